@@ -1,0 +1,184 @@
+"""FP64 CPU oracle of the Hs-descent path of ``ddg-repulsive-surfaces`` — TEST INFRASTRUCTURE.
+
+A C++ restatement of the reference's Rust code (``oracle/src/*.cpp``, each function citing the
+reference file and line it follows), compiled with ``-ffp-contract=off`` by ``oracle/Makefile``
+into ``oracle/_build/librs_oracle.so`` and loaded here through ctypes.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may use
+this module, and only as the checker or the timed CPU baseline.  The GPU library never loads it.
+
+Parity pins (see the oracle tests, ``tests/test_oracle_*.py``):
+* exact energy / differential: the Lean golden fixtures ``rs-*-energy*`` (1e-12 normwise), the
+  authors'-program fixtures (1e-12 / 1e-11), closed forms (two parallel / perpendicular
+  triangles), brute force, the scaling and rigid-motion identities and central differences;
+* BVH / face tree / Barnes-Hut lists: structural invariants (leaf bound, contiguous ranges, the
+  quarter-balance window, the cost minimality of every split by brute force), cluster data equal
+  to direct sums, the lists partitioning every ordered pair exactly once;
+* Barnes-Hut energy / gradient: theta = 0 equals the exact evaluation, central differences
+  with frozen lists, the O(theta^2) error and the authors' Barnes-Hut value within 1e-2.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "librs_oracle.so")
+_lib = None
+
+STATUS_NAMES = {
+    0: "Ok",
+    1: "InvalidOption",
+    2: "InvalidExponents",
+    3: "NonFiniteEnergy",
+    4: "SolverDiverged",
+    5: "SingularSaddle",
+    6: "DegenerateFace",
+    7: "NonFinitePosition",
+}
+
+
+class OracleError(RuntimeError):
+    def __init__(self, status: int, detail: str = "", culprit=None):
+        self.status = status
+        self.kind = STATUS_NAMES.get(status, str(status))
+        self.detail = detail
+        self.culprit = culprit
+        super().__init__(f"{self.kind}: {detail}")
+
+
+def build() -> str:
+    """Compiles the oracle with its Makefile (g++ only)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_D = ctypes.c_double
+_INT = ctypes.c_int
+
+
+def _declare(L):
+    L.orc_exact.argtypes = [_I64, _I64, _P, _P, _D, _D, _INT, _P, _P, _P, _P, ctypes.c_char_p, _I64]
+    L.orc_exact.restype = _INT
+    L.orc_bh_build.argtypes = [_I64, _I64, _P, _P, _D, _I64, _INT, ctypes.POINTER(_INT), ctypes.c_char_p, _I64]
+    L.orc_bh_build.restype = _P
+    L.orc_bh_free.argtypes = [_P]
+    L.orc_bh_refit.argtypes = [_P, _P]
+    L.orc_bh_refit.restype = _INT
+    L.orc_bh_counts.argtypes = [_P, _P]
+    L.orc_bh_tree.argtypes = [_P, _P, _P, _P, _P, _P]
+    L.orc_bh_lists.argtypes = [_P, _P, _P, _P, _P]
+    L.orc_bh_energy.argtypes = [_P, _D, _D, _INT, _P]
+    L.orc_bh_energy.restype = _INT
+    L.orc_bh_energy_and_differential.argtypes = [_P, _D, _D, _INT, _P, _P, _P]
+    L.orc_bh_energy_and_differential.restype = _INT
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _mesh(faces, positions):
+    f = np.ascontiguousarray(faces, dtype=np.uint32).reshape(-1, 3)
+    p = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+    return f, p
+
+
+def exact(faces, positions, alpha=6.0, beta=12.0, threads=0):
+    """All-pairs energy, differential (nv x 3) and per-face partials (nf x 7) in FP64
+    (exact.rs:146 ``energy_and_differential``)."""
+    f, p = _mesh(faces, positions)
+    nv, nf = p.shape[0], f.shape[0]
+    e = np.zeros(1)
+    d = np.zeros((nv, 3))
+    fp = np.zeros((nf, 7))
+    cul = np.full(2, -1, dtype=np.int64)
+    msg = ctypes.create_string_buffer(256)
+    st = lib().orc_exact(nv, nf, _ptr(f), _ptr(p), alpha, beta, threads, _ptr(e), _ptr(d), _ptr(fp), _ptr(cul), msg, 256)
+    if st != 0:
+        raise OracleError(st, msg.value.decode(), tuple(cul))
+    return float(e[0]), d, fp
+
+
+class BhTree:
+    """The oracle's Barnes-Hut evaluator (barnes_hut.rs:114 ``BhTree``)."""
+
+    def __init__(self, faces, positions, theta=0.5, leaf_size=8, threads=0):
+        f, p = _mesh(faces, positions)
+        self.nv, self.nf = p.shape[0], f.shape[0]
+        self.threads = threads
+        st = _INT(0)
+        msg = ctypes.create_string_buffer(256)
+        self._h = lib().orc_bh_build(self.nv, self.nf, _ptr(f), _ptr(p), theta, leaf_size, threads, ctypes.byref(st), msg, 256)
+        if not self._h:
+            raise OracleError(st.value, msg.value.decode())
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().orc_bh_free(h)
+            self._h = None
+
+    def refit(self, positions):
+        p = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+        st = lib().orc_bh_refit(self._h, _ptr(p))
+        if st != 0:
+            raise OracleError(st)
+
+    def counts(self):
+        c = np.zeros(6, dtype=np.int64)
+        lib().orc_bh_counts(self._h, _ptr(c))
+        keys = ["nodes", "leaves", "lumped", "exact_blocks", "pairs_exact", "partner_slots"]
+        return dict(zip(keys, (int(x) for x in c)))
+
+    def tree(self):
+        c = self.counts()
+        nn = c["nodes"]
+        order = np.zeros(self.nf, dtype=np.uint32)
+        start = np.zeros(nn, dtype=np.uint32)
+        end = np.zeros(nn, dtype=np.uint32)
+        right = np.zeros(nn, dtype=np.uint32)
+        cl = np.zeros((nn, 17))
+        lib().orc_bh_tree(self._h, _ptr(order), _ptr(start), _ptr(end), _ptr(right), _ptr(cl))
+        return {"order": order, "start": start, "end": end, "right": right, "clusters": cl}
+
+    def lists(self):
+        c = self.counts()
+        nl = c["leaves"]
+        lo = np.zeros(nl + 1, dtype=np.uint64)
+        lm = np.zeros(c["lumped"], dtype=np.uint32)
+        eo = np.zeros(nl + 1, dtype=np.uint64)
+        ex = np.zeros(c["exact_blocks"], dtype=np.uint32)
+        lib().orc_bh_lists(self._h, _ptr(lo), _ptr(lm), _ptr(eo), _ptr(ex))
+        return {"lumped_off": lo, "lumped": lm, "exact_off": eo, "exact": ex}
+
+    def energy(self, alpha=6.0, beta=12.0):
+        e = np.zeros(1)
+        st = lib().orc_bh_energy(self._h, alpha, beta, self.threads, _ptr(e))
+        if st != 0:
+            raise OracleError(st)
+        return float(e[0])
+
+    def energy_and_differential(self, alpha=6.0, beta=12.0):
+        e = np.zeros(1)
+        d = np.zeros((self.nv, 3))
+        fp = np.zeros((self.nf, 7))
+        st = lib().orc_bh_energy_and_differential(self._h, alpha, beta, self.threads, _ptr(e), _ptr(d), _ptr(fp))
+        if st != 0:
+            raise OracleError(st)
+        return float(e[0]), d, fp

@@ -1,0 +1,159 @@
+// FP64 oracle — TEST INFRASTRUCTURE ONLY: a flat C interface for the Python test harness
+// (oracle/__init__.py loads it through ctypes).  Arrays are flat: faces `3·nf` u32, positions and
+// differentials `3·nv` f64 interleaved (`3 v + axis`), per-face partials `7·nf`
+// (∂/∂a, ∂/∂X, ∂/∂N).
+#include <cstring>
+
+#include "oracle.hpp"
+
+using namespace orc;
+
+namespace {
+Mesh make_mesh(int64_t nv, int64_t nf, const uint32_t* faces) {
+  Mesh m;
+  m.nv = nv;
+  m.faces.resize(nf);
+  for (int64_t f = 0; f < nf; ++f) m.faces[f] = {faces[3 * f], faces[3 * f + 1], faces[3 * f + 2]};
+  return m;
+}
+Positions make_pos(int64_t nv, const double* p) {
+  Positions out(nv);
+  for (int64_t v = 0; v < nv; ++v) out[v] = {p[3 * v], p[3 * v + 1], p[3 * v + 2]};
+  return out;
+}
+int fail(const Error& e, char* msg, int64_t len, int64_t* culprit) {
+  if (msg && len > 0) {
+    std::strncpy(msg, e.detail.c_str(), (size_t)len - 1);
+    msg[len - 1] = 0;
+  }
+  if (culprit) {
+    culprit[0] = e.a;
+    culprit[1] = e.b;
+  }
+  return e.status;
+}
+bool make_exps(double a, double b, Exponents* e, Error* err) {
+  std::string why;
+  if (!Exponents::make(a, b, e, &why)) {
+    err->status = INVALID_EXPONENTS;
+    err->detail = why;
+    return false;
+  }
+  return true;
+}
+void write_partials(const FacePartials& fp, double* out) {
+  for (size_t f = 0; f < fp.d_area.size(); ++f) {
+    out[7 * f] = fp.d_area[f];
+    for (int k = 0; k < 3; ++k) {
+      out[7 * f + 1 + k] = fp.d_bary[f][k];
+      out[7 * f + 4 + k] = fp.d_normal[f][k];
+    }
+  }
+}
+}  // namespace
+
+extern "C" {
+
+int orc_exact(int64_t nv, int64_t nf, const uint32_t* faces, const double* pos, double alpha, double beta,
+              int threads, double* energy, double* diff, double* partials, int64_t* culprit, char* msg,
+              int64_t msglen) {
+  Error err;
+  Exponents e;
+  if (!make_exps(alpha, beta, &e, &err)) return fail(err, msg, msglen, culprit);
+  Mesh m = make_mesh(nv, nf, faces);
+  Positions p = make_pos(nv, pos);
+  std::vector<V3> d;
+  FacePartials fp;
+  if (!exact_energy_and_differential(m, p, e, threads, energy, &d, &fp, &err)) return fail(err, msg, msglen, culprit);
+  for (int64_t v = 0; v < nv; ++v)
+    for (int k = 0; k < 3; ++k) diff[3 * v + k] = d[v][k];
+  if (partials) write_partials(fp, partials);
+  return OK;
+}
+
+void* orc_bh_build(int64_t nv, int64_t nf, const uint32_t* faces, const double* pos, double theta,
+                   int64_t leaf_size, int threads, int* status, char* msg, int64_t msglen) {
+  Error err;
+  auto* bh = new BhTree();
+  if (!bh_build(make_mesh(nv, nf, faces), make_pos(nv, pos), theta, (size_t)leaf_size, threads, bh, &err)) {
+    delete bh;
+    *status = fail(err, msg, msglen, nullptr);
+    return nullptr;
+  }
+  *status = OK;
+  return bh;
+}
+
+void orc_bh_free(void* h) { delete static_cast<BhTree*>(h); }
+
+int orc_bh_refit(void* h, const double* pos) {
+  auto* bh = static_cast<BhTree*>(h);
+  Error err;
+  if (!bh_refit(*bh, make_pos(bh->mesh.nv, pos), &err)) return err.status;
+  return OK;
+}
+
+// counts: [nodes, leaves, lumped entries, exact entries, pairs_exact, partner slots]
+void orc_bh_counts(void* h, int64_t* out) {
+  auto* bh = static_cast<BhTree*>(h);
+  out[0] = (int64_t)bh->tree.bvh.nodes.size();
+  out[1] = (int64_t)bh->tree.leaves.size();
+  out[2] = (int64_t)bh->lists.lumped.size();
+  out[3] = (int64_t)bh->lists.exact.size();
+  out[4] = (int64_t)bh->lists.pairs_exact;
+  out[5] = (int64_t)bh->lists.partner_off.back();
+}
+
+// The tree: order (nf), per node start/end/right, and per node
+// [area, centroid(3), projector(6), bbox lo(3), bbox hi(3), radius] (17 f64).
+void orc_bh_tree(void* h, uint32_t* order, uint32_t* start, uint32_t* end, uint32_t* right, double* cl) {
+  auto* bh = static_cast<BhTree*>(h);
+  const auto& t = bh->tree;
+  std::memcpy(order, t.bvh.order.data(), t.bvh.order.size() * 4);
+  for (size_t n = 0; n < t.bvh.nodes.size(); ++n) {
+    start[n] = t.bvh.nodes[n].start;
+    end[n] = t.bvh.nodes[n].end;
+    right[n] = t.bvh.nodes[n].right;
+    const Cluster& c = t.clusters[n];
+    double* o = cl + 17 * n;
+    o[0] = c.area;
+    for (int k = 0; k < 3; ++k) o[1 + k] = c.centroid[k];
+    for (int k = 0; k < 6; ++k) o[4 + k] = c.projector[k];
+    for (int k = 0; k < 3; ++k) o[10 + k] = c.bbox.lo[k];
+    for (int k = 0; k < 3; ++k) o[13 + k] = c.bbox.hi[k];
+    o[16] = c.radius;
+  }
+}
+
+void orc_bh_lists(void* h, uint64_t* lumped_off, uint32_t* lumped, uint64_t* exact_off, uint32_t* exact) {
+  auto* bh = static_cast<BhTree*>(h);
+  const auto& L = bh->lists;
+  std::memcpy(lumped_off, L.lumped_off.data(), L.lumped_off.size() * 8);
+  std::memcpy(lumped, L.lumped.data(), L.lumped.size() * 4);
+  std::memcpy(exact_off, L.exact_off.data(), L.exact_off.size() * 8);
+  std::memcpy(exact, L.exact.data(), L.exact.size() * 4);
+}
+
+int orc_bh_energy(void* h, double alpha, double beta, int threads, double* energy) {
+  Error err;
+  Exponents e;
+  if (!make_exps(alpha, beta, &e, &err)) return err.status;
+  *energy = bh_energy(*static_cast<BhTree*>(h), e, threads);
+  return OK;
+}
+
+int orc_bh_energy_and_differential(void* h, double alpha, double beta, int threads, double* energy,
+                                   double* diff, double* partials) {
+  Error err;
+  Exponents e;
+  if (!make_exps(alpha, beta, &e, &err)) return err.status;
+  auto* bh = static_cast<BhTree*>(h);
+  if (partials) write_partials(bh_face_partials(*bh, e, threads), partials);
+  std::vector<V3> d;
+  if (!bh_energy_and_differential(*bh, e, threads, energy, &d, &err)) return err.status;
+  for (int64_t v = 0; v < bh->mesh.nv; ++v)
+    for (int k = 0; k < 3; ++k) diff[3 * v + k] = d[v][k];
+  return OK;
+}
+
+}  // extern "C"

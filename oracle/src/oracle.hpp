@@ -1,0 +1,153 @@
+// FP64 oracle — TEST INFRASTRUCTURE ONLY.  Declarations shared by the oracle's translation units.
+#pragma once
+#include <atomic>
+
+#include "core.hpp"
+
+namespace orc {
+
+struct FacePartials {
+  double energy = 0.0;
+  std::vector<double> d_area;
+  std::vector<V3> d_bary, d_normal;
+};
+
+FacePartials exact_face_partials(const FaceGeometry& g, const Exponents& e, int threads);
+bool exact_energy_and_differential(const Mesh& m, const Positions& p, const Exponents& e, int threads,
+                                   double* energy, std::vector<V3>* diff, FacePartials* partials,
+                                   Error* err);
+
+// ---------------------------------------------------------------------------------------------
+// Axis-aligned boxes (crates/ddg-repulsive-curves/src/spatial/aabb.rs)
+
+struct Aabb {
+  V3 lo{INFINITY, INFINITY, INFINITY};
+  V3 hi{-INFINITY, -INFINITY, -INFINITY};
+  bool is_empty() const { return lo[0] > hi[0] || lo[1] > hi[1] || lo[2] > hi[2]; }
+  void grow(V3 p) {
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = fmin_(lo[k], p[k]);
+      hi[k] = fmax_(hi[k], p[k]);
+    }
+  }
+  Aabb unite(const Aabb& o) const {
+    Aabb r = *this;
+    for (int k = 0; k < 3; ++k) {
+      r.lo[k] = fmin_(r.lo[k], o.lo[k]);
+      r.hi[k] = fmax_(r.hi[k], o.hi[k]);
+    }
+    return r;
+  }
+  // aabb.rs:73 `diameter_sq`
+  double diameter_sq() const {
+    if (is_empty()) return 0.0;
+    double s = 0.0;
+    for (int k = 0; k < 3; ++k) {
+      double w = hi[k] - lo[k];
+      s += w * w;
+    }
+    return s;
+  }
+};
+
+// spatial/mod.rs:113 `box_distance_sq`
+inline double box_distance_sq(const Aabb& a, const Aabb& b) {
+  double d2 = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    double d = fmax_(fmax_(a.lo[k], b.lo[k]) - fmin_(a.hi[k], b.hi[k]), 0.0);
+    d2 += d * d;
+  }
+  return d2;
+}
+
+// spatial/mod.rs:125 `farthest_corner_distance`
+inline double farthest_corner_distance(const Aabb& b, V3 x) {
+  double r2 = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    double d = fmax_(b.hi[k] - x[k], x[k] - b.lo[k]);
+    r2 += d * d;
+  }
+  return std::sqrt(r2);
+}
+
+// ---------------------------------------------------------------------------------------------
+// BVH (crates/ddg-repulsive-curves/src/spatial/bvh.rs)
+
+struct BvhNode {
+  Aabb bbox;
+  double mass = 0.0;
+  V3 moment{0.0, 0.0, 0.0};
+  uint32_t start = 0, end = 0, right = 0;
+  bool is_leaf() const { return right == 0; }
+  V3 centroid() const { return {moment[0] / mass, moment[1] / mass, moment[2] / mass}; }
+};
+
+struct Bvh {
+  std::vector<BvhNode> nodes;
+  std::vector<uint32_t> order;
+  size_t leaf_size = 8;
+};
+
+// bvh.rs:126 `build_with` (min_fraction 0.25) followed by :221 `refit`.
+Bvh bvh_build(const std::vector<V3>& pts, const std::vector<Aabb>& bnds, const std::vector<double>& mass,
+              size_t leaf_size, double min_fraction = 0.25);
+void bvh_refit(Bvh& b, const std::vector<V3>& pts, const std::vector<Aabb>& bnds,
+               const std::vector<double>& mass);
+
+// ---------------------------------------------------------------------------------------------
+// Face tree (crates/ddg-repulsive-surfaces/src/spatial/mod.rs)
+
+struct Cluster {
+  double area = 0.0;
+  V3 centroid{0, 0, 0};
+  std::array<double, 6> projector{};
+  Aabb bbox;
+  double radius = 0.0;
+  // spatial/mod.rs:101 `projector_quadratic_form`
+  double pq(V3 v) const {
+    const auto& p = projector;
+    return p[0] * v[0] * v[0] + p[3] * v[1] * v[1] + p[5] * v[2] * v[2] +
+           2.0 * (p[1] * v[0] * v[1] + p[2] * v[0] * v[2] + p[4] * v[1] * v[2]);
+  }
+};
+
+struct FaceTree {
+  Bvh bvh;
+  std::vector<Cluster> clusters;
+  std::vector<uint32_t> leaves, leaf_ordinal, leaf_of_face;
+};
+
+std::vector<Aabb> face_bounds(const Mesh& m, const Positions& p);
+FaceTree face_tree_build(const Mesh& m, const Positions& p, const FaceGeometry& g, size_t leaf_size);
+void face_tree_refit(FaceTree& t, const Mesh& m, const Positions& p, const FaceGeometry& g);
+
+// ---------------------------------------------------------------------------------------------
+// Barnes–Hut (crates/ddg-repulsive-surfaces/src/energy/barnes_hut.rs)
+
+struct InteractionLists {
+  std::vector<uint64_t> lumped_off, exact_off, partner_off, lumped_t_off, exact_t_off;
+  std::vector<uint32_t> lumped, exact, lumped_t, exact_t;
+  uint64_t pairs_exact = 0;
+};
+
+InteractionLists bh_build_lists(const FaceTree& t, double theta, int threads);
+
+struct BhTree {
+  double theta = 0.5;
+  size_t leaf_size = 8;
+  Mesh mesh;
+  Positions positions;
+  FaceGeometry geom;
+  FaceTree tree;
+  InteractionLists lists;
+};
+
+bool bh_build(const Mesh& m, const Positions& p, double theta, size_t leaf_size, int threads, BhTree* out,
+              Error* err);
+bool bh_refit(BhTree& bh, const Positions& p, Error* err);
+double bh_energy(const BhTree& bh, const Exponents& e, int threads);
+FacePartials bh_face_partials(const BhTree& bh, const Exponents& e, int threads);
+bool bh_energy_and_differential(const BhTree& bh, const Exponents& e, int threads, double* energy,
+                                std::vector<V3>* diff, Error* err);
+
+}  // namespace orc
