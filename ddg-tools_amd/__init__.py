@@ -1,0 +1,265 @@
+"""ddg-tools_amd — the MI355X (gfx950) Hs-descent path of ``ddg-repulsive-surfaces``.
+
+Python mirror of the reference's API for this path, over the C ABI of ``include/ddgrs.h``
+(``lib/libddgrs_hip.so``, built by ``build.py``).  Names, argument meaning and errors follow the
+reference (``crates/ddg-repulsive-surfaces``):
+
+* ``Context`` — ``GpuContext`` / ``Device::Cuda`` (``src/gpu/mod.rs:47-151``);
+* ``Surface`` — the validated mesh with device-resident positions (``src/surface.rs:132,513``);
+* ``energy(surface, exponents, eval)`` — ``energy()`` (``src/energy/mod.rs:168``) for
+  ``EnergyEval.Exact`` and ``EnergyEval.BarnesHut``;
+* errors raise ``RsError`` whose ``kind`` names the reference's ``RsError`` variant.
+
+The library is the only compute path: if ``lib/libddgrs_hip.so`` is missing, importing the
+binding raises; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libddgrs_hip.so")
+
+KINDS = {
+    0: "Ok",
+    1: "InvalidOption",
+    2: "InvalidExponents",
+    3: "NonFiniteEnergy",
+    4: "SolverDiverged",
+    5: "SingularSaddle",
+    6: "DegenerateFace",
+    7: "NonFinitePosition",
+    8: "Gpu",
+}
+
+
+class RsError(RuntimeError):
+    """A failed call: ``kind`` (the reference's RsError variant), ``operation``, ``detail``."""
+
+    def __init__(self, status, operation, detail, a=-1, b=-1, value=0.0):
+        self.status = status
+        self.kind = KINDS.get(status, str(status))
+        self.operation = operation
+        self.detail = detail
+        self.a, self.b, self.value = a, b, value
+        super().__init__(f"{self.kind} [{operation}]: {detail}")
+
+
+class _Err(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int32),
+        ("operation", ctypes.c_char * 64),
+        ("detail", ctypes.c_char * 256),
+        ("a", ctypes.c_int64),
+        ("b", ctypes.c_int64),
+        ("value", ctypes.c_double),
+    ]
+
+
+_lib = None
+
+
+def library_path() -> str:
+    return LIB_PATH
+
+
+def lib():
+    """The loaded C ABI library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with ddg-tools_amd/build.py (no CPU fallback exists)")
+        _lib = ctypes.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+_P = ctypes.c_void_p
+_I32 = ctypes.c_int32
+_I64 = ctypes.c_int64
+_D = ctypes.c_double
+_E = ctypes.POINTER(_Err)
+
+# name -> (restype, argtypes); every int-returning entry point takes a trailing ddgrs_error*.
+SIGNATURES = {
+    "ddgrs_ctx_create": (_I32, [_I32, _I32, ctypes.POINTER(_P), _E]),
+    "ddgrs_ctx_destroy": (None, [_P]),
+    "ddgrs_device_info": (_I32, [_P, ctypes.c_char_p, _I64, ctypes.POINTER(_I64), ctypes.POINTER(_I64), _E]),
+    "ddgrs_ctx_synchronize": (_I32, [_P, _E]),
+    "ddgrs_ctx_set_profiling": (_I32, [_P, _I32, _E]),
+    "ddgrs_ctx_kernel_times": (_I32, [_P, _I64, _P, _P, _P, ctypes.POINTER(_I64), _E]),
+    "ddgrs_surface_create": (_I32, [_P, _I64, _I64, _P, _P, ctypes.POINTER(_P), _E]),
+    "ddgrs_surface_destroy": (None, [_P]),
+    "ddgrs_surface_set_positions": (_I32, [_P, _P, _E]),
+    "ddgrs_surface_get_positions": (_I32, [_P, _P, _E]),
+    "ddgrs_exact_energy_and_differential": (_I32, [_P, _D, _D, ctypes.POINTER(_D), _P, _P, _E]),
+}
+
+
+def _declare(L):
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+def _call(fn, *args):
+    err = _Err()
+    st = fn(*args, ctypes.byref(err))
+    if st != 0:
+        raise RsError(st, err.operation.decode(errors="replace"), err.detail.decode(errors="replace"), err.a, err.b, err.value)
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _f64(a, shape=None) -> np.ndarray:
+    x = np.ascontiguousarray(a, dtype=np.float64)
+    return x.reshape(shape) if shape is not None else x
+
+
+class Context:
+    """One device and one in-order stream (``GpuContext``, src/gpu/mod.rs:129)."""
+
+    def __init__(self, ordinal: int = 0, determinism: int = 1):
+        h = _P()
+        _call(lib().ddgrs_ctx_create, ordinal, determinism, ctypes.byref(h))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ddgrs_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def device_info(self) -> dict:
+        name = ctypes.create_string_buffer(256)
+        cu, mem = _I64(), _I64()
+        _call(lib().ddgrs_device_info, self._h, name, 256, ctypes.byref(cu), ctypes.byref(mem))
+        return {"name": name.value.decode(), "compute_units": cu.value, "memory_bytes": mem.value}
+
+    def synchronize(self):
+        _call(lib().ddgrs_ctx_synchronize, self._h)
+
+    def set_profiling(self, enabled: bool):
+        _call(lib().ddgrs_ctx_set_profiling, self._h, 1 if enabled else 0)
+
+    def kernel_times(self) -> dict:
+        cap = 256
+        names = ctypes.create_string_buffer(64 * cap)
+        launches = np.zeros(cap, dtype=np.int64)
+        ms = np.zeros(cap)
+        n = _I64()
+        _call(lib().ddgrs_ctx_kernel_times, self._h, cap, names, _ptr(launches), _ptr(ms), ctypes.byref(n))
+        out = {}
+        for i in range(min(n.value, cap)):
+            nm = names.raw[64 * i: 64 * (i + 1)].split(b"\0", 1)[0].decode()
+            out[nm] = {"launches": int(launches[i]), "ms": float(ms[i])}
+        return out
+
+
+class Surface:
+    """A triangle mesh with its positions on the device (``Surface``, src/surface.rs:132)."""
+
+    def __init__(self, ctx: Context, faces, positions):
+        self.ctx = ctx
+        self.faces = np.ascontiguousarray(faces, dtype=np.uint32).reshape(-1, 3)
+        p = _f64(positions).reshape(-1, 3)
+        self.nv, self.nf = p.shape[0], self.faces.shape[0]
+        h = _P()
+        _call(lib().ddgrs_surface_create, ctx.handle, self.nv, self.nf, _ptr(self.faces), _ptr(p), ctypes.byref(h))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ddgrs_surface_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_positions(self, positions):
+        p = _f64(positions).reshape(-1, 3)
+        if p.shape[0] != self.nv:
+            raise RsError(1, "set_positions", f"{p.shape[0]} positions for {self.nv} vertices")
+        _call(lib().ddgrs_surface_set_positions, self._h, _ptr(p))
+
+    def positions(self) -> np.ndarray:
+        out = np.zeros((self.nv, 3))
+        _call(lib().ddgrs_surface_get_positions, self._h, _ptr(out))
+        return out
+
+
+# ---- energy (src/energy/mod.rs) ----------------------------------------------------------------
+
+
+@dataclasses.dataclass(frozen=True)
+class Exponents:
+    """``Exponents`` (src/kernel.rs:26); validated by the library."""
+
+    alpha: float = 6.0
+    beta: float = 12.0
+
+
+@dataclasses.dataclass(frozen=True)
+class BhOptions:
+    """``BhOptions`` (src/energy/mod.rs:62): θ = 0.5, leaf size 8 by default."""
+
+    theta: float = 0.5
+    leaf_size: int = 8
+
+
+class EnergyEval:
+    """``EnergyEval::{Exact, BarnesHut}`` (src/energy/mod.rs:106)."""
+
+    Exact = "exact"
+
+    @staticmethod
+    def BarnesHut(opts: BhOptions = BhOptions()):
+        return ("barnes_hut", opts)
+
+
+@dataclasses.dataclass
+class EnergyReport:
+    """``EnergyReport`` (src/energy/mod.rs:130) without penalties."""
+
+    energy: float
+    differential: np.ndarray
+    pairs_exact: int
+    nodes_lumped: int
+
+
+def exact_energy_and_differential(surface: Surface, exponents: Exponents = Exponents(), partials: bool = False):
+    """``exact::energy_and_differential`` (src/energy/exact.rs:146): (E, dE (nv×3)[, partials (nf×7)])."""
+    e = _D()
+    d = np.zeros((surface.nv, 3))
+    fp = np.zeros((surface.nf, 7)) if partials else None
+    _call(lib().ddgrs_exact_energy_and_differential, surface.handle, exponents.alpha, exponents.beta, ctypes.byref(e),
+          _ptr(d), _ptr(fp) if partials else None)
+    return (e.value, d, fp) if partials else (e.value, d)
+
+
+def energy(surface: Surface, exponents: Exponents = Exponents(), eval=EnergyEval.Exact) -> EnergyReport:
+    """``energy()`` (src/energy/mod.rs:168) without penalties."""
+    if eval == EnergyEval.Exact:
+        e, d = exact_energy_and_differential(surface, exponents)
+        return EnergyReport(e, d, surface.nf * (surface.nf - 1), 0)
+    raise RsError(1, "energy", f"unsupported evaluation {eval!r}")
+
+
+from . import generate  # noqa: E402,F401

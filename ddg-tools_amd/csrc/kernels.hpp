@@ -1,0 +1,31 @@
+// Shared declarations of the library's kernels' host wrappers and device helpers.
+#pragma once
+#include "runtime.hpp"
+
+namespace ddgrs {
+
+
+inline Scratch& scratch_of(ddgrs_ctx* ctx) { return ctx->scratch; }
+
+// Exponents as the kernels take them (RS/src/kernel.rs:26 Exponents): integer exponents use
+// products, others powf.
+struct Exps {
+  float alpha, beta;
+  int alpha_int, beta_int;  // -1 when not an integer (kernel.rs:37 as_small_int)
+  double alpha64, beta64;
+};
+Exps make_exps(double alpha, double beta);  // throws DDGRS_INVALID_EXPONENTS
+
+// Geometry of the surface's current positions into s->area/normal/bary and the packs;
+// `validate` checks the positions as RS/src/surface.rs:513 (synchronizes).
+void compute_face_geometry(ddgrs_surface* s, FacePacks& packs, bool validate);
+// out[3v+k] = Σ_{S∋v} face_pull_back(S)[corner]; partials: 7 per face.
+void pull_back(ddgrs_surface* s, const double* partials, double* out);
+// out[0] = Σ x (fixed order).
+void sum_f64(ddgrs_ctx* ctx, int64_t n, const double* x, double* out);
+
+// Exact all-pairs energy: per-face partials (7·nf, FP64) and per-face energy rows (nf, FP64).
+void exact_pairs(ddgrs_ctx* ctx, int64_t nf, const FacePacks& packs, const double* area, const Exps& e,
+                 double* partials, double* row_energy);
+
+}  // namespace ddgrs

@@ -1,0 +1,97 @@
+/*
+ * ddgrs — MI355X (gfx950) implementation of the Hs-descent hot path of ddg-repulsive-surfaces.
+ *
+ * C ABI only: opaque handles, plain pointers and sizes, no C++ or torch types.  Every entry point
+ * returns a status (0 = ok) and fills `ddgrs_error` on failure; nothing aborts.  All arrays at the
+ * boundary are the reference's: faces are `3·nf` u32 (caller's vertex order), positions and
+ * differentials are `3·nv` f64 interleaved `[3 v + axis]` (crates/ddg-repulsive-surfaces/src/metric/
+ * hierarchical.rs:39-42), saddle vectors are `3·nv + k_Q` f64.  Input slices are borrowed for the
+ * duration of the call; outputs are caller-allocated and overwritten (solve/gmres.rs:36-40).  Device
+ * copies belong to the handle until it is destroyed.  One context is one in-order HIP stream; calls on
+ * a context are serialized by the caller (gpu/cuda.rs:111-114).
+ *
+ * Each declaration names the reference interface it replaces (path relative to the reference root,
+ * `RS/` = crates/ddg-repulsive-surfaces/).
+ */
+#ifndef DDGRS_H
+#define DDGRS_H
+
+#include <stdint.h>
+
+#if defined(__GNUC__)
+#define DDGRS_API __attribute__((visibility("default")))
+#else
+#define DDGRS_API
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes; the kinds mirror RsError (RS/src/error.rs). */
+enum ddgrs_status {
+  DDGRS_OK = 0,
+  DDGRS_INVALID_OPTION = 1,     /* RsError::InvalidOption{name, value, requirement} */
+  DDGRS_INVALID_EXPONENTS = 2,  /* RsError::InvalidExponents */
+  DDGRS_NON_FINITE_ENERGY = 3,  /* RsError::NonFiniteEnergy{faces} */
+  DDGRS_SOLVER_DIVERGED = 4,    /* RsError::SolverDiverged{residual} */
+  DDGRS_SINGULAR_SADDLE = 5,    /* RsError::SingularSaddle{pivot} */
+  DDGRS_DEGENERATE_FACE = 6,    /* RsError::DegenerateFace{face, area} */
+  DDGRS_NON_FINITE_POSITION = 7,/* RsError::NonFinitePosition{vertex} */
+  DDGRS_GPU = 8                 /* RsError::Gpu{operation, detail} (RS/src/gpu/cuda.rs:19-24) */
+};
+
+typedef struct ddgrs_error {
+  int32_t status;
+  char operation[64];
+  char detail[256];
+  int64_t a, b; /* e.g. the face pair of NonFiniteEnergy, the face of DegenerateFace */
+  double value; /* e.g. the residual of SolverDiverged */
+} ddgrs_error;
+
+typedef struct ddgrs_ctx ddgrs_ctx;
+typedef struct ddgrs_surface ddgrs_surface;
+
+/* ---- context ------------------------------------------------------------------------------- */
+
+/* Replaces GpuContext::new / CudaBackend::new (RS/src/gpu/mod.rs:151, RS/src/gpu/cuda.rs:106),
+ * selected by Device::Cuda(CudaOptions{ordinal, determinism}) (RS/src/gpu/mod.rs:47-87).
+ * determinism: 1 = run-to-run bitwise reproducible (the only level; results never depend on the
+ * launch shape). */
+DDGRS_API int ddgrs_ctx_create(int32_t ordinal, int32_t determinism, ddgrs_ctx** out, ddgrs_error* err);
+DDGRS_API void ddgrs_ctx_destroy(ddgrs_ctx* ctx);
+/* Replaces gpu_info() (crates/ddg-py/src/python/repulsive_surfaces.rs:683). */
+DDGRS_API int ddgrs_device_info(ddgrs_ctx* ctx, char* name, int64_t name_len, int64_t* compute_units,
+                      int64_t* memory_bytes, ddgrs_error* err);
+/* Blocks until the context's stream is idle. */
+DDGRS_API int ddgrs_ctx_synchronize(ddgrs_ctx* ctx, ddgrs_error* err);
+/* Kernel timing: when enabled, every launch is bracketed by HIP events on the context's stream;
+ * ddgrs_ctx_kernel_times reports per kernel name the launch count and the summed milliseconds. */
+DDGRS_API int ddgrs_ctx_set_profiling(ddgrs_ctx* ctx, int32_t enabled, ddgrs_error* err);
+DDGRS_API int ddgrs_ctx_kernel_times(ddgrs_ctx* ctx, int64_t capacity, char* names /* capacity*64 */,
+                           int64_t* launches, double* milliseconds, int64_t* count, ddgrs_error* err);
+
+/* ---- surface (device-resident mesh and positions) ------------------------------------------ */
+
+/* The validated triangle mesh and its positions on the device (RS/src/surface.rs:132 Surface::new,
+ * :513 validate_positions). */
+DDGRS_API int ddgrs_surface_create(ddgrs_ctx* ctx, int64_t nv, int64_t nf, const uint32_t* faces,
+                         const double* positions, ddgrs_surface** out, ddgrs_error* err);
+DDGRS_API void ddgrs_surface_destroy(ddgrs_surface* s);
+/* Moves the surface (validated as RS/src/surface.rs:513). */
+DDGRS_API int ddgrs_surface_set_positions(ddgrs_surface* s, const double* positions, ddgrs_error* err);
+DDGRS_API int ddgrs_surface_get_positions(ddgrs_surface* s, double* positions, ddgrs_error* err);
+
+/* ---- energy --------------------------------------------------------------------------------- */
+
+/* The exact all-pairs tangent-point energy and its differential `∂E/∂f_v` (3·nv, optional),
+ * FP32 pair kernel with compensated accumulation.  Replaces energy() with EnergyEval::Exact
+ * (RS/src/energy/mod.rs:168 → RS/src/energy/exact.rs:146 energy_and_differential).
+ * face_partials (optional, 7·nf): (∂/∂a_S, ∂/∂X_S, ∂/∂N_S) as exact.rs:87 face_partials. */
+DDGRS_API int ddgrs_exact_energy_and_differential(ddgrs_surface* s, double alpha, double beta, double* energy,
+                                        double* differential, double* face_partials, ddgrs_error* err);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DDGRS_H */
