@@ -87,6 +87,17 @@ def _declare(L):
     L.orc_bh_energy.restype = _INT
     L.orc_bh_energy_and_differential.argtypes = [_P, _D, _D, _INT, _P, _P, _P]
     L.orc_bh_energy_and_differential.restype = _INT
+    L.orc_metric_build.argtypes = [_I64, _I64, _P, _P, _D, _D, _D, _I64, _I64, _INT, ctypes.POINTER(_INT), ctypes.c_char_p, _I64]
+    L.orc_metric_build.restype = _P
+    L.orc_metric_free.argtypes = [_P]
+    L.orc_metric_apply.argtypes = [_P, _D, _D, _I64, _P, _P, _INT]
+    L.orc_metric_stats.argtypes = [_P, _P]
+    L.orc_metric_ha.argtypes = [_P, _P]
+    L.orc_vfrac_build.argtypes = [_I64, _P, _P, _D, _D, _I64, _I64, _INT, ctypes.POINTER(_INT)]
+    L.orc_vfrac_build.restype = _P
+    L.orc_vfrac_free.argtypes = [_P]
+    L.orc_vfrac_apply.argtypes = [_P, _I64, _P, _P, _INT]
+    L.orc_vertex_areas.argtypes = [_I64, _I64, _P, _P, _P]
 
 
 def _ptr(a: np.ndarray):
@@ -182,3 +193,77 @@ class BhTree:
         if st != 0:
             raise OracleError(st)
         return float(e[0]), d, fp
+
+
+class FaceMetric:
+    """The oracle's hierarchical B̃, B̃⁰ (hierarchical.rs:283 ``FaceMetricTree``)."""
+
+    def __init__(self, faces, positions, alpha=6.0, beta=12.0, chi=0.5, leaf_size=8, leaf_pairs=64, threads=0):
+        f, p = _mesh(faces, positions)
+        self.nv, self.nf, self.threads = p.shape[0], f.shape[0], threads
+        st = _INT(0)
+        msg = ctypes.create_string_buffer(256)
+        self._h = lib().orc_metric_build(self.nv, self.nf, _ptr(f), _ptr(p), alpha, beta, chi, leaf_size, leaf_pairs,
+                                         threads, ctypes.byref(st), msg, 256)
+        if not self._h:
+            raise OracleError(st.value, msg.value.decode())
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().orc_metric_free(h)
+            self._h = None
+
+    def apply(self, x, w_hi=1.0, w_lo=1.0):
+        """y = (w_hi B̃ + w_lo B̃⁰) x for x of shape (nb, 3 nv) or (3 nv,)."""
+        xs = np.ascontiguousarray(x, dtype=np.float64)
+        one = xs.ndim == 1
+        xs = xs.reshape(-1, 3 * self.nv)
+        ys = np.zeros_like(xs)
+        lib().orc_metric_apply(self._h, w_hi, w_lo, xs.shape[0], _ptr(xs), _ptr(ys), self.threads)
+        return ys[0] if one else ys
+
+    def stats(self):
+        o = np.zeros(7, dtype=np.int64)
+        lib().orc_metric_stats(self._h, _ptr(o))
+        keys = ["admissible_blocks", "inadmissible_blocks", "admissible_pairs", "near_entries", "nodes", "depth", "near_total"]
+        return dict(zip(keys, (int(x) for x in o)))
+
+    def ha(self):
+        out = np.zeros((self.nf, 2))
+        lib().orc_metric_ha(self._h, _ptr(out))
+        return out
+
+
+class VertexFractional:
+    """The oracle's vertex-pair L̃^V_τ (hierarchical.rs:1003 ``VertexFractionalTree``)."""
+
+    def __init__(self, positions, mass, tau, chi=0.5, leaf_size=8, leaf_pairs=64, threads=0):
+        p = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+        m = np.ascontiguousarray(mass, dtype=np.float64)
+        self.nv, self.threads = p.shape[0], threads
+        st = _INT(0)
+        self._h = lib().orc_vfrac_build(self.nv, _ptr(p), _ptr(m), tau, chi, leaf_size, leaf_pairs, threads, ctypes.byref(st))
+        if not self._h:
+            raise OracleError(st.value, "vertex fractional build")
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().orc_vfrac_free(h)
+            self._h = None
+
+    def apply(self, x):
+        xs = np.ascontiguousarray(x, dtype=np.float64)
+        one = xs.ndim == 1
+        xs = xs.reshape(-1, 3 * self.nv)
+        ys = np.zeros_like(xs)
+        lib().orc_vfrac_apply(self._h, xs.shape[0], _ptr(xs), _ptr(ys), self.threads)
+        return ys[0] if one else ys
+
+
+def vertex_areas(faces, positions):
+    f, p = _mesh(faces, positions)
+    out = np.zeros(p.shape[0])
+    lib().orc_vertex_areas(p.shape[0], f.shape[0], _ptr(f), _ptr(p), _ptr(out))
+    return out

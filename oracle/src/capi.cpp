@@ -156,4 +156,77 @@ int orc_bh_energy_and_differential(void* h, double alpha, double beta, int threa
   return OK;
 }
 
+// ---- hierarchical metric ----
+
+void* orc_metric_build(int64_t nv, int64_t nf, const uint32_t* faces, const double* pos, double alpha, double beta,
+                       double chi, int64_t leaf_size, int64_t leaf_pairs, int threads, int* status, char* msg,
+                       int64_t msglen) {
+  Error err;
+  Exponents e;
+  if (!make_exps(alpha, beta, &e, &err)) {
+    *status = fail(err, msg, msglen, nullptr);
+    return nullptr;
+  }
+  auto* fm = new FaceMetric();
+  if (!face_metric_build(make_mesh(nv, nf, faces), make_pos(nv, pos), e, chi, (size_t)leaf_size, (size_t)leaf_pairs,
+                         threads, fm, &err)) {
+    delete fm;
+    *status = fail(err, msg, msglen, nullptr);
+    return nullptr;
+  }
+  *status = OK;
+  return fm;
+}
+
+void orc_metric_free(void* h) { delete static_cast<FaceMetric*>(h); }
+
+void orc_metric_apply(void* h, double w_hi, double w_lo, int64_t nb, const double* xs, double* ys, int threads) {
+  face_metric_apply(*static_cast<FaceMetric*>(h), w_hi, w_lo, (size_t)nb, xs, ys, threads);
+}
+
+// [admissible_blocks, inadmissible_blocks, admissible_pairs, near_entries, nodes, depth, near_total]
+void orc_metric_stats(void* h, int64_t* out) {
+  const auto& s = static_cast<FaceMetric*>(h)->bct.stats;
+  out[0] = (int64_t)s.admissible_blocks;
+  out[1] = (int64_t)s.inadmissible_blocks;
+  out[2] = (int64_t)s.admissible_pairs;
+  out[3] = (int64_t)s.near_entries;
+  out[4] = (int64_t)s.nodes;
+  out[5] = (int64_t)s.depth;
+  out[6] = (int64_t)static_cast<FaceMetric*>(h)->bct.near_total;
+}
+
+// H̃a per face (2 per face: hi, lo), original order.
+void orc_metric_ha(void* h, double* out) {
+  auto* fm = static_cast<FaceMetric*>(h);
+  std::memcpy(out, fm->ha.data(), fm->ha.size() * 8);
+}
+
+void* orc_vfrac_build(int64_t nv, const double* pos, const double* mass, double tau, double chi, int64_t leaf_size,
+                      int64_t leaf_pairs, int threads, int* status) {
+  Error err;
+  auto* vf = new VertexFractional();
+  std::vector<double> m(mass, mass + nv);
+  if (!vertex_fractional_build(make_pos(nv, pos), m, tau, chi, (size_t)leaf_size, (size_t)leaf_pairs, threads, vf, &err)) {
+    delete vf;
+    *status = err.status;
+    return nullptr;
+  }
+  *status = OK;
+  return vf;
+}
+
+void orc_vfrac_free(void* h) { delete static_cast<VertexFractional*>(h); }
+
+void orc_vfrac_apply(void* h, int64_t nb, const double* xs, double* ys, int threads) {
+  vertex_fractional_apply(*static_cast<VertexFractional*>(h), (size_t)nb, xs, ys, threads);
+}
+
+void orc_vertex_areas(int64_t nv, int64_t nf, const uint32_t* faces, const double* pos, double* out) {
+  Mesh m = make_mesh(nv, nf, faces);
+  FaceGeometry g = face_geometry(m, make_pos(nv, pos));
+  auto va = vertex_areas(m, g.areas);
+  std::memcpy(out, va.data(), va.size() * 8);
+}
+
 }  // extern "C"

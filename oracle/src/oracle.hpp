@@ -150,4 +150,67 @@ FacePartials bh_face_partials(const BhTree& bh, const Exponents& e, int threads)
 bool bh_energy_and_differential(const BhTree& bh, const Exponents& e, int threads, double* energy,
                                 std::vector<V3>* diff, Error* err);
 
+// ---------------------------------------------------------------------------------------------
+// Block cluster tree and hierarchical products (spatial/block_tree.rs, metric/hierarchical.rs)
+
+struct ClusterKernelFns {
+  std::function<bool(size_t, size_t)> admissible;
+  std::function<void(size_t, size_t, double*)> far;   // N values
+  std::function<void(size_t, size_t, double*)> near;  // N values
+  std::function<double(size_t)> weight;
+};
+
+struct BctStats {
+  uint64_t admissible_blocks = 0, inadmissible_blocks = 0, admissible_pairs = 0, near_entries = 0, nodes = 0,
+           depth = 0;
+};
+
+struct BlockTree {
+  int N = 1;
+  std::vector<uint32_t> start, end, right, order, position, leaves, chain;
+  std::vector<uint64_t> chain_ptr;
+  std::vector<double> weight;
+  std::vector<uint64_t> far_ptr;
+  std::vector<uint32_t> far_col;
+  std::vector<double> far_coef;
+  std::vector<uint64_t> near_ptr, near_off;
+  std::vector<uint32_t> near_col;
+  std::vector<double> near_val;
+  uint64_t near_total = 0;
+  BctStats stats;
+};
+
+BlockTree block_tree_build(const Bvh& bvh, const ClusterKernelFns& K, int N, size_t leaf_pairs, bool fill);
+std::vector<double> block_tree_multiply_permuted(const BlockTree& t, const std::vector<int>& map,
+                                                 const std::vector<double>& psi, int threads);
+std::vector<double> block_tree_multiply(const BlockTree& t, const std::vector<int>& map, const std::vector<double>& psi,
+                                        int threads);
+std::array<V3, 3> df_coefficients(const std::array<V3, 3>& p);
+
+struct FaceMetric {
+  Mesh mesh;
+  FaceGeometry geom;
+  FaceTree tree;
+  std::vector<std::array<V3, 3>> df;
+  BlockTree bct;
+  std::vector<double> ha, ha_perm;  // 2 per face
+  std::vector<uint64_t> inc_ptr;
+  std::vector<std::pair<uint32_t, uint8_t>> inc;
+};
+
+bool face_metric_build(const Mesh& m, const Positions& p, const Exponents& e, double chi, size_t leaf_size,
+                       size_t leaf_pairs, int threads, FaceMetric* out, Error* err);
+void face_metric_apply(const FaceMetric& fm, double w_hi, double w_lo, size_t nb, const double* xs, double* ys,
+                       int threads);
+
+struct VertexFractional {
+  Bvh bvh;
+  BlockTree bct;
+  std::vector<double> mass, ha;
+};
+
+bool vertex_fractional_build(const Positions& p, const std::vector<double>& mass, double tau, double chi,
+                             size_t leaf_size, size_t leaf_pairs, int threads, VertexFractional* out, Error* err);
+void vertex_fractional_apply(const VertexFractional& vf, size_t nb, const double* xs, double* ys, int threads);
+
 }  // namespace orc
