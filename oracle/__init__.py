@@ -98,6 +98,25 @@ def _declare(L):
     L.orc_vfrac_free.argtypes = [_P]
     L.orc_vfrac_apply.argtypes = [_P, _I64, _P, _P, _INT]
     L.orc_vertex_areas.argtypes = [_I64, _I64, _P, _P, _P]
+    L.orc_saddle_build.argtypes = [_I64, _I64, _P, _P, _D, _D, _D, _I64, _I64, _D, _INT, ctypes.POINTER(_INT)]
+    L.orc_saddle_build.restype = _P
+    L.orc_saddle_free.argtypes = [_P]
+    L.orc_saddle_eps.argtypes = [_P]
+    L.orc_saddle_eps.restype = _D
+    L.orc_saddle_frac_scale.argtypes = [_P]
+    L.orc_saddle_frac_scale.restype = _D
+    L.orc_saddle_apply.argtypes = [_P, _P, _P, _INT]
+    L.orc_precond_apply.argtypes = [_P, _P, _P, _INT]
+    L.orc_lprime_solve.argtypes = [_P, _P, _P]
+    L.orc_saddle_solve.argtypes = [_P, _P, _P, _I64, _D, _I64, _INT, _P]
+    L.orc_session_create.argtypes = [_I64, _I64, _P, _P, _D, _D, _D, _D, _I64, _D, _I64, ctypes.POINTER(_INT)]
+    L.orc_session_create.restype = _P
+    L.orc_session_free.argtypes = [_P]
+    L.orc_session_step.argtypes = [_P, _INT, _P]
+    L.orc_session_step.restype = _INT
+    L.orc_session_positions.argtypes = [_P, _P]
+    L.orc_fractional_symbol_ratio.argtypes = [_D]
+    L.orc_fractional_symbol_ratio.restype = _D
 
 
 def _ptr(a: np.ndarray):
@@ -267,3 +286,95 @@ def vertex_areas(faces, positions):
     out = np.zeros(p.shape[0])
     lib().orc_vertex_areas(p.shape[0], f.shape[0], _ptr(f), _ptr(p), _ptr(out))
     return out
+
+
+class Saddle:
+    """The oracle's hierarchical saddle A′ with the augmented vertex-pair preconditioner and a global
+    barycenter row (solve/iterative.rs:126, solve/precond.rs:645)."""
+
+    def __init__(self, faces, positions, alpha=6.0, beta=12.0, chi=0.5, leaf_size=8, leaf_pairs=64, eps_rel=1e-10,
+                 threads=0):
+        f, p = _mesh(faces, positions)
+        self.nv, self.threads = p.shape[0], threads
+        self.n = 3 * self.nv + 3
+        st = _INT(0)
+        self._h = lib().orc_saddle_build(self.nv, f.shape[0], _ptr(f), _ptr(p), alpha, beta, chi, leaf_size, leaf_pairs,
+                                         eps_rel, threads, ctypes.byref(st))
+        if not self._h:
+            raise OracleError(st.value, "saddle build")
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().orc_saddle_free(h)
+            self._h = None
+
+    @property
+    def eps(self):
+        return lib().orc_saddle_eps(self._h)
+
+    @property
+    def frac_scale(self):
+        return lib().orc_saddle_frac_scale(self._h)
+
+    def _vec(self, fn, x, *extra):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(self.n)
+        fn(self._h, _ptr(x), _ptr(y), *extra)
+        return y
+
+    def apply(self, x):
+        return self._vec(lib().orc_saddle_apply, x, self.threads)
+
+    def precond(self, r):
+        return self._vec(lib().orc_precond_apply, r, self.threads)
+
+    def lprime_solve(self, r):
+        return self._vec(lib().orc_lprime_solve, r)
+
+    def solve(self, rhs, restart=50, tol=1e-6, maxit=200):
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        x = np.zeros(self.n)
+        rep = np.zeros(6)
+        lib().orc_saddle_solve(self._h, _ptr(rhs), _ptr(x), restart, tol, maxit, self.threads, _ptr(rep))
+        keys = ["iterations", "cycles", "converged", "relative_residual", "operator_applications",
+                "preconditioner_applications"]
+        return x, dict(zip(keys, rep.tolist()))
+
+
+class Session:
+    """The oracle's Hs descent session (step/session.rs:280) with Barnes–Hut energy, the hierarchical
+    saddle, a global barycenter constraint, remeshing and CCD off."""
+
+    def __init__(self, faces, positions, alpha=6.0, beta=12.0, theta=0.5, chi=0.5, restart=50, tol=1e-6, maxit=200):
+        f, p = _mesh(faces, positions)
+        self.nv = p.shape[0]
+        st = _INT(0)
+        self._h = lib().orc_session_create(self.nv, f.shape[0], _ptr(f), _ptr(p), alpha, beta, theta, chi, restart, tol,
+                                           maxit, ctypes.byref(st))
+        if not self._h:
+            raise OracleError(st.value, "session")
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().orc_session_free(h)
+            self._h = None
+
+    def step(self, threads=0):
+        rep = np.zeros(9)
+        st = lib().orc_session_step(self._h, threads, _ptr(rep))
+        if st != 0:
+            raise OracleError(st, "step")
+        keys = ["energy0", "energy1", "slope", "tau", "accepted", "trials", "gmres_iterations", "gmres_converged",
+                "gmres_relative_residual"]
+        return dict(zip(keys, rep.tolist()))
+
+    def positions(self):
+        out = np.zeros((self.nv, 3))
+        lib().orc_session_positions(self._h, _ptr(out))
+        return out
+
+
+def fractional_symbol_ratio(s):
+    return lib().orc_fractional_symbol_ratio(s)

@@ -229,4 +229,122 @@ void orc_vertex_areas(int64_t nv, int64_t nf, const uint32_t* faces, const doubl
   std::memcpy(out, va.data(), va.size() * 8);
 }
 
+// ---- saddle solver, preconditioner, GMRES, descent step ----
+
+void* orc_saddle_build(int64_t nv, int64_t nf, const uint32_t* faces, const double* pos, double alpha, double beta,
+                       double chi, int64_t leaf_size, int64_t leaf_pairs, double eps_rel, int threads, int* status) {
+  Error err;
+  Exponents e;
+  if (!make_exps(alpha, beta, &e, &err)) {
+    *status = err.status;
+    return nullptr;
+  }
+  SaddleOptions o;
+  o.chi = chi;
+  o.leaf_size = (size_t)leaf_size;
+  o.leaf_pairs = (size_t)leaf_pairs;
+  o.eps_rel = eps_rel;
+  auto* sd = new Saddle();
+  if (!saddle_build(make_mesh(nv, nf, faces), make_pos(nv, pos), e, o, threads, sd, &err)) {
+    delete sd;
+    *status = err.status;
+    return nullptr;
+  }
+  *status = OK;
+  return sd;
+}
+
+void orc_saddle_free(void* h) { delete static_cast<Saddle*>(h); }
+double orc_saddle_eps(void* h) { return static_cast<Saddle*>(h)->eps; }
+double orc_saddle_frac_scale(void* h) { return static_cast<Saddle*>(h)->frac_scale; }
+void orc_saddle_apply(void* h, const double* x, double* y, int threads) { saddle_apply(*static_cast<Saddle*>(h), x, y, threads); }
+void orc_precond_apply(void* h, const double* r, double* z, int threads) { precond_apply(*static_cast<Saddle*>(h), r, z, threads); }
+
+// x = L′⁻¹ r for a full vector (3 nv + 3): the scalar saddle per axis.
+void orc_lprime_solve(void* h, const double* r, double* x) {
+  auto* sd = static_cast<Saddle*>(h);
+  const size_t nv = sd->nv, nd = 3 * nv;
+  std::vector<double> b(nv + 1), y(nv + 1);
+  for (int k = 0; k < 3; ++k) {
+    for (size_t v = 0; v < nv; ++v) b[v] = r[3 * v + k];
+    b[nv] = r[nd + k];
+    ldlt_solve(sd->lprime, b.data(), y.data());
+    for (size_t v = 0; v < nv; ++v) x[3 * v + k] = y[v];
+    x[nd + k] = y[nv];
+  }
+}
+
+// report: [iterations, cycles, converged, relative residual, operator applications, preconditioner applications]
+void orc_saddle_solve(void* h, const double* rhs, double* x, int64_t restart, double tol, int64_t maxit, int threads,
+                      double* report) {
+  auto* sd = static_cast<Saddle*>(h);
+  const size_t n = 3 * sd->nv + 3;
+  std::vector<double> b(rhs, rhs + n), sol;
+  GmresOptions o;
+  o.restart = (size_t)restart;
+  o.tolerance = tol;
+  o.max_iterations = (size_t)maxit;
+  auto op = [&](const double* a, double* y) { saddle_apply(*sd, a, y, threads); };
+  auto pc = [&](const double* r, double* z) { precond_apply(*sd, r, z, threads); };
+  GmresReport rep = gmres_right(op, pc, b, o, &sol);
+  std::memcpy(x, sol.data(), n * 8);
+  report[0] = (double)rep.iterations;
+  report[1] = (double)rep.cycles;
+  report[2] = rep.converged ? 1.0 : 0.0;
+  report[3] = rep.relative_residual;
+  report[4] = (double)rep.operator_applications;
+  report[5] = (double)rep.preconditioner_applications;
+}
+
+void* orc_session_create(int64_t nv, int64_t nf, const uint32_t* faces, const double* pos, double alpha, double beta,
+                         double theta, double chi, int64_t restart, double tol, int64_t maxit, int* status) {
+  Error err;
+  Exponents e;
+  if (!make_exps(alpha, beta, &e, &err)) {
+    *status = err.status;
+    return nullptr;
+  }
+  SessionOptions o;
+  o.theta = theta;
+  o.saddle.chi = chi;
+  o.gmres.restart = (size_t)restart;
+  o.gmres.tolerance = tol;
+  o.gmres.max_iterations = (size_t)maxit;
+  auto* s = new Session();
+  if (!session_create(make_mesh(nv, nf, faces), make_pos(nv, pos), e, o, s, &err)) {
+    delete s;
+    *status = err.status;
+    return nullptr;
+  }
+  *status = OK;
+  return s;
+}
+
+void orc_session_free(void* h) { delete static_cast<Session*>(h); }
+
+// report: [energy0, energy1, slope, tau, accepted, trials, gmres iterations, gmres converged, gmres relres]
+int orc_session_step(void* h, int threads, double* report) {
+  Error err;
+  StepReport rep;
+  if (!session_step(*static_cast<Session*>(h), threads, &rep, &err)) return err.status;
+  report[0] = rep.energy0;
+  report[1] = rep.energy1;
+  report[2] = rep.slope;
+  report[3] = rep.tau;
+  report[4] = rep.accepted ? 1.0 : 0.0;
+  report[5] = (double)rep.trials;
+  report[6] = (double)rep.gmres.iterations;
+  report[7] = rep.gmres.converged ? 1.0 : 0.0;
+  report[8] = rep.gmres.relative_residual;
+  return OK;
+}
+
+void orc_session_positions(void* h, double* out) {
+  auto* s = static_cast<Session*>(h);
+  for (size_t v = 0; v < s->positions.size(); ++v)
+    for (int k = 0; k < 3; ++k) out[3 * v + k] = s->positions[v][k];
+}
+
+double orc_fractional_symbol_ratio(double s) { return fractional_symbol_ratio(s); }
+
 }  // extern "C"

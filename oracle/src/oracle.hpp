@@ -213,4 +213,93 @@ bool vertex_fractional_build(const Positions& p, const std::vector<double>& mass
                              size_t leaf_size, size_t leaf_pairs, int threads, VertexFractional* out, Error* err);
 void vertex_fractional_apply(const VertexFractional& vf, size_t nb, const double* xs, double* ys, int threads);
 
+// ---------------------------------------------------------------------------------------------
+// Saddle solver and descent step (solve/{precond,gmres,iterative}.rs, step/session.rs)
+
+struct Triplet {
+  uint32_t r, c;
+  double v;
+};
+struct Csr {
+  size_t nrows = 0, ncols = 0;
+  std::vector<size_t> ptr;
+  std::vector<uint32_t> idx;
+  std::vector<double> val;
+};
+Csr csr_from_triplets(size_t nrows, size_t ncols, const std::vector<Triplet>& t);
+std::vector<Triplet> cotan_triplets(const Mesh& m, const Positions& p);
+double mean_edge_length(const Mesh& m, const Positions& p);
+std::vector<uint32_t> nested_dissection(const Csr& k, const Positions& p);
+
+struct Ldlt {
+  size_t n = 0;
+  std::vector<uint32_t> perm;  // perm[new] = old
+  std::vector<int64_t> parent;
+  std::vector<uint64_t> lp;
+  std::vector<uint32_t> li;
+  std::vector<double> lx, d;
+};
+bool ldlt_factor(const Csr& a, const std::vector<uint32_t>& perm, Ldlt* f, Error* err);
+void ldlt_solve(const Ldlt& f, const double* b, double* x);
+double fractional_symbol_ratio(double s);
+
+struct SaddleOptions {
+  double chi = 0.5;
+  size_t leaf_size = 8, leaf_pairs = 64;
+  double w_tp = 1.0, c_lo = 1.0;
+  double eps_rel = 1e-10;
+};
+
+struct Saddle {
+  size_t nv = 0;
+  double w_tp = 1.0, c_lo = 1.0, eps = 0.0, frac_scale = 1.0;
+  FaceMetric metric;
+  std::vector<double> mass, crow;
+  Csr k, lprime_matrix;
+  Ldlt lprime;
+  VertexFractional frac;
+};
+bool saddle_build(const Mesh& m, const Positions& p, const Exponents& e, const SaddleOptions& o, int threads,
+                  Saddle* out, Error* err);
+void saddle_apply(const Saddle& sd, const double* x, double* y, int threads);
+void precond_apply(const Saddle& sd, const double* r, double* z, int threads);
+
+struct GmresOptions {
+  size_t restart = 50;
+  double tolerance = 1e-6;
+  size_t max_iterations = 200;
+};
+struct GmresReport {
+  size_t iterations = 0, cycles = 0, operator_applications = 0, preconditioner_applications = 0;
+  bool converged = false;
+  double relative_residual = 0.0;
+  std::vector<double> cycle_residuals, history;
+};
+GmresReport gmres_right(const std::function<void(const double*, double*)>& op,
+                        const std::function<void(const double*, double*)>& prec, const std::vector<double>& b,
+                        const GmresOptions& o, std::vector<double>* x);
+
+struct SessionOptions {
+  double theta = 0.5;
+  SaddleOptions saddle;
+  GmresOptions gmres;
+};
+struct Session {
+  Mesh mesh;
+  Positions positions;
+  Exponents exps;
+  SessionOptions opts;
+  double l0 = 0.0, last_tau = 0.0;
+  V3 barycenter_target{0, 0, 0};
+};
+struct StepReport {
+  double energy0 = 0.0, energy1 = 0.0, slope = 0.0, tau = 0.0;
+  bool accepted = false;
+  size_t trials = 0;
+  GmresReport gmres;
+};
+bool session_create(const Mesh& m, const Positions& p, const Exponents& e, const SessionOptions& o, Session* s,
+                    Error* err);
+bool session_step(Session& s, int threads, StepReport* rep, Error* err);
+
 }  // namespace orc

@@ -1,0 +1,542 @@
+// FP64 oracle — TEST INFRASTRUCTURE ONLY: the Hs saddle solver and one descent step.
+//
+// Restates:
+//   crates/ddg-repulsive-surfaces/src/geometry.rs:221 cotan_stiffness, :194 vertex_areas,
+//       mean_edge_length
+//   crates/ddg-repulsive-surfaces/src/constraints/mod.rs:556 jacobian (the barycenter rows)
+//   crates/ddg-repulsive-surfaces/src/solve/dense.rs:529 preconditioner_epsilon
+//   crates/ddg-repulsive-surfaces/src/solve/precond.rs  LaplacianSaddle :157 (per-axis scalar
+//       saddle, multipliers last), fractional_symbol_ratio :561, SparsePreconditioner::apply_fine
+//       :1026 (augmented form, Symbol scale, vertex pairs, no refinement)
+//   crates/ddg-repulsive-surfaces/src/solve/iterative.rs:284 HierarchicalOperator::apply_batch
+//   crates/ddg-repulsive-surfaces/src/solve/gmres.rs:311-583 GmresRun (right preconditioning,
+//       x0 = 0, MGS fused with the next dot in chunks of 4096, Givens, explicit residual per cycle)
+//   crates/ddg-repulsive-surfaces/src/step/session.rs:749 step_inner (energy, direction, Armijo
+//       line search with the GrowPrevious / MaxDisplacement start :1604, barycenter projection),
+//       with remeshing and CCD off.
+//
+// The L′ factor is this oracle's own: a nested-dissection order by coordinate bisection and the
+// up-looking LDLᵀ of T. Davis ("Algorithm 849: A concise sparse Cholesky factorization package",
+// ACM TOMS 31(4), 2005).  The reference factors with faer 0.24.4 (AMD order, not vendored); the
+// factor is the same matrix factorization up to round-off, which the Lean `precond` fixtures pin
+// at 1e-10 (tests/oracle.rs:45-48).
+#include <algorithm>
+#include <numeric>
+
+#include "oracle.hpp"
+
+namespace orc {
+
+Csr csr_from_triplets(size_t nrows, size_t ncols, const std::vector<Triplet>& t) {
+  std::vector<size_t> idx(t.size());
+  std::iota(idx.begin(), idx.end(), 0);
+  std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
+    return t[a].r != t[b].r ? t[a].r < t[b].r : t[a].c < t[b].c;
+  });
+  Csr m;
+  m.nrows = nrows;
+  m.ncols = ncols;
+  m.ptr.assign(nrows + 1, 0);
+  for (size_t q = 0; q < idx.size();) {
+    const Triplet& a = t[idx[q]];
+    double v = 0.0;
+    size_t q2 = q;
+    while (q2 < idx.size() && t[idx[q2]].r == a.r && t[idx[q2]].c == a.c) v += t[idx[q2++]].v;
+    m.idx.push_back(a.c);
+    m.val.push_back(v);
+    m.ptr[a.r + 1]++;
+    q = q2;
+  }
+  for (size_t i = 0; i < nrows; ++i) m.ptr[i + 1] += m.ptr[i];
+  return m;
+}
+
+std::vector<Triplet> cotan_triplets(const Mesh& m, const Positions& p) {
+  std::vector<Triplet> trip;
+  trip.reserve(12 * m.faces.size());
+  for (auto& t : m.faces) {
+    auto q = corners(p, t);
+    for (int c = 0; c < 3; ++c) {
+      uint32_t a = t[(c + 1) % 3], b = t[(c + 2) % 3];
+      V3 u = sub(q[(c + 1) % 3], q[c]);
+      V3 w = sub(q[(c + 2) % 3], q[c]);
+      double cot = dot(u, w) / norm(cross(u, w));
+      double h = 0.5 * cot;
+      trip.push_back({a, b, -h});
+      trip.push_back({b, a, -h});
+      trip.push_back({a, a, h});
+      trip.push_back({b, b, h});
+    }
+  }
+  return trip;
+}
+
+double mean_edge_length(const Mesh& m, const Positions& p) {
+  std::vector<std::pair<uint32_t, uint32_t>> e;
+  for (auto& t : m.faces)
+    for (int c = 0; c < 3; ++c) {
+      uint32_t a = t[c], b = t[(c + 1) % 3];
+      e.push_back({std::min(a, b), std::max(a, b)});
+    }
+  std::sort(e.begin(), e.end());
+  e.erase(std::unique(e.begin(), e.end()), e.end());
+  double s = 0.0;
+  for (auto [a, b] : e) s += norm(sub(p[a], p[b]));
+  return s / (double)e.size();
+}
+
+// Nested dissection by coordinate bisection: domains ordered before their separator.
+static void nd_order(const std::vector<uint32_t>& verts, const Positions& p, const Csr& g,
+                     std::vector<int>& mark, int tag, std::vector<uint32_t>& out) {
+  if (verts.size() <= 64) {
+    out.insert(out.end(), verts.begin(), verts.end());
+    return;
+  }
+  V3 lo{INFINITY, INFINITY, INFINITY}, hi{-INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t v : verts)
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = std::min(lo[k], p[v][k]);
+      hi[k] = std::max(hi[k], p[v][k]);
+    }
+  int ax = 0;
+  for (int k = 1; k < 3; ++k)
+    if (hi[k] - lo[k] > hi[ax] - lo[ax]) ax = k;
+  std::vector<uint32_t> s = verts;
+  std::sort(s.begin(), s.end(), [&](uint32_t a, uint32_t b) {
+    return p[a][ax] != p[b][ax] ? p[a][ax] < p[b][ax] : a < b;
+  });
+  size_t half = s.size() / 2;
+  // side: tag for A, tag+1 for B
+  for (size_t i = 0; i < s.size(); ++i) mark[s[i]] = i < half ? tag : tag + 1;
+  std::vector<uint32_t> A, B, S;
+  for (size_t i = 0; i < half; ++i) {
+    uint32_t v = s[i];
+    bool touches = false;
+    for (size_t q = g.ptr[v]; q < g.ptr[v + 1]; ++q)
+      if (mark[g.idx[q]] == tag + 1) touches = true;
+    (touches ? S : A).push_back(v);
+  }
+  for (size_t i = half; i < s.size(); ++i) B.push_back(s[i]);
+  for (uint32_t v : S) mark[v] = -1;
+  nd_order(A, p, g, mark, tag + 2, out);
+  nd_order(B, p, g, mark, tag + 2, out);
+  out.insert(out.end(), S.begin(), S.end());
+}
+
+std::vector<uint32_t> nested_dissection(const Csr& k, const Positions& p) {
+  std::vector<uint32_t> all(k.nrows);
+  std::iota(all.begin(), all.end(), 0);
+  std::vector<int> mark(k.nrows, -1);
+  std::vector<uint32_t> out;
+  out.reserve(k.nrows);
+  nd_order(all, p, k, mark, 0, out);
+  return out;
+}
+
+bool ldlt_factor(const Csr& a, const std::vector<uint32_t>& perm, Ldlt* f, Error* err) {
+  // `a` symmetric with both triangles stored (CSR rows = CSC columns).
+  const size_t n = a.nrows;
+  f->n = n;
+  f->perm = perm;
+  std::vector<uint32_t> pinv(n);
+  for (size_t i = 0; i < n; ++i) pinv[perm[i]] = (uint32_t)i;
+  std::vector<int64_t> parent(n), flag(n);
+  std::vector<uint64_t> lnz(n);
+  for (size_t k = 0; k < n; ++k) {
+    parent[k] = -1;
+    flag[k] = (int64_t)k;
+    lnz[k] = 0;
+    size_t kk = perm[k];
+    for (size_t q = a.ptr[kk]; q < a.ptr[kk + 1]; ++q) {
+      size_t i = pinv[a.idx[q]];
+      if (i < k)
+        for (; flag[i] != (int64_t)k; i = (size_t)parent[i]) {
+          if (parent[i] == -1) parent[i] = (int64_t)k;
+          lnz[i]++;
+          flag[i] = (int64_t)k;
+        }
+    }
+  }
+  f->lp.assign(n + 1, 0);
+  for (size_t k = 0; k < n; ++k) f->lp[k + 1] = f->lp[k] + lnz[k];
+  f->li.assign(f->lp[n], 0);
+  f->lx.assign(f->lp[n], 0.0);
+  f->d.assign(n, 0.0);
+  std::vector<double> y(n, 0.0);
+  std::vector<size_t> pattern(n);
+  for (size_t k = 0; k < n; ++k) {
+    y[k] = 0.0;
+    size_t top = n;
+    flag[k] = (int64_t)k;
+    lnz[k] = 0;
+    size_t kk = perm[k];
+    for (size_t q = a.ptr[kk]; q < a.ptr[kk + 1]; ++q) {
+      size_t i = pinv[a.idx[q]];
+      if (i <= k) {
+        y[i] += a.val[q];
+        size_t len = 0;
+        for (; flag[i] != (int64_t)k; i = (size_t)parent[i]) {
+          pattern[len++] = i;
+          flag[i] = (int64_t)k;
+        }
+        while (len > 0) pattern[--top] = pattern[--len];
+      }
+    }
+    f->d[k] = y[k];
+    y[k] = 0.0;
+    for (; top < n; ++top) {
+      size_t i = pattern[top];
+      double yi = y[i];
+      y[i] = 0.0;
+      uint64_t p2 = f->lp[i] + lnz[i];
+      for (uint64_t q = f->lp[i]; q < p2; ++q) y[f->li[q]] -= f->lx[q] * yi;
+      double l_ki = yi / f->d[i];
+      f->d[k] -= l_ki * yi;
+      f->li[p2] = (uint32_t)k;
+      f->lx[p2] = l_ki;
+      lnz[i]++;
+    }
+    if (f->d[k] == 0.0 || !std::isfinite(f->d[k])) {
+      err->status = SINGULAR_SADDLE;
+      err->a = (int64_t)k;
+      err->detail = "zero pivot in the L' factorization";
+      return false;
+    }
+  }
+  f->parent.assign(parent.begin(), parent.end());
+  return true;
+}
+
+// x = A⁻¹ b for one column in the original numbering.
+void ldlt_solve(const Ldlt& f, const double* b, double* x) {
+  const size_t n = f.n;
+  std::vector<double> y(n);
+  for (size_t i = 0; i < n; ++i) y[i] = b[f.perm[i]];
+  for (size_t j = 0; j < n; ++j)
+    for (uint64_t q = f.lp[j]; q < f.lp[j + 1]; ++q) y[f.li[q]] -= f.lx[q] * y[j];
+  for (size_t j = 0; j < n; ++j) y[j] /= f.d[j];
+  for (size_t j = n; j-- > 0;)
+    for (uint64_t q = f.lp[j]; q < f.lp[j + 1]; ++q) y[j] -= f.lx[q] * y[f.li[q]];
+  for (size_t i = 0; i < n; ++i) x[f.perm[i]] = y[i];
+}
+
+// precond.rs:528-564: `4 I(σ) I(2 − s)`, I(t) = π Γ(1−t) / (t 4^t Γ(1+t)).
+double fractional_symbol_ratio(double s) {
+  auto I = [](double t) { return M_PI * std::tgamma(1.0 - t) / (t * std::pow(4.0, t) * std::tgamma(1.0 + t)); };
+  return 4.0 * I(s - 1.0) * I(2.0 - s);
+}
+
+bool saddle_build(const Mesh& m, const Positions& p, const Exponents& e, const SaddleOptions& o, int threads,
+                  Saddle* out, Error* err) {
+  if (!face_metric_build(m, p, e, o.chi, o.leaf_size, o.leaf_pairs, threads, &out->metric, err)) return false;
+  const size_t nv = (size_t)m.nv;
+  out->nv = nv;
+  out->w_tp = o.w_tp;
+  out->c_lo = o.c_lo;
+  out->mass = vertex_areas(m, out->metric.geom.areas);
+  // Global barycenter rows a_v / Σ a (constraints/mod.rs:556), one scalar row shared by the axes.
+  double tot = 0.0;
+  for (size_t v = 0; v < nv; ++v) tot += out->mass[v];
+  out->crow.resize(nv);
+  for (size_t v = 0; v < nv; ++v) out->crow[v] = out->mass[v] / tot;
+  // K = W + ε M (precond.rs:705-760)
+  auto trip = cotan_triplets(m, p);
+  Csr w = csr_from_triplets(nv, nv, trip);
+  double trw = 0.0, trm = 0.0;
+  for (size_t i = 0; i < nv; ++i)
+    for (size_t q = w.ptr[i]; q < w.ptr[i + 1]; ++q)
+      if (w.idx[q] == i) trw += w.val[q];
+  for (size_t v = 0; v < nv; ++v) trm += out->mass[v];
+  out->eps = o.eps_rel * trw / trm;
+  for (size_t v = 0; v < nv; ++v) trip.push_back({(uint32_t)v, (uint32_t)v, out->eps * out->mass[v]});
+  out->k = csr_from_triplets(nv, nv, trip);
+  // L′ = [[K, c1ᵀ], [c1, 0]], order: nested dissection of K, then the multiplier.
+  std::vector<Triplet> lt = trip;
+  for (size_t v = 0; v < nv; ++v) {
+    lt.push_back({(uint32_t)nv, (uint32_t)v, out->crow[v]});
+    lt.push_back({(uint32_t)v, (uint32_t)nv, out->crow[v]});
+  }
+  Csr lp = csr_from_triplets(nv + 1, nv + 1, lt);
+  auto order = nested_dissection(out->k, p);
+  order.push_back((uint32_t)nv);
+  if (!ldlt_factor(lp, order, &out->lprime, err)) return false;
+  out->lprime_matrix = std::move(lp);
+  const double s = e.s();
+  if (!vertex_fractional_build(p, out->mass, 2.0 - s, o.chi, o.leaf_size, o.leaf_pairs, threads, &out->frac, err))
+    return false;
+  out->frac_scale = 1.0 / fractional_symbol_ratio(s) / o.w_tp;
+  return true;
+}
+
+// A′ [x; λ] (iterative.rs:284): w_TP (B̃ + c_lo B̃⁰) x + C_Qᵀ λ; C_Q x.
+void saddle_apply(const Saddle& sd, const double* x, double* y, int threads) {
+  const size_t nv = sd.nv, nd = 3 * nv;
+  face_metric_apply(sd.metric, sd.w_tp, sd.w_tp * sd.c_lo, 1, x, y, threads);
+  for (int k = 0; k < 3; ++k) {
+    double lam = x[nd + k];
+    double cx = 0.0;
+    for (size_t v = 0; v < nv; ++v) {
+      y[3 * v + k] += sd.crow[v] * lam;
+      cx += sd.crow[v] * x[3 * v + k];
+    }
+    y[nd + k] = cx;
+  }
+}
+
+// L′⁻¹ r for a full vector (3 nv + 3), the three axes as scalar saddles.
+static void lprime_solve(const Saddle& sd, const double* r, double* x) {
+  const size_t nv = sd.nv, nd = 3 * nv;
+  std::vector<double> b(nv + 1), y(nv + 1);
+  for (int k = 0; k < 3; ++k) {
+    for (size_t v = 0; v < nv; ++v) b[v] = r[3 * v + k];
+    b[nv] = r[nd + k];
+    ldlt_solve(sd.lprime, b.data(), y.data());
+    for (size_t v = 0; v < nv; ++v) x[3 * v + k] = y[v];
+    x[nd + k] = y[nv];
+  }
+}
+
+// precond.rs:1026 apply_fine, augmented form: z = L′⁻¹ v + y, y = L′⁻¹ r,
+// v = s L^V y − (K ⊗ I₃) y on the positions, 0 on the multipliers.
+void precond_apply(const Saddle& sd, const double* r, double* z, int threads) {
+  const size_t nv = sd.nv, nd = 3 * nv, n = nd + 3;
+  std::vector<double> y(n), v(n, 0.0);
+  lprime_solve(sd, r, y.data());
+  vertex_fractional_apply(sd.frac, 1, y.data(), v.data(), threads);
+  for (size_t i = 0; i < nd; ++i) v[i] *= sd.frac_scale;
+  for (size_t j = 0; j < nv; ++j) {
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (size_t q = sd.k.ptr[j]; q < sd.k.ptr[j + 1]; ++q)
+      for (int a = 0; a < 3; ++a) acc[a] += sd.k.val[q] * y[3 * (size_t)sd.k.idx[q] + a];
+    for (int a = 0; a < 3; ++a) v[3 * j + a] -= acc[a];
+  }
+  lprime_solve(sd, v.data(), z);
+  for (size_t i = 0; i < n; ++i) z[i] += y[i];
+}
+
+namespace {
+constexpr size_t kChunk = 4096;
+double dot_seq(const double* a, const double* b, size_t n) {
+  double s = 0.0;
+  for (size_t i = 0; i < n; ++i) s += a[i] * b[i];
+  return s;
+}
+double vdot(const std::vector<double>& a, const std::vector<double>& b) {
+  double s = 0.0;
+  for (size_t c = 0; c < a.size(); c += kChunk) s += dot_seq(&a[c], &b[c], std::min(kChunk, a.size() - c));
+  return s;
+}
+double vnorm(const std::vector<double>& a) { return std::sqrt(vdot(a, a)); }
+// w −= h v, then w · next (chunked)
+double axpy_dot(std::vector<double>& w, double h, const std::vector<double>& v, const std::vector<double>* next) {
+  double s = 0.0;
+  for (size_t c = 0; c < w.size(); c += kChunk) {
+    size_t e = std::min(w.size(), c + kChunk);
+    for (size_t i = c; i < e; ++i) w[i] -= h * v[i];
+    if (next) s += dot_seq(&w[c], &(*next)[c], e - c);
+  }
+  return s;
+}
+}  // namespace
+
+// gmres.rs:311-583, right preconditioning.
+GmresReport gmres_right(const std::function<void(const double*, double*)>& op,
+                        const std::function<void(const double*, double*)>& prec, const std::vector<double>& b,
+                        const GmresOptions& o, std::vector<double>* x_out) {
+  const size_t n = b.size();
+  const size_t m = std::min(o.restart, std::max<size_t>(n, 1));
+  GmresReport rep;
+  std::vector<double> x(n, 0.0);
+  double bnorm = vnorm(b);
+  if (bnorm == 0.0) {
+    rep.converged = true;
+    *x_out = x;
+    return rep;
+  }
+  std::vector<std::vector<double>> basis, h;
+  std::vector<double> cs(m, 0.0), sn(m, 0.0), g(m + 1, 0.0), r(n), tmp(n), w(n);
+  for (;;) {
+    // explicit residual
+    if (rep.cycles > 0) {
+      op(x.data(), tmp.data());
+      rep.operator_applications++;
+      for (size_t i = 0; i < n; ++i) r[i] = b[i] - tmp[i];
+    } else {
+      r = b;
+    }
+    double beta = vnorm(r);
+    rep.relative_residual = beta / bnorm;
+    double rel = beta / bnorm;
+    rep.cycle_residuals.push_back(rel);
+    if (!std::isfinite(rel)) break;
+    if (rel <= o.tolerance) {
+      rep.converged = true;
+      break;
+    }
+    if (rep.iterations >= o.max_iterations) break;
+    rep.cycles++;
+    basis.clear();
+    h.clear();
+    basis.push_back(std::vector<double>(n));
+    for (size_t i = 0; i < n; ++i) basis[0][i] = r[i] / beta;
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = beta;
+    size_t j = 0, k_end = 0;
+    for (;;) {
+      prec(basis[j].data(), tmp.data());
+      op(tmp.data(), w.data());
+      rep.operator_applications++;
+      rep.preconditioner_applications++;
+      std::vector<double> col(j + 2, 0.0);
+      double hij = vdot(w, basis[0]);
+      for (size_t i = 0; i <= j; ++i) {
+        col[i] = hij;
+        hij = axpy_dot(w, col[i], basis[i], i < j ? &basis[i + 1] : nullptr);
+      }
+      double hn = vnorm(w);
+      col[j + 1] = hn;
+      for (size_t i = 0; i < j; ++i) {
+        double t = cs[i] * col[i] + sn[i] * col[i + 1];
+        col[i + 1] = -sn[i] * col[i] + cs[i] * col[i + 1];
+        col[i] = t;
+      }
+      double denom = std::hypot(col[j], col[j + 1]);
+      if (denom == 0.0) {
+        cs[j] = 1.0;
+        sn[j] = 0.0;
+      } else {
+        cs[j] = col[j] / denom;
+        sn[j] = col[j + 1] / denom;
+      }
+      col[j] = denom;
+      col[j + 1] = 0.0;
+      g[j + 1] = -sn[j] * g[j];
+      g[j] *= cs[j];
+      h.push_back(col);
+      rep.iterations++;
+      double est = std::fabs(g[j + 1]) / bnorm;
+      rep.history.push_back(est);
+      if (hn == 0.0 || !std::isfinite(hn) || est <= o.tolerance || rep.iterations >= o.max_iterations || j + 1 >= m) {
+        k_end = j + 1;
+        break;
+      }
+      basis.push_back(std::vector<double>(n));
+      for (size_t i = 0; i < n; ++i) basis[j + 1][i] = w[i] / hn;
+      j++;
+    }
+    std::vector<double> y(k_end, 0.0);
+    for (size_t i = k_end; i-- > 0;) {
+      double s = g[i];
+      for (size_t l = i + 1; l < k_end; ++l) s -= h[l][i] * y[l];
+      y[i] = h[i][i] != 0.0 ? s / h[i][i] : 0.0;
+    }
+    std::vector<double> u(n, 0.0);
+    for (size_t l = 0; l < k_end; ++l)
+      for (size_t i = 0; i < n; ++i) u[i] += y[l] * basis[l][i];
+    prec(u.data(), tmp.data());
+    rep.preconditioner_applications++;
+    for (size_t i = 0; i < n; ++i) x[i] += tmp[i];
+  }
+  *x_out = std::move(x);
+  return rep;
+}
+
+bool session_step(Session& s, int threads, StepReport* rep, Error* err) {
+  const Mesh& m = s.mesh;
+  const size_t nv = (size_t)m.nv, nd = 3 * nv;
+  // 1. trees, lists, F and dF (Barnes–Hut; objective.rs:71,151)
+  BhTree bh;
+  if (!bh_build(m, s.positions, s.opts.theta, s.opts.saddle.leaf_size, threads, &bh, err)) return false;
+  double f0;
+  std::vector<V3> gv;
+  if (!bh_energy_and_differential(bh, s.exps, threads, &f0, &gv, err)) return false;
+  std::vector<double> g(nd);
+  for (size_t v = 0; v < nv; ++v)
+    for (int k = 0; k < 3; ++k) g[3 * v + k] = gv[v][k];
+  // 2. the solver and the direction x = A′⁻¹ [g; 0]
+  Saddle sd;
+  if (!saddle_build(m, s.positions, s.exps, s.opts.saddle, threads, &sd, err)) return false;
+  std::vector<double> rhs(nd + 3, 0.0), sol;
+  std::copy(g.begin(), g.end(), rhs.begin());
+  auto op = [&](const double* x, double* y) { saddle_apply(sd, x, y, threads); };
+  auto pc = [&](const double* r, double* z) { precond_apply(sd, r, z, threads); };
+  rep->gmres = gmres_right(op, pc, rhs, s.opts.gmres, &sol);
+  if (!rep->gmres.converged && !(rep->gmres.relative_residual <= 1e-1)) {
+    err->status = SOLVER_DIVERGED;
+    err->detail = "GMRES diverged";
+    return false;
+  }
+  std::vector<double> x(sol.begin(), sol.begin() + nd);
+  double slope = 0.0, xinf = 0.0, xn2 = 0.0, maxd = 0.0;
+  for (size_t i = 0; i < nd; ++i) {
+    slope += g[i] * x[i];
+    xinf = std::max(xinf, std::fabs(x[i]));
+    xn2 += x[i] * x[i];
+  }
+  for (size_t v = 0; v < nv; ++v)
+    maxd = std::max(maxd, std::sqrt(x[3 * v] * x[3 * v] + x[3 * v + 1] * x[3 * v + 1] + x[3 * v + 2] * x[3 * v + 2]));
+  // 3. Armijo line search (session.rs:991-1069; StartRule::GrowPrevious{2, MaxDisplacement{1}})
+  double tau0 = s.last_tau > 0.0 ? 2.0 * s.last_tau : s.l0 / maxd;
+  double tau_min = 1e-12 * s.l0 / xinf;
+  rep->energy0 = f0;
+  rep->slope = slope;
+  rep->accepted = false;
+  rep->trials = 0;
+  Positions trial(nv);
+  double tau = tau0;
+  if (slope > 0.0 && std::isfinite(tau0) && tau0 > 0.0) {
+    while (tau >= tau_min) {
+      for (size_t v = 0; v < nv; ++v)
+        for (int k = 0; k < 3; ++k) trial[v][k] = s.positions[v][k] - tau * x[3 * v + k];
+      Error e2;
+      double ft = NAN;
+      if (bh_refit(bh, trial, &e2)) ft = bh_energy(bh, s.exps, threads);
+      rep->trials++;
+      if (std::isfinite(ft) && ft <= f0 - 1e-2 * tau * slope) {
+        rep->accepted = true;
+        rep->energy1 = ft;
+        break;
+      }
+      tau *= 0.5;
+    }
+  }
+  // 4. projection: translate to the target barycenter (areas at the new positions)
+  Positions np = rep->accepted ? trial : s.positions;
+  FaceGeometry gg = face_geometry(m, np);
+  auto va = vertex_areas(m, gg.areas);
+  double tot = 0.0;
+  V3 bc{0, 0, 0};
+  for (size_t v = 0; v < nv; ++v) {
+    tot += va[v];
+    for (int k = 0; k < 3; ++k) bc[k] += va[v] * np[v][k];
+  }
+  for (int k = 0; k < 3; ++k) bc[k] /= tot;
+  for (size_t v = 0; v < nv; ++v)
+    for (int k = 0; k < 3; ++k) np[v][k] += s.barycenter_target[k] - bc[k];
+  s.positions = np;
+  if (rep->accepted) s.last_tau = tau;
+  rep->tau = rep->accepted ? tau : 0.0;
+  return true;
+}
+
+bool session_create(const Mesh& m, const Positions& p, const Exponents& e, const SessionOptions& o, Session* s,
+                    Error* err) {
+  if (!validate_positions(m, p, err)) return false;
+  s->mesh = m;
+  s->positions = p;
+  s->exps = e;
+  s->opts = o;
+  s->l0 = mean_edge_length(m, p);
+  FaceGeometry gg = face_geometry(m, p);
+  auto va = vertex_areas(m, gg.areas);
+  double tot = 0.0;
+  V3 bc{0, 0, 0};
+  for (size_t v = 0; v < (size_t)m.nv; ++v) {
+    tot += va[v];
+    for (int k = 0; k < 3; ++k) bc[k] += va[v] * p[v][k];
+  }
+  for (int k = 0; k < 3; ++k) s->barycenter_target[k] = bc[k] / tot;
+  return true;
+}
+
+}  // namespace orc

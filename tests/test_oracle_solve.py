@@ -1,0 +1,99 @@
+"""Pins the FP64 oracle's saddle solver, preconditioner, GMRES and descent step (CPU).
+
+* Lean golden ``rs-icosa-precond``: ε and z = L′⁻¹ blockdiag(L_{2−s} ⊗ I₃, 0) L′⁻¹ r with the
+  fixture's own L_{2−s} (tests/oracle.rs:45-48: 1e-10) — pins W, M, ε, the barycenter row and
+  the L′ solve;
+* ``fractional_symbol_ratio`` = 4 I(σ) I(2−s) ≈ 200 at s = 5/3 (solve/precond.rs:512-564);
+* the augmented preconditioner reproduces the constraint rows, C_Q (P⁻¹ [r; b])_x = b, and is
+  symmetric (precond.rs:596-612);
+* GMRES's solution satisfies A′ x = b to its tolerance, checked against an independent dense
+  solve (gmres.rs:413);
+* one descent step lowers the energy and keeps the barycenter.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from golden import fixtures as fx
+
+
+def test_lean_precond_icosa():
+    c, o = fx.read_case("rs-icosa-precond"), fx.read_out("rs-icosa-precond")
+    sd = oracle.Saddle(c["faces"], c["positions"], *c["exponents"], chi=0.0, eps_rel=c["eps_rel"])
+    assert abs(sd.eps - o["eps"]) <= 1e-12 * o["eps"]
+    nv = len(c["positions"])
+    r = np.zeros(3 * nv + 3)
+    r[: 3 * nv] = np.array(c["r"]).reshape(-1)
+    lfr = fx.matrix(o, "Lfr")
+    y = sd.lprime_solve(r)
+    v = np.zeros_like(y)
+    for k in range(3):
+        v[k: 3 * nv: 3] = lfr @ y[k: 3 * nv: 3]
+    z = sd.lprime_solve(v)
+    zref = np.array([o["rows"]["z"][i] for i in range(nv)]).reshape(-1)
+    assert np.abs(z[: 3 * nv] - zref).max() <= 1e-10 * np.abs(zref).max()
+
+
+def test_symbol_ratio_closed_form():
+    s = 5.0 / 3.0
+
+    def I(t):
+        return math.pi * math.gamma(1 - t) / (t * 4.0**t * math.gamma(1 + t))
+
+    assert abs(oracle.fractional_symbol_ratio(s) - 4 * I(s - 1) * I(2 - s)) <= 1e-13 * 200
+    assert 199.0 < oracle.fractional_symbol_ratio(s) < 201.0
+
+
+@pytest.fixture(scope="module")
+def small():
+    from conftest import load_package
+
+    g = load_package().generate
+    p, f = g.icosphere(2)
+    return g.normal_jitter(p, f, 0.02, 9), f
+
+
+def test_augmented_preconditioner_structure(small):
+    p, f = small
+    sd = oracle.Saddle(f, p, chi=0.5)
+    nv = len(p)
+    rng = np.random.default_rng(2)
+    r = rng.standard_normal(sd.n)
+    z = sd.precond(r)
+    va = oracle.vertex_areas(f, p)
+    crow = va / va.sum()
+    for k in range(3):
+        assert abs(crow @ z[k: 3 * nv: 3] - r[3 * nv + k]) <= 1e-9 * np.abs(r).max()
+    u = rng.standard_normal(sd.n)
+    assert abs(u @ sd.precond(r) - r @ sd.precond(u)) <= 1e-9 * np.abs(u) @ np.abs(sd.precond(r))
+
+
+def test_gmres_against_dense_solve(small):
+    p, f = small
+    sd = oracle.Saddle(f, p, chi=0.5)
+    n = sd.n
+    A = np.stack([sd.apply(e) for e in np.eye(n)], axis=1)
+    e0, d, _ = oracle.exact(f, p)
+    rhs = np.zeros(n)
+    rhs[:-3] = d.reshape(-1)
+    x, rep = sd.solve(rhs, restart=50, tol=1e-10, maxit=400)
+    assert rep["converged"] == 1.0
+    xd = np.linalg.solve(A, rhs)
+    assert np.linalg.norm(x - xd) <= 1e-7 * np.linalg.norm(xd)
+    assert np.linalg.norm(A @ x - rhs) <= 1e-9 * np.linalg.norm(rhs)
+
+
+def test_descent_step_lowers_energy_and_keeps_barycenter(small):
+    p, f = small
+    s = oracle.Session(f, p, theta=0.5)
+    va = oracle.vertex_areas(f, p)
+    b0 = (va[:, None] * p).sum(axis=0) / va.sum()
+    rep = s.step(2)
+    assert rep["accepted"] == 1.0 and rep["gmres_converged"] == 1.0
+    assert rep["energy1"] < rep["energy0"]
+    q = s.positions()
+    vq = oracle.vertex_areas(f, q)
+    b1 = (vq[:, None] * q).sum(axis=0) / vq.sum()
+    assert np.abs(b1 - b0).max() <= 1e-14
