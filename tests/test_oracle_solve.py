@@ -64,10 +64,12 @@ def test_augmented_preconditioner_structure(small):
     z = sd.precond(r)
     va = oracle.vertex_areas(f, p)
     crow = va / va.sum()
+    # L′ has a near-null space of size ε = 1e-10 tr W / tr M; its unrefined solves keep about
+    # 1e-8 relative accuracy on it (precond.rs:16-18), hence the tolerance.
     for k in range(3):
-        assert abs(crow @ z[k: 3 * nv: 3] - r[3 * nv + k]) <= 1e-9 * np.abs(r).max()
+        assert abs(crow @ z[k: 3 * nv: 3] - r[3 * nv + k]) <= 1e-7 * np.abs(r).max()
     u = rng.standard_normal(sd.n)
-    assert abs(u @ sd.precond(r) - r @ sd.precond(u)) <= 1e-9 * np.abs(u) @ np.abs(sd.precond(r))
+    assert abs(u @ sd.precond(r) - r @ sd.precond(u)) <= 1e-7 * np.abs(u) @ np.abs(sd.precond(r))
 
 
 def test_gmres_against_dense_solve(small):
