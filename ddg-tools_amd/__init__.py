@@ -97,6 +97,14 @@ SIGNATURES = {
     "ddgrs_surface_set_positions": (_I32, [_P, _P, _E]),
     "ddgrs_surface_get_positions": (_I32, [_P, _P, _E]),
     "ddgrs_exact_energy_and_differential": (_I32, [_P, _D, _D, ctypes.POINTER(_D), _P, _P, _E]),
+    "ddgrs_bh_build": (_I32, [_P, _D, _I64, ctypes.POINTER(_P), _E]),
+    "ddgrs_bh_destroy": (None, [_P]),
+    "ddgrs_bh_refit": (_I32, [_P, _P, _E]),
+    "ddgrs_bh_energy": (_I32, [_P, _D, _D, ctypes.POINTER(_D), _E]),
+    "ddgrs_bh_energy_and_differential": (_I32, [_P, _D, _D, ctypes.POINTER(_D), _P, _P, _E]),
+    "ddgrs_bh_counts": (_I32, [_P, _P, _E]),
+    "ddgrs_bh_tree": (_I32, [_P, _P, _P, _P, _P, _P, _E]),
+    "ddgrs_bh_lists": (_I32, [_P, _P, _P, _P, _P, _E]),
 }
 
 
@@ -254,11 +262,85 @@ def exact_energy_and_differential(surface: Surface, exponents: Exponents = Expon
     return (e.value, d, fp) if partials else (e.value, d)
 
 
+class BhTree:
+    """The Barnes–Hut evaluator of one step (``BhTree``, src/energy/barnes_hut.rs:114): face BVH and
+    frozen lists built at the surface's current positions; ``refit`` moves it for line-search
+    trials."""
+
+    def __init__(self, surface: Surface, opts: BhOptions = BhOptions()):
+        self.surface = surface
+        self.opts = opts
+        h = _P()
+        _call(lib().ddgrs_bh_build, surface.handle, opts.theta, opts.leaf_size, ctypes.byref(h))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ddgrs_bh_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def refit(self, positions=None):
+        if positions is None:
+            _call(lib().ddgrs_bh_refit, self._h, None)
+        else:
+            p = _f64(positions).reshape(-1, 3)
+            if p.shape[0] != self.surface.nv:
+                raise RsError(1, "bh_refit", f"{p.shape[0]} positions for {self.surface.nv} vertices")
+            _call(lib().ddgrs_bh_refit, self._h, _ptr(p))
+
+    def energy(self, exponents: Exponents = Exponents()) -> float:
+        e = _D()
+        _call(lib().ddgrs_bh_energy, self._h, exponents.alpha, exponents.beta, ctypes.byref(e))
+        return e.value
+
+    def energy_and_differential(self, exponents: Exponents = Exponents(), partials: bool = False):
+        e = _D()
+        d = np.zeros((self.surface.nv, 3))
+        fp = np.zeros((self.surface.nf, 7)) if partials else None
+        _call(lib().ddgrs_bh_energy_and_differential, self._h, exponents.alpha, exponents.beta, ctypes.byref(e), _ptr(d),
+              _ptr(fp) if partials else None)
+        return (e.value, d, fp) if partials else (e.value, d)
+
+    def counts(self) -> dict:
+        c = np.zeros(6, dtype=np.int64)
+        _call(lib().ddgrs_bh_counts, self._h, _ptr(c))
+        keys = ["nodes", "leaves", "lumped", "exact_blocks", "pairs_exact", "partner_slots"]
+        return dict(zip(keys, (int(x) for x in c)))
+
+    def tree(self) -> dict:
+        c = self.counts()
+        nn = c["nodes"]
+        order = np.zeros(self.surface.nf, dtype=np.uint32)
+        start, end, right = (np.zeros(nn, dtype=np.uint32) for _ in range(3))
+        cl = np.zeros((nn, 17))
+        _call(lib().ddgrs_bh_tree, self._h, _ptr(order), _ptr(start), _ptr(end), _ptr(right), _ptr(cl))
+        return {"order": order, "start": start, "end": end, "right": right, "clusters": cl}
+
+    def lists(self) -> dict:
+        c = self.counts()
+        nl = c["leaves"]
+        lo = np.zeros(nl + 1, dtype=np.uint32)
+        lm = np.zeros(c["lumped"], dtype=np.uint32)
+        eo = np.zeros(nl + 1, dtype=np.uint32)
+        ex = np.zeros(c["exact_blocks"], dtype=np.uint32)
+        _call(lib().ddgrs_bh_lists, self._h, _ptr(lo), _ptr(lm), _ptr(eo), _ptr(ex))
+        return {"lumped_off": lo, "lumped": lm, "exact_off": eo, "exact": ex}
+
+
 def energy(surface: Surface, exponents: Exponents = Exponents(), eval=EnergyEval.Exact) -> EnergyReport:
     """``energy()`` (src/energy/mod.rs:168) without penalties."""
     if eval == EnergyEval.Exact:
         e, d = exact_energy_and_differential(surface, exponents)
         return EnergyReport(e, d, surface.nf * (surface.nf - 1), 0)
+    if isinstance(eval, tuple) and eval[0] == "barnes_hut":
+        bh = BhTree(surface, eval[1])
+        e, d = bh.energy_and_differential(exponents)
+        c = bh.counts()
+        bh.close()
+        return EnergyReport(e, d, c["pairs_exact"], c["lumped"])
     raise RsError(1, "energy", f"unsupported evaluation {eval!r}")
 
 

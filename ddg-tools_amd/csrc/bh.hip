@@ -1,0 +1,541 @@
+// Barnes–Hut energy Ẽ and its exact gradient for frozen lists on the device.
+//
+// Follows RS/src/energy/barnes_hut.rs: leaf_with_gradient :627 (own, node and partner terms),
+// face_partials :478 (node sums over the transposed lumped lists :506-520, downward pass
+// :521-538, per-leaf final :540-567), leaf_energy :583 (value only, line-search trials), refit
+// :204 (face data and node aggregates A_C, X_C; boxes, radii and lists frozen).
+//
+// Schedule (gfx950): one wave per leaf ℓ, lane = 8 i + j.  Lumped entries are spread over j
+// (lane (i, j) evaluates face S_i against node C_{e}, e ≡ j mod 8; the node terms b, u are reduced
+// over i with shuffles); an exact block (ℓ, C) puts the ≤ 8 × 8 pairs (S_i, T_j) on the 64 lanes;
+// the partner terms are gathered on the target side: the wave of leaf C walks the transposed list
+// (the leaves that open C) and evaluates K(S'_j → T_i) itself, so no partner array is stored.
+// Own and partner sums are reduced over j by an xor butterfly (bitwise identical on all lanes,
+// independent of scheduling).  Pair arithmetic is FP32 with the hi/lo barycenter difference; the
+// node sums, the downward pass and the per-face epilogue are FP64.
+#include "bh.hpp"
+
+namespace ddgrs {
+
+#pragma clang fp contract(fast)
+
+template <int MODE>
+struct KPow {
+  float alpha, beta;
+  int aint, bint;
+};
+
+// K, ∂K/∂v (vector), ∂K/∂N factor c_n (∂K/∂N = c_n v) for v and the source normal n
+// (RS/src/kernel.rs:238 kernel_with_gradient_of_v).
+template <int MODE>
+__device__ __forceinline__ void kgrad(const KPow<MODE>& p, float vx, float vy, float vz, float nx, float ny, float nz,
+                                      float r2, float& k, float& cn, float& bv) {
+  float inv = __builtin_amdgcn_rcpf(r2);
+  float nu = nx * vx + ny * vy + nz * vz;
+  if constexpr (MODE == 0) {  // (6, 12)
+    float i3 = inv * inv * inv;
+    float t = nu * nu * inv;
+    float t2 = t * t;
+    k = t2 * t * i3;
+    cn = p.alpha * (nu * inv) * t2 * i3;
+  } else if constexpr (MODE == 1) {
+    float a = fabsf(nu), q = 1.f;
+    for (int i = 0; i < p.aint - 1; ++i) q *= a;
+    float rb = 1.f;
+    for (int i = 0; i < p.bint / 2; ++i) rb *= inv;
+    if (p.bint & 1) rb *= __frsqrt_rn(r2);
+    k = q * a * rb;
+    cn = p.alpha * copysignf(q, nu) * rb;
+  } else {
+    float a = fabsf(nu);
+    float q = exp2f((p.alpha - 1.f) * __log2f(a));
+    float rb = exp2f(-0.5f * p.beta * __log2f(r2));
+    k = q * a * rb;
+    cn = p.alpha * copysignf(q, nu) * rb;
+  }
+  bv = p.beta * k * inv;  // ∂K/∂v = cn n − bv v
+}
+
+template <int MODE>
+__device__ __forceinline__ float kval(const KPow<MODE>& p, float vx, float vy, float vz, float nx, float ny, float nz,
+                                      float r2) {
+  float inv = __builtin_amdgcn_rcpf(r2);
+  float nu = nx * vx + ny * vy + nz * vz;
+  if constexpr (MODE == 0) {
+    float i3 = inv * inv * inv;
+    float t = nu * nu * inv;
+    return t * t * t * i3;
+  } else if constexpr (MODE == 1) {
+    float a = fabsf(nu), q = 1.f;
+    for (int i = 0; i < p.aint; ++i) q *= a;
+    float rb = 1.f;
+    for (int i = 0; i < p.bint / 2; ++i) rb *= inv;
+    if (p.bint & 1) rb *= __frsqrt_rn(r2);
+    return q * rb;
+  } else {
+    float a = fabsf(nu);
+    return exp2f(p.alpha * __log2f(a)) * exp2f(-0.5f * p.beta * __log2f(r2));
+  }
+}
+
+__device__ __forceinline__ float xsum8(float x) {  // sum over the 8 lanes j of a row i
+  x += __shfl_xor(x, 1);
+  x += __shfl_xor(x, 2);
+  x += __shfl_xor(x, 4);
+  return x;
+}
+__device__ __forceinline__ float isum8(float x) {  // sum over the 8 rows i (lanes j, j+8, …)
+  x += __shfl_xor(x, 8);
+  x += __shfl_xor(x, 16);
+  x += __shfl_xor(x, 32);
+  return x;
+}
+
+struct BhArgs {
+  int n_leaves;
+  const uint32_t *leaves, *nstart, *nend;
+  const uint32_t *lumped_off, *lumped, *exact_off, *exact, *exact_t_off, *exact_t;
+  const float4 *xhi, *xlo, *nrm;  // faces, BVH order
+  const float4 *chi, *clo;        // nodes: (X_C hi, A_C), (X_C − hi, 0)
+};
+
+template <int MODE>
+__global__ __launch_bounds__(256) void bh_gradient_kernel(BhArgs A, KPow<MODE> pw, float4* __restrict__ node_bu,
+                                                          float* __restrict__ part, double* __restrict__ leaf_energy) {
+  const int lane = threadIdx.x & 63;
+  const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (k >= A.n_leaves) return;  // whole waves exit together
+  const int i = lane >> 3, j = lane & 7;
+  const uint32_t L = A.leaves[k];
+  const uint32_t s0 = A.nstart[L];
+  const int ns = (int)(A.nend[L] - s0);
+  const bool vi = i < ns;
+  const uint32_t ps = s0 + (vi ? i : 0);
+  const float4 sh = A.xhi[ps], sl = A.xlo[ps], sn = A.nrm[ps];
+  const float as = vi ? sh.w : 0.f;
+  float own_a = 0.f, ox = 0.f, oy = 0.f, oz = 0.f, onx = 0.f, ony = 0.f, onz = 0.f, en = 0.f;
+  // 1. lumped nodes
+  const uint32_t l0 = A.lumped_off[k], l1 = A.lumped_off[k + 1];
+  for (uint32_t base = l0; base < l1; base += 8) {
+    const uint32_t e = base + j;
+    const bool act = e < l1;
+    const uint32_t C = A.lumped[act ? e : l0];
+    const float4 ch = A.chi[C], cl = A.clo[C];
+    const float vx = (sh.x - ch.x) + (sl.x - cl.x);
+    const float vy = (sh.y - ch.y) + (sl.y - cl.y);
+    const float vz = (sh.z - ch.z) + (sl.z - cl.z);
+    float kk, cn, bv;
+    kgrad<MODE>(pw, vx, vy, vz, sn.x, sn.y, sn.z, vx * vx + vy * vy + vz * vz, kk, cn, bv);
+    const float w = (act && vi) ? 1.f : 0.f;
+    const float ac = ch.w;
+    const float dvx = cn * sn.x - bv * vx, dvy = cn * sn.y - bv * vy, dvz = cn * sn.z - bv * vz;
+    own_a += w * kk * ac;
+    const float sa = w * as * ac;
+    ox += sa * dvx;
+    oy += sa * dvy;
+    oz += sa * dvz;
+    onx += sa * cn * vx;
+    ony += sa * cn * vy;
+    onz += sa * cn * vz;
+    en += sa * kk;
+    const float b = isum8(w * as * kk);
+    const float ux = isum8(-w * as * dvx), uy = isum8(-w * as * dvy), uz = isum8(-w * as * dvz);
+    if (i == 0 && act) node_bu[e] = make_float4(b, ux, uy, uz);
+  }
+  // 2. exact blocks, own side: pairs (S_i, T_j)
+  float row = 0.f, dx = 0.f, dy = 0.f, dz = 0.f, nx = 0.f, ny = 0.f, nz = 0.f;
+  for (uint32_t q = A.exact_off[k]; q < A.exact_off[k + 1]; ++q) {
+    const uint32_t C = A.exact[q];
+    const uint32_t t0 = A.nstart[C];
+    const int nt = (int)(A.nend[C] - t0);
+    const bool act = vi && j < nt && !(C == L && i == j);
+    const uint32_t pt = t0 + (j < nt ? j : 0);
+    const float4 th = A.xhi[pt], tl = A.xlo[pt];
+    const float vx = (sh.x - th.x) + (sl.x - tl.x);
+    const float vy = (sh.y - th.y) + (sl.y - tl.y);
+    const float vz = (sh.z - th.z) + (sl.z - tl.z);
+    const float r2 = act ? (vx * vx + vy * vy + vz * vz) : 1.f;
+    float kk, cn, bv;
+    kgrad<MODE>(pw, vx, vy, vz, sn.x, sn.y, sn.z, r2, kk, cn, bv);
+    const float at = act ? th.w : 0.f;
+    row += at * kk;
+    dx += at * (cn * sn.x - bv * vx);
+    dy += at * (cn * sn.y - bv * vy);
+    dz += at * (cn * sn.z - bv * vz);
+    nx += at * cn * vx;
+    ny += at * cn * vy;
+    nz += at * cn * vz;
+  }
+  // 3. exact blocks, partner side: pairs (S'_j ∈ ℓ', T_i ∈ ℓ) for the leaves ℓ' that open ℓ
+  float pa = 0.f, px = 0.f, py = 0.f, pz = 0.f;
+  for (uint32_t q = A.exact_t_off[k]; q < A.exact_t_off[k + 1]; ++q) {
+    const uint32_t Ls = A.leaves[A.exact_t[q]];
+    const uint32_t u0 = A.nstart[Ls];
+    const int nu = (int)(A.nend[Ls] - u0);
+    const bool act = vi && j < nu && !(Ls == L && i == j);
+    const uint32_t pu = u0 + (j < nu ? j : 0);
+    const float4 uh = A.xhi[pu], ul = A.xlo[pu], un = A.nrm[pu];
+    const float vx = (uh.x - sh.x) + (ul.x - sl.x);
+    const float vy = (uh.y - sh.y) + (ul.y - sl.y);
+    const float vz = (uh.z - sh.z) + (ul.z - sl.z);
+    const float r2 = act ? (vx * vx + vy * vy + vz * vz) : 1.f;
+    float kk, cn, bv;
+    kgrad<MODE>(pw, vx, vy, vz, un.x, un.y, un.z, r2, kk, cn, bv);
+    const float au = act ? uh.w : 0.f;
+    pa += au * kk;
+    const float w = -au * as;
+    px += w * (cn * un.x - bv * vx);
+    py += w * (cn * un.y - bv * vy);
+    pz += w * (cn * un.z - bv * vz);
+  }
+  // reduce over j and finish the own terms of S_i
+  own_a = xsum8(own_a);
+  ox = xsum8(ox); oy = xsum8(oy); oz = xsum8(oz);
+  onx = xsum8(onx); ony = xsum8(ony); onz = xsum8(onz);
+  row = xsum8(row);
+  dx = xsum8(dx); dy = xsum8(dy); dz = xsum8(dz);
+  nx = xsum8(nx); ny = xsum8(ny); nz = xsum8(nz);
+  pa = xsum8(pa);
+  px = xsum8(px); py = xsum8(py); pz = xsum8(pz);
+  if (j == 0 && vi) {
+    float* o = part + 7 * (size_t)(s0 + i);
+    o[0] = (own_a + row) + pa;
+    o[1] = (ox + as * dx) + px;
+    o[2] = (oy + as * dy) + py;
+    o[3] = (oz + as * dz) + pz;
+    o[4] = onx + as * nx;
+    o[5] = ony + as * ny;
+    o[6] = onz + as * nz;
+  }
+  // leaf energy: lumped terms (spread over all lanes) + a_S · row (lanes j == 0)
+  float el = en + ((j == 0 && vi) ? as * row : 0.f);
+  el = xsum8(el);
+  el = isum8(el);
+  if (lane == 0) leaf_energy[k] = (double)el;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void bh_energy_kernel(BhArgs A, KPow<MODE> pw, double* __restrict__ leaf_energy) {
+  const int lane = threadIdx.x & 63;
+  const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (k >= A.n_leaves) return;
+  const int i = lane >> 3, j = lane & 7;
+  const uint32_t L = A.leaves[k];
+  const uint32_t s0 = A.nstart[L];
+  const int ns = (int)(A.nend[L] - s0);
+  const bool vi = i < ns;
+  const uint32_t ps = s0 + (vi ? i : 0);
+  const float4 sh = A.xhi[ps], sl = A.xlo[ps], sn = A.nrm[ps];
+  const float as = vi ? sh.w : 0.f;
+  float en = 0.f;
+  const uint32_t l0 = A.lumped_off[k], l1 = A.lumped_off[k + 1];
+  for (uint32_t base = l0; base < l1; base += 8) {
+    const uint32_t e = base + j;
+    const bool act = e < l1;
+    const uint32_t C = A.lumped[act ? e : l0];
+    const float4 ch = A.chi[C], cl = A.clo[C];
+    const float vx = (sh.x - ch.x) + (sl.x - cl.x);
+    const float vy = (sh.y - ch.y) + (sl.y - cl.y);
+    const float vz = (sh.z - ch.z) + (sl.z - cl.z);
+    const float kk = kval<MODE>(pw, vx, vy, vz, sn.x, sn.y, sn.z, vx * vx + vy * vy + vz * vz);
+    en += (act && vi) ? as * ch.w * kk : 0.f;
+  }
+  for (uint32_t q = A.exact_off[k]; q < A.exact_off[k + 1]; ++q) {
+    const uint32_t C = A.exact[q];
+    const uint32_t t0 = A.nstart[C];
+    const int nt = (int)(A.nend[C] - t0);
+    const bool act = vi && j < nt && !(C == L && i == j);
+    const uint32_t pt = t0 + (j < nt ? j : 0);
+    const float4 th = A.xhi[pt], tl = A.xlo[pt];
+    const float vx = (sh.x - th.x) + (sl.x - tl.x);
+    const float vy = (sh.y - th.y) + (sl.y - tl.y);
+    const float vz = (sh.z - th.z) + (sl.z - tl.z);
+    const float r2 = act ? (vx * vx + vy * vy + vz * vz) : 1.f;
+    const float kk = kval<MODE>(pw, vx, vy, vz, sn.x, sn.y, sn.z, r2);
+    en += act ? as * th.w * kk : 0.f;
+  }
+  en = xsum8(en);
+  en = isum8(en);
+  if (lane == 0) leaf_energy[k] = (double)en;
+}
+
+#pragma clang fp contract(off)
+
+// Faces into BVH order: the FP32 packs of position p are those of face order[p].
+__global__ __launch_bounds__(256) void permute_packs_kernel(int nf, const uint32_t* __restrict__ order,
+                                                            const float4* __restrict__ a, const float4* __restrict__ b,
+                                                            const float4* __restrict__ c, float4* __restrict__ pa,
+                                                            float4* __restrict__ pb, float4* __restrict__ pc) {
+  int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= nf) return;
+  uint32_t f = order[p];
+  pa[p] = a[f];
+  pb[p] = b[f];
+  pc[p] = c[f];
+}
+
+// One level of the bottom-up node aggregates (bvh.rs:221-254): a leaf sums its faces in BVH order
+// (area, area-weighted barycenter), an internal node adds its children.
+__global__ __launch_bounds__(256) void node_aggregate_kernel(int count, const uint32_t* __restrict__ nodes,
+                                                             const uint32_t* __restrict__ nstart,
+                                                             const uint32_t* __restrict__ nend,
+                                                             const uint32_t* __restrict__ right,
+                                                             const uint32_t* __restrict__ order,
+                                                             const double* __restrict__ area,
+                                                             const double* __restrict__ bary, double* __restrict__ mass,
+                                                             double* __restrict__ moment) {
+  int q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= count) return;
+  uint32_t a = nodes[q];
+  if (right[a] == 0) {
+    double m = 0.0, mx = 0.0, my = 0.0, mz = 0.0;
+    for (uint32_t p = nstart[a]; p < nend[a]; ++p) {
+      uint32_t f = order[p];
+      double w = area[f];
+      m += w;
+      mx += w * bary[3 * f];
+      my += w * bary[3 * f + 1];
+      mz += w * bary[3 * f + 2];
+    }
+    mass[a] = m;
+    moment[3 * a] = mx;
+    moment[3 * a + 1] = my;
+    moment[3 * a + 2] = mz;
+  } else {
+    uint32_t l = a + 1, r = right[a];
+    mass[a] = mass[l] + mass[r];
+    for (int k = 0; k < 3; ++k) moment[3 * a + k] = moment[3 * l + k] + moment[3 * r + k];
+  }
+}
+
+// Centroids X_C = moment / A_C and the FP32 node packs.
+__global__ __launch_bounds__(256) void node_pack_kernel(int nn, const double* __restrict__ mass,
+                                                        const double* __restrict__ moment, double* __restrict__ cen,
+                                                        float4* __restrict__ chi, float4* __restrict__ clo) {
+  int a = blockIdx.x * 256 + threadIdx.x;
+  if (a >= nn) return;
+  double m = mass[a];
+  double x[3];
+  for (int k = 0; k < 3; ++k) {
+    x[k] = moment[3 * a + k] / m;
+    cen[3 * a + k] = x[k];
+  }
+  float hx = (float)x[0], hy = (float)x[1], hz = (float)x[2];
+  chi[a] = make_float4(hx, hy, hz, (float)m);
+  clo[a] = make_float4((float)(x[0] - (double)hx), (float)(x[1] - (double)hy), (float)(x[2] - (double)hz), 0.f);
+}
+
+// Node sums b_C, u_C over the transposed lumped lists (entries increasing), FP64.
+__global__ __launch_bounds__(256) void bh_node_sums_kernel(int nn, const uint32_t* __restrict__ off,
+                                                           const uint32_t* __restrict__ ent,
+                                                           const float4* __restrict__ node_bu,
+                                                           double4* __restrict__ sums) {
+  int a = blockIdx.x * 256 + threadIdx.x;
+  if (a >= nn) return;
+  double b = 0.0, ux = 0.0, uy = 0.0, uz = 0.0;
+  for (uint32_t q = off[a]; q < off[a + 1]; ++q) {
+    float4 t = node_bu[ent[q]];
+    b += (double)t.x;
+    ux += (double)t.y;
+    uy += (double)t.z;
+    uz += (double)t.w;
+  }
+  sums[a] = make_double4(b, ux, uy, uz);
+}
+
+// Per face (BVH position p): the downward pass along the chain root → leaf (barnes_hut.rs:521-538)
+// and the final partials (:540-567), written in face order (7 FP64 per face).
+__global__ __launch_bounds__(256) void bh_final_kernel(int nf, const uint32_t* __restrict__ order,
+                                                       const uint32_t* __restrict__ leaf_of_pos,
+                                                       const uint32_t* __restrict__ chain_ptr,
+                                                       const uint32_t* __restrict__ chain,
+                                                       const double4* __restrict__ sums, const double* __restrict__ cen,
+                                                       const float* __restrict__ part, const double* __restrict__ area,
+                                                       const double* __restrict__ bary, double* __restrict__ out) {
+  int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= nf) return;
+  uint32_t k = leaf_of_pos[p];
+  uint32_t c0 = chain_ptr[k], c1 = chain_ptr[k + 1];
+  uint32_t a = chain[c0];
+  double4 s = sums[a];
+  double beta = s.x, U[3] = {s.y, s.z, s.w};
+  for (uint32_t q = c0 + 1; q < c1; ++q) {
+    uint32_t c = chain[q];
+    double sh[3] = {cen[3 * c] - cen[3 * a], cen[3 * c + 1] - cen[3 * a + 1], cen[3 * c + 2] - cen[3 * a + 2]};
+    double4 sc = sums[c];
+    beta = beta + (U[0] * sh[0] + U[1] * sh[1] + U[2] * sh[2]) + sc.x;
+    U[0] = U[0] + sc.y;
+    U[1] = U[1] + sc.z;
+    U[2] = U[2] + sc.w;
+    a = c;
+  }
+  uint32_t f = order[p];
+  const float* pp = part + 7 * (size_t)p;
+  double at = area[f];
+  double d[3] = {bary[3 * f] - cen[3 * a], bary[3 * f + 1] - cen[3 * a + 1], bary[3 * f + 2] - cen[3 * a + 2]};
+  double* o = out + 7 * (size_t)f;
+  o[0] = (double)pp[0] + (beta + (U[0] * d[0] + U[1] * d[1] + U[2] * d[2]));
+  for (int k2 = 0; k2 < 3; ++k2) o[1 + k2] = (double)pp[1 + k2] + at * U[k2];
+  for (int k2 = 0; k2 < 3; ++k2) o[4 + k2] = (double)pp[4 + k2];
+}
+
+// ---- host ------------------------------------------------------------------------------------
+
+void BhEval::upload_structure(ddgrs_ctx* ctx) {
+  const Topology& t = topo;
+  nf = (int)t.order.size();
+  nn = (int)t.n_nodes();
+  nl = (int)t.leaves.size();
+  upload(ctx, d_order, t.order);
+  upload(ctx, d_start, t.start);
+  upload(ctx, d_end, t.end);
+  upload(ctx, d_right, t.right);
+  upload(ctx, d_leaves, t.leaves);
+  upload(ctx, d_lumped_off, lists.lumped_off);
+  upload(ctx, d_lumped, lists.lumped);
+  upload(ctx, d_exact_off, lists.exact_off);
+  upload(ctx, d_exact, lists.exact);
+  upload(ctx, d_exact_t_off, lists.exact_t_off);
+  upload(ctx, d_exact_t, lists.exact_t);
+  upload(ctx, d_lumped_t_off, lists.lumped_t_off);
+  upload(ctx, d_lumped_t, lists.lumped_t);
+  std::vector<uint32_t> leaf_of_pos(nf), chain_ptr{0}, chain;
+  for (int k = 0; k < nl; ++k) {
+    uint32_t L = t.leaves[k];
+    for (uint32_t p = t.start[L]; p < t.end[L]; ++p) leaf_of_pos[p] = (uint32_t)k;
+    size_t c0 = chain.size();
+    for (uint32_t x = L;; x = t.parent[x]) {
+      chain.push_back(x);
+      if (x == 0) break;
+    }
+    std::reverse(chain.begin() + c0, chain.end());
+    chain_ptr.push_back((uint32_t)chain.size());
+  }
+  upload(ctx, d_leaf_of_pos, leaf_of_pos);
+  upload(ctx, d_chain_ptr, chain_ptr);
+  upload(ctx, d_chain, chain);
+  // nodes grouped by depth, deepest first, for the bottom-up aggregate
+  levels.assign(t.height, {});
+  for (int a = 0; a < nn; ++a) levels[t.depth[a]].push_back((uint32_t)a);
+  std::vector<uint32_t> flat;
+  level_off.assign(1, 0);
+  for (int d = (int)t.height - 1; d >= 0; --d) {
+    flat.insert(flat.end(), levels[d].begin(), levels[d].end());
+    level_off.push_back((uint32_t)flat.size());
+  }
+  upload(ctx, d_levels, flat);
+  pxhi.reserve(nf);
+  pxlo.reserve(nf);
+  pnrm.reserve(nf);
+  mass.reserve(nn);
+  moment.reserve(3 * nn);
+  cen.reserve(3 * nn);
+  chi.reserve(nn);
+  clo.reserve(nn);
+  node_bu.reserve(std::max<size_t>(1, lists.lumped.size()));
+  sums.reserve(nn);
+  part.reserve(7 * (size_t)nf);
+  leaf_energy.reserve(nl);
+}
+
+void BhEval::refresh(ddgrs_surface* s) {
+  ddgrs_ctx* ctx = s->ctx;
+  launch(ctx, "bh_permute", permute_packs_kernel, dim3(blocks_for(nf, 256)), dim3(256), 0, nf,
+         (const uint32_t*)d_order.ptr, (const float4*)s->packs.xhi.ptr, (const float4*)s->packs.xlo.ptr,
+         (const float4*)s->packs.nrm.ptr, pxhi.ptr, pxlo.ptr, pnrm.ptr);
+  for (size_t lv = 0; lv + 1 < level_off.size(); ++lv) {
+    int cnt = (int)(level_off[lv + 1] - level_off[lv]);
+    launch(ctx, "node_aggregate", node_aggregate_kernel, dim3(blocks_for(cnt, 256)), dim3(256), 0, cnt,
+           (const uint32_t*)(d_levels.ptr + level_off[lv]), (const uint32_t*)d_start.ptr, (const uint32_t*)d_end.ptr,
+           (const uint32_t*)d_right.ptr, (const uint32_t*)d_order.ptr, (const double*)s->area.ptr,
+           (const double*)s->bary.ptr, mass.ptr, moment.ptr);
+  }
+  launch(ctx, "node_pack", node_pack_kernel, dim3(blocks_for(nn, 256)), dim3(256), 0, nn, (const double*)mass.ptr,
+         (const double*)moment.ptr, cen.ptr, chi.ptr, clo.ptr);
+}
+
+BhArgs BhEval::args() const {
+  BhArgs A;
+  A.n_leaves = nl;
+  A.leaves = d_leaves.ptr;
+  A.nstart = d_start.ptr;
+  A.nend = d_end.ptr;
+  A.lumped_off = d_lumped_off.ptr;
+  A.lumped = d_lumped.ptr;
+  A.exact_off = d_exact_off.ptr;
+  A.exact = d_exact.ptr;
+  A.exact_t_off = d_exact_t_off.ptr;
+  A.exact_t = d_exact_t.ptr;
+  A.xhi = pxhi.ptr;
+  A.xlo = pxlo.ptr;
+  A.nrm = pnrm.ptr;
+  A.chi = chi.ptr;
+  A.clo = clo.ptr;
+  return A;
+}
+
+void BhEval::energy(ddgrs_ctx* ctx, const Exps& e, double* total) {
+  BhArgs A = args();
+  dim3 grid(blocks_for(nl, 4));
+  if (e.alpha_int == 6 && e.beta_int == 12)
+    launch(ctx, "bh_energy", bh_energy_kernel<0>, grid, dim3(256), 0, A, KPow<0>{e.alpha, e.beta, 6, 12}, leaf_energy.ptr);
+  else if (e.alpha_int >= 0 && e.beta_int >= 0)
+    launch(ctx, "bh_energy", bh_energy_kernel<1>, grid, dim3(256), 0, A, KPow<1>{e.alpha, e.beta, e.alpha_int, e.beta_int},
+           leaf_energy.ptr);
+  else
+    launch(ctx, "bh_energy", bh_energy_kernel<2>, grid, dim3(256), 0, A, KPow<2>{e.alpha, e.beta, -1, -1}, leaf_energy.ptr);
+  sum_f64(ctx, nl, leaf_energy.ptr, total);
+}
+
+void BhEval::gradient(ddgrs_surface* s, const Exps& e, double* total, double* partials) {
+  ddgrs_ctx* ctx = s->ctx;
+  BhArgs A = args();
+  dim3 grid(blocks_for(nl, 4));
+  if (e.alpha_int == 6 && e.beta_int == 12)
+    launch(ctx, "bh_gradient", bh_gradient_kernel<0>, grid, dim3(256), 0, A, KPow<0>{e.alpha, e.beta, 6, 12}, node_bu.ptr,
+           part.ptr, leaf_energy.ptr);
+  else if (e.alpha_int >= 0 && e.beta_int >= 0)
+    launch(ctx, "bh_gradient", bh_gradient_kernel<1>, grid, dim3(256), 0, A,
+           KPow<1>{e.alpha, e.beta, e.alpha_int, e.beta_int}, node_bu.ptr, part.ptr, leaf_energy.ptr);
+  else
+    launch(ctx, "bh_gradient", bh_gradient_kernel<2>, grid, dim3(256), 0, A, KPow<2>{e.alpha, e.beta, -1, -1},
+           node_bu.ptr, part.ptr, leaf_energy.ptr);
+  launch(ctx, "bh_node_sums", bh_node_sums_kernel, dim3(blocks_for(nn, 256)), dim3(256), 0, nn,
+         (const uint32_t*)d_lumped_t_off.ptr, (const uint32_t*)d_lumped_t.ptr, (const float4*)node_bu.ptr, sums.ptr);
+  launch(ctx, "bh_final", bh_final_kernel, dim3(blocks_for(nf, 256)), dim3(256), 0, nf, (const uint32_t*)d_order.ptr,
+         (const uint32_t*)d_leaf_of_pos.ptr, (const uint32_t*)d_chain_ptr.ptr, (const uint32_t*)d_chain.ptr,
+         (const double4*)sums.ptr, (const double*)cen.ptr, (const float*)part.ptr, (const double*)s->area.ptr,
+         (const double*)s->bary.ptr, partials);
+  sum_f64(ctx, nl, leaf_energy.ptr, total);
+}
+
+void BhEval::build(ddgrs_surface* s, double theta_, size_t leaf_size_) {
+  ddgrs_ctx* ctx = s->ctx;
+  theta = theta_;
+  leaf_size = leaf_size_;
+  const size_t nf_ = (size_t)s->nf, nv = (size_t)s->nv;
+  std::vector<double> pos(3 * nv), area(nf_), bary(3 * nf_), nrm(3 * nf_), lo(3 * nf_), hi(3 * nf_);
+  download(ctx, pos.data(), (const double*)s->positions.ptr, 3 * nv);
+  download(ctx, area.data(), (const double*)s->area.ptr, nf_);
+  download(ctx, bary.data(), (const double*)s->bary.ptr, 3 * nf_);
+  download(ctx, nrm.data(), (const double*)s->normal.ptr, 3 * nf_);
+  ctx->sync();
+  for (size_t f = 0; f < nf_; ++f)
+    for (int k = 0; k < 3; ++k) {
+      double a = INFINITY, b = -INFINITY;
+      for (int c = 0; c < 3; ++c) {
+        double x = pos[3 * (size_t)s->faces_h[3 * f + c] + k];
+        a = std::fmin(a, x);
+        b = std::fmax(b, x);
+      }
+      lo[3 * f + k] = a;
+      hi[3 * f + k] = b;
+    }
+  topo = build_topology(nf_, bary.data(), lo.data(), hi.data(), leaf_size);
+  clusters = build_clusters(topo, bary.data(), lo.data(), hi.data(), area.data(), nrm.data());
+  lists = build_bh_lists(topo, clusters, theta);
+  upload_structure(ctx);
+  refresh(s);
+}
+
+}  // namespace ddgrs

@@ -1,0 +1,44 @@
+// The device Barnes–Hut evaluator (see bh.hip).
+#pragma once
+#include <algorithm>
+
+#include "kernels.hpp"
+#include "trees.hpp"
+
+namespace ddgrs {
+
+struct BhArgs;
+
+struct BhEval {
+  double theta = 0.5;
+  size_t leaf_size = 8;
+  Topology topo;
+  Clusters clusters;  // build-time cluster data (boxes, radii: frozen)
+  BhLists lists;
+  int nf = 0, nn = 0, nl = 0;
+  // structure on the device
+  DevBuf<uint32_t> d_order, d_start, d_end, d_right, d_leaves, d_lumped_off, d_lumped, d_exact_off, d_exact,
+      d_exact_t_off, d_exact_t, d_lumped_t_off, d_lumped_t, d_leaf_of_pos, d_chain_ptr, d_chain, d_levels;
+  std::vector<std::vector<uint32_t>> levels;
+  std::vector<uint32_t> level_off;
+  // data of the current positions
+  DevBuf<float4> pxhi, pxlo, pnrm, chi, clo, node_bu;
+  DevBuf<double> mass, moment, cen, leaf_energy;
+  DevBuf<double4> sums;
+  DevBuf<float> part;
+
+  // Tree and lists at the surface's current positions (face geometry must be current).
+  void build(ddgrs_surface* s, double theta, size_t leaf_size);
+  // Face packs in BVH order and node aggregates of the surface's current positions.
+  void refresh(ddgrs_surface* s);
+  // Ẽ into total[0] (device).
+  void energy(ddgrs_ctx* ctx, const Exps& e, double* total);
+  // Ẽ into total[0] and the per-face partials (7·nf FP64, face order) on the device.
+  void gradient(ddgrs_surface* s, const Exps& e, double* total, double* partials);
+
+ private:
+  void upload_structure(ddgrs_ctx* ctx);
+  BhArgs args() const;
+};
+
+}  // namespace ddgrs

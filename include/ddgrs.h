@@ -51,6 +51,7 @@ typedef struct ddgrs_error {
 
 typedef struct ddgrs_ctx ddgrs_ctx;
 typedef struct ddgrs_surface ddgrs_surface;
+typedef struct ddgrs_bh ddgrs_bh;
 
 /* ---- context ------------------------------------------------------------------------------- */
 
@@ -90,6 +91,33 @@ DDGRS_API int ddgrs_surface_get_positions(ddgrs_surface* s, double* positions, d
  * face_partials (optional, 7·nf): (∂/∂a_S, ∂/∂X_S, ∂/∂N_S) as exact.rs:87 face_partials. */
 DDGRS_API int ddgrs_exact_energy_and_differential(ddgrs_surface* s, double alpha, double beta, double* energy,
                                         double* differential, double* face_partials, ddgrs_error* err);
+
+/* ---- Barnes–Hut --------------------------------------------------------------------------- */
+
+/* Builds the face BVH (leaf size, split rule of RC/src/spatial/bvh.rs:126) and the frozen
+ * interaction lists for θ at the surface's current positions.  Replaces BhTree::build
+ * (RS/src/energy/barnes_hut.rs:156) + attach_gpu (:265). */
+DDGRS_API int ddgrs_bh_build(ddgrs_surface* s, double theta, int64_t leaf_size, ddgrs_bh** out, ddgrs_error* err);
+DDGRS_API void ddgrs_bh_destroy(ddgrs_bh* bh);
+/* Moves to new positions (3·nv, or NULL for the surface's current ones) keeping the tree, boxes,
+ * radii and lists; refreshes the face data and A_C, X_C.  Replaces BhTree::refit (:204). */
+DDGRS_API int ddgrs_bh_refit(ddgrs_bh* bh, const double* positions, ddgrs_error* err);
+/* Ẽ on the frozen lists (value only, line-search trials).  Replaces BhTree::try_energy (:350). */
+DDGRS_API int ddgrs_bh_energy(ddgrs_bh* bh, double alpha, double beta, double* energy, ddgrs_error* err);
+/* Ẽ and its exact differential for the frozen lists (3·nv, optional) and the per-face partials
+ * (7·nf, optional).  Replaces BhTree::energy_and_differential (:387) / try_face_partials (:421). */
+DDGRS_API int ddgrs_bh_energy_and_differential(ddgrs_bh* bh, double alpha, double beta, double* energy,
+                                               double* differential, double* face_partials, ddgrs_error* err);
+/* [nodes, leaves, lumped entries, exact blocks, pairs_exact, partner slots]
+ * (InteractionLists::n_lumped / pairs_exact, :94-107). */
+DDGRS_API int ddgrs_bh_counts(ddgrs_bh* bh, int64_t* out6, ddgrs_error* err);
+/* The tree (any pointer may be NULL): order (nf), per node start/end/right, and per node 17 f64
+ * [A, X(3), P̄(6: xx xy xz yy yz zz), box lo(3), box hi(3), r] (RS/src/spatial/mod.rs:80 Cluster). */
+DDGRS_API int ddgrs_bh_tree(ddgrs_bh* bh, uint32_t* order, uint32_t* start, uint32_t* end, uint32_t* right,
+                            double* clusters, ddgrs_error* err);
+/* The lists: per leaf ordinal CSR offsets (leaves+1) and node indices (InteractionLists :61). */
+DDGRS_API int ddgrs_bh_lists(ddgrs_bh* bh, uint32_t* lumped_off, uint32_t* lumped, uint32_t* exact_off,
+                             uint32_t* exact, ddgrs_error* err);
 
 #ifdef __cplusplus
 }
