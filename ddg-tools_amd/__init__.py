@@ -105,6 +105,13 @@ SIGNATURES = {
     "ddgrs_bh_counts": (_I32, [_P, _P, _E]),
     "ddgrs_bh_tree": (_I32, [_P, _P, _P, _P, _P, _P, _E]),
     "ddgrs_bh_lists": (_I32, [_P, _P, _P, _P, _P, _E]),
+    "ddgrs_metric_build": (_I32, [_P, _D, _D, _D, _I64, _I64, ctypes.POINTER(_P), _E]),
+    "ddgrs_metric_destroy": (None, [_P]),
+    "ddgrs_metric_apply_batch": (_I32, [_P, _D, _D, _I64, _P, _P, _E]),
+    "ddgrs_metric_stats": (_I32, [_P, _P, _E]),
+    "ddgrs_vfrac_build": (_I32, [_P, _D, _D, _I64, _I64, ctypes.POINTER(_P), _E]),
+    "ddgrs_vfrac_destroy": (None, [_P]),
+    "ddgrs_vfrac_apply_batch": (_I32, [_P, _I64, _P, _P, _E]),
 }
 
 
@@ -328,6 +335,74 @@ class BhTree:
         ex = np.zeros(c["exact_blocks"], dtype=np.uint32)
         _call(lib().ddgrs_bh_lists, self._h, _ptr(lo), _ptr(lm), _ptr(eo), _ptr(ex))
         return {"lumped_off": lo, "lumped": lm, "exact_off": eo, "exact": ex}
+
+
+@dataclasses.dataclass(frozen=True)
+class ClusterTreeOptions:
+    """``ClusterTreeOptions`` (src/metric/hierarchical.rs:65): χ = 0.5, leaf 8, leaf_pairs 64."""
+
+    chi: float = 0.5
+    leaf_size: int = 8
+    leaf_pairs: int = 64
+
+
+class _Handle:
+    _destroy = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            getattr(lib(), self._destroy)(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+def _batch(fn, h, n, x, *extra):
+    xs = _f64(x)
+    one = xs.ndim == 1
+    xs = np.ascontiguousarray(xs.reshape(-1, n))
+    ys = np.zeros_like(xs)
+    _call(fn, h, *extra, xs.shape[0], _ptr(xs), _ptr(ys))
+    return ys[0] if one else ys
+
+
+class FaceMetricTree(_Handle):
+    """B̃ and B̃⁰ of one configuration (``FaceMetricTree``, src/metric/hierarchical.rs:283)."""
+
+    _destroy = "ddgrs_metric_destroy"
+
+    def __init__(self, surface: Surface, exponents: Exponents = Exponents(), options: ClusterTreeOptions = ClusterTreeOptions()):
+        self.surface = surface
+        h = _P()
+        _call(lib().ddgrs_metric_build, surface.handle, exponents.alpha, exponents.beta, options.chi, options.leaf_size,
+              options.leaf_pairs, ctypes.byref(h))
+        self._h = h
+
+    def apply_batch(self, x, w_hi: float = 1.0, w_lo: float = 1.0):
+        """y = (w_hi B̃ + w_lo B̃⁰) x for x of shape (nb, 3 nv) or (3 nv,)."""
+        return _batch(lib().ddgrs_metric_apply_batch, self._h, 3 * self.surface.nv, x, w_hi, w_lo)
+
+    def stats(self) -> dict:
+        o = np.zeros(7, dtype=np.int64)
+        _call(lib().ddgrs_metric_stats, self._h, _ptr(o))
+        keys = ["admissible_blocks", "inadmissible_blocks", "admissible_pairs", "near_entries", "nodes", "depth", "near_total"]
+        return dict(zip(keys, (int(v) for v in o)))
+
+
+class VertexFractionalTree(_Handle):
+    """L̃^V_τ over vertex pairs (``VertexFractionalTree``, src/metric/hierarchical.rs:1003)."""
+
+    _destroy = "ddgrs_vfrac_destroy"
+
+    def __init__(self, surface: Surface, tau: float, options: ClusterTreeOptions = ClusterTreeOptions()):
+        self.surface = surface
+        h = _P()
+        _call(lib().ddgrs_vfrac_build, surface.handle, tau, options.chi, options.leaf_size, options.leaf_pairs, ctypes.byref(h))
+        self._h = h
+
+    def apply_batch(self, x):
+        return _batch(lib().ddgrs_vfrac_apply_batch, self._h, 3 * self.surface.nv, x)
 
 
 def energy(surface: Surface, exponents: Exponents = Exponents(), eval=EnergyEval.Exact) -> EnergyReport:

@@ -509,10 +509,8 @@ void BhEval::gradient(ddgrs_surface* s, const Exps& e, double* total, double* pa
   sum_f64(ctx, nl, leaf_energy.ptr, total);
 }
 
-void BhEval::build(ddgrs_surface* s, double theta_, size_t leaf_size_) {
+void build_face_tree(ddgrs_surface* s, size_t leaf_size, Topology* topo, Clusters* clusters) {
   ddgrs_ctx* ctx = s->ctx;
-  theta = theta_;
-  leaf_size = leaf_size_;
   const size_t nf_ = (size_t)s->nf, nv = (size_t)s->nv;
   std::vector<double> pos(3 * nv), area(nf_), bary(3 * nf_), nrm(3 * nf_), lo(3 * nf_), hi(3 * nf_);
   download(ctx, pos.data(), (const double*)s->positions.ptr, 3 * nv);
@@ -531,10 +529,31 @@ void BhEval::build(ddgrs_surface* s, double theta_, size_t leaf_size_) {
       lo[3 * f + k] = a;
       hi[3 * f + k] = b;
     }
-  topo = build_topology(nf_, bary.data(), lo.data(), hi.data(), leaf_size);
-  clusters = build_clusters(topo, bary.data(), lo.data(), hi.data(), area.data(), nrm.data());
+  *topo = build_topology(nf_, bary.data(), lo.data(), hi.data(), leaf_size);
+  *clusters = build_clusters(*topo, bary.data(), lo.data(), hi.data(), area.data(), nrm.data());
+}
+
+void vertex_areas_host(ddgrs_surface* s, std::vector<double>* pos, std::vector<double>* mass) {
+  ddgrs_ctx* ctx = s->ctx;
+  const size_t nf_ = (size_t)s->nf, nv = (size_t)s->nv;
+  std::vector<double> area(nf_);
+  pos->resize(3 * nv);
+  download(ctx, pos->data(), (const double*)s->positions.ptr, 3 * nv);
+  download(ctx, area.data(), (const double*)s->area.ptr, nf_);
+  ctx->sync();
+  mass->assign(nv, 0.0);
+  for (size_t f = 0; f < nf_; ++f) {
+    double a = area[f] / 3.0;
+    for (int c = 0; c < 3; ++c) (*mass)[s->faces_h[3 * f + c]] += a;
+  }
+}
+
+void BhEval::build(ddgrs_surface* s, double theta_, size_t leaf_size_) {
+  theta = theta_;
+  leaf_size = leaf_size_;
+  build_face_tree(s, leaf_size, &topo, &clusters);
   lists = build_bh_lists(topo, clusters, theta);
-  upload_structure(ctx);
+  upload_structure(s->ctx);
   refresh(s);
 }
 

@@ -42,3 +42,10 @@ struct BhEval {
 };
 
 }  // namespace ddgrs
+
+namespace ddgrs {
+// Face BVH (topology + build-time clusters with projectors) at the surface's current positions.
+void build_face_tree(ddgrs_surface* s, size_t leaf_size, Topology* topo, Clusters* clusters);
+// Vertex areas a_v = ⅓ Σ_{S∋v} a_S (RS/src/geometry.rs:194) and positions of the surface (host).
+void vertex_areas_host(ddgrs_surface* s, std::vector<double>* pos, std::vector<double>* mass);
+}  // namespace ddgrs

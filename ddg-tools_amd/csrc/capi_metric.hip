@@ -1,0 +1,134 @@
+// C ABI of the hierarchical metric B̃, B̃⁰ and of the vertex-pair fractional operator.
+#include <cmath>
+
+#include "bh.hpp"
+#include "blocktree.hpp"
+#include "capi_util.hpp"
+
+using namespace ddgrs;
+
+struct ddgrs_metric {
+  ddgrs_surface* s = nullptr;
+  FaceMetricDev fm;
+};
+
+struct ddgrs_vfrac {
+  ddgrs_surface* s = nullptr;
+  VertexFractionalDev vf;
+};
+
+namespace {
+void check_tree_options(double chi, int64_t leaf_size, int64_t leaf_pairs) {
+  if (!(std::isfinite(chi) && chi >= 0.0)) invalid("chi", std::to_string(chi), "a finite number >= 0 (0: exact products)");
+  if (leaf_size < 1) invalid("leaf_size", std::to_string(leaf_size), "at least 1");
+  if (leaf_pairs < 0) invalid("leaf_pairs", std::to_string(leaf_pairs), "non-negative");
+}
+
+// Runs `apply(x_dev, y_dev)` on nb host vectors of length n.
+template <class F>
+void batch(ddgrs_ctx* ctx, int64_t nb, int64_t n, const double* xs, double* ys, F&& apply) {
+  DevBuf<double> dx(n), dy(n);
+  for (int64_t q = 0; q < nb; ++q) {
+    HIPCHECK(hipMemcpyAsync(dx.ptr, xs + q * n, n * 8, hipMemcpyHostToDevice, ctx->stream));
+    apply(dx.ptr, dy.ptr);
+    HIPCHECK(hipMemcpyAsync(ys + q * n, dy.ptr, n * 8, hipMemcpyDeviceToHost, ctx->stream));
+  }
+  ctx->sync();
+}
+}  // namespace
+
+extern "C" {
+
+int ddgrs_metric_build(ddgrs_surface* s, double alpha, double beta, double chi, int64_t leaf_size, int64_t leaf_pairs,
+                       ddgrs_metric** out, ddgrs_error* err) {
+  return guarded(err, "metric_build", [&] {
+    need(s, "surface");
+    need(out, "out");
+    *out = nullptr;
+    Exps e = make_exps(alpha, beta);
+    check_tree_options(chi, leaf_size, leaf_pairs);
+    auto h = std::make_unique<ddgrs_metric>();
+    h->s = s;
+    Topology t;
+    Clusters c;
+    build_face_tree(s, (size_t)leaf_size, &t, &c);
+    h->fm.build(s, t, c, chi, (size_t)leaf_pairs, (e.beta64 - 2.0) / e.alpha64);
+    *out = h.release();
+  });
+}
+
+void ddgrs_metric_destroy(ddgrs_metric* h) {
+  if (!h) return;
+  try {
+    h->s->ctx->sync();
+  } catch (...) {
+  }
+  delete h;
+}
+
+int ddgrs_metric_apply_batch(ddgrs_metric* h, double w_hi, double w_lo, int64_t nb, const double* xs, double* ys,
+                             ddgrs_error* err) {
+  return guarded(err, "metric_apply", [&] {
+    need(h, "metric");
+    need(xs, "xs");
+    need(ys, "ys");
+    if (!(std::isfinite(w_hi) && std::isfinite(w_lo))) invalid("metric weights", "non-finite", "finite weights");
+    h->fm.w_hi = w_hi;
+    h->fm.w_lo = w_lo;
+    ddgrs_ctx* ctx = h->s->ctx;
+    batch(ctx, nb, 3 * h->s->nv, xs, ys, [&](const double* x, double* y) { h->fm.apply(ctx, x, y); });
+  });
+}
+
+int ddgrs_metric_stats(ddgrs_metric* h, int64_t* out, ddgrs_error* err) {
+  return guarded(err, "metric_stats", [&] {
+    need(h, "metric");
+    need(out, "out");
+    const auto& b = h->fm.bs;
+    out[0] = (int64_t)b.far_col.size();
+    out[1] = (int64_t)b.near_col.size();
+    out[2] = (int64_t)b.admissible_pairs;
+    out[3] = (int64_t)b.near_entries;
+    out[4] = (int64_t)h->fm.topo.n_nodes();
+    out[5] = (int64_t)b.depth;
+    out[6] = (int64_t)b.near_total;
+  });
+}
+
+int ddgrs_vfrac_build(ddgrs_surface* s, double tau, double chi, int64_t leaf_size, int64_t leaf_pairs,
+                      ddgrs_vfrac** out, ddgrs_error* err) {
+  return guarded(err, "vfrac_build", [&] {
+    need(s, "surface");
+    need(out, "out");
+    *out = nullptr;
+    if (!std::isfinite(tau)) invalid("tau", std::to_string(tau), "a finite order");
+    check_tree_options(chi, leaf_size, leaf_pairs);
+    auto h = std::make_unique<ddgrs_vfrac>();
+    h->s = s;
+    std::vector<double> pos, mass;
+    vertex_areas_host(s, &pos, &mass);
+    h->vf.build(s->ctx, pos, mass, tau, chi, (size_t)leaf_size, (size_t)leaf_pairs);
+    *out = h.release();
+  });
+}
+
+void ddgrs_vfrac_destroy(ddgrs_vfrac* h) {
+  if (!h) return;
+  try {
+    h->s->ctx->sync();
+  } catch (...) {
+  }
+  delete h;
+}
+
+int ddgrs_vfrac_apply_batch(ddgrs_vfrac* h, int64_t nb, const double* xs, double* ys, ddgrs_error* err) {
+  return guarded(err, "vfrac_apply", [&] {
+    need(h, "vfrac");
+    need(xs, "xs");
+    need(ys, "ys");
+    ddgrs_ctx* ctx = h->s->ctx;
+    batch(ctx, nb, 3 * h->s->nv, xs, ys, [&](const double* x, double* y) { h->vf.apply(ctx, x, y); });
+  });
+}
+
+}  // extern "C"

@@ -52,6 +52,8 @@ typedef struct ddgrs_error {
 typedef struct ddgrs_ctx ddgrs_ctx;
 typedef struct ddgrs_surface ddgrs_surface;
 typedef struct ddgrs_bh ddgrs_bh;
+typedef struct ddgrs_metric ddgrs_metric;
+typedef struct ddgrs_vfrac ddgrs_vfrac;
 
 /* ---- context ------------------------------------------------------------------------------- */
 
@@ -118,6 +120,30 @@ DDGRS_API int ddgrs_bh_tree(ddgrs_bh* bh, uint32_t* order, uint32_t* start, uint
 /* The lists: per leaf ordinal CSR offsets (leaves+1) and node indices (InteractionLists :61). */
 DDGRS_API int ddgrs_bh_lists(ddgrs_bh* bh, uint32_t* lumped_off, uint32_t* lumped, uint32_t* exact_off,
                              uint32_t* exact, ddgrs_error* err);
+
+/* ---- hierarchical metric ------------------------------------------------------------------- */
+
+/* The block cluster tree of B, B⁰ on a face BVH at the surface's current positions: separation χ,
+ * leaf size, leaf_pairs (RS/src/metric/hierarchical.rs:65 ClusterTreeOptions), near field and
+ * H̃a on the device.  Replaces FaceMetricTree::build / build_on_gpu (hierarchical.rs:296,374). */
+DDGRS_API int ddgrs_metric_build(ddgrs_surface* s, double alpha, double beta, double chi, int64_t leaf_size,
+                                 int64_t leaf_pairs, ddgrs_metric** out, ddgrs_error* err);
+DDGRS_API void ddgrs_metric_destroy(ddgrs_metric* m);
+/* ys_k = (w_hi B̃ + w_lo B̃⁰) xs_k for nb interleaved 3·nv vectors.  Replaces
+ * FaceMetricTree::apply_batch (hierarchical.rs:540) / GpuFaceMetric::apply_batch (gpu/mod.rs:318). */
+DDGRS_API int ddgrs_metric_apply_batch(ddgrs_metric* m, double w_hi, double w_lo, int64_t nb, const double* xs,
+                                       double* ys, ddgrs_error* err);
+/* [admissible blocks, inadmissible blocks, admissible pairs, near entries, nodes, depth, stored
+ * near entries] (BctStats, RS/src/spatial/block_tree.rs:401). */
+DDGRS_API int ddgrs_metric_stats(ddgrs_metric* m, int64_t* out7, ddgrs_error* err);
+
+/* The vertex-pair L̃^V_τ (dual areas of the surface) on a vertex BVH.  Replaces
+ * VertexFractionalTree::for_surface / build_on_gpu (hierarchical.rs:1157,1092). */
+DDGRS_API int ddgrs_vfrac_build(ddgrs_surface* s, double tau, double chi, int64_t leaf_size, int64_t leaf_pairs,
+                                ddgrs_vfrac** out, ddgrs_error* err);
+DDGRS_API void ddgrs_vfrac_destroy(ddgrs_vfrac* v);
+/* Replaces VertexFractionalTree::apply_batch (hierarchical.rs:1274). */
+DDGRS_API int ddgrs_vfrac_apply_batch(ddgrs_vfrac* v, int64_t nb, const double* xs, double* ys, ddgrs_error* err);
 
 #ifdef __cplusplus
 }
