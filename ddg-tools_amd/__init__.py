@@ -8,6 +8,10 @@ reference (``crates/ddg-repulsive-surfaces``):
 * ``Surface`` — the validated mesh with device-resident positions (``src/surface.rs:132,513``);
 * ``energy(surface, exponents, eval)`` — ``energy()`` (``src/energy/mod.rs:168``) for
   ``EnergyEval.Exact`` and ``EnergyEval.BarnesHut``;
+* ``BhTree``, ``FaceMetricTree``, ``VertexFractionalTree`` — the Barnes–Hut tree and the block
+  cluster tree products (``src/energy/barnes_hut.rs``, ``src/metric/hierarchical.rs``);
+* ``Saddle`` — the step's saddle solver (``hierarchical_saddle``, ``src/solve/iterative.rs:455``);
+* ``Session`` — the Hs descent session (``Session::new`` / ``step``, ``src/step/session.rs:371,724``);
 * errors raise ``RsError`` whose ``kind`` names the reference's ``RsError`` variant.
 
 The library is the only compute path: if ``lib/libddgrs_hip.so`` is missing, importing the
@@ -112,6 +116,19 @@ SIGNATURES = {
     "ddgrs_vfrac_build": (_I32, [_P, _D, _D, _I64, _I64, ctypes.POINTER(_P), _E]),
     "ddgrs_vfrac_destroy": (None, [_P]),
     "ddgrs_vfrac_apply_batch": (_I32, [_P, _I64, _P, _P, _E]),
+    "ddgrs_saddle_options_default": (None, [_P]),
+    "ddgrs_saddle_build": (_I32, [_P, _D, _D, _P, ctypes.POINTER(_P), _E]),
+    "ddgrs_saddle_destroy": (None, [_P]),
+    "ddgrs_saddle_info": (_I32, [_P, _P, _E]),
+    "ddgrs_saddle_apply_batch": (_I32, [_P, _I64, _P, _P, _E]),
+    "ddgrs_saddle_precond_batch": (_I32, [_P, _I64, _P, _P, _E]),
+    "ddgrs_saddle_lprime_batch": (_I32, [_P, _I64, _P, _P, _E]),
+    "ddgrs_saddle_solve": (_I32, [_P, _P, _P, _I64, _D, _I64, _P, _E]),
+    "ddgrs_step_options_default": (None, [_P]),
+    "ddgrs_session_create": (_I32, [_P, _P, ctypes.POINTER(_P), _E]),
+    "ddgrs_session_destroy": (None, [_P]),
+    "ddgrs_session_step": (_I32, [_P, _P, _E]),
+    "ddgrs_session_state": (_I32, [_P, _P, _E]),
 }
 
 
@@ -403,6 +420,211 @@ class VertexFractionalTree(_Handle):
 
     def apply_batch(self, x):
         return _batch(lib().ddgrs_vfrac_apply_batch, self._h, 3 * self.surface.nv, x)
+
+
+class _SaddleOptionsC(ctypes.Structure):
+    _fields_ = [("chi", _D), ("leaf_size", _I64), ("leaf_pairs", _I64), ("w_tp", _D), ("c_lo", _D), ("eps_rel", _D)]
+
+
+class _GmresReportC(ctypes.Structure):
+    _fields_ = [
+        ("iterations", _I64),
+        ("cycles", _I64),
+        ("operator_applications", _I64),
+        ("preconditioner_applications", _I64),
+        ("converged", _I32),
+        ("relative_residual", _D),
+    ]
+
+
+class _StepOptionsC(ctypes.Structure):
+    _fields_ = [
+        ("alpha", _D),
+        ("beta", _D),
+        ("energy_exact", _I32),
+        ("theta", _D),
+        ("saddle", _SaddleOptionsC),
+        ("restart", _I64),
+        ("tolerance", _D),
+        ("max_iterations", _I64),
+        ("armijo_c", _D),
+        ("tau_min_rel", _D),
+        ("max_halvings", _I64),
+    ]
+
+
+class _StepReportC(ctypes.Structure):
+    _fields_ = [
+        ("energy_before", _D),
+        ("energy_trial", _D),
+        ("gradient_norm", _D),
+        ("sobolev_norm_sq", _D),
+        ("tau", _D),
+        ("tau0", _D),
+        ("accepted", _I32),
+        ("trials", _I32),
+        ("gmres", _GmresReportC),
+        ("pairs_exact", _I64),
+        ("nodes_lumped", _I64),
+        ("seconds_trees", _D),
+        ("seconds_energy", _D),
+        ("seconds_metric", _D),
+        ("seconds_precond", _D),
+        ("seconds_solve", _D),
+        ("seconds_line_search", _D),
+        ("seconds_projection", _D),
+        ("seconds_total", _D),
+    ]
+
+
+@dataclasses.dataclass(frozen=True)
+class SaddleOptions:
+    """The step's hierarchical solver options (src/solve/iterative.rs:350 ``HierarchicalOptions``,
+    src/solve/precond.rs:466 ``PreconditionerOptions``; metric weights of ``SobolevWeights``)."""
+
+    chi: float = 0.5
+    leaf_size: int = 8
+    leaf_pairs: int = 64
+    w_tp: float = 1.0
+    c_lo: float = 1.0
+    eps_rel: float = 1e-10
+
+    def _c(self) -> _SaddleOptionsC:
+        return _SaddleOptionsC(self.chi, self.leaf_size, self.leaf_pairs, self.w_tp, self.c_lo, self.eps_rel)
+
+
+@dataclasses.dataclass(frozen=True)
+class GmresOptions:
+    """``GmresOptions`` (src/solve/gmres.rs:138): restart 50, relative tolerance 1e-6, 200 iterations."""
+
+    restart: int = 50
+    tolerance: float = 1e-6
+    max_iterations: int = 200
+
+
+@dataclasses.dataclass(frozen=True)
+class GmresReport:
+    """``GmresReport`` (src/solve/gmres.rs:200)."""
+
+    iterations: int
+    cycles: int
+    operator_applications: int
+    preconditioner_applications: int
+    converged: bool
+    relative_residual: float
+
+
+def _gmres_report(c: _GmresReportC) -> GmresReport:
+    return GmresReport(int(c.iterations), int(c.cycles), int(c.operator_applications), int(c.preconditioner_applications),
+                       bool(c.converged), float(c.relative_residual))
+
+
+class Saddle(_Handle):
+    """The step's saddle solver: A′ = [[w(B̃ + c_lo B̃⁰), C_Qᵀ], [C_Q, 0]] with the global barycenter
+    rows, its augmented preconditioner and GMRES (``hierarchical_saddle``, src/solve/iterative.rs:455;
+    ``GmresSaddle``, src/solve/gmres.rs:717).  Vectors have 3 nv + 3 entries."""
+
+    _destroy = "ddgrs_saddle_destroy"
+
+    def __init__(self, surface: Surface, exponents: Exponents = Exponents(), options: SaddleOptions = SaddleOptions()):
+        self.surface = surface
+        self.n = 3 * surface.nv + 3
+        h = _P()
+        o = options._c()
+        _call(lib().ddgrs_saddle_build, surface.handle, exponents.alpha, exponents.beta, ctypes.byref(o), ctypes.byref(h))
+        self._h = h
+
+    def info(self) -> dict:
+        o = np.zeros(2)
+        _call(lib().ddgrs_saddle_info, self._h, _ptr(o))
+        return {"eps": float(o[0]), "fractional_scale": float(o[1])}
+
+    def apply_batch(self, x):
+        return _batch(lib().ddgrs_saddle_apply_batch, self._h, self.n, x)
+
+    def precondition(self, r):
+        return _batch(lib().ddgrs_saddle_precond_batch, self._h, self.n, r)
+
+    def lprime_solve(self, r):
+        return _batch(lib().ddgrs_saddle_lprime_batch, self._h, self.n, r)
+
+    def solve(self, b, options: GmresOptions = GmresOptions()):
+        """x = A′⁻¹ b and the ``GmresReport``."""
+        bb = _f64(b, (self.n,))
+        x = np.zeros(self.n)
+        rep = _GmresReportC()
+        _call(lib().ddgrs_saddle_solve, self._h, _ptr(bb), _ptr(x), options.restart, options.tolerance,
+              options.max_iterations, ctypes.byref(rep))
+        return x, _gmres_report(rep)
+
+
+@dataclasses.dataclass(frozen=True)
+class StepOptions:
+    """A step of the Hs flow (src/step/session.rs ``FlowOptions`` with remeshing and CCD off, the
+    global barycenter constraint, ``StartRule::GrowPrevious{2, MaxDisplacement{1}}``)."""
+
+    exponents: Exponents = Exponents()
+    energy_exact: bool = False
+    theta: float = 0.5
+    saddle: SaddleOptions = SaddleOptions()
+    gmres: GmresOptions = GmresOptions()
+    armijo_c: float = 1e-2
+    tau_min_rel: float = 1e-12
+    max_halvings: int = -1
+
+    def _c(self) -> _StepOptionsC:
+        return _StepOptionsC(self.exponents.alpha, self.exponents.beta, 1 if self.energy_exact else 0, self.theta,
+                             self.saddle._c(), self.gmres.restart, self.gmres.tolerance, self.gmres.max_iterations,
+                             self.armijo_c, self.tau_min_rel, self.max_halvings)
+
+
+@dataclasses.dataclass(frozen=True)
+class StepReport:
+    """The report of one step (src/step/session.rs:146 ``StepReport``, the fields of this path)."""
+
+    energy_before: float
+    energy_trial: float
+    gradient_norm: float
+    sobolev_norm_sq: float
+    tau: float
+    tau0: float
+    accepted: bool
+    trials: int
+    gmres: GmresReport
+    pairs_exact: int
+    nodes_lumped: int
+    seconds: dict
+
+
+class Session(_Handle):
+    """A descent session on a device-resident surface (``Session``, src/step/session.rs:280): ``step()``
+    moves the surface (``Session::step``, :724)."""
+
+    _destroy = "ddgrs_session_destroy"
+
+    def __init__(self, surface: Surface, options: StepOptions = StepOptions()):
+        self.surface = surface
+        self.options = options
+        h = _P()
+        o = options._c()
+        _call(lib().ddgrs_session_create, surface.handle, ctypes.byref(o), ctypes.byref(h))
+        self._h = h
+
+    def step(self) -> StepReport:
+        r = _StepReportC()
+        _call(lib().ddgrs_session_step, self._h, ctypes.byref(r))
+        sec = {k[len("seconds_"):]: float(getattr(r, k)) for k, _ in _StepReportC._fields_ if k.startswith("seconds_")}
+        return StepReport(float(r.energy_before), float(r.energy_trial), float(r.gradient_norm), float(r.sobolev_norm_sq),
+                          float(r.tau), float(r.tau0), bool(r.accepted), int(r.trials), _gmres_report(r.gmres),
+                          int(r.pairs_exact), int(r.nodes_lumped), sec)
+
+    def state(self) -> dict:
+        o = np.zeros(5)
+        _call(lib().ddgrs_session_state, self._h, _ptr(o))
+        return {"mean_edge_length0": float(o[0]), "last_tau": float(o[1]), "barycenter_target": o[2:5].copy()}
+
+    def positions(self) -> np.ndarray:
+        return self.surface.positions()
 
 
 def energy(surface: Surface, exponents: Exponents = Exponents(), eval=EnergyEval.Exact) -> EnergyReport:

@@ -1,0 +1,248 @@
+// C ABI of the step's saddle solver and of the descent session.
+#include <cmath>
+#include <cstring>
+
+#include "capi_util.hpp"
+#include "session.hpp"
+
+using namespace ddgrs;
+
+struct ddgrs_saddle {
+  ddgrs_surface* s = nullptr;
+  FaceMetricDev fm;
+  Preconditioner pc;
+  SaddleOp op;
+  DeviceVecs vec;
+};
+
+struct ddgrs_session {
+  Session ses;
+};
+
+namespace {
+void check_saddle_options(const ddgrs_saddle_options& o) {
+  if (!(std::isfinite(o.chi) && o.chi >= 0.0)) invalid("chi", std::to_string(o.chi), "a finite number >= 0 (0: exact products)");
+  if (o.leaf_size < 1) invalid("leaf_size", std::to_string(o.leaf_size), "at least 1");
+  if (o.leaf_pairs < 0) invalid("leaf_pairs", std::to_string(o.leaf_pairs), "non-negative");
+  if (!(std::isfinite(o.w_tp) && o.w_tp > 0.0)) invalid("w_tp", std::to_string(o.w_tp), "a finite weight > 0");
+  if (!(std::isfinite(o.c_lo) && o.c_lo >= 0.0)) invalid("c_lo", std::to_string(o.c_lo), "a finite weight >= 0");
+  if (!(std::isfinite(o.eps_rel) && o.eps_rel > 0.0)) invalid("eps_rel", std::to_string(o.eps_rel), "a finite number > 0");
+}
+
+// gmres.rs:168 GmresOptions::validate
+void check_gmres(int64_t restart, double tol, int64_t maxit) {
+  if (restart < 1) invalid("gmres.restart", std::to_string(restart), "at least 1");
+  if (!(std::isfinite(tol) && tol > 0.0)) invalid("gmres.tolerance", std::to_string(tol), "a finite number > 0");
+  if (maxit < 1) invalid("gmres.max_iterations", std::to_string(maxit), "at least 1");
+}
+
+template <class F>
+void batch(ddgrs_ctx* ctx, int64_t nb, int64_t n, const double* xs, double* ys, F&& apply) {
+  DevBuf<double> dx(n), dy(n);
+  for (int64_t q = 0; q < nb; ++q) {
+    HIPCHECK(hipMemcpyAsync(dx.ptr, xs + q * n, n * 8, hipMemcpyHostToDevice, ctx->stream));
+    apply(dx.ptr, dy.ptr);
+    HIPCHECK(hipMemcpyAsync(ys + q * n, dy.ptr, n * 8, hipMemcpyDeviceToHost, ctx->stream));
+  }
+  ctx->sync();
+}
+
+void fill_report(const GmresResult& r, ddgrs_gmres_report* out) {
+  out->iterations = (int64_t)r.iterations;
+  out->cycles = (int64_t)r.cycles;
+  out->operator_applications = (int64_t)r.operator_applications;
+  out->preconditioner_applications = (int64_t)r.preconditioner_applications;
+  out->converged = r.converged ? 1 : 0;
+  out->relative_residual = r.relative_residual;
+}
+}  // namespace
+
+extern "C" {
+
+void ddgrs_saddle_options_default(ddgrs_saddle_options* o) {
+  if (!o) return;
+  o->chi = 0.5;
+  o->leaf_size = 8;
+  o->leaf_pairs = 64;
+  o->w_tp = 1.0;
+  o->c_lo = 1.0;
+  o->eps_rel = 1e-10;
+}
+
+int ddgrs_saddle_build(ddgrs_surface* s, double alpha, double beta, const ddgrs_saddle_options* o, ddgrs_saddle** out,
+                       ddgrs_error* err) {
+  return guarded(err, "saddle_build", [&] {
+    need(s, "surface");
+    need(o, "options");
+    need(out, "out");
+    *out = nullptr;
+    Exps e = make_exps(alpha, beta);
+    check_saddle_options(*o);
+    auto h = std::make_unique<ddgrs_saddle>();
+    h->s = s;
+    const double s_exp = (e.beta64 - 2.0) / e.alpha64;
+    Topology t;
+    Clusters c;
+    build_face_tree(s, (size_t)o->leaf_size, &t, &c);
+    h->fm.w_hi = o->w_tp;
+    h->fm.w_lo = o->w_tp * o->c_lo;
+    h->fm.build(s, t, c, o->chi, (size_t)o->leaf_pairs, s_exp);
+    h->pc.build(s, s_exp, o->w_tp, o->eps_rel, o->chi, (size_t)o->leaf_size, (size_t)o->leaf_pairs);
+    h->op.fm = &h->fm;
+    h->op.crow = h->pc.crow_d.ptr;
+    h->op.nv = (size_t)s->nv;
+    h->vec.init(3 * (size_t)s->nv + 3);
+    *out = h.release();
+  });
+}
+
+void ddgrs_saddle_destroy(ddgrs_saddle* h) {
+  if (!h) return;
+  try {
+    h->s->ctx->sync();
+  } catch (...) {
+  }
+  delete h;
+}
+
+int ddgrs_saddle_info(ddgrs_saddle* h, double* out2, ddgrs_error* err) {
+  return guarded(err, "saddle_info", [&] {
+    need(h, "saddle");
+    need(out2, "out");
+    out2[0] = h->pc.eps;
+    out2[1] = h->pc.frac_scale;
+  });
+}
+
+int ddgrs_saddle_apply_batch(ddgrs_saddle* h, int64_t nb, const double* xs, double* ys, ddgrs_error* err) {
+  return guarded(err, "saddle_apply", [&] {
+    need(h, "saddle");
+    need(xs, "xs");
+    need(ys, "ys");
+    ddgrs_ctx* ctx = h->s->ctx;
+    batch(ctx, nb, 3 * h->s->nv + 3, xs, ys, [&](const double* x, double* y) { h->op.apply(ctx, x, y); });
+  });
+}
+
+int ddgrs_saddle_precond_batch(ddgrs_saddle* h, int64_t nb, const double* rs, double* zs, ddgrs_error* err) {
+  return guarded(err, "saddle_precond", [&] {
+    need(h, "saddle");
+    need(rs, "rs");
+    need(zs, "zs");
+    ddgrs_ctx* ctx = h->s->ctx;
+    batch(ctx, nb, 3 * h->s->nv + 3, rs, zs, [&](const double* r, double* z) { h->pc.apply(ctx, r, z); });
+  });
+}
+
+int ddgrs_saddle_lprime_batch(ddgrs_saddle* h, int64_t nb, const double* rs, double* xs, ddgrs_error* err) {
+  return guarded(err, "saddle_lprime", [&] {
+    need(h, "saddle");
+    need(rs, "rs");
+    need(xs, "xs");
+    ddgrs_ctx* ctx = h->s->ctx;
+    batch(ctx, nb, 3 * h->s->nv + 3, rs, xs, [&](const double* r, double* x) { h->pc.lp.solve(ctx, r, x); });
+  });
+}
+
+int ddgrs_saddle_solve(ddgrs_saddle* h, const double* b, double* x, int64_t restart, double tolerance,
+                       int64_t max_iterations, ddgrs_gmres_report* report, ddgrs_error* err) {
+  return guarded(err, "saddle_solve", [&] {
+    need(h, "saddle");
+    need(b, "b");
+    need(x, "x");
+    check_gmres(restart, tolerance, max_iterations);
+    ddgrs_ctx* ctx = h->s->ctx;
+    const size_t n = 3 * (size_t)h->s->nv + 3;
+    DevBuf<double> db(n), dx(n);
+    upload(ctx, db, b, n);
+    GmresSettings gs{(size_t)restart, tolerance, (size_t)max_iterations};
+    auto A = [&](const double* u, double* y) { h->op.apply(ctx, u, y); };
+    auto P = [&](const double* r, double* z) { h->pc.apply(ctx, r, z); };
+    GmresResult r = gmres_device(ctx, n, A, P, db.ptr, dx.ptr, gs, h->vec);
+    download(ctx, x, (const double*)dx.ptr, n);
+    ctx->sync();
+    if (report) fill_report(r, report);
+  });
+}
+
+void ddgrs_step_options_default(ddgrs_step_options* o) {
+  if (!o) return;
+  o->alpha = 6.0;
+  o->beta = 12.0;
+  o->energy_exact = 0;
+  o->theta = 0.5;
+  ddgrs_saddle_options_default(&o->saddle);
+  o->restart = 50;
+  o->tolerance = 1e-6;
+  o->max_iterations = 200;
+  o->armijo_c = 1e-2;
+  o->tau_min_rel = 1e-12;
+  o->max_halvings = -1;
+}
+
+int ddgrs_session_create(ddgrs_surface* s, const ddgrs_step_options* o, ddgrs_session** out, ddgrs_error* err) {
+  return guarded(err, "session_create", [&] {
+    need(s, "surface");
+    need(o, "options");
+    need(out, "out");
+    *out = nullptr;
+    (void)make_exps(o->alpha, o->beta);
+    check_saddle_options(o->saddle);
+    check_gmres(o->restart, o->tolerance, o->max_iterations);
+    if (!o->energy_exact && !(std::isfinite(o->theta) && o->theta >= 0.0))
+      invalid("theta", std::to_string(o->theta), "a finite number >= 0");
+    if (!(std::isfinite(o->armijo_c) && o->armijo_c > 0.0 && o->armijo_c < 1.0))
+      invalid("armijo_c", std::to_string(o->armijo_c), "in (0, 1)");
+    if (!(std::isfinite(o->tau_min_rel) && o->tau_min_rel > 0.0))
+      invalid("tau_min_rel", std::to_string(o->tau_min_rel), "a finite number > 0");
+    StepOptions so;
+    so.alpha = o->alpha;
+    so.beta = o->beta;
+    so.energy_exact = o->energy_exact != 0;
+    so.theta = o->theta;
+    so.leaf_size = (size_t)o->saddle.leaf_size;
+    so.chi = o->saddle.chi;
+    so.leaf_pairs = (size_t)o->saddle.leaf_pairs;
+    so.w_tp = o->saddle.w_tp;
+    so.c_lo = o->saddle.c_lo;
+    so.eps_rel = o->saddle.eps_rel;
+    so.restart = (size_t)o->restart;
+    so.tolerance = o->tolerance;
+    so.max_iterations = (size_t)o->max_iterations;
+    so.armijo_c = o->armijo_c;
+    so.tau_min_rel = o->tau_min_rel;
+    so.max_halvings = (long)o->max_halvings;
+    auto h = std::make_unique<ddgrs_session>();
+    h->ses.init(s, so);
+    *out = h.release();
+  });
+}
+
+void ddgrs_session_destroy(ddgrs_session* h) {
+  if (!h) return;
+  try {
+    h->ses.s->ctx->sync();
+  } catch (...) {
+  }
+  delete h;
+}
+
+int ddgrs_session_step(ddgrs_session* h, ddgrs_step_report* report, ddgrs_error* err) {
+  return guarded(err, "session_step", [&] {
+    need(h, "session");
+    ddgrs_step_report local;
+    h->ses.step(report ? report : &local);
+  });
+}
+
+int ddgrs_session_state(ddgrs_session* h, double* out5, ddgrs_error* err) {
+  return guarded(err, "session_state", [&] {
+    need(h, "session");
+    need(out5, "out");
+    out5[0] = h->ses.l0;
+    out5[1] = h->ses.last_tau;
+    for (int k = 0; k < 3; ++k) out5[2 + k] = h->ses.target[k];
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,282 @@
+// One Hs descent step on the device (RS/src/step/session.rs:749 step_inner, remeshing and CCD off):
+// energy and differential (Barnes–Hut on frozen lists, or exact all pairs), the hierarchical
+// saddle solver of the step (metric tree on the energy's face tree, iterative.rs:487-560), the
+// direction by right-preconditioned GMRES, the Armijo line search with the GrowPrevious /
+// MaxDisplacement start (session.rs:991-1069, :1604), and the projection onto the global
+// barycenter constraint.  Positions and every vector stay on the device; the host reads scalars
+// (energies, the Armijo test, norms) and builds the tree structures.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "bh.hpp"
+#include "capi_util.hpp"
+#include "gmres.hpp"
+#include "session.hpp"
+
+namespace ddgrs {
+
+#pragma clang fp contract(off)
+
+// positions = p0 − τ x (x: the position block of the direction)
+__global__ __launch_bounds__(256) void trial_kernel(size_t n, const double* __restrict__ p0, const double* __restrict__ x,
+                                                    double tau, double* __restrict__ p) {
+  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = p0[i] - tau * x[i];
+}
+
+// a_v = Σ_{S∋v} a_S / 3 (faces ascending, RS/src/geometry.rs:194) and a_v f_v
+__global__ __launch_bounds__(256) void vertex_mass_kernel(int64_t nv, const uint32_t* __restrict__ inc_ptr,
+                                                          const uint32_t* __restrict__ inc, const double* __restrict__ area,
+                                                          const double* __restrict__ pos, double* __restrict__ out) {
+  int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (v >= nv) return;
+  double a = 0.0;
+  for (uint32_t q = inc_ptr[v]; q < inc_ptr[v + 1]; ++q) a += area[inc[q] >> 2] / 3.0;
+  out[v] = a;
+  for (int k = 0; k < 3; ++k) out[(size_t)nv * (1 + k) + v] = a * pos[3 * v + k];
+}
+
+__global__ __launch_bounds__(256) void translate_kernel(int64_t nv, double* __restrict__ pos, double sx, double sy,
+                                                        double sz) {
+  int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (v >= nv) return;
+  pos[3 * v] += sx;
+  pos[3 * v + 1] += sy;
+  pos[3 * v + 2] += sz;
+}
+
+// max over v of |x_v| (Euclidean) and of |x_i|: order-free maxima, two levels.
+__global__ __launch_bounds__(256) void xmax_kernel(int64_t nv, const double* __restrict__ x, double* __restrict__ part) {
+  __shared__ double s0[256], s1[256];
+  int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  double d = 0.0, m = 0.0;
+  if (v < nv) {
+    double a = x[3 * v], b = x[3 * v + 1], c = x[3 * v + 2];
+    d = sqrt(a * a + b * b + c * c);
+    m = fmax(fabs(a), fmax(fabs(b), fabs(c)));
+  }
+  s0[threadIdx.x] = d;
+  s1[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s0[threadIdx.x] = fmax(s0[threadIdx.x], s0[threadIdx.x + w]);
+      s1[threadIdx.x] = fmax(s1[threadIdx.x], s1[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = s0[0];
+    part[2 * blockIdx.x + 1] = s1[0];
+  }
+}
+
+namespace {
+double seconds_since(std::chrono::steady_clock::time_point t) {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
+}
+}  // namespace
+
+void Session::init(ddgrs_surface* s_, const StepOptions& o) {
+  s = s_;
+  opts = o;
+  exps = make_exps(o.alpha, o.beta);
+  const size_t nv = (size_t)s->nv;
+  std::vector<double> pos, mass;
+  vertex_areas_host(s, &pos, &mass);
+  // mean edge length ℓ̄₀ (RS/src/geometry.rs mean_edge_length) over the unique edges
+  std::vector<std::pair<uint32_t, uint32_t>> e;
+  for (int64_t f = 0; f < s->nf; ++f)
+    for (int c = 0; c < 3; ++c) {
+      uint32_t a = s->faces_h[3 * f + c], b = s->faces_h[3 * f + (c + 1) % 3];
+      e.push_back({std::min(a, b), std::max(a, b)});
+    }
+  std::sort(e.begin(), e.end());
+  e.erase(std::unique(e.begin(), e.end()), e.end());
+  double sum = 0.0;
+  for (auto [a, b] : e) {
+    double d[3] = {pos[3 * a] - pos[3 * b], pos[3 * a + 1] - pos[3 * b + 1], pos[3 * a + 2] - pos[3 * b + 2]};
+    sum += std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  }
+  l0 = sum / (double)e.size();
+  double tot = 0.0, bc[3] = {0.0, 0.0, 0.0};
+  for (size_t v = 0; v < nv; ++v) {
+    tot += mass[v];
+    for (int k = 0; k < 3; ++k) bc[k] += mass[v] * pos[3 * v + k];
+  }
+  for (int k = 0; k < 3; ++k) target[k] = bc[k] / tot;
+  last_tau = 0.0;
+  const size_t n = 3 * nv + 3;
+  vec.init(n);
+  g.reserve(n);
+  sol.reserve(n);
+  p0.reserve(3 * nv);
+  partials.reserve(7 * (size_t)s->nf);
+  rows.reserve((size_t)s->nf);
+  scal.reserve(16);
+  mwork.reserve(4 * nv);
+  xpart.reserve(2 * (size_t)blocks_for((int64_t)nv, 256));
+}
+
+double Session::read_scalar(const double* d) {
+  double* pin = (double*)s->ctx->staging(8);
+  download(s->ctx, pin, d, 1);
+  s->ctx->sync();
+  return pin[0];
+}
+
+double Session::energy_value() {
+  ddgrs_ctx* ctx = s->ctx;
+  if (opts.energy_exact) {
+    exact_pairs(ctx, s->nf, s->packs, s->area.ptr, exps, partials.ptr, rows.ptr);
+    sum_f64(ctx, s->nf, rows.ptr, scal.ptr);
+  } else {
+    bh.energy(ctx, exps, scal.ptr);
+  }
+  return read_scalar(scal.ptr);
+}
+
+void Session::project() {
+  ddgrs_ctx* ctx = s->ctx;
+  const int64_t nv = s->nv;
+  launch(ctx, "vertex_mass", vertex_mass_kernel, dim3(blocks_for(nv, 256)), dim3(256), 0, nv, (const uint32_t*)s->inc_ptr.ptr,
+         (const uint32_t*)s->inc.ptr, (const double*)s->area.ptr, (const double*)s->positions.ptr, mwork.ptr);
+  // Σ a_v and Σ a_v f_v as fixed-order sums
+  for (int k = 0; k < 4; ++k) sum_f64(ctx, nv, mwork.ptr + (size_t)k * nv, scal.ptr + 4 + k);
+  double* pin = (double*)ctx->staging(32);
+  download(ctx, pin, (const double*)(scal.ptr + 4), 4);
+  ctx->sync();
+  double tot = pin[0];
+  double shift[3];
+  for (int k = 0; k < 3; ++k) shift[k] = target[k] - pin[1 + k] / tot;
+  launch(ctx, "translate", translate_kernel, dim3(blocks_for(nv, 256)), dim3(256), 0, nv, s->positions.ptr, shift[0],
+         shift[1], shift[2]);
+  compute_face_geometry(s, s->packs, false);
+}
+
+void Session::step(StepReport* rep) {
+  auto t_step = std::chrono::steady_clock::now();
+  ddgrs_ctx* ctx = s->ctx;
+  const int64_t nv = s->nv, nf = s->nf;
+  const size_t nd = 3 * (size_t)nv, n = nd + 3;
+  std::memset(rep, 0, sizeof(*rep));
+  // 1. trees, lists; F, dF
+  auto t = std::chrono::steady_clock::now();
+  compute_face_geometry(s, s->packs, true);
+  Topology topo;
+  Clusters clusters;
+  if (opts.energy_exact) {
+    build_face_tree(s, opts.leaf_size, &topo, &clusters);
+  } else {
+    bh.build(s, opts.theta, opts.leaf_size);
+  }
+  rep->seconds_trees = seconds_since(t);
+  t = std::chrono::steady_clock::now();
+  if (opts.energy_exact) {
+    exact_pairs(ctx, nf, s->packs, s->area.ptr, exps, partials.ptr, rows.ptr);
+    sum_f64(ctx, nf, rows.ptr, scal.ptr);
+    rep->pairs_exact = (int64_t)nf * (nf - 1);
+  } else {
+    bh.gradient(s, exps, scal.ptr, partials.ptr);
+    rep->pairs_exact = (int64_t)bh.lists.pairs_exact;
+    rep->nodes_lumped = (int64_t)bh.lists.lumped.size();
+  }
+  pull_back(s, partials.ptr, g.ptr);
+  HIPCHECK(hipMemsetAsync(g.ptr + nd, 0, 3 * sizeof(double), ctx->stream));
+  const double f0 = read_scalar(scal.ptr);
+  const double gnorm = vec.norm_host(ctx, g.ptr);
+  rep->seconds_energy = seconds_since(t);
+  rep->energy_before = f0;
+  rep->gradient_norm = gnorm;
+  if (!std::isfinite(f0) || !std::isfinite(gnorm)) {
+    Error e(DDGRS_NON_FINITE_ENERGY, "step", "the energy or its differential is not finite");
+    throw e;
+  }
+  // 2. the solver of A′ and the direction x = A′⁻¹ [g; 0]
+  t = std::chrono::steady_clock::now();
+  const double s_exp = (exps.beta64 - 2.0) / exps.alpha64;
+  fm.w_hi = opts.w_tp;
+  fm.w_lo = opts.w_tp * opts.c_lo;
+  fm.build(s, opts.energy_exact ? topo : bh.topo, opts.energy_exact ? clusters : bh.clusters, opts.chi, opts.leaf_pairs,
+           s_exp);
+  rep->seconds_metric = seconds_since(t);
+  t = std::chrono::steady_clock::now();
+  pc.build(s, s_exp, opts.w_tp, opts.eps_rel, opts.chi, opts.leaf_size, opts.leaf_pairs);
+  rep->seconds_precond = seconds_since(t);
+  op.fm = &fm;
+  op.crow = pc.crow_d.ptr;
+  op.nv = (size_t)nv;
+  t = std::chrono::steady_clock::now();
+  GmresSettings gs{opts.restart, opts.tolerance, opts.max_iterations};
+  auto A = [&](const double* x, double* y) { op.apply(ctx, x, y); };
+  auto P = [&](const double* r, double* z) { pc.apply(ctx, r, z); };
+  GmresResult gr = gmres_device(ctx, n, A, P, g.ptr, sol.ptr, gs, vec);
+  rep->seconds_solve = seconds_since(t);
+  rep->gmres.iterations = (int64_t)gr.iterations;
+  rep->gmres.cycles = (int64_t)gr.cycles;
+  rep->gmres.operator_applications = (int64_t)gr.operator_applications;
+  rep->gmres.preconditioner_applications = (int64_t)gr.preconditioner_applications;
+  rep->gmres.converged = gr.converged ? 1 : 0;
+  rep->gmres.relative_residual = gr.relative_residual;
+  last_gmres = gr;
+  if (!gr.converged && !(gr.relative_residual <= 1e-1)) {
+    Error e(DDGRS_SOLVER_DIVERGED, "gmres", "true relative residual above 0.1 after the iteration cap");
+    e.value = gr.relative_residual;
+    throw e;
+  }
+  // ⟨g, x⟩, ‖x‖_∞ and the largest vertex displacement
+  vec.dots(ctx, g.ptr, 1, sol.ptr, scal.ptr + 1);
+  const unsigned nbv = blocks_for(nv, 256);
+  launch(ctx, "xmax", xmax_kernel, dim3(nbv), dim3(256), 0, nv, (const double*)sol.ptr, xpart.ptr);
+  std::vector<double> xp(2 * (size_t)nbv);
+  download(ctx, xp.data(), (const double*)xpart.ptr, 2 * (size_t)nbv);
+  const double slope = read_scalar(scal.ptr + 1);
+  double maxd = 0.0, xinf = 0.0;
+  for (unsigned b = 0; b < nbv; ++b) {
+    maxd = std::fmax(maxd, xp[2 * b]);
+    xinf = std::fmax(xinf, xp[2 * b + 1]);
+  }
+  rep->sobolev_norm_sq = slope;
+  // 3. line search (Armijo, halving; start GrowPrevious{2, MaxDisplacement{1}})
+  t = std::chrono::steady_clock::now();
+  const double tau0 = last_tau > 0.0 ? 2.0 * last_tau : l0 / maxd;
+  const double tau_min = opts.tau_min_rel * l0 / xinf;
+  rep->tau0 = tau0;
+  HIPCHECK(hipMemcpyAsync(p0.ptr, s->positions.ptr, nd * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+  bool accepted = false;
+  double tau = tau0;
+  if (slope > 0.0 && std::isfinite(tau0) && tau0 > 0.0) {
+    while (tau >= tau_min && (opts.max_halvings < 0 || rep->trials <= opts.max_halvings)) {
+      launch(ctx, "trial", trial_kernel, dim3(blocks_for((int64_t)nd, 256)), dim3(256), 0, nd, (const double*)p0.ptr,
+             (const double*)sol.ptr, tau, s->positions.ptr);
+      compute_face_geometry(s, s->packs, false);
+      if (!opts.energy_exact) bh.refresh(s);
+      const double ft = energy_value();
+      rep->trials++;
+      if (std::isfinite(ft) && ft <= f0 - opts.armijo_c * tau * slope) {
+        accepted = true;
+        rep->energy_trial = ft;
+        break;
+      }
+      tau *= 0.5;
+    }
+  }
+  if (!accepted) {
+    HIPCHECK(hipMemcpyAsync(s->positions.ptr, p0.ptr, nd * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    compute_face_geometry(s, s->packs, false);
+    rep->energy_trial = f0;
+  }
+  rep->seconds_line_search = seconds_since(t);
+  // 4. projection onto the barycenter constraint
+  t = std::chrono::steady_clock::now();
+  project();
+  rep->seconds_projection = seconds_since(t);
+  rep->accepted = accepted ? 1 : 0;
+  rep->tau = accepted ? tau : 0.0;
+  if (accepted) last_tau = tau;
+  rep->seconds_total = seconds_since(t_step);
+}
+
+}  // namespace ddgrs

@@ -1,0 +1,54 @@
+// The device Hs descent session (RS/src/step/session.rs): positions resident in HBM, one step =
+// energy and differential, the saddle solve for the direction, the Armijo line search and the
+// barycenter projection.
+#pragma once
+#include "bh.hpp"
+#include "gmres.hpp"
+
+namespace ddgrs {
+
+struct StepOptions {
+  double alpha = 6.0, beta = 12.0;
+  bool energy_exact = false;     // all pairs instead of Barnes–Hut
+  double theta = 0.5;            // Barnes–Hut opening parameter
+  size_t leaf_size = 8;          // face and vertex BVH leaves
+  double chi = 0.5;              // block cluster tree separation
+  size_t leaf_pairs = 64;        // dense block threshold
+  double w_tp = 1.0, c_lo = 1.0; // metric weights of B̃ and B̃⁰
+  double eps_rel = 1e-10;        // ε of K = W + ε M relative to trW / trM
+  size_t restart = 50;
+  double tolerance = 1e-6;
+  size_t max_iterations = 200;
+  double armijo_c = 1e-2;
+  double tau_min_rel = 1e-12;    // τ_min = tau_min_rel · ℓ̄₀ / ‖x‖_∞
+  long max_halvings = -1;        // no cap when negative
+};
+
+using StepReport = ddgrs_step_report;
+
+
+struct Session {
+  ddgrs_surface* s = nullptr;
+  StepOptions opts;
+  Exps exps{};
+  double l0 = 0.0, last_tau = 0.0;
+  double target[3] = {0.0, 0.0, 0.0};
+  BhEval bh;
+  FaceMetricDev fm;
+  Preconditioner pc;
+  SaddleOp op;
+  DeviceVecs vec;
+  GmresResult last_gmres;
+  DevBuf<double> g, sol, p0, partials, rows, scal, mwork, xpart;
+
+  // ℓ̄₀ and the barycenter target at the surface's current positions (session.rs:412 new).
+  void init(ddgrs_surface* s, const StepOptions& o);
+  void step(StepReport* rep);
+
+ private:
+  double read_scalar(const double* d);
+  double energy_value();
+  void project();
+};
+
+}  // namespace ddgrs

@@ -54,6 +54,8 @@ typedef struct ddgrs_surface ddgrs_surface;
 typedef struct ddgrs_bh ddgrs_bh;
 typedef struct ddgrs_metric ddgrs_metric;
 typedef struct ddgrs_vfrac ddgrs_vfrac;
+typedef struct ddgrs_saddle ddgrs_saddle;
+typedef struct ddgrs_session ddgrs_session;
 
 /* ---- context ------------------------------------------------------------------------------- */
 
@@ -144,6 +146,88 @@ DDGRS_API int ddgrs_vfrac_build(ddgrs_surface* s, double tau, double chi, int64_
 DDGRS_API void ddgrs_vfrac_destroy(ddgrs_vfrac* v);
 /* Replaces VertexFractionalTree::apply_batch (hierarchical.rs:1274). */
 DDGRS_API int ddgrs_vfrac_apply_batch(ddgrs_vfrac* v, int64_t nb, const double* xs, double* ys, ddgrs_error* err);
+
+/* ---- the step's saddle solver --------------------------------------------------------------- */
+
+/* Options of the hierarchical saddle solver (RS/src/solve/iterative.rs:350 HierarchicalOptions with
+ * RS/src/solve/precond.rs:466 PreconditionerOptions; the metric weights of SobolevWeights). */
+typedef struct ddgrs_saddle_options {
+  double chi;         /* block cluster tree separation (0: exact products) */
+  int64_t leaf_size;  /* face and vertex BVH leaf size */
+  int64_t leaf_pairs; /* blocks with at most this many pairs are dense */
+  double w_tp;        /* weight of B̃ */
+  double c_lo;        /* weight of B̃⁰ relative to w_tp */
+  double eps_rel;     /* K = W + ε M with ε = eps_rel · tr W / tr M */
+} ddgrs_saddle_options;
+
+/* The GMRES report (RS/src/solve/gmres.rs:200 GmresReport). */
+typedef struct ddgrs_gmres_report {
+  int64_t iterations, cycles, operator_applications, preconditioner_applications;
+  int32_t converged;
+  double relative_residual; /* explicit ‖b − A x‖ / ‖b‖ at the last cycle start */
+} ddgrs_gmres_report;
+
+/* Fills the reference defaults (χ 0.5, leaf 8, 64 pairs, w 1, c_lo 1, ε 1e-10). */
+DDGRS_API void ddgrs_saddle_options_default(ddgrs_saddle_options* o);
+/* The solver of A′ = [[w (B̃ + c_lo B̃⁰), C_Qᵀ], [C_Q, 0]] with the global barycenter rows C_Q
+ * (k_Q = 3) at the surface's current positions: the metric tree, L′ (sparse LDLᵀ in a nested
+ * dissection order, solved on the device) and L̃^V_{2−s}.  Replaces hierarchical_saddle
+ * (RS/src/solve/iterative.rs:455) with SparsePreconditioner::new (RS/src/solve/precond.rs:686). */
+DDGRS_API int ddgrs_saddle_build(ddgrs_surface* s, double alpha, double beta, const ddgrs_saddle_options* o,
+                                 ddgrs_saddle** out, ddgrs_error* err);
+DDGRS_API void ddgrs_saddle_destroy(ddgrs_saddle* h);
+/* [ε, fractional scale s_frac] (precond.rs:931 fractional_scale). */
+DDGRS_API int ddgrs_saddle_info(ddgrs_saddle* h, double* out2, ddgrs_error* err);
+/* y = A′ x for nb vectors of 3·nv + 3 (RS/src/solve/iterative.rs:284 apply_batch). */
+DDGRS_API int ddgrs_saddle_apply_batch(ddgrs_saddle* h, int64_t nb, const double* xs, double* ys, ddgrs_error* err);
+/* z = P⁻¹ r, the augmented preconditioner (RS/src/solve/precond.rs:1099 apply_batch → :1026). */
+DDGRS_API int ddgrs_saddle_precond_batch(ddgrs_saddle* h, int64_t nb, const double* rs, double* zs, ddgrs_error* err);
+/* x = L′⁻¹ r (RS/src/solve/precond.rs:384 LaplacianSaddle::solve). */
+DDGRS_API int ddgrs_saddle_lprime_batch(ddgrs_saddle* h, int64_t nb, const double* rs, double* xs, ddgrs_error* err);
+/* x = A′⁻¹ b by right-preconditioned restarted GMRES on the device (RS/src/solve/gmres.rs:801
+ * GmresSaddle::solve_many, options gmres.rs:138). */
+DDGRS_API int ddgrs_saddle_solve(ddgrs_saddle* h, const double* b, double* x, int64_t restart, double tolerance,
+                                 int64_t max_iterations, ddgrs_gmres_report* report, ddgrs_error* err);
+
+/* ---- the descent session -------------------------------------------------------------------- */
+
+/* Options of a step (RS/src/step/session.rs FlowOptions with the Hs method, remeshing and CCD
+ * off, the global barycenter constraint, StartRule::GrowPrevious{2, MaxDisplacement{1}}). */
+typedef struct ddgrs_step_options {
+  double alpha, beta;
+  int32_t energy_exact;    /* 1: exact all pairs; 0: Barnes–Hut */
+  double theta;            /* Barnes–Hut θ */
+  ddgrs_saddle_options saddle;
+  int64_t restart;
+  double tolerance;
+  int64_t max_iterations;
+  double armijo_c;         /* sufficient decrease constant */
+  double tau_min_rel;      /* τ_min = tau_min_rel · ℓ̄₀ / ‖x‖_∞ */
+  int64_t max_halvings;    /* < 0: no cap */
+} ddgrs_step_options;
+
+/* A step's report (RS/src/step/session.rs:146 StepReport, the fields this path produces). */
+typedef struct ddgrs_step_report {
+  double energy_before, energy_trial, gradient_norm, sobolev_norm_sq, tau, tau0;
+  int32_t accepted, trials;
+  ddgrs_gmres_report gmres;
+  int64_t pairs_exact, nodes_lumped;
+  double seconds_trees, seconds_energy, seconds_metric, seconds_precond, seconds_solve, seconds_line_search,
+      seconds_projection, seconds_total;
+} ddgrs_step_report;
+
+/* Fills the reference defaults ((6, 12), Barnes–Hut θ 0.5, restart 50, 1e-6, 200, c 1e-2). */
+DDGRS_API void ddgrs_step_options_default(ddgrs_step_options* o);
+/* A session on the surface (which it moves): ℓ̄₀ and the barycenter target at the current
+ * positions.  Replaces Session::new (RS/src/step/session.rs:371). */
+DDGRS_API int ddgrs_session_create(ddgrs_surface* s, const ddgrs_step_options* o, ddgrs_session** out,
+                                   ddgrs_error* err);
+DDGRS_API void ddgrs_session_destroy(ddgrs_session* h);
+/* One descent step; the surface holds the new positions.  Replaces Session::step
+ * (RS/src/step/session.rs:724). */
+DDGRS_API int ddgrs_session_step(ddgrs_session* h, ddgrs_step_report* report, ddgrs_error* err);
+/* [ℓ̄₀, last τ, barycenter target (3)] (session.rs:647 mean_edge_length0). */
+DDGRS_API int ddgrs_session_state(ddgrs_session* h, double* out5, ddgrs_error* err);
 
 #ifdef __cplusplus
 }

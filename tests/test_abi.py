@@ -30,3 +30,37 @@ def test_binding_covers_every_declared_symbol():
     pkg = load_package()
     missing = [s for s in declared_symbols() if s not in pkg.SIGNATURES]
     assert not missing, missing
+
+
+def _c_layout(tmp_path, structs):
+    """sizeof and field offsets of the header's structs, from gcc."""
+    import subprocess
+
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "ddgrs.h"', "int main(void) {"]
+    for name, fields in structs.items():
+        lines.append(f'  printf("{name} %zu\\n", sizeof({name}));')
+        for f in fields:
+            lines.append(f'  printf("{name}.{f} %zu\\n", offsetof({name}, {f}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    return {k: int(v) for k, v in (line.split() for line in out.splitlines())}
+
+
+def test_struct_layouts_match_binding(tmp_path):
+    pkg = load_package()
+    pairs = {
+        "ddgrs_error": pkg._Err,
+        "ddgrs_saddle_options": pkg._SaddleOptionsC,
+        "ddgrs_gmres_report": pkg._GmresReportC,
+        "ddgrs_step_options": pkg._StepOptionsC,
+        "ddgrs_step_report": pkg._StepReportC,
+    }
+    c = _c_layout(tmp_path, {k: [f for f, _ in v._fields_] for k, v in pairs.items()})
+    for name, cls in pairs.items():
+        assert c[name] == ctypes.sizeof(cls), name
+        for f, _ in cls._fields_:
+            assert c[f"{name}.{f}"] == getattr(cls, f).offset, (name, f)

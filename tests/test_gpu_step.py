@@ -1,0 +1,136 @@
+"""The step's saddle solver and the Hs descent step on the device against the FP64 oracle (GPU).
+
+* L′⁻¹ (FP64 LDLᵀ solves on the device): relative 2-norm error ≤ 1e-9; ε and the fractional scale
+  to 1e-12.
+* A′ x and P⁻¹ r (FP32 hierarchical products inside): relative 2-norm error ≤ 1e-5.
+* GMRES: converged to 1e-6 on the device's operator, iteration count within 1 of the oracle's, and
+  the solution within 1e-4 (relative 2-norm) of the oracle's.
+* The step: energy before ≤ 1e-5 relative (the Barnes–Hut energy on the same tree and θ), the
+  accepted trial energy ≤ 1e-5, the same number of line-search trials, τ ≤ 1e-4 relative, moved
+  positions ≤ 1e-4 of the displacement, the barycenter kept to 1e-12.
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(b))
+
+
+def meshes(pkg):
+    return {
+        "icosphere_l3": pkg.generate.config_icosphere_l3,
+        "torus_16k": pkg.generate.config_torus_16k,
+    }
+
+
+@pytest.mark.parametrize("name", ["icosphere_l3", "torus_16k"])
+def test_saddle_parts(pkg, ctx, name):
+    p, f = meshes(pkg)[name]()
+    s = pkg.Surface(ctx, f, p)
+    sd = pkg.Saddle(s)
+    od = oracle.Saddle(f, p)
+    info = sd.info()
+    assert abs(info["eps"] - od.eps) <= 1e-12 * od.eps
+    assert abs(info["fractional_scale"] - od.frac_scale) <= 1e-12 * od.frac_scale
+    n = od.n
+    rng = np.random.default_rng(21)
+    x = rng.standard_normal((2, n))
+    e_l = max(rel(sd.lprime_solve(x[q]), od.lprime_solve(x[q])) for q in range(2))
+    e_a = max(rel(sd.apply_batch(x[q]), od.apply(x[q])) for q in range(2))
+    e_p = max(rel(sd.precondition(x[q]), od.precond(x[q])) for q in range(2))
+    print(f"{name}: L'^-1 {e_l:.2e}  A' {e_a:.2e}  P^-1 {e_p:.2e}")
+    assert e_l <= 1e-9
+    assert e_a <= 1e-5
+    assert e_p <= 1e-5
+
+
+@pytest.mark.parametrize("name", ["icosphere_l3", "torus_16k"])
+def test_saddle_solve(pkg, ctx, name):
+    p, f = meshes(pkg)[name]()
+    s = pkg.Surface(ctx, f, p)
+    sd = pkg.Saddle(s)
+    od = oracle.Saddle(f, p)
+    e, d = pkg.exact_energy_and_differential(s) if name == "icosphere_l3" else (None, None)
+    if d is None:
+        d = pkg.BhTree(s, pkg.BhOptions(theta=0.25)).energy_and_differential(pkg.Exponents())[1]
+    b = np.zeros(od.n)
+    b[: 3 * len(p)] = np.asarray(d).reshape(-1)
+    x, rep = sd.solve(b)
+    x0, rep0 = od.solve(b)
+    err = rel(x, x0)
+    print(f"{name}: iterations {rep.iterations} (oracle {rep0['iterations']:.0f}), residual {rep.relative_residual:.2e}, "
+          f"solution rel {err:.2e}")
+    assert rep.converged and rep.relative_residual <= 1e-6
+    assert abs(rep.iterations - rep0["iterations"]) <= 1
+    assert err <= 1e-4
+    # the device solution solves the oracle's FP64 system to the metric's FP32 accuracy
+    assert rel(od.apply(x), b) <= 1e-4
+
+
+@pytest.mark.parametrize("name,theta,steps", [("icosphere_l3", 0.5, 3), ("torus_16k", 0.25, 1)])
+def test_step_parity(pkg, ctx, name, theta, steps):
+    p, f = meshes(pkg)[name]()
+    s = pkg.Surface(ctx, f, p)
+    ses = pkg.Session(s, pkg.StepOptions(theta=theta))
+    ors = oracle.Session(f, p, theta=theta)
+    st = ses.state()
+    for k in range(steps):
+        r = ses.step()
+        r0 = ors.step(threads=16)
+        x0 = ors.positions()
+        x = s.positions()
+        print(f"{name} step {k}: E {r.energy_before:.12g} / {r0['energy0']:.12g} -> {r.energy_trial:.12g} / "
+              f"{r0['energy1']:.12g}; gmres {r.gmres.iterations} / {r0['gmres_iterations']:.0f}; "
+              f"trials {r.trials} / {r0['trials']:.0f}; tau {r.tau:.6g} / {r0['tau']:.6g}")
+        assert abs(r.energy_before - r0["energy0"]) <= 1e-5 * abs(r0["energy0"])
+        assert r.accepted == bool(r0["accepted"])
+        assert r.trials == int(r0["trials"])
+        assert abs(r.tau - r0["tau"]) <= 1e-4 * r0["tau"]
+        assert abs(r.energy_trial - r0["energy1"]) <= 1e-5 * abs(r0["energy1"])
+        assert abs(r.gmres.iterations - r0["gmres_iterations"]) <= 1
+        assert r.gmres.converged and r.gmres.relative_residual <= 1e-6
+        assert r.energy_trial < r.energy_before
+        moved = np.abs(x0 - p).max()
+        assert np.abs(x - x0).max() <= 1e-4 * moved
+        p = x0
+    # the barycenter target (area-weighted, fixed at creation) is kept
+    a = oracle.vertex_areas(f, x)
+    bc = (a[:, None] * x).sum(0) / a.sum()
+    assert np.abs(bc - st["barycenter_target"]).max() <= 1e-12 * max(1.0, np.abs(x).max())
+
+
+def test_step_l6(pkg, ctx):
+    """The bench workload (81,920 faces): one step against the oracle's."""
+    p, f = pkg.generate.config_bumpy_l6()
+    s = pkg.Surface(ctx, f, p)
+    ses = pkg.Session(s)
+    ors = oracle.Session(f, p)
+    r = ses.step()
+    r0 = ors.step(threads=16)
+    x, x0 = s.positions(), ors.positions()
+    print(f"L6: E {r.energy_before:.12g} / {r0['energy0']:.12g}; gmres {r.gmres.iterations} / "
+          f"{r0['gmres_iterations']:.0f}; tau {r.tau:.6g} / {r0['tau']:.6g}; seconds {r.seconds}")
+    assert abs(r.energy_before - r0["energy0"]) <= 1e-5 * abs(r0["energy0"])
+    assert abs(r.energy_trial - r0["energy1"]) <= 1e-5 * abs(r0["energy1"])
+    assert r.trials == int(r0["trials"]) and abs(r.tau - r0["tau"]) <= 1e-4 * r0["tau"]
+    assert abs(r.gmres.iterations - r0["gmres_iterations"]) <= 1
+    assert np.abs(x - x0).max() <= 1e-4 * np.abs(x0 - p).max()
+
+
+def test_step_errors(pkg, ctx):
+    p, f = pkg.generate.config_icosphere_l3()
+    s = pkg.Surface(ctx, f, p)
+    with pytest.raises(pkg.RsError) as e:
+        pkg.Session(s, pkg.StepOptions(gmres=pkg.GmresOptions(tolerance=0.0)))
+    assert e.value.kind == "InvalidOption"
+    with pytest.raises(pkg.RsError) as e:
+        pkg.Saddle(s, options=pkg.SaddleOptions(leaf_size=0))
+    assert e.value.kind == "InvalidOption"
+    with pytest.raises(pkg.RsError) as e:
+        pkg.Session(s, pkg.StepOptions(exponents=pkg.Exponents(alpha=1.0, beta=2.0)))
+    assert e.value.kind == "InvalidExponents"
