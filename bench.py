@@ -2,13 +2,26 @@
 
 ``python bench.py [--gpus 1] [--steps K] [--warmup W]`` prints one JSON line.
 
-Workload: the bumpy icosphere at level 6 (40,962 vertices, 81,920 faces, 16 seeded bumps), inputs
-resident in HBM before the timed region.  A "step" here is the exact all-pairs tangent-point energy
-and its differential (the energy stage of an Hs step; the metric, preconditioner and GMRES stages
-are not in this build yet, see ``config.stages``).  The roofline object describes the dominant
-kernel (``exact_pairs``: FP32 VALU bound, 63 FLOP per ordered face pair), timed with HIP events on
-the library's own stream.  ``cpu_baseline`` times the FP64 oracle (a C++ restatement of the
-reference's Rust code) on the host on a bounded sample, scaled by the pair count.
+Workload: the plain Hs descent step on the bumpy icosphere at level 6 (40,962 vertices, 81,920
+faces, 16 seeded bumps; α = 6, β = 12): Barnes–Hut energy and differential (θ = 0.5, leaf 8), the
+block cluster tree metric (χ = 0.5, 64-pair dense blocks, FP32 values), the reference
+preconditioner (L′ = sparse LDLᵀ with the global barycenter row, augmented with L̃^V_{2−s}), GMRES
+(restart 50) to relative residual 1e-6 in FP64, the Armijo line search and the barycenter
+projection.  Positions live in HBM for the whole run; the timed region is K consecutive steps of
+one session started from the seeded input (after W warm-up steps of another session from the same
+input), bracketed by device synchronisations.
+
+The roofline object describes the kernel with the largest device time per step among the two
+modelled ones:
+
+* ``bh_gradient`` (FP32 VALU bound): 63 FLOP per interaction — an ordered exact face pair (S, T)
+  or a lumped (face, node) pair — value and gradient in the shared (6, 12) form;
+* ``bt_rows_c12`` (HBM bound), the near-field rows of the metric product B̃/B̃⁰ (12 channels):
+  8 bytes per near entry (two FP32 values), plus 48 bytes in and 48 bytes out per face.
+
+Both are reported (``rooflines``); their launch durations come from HIP events on the library's own
+stream during a profiled replay of the first timed step.  ``cpu_baseline`` runs the FP64 oracle (a
+C++ restatement of the reference's Rust step) on the host for the same step from the same input.
 """
 from __future__ import annotations
 
@@ -18,23 +31,40 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-FLOP_PER_PAIR = 63  # (6,12) value + gradient per ordered pair (both directions share r^-β)
+FLOP_PER_INTERACTION = 63
+BYTES_PER_NEAR_ENTRY = 8
+BYTES_PER_FACE_ROWS = 96
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
+METRIC = "Hs descent steps/sec at 81,920 faces; GMRES iters, learned vs plain precond"
+
+
+def lumped_face_node_pairs(tree: dict, lists: dict) -> int:
+    """Σ over leaves of (lumped nodes) × (faces of the leaf): the (face, node) interactions."""
+    start, end = tree["start"].astype(np.int64), tree["end"].astype(np.int64)
+    nn = start.size
+    leaf = np.ones(nn, dtype=bool)
+    leaf[:-1] = start[1:] != start[:-1]  # an internal node's next pre-order node is its left child
+    sizes = (end - start)[leaf]
+    counts = np.diff(lists["lumped_off"].astype(np.int64))
+    return int((counts * sizes).sum())
 
 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is described, not built.",
+        print(f"bench.py runs on one GPU (asked for {args.gpus}); how the step shards over GPUs is described, not built.",
               file=sys.stderr)
         return 2
 
@@ -44,52 +74,93 @@ def main() -> int:
     ctx = pkg.Context(0)
     p, f = pkg.generate.config_bumpy_l6()
     nf = f.shape[0]
+    opts = pkg.StepOptions()
     surf = pkg.Surface(ctx, f, p)
-    exps = pkg.Exponents(6.0, 12.0)
 
+    print(f"bench: warmup {args.warmup} step(s)", file=sys.stderr, flush=True)
+    ses = pkg.Session(surf, opts)
     for _ in range(args.warmup):
-        pkg.exact_energy_and_differential(surf, exps)
+        ses.step()
+    ses.close()
+
+    surf.set_positions(p)
+    ses = pkg.Session(surf, opts)
     ctx.synchronize()
+    print(f"bench: timing {args.steps} step(s)", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        e, d = pkg.exact_energy_and_differential(surf, exps)
+    reps = [ses.step() for _ in range(args.steps)]
     ctx.synchronize()
     t1 = time.perf_counter()
+    ses.close()
     ms = (t1 - t0) * 1e3 / args.steps
 
-    # Kernel-level timing pass (HIP events on the library's stream).
+    # Profiled replay of the first step (same input): per-kernel device times.
+    surf.set_positions(p)
+    bh = pkg.BhTree(surf, pkg.BhOptions(theta=opts.theta, leaf_size=opts.saddle.leaf_size))
+    counts = bh.counts()
+    face_node = lumped_face_node_pairs(bh.tree(), bh.lists())
+    bh.close()
+    fm = pkg.FaceMetricTree(surf, opts.exponents, pkg.ClusterTreeOptions(opts.saddle.chi, opts.saddle.leaf_size,
+                                                                          opts.saddle.leaf_pairs))
+    mstats = fm.stats()
+    fm.close()
+    ses = pkg.Session(surf, opts)
     ctx.set_profiling(True)
-    for _ in range(3):
-        pkg.exact_energy_and_differential(surf, exps)
+    r1 = ses.step()
     kt = ctx.kernel_times()
     ctx.set_profiling(False)
-    kp = kt["exact_pairs"]
-    kern_ms = kp["ms"] / kp["launches"]
-    flop = FLOP_PER_PAIR * nf * (nf - 1)
-    achieved = flop / (kern_ms * 1e-3) / 1e12
+    ses.close()
+    device_ms = sum(v["ms"] for v in kt.values())
+
+    rooflines = {}
+    g = kt.get("bh_gradient")
+    if g:
+        kms = g["ms"] / g["launches"]
+        flop = FLOP_PER_INTERACTION * (counts["pairs_exact"] + face_node)
+        a = flop / (kms * 1e-3) / 1e12
+        rooflines["bh_gradient"] = {
+            "bound": "valu", "achieved": a, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": a / PEAK_FP32_TFLOPS,
+            "traffic": None, "kernel_ms": kms, "ms_per_step": g["ms"],
+            "work": f"{FLOP_PER_INTERACTION} FLOP x ({counts['pairs_exact']} exact ordered pairs + {face_node} "
+                    f"lumped (face, node) pairs)",
+        }
+    m = kt.get("bt_rows_c12")
+    if m:
+        kms = m["ms"] / m["launches"]
+        nbytes = BYTES_PER_NEAR_ENTRY * mstats["near_total"] + BYTES_PER_FACE_ROWS * nf
+        a = nbytes / (kms * 1e-3) / 1e9
+        rooflines["bt_rows_c12"] = {
+            "bound": "hbm", "achieved": a, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": a / PEAK_HBM_GBS,
+            "traffic": None, "kernel_ms": kms, "ms_per_step": m["ms"], "launches_per_step": m["launches"],
+            "work": f"{BYTES_PER_NEAR_ENTRY} B x {mstats['near_total']} near entries + {BYTES_PER_FACE_ROWS} B x {nf} faces",
+        }
+    dominant = max(rooflines, key=lambda k: rooflines[k]["ms_per_step"]) if rooflines else None
+    roof = dict(rooflines[dominant], kernel=dominant) if dominant else None
 
     cpu = None
     if not args.no_cpu_baseline:
         import oracle
 
-        pt, ft = pkg.generate.config_torus_16k()
         threads = min(16, os.cpu_count() or 1)
+        print(f"bench: oracle baseline, {args.cpu_steps} step(s) on {threads} threads", file=sys.stderr, flush=True)
+        ors = oracle.Session(f, p, theta=opts.theta)
         c0 = time.perf_counter()
-        oracle.exact(ft, pt, threads=threads)
+        for _ in range(args.cpu_steps):
+            ors.step(threads=threads)
         c1 = time.perf_counter()
-        ntf = ft.shape[0]
-        pairs_per_s = ntf * (ntf - 1) / (c1 - c0)
         cpu = {
-            "value": pairs_per_s / (nf * (nf - 1)),
+            "value": args.cpu_steps / (c1 - c0),
             "unit": "steps/s",
             "cores": threads,
             "kind": "port",
-            "sample": f"FP64 oracle all-pairs energy+differential on the 16,384-face torus ({c1 - c0:.2f} s), "
-                      f"scaled by ordered pair count to 81,920 faces",
+            "sample": f"FP64 oracle Hs step (same workload, Barnes-Hut theta 0.5), the first {args.cpu_steps} steps "
+                      f"from the seeded input, {c1 - c0:.1f} s",
         }
 
+    its = [r.gmres.iterations for r in reps]
+    stage = {k: float(np.mean([r.seconds[k] for r in reps]) * 1e3) for k in reps[0].seconds}
     line = {
-        "metric": "Hs descent steps/sec at 81,920 faces; GMRES iters, learned vs plain precond",
+        "metric": METRIC,
         "value": 1e3 / ms,
         "unit": "steps/s",
         "n_gpus": 1,
@@ -99,27 +170,22 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 pair kernel, fp64 epilogue",
+        "dtype": "fp32 pair and metric kernels, fp64 solver",
         "data": "synthetic (seeded bumpy icosphere)",
         "config": {
-            "workload": "bumpy icosphere L6, 81,920 faces, exact all-pairs energy + differential",
-            "stages": "energy only (metric, preconditioner, GMRES, line search not yet built)",
+            "workload": "plain Hs descent step, bumpy icosphere L6 (40,962 vertices, 81,920 faces), Barnes-Hut "
+                        "theta 0.5, chi 0.5, GMRES restart 50 to 1e-6",
             "faces": nf,
-            "energy": e,
+            "preconditioner": "plain (reference); learned not built",
+            "gmres_iterations": its,
+            "energy": [r.energy_before for r in reps] + [reps[-1].energy_trial],
+            "stage_ms": stage,
+            "device_kernel_ms_step1": device_ms,
         },
-        "roofline": {
-            "bound": "valu",
-            "kernel": "exact_pairs",
-            "achieved": achieved,
-            "peak": PEAK_FP32_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": None,
-            "kernel_ms": kern_ms,
-            "work": f"{FLOP_PER_PAIR} FLOP x {nf}x{nf - 1} ordered pairs",
-        },
+        "roofline": roof,
+        "rooflines": rooflines,
         "cpu_baseline": cpu,
-        "kernel_times_ms": {k: v["ms"] / v["launches"] for k, v in kt.items()},
+        "kernel_ms_step1": {k: round(v["ms"], 4) for k, v in sorted(kt.items(), key=lambda kv: -kv[1]["ms"])[:16]},
     }
     print(json.dumps(line))
     return 0

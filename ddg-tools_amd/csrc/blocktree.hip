@@ -157,17 +157,20 @@ void BlockTreeDev::upload(ddgrs_ctx* ctx, const Topology& t, const BlockStructur
   near_val.reserve((size_t)std::max<uint64_t>(near_total, 1) * nk);
 }
 
+// Kernel names carry the channel count (timing per product kind).
+#define BT_NAME(base) (C == 12 ? base "_c12" : C == 3 ? base "_c3" : C == 2 ? base "_c2" : base "_c1")
+
 template <int C, int H>
 void BlockTreeDev::multiply(ddgrs_ctx* ctx, const float* psi) {
   up.reserve((size_t)nn * C);
   farv.reserve((size_t)nn * C);
   hx.reserve((size_t)n * C);
-  launch(ctx, "bt_up", bt_up_kernel<C>, dim3(blocks_for(n_needed, 4)), dim3(256), 0, n_needed,
+  launch(ctx, BT_NAME("bt_up"), bt_up_kernel<C>, dim3(blocks_for(n_needed, 4)), dim3(256), 0, n_needed,
          (const uint32_t*)needed.ptr, (const uint32_t*)start.ptr, (const uint32_t*)end.ptr, psi, up.ptr);
-  launch(ctx, "bt_far", bt_far_kernel<C, H>, dim3(blocks_for((int64_t)nn * C, 256)), dim3(256), 0, nn,
+  launch(ctx, BT_NAME("bt_far"), bt_far_kernel<C, H>, dim3(blocks_for((int64_t)nn * C, 256)), dim3(256), 0, nn,
          (const uint32_t*)far_ptr.ptr, (const uint32_t*)far_col.ptr, (const float*)far_coef.ptr, nk,
          (const float*)up.ptr, farv.ptr);
-  launch(ctx, "bt_rows", bt_rows_kernel<C, H>, dim3(blocks_for(nl, 4)), dim3(256), 0, nl, (const uint32_t*)leaves.ptr,
+  launch(ctx, BT_NAME("bt_rows"), bt_rows_kernel<C, H>, dim3(blocks_for(nl, 4)), dim3(256), 0, nl, (const uint32_t*)leaves.ptr,
          (const uint32_t*)start.ptr, (const uint32_t*)end.ptr, (const uint32_t*)chain_ptr.ptr,
          (const uint32_t*)chain.ptr, (const uint32_t*)near_ptr.ptr, (const uint32_t*)near_col.ptr,
          (const uint64_t*)near_off.ptr, (const float*)near_val.ptr, nk, (const float*)farv.ptr, psi, hx.ptr);

@@ -35,6 +35,40 @@ Exps make_exps(double alpha, double beta) {
 
 }  // namespace ddgrs
 
+namespace ddgrs {
+
+void ctx_release(ddgrs_ctx* ctx) {
+  if (!ctx || --ctx->refs > 0) return;
+  try {
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& p : ctx->pending) {
+      (void)hipEventDestroy(p.start);
+      (void)hipEventDestroy(p.stop);
+    }
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    ctx->scratch = Scratch();
+    (void)hipStreamDestroy(ctx->stream);
+  } catch (...) {
+  }
+  delete ctx;
+}
+
+void surface_retain(ddgrs_surface* s) { ++s->refs; }
+
+void surface_release(ddgrs_surface* s) {
+  if (!s || --s->refs > 0) return;
+  ddgrs_ctx* ctx = s->ctx;
+  try {
+    ctx->sync();
+  } catch (...) {
+  }
+  delete s;
+  ctx_release(ctx);
+}
+
+}  // namespace ddgrs
+
 extern "C" {
 
 int ddgrs_ctx_create(int32_t ordinal, int32_t determinism, ddgrs_ctx** out, ddgrs_error* err) {
@@ -56,22 +90,7 @@ int ddgrs_ctx_create(int32_t ordinal, int32_t determinism, ddgrs_ctx** out, ddgr
   });
 }
 
-void ddgrs_ctx_destroy(ddgrs_ctx* ctx) {
-  if (!ctx) return;
-  try {
-    (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
-    for (auto& p : ctx->pending) {
-      (void)hipEventDestroy(p.start);
-      (void)hipEventDestroy(p.stop);
-    }
-    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
-    ctx->scratch = Scratch();
-    (void)hipStreamDestroy(ctx->stream);
-  } catch (...) {
-  }
-  delete ctx;
-}
+void ddgrs_ctx_destroy(ddgrs_ctx* ctx) { ctx_release(ctx); }
 
 int ddgrs_device_info(ddgrs_ctx* ctx, char* name, int64_t name_len, int64_t* compute_units, int64_t* memory_bytes,
                       ddgrs_error* err) {
@@ -161,18 +180,12 @@ int ddgrs_surface_create(ddgrs_ctx* ctx, int64_t nv, int64_t nf, const uint32_t*
     upload(ctx, s->inc, inc);
     upload(ctx, s->positions, positions, (size_t)(3 * nv));
     compute_face_geometry(s.get(), s->packs, true);
+    ++ctx->refs;
     *out = s.release();
   });
 }
 
-void ddgrs_surface_destroy(ddgrs_surface* s) {
-  if (!s) return;
-  try {
-    s->ctx->sync();
-  } catch (...) {
-  }
-  delete s;
-}
+void ddgrs_surface_destroy(ddgrs_surface* s) { surface_release(s); }
 
 int ddgrs_surface_set_positions(ddgrs_surface* s, const double* positions, ddgrs_error* err) {
   return guarded(err, "set_positions", [&] {

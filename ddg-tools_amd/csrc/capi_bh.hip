@@ -24,18 +24,12 @@ int ddgrs_bh_build(ddgrs_surface* s, double theta, int64_t leaf_size, ddgrs_bh**
     auto h = std::make_unique<ddgrs_bh>();
     h->s = s;
     h->ev.build(s, theta, (size_t)leaf_size);
+    surface_retain(s);
     *out = h.release();
   });
 }
 
-void ddgrs_bh_destroy(ddgrs_bh* h) {
-  if (!h) return;
-  try {
-    h->s->ctx->sync();
-  } catch (...) {
-  }
-  delete h;
-}
+void ddgrs_bh_destroy(ddgrs_bh* h) { destroy_handle(h); }
 
 int ddgrs_bh_refit(ddgrs_bh* h, const double* positions, ddgrs_error* err) {
   return guarded(err, "bh_refit", [&] {

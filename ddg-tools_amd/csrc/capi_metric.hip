@@ -53,18 +53,12 @@ int ddgrs_metric_build(ddgrs_surface* s, double alpha, double beta, double chi, 
     Clusters c;
     build_face_tree(s, (size_t)leaf_size, &t, &c);
     h->fm.build(s, t, c, chi, (size_t)leaf_pairs, (e.beta64 - 2.0) / e.alpha64);
+    surface_retain(s);
     *out = h.release();
   });
 }
 
-void ddgrs_metric_destroy(ddgrs_metric* h) {
-  if (!h) return;
-  try {
-    h->s->ctx->sync();
-  } catch (...) {
-  }
-  delete h;
-}
+void ddgrs_metric_destroy(ddgrs_metric* h) { destroy_handle(h); }
 
 int ddgrs_metric_apply_batch(ddgrs_metric* h, double w_hi, double w_lo, int64_t nb, const double* xs, double* ys,
                              ddgrs_error* err) {
@@ -108,18 +102,12 @@ int ddgrs_vfrac_build(ddgrs_surface* s, double tau, double chi, int64_t leaf_siz
     std::vector<double> pos, mass;
     vertex_areas_host(s, &pos, &mass);
     h->vf.build(s->ctx, pos, mass, tau, chi, (size_t)leaf_size, (size_t)leaf_pairs);
+    surface_retain(s);
     *out = h.release();
   });
 }
 
-void ddgrs_vfrac_destroy(ddgrs_vfrac* h) {
-  if (!h) return;
-  try {
-    h->s->ctx->sync();
-  } catch (...) {
-  }
-  delete h;
-}
+void ddgrs_vfrac_destroy(ddgrs_vfrac* h) { destroy_handle(h); }
 
 int ddgrs_vfrac_apply_batch(ddgrs_vfrac* h, int64_t nb, const double* xs, double* ys, ddgrs_error* err) {
   return guarded(err, "vfrac_apply", [&] {

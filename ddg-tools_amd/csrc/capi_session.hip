@@ -16,6 +16,7 @@ struct ddgrs_saddle {
 };
 
 struct ddgrs_session {
+  ddgrs_surface* s = nullptr;
   Session ses;
 };
 
@@ -92,18 +93,12 @@ int ddgrs_saddle_build(ddgrs_surface* s, double alpha, double beta, const ddgrs_
     h->op.crow = h->pc.crow_d.ptr;
     h->op.nv = (size_t)s->nv;
     h->vec.init(3 * (size_t)s->nv + 3);
+    surface_retain(s);
     *out = h.release();
   });
 }
 
-void ddgrs_saddle_destroy(ddgrs_saddle* h) {
-  if (!h) return;
-  try {
-    h->s->ctx->sync();
-  } catch (...) {
-  }
-  delete h;
-}
+void ddgrs_saddle_destroy(ddgrs_saddle* h) { destroy_handle(h); }
 
 int ddgrs_saddle_info(ddgrs_saddle* h, double* out2, ddgrs_error* err) {
   return guarded(err, "saddle_info", [&] {
@@ -213,19 +208,14 @@ int ddgrs_session_create(ddgrs_surface* s, const ddgrs_step_options* o, ddgrs_se
     so.tau_min_rel = o->tau_min_rel;
     so.max_halvings = (long)o->max_halvings;
     auto h = std::make_unique<ddgrs_session>();
+    h->s = s;
     h->ses.init(s, so);
+    surface_retain(s);
     *out = h.release();
   });
 }
 
-void ddgrs_session_destroy(ddgrs_session* h) {
-  if (!h) return;
-  try {
-    h->ses.s->ctx->sync();
-  } catch (...) {
-  }
-  delete h;
-}
+void ddgrs_session_destroy(ddgrs_session* h) { destroy_handle(h); }
 
 int ddgrs_session_step(ddgrs_session* h, ddgrs_step_report* report, ddgrs_error* err) {
   return guarded(err, "session_step", [&] {

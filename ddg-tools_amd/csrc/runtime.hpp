@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <cstring>
 #include <map>
@@ -100,6 +101,7 @@ struct KernelTime {
 };
 
 struct ddgrs_ctx {
+  std::atomic<int> refs{1};  // the owner's reference plus one per surface
   int device = 0;
   int determinism = 1;
   hipStream_t stream = nullptr;
@@ -184,6 +186,7 @@ inline unsigned blocks_for(int64_t n, int per) { return (unsigned)((n + per - 1)
 
 // The triangle mesh and its positions on the device.
 struct ddgrs_surface {
+  std::atomic<int> refs{1};  // the owner's reference plus one per handle built on it
   ddgrs_ctx* ctx = nullptr;
   int64_t nv = 0, nf = 0;
   std::vector<uint32_t> faces_h;          // 3·nf
@@ -195,3 +198,24 @@ struct ddgrs_surface {
   ddgrs::DevBuf<double> area, normal, bary;  // nf, 3nf, 3nf
   ddgrs::FacePacks packs;                 // FP32 face data of the current positions
 };
+
+namespace ddgrs {
+// Lifetimes: a context lives until its owner and every surface on it have released it, a surface
+// until its owner and every handle built on it have; the destroy calls may come in any order.
+void ctx_release(ddgrs_ctx* ctx);
+void surface_retain(ddgrs_surface* s);
+void surface_release(ddgrs_surface* s);
+
+// Destroys a handle that holds `ddgrs_surface* s` and releases the surface.
+template <class H>
+void destroy_handle(H* h) {
+  if (!h) return;
+  ddgrs_surface* s = h->s;
+  try {
+    s->ctx->sync();
+  } catch (...) {
+  }
+  delete h;
+  surface_release(s);
+}
+}  // namespace ddgrs

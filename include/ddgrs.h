@@ -10,6 +10,10 @@
  * copies belong to the handle until it is destroyed.  One context is one in-order HIP stream; calls on
  * a context are serialized by the caller (gpu/cuda.rs:111-114).
  *
+ * Lifetimes: a context lives until it and every surface on it are destroyed, a surface until it and
+ * every handle built on it (Barnes–Hut, metric, saddle, session) are destroyed; the destroy calls may
+ * come in any order.
+ *
  * Each declaration names the reference interface it replaces (path relative to the reference root,
  * `RS/` = crates/ddg-repulsive-surfaces/).
  */

@@ -1,7 +1,13 @@
 """The step's saddle solver and the Hs descent step on the device against the FP64 oracle (GPU).
 
-* L′⁻¹ (FP64 LDLᵀ solves on the device): relative 2-norm error ≤ 1e-9; ε and the fractional scale
-  to 1e-12.
+* L′⁻¹ (FP64 LDLᵀ solves on the device): L′'s last pivot is −cᵀK⁻¹c ~ 1/ε (ε = 1e-10 tr W/tr M),
+  so any FP64 solve of it has a relative residual near 1e-10 and two solves with different
+  elimination or summation orders differ by up to ~1e-8 (scipy's spsolve against the oracle:
+  4.8e-9 on the level-3 icosphere).  The test asks for a residual against an independent scipy
+  assembly of L′ within 10× the oracle's own, and a difference from the oracle ≤ 1e-7; ε and the
+  fractional scale to 1e-12.
+* Lean golden ``rs-octa-saddle-none``: the device direction x = A′⁻¹ [g; 0] (χ = 0, FP32 metric
+  values) within 1e-5 of the fixture's x.
 * A′ x and P⁻¹ r (FP32 hierarchical products inside): relative 2-norm error ≤ 1e-5.
 * GMRES: converged to 1e-6 on the device's operator, iteration count within 1 of the oracle's, and
   the solution within 1e-4 (relative 2-norm) of the oracle's.
@@ -13,6 +19,8 @@ import numpy as np
 import pytest
 
 import oracle
+from conftest import lprime_axis_matrix, lprime_residual
+from golden import fixtures as fx
 
 pytestmark = pytest.mark.gpu
 
@@ -40,13 +48,37 @@ def test_saddle_parts(pkg, ctx, name):
     n = od.n
     rng = np.random.default_rng(21)
     x = rng.standard_normal((2, n))
-    e_l = max(rel(sd.lprime_solve(x[q]), od.lprime_solve(x[q])) for q in range(2))
+    L, _ = lprime_axis_matrix(f, p, oracle.vertex_areas(f, p))
+    e_l, res, res0 = 0.0, 0.0, 0.0
+    for q in range(2):
+        xl, xl0 = sd.lprime_solve(x[q]), od.lprime_solve(x[q])
+        e_l = max(e_l, rel(xl, xl0))
+        res, res0 = max(res, lprime_residual(L, x[q], xl)), max(res0, lprime_residual(L, x[q], xl0))
     e_a = max(rel(sd.apply_batch(x[q]), od.apply(x[q])) for q in range(2))
     e_p = max(rel(sd.precondition(x[q]), od.precond(x[q])) for q in range(2))
-    print(f"{name}: L'^-1 {e_l:.2e}  A' {e_a:.2e}  P^-1 {e_p:.2e}")
-    assert e_l <= 1e-9
+    print(f"{name}: L'^-1 {e_l:.2e} (residual {res:.2e}, oracle {res0:.2e})  A' {e_a:.2e}  P^-1 {e_p:.2e}")
+    assert res <= max(10.0 * res0, 1e-13)
+    assert e_l <= 1e-7
     assert e_a <= 1e-5
     assert e_p <= 1e-5
+
+
+def test_lean_saddle_octa(pkg, ctx):
+    c, o = fx.read_case("rs-octa-saddle-none"), fx.read_out("rs-octa-saddle-none")
+    p, f = np.array(c["positions"]), np.array(c["faces"])
+    nv = len(p)
+    g = np.array([o["rows"]["g"][i] for i in range(nv)]).reshape(-1)
+    xr = np.array([o["rows"]["x"][i] for i in range(nv)]).reshape(-1)
+    s = pkg.Surface(ctx, f, p)
+    sd = pkg.Saddle(s, pkg.Exponents(*c["exponents"]), pkg.SaddleOptions(chi=0.0))
+    b = np.zeros(3 * nv + 3)
+    b[: 3 * nv] = g
+    x, rep = sd.solve(b, pkg.GmresOptions(tolerance=1e-10, max_iterations=500))
+    err = np.abs(x[: 3 * nv] - xr).max() / np.abs(xr).max()
+    print(f"octa saddle: {rep.iterations} iterations, x rel {err:.2e}, <g,x> {g @ x[:3 * nv]:.12g} / {o['gx']:.12g}")
+    assert rep.converged
+    assert err <= 1e-5
+    assert abs(g @ x[: 3 * nv] - o["gx"]) <= 1e-5 * abs(o["gx"])
 
 
 @pytest.mark.parametrize("name", ["icosphere_l3", "torus_16k"])

@@ -8,6 +8,9 @@
   symmetric (precond.rs:596-612);
 * GMRES's solution satisfies A′ x = b to its tolerance, checked against an independent dense
   solve (gmres.rs:413);
+* Lean golden ``rs-octa-saddle-none``: the direction x = A′⁻¹ [g; 0] with the dense metric
+  (χ = 0, w = (1, 1), (4, 8), barycenter rows) and ⟨g, x⟩ — pins A′, the constraint rows and GMRES;
+* ε and L′ against an independent numpy assembly (cotan W, lumped M, barycenter row);
 * one descent step lowers the energy and keeps the barycenter.
 """
 import math
@@ -34,6 +37,36 @@ def test_lean_precond_icosa():
     z = sd.lprime_solve(v)
     zref = np.array([o["rows"]["z"][i] for i in range(nv)]).reshape(-1)
     assert np.abs(z[: 3 * nv] - zref).max() <= 1e-10 * np.abs(zref).max()
+
+
+def test_lean_saddle_octa():
+    c, o = fx.read_case("rs-octa-saddle-none"), fx.read_out("rs-octa-saddle-none")
+    assert c["weights"] == (1.0, 1.0, 0.0)
+    p, f = np.array(c["positions"]), np.array(c["faces"])
+    nv = len(p)
+    g = np.array([o["rows"]["g"][i] for i in range(nv)]).reshape(-1)
+    x = np.array([o["rows"]["x"][i] for i in range(nv)]).reshape(-1)
+    _, d, _ = oracle.exact(f, p, *c["exponents"])
+    assert np.abs(d.reshape(-1) - g).max() <= 1e-12 * np.abs(g).max()
+    sd = oracle.Saddle(f, p, *c["exponents"], chi=0.0)
+    b = np.zeros(sd.n)
+    b[: 3 * nv] = g
+    xs, rep = sd.solve(b, tol=1e-13, maxit=500)
+    assert rep["converged"] == 1.0
+    assert np.abs(xs[: 3 * nv] - x).max() <= 1e-10 * np.abs(x).max()
+    assert abs(g @ xs[: 3 * nv] - o["gx"]) <= 1e-10 * abs(o["gx"])
+
+
+def test_lprime_matches_independent_assembly():
+    from conftest import load_package, lprime_axis_matrix, lprime_residual
+
+    p, f = load_package().generate.config_icosphere_l3()
+    sd = oracle.Saddle(f, p)
+    L, eps = lprime_axis_matrix(f, p, oracle.vertex_areas(f, p))
+    assert abs(sd.eps - eps) <= 1e-13 * eps
+    r = np.random.default_rng(21).standard_normal(sd.n)
+    # L′'s last pivot is −cᵀK⁻¹c ~ 1/ε: solves keep about 1e-10 relative residual (precond.rs:16-18)
+    assert lprime_residual(L, r, sd.lprime_solve(r)) <= 1e-8
 
 
 def test_symbol_ratio_closed_form():
