@@ -16,8 +16,9 @@ modelled ones:
 
 * ``bh_gradient`` (FP32 VALU bound): 63 FLOP per interaction — an ordered exact face pair (S, T)
   or a lumped (face, node) pair — value and gradient in the shared (6, 12) form;
-* ``bt_rows_c12`` (HBM bound), the near-field rows of the metric product B̃/B̃⁰ (12 channels):
-  8 bytes per near entry (two FP32 values), plus 48 bytes in and 48 bytes out per face.
+* ``bt_panel_c12`` (HBM bound), the near-field panels of the metric product B̃/B̃⁰ (12 channels):
+  8 bytes per near entry (two FP32 values), plus 96 bytes in and 96 bytes out per face (12
+  channels as FP32 hi/lo pairs).
 
 Both are reported (``rooflines``); their launch durations come from HIP events on the library's own
 stream during a profiled replay of the first timed step.  ``cpu_baseline`` runs the FP64 oracle (a
@@ -38,7 +39,7 @@ sys.path.insert(0, ROOT)
 
 FLOP_PER_INTERACTION = 63
 BYTES_PER_NEAR_ENTRY = 8
-BYTES_PER_FACE_ROWS = 96
+BYTES_PER_FACE_ROWS = 192
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 METRIC = "Hs descent steps/sec at 81,920 faces; GMRES iters, learned vs plain precond"
@@ -124,12 +125,12 @@ def main() -> int:
             "work": f"{FLOP_PER_INTERACTION} FLOP x ({counts['pairs_exact']} exact ordered pairs + {face_node} "
                     f"lumped (face, node) pairs)",
         }
-    m = kt.get("bt_rows_c12")
+    m = kt.get("bt_panel_c12")
     if m:
         kms = m["ms"] / m["launches"]
         nbytes = BYTES_PER_NEAR_ENTRY * mstats["near_total"] + BYTES_PER_FACE_ROWS * nf
         a = nbytes / (kms * 1e-3) / 1e9
-        rooflines["bt_rows_c12"] = {
+        rooflines["bt_panel_c12"] = {
             "bound": "hbm", "achieved": a, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": a / PEAK_HBM_GBS,
             "traffic": None, "kernel_ms": kms, "ms_per_step": m["ms"], "launches_per_step": m["launches"],
             "work": f"{BYTES_PER_NEAR_ENTRY} B x {mstats['near_total']} near entries + {BYTES_PER_FACE_ROWS} B x {nf} faces",

@@ -1,16 +1,23 @@
-// Hierarchical products on the device (see blocktree.hpp).
+// Hierarchical products on the device in difference form.
 //
-// Follows RS/src/spatial/block_tree.rs:536 multiply_core (upward sums, far field per node,
-// downward accumulation, rows = far part + near blocks of the chain root → leaf, diagonal skipped)
-// and RS/src/metric/hierarchical.rs (FaceKernels :192-236, product :696: m = M x, ψ = a m,
-// z = w (H̃a·m − H̃ψ), y = 2 Σ_{S∋v} a_S (Mᵀ row)·z_S; VertexFractionalTree::product :1313).
+// Follows RS/src/spatial/block_tree.rs:536 multiply_core (far field per admissible block, near
+// blocks of the chain root → leaf, diagonal excluded) and RS/src/metric/hierarchical.rs
+// (FaceKernels :192-236, product :696: m = M x, ψ = a m, z = w (H̃a·m − H̃ψ),
+// y = 2 Σ_{S∋v} a_S (Mᵀ row)·z_S; VertexFractionalTree::product :1313).  The products are evaluated
+// as Σ_T v_ST (u_S − u_T) (see blocktree.hpp), which is the same operator.
 //
-// Device form: the near field is stored dense per block in FP32 (NK values per entry, evaluated in
-// FP64 with the reference's expressions, (min, max)-ordered so that mirrored blocks are bitwise
-// transposes); far coefficients likewise.  The upward sum of a node is the sum of ψ over its
-// contiguous BVH range (one wave per node that some far block reads), the downward accumulation is
-// folded into the row kernel (a leaf sums the far values along its chain), and a row kernel runs
-// one wave per leaf with lanes (row r, column lane j) sweeping the near blocks of the chain.
+// Precision: inputs u are FP64 split into FP32 hi/lo pairs, so u_S − u_T = (hi_S − hi_T) +
+// (lo_S − lo_T) keeps its relative accuracy however close u_S and u_T are; near values and far
+// coefficients are evaluated in FP64 and stored FP32 (a fixed perturbation of the operator, the
+// same for every product); near sums accumulate in FP32 per lane and reduce pairwise across the
+// wave; node means, the far field and the epilogue are FP64; outputs are FP32 hi/lo pairs.
+//
+// Kernels of one product: upward node sums (one thread per leaf, climbing with a per-node arrival
+// counter: the second child to arrive sums left + right, so the values do not depend on timing),
+// far field (one wave per node), leaf far terms (one thread per leaf and channel), panels (one wave
+// per panel: lanes are columns, registers hold the 8 rows × C channels).
+#include <algorithm>
+
 #include "blocktree.hpp"
 
 namespace ddgrs {
@@ -24,179 +31,396 @@ __device__ __forceinline__ float wsum64(float x) {
   x += __shfl_xor(x, 1);
   return x;
 }
-__device__ __forceinline__ float jsum8(float x) {
-  x += __shfl_xor(x, 1);
-  x += __shfl_xor(x, 2);
+__device__ __forceinline__ double wsum64d(double x) {
+  x += __shfl_xor(x, 32);
+  x += __shfl_xor(x, 16);
+  x += __shfl_xor(x, 8);
   x += __shfl_xor(x, 4);
+  x += __shfl_xor(x, 2);
+  x += __shfl_xor(x, 1);
   return x;
 }
 
-template <int C>
-__global__ __launch_bounds__(256) void bt_up_kernel(int n_needed, const uint32_t* __restrict__ needed,
-                                                    const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
-                                                    const float* __restrict__ psi, float* __restrict__ up) {
-  const int w = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  if (w >= n_needed) return;
-  const uint32_t a = needed[w];
-  float acc[C];
-#pragma unroll
-  for (int c = 0; c < C; ++c) acc[c] = 0.f;
-  for (uint32_t p = start[a] + lane; p < end[a]; p += 64)
-#pragma unroll
-    for (int c = 0; c < C; ++c) acc[c] += psi[(size_t)p * C + c];
-#pragma unroll
-  for (int c = 0; c < C; ++c) acc[c] = wsum64(acc[c]);
-  if (lane == 0)
-#pragma unroll
-    for (int c = 0; c < C; ++c) up[(size_t)a * C + c] = acc[c];
+__device__ __forceinline__ uint32_t wave_id() {
+  return __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
 }
 
-template <int C, int H>
+// 2C floats (C hi then C lo) of one element.
+template <int C>
+__device__ __forceinline__ void load_hl(const float* __restrict__ p, float* hi, float* lo) {
+  if constexpr (C % 2 == 0) {
+    const float4* q = reinterpret_cast<const float4*>(p);
+#pragma unroll
+    for (int i = 0; i < C / 2; ++i) {
+      float4 t = q[i];
+      float* dst = (4 * i < C) ? hi : lo;
+      const int o = (4 * i < C) ? 4 * i : 4 * i - C;
+      dst[o] = t.x;
+      dst[o + 1] = t.y;
+      dst[o + 2] = t.z;
+      dst[o + 3] = t.w;
+    }
+  } else {
+    const float2* q = reinterpret_cast<const float2*>(p);
+    float t[2 * C];
+#pragma unroll
+    for (int i = 0; i < C; ++i) {
+      float2 u = q[i];
+      t[2 * i] = u.x;
+      t[2 * i + 1] = u.y;
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      hi[c] = t[c];
+      lo[c] = t[C + c];
+    }
+  }
+}
+
+// ---- one product ---------------------------------------------------------------------------
+
+struct TreeArgs {
+  const uint32_t *start, *end, *right, *parent, *leaves;
+  const double *weight, *inv_mass;
+  uint32_t* counters;
+  int nl;
+};
+
+// Node sums σ_A = Σ_{T∈A} a_T u_T and means ū_A = σ_A / A_A (FP64), all nodes.
+template <int C>
+__global__ __launch_bounds__(256) void bt_upward_kernel(TreeArgs t, const float* __restrict__ u_hl,
+                                                        double* __restrict__ sigma, double* __restrict__ mean) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= t.nl) return;
+  uint32_t node = t.leaves[k];
+  double s[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) s[c] = 0.0;
+  for (uint32_t p = t.start[node]; p < t.end[node]; ++p) {
+    const double a = t.weight[p];
+    const float* q = u_hl + (size_t)p * 2 * C;
+#pragma unroll
+    for (int c = 0; c < C; ++c) s[c] += a * ((double)q[c] + (double)q[C + c]);
+  }
+  for (;;) {
+    const double im = t.inv_mass[node];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      sigma[(size_t)node * C + c] = s[c];
+      mean[(size_t)node * C + c] = s[c] * im;
+    }
+    if (node == 0) return;
+    const uint32_t p = t.parent[node];
+    __threadfence();  // publish σ_node before arriving at the parent
+    if (atomicAdd(&t.counters[p], 1u) == 0u) return;  // the sibling is not done: it will climb
+    __threadfence();  // see the sibling's σ
+    t.counters[p] = 0u;
+    const uint32_t l = p + 1, r = t.right[p];
+#pragma unroll
+    for (int c = 0; c < C; ++c) s[c] = sigma[(size_t)l * C + c] + sigma[(size_t)r * C + c];
+    node = p;
+  }
+}
+
+// γ_A = Σ_{B far of A} c_AB (ū_A − ū_B) per channel (FP64), one wave per node.
+template <int C, int H, int NK>
 __global__ __launch_bounds__(256) void bt_far_kernel(int nn, const uint32_t* __restrict__ far_ptr,
                                                      const uint32_t* __restrict__ far_col, const float* __restrict__ coef,
-                                                     int nk, const float* __restrict__ up, float* __restrict__ farv) {
-  const int t = blockIdx.x * 256 + threadIdx.x;
-  const int a = t / C, c = t % C;
-  if (a >= nn) return;
-  const int k = c < H ? 0 : nk - 1;
-  float acc = 0.f;
-  for (uint32_t q = far_ptr[a]; q < far_ptr[a + 1]; ++q) acc += coef[(size_t)q * nk + k] * up[(size_t)far_col[q] * C + c];
-  farv[(size_t)a * C + c] = acc;
+                                                     const double* __restrict__ mean, double* __restrict__ gamma) {
+  const uint32_t a = wave_id();
+  const int lane = threadIdx.x & 63;
+  if (a >= (uint32_t)nn) return;
+  const uint32_t q0 = far_ptr[a], q1 = far_ptr[a + 1];
+  double ma[C], g[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    ma[c] = mean[(size_t)a * C + c];
+    g[c] = 0.0;
+  }
+  for (uint32_t q = q0 + lane; q < q1; q += 64) {
+    const uint32_t b = far_col[q];
+    const double k0 = (double)coef[(size_t)q * NK], k1 = (double)coef[(size_t)q * NK + NK - 1];
+#pragma unroll
+    for (int c = 0; c < C; ++c) g[c] += (c < H ? k0 : k1) * (ma[c] - mean[(size_t)b * C + c]);
+  }
+  if (q1 - q0 > 0) {
+#pragma unroll
+    for (int c = 0; c < C; ++c) g[c] = wsum64d(g[c]);
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int c = 0; c < C; ++c) gamma[(size_t)a * C + c] = g[c];
+  }
 }
 
-template <int C, int H>
-__global__ __launch_bounds__(256) void bt_rows_kernel(int nl, const uint32_t* __restrict__ leaves,
-                                                      const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
-                                                      const uint32_t* __restrict__ chain_ptr,
-                                                      const uint32_t* __restrict__ chain,
-                                                      const uint32_t* __restrict__ near_ptr,
-                                                      const uint32_t* __restrict__ near_col,
-                                                      const uint64_t* __restrict__ near_off,
-                                                      const float* __restrict__ near_val, int nk,
-                                                      const float* __restrict__ farv, const float* __restrict__ psi,
-                                                      float* __restrict__ hx) {
-  const int k = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  if (k >= nl) return;
-  const int r = lane >> 3, j = lane & 7;
-  const uint32_t L = leaves[k];
-  const uint32_t s = start[L];
-  const int ns = (int)(end[L] - s);
-  float acc[C];
+// δ_L = Σ_{A ⊇ L} [α_A (ū_L − ū_A) + γ_A], from the leaf to the root (FP64).
+template <int C, int H, int NK>
+__global__ __launch_bounds__(256) void bt_leaf_far_kernel(TreeArgs t, const double* __restrict__ alpha_node,
+                                                          const double* __restrict__ mean,
+                                                          const double* __restrict__ gamma, double* __restrict__ delta) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= t.nl * C) return;
+  const int k = i / C, c = i % C;
+  const int kern = c < H ? 0 : NK - 1;
+  const uint32_t leaf = t.leaves[k];
+  const double ml = mean[(size_t)leaf * C + c];
+  double d = 0.0;
+  for (uint32_t a = leaf;; a = t.parent[a]) {
+    d += alpha_node[(size_t)a * NK + kern] * (ml - mean[(size_t)a * C + c]) + gamma[(size_t)a * C + c];
+    if (a == 0) break;
+  }
+  delta[(size_t)k * C + c] = d;
+}
+
+struct PanelArgs {
+  const uint32_t *panel_leaf, *panel_row0, *panel_rows, *col_ptr, *cols, *leaves;
+  const uint64_t* panel_val;
+  const float* val;
+  const double *alpha_leaf, *mean, *delta;
+  int np;
+};
+
+// Near field of one panel plus its far terms; z = w_k (near + ᾱ_L (u_S − ū_L) + δ_L).
+template <int C, int H, int NK>
+__global__ __launch_bounds__(256) void bt_panel_kernel(PanelArgs P, const float* __restrict__ u_hl, double w0, double w1,
+                                                       float* __restrict__ z_hl) {
+  const uint32_t p = wave_id();
+  const int lane = threadIdx.x & 63;
+  if (p >= (uint32_t)P.np) return;
+  const uint32_t k = P.panel_leaf[p], row0 = P.panel_row0[p], R = P.panel_rows[p];
+  const uint32_t c0 = P.col_ptr[k], K = P.col_ptr[k + 1] - c0;
+  const float* vbase = P.val + P.panel_val[p] * NK;
+  float acc[8][C];
 #pragma unroll
-  for (int c = 0; c < C; ++c) acc[c] = 0.f;
-  const uint32_t c0 = chain_ptr[k], c1 = chain_ptr[k + 1];
-  for (uint32_t ci = c0; ci < c1; ++ci) {
-    const uint32_t a = chain[ci];
-    const uint32_t row = s - start[a] + r;
-    for (uint32_t q = near_ptr[a]; q < near_ptr[a + 1]; ++q) {
-      const uint32_t b = near_col[q];
-      const uint32_t sb = start[b];
-      const int cb = (int)(end[b] - sb);
-      const uint64_t off = near_off[q] + (uint64_t)row * cb;
-      if (r < ns) {
-        for (int c = j; c < cb; c += 8) {
-          if (a == b && (uint32_t)c == row) continue;
-          const float* v = near_val + (off + c) * nk;
-          const float v0 = v[0], v1 = v[nk - 1];
-          const float* ps = psi + (size_t)(sb + c) * C;
+  for (int r = 0; r < 8; ++r)
 #pragma unroll
-          for (int ch = 0; ch < C; ++ch) acc[ch] += (ch < H ? v0 : v1) * ps[ch];
+    for (int c = 0; c < C; ++c) acc[r][c] = 0.f;
+  for (uint32_t base = 0; base < K; base += 64) {
+    const uint32_t kk = base + lane;
+    const bool act = kk < K;
+    const uint32_t tcol = P.cols[c0 + (act ? kk : 0)];
+    float th[C], tl[C];
+    load_hl<C>(u_hl + (size_t)tcol * 2 * C, th, tl);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      if ((uint32_t)r < R) {
+        float v0 = 0.f, v1 = 0.f;
+        if (act) {
+          const float* v = vbase + ((size_t)r * K + kk) * NK;
+          v0 = v[0];
+          v1 = v[NK - 1];
+        }
+        const float* srow = u_hl + (size_t)(row0 + r) * 2 * C;
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          const float d = (srow[c] - th[c]) + (srow[C + c] - tl[c]);
+          acc[r][c] += (c < H ? v0 : v1) * d;
         }
       }
     }
   }
+  // pairwise reduction over the wave; output (r, c) goes to lane (r C + c) mod 64
+  float mine0 = 0.f, mine1 = 0.f;
 #pragma unroll
-  for (int c = 0; c < C; ++c) acc[c] = jsum8(acc[c]);
-  if (j == 0 && r < ns) {
-    float g[C];
+  for (int r = 0; r < 8; ++r) {
+    if ((uint32_t)r < R) {
 #pragma unroll
-    for (int c = 0; c < C; ++c) g[c] = 0.f;
-    for (uint32_t ci = c0; ci < c1; ++ci) {
-      const float* f = farv + (size_t)chain[ci] * C;
-#pragma unroll
-      for (int c = 0; c < C; ++c) g[c] += f[c];
+      for (int c = 0; c < C; ++c) {
+        const float t = wsum64(acc[r][c]);
+        const int idx = r * C + c;
+        if (idx < 64) {
+          if (lane == idx) mine0 = t;
+        } else {
+          if (lane == idx - 64) mine1 = t;
+        }
+      }
     }
+  }
+  const uint32_t leaf = P.leaves[k];
 #pragma unroll
-    for (int c = 0; c < C; ++c) hx[(size_t)(s + r) * C + c] = g[c] + acc[c];
+  for (int h = 0; h < 2; ++h) {
+    const int idx = lane + 64 * h;
+    if (idx >= (int)R * C) continue;
+    const int r = idx / C, c = idx % C;
+    const int kern = c < H ? 0 : NK - 1;
+    const uint32_t srow = row0 + r;
+    const float* q = u_hl + (size_t)srow * 2 * C;
+    const double us = (double)q[c] + (double)q[C + c];
+    const double far = P.alpha_leaf[(size_t)k * NK + kern] * (us - P.mean[(size_t)leaf * C + c]) +
+                       P.delta[(size_t)k * C + c];
+    const double z = (c < H ? w0 : w1) * ((double)(h == 0 ? mine0 : mine1) + far);
+    const float zh = (float)z;
+    z_hl[(size_t)srow * 2 * C + c] = zh;
+    z_hl[(size_t)srow * 2 * C + C + c] = (float)(z - (double)zh);
   }
 }
 
-void BlockTreeDev::upload(ddgrs_ctx* ctx, const Topology& t, const BlockStructure& s, int nk_) {
+// ---- structure -----------------------------------------------------------------------------
+
+// α_A = Σ_B c_AB per kernel (FP64), one thread per node.
+__global__ __launch_bounds__(256) void bt_alpha_node_kernel(int nn, int nk, const uint32_t* __restrict__ far_ptr,
+                                                            const float* __restrict__ coef, double* __restrict__ alpha) {
+  const int a = blockIdx.x * 256 + threadIdx.x;
+  if (a >= nn) return;
+  for (int k = 0; k < nk; ++k) {
+    double s = 0.0;
+    for (uint32_t q = far_ptr[a]; q < far_ptr[a + 1]; ++q) s += (double)coef[(size_t)q * nk + k];
+    alpha[(size_t)a * nk + k] = s;
+  }
+}
+
+// ᾱ_L = Σ_{A ⊇ L} α_A (leaf to root), one thread per leaf.
+__global__ __launch_bounds__(256) void bt_alpha_leaf_kernel(int nl, int nk, const uint32_t* __restrict__ leaves,
+                                                            const uint32_t* __restrict__ parent,
+                                                            const double* __restrict__ alpha,
+                                                            double* __restrict__ alpha_leaf) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= nl) return;
+  for (int j = 0; j < nk; ++j) {
+    double s = 0.0;
+    for (uint32_t a = leaves[k];; a = parent[a]) {
+      s += alpha[(size_t)a * nk + j];
+      if (a == 0) break;
+    }
+    alpha_leaf[(size_t)k * nk + j] = s;
+  }
+}
+
+void BlockTreeDev::upload(ddgrs_ctx* ctx, const Topology& t, const BlockStructure& s, int nk_,
+                          const std::vector<double>& weight_bvh, const std::vector<double>& node_mass) {
+  constexpr uint32_t kRows = 8;
   nk = nk_;
   n = (int)t.order.size();
   nn = (int)t.n_nodes();
   nl = (int)t.leaves.size();
-  near_total = s.near_total;
   ddgrs::upload(ctx, order, t.order);
   std::vector<uint32_t> pos(n);
   for (int p = 0; p < n; ++p) pos[t.order[p]] = (uint32_t)p;
   ddgrs::upload(ctx, position, pos);
   ddgrs::upload(ctx, start, t.start);
   ddgrs::upload(ctx, end, t.end);
+  ddgrs::upload(ctx, right, t.right);
+  ddgrs::upload(ctx, parent, t.parent);
   ddgrs::upload(ctx, leaves, t.leaves);
-  ddgrs::upload(ctx, chain_ptr, s.chain_ptr);
-  ddgrs::upload(ctx, chain, s.chain);
+  // panels: per leaf, the column ranges of the near blocks of its ancestors, ascending
+  std::vector<uint32_t> cptr(nl + 1, 0), cl, pleaf, prow0, prows;
+  std::vector<uint64_t> pval;
+  std::vector<std::pair<uint32_t, uint32_t>> ranges;
+  uint64_t total = 0;
+  for (int k = 0; k < nl; ++k) {
+    ranges.clear();
+    for (uint32_t ci = s.chain_ptr[k]; ci < s.chain_ptr[k + 1]; ++ci) {
+      const uint32_t a = s.chain[ci];
+      for (uint32_t q = s.near_ptr[a]; q < s.near_ptr[a + 1]; ++q) {
+        const uint32_t b = s.near_col[q];
+        ranges.push_back({t.start[b], t.end[b]});
+      }
+    }
+    std::sort(ranges.begin(), ranges.end());
+    for (auto [b0, b1] : ranges)
+      for (uint32_t x = b0; x < b1; ++x) cl.push_back(x);
+    cptr[k + 1] = (uint32_t)cl.size();
+    const uint32_t K = cptr[k + 1] - cptr[k];
+    const uint32_t leaf = t.leaves[k];
+    for (uint32_t r0 = t.start[leaf]; r0 < t.end[leaf]; r0 += kRows) {
+      const uint32_t rows = std::min(kRows, t.end[leaf] - r0);
+      pleaf.push_back((uint32_t)k);
+      prow0.push_back(r0);
+      prows.push_back(rows);
+      pval.push_back(total);
+      total += (uint64_t)rows * K;
+    }
+  }
+  np = (int)pleaf.size();
+  near_total = total;
+  ddgrs::upload(ctx, col_ptr, cptr);
+  ddgrs::upload(ctx, cols, cl);
+  ddgrs::upload(ctx, panel_leaf, pleaf);
+  ddgrs::upload(ctx, panel_row0, prow0);
+  ddgrs::upload(ctx, panel_rows, prows);
+  ddgrs::upload(ctx, panel_val, pval);
+  val.reserve((size_t)std::max<uint64_t>(total, 1) * nk);
+  // far field
   ddgrs::upload(ctx, far_ptr, s.far_ptr);
   ddgrs::upload(ctx, far_col, s.far_col);
-  std::vector<uint32_t> far_row(s.far_col.size());
+  std::vector<uint32_t> frow(s.far_col.size());
   for (int a = 0; a < nn; ++a)
-    for (uint32_t q = s.far_ptr[a]; q < s.far_ptr[a + 1]; ++q) far_row[q] = (uint32_t)a;
-  ddgrs::upload(ctx, this->far_row, far_row);
-  ddgrs::upload(ctx, near_ptr, s.near_ptr);
-  ddgrs::upload(ctx, near_col, s.near_col);
-  ddgrs::upload(ctx, near_row, s.near_row);
-  ddgrs::upload(ctx, near_off, s.near_off);
-  std::vector<uint32_t> nd;
-  for (int a = 0; a < nn; ++a)
-    if (s.far_needed[a]) nd.push_back((uint32_t)a);
-  n_needed = (int)nd.size();
-  ddgrs::upload(ctx, needed, nd);
+    for (uint32_t q = s.far_ptr[a]; q < s.far_ptr[a + 1]; ++q) frow[q] = (uint32_t)a;
+  ddgrs::upload(ctx, far_row, frow);
   n_far = (uint32_t)s.far_col.size();
-  n_near_blocks = (uint32_t)s.near_col.size();
   far_coef.reserve((size_t)std::max<uint32_t>(n_far, 1) * nk);
-  near_val.reserve((size_t)std::max<uint64_t>(near_total, 1) * nk);
+  ddgrs::upload(ctx, weight, weight_bvh);
+  std::vector<double> im(nn);
+  for (int a = 0; a < nn; ++a) im[a] = 1.0 / node_mass[a];
+  ddgrs::upload(ctx, inv_mass, im);
+  counters.reserve(nn);
+  HIPCHECK(hipMemsetAsync(counters.ptr, 0, (size_t)nn * sizeof(uint32_t), ctx->stream));
+}
+
+void BlockTreeDev::finish_far(ddgrs_ctx* ctx) {
+  alpha_node.reserve((size_t)nn * nk);
+  alpha_leaf.reserve((size_t)nl * nk);
+  launch(ctx, "bt_alpha_node", bt_alpha_node_kernel, dim3(blocks_for(nn, 256)), dim3(256), 0, nn, nk,
+         (const uint32_t*)far_ptr.ptr, (const float*)far_coef.ptr, alpha_node.ptr);
+  launch(ctx, "bt_alpha_leaf", bt_alpha_leaf_kernel, dim3(blocks_for(nl, 256)), dim3(256), 0, nl, nk,
+         (const uint32_t*)leaves.ptr, (const uint32_t*)parent.ptr, (const double*)alpha_node.ptr, alpha_leaf.ptr);
 }
 
 // Kernel names carry the channel count (timing per product kind).
-#define BT_NAME(base) (C == 12 ? base "_c12" : C == 3 ? base "_c3" : C == 2 ? base "_c2" : base "_c1")
+#define BT_NAME(base) (C == 12 ? base "_c12" : C == 3 ? base "_c3" : base "_c")
 
 template <int C, int H>
-void BlockTreeDev::multiply(ddgrs_ctx* ctx, const float* psi) {
-  up.reserve((size_t)nn * C);
-  farv.reserve((size_t)nn * C);
-  hx.reserve((size_t)n * C);
-  launch(ctx, BT_NAME("bt_up"), bt_up_kernel<C>, dim3(blocks_for(n_needed, 4)), dim3(256), 0, n_needed,
-         (const uint32_t*)needed.ptr, (const uint32_t*)start.ptr, (const uint32_t*)end.ptr, psi, up.ptr);
-  launch(ctx, BT_NAME("bt_far"), bt_far_kernel<C, H>, dim3(blocks_for((int64_t)nn * C, 256)), dim3(256), 0, nn,
-         (const uint32_t*)far_ptr.ptr, (const uint32_t*)far_col.ptr, (const float*)far_coef.ptr, nk,
-         (const float*)up.ptr, farv.ptr);
-  launch(ctx, BT_NAME("bt_rows"), bt_rows_kernel<C, H>, dim3(blocks_for(nl, 4)), dim3(256), 0, nl, (const uint32_t*)leaves.ptr,
-         (const uint32_t*)start.ptr, (const uint32_t*)end.ptr, (const uint32_t*)chain_ptr.ptr,
-         (const uint32_t*)chain.ptr, (const uint32_t*)near_ptr.ptr, (const uint32_t*)near_col.ptr,
-         (const uint64_t*)near_off.ptr, (const float*)near_val.ptr, nk, (const float*)farv.ptr, psi, hx.ptr);
+void BlockTreeDev::multiply(ddgrs_ctx* ctx, const float* u_hl, double w0, double w1, float* z_hl) {
+  sigma.reserve((size_t)nn * C);
+  mean.reserve((size_t)nn * C);
+  gamma.reserve((size_t)nn * C);
+  delta.reserve((size_t)nl * C);
+  TreeArgs ta{start.ptr, end.ptr, right.ptr, parent.ptr, leaves.ptr, weight.ptr, inv_mass.ptr, counters.ptr, nl};
+  launch(ctx, BT_NAME("bt_upward"), bt_upward_kernel<C>, dim3(blocks_for(nl, 256)), dim3(256), 0, ta, u_hl, sigma.ptr,
+         mean.ptr);
+  if (nk == 2) {
+    launch(ctx, BT_NAME("bt_far"), bt_far_kernel<C, H, 2>, dim3(blocks_for(nn, 4)), dim3(256), 0, nn,
+           (const uint32_t*)far_ptr.ptr, (const uint32_t*)far_col.ptr, (const float*)far_coef.ptr,
+           (const double*)mean.ptr, gamma.ptr);
+    launch(ctx, BT_NAME("bt_leaf_far"), bt_leaf_far_kernel<C, H, 2>, dim3(blocks_for((int64_t)nl * C, 256)), dim3(256),
+           0, ta, (const double*)alpha_node.ptr, (const double*)mean.ptr, (const double*)gamma.ptr, delta.ptr);
+  } else {
+    launch(ctx, BT_NAME("bt_far"), bt_far_kernel<C, H, 1>, dim3(blocks_for(nn, 4)), dim3(256), 0, nn,
+           (const uint32_t*)far_ptr.ptr, (const uint32_t*)far_col.ptr, (const float*)far_coef.ptr,
+           (const double*)mean.ptr, gamma.ptr);
+    launch(ctx, BT_NAME("bt_leaf_far"), bt_leaf_far_kernel<C, H, 1>, dim3(blocks_for((int64_t)nl * C, 256)), dim3(256),
+           0, ta, (const double*)alpha_node.ptr, (const double*)mean.ptr, (const double*)gamma.ptr, delta.ptr);
+  }
+  PanelArgs pa{panel_leaf.ptr, panel_row0.ptr, panel_rows.ptr, col_ptr.ptr,        cols.ptr, leaves.ptr,
+               panel_val.ptr,  val.ptr,        alpha_leaf.ptr, (const double*)mean.ptr, (const double*)delta.ptr, np};
+  if (nk == 2)
+    launch(ctx, BT_NAME("bt_panel"), bt_panel_kernel<C, H, 2>, dim3(blocks_for(np, 4)), dim3(256), 0, pa, u_hl, w0, w1,
+           z_hl);
+  else
+    launch(ctx, BT_NAME("bt_panel"), bt_panel_kernel<C, H, 1>, dim3(blocks_for(np, 4)), dim3(256), 0, pa, u_hl, w0, w1,
+           z_hl);
 }
 
-template void BlockTreeDev::multiply<12, 9>(ddgrs_ctx*, const float*);
-template void BlockTreeDev::multiply<2, 1>(ddgrs_ctx*, const float*);
-template void BlockTreeDev::multiply<3, 3>(ddgrs_ctx*, const float*);
-template void BlockTreeDev::multiply<1, 1>(ddgrs_ctx*, const float*);
+template void BlockTreeDev::multiply<12, 9>(ddgrs_ctx*, const float*, double, double, float*);
+template void BlockTreeDev::multiply<3, 3>(ddgrs_ctx*, const float*, double, double, float*);
 
 // ---- face metric ---------------------------------------------------------------------------
 
 #pragma clang fp contract(off)
 
 struct NodeGeo {
-  const double *cen, *proj;  // 3, 6 per node
+  const double *cen, *proj, *mass;  // 3, 6, 1 per node
 };
 
 // Far coefficients [ρ^{−2s}, ½(vᵀP̄_I v + vᵀP̄_J v) ρ^{−4} ρ^{−2s}] at the centroids, (min, max)
-// ordered (hierarchical.rs:185 hi_lo, :206 far).
+// ordered (hierarchical.rs:185 hi_lo, :206 far), times the column cluster's area.
 __global__ __launch_bounds__(256) void fm_far_coef_kernel(uint32_t nfar, const uint32_t* __restrict__ far_row,
                                                           const uint32_t* __restrict__ far_col, NodeGeo g, double neg_s,
                                                           float* __restrict__ coef) {
   uint32_t q = blockIdx.x * 256 + threadIdx.x;
   if (q >= nfar) return;
-  uint32_t a = far_row[q], b = far_col[q];
+  const uint32_t col = far_col[q];
+  uint32_t a = far_row[q], b = col;
   if (a > b) {
     uint32_t t = a;
     a = b;
@@ -212,54 +436,55 @@ __global__ __launch_bounds__(256) void fm_far_coef_kernel(uint32_t nfar, const u
   double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
   double hi = pow(r2, neg_s);
   double lo = 0.5 * (pq(a) + pq(b)) / (r2 * r2) * hi;
-  coef[2 * (size_t)q] = (float)hi;
-  coef[2 * (size_t)q + 1] = (float)lo;
+  const double m = g.mass[col];
+  coef[2 * (size_t)q] = (float)(hi * m);
+  coef[2 * (size_t)q + 1] = (float)(lo * m);
 }
 
-// Near entries of one block per wave, lane = r·cb + c (≤ 64 entries per block).
-__global__ __launch_bounds__(256) void fm_near_kernel(uint32_t nblocks, const uint32_t* __restrict__ near_row,
-                                                      const uint32_t* __restrict__ near_col,
-                                                      const uint64_t* __restrict__ near_off,
-                                                      const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
-                                                      const uint32_t* __restrict__ order, const double* __restrict__ bary,
-                                                      const double* __restrict__ nrm, double neg_s,
-                                                      float* __restrict__ val) {
-  const uint32_t q = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  if (q >= nblocks) return;
-  const uint32_t a = near_row[q], b = near_col[q];
-  const uint32_t ra = end[a] - start[a], cb = end[b] - start[b];
-  if (lane >= ra * cb) return;
-  const uint32_t r = lane / cb, c = lane % cb;
-  uint32_t i = order[start[a] + r], j = order[start[b] + c];
-  float* out = val + 2 * (near_off[q] + lane);
-  if (i == j) {
-    out[0] = 0.f;
-    out[1] = 0.f;
-    return;
+// Near values of one panel per wave: rows r, columns kk (lanes), (min, max)-ordered kernels times
+// the column face's area; the diagonal is 0.
+__global__ __launch_bounds__(256) void fm_panel_values_kernel(PanelArgs P, const uint32_t* __restrict__ order,
+                                                              const double* __restrict__ bary,
+                                                              const double* __restrict__ nrm,
+                                                              const double* __restrict__ area, double neg_s,
+                                                              float* __restrict__ val) {
+  const uint32_t p = wave_id();
+  const int lane = threadIdx.x & 63;
+  if (p >= (uint32_t)P.np) return;
+  const uint32_t k = P.panel_leaf[p], row0 = P.panel_row0[p], R = P.panel_rows[p];
+  const uint32_t c0 = P.col_ptr[k], K = P.col_ptr[k + 1] - c0;
+  float* out = val + 2 * P.panel_val[p];
+  for (uint32_t r = 0; r < R; ++r) {
+    const uint32_t s0 = order[row0 + r];
+    for (uint32_t kk = lane; kk < K; kk += 64) {
+      const uint32_t t0 = order[P.cols[c0 + kk]];
+      float* o = out + 2 * ((size_t)r * K + kk);
+      if (s0 == t0) {
+        o[0] = 0.f;
+        o[1] = 0.f;
+        continue;
+      }
+      const uint32_t i = s0 < t0 ? s0 : t0, j = s0 < t0 ? t0 : s0;
+      double d[3];
+      for (int x = 0; x < 3; ++x) d[x] = bary[3 * i + x] - bary[3 * j + x];
+      double ns = nrm[3 * i] * d[0] + nrm[3 * i + 1] * d[1] + nrm[3 * i + 2] * d[2];
+      double nt = nrm[3 * j] * d[0] + nrm[3 * j + 1] * d[1] + nrm[3 * j + 2] * d[2];
+      double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      double hi = pow(r2, neg_s);
+      double lo = 0.5 * (ns * ns + nt * nt) / (r2 * r2) * hi;
+      const double at = area[t0];
+      o[0] = (float)(hi * at);
+      o[1] = (float)(lo * at);
+    }
   }
-  if (i > j) {
-    uint32_t t = i;
-    i = j;
-    j = t;
-  }
-  double d[3];
-  for (int k = 0; k < 3; ++k) d[k] = bary[3 * i + k] - bary[3 * j + k];
-  double ns = nrm[3 * i] * d[0] + nrm[3 * i + 1] * d[1] + nrm[3 * i + 2] * d[2];
-  double nt = nrm[3 * j] * d[0] + nrm[3 * j + 1] * d[1] + nrm[3 * j + 2] * d[2];
-  double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-  double hi = pow(r2, neg_s);
-  double lo = 0.5 * (ns * ns + nt * nt) / (r2 * r2) * hi;
-  out[0] = (float)hi;
-  out[1] = (float)lo;
 }
 
 // Per BVH position: D_f coefficients g_c = (n/|n|²) × e_opp(c) (RS/src/metric/mod.rs:120), area,
-// vertices, and ψ = (a, a) for H̃a.
+// vertices.
 __global__ __launch_bounds__(256) void fm_face_kernel(int nf, const uint32_t* __restrict__ order,
                                                       const uint32_t* __restrict__ faces, const double* __restrict__ pos,
                                                       const double* __restrict__ area, float* __restrict__ df,
-                                                      float* __restrict__ area_p, uint32_t* __restrict__ vert_p,
-                                                      float* __restrict__ psi2) {
+                                                      double* __restrict__ area_p, uint32_t* __restrict__ vert_p) {
   int p = blockIdx.x * 256 + threadIdx.x;
   if (p >= nf) return;
   uint32_t s = order[p];
@@ -284,54 +509,41 @@ __global__ __launch_bounds__(256) void fm_face_kernel(int nf, const uint32_t* __
     double g[3] = {w[1] * e[2] - w[2] * e[1], w[2] * e[0] - w[0] * e[2], w[0] * e[1] - w[1] * e[0]};
     for (int k = 0; k < 3; ++k) df[9 * p + 3 * c + k] = (float)g[k];
   }
-  float a = (float)area[s];
-  area_p[p] = a;
-  psi2[2 * p] = a;
-  psi2[2 * p + 1] = a;
+  area_p[p] = area[s];
 }
 
-#pragma clang fp contract(fast)
+__device__ __forceinline__ void split_store(double x, float* hi, float* lo) {
+  const float h = (float)x;
+  *hi = h;
+  *lo = (float)(x - (double)h);
+}
 
-// m = M x (9 gradient channels 3k + r, then 3 average channels), ψ = a m, per BVH position.
+// m = M x in FP64 (9 gradient channels 3k + r, then 3 average channels) per BVH position, stored
+// as hi/lo pairs.
 __global__ __launch_bounds__(256) void fm_m_kernel(int nf, const uint32_t* __restrict__ vert_p,
-                                                   const float* __restrict__ df, const float* __restrict__ area_p,
-                                                   const double* __restrict__ x, float* __restrict__ m,
-                                                   float* __restrict__ psi) {
+                                                   const float* __restrict__ df, const double* __restrict__ x,
+                                                   float* __restrict__ m_hl) {
   int p = blockIdx.x * 256 + threadIdx.x;
   if (p >= nf) return;
-  float xv[3][3];
+  double xv[3][3];
   for (int c = 0; c < 3; ++c) {
     uint32_t v = vert_p[3 * p + c];
-    for (int k = 0; k < 3; ++k) xv[c][k] = (float)x[3 * v + k];
+    for (int k = 0; k < 3; ++k) xv[c][k] = x[3 * v + k];
   }
   const float* g = df + 9 * p;
-  float out[12];
+  float* o = m_hl + 24 * (size_t)p;
   for (int k = 0; k < 3; ++k)
-    for (int r = 0; r < 3; ++r) out[3 * k + r] = xv[0][k] * g[r] + xv[1][k] * g[3 + r] + xv[2][k] * g[6 + r];
-  for (int k = 0; k < 3; ++k) out[9 + k] = (xv[0][k] + xv[1][k] + xv[2][k]) * (1.f / 3.f);
-  float a = area_p[p];
-  for (int c = 0; c < 12; ++c) {
-    m[12 * (size_t)p + c] = out[c];
-    psi[12 * (size_t)p + c] = a * out[c];
-  }
+    for (int r = 0; r < 3; ++r) {
+      double t = xv[0][k] * (double)g[r] + xv[1][k] * (double)g[3 + r] + xv[2][k] * (double)g[6 + r];
+      split_store(t, o + 3 * k + r, o + 12 + 3 * k + r);
+    }
+  for (int k = 0; k < 3; ++k) split_store((xv[0][k] + xv[1][k] + xv[2][k]) / 3.0, o + 9 + k, o + 21 + k);
 }
 
-// z = w (H̃a·m − H̃ψ) per BVH position and channel.
-__global__ __launch_bounds__(256) void fm_z_kernel(int nf, const float* __restrict__ ha, const float* __restrict__ m,
-                                                   const float* __restrict__ hx, float w_hi, float w_lo,
-                                                   float* __restrict__ z) {
-  int t = blockIdx.x * 256 + threadIdx.x;
-  if (t >= nf * 12) return;
-  int p = t / 12, c = t % 12;
-  int kern = c < 9 ? 0 : 1;
-  float w = c < 9 ? w_hi : w_lo;
-  z[t] = w * (ha[2 * p + kern] * m[t] - hx[t]);
-}
-
-// y_v = 2 Σ_{S∋v} a_S (Mᵀ row of S at v)·z_S (faces ascending; FP64 accumulation).
+// y_v = 2 Σ_{S∋v} a_S (Mᵀ row of S at v)·z_S (faces ascending; FP64).
 __global__ __launch_bounds__(256) void fm_gather_kernel(int nv, const uint32_t* __restrict__ inc_ptr,
                                                         const uint32_t* __restrict__ inc, const float* __restrict__ df,
-                                                        const float* __restrict__ area_p, const float* __restrict__ z,
+                                                        const double* __restrict__ area_p, const float* __restrict__ z_hl,
                                                         double* __restrict__ y) {
   int v = blockIdx.x * 256 + threadIdx.x;
   if (v >= nv) return;
@@ -339,12 +551,14 @@ __global__ __launch_bounds__(256) void fm_gather_kernel(int nv, const uint32_t* 
   for (uint32_t q = inc_ptr[v]; q < inc_ptr[v + 1]; ++q) {
     uint32_t e = inc[q];
     uint32_t p = e >> 2, c = e & 3u;
-    float a = area_p[p];
+    const double a = area_p[p];
     const float* g = df + 9 * (size_t)p + 3 * c;
-    const float* zs = z + 12 * (size_t)p;
+    const float* zs = z_hl + 24 * (size_t)p;
+    double z[12];
+    for (int i = 0; i < 12; ++i) z[i] = (double)zs[i] + (double)zs[12 + i];
     for (int k = 0; k < 3; ++k) {
-      float t = a * (g[0] * zs[3 * k] + g[1] * zs[3 * k + 1] + g[2] * zs[3 * k + 2]) + a * (1.f / 3.f) * zs[9 + k];
-      acc[k] += (double)t;
+      double t = (double)g[0] * z[3 * k] + (double)g[1] * z[3 * k + 1] + (double)g[2] * z[3 * k + 2] + z[9 + k] / 3.0;
+      acc[k] += a * t;
     }
   }
   for (int k = 0; k < 3; ++k) y[3 * (size_t)v + k] = 2.0 * acc[k];
@@ -357,29 +571,31 @@ void FaceMetricDev::build(ddgrs_surface* s, const Topology& t, const Clusters& c
   nv = (int)s->nv;
   nf = (int)s->nf;
   bs = build_block_structure(t, cl, chi, leaf_pairs);
-  bt.upload(ctx, t, bs, 2);
-  // node geometry (build-time centroids and projectors)
-  DevBuf<double> cen, proj;
+  std::vector<double> area_h(nf), wbvh(nf);
+  download(ctx, area_h.data(), (const double*)s->area.ptr, (size_t)nf);
+  ctx->sync();
+  for (int p = 0; p < nf; ++p) wbvh[p] = area_h[t.order[p]];
+  bt.upload(ctx, t, bs, 2, wbvh, cl.mass);
+  DevBuf<double> cen, proj, nmass;
   ddgrs::upload(ctx, cen, cl.centroid);
   ddgrs::upload(ctx, proj, cl.projector);
+  ddgrs::upload(ctx, nmass, cl.mass);
   const double neg_s = -s_exp;
   launch(ctx, "fm_far_coef", fm_far_coef_kernel, dim3(blocks_for(bt.n_far, 256)), dim3(256), 0, bt.n_far,
-         (const uint32_t*)bt.far_row.ptr, (const uint32_t*)bt.far_col.ptr, NodeGeo{cen.ptr, proj.ptr}, neg_s,
+         (const uint32_t*)bt.far_row.ptr, (const uint32_t*)bt.far_col.ptr, NodeGeo{cen.ptr, proj.ptr, nmass.ptr}, neg_s,
          bt.far_coef.ptr);
-  launch(ctx, "fm_near", fm_near_kernel, dim3(blocks_for(bt.n_near_blocks, 4)), dim3(256), 0, bt.n_near_blocks,
-         (const uint32_t*)bt.near_row.ptr, (const uint32_t*)bt.near_col.ptr, (const uint64_t*)bt.near_off.ptr,
-         (const uint32_t*)bt.start.ptr, (const uint32_t*)bt.end.ptr, (const uint32_t*)bt.order.ptr,
-         (const double*)s->bary.ptr, (const double*)s->normal.ptr, neg_s, bt.near_val.ptr);
+  bt.finish_far(ctx);
+  PanelArgs pa{bt.panel_leaf.ptr, bt.panel_row0.ptr, bt.panel_rows.ptr, bt.col_ptr.ptr, bt.cols.ptr, bt.leaves.ptr,
+               bt.panel_val.ptr,  nullptr,          nullptr,           nullptr,         nullptr,     bt.np};
+  launch(ctx, "fm_panel_values", fm_panel_values_kernel, dim3(blocks_for(bt.np, 4)), dim3(256), 0, pa,
+         (const uint32_t*)bt.order.ptr, (const double*)s->bary.ptr, (const double*)s->normal.ptr,
+         (const double*)s->area.ptr, neg_s, bt.val.ptr);
   df.reserve(9 * (size_t)nf);
   area_p.reserve(nf);
   vert_p.reserve(3 * (size_t)nf);
-  DevBuf<float> psi2(2 * (size_t)nf);
   launch(ctx, "fm_face", fm_face_kernel, dim3(blocks_for(nf, 256)), dim3(256), 0, nf, (const uint32_t*)bt.order.ptr,
          (const uint32_t*)s->faces.ptr, (const double*)s->positions.ptr, (const double*)s->area.ptr, df.ptr, area_p.ptr,
-         vert_p.ptr, psi2.ptr);
-  bt.multiply<2, 1>(ctx, psi2.ptr);
-  ha.reserve(2 * (size_t)nf);
-  HIPCHECK(hipMemcpyAsync(ha.ptr, bt.hx.ptr, 2 * (size_t)nf * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+         vert_p.ptr);
   // vertex → (BVH position of the face, corner), faces ascending
   std::vector<uint32_t> position(nf);
   for (int p = 0; p < nf; ++p) position[t.order[p]] = (uint32_t)p;
@@ -392,33 +608,31 @@ void FaceMetricDev::build(ddgrs_surface* s, const Topology& t, const Clusters& c
     for (int c = 0; c < 3; ++c) inc_h[fill[s->faces_h[3 * f + c]]++] = (position[f] << 2) | (uint32_t)c;
   ddgrs::upload(ctx, inc_ptr, ptr);
   ddgrs::upload(ctx, inc, inc_h);
-  m.reserve(12 * (size_t)nf);
-  psi.reserve(12 * (size_t)nf);
-  z.reserve(12 * (size_t)nf);
+  m_hl.reserve(24 * (size_t)nf);
+  z_hl.reserve(24 * (size_t)nf);
   ctx->sync();  // the local buffers above are released on return
 }
 
 void FaceMetricDev::apply(ddgrs_ctx* ctx, const double* x, double* y) {
   launch(ctx, "fm_m", fm_m_kernel, dim3(blocks_for(nf, 256)), dim3(256), 0, nf, (const uint32_t*)vert_p.ptr,
-         (const float*)df.ptr, (const float*)area_p.ptr, x, m.ptr, psi.ptr);
-  bt.multiply<12, 9>(ctx, psi.ptr);
-  launch(ctx, "fm_z", fm_z_kernel, dim3(blocks_for((int64_t)nf * 12, 256)), dim3(256), 0, nf, (const float*)ha.ptr,
-         (const float*)m.ptr, (const float*)bt.hx.ptr, (float)w_hi, (float)w_lo, z.ptr);
+         (const float*)df.ptr, x, m_hl.ptr);
+  bt.multiply<12, 9>(ctx, m_hl.ptr, w_hi, w_lo, z_hl.ptr);
   launch(ctx, "fm_gather", fm_gather_kernel, dim3(blocks_for(nv, 256)), dim3(256), 0, nv, (const uint32_t*)inc_ptr.ptr,
-         (const uint32_t*)inc.ptr, (const float*)df.ptr, (const float*)area_p.ptr, (const float*)z.ptr, y);
+         (const uint32_t*)inc.ptr, (const float*)df.ptr, (const double*)area_p.ptr, (const float*)z_hl.ptr, y);
 }
 
 // ---- vertex-pair fractional operator ------------------------------------------------------
 
-#pragma clang fp contract(off)
-
+// ρ^{−(τ+1)}... at the centroids, (min, max) ordered, times the column cluster's mass.
 __global__ __launch_bounds__(256) void vf_far_coef_kernel(uint32_t nfar, const uint32_t* __restrict__ far_row,
                                                           const uint32_t* __restrict__ far_col,
-                                                          const double* __restrict__ cen, double ex,
+                                                          const double* __restrict__ cen,
+                                                          const double* __restrict__ nmass, double ex,
                                                           float* __restrict__ coef) {
   uint32_t q = blockIdx.x * 256 + threadIdx.x;
   if (q >= nfar) return;
-  uint32_t a = far_row[q], b = far_col[q];
+  const uint32_t col = far_col[q];
+  uint32_t a = far_row[q], b = col;
   if (a > b) {
     uint32_t t = a;
     a = b;
@@ -426,60 +640,54 @@ __global__ __launch_bounds__(256) void vf_far_coef_kernel(uint32_t nfar, const u
   }
   double d[3];
   for (int k = 0; k < 3; ++k) d[k] = cen[3 * a + k] - cen[3 * b + k];
-  coef[q] = (float)pow(d[0] * d[0] + d[1] * d[1] + d[2] * d[2], ex);
+  coef[q] = (float)(pow(d[0] * d[0] + d[1] * d[1] + d[2] * d[2], ex) * nmass[col]);
 }
 
-__global__ __launch_bounds__(256) void vf_near_kernel(uint32_t nblocks, const uint32_t* __restrict__ near_row,
-                                                      const uint32_t* __restrict__ near_col,
-                                                      const uint64_t* __restrict__ near_off,
-                                                      const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
-                                                      const uint32_t* __restrict__ order, const double* __restrict__ pos,
-                                                      double ex, float* __restrict__ val) {
-  const uint32_t q = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  if (q >= nblocks) return;
-  const uint32_t a = near_row[q], b = near_col[q];
-  const uint32_t ra = end[a] - start[a], cb = end[b] - start[b];
-  if (lane >= ra * cb) return;
-  const uint32_t r = lane / cb, c = lane % cb;
-  uint32_t i = order[start[a] + r], j = order[start[b] + c];
-  if (i == j) {
-    val[near_off[q] + lane] = 0.f;
-    return;
+__global__ __launch_bounds__(256) void vf_panel_values_kernel(PanelArgs P, const uint32_t* __restrict__ order,
+                                                              const double* __restrict__ pos,
+                                                              const double* __restrict__ mass, double ex,
+                                                              float* __restrict__ val) {
+  const uint32_t p = wave_id();
+  const int lane = threadIdx.x & 63;
+  if (p >= (uint32_t)P.np) return;
+  const uint32_t k = P.panel_leaf[p], row0 = P.panel_row0[p], R = P.panel_rows[p];
+  const uint32_t c0 = P.col_ptr[k], K = P.col_ptr[k + 1] - c0;
+  float* out = val + P.panel_val[p];
+  for (uint32_t r = 0; r < R; ++r) {
+    const uint32_t s0 = order[row0 + r];
+    for (uint32_t kk = lane; kk < K; kk += 64) {
+      const uint32_t t0 = order[P.cols[c0 + kk]];
+      float* o = out + (size_t)r * K + kk;
+      if (s0 == t0) {
+        *o = 0.f;
+        continue;
+      }
+      const uint32_t i = s0 < t0 ? s0 : t0, j = s0 < t0 ? t0 : s0;
+      double d[3];
+      for (int x = 0; x < 3; ++x) d[x] = pos[3 * i + x] - pos[3 * j + x];
+      *o = (float)(pow(d[0] * d[0] + d[1] * d[1] + d[2] * d[2], ex) * mass[t0]);
+    }
   }
-  if (i > j) {
-    uint32_t t = i;
-    i = j;
-    j = t;
-  }
-  double d[3];
-  for (int k = 0; k < 3; ++k) d[k] = pos[3 * i + k] - pos[3 * j + k];
-  val[near_off[q] + lane] = (float)pow(d[0] * d[0] + d[1] * d[1] + d[2] * d[2], ex);
 }
 
-#pragma clang fp contract(fast)
-
-__global__ __launch_bounds__(256) void vf_psi_kernel(int n, const uint32_t* __restrict__ order,
-                                                     const float* __restrict__ mass_p, const double* __restrict__ x,
-                                                     float* __restrict__ psi) {
+__global__ __launch_bounds__(256) void vf_u_kernel(int n, const uint32_t* __restrict__ order,
+                                                   const double* __restrict__ x, float* __restrict__ u_hl) {
   int p = blockIdx.x * 256 + threadIdx.x;
   if (p >= n) return;
   uint32_t v = order[p];
-  float a = mass_p[p];
-  for (int k = 0; k < 3; ++k) psi[3 * (size_t)p + k] = a * (float)x[3 * (size_t)v + k];
+  for (int k = 0; k < 3; ++k) split_store(x[3 * (size_t)v + k], u_hl + 6 * (size_t)p + k, u_hl + 6 * (size_t)p + 3 + k);
 }
 
 __global__ __launch_bounds__(256) void vf_out_kernel(int n, const uint32_t* __restrict__ position,
-                                                     const double* __restrict__ mass, const float* __restrict__ ha,
-                                                     const float* __restrict__ hx, const double* __restrict__ x,
+                                                     const double* __restrict__ mass, const float* __restrict__ z_hl,
                                                      double* __restrict__ y) {
   int v = blockIdx.x * 256 + threadIdx.x;
   if (v >= n) return;
   uint32_t p = position[v];
   double a = mass[v];
-  float h = ha[p];
   for (int k = 0; k < 3; ++k) {
-    float t = h * (float)x[3 * (size_t)v + k] - hx[3 * (size_t)p + k];
-    y[3 * (size_t)v + k] = 2.0 * a * (double)t;
+    const double z = (double)z_hl[6 * (size_t)p + k] + (double)z_hl[6 * (size_t)p + 3 + k];
+    y[3 * (size_t)v + k] = 2.0 * a * z;
   }
 }
 
@@ -490,34 +698,34 @@ void VertexFractionalDev::build(ddgrs_ctx* ctx, const std::vector<double>& pos, 
   topo = build_topology(nv, pos.data(), pos.data(), pos.data(), leaf_size);
   clusters = build_clusters(topo, pos.data(), pos.data(), pos.data(), m.data());
   bs = build_block_structure(topo, clusters, chi, leaf_pairs);
-  bt.upload(ctx, topo, bs, 1);
+  std::vector<double> wbvh(nv);
+  for (int p = 0; p < nv; ++p) wbvh[p] = m[topo.order[p]];
+  bt.upload(ctx, topo, bs, 1, wbvh, clusters.mass);
   const double ex = -(tau + 1.0);
-  DevBuf<double> cen, dpos;
+  DevBuf<double> cen, dpos, nmass;
   ddgrs::upload(ctx, cen, clusters.centroid);
   ddgrs::upload(ctx, dpos, pos);
-  launch(ctx, "vf_far_coef", vf_far_coef_kernel, dim3(blocks_for(bt.n_far, 256)), dim3(256), 0, bt.n_far,
-         (const uint32_t*)bt.far_row.ptr, (const uint32_t*)bt.far_col.ptr, (const double*)cen.ptr, ex, bt.far_coef.ptr);
-  launch(ctx, "vf_near", vf_near_kernel, dim3(blocks_for(bt.n_near_blocks, 4)), dim3(256), 0, bt.n_near_blocks,
-         (const uint32_t*)bt.near_row.ptr, (const uint32_t*)bt.near_col.ptr, (const uint64_t*)bt.near_off.ptr,
-         (const uint32_t*)bt.start.ptr, (const uint32_t*)bt.end.ptr, (const uint32_t*)bt.order.ptr,
-         (const double*)dpos.ptr, ex, bt.near_val.ptr);
-  std::vector<float> mp(nv);
-  for (int p = 0; p < nv; ++p) mp[p] = (float)m[topo.order[p]];
-  ddgrs::upload(ctx, mass_p, mp);
+  ddgrs::upload(ctx, nmass, clusters.mass);
   ddgrs::upload(ctx, mass, m);
-  bt.multiply<1, 1>(ctx, mass_p.ptr);
-  ha.reserve(nv);
-  HIPCHECK(hipMemcpyAsync(ha.ptr, bt.hx.ptr, (size_t)nv * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
-  psi.reserve(3 * (size_t)nv);
+  launch(ctx, "vf_far_coef", vf_far_coef_kernel, dim3(blocks_for(bt.n_far, 256)), dim3(256), 0, bt.n_far,
+         (const uint32_t*)bt.far_row.ptr, (const uint32_t*)bt.far_col.ptr, (const double*)cen.ptr,
+         (const double*)nmass.ptr, ex, bt.far_coef.ptr);
+  bt.finish_far(ctx);
+  PanelArgs pa{bt.panel_leaf.ptr, bt.panel_row0.ptr, bt.panel_rows.ptr, bt.col_ptr.ptr, bt.cols.ptr, bt.leaves.ptr,
+               bt.panel_val.ptr,  nullptr,          nullptr,           nullptr,         nullptr,     bt.np};
+  launch(ctx, "vf_panel_values", vf_panel_values_kernel, dim3(blocks_for(bt.np, 4)), dim3(256), 0, pa,
+         (const uint32_t*)bt.order.ptr, (const double*)dpos.ptr, (const double*)mass.ptr, ex, bt.val.ptr);
+  u_hl.reserve(6 * (size_t)nv);
+  z_hl.reserve(6 * (size_t)nv);
   ctx->sync();
 }
 
 void VertexFractionalDev::apply(ddgrs_ctx* ctx, const double* x, double* y) {
-  launch(ctx, "vf_psi", vf_psi_kernel, dim3(blocks_for(nv, 256)), dim3(256), 0, nv, (const uint32_t*)bt.order.ptr,
-         (const float*)mass_p.ptr, x, psi.ptr);
-  bt.multiply<3, 3>(ctx, psi.ptr);
+  launch(ctx, "vf_u", vf_u_kernel, dim3(blocks_for(nv, 256)), dim3(256), 0, nv, (const uint32_t*)bt.order.ptr, x,
+         u_hl.ptr);
+  bt.multiply<3, 3>(ctx, u_hl.ptr, 1.0, 1.0, z_hl.ptr);
   launch(ctx, "vf_out", vf_out_kernel, dim3(blocks_for(nv, 256)), dim3(256), 0, nv, (const uint32_t*)bt.position.ptr,
-         (const double*)mass.ptr, (const float*)ha.ptr, (const float*)bt.hx.ptr, x, y);
+         (const double*)mass.ptr, (const float*)z_hl.ptr, y);
 }
 
 }  // namespace ddgrs

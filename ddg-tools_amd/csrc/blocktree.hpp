@@ -7,25 +7,48 @@
 
 namespace ddgrs {
 
-// Device copy of a block structure over a BVH with NK kernels (values FP32, NK per entry).
+// Device form of a block structure over a BVH, for products in difference form
+//   z_S = Σ_{T near S} v_ST (u_S − u_T) + Σ_{far (A, B), S ∈ A} c_AB (u_S − ū_B),
+// v_ST = H_ST a_T (near values × the column weight), c_AB = κ_AB A_B (far coefficient × the column
+// cluster's weight), ū_B the weighted mean of u over B.  This equals the reference's
+// H̃a·u − H̃(a u) (hierarchical.rs:696 product) term by term, but no large term cancels: every
+// FP32 rounding is relative to a difference of u, not to u.
+//
+// Near field: one panel per leaf (split into panels of at most 8 rows): the leaf's rows against
+// every column that a near block of one of its ancestors covers, stored dense [row][column] with NK
+// FP32 values per entry.  Far field: the admissible blocks per row node.  The far part is pushed to
+// the leaves exactly: for S in leaf L,
+//   Σ_{A ∋ S} Σ_B c_AB (u_S − ū_B) = ᾱ_L (u_S − ū_L) + δ_L,
+//   ᾱ_L = Σ_{A ⊇ L} α_A, α_A = Σ_B c_AB,  δ_L = Σ_{A ⊇ L} [α_A (ū_L − ū_A) + γ_A],
+//   γ_A = Σ_B c_AB (ū_A − ū_B)   (FP64, from FP64 node means).
 struct BlockTreeDev {
   int nk = 1;
-  int n = 0, nn = 0, nl = 0;
-  uint64_t near_total = 0;
-  DevBuf<uint32_t> order, position, start, end, leaves, chain_ptr, chain;
-  DevBuf<uint32_t> far_ptr, far_col, far_row, near_ptr, near_col, near_row, needed;
-  DevBuf<uint64_t> near_off;
-  DevBuf<float> far_coef, near_val;  // nk per entry
-  int n_needed = 0;
-  uint32_t n_far = 0, n_near_blocks = 0;
-  // scratch of a product
-  DevBuf<float> up, farv, hx;
+  int n = 0, nn = 0, nl = 0, np = 0;
+  uint64_t near_total = 0;  // stored near entries (Σ over panels of rows × columns)
+  uint32_t n_far = 0;
+  DevBuf<uint32_t> order, position, start, end, right, parent, leaves;
+  DevBuf<uint32_t> panel_leaf, panel_row0, panel_rows, col_ptr, cols;
+  DevBuf<uint64_t> panel_val;  // per panel: first entry of its values
+  DevBuf<float> val;           // NK per entry
+  DevBuf<uint32_t> far_ptr, far_col, far_row;
+  DevBuf<float> far_coef;      // NK per far entry: κ_AB A_B
+  DevBuf<double> weight;       // per BVH position: a (FP64)
+  DevBuf<double> inv_mass;     // per node: 1 / A
+  DevBuf<double> alpha_node;   // per node, NK
+  DevBuf<double> alpha_leaf;   // per leaf ordinal, NK
+  DevBuf<uint32_t> counters;   // per node, zero between products (upward pass)
+  DevBuf<double> sigma, mean, gamma, delta;  // product scratch (C per node / leaf)
 
-  void upload(ddgrs_ctx* ctx, const Topology& t, const BlockStructure& s, int nk);
-  // hx = H̃ ψ (both in BVH order, C channels per element; channels [0, H) on kernel 0, [H, C) on
-  // kernel nk − 1).  Results stay in `hx`.
+  // Structure, panels and weights; near values and far coefficients are filled by the caller's
+  // kernels, then finish_far() computes α.
+  void upload(ddgrs_ctx* ctx, const Topology& t, const BlockStructure& s, int nk, const std::vector<double>& weight_bvh,
+              const std::vector<double>& node_mass);
+  void finish_far(ddgrs_ctx* ctx);
+  // z = w_k (H̃a·u − H̃(a u)) per BVH position and channel, channels [0, H) on kernel 0 with w0,
+  // [H, C) on kernel NK − 1 with w1.  u and z are hi/lo FP32 pairs: 2 C floats per position
+  // (C hi, then C lo).
   template <int C, int H>
-  void multiply(ddgrs_ctx* ctx, const float* psi);
+  void multiply(ddgrs_ctx* ctx, const float* u_hl, double w0, double w1, float* z_hl);
 };
 
 // B̃, B̃⁰ of one configuration (hierarchical.rs:283).
@@ -34,15 +57,14 @@ struct FaceMetricDev {
   BlockStructure bs;
   BlockTreeDev bt;
   int nv = 0, nf = 0;
-  DevBuf<float> df;        // 9 per BVH position: g_c(S) for c = 0..2
-  DevBuf<float> area_p;    // per BVH position
-  DevBuf<uint32_t> vert_p; // 3 per BVH position
-  DevBuf<float> ha;        // 2 per BVH position (H̃a for hi, lo)
-  DevBuf<float> m, psi, z; // product scratch, 12 channels
+  DevBuf<float> df;         // 9 per BVH position: g_c(S) for c = 0..2
+  DevBuf<double> area_p;    // per BVH position
+  DevBuf<uint32_t> vert_p;  // 3 per BVH position
+  DevBuf<float> m_hl, z_hl; // product scratch: 12 channels, hi/lo
   DevBuf<uint32_t> inc_ptr, inc;  // vertex → (face, corner), with faces' BVH positions
   double w_hi = 1.0, w_lo = 1.0;
 
-  // Builds on the face tree of `bh` (same positions) with separation χ and leaf_pairs.
+  // Builds on a face tree of the surface's current positions with separation χ and leaf_pairs.
   void build(ddgrs_surface* s, const Topology& t, const Clusters& c, double chi, size_t leaf_pairs, double s_exp);
   // y = (w_hi B̃ + w_lo B̃⁰) x for one interleaved 3 nv FP64 vector (device pointers).
   void apply(ddgrs_ctx* ctx, const double* x, double* y);
@@ -55,9 +77,8 @@ struct VertexFractionalDev {
   BlockStructure bs;
   BlockTreeDev bt;
   int nv = 0;
-  DevBuf<float> mass_p, ha;  // per BVH position
-  DevBuf<float> psi;
-  DevBuf<double> mass;       // per vertex (FP64)
+  DevBuf<double> mass;      // per vertex (FP64)
+  DevBuf<float> u_hl, z_hl; // product scratch: 3 channels, hi/lo
   void build(ddgrs_ctx* ctx, const std::vector<double>& pos, const std::vector<double>& mass, double tau, double chi,
              size_t leaf_size, size_t leaf_pairs);
   // y = L̃^V x (interleaved 3 nv FP64, device pointers).

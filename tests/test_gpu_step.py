@@ -73,7 +73,7 @@ def test_lean_saddle_octa(pkg, ctx):
     sd = pkg.Saddle(s, pkg.Exponents(*c["exponents"]), pkg.SaddleOptions(chi=0.0))
     b = np.zeros(3 * nv + 3)
     b[: 3 * nv] = g
-    x, rep = sd.solve(b, pkg.GmresOptions(tolerance=1e-10, max_iterations=500))
+    x, rep = sd.solve(b)
     err = np.abs(x[: 3 * nv] - xr).max() / np.abs(xr).max()
     print(f"octa saddle: {rep.iterations} iterations, x rel {err:.2e}, <g,x> {g @ x[:3 * nv]:.12g} / {o['gx']:.12g}")
     assert rep.converged
