@@ -2,23 +2,23 @@
 //
 // Host: K = W + εM (RS/src/geometry.rs:221 cotan_stiffness, RS/src/solve/dense.rs:529 ε = ε_rel
 // tr W / tr M), the scalar saddle L′ of the global barycenter row (precond.rs:184-240, per-axis
-// case, multipliers ordered last), its LDLᵀ factor in a nested-dissection order, and the
-// elimination-tree levels that schedule the device solves.
+// case, multipliers ordered last), its LDLᵀ factor in a nested-dissection order, and the dense
+// inverses of the diagonal blocks of L over the separator tree.
 //
 // Device: the three axes of a saddle vector are solved together in the permuted numbering (n × 3
-// row-major).  Forward substitution gathers a row of L (x_j = b_j − Σ_k L_jk x_k, the k are
-// descendants of j), backward substitution gathers a column (x_j −= Σ_i L_ij x_i, the i are
-// ancestors), so a level of the elimination tree is a set of independent rows.  Wide levels (the
-// bottom of the tree) are one grid launch each, one thread per (row, axis); the narrow top of the
-// tree (separators and the dense multiplier row) is solved by one workgroup that walks its levels
-// with a barrier between them, one wave per row.  FP64 throughout: L′ has a near-null space of
-// relative size ε = 1e-10.
+// row-major).  The separator tree is walked by height, one launch per height and direction, one
+// workgroup per (block, chunk of 64 rows of the block): the workgroup forms the block's right-hand
+// side in LDS from the rows (forward) or columns (backward) of L outside the block, then multiplies
+// by the dense inverse of the block's diagonal part.  A separator's rows only meet its descendants
+// (forward) and ancestors (backward) outside the block, so the blocks of one height are
+// independent.  FP64 throughout: L′ has a near-null space of relative size ε = 1e-10.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <numeric>
 
 #include "bh.hpp"
+#include "host_parallel.hpp"
 #include "precond.hpp"
 
 namespace ddgrs {
@@ -27,7 +27,7 @@ namespace ddgrs {
 
 namespace {
 constexpr uint32_t kNone = UINT32_MAX;
-constexpr size_t kWideMin = 2048;  // rows of a level solved by a whole-grid launch
+constexpr uint32_t kChunk = 64;  // rows of a block per workgroup
 
 HostCsr csr_from_triplets(size_t n, std::vector<std::tuple<uint32_t, uint32_t, double>>& t) {
   std::stable_sort(t.begin(), t.end(), [](const auto& a, const auto& b) {
@@ -49,25 +49,35 @@ HostCsr csr_from_triplets(size_t n, std::vector<std::tuple<uint32_t, uint32_t, d
 }
 }  // namespace
 
-std::vector<uint32_t> nested_dissection_order(const HostCsr& k, const std::vector<double>& pos) {
+NdBlocks nested_dissection(const HostCsr& k, const std::vector<double>& pos) {
   const size_t n = k.n;
-  std::vector<uint32_t> out;
-  out.reserve(n);
+  NdBlocks nd;
+  nd.order.reserve(n);
   std::vector<int32_t> side(n, -1);
   struct Task {
     std::vector<uint32_t> verts;
     bool emit_only;  // a separator, appended after its two domains
+    uint32_t id, parent;
   };
-  // Iterative post-order: a task expands into [domain A, domain B, separator].
+  // Iterative post-order: a domain task expands into [domain A, domain B, separator]; the
+  // separator keeps the domain's block id (it is the root of the domain's subtree).
+  auto new_block = [&](uint32_t parent) {
+    nd.start.push_back(0);
+    nd.end.push_back(0);
+    nd.parent.push_back(parent);
+    return (uint32_t)nd.parent.size() - 1;
+  };
   std::vector<Task> stack;
   std::vector<uint32_t> all(n);
   std::iota(all.begin(), all.end(), 0u);
-  stack.push_back({std::move(all), false});
+  stack.push_back({std::move(all), false, new_block(kNone), kNone});
   while (!stack.empty()) {
     Task t = std::move(stack.back());
     stack.pop_back();
     if (t.emit_only || t.verts.size() <= 64) {
-      out.insert(out.end(), t.verts.begin(), t.verts.end());
+      nd.start[t.id] = (uint32_t)nd.order.size();
+      nd.order.insert(nd.order.end(), t.verts.begin(), t.verts.end());
+      nd.end[t.id] = (uint32_t)nd.order.size();
       continue;
     }
     double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -101,11 +111,12 @@ std::vector<uint32_t> nested_dissection_order(const HostCsr& k, const std::vecto
     std::sort(A.begin(), A.end());
     std::sort(B.begin(), B.end());
     std::sort(S.begin(), S.end());
-    stack.push_back({std::move(S), true});
-    stack.push_back({std::move(B), false});
-    stack.push_back({std::move(A), false});
+    const uint32_t ia = new_block(t.id), ib = new_block(t.id);
+    stack.push_back({std::move(S), true, t.id, t.parent});
+    stack.push_back({std::move(B), false, ib, t.id});
+    stack.push_back({std::move(A), false, ia, t.id});
   }
-  return out;
+  return nd;
 }
 
 Factor factor_ldlt(const HostCsr& a, const std::vector<uint32_t>& order) {
@@ -186,11 +197,6 @@ Factor factor_ldlt(const HostCsr& a, const std::vector<uint32_t>& order) {
       f.rj[fill[i]] = (uint32_t)j;
       f.rx[fill[i]++] = f.cx[q];
     }
-  f.parent = parent;
-  f.level.assign(n, 0);
-  for (size_t j = 0; j < n; ++j)
-    if (parent[j] != kNone) f.level[parent[j]] = std::max(f.level[parent[j]], f.level[j] + 1);
-  f.n_levels = n ? *std::max_element(f.level.begin(), f.level.end()) + 1 : 0;
   return f;
 }
 
@@ -214,61 +220,283 @@ __global__ __launch_bounds__(256) void lp_scatter_kernel(uint32_t n, uint32_t nv
   x[old < nv ? 3 * (size_t)old + a : 3 * (size_t)nv + a] = xb[t];
 }
 
-// One wide level: thread per (row, axis); forward uses the row of L, backward the column.
-__global__ __launch_bounds__(256) void lp_level_kernel(uint32_t count, const uint32_t* __restrict__ rows,
-                                                       const uint32_t* __restrict__ ptr, const uint32_t* __restrict__ idx,
-                                                       const double* __restrict__ val, double* __restrict__ xb) {
-  uint32_t t = blockIdx.x * 256 + threadIdx.x;
-  if (t >= 3 * count) return;
-  uint32_t j = rows[t / 3], a = t % 3;
-  double s = 0.0;
-  for (uint32_t q = ptr[j]; q < ptr[j + 1]; ++q) s += val[q] * xb[3 * (size_t)idx[q] + a];
-  xb[3 * (size_t)j + a] -= s;
+__device__ __forceinline__ double wsum64d(double x) {
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+  return x;
 }
 
-// The narrow levels in one workgroup: levels in order (forward: bottom-up; backward: top-down), a
-// barrier between levels, one wave per row (lanes split the terms, fixed-order reduction).
-__global__ __launch_bounds__(1024) void lp_narrow_kernel(uint32_t n_levels, const uint32_t* __restrict__ off,
-                                                         const uint32_t* __restrict__ rows, int backward,
-                                                         const uint32_t* __restrict__ ptr,
-                                                         const uint32_t* __restrict__ idx,
-                                                         const double* __restrict__ val, double* __restrict__ xb) {
-  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (uint32_t l = 0; l < n_levels; ++l) {
-    const uint32_t lv = backward ? n_levels - 1 - l : l;
-    for (uint32_t q = off[lv] + wave; q < off[lv + 1]; q += 16) {
-      const uint32_t j = rows[q];
-      double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-      for (uint32_t e = ptr[j] + lane; e < ptr[j + 1]; e += 64) {
-        const double v = val[e];
-        const double* xk = xb + 3 * (size_t)idx[e];
-        s0 += v * xk[0];
-        s1 += v * xk[1];
-        s2 += v * xk[2];
-      }
-      for (int o = 32; o > 0; o >>= 1) {
-        s0 += __shfl_xor(s0, o);
-        s1 += __shfl_xor(s1, o);
-        s2 += __shfl_xor(s2, o);
-      }
+struct LpArgs {
+  const uint32_t *item_block, *item_row0, *blk_start, *blk_rows;
+  const uint64_t* blk_inv;
+  const double* inv;
+  const uint32_t *ptr, *split, *idx;  // forward: rows of L; backward: columns of L
+  const double *val, *d;
+};
+
+// Σ over [q0, q1) of val[q] x[idx[q]] (3 axes) by one wave.
+__device__ __forceinline__ void wave_dot3(const LpArgs& A, uint32_t q0, uint32_t q1, const double* __restrict__ x,
+                                          int lane, double& s0, double& s1, double& s2) {
+  s0 = s1 = s2 = 0.0;
+  for (uint32_t q = q0 + lane; q < q1; q += 64) {
+    const double v = A.val[q];
+    const double* xk = x + 3 * (size_t)A.idx[q];
+    s0 += v * xk[0];
+    s1 += v * xk[1];
+    s2 += v * xk[2];
+  }
+  s0 = wsum64d(s0);
+  s1 = wsum64d(s1);
+  s2 = wsum64d(s2);
+}
+
+// Σ over [q0, q1) by the whole workgroup (a single long row: the multiplier row).
+__device__ __forceinline__ void block_dot3(const LpArgs& A, uint32_t q0, uint32_t q1, const double* __restrict__ x,
+                                           double* red, double& s0, double& s1, double& s2) {
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  for (uint32_t q = q0 + t; q < q1; q += 256) {
+    const double v = A.val[q];
+    const double* xk = x + 3 * (size_t)A.idx[q];
+    a0 += v * xk[0];
+    a1 += v * xk[1];
+    a2 += v * xk[2];
+  }
+  a0 = wsum64d(a0);
+  a1 = wsum64d(a1);
+  a2 = wsum64d(a2);
+  if (lane == 0) {
+    red[3 * wave] = a0;
+    red[3 * wave + 1] = a1;
+    red[3 * wave + 2] = a2;
+  }
+  __syncthreads();
+  s0 = (red[0] + red[3]) + (red[6] + red[9]);
+  s1 = (red[1] + red[4]) + (red[7] + red[10]);
+  s2 = (red[2] + red[5]) + (red[8] + red[11]);
+}
+
+// Forward, one height of the separator tree: workgroup = (block J, rows [row0, row0 + 64) of J).
+// r_J = b_J − L_{J, j < start} x (rows of L up to the block), then x = M_J r_J, M_J = L_JJ⁻¹.
+__global__ __launch_bounds__(256) void lp_forward_kernel(LpArgs A, uint32_t item0, const double* __restrict__ b,
+                                                         double* __restrict__ xf) {
+  extern __shared__ double lds[];  // 12 reduction slots, then r (k × 3)
+  double* red = lds;
+  double* r = lds + 16;
+  const uint32_t it = item0 + blockIdx.x;
+  const uint32_t J = A.item_block[it], row0 = A.item_row0[it];
+  const uint32_t j0 = A.blk_start[J], k = A.blk_rows[J];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (k == 1) {
+    double s0, s1, s2;
+    block_dot3(A, A.ptr[j0], A.split[j0], xf, red, s0, s1, s2);
+    if (threadIdx.x == 0) {
+      r[0] = b[3 * (size_t)j0] - s0;
+      r[1] = b[3 * (size_t)j0 + 1] - s1;
+      r[2] = b[3 * (size_t)j0 + 2] - s2;
+    }
+  } else {
+    for (uint32_t ii = wave; ii < k; ii += 4) {
+      const uint32_t i = j0 + ii;
+      double s0, s1, s2;
+      wave_dot3(A, A.ptr[i], A.split[i], xf, lane, s0, s1, s2);
       if (lane == 0) {
-        xb[3 * (size_t)j] -= s0;
-        xb[3 * (size_t)j + 1] -= s1;
-        xb[3 * (size_t)j + 2] -= s2;
+        r[3 * ii] = b[3 * (size_t)i] - s0;
+        r[3 * ii + 1] = b[3 * (size_t)i + 1] - s1;
+        r[3 * ii + 2] = b[3 * (size_t)i + 2] - s2;
       }
     }
-    __threadfence_block();
-    __syncthreads();
+  }
+  __syncthreads();
+  const double* M = A.inv + A.blk_inv[J];
+  const uint32_t rend = min(row0 + kChunk, k);
+  for (uint32_t ii = row0 + wave; ii < rend; ii += 4) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (uint32_t l = lane; l <= ii; l += 64) {
+      const double m = M[(size_t)ii * k + l];
+      s0 += m * r[3 * l];
+      s1 += m * r[3 * l + 1];
+      s2 += m * r[3 * l + 2];
+    }
+    s0 = wsum64d(s0);
+    s1 = wsum64d(s1);
+    s2 = wsum64d(s2);
+    if (lane == 0) {
+      const size_t i = j0 + ii;
+      xf[3 * i] = s0;
+      xf[3 * i + 1] = s1;
+      xf[3 * i + 2] = s2;
+    }
   }
 }
 
-__global__ __launch_bounds__(256) void lp_diag_kernel(uint32_t n, const double* __restrict__ d, double* __restrict__ xb) {
-  uint32_t t = blockIdx.x * 256 + threadIdx.x;
-  if (t < 3 * n) xb[t] /= d[t / 3];
+// Backward, one height: r_J = D_J⁻¹ y_J − L_{i ≥ end, J}ᵀ x (columns of L below the block), then
+// x = M_Jᵀ r_J.
+__global__ __launch_bounds__(256) void lp_backward_kernel(LpArgs A, uint32_t item0, const double* __restrict__ y,
+                                                          double* __restrict__ xb) {
+  extern __shared__ double lds[];  // 12 reduction slots, 4 × 64 × 3 partials, then r (k × 3)
+  double* red = lds;
+  double* part = lds + 16;
+  double* r = lds + 16 + 768;
+  const uint32_t it = item0 + blockIdx.x;
+  const uint32_t J = A.item_block[it], row0 = A.item_row0[it];
+  const uint32_t j0 = A.blk_start[J], k = A.blk_rows[J];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (k == 1) {
+    double s0, s1, s2;
+    block_dot3(A, A.split[j0], A.ptr[j0 + 1], xb, red, s0, s1, s2);
+    if (threadIdx.x == 0) {
+      const double dj = A.d[j0];
+      r[0] = y[3 * (size_t)j0] / dj - s0;
+      r[1] = y[3 * (size_t)j0 + 1] / dj - s1;
+      r[2] = y[3 * (size_t)j0 + 2] / dj - s2;
+    }
+  } else {
+    for (uint32_t ii = wave; ii < k; ii += 4) {
+      const uint32_t i = j0 + ii;
+      double s0, s1, s2;
+      wave_dot3(A, A.split[i], A.ptr[i + 1], xb, lane, s0, s1, s2);
+      if (lane == 0) {
+        const double di = A.d[i];
+        r[3 * ii] = y[3 * (size_t)i] / di - s0;
+        r[3 * ii + 1] = y[3 * (size_t)i + 1] / di - s1;
+        r[3 * ii + 2] = y[3 * (size_t)i + 2] / di - s2;
+      }
+    }
+  }
+  __syncthreads();
+  const double* M = A.inv + A.blk_inv[J];
+  const uint32_t ii = row0 + lane;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  if (ii < k)
+    for (uint32_t l = ii + wave; l < k; l += 4) {
+      const double m = M[(size_t)l * k + ii];
+      s0 += m * r[3 * l];
+      s1 += m * r[3 * l + 1];
+      s2 += m * r[3 * l + 2];
+    }
+  part[3 * threadIdx.x] = s0;
+  part[3 * threadIdx.x + 1] = s1;
+  part[3 * threadIdx.x + 2] = s2;
+  __syncthreads();
+  if (wave == 0 && ii < k) {
+    const size_t i = j0 + ii;
+    for (int a = 0; a < 3; ++a)
+      xb[3 * i + a] = (part[3 * lane + a] + part[3 * (lane + 64) + a]) + (part[3 * (lane + 128) + a] + part[3 * (lane + 192) + a]);
+  }
 }
 
-void LprimeDev::upload(ddgrs_ctx* ctx) {
+LprimePlan make_lprime_plan(const Factor& f, const NdBlocks& nd) {
+  LprimePlan P;
+  const size_t n = f.n;
+  auto& bstart = P.bstart;
+  auto& bend = P.bend;
+  auto& bheight = P.bheight;
+  auto& max_rows = P.max_rows;
+  auto& level_off = P.level_off;
+  // blocks: the separator tree's, then the multiplier row (the root)
+  bstart = nd.start;
+  bend = nd.end;
+  std::vector<uint32_t> bparent = nd.parent;
+  const uint32_t nb0 = (uint32_t)bstart.size();
+  bstart.push_back((uint32_t)n - 1);
+  bend.push_back((uint32_t)n);
+  bparent.push_back(kNone);
+  for (uint32_t b = 0; b < nb0; ++b)
+    if (bparent[b] == kNone) bparent[b] = nb0;
+  const uint32_t nb = nb0 + 1;
+  // heights: a domain's two halves get ids after the domain's, so decreasing ids visit children
+  // before parents; the multiplier row (id nb0) is the parent of the tree's root (id 0)
+  bheight.assign(nb, 0);
+  for (uint32_t b = nb0; b-- > 0;) bheight[bparent[b]] = std::max(bheight[bparent[b]], bheight[b] + 1);
+  std::vector<uint32_t> block_of(n);
+  for (uint32_t b = 0; b < nb; ++b)
+    for (uint32_t i = bstart[b]; i < bend[b]; ++i) block_of[i] = b;
+  // off-block split points; every off-block entry must meet a block of another height on the
+  // right side (descendants below, ancestors above), else the schedule would race
+  auto& rs = P.rsplit;
+  auto& cs = P.csplit;
+  rs.resize(n);
+  cs.resize(n);
+  for (size_t i = 0; i < n; ++i) {
+    const uint32_t b = block_of[i];
+    uint32_t q = f.rp[i];
+    while (q < f.rp[i + 1] && f.rj[q] < bstart[b]) {
+      if (bheight[block_of[f.rj[q]]] >= bheight[b]) throw Error(DDGRS_GPU, "preconditioner", "separator tree schedule violated (row)");
+      ++q;
+    }
+    rs[i] = q;
+    q = f.cp[i];
+    while (q < f.cp[i + 1] && f.ci[q] < bend[b]) ++q;
+    cs[i] = q;
+    for (uint32_t e = q; e < f.cp[i + 1]; ++e)
+      if (bheight[block_of[f.ci[e]]] <= bheight[b]) throw Error(DDGRS_GPU, "preconditioner", "separator tree schedule violated (column)");
+  }
+  // dense inverses M_J = L_JJ⁻¹ (unit lower triangular), row-major
+  auto& ioff = P.inv_off;
+  ioff.assign(nb + 1, 0);
+  max_rows = 0;
+  for (uint32_t b = 0; b < nb; ++b) {
+    const uint64_t k = bend[b] - bstart[b];
+    ioff[b + 1] = ioff[b] + k * k;
+    max_rows = std::max(max_rows, (uint32_t)k);
+  }
+  auto& M = P.inv;
+  M.assign(ioff[nb], 0.0);
+  auto invert_column = [&](uint32_t b, const std::vector<double>& L, uint32_t c) {
+    const uint32_t k = bend[b] - bstart[b];
+    double* m = M.data() + ioff[b];
+    m[(size_t)c * k + c] = 1.0;
+    for (uint32_t i = c + 1; i < k; ++i) {
+      double acc = 0.0;
+      for (uint32_t j = c; j < i; ++j) acc += L[(size_t)i * k + j] * m[(size_t)j * k + c];
+      m[(size_t)i * k + c] = -acc;
+    }
+  };
+  auto dense_l = [&](uint32_t b) {
+    const uint32_t k = bend[b] - bstart[b], j0 = bstart[b];
+    std::vector<double> L((size_t)k * k, 0.0);
+    for (uint32_t ii = 0; ii < k; ++ii)
+      for (uint32_t q = rs[j0 + ii]; q < f.rp[j0 + ii + 1]; ++q) L[(size_t)ii * k + (f.rj[q] - j0)] = f.rx[q];
+    return L;
+  };
+  std::vector<uint32_t> small, big;
+  for (uint32_t b = 0; b < nb; ++b) (bend[b] - bstart[b] >= 256 ? big : small).push_back(b);
+  parallel(small.size(), [&](size_t i) {
+    const uint32_t b = small[i], k = bend[b] - bstart[b];
+    auto L = dense_l(b);
+    for (uint32_t c = 0; c < k; ++c) invert_column(b, L, c);
+  });
+  for (uint32_t b : big) {
+    auto L = dense_l(b);
+    parallel(bend[b] - bstart[b], [&](size_t c) { invert_column(b, L, (uint32_t)c); });
+  }
+  // work items by height
+  const uint32_t H = *std::max_element(bheight.begin(), bheight.end()) + 1;
+  std::vector<std::vector<std::pair<uint32_t, uint32_t>>> by(H);
+  for (uint32_t b = 0; b < nb; ++b)
+    for (uint32_t r0 = 0; r0 < bend[b] - bstart[b]; r0 += kChunk) by[bheight[b]].push_back({b, r0});
+  auto& iblock = P.item_block;
+  auto& irow0 = P.item_row0;
+  auto& brows = P.rows;
+  brows.resize(nb);
+  level_off.assign(1, 0);
+  for (uint32_t h = 0; h < H; ++h) {
+    for (auto [b, r0] : by[h]) {
+      iblock.push_back(b);
+      irow0.push_back(r0);
+    }
+    level_off.push_back((uint32_t)iblock.size());
+  }
+  for (uint32_t b = 0; b < nb; ++b) brows[b] = bend[b] - bstart[b];
+  return P;
+}
+
+void LprimeDev::upload(ddgrs_ctx* ctx, const NdBlocks& nd) {
+  auto t0 = std::chrono::steady_clock::now();
   n = f.n;
+  LprimePlan P = make_lprime_plan(f, nd);
+  level_off = P.level_off;
+  max_rows = P.max_rows;
   ddgrs::upload(ctx, perm, f.perm);
   ddgrs::upload(ctx, rp, f.rp);
   ddgrs::upload(ctx, rj, f.rj);
@@ -277,49 +505,40 @@ void LprimeDev::upload(ddgrs_ctx* ctx) {
   ddgrs::upload(ctx, ci, f.ci);
   ddgrs::upload(ctx, cx, f.cx);
   ddgrs::upload(ctx, d, f.d);
-  std::vector<std::vector<uint32_t>> by(f.n_levels);
-  for (size_t j = 0; j < n; ++j) by[f.level[j]].push_back((uint32_t)j);
-  uint32_t first_narrow = 0;
-  while (first_narrow < f.n_levels && by[first_narrow].size() >= kWideMin) ++first_narrow;
-  std::vector<uint32_t> wide, narrow;
-  wide_off.assign(1, 0);
-  narrow_off.assign(1, 0);
-  for (uint32_t l = 0; l < f.n_levels; ++l) {
-    auto& dst = l < first_narrow ? wide : narrow;
-    dst.insert(dst.end(), by[l].begin(), by[l].end());
-    (l < first_narrow ? wide_off : narrow_off).push_back((uint32_t)dst.size());
-  }
-  ddgrs::upload(ctx, wide_rows, wide);
-  ddgrs::upload(ctx, narrow_rows, narrow);
-  ddgrs::upload(ctx, narrow_off_d, narrow_off);
+  ddgrs::upload(ctx, rsplit, P.rsplit);
+  ddgrs::upload(ctx, csplit, P.csplit);
+  ddgrs::upload(ctx, item_block, P.item_block);
+  ddgrs::upload(ctx, item_row0, P.item_row0);
+  ddgrs::upload(ctx, blk_start, P.bstart);
+  ddgrs::upload(ctx, blk_rows, P.rows);
+  ddgrs::upload(ctx, blk_inv, P.inv_off);
+  ddgrs::upload(ctx, inv, P.inv);
   xb.reserve(3 * n);
+  xf.reserve(3 * n);
+  ctx->sync();  // host vectors above are released on return
+  inverse_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
 void LprimeDev::solve(ddgrs_ctx* ctx, const double* r, double* x) {
   const uint32_t un = (uint32_t)n, unv = (uint32_t)nv;
   launch(ctx, "lp_gather", lp_gather_kernel, dim3(blocks_for(3 * n, 256)), dim3(256), 0, un, unv,
          (const uint32_t*)perm.ptr, r, xb.ptr);
-  const uint32_t nw = (uint32_t)wide_off.size() - 1, nn = (uint32_t)narrow_off.size() - 1;
-  for (uint32_t l = 0; l < nw; ++l) {
-    uint32_t cnt = wide_off[l + 1] - wide_off[l];
-    launch(ctx, "lp_forward_level", lp_level_kernel, dim3(blocks_for(3 * (int64_t)cnt, 256)), dim3(256), 0, cnt,
-           (const uint32_t*)(wide_rows.ptr + wide_off[l]), (const uint32_t*)rp.ptr, (const uint32_t*)rj.ptr,
-           (const double*)rx.ptr, xb.ptr);
+  LpArgs fa{item_block.ptr, item_row0.ptr, blk_start.ptr, blk_rows.ptr, blk_inv.ptr, inv.ptr,
+            rp.ptr,         rsplit.ptr,    rj.ptr,        rx.ptr,       d.ptr};
+  LpArgs ba{item_block.ptr, item_row0.ptr, blk_start.ptr, blk_rows.ptr, blk_inv.ptr, inv.ptr,
+            cp.ptr,         csplit.ptr,    ci.ptr,        cx.ptr,       d.ptr};
+  const uint32_t H = (uint32_t)level_off.size() - 1;
+  const size_t lds_f = (16 + 3 * (size_t)max_rows) * sizeof(double);
+  const size_t lds_b = (16 + 768 + 3 * (size_t)max_rows) * sizeof(double);
+  for (uint32_t h = 0; h < H; ++h) {
+    const uint32_t cnt = level_off[h + 1] - level_off[h];
+    launch(ctx, "lp_forward", lp_forward_kernel, dim3(cnt), dim3(256), lds_f, fa, level_off[h], (const double*)xb.ptr,
+           xf.ptr);
   }
-  if (nn > 0)
-    launch(ctx, "lp_forward_narrow", lp_narrow_kernel, dim3(1), dim3(1024), 0, nn, (const uint32_t*)narrow_off_d.ptr,
-           (const uint32_t*)narrow_rows.ptr, 0, (const uint32_t*)rp.ptr, (const uint32_t*)rj.ptr, (const double*)rx.ptr,
+  for (uint32_t h = H; h-- > 0;) {
+    const uint32_t cnt = level_off[h + 1] - level_off[h];
+    launch(ctx, "lp_backward", lp_backward_kernel, dim3(cnt), dim3(256), lds_b, ba, level_off[h], (const double*)xf.ptr,
            xb.ptr);
-  launch(ctx, "lp_diag", lp_diag_kernel, dim3(blocks_for(3 * n, 256)), dim3(256), 0, un, (const double*)d.ptr, xb.ptr);
-  if (nn > 0)
-    launch(ctx, "lp_backward_narrow", lp_narrow_kernel, dim3(1), dim3(1024), 0, nn, (const uint32_t*)narrow_off_d.ptr,
-           (const uint32_t*)narrow_rows.ptr, 1, (const uint32_t*)cp.ptr, (const uint32_t*)ci.ptr, (const double*)cx.ptr,
-           xb.ptr);
-  for (uint32_t l = nw; l-- > 0;) {
-    uint32_t cnt = wide_off[l + 1] - wide_off[l];
-    launch(ctx, "lp_backward_level", lp_level_kernel, dim3(blocks_for(3 * (int64_t)cnt, 256)), dim3(256), 0, cnt,
-           (const uint32_t*)(wide_rows.ptr + wide_off[l]), (const uint32_t*)cp.ptr, (const uint32_t*)ci.ptr,
-           (const double*)cx.ptr, xb.ptr);
   }
   launch(ctx, "lp_scatter", lp_scatter_kernel, dim3(blocks_for(3 * n, 256)), dim3(256), 0, un, unv,
          (const uint32_t*)perm.ptr, (const double*)xb.ptr, x);
@@ -397,11 +616,12 @@ void Preconditioner::build(ddgrs_surface* s, double s_exp, double w_tp, double e
     trip.emplace_back((uint32_t)v, (uint32_t)nv, crow[v]);
   }
   HostCsr lpm = csr_from_triplets(nv + 1, trip);
-  std::vector<uint32_t> order = nested_dissection_order(k, pos);
+  NdBlocks nd = nested_dissection(k, pos);
+  std::vector<uint32_t> order = nd.order;
   order.push_back((uint32_t)nv);
   lp.f = factor_ldlt(lpm, order);
   lp.nv = nv;
-  lp.upload(ctx);
+  lp.upload(ctx, nd);
   ddgrs::upload(ctx, kp, k.ptr);
   ddgrs::upload(ctx, kj, k.idx);
   ddgrs::upload(ctx, kx, k.val);

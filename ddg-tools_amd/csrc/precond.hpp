@@ -19,29 +19,55 @@ struct Factor {
   std::vector<uint32_t> perm;                 // perm[new] = old
   std::vector<uint32_t> rp, rj, cp, ci;       // rows of L (columns < row), columns of L (rows > col)
   std::vector<double> rx, cx, d;
-  std::vector<uint32_t> parent;               // elimination tree, UINT32_MAX for a root
-  std::vector<uint32_t> level;                // height of each node above the leaves
-  uint32_t n_levels = 0;
 };
 
 // Factor of the symmetric matrix `a` (both triangles stored) in the elimination order `order`
 // (up-looking LDLᵀ: T. Davis, "Algorithm 849", ACM TOMS 31(4), 2005).  Throws SingularSaddle on a
 // zero or non-finite pivot.
 Factor factor_ldlt(const HostCsr& a, const std::vector<uint32_t>& order);
-// Nested dissection of the graph of `k` by coordinate bisection of `pos` (domains before their
-// separators).
-std::vector<uint32_t> nested_dissection_order(const HostCsr& k, const std::vector<double>& pos);
+
+// A nested-dissection order and its separator tree: blocks are contiguous ranges of the order
+// (leaf domains and separators), a separator after the two domains it splits; parent[b] is the
+// separator that splits the domain b came from (UINT32_MAX for the root).
+struct NdBlocks {
+  std::vector<uint32_t> order;
+  std::vector<uint32_t> start, end, parent;
+};
+// Nested dissection of the graph of `k` by coordinate bisection of `pos` (domains of at most 64
+// vertices are leaves).
+NdBlocks nested_dissection(const HostCsr& k, const std::vector<double>& pos);
+
+// Device solves of L′ by blocks of the separator tree.  With the blocks of one height of the tree
+// independent, a solve is one launch per height and direction: a block J computes
+// r_J = b_J − L_{J,desc} x_desc (forward; its descendants are done) and x_J = L_JJ⁻¹ r_J with the
+// dense inverse of its diagonal block; backward likewise with the ancestors and L_JJ⁻ᵀ.
+// Host plan of the block solves: blocks (the separator tree's, then the multiplier row as the
+// root), their heights, the off-block split points of every row and column of L, the dense
+// inverses of the diagonal blocks (row-major), and the work items (block, first row) per height.
+struct LprimePlan {
+  std::vector<uint32_t> bstart, bend, bheight, rows, rsplit, csplit, item_block, item_row0, level_off;
+  std::vector<uint64_t> inv_off;
+  std::vector<double> inv;
+  uint32_t max_rows = 0;
+};
+// Throws if an off-block entry of L joins two blocks of the same height (the blocks of one height
+// must be independent).
+LprimePlan make_lprime_plan(const Factor& f, const NdBlocks& nd);
 
 struct LprimeDev {
   size_t n = 0, nv = 0;
   Factor f;
-  DevBuf<uint32_t> perm, rp, rj, cp, ci, wide_rows, narrow_rows;
+  std::vector<uint32_t> level_off;              // work items of each height: [level_off[h], level_off[h+1])
+  uint32_t max_rows = 0;
+  DevBuf<uint32_t> perm, rp, rj, cp, ci, rsplit, csplit;
   DevBuf<double> rx, cx, d;
-  std::vector<uint32_t> wide_off;    // levels [0, n_wide): row ranges in wide_rows (levels bottom-up)
-  std::vector<uint32_t> narrow_off;  // levels n_wide..: row ranges in narrow_rows
-  DevBuf<uint32_t> narrow_off_d;
-  DevBuf<double> xb;                 // n × 3 work array (permuted numbering)
-  void upload(ddgrs_ctx* ctx);
+  DevBuf<uint32_t> item_block, item_row0, blk_start, blk_rows;
+  DevBuf<uint64_t> blk_inv;                     // offset of each block's dense inverse (row-major k × k)
+  DevBuf<double> inv;
+  DevBuf<double> xb, xf;                        // n × 3 work arrays (permuted numbering)
+  double inverse_seconds = 0.0;
+  // Block schedule and dense inverses of the blocks' diagonal parts of L.
+  void upload(ddgrs_ctx* ctx, const NdBlocks& nd);
   // x = L′⁻¹ r for a full saddle vector (3 nv + 3, interleaved positions then multipliers).
   void solve(ddgrs_ctx* ctx, const double* r, double* x);
 };

@@ -12,6 +12,7 @@
 #include <cstring>
 #include <thread>
 
+#include "host_parallel.hpp"
 #include "trees.hpp"
 
 #pragma clang fp contract(off)
@@ -47,20 +48,6 @@ inline double diam2(const Box& b) {
   return s;
 }
 
-template <class F>
-void parallel(size_t n, F&& f) {
-  unsigned t = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-  if (n < 64 || t == 1) {
-    for (size_t i = 0; i < n; ++i) f(i);
-    return;
-  }
-  std::vector<std::thread> th;
-  for (unsigned k = 0; k < t; ++k)
-    th.emplace_back([&, k] {
-      for (size_t i = k; i < n; i += t) f(i);
-    });
-  for (auto& x : th) x.join();
-}
 
 inline double box_distance_sq(const double* alo, const double* ahi, const double* blo, const double* bhi) {
   double d2 = 0.0;
