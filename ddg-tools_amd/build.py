@@ -1,4 +1,5 @@
-"""Builds the gfx950 library ``ddg-tools_amd/lib/libddgrs_hip.so`` from ``csrc/*.hip`` with hipcc.
+"""Builds the gfx950 library ``ddg-tools_amd/lib/libddgrs_hip.so`` from ``csrc/*.hip`` with hipcc,
+and the standalone probes ``probes/*.hip`` into ``lib/``.
 
 Every translation unit is compiled for gfx950 only, with packed FP32 instructions disabled
 (``-Xclang -target-feature -Xclang -packed-fp32-ops``), then linked into one shared library
@@ -50,6 +51,12 @@ def build(jobs: int = 8) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    # probes: standalone programs (the HBM copy reference used to scale byte counters)
+    for src in sorted(glob.glob(os.path.join(HERE, "probes", "*.hip"))):
+        exe = os.path.join(LIBDIR, os.path.splitext(os.path.basename(src))[0])
+        r = subprocess.run([HIPCC, *FLAGS[:-3], "-o", exe, src], capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"hipcc failed on {src}:\n{r.stderr}")
     return LIB
 
 

@@ -27,7 +27,8 @@ namespace ddgrs {
 
 namespace {
 constexpr uint32_t kNone = UINT32_MAX;
-constexpr uint32_t kChunk = 64;  // rows of a block per workgroup
+constexpr uint32_t kChunk = 64;      // rows of a block per workgroup (fused heights)
+constexpr uint32_t kLongParts = 128; // workgroups summing the multiplier row's forward residual
 
 HostCsr csr_from_triplets(size_t n, std::vector<std::tuple<uint32_t, uint32_t, double>>& t) {
   std::stable_sort(t.begin(), t.end(), [](const auto& a, const auto& b) {
@@ -385,6 +386,89 @@ __global__ __launch_bounds__(256) void lp_backward_kernel(LpArgs A, uint32_t ite
   }
 }
 
+// Split heights, residual phase: one wave per row i of the height.
+// forward: r_i = b_i − Σ_{j < block start} L_ij x_j;  backward: r_i = y_i / d_i − Σ_{j ≥ block end} L_ji x_j.
+__global__ __launch_bounds__(256) void lp_residual_kernel(LpArgs A, const uint32_t* __restrict__ rows, uint32_t count,
+                                                          int backward, const double* __restrict__ rhs,
+                                                          const double* __restrict__ x, double* __restrict__ rbuf) {
+  const uint32_t w = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (w >= count) return;
+  const uint32_t i = rows[w];
+  const uint32_t q0 = backward ? A.split[i] : A.ptr[i], q1 = backward ? A.ptr[i + 1] : A.split[i];
+  double s0, s1, s2;
+  wave_dot3(A, q0, q1, x, lane, s0, s1, s2);
+  if (lane == 0) {
+    if (backward) {
+      rbuf[3 * (size_t)i] = rhs[3 * (size_t)i] / A.d[i] - s0;
+      rbuf[3 * (size_t)i + 1] = rhs[3 * (size_t)i + 1] / A.d[i] - s1;
+      rbuf[3 * (size_t)i + 2] = rhs[3 * (size_t)i + 2] / A.d[i] - s2;
+    } else {
+      rbuf[3 * (size_t)i] = rhs[3 * (size_t)i] - s0;
+      rbuf[3 * (size_t)i + 1] = rhs[3 * (size_t)i + 1] - s1;
+      rbuf[3 * (size_t)i + 2] = rhs[3 * (size_t)i + 2] - s2;
+    }
+  }
+}
+
+// Split heights, dense phase: one wave per row i of block J, ii = i − start(J):
+// forward x_i = Σ_{l ≤ ii} M[ii][l] r_l;  backward x_i = Σ_{l ≥ ii} Mᵀ[ii][l] r_l.
+__global__ __launch_bounds__(256) void lp_dense_kernel(LpArgs A, const double* __restrict__ g,
+                                                       const uint32_t* __restrict__ block_of,
+                                                       const uint32_t* __restrict__ rows, uint32_t count, int backward,
+                                                       const double* __restrict__ rbuf, double* __restrict__ x) {
+  const uint32_t w = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (w >= count) return;
+  const uint32_t i = rows[w];
+  const uint32_t J = block_of[i], j0 = A.blk_start[J], k = A.blk_rows[J], ii = i - j0;
+  const double* row = g + A.blk_inv[J] + (size_t)ii * k;
+  const uint32_t l0 = backward ? ii : 0, l1 = backward ? k : ii + 1;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (uint32_t l = l0 + lane; l < l1; l += 64) {
+    const double m = row[l];
+    const double* r = rbuf + 3 * (size_t)(j0 + l);
+    s0 += m * r[0];
+    s1 += m * r[1];
+    s2 += m * r[2];
+  }
+  s0 = wsum64d(s0);
+  s1 = wsum64d(s1);
+  s2 = wsum64d(s2);
+  if (lane == 0) {
+    x[3 * (size_t)i] = s0;
+    x[3 * (size_t)i + 1] = s1;
+    x[3 * (size_t)i + 2] = s2;
+  }
+}
+
+// The multiplier row's forward residual (its row of L spans every column): kLongParts workgroups
+// each sum a strided share into parts[3 × part]; the final kernel adds the parts in order and
+// writes x_n = b_n − Σ (its block is 1 × 1 with inverse 1).
+__global__ __launch_bounds__(256) void lp_long_partial_kernel(LpArgs A, uint32_t i, const double* __restrict__ x,
+                                                              double* __restrict__ parts) {
+  __shared__ double red[16];
+  double s0, s1, s2;
+  const uint32_t q0 = A.ptr[i], q1 = A.split[i];
+  const uint32_t per = (q1 - q0 + kLongParts - 1) / kLongParts;
+  const uint32_t lo = min(q1, q0 + blockIdx.x * per), hi = min(q1, lo + per);
+  block_dot3(A, lo, hi, x, red, s0, s1, s2);
+  if (threadIdx.x == 0) {
+    parts[3 * blockIdx.x] = s0;
+    parts[3 * blockIdx.x + 1] = s1;
+    parts[3 * blockIdx.x + 2] = s2;
+  }
+}
+
+__global__ void lp_long_final_kernel(uint32_t i, const double* __restrict__ parts, const double* __restrict__ b,
+                                     double* __restrict__ x) {
+  const int a = threadIdx.x;
+  if (a >= 3) return;
+  double s = 0.0;
+  for (uint32_t p = 0; p < kLongParts; ++p) s += parts[3 * p + a];
+  x[3 * (size_t)i + a] = b[3 * (size_t)i + a] - s;
+}
+
 LprimePlan make_lprime_plan(const Factor& f, const NdBlocks& nd) {
   LprimePlan P;
   const size_t n = f.n;
@@ -488,6 +572,26 @@ LprimePlan make_lprime_plan(const Factor& f, const NdBlocks& nd) {
     level_off.push_back((uint32_t)iblock.size());
   }
   for (uint32_t b = 0; b < nb; ++b) brows[b] = bend[b] - bstart[b];
+  // transposed inverses, row → block, fused or split heights
+  P.inv_t.assign(M.size(), 0.0);
+  for (uint32_t b = 0; b < nb; ++b) {
+    const uint32_t k = brows[b];
+    const double* m = M.data() + ioff[b];
+    double* mt = P.inv_t.data() + ioff[b];
+    for (uint32_t i = 0; i < k; ++i)
+      for (uint32_t j = 0; j < k; ++j) mt[(size_t)j * k + i] = m[(size_t)i * k + j];
+  }
+  P.block_of = block_of;
+  P.fused.assign(H, 1);
+  for (uint32_t b = 0; b < nb; ++b)
+    if (brows[b] > kChunk) P.fused[bheight[b]] = 0;
+  P.split_off.assign(1, 0);
+  for (uint32_t h = 0; h < H; ++h) {
+    if (!P.fused[h])
+      for (auto [b, r0] : by[h])
+        for (uint32_t i = bstart[b] + r0; i < std::min(bstart[b] + r0 + kChunk, bend[b]); ++i) P.split_rows.push_back(i);
+    P.split_off.push_back((uint32_t)P.split_rows.size());
+  }
   return P;
 }
 
@@ -513,8 +617,15 @@ void LprimeDev::upload(ddgrs_ctx* ctx, const NdBlocks& nd) {
   ddgrs::upload(ctx, blk_rows, P.rows);
   ddgrs::upload(ctx, blk_inv, P.inv_off);
   ddgrs::upload(ctx, inv, P.inv);
+  ddgrs::upload(ctx, inv_t, P.inv_t);
+  ddgrs::upload(ctx, block_of, P.block_of);
+  ddgrs::upload(ctx, split_rows, P.split_rows);
+  split_off = P.split_off;
+  fused = P.fused;
   xb.reserve(3 * n);
   xf.reserve(3 * n);
+  rbuf.reserve(3 * n);
+  parts.reserve(3 * kLongParts);
   ctx->sync();  // host vectors above are released on return
   inverse_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -528,17 +639,40 @@ void LprimeDev::solve(ddgrs_ctx* ctx, const double* r, double* x) {
   LpArgs ba{item_block.ptr, item_row0.ptr, blk_start.ptr, blk_rows.ptr, blk_inv.ptr, inv.ptr,
             cp.ptr,         csplit.ptr,    ci.ptr,        cx.ptr,       d.ptr};
   const uint32_t H = (uint32_t)level_off.size() - 1;
-  const size_t lds_f = (16 + 3 * (size_t)max_rows) * sizeof(double);
-  const size_t lds_b = (16 + 768 + 3 * (size_t)max_rows) * sizeof(double);
+  const size_t lds_f = (16 + 3 * (size_t)kChunk) * sizeof(double);
+  const size_t lds_b = (16 + 768 + 3 * (size_t)kChunk) * sizeof(double);
+  // forward: heights bottom-up; the top height is the multiplier row (row n − 1)
   for (uint32_t h = 0; h < H; ++h) {
-    const uint32_t cnt = level_off[h + 1] - level_off[h];
-    launch(ctx, "lp_forward", lp_forward_kernel, dim3(cnt), dim3(256), lds_f, fa, level_off[h], (const double*)xb.ptr,
-           xf.ptr);
+    if (h == H - 1) {
+      launch(ctx, "lp_long_partial", lp_long_partial_kernel, dim3(kLongParts), dim3(256), 0, fa, un - 1,
+             (const double*)xf.ptr, parts.ptr);
+      launch(ctx, "lp_long_final", lp_long_final_kernel, dim3(1), dim3(64), 0, un - 1, (const double*)parts.ptr,
+             (const double*)xb.ptr, xf.ptr);
+    } else if (fused[h]) {
+      launch(ctx, "lp_forward", lp_forward_kernel, dim3(level_off[h + 1] - level_off[h]), dim3(256), lds_f, fa,
+             level_off[h], (const double*)xb.ptr, xf.ptr);
+    } else {
+      const uint32_t cnt = split_off[h + 1] - split_off[h];
+      const uint32_t* rows = split_rows.ptr + split_off[h];
+      launch(ctx, "lp_residual", lp_residual_kernel, dim3(blocks_for(cnt, 4)), dim3(256), 0, fa, rows, cnt, 0,
+             (const double*)xb.ptr, (const double*)xf.ptr, rbuf.ptr);
+      launch(ctx, "lp_dense", lp_dense_kernel, dim3(blocks_for(cnt, 4)), dim3(256), 0, fa, (const double*)inv.ptr,
+             (const uint32_t*)block_of.ptr, rows, cnt, 0, (const double*)rbuf.ptr, xf.ptr);
+    }
   }
+  // backward: heights top-down (the multiplier row is a fused 1 × 1 block)
   for (uint32_t h = H; h-- > 0;) {
-    const uint32_t cnt = level_off[h + 1] - level_off[h];
-    launch(ctx, "lp_backward", lp_backward_kernel, dim3(cnt), dim3(256), lds_b, ba, level_off[h], (const double*)xf.ptr,
-           xb.ptr);
+    if (fused[h] || h == H - 1) {
+      launch(ctx, "lp_backward", lp_backward_kernel, dim3(level_off[h + 1] - level_off[h]), dim3(256), lds_b, ba,
+             level_off[h], (const double*)xf.ptr, xb.ptr);
+    } else {
+      const uint32_t cnt = split_off[h + 1] - split_off[h];
+      const uint32_t* rows = split_rows.ptr + split_off[h];
+      launch(ctx, "lp_residual", lp_residual_kernel, dim3(blocks_for(cnt, 4)), dim3(256), 0, ba, rows, cnt, 1,
+             (const double*)xf.ptr, (const double*)xb.ptr, rbuf.ptr);
+      launch(ctx, "lp_dense", lp_dense_kernel, dim3(blocks_for(cnt, 4)), dim3(256), 0, ba, (const double*)inv_t.ptr,
+             (const uint32_t*)block_of.ptr, rows, cnt, 1, (const double*)rbuf.ptr, xb.ptr);
+    }
   }
   launch(ctx, "lp_scatter", lp_scatter_kernel, dim3(blocks_for(3 * n, 256)), dim3(256), 0, un, unv,
          (const uint32_t*)perm.ptr, (const double*)xb.ptr, x);

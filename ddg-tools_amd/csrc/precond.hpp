@@ -44,10 +44,17 @@ NdBlocks nested_dissection(const HostCsr& k, const std::vector<double>& pos);
 // Host plan of the block solves: blocks (the separator tree's, then the multiplier row as the
 // root), their heights, the off-block split points of every row and column of L, the dense
 // inverses of the diagonal blocks (row-major), and the work items (block, first row) per height.
+// A height whose blocks all fit one workgroup chunk (≤ 64 rows) is solved by one fused launch
+// per direction (items = blocks); a height with a larger block is split into a residual launch
+// and a dense launch, one wave per row each (rows of the height in split_rows).  The top height
+// is the multiplier row alone: its forward residual spans every column and is summed by
+// kLongParts workgroups, then reduced in a fixed order.
 struct LprimePlan {
   std::vector<uint32_t> bstart, bend, bheight, rows, rsplit, csplit, item_block, item_row0, level_off;
+  std::vector<uint32_t> block_of, split_rows, split_off;  // split_off[h]: rows of height h (empty if fused)
+  std::vector<uint8_t> fused;
   std::vector<uint64_t> inv_off;
-  std::vector<double> inv;
+  std::vector<double> inv, inv_t;  // per block: M = L_JJ⁻¹ and Mᵀ, row-major
   uint32_t max_rows = 0;
 };
 // Throws if an off-block entry of L joins two blocks of the same height (the blocks of one height
@@ -58,13 +65,15 @@ struct LprimeDev {
   size_t n = 0, nv = 0;
   Factor f;
   std::vector<uint32_t> level_off;              // work items of each height: [level_off[h], level_off[h+1])
+  std::vector<uint32_t> split_off;
+  std::vector<uint8_t> fused;
   uint32_t max_rows = 0;
   DevBuf<uint32_t> perm, rp, rj, cp, ci, rsplit, csplit;
   DevBuf<double> rx, cx, d;
-  DevBuf<uint32_t> item_block, item_row0, blk_start, blk_rows;
+  DevBuf<uint32_t> item_block, item_row0, blk_start, blk_rows, block_of, split_rows;
   DevBuf<uint64_t> blk_inv;                     // offset of each block's dense inverse (row-major k × k)
-  DevBuf<double> inv;
-  DevBuf<double> xb, xf;                        // n × 3 work arrays (permuted numbering)
+  DevBuf<double> inv, inv_t;
+  DevBuf<double> xb, xf, rbuf, parts;           // n × 3 work arrays (permuted numbering), partial sums
   double inverse_seconds = 0.0;
   // Block schedule and dense inverses of the blocks' diagonal parts of L.
   void upload(ddgrs_ctx* ctx, const NdBlocks& nd);
