@@ -21,15 +21,24 @@ modelled ones:
   channels as FP32 hi/lo pairs).
 
 Both are reported (``rooflines``); their launch durations come from HIP events on the library's own
-stream during a profiled replay of the first timed step.  ``cpu_baseline`` runs the FP64 oracle (a
-C++ restatement of the reference's Rust step) on the host for the same step from the same input.
+stream during a profiled replay of the first timed step.  ``traffic`` is the HBM bytes per launch
+from ``rocprofv3 --pmc`` FETCH_SIZE and WRITE_SIZE passes over a one-step run, each scaled by the
+counter's reading for the 256 MiB copy of ``ddg-tools_amd/lib/hbm_copy`` (run as child processes
+before this process touches the GPU; ``--no-traffic`` skips them).  ``cpu_baseline`` runs the FP64
+oracle (a C++ restatement of the reference's Rust step) on the host for the same step from the
+same input.
 """
 from __future__ import annotations
 
 import argparse
+import csv
+import glob
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -56,6 +65,62 @@ def lumped_face_node_pairs(tree: dict, lists: dict) -> int:
     return int((counts * sizes).sum())
 
 
+KERNEL_NAMES = {"bt_panel_c12": "bt_panel_kernel<12, 9, 2>", "bh_gradient": "bh_gradient_kernel<0>"}
+
+
+def _counter_kib(directory: str, counter: str, substr: str):
+    """Mean counter value (KiB) per dispatch of the kernels whose name contains `substr`."""
+    vals = []
+    for path in glob.glob(os.path.join(directory, "**", "*counter_collection.csv"), recursive=True):
+        with open(path) as fh:
+            for r in csv.DictReader(fh):
+                if r["Counter_Name"] == counter and substr in r["Kernel_Name"]:
+                    vals.append(float(r["Counter_Value"]))
+    return sum(vals) / len(vals) if vals else None
+
+
+def measure_traffic(timeout_s: int = 240):
+    """HBM bytes per launch of the modelled kernels from `rocprofv3 --pmc` (FETCH_SIZE, then
+    WRITE_SIZE, each in its own run of a one-step bench and of the 256 MiB copy probe, scaled by
+    what the counter reads for the copy).  Runs as child processes before this process touches
+    the GPU; returns {} when rocprofv3 or the probe is missing or a pass fails."""
+    prof = shutil.which("rocprofv3")
+    probe = os.path.join(ROOT, "ddg-tools_amd", "lib", "hbm_copy")
+    if not prof or not os.path.exists(probe):
+        return {}
+    known_kib = 256 * 1024
+    env = {**os.environ, "TMPDIR": "/tmp"}
+    res = {}
+    with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
+        scaled = {k: 0.0 for k in KERNEL_NAMES}
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            runs = {
+                "bench": [sys.executable, os.path.abspath(__file__), "--steps", "1", "--warmup", "0",
+                          "--no-cpu-baseline", "--no-traffic"],
+                "copy": [probe, "256", "3"],
+            }
+            for tag, cmd in runs.items():
+                d = os.path.join(tmp, f"{counter}_{tag}")
+                r = subprocess.run(["timeout", "-s", "KILL", str(timeout_s), prof, "--pmc", counter, "--output-format",
+                                    "csv", "-d", d, "-o", "run", "--", *cmd], cwd="/tmp", env=env,
+                                   stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+                if r.returncode not in (0, 139) and not glob.glob(os.path.join(d, "**", "*.csv"), recursive=True):
+                    return {}
+            copy_kib = _counter_kib(os.path.join(tmp, f"{counter}_copy"), counter, "copy_kernel")
+            if not copy_kib:
+                return {}
+            factor = known_kib / copy_kib
+            res[counter + "_scale"] = factor
+            for key, name in KERNEL_NAMES.items():
+                kib = _counter_kib(os.path.join(tmp, f"{counter}_bench"), counter, name)
+                if kib is None:
+                    scaled[key] = None
+                elif scaled[key] is not None:
+                    scaled[key] += kib * factor * 1024.0
+        res.update({k: v for k, v in scaled.items() if v is not None})
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -63,11 +128,14 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 --pmc traffic passes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU (asked for {args.gpus}); how the step shards over GPUs is described, not built.",
               file=sys.stderr)
         return 2
+
+    traffic = {} if args.no_traffic else measure_traffic()
 
     from __graft_entry__ import load_package
 
@@ -121,7 +189,7 @@ def main() -> int:
         a = flop / (kms * 1e-3) / 1e12
         rooflines["bh_gradient"] = {
             "bound": "valu", "achieved": a, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": a / PEAK_FP32_TFLOPS,
-            "traffic": None, "kernel_ms": kms, "ms_per_step": g["ms"],
+            "traffic": traffic.get("bh_gradient"), "kernel_ms": kms, "ms_per_step": g["ms"],
             "work": f"{FLOP_PER_INTERACTION} FLOP x ({counts['pairs_exact']} exact ordered pairs + {face_node} "
                     f"lumped (face, node) pairs)",
         }
@@ -132,7 +200,8 @@ def main() -> int:
         a = nbytes / (kms * 1e-3) / 1e9
         rooflines["bt_panel_c12"] = {
             "bound": "hbm", "achieved": a, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": a / PEAK_HBM_GBS,
-            "traffic": None, "kernel_ms": kms, "ms_per_step": m["ms"], "launches_per_step": m["launches"],
+            "traffic": traffic.get("bt_panel_c12"), "algorithmic_bytes": nbytes, "kernel_ms": kms,
+            "ms_per_step": m["ms"], "launches_per_step": m["launches"],
             "work": f"{BYTES_PER_NEAR_ENTRY} B x {mstats['near_total']} near entries + {BYTES_PER_FACE_ROWS} B x {nf} faces",
         }
     dominant = max(rooflines, key=lambda k: rooflines[k]["ms_per_step"]) if rooflines else None
@@ -186,6 +255,7 @@ def main() -> int:
         "roofline": roof,
         "rooflines": rooflines,
         "cpu_baseline": cpu,
+        "traffic_scale": {k: v for k, v in traffic.items() if k.endswith("_scale")},
         "kernel_ms_step1": {k: round(v["ms"], 4) for k, v in sorted(kt.items(), key=lambda kv: -kv[1]["ms"])[:16]},
     }
     print(json.dumps(line))

@@ -325,22 +325,30 @@ __global__ __launch_bounds__(256) void node_pack_kernel(int nn, const double* __
   clo[a] = make_float4((float)(x[0] - (double)hx), (float)(x[1] - (double)hy), (float)(x[2] - (double)hz), 0.f);
 }
 
-// Node sums b_C, u_C over the transposed lumped lists (entries increasing), FP64.
+// Node sums b_C, u_C over the transposed lumped lists, FP64: one wave per node, lanes strided over
+// the entries, then a fixed-order butterfly (the result does not depend on timing).
 __global__ __launch_bounds__(256) void bh_node_sums_kernel(int nn, const uint32_t* __restrict__ off,
                                                            const uint32_t* __restrict__ ent,
                                                            const float4* __restrict__ node_bu,
                                                            double4* __restrict__ sums) {
-  int a = blockIdx.x * 256 + threadIdx.x;
+  const int a = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
   if (a >= nn) return;
   double b = 0.0, ux = 0.0, uy = 0.0, uz = 0.0;
-  for (uint32_t q = off[a]; q < off[a + 1]; ++q) {
+  for (uint32_t q = off[a] + lane; q < off[a + 1]; q += 64) {
     float4 t = node_bu[ent[q]];
     b += (double)t.x;
     ux += (double)t.y;
     uy += (double)t.z;
     uz += (double)t.w;
   }
-  sums[a] = make_double4(b, ux, uy, uz);
+  for (int o = 32; o > 0; o >>= 1) {
+    b += __shfl_xor(b, o);
+    ux += __shfl_xor(ux, o);
+    uy += __shfl_xor(uy, o);
+    uz += __shfl_xor(uz, o);
+  }
+  if (lane == 0) sums[a] = make_double4(b, ux, uy, uz);
 }
 
 // Per face (BVH position p): the downward pass along the chain root → leaf (barnes_hut.rs:521-538)
@@ -500,7 +508,7 @@ void BhEval::gradient(ddgrs_surface* s, const Exps& e, double* total, double* pa
   else
     launch(ctx, "bh_gradient", bh_gradient_kernel<2>, grid, dim3(256), 0, A, KPow<2>{e.alpha, e.beta, -1, -1},
            node_bu.ptr, part.ptr, leaf_energy.ptr);
-  launch(ctx, "bh_node_sums", bh_node_sums_kernel, dim3(blocks_for(nn, 256)), dim3(256), 0, nn,
+  launch(ctx, "bh_node_sums", bh_node_sums_kernel, dim3(blocks_for(nn, 4)), dim3(256), 0, nn,
          (const uint32_t*)d_lumped_t_off.ptr, (const uint32_t*)d_lumped_t.ptr, (const float4*)node_bu.ptr, sums.ptr);
   launch(ctx, "bh_final", bh_final_kernel, dim3(blocks_for(nf, 256)), dim3(256), 0, nf, (const uint32_t*)d_order.ptr,
          (const uint32_t*)d_leaf_of_pos.ptr, (const uint32_t*)d_chain_ptr.ptr, (const uint32_t*)d_chain.ptr,

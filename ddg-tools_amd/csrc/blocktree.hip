@@ -195,26 +195,34 @@ __global__ __launch_bounds__(256) void bt_panel_kernel(PanelArgs P, const float*
   for (int r = 0; r < 8; ++r)
 #pragma unroll
     for (int c = 0; c < C; ++c) acc[r][c] = 0.f;
+  // The column index of the next chunk is loaded one chunk ahead; all rows' values of a chunk are
+  // loaded before any is used (clamped indices, zeroed outside the panel), so a chunk costs one
+  // round trip to memory, not one per row.
+  uint32_t tnext = P.cols[c0 + (lane < K ? lane : 0)];
   for (uint32_t base = 0; base < K; base += 64) {
     const uint32_t kk = base + lane;
     const bool act = kk < K;
-    const uint32_t tcol = P.cols[c0 + (act ? kk : 0)];
+    const uint32_t tcol = tnext;
+    tnext = P.cols[c0 + (kk + 64 < K ? kk + 64 : 0)];
     float th[C], tl[C];
     load_hl<C>(u_hl + (size_t)tcol * 2 * C, th, tl);
+    float v0[8], v1[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const bool ok = act && (uint32_t)r < R;
+      const float* v = vbase + (ok ? ((size_t)r * K + kk) * NK : 0);
+      v0[r] = v[0];
+      v1[r] = v[NK - 1];
+      if (!ok) v0[r] = v1[r] = 0.f;
+    }
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
       if ((uint32_t)r < R) {
-        float v0 = 0.f, v1 = 0.f;
-        if (act) {
-          const float* v = vbase + ((size_t)r * K + kk) * NK;
-          v0 = v[0];
-          v1 = v[NK - 1];
-        }
         const float* srow = u_hl + (size_t)(row0 + r) * 2 * C;
 #pragma unroll
         for (int c = 0; c < C; ++c) {
           const float d = (srow[c] - th[c]) + (srow[C + c] - tl[c]);
-          acc[r][c] += (c < H ? v0 : v1) * d;
+          acc[r][c] += (c < H ? v0[r] : v1[r]) * d;
         }
       }
     }
@@ -257,15 +265,17 @@ __global__ __launch_bounds__(256) void bt_panel_kernel(PanelArgs P, const float*
 
 // ---- structure -----------------------------------------------------------------------------
 
-// α_A = Σ_B c_AB per kernel (FP64), one thread per node.
+// α_A = Σ_B c_AB per kernel (FP64), one wave per node.
 __global__ __launch_bounds__(256) void bt_alpha_node_kernel(int nn, int nk, const uint32_t* __restrict__ far_ptr,
                                                             const float* __restrict__ coef, double* __restrict__ alpha) {
-  const int a = blockIdx.x * 256 + threadIdx.x;
-  if (a >= nn) return;
+  const uint32_t a = wave_id();
+  const int lane = threadIdx.x & 63;
+  if (a >= (uint32_t)nn) return;
   for (int k = 0; k < nk; ++k) {
     double s = 0.0;
-    for (uint32_t q = far_ptr[a]; q < far_ptr[a + 1]; ++q) s += (double)coef[(size_t)q * nk + k];
-    alpha[(size_t)a * nk + k] = s;
+    for (uint32_t q = far_ptr[a] + lane; q < far_ptr[a + 1]; q += 64) s += (double)coef[(size_t)q * nk + k];
+    s = wsum64d(s);
+    if (lane == 0) alpha[(size_t)a * nk + k] = s;
   }
 }
 
@@ -360,7 +370,7 @@ void BlockTreeDev::upload(ddgrs_ctx* ctx, const Topology& t, const BlockStructur
 void BlockTreeDev::finish_far(ddgrs_ctx* ctx) {
   alpha_node.reserve((size_t)nn * nk);
   alpha_leaf.reserve((size_t)nl * nk);
-  launch(ctx, "bt_alpha_node", bt_alpha_node_kernel, dim3(blocks_for(nn, 256)), dim3(256), 0, nn, nk,
+  launch(ctx, "bt_alpha_node", bt_alpha_node_kernel, dim3(blocks_for(nn, 4)), dim3(256), 0, nn, nk,
          (const uint32_t*)far_ptr.ptr, (const float*)far_coef.ptr, alpha_node.ptr);
   launch(ctx, "bt_alpha_leaf", bt_alpha_leaf_kernel, dim3(blocks_for(nl, 256)), dim3(256), 0, nl, nk,
          (const uint32_t*)leaves.ptr, (const uint32_t*)parent.ptr, (const double*)alpha_node.ptr, alpha_leaf.ptr);
