@@ -517,49 +517,69 @@ void BhEval::gradient(ddgrs_surface* s, const Exps& e, double* total, double* pa
   sum_f64(ctx, nl, leaf_energy.ptr, total);
 }
 
-void build_face_tree(ddgrs_surface* s, size_t leaf_size, Topology* topo, Clusters* clusters) {
+HostGeometry download_geometry(ddgrs_surface* s) {
   ddgrs_ctx* ctx = s->ctx;
   const size_t nf_ = (size_t)s->nf, nv = (size_t)s->nv;
-  std::vector<double> pos(3 * nv), area(nf_), bary(3 * nf_), nrm(3 * nf_), lo(3 * nf_), hi(3 * nf_);
-  download(ctx, pos.data(), (const double*)s->positions.ptr, 3 * nv);
-  download(ctx, area.data(), (const double*)s->area.ptr, nf_);
-  download(ctx, bary.data(), (const double*)s->bary.ptr, 3 * nf_);
-  download(ctx, nrm.data(), (const double*)s->normal.ptr, 3 * nf_);
+  HostGeometry g;
+  g.pos.resize(3 * nv);
+  g.area.resize(nf_);
+  g.bary.resize(3 * nf_);
+  g.nrm.resize(3 * nf_);
+  download(ctx, g.pos.data(), (const double*)s->positions.ptr, 3 * nv);
+  download(ctx, g.area.data(), (const double*)s->area.ptr, nf_);
+  download(ctx, g.bary.data(), (const double*)s->bary.ptr, 3 * nf_);
+  download(ctx, g.nrm.data(), (const double*)s->normal.ptr, 3 * nf_);
   ctx->sync();
+  g.lo.resize(3 * nf_);
+  g.hi.resize(3 * nf_);
   for (size_t f = 0; f < nf_; ++f)
     for (int k = 0; k < 3; ++k) {
       double a = INFINITY, b = -INFINITY;
       for (int c = 0; c < 3; ++c) {
-        double x = pos[3 * (size_t)s->faces_h[3 * f + c] + k];
+        double x = g.pos[3 * (size_t)s->faces_h[3 * f + c] + k];
         a = std::fmin(a, x);
         b = std::fmax(b, x);
       }
-      lo[3 * f + k] = a;
-      hi[3 * f + k] = b;
+      g.lo[3 * f + k] = a;
+      g.hi[3 * f + k] = b;
     }
-  *topo = build_topology(nf_, bary.data(), lo.data(), hi.data(), leaf_size);
-  *clusters = build_clusters(*topo, bary.data(), lo.data(), hi.data(), area.data(), nrm.data());
+  // RS/src/geometry.rs:194 vertex_areas (faces ascending)
+  g.vmass.assign(nv, 0.0);
+  for (size_t f = 0; f < nf_; ++f) {
+    double a = g.area[f] / 3.0;
+    for (int c = 0; c < 3; ++c) g.vmass[s->faces_h[3 * f + c]] += a;
+  }
+  return g;
+}
+
+void build_face_tree_host(const HostGeometry& g, size_t leaf_size, Topology* topo, Clusters* clusters) {
+  const size_t nf_ = g.area.size();
+  *topo = build_topology(nf_, g.bary.data(), g.lo.data(), g.hi.data(), leaf_size);
+  *clusters = build_clusters(*topo, g.bary.data(), g.lo.data(), g.hi.data(), g.area.data(), g.nrm.data());
+}
+
+void build_face_tree(ddgrs_surface* s, size_t leaf_size, Topology* topo, Clusters* clusters) {
+  build_face_tree_host(download_geometry(s), leaf_size, topo, clusters);
 }
 
 void vertex_areas_host(ddgrs_surface* s, std::vector<double>* pos, std::vector<double>* mass) {
-  ddgrs_ctx* ctx = s->ctx;
-  const size_t nf_ = (size_t)s->nf, nv = (size_t)s->nv;
-  std::vector<double> area(nf_);
-  pos->resize(3 * nv);
-  download(ctx, pos->data(), (const double*)s->positions.ptr, 3 * nv);
-  download(ctx, area.data(), (const double*)s->area.ptr, nf_);
-  ctx->sync();
-  mass->assign(nv, 0.0);
-  for (size_t f = 0; f < nf_; ++f) {
-    double a = area[f] / 3.0;
-    for (int c = 0; c < 3; ++c) (*mass)[s->faces_h[3 * f + c]] += a;
-  }
+  HostGeometry g = download_geometry(s);
+  *pos = std::move(g.pos);
+  *mass = std::move(g.vmass);
 }
 
 void BhEval::build(ddgrs_surface* s, double theta_, size_t leaf_size_) {
+  Topology t;
+  Clusters c;
+  build_face_tree(s, leaf_size_, &t, &c);
+  build_from(s, t, c, theta_, leaf_size_);
+}
+
+void BhEval::build_from(ddgrs_surface* s, const Topology& t, const Clusters& c, double theta_, size_t leaf_size_) {
   theta = theta_;
   leaf_size = leaf_size_;
-  build_face_tree(s, leaf_size, &topo, &clusters);
+  topo = t;
+  clusters = c;
   lists = build_bh_lists(topo, clusters, theta);
   upload_structure(s->ctx);
   refresh(s);

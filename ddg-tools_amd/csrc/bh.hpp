@@ -29,6 +29,8 @@ struct BhEval {
 
   // Tree and lists at the surface's current positions (face geometry must be current).
   void build(ddgrs_surface* s, double theta, size_t leaf_size);
+  // Lists on a face tree of the surface's current positions.
+  void build_from(ddgrs_surface* s, const Topology& t, const Clusters& c, double theta, size_t leaf_size);
   // Face packs in BVH order and node aggregates of the surface's current positions.
   void refresh(ddgrs_surface* s);
   // Ẽ into total[0] (device).
@@ -44,7 +46,15 @@ struct BhEval {
 }  // namespace ddgrs
 
 namespace ddgrs {
-// Face BVH (topology + build-time clusters with projectors) at the surface's current positions.
+// The surface's current positions and face geometry on the host (one synchronisation).
+struct HostGeometry {
+  std::vector<double> pos, area, bary, nrm, lo, hi;  // 3 nv, nf, 3 nf, 3 nf, 3 nf (face boxes), 3 nf
+  std::vector<double> vmass;                           // vertex areas a_v = ⅓ Σ_{S∋v} a_S
+};
+HostGeometry download_geometry(ddgrs_surface* s);
+// Face BVH (topology + build-time clusters with projectors) of a host snapshot.
+void build_face_tree_host(const HostGeometry& g, size_t leaf_size, Topology* topo, Clusters* clusters);
+// Face BVH at the surface's current positions.
 void build_face_tree(ddgrs_surface* s, size_t leaf_size, Topology* topo, Clusters* clusters);
 // Vertex areas a_v = ⅓ Σ_{S∋v} a_S (RS/src/geometry.rs:194) and positions of the surface (host).
 void vertex_areas_host(ddgrs_surface* s, std::vector<double>* pos, std::vector<double>* mass);

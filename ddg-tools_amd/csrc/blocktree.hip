@@ -19,6 +19,7 @@
 #include <algorithm>
 
 #include "blocktree.hpp"
+#include "host_parallel.hpp"
 
 namespace ddgrs {
 
@@ -296,73 +297,101 @@ __global__ __launch_bounds__(256) void bt_alpha_leaf_kernel(int nl, int nk, cons
   }
 }
 
-void BlockTreeDev::upload(ddgrs_ctx* ctx, const Topology& t, const BlockStructure& s, int nk_,
-                          const std::vector<double>& weight_bvh, const std::vector<double>& node_mass) {
+BlockTreePlan plan_block_tree(const Topology& t, const BlockStructure& s, int nk, std::vector<double> weight_bvh,
+                              const std::vector<double>& node_mass) {
   constexpr uint32_t kRows = 8;
-  nk = nk_;
-  n = (int)t.order.size();
-  nn = (int)t.n_nodes();
-  nl = (int)t.leaves.size();
-  ddgrs::upload(ctx, order, t.order);
-  std::vector<uint32_t> pos(n);
-  for (int p = 0; p < n; ++p) pos[t.order[p]] = (uint32_t)p;
-  ddgrs::upload(ctx, position, pos);
-  ddgrs::upload(ctx, start, t.start);
-  ddgrs::upload(ctx, end, t.end);
-  ddgrs::upload(ctx, right, t.right);
-  ddgrs::upload(ctx, parent, t.parent);
-  ddgrs::upload(ctx, leaves, t.leaves);
+  BlockTreePlan P;
+  P.nk = nk;
+  P.n = (int)t.order.size();
+  P.nn = (int)t.n_nodes();
+  P.nl = (int)t.leaves.size();
+  P.order = t.order;
+  P.position.resize(P.n);
+  for (int p = 0; p < P.n; ++p) P.position[t.order[p]] = (uint32_t)p;
+  P.start = t.start;
+  P.end = t.end;
+  P.right = t.right;
+  P.parent = t.parent;
+  P.leaves = t.leaves;
   // panels: per leaf, the column ranges of the near blocks of its ancestors, ascending
-  std::vector<uint32_t> cptr(nl + 1, 0), cl, pleaf, prow0, prows;
-  std::vector<uint64_t> pval;
-  std::vector<std::pair<uint32_t, uint32_t>> ranges;
-  uint64_t total = 0;
-  for (int k = 0; k < nl; ++k) {
-    ranges.clear();
+  const int nl = P.nl;
+  std::vector<std::vector<std::pair<uint32_t, uint32_t>>> ranges(nl);
+  parallel((size_t)nl, [&](size_t k) {
+    auto& rg = ranges[k];
     for (uint32_t ci = s.chain_ptr[k]; ci < s.chain_ptr[k + 1]; ++ci) {
       const uint32_t a = s.chain[ci];
       for (uint32_t q = s.near_ptr[a]; q < s.near_ptr[a + 1]; ++q) {
         const uint32_t b = s.near_col[q];
-        ranges.push_back({t.start[b], t.end[b]});
+        rg.push_back({t.start[b], t.end[b]});
       }
     }
-    std::sort(ranges.begin(), ranges.end());
-    for (auto [b0, b1] : ranges)
-      for (uint32_t x = b0; x < b1; ++x) cl.push_back(x);
-    cptr[k + 1] = (uint32_t)cl.size();
-    const uint32_t K = cptr[k + 1] - cptr[k];
-    const uint32_t leaf = t.leaves[k];
+    std::sort(rg.begin(), rg.end());
+  });
+  P.col_ptr.assign(nl + 1, 0);
+  for (int k = 0; k < nl; ++k) {
+    uint32_t K = 0;
+    for (auto [b0, b1] : ranges[k]) K += b1 - b0;
+    P.col_ptr[k + 1] = P.col_ptr[k] + K;
+  }
+  P.cols.resize(P.col_ptr[nl]);
+  parallel((size_t)nl, [&](size_t k) {
+    uint32_t o = P.col_ptr[k];
+    for (auto [b0, b1] : ranges[k])
+      for (uint32_t x = b0; x < b1; ++x) P.cols[o++] = x;
+  });
+  uint64_t total = 0;
+  for (int k = 0; k < nl; ++k) {
+    const uint32_t K = P.col_ptr[k + 1] - P.col_ptr[k], leaf = t.leaves[k];
     for (uint32_t r0 = t.start[leaf]; r0 < t.end[leaf]; r0 += kRows) {
       const uint32_t rows = std::min(kRows, t.end[leaf] - r0);
-      pleaf.push_back((uint32_t)k);
-      prow0.push_back(r0);
-      prows.push_back(rows);
-      pval.push_back(total);
+      P.panel_leaf.push_back((uint32_t)k);
+      P.panel_row0.push_back(r0);
+      P.panel_rows.push_back(rows);
+      P.panel_val.push_back(total);
       total += (uint64_t)rows * K;
     }
   }
-  np = (int)pleaf.size();
-  near_total = total;
-  ddgrs::upload(ctx, col_ptr, cptr);
-  ddgrs::upload(ctx, cols, cl);
-  ddgrs::upload(ctx, panel_leaf, pleaf);
-  ddgrs::upload(ctx, panel_row0, prow0);
-  ddgrs::upload(ctx, panel_rows, prows);
-  ddgrs::upload(ctx, panel_val, pval);
-  val.reserve((size_t)std::max<uint64_t>(total, 1) * nk);
-  // far field
-  ddgrs::upload(ctx, far_ptr, s.far_ptr);
-  ddgrs::upload(ctx, far_col, s.far_col);
-  std::vector<uint32_t> frow(s.far_col.size());
-  for (int a = 0; a < nn; ++a)
-    for (uint32_t q = s.far_ptr[a]; q < s.far_ptr[a + 1]; ++q) frow[q] = (uint32_t)a;
-  ddgrs::upload(ctx, far_row, frow);
-  n_far = (uint32_t)s.far_col.size();
+  P.np = (int)P.panel_leaf.size();
+  P.near_total = total;
+  P.far_ptr = s.far_ptr;
+  P.far_col = s.far_col;
+  P.far_row.resize(s.far_col.size());
+  for (int a = 0; a < P.nn; ++a)
+    for (uint32_t q = s.far_ptr[a]; q < s.far_ptr[a + 1]; ++q) P.far_row[q] = (uint32_t)a;
+  P.weight = std::move(weight_bvh);
+  P.inv_mass.resize(P.nn);
+  for (int a = 0; a < P.nn; ++a) P.inv_mass[a] = 1.0 / node_mass[a];
+  return P;
+}
+
+void BlockTreeDev::upload(ddgrs_ctx* ctx, const BlockTreePlan& P) {
+  nk = P.nk;
+  n = P.n;
+  nn = P.nn;
+  nl = P.nl;
+  np = P.np;
+  near_total = P.near_total;
+  n_far = (uint32_t)P.far_col.size();
+  ddgrs::upload(ctx, order, P.order);
+  ddgrs::upload(ctx, position, P.position);
+  ddgrs::upload(ctx, start, P.start);
+  ddgrs::upload(ctx, end, P.end);
+  ddgrs::upload(ctx, right, P.right);
+  ddgrs::upload(ctx, parent, P.parent);
+  ddgrs::upload(ctx, leaves, P.leaves);
+  ddgrs::upload(ctx, col_ptr, P.col_ptr);
+  ddgrs::upload(ctx, cols, P.cols);
+  ddgrs::upload(ctx, panel_leaf, P.panel_leaf);
+  ddgrs::upload(ctx, panel_row0, P.panel_row0);
+  ddgrs::upload(ctx, panel_rows, P.panel_rows);
+  ddgrs::upload(ctx, panel_val, P.panel_val);
+  val.reserve((size_t)std::max<uint64_t>(near_total, 1) * nk);
+  ddgrs::upload(ctx, far_ptr, P.far_ptr);
+  ddgrs::upload(ctx, far_col, P.far_col);
+  ddgrs::upload(ctx, far_row, P.far_row);
   far_coef.reserve((size_t)std::max<uint32_t>(n_far, 1) * nk);
-  ddgrs::upload(ctx, weight, weight_bvh);
-  std::vector<double> im(nn);
-  for (int a = 0; a < nn; ++a) im[a] = 1.0 / node_mass[a];
-  ddgrs::upload(ctx, inv_mass, im);
+  ddgrs::upload(ctx, weight, P.weight);
+  ddgrs::upload(ctx, inv_mass, P.inv_mass);
   counters.reserve(nn);
   HIPCHECK(hipMemsetAsync(counters.ptr, 0, (size_t)nn * sizeof(uint32_t), ctx->stream));
 }
@@ -574,22 +603,50 @@ __global__ __launch_bounds__(256) void fm_gather_kernel(int nv, const uint32_t* 
   for (int k = 0; k < 3; ++k) y[3 * (size_t)v + k] = 2.0 * acc[k];
 }
 
+FaceMetricPlan plan_face_metric(const std::vector<uint32_t>& faces, size_t nv, const Topology& t, const Clusters& cl,
+                                double chi, size_t leaf_pairs, const std::vector<double>& face_area) {
+  FaceMetricPlan P;
+  const size_t nf = t.order.size();
+  P.topo = t;
+  P.bs = build_block_structure(t, cl, chi, leaf_pairs);
+  std::vector<double> wbvh(nf);
+  for (size_t p = 0; p < nf; ++p) wbvh[p] = face_area[t.order[p]];
+  P.bt = plan_block_tree(t, P.bs, 2, std::move(wbvh), cl.mass);
+  P.centroid = cl.centroid;
+  P.projector = cl.projector;
+  P.node_mass = cl.mass;
+  // vertex → (BVH position of the face, corner), faces ascending
+  const std::vector<uint32_t>& position = P.bt.position;
+  P.inc_ptr.assign(nv + 1, 0);
+  P.inc.resize(3 * nf);
+  for (size_t f = 0; f < nf; ++f)
+    for (int c = 0; c < 3; ++c) P.inc_ptr[faces[3 * f + c] + 1]++;
+  for (size_t v = 0; v < nv; ++v) P.inc_ptr[v + 1] += P.inc_ptr[v];
+  std::vector<uint32_t> fill(P.inc_ptr.begin(), P.inc_ptr.end() - 1);
+  for (size_t f = 0; f < nf; ++f)
+    for (int c = 0; c < 3; ++c) P.inc[fill[faces[3 * f + c]]++] = (position[f] << 2) | (uint32_t)c;
+  return P;
+}
+
 void FaceMetricDev::build(ddgrs_surface* s, const Topology& t, const Clusters& cl, double chi, size_t leaf_pairs,
                           double s_exp) {
+  std::vector<double> area_h((size_t)s->nf);
+  download(s->ctx, area_h.data(), (const double*)s->area.ptr, (size_t)s->nf);
+  s->ctx->sync();
+  build(s, plan_face_metric(s->faces_h, (size_t)s->nv, t, cl, chi, leaf_pairs, area_h), s_exp);
+}
+
+void FaceMetricDev::build(ddgrs_surface* s, const FaceMetricPlan& P, double s_exp) {
   ddgrs_ctx* ctx = s->ctx;
-  topo = t;
+  topo = P.topo;
+  bs = P.bs;
   nv = (int)s->nv;
   nf = (int)s->nf;
-  bs = build_block_structure(t, cl, chi, leaf_pairs);
-  std::vector<double> area_h(nf), wbvh(nf);
-  download(ctx, area_h.data(), (const double*)s->area.ptr, (size_t)nf);
-  ctx->sync();
-  for (int p = 0; p < nf; ++p) wbvh[p] = area_h[t.order[p]];
-  bt.upload(ctx, t, bs, 2, wbvh, cl.mass);
+  bt.upload(ctx, P.bt);
   DevBuf<double> cen, proj, nmass;
-  ddgrs::upload(ctx, cen, cl.centroid);
-  ddgrs::upload(ctx, proj, cl.projector);
-  ddgrs::upload(ctx, nmass, cl.mass);
+  ddgrs::upload(ctx, cen, P.centroid);
+  ddgrs::upload(ctx, proj, P.projector);
+  ddgrs::upload(ctx, nmass, P.node_mass);
   const double neg_s = -s_exp;
   launch(ctx, "fm_far_coef", fm_far_coef_kernel, dim3(blocks_for(bt.n_far, 256)), dim3(256), 0, bt.n_far,
          (const uint32_t*)bt.far_row.ptr, (const uint32_t*)bt.far_col.ptr, NodeGeo{cen.ptr, proj.ptr, nmass.ptr}, neg_s,
@@ -606,18 +663,8 @@ void FaceMetricDev::build(ddgrs_surface* s, const Topology& t, const Clusters& c
   launch(ctx, "fm_face", fm_face_kernel, dim3(blocks_for(nf, 256)), dim3(256), 0, nf, (const uint32_t*)bt.order.ptr,
          (const uint32_t*)s->faces.ptr, (const double*)s->positions.ptr, (const double*)s->area.ptr, df.ptr, area_p.ptr,
          vert_p.ptr);
-  // vertex → (BVH position of the face, corner), faces ascending
-  std::vector<uint32_t> position(nf);
-  for (int p = 0; p < nf; ++p) position[t.order[p]] = (uint32_t)p;
-  std::vector<uint32_t> ptr(nv + 1, 0), inc_h(3 * (size_t)nf);
-  for (int f = 0; f < nf; ++f)
-    for (int c = 0; c < 3; ++c) ptr[s->faces_h[3 * f + c] + 1]++;
-  for (int v = 0; v < nv; ++v) ptr[v + 1] += ptr[v];
-  std::vector<uint32_t> fill(ptr.begin(), ptr.end() - 1);
-  for (int f = 0; f < nf; ++f)
-    for (int c = 0; c < 3; ++c) inc_h[fill[s->faces_h[3 * f + c]]++] = (position[f] << 2) | (uint32_t)c;
-  ddgrs::upload(ctx, inc_ptr, ptr);
-  ddgrs::upload(ctx, inc, inc_h);
+  ddgrs::upload(ctx, inc_ptr, P.inc_ptr);
+  ddgrs::upload(ctx, inc, P.inc);
   m_hl.reserve(24 * (size_t)nf);
   z_hl.reserve(24 * (size_t)nf);
   ctx->sync();  // the local buffers above are released on return
@@ -701,22 +748,40 @@ __global__ __launch_bounds__(256) void vf_out_kernel(int n, const uint32_t* __re
   }
 }
 
+VertexFractionalPlan plan_vertex_fractional(const std::vector<double>& pos, const std::vector<double>& m,
+                                            double tau, double chi, size_t leaf_size, size_t leaf_pairs) {
+  VertexFractionalPlan P;
+  const size_t nv = m.size();
+  // vertex BVH: point f_v, box the point, mass a_v (hierarchical.rs:1017-1060)
+  P.topo = build_topology(nv, pos.data(), pos.data(), pos.data(), leaf_size);
+  P.clusters = build_clusters(P.topo, pos.data(), pos.data(), pos.data(), m.data());
+  P.bs = build_block_structure(P.topo, P.clusters, chi, leaf_pairs);
+  std::vector<double> wbvh(nv);
+  for (size_t p = 0; p < nv; ++p) wbvh[p] = m[P.topo.order[p]];
+  P.bt = plan_block_tree(P.topo, P.bs, 1, std::move(wbvh), P.clusters.mass);
+  P.pos = pos;
+  P.mass = m;
+  P.tau = tau;
+  return P;
+}
+
 void VertexFractionalDev::build(ddgrs_ctx* ctx, const std::vector<double>& pos, const std::vector<double>& m, double tau,
                                 double chi, size_t leaf_size, size_t leaf_pairs) {
-  nv = (int)m.size();
-  // vertex BVH: point f_v, box the point, mass a_v (hierarchical.rs:1017-1060)
-  topo = build_topology(nv, pos.data(), pos.data(), pos.data(), leaf_size);
-  clusters = build_clusters(topo, pos.data(), pos.data(), pos.data(), m.data());
-  bs = build_block_structure(topo, clusters, chi, leaf_pairs);
-  std::vector<double> wbvh(nv);
-  for (int p = 0; p < nv; ++p) wbvh[p] = m[topo.order[p]];
-  bt.upload(ctx, topo, bs, 1, wbvh, clusters.mass);
-  const double ex = -(tau + 1.0);
+  build(ctx, plan_vertex_fractional(pos, m, tau, chi, leaf_size, leaf_pairs));
+}
+
+void VertexFractionalDev::build(ddgrs_ctx* ctx, const VertexFractionalPlan& P) {
+  nv = (int)P.mass.size();
+  topo = P.topo;
+  clusters = P.clusters;
+  bs = P.bs;
+  bt.upload(ctx, P.bt);
+  const double ex = -(P.tau + 1.0);
   DevBuf<double> cen, dpos, nmass;
-  ddgrs::upload(ctx, cen, clusters.centroid);
-  ddgrs::upload(ctx, dpos, pos);
-  ddgrs::upload(ctx, nmass, clusters.mass);
-  ddgrs::upload(ctx, mass, m);
+  ddgrs::upload(ctx, cen, P.clusters.centroid);
+  ddgrs::upload(ctx, dpos, P.pos);
+  ddgrs::upload(ctx, nmass, P.clusters.mass);
+  ddgrs::upload(ctx, mass, P.mass);
   launch(ctx, "vf_far_coef", vf_far_coef_kernel, dim3(blocks_for(bt.n_far, 256)), dim3(256), 0, bt.n_far,
          (const uint32_t*)bt.far_row.ptr, (const uint32_t*)bt.far_col.ptr, (const double*)cen.ptr,
          (const double*)nmass.ptr, ex, bt.far_coef.ptr);

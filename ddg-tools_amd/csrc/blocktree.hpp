@@ -7,6 +7,19 @@
 
 namespace ddgrs {
 
+// Host part of a BlockTreeDev: the structure, panels and weights (built without the device).
+struct BlockTreePlan {
+  int nk = 1;
+  int n = 0, nn = 0, nl = 0, np = 0;
+  uint64_t near_total = 0;
+  std::vector<uint32_t> order, position, start, end, right, parent, leaves;
+  std::vector<uint32_t> col_ptr, cols, panel_leaf, panel_row0, panel_rows, far_ptr, far_col, far_row;
+  std::vector<uint64_t> panel_val;
+  std::vector<double> weight, inv_mass;
+};
+BlockTreePlan plan_block_tree(const Topology& t, const BlockStructure& s, int nk, std::vector<double> weight_bvh,
+                              const std::vector<double>& node_mass);
+
 // Device form of a block structure over a BVH, for products in difference form
 //   z_S = Σ_{T near S} v_ST (u_S − u_T) + Σ_{far (A, B), S ∈ A} c_AB (u_S − ū_B),
 // v_ST = H_ST a_T (near values × the column weight), c_AB = κ_AB A_B (far coefficient × the column
@@ -41,8 +54,7 @@ struct BlockTreeDev {
 
   // Structure, panels and weights; near values and far coefficients are filled by the caller's
   // kernels, then finish_far() computes α.
-  void upload(ddgrs_ctx* ctx, const Topology& t, const BlockStructure& s, int nk, const std::vector<double>& weight_bvh,
-              const std::vector<double>& node_mass);
+  void upload(ddgrs_ctx* ctx, const BlockTreePlan& p);
   void finish_far(ddgrs_ctx* ctx);
   // z = w_k (H̃a·u − H̃(a u)) per BVH position and channel, channels [0, H) on kernel 0 with w0,
   // [H, C) on kernel NK − 1 with w1.  u and z are hi/lo FP32 pairs: 2 C floats per position
@@ -50,6 +62,17 @@ struct BlockTreeDev {
   template <int C, int H>
   void multiply(ddgrs_ctx* ctx, const float* u_hl, double w0, double w1, float* z_hl);
 };
+
+// Host part of a FaceMetricDev on a given face tree.
+struct FaceMetricPlan {
+  Topology topo;
+  BlockStructure bs;
+  BlockTreePlan bt;
+  std::vector<double> centroid, projector, node_mass;
+  std::vector<uint32_t> inc_ptr, inc;  // vertex → (BVH position of the face, corner), faces ascending
+};
+FaceMetricPlan plan_face_metric(const std::vector<uint32_t>& faces, size_t nv, const Topology& t, const Clusters& c,
+                                double chi, size_t leaf_pairs, const std::vector<double>& face_area);
 
 // B̃, B̃⁰ of one configuration (hierarchical.rs:283).
 struct FaceMetricDev {
@@ -66,9 +89,22 @@ struct FaceMetricDev {
 
   // Builds on a face tree of the surface's current positions with separation χ and leaf_pairs.
   void build(ddgrs_surface* s, const Topology& t, const Clusters& c, double chi, size_t leaf_pairs, double s_exp);
+  void build(ddgrs_surface* s, const FaceMetricPlan& p, double s_exp);
   // y = (w_hi B̃ + w_lo B̃⁰) x for one interleaved 3 nv FP64 vector (device pointers).
   void apply(ddgrs_ctx* ctx, const double* x, double* y);
 };
+
+// Host part of a VertexFractionalDev: vertex BVH (points f_v, masses a_v), block structure, panels.
+struct VertexFractionalPlan {
+  Topology topo;
+  Clusters clusters;
+  BlockStructure bs;
+  BlockTreePlan bt;
+  std::vector<double> pos, mass;
+  double tau = 0.0;
+};
+VertexFractionalPlan plan_vertex_fractional(const std::vector<double>& pos, const std::vector<double>& mass,
+                                            double tau, double chi, size_t leaf_size, size_t leaf_pairs);
 
 // L̃^V_τ over vertex pairs (hierarchical.rs:1003).
 struct VertexFractionalDev {
@@ -81,6 +117,7 @@ struct VertexFractionalDev {
   DevBuf<float> u_hl, z_hl; // product scratch: 3 channels, hi/lo
   void build(ddgrs_ctx* ctx, const std::vector<double>& pos, const std::vector<double>& mass, double tau, double chi,
              size_t leaf_size, size_t leaf_pairs);
+  void build(ddgrs_ctx* ctx, const VertexFractionalPlan& p);
   // y = L̃^V x (interleaved 3 nv FP64, device pointers).
   void apply(ddgrs_ctx* ctx, const double* x, double* y);
 };

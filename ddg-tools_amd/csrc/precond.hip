@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <future>
 #include <numeric>
 
 #include "bh.hpp"
@@ -595,10 +596,8 @@ LprimePlan make_lprime_plan(const Factor& f, const NdBlocks& nd) {
   return P;
 }
 
-void LprimeDev::upload(ddgrs_ctx* ctx, const NdBlocks& nd) {
-  auto t0 = std::chrono::steady_clock::now();
+void LprimeDev::upload(ddgrs_ctx* ctx, const Factor& f, const LprimePlan& P) {
   n = f.n;
-  LprimePlan P = make_lprime_plan(f, nd);
   level_off = P.level_off;
   max_rows = P.max_rows;
   ddgrs::upload(ctx, perm, f.perm);
@@ -626,8 +625,6 @@ void LprimeDev::upload(ddgrs_ctx* ctx, const NdBlocks& nd) {
   xf.reserve(3 * n);
   rbuf.reserve(3 * n);
   parts.reserve(3 * kLongParts);
-  ctx->sync();  // host vectors above are released on return
-  inverse_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
 void LprimeDev::solve(ddgrs_ctx* ctx, const double* r, double* x) {
@@ -700,73 +697,146 @@ __global__ __launch_bounds__(256) void pc_add_kernel(uint32_t n, const double* _
   if (t < n) z[t] += y[t];
 }
 
-void Preconditioner::build(ddgrs_surface* s, double s_exp, double w_tp, double eps_rel, double chi, size_t leaf_size,
-                           size_t leaf_pairs) {
-  auto t0 = std::chrono::steady_clock::now();
-  ddgrs_ctx* ctx = s->ctx;
-  std::vector<double> pos, mass;
-  vertex_areas_host(s, &pos, &mass);
-  nv = (size_t)s->nv;
-  n = 3 * nv + 3;
-  // K = W + ε M (cotan stiffness from the face list)
-  std::vector<std::tuple<uint32_t, uint32_t, double>> trip;
-  trip.reserve(12 * (size_t)s->nf);
-  for (int64_t f = 0; f < s->nf; ++f) {
-    const uint32_t* t = &s->faces_h[3 * f];
-    for (int c = 0; c < 3; ++c) {
-      uint32_t a = t[(c + 1) % 3], b = t[(c + 2) % 3];
-      const double* p0 = &pos[3 * (size_t)t[c]];
-      const double* p1 = &pos[3 * (size_t)a];
-      const double* p2 = &pos[3 * (size_t)b];
-      double u[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
-      double w[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
-      double cr[3] = {u[1] * w[2] - u[2] * w[1], u[2] * w[0] - u[0] * w[2], u[0] * w[1] - u[1] * w[0]};
-      double cot = (u[0] * w[0] + u[1] * w[1] + u[2] * w[2]) / std::sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
-      double h = 0.5 * cot;
-      trip.emplace_back(a, b, -h);
-      trip.emplace_back(b, a, -h);
-      trip.emplace_back(a, a, h);
-      trip.emplace_back(b, b, h);
-    }
-  }
-  double trw = 0.0, trm = 0.0;
+HostCsr cotan_stiffness(const std::vector<uint32_t>& faces, size_t nv, const std::vector<double>& pos,
+                        const std::vector<double>& mass, double eps_rel, double* eps) {
+  const size_t nf = faces.size() / 3;
+  // vertex → incident faces, ascending
+  std::vector<uint32_t> vptr(nv + 1, 0), vinc(3 * nf);
+  for (size_t f = 0; f < nf; ++f)
+    for (int c = 0; c < 3; ++c) vptr[faces[3 * f + c] + 1]++;
+  for (size_t v = 0; v < nv; ++v) vptr[v + 1] += vptr[v];
   {
-    HostCsr w = csr_from_triplets(nv, trip);
-    for (size_t i = 0; i < nv; ++i)
-      for (uint32_t q = w.ptr[i]; q < w.ptr[i + 1]; ++q)
-        if (w.idx[q] == i) trw += w.val[q];
+    std::vector<uint32_t> fill(vptr.begin(), vptr.end() - 1);
+    for (size_t f = 0; f < nf; ++f)
+      for (int c = 0; c < 3; ++c) vinc[fill[faces[3 * f + c]]++] = (uint32_t)f;
+  }
+  // row patterns: the vertex and its neighbours, ascending
+  HostCsr k;
+  k.n = nv;
+  k.ptr.assign(nv + 1, 0);
+  std::vector<std::vector<uint32_t>> pat(nv);
+  parallel(nv, [&](size_t a) {
+    auto& p = pat[a];
+    p.push_back((uint32_t)a);
+    for (uint32_t q = vptr[a]; q < vptr[a + 1]; ++q)
+      for (int c = 0; c < 3; ++c) p.push_back(faces[3 * (size_t)vinc[q] + c]);
+    std::sort(p.begin(), p.end());
+    p.erase(std::unique(p.begin(), p.end()), p.end());
+  });
+  for (size_t a = 0; a < nv; ++a) k.ptr[a + 1] = k.ptr[a] + (uint32_t)pat[a].size();
+  k.idx.resize(k.ptr[nv]);
+  k.val.assign(k.ptr[nv], 0.0);
+  // values: for each incident face (ascending) and corner c, the edge opposite c gets
+  // −½ cot(angle at c) off the diagonal and +½ cot on it, summed in the triplets' order
+  parallel(nv, [&](size_t a) {
+    const uint32_t r0 = k.ptr[a];
+    std::copy(pat[a].begin(), pat[a].end(), k.idx.begin() + r0);
+    auto at = [&](uint32_t col) -> double& {
+      const uint32_t* b = k.idx.data() + r0;
+      return k.val[r0 + (std::lower_bound(b, b + pat[a].size(), col) - b)];
+    };
+    for (uint32_t q = vptr[a]; q < vptr[a + 1]; ++q) {
+      const uint32_t* t = &faces[3 * (size_t)vinc[q]];
+      for (int c = 0; c < 3; ++c) {
+        const uint32_t x = t[(c + 1) % 3], y = t[(c + 2) % 3];
+        if (x != a && y != a) continue;
+        const double* p0 = &pos[3 * (size_t)t[c]];
+        const double* p1 = &pos[3 * (size_t)x];
+        const double* p2 = &pos[3 * (size_t)y];
+        double u[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+        double w[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+        double cr[3] = {u[1] * w[2] - u[2] * w[1], u[2] * w[0] - u[0] * w[2], u[0] * w[1] - u[1] * w[0]};
+        double cot = (u[0] * w[0] + u[1] * w[1] + u[2] * w[2]) / std::sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+        double h = 0.5 * cot;
+        at(x == a ? y : x) += -h;
+        at((uint32_t)a) += h;
+      }
+    }
+  });
+  double trw = 0.0, trm = 0.0;
+  std::vector<uint32_t> diag(nv);
+  for (size_t a = 0; a < nv; ++a) {
+    const uint32_t* b = k.idx.data() + k.ptr[a];
+    diag[a] = k.ptr[a] + (uint32_t)(std::lower_bound(b, b + (k.ptr[a + 1] - k.ptr[a]), (uint32_t)a) - b);
+    trw += k.val[diag[a]];
   }
   for (size_t v = 0; v < nv; ++v) trm += mass[v];
-  eps = eps_rel * trw / trm;
-  for (size_t v = 0; v < nv; ++v) trip.emplace_back((uint32_t)v, (uint32_t)v, eps * mass[v]);
-  auto ktrip = trip;
-  HostCsr k = csr_from_triplets(nv, ktrip);
-  crow.resize(nv);
+  *eps = eps_rel * trw / trm;
+  for (size_t v = 0; v < nv; ++v) k.val[diag[v]] += *eps * mass[v];
+  return k;
+}
+
+PrecondPlan plan_preconditioner(const std::vector<uint32_t>& faces, size_t nv, const std::vector<double>& pos,
+                                const std::vector<double>& mass, double s_exp, double w_tp, double eps_rel, double chi,
+                                size_t leaf_size, size_t leaf_pairs) {
+  PrecondPlan P;
+  P.nv = nv;
+  // L̃^V_{2−s} is independent of the factor: build it alongside
+  auto vf_future = std::async(std::launch::async, [&] {
+    return plan_vertex_fractional(pos, mass, 2.0 - s_exp, chi, leaf_size, leaf_pairs);
+  });
+  P.k = cotan_stiffness(faces, nv, pos, mass, eps_rel, &P.eps);
+  const HostCsr& k = P.k;
+  P.crow.resize(nv);
   double tot = 0.0;
   for (size_t v = 0; v < nv; ++v) tot += mass[v];
-  for (size_t v = 0; v < nv; ++v) crow[v] = mass[v] / tot;
+  for (size_t v = 0; v < nv; ++v) P.crow[v] = mass[v] / tot;
+  // L′ = [[K, c], [cᵀ, 0]] (both triangles), the multiplier last
+  HostCsr lpm;
+  lpm.n = nv + 1;
+  lpm.ptr.assign(nv + 2, 0);
+  lpm.idx.reserve(k.idx.size() + 2 * nv);
+  lpm.val.reserve(k.idx.size() + 2 * nv);
   for (size_t v = 0; v < nv; ++v) {
-    trip.emplace_back((uint32_t)nv, (uint32_t)v, crow[v]);
-    trip.emplace_back((uint32_t)v, (uint32_t)nv, crow[v]);
+    for (uint32_t q = k.ptr[v]; q < k.ptr[v + 1]; ++q) {
+      lpm.idx.push_back(k.idx[q]);
+      lpm.val.push_back(k.val[q]);
+    }
+    lpm.idx.push_back((uint32_t)nv);
+    lpm.val.push_back(P.crow[v]);
+    lpm.ptr[v + 1] = (uint32_t)lpm.idx.size();
   }
-  HostCsr lpm = csr_from_triplets(nv + 1, trip);
+  for (size_t v = 0; v < nv; ++v) {
+    lpm.idx.push_back((uint32_t)v);
+    lpm.val.push_back(P.crow[v]);
+  }
+  lpm.ptr[nv + 1] = (uint32_t)lpm.idx.size();
   NdBlocks nd = nested_dissection(k, pos);
   std::vector<uint32_t> order = nd.order;
   order.push_back((uint32_t)nv);
-  lp.f = factor_ldlt(lpm, order);
-  lp.nv = nv;
-  lp.upload(ctx, nd);
-  ddgrs::upload(ctx, kp, k.ptr);
-  ddgrs::upload(ctx, kj, k.idx);
-  ddgrs::upload(ctx, kx, k.val);
-  ddgrs::upload(ctx, crow_d, crow);
+  P.f = factor_ldlt(lpm, order);
+  P.lp = make_lprime_plan(P.f, nd);
   // L̃^V_{2−s} and its scale 1/(w_TP 4 I(σ) I(2−s)) (precond.rs:528-564, :893-906)
   auto I = [](double t) { return M_PI * std::tgamma(1.0 - t) / (t * std::pow(4.0, t) * std::tgamma(1.0 + t)); };
-  frac_scale = 1.0 / (4.0 * I(s_exp - 1.0) * I(2.0 - s_exp)) / w_tp;
-  vf.build(ctx, pos, mass, 2.0 - s_exp, chi, leaf_size, leaf_pairs);
+  P.frac_scale = 1.0 / (4.0 * I(s_exp - 1.0) * I(2.0 - s_exp)) / w_tp;
+  P.vf = vf_future.get();
+  return P;
+}
+
+void Preconditioner::build(ddgrs_surface* s, double s_exp, double w_tp, double eps_rel, double chi, size_t leaf_size,
+                           size_t leaf_pairs) {
+  std::vector<double> pos, mass;
+  vertex_areas_host(s, &pos, &mass);
+  build(s->ctx, plan_preconditioner(s->faces_h, (size_t)s->nv, pos, mass, s_exp, w_tp, eps_rel, chi, leaf_size,
+                                    leaf_pairs));
+}
+
+void Preconditioner::build(ddgrs_ctx* ctx, const PrecondPlan& P) {
+  nv = P.nv;
+  n = 3 * nv + 3;
+  eps = P.eps;
+  frac_scale = P.frac_scale;
+  crow = P.crow;
+  lp.nv = nv;
+  lp.upload(ctx, P.f, P.lp);
+  ddgrs::upload(ctx, kp, P.k.ptr);
+  ddgrs::upload(ctx, kj, P.k.idx);
+  ddgrs::upload(ctx, kx, P.k.val);
+  ddgrs::upload(ctx, crow_d, crow);
+  vf.build(ctx, P.vf);
   y.reserve(n);
   v.reserve(n);
-  factor_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  ctx->sync();  // the plan's host buffers may be released after return
 }
 
 void Preconditioner::apply(ddgrs_ctx* ctx, const double* r, double* z) {

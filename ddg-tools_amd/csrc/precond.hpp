@@ -63,7 +63,6 @@ LprimePlan make_lprime_plan(const Factor& f, const NdBlocks& nd);
 
 struct LprimeDev {
   size_t n = 0, nv = 0;
-  Factor f;
   std::vector<uint32_t> level_off;              // work items of each height: [level_off[h], level_off[h+1])
   std::vector<uint32_t> split_off;
   std::vector<uint8_t> fused;
@@ -76,10 +75,29 @@ struct LprimeDev {
   DevBuf<double> xb, xf, rbuf, parts;           // n × 3 work arrays (permuted numbering), partial sums
   double inverse_seconds = 0.0;
   // Block schedule and dense inverses of the blocks' diagonal parts of L.
-  void upload(ddgrs_ctx* ctx, const NdBlocks& nd);
+  void upload(ddgrs_ctx* ctx, const Factor& f, const LprimePlan& p);
   // x = L′⁻¹ r for a full saddle vector (3 nv + 3, interleaved positions then multipliers).
   void solve(ddgrs_ctx* ctx, const double* r, double* x);
 };
+
+// Host part of the preconditioner at given positions: K (cotan + εM), the barycenter row, L′'s
+// factor and block plan, L̃^V_{2−s}'s plan and the fractional scale.
+struct PrecondPlan {
+  size_t nv = 0;
+  double eps = 0.0, frac_scale = 1.0;
+  std::vector<double> crow;
+  HostCsr k;
+  Factor f;
+  LprimePlan lp;
+  VertexFractionalPlan vf;
+};
+PrecondPlan plan_preconditioner(const std::vector<uint32_t>& faces, size_t nv, const std::vector<double>& pos,
+                                const std::vector<double>& mass, double s_exp, double w_tp, double eps_rel, double chi,
+                                size_t leaf_size, size_t leaf_pairs);
+// K = W + ε M with W the cotan stiffness (RS/src/geometry.rs:221), assembled row by row in the
+// order of the reference's triplets (faces ascending, corners ascending); tr W into *trw.
+HostCsr cotan_stiffness(const std::vector<uint32_t>& faces, size_t nv, const std::vector<double>& pos,
+                        const std::vector<double>& mass, double eps_rel, double* eps);
 
 struct Preconditioner {
   size_t nv = 0, n = 0;
@@ -91,10 +109,10 @@ struct Preconditioner {
   DevBuf<double> y, v;      // work vectors (n)
   std::vector<double> crow; // barycenter row a_v / Σ a
   DevBuf<double> crow_d;
-  double factor_seconds = 0.0;
 
   // Builds K = W + ε M, L′ and L̃^V_{2−s} at the surface's current positions.
   void build(ddgrs_surface* s, double s_exp, double w_tp, double eps_rel, double chi, size_t leaf_size, size_t leaf_pairs);
+  void build(ddgrs_ctx* ctx, const PrecondPlan& p);
   // z = P⁻¹ r (device pointers, length 3 nv + 3).
   void apply(ddgrs_ctx* ctx, const double* r, double* z);
 };

@@ -9,6 +9,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <future>
 
 #include "bh.hpp"
 #include "capi_util.hpp"
@@ -162,16 +163,23 @@ void Session::step(StepReport* rep) {
   const int64_t nv = s->nv, nf = s->nf;
   const size_t nd = 3 * (size_t)nv, n = nd + 3;
   std::memset(rep, 0, sizeof(*rep));
-  // 1. trees, lists; F, dF
+  // 1. host snapshot; the preconditioner's and the metric's host plans are built on other threads
+  // while this thread builds the face tree and the Barnes–Hut lists and the device computes F, dF
   auto t = std::chrono::steady_clock::now();
   compute_face_geometry(s, s->packs, true);
+  const HostGeometry geo = download_geometry(s);
+  const double s_exp = (exps.beta64 - 2.0) / exps.alpha64;
+  auto pc_plan = std::async(std::launch::async, [&] {
+    return plan_preconditioner(s->faces_h, (size_t)nv, geo.pos, geo.vmass, s_exp, opts.w_tp, opts.eps_rel, opts.chi,
+                               opts.leaf_size, opts.leaf_pairs);
+  });
   Topology topo;
   Clusters clusters;
-  if (opts.energy_exact) {
-    build_face_tree(s, opts.leaf_size, &topo, &clusters);
-  } else {
-    bh.build(s, opts.theta, opts.leaf_size);
-  }
+  build_face_tree_host(geo, opts.leaf_size, &topo, &clusters);
+  auto fm_plan = std::async(std::launch::async, [&] {
+    return plan_face_metric(s->faces_h, (size_t)nv, topo, clusters, opts.chi, opts.leaf_pairs, geo.area);
+  });
+  if (!opts.energy_exact) bh.build_from(s, topo, clusters, opts.theta, opts.leaf_size);
   rep->seconds_trees = seconds_since(t);
   t = std::chrono::steady_clock::now();
   if (opts.energy_exact) {
@@ -196,14 +204,12 @@ void Session::step(StepReport* rep) {
   }
   // 2. the solver of A′ and the direction x = A′⁻¹ [g; 0]
   t = std::chrono::steady_clock::now();
-  const double s_exp = (exps.beta64 - 2.0) / exps.alpha64;
   fm.w_hi = opts.w_tp;
   fm.w_lo = opts.w_tp * opts.c_lo;
-  fm.build(s, opts.energy_exact ? topo : bh.topo, opts.energy_exact ? clusters : bh.clusters, opts.chi, opts.leaf_pairs,
-           s_exp);
+  fm.build(s, fm_plan.get(), s_exp);
   rep->seconds_metric = seconds_since(t);
   t = std::chrono::steady_clock::now();
-  pc.build(s, s_exp, opts.w_tp, opts.eps_rel, opts.chi, opts.leaf_size, opts.leaf_pairs);
+  pc.build(ctx, pc_plan.get());
   rep->seconds_precond = seconds_since(t);
   op.fm = &fm;
   op.crow = pc.crow_d.ptr;
