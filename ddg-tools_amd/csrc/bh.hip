@@ -552,14 +552,43 @@ HostGeometry download_geometry(ddgrs_surface* s) {
   return g;
 }
 
-void build_face_tree_host(const HostGeometry& g, size_t leaf_size, Topology* topo, Clusters* clusters) {
-  const size_t nf_ = g.area.size();
-  *topo = build_topology(nf_, g.bary.data(), g.lo.data(), g.hi.data(), leaf_size);
+// Face boxes: per-axis min and max of the corners (FP64, exact).
+__global__ __launch_bounds__(256) void face_boxes_kernel(int64_t nf, const uint32_t* __restrict__ faces,
+                                                         const double* __restrict__ pos, double* __restrict__ lo,
+                                                         double* __restrict__ hi) {
+  const int64_t f = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (f >= nf) return;
+  for (int k = 0; k < 3; ++k) {
+    double a = INFINITY, b = -INFINITY;
+    for (int c = 0; c < 3; ++c) {
+      const double x = pos[3 * (size_t)faces[3 * f + c] + k];
+      a = fmin(a, x);
+      b = fmax(b, x);
+    }
+    lo[3 * f + k] = a;
+    hi[3 * f + k] = b;
+  }
+}
+
+void build_face_tree(ddgrs_surface* s, const HostGeometry& g, size_t leaf_size, GpuBvhBuilder& b, Topology* topo,
+                     Clusters* clusters) {
+  ddgrs_ctx* ctx = s->ctx;
+  const size_t nf_ = (size_t)s->nf;
+  b.box_lo.reserve(3 * nf_);
+  b.box_hi.reserve(3 * nf_);
+  launch(ctx, "face_boxes", face_boxes_kernel, dim3(blocks_for(s->nf, 256)), dim3(256), 0, s->nf,
+         (const uint32_t*)s->faces.ptr, (const double*)s->positions.ptr, b.box_lo.ptr, b.box_hi.ptr);
+  *topo = b.build(ctx, nf_, s->bary.ptr, b.box_lo.ptr, b.box_hi.ptr, leaf_size);
   *clusters = build_clusters(*topo, g.bary.data(), g.lo.data(), g.hi.data(), g.area.data(), g.nrm.data());
 }
 
 void build_face_tree(ddgrs_surface* s, size_t leaf_size, Topology* topo, Clusters* clusters) {
-  build_face_tree_host(download_geometry(s), leaf_size, topo, clusters);
+  GpuBvhBuilder b;
+  build_face_tree(s, download_geometry(s), leaf_size, b, topo, clusters);
+}
+
+Topology build_vertex_tree(ddgrs_surface* s, size_t leaf_size, GpuBvhBuilder& b) {
+  return b.build(s->ctx, (size_t)s->nv, s->positions.ptr, s->positions.ptr, s->positions.ptr, leaf_size);
 }
 
 void vertex_areas_host(ddgrs_surface* s, std::vector<double>* pos, std::vector<double>* mass) {

@@ -3,6 +3,7 @@
 #include <algorithm>
 
 #include "kernels.hpp"
+#include "tree_gpu.hpp"
 #include "trees.hpp"
 
 namespace ddgrs {
@@ -52,10 +53,14 @@ struct HostGeometry {
   std::vector<double> vmass;                           // vertex areas a_v = ⅓ Σ_{S∋v} a_S
 };
 HostGeometry download_geometry(ddgrs_surface* s);
-// Face BVH (topology + build-time clusters with projectors) of a host snapshot.
-void build_face_tree_host(const HostGeometry& g, size_t leaf_size, Topology* topo, Clusters* clusters);
+// Face BVH (topology built on the device, build-time clusters with projectors on the host from the
+// snapshot g of the same positions).
+void build_face_tree(ddgrs_surface* s, const HostGeometry& g, size_t leaf_size, GpuBvhBuilder& b, Topology* topo,
+                     Clusters* clusters);
 // Face BVH at the surface's current positions.
 void build_face_tree(ddgrs_surface* s, size_t leaf_size, Topology* topo, Clusters* clusters);
+// Vertex BVH (points f_v) at the surface's current positions, built on the device.
+Topology build_vertex_tree(ddgrs_surface* s, size_t leaf_size, GpuBvhBuilder& b);
 // Vertex areas a_v = ⅓ Σ_{S∋v} a_S (RS/src/geometry.rs:194) and positions of the surface (host).
 void vertex_areas_host(ddgrs_surface* s, std::vector<double>* pos, std::vector<double>* mass);
 }  // namespace ddgrs

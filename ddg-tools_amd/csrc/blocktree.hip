@@ -749,11 +749,11 @@ __global__ __launch_bounds__(256) void vf_out_kernel(int n, const uint32_t* __re
 }
 
 VertexFractionalPlan plan_vertex_fractional(const std::vector<double>& pos, const std::vector<double>& m,
-                                            double tau, double chi, size_t leaf_size, size_t leaf_pairs) {
+                                            double tau, double chi, size_t leaf_pairs, Topology topo) {
   VertexFractionalPlan P;
   const size_t nv = m.size();
   // vertex BVH: point f_v, box the point, mass a_v (hierarchical.rs:1017-1060)
-  P.topo = build_topology(nv, pos.data(), pos.data(), pos.data(), leaf_size);
+  P.topo = std::move(topo);
   P.clusters = build_clusters(P.topo, pos.data(), pos.data(), pos.data(), m.data());
   P.bs = build_block_structure(P.topo, P.clusters, chi, leaf_pairs);
   std::vector<double> wbvh(nv);
@@ -763,11 +763,6 @@ VertexFractionalPlan plan_vertex_fractional(const std::vector<double>& pos, cons
   P.mass = m;
   P.tau = tau;
   return P;
-}
-
-void VertexFractionalDev::build(ddgrs_ctx* ctx, const std::vector<double>& pos, const std::vector<double>& m, double tau,
-                                double chi, size_t leaf_size, size_t leaf_pairs) {
-  build(ctx, plan_vertex_fractional(pos, m, tau, chi, leaf_size, leaf_pairs));
 }
 
 void VertexFractionalDev::build(ddgrs_ctx* ctx, const VertexFractionalPlan& P) {

@@ -103,8 +103,9 @@ struct VertexFractionalPlan {
   std::vector<double> pos, mass;
   double tau = 0.0;
 };
+// `topo`: the vertex BVH (points f_v, leaf size as given) of the same positions.
 VertexFractionalPlan plan_vertex_fractional(const std::vector<double>& pos, const std::vector<double>& mass,
-                                            double tau, double chi, size_t leaf_size, size_t leaf_pairs);
+                                            double tau, double chi, size_t leaf_pairs, Topology topo);
 
 // L̃^V_τ over vertex pairs (hierarchical.rs:1003).
 struct VertexFractionalDev {
@@ -115,8 +116,6 @@ struct VertexFractionalDev {
   int nv = 0;
   DevBuf<double> mass;      // per vertex (FP64)
   DevBuf<float> u_hl, z_hl; // product scratch: 3 channels, hi/lo
-  void build(ddgrs_ctx* ctx, const std::vector<double>& pos, const std::vector<double>& mass, double tau, double chi,
-             size_t leaf_size, size_t leaf_pairs);
   void build(ddgrs_ctx* ctx, const VertexFractionalPlan& p);
   // y = L̃^V x (interleaved 3 nv FP64, device pointers).
   void apply(ddgrs_ctx* ctx, const double* x, double* y);

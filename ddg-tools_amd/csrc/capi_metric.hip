@@ -101,7 +101,9 @@ int ddgrs_vfrac_build(ddgrs_surface* s, double tau, double chi, int64_t leaf_siz
     h->s = s;
     std::vector<double> pos, mass;
     vertex_areas_host(s, &pos, &mass);
-    h->vf.build(s->ctx, pos, mass, tau, chi, (size_t)leaf_size, (size_t)leaf_pairs);
+    GpuBvhBuilder b;
+    Topology vt = build_vertex_tree(s, (size_t)leaf_size, b);
+    h->vf.build(s->ctx, plan_vertex_fractional(pos, mass, tau, chi, (size_t)leaf_pairs, std::move(vt)));
     surface_retain(s);
     *out = h.release();
   });

@@ -768,12 +768,12 @@ HostCsr cotan_stiffness(const std::vector<uint32_t>& faces, size_t nv, const std
 
 PrecondPlan plan_preconditioner(const std::vector<uint32_t>& faces, size_t nv, const std::vector<double>& pos,
                                 const std::vector<double>& mass, double s_exp, double w_tp, double eps_rel, double chi,
-                                size_t leaf_size, size_t leaf_pairs) {
+                                size_t leaf_pairs, Topology vertex_tree) {
   PrecondPlan P;
   P.nv = nv;
   // L̃^V_{2−s} is independent of the factor: build it alongside
   auto vf_future = std::async(std::launch::async, [&] {
-    return plan_vertex_fractional(pos, mass, 2.0 - s_exp, chi, leaf_size, leaf_pairs);
+    return plan_vertex_fractional(pos, mass, 2.0 - s_exp, chi, leaf_pairs, std::move(vertex_tree));
   });
   P.k = cotan_stiffness(faces, nv, pos, mass, eps_rel, &P.eps);
   const HostCsr& k = P.k;
@@ -817,8 +817,10 @@ void Preconditioner::build(ddgrs_surface* s, double s_exp, double w_tp, double e
                            size_t leaf_pairs) {
   std::vector<double> pos, mass;
   vertex_areas_host(s, &pos, &mass);
-  build(s->ctx, plan_preconditioner(s->faces_h, (size_t)s->nv, pos, mass, s_exp, w_tp, eps_rel, chi, leaf_size,
-                                    leaf_pairs));
+  GpuBvhBuilder b;
+  Topology vt = build_vertex_tree(s, leaf_size, b);
+  build(s->ctx, plan_preconditioner(s->faces_h, (size_t)s->nv, pos, mass, s_exp, w_tp, eps_rel, chi, leaf_pairs,
+                                    std::move(vt)));
 }
 
 void Preconditioner::build(ddgrs_ctx* ctx, const PrecondPlan& P) {

@@ -91,9 +91,10 @@ struct PrecondPlan {
   LprimePlan lp;
   VertexFractionalPlan vf;
 };
+// `vertex_tree`: the vertex BVH of the same positions (built on the device by the caller).
 PrecondPlan plan_preconditioner(const std::vector<uint32_t>& faces, size_t nv, const std::vector<double>& pos,
                                 const std::vector<double>& mass, double s_exp, double w_tp, double eps_rel, double chi,
-                                size_t leaf_size, size_t leaf_pairs);
+                                size_t leaf_pairs, Topology vertex_tree);
 // K = W + ε M with W the cotan stiffness (RS/src/geometry.rs:221), assembled row by row in the
 // order of the reference's triplets (faces ascending, corners ascending); tr W into *trw.
 HostCsr cotan_stiffness(const std::vector<uint32_t>& faces, size_t nv, const std::vector<double>& pos,

@@ -169,13 +169,14 @@ void Session::step(StepReport* rep) {
   compute_face_geometry(s, s->packs, true);
   const HostGeometry geo = download_geometry(s);
   const double s_exp = (exps.beta64 - 2.0) / exps.alpha64;
+  Topology vtopo = build_vertex_tree(s, opts.leaf_size, vert_bvh);
   auto pc_plan = std::async(std::launch::async, [&] {
     return plan_preconditioner(s->faces_h, (size_t)nv, geo.pos, geo.vmass, s_exp, opts.w_tp, opts.eps_rel, opts.chi,
-                               opts.leaf_size, opts.leaf_pairs);
+                               opts.leaf_pairs, std::move(vtopo));
   });
   Topology topo;
   Clusters clusters;
-  build_face_tree_host(geo, opts.leaf_size, &topo, &clusters);
+  build_face_tree(s, geo, opts.leaf_size, face_bvh, &topo, &clusters);
   auto fm_plan = std::async(std::launch::async, [&] {
     return plan_face_metric(s->faces_h, (size_t)nv, topo, clusters, opts.chi, opts.leaf_pairs, geo.area);
   });

@@ -34,6 +34,7 @@ struct Session {
   double l0 = 0.0, last_tau = 0.0;
   double target[3] = {0.0, 0.0, 0.0};
   BhEval bh;
+  GpuBvhBuilder face_bvh, vert_bvh;
   FaceMetricDev fm;
   Preconditioner pc;
   SaddleOp op;

@@ -1,0 +1,31 @@
+// BVH construction on the device with the reference's split rule (RC/src/spatial/bvh.rs:126
+// build_with): at depth d the elements of a node are ordered along axis d mod 3 by (total order key
+// of the coordinate, element index); the split k minimises diam²(box of the first k) +
+// diam²(box of the rest) over k in [⌈n/4⌉, n − ⌈n/4⌉], the first minimum winning; nodes of at most
+// leaf_size elements are leaves.  The result is bitwise the host build's (box unions are exact,
+// the diameters are evaluated with the same operations, no contraction).
+//
+// Device form, level by level: each axis is sorted once by (key, index) with a radix sort; a level
+// orders every splitting node's elements along its axis by a stable radix sort of that global
+// order on the node index (a stable partition), then one workgroup per node scans the suffix boxes,
+// the prefix boxes and the split costs and reduces to the first minimum.  The host keeps the node
+// list (a few kilobytes per level) and numbers the nodes in pre-order at the end.
+#pragma once
+#include "runtime.hpp"
+#include "trees.hpp"
+
+namespace ddgrs {
+
+struct GpuBvhBuilder {
+  size_t cap = 0;
+  DevBuf<uint64_t> key_in, key_out;
+  DevBuf<uint32_t> iota, axis_order, order, seg_of, part_key_in, part_key_out, part_val_in, part_val_out;
+  DevBuf<uint32_t> act_start, act_end, act_prefix, act_split;
+  DevBuf<double> suf;  // 6 per element: suffix box lo, hi
+  DevBuf<double> box_lo, box_hi;  // element boxes when the caller has none on the device
+  DevBuf<uint8_t> temp;
+  // Builds over n elements with centres pts and boxes [lo, hi] (3 FP64 each, device pointers).
+  Topology build(ddgrs_ctx* ctx, size_t n, const double* pts, const double* lo, const double* hi, size_t leaf_size);
+};
+
+}  // namespace ddgrs

@@ -20,6 +20,14 @@ struct Topology {
 
 Topology build_topology(size_t n, const double* pts, const double* lo, const double* hi, size_t leaf_size);
 
+// A node of a BVH under construction (creation order: children are appended after their parent).
+struct TreeSegment {
+  uint32_t start, end, depth;
+  int64_t left = -1, right = -1;
+};
+// Pre-order ids (left child = i + 1), parents, depths and leaves from the segments.
+Topology finish_topology(const std::vector<TreeSegment>& segs, std::vector<uint32_t> order);
+
 // Build-time cluster data (RS/src/spatial/mod.rs:80 Cluster), summed bottom-up in the reference's
 // order: area, centroid, box of the element boxes, radius to the farthest box corner.
 struct Clusters {

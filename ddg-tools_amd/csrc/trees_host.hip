@@ -61,10 +61,7 @@ inline double box_distance_sq(const double* alo, const double* ahi, const double
 }  // namespace
 
 Topology build_topology(size_t n, const double* pts, const double* lo, const double* hi, size_t leaf_size) {
-  struct Seg {
-    uint32_t start, end, depth;
-    int64_t left = -1, right = -1;
-  };
+  using Seg = TreeSegment;
   std::vector<uint32_t> order(n);
   for (size_t i = 0; i < n; ++i) order[i] = (uint32_t)i;
   std::vector<Seg> segs{{0, (uint32_t)n, 0}};
@@ -118,6 +115,10 @@ Topology build_topology(size_t n, const double* pts, const double* lo, const dou
     }
     level = std::move(next);
   }
+  return finish_topology(segs, std::move(order));
+}
+
+Topology finish_topology(const std::vector<TreeSegment>& segs, std::vector<uint32_t> order) {
   // Subtree sizes (children were appended after their parents), then pre-order ids.
   std::vector<uint32_t> count(segs.size(), 1), id(segs.size(), 0);
   for (size_t i = segs.size(); i-- > 0;)
