@@ -15,7 +15,13 @@ Parity pins (see the oracle tests, ``tests/test_oracle_*.py``):
   quarter-balance window, the cost minimality of every split by brute force), cluster data equal
   to direct sums, the lists partitioning every ordered pair exactly once;
 * Barnes-Hut energy / gradient: theta = 0 equals the exact evaluation, central differences
-  with frozen lists, the O(theta^2) error and the authors' Barnes-Hut value within 1e-2.
+  with frozen lists, the O(theta^2) error and the authors' Barnes-Hut value within 1e-2;
+* metric B, B0 (chi = 0): the Lean ``rs-*-metric`` fixtures; the null space; symmetry;
+* preconditioner and saddle solve: the Lean ``rs-icosa-precond`` and ``rs-octa-saddle-none``
+  fixtures, an independent numpy assembly of K and L', GMRES against a dense solve;
+* the descent step (``Session.step``): parity unpinned beyond properties (Armijo condition,
+  energy decrease, barycenter kept) -- the reference's step fixtures all carry area, volume or
+  pin constraints that this path does not build.
 """
 from __future__ import annotations
 

@@ -20,6 +20,13 @@
 // ACM TOMS 31(4), 2005).  The reference factors with faer 0.24.4 (AMD order, not vendored); the
 // factor is the same matrix factorization up to round-off, which the Lean `precond` fixtures pin
 // at 1e-10 (tests/oracle.rs:45-48).
+//
+// Pins: the Lean `rs-icosa-precond` fixture (ε, L′ solves, P⁻¹ r), the Lean `rs-octa-saddle-none`
+// fixture (the direction x = A′⁻¹ [g; 0] and ⟨g, x⟩ with the dense metric), an independent numpy
+// assembly of K and L′, GMRES against a dense solve.  session_step: parity unpinned beyond the
+// properties its test checks (the accepted step satisfies the Armijo condition, lowers the
+// energy and keeps the barycenter): the reference's step fixtures all carry area, volume or pin
+// constraints, which this path does not build.
 #include <algorithm>
 #include <numeric>
 
