@@ -15,7 +15,7 @@
 // Kernels of one product: upward node sums (one thread per leaf, climbing with a per-node arrival
 // counter: the second child to arrive sums left + right, so the values do not depend on timing),
 // far field (one wave per node), leaf far terms (one thread per leaf and channel), panels (one wave
-// per panel: lanes are columns, registers hold the 8 rows × C channels).
+// per panel: lanes are columns, registers hold the panel's rows × C channels).
 #include <algorithm>
 
 #include "blocktree.hpp"
@@ -183,7 +183,7 @@ struct PanelArgs {
 
 // Near field of one panel plus its far terms; z = w_k (near + ᾱ_L (u_S − ū_L) + δ_L).
 template <int C, int H, int NK>
-__global__ __launch_bounds__(256) void bt_panel_kernel(PanelArgs P, const float* __restrict__ u_hl, double w0, double w1,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void bt_panel_kernel(PanelArgs P, const float* __restrict__ u_hl, double w0, double w1,
                                                        float* __restrict__ z_hl) {
   const uint32_t p = wave_id();
   const int lane = threadIdx.x & 63;
@@ -191,9 +191,9 @@ __global__ __launch_bounds__(256) void bt_panel_kernel(PanelArgs P, const float*
   const uint32_t k = P.panel_leaf[p], row0 = P.panel_row0[p], R = P.panel_rows[p];
   const uint32_t c0 = P.col_ptr[k], K = P.col_ptr[k + 1] - c0;
   const float* vbase = P.val + P.panel_val[p] * NK;
-  float acc[8][C];
+  float acc[kPanelRows][C];
 #pragma unroll
-  for (int r = 0; r < 8; ++r)
+  for (int r = 0; r < (int)kPanelRows; ++r)
 #pragma unroll
     for (int c = 0; c < C; ++c) acc[r][c] = 0.f;
   // The column index of the next chunk is loaded one chunk ahead; all rows' values of a chunk are
@@ -207,9 +207,9 @@ __global__ __launch_bounds__(256) void bt_panel_kernel(PanelArgs P, const float*
     tnext = P.cols[c0 + (kk + 64 < K ? kk + 64 : 0)];
     float th[C], tl[C];
     load_hl<C>(u_hl + (size_t)tcol * 2 * C, th, tl);
-    float v0[8], v1[8];
+    float v0[kPanelRows], v1[kPanelRows];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
+    for (int r = 0; r < (int)kPanelRows; ++r) {
       const bool ok = act && (uint32_t)r < R;
       const float* v = vbase + (ok ? ((size_t)r * K + kk) * NK : 0);
       v0[r] = v[0];
@@ -217,7 +217,7 @@ __global__ __launch_bounds__(256) void bt_panel_kernel(PanelArgs P, const float*
       if (!ok) v0[r] = v1[r] = 0.f;
     }
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
+    for (int r = 0; r < (int)kPanelRows; ++r) {
       if ((uint32_t)r < R) {
         const float* srow = u_hl + (size_t)(row0 + r) * 2 * C;
 #pragma unroll
@@ -231,7 +231,7 @@ __global__ __launch_bounds__(256) void bt_panel_kernel(PanelArgs P, const float*
   // pairwise reduction over the wave; output (r, c) goes to lane (r C + c) mod 64
   float mine0 = 0.f, mine1 = 0.f;
 #pragma unroll
-  for (int r = 0; r < 8; ++r) {
+  for (int r = 0; r < (int)kPanelRows; ++r) {
     if ((uint32_t)r < R) {
 #pragma unroll
       for (int c = 0; c < C; ++c) {
@@ -299,7 +299,7 @@ __global__ __launch_bounds__(256) void bt_alpha_leaf_kernel(int nl, int nk, cons
 
 BlockTreePlan plan_block_tree(const Topology& t, const BlockStructure& s, int nk, std::vector<double> weight_bvh,
                               const std::vector<double>& node_mass) {
-  constexpr uint32_t kRows = 8;
+  constexpr uint32_t kRows = kPanelRows;
   BlockTreePlan P;
   P.nk = nk;
   P.n = (int)t.order.size();

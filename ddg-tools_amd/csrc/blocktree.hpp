@@ -7,6 +7,9 @@
 
 namespace ddgrs {
 
+// Rows of a near-field panel (a leaf's rows are split into panels of at most this many).
+constexpr uint32_t kPanelRows = 4;
+
 // Host part of a BlockTreeDev: the structure, panels and weights (built without the device).
 struct BlockTreePlan {
   int nk = 1;
@@ -27,7 +30,7 @@ BlockTreePlan plan_block_tree(const Topology& t, const BlockStructure& s, int nk
 // H̃a·u − H̃(a u) (hierarchical.rs:696 product) term by term, but no large term cancels: every
 // FP32 rounding is relative to a difference of u, not to u.
 //
-// Near field: one panel per leaf (split into panels of at most 8 rows): the leaf's rows against
+// Near field: one panel per leaf (split into panels of at most kPanelRows rows): the leaf's rows against
 // every column that a near block of one of its ancestors covers, stored dense [row][column] with NK
 // FP32 values per entry.  Far field: the admissible blocks per row node.  The far part is pushed to
 // the leaves exactly: for S in leaf L,

@@ -13,8 +13,11 @@
 // (forward) and ancestors (backward) outside the block, so the blocks of one height are
 // independent.  FP64 throughout: L′ has a near-null space of relative size ε = 1e-10.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <future>
 #include <numeric>
 
@@ -200,6 +203,195 @@ Factor factor_ldlt(const HostCsr& a, const std::vector<uint32_t>& order) {
       f.rx[fill[i]++] = f.cx[q];
     }
   return f;
+}
+
+// ---- multifrontal factorization over the separator tree --------------------------------------
+
+Factor factor_multifrontal(const HostCsr& a, const NdBlocks& nd) {
+  const size_t nv = nd.order.size(), n = nv + 1;
+  Factor F;
+  F.n = n;
+  F.perm = nd.order;
+  F.perm.push_back((uint32_t)nv);
+  std::vector<uint32_t> pinv(n);
+  for (size_t i = 0; i < n; ++i) pinv[F.perm[i]] = (uint32_t)i;
+  // blocks: the separator tree's, then the multiplier row as the root
+  std::vector<uint32_t> bstart = nd.start, bend = nd.end, bparent = nd.parent;
+  const uint32_t nb0 = (uint32_t)bstart.size(), nb = nb0 + 1;
+  bstart.push_back((uint32_t)n - 1);
+  bend.push_back((uint32_t)n);
+  bparent.push_back(kNone);
+  for (uint32_t b = 0; b < nb0; ++b)
+    if (bparent[b] == kNone) bparent[b] = nb0;
+  std::vector<std::vector<uint32_t>> children(nb);
+  for (uint32_t b = 0; b < nb0; ++b) children[bparent[b]].push_back(b);
+  std::vector<uint32_t> height(nb, 0);
+  for (uint32_t b = nb0; b-- > 0;) height[bparent[b]] = std::max(height[bparent[b]], height[b] + 1);
+  const uint32_t H = height[nb0] + 1;
+  // symbolic: front rows = the block's rows, then struct(J) = rows ≥ end(J) of A's columns in J and
+  // of the children's structs (children before parents: decreasing ids, the root last)
+  std::vector<std::vector<uint32_t>> front(nb);
+  auto symbolic = [&](uint32_t b) {
+    const uint32_t j0 = bstart[b], j1 = bend[b];
+    std::vector<uint32_t> st;
+    for (uint32_t j = j0; j < j1; ++j) {
+      const uint32_t o = F.perm[j];
+      for (uint32_t q = a.ptr[o]; q < a.ptr[o + 1]; ++q) {
+        const uint32_t i = pinv[a.idx[q]];
+        if (i >= j1) st.push_back(i);
+      }
+    }
+    for (uint32_t c : children[b])
+      for (size_t r = bend[c] - bstart[c]; r < front[c].size(); ++r) {
+        if (front[c][r] < j0) throw Error(DDGRS_GPU, "preconditioner", "separator tree: a front row precedes its parent");
+        if (front[c][r] >= j1) st.push_back(front[c][r]);
+      }
+    std::sort(st.begin(), st.end());
+    st.erase(std::unique(st.begin(), st.end()), st.end());
+    auto& fr = front[b];
+    fr.resize(j1 - j0 + st.size());
+    for (uint32_t j = j0; j < j1; ++j) fr[j - j0] = j;
+    std::copy(st.begin(), st.end(), fr.begin() + (j1 - j0));
+  };
+  for (uint32_t b = nb0; b-- > 0;) symbolic(b);
+  symbolic(nb0);
+  // numeric, by height; update matrices live until the parent has assembled them
+  std::vector<std::vector<double>> upd(nb);
+  std::vector<std::vector<std::pair<uint32_t, double>>> lcol(n);  // column entries (row, value), rows ascending
+  F.d.assign(n, 0.0);
+  std::vector<std::vector<uint32_t>> by_height(H);
+  for (uint32_t b = 0; b < nb; ++b) by_height[height[b]].push_back(b);
+  std::atomic<int64_t> bad{-1};
+  auto eliminate = [&](uint32_t b, bool inner_parallel) {
+    const uint32_t j0 = bstart[b], k = bend[b] - bstart[b];
+    const auto& fr = front[b];
+    const size_t f = fr.size();
+    auto local = [&](uint32_t i) -> size_t {
+      if (i < bend[b]) return i - j0;
+      return k + (size_t)(std::lower_bound(fr.begin() + k, fr.end(), i) - (fr.begin() + k));
+    };
+    std::vector<double> M(f * f, 0.0);  // row-major, lower triangle used
+    // assemble A's columns of the block (entries on or below the diagonal)
+    for (uint32_t c = 0; c < k; ++c) {
+      const uint32_t j = j0 + c, o = F.perm[j];
+      for (uint32_t q = a.ptr[o]; q < a.ptr[o + 1]; ++q) {
+        const uint32_t i = pinv[a.idx[q]];
+        if (i >= j) M[local(i) * f + c] += a.val[q];
+      }
+    }
+    // extend-add the children's update matrices
+    for (uint32_t cb : children[b]) {
+      const auto& cf = front[cb];
+      const size_t ck = bend[cb] - bstart[cb], cs = cf.size() - ck;
+      std::vector<size_t> map(cs);
+      for (size_t r = 0; r < cs; ++r) map[r] = local(cf[ck + r]);
+      const auto& U = upd[cb];
+      for (size_t r = 0; r < cs; ++r)
+        for (size_t c = 0; c <= r; ++c) M[map[r] * f + map[c]] += U[r * cs + c];
+      std::vector<double>().swap(upd[cb]);
+    }
+    // blocked right-looking LDLᵀ of the first k columns (rank-NB trailing updates)
+    constexpr size_t NB = 32;
+    std::vector<double> W(f * NB);  // unscaled panel columns l·d
+    for (size_t c0 = 0; c0 < k; c0 += NB) {
+      const size_t c1 = std::min<size_t>(c0 + NB, k), nbk = c1 - c0;
+      for (size_t c = c0; c < c1; ++c) {
+        const double dc = M[c * f + c];
+        if (dc == 0.0 || !std::isfinite(dc)) {
+          bad = (int64_t)(j0 + c);
+          return;
+        }
+        for (size_t i = c + 1; i < f; ++i) {
+          const double w = M[i * f + c];
+          W[i * NB + (c - c0)] = w;
+          M[i * f + c] = w / dc;
+        }
+        // update the rest of the panel's columns
+        for (size_t i = c + 1; i < f; ++i) {
+          const double li = M[i * f + c];
+          const size_t mend = std::min(i + 1, c1);
+          for (size_t m = c + 1; m < mend; ++m) M[i * f + m] -= li * W[m * NB + (c - c0)];
+        }
+      }
+      // trailing update of columns ≥ c1: M[i][m] −= Σ_c L[i][c] W[m][c]
+      auto rows = [&](size_t i) {
+        double* mi = &M[i * f];
+        const double* li = &M[i * f + c0];
+        for (size_t m = c1; m <= i; ++m) {
+          const double* wm = &W[m * NB];
+          double acc = 0.0;
+          for (size_t t = 0; t < nbk; ++t) acc += li[t] * wm[t];
+          mi[m] -= acc;
+        }
+      };
+      if (inner_parallel && f - c1 >= 256)
+        parallel(f - c1, [&](size_t r) { rows(c1 + r); });
+      else
+        for (size_t i = c1; i < f; ++i) rows(i);
+    }
+    for (uint32_t c = 0; c < k; ++c) {
+      F.d[j0 + c] = M[c * f + c];
+      auto& col = lcol[j0 + c];
+      for (size_t i = c + 1; i < f; ++i) {
+        const double v = M[i * f + c];
+        if (v != 0.0) col.push_back({fr[i], v});
+      }
+    }
+    const size_t s2 = f - k;
+    std::vector<double> U(s2 * s2, 0.0);
+    for (size_t r = 0; r < s2; ++r)
+      for (size_t c = 0; c <= r; ++c) U[r * s2 + c] = M[(k + r) * f + (k + c)];
+    upd[b] = std::move(U);
+  };
+  static const bool trace = std::getenv("DDGRS_TRACE_FACTOR") != nullptr;
+  auto t_num = std::chrono::steady_clock::now();
+  for (uint32_t h = 0; h < H; ++h) {
+    auto t_h = std::chrono::steady_clock::now();
+    const auto& bl = by_height[h];
+    if (bl.size() >= 8)
+      parallel(bl.size(), [&](size_t i) { eliminate(bl[i], false); });
+    else
+      for (uint32_t b : bl) eliminate(b, true);
+    if (trace) {
+      size_t fmax = 0;
+      for (uint32_t b : bl) fmax = std::max(fmax, front[b].size());
+      std::fprintf(stderr, "factor height %u: %zu blocks, largest front %zu, %.2f ms\n", h, bl.size(), fmax,
+                   std::chrono::duration<double>(std::chrono::steady_clock::now() - t_h).count() * 1e3);
+    }
+    if (bad >= 0) {
+      Error e(DDGRS_SINGULAR_SADDLE, "preconditioner factorization", "zero or non-finite pivot " + std::to_string(bad.load()));
+      e.a = bad;
+      throw e;
+    }
+  }
+  if (trace)
+    std::fprintf(stderr, "factor numeric %.2f ms\n",
+                 std::chrono::duration<double>(std::chrono::steady_clock::now() - t_num).count() * 1e3);
+  // column and row storage of L
+  F.cp.assign(n + 1, 0);
+  for (size_t j = 0; j < n; ++j) F.cp[j + 1] = F.cp[j] + (uint32_t)lcol[j].size();
+  F.ci.resize(F.cp[n]);
+  F.cx.resize(F.cp[n]);
+  parallel(n, [&](size_t j) {
+    uint32_t q = F.cp[j];
+    for (auto& e : lcol[j]) {
+      F.ci[q] = e.first;
+      F.cx[q++] = e.second;
+    }
+  });
+  F.rp.assign(n + 1, 0);
+  for (uint32_t i : F.ci) F.rp[i + 1]++;
+  for (size_t i = 0; i < n; ++i) F.rp[i + 1] += F.rp[i];
+  F.rj.resize(F.ci.size());
+  F.rx.resize(F.ci.size());
+  std::vector<uint32_t> fill(F.rp.begin(), F.rp.end() - 1);
+  for (size_t j = 0; j < n; ++j)
+    for (uint32_t q = F.cp[j]; q < F.cp[j + 1]; ++q) {
+      const uint32_t i = F.ci[q];
+      F.rj[fill[i]] = (uint32_t)j;
+      F.rx[fill[i]++] = F.cx[q];
+    }
+  return F;
 }
 
 // ---- device solves ---------------------------------------------------------------------------
@@ -802,9 +994,7 @@ PrecondPlan plan_preconditioner(const std::vector<uint32_t>& faces, size_t nv, c
   }
   lpm.ptr[nv + 1] = (uint32_t)lpm.idx.size();
   NdBlocks nd = nested_dissection(k, pos);
-  std::vector<uint32_t> order = nd.order;
-  order.push_back((uint32_t)nv);
-  P.f = factor_ldlt(lpm, order);
+  P.f = factor_multifrontal(lpm, nd);
   P.lp = make_lprime_plan(P.f, nd);
   // L̃^V_{2−s} and its scale 1/(w_TP 4 I(σ) I(2−s)) (precond.rs:528-564, :893-906)
   auto I = [](double t) { return M_PI * std::tgamma(1.0 - t) / (t * std::pow(4.0, t) * std::tgamma(1.0 + t)); };

@@ -26,6 +26,13 @@ struct Factor {
 // zero or non-finite pivot.
 Factor factor_ldlt(const HostCsr& a, const std::vector<uint32_t>& order);
 
+struct NdBlocks;
+// The same factor by the multifrontal method over the separator tree of `nd` (order nd.order, the
+// multiplier row last): one dense front per block (the block's rows and the later rows coupled to
+// them), children's update matrices added in, blocked right-looking LDLᵀ of the block's columns;
+// the blocks of one height are independent.  Exact zeros are not stored.
+Factor factor_multifrontal(const HostCsr& a, const NdBlocks& nd);
+
 // A nested-dissection order and its separator tree: blocks are contiguous ranges of the order
 // (leaf domains and separators), a separator after the two domains it splits; parent[b] is the
 // separator that splits the domain b came from (UINT32_MAX for the root).
