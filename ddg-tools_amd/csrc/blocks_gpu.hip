@@ -1,0 +1,474 @@
+// Block cluster tree structure and near-field panels built on the device.
+//
+// Follows RS/src/spatial/block_tree.rs:129 build_impl: starting from (root, root), every pair of
+// the current level is admissible (a ≠ b and the boxes χ-separated: d² > 0 and
+// max(r_a, r_b)² ≤ χ² d², RS/src/metric/hierarchical.rs:116), near (both leaves, or
+// |a|·|b| ≤ leaf_pairs), or split into the pairs of children (the row node's children outer, left
+// child first).  Admissible and near pairs are kept in creation order and then grouped by row node
+// stably, which is the order of build_block_structure.  The separation test uses the same FP64
+// operations (no contraction), so the lists are bitwise the host build's.
+//
+// Per level: a classification kernel, a scan of the (admissible, near, children) counts and a
+// scatter kernel; the host reads the three totals.  Grouping: a stable radix sort by row node and a
+// binary search for the row pointers.  Panels: per leaf the column ranges of the near blocks of the
+// leaf and its ancestors, sorted by first column (the ranges of one leaf are disjoint: the blocks of
+// a chain partition the columns), then the column lists and the panel table.
+#include <cstring>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include <algorithm>
+
+#include "blocktree.hpp"
+
+namespace ddgrs {
+
+#pragma clang fp contract(off)
+
+BlockStats block_stats(const BlockStructure& s) {
+  BlockStats b;
+  b.far_blocks = s.far_col.size();
+  b.near_blocks = s.near_col.size();
+  b.admissible_pairs = s.admissible_pairs;
+  b.near_entries = s.near_entries;
+  b.near_total = s.near_total;
+  b.depth = s.depth;
+  return b;
+}
+
+namespace {
+
+struct AddCounts {
+  __host__ __device__ BlockCounts operator()(const BlockCounts& x, const BlockCounts& y) const {
+    return {x.adm + y.adm, x.nearb + y.nearb, x.kids + y.kids};
+  }
+};
+
+struct BlockGeo {
+  const uint32_t *start, *end, *right;
+  const double *lo, *hi, *radius;
+  double chi;
+  uint64_t leaf_pairs;
+};
+
+__device__ __forceinline__ bool separated(const BlockGeo& g, uint32_t a, uint32_t b) {
+  if (!(g.chi > 0.0)) return false;
+  double d2 = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    const double d =
+        fmax(fmax(g.lo[3 * a + k], g.lo[3 * b + k]) - fmin(g.hi[3 * a + k], g.hi[3 * b + k]), 0.0);
+    d2 += d * d;
+  }
+  const double h = fmax(g.radius[a], g.radius[b]);
+  return d2 > 0.0 && h * h <= g.chi * g.chi * d2;
+}
+
+}  // namespace
+
+// cls: 0 admissible, 1 near, 2 split; cnt: what the pair adds to each list.
+__global__ __launch_bounds__(256) void blk_classify_kernel(uint32_t m, const uint32_t* __restrict__ pa,
+                                                           const uint32_t* __restrict__ pb, BlockGeo g,
+                                                           uint8_t* __restrict__ cls, BlockCounts* __restrict__ cnt) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= m) return;
+  const uint32_t a = pa[i], b = pb[i];
+  const bool la = g.right[a] == 0, lb = g.right[b] == 0;
+  const uint64_t na = g.end[a] - g.start[a], nb = g.end[b] - g.start[b];
+  uint8_t c;
+  if (a != b && separated(g, a, b))
+    c = 0;
+  else if ((la && lb) || na * nb <= g.leaf_pairs)
+    c = 1;
+  else
+    c = 2;
+  cls[i] = c;
+  cnt[i] = {c == 0 ? 1u : 0u, c == 1 ? 1u : 0u, c == 2 ? (la ? 1u : 2u) * (lb ? 1u : 2u) : 0u};
+}
+
+__global__ __launch_bounds__(256) void blk_scatter_kernel(uint32_t m, const uint32_t* __restrict__ pa,
+                                                          const uint32_t* __restrict__ pb,
+                                                          const uint32_t* __restrict__ right,
+                                                          const uint8_t* __restrict__ cls,
+                                                          const BlockCounts* __restrict__ off, uint32_t adm0,
+                                                          uint32_t near0, uint32_t* __restrict__ adm_a,
+                                                          uint32_t* __restrict__ adm_b, uint32_t* __restrict__ near_a,
+                                                          uint32_t* __restrict__ near_b, uint32_t* __restrict__ qa,
+                                                          uint32_t* __restrict__ qb) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= m) return;
+  const uint32_t a = pa[i], b = pb[i];
+  const BlockCounts o = off[i];
+  if (cls[i] == 0) {
+    adm_a[adm0 + o.adm] = a;
+    adm_b[adm0 + o.adm] = b;
+  } else if (cls[i] == 1) {
+    near_a[near0 + o.nearb] = a;
+    near_b[near0 + o.nearb] = b;
+  } else {
+    const uint32_t ra = right[a], rb = right[b];
+    const uint32_t ka[2] = {ra ? a + 1 : a, ra}, kb[2] = {rb ? b + 1 : b, rb};
+    const int xa = ra ? 2 : 1, xb = rb ? 2 : 1;
+    uint32_t q = o.kids;
+    for (int x = 0; x < xa; ++x)
+      for (int y = 0; y < xb; ++y) {
+        qa[q] = ka[x];
+        qb[q] = kb[y];
+        ++q;
+      }
+  }
+}
+
+// ptr[a] = first position of row a in the sorted rows (a in [0, nn]).
+__global__ __launch_bounds__(256) void blk_row_ptr_kernel(uint32_t nn, uint32_t m, const uint32_t* __restrict__ rows,
+                                                          uint32_t* __restrict__ ptr) {
+  const uint32_t a = blockIdx.x * 256 + threadIdx.x;
+  if (a > nn) return;
+  uint32_t lo = 0, hi = m;
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (rows[mid] < a)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  ptr[a] = lo;
+}
+
+// out[0] += Σ |a||b|;  out[1] += Σ (a = b ? |a|(|a| − 1) : |a||b|)  (integer sums).
+__global__ __launch_bounds__(256) void blk_stats_kernel(uint32_t m, const uint32_t* __restrict__ pa,
+                                                        const uint32_t* __restrict__ pb,
+                                                        const uint32_t* __restrict__ start,
+                                                        const uint32_t* __restrict__ end,
+                                                        unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long s0[256], s1[256];
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  unsigned long long x = 0, y = 0;
+  if (i < m) {
+    const uint32_t a = pa[i], b = pb[i];
+    const uint64_t na = end[a] - start[a], nb = end[b] - start[b];
+    x = na * nb;
+    y = a == b ? na * (na - 1) : na * nb;
+  }
+  s0[threadIdx.x] = x;
+  s1[threadIdx.x] = y;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s0[threadIdx.x] += s0[threadIdx.x + w];
+      s1[threadIdx.x] += s1[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    atomicAdd(&out[0], s0[0]);
+    atomicAdd(&out[1], s1[0]);
+  }
+}
+
+// Per leaf ordinal k < nl: near ranges, panel columns, panels and stored entries; zeros at k = nl
+// (the scans then end with the totals).
+__global__ __launch_bounds__(256) void blk_panel_count_kernel(uint32_t nl, const uint32_t* __restrict__ leaves,
+                                                              const uint32_t* __restrict__ parent,
+                                                              const uint32_t* __restrict__ start,
+                                                              const uint32_t* __restrict__ end,
+                                                              const uint32_t* __restrict__ near_ptr,
+                                                              const uint32_t* __restrict__ near_col,
+                                                              uint32_t* __restrict__ rcount,
+                                                              uint32_t* __restrict__ kcount,
+                                                              uint32_t* __restrict__ pcount,
+                                                              uint64_t* __restrict__ entries) {
+  const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+  if (k > nl) return;
+  if (k == nl) {
+    rcount[k] = kcount[k] = pcount[k] = 0;
+    entries[k] = 0;
+    return;
+  }
+  const uint32_t leaf = leaves[k];
+  uint32_t R = 0, K = 0;
+  for (uint32_t a = leaf;; a = parent[a]) {
+    for (uint32_t q = near_ptr[a]; q < near_ptr[a + 1]; ++q) {
+      const uint32_t b = near_col[q];
+      K += end[b] - start[b];
+      ++R;
+    }
+    if (a == 0) break;
+  }
+  const uint32_t ns = end[leaf] - start[leaf];
+  rcount[k] = R;
+  kcount[k] = K;
+  pcount[k] = (ns + kPanelRows - 1) / kPanelRows;
+  entries[k] = (uint64_t)ns * K;
+}
+
+__global__ __launch_bounds__(256) void blk_range_fill_kernel(uint32_t nl, const uint32_t* __restrict__ leaves,
+                                                             const uint32_t* __restrict__ parent,
+                                                             const uint32_t* __restrict__ start,
+                                                             const uint32_t* __restrict__ end,
+                                                             const uint32_t* __restrict__ near_ptr,
+                                                             const uint32_t* __restrict__ near_col,
+                                                             const uint32_t* __restrict__ range_ptr,
+                                                             uint64_t* __restrict__ key, uint32_t* __restrict__ rend) {
+  const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= nl) return;
+  uint32_t o = range_ptr[k];
+  for (uint32_t a = leaves[k];; a = parent[a]) {
+    for (uint32_t q = near_ptr[a]; q < near_ptr[a + 1]; ++q) {
+      const uint32_t b = near_col[q];
+      key[o] = ((uint64_t)k << 32) | start[b];
+      rend[o] = end[b];
+      ++o;
+    }
+    if (a == 0) break;
+  }
+}
+
+// One wave per leaf: the columns of its sorted ranges, in order.
+__global__ __launch_bounds__(256) void blk_cols_kernel(uint32_t nl, const uint32_t* __restrict__ range_ptr,
+                                                       const uint64_t* __restrict__ key,
+                                                       const uint32_t* __restrict__ rend,
+                                                       const uint32_t* __restrict__ col_ptr,
+                                                       uint32_t* __restrict__ cols) {
+  const uint32_t k = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const uint32_t lane = threadIdx.x & 63;
+  if (k >= nl) return;
+  uint32_t o = col_ptr[k];
+  for (uint32_t r = range_ptr[k]; r < range_ptr[k + 1]; ++r) {
+    const uint32_t b0 = (uint32_t)key[r], b1 = rend[r];
+    for (uint32_t x = b0 + lane; x < b1; x += 64) cols[o + (x - b0)] = x;
+    o += b1 - b0;
+  }
+}
+
+__global__ __launch_bounds__(256) void blk_panel_fill_kernel(uint32_t nl, const uint32_t* __restrict__ leaves,
+                                                             const uint32_t* __restrict__ start,
+                                                             const uint32_t* __restrict__ end,
+                                                             const uint32_t* __restrict__ col_ptr,
+                                                             const uint32_t* __restrict__ pan_ptr,
+                                                             const uint64_t* __restrict__ val_base,
+                                                             uint32_t* __restrict__ panel_leaf,
+                                                             uint32_t* __restrict__ panel_row0,
+                                                             uint32_t* __restrict__ panel_rows,
+                                                             uint64_t* __restrict__ panel_val) {
+  const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= nl) return;
+  const uint32_t leaf = leaves[k], K = col_ptr[k + 1] - col_ptr[k];
+  uint32_t p = pan_ptr[k];
+  for (uint32_t r0 = start[leaf]; r0 < end[leaf]; r0 += kPanelRows, ++p) {
+    panel_leaf[p] = k;
+    panel_row0[p] = r0;
+    panel_rows[p] = min(kPanelRows, end[leaf] - r0);
+    panel_val[p] = val_base[k] + (uint64_t)(r0 - start[leaf]) * K;
+  }
+}
+
+__global__ __launch_bounds__(256) void blk_weight_kernel(uint32_t n, const uint32_t* __restrict__ order,
+                                                         const double* __restrict__ w, double* __restrict__ out) {
+  const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+  if (p < n) out[p] = w[order[p]];
+}
+
+namespace {
+
+template <class T>
+void scan_exclusive(ddgrs_ctx* ctx, DevBuf<uint8_t>& temp, const T* in, T* out, size_t n) {
+  size_t tb = 0;
+  HIPCHECK(rocprim::exclusive_scan(nullptr, tb, in, out, T(0), n, rocprim::plus<T>(), ctx->stream));
+  temp.reserve(tb);
+  tb = temp.n;
+  HIPCHECK(rocprim::exclusive_scan(temp.ptr, tb, in, out, T(0), n, rocprim::plus<T>(), ctx->stream));
+}
+
+template <class K>
+void sort_pairs(ddgrs_ctx* ctx, DevBuf<uint8_t>& temp, const K* ki, K* ko, const uint32_t* vi, uint32_t* vo, size_t n,
+                int bits) {
+  size_t tb = 0;
+  HIPCHECK(rocprim::radix_sort_pairs(nullptr, tb, ki, ko, vi, vo, n, 0, bits, ctx->stream));
+  temp.reserve(tb);
+  tb = temp.n;
+  HIPCHECK(rocprim::radix_sort_pairs(temp.ptr, tb, ki, ko, vi, vo, n, 0, bits, ctx->stream));
+}
+
+int bits_for(uint64_t maxval) {
+  int b = 1;
+  while (b < 64 && (1ull << b) <= maxval) ++b;
+  return b;
+}
+
+// Grows `b` to hold `need` elements, keeping the first `used`.
+void grow_keep(ddgrs_ctx* ctx, DevBuf<uint32_t>& b, size_t used, size_t need) {
+  if (need <= b.n && b.ptr) return;
+  DevBuf<uint32_t> nb(std::max<size_t>(need, 2 * b.n));
+  if (used) HIPCHECK(hipMemcpyAsync(nb.ptr, b.ptr, used * sizeof(uint32_t), hipMemcpyDeviceToDevice, ctx->stream));
+  ctx->sync();
+  b = std::move(nb);
+}
+
+}  // namespace
+
+BlockStats BlockTreeDev::build_structure(ddgrs_ctx* ctx, const Topology& t, const Clusters& cl, double chi,
+                                         size_t leaf_pairs, int nk_, const double* elem_weight) {
+  BlockBuildScratch& S = scratch;
+  nk = nk_;
+  n = (int)t.order.size();
+  nn = (int)t.n_nodes();
+  nl = (int)t.leaves.size();
+  ddgrs::upload(ctx, order, t.order);
+  std::vector<uint32_t> pos(n);
+  for (int p = 0; p < n; ++p) pos[t.order[p]] = (uint32_t)p;
+  ddgrs::upload(ctx, position, pos);
+  ddgrs::upload(ctx, start, t.start);
+  ddgrs::upload(ctx, end, t.end);
+  ddgrs::upload(ctx, right, t.right);
+  ddgrs::upload(ctx, parent, t.parent);
+  ddgrs::upload(ctx, leaves, t.leaves);
+  std::vector<double> im(nn);
+  for (int a = 0; a < nn; ++a) im[a] = 1.0 / cl.mass[a];
+  ddgrs::upload(ctx, inv_mass, im);
+  ddgrs::upload(ctx, S.lo, cl.lo);
+  ddgrs::upload(ctx, S.hi, cl.hi);
+  ddgrs::upload(ctx, S.radius, cl.radius);
+  weight.reserve(n);
+  launch(ctx, "blk_weight", blk_weight_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, (uint32_t)n,
+         (const uint32_t*)order.ptr, elem_weight, weight.ptr);
+  const BlockGeo g{start.ptr, end.ptr, right.ptr, S.lo.ptr, S.hi.ptr, S.radius.ptr, chi, (uint64_t)leaf_pairs};
+
+  // level-synchronous classification
+  S.pa.reserve(1);
+  S.pb.reserve(1);
+  const uint32_t root = 0;
+  HIPCHECK(hipMemcpyAsync(S.pa.ptr, &root, sizeof(root), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHECK(hipMemcpyAsync(S.pb.ptr, &root, sizeof(root), hipMemcpyHostToDevice, ctx->stream));
+  uint32_t m = 1, n_adm = 0, n_near = 0, depth = 0;
+  BlockCounts* pin = (BlockCounts*)ctx->staging(2 * sizeof(BlockCounts));
+  while (m > 0) {
+    ++depth;
+    S.cls.reserve(m);
+    S.cnt.reserve(m);
+    S.off.reserve(m);
+    launch(ctx, "blk_classify", blk_classify_kernel, dim3(blocks_for(m, 256)), dim3(256), 0, m,
+           (const uint32_t*)S.pa.ptr, (const uint32_t*)S.pb.ptr, g, S.cls.ptr, S.cnt.ptr);
+    size_t tb = 0;
+    const BlockCounts zero{0, 0, 0};
+    HIPCHECK(rocprim::exclusive_scan(nullptr, tb, S.cnt.ptr, S.off.ptr, zero, m, AddCounts(), ctx->stream));
+    S.temp.reserve(tb);
+    tb = S.temp.n;
+    HIPCHECK(rocprim::exclusive_scan(S.temp.ptr, tb, S.cnt.ptr, S.off.ptr, zero, m, AddCounts(), ctx->stream));
+    download(ctx, pin, (const BlockCounts*)S.off.ptr + (m - 1), 1);
+    download(ctx, pin + 1, (const BlockCounts*)S.cnt.ptr + (m - 1), 1);
+    ctx->sync();
+    const BlockCounts tot = AddCounts()(pin[0], pin[1]);
+    grow_keep(ctx, S.adm_a, n_adm, (size_t)n_adm + tot.adm);
+    grow_keep(ctx, S.adm_b, n_adm, (size_t)n_adm + tot.adm);
+    grow_keep(ctx, S.near_a, n_near, (size_t)n_near + tot.nearb);
+    grow_keep(ctx, S.near_b, n_near, (size_t)n_near + tot.nearb);
+    S.qa.reserve(std::max<uint32_t>(tot.kids, 1));
+    S.qb.reserve(std::max<uint32_t>(tot.kids, 1));
+    launch(ctx, "blk_scatter", blk_scatter_kernel, dim3(blocks_for(m, 256)), dim3(256), 0, m,
+           (const uint32_t*)S.pa.ptr, (const uint32_t*)S.pb.ptr, (const uint32_t*)right.ptr, (const uint8_t*)S.cls.ptr,
+           (const BlockCounts*)S.off.ptr, n_adm, n_near, S.adm_a.ptr, S.adm_b.ptr, S.near_a.ptr, S.near_b.ptr,
+           S.qa.ptr, S.qb.ptr);
+    n_adm += tot.adm;
+    n_near += tot.nearb;
+    m = tot.kids;
+    std::swap(S.pa, S.qa);
+    std::swap(S.pb, S.qb);
+  }
+
+  // grouped by row node, creation order kept
+  const int row_bits = bits_for((uint64_t)nn);
+  n_far = n_adm;
+  far_row.reserve(std::max<uint32_t>(n_far, 1));
+  far_col.reserve(std::max<uint32_t>(n_far, 1));
+  far_ptr.reserve(nn + 1);
+  if (n_far) sort_pairs<uint32_t>(ctx, S.temp, S.adm_a.ptr, far_row.ptr, S.adm_b.ptr, far_col.ptr, n_far, row_bits);
+  launch(ctx, "blk_row_ptr", blk_row_ptr_kernel, dim3(blocks_for(nn + 1, 256)), dim3(256), 0, (uint32_t)nn, n_far,
+         (const uint32_t*)far_row.ptr, far_ptr.ptr);
+  S.near_row.reserve(std::max<uint32_t>(n_near, 1));
+  S.near_col.reserve(std::max<uint32_t>(n_near, 1));
+  S.near_ptr.reserve(nn + 1);
+  if (n_near)
+    sort_pairs<uint32_t>(ctx, S.temp, S.near_a.ptr, S.near_row.ptr, S.near_b.ptr, S.near_col.ptr, n_near, row_bits);
+  launch(ctx, "blk_row_ptr", blk_row_ptr_kernel, dim3(blocks_for(nn + 1, 256)), dim3(256), 0, (uint32_t)nn, n_near,
+         (const uint32_t*)S.near_row.ptr, S.near_ptr.ptr);
+  S.sums.reserve(4);
+  HIPCHECK(hipMemsetAsync(S.sums.ptr, 0, 4 * sizeof(unsigned long long), ctx->stream));
+  launch(ctx, "blk_stats", blk_stats_kernel, dim3(blocks_for(n_far, 256)), dim3(256), 0, n_far,
+         (const uint32_t*)far_row.ptr, (const uint32_t*)far_col.ptr, (const uint32_t*)start.ptr,
+         (const uint32_t*)end.ptr, S.sums.ptr);
+  launch(ctx, "blk_stats", blk_stats_kernel, dim3(blocks_for(n_near, 256)), dim3(256), 0, n_near,
+         (const uint32_t*)S.near_row.ptr, (const uint32_t*)S.near_col.ptr, (const uint32_t*)start.ptr,
+         (const uint32_t*)end.ptr, S.sums.ptr + 2);
+
+  // panels
+  const size_t nl1 = (size_t)nl + 1;
+  S.rcount.reserve(nl1);
+  S.kcount.reserve(nl1);
+  S.pcount.reserve(nl1);
+  S.entries.reserve(nl1);
+  S.range_ptr.reserve(nl1);
+  S.pan_ptr.reserve(nl1);
+  S.val_base.reserve(nl1);
+  col_ptr.reserve(nl1);
+  launch(ctx, "blk_panel_count", blk_panel_count_kernel, dim3(blocks_for(nl1, 256)), dim3(256), 0, (uint32_t)nl,
+         (const uint32_t*)leaves.ptr, (const uint32_t*)parent.ptr, (const uint32_t*)start.ptr, (const uint32_t*)end.ptr,
+         (const uint32_t*)S.near_ptr.ptr, (const uint32_t*)S.near_col.ptr, S.rcount.ptr, S.kcount.ptr, S.pcount.ptr,
+         S.entries.ptr);
+  scan_exclusive<uint32_t>(ctx, S.temp, S.rcount.ptr, S.range_ptr.ptr, nl1);
+  scan_exclusive<uint32_t>(ctx, S.temp, S.kcount.ptr, col_ptr.ptr, nl1);
+  scan_exclusive<uint32_t>(ctx, S.temp, S.pcount.ptr, S.pan_ptr.ptr, nl1);
+  scan_exclusive<uint64_t>(ctx, S.temp, S.entries.ptr, S.val_base.ptr, nl1);
+  struct Tails {
+    uint32_t ranges, cols, panels, pad;
+    uint64_t entries;
+    unsigned long long sums[4];
+  };
+  Tails* tl = (Tails*)ctx->staging(sizeof(Tails));
+  download(ctx, &tl->ranges, (const uint32_t*)S.range_ptr.ptr + nl, 1);
+  download(ctx, &tl->cols, (const uint32_t*)col_ptr.ptr + nl, 1);
+  download(ctx, &tl->panels, (const uint32_t*)S.pan_ptr.ptr + nl, 1);
+  download(ctx, &tl->entries, (const uint64_t*)S.val_base.ptr + nl, 1);
+  download(ctx, tl->sums, (const unsigned long long*)S.sums.ptr, 4);
+  ctx->sync();
+  const Tails T = *tl;
+  np = (int)T.panels;
+  near_total = T.entries;
+  const size_t nr = std::max<uint32_t>(T.ranges, 1);
+  S.range_key.reserve(nr);
+  S.range_key_s.reserve(nr);
+  S.range_end.reserve(nr);
+  S.range_end_s.reserve(nr);
+  launch(ctx, "blk_range_fill", blk_range_fill_kernel, dim3(blocks_for(nl, 256)), dim3(256), 0, (uint32_t)nl,
+         (const uint32_t*)leaves.ptr, (const uint32_t*)parent.ptr, (const uint32_t*)start.ptr, (const uint32_t*)end.ptr,
+         (const uint32_t*)S.near_ptr.ptr, (const uint32_t*)S.near_col.ptr, (const uint32_t*)S.range_ptr.ptr,
+         S.range_key.ptr, S.range_end.ptr);
+  if (T.ranges)
+    sort_pairs<uint64_t>(ctx, S.temp, S.range_key.ptr, S.range_key_s.ptr, S.range_end.ptr, S.range_end_s.ptr, T.ranges,
+                         32 + bits_for((uint64_t)nl));
+  cols.reserve(std::max<uint32_t>(T.cols, 1));
+  launch(ctx, "blk_cols", blk_cols_kernel, dim3(blocks_for(nl, 4)), dim3(256), 0, (uint32_t)nl,
+         (const uint32_t*)S.range_ptr.ptr, (const uint64_t*)S.range_key_s.ptr, (const uint32_t*)S.range_end_s.ptr,
+         (const uint32_t*)col_ptr.ptr, cols.ptr);
+  panel_leaf.reserve(std::max(np, 1));
+  panel_row0.reserve(std::max(np, 1));
+  panel_rows.reserve(std::max(np, 1));
+  panel_val.reserve(std::max(np, 1));
+  launch(ctx, "blk_panel_fill", blk_panel_fill_kernel, dim3(blocks_for(nl, 256)), dim3(256), 0, (uint32_t)nl,
+         (const uint32_t*)leaves.ptr, (const uint32_t*)start.ptr, (const uint32_t*)end.ptr, (const uint32_t*)col_ptr.ptr,
+         (const uint32_t*)S.pan_ptr.ptr, (const uint64_t*)S.val_base.ptr, panel_leaf.ptr, panel_row0.ptr,
+         panel_rows.ptr, panel_val.ptr);
+  val.reserve((size_t)std::max<uint64_t>(near_total, 1) * nk);
+  far_coef.reserve((size_t)std::max<uint32_t>(n_far, 1) * nk);
+  counters.reserve(nn);
+  HIPCHECK(hipMemsetAsync(counters.ptr, 0, (size_t)nn * sizeof(uint32_t), ctx->stream));
+  BlockStats st;
+  st.far_blocks = n_far;
+  st.near_blocks = n_near;
+  st.admissible_pairs = T.sums[0];
+  st.near_total = T.sums[2];
+  st.near_entries = T.sums[3];
+  st.depth = depth;
+  return st;
+}
+
+}  // namespace ddgrs

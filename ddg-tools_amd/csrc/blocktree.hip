@@ -628,25 +628,24 @@ FaceMetricPlan plan_face_metric(const std::vector<uint32_t>& faces, size_t nv, c
   return P;
 }
 
-void FaceMetricDev::build(ddgrs_surface* s, const Topology& t, const Clusters& cl, double chi, size_t leaf_pairs,
-                          double s_exp) {
-  std::vector<double> area_h((size_t)s->nf);
-  download(s->ctx, area_h.data(), (const double*)s->area.ptr, (size_t)s->nf);
-  s->ctx->sync();
-  build(s, plan_face_metric(s->faces_h, (size_t)s->nv, t, cl, chi, leaf_pairs, area_h), s_exp);
+// vertex → (BVH position of the face, corner) from the surface's vertex → (face, corner) lists
+__global__ __launch_bounds__(256) void fm_inc_kernel(uint32_t m, const uint32_t* __restrict__ position,
+                                                     const uint32_t* __restrict__ inc, uint32_t* __restrict__ out) {
+  const uint32_t q = blockIdx.x * 256 + threadIdx.x;
+  if (q < m) out[q] = (position[inc[q] >> 2] << 2) | (inc[q] & 3u);
 }
 
-void FaceMetricDev::build(ddgrs_surface* s, const FaceMetricPlan& P, double s_exp) {
+namespace {
+// Far coefficients, α, near values and the per-face data of a face metric whose block tree is on
+// the device.
+void face_metric_values(ddgrs_surface* s, FaceMetricDev& F, const std::vector<double>& centroid,
+                        const std::vector<double>& projector, const std::vector<double>& node_mass, double s_exp) {
   ddgrs_ctx* ctx = s->ctx;
-  topo = P.topo;
-  bs = P.bs;
-  nv = (int)s->nv;
-  nf = (int)s->nf;
-  bt.upload(ctx, P.bt);
+  BlockTreeDev& bt = F.bt;
   DevBuf<double> cen, proj, nmass;
-  ddgrs::upload(ctx, cen, P.centroid);
-  ddgrs::upload(ctx, proj, P.projector);
-  ddgrs::upload(ctx, nmass, P.node_mass);
+  ddgrs::upload(ctx, cen, centroid);
+  ddgrs::upload(ctx, proj, projector);
+  ddgrs::upload(ctx, nmass, node_mass);
   const double neg_s = -s_exp;
   launch(ctx, "fm_far_coef", fm_far_coef_kernel, dim3(blocks_for(bt.n_far, 256)), dim3(256), 0, bt.n_far,
          (const uint32_t*)bt.far_row.ptr, (const uint32_t*)bt.far_col.ptr, NodeGeo{cen.ptr, proj.ptr, nmass.ptr}, neg_s,
@@ -657,17 +656,45 @@ void FaceMetricDev::build(ddgrs_surface* s, const FaceMetricPlan& P, double s_ex
   launch(ctx, "fm_panel_values", fm_panel_values_kernel, dim3(blocks_for(bt.np, 4)), dim3(256), 0, pa,
          (const uint32_t*)bt.order.ptr, (const double*)s->bary.ptr, (const double*)s->normal.ptr,
          (const double*)s->area.ptr, neg_s, bt.val.ptr);
-  df.reserve(9 * (size_t)nf);
-  area_p.reserve(nf);
-  vert_p.reserve(3 * (size_t)nf);
+  const int nf = F.nf;
+  F.df.reserve(9 * (size_t)nf);
+  F.area_p.reserve(nf);
+  F.vert_p.reserve(3 * (size_t)nf);
   launch(ctx, "fm_face", fm_face_kernel, dim3(blocks_for(nf, 256)), dim3(256), 0, nf, (const uint32_t*)bt.order.ptr,
-         (const uint32_t*)s->faces.ptr, (const double*)s->positions.ptr, (const double*)s->area.ptr, df.ptr, area_p.ptr,
-         vert_p.ptr);
+         (const uint32_t*)s->faces.ptr, (const double*)s->positions.ptr, (const double*)s->area.ptr, F.df.ptr,
+         F.area_p.ptr, F.vert_p.ptr);
+  F.m_hl.reserve(24 * (size_t)nf);
+  F.z_hl.reserve(24 * (size_t)nf);
+  ctx->sync();  // the local buffers above are released on return
+}
+}  // namespace
+
+void FaceMetricDev::build(ddgrs_surface* s, const Topology& t, const Clusters& cl, double chi, size_t leaf_pairs,
+                          double s_exp) {
+  ddgrs_ctx* ctx = s->ctx;
+  topo = t;
+  nv = (int)s->nv;
+  nf = (int)s->nf;
+  stats = bt.build_structure(ctx, t, cl, chi, leaf_pairs, 2, s->area.ptr);
+  inc_ptr.reserve((size_t)nv + 1);
+  HIPCHECK(hipMemcpyAsync(inc_ptr.ptr, s->inc_ptr.ptr, ((size_t)nv + 1) * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                          ctx->stream));
+  inc.reserve(3 * (size_t)nf);
+  launch(ctx, "fm_inc", fm_inc_kernel, dim3(blocks_for(3 * (int64_t)nf, 256)), dim3(256), 0, (uint32_t)(3 * nf),
+         (const uint32_t*)bt.position.ptr, (const uint32_t*)s->inc.ptr, inc.ptr);
+  face_metric_values(s, *this, cl.centroid, cl.projector, cl.mass, s_exp);
+}
+
+void FaceMetricDev::build(ddgrs_surface* s, const FaceMetricPlan& P, double s_exp) {
+  ddgrs_ctx* ctx = s->ctx;
+  topo = P.topo;
+  stats = block_stats(P.bs);
+  nv = (int)s->nv;
+  nf = (int)s->nf;
+  bt.upload(ctx, P.bt);
   ddgrs::upload(ctx, inc_ptr, P.inc_ptr);
   ddgrs::upload(ctx, inc, P.inc);
-  m_hl.reserve(24 * (size_t)nf);
-  z_hl.reserve(24 * (size_t)nf);
-  ctx->sync();  // the local buffers above are released on return
+  face_metric_values(s, *this, P.centroid, P.projector, P.node_mass, s_exp);
 }
 
 void FaceMetricDev::apply(ddgrs_ctx* ctx, const double* x, double* y) {
@@ -765,18 +792,15 @@ VertexFractionalPlan plan_vertex_fractional(const std::vector<double>& pos, cons
   return P;
 }
 
-void VertexFractionalDev::build(ddgrs_ctx* ctx, const VertexFractionalPlan& P) {
-  nv = (int)P.mass.size();
-  topo = P.topo;
-  clusters = P.clusters;
-  bs = P.bs;
-  bt.upload(ctx, P.bt);
-  const double ex = -(P.tau + 1.0);
+namespace {
+void vertex_fractional_values(ddgrs_ctx* ctx, VertexFractionalDev& V, const Clusters& cl, const std::vector<double>& pos,
+                              double tau) {
+  BlockTreeDev& bt = V.bt;
+  const double ex = -(tau + 1.0);
   DevBuf<double> cen, dpos, nmass;
-  ddgrs::upload(ctx, cen, P.clusters.centroid);
-  ddgrs::upload(ctx, dpos, P.pos);
-  ddgrs::upload(ctx, nmass, P.clusters.mass);
-  ddgrs::upload(ctx, mass, P.mass);
+  ddgrs::upload(ctx, cen, cl.centroid);
+  ddgrs::upload(ctx, dpos, pos);
+  ddgrs::upload(ctx, nmass, cl.mass);
   launch(ctx, "vf_far_coef", vf_far_coef_kernel, dim3(blocks_for(bt.n_far, 256)), dim3(256), 0, bt.n_far,
          (const uint32_t*)bt.far_row.ptr, (const uint32_t*)bt.far_col.ptr, (const double*)cen.ptr,
          (const double*)nmass.ptr, ex, bt.far_coef.ptr);
@@ -784,10 +808,31 @@ void VertexFractionalDev::build(ddgrs_ctx* ctx, const VertexFractionalPlan& P) {
   PanelArgs pa{bt.panel_leaf.ptr, bt.panel_row0.ptr, bt.panel_rows.ptr, bt.col_ptr.ptr, bt.cols.ptr, bt.leaves.ptr,
                bt.panel_val.ptr,  nullptr,          nullptr,           nullptr,         nullptr,     bt.np};
   launch(ctx, "vf_panel_values", vf_panel_values_kernel, dim3(blocks_for(bt.np, 4)), dim3(256), 0, pa,
-         (const uint32_t*)bt.order.ptr, (const double*)dpos.ptr, (const double*)mass.ptr, ex, bt.val.ptr);
-  u_hl.reserve(6 * (size_t)nv);
-  z_hl.reserve(6 * (size_t)nv);
+         (const uint32_t*)bt.order.ptr, (const double*)dpos.ptr, (const double*)V.mass.ptr, ex, bt.val.ptr);
+  V.u_hl.reserve(6 * (size_t)V.nv);
+  V.z_hl.reserve(6 * (size_t)V.nv);
   ctx->sync();
+}
+}  // namespace
+
+void VertexFractionalDev::build(ddgrs_ctx* ctx, const VertexFractionalPlan& P) {
+  nv = (int)P.mass.size();
+  topo = P.topo;
+  stats = block_stats(P.bs);
+  bt.upload(ctx, P.bt);
+  ddgrs::upload(ctx, mass, P.mass);
+  vertex_fractional_values(ctx, *this, P.clusters, P.pos, P.tau);
+}
+
+void VertexFractionalDev::build(ddgrs_ctx* ctx, const std::vector<double>& pos, const std::vector<double>& m, double tau,
+                                double chi, size_t leaf_pairs, Topology t) {
+  nv = (int)m.size();
+  topo = std::move(t);
+  // vertex BVH: point f_v, box the point, mass a_v (hierarchical.rs:1017-1060)
+  const Clusters cl = build_clusters(topo, pos.data(), pos.data(), pos.data(), m.data());
+  ddgrs::upload(ctx, mass, m);
+  stats = bt.build_structure(ctx, topo, cl, chi, leaf_pairs, 1, mass.ptr);
+  vertex_fractional_values(ctx, *this, cl, pos, tau);
 }
 
 void VertexFractionalDev::apply(ddgrs_ctx* ctx, const double* x, double* y) {

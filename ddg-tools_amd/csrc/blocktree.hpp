@@ -23,6 +23,29 @@ struct BlockTreePlan {
 BlockTreePlan plan_block_tree(const Topology& t, const BlockStructure& s, int nk, std::vector<double> weight_bvh,
                               const std::vector<double>& node_mass);
 
+// Counts of a block structure (RS/src/spatial/block_tree.rs:340-346): blocks, admissible pairs,
+// near entries (diagonal blocks without their diagonal), stored near-block entries, levels.
+struct BlockStats {
+  uint64_t far_blocks = 0, near_blocks = 0, admissible_pairs = 0, near_entries = 0, near_total = 0;
+  uint32_t depth = 0;
+};
+BlockStats block_stats(const BlockStructure& s);
+
+// Per block pair of a level: admissible, near and child pairs it produces (and their prefix sums).
+struct BlockCounts {
+  uint32_t adm, nearb, kids;
+};
+// Device scratch of a structure build (kept between builds).
+struct BlockBuildScratch {
+  DevBuf<uint32_t> pa, pb, qa, qb, adm_a, adm_b, near_a, near_b, near_row, near_col, near_ptr;
+  DevBuf<uint32_t> rcount, kcount, pcount, range_ptr, pan_ptr, range_end, range_end_s;
+  DevBuf<uint64_t> entries, val_base, range_key, range_key_s;
+  DevBuf<double> lo, hi, radius;
+  DevBuf<uint8_t> cls, temp;
+  DevBuf<BlockCounts> cnt, off;
+  DevBuf<unsigned long long> sums;
+};
+
 // Device form of a block structure over a BVH, for products in difference form
 //   z_S = Σ_{T near S} v_ST (u_S − u_T) + Σ_{far (A, B), S ∈ A} c_AB (u_S − ū_B),
 // v_ST = H_ST a_T (near values × the column weight), c_AB = κ_AB A_B (far coefficient × the column
@@ -54,10 +77,17 @@ struct BlockTreeDev {
   DevBuf<double> alpha_leaf;   // per leaf ordinal, NK
   DevBuf<uint32_t> counters;   // per node, zero between products (upward pass)
   DevBuf<double> sigma, mean, gamma, delta;  // product scratch (C per node / leaf)
+  BlockBuildScratch scratch;
 
   // Structure, panels and weights; near values and far coefficients are filled by the caller's
   // kernels, then finish_far() computes α.
   void upload(ddgrs_ctx* ctx, const BlockTreePlan& p);
+  // The same structure, panels and weights built on the device (bitwise equal to
+  // upload(plan_block_tree(...)) of build_block_structure's lists): level-synchronous
+  // classification of the block pairs, stable grouping by row node, panels from the near lists.
+  // `elem_weight`: per element in the tree's input numbering (device pointer).
+  BlockStats build_structure(ddgrs_ctx* ctx, const Topology& t, const Clusters& c, double chi, size_t leaf_pairs,
+                             int nk, const double* elem_weight);
   void finish_far(ddgrs_ctx* ctx);
   // z = w_k (H̃a·u − H̃(a u)) per BVH position and channel, channels [0, H) on kernel 0 with w0,
   // [H, C) on kernel NK − 1 with w1.  u and z are hi/lo FP32 pairs: 2 C floats per position
@@ -80,7 +110,7 @@ FaceMetricPlan plan_face_metric(const std::vector<uint32_t>& faces, size_t nv, c
 // B̃, B̃⁰ of one configuration (hierarchical.rs:283).
 struct FaceMetricDev {
   Topology topo;
-  BlockStructure bs;
+  BlockStats stats;
   BlockTreeDev bt;
   int nv = 0, nf = 0;
   DevBuf<float> df;         // 9 per BVH position: g_c(S) for c = 0..2
@@ -90,8 +120,10 @@ struct FaceMetricDev {
   DevBuf<uint32_t> inc_ptr, inc;  // vertex → (face, corner), with faces' BVH positions
   double w_hi = 1.0, w_lo = 1.0;
 
-  // Builds on a face tree of the surface's current positions with separation χ and leaf_pairs.
+  // Builds on a face tree of the surface's current positions with separation χ and leaf_pairs
+  // (structure and panels on the device).
   void build(ddgrs_surface* s, const Topology& t, const Clusters& c, double chi, size_t leaf_pairs, double s_exp);
+  // The same from a host plan.
   void build(ddgrs_surface* s, const FaceMetricPlan& p, double s_exp);
   // y = (w_hi B̃ + w_lo B̃⁰) x for one interleaved 3 nv FP64 vector (device pointers).
   void apply(ddgrs_ctx* ctx, const double* x, double* y);
@@ -113,13 +145,15 @@ VertexFractionalPlan plan_vertex_fractional(const std::vector<double>& pos, cons
 // L̃^V_τ over vertex pairs (hierarchical.rs:1003).
 struct VertexFractionalDev {
   Topology topo;
-  Clusters clusters;
-  BlockStructure bs;
+  BlockStats stats;
   BlockTreeDev bt;
   int nv = 0;
   DevBuf<double> mass;      // per vertex (FP64)
   DevBuf<float> u_hl, z_hl; // product scratch: 3 channels, hi/lo
   void build(ddgrs_ctx* ctx, const VertexFractionalPlan& p);
+  // The same with the structure and panels built on the device; `topo` the vertex BVH of `pos`.
+  void build(ddgrs_ctx* ctx, const std::vector<double>& pos, const std::vector<double>& mass, double tau, double chi,
+             size_t leaf_pairs, Topology topo);
   // y = L̃^V x (interleaved 3 nv FP64, device pointers).
   void apply(ddgrs_ctx* ctx, const double* x, double* y);
 };

@@ -1,5 +1,6 @@
 // C ABI of the hierarchical metric B̃, B̃⁰ and of the vertex-pair fractional operator.
 #include <cmath>
+#include <cstdlib>
 
 #include "bh.hpp"
 #include "blocktree.hpp"
@@ -22,6 +23,13 @@ void check_tree_options(double chi, int64_t leaf_size, int64_t leaf_pairs) {
   if (!(std::isfinite(chi) && chi >= 0.0)) invalid("chi", std::to_string(chi), "a finite number >= 0 (0: exact products)");
   if (leaf_size < 1) invalid("leaf_size", std::to_string(leaf_size), "at least 1");
   if (leaf_pairs < 0) invalid("leaf_pairs", std::to_string(leaf_pairs), "non-negative");
+}
+
+// DDGRS_BLOCKS_ON_HOST=1 builds the block structures and panels with the host planner instead of
+// on the device (the two are bitwise equal; the switch exists to check that).
+bool blocks_on_host() {
+  const char* e = std::getenv("DDGRS_BLOCKS_ON_HOST");
+  return e && e[0] == '1';
 }
 
 // Runs `apply(x_dev, y_dev)` on nb host vectors of length n.
@@ -52,7 +60,15 @@ int ddgrs_metric_build(ddgrs_surface* s, double alpha, double beta, double chi, 
     Topology t;
     Clusters c;
     build_face_tree(s, (size_t)leaf_size, &t, &c);
-    h->fm.build(s, t, c, chi, (size_t)leaf_pairs, (e.beta64 - 2.0) / e.alpha64);
+    const double s_exp = (e.beta64 - 2.0) / e.alpha64;
+    if (blocks_on_host()) {
+      std::vector<double> area((size_t)s->nf);
+      download(s->ctx, area.data(), (const double*)s->area.ptr, area.size());
+      s->ctx->sync();
+      h->fm.build(s, plan_face_metric(s->faces_h, (size_t)s->nv, t, c, chi, (size_t)leaf_pairs, area), s_exp);
+    } else {
+      h->fm.build(s, t, c, chi, (size_t)leaf_pairs, s_exp);
+    }
     surface_retain(s);
     *out = h.release();
   });
@@ -78,9 +94,9 @@ int ddgrs_metric_stats(ddgrs_metric* h, int64_t* out, ddgrs_error* err) {
   return guarded(err, "metric_stats", [&] {
     need(h, "metric");
     need(out, "out");
-    const auto& b = h->fm.bs;
-    out[0] = (int64_t)b.far_col.size();
-    out[1] = (int64_t)b.near_col.size();
+    const auto& b = h->fm.stats;
+    out[0] = (int64_t)b.far_blocks;
+    out[1] = (int64_t)b.near_blocks;
     out[2] = (int64_t)b.admissible_pairs;
     out[3] = (int64_t)b.near_entries;
     out[4] = (int64_t)h->fm.topo.n_nodes();
@@ -103,7 +119,10 @@ int ddgrs_vfrac_build(ddgrs_surface* s, double tau, double chi, int64_t leaf_siz
     vertex_areas_host(s, &pos, &mass);
     GpuBvhBuilder b;
     Topology vt = build_vertex_tree(s, (size_t)leaf_size, b);
-    h->vf.build(s->ctx, plan_vertex_fractional(pos, mass, tau, chi, (size_t)leaf_pairs, std::move(vt)));
+    if (blocks_on_host())
+      h->vf.build(s->ctx, plan_vertex_fractional(pos, mass, tau, chi, (size_t)leaf_pairs, std::move(vt)));
+    else
+      h->vf.build(s->ctx, pos, mass, tau, chi, (size_t)leaf_pairs, std::move(vt));
     surface_retain(s);
     *out = h.release();
   });

@@ -959,14 +959,9 @@ HostCsr cotan_stiffness(const std::vector<uint32_t>& faces, size_t nv, const std
 }
 
 PrecondPlan plan_preconditioner(const std::vector<uint32_t>& faces, size_t nv, const std::vector<double>& pos,
-                                const std::vector<double>& mass, double s_exp, double w_tp, double eps_rel, double chi,
-                                size_t leaf_pairs, Topology vertex_tree) {
+                                const std::vector<double>& mass, double s_exp, double w_tp, double eps_rel) {
   PrecondPlan P;
   P.nv = nv;
-  // L̃^V_{2−s} is independent of the factor: build it alongside
-  auto vf_future = std::async(std::launch::async, [&] {
-    return plan_vertex_fractional(pos, mass, 2.0 - s_exp, chi, leaf_pairs, std::move(vertex_tree));
-  });
   P.k = cotan_stiffness(faces, nv, pos, mass, eps_rel, &P.eps);
   const HostCsr& k = P.k;
   P.crow.resize(nv);
@@ -996,10 +991,9 @@ PrecondPlan plan_preconditioner(const std::vector<uint32_t>& faces, size_t nv, c
   NdBlocks nd = nested_dissection(k, pos);
   P.f = factor_multifrontal(lpm, nd);
   P.lp = make_lprime_plan(P.f, nd);
-  // L̃^V_{2−s} and its scale 1/(w_TP 4 I(σ) I(2−s)) (precond.rs:528-564, :893-906)
+  // the scale of L̃^V_{2−s}: 1/(w_TP 4 I(σ) I(2−s)) (precond.rs:528-564, :893-906)
   auto I = [](double t) { return M_PI * std::tgamma(1.0 - t) / (t * std::pow(4.0, t) * std::tgamma(1.0 + t)); };
   P.frac_scale = 1.0 / (4.0 * I(s_exp - 1.0) * I(2.0 - s_exp)) / w_tp;
-  P.vf = vf_future.get();
   return P;
 }
 
@@ -1009,8 +1003,8 @@ void Preconditioner::build(ddgrs_surface* s, double s_exp, double w_tp, double e
   vertex_areas_host(s, &pos, &mass);
   GpuBvhBuilder b;
   Topology vt = build_vertex_tree(s, leaf_size, b);
-  build(s->ctx, plan_preconditioner(s->faces_h, (size_t)s->nv, pos, mass, s_exp, w_tp, eps_rel, chi, leaf_pairs,
-                                    std::move(vt)));
+  build(s->ctx, plan_preconditioner(s->faces_h, (size_t)s->nv, pos, mass, s_exp, w_tp, eps_rel));
+  build_fractional(s->ctx, pos, mass, 2.0 - s_exp, chi, leaf_pairs, std::move(vt));
 }
 
 void Preconditioner::build(ddgrs_ctx* ctx, const PrecondPlan& P) {
@@ -1025,10 +1019,14 @@ void Preconditioner::build(ddgrs_ctx* ctx, const PrecondPlan& P) {
   ddgrs::upload(ctx, kj, P.k.idx);
   ddgrs::upload(ctx, kx, P.k.val);
   ddgrs::upload(ctx, crow_d, crow);
-  vf.build(ctx, P.vf);
   y.reserve(n);
   v.reserve(n);
   ctx->sync();  // the plan's host buffers may be released after return
+}
+
+void Preconditioner::build_fractional(ddgrs_ctx* ctx, const std::vector<double>& pos, const std::vector<double>& mass,
+                                      double tau, double chi, size_t leaf_pairs, Topology vertex_tree) {
+  vf.build(ctx, pos, mass, tau, chi, leaf_pairs, std::move(vertex_tree));
 }
 
 void Preconditioner::apply(ddgrs_ctx* ctx, const double* r, double* z) {

@@ -88,7 +88,7 @@ struct LprimeDev {
 };
 
 // Host part of the preconditioner at given positions: K (cotan + εM), the barycenter row, L′'s
-// factor and block plan, L̃^V_{2−s}'s plan and the fractional scale.
+// factor and block plan and the fractional scale.
 struct PrecondPlan {
   size_t nv = 0;
   double eps = 0.0, frac_scale = 1.0;
@@ -96,12 +96,9 @@ struct PrecondPlan {
   HostCsr k;
   Factor f;
   LprimePlan lp;
-  VertexFractionalPlan vf;
 };
-// `vertex_tree`: the vertex BVH of the same positions (built on the device by the caller).
 PrecondPlan plan_preconditioner(const std::vector<uint32_t>& faces, size_t nv, const std::vector<double>& pos,
-                                const std::vector<double>& mass, double s_exp, double w_tp, double eps_rel, double chi,
-                                size_t leaf_pairs, Topology vertex_tree);
+                                const std::vector<double>& mass, double s_exp, double w_tp, double eps_rel);
 // K = W + ε M with W the cotan stiffness (RS/src/geometry.rs:221), assembled row by row in the
 // order of the reference's triplets (faces ascending, corners ascending); tr W into *trw.
 HostCsr cotan_stiffness(const std::vector<uint32_t>& faces, size_t nv, const std::vector<double>& pos,
@@ -121,6 +118,9 @@ struct Preconditioner {
   // Builds K = W + ε M, L′ and L̃^V_{2−s} at the surface's current positions.
   void build(ddgrs_surface* s, double s_exp, double w_tp, double eps_rel, double chi, size_t leaf_size, size_t leaf_pairs);
   void build(ddgrs_ctx* ctx, const PrecondPlan& p);
+  // L̃^V_{2−s} on the vertex BVH `vertex_tree` of `pos` (structure and panels on the device).
+  void build_fractional(ddgrs_ctx* ctx, const std::vector<double>& pos, const std::vector<double>& mass, double tau,
+                        double chi, size_t leaf_pairs, Topology vertex_tree);
   // z = P⁻¹ r (device pointers, length 3 nv + 3).
   void apply(ddgrs_ctx* ctx, const double* r, double* z);
 };

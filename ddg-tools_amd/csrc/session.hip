@@ -163,23 +163,19 @@ void Session::step(StepReport* rep) {
   const int64_t nv = s->nv, nf = s->nf;
   const size_t nd = 3 * (size_t)nv, n = nd + 3;
   std::memset(rep, 0, sizeof(*rep));
-  // 1. host snapshot; the preconditioner's and the metric's host plans are built on other threads
-  // while this thread builds the face tree and the Barnes–Hut lists and the device computes F, dF
+  // 1. host snapshot; the preconditioner's host plan is built on another thread while this thread
+  // builds the trees and the Barnes–Hut lists and the device computes F, dF and the block trees
   auto t = std::chrono::steady_clock::now();
   compute_face_geometry(s, s->packs, true);
   const HostGeometry geo = download_geometry(s);
   const double s_exp = (exps.beta64 - 2.0) / exps.alpha64;
-  Topology vtopo = build_vertex_tree(s, opts.leaf_size, vert_bvh);
   auto pc_plan = std::async(std::launch::async, [&] {
-    return plan_preconditioner(s->faces_h, (size_t)nv, geo.pos, geo.vmass, s_exp, opts.w_tp, opts.eps_rel, opts.chi,
-                               opts.leaf_pairs, std::move(vtopo));
+    return plan_preconditioner(s->faces_h, (size_t)nv, geo.pos, geo.vmass, s_exp, opts.w_tp, opts.eps_rel);
   });
+  Topology vtopo = build_vertex_tree(s, opts.leaf_size, vert_bvh);
   Topology topo;
   Clusters clusters;
   build_face_tree(s, geo, opts.leaf_size, face_bvh, &topo, &clusters);
-  auto fm_plan = std::async(std::launch::async, [&] {
-    return plan_face_metric(s->faces_h, (size_t)nv, topo, clusters, opts.chi, opts.leaf_pairs, geo.area);
-  });
   if (!opts.energy_exact) bh.build_from(s, topo, clusters, opts.theta, opts.leaf_size);
   rep->seconds_trees = seconds_since(t);
   t = std::chrono::steady_clock::now();
@@ -207,7 +203,8 @@ void Session::step(StepReport* rep) {
   t = std::chrono::steady_clock::now();
   fm.w_hi = opts.w_tp;
   fm.w_lo = opts.w_tp * opts.c_lo;
-  fm.build(s, fm_plan.get(), s_exp);
+  fm.build(s, topo, clusters, opts.chi, opts.leaf_pairs, s_exp);
+  pc.build_fractional(ctx, geo.pos, geo.vmass, 2.0 - s_exp, opts.chi, opts.leaf_pairs, std::move(vtopo));
   rep->seconds_metric = seconds_since(t);
   t = std::chrono::steady_clock::now();
   pc.build(ctx, pc_plan.get());

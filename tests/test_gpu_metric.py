@@ -70,3 +70,20 @@ def test_face_metric_l6(pkg, ctx):
     err = np.linalg.norm(y - y0) / np.linalg.norm(y0)
     print(f"L6 metric: rel {err:.2e}; stats {fm.stats()}")
     assert err <= TOL
+
+
+@pytest.mark.parametrize("which", [0, 1])
+def test_block_build_device_equals_host(pkg, ctx, which, monkeypatch):
+    """The device build of the block structures and panels equals the host planner's bitwise."""
+    name, p, f = list(meshes(pkg))[which]
+    s = pkg.Surface(ctx, f, p)
+    tau = 2.0 - (12.0 - 2.0) / 6.0
+    rng = np.random.default_rng(14)
+    x = rng.standard_normal((2, 3 * len(p)))
+    fm_d, vf_d = pkg.FaceMetricTree(s), pkg.VertexFractionalTree(s, tau)
+    monkeypatch.setenv("DDGRS_BLOCKS_ON_HOST", "1")
+    fm_h, vf_h = pkg.FaceMetricTree(s), pkg.VertexFractionalTree(s, tau)
+    monkeypatch.delenv("DDGRS_BLOCKS_ON_HOST")
+    assert fm_d.stats() == fm_h.stats()
+    assert np.array_equal(fm_d.apply_batch(x, 1.0, 0.3), fm_h.apply_batch(x, 1.0, 0.3))
+    assert np.array_equal(vf_d.apply_batch(x), vf_h.apply_batch(x))
