@@ -90,7 +90,7 @@ int ddgrs_saddle_build(ddgrs_surface* s, double alpha, double beta, const ddgrs_
     h->fm.build(s, t, c, o->chi, (size_t)o->leaf_pairs, s_exp);
     h->pc.build(s, s_exp, o->w_tp, o->eps_rel, o->chi, (size_t)o->leaf_size, (size_t)o->leaf_pairs);
     h->op.fm = &h->fm;
-    h->op.crow = h->pc.crow_d.ptr;
+    h->op.crow = h->pc.crow_dev;
     h->op.nv = (size_t)s->nv;
     h->vec.init(3 * (size_t)s->nv + 3);
     surface_retain(s);
@@ -104,7 +104,7 @@ int ddgrs_saddle_info(ddgrs_saddle* h, double* out2, ddgrs_error* err) {
   return guarded(err, "saddle_info", [&] {
     need(h, "saddle");
     need(out2, "out");
-    out2[0] = h->pc.eps;
+    out2[0] = h->pc.eps_value(h->s->ctx);
     out2[1] = h->pc.frac_scale;
   });
 }

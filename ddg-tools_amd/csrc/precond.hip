@@ -23,6 +23,7 @@
 
 #include "bh.hpp"
 #include "host_parallel.hpp"
+#include "lprime_gpu.hpp"
 #include "precond.hpp"
 
 namespace ddgrs {
@@ -662,7 +663,7 @@ __global__ void lp_long_final_kernel(uint32_t i, const double* __restrict__ part
   x[3 * (size_t)i + a] = b[3 * (size_t)i + a] - s;
 }
 
-LprimePlan make_lprime_plan(const Factor& f, const NdBlocks& nd) {
+LprimePlan make_lprime_schedule(const Factor& f, const NdBlocks& nd) {
   LprimePlan P;
   const size_t n = f.n;
   auto& bstart = P.bstart;
@@ -708,15 +709,54 @@ LprimePlan make_lprime_plan(const Factor& f, const NdBlocks& nd) {
     for (uint32_t e = q; e < f.cp[i + 1]; ++e)
       if (bheight[block_of[f.ci[e]]] <= bheight[b]) throw Error(DDGRS_GPU, "preconditioner", "separator tree schedule violated (column)");
   }
-  // dense inverses M_J = L_JJ⁻¹ (unit lower triangular), row-major
-  auto& ioff = P.inv_off;
-  ioff.assign(nb + 1, 0);
+  // offsets of the dense inverses M_J = L_JJ⁻¹ (row-major k × k)
+  P.inv_off.assign(nb + 1, 0);
   max_rows = 0;
   for (uint32_t b = 0; b < nb; ++b) {
     const uint64_t k = bend[b] - bstart[b];
-    ioff[b + 1] = ioff[b] + k * k;
+    P.inv_off[b + 1] = P.inv_off[b] + k * k;
     max_rows = std::max(max_rows, (uint32_t)k);
   }
+  // work items by height
+  const uint32_t H = *std::max_element(bheight.begin(), bheight.end()) + 1;
+  std::vector<std::vector<std::pair<uint32_t, uint32_t>>> by(H);
+  for (uint32_t b = 0; b < nb; ++b)
+    for (uint32_t r0 = 0; r0 < bend[b] - bstart[b]; r0 += kChunk) by[bheight[b]].push_back({b, r0});
+  auto& iblock = P.item_block;
+  auto& irow0 = P.item_row0;
+  auto& brows = P.rows;
+  brows.resize(nb);
+  level_off.assign(1, 0);
+  for (uint32_t h = 0; h < H; ++h) {
+    for (auto [b, r0] : by[h]) {
+      iblock.push_back(b);
+      irow0.push_back(r0);
+    }
+    level_off.push_back((uint32_t)iblock.size());
+  }
+  for (uint32_t b = 0; b < nb; ++b) brows[b] = bend[b] - bstart[b];
+  // row → block, fused or split heights
+  P.block_of = block_of;
+  P.fused.assign(H, 1);
+  for (uint32_t b = 0; b < nb; ++b)
+    if (brows[b] > kChunk) P.fused[bheight[b]] = 0;
+  P.split_off.assign(1, 0);
+  for (uint32_t h = 0; h < H; ++h) {
+    if (!P.fused[h])
+      for (auto [b, r0] : by[h])
+        for (uint32_t i = bstart[b] + r0; i < std::min(bstart[b] + r0 + kChunk, bend[b]); ++i) P.split_rows.push_back(i);
+    P.split_off.push_back((uint32_t)P.split_rows.size());
+  }
+  return P;
+}
+
+// Dense inverses of the diagonal blocks (and their transposes) from the rows of L.
+void host_block_inverses(const Factor& f, LprimePlan& P) {
+  const uint32_t nb = (uint32_t)P.bstart.size();
+  const auto& bstart = P.bstart;
+  const auto& bend = P.bend;
+  const auto& rs = P.rsplit;
+  const auto& ioff = P.inv_off;
   auto& M = P.inv;
   M.assign(ioff[nb], 0.0);
   auto invert_column = [&](uint32_t b, const std::vector<double>& L, uint32_t c) {
@@ -747,59 +787,31 @@ LprimePlan make_lprime_plan(const Factor& f, const NdBlocks& nd) {
     auto L = dense_l(b);
     parallel(bend[b] - bstart[b], [&](size_t c) { invert_column(b, L, (uint32_t)c); });
   }
-  // work items by height
-  const uint32_t H = *std::max_element(bheight.begin(), bheight.end()) + 1;
-  std::vector<std::vector<std::pair<uint32_t, uint32_t>>> by(H);
-  for (uint32_t b = 0; b < nb; ++b)
-    for (uint32_t r0 = 0; r0 < bend[b] - bstart[b]; r0 += kChunk) by[bheight[b]].push_back({b, r0});
-  auto& iblock = P.item_block;
-  auto& irow0 = P.item_row0;
-  auto& brows = P.rows;
-  brows.resize(nb);
-  level_off.assign(1, 0);
-  for (uint32_t h = 0; h < H; ++h) {
-    for (auto [b, r0] : by[h]) {
-      iblock.push_back(b);
-      irow0.push_back(r0);
-    }
-    level_off.push_back((uint32_t)iblock.size());
-  }
-  for (uint32_t b = 0; b < nb; ++b) brows[b] = bend[b] - bstart[b];
-  // transposed inverses, row → block, fused or split heights
   P.inv_t.assign(M.size(), 0.0);
   for (uint32_t b = 0; b < nb; ++b) {
-    const uint32_t k = brows[b];
+    const uint32_t k = bend[b] - bstart[b];
     const double* m = M.data() + ioff[b];
     double* mt = P.inv_t.data() + ioff[b];
     for (uint32_t i = 0; i < k; ++i)
       for (uint32_t j = 0; j < k; ++j) mt[(size_t)j * k + i] = m[(size_t)i * k + j];
   }
-  P.block_of = block_of;
-  P.fused.assign(H, 1);
-  for (uint32_t b = 0; b < nb; ++b)
-    if (brows[b] > kChunk) P.fused[bheight[b]] = 0;
-  P.split_off.assign(1, 0);
-  for (uint32_t h = 0; h < H; ++h) {
-    if (!P.fused[h])
-      for (auto [b, r0] : by[h])
-        for (uint32_t i = bstart[b] + r0; i < std::min(bstart[b] + r0 + kChunk, bend[b]); ++i) P.split_rows.push_back(i);
-    P.split_off.push_back((uint32_t)P.split_rows.size());
-  }
+}
+
+LprimePlan make_lprime_plan(const Factor& f, const NdBlocks& nd) {
+  LprimePlan P = make_lprime_schedule(f, nd);
+  host_block_inverses(f, P);
   return P;
 }
 
-void LprimeDev::upload(ddgrs_ctx* ctx, const Factor& f, const LprimePlan& P) {
+void LprimeDev::setup(ddgrs_ctx* ctx, const Factor& f, const LprimePlan& P) {
   n = f.n;
   level_off = P.level_off;
   max_rows = P.max_rows;
   ddgrs::upload(ctx, perm, f.perm);
   ddgrs::upload(ctx, rp, f.rp);
   ddgrs::upload(ctx, rj, f.rj);
-  ddgrs::upload(ctx, rx, f.rx);
   ddgrs::upload(ctx, cp, f.cp);
   ddgrs::upload(ctx, ci, f.ci);
-  ddgrs::upload(ctx, cx, f.cx);
-  ddgrs::upload(ctx, d, f.d);
   ddgrs::upload(ctx, rsplit, P.rsplit);
   ddgrs::upload(ctx, csplit, P.csplit);
   ddgrs::upload(ctx, item_block, P.item_block);
@@ -807,16 +819,28 @@ void LprimeDev::upload(ddgrs_ctx* ctx, const Factor& f, const LprimePlan& P) {
   ddgrs::upload(ctx, blk_start, P.bstart);
   ddgrs::upload(ctx, blk_rows, P.rows);
   ddgrs::upload(ctx, blk_inv, P.inv_off);
-  ddgrs::upload(ctx, inv, P.inv);
-  ddgrs::upload(ctx, inv_t, P.inv_t);
   ddgrs::upload(ctx, block_of, P.block_of);
   ddgrs::upload(ctx, split_rows, P.split_rows);
   split_off = P.split_off;
   fused = P.fused;
+  rx.reserve(std::max<size_t>(f.rj.size(), 1));
+  cx.reserve(std::max<size_t>(f.ci.size(), 1));
+  d.reserve(n);
+  inv.reserve(std::max<uint64_t>(P.inv_off.back(), 1));
+  inv_t.reserve(std::max<uint64_t>(P.inv_off.back(), 1));
   xb.reserve(3 * n);
   xf.reserve(3 * n);
   rbuf.reserve(3 * n);
   parts.reserve(3 * kLongParts);
+}
+
+void LprimeDev::upload(ddgrs_ctx* ctx, const Factor& f, const LprimePlan& P) {
+  setup(ctx, f, P);
+  ddgrs::upload(ctx, rx, f.rx);
+  ddgrs::upload(ctx, cx, f.cx);
+  ddgrs::upload(ctx, d, f.d);
+  ddgrs::upload(ctx, inv, P.inv);
+  ddgrs::upload(ctx, inv_t, P.inv_t);
 }
 
 void LprimeDev::solve(ddgrs_ctx* ctx, const double* r, double* x) {
@@ -997,13 +1021,35 @@ PrecondPlan plan_preconditioner(const std::vector<uint32_t>& faces, size_t nv, c
   return P;
 }
 
+Preconditioner::Preconditioner() = default;
+Preconditioner::~Preconditioner() = default;
+
+namespace {
+double fractional_scale(double s_exp, double w_tp) {
+  // the scale of L̃^V_{2−s}: 1/(w_TP 4 I(σ) I(2−s)) (precond.rs:528-564, :893-906)
+  auto I = [](double t) { return M_PI * std::tgamma(1.0 - t) / (t * std::pow(4.0, t) * std::tgamma(1.0 + t)); };
+  return 1.0 / (4.0 * I(s_exp - 1.0) * I(2.0 - s_exp)) / w_tp;
+}
+bool lprime_on_host() {
+  const char* e = std::getenv("DDGRS_LPRIME_ON_HOST");
+  return e && e[0] == '1';
+}
+}  // namespace
+
 void Preconditioner::build(ddgrs_surface* s, double s_exp, double w_tp, double eps_rel, double chi, size_t leaf_size,
                            size_t leaf_pairs) {
   std::vector<double> pos, mass;
   vertex_areas_host(s, &pos, &mass);
   GpuBvhBuilder b;
   Topology vt = build_vertex_tree(s, leaf_size, b);
-  build(s->ctx, plan_preconditioner(s->faces_h, (size_t)s->nv, pos, mass, s_exp, w_tp, eps_rel));
+  if (lprime_on_host()) {
+    build(s->ctx, plan_preconditioner(s->faces_h, (size_t)s->nv, pos, mass, s_exp, w_tp, eps_rel));
+  } else {
+    setup_device(s->ctx, lprime_symbolic(s->faces_h, (size_t)s->nv, pos, s->ctx->prop.sharedMemPerBlock));
+    factor_device(s, s_exp, w_tp, eps_rel);
+    s->ctx->sync();
+    check_factor(s->ctx);
+  }
   build_fractional(s->ctx, pos, mass, 2.0 - s_exp, chi, leaf_pairs, std::move(vt));
 }
 
@@ -1019,9 +1065,43 @@ void Preconditioner::build(ddgrs_ctx* ctx, const PrecondPlan& P) {
   ddgrs::upload(ctx, kj, P.k.idx);
   ddgrs::upload(ctx, kx, P.k.val);
   ddgrs::upload(ctx, crow_d, crow);
+  k_ptr = kp.ptr;
+  k_idx = kj.ptr;
+  k_val = kx.ptr;
+  crow_dev = crow_d.ptr;
+  fac.reset();
   y.reserve(n);
   v.reserve(n);
   ctx->sync();  // the plan's host buffers may be released after return
+}
+
+void Preconditioner::setup_device(ddgrs_ctx* ctx, const LprimeSymbolic& sym) {
+  nv = sym.nv;
+  n = 3 * nv + 3;
+  fac = std::make_unique<LprimeFactorDev>();
+  fac->setup(ctx, sym, lp);
+  k_ptr = fac->kp.ptr;
+  k_idx = fac->kj.ptr;
+  k_val = fac->kx.ptr;
+  crow_dev = fac->crow.ptr;
+  y.reserve(n);
+  v.reserve(n);
+}
+
+void Preconditioner::factor_device(ddgrs_surface* s, double s_exp, double w_tp, double eps_rel) {
+  if (!fac) throw Error(DDGRS_GPU, "preconditioner", "device factorization used before its symbolic set-up");
+  frac_scale = fractional_scale(s_exp, w_tp);
+  fac->factor(s, eps_rel, lp);
+  eps = -1.0;  // read on demand
+}
+
+void Preconditioner::check_factor(ddgrs_ctx* ctx) {
+  if (fac) fac->check(ctx);
+}
+
+double Preconditioner::eps_value(ddgrs_ctx* ctx) {
+  if (fac && eps < 0.0) eps = fac->eps(ctx);
+  return eps;
 }
 
 void Preconditioner::build_fractional(ddgrs_ctx* ctx, const std::vector<double>& pos, const std::vector<double>& mass,
@@ -1032,8 +1112,8 @@ void Preconditioner::build_fractional(ddgrs_ctx* ctx, const std::vector<double>&
 void Preconditioner::apply(ddgrs_ctx* ctx, const double* r, double* z) {
   lp.solve(ctx, r, y.ptr);
   vf.apply(ctx, y.ptr, v.ptr);
-  launch(ctx, "pc_augment", pc_augment_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, (uint32_t)nv,
-         (const uint32_t*)kp.ptr, (const uint32_t*)kj.ptr, (const double*)kx.ptr, frac_scale, (const double*)y.ptr, v.ptr);
+  launch(ctx, "pc_augment", pc_augment_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, (uint32_t)nv, k_ptr, k_idx,
+         k_val, frac_scale, (const double*)y.ptr, v.ptr);
   lp.solve(ctx, v.ptr, z);
   launch(ctx, "pc_add", pc_add_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, (uint32_t)n, (const double*)y.ptr, z);
 }

@@ -3,6 +3,8 @@
 // host (sparse LDLᵀ in a nested-dissection order), solved on the device in FP64; the augmented
 // form z = L′⁻¹ v + y, y = L′⁻¹ r, v = s L̃^V y − K y (precond.rs:1026 apply_fine).
 #pragma once
+#include <memory>
+
 #include "blocktree.hpp"
 
 namespace ddgrs {
@@ -67,6 +69,10 @@ struct LprimePlan {
 // Throws if an off-block entry of L joins two blocks of the same height (the blocks of one height
 // must be independent).
 LprimePlan make_lprime_plan(const Factor& f, const NdBlocks& nd);
+// The same without the inverses (only the pattern of `f` is read; inv_off is set).
+LprimePlan make_lprime_schedule(const Factor& f, const NdBlocks& nd);
+// The dense inverses (and transposes) of a schedule from the rows of `f`.
+void host_block_inverses(const Factor& f, LprimePlan& p);
 
 struct LprimeDev {
   size_t n = 0, nv = 0;
@@ -83,6 +89,9 @@ struct LprimeDev {
   double inverse_seconds = 0.0;
   // Block schedule and dense inverses of the blocks' diagonal parts of L.
   void upload(ddgrs_ctx* ctx, const Factor& f, const LprimePlan& p);
+  // The schedule and the pattern of L only; rx, cx, d, inv and inv_t are sized for a device
+  // factorization to fill.
+  void setup(ddgrs_ctx* ctx, const Factor& pattern, const LprimePlan& p);
   // x = L′⁻¹ r for a full saddle vector (3 nv + 3, interleaved positions then multipliers).
   void solve(ddgrs_ctx* ctx, const double* r, double* x);
 };
@@ -104,20 +113,36 @@ PrecondPlan plan_preconditioner(const std::vector<uint32_t>& faces, size_t nv, c
 HostCsr cotan_stiffness(const std::vector<uint32_t>& faces, size_t nv, const std::vector<double>& pos,
                         const std::vector<double>& mass, double eps_rel, double* eps);
 
+struct LprimeSymbolic;
+struct LprimeFactorDev;
+
 struct Preconditioner {
   size_t nv = 0, n = 0;
   double eps = 0.0, frac_scale = 1.0;
   LprimeDev lp;
   VertexFractionalDev vf;
-  DevBuf<uint32_t> kp, kj;  // K (CSR, symmetric)
+  DevBuf<uint32_t> kp, kj;  // K (CSR, symmetric) of a host plan
   DevBuf<double> kx;
   DevBuf<double> y, v;      // work vectors (n)
-  std::vector<double> crow; // barycenter row a_v / Σ a
+  std::vector<double> crow; // barycenter row a_v / Σ a of a host plan
   DevBuf<double> crow_d;
+  std::unique_ptr<LprimeFactorDev> fac;  // the device factorization (set up once per mesh)
+  // What apply() reads: K and the barycenter row, from the host plan or the device factorization.
+  const uint32_t *k_ptr = nullptr, *k_idx = nullptr;
+  const double *k_val = nullptr, *crow_dev = nullptr;
 
-  // Builds K = W + ε M, L′ and L̃^V_{2−s} at the surface's current positions.
+  Preconditioner();
+  ~Preconditioner();
+  // Builds K = W + ε M, L′ and L̃^V_{2−s} at the surface's current positions (L′ factored on the
+  // device unless DDGRS_LPRIME_ON_HOST=1 asks for the host factorization).
   void build(ddgrs_surface* s, double s_exp, double w_tp, double eps_rel, double chi, size_t leaf_size, size_t leaf_pairs);
   void build(ddgrs_ctx* ctx, const PrecondPlan& p);
+  // Device factorization: the symbolic structure (once per mesh), then per configuration K, L′'s
+  // factor and block inverses (enqueued; check_factor() after the next synchronization).
+  void setup_device(ddgrs_ctx* ctx, const LprimeSymbolic& sym);
+  void factor_device(ddgrs_surface* s, double s_exp, double w_tp, double eps_rel);
+  void check_factor(ddgrs_ctx* ctx);
+  double eps_value(ddgrs_ctx* ctx);
   // L̃^V_{2−s} on the vertex BVH `vertex_tree` of `pos` (structure and panels on the device).
   void build_fractional(ddgrs_ctx* ctx, const std::vector<double>& pos, const std::vector<double>& mass, double tau,
                         double chi, size_t leaf_pairs, Topology vertex_tree);

@@ -109,6 +109,12 @@ void Session::init(ddgrs_surface* s_, const StepOptions& o) {
   }
   for (int k = 0; k < 3; ++k) target[k] = bc[k] / tot;
   last_tau = 0.0;
+  // the mesh's L′ structure (fixed while only the positions change), on another thread
+  lp_ready = false;
+  const size_t lds = s->ctx->prop.sharedMemPerBlock;
+  lp_symbolic = std::async(std::launch::async, [this, nv, lds, p = std::move(pos)] {
+    return lprime_symbolic(s->faces_h, nv, p, lds);
+  });
   const size_t n = 3 * nv + 3;
   vec.init(n);
   g.reserve(n);
@@ -163,19 +169,21 @@ void Session::step(StepReport* rep) {
   const int64_t nv = s->nv, nf = s->nf;
   const size_t nd = 3 * (size_t)nv, n = nd + 3;
   std::memset(rep, 0, sizeof(*rep));
-  // 1. host snapshot; the preconditioner's host plan is built on another thread while this thread
-  // builds the trees and the Barnes–Hut lists and the device computes F, dF and the block trees
+  // 1. trees on the device; L′'s factorization is enqueued before the host builds the Barnes–Hut
+  // lists, so the two overlap
   auto t = std::chrono::steady_clock::now();
   compute_face_geometry(s, s->packs, true);
   const HostGeometry geo = download_geometry(s);
   const double s_exp = (exps.beta64 - 2.0) / exps.alpha64;
-  auto pc_plan = std::async(std::launch::async, [&] {
-    return plan_preconditioner(s->faces_h, (size_t)nv, geo.pos, geo.vmass, s_exp, opts.w_tp, opts.eps_rel);
-  });
   Topology vtopo = build_vertex_tree(s, opts.leaf_size, vert_bvh);
   Topology topo;
   Clusters clusters;
   build_face_tree(s, geo, opts.leaf_size, face_bvh, &topo, &clusters);
+  if (!lp_ready) {
+    pc.setup_device(ctx, lp_symbolic.get());
+    lp_ready = true;
+  }
+  pc.factor_device(s, s_exp, opts.w_tp, opts.eps_rel);
   if (!opts.energy_exact) bh.build_from(s, topo, clusters, opts.theta, opts.leaf_size);
   rep->seconds_trees = seconds_since(t);
   t = std::chrono::steady_clock::now();
@@ -204,13 +212,13 @@ void Session::step(StepReport* rep) {
   fm.w_hi = opts.w_tp;
   fm.w_lo = opts.w_tp * opts.c_lo;
   fm.build(s, topo, clusters, opts.chi, opts.leaf_pairs, s_exp);
-  pc.build_fractional(ctx, geo.pos, geo.vmass, 2.0 - s_exp, opts.chi, opts.leaf_pairs, std::move(vtopo));
   rep->seconds_metric = seconds_since(t);
   t = std::chrono::steady_clock::now();
-  pc.build(ctx, pc_plan.get());
+  pc.build_fractional(ctx, geo.pos, geo.vmass, 2.0 - s_exp, opts.chi, opts.leaf_pairs, std::move(vtopo));
+  pc.check_factor(ctx);  // the factorization ended before the synchronizations above
   rep->seconds_precond = seconds_since(t);
   op.fm = &fm;
-  op.crow = pc.crow_d.ptr;
+  op.crow = pc.crow_dev;
   op.nv = (size_t)nv;
   t = std::chrono::steady_clock::now();
   GmresSettings gs{opts.restart, opts.tolerance, opts.max_iterations};

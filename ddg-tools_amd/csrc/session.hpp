@@ -2,8 +2,11 @@
 // energy and differential, the saddle solve for the direction, the Armijo line search and the
 // barycenter projection.
 #pragma once
+#include <future>
+
 #include "bh.hpp"
 #include "gmres.hpp"
+#include "lprime_gpu.hpp"
 
 namespace ddgrs {
 
@@ -37,6 +40,8 @@ struct Session {
   GpuBvhBuilder face_bvh, vert_bvh;
   FaceMetricDev fm;
   Preconditioner pc;
+  std::future<LprimeSymbolic> lp_symbolic;  // L′'s symbolic analysis, started at init
+  bool lp_ready = false;
   SaddleOp op;
   DeviceVecs vec;
   GmresResult last_gmres;
