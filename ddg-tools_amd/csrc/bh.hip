@@ -13,6 +13,8 @@
 // Own and partner sums are reduced over j by an xor butterfly (bitwise identical on all lanes,
 // independent of scheduling).  Pair arithmetic is FP32 with the hi/lo barycenter difference; the
 // node sums, the downward pass and the per-face epilogue are FP64.
+#include <cstdlib>
+
 #include "bh.hpp"
 
 namespace ddgrs {
@@ -399,14 +401,6 @@ void BhEval::upload_structure(ddgrs_ctx* ctx) {
   upload(ctx, d_end, t.end);
   upload(ctx, d_right, t.right);
   upload(ctx, d_leaves, t.leaves);
-  upload(ctx, d_lumped_off, lists.lumped_off);
-  upload(ctx, d_lumped, lists.lumped);
-  upload(ctx, d_exact_off, lists.exact_off);
-  upload(ctx, d_exact, lists.exact);
-  upload(ctx, d_exact_t_off, lists.exact_t_off);
-  upload(ctx, d_exact_t, lists.exact_t);
-  upload(ctx, d_lumped_t_off, lists.lumped_t_off);
-  upload(ctx, d_lumped_t, lists.lumped_t);
   std::vector<uint32_t> leaf_of_pos(nf), chain_ptr{0}, chain;
   for (int k = 0; k < nl; ++k) {
     uint32_t L = t.leaves[k];
@@ -440,7 +434,6 @@ void BhEval::upload_structure(ddgrs_ctx* ctx) {
   cen.reserve(3 * nn);
   chi.reserve(nn);
   clo.reserve(nn);
-  node_bu.reserve(std::max<size_t>(1, lists.lumped.size()));
   sums.reserve(nn);
   part.reserve(7 * (size_t)nf);
   leaf_energy.reserve(nl);
@@ -609,9 +602,51 @@ void BhEval::build_from(ddgrs_surface* s, const Topology& t, const Clusters& c, 
   leaf_size = leaf_size_;
   topo = t;
   clusters = c;
-  lists = build_bh_lists(topo, clusters, theta);
-  upload_structure(s->ctx);
+  ddgrs_ctx* ctx = s->ctx;
+  upload_structure(ctx);
+  const char* host = std::getenv("DDGRS_BH_LISTS_ON_HOST");
+  if (host && host[0] == '1') {
+    lists = build_bh_lists(topo, clusters, theta);
+    upload(ctx, d_lumped_off, lists.lumped_off);
+    upload(ctx, d_lumped, lists.lumped);
+    upload(ctx, d_exact_off, lists.exact_off);
+    upload(ctx, d_exact, lists.exact);
+    upload(ctx, d_exact_t_off, lists.exact_t_off);
+    upload(ctx, d_exact_t, lists.exact_t);
+    upload(ctx, d_lumped_t_off, lists.lumped_t_off);
+    upload(ctx, d_lumped_t, lists.lumped_t);
+    n_lumped = lists.lumped.size();
+    n_exact = lists.exact.size();
+    pairs_exact = lists.pairs_exact;
+  } else {
+    lists = BhLists();
+    build_lists_device(ctx);
+  }
+  node_bu.reserve(std::max<size_t>(1, n_lumped));
   refresh(s);
+}
+
+BhLists BhEval::download_lists(ddgrs_ctx* ctx) const {
+  BhLists L;
+  L.lumped_off.resize((size_t)nl + 1);
+  L.exact_off.resize((size_t)nl + 1);
+  L.lumped.resize(n_lumped);
+  L.exact.resize(n_exact);
+  L.lumped_t_off.resize((size_t)nn + 1);
+  L.lumped_t.resize(n_lumped);
+  L.exact_t_off.resize((size_t)nl + 1);
+  L.exact_t.resize(n_exact);
+  download(ctx, L.lumped_off.data(), (const uint32_t*)d_lumped_off.ptr, L.lumped_off.size());
+  download(ctx, L.exact_off.data(), (const uint32_t*)d_exact_off.ptr, L.exact_off.size());
+  download(ctx, L.lumped.data(), (const uint32_t*)d_lumped.ptr, L.lumped.size());
+  download(ctx, L.exact.data(), (const uint32_t*)d_exact.ptr, L.exact.size());
+  download(ctx, L.lumped_t_off.data(), (const uint32_t*)d_lumped_t_off.ptr, L.lumped_t_off.size());
+  download(ctx, L.lumped_t.data(), (const uint32_t*)d_lumped_t.ptr, L.lumped_t.size());
+  download(ctx, L.exact_t_off.data(), (const uint32_t*)d_exact_t_off.ptr, L.exact_t_off.size());
+  download(ctx, L.exact_t.data(), (const uint32_t*)d_exact_t.ptr, L.exact_t.size());
+  ctx->sync();
+  L.pairs_exact = pairs_exact;
+  return L;
 }
 
 }  // namespace ddgrs

@@ -15,8 +15,9 @@ struct BhEval {
   size_t leaf_size = 8;
   Topology topo;
   Clusters clusters;  // build-time cluster data (boxes, radii: frozen)
-  BhLists lists;
+  BhLists lists;      // host lists (DDGRS_BH_LISTS_ON_HOST=1 only; the lists live on the device)
   int nf = 0, nn = 0, nl = 0;
+  uint64_t n_lumped = 0, n_exact = 0, pairs_exact = 0;
   // structure on the device
   DevBuf<uint32_t> d_order, d_start, d_end, d_right, d_leaves, d_lumped_off, d_lumped, d_exact_off, d_exact,
       d_exact_t_off, d_exact_t, d_lumped_t_off, d_lumped_t, d_leaf_of_pos, d_chain_ptr, d_chain, d_levels;
@@ -27,6 +28,11 @@ struct BhEval {
   DevBuf<double> mass, moment, cen, leaf_energy;
   DevBuf<double4> sums;
   DevBuf<float> part;
+  // device list build
+  DevBuf<uint32_t> d_esc, d_leaf_ordinal, l_count, e_count, l_idx, l_key_s, e_tgt, e_src, e_tgt_s;
+  DevBuf<double> d_nlo, d_nhi, d_nrad;
+  DevBuf<unsigned long long> d_pairs;
+  DevBuf<uint8_t> temp;
 
   // Tree and lists at the surface's current positions (face geometry must be current).
   void build(ddgrs_surface* s, double theta, size_t leaf_size);
@@ -38,9 +44,13 @@ struct BhEval {
   void energy(ddgrs_ctx* ctx, const Exps& e, double* total);
   // Ẽ into total[0] and the per-face partials (7·nf FP64, face order) on the device.
   void gradient(ddgrs_surface* s, const Exps& e, double* total, double* partials);
+  // The lists on the host (copied from the device).
+  BhLists download_lists(ddgrs_ctx* ctx) const;
 
  private:
   void upload_structure(ddgrs_ctx* ctx);
+  // Lists, transposes and counts on the device from topo and clusters (bh_lists_gpu.hip).
+  void build_lists_device(ddgrs_ctx* ctx);
   BhArgs args() const;
 };
 

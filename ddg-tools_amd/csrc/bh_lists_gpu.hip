@@ -1,0 +1,212 @@
+// Barnes–Hut interaction lists built on the device (RS/src/energy/barnes_hut.rs:743 build_lists).
+//
+// For each leaf L the reference walks the tree from the root depth first, left child first: a node
+// A is lumped when max(r_L, r_A)² < θ² dist²(box_L, box_A), an exact leaf when it is a leaf, and
+// opened otherwise.  With the nodes numbered in pre-order the walk visits nodes in increasing
+// order, so it is done here without a stack: after a lumped node or a leaf the walk continues at
+// the node after A's subtree (its escape index), after an opened node at A + 1.  One thread per
+// leaf walks twice (counts, then the lists after a scan); the transposes are stable radix sorts of
+// the entries by target.  The admissibility test uses the host's FP64 operations without
+// contraction, so the lists are the host build's.
+#include <cstring>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include <algorithm>
+
+#include "bh.hpp"
+
+namespace ddgrs {
+
+#pragma clang fp contract(off)
+
+namespace {
+struct BhTree {
+  const uint32_t *right, *esc, *start, *end, *leaves;
+  const double *lo, *hi, *radius;
+  uint32_t nn;
+  double th2;
+};
+
+__device__ __forceinline__ bool bh_lumped(const BhTree& T, uint32_t l, uint32_t a) {
+  double d2 = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    const double d = fmax(fmax(T.lo[3 * l + k], T.lo[3 * a + k]) - fmin(T.hi[3 * l + k], T.hi[3 * a + k]), 0.0);
+    d2 += d * d;
+  }
+  const double h = fmax(T.radius[l], T.radius[a]);
+  return h * h < T.th2 * d2;
+}
+}  // namespace
+
+// Per leaf ordinal k < nl: lumped and exact counts, and Σ over exact nodes of the ordered pairs;
+// zeros at k = nl (the scans end with the totals).
+__global__ __launch_bounds__(256) void bhl_count_kernel(BhTree T, uint32_t nl, uint32_t* __restrict__ lcount,
+                                                        uint32_t* __restrict__ ecount,
+                                                        unsigned long long* __restrict__ pairs) {
+  __shared__ unsigned long long red[256];
+  const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+  unsigned long long pk = 0;
+  if (k < nl) {
+    const uint32_t l = T.leaves[k];
+    const uint64_t ns = T.end[l] - T.start[l];
+    uint32_t nlum = 0, nex = 0;
+    for (uint32_t a = 0; a < T.nn;) {
+      if (bh_lumped(T, l, a)) {
+        ++nlum;
+        a = T.esc[a];
+      } else if (T.right[a] == 0) {
+        ++nex;
+        pk += a == l ? ns * (ns - 1) : ns * (uint64_t)(T.end[a] - T.start[a]);
+        a = T.esc[a];
+      } else {
+        a = a + 1;
+      }
+    }
+    lcount[k] = nlum;
+    ecount[k] = nex;
+  } else if (k == nl) {
+    lcount[k] = ecount[k] = 0;
+  }
+  red[threadIdx.x] = pk;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(pairs, red[0]);  // an integer sum
+}
+
+// The lists, with each exact entry's target leaf ordinal and source leaf ordinal, and the lumped
+// entries' indices.
+__global__ __launch_bounds__(256) void bhl_fill_kernel(BhTree T, uint32_t nl, const uint32_t* __restrict__ leaf_ordinal,
+                                                       const uint32_t* __restrict__ lumped_off,
+                                                       const uint32_t* __restrict__ exact_off,
+                                                       uint32_t* __restrict__ lumped, uint32_t* __restrict__ lumped_e,
+                                                       uint32_t* __restrict__ exact, uint32_t* __restrict__ exact_tgt,
+                                                       uint32_t* __restrict__ exact_src) {
+  const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= nl) return;
+  const uint32_t l = T.leaves[k];
+  uint32_t ql = lumped_off[k], qe = exact_off[k];
+  for (uint32_t a = 0; a < T.nn;) {
+    if (bh_lumped(T, l, a)) {
+      lumped[ql] = a;
+      lumped_e[ql] = ql;
+      ++ql;
+      a = T.esc[a];
+    } else if (T.right[a] == 0) {
+      exact[qe] = a;
+      exact_tgt[qe] = leaf_ordinal[a];
+      exact_src[qe] = k;
+      ++qe;
+      a = T.esc[a];
+    } else {
+      a = a + 1;
+    }
+  }
+}
+
+// ptr[x] = first position of key x in the sorted keys (x in [0, m]).
+__global__ __launch_bounds__(256) void bhl_ptr_kernel(uint32_t m, uint32_t count, const uint32_t* __restrict__ keys,
+                                                      uint32_t* __restrict__ ptr) {
+  const uint32_t x = blockIdx.x * 256 + threadIdx.x;
+  if (x > m) return;
+  uint32_t lo = 0, hi = count;
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (keys[mid] < x)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  ptr[x] = lo;
+}
+
+namespace {
+int bits_for(uint64_t maxval) {
+  int b = 1;
+  while (b < 32 && (1ull << b) <= maxval) ++b;
+  return b;
+}
+}  // namespace
+
+void BhEval::build_lists_device(ddgrs_ctx* ctx) {
+  const Topology& t = topo;
+  // escape indices: the node after each subtree in pre-order
+  std::vector<uint32_t> esc(nn);
+  esc[0] = (uint32_t)nn;
+  for (int a = 0; a < nn; ++a)
+    if (t.right[a] != 0) {
+      esc[a + 1] = t.right[a];
+      esc[t.right[a]] = esc[a];
+    }
+  ddgrs::upload(ctx, d_esc, esc);
+  ddgrs::upload(ctx, d_leaf_ordinal, t.leaf_ordinal);
+  ddgrs::upload(ctx, d_nlo, clusters.lo);
+  ddgrs::upload(ctx, d_nhi, clusters.hi);
+  ddgrs::upload(ctx, d_nrad, clusters.radius);
+  const BhTree T{d_right.ptr, d_esc.ptr, d_start.ptr, d_end.ptr, d_leaves.ptr, d_nlo.ptr, d_nhi.ptr, d_nrad.ptr,
+                 (uint32_t)nn, theta * theta};
+  const size_t nl1 = (size_t)nl + 1;
+  l_count.reserve(nl1);
+  e_count.reserve(nl1);
+  d_lumped_off.reserve(nl1);
+  d_exact_off.reserve(nl1);
+  d_pairs.reserve(1);
+  HIPCHECK(hipMemsetAsync(d_pairs.ptr, 0, sizeof(unsigned long long), ctx->stream));
+  launch(ctx, "bhl_count", bhl_count_kernel, dim3(blocks_for(nl1, 256)), dim3(256), 0, T, (uint32_t)nl, l_count.ptr,
+         e_count.ptr, d_pairs.ptr);
+  for (int which = 0; which < 2; ++which) {
+    const uint32_t* in = which ? e_count.ptr : l_count.ptr;
+    uint32_t* out = which ? d_exact_off.ptr : d_lumped_off.ptr;
+    size_t tb = 0;
+    HIPCHECK(rocprim::exclusive_scan(nullptr, tb, in, out, 0u, nl1, rocprim::plus<uint32_t>(), ctx->stream));
+    temp.reserve(tb);
+    tb = temp.n;
+    HIPCHECK(rocprim::exclusive_scan(temp.ptr, tb, in, out, 0u, nl1, rocprim::plus<uint32_t>(), ctx->stream));
+  }
+  struct Totals {
+    uint32_t lumped, exact;
+    unsigned long long pairs;
+  };
+  Totals* tot = (Totals*)ctx->staging(sizeof(Totals));
+  download(ctx, &tot->lumped, (const uint32_t*)d_lumped_off.ptr + nl, 1);
+  download(ctx, &tot->exact, (const uint32_t*)d_exact_off.ptr + nl, 1);
+  download(ctx, &tot->pairs, (const unsigned long long*)d_pairs.ptr, 1);
+  ctx->sync();
+  n_lumped = tot->lumped;
+  n_exact = tot->exact;
+  pairs_exact = tot->pairs;
+  const size_t ml = std::max<size_t>(n_lumped, 1), me = std::max<size_t>(n_exact, 1);
+  d_lumped.reserve(ml);
+  l_idx.reserve(ml);
+  l_key_s.reserve(ml);
+  d_lumped_t.reserve(ml);
+  d_exact.reserve(me);
+  e_tgt.reserve(me);
+  e_src.reserve(me);
+  e_tgt_s.reserve(me);
+  d_exact_t.reserve(me);
+  d_lumped_t_off.reserve((size_t)nn + 1);
+  d_exact_t_off.reserve(nl1);
+  launch(ctx, "bhl_fill", bhl_fill_kernel, dim3(blocks_for(nl, 256)), dim3(256), 0, T, (uint32_t)nl,
+         (const uint32_t*)d_leaf_ordinal.ptr, (const uint32_t*)d_lumped_off.ptr, (const uint32_t*)d_exact_off.ptr,
+         d_lumped.ptr, l_idx.ptr, d_exact.ptr, e_tgt.ptr, e_src.ptr);
+  // transposes: entries grouped by target, in entry order within a target
+  auto sort = [&](const uint32_t* ki, uint32_t* ko, const uint32_t* vi, uint32_t* vo, size_t n, int bits) {
+    size_t tb = 0;
+    HIPCHECK(rocprim::radix_sort_pairs(nullptr, tb, ki, ko, vi, vo, n, 0, bits, ctx->stream));
+    temp.reserve(tb);
+    tb = temp.n;
+    HIPCHECK(rocprim::radix_sort_pairs(temp.ptr, tb, ki, ko, vi, vo, n, 0, bits, ctx->stream));
+  };
+  if (n_lumped) sort(d_lumped.ptr, l_key_s.ptr, l_idx.ptr, d_lumped_t.ptr, n_lumped, bits_for((uint64_t)nn));
+  launch(ctx, "bhl_ptr", bhl_ptr_kernel, dim3(blocks_for((int64_t)nn + 1, 256)), dim3(256), 0, (uint32_t)nn,
+         (uint32_t)n_lumped, (const uint32_t*)l_key_s.ptr, d_lumped_t_off.ptr);
+  if (n_exact) sort(e_tgt.ptr, e_tgt_s.ptr, e_src.ptr, d_exact_t.ptr, n_exact, bits_for((uint64_t)nl));
+  launch(ctx, "bhl_ptr", bhl_ptr_kernel, dim3(blocks_for((int64_t)nl + 1, 256)), dim3(256), 0, (uint32_t)nl,
+         (uint32_t)n_exact, (const uint32_t*)e_tgt_s.ptr, d_exact_t_off.ptr);
+}
+
+}  // namespace ddgrs

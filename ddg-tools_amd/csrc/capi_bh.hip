@@ -89,11 +89,12 @@ int ddgrs_bh_counts(ddgrs_bh* h, int64_t* out, ddgrs_error* err) {
     const auto& ev = h->ev;
     out[0] = (int64_t)ev.topo.n_nodes();
     out[1] = (int64_t)ev.topo.leaves.size();
-    out[2] = (int64_t)ev.lists.lumped.size();
-    out[3] = (int64_t)ev.lists.exact.size();
-    out[4] = (int64_t)ev.lists.pairs_exact;
+    out[2] = (int64_t)ev.n_lumped;
+    out[3] = (int64_t)ev.n_exact;
+    out[4] = (int64_t)ev.pairs_exact;
+    const BhLists L = ev.download_lists(h->s->ctx);
     int64_t slots = 0;
-    for (uint32_t a : ev.lists.exact) slots += ev.topo.end[a] - ev.topo.start[a];
+    for (uint32_t a : L.exact) slots += ev.topo.end[a] - ev.topo.start[a];
     out[5] = slots;
   });
 }
@@ -126,7 +127,7 @@ int ddgrs_bh_lists(ddgrs_bh* h, uint32_t* lumped_off, uint32_t* lumped, uint32_t
                    ddgrs_error* err) {
   return guarded(err, "bh_lists", [&] {
     need(h, "bh");
-    const auto& L = h->ev.lists;
+    const BhLists L = h->ev.download_lists(h->s->ctx);
     if (lumped_off) std::copy(L.lumped_off.begin(), L.lumped_off.end(), lumped_off);
     if (lumped) std::copy(L.lumped.begin(), L.lumped.end(), lumped);
     if (exact_off) std::copy(L.exact_off.begin(), L.exact_off.end(), exact_off);

@@ -8,6 +8,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <future>
 
@@ -172,19 +174,33 @@ void Session::step(StepReport* rep) {
   // 1. trees on the device; L′'s factorization is enqueued before the host builds the Barnes–Hut
   // lists, so the two overlap
   auto t = std::chrono::steady_clock::now();
+  // DDGRS_TRACE_STEP=1: wall times of the tree stage's parts on stderr
+  static const bool trace = std::getenv("DDGRS_TRACE_STEP") != nullptr;
+  auto t_part = t;
+  auto mark = [&](const char* what) {
+    if (!trace) return;
+    std::fprintf(stderr, "step: %-18s %8.2f ms\n", what, seconds_since(t_part) * 1e3);
+    t_part = std::chrono::steady_clock::now();
+  };
   compute_face_geometry(s, s->packs, true);
   const HostGeometry geo = download_geometry(s);
   const double s_exp = (exps.beta64 - 2.0) / exps.alpha64;
+  mark("geometry");
   Topology vtopo = build_vertex_tree(s, opts.leaf_size, vert_bvh);
+  mark("vertex tree");
   Topology topo;
   Clusters clusters;
   build_face_tree(s, geo, opts.leaf_size, face_bvh, &topo, &clusters);
+  mark("face tree");
   if (!lp_ready) {
     pc.setup_device(ctx, lp_symbolic.get());
     lp_ready = true;
+    mark("L' set-up");
   }
   pc.factor_device(s, s_exp, opts.w_tp, opts.eps_rel);
+  mark("L' enqueue");
   if (!opts.energy_exact) bh.build_from(s, topo, clusters, opts.theta, opts.leaf_size);
+  mark("Barnes-Hut lists");
   rep->seconds_trees = seconds_since(t);
   t = std::chrono::steady_clock::now();
   if (opts.energy_exact) {
@@ -193,8 +209,8 @@ void Session::step(StepReport* rep) {
     rep->pairs_exact = (int64_t)nf * (nf - 1);
   } else {
     bh.gradient(s, exps, scal.ptr, partials.ptr);
-    rep->pairs_exact = (int64_t)bh.lists.pairs_exact;
-    rep->nodes_lumped = (int64_t)bh.lists.lumped.size();
+    rep->pairs_exact = (int64_t)bh.pairs_exact;
+    rep->nodes_lumped = (int64_t)bh.n_lumped;
   }
   pull_back(s, partials.ptr, g.ptr);
   HIPCHECK(hipMemsetAsync(g.ptr + nd, 0, 3 * sizeof(double), ctx->stream));

@@ -81,3 +81,21 @@ def test_bumpy_l6(pkg, ctx):
     print(f"L6 theta 0.5: E rel {abs(e - e0) / e0:.2e}, dE max-rel {rel_max(d, d0):.2e}")
     assert abs(e - e0) <= 1e-5 * e0
     assert rel_max(d, d0) <= 1e-4
+
+
+def test_device_lists_equal_host_lists(pkg, ctx, monkeypatch):
+    """Lists and transposes built on the device give bitwise the evaluation of the host-built ones."""
+    p, f = pkg.generate.config_torus_16k()
+    s = pkg.Surface(ctx, f, p)
+    dev = pkg.BhTree(s, pkg.BhOptions(0.5, 8))
+    monkeypatch.setenv("DDGRS_BH_LISTS_ON_HOST", "1")
+    host = pkg.BhTree(s, pkg.BhOptions(0.5, 8))
+    monkeypatch.delenv("DDGRS_BH_LISTS_ON_HOST")
+    assert dev.counts() == host.counts()
+    Ld, Lh = dev.lists(), host.lists()
+    for k in Ld:
+        assert np.array_equal(Ld[k], Lh[k]), k
+    e, d = dev.energy_and_differential()
+    eh, dh = host.energy_and_differential()
+    assert e == eh
+    assert np.array_equal(d, dh)
