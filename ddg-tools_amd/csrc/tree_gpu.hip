@@ -13,6 +13,8 @@ namespace ddgrs {
 
 namespace {
 
+constexpr uint32_t kSplitChunk = 2048;  // elements per workgroup in the chunked split
+
 // Total order key of a double (the order of f64::total_cmp), as an unsigned integer.
 __device__ __forceinline__ uint64_t total_key_u(double x) {
   const int64_t b = __double_as_longlong(x);
@@ -196,6 +198,165 @@ __global__ __launch_bounds__(256) void bvh_split_kernel(const uint32_t* __restri
   if (t == 0) split[a] = red_k[0] == UINT32_MAX ? cnt / 2 : red_k[0];
 }
 
+// ---- chunked split for large nodes: several workgroups per node ------------------------------
+// A node's elements are cut into chunks of kSplitChunk; the chunk totals are scanned per node, then
+// every chunk writes its suffix boxes (one launch) and evaluates its splits (the next launch: a
+// split's right box may lie in the next chunk), and the chunks' first minima are reduced per node.
+// Box unions are exact, so the boxes, costs and the chosen split equal the one-workgroup kernel's.
+
+__global__ __launch_bounds__(256) void bvh_chunk_total_kernel(const uint32_t* __restrict__ ch_seg,
+                                                              const uint32_t* __restrict__ ch_c0,
+                                                              const uint32_t* __restrict__ act_start,
+                                                              const uint32_t* __restrict__ act_end,
+                                                              const uint32_t* __restrict__ order,
+                                                              const double* __restrict__ lo,
+                                                              const double* __restrict__ hi, Box6* __restrict__ ctot) {
+  __shared__ Box6 wave_tot[4];
+  const uint32_t c = blockIdx.x, a = ch_seg[c];
+  const uint32_t start = act_start[a], cnt = act_end[a] - start;
+  const uint32_t c0 = ch_c0[c], c1 = min(c0 + kSplitChunk, cnt);
+  Box6 v = empty6();
+  for (uint32_t p = c0 + threadIdx.x; p < c1; p += 256) v = unite(v, load_box(lo, hi, order[start + p]));
+  Box6 tot;
+  (void)block_scan(v, wave_tot, &tot);
+  if (threadIdx.x == 0) ctot[c] = tot;
+}
+
+// Per chunk: the union of the node's earlier chunks (cpre) and of its later chunks (csuf).
+__global__ __launch_bounds__(256) void bvh_chunk_scan_kernel(uint32_t nch, const uint32_t* __restrict__ ch_seg,
+                                                             const uint32_t* __restrict__ seg_chunk0,
+                                                             const Box6* __restrict__ ctot, Box6* __restrict__ cpre,
+                                                             Box6* __restrict__ csuf) {
+  const uint32_t c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= nch) return;
+  const uint32_t a = ch_seg[c], f0 = seg_chunk0[a], f1 = seg_chunk0[a + 1];
+  Box6 pre = empty6(), suf = empty6();
+  for (uint32_t j = f0; j < c; ++j) pre = unite(pre, ctot[j]);
+  for (uint32_t j = c + 1; j < f1; ++j) suf = unite(suf, ctot[j]);
+  cpre[c] = pre;
+  csuf[c] = suf;
+}
+
+__global__ __launch_bounds__(256) void bvh_chunk_suffix_kernel(const uint32_t* __restrict__ ch_seg,
+                                                               const uint32_t* __restrict__ ch_c0,
+                                                               const uint32_t* __restrict__ act_start,
+                                                               const uint32_t* __restrict__ act_end,
+                                                               const uint32_t* __restrict__ order,
+                                                               const double* __restrict__ lo,
+                                                               const double* __restrict__ hi,
+                                                               const Box6* __restrict__ csuf, double* __restrict__ suf) {
+  __shared__ Box6 wave_tot[4];
+  const uint32_t c = blockIdx.x, a = ch_seg[c], t = threadIdx.x;
+  const uint32_t start = act_start[a], cnt = act_end[a] - start;
+  const uint32_t c0 = ch_c0[c], c1 = min(c0 + kSplitChunk, cnt);
+  const uint32_t* sl = order + start;
+  double* sf = suf + 6 * (size_t)start;
+  Box6 carry = csuf[c];
+  for (int64_t e1 = c1; e1 > (int64_t)c0; e1 -= 256) {
+    const int64_t e0 = e1 - 256 > (int64_t)c0 ? e1 - 256 : (int64_t)c0;
+    const int64_t p = e1 - 1 - (int64_t)t;
+    Box6 v = p >= e0 ? load_box(lo, hi, sl[p]) : empty6();
+    Box6 tot;
+    v = block_scan(v, wave_tot, &tot);
+    v = unite(v, carry);
+    if (p >= e0) {
+      double* o = sf + 6 * p;
+      o[0] = v.l0;
+      o[1] = v.l1;
+      o[2] = v.l2;
+      o[3] = v.h0;
+      o[4] = v.h1;
+      o[5] = v.h2;
+    }
+    carry = unite(tot, carry);
+  }
+}
+
+struct SplitBest {
+  double cost;
+  uint32_t k;
+};
+
+__global__ __launch_bounds__(256) void bvh_chunk_cost_kernel(const uint32_t* __restrict__ ch_seg,
+                                                             const uint32_t* __restrict__ ch_c0,
+                                                             const uint32_t* __restrict__ act_start,
+                                                             const uint32_t* __restrict__ act_end,
+                                                             const uint32_t* __restrict__ order,
+                                                             const double* __restrict__ lo,
+                                                             const double* __restrict__ hi,
+                                                             const Box6* __restrict__ cpre,
+                                                             const double* __restrict__ suf,
+                                                             SplitBest* __restrict__ cbest) {
+  __shared__ Box6 wave_tot[4];
+  __shared__ double red_c[256];
+  __shared__ uint32_t red_k[256];
+  const uint32_t c = blockIdx.x, a = ch_seg[c], t = threadIdx.x;
+  const uint32_t start = act_start[a], cnt = act_end[a] - start;
+  const uint32_t c0 = ch_c0[c], c1 = min(c0 + kSplitChunk, cnt);
+  const uint32_t* sl = order + start;
+  const double* sf = suf + 6 * (size_t)start;
+  size_t low = (size_t)ceil(0.25 * (double)cnt);
+  if (low < 1) low = 1;
+  if (low > cnt / 2) low = cnt / 2;
+  double best = INFINITY;
+  uint32_t best_k = UINT32_MAX;
+  Box6 carry = cpre[c];
+  for (uint32_t e0 = c0; e0 < c1; e0 += 256) {
+    const uint32_t p = e0 + t;
+    Box6 v = p < c1 ? load_box(lo, hi, sl[p]) : empty6();
+    Box6 tot;
+    v = block_scan(v, wave_tot, &tot);
+    v = unite(carry, v);
+    const size_t k = (size_t)p + 1;
+    if (p < c1 && k >= low && k <= cnt - low) {
+      const double* o = sf + 6 * k;
+      const Box6 r = {o[0], o[1], o[2], o[3], o[4], o[5]};
+      const double cost = diam2(v) + diam2(r);
+      if (cost < best) {
+        best = cost;
+        best_k = (uint32_t)k;
+      }
+    }
+    carry = unite(carry, tot);
+  }
+  red_c[t] = best;
+  red_k[t] = best_k;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)t < w) {
+      const double c2 = red_c[t + w];
+      const uint32_t k2 = red_k[t + w];
+      if (k2 != UINT32_MAX && (red_k[t] == UINT32_MAX || c2 < red_c[t] || (c2 == red_c[t] && k2 < red_k[t]))) {
+        red_c[t] = c2;
+        red_k[t] = k2;
+      }
+    }
+    __syncthreads();
+  }
+  if (t == 0) cbest[c] = {red_c[0], red_k[0]};
+}
+
+// Per node: the first minimum over its chunks (chunks in increasing k).
+__global__ __launch_bounds__(256) void bvh_chunk_pick_kernel(uint32_t na, const uint32_t* __restrict__ seg_chunk0,
+                                                             const uint32_t* __restrict__ act_start,
+                                                             const uint32_t* __restrict__ act_end,
+                                                             const SplitBest* __restrict__ cbest,
+                                                             uint32_t* __restrict__ split) {
+  const uint32_t a = blockIdx.x * 256 + threadIdx.x;
+  if (a >= na) return;
+  double best = INFINITY;
+  uint32_t best_k = UINT32_MAX;
+  for (uint32_t j = seg_chunk0[a]; j < seg_chunk0[a + 1]; ++j) {
+    const SplitBest b = cbest[j];
+    if (b.k != UINT32_MAX && (best_k == UINT32_MAX || b.cost < best)) {
+      best = b.cost;
+      best_k = b.k;
+    }
+  }
+  const uint32_t cnt = act_end[a] - act_start[a];
+  split[a] = best_k == UINT32_MAX ? cnt / 2 : best_k;
+}
+
 Topology GpuBvhBuilder::build(ddgrs_ctx* ctx, size_t n, const double* pts, const double* lo, const double* hi,
                               size_t leaf_size) {
   const uint32_t un = (uint32_t)n;
@@ -264,8 +425,47 @@ Topology GpuBvhBuilder::build(ddgrs_ctx* ctx, size_t n, const double* pts, const
     launch(ctx, "bvh_scatter", bvh_scatter_kernel, dim3(blocks_for(total, 256)), dim3(256), 0, total,
            (const uint32_t*)part_key_out.ptr, (const uint32_t*)part_val_out.ptr, (const uint32_t*)act_start.ptr,
            (const uint32_t*)act_prefix.ptr, order.ptr);
-    launch(ctx, "bvh_split", bvh_split_kernel, dim3(na), dim3(256), 0, (const uint32_t*)act_start.ptr,
-           (const uint32_t*)act_end.ptr, (const uint32_t*)order.ptr, lo, hi, suf.ptr, act_split.ptr);
+    uint32_t longest = 0;
+    for (uint32_t i = 0; i < na; ++i) longest = std::max(longest, a_end[i] - a_start[i]);
+    if (longest <= kSplitChunk) {
+      launch(ctx, "bvh_split", bvh_split_kernel, dim3(na), dim3(256), 0, (const uint32_t*)act_start.ptr,
+             (const uint32_t*)act_end.ptr, (const uint32_t*)order.ptr, lo, hi, suf.ptr, act_split.ptr);
+    } else {
+      c_seg.clear();
+      c_c0.clear();
+      c_first.assign(1, 0);
+      for (uint32_t i = 0; i < na; ++i) {
+        for (uint32_t c0 = 0; c0 < a_end[i] - a_start[i]; c0 += kSplitChunk) {
+          c_seg.push_back(i);
+          c_c0.push_back(c0);
+        }
+        c_first.push_back((uint32_t)c_seg.size());
+      }
+      const uint32_t nch = (uint32_t)c_seg.size();
+      upload(ctx, ch_seg, c_seg);
+      upload(ctx, ch_c0, c_c0);
+      upload(ctx, seg_chunk0, c_first);
+      static_assert(sizeof(Box6) == 6 * sizeof(double) && sizeof(SplitBest) == 2 * sizeof(uint64_t), "scratch layout");
+      ch_box.reserve(18 * (size_t)nch);
+      ch_best.reserve(2 * (size_t)nch);
+      Box6* ctot = (Box6*)ch_box.ptr;
+      Box6* cpre = ctot + nch;
+      Box6* csuf = cpre + nch;
+      launch(ctx, "bvh_chunk_total", bvh_chunk_total_kernel, dim3(nch), dim3(256), 0, (const uint32_t*)ch_seg.ptr,
+             (const uint32_t*)ch_c0.ptr, (const uint32_t*)act_start.ptr, (const uint32_t*)act_end.ptr,
+             (const uint32_t*)order.ptr, lo, hi, ctot);
+      launch(ctx, "bvh_chunk_scan", bvh_chunk_scan_kernel, dim3(blocks_for(nch, 256)), dim3(256), 0, nch,
+             (const uint32_t*)ch_seg.ptr, (const uint32_t*)seg_chunk0.ptr, (const Box6*)ctot, cpre, csuf);
+      launch(ctx, "bvh_chunk_suffix", bvh_chunk_suffix_kernel, dim3(nch), dim3(256), 0, (const uint32_t*)ch_seg.ptr,
+             (const uint32_t*)ch_c0.ptr, (const uint32_t*)act_start.ptr, (const uint32_t*)act_end.ptr,
+             (const uint32_t*)order.ptr, lo, hi, (const Box6*)csuf, suf.ptr);
+      launch(ctx, "bvh_chunk_cost", bvh_chunk_cost_kernel, dim3(nch), dim3(256), 0, (const uint32_t*)ch_seg.ptr,
+             (const uint32_t*)ch_c0.ptr, (const uint32_t*)act_start.ptr, (const uint32_t*)act_end.ptr,
+             (const uint32_t*)order.ptr, lo, hi, (const Box6*)cpre, (const double*)suf.ptr, (SplitBest*)ch_best.ptr);
+      launch(ctx, "bvh_chunk_pick", bvh_chunk_pick_kernel, dim3(blocks_for(na, 256)), dim3(256), 0, na,
+             (const uint32_t*)seg_chunk0.ptr, (const uint32_t*)act_start.ptr, (const uint32_t*)act_end.ptr,
+             (const SplitBest*)ch_best.ptr, act_split.ptr);
+    }
     a_split.resize(na);
     download(ctx, a_split.data(), (const uint32_t*)act_split.ptr, na);
     ctx->sync();

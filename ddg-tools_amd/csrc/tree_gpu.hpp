@@ -24,6 +24,11 @@ struct GpuBvhBuilder {
   DevBuf<double> suf;  // 6 per element: suffix box lo, hi
   DevBuf<double> box_lo, box_hi;  // element boxes when the caller has none on the device
   DevBuf<uint8_t> temp;
+  // chunked split of large nodes: per chunk its node and first element, per node its first chunk
+  DevBuf<uint32_t> ch_seg, ch_c0, seg_chunk0;
+  DevBuf<double> ch_box;    // per chunk 3 boxes of 6 doubles: total, earlier chunks, later chunks
+  DevBuf<uint64_t> ch_best; // per chunk (cost, split): 2 words
+  std::vector<uint32_t> c_seg, c_c0, c_first;
   // Builds over n elements with centres pts and boxes [lo, hi] (3 FP64 each, device pointers).
   Topology build(ddgrs_ctx* ctx, size_t n, const double* pts, const double* lo, const double* hi, size_t leaf_size);
 };
