@@ -180,6 +180,10 @@ def main() -> int:
     ctx.set_profiling(False)
     ses.close()
     device_ms = sum(v["ms"] for v in kt.values())
+    launches = sum(v["launches"] for v in kt.values())
+    print(f"bench: profiled step: {launches} launches, {device_ms:.2f} ms of kernels", file=sys.stderr)
+    for k, v in sorted(kt.items(), key=lambda kv: -kv[1]["ms"]):
+        print(f"bench:   {k:24s} {v['launches']:6d} launches {v['ms']:9.3f} ms", file=sys.stderr)
 
     rooflines = {}
     g = kt.get("bh_gradient")

@@ -241,13 +241,13 @@ LprimeSymbolic lprime_symbolic(const std::vector<uint32_t>& faces, size_t nv, co
     }
     for (uint32_t c0 = 0; c0 < kmax; c0 += kPanelCols) {
       S.step_c0.push_back(c0);
-      S.panel_off.push_back((uint32_t)S.panel_blocks.size());
+      S.panel_off.push_back((uint32_t)S.panels.size());
       S.tile_off.push_back((uint32_t)S.tiles.size());
       for (uint32_t b : big_h[h]) {
         const uint32_t k = S.bk[b], f = S.bf[b];
         if (k <= c0) continue;
-        S.panel_blocks.push_back(b);
         const uint32_t c1 = std::min(c0 + kPanelCols, k);
+        for (uint32_t r0 = 0; r0 == 0 || c1 + r0 < f; r0 += kPanelItemRows) S.panels.push_back({b, r0});
         const uint32_t T = (f - c1 + kTile - 1) / kTile;
         for (uint32_t I = 0; I < T; ++I)
           for (uint32_t J = 0; J <= I; ++J) S.tiles.push_back({b, c1 + kTile * I, c1 + kTile * J});
@@ -255,7 +255,7 @@ LprimeSymbolic lprime_symbolic(const std::vector<uint32_t>& faces, size_t nv, co
     }
     S.step_off.push_back((uint32_t)S.step_c0.size());
   }
-  S.panel_off.push_back((uint32_t)S.panel_blocks.size());
+  S.panel_off.push_back((uint32_t)S.panels.size());
   S.tile_off.push_back((uint32_t)S.tiles.size());
   S.max_children = MC;
   S.ea_item_off.assign(1, 0);
@@ -439,32 +439,85 @@ __global__ __launch_bounds__(256) void lf_extend_add_kernel(FrontArgs A, const F
   }
 }
 
-// One workgroup per large front: eliminate columns [c0, c1) (rows below in the panel updated).
-__global__ __launch_bounds__(256) void lf_panel_kernel(FrontArgs A, const uint32_t* __restrict__ list, uint32_t c0,
-                                                       double* __restrict__ front, uint32_t* __restrict__ bad) {
+// Diagonal block of panel [c0, c1) of a large front, factored in place: D = L_D diag(d) L_Dᵀ
+// (one wave, one row per lane; items with r0 ≠ 0 have nothing to do).
+__global__ __launch_bounds__(64) void lf_panel_diag_kernel(FrontArgs A, const FrontPanel* __restrict__ items,
+                                                           uint32_t c0, double* __restrict__ front,
+                                                           uint32_t* __restrict__ bad) {
+  __shared__ double Ds[kPanelCols][kPanelCols + 1];
   __shared__ double wv[kPanelCols];
-  const uint32_t b = list[blockIdx.x];
-  const uint32_t f = A.bf[b], k = A.bk[b], j0 = A.bstart[b];
-  const uint32_t c1 = min(c0 + kPanelCols, k);
+  const FrontPanel it = items[blockIdx.x];
+  if (it.r0 != 0) return;
+  const uint32_t b = it.block, f = A.bf[b], k = A.bk[b], j0 = A.bstart[b];
+  const uint32_t c1 = min(c0 + kPanelCols, k), nbk = c1 - c0;
   double* G = front + A.foff[b];
-  for (uint32_t c = c0; c < c1; ++c) {
-    const double d = G[(size_t)c * f + c];
-    if (threadIdx.x == 0 && !(d != 0.0 && isfinite(d))) atomicMin(bad, j0 + c);
-    for (uint32_t r = c + 1 + threadIdx.x; r < f; r += 256) {
-      const double w = G[(size_t)r * f + c];
-      if (r < c1) wv[r - c0] = w;
-      G[(size_t)r * f + c] = w / d;
+  const uint32_t lane = threadIdx.x;
+  if (lane < nbk)
+    for (uint32_t j = 0; j <= lane; ++j) Ds[lane][j] = G[(size_t)(c0 + lane) * f + c0 + j];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  for (uint32_t t = 0; t < nbk; ++t) {
+    const double d = Ds[t][t];
+    if (lane > t && lane < nbk) {
+      const double w = Ds[lane][t];
+      wv[lane] = w;
+      Ds[lane][t] = w / d;
     }
-    __syncthreads();
-    const uint32_t nc = c1 - c - 1;
-    if (nc > 0) {
-      const uint32_t total = (f - c - 1) * nc;
-      for (uint32_t e = threadIdx.x; e < total; e += 256) {
-        const uint32_t r = c + 1 + e / nc, m = c + 1 + e % nc;
-        if (m <= r) G[(size_t)r * f + m] = fma(-G[(size_t)r * f + c], wv[m - c0], G[(size_t)r * f + m]);
-      }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (lane > t && lane < nbk) {
+      const double l = Ds[lane][t];
+      for (uint32_t m = t + 1; m <= lane; ++m) Ds[lane][m] = fma(-l, wv[m], Ds[lane][m]);
     }
-    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+  if (lane < nbk) {
+    for (uint32_t j = 0; j <= lane; ++j) G[(size_t)(c0 + lane) * f + c0 + j] = Ds[lane][j];
+    const double d = Ds[lane][lane];
+    if (!(d != 0.0 && isfinite(d))) atomicMin(bad, j0 + c0 + lane);
+  }
+}
+
+// Rows below the diagonal block of panel [c0, c1) (the block factored): one thread per row r,
+// w_t = a_rt − Σ_{s<t} w_s (L_D)_ts, l_rt = w_t / d_t; the rows are independent.
+__global__ __launch_bounds__(kPanelItemRows) void lf_panel_kernel(FrontArgs A, const FrontPanel* __restrict__ items,
+                                                                  uint32_t c0, double* __restrict__ front) {
+  __shared__ double Ds[kPanelCols][kPanelCols + 1];
+  __shared__ double R[kPanelItemRows][kPanelCols + 1];
+  const FrontPanel it = items[blockIdx.x];
+  const uint32_t b = it.block, f = A.bf[b], k = A.bk[b];
+  const uint32_t c1 = min(c0 + kPanelCols, k), nbk = c1 - c0;
+  const uint32_t rbase = c1 + it.r0;
+  if (rbase >= f) return;  // uniform per workgroup
+  const uint32_t nrows = min(kPanelItemRows, f - rbase);
+  double* G = front + A.foff[b];
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t e = tid; e < kPanelCols * kPanelCols; e += kPanelItemRows) {
+    const uint32_t i = e / kPanelCols, j = e % kPanelCols;
+    Ds[i][j] = (i < nbk && j <= i) ? G[(size_t)(c0 + i) * f + c0 + j] : (i == j ? 1.0 : 0.0);
+  }
+  for (uint32_t e = tid; e < nrows * kPanelCols; e += kPanelItemRows) {
+    const uint32_t i = e / kPanelCols, t = e % kPanelCols;
+    R[i][t] = t < nbk ? G[(size_t)(rbase + i) * f + c0 + t] : 0.0;
+  }
+  __syncthreads();
+  if (tid < nrows) {
+    double w[kPanelCols];
+#pragma unroll
+    for (uint32_t t = 0; t < kPanelCols; ++t) {
+      double x = R[tid][t];
+#pragma unroll
+      for (uint32_t s2 = 0; s2 < t; ++s2) x = fma(-w[s2], Ds[t][s2], x);
+      w[t] = x;
+    }
+#pragma unroll
+    for (uint32_t t = 0; t < kPanelCols; ++t) R[tid][t] = w[t] / Ds[t][t];
+  }
+  __syncthreads();
+  for (uint32_t e = tid; e < nrows * kPanelCols; e += kPanelItemRows) {
+    const uint32_t i = e / kPanelCols, t = e % kPanelCols;
+    if (t < nbk) G[(size_t)(rbase + i) * f + c0 + t] = R[i][t];
   }
 }
 
@@ -592,7 +645,7 @@ void LprimeFactorDev::setup(ddgrs_ctx* ctx, const LprimeSymbolic& S, LprimeDev& 
   ddgrs::upload(ctx, ea_off, S.ea_off);
   ddgrs::upload(ctx, ea_map, S.ea_map);
   ddgrs::upload(ctx, lds_blocks, S.lds_blocks);
-  ddgrs::upload(ctx, panel_blocks, S.panel_blocks);
+  ddgrs::upload(ctx, panels, S.panels);
   ddgrs::upload(ctx, foff, S.foff);
   ddgrs::upload(ctx, amap_k, S.amap_k);
   ddgrs::upload(ctx, amap_c, S.amap_c);
@@ -657,8 +710,10 @@ void LprimeFactorDev::factor(ddgrs_surface* s, double eps_rel, LprimeDev& lp) {
              (const FrontItem*)ea_items.ptr + ea_item_off[q], front.ptr);
     }
     for (uint32_t st = step_off[h]; st < step_off[h + 1]; ++st) {
-      launch(ctx, "lf_panel", lf_panel_kernel, dim3(panel_off[st + 1] - panel_off[st]), dim3(256), 0, A,
-             (const uint32_t*)panel_blocks.ptr + panel_off[st], step_c0[st], front.ptr, bad.ptr);
+      const uint32_t np = panel_off[st + 1] - panel_off[st];
+      const FrontPanel* items = (const FrontPanel*)panels.ptr + panel_off[st];
+      launch(ctx, "lf_panel_diag", lf_panel_diag_kernel, dim3(np), dim3(64), 0, A, items, step_c0[st], front.ptr, bad.ptr);
+      launch(ctx, "lf_panel", lf_panel_kernel, dim3(np), dim3(kPanelItemRows), 0, A, items, step_c0[st], front.ptr);
       launch(ctx, "lf_update", lf_update_kernel, dim3(tile_off[st + 1] - tile_off[st]), dim3(256), 0, A,
              (const FrontTile*)tiles.ptr + tile_off[st], step_c0[st], front.ptr);
     }

@@ -14,12 +14,17 @@ namespace ddgrs {
 constexpr uint32_t kPanelCols = 32;
 // Rows and columns of an update tile.
 constexpr uint32_t kTile = 64;
+// Rows below a panel's diagonal block solved by one workgroup (one row per thread).
+constexpr uint32_t kPanelItemRows = 128;
 
 struct FrontItem {  // rows [r0, r1) of a child's update matrix, added into its parent's front
   uint32_t child, r0, r1;
 };
 struct FrontTile {  // trailing-update tile of a large front: rows from i0, columns from j0
   uint32_t block, i0, j0;
+};
+struct FrontPanel {  // a panel's rows [c1 + r0, c1 + r0 + kPanelItemRows) below its diagonal block
+  uint32_t block, r0;
 };
 
 struct LprimeSymbolic {
@@ -45,7 +50,8 @@ struct LprimeSymbolic {
   std::vector<FrontItem> ea_items;             // large fronts' extend-add work
   std::vector<uint32_t> ea_item_off;           // per (height, child slot): H × max_children + 1
   uint32_t max_children = 0;
-  std::vector<uint32_t> panel_blocks, tile_off, panel_off, step_c0, step_off;  // step_off: per height
+  std::vector<FrontPanel> panels;
+  std::vector<uint32_t> tile_off, panel_off, step_c0, step_off;  // step_off: per height
   std::vector<FrontTile> tiles;
   static constexpr uint64_t kNoPos = ~0ull;
 };
@@ -59,7 +65,8 @@ struct LprimeFactorDev {
   uint32_t H = 0, max_children = 0, max_rows = 0;
   uint64_t front_total = 0;
   std::vector<uint32_t> lds_off, lds_rows, ea_item_off, step_off, step_c0, panel_off, tile_off;
-  DevBuf<uint32_t> kp, kj, kdiag, bstart, bk, bf, bparent, child_ptr, child, ea_off, ea_map, lds_blocks, panel_blocks;
+  DevBuf<uint32_t> kp, kj, kdiag, bstart, bk, bf, bparent, child_ptr, child, ea_off, ea_map, lds_blocks;
+  DevBuf<FrontPanel> panels;
   DevBuf<uint64_t> foff, amap_k, amap_c, gmap_r, gmap_c, dpos;
   DevBuf<FrontItem> ea_items;
   DevBuf<FrontTile> tiles;
