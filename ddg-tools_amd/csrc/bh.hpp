@@ -29,7 +29,8 @@ struct BhEval {
   DevBuf<double4> sums;
   DevBuf<float> part;
   // device list build
-  DevBuf<uint32_t> d_esc, d_leaf_ordinal, l_count, e_count, l_idx, l_key_s, e_tgt, e_src, e_tgt_s;
+  DevBuf<uint32_t> d_esc, d_leaf_ordinal, d_parent, d_task_root, d_task_depth, l_count, e_count, l_task_off, e_task_off;
+  DevBuf<uint32_t> l_idx, l_key_s, e_tgt, e_src, e_tgt_s;
   DevBuf<double> d_nlo, d_nhi, d_nrad;
   DevBuf<unsigned long long> d_pairs;
   DevBuf<uint8_t> temp;

@@ -4,10 +4,17 @@
 // A is lumped when max(r_L, r_A)² < θ² dist²(box_L, box_A), an exact leaf when it is a leaf, and
 // opened otherwise.  With the nodes numbered in pre-order the walk visits nodes in increasing
 // order, so it is done here without a stack: after a lumped node or a leaf the walk continues at
-// the node after A's subtree (its escape index), after an opened node at A + 1.  One thread per
-// leaf walks twice (counts, then the lists after a scan); the transposes are stable radix sorts of
-// the entries by target.  The admissibility test uses the host's FP64 operations without
-// contraction, so the lists are the host build's.
+// the node after A's subtree (its escape index), after an opened node at A + 1.
+//
+// A leaf's walk is split into tasks, one per root of a cut of the tree (the nodes at depth D and
+// the shallower leaves, in pre-order): a task first tests the ancestors of its root from the top;
+// if one of them is lumped or a leaf, the walk never enters the root's subtree and the task emits
+// that ancestor only if its root is the first cut node below it (reached from it by left children
+// alone); otherwise the task walks its root's subtree.  The tasks of a leaf, in pre-order of their
+// roots, emit exactly the walk's sequence, so counts scanned in (leaf, task) order place every
+// entry.  One thread per (leaf, task) walks twice (counts, then the entries); the transposes are
+// stable radix sorts of the entries by target.  The admissibility test uses the host's FP64
+// operations without contraction, so the lists are the host build's.
 #include <cstring>
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
@@ -39,34 +46,74 @@ __device__ __forceinline__ bool bh_lumped(const BhTree& T, uint32_t l, uint32_t 
 }
 }  // namespace
 
-// Per leaf ordinal k < nl: lumped and exact counts, and Σ over exact nodes of the ordered pairs;
-// zeros at k = nl (the scans end with the totals).
-__global__ __launch_bounds__(256) void bhl_count_kernel(BhTree T, uint32_t nl, uint32_t* __restrict__ lcount,
+constexpr int kMaxCutDepth = 5;
+
+struct BhTasks {
+  const uint32_t* root;   // per task: its cut node
+  const uint32_t* depth;  // per task: the cut node's depth
+  const uint32_t* parent;
+  uint32_t count;
+};
+
+// Walks task j of leaf k, calling emit(node, lumped) for each entry in order.
+template <class Emit>
+__device__ __forceinline__ void bh_task_walk(const BhTree& T, const BhTasks& W, uint32_t k, uint32_t j, Emit&& emit) {
+  const uint32_t l = T.leaves[k], c = W.root[j], m = W.depth[j];
+  uint32_t path[kMaxCutDepth];
+  {
+    uint32_t x = c;
+    for (int i = (int)m - 1; i >= 0; --i) {
+      x = W.parent[x];
+      path[i] = x;
+    }
+  }
+  for (uint32_t i = 0; i < m; ++i) {
+    const uint32_t a = path[i];
+    const bool lum = bh_lumped(T, l, a);
+    if (lum || T.right[a] == 0) {
+      if (c == a + (m - i)) emit(a, lum);
+      return;
+    }
+  }
+  const uint32_t stop = T.esc[c];
+  for (uint32_t a = c; a < stop;) {
+    if (bh_lumped(T, l, a)) {
+      emit(a, true);
+      a = T.esc[a];
+    } else if (T.right[a] == 0) {
+      emit(a, false);
+      a = T.esc[a];
+    } else {
+      a = a + 1;
+    }
+  }
+}
+
+// Per (leaf, task) q = k·tasks + j < nl·tasks: lumped and exact counts, and Σ over exact nodes of
+// the ordered pairs; zeros at q = nl·tasks (the scans end with the totals).
+__global__ __launch_bounds__(256) void bhl_count_kernel(BhTree T, BhTasks W, uint32_t nl, uint32_t* __restrict__ lcount,
                                                         uint32_t* __restrict__ ecount,
                                                         unsigned long long* __restrict__ pairs) {
   __shared__ unsigned long long red[256];
-  const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t q = blockIdx.x * 256 + threadIdx.x, nq = nl * W.count;
   unsigned long long pk = 0;
-  if (k < nl) {
+  if (q < nq) {
+    const uint32_t k = q / W.count, j = q % W.count;
     const uint32_t l = T.leaves[k];
     const uint64_t ns = T.end[l] - T.start[l];
     uint32_t nlum = 0, nex = 0;
-    for (uint32_t a = 0; a < T.nn;) {
-      if (bh_lumped(T, l, a)) {
+    bh_task_walk(T, W, k, j, [&](uint32_t a, bool lum) {
+      if (lum) {
         ++nlum;
-        a = T.esc[a];
-      } else if (T.right[a] == 0) {
+      } else {
         ++nex;
         pk += a == l ? ns * (ns - 1) : ns * (uint64_t)(T.end[a] - T.start[a]);
-        a = T.esc[a];
-      } else {
-        a = a + 1;
       }
-    }
-    lcount[k] = nlum;
-    ecount[k] = nex;
-  } else if (k == nl) {
-    lcount[k] = ecount[k] = 0;
+    });
+    lcount[q] = nlum;
+    ecount[q] = nex;
+  } else if (q == nq) {
+    lcount[q] = ecount[q] = 0;
   }
   red[threadIdx.x] = pk;
   __syncthreads();
@@ -77,34 +124,37 @@ __global__ __launch_bounds__(256) void bhl_count_kernel(BhTree T, uint32_t nl, u
   if (threadIdx.x == 0) atomicAdd(pairs, red[0]);  // an integer sum
 }
 
-// The lists, with each exact entry's target leaf ordinal and source leaf ordinal, and the lumped
-// entries' indices.
-__global__ __launch_bounds__(256) void bhl_fill_kernel(BhTree T, uint32_t nl, const uint32_t* __restrict__ leaf_ordinal,
-                                                       const uint32_t* __restrict__ lumped_off,
-                                                       const uint32_t* __restrict__ exact_off,
+// The entries of each (leaf, task), with each exact entry's target leaf ordinal and source leaf
+// ordinal and the lumped entries' indices; per leaf the offsets of its first task.
+__global__ __launch_bounds__(256) void bhl_fill_kernel(BhTree T, BhTasks W, uint32_t nl,
+                                                       const uint32_t* __restrict__ leaf_ordinal,
+                                                       const uint32_t* __restrict__ ltask_off,
+                                                       const uint32_t* __restrict__ etask_off,
+                                                       uint32_t* __restrict__ lumped_off, uint32_t* __restrict__ exact_off,
                                                        uint32_t* __restrict__ lumped, uint32_t* __restrict__ lumped_e,
                                                        uint32_t* __restrict__ exact, uint32_t* __restrict__ exact_tgt,
                                                        uint32_t* __restrict__ exact_src) {
-  const uint32_t k = blockIdx.x * 256 + threadIdx.x;
-  if (k >= nl) return;
-  const uint32_t l = T.leaves[k];
-  uint32_t ql = lumped_off[k], qe = exact_off[k];
-  for (uint32_t a = 0; a < T.nn;) {
-    if (bh_lumped(T, l, a)) {
+  const uint32_t q = blockIdx.x * 256 + threadIdx.x, nq = nl * W.count;
+  if (q > nq) return;
+  const uint32_t k = q / W.count, j = q % W.count;
+  if (j == 0) {  // the leaf's first task (q = nq: the totals)
+    lumped_off[k] = ltask_off[q];
+    exact_off[k] = etask_off[q];
+  }
+  if (q == nq) return;
+  uint32_t ql = ltask_off[q], qe = etask_off[q];
+  bh_task_walk(T, W, k, j, [&](uint32_t a, bool lum) {
+    if (lum) {
       lumped[ql] = a;
       lumped_e[ql] = ql;
       ++ql;
-      a = T.esc[a];
-    } else if (T.right[a] == 0) {
+    } else {
       exact[qe] = a;
       exact_tgt[qe] = leaf_ordinal[a];
       exact_src[qe] = k;
       ++qe;
-      a = T.esc[a];
-    } else {
-      a = a + 1;
     }
-  }
+  });
 }
 
 // ptr[x] = first position of key x in the sorted keys (x in [0, m]).
@@ -141,38 +191,54 @@ void BhEval::build_lists_device(ddgrs_ctx* ctx) {
       esc[a + 1] = t.right[a];
       esc[t.right[a]] = esc[a];
     }
+  // the cut: nodes at depth D and shallower leaves, in pre-order
+  const uint32_t D = std::min<uint32_t>(kMaxCutDepth, t.height > 0 ? t.height - 1 : 0);
+  std::vector<uint32_t> troot, tdepth;
+  for (int a = 0; a < nn; ++a)
+    if (t.depth[a] == D || (t.depth[a] < D && t.right[a] == 0)) {
+      troot.push_back((uint32_t)a);
+      tdepth.push_back(t.depth[a]);
+    }
+  const uint32_t ntask = (uint32_t)troot.size();
   ddgrs::upload(ctx, d_esc, esc);
   ddgrs::upload(ctx, d_leaf_ordinal, t.leaf_ordinal);
+  ddgrs::upload(ctx, d_parent, t.parent);
+  ddgrs::upload(ctx, d_task_root, troot);
+  ddgrs::upload(ctx, d_task_depth, tdepth);
   ddgrs::upload(ctx, d_nlo, clusters.lo);
   ddgrs::upload(ctx, d_nhi, clusters.hi);
   ddgrs::upload(ctx, d_nrad, clusters.radius);
   const BhTree T{d_right.ptr, d_esc.ptr, d_start.ptr, d_end.ptr, d_leaves.ptr, d_nlo.ptr, d_nhi.ptr, d_nrad.ptr,
                  (uint32_t)nn, theta * theta};
-  const size_t nl1 = (size_t)nl + 1;
-  l_count.reserve(nl1);
-  e_count.reserve(nl1);
+  const BhTasks W{d_task_root.ptr, d_task_depth.ptr, d_parent.ptr, ntask};
+  const size_t nq1 = (size_t)nl * ntask + 1, nl1 = (size_t)nl + 1;
+  if (nq1 > UINT32_MAX) throw Error(DDGRS_GPU, "bh_lists", "too many (leaf, task) pairs");
+  l_count.reserve(nq1);
+  e_count.reserve(nq1);
+  l_task_off.reserve(nq1);
+  e_task_off.reserve(nq1);
   d_lumped_off.reserve(nl1);
   d_exact_off.reserve(nl1);
   d_pairs.reserve(1);
   HIPCHECK(hipMemsetAsync(d_pairs.ptr, 0, sizeof(unsigned long long), ctx->stream));
-  launch(ctx, "bhl_count", bhl_count_kernel, dim3(blocks_for(nl1, 256)), dim3(256), 0, T, (uint32_t)nl, l_count.ptr,
-         e_count.ptr, d_pairs.ptr);
+  launch(ctx, "bhl_count", bhl_count_kernel, dim3(blocks_for((int64_t)nq1, 256)), dim3(256), 0, T, W, (uint32_t)nl,
+         l_count.ptr, e_count.ptr, d_pairs.ptr);
   for (int which = 0; which < 2; ++which) {
     const uint32_t* in = which ? e_count.ptr : l_count.ptr;
-    uint32_t* out = which ? d_exact_off.ptr : d_lumped_off.ptr;
+    uint32_t* out = which ? e_task_off.ptr : l_task_off.ptr;
     size_t tb = 0;
-    HIPCHECK(rocprim::exclusive_scan(nullptr, tb, in, out, 0u, nl1, rocprim::plus<uint32_t>(), ctx->stream));
+    HIPCHECK(rocprim::exclusive_scan(nullptr, tb, in, out, 0u, nq1, rocprim::plus<uint32_t>(), ctx->stream));
     temp.reserve(tb);
     tb = temp.n;
-    HIPCHECK(rocprim::exclusive_scan(temp.ptr, tb, in, out, 0u, nl1, rocprim::plus<uint32_t>(), ctx->stream));
+    HIPCHECK(rocprim::exclusive_scan(temp.ptr, tb, in, out, 0u, nq1, rocprim::plus<uint32_t>(), ctx->stream));
   }
   struct Totals {
     uint32_t lumped, exact;
     unsigned long long pairs;
   };
   Totals* tot = (Totals*)ctx->staging(sizeof(Totals));
-  download(ctx, &tot->lumped, (const uint32_t*)d_lumped_off.ptr + nl, 1);
-  download(ctx, &tot->exact, (const uint32_t*)d_exact_off.ptr + nl, 1);
+  download(ctx, &tot->lumped, (const uint32_t*)l_task_off.ptr + (nq1 - 1), 1);
+  download(ctx, &tot->exact, (const uint32_t*)e_task_off.ptr + (nq1 - 1), 1);
   download(ctx, &tot->pairs, (const unsigned long long*)d_pairs.ptr, 1);
   ctx->sync();
   n_lumped = tot->lumped;
@@ -190,9 +256,9 @@ void BhEval::build_lists_device(ddgrs_ctx* ctx) {
   d_exact_t.reserve(me);
   d_lumped_t_off.reserve((size_t)nn + 1);
   d_exact_t_off.reserve(nl1);
-  launch(ctx, "bhl_fill", bhl_fill_kernel, dim3(blocks_for(nl, 256)), dim3(256), 0, T, (uint32_t)nl,
-         (const uint32_t*)d_leaf_ordinal.ptr, (const uint32_t*)d_lumped_off.ptr, (const uint32_t*)d_exact_off.ptr,
-         d_lumped.ptr, l_idx.ptr, d_exact.ptr, e_tgt.ptr, e_src.ptr);
+  launch(ctx, "bhl_fill", bhl_fill_kernel, dim3(blocks_for((int64_t)nq1, 256)), dim3(256), 0, T, W, (uint32_t)nl,
+         (const uint32_t*)d_leaf_ordinal.ptr, (const uint32_t*)l_task_off.ptr, (const uint32_t*)e_task_off.ptr,
+         d_lumped_off.ptr, d_exact_off.ptr, d_lumped.ptr, l_idx.ptr, d_exact.ptr, e_tgt.ptr, e_src.ptr);
   // transposes: entries grouped by target, in entry order within a target
   auto sort = [&](const uint32_t* ki, uint32_t* ko, const uint32_t* vi, uint32_t* vo, size_t n, int bits) {
     size_t tb = 0;
