@@ -135,7 +135,7 @@ int ddgrs_saddle_lprime_batch(ddgrs_saddle* h, int64_t nb, const double* rs, dou
     need(rs, "rs");
     need(xs, "xs");
     ddgrs_ctx* ctx = h->s->ctx;
-    batch(ctx, nb, 3 * h->s->nv + 3, rs, xs, [&](const double* r, double* x) { h->pc.lp.solve(ctx, r, x); });
+    batch(ctx, nb, 3 * h->s->nv + 3, rs, xs, [&](const double* r, double* x) { h->pc.lprime_solve(ctx, r, x); });
   });
 }
 

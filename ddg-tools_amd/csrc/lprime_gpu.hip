@@ -18,6 +18,7 @@
 // formed by forward substitution, one wave per column.  FP64 throughout.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <numeric>
 
 #include "host_parallel.hpp"
@@ -71,12 +72,11 @@ LprimeSymbolic lprime_symbolic(const std::vector<uint32_t>& faces, size_t nv, co
   S.nd = nested_dissection(K, pos);
   const NdBlocks& nd = S.nd;
   // order, the multiplier row last
-  Factor& F = S.pattern;
-  F.n = n;
-  F.perm = nd.order;
-  F.perm.push_back((uint32_t)nv);
+  S.perm = nd.order;
+  S.perm.push_back((uint32_t)nv);
+  const std::vector<uint32_t>& perm = S.perm;
   std::vector<uint32_t> pinv(n);
-  for (size_t i = 0; i < n; ++i) pinv[F.perm[i]] = (uint32_t)i;
+  for (size_t i = 0; i < n; ++i) pinv[perm[i]] = (uint32_t)i;
   // blocks: the separator tree's, then the multiplier row as the root
   const uint32_t nb0 = (uint32_t)nd.start.size(), nb = nb0 + 1;
   S.nb = nb;
@@ -103,7 +103,7 @@ LprimeSymbolic lprime_symbolic(const std::vector<uint32_t>& faces, size_t nv, co
     const uint32_t j0 = S.bstart[b], j1 = bend[b];
     std::vector<uint32_t> st;
     for (uint32_t j = j0; j < j1; ++j) {
-      const uint32_t o = F.perm[j];
+      const uint32_t o = perm[j];
       if (o == nv) continue;  // the multiplier column has no entries below the diagonal
       for (uint32_t q = K.ptr[o]; q < K.ptr[o + 1]; ++q) {
         const uint32_t i = pinv[K.idx[q]];
@@ -174,43 +174,50 @@ LprimeSymbolic lprime_symbolic(const std::vector<uint32_t>& faces, size_t nv, co
     const uint32_t p = S.bparent[b];
     for (uint32_t r = 0; r < S.bf[b] - S.bk[b]; ++r) S.ea_map[S.ea_off[b] + r] = (uint32_t)local(p, front[b][S.bk[b] + r]);
   }
-  // L's columns (every front entry below the diagonal), its rows, D
-  F.cp.assign(n + 1, 0);
-  S.dpos.resize(n);
+  // solves: blocks with a front by height, later front rows, inverse offsets
+  S.block_of.resize(n);
   for (uint32_t b = 0; b < nb; ++b)
-    for (uint32_t c = 0; c < S.bk[b]; ++c) {
-      const uint32_t j = S.bstart[b] + c;
-      F.cp[j + 1] = S.bf[b] - c - 1;
-      S.dpos[j] = S.foff[b] + (uint64_t)c * S.bf[b] + c;
-    }
-  for (size_t j = 0; j < n; ++j) F.cp[j + 1] += F.cp[j];
-  F.ci.resize(F.cp[n]);
-  S.gmap_c.resize(F.cp[n]);
-  parallel(nb, [&](size_t b) {
-    const uint32_t f = S.bf[b];
-    for (uint32_t c = 0; c < S.bk[b]; ++c) {
-      uint32_t q = F.cp[S.bstart[b] + c];
-      for (uint32_t r = c + 1; r < f; ++r, ++q) {
-        F.ci[q] = front[b][r];
-        S.gmap_c[q] = S.foff[b] + (uint64_t)r * f + c;
-      }
-    }
-  });
-  F.rp.assign(n + 1, 0);
-  for (uint32_t i : F.ci) F.rp[i + 1]++;
-  for (size_t i = 0; i < n; ++i) F.rp[i + 1] += F.rp[i];
-  F.rj.resize(F.ci.size());
-  S.gmap_r.resize(F.ci.size());
-  {
-    std::vector<uint32_t> fill(F.rp.begin(), F.rp.end() - 1);
-    for (size_t j = 0; j < n; ++j)
-      for (uint32_t q = F.cp[j]; q < F.cp[j + 1]; ++q) {
-        const uint32_t i = F.ci[q];
-        F.rj[fill[i]] = (uint32_t)j;
-        S.gmap_r[fill[i]++] = S.gmap_c[q];
-      }
+    for (uint32_t i = S.bstart[b]; i < bend[b]; ++i) S.block_of[i] = b;
+  S.srows.resize(S.ea_off[nb]);
+  S.moff.assign(nb + 1, 0);
+  for (uint32_t b = 0; b < nb; ++b) {
+    std::copy(front[b].begin() + S.bk[b], front[b].end(), S.srows.begin() + S.ea_off[b]);
+    S.moff[b + 1] = S.moff[b] + (uint64_t)S.bk[b] * S.bk[b];
   }
-  S.plan = make_lprime_schedule(F, nd);
+  S.hoff.assign(1, 0);
+  S.hsplit.assign(S.H, 0);
+  S.hlds.assign(S.H, 0);
+  S.krows_off.assign(1, 0);
+  S.urows_off.assign(1, 0);
+  S.ltoff.assign(nb + 1, 0);
+  // DDGRS_LPRIME_LARGE_PATH=1 solves every height on the large-front path (to test it on small meshes)
+  const char* force = std::getenv("DDGRS_LPRIME_LARGE_PATH");
+  const size_t small_lds = (force && force[0] == '1') ? 0 : std::min(lds_bytes, kSolveSmallLds);
+  for (uint32_t h = 0; h < S.H; ++h) {
+    const uint32_t h0 = (uint32_t)S.hblocks.size();
+    size_t need = 0;
+    for (uint32_t b = 0; b < nb; ++b)
+      if (S.bheight[b] == h && S.bf[b] > 0) {
+        S.hblocks.push_back(b);
+        const size_t f = S.bf[b], k = S.bk[b];
+        need = std::max(need, ((k + 1) * f + 3 * f + 3 * k) * sizeof(double));
+      }
+    S.hoff.push_back((uint32_t)S.hblocks.size());
+    S.hsplit[h] = need > small_lds;
+    S.hlds[h] = (uint32_t)need;
+    if (S.hsplit[h])
+      for (uint32_t q = h0; q < S.hblocks.size(); ++q) {
+        const uint32_t b = S.hblocks[q], k = S.bk[b], ns = S.bf[b] - k;
+        for (uint32_t r0 = 0; r0 < k; r0 += kSolveRows) S.krows.push_back({b, r0});
+        for (uint32_t r0 = 0; r0 < ns; r0 += kSolveRows) S.urows.push_back({b, r0});
+        for (uint32_t r0 = 0; r0 < ns; r0 += 32)
+          for (uint32_t i0 = 0; i0 < k; i0 += 32) S.lt_tiles.push_back({b, r0, i0});
+      }
+    S.krows_off.push_back((uint32_t)S.krows.size());
+    S.urows_off.push_back((uint32_t)S.urows.size());
+  }
+  for (uint32_t b = 0; b < nb; ++b)
+    S.ltoff[b + 1] = S.ltoff[b] + (S.hsplit[S.bheight[b]] ? (uint64_t)S.bk[b] * (S.bf[b] - S.bk[b]) : 0);
   // work lists per height
   const size_t lds_doubles = lds_bytes / sizeof(double);
   auto fits_lds = [&](uint32_t f) { return (uint64_t)f * (f + 1) / 2 + f <= lds_doubles; };
@@ -559,27 +566,14 @@ __global__ __launch_bounds__(256) void lf_update_kernel(FrontArgs A, const Front
   }
 }
 
-__global__ __launch_bounds__(256) void lf_gather_kernel(size_t nnz, uint32_t n, const uint64_t* __restrict__ gmap_r,
-                                                        const uint64_t* __restrict__ gmap_c,
-                                                        const uint64_t* __restrict__ dpos,
-                                                        const double* __restrict__ front, double* __restrict__ rx,
-                                                        double* __restrict__ cx, double* __restrict__ d) {
-  for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < nnz || q < n; q += (size_t)gridDim.x * 256) {
-    if (q < nnz) {
-      rx[q] = front[gmap_r[q]];
-      cx[q] = front[gmap_c[q]];
-    }
-    if (q < n) d[q] = front[dpos[q]];
-  }
-}
-
 // One wave per row j of the order: column c = j − start(J) of M_J = L_JJ⁻¹ (unit lower) by forward
-// substitution, m_i = −Σ_{c ≤ l < i} L_il m_l; written to M_J (row-major) and M_Jᵀ.
+// substitution, m_i = −Σ_{c ≤ l < i} L_il m_l (L_JJ: the top-left k × k of J's front); M_J and
+// M_Jᵀ, both row-major.
 __global__ __launch_bounds__(256) void lf_inverse_kernel(uint32_t n, uint32_t max_rows, FrontArgs A,
                                                          const uint32_t* __restrict__ block_of,
-                                                         const uint64_t* __restrict__ inv_off,
-                                                         const double* __restrict__ front, double* __restrict__ inv,
-                                                         double* __restrict__ inv_t) {
+                                                         const uint64_t* __restrict__ moff,
+                                                         const double* __restrict__ front, double* __restrict__ minv,
+                                                         double* __restrict__ minv_t) {
   extern __shared__ double lds[];
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t j = blockIdx.x * 4 + wave;
@@ -587,8 +581,8 @@ __global__ __launch_bounds__(256) void lf_inverse_kernel(uint32_t n, uint32_t ma
   const uint32_t J = block_of[j], j0 = A.bstart[J], k = A.bk[J], f = A.bf[J], c = j - j0;
   const double* L = front + A.foff[J];
   double* m = lds + (size_t)wave * max_rows;
-  double* M = inv + inv_off[J];
-  double* Mt = inv_t + inv_off[J];
+  double* M = minv + moff[J];
+  double* Mt = minv_t + moff[J];
   if (lane == 0) {
     m[c] = 1.0;
     M[(size_t)c * k + c] = 1.0;
@@ -611,18 +605,349 @@ __global__ __launch_bounds__(256) void lf_inverse_kernel(uint32_t n, uint32_t ma
   }
 }
 
+// ---- solves with the fronts -------------------------------------------------------------------
+
+// b (order numbering, 3 per row) from a saddle vector (positions interleaved, multipliers last).
+__global__ __launch_bounds__(256) void lf_gather_rhs_kernel(uint32_t n, uint32_t nv, const uint32_t* __restrict__ perm,
+                                                            const double* __restrict__ r, double* __restrict__ b) {
+  const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= 3 * n) return;
+  const uint32_t j = t / 3, a = t % 3, o = perm[j];
+  b[t] = o < nv ? r[3 * (size_t)o + a] : r[3 * (size_t)nv + a];
+}
+
+__global__ __launch_bounds__(256) void lf_scatter_sol_kernel(uint32_t n, uint32_t nv, const uint32_t* __restrict__ perm,
+                                                             const double* __restrict__ xb, double* __restrict__ x) {
+  const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= 3 * n) return;
+  const uint32_t j = t / 3, a = t % 3, o = perm[j];
+  x[o < nv ? 3 * (size_t)o + a : 3 * (size_t)nv + a] = xb[t];
+}
+
+// w = [b_J; 0] (3 per row, f rows) plus the children's update vectors, one child after another
+// (the whole workgroup; ends synchronized).
+__device__ void assemble_front_vector(const FrontArgs& A, uint32_t J, const double* __restrict__ b,
+                                      const double* __restrict__ ubuf, double* w) {
+  const uint32_t f = A.bf[J], k = A.bk[J], j0 = A.bstart[J], tid = threadIdx.x;
+  for (uint32_t i = tid; i < f; i += 256)
+    for (int a = 0; a < 3; ++a) w[3 * i + a] = i < k ? b[3 * (size_t)(j0 + i) + a] : 0.0;
+  __syncthreads();
+  for (uint32_t q = A.child_ptr[J]; q < A.child_ptr[J + 1]; ++q) {
+    const uint32_t ch = A.child[q], cs = A.bf[ch] - A.bk[ch];
+    const uint32_t* map = A.ea_map + A.ea_off[ch];
+    const double* uc = ubuf + 3 * (size_t)A.ea_off[ch];
+    for (uint32_t r = tid; r < cs; r += 256)
+      for (int a = 0; a < 3; ++a) w[3 * map[r] + a] += uc[3 * r + a];
+    __syncthreads();
+  }
+}
+
+// ---- small fronts: one workgroup per block, its matrices loaded into LDS first ------------------
+
+// Forward: w = [b_J; 0] + children's update vectors, y_J = M_J w_J, u_J = w_S − L_SJ y_J.
+// LDS: M_J (k × (k+1)), L_SJ ((f−k) × (k+1)), w (3 f), y_J (3 k).
+__global__ __launch_bounds__(256) void lf_forward_small_kernel(FrontArgs A, const uint32_t* __restrict__ list,
+                                                               const uint64_t* __restrict__ moff,
+                                                               const double* __restrict__ minv,
+                                                               const double* __restrict__ front,
+                                                               const double* __restrict__ b, double* __restrict__ y,
+                                                               double* __restrict__ ubuf) {
+  extern __shared__ double sh[];
+  const uint32_t J = list[blockIdx.x];
+  const uint32_t f = A.bf[J], k = A.bk[J], j0 = A.bstart[J], ns = f - k, ks = k + 1, tid = threadIdx.x;
+  double* Ms = sh;
+  double* Ls = Ms + (size_t)k * ks;
+  double* w = Ls + (size_t)ns * ks;
+  double* yj = w + 3 * (size_t)f;
+  const double* M = minv + moff[J];
+  const double* G = front + A.foff[J];
+  for (uint32_t e = tid; e < k * k; e += 256) Ms[(e / k) * ks + e % k] = M[e];
+  for (uint32_t e = tid; e < ns * k; e += 256) {
+    const uint32_t r = e / k, l = e % k;
+    Ls[r * ks + l] = G[(size_t)(k + r) * f + l];
+  }
+  assemble_front_vector(A, J, b, ubuf, w);
+  for (uint32_t i = tid; i < k; i += 256) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (uint32_t l = 0; l <= i; ++l) {
+      const double m = Ms[i * ks + l];
+      s0 = fma(m, w[3 * l], s0);
+      s1 = fma(m, w[3 * l + 1], s1);
+      s2 = fma(m, w[3 * l + 2], s2);
+    }
+    yj[3 * i] = s0;
+    yj[3 * i + 1] = s1;
+    yj[3 * i + 2] = s2;
+    y[3 * (size_t)(j0 + i)] = s0;
+    y[3 * (size_t)(j0 + i) + 1] = s1;
+    y[3 * (size_t)(j0 + i) + 2] = s2;
+  }
+  __syncthreads();
+  double* u = ubuf + 3 * (size_t)A.ea_off[J];
+  for (uint32_t r = tid; r < ns; r += 256) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (uint32_t l = 0; l < k; ++l) {
+      const double v = Ls[r * ks + l];
+      s0 = fma(v, yj[3 * l], s0);
+      s1 = fma(v, yj[3 * l + 1], s1);
+      s2 = fma(v, yj[3 * l + 2], s2);
+    }
+    u[3 * r] = w[3 * (k + r)] - s0;
+    u[3 * r + 1] = w[3 * (k + r) + 1] - s1;
+    u[3 * r + 2] = w[3 * (k + r) + 2] - s2;
+  }
+}
+
+// Backward: t = D_J⁻¹ y_J − L_SJᵀ x_S, x_J = M_Jᵀ t.  LDS: M_J, L_SJ (as forward), x_S (3 (f−k)), t (3 k).
+__global__ __launch_bounds__(256) void lf_backward_small_kernel(FrontArgs A, const uint32_t* __restrict__ list,
+                                                                const uint32_t* __restrict__ srows,
+                                                                const uint64_t* __restrict__ moff,
+                                                                const double* __restrict__ minv,
+                                                                const double* __restrict__ front,
+                                                                const double* __restrict__ y, double* __restrict__ x) {
+  extern __shared__ double sh[];
+  const uint32_t J = list[blockIdx.x];
+  const uint32_t f = A.bf[J], k = A.bk[J], j0 = A.bstart[J], ns = f - k, ks = k + 1, tid = threadIdx.x;
+  if (k == 0) return;  // uniform per workgroup
+  double* Ms = sh;
+  double* Ls = Ms + (size_t)k * ks;
+  double* xs = Ls + (size_t)ns * ks;
+  double* t = xs + 3 * (size_t)ns;
+  const double* M = minv + moff[J];
+  const double* G = front + A.foff[J];
+  const uint32_t* S = srows + A.ea_off[J];
+  for (uint32_t e = tid; e < k * k; e += 256) Ms[(e / k) * ks + e % k] = M[e];
+  for (uint32_t e = tid; e < ns * k; e += 256) {
+    const uint32_t r = e / k, l = e % k;
+    Ls[r * ks + l] = G[(size_t)(k + r) * f + l];
+  }
+  for (uint32_t r = tid; r < ns; r += 256)
+    for (int a = 0; a < 3; ++a) xs[3 * r + a] = x[3 * (size_t)S[r] + a];
+  __syncthreads();
+  for (uint32_t i = tid; i < k; i += 256) {
+    const double d = G[(size_t)i * f + i];
+    double s0 = y[3 * (size_t)(j0 + i)] / d, s1 = y[3 * (size_t)(j0 + i) + 1] / d, s2 = y[3 * (size_t)(j0 + i) + 2] / d;
+    for (uint32_t r = 0; r < ns; ++r) {
+      const double l = Ls[r * ks + i];
+      s0 = fma(-l, xs[3 * r], s0);
+      s1 = fma(-l, xs[3 * r + 1], s1);
+      s2 = fma(-l, xs[3 * r + 2], s2);
+    }
+    t[3 * i] = s0;
+    t[3 * i + 1] = s1;
+    t[3 * i + 2] = s2;
+  }
+  __syncthreads();
+  for (uint32_t i = tid; i < k; i += 256) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (uint32_t l = i; l < k; ++l) {
+      const double m = Ms[l * ks + i];
+      s0 = fma(m, t[3 * l], s0);
+      s1 = fma(m, t[3 * l + 1], s1);
+      s2 = fma(m, t[3 * l + 2], s2);
+    }
+    x[3 * (size_t)(j0 + i)] = s0;
+    x[3 * (size_t)(j0 + i) + 1] = s1;
+    x[3 * (size_t)(j0 + i) + 2] = s2;
+  }
+}
+
+// ---- large fronts: rows spread over many workgroups, four rows per wave -----------------------
+
+// Σ_l row_q[l] v[l] (3 axes) for the four rows of a wave: rows q with length len[q], reduced over
+// the wave (every lane gets the sums).
+__device__ __forceinline__ void wave_rows4(const double* const row[4], const uint32_t lo[4], const uint32_t hi[4],
+                                           const double* v, uint32_t lane, double out[4][3]) {
+  double acc[4][3] = {};
+  uint32_t lmin = lo[0], lmax = hi[0];
+  for (int q = 1; q < 4; ++q) {
+    lmin = min(lmin, lo[q]);
+    lmax = max(lmax, hi[q]);
+  }
+  for (uint32_t l0 = lmin; l0 < lmax; l0 += 64) {
+    const uint32_t l = l0 + lane;
+    double m[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) m[q] = (l >= lo[q] && l < hi[q]) ? row[q][l] : 0.0;
+    if (l < lmax) {
+      const double v0 = v[3 * l], v1 = v[3 * l + 1], v2 = v[3 * l + 2];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        acc[q][0] = fma(m[q], v0, acc[q][0]);
+        acc[q][1] = fma(m[q], v1, acc[q][1]);
+        acc[q][2] = fma(m[q], v2, acc[q][2]);
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    for (int a = 0; a < 3; ++a) out[q][a] = wsum64(acc[q][a]);
+}
+
+// Forward y_J = M_J w_J, rows [r0, r0 + 16) (w assembled by each workgroup).
+__global__ __launch_bounds__(256) void lf_forward_y_kernel(FrontArgs A, const SolveItem* __restrict__ items,
+                                                           const uint64_t* __restrict__ moff,
+                                                           const double* __restrict__ minv,
+                                                           const double* __restrict__ b, double* __restrict__ y,
+                                                           const double* __restrict__ ubuf) {
+  extern __shared__ double w[];
+  const SolveItem it = items[blockIdx.x];
+  const uint32_t J = it.block, k = A.bk[J], j0 = A.bstart[J], lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  assemble_front_vector(A, J, b, ubuf, w);
+  const double* M = minv + moff[J];
+  const double* row[4];
+  uint32_t lo[4], hi[4], ri[4];
+  for (int q = 0; q < 4; ++q) {
+    ri[q] = it.r0 + 4 * wave + q;
+    const bool ok = ri[q] < k;
+    row[q] = M + (size_t)(ok ? ri[q] : 0) * k;
+    lo[q] = 0;
+    hi[q] = ok ? ri[q] + 1 : 0;
+  }
+  double out[4][3];
+  wave_rows4(row, lo, hi, w, lane, out);
+  if (lane == 0)
+    for (int q = 0; q < 4; ++q)
+      if (ri[q] < k)
+        for (int a = 0; a < 3; ++a) y[3 * (size_t)(j0 + ri[q]) + a] = out[q][a];
+}
+
+// Forward u_J = w_S − L_SJ y_J, later rows [r0, r0 + 16).
+__global__ __launch_bounds__(256) void lf_forward_u_kernel(FrontArgs A, const SolveItem* __restrict__ items,
+                                                           const double* __restrict__ front,
+                                                           const double* __restrict__ b, const double* __restrict__ y,
+                                                           double* __restrict__ ubuf) {
+  extern __shared__ double w[];  // 3 f, then y_J: 3 k
+  const SolveItem it = items[blockIdx.x];
+  const uint32_t J = it.block, f = A.bf[J], k = A.bk[J], j0 = A.bstart[J], ns = f - k;
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double* yj = w + 3 * (size_t)f;
+  for (uint32_t i = threadIdx.x; i < 3 * k; i += 256) yj[i] = y[3 * (size_t)j0 + i];
+  assemble_front_vector(A, J, b, ubuf, w);
+  const double* G = front + A.foff[J];
+  const double* row[4];
+  uint32_t lo[4], hi[4], ri[4];
+  for (int q = 0; q < 4; ++q) {
+    ri[q] = it.r0 + 4 * wave + q;
+    const bool ok = ri[q] < ns;
+    row[q] = G + (size_t)(k + (ok ? ri[q] : 0)) * f;
+    lo[q] = 0;
+    hi[q] = ok ? k : 0;
+  }
+  double out[4][3];
+  wave_rows4(row, lo, hi, yj, lane, out);
+  double* u = ubuf + 3 * (size_t)A.ea_off[J];
+  if (lane == 0)
+    for (int q = 0; q < 4; ++q)
+      if (ri[q] < ns)
+        for (int a = 0; a < 3; ++a) u[3 * ri[q] + a] = w[3 * (k + ri[q]) + a] - out[q][a];
+}
+
+// Backward t = D_J⁻¹ y_J − L_SJᵀ x_S, rows [r0, r0 + 16) (rows of the transposed L_SJ).
+__global__ __launch_bounds__(256) void lf_backward_t_kernel(FrontArgs A, const SolveItem* __restrict__ items,
+                                                            const uint32_t* __restrict__ srows,
+                                                            const uint64_t* __restrict__ ltoff,
+                                                            const double* __restrict__ lt,
+                                                            const double* __restrict__ front,
+                                                            const double* __restrict__ y, const double* __restrict__ x,
+                                                            double* __restrict__ tbuf) {
+  extern __shared__ double xs[];  // 3 (f − k)
+  const SolveItem it = items[blockIdx.x];
+  const uint32_t J = it.block, f = A.bf[J], k = A.bk[J], j0 = A.bstart[J], ns = f - k;
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t* S = srows + A.ea_off[J];
+  for (uint32_t r = threadIdx.x; r < ns; r += 256)
+    for (int a = 0; a < 3; ++a) xs[3 * r + a] = x[3 * (size_t)S[r] + a];
+  __syncthreads();
+  const double* T = lt + ltoff[J];
+  const double* row[4];
+  uint32_t lo[4], hi[4], ri[4];
+  for (int q = 0; q < 4; ++q) {
+    ri[q] = it.r0 + 4 * wave + q;
+    const bool ok = ri[q] < k;
+    row[q] = T + (size_t)(ok ? ri[q] : 0) * ns;
+    lo[q] = 0;
+    hi[q] = ok ? ns : 0;
+  }
+  double out[4][3];
+  wave_rows4(row, lo, hi, xs, lane, out);
+  const double* G = front + A.foff[J];
+  if (lane == 0)
+    for (int q = 0; q < 4; ++q)
+      if (ri[q] < k) {
+        const size_t i = j0 + ri[q];
+        const double d = G[(size_t)ri[q] * f + ri[q]];
+        for (int a = 0; a < 3; ++a) tbuf[3 * i + a] = y[3 * i + a] / d - out[q][a];
+      }
+}
+
+// Backward x_J = M_Jᵀ t, rows [r0, r0 + 16) (rows of M_Jᵀ).
+__global__ __launch_bounds__(256) void lf_backward_x_kernel(FrontArgs A, const SolveItem* __restrict__ items,
+                                                            const uint64_t* __restrict__ moff,
+                                                            const double* __restrict__ minv_t,
+                                                            const double* __restrict__ tbuf, double* __restrict__ x) {
+  extern __shared__ double t[];  // 3 k
+  const SolveItem it = items[blockIdx.x];
+  const uint32_t J = it.block, k = A.bk[J], j0 = A.bstart[J], lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (uint32_t i = threadIdx.x; i < 3 * k; i += 256) t[i] = tbuf[3 * (size_t)j0 + i];
+  __syncthreads();
+  const double* Mt = minv_t + moff[J];
+  const double* row[4];
+  uint32_t lo[4], hi[4], ri[4];
+  for (int q = 0; q < 4; ++q) {
+    ri[q] = it.r0 + 4 * wave + q;
+    const bool ok = ri[q] < k;
+    row[q] = Mt + (size_t)(ok ? ri[q] : 0) * k;
+    lo[q] = ok ? ri[q] : 0;
+    hi[q] = ok ? k : 0;
+  }
+  double out[4][3];
+  wave_rows4(row, lo, hi, t, lane, out);
+  if (lane == 0)
+    for (int q = 0; q < 4; ++q)
+      if (ri[q] < k)
+        for (int a = 0; a < 3; ++a) x[3 * (size_t)(j0 + ri[q]) + a] = out[q][a];
+}
+
+// L_SJᵀ of the large fronts (k × (f − k), row-major), 32 × 32 tiles through LDS.
+__global__ __launch_bounds__(256) void lf_transpose_kernel(FrontArgs A, const FrontTile* __restrict__ items,
+                                                           const uint64_t* __restrict__ ltoff,
+                                                           const double* __restrict__ front, double* __restrict__ lt) {
+  __shared__ double tile[32][33];
+  const FrontTile it = items[blockIdx.x];  // block, i0: first later row r, j0: first column i
+  const uint32_t J = it.block, f = A.bf[J], k = A.bk[J], ns = f - k;
+  const double* G = front + A.foff[J];
+  double* T = lt + ltoff[J];
+  const uint32_t tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (uint32_t yy = ty; yy < 32; yy += 8) {
+    const uint32_t r = it.i0 + yy, i = it.j0 + tx;
+    tile[yy][tx] = (r < ns && i < k) ? G[(size_t)(k + r) * f + i] : 0.0;
+  }
+  __syncthreads();
+  for (uint32_t yy = ty; yy < 32; yy += 8) {
+    const uint32_t i = it.j0 + yy, r = it.i0 + tx;
+    if (i < k && r < ns) T[(size_t)i * ns + r] = tile[tx][yy];
+  }
+}
+
 // ---- host side -------------------------------------------------------------------------------------
 
 LprimeFactorDev::~LprimeFactorDev() {
   if (bad_host) (void)hipHostFree(bad_host);
 }
 
-void LprimeFactorDev::setup(ddgrs_ctx* ctx, const LprimeSymbolic& S, LprimeDev& lp) {
+void LprimeFactorDev::setup(ddgrs_ctx* ctx, const LprimeSymbolic& S) {
   nv = S.nv;
   n = S.n;
   H = S.H;
   max_children = S.max_children;
   max_rows = S.max_rows;
+  max_front = S.max_front;
+  hoff = S.hoff;
+  hsplit = S.hsplit;
+  hlds = S.hlds;
+  krows_off = S.krows_off;
+  urows_off = S.urows_off;
   front_total = S.front_total;
   lds_off = S.lds_off;
   lds_rows = S.lds_rows;
@@ -631,7 +956,6 @@ void LprimeFactorDev::setup(ddgrs_ctx* ctx, const LprimeSymbolic& S, LprimeDev& 
   step_c0 = S.step_c0;
   panel_off = S.panel_off;
   tile_off = S.tile_off;
-  nnz_l = S.pattern.ci.size();
   nnz_k = S.kpat.idx.size();
   ddgrs::upload(ctx, kp, S.kpat.ptr);
   ddgrs::upload(ctx, kj, S.kpat.idx);
@@ -649,9 +973,23 @@ void LprimeFactorDev::setup(ddgrs_ctx* ctx, const LprimeSymbolic& S, LprimeDev& 
   ddgrs::upload(ctx, foff, S.foff);
   ddgrs::upload(ctx, amap_k, S.amap_k);
   ddgrs::upload(ctx, amap_c, S.amap_c);
-  ddgrs::upload(ctx, gmap_r, S.gmap_r);
-  ddgrs::upload(ctx, gmap_c, S.gmap_c);
-  ddgrs::upload(ctx, dpos, S.dpos);
+  ddgrs::upload(ctx, perm, S.perm);
+  ddgrs::upload(ctx, block_of, S.block_of);
+  ddgrs::upload(ctx, hblocks, S.hblocks);
+  ddgrs::upload(ctx, krows, S.krows);
+  ddgrs::upload(ctx, urows, S.urows);
+  ddgrs::upload(ctx, ltoff, S.ltoff);
+  ddgrs::upload(ctx, lt_tiles, S.lt_tiles);
+  n_lt_tiles = (uint32_t)S.lt_tiles.size();
+  lt.reserve(std::max<uint64_t>(S.ltoff.back(), 1));
+  tb.reserve(3 * n);
+  ddgrs::upload(ctx, srows, S.srows);
+  ddgrs::upload(ctx, moff, S.moff);
+  minv.reserve(std::max<uint64_t>(S.moff.back(), 1));
+  minv_t.reserve(std::max<uint64_t>(S.moff.back(), 1));
+  ubuf.reserve(3 * std::max<size_t>(S.ea_off.back(), 1));
+  yb.reserve(3 * n);
+  xb.reserve(3 * n);
   ddgrs::upload(ctx, ea_items, S.ea_items);
   ddgrs::upload(ctx, tiles, S.tiles);
   front.reserve(std::max<uint64_t>(front_total, 1));
@@ -668,12 +1006,10 @@ void LprimeFactorDev::setup(ddgrs_ctx* ctx, const LprimeSymbolic& S, LprimeDev& 
   if (lds_max > 0)
     HIPCHECK(hipFuncSetAttribute((const void*)lf_lds_front_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)lds_max));
-  lp.nv = nv;
-  lp.setup(ctx, S.pattern, S.plan);
   ctx->sync();  // the symbolic structure may be released after return
 }
 
-void LprimeFactorDev::factor(ddgrs_surface* s, double eps_rel, LprimeDev& lp) {
+void LprimeFactorDev::factor(ddgrs_surface* s, double eps_rel) {
   ddgrs_ctx* ctx = s->ctx;
   const uint32_t unv = (uint32_t)nv;
   const unsigned gv = blocks_for((int64_t)nv, 256);
@@ -718,15 +1054,67 @@ void LprimeFactorDev::factor(ddgrs_surface* s, double eps_rel, LprimeDev& lp) {
              (const FrontTile*)tiles.ptr + tile_off[st], step_c0[st], front.ptr);
     }
   }
-  // L's rows and columns, D, the block inverses
-  launch(ctx, "lf_gather", lf_gather_kernel,
-         dim3((unsigned)std::min<uint64_t>(blocks_for((int64_t)std::max<size_t>(nnz_l, n), 256), 8192)), dim3(256), 0,
-         nnz_l, (uint32_t)n, (const uint64_t*)gmap_r.ptr, (const uint64_t*)gmap_c.ptr, (const uint64_t*)dpos.ptr,
-         (const double*)front.ptr, lp.rx.ptr, lp.cx.ptr, lp.d.ptr);
+  // the dense inverses of the diagonal blocks
   launch(ctx, "lf_inverse", lf_inverse_kernel, dim3(blocks_for((int64_t)n, 4)), dim3(256),
-         4 * (size_t)max_rows * sizeof(double), (uint32_t)n, max_rows, A, (const uint32_t*)lp.block_of.ptr,
-         (const uint64_t*)lp.blk_inv.ptr, (const double*)front.ptr, lp.inv.ptr, lp.inv_t.ptr);
+         4 * (size_t)max_rows * sizeof(double), (uint32_t)n, max_rows, A, (const uint32_t*)block_of.ptr,
+         (const uint64_t*)moff.ptr, (const double*)front.ptr, minv.ptr, minv_t.ptr);
+  launch(ctx, "lf_transpose", lf_transpose_kernel, dim3(n_lt_tiles), dim3(256), 0, A,
+         (const FrontTile*)lt_tiles.ptr, (const uint64_t*)ltoff.ptr, (const double*)front.ptr, lt.ptr);
   download(ctx, bad_host, (const uint32_t*)bad.ptr, 1);
+}
+
+namespace {
+// Per-height kernel names (timings by height of the tree).
+const char* height_name(const char* base, uint32_t h) {
+  static const char* fwd[] = {"lf_forward_h00", "lf_forward_h01", "lf_forward_h02", "lf_forward_h03", "lf_forward_h04",
+                              "lf_forward_h05", "lf_forward_h06", "lf_forward_h07", "lf_forward_h08", "lf_forward_h09",
+                              "lf_forward_h10", "lf_forward_h11", "lf_forward_h12", "lf_forward_h13", "lf_forward_h14",
+                              "lf_forward_h15"};
+  static const char* bwd[] = {"lf_backward_h00", "lf_backward_h01", "lf_backward_h02", "lf_backward_h03",
+                              "lf_backward_h04", "lf_backward_h05", "lf_backward_h06", "lf_backward_h07",
+                              "lf_backward_h08", "lf_backward_h09", "lf_backward_h10", "lf_backward_h11",
+                              "lf_backward_h12", "lf_backward_h13", "lf_backward_h14", "lf_backward_h15"};
+  if (h > 15) return base;
+  return base[3] == 'f' ? fwd[h] : bwd[h];
+}
+}  // namespace
+
+void LprimeFactorDev::solve(ddgrs_ctx* ctx, const double* r, double* x) {
+  const uint32_t un = (uint32_t)n, unv = (uint32_t)nv;
+  const FrontArgs A{bstart.ptr, bk.ptr, bf.ptr, bparent.ptr, child_ptr.ptr, child.ptr, ea_off.ptr, ea_map.ptr, foff.ptr};
+  launch(ctx, "lf_gather_rhs", lf_gather_rhs_kernel, dim3(blocks_for(3 * (int64_t)n, 256)), dim3(256), 0, un, unv,
+         (const uint32_t*)perm.ptr, r, xb.ptr);
+  const size_t lds = (3 * (size_t)max_front + 3 * (size_t)max_rows) * sizeof(double);
+  for (uint32_t h = 0; h < H; ++h) {
+    if (!hsplit[h]) {
+      launch(ctx, height_name("lf_forward", h), lf_forward_small_kernel, dim3(hoff[h + 1] - hoff[h]), dim3(256),
+             (size_t)hlds[h], A, (const uint32_t*)hblocks.ptr + hoff[h], (const uint64_t*)moff.ptr,
+             (const double*)minv.ptr, (const double*)front.ptr, (const double*)xb.ptr, yb.ptr, ubuf.ptr);
+    } else {
+      launch(ctx, height_name("lf_forward", h), lf_forward_y_kernel, dim3(krows_off[h + 1] - krows_off[h]), dim3(256),
+             lds, A, (const SolveItem*)krows.ptr + krows_off[h], (const uint64_t*)moff.ptr, (const double*)minv.ptr,
+             (const double*)xb.ptr, yb.ptr, (const double*)ubuf.ptr);
+      launch(ctx, height_name("lf_forward", h), lf_forward_u_kernel, dim3(urows_off[h + 1] - urows_off[h]), dim3(256),
+             lds, A, (const SolveItem*)urows.ptr + urows_off[h], (const double*)front.ptr, (const double*)xb.ptr,
+             (const double*)yb.ptr, ubuf.ptr);
+    }
+  }
+  for (uint32_t h = H; h-- > 0;) {
+    if (!hsplit[h]) {
+      launch(ctx, height_name("lf_backward", h), lf_backward_small_kernel, dim3(hoff[h + 1] - hoff[h]), dim3(256),
+             (size_t)hlds[h], A, (const uint32_t*)hblocks.ptr + hoff[h], (const uint32_t*)srows.ptr,
+             (const uint64_t*)moff.ptr, (const double*)minv.ptr, (const double*)front.ptr, (const double*)yb.ptr, xb.ptr);
+    } else {
+      launch(ctx, height_name("lf_backward", h), lf_backward_t_kernel, dim3(krows_off[h + 1] - krows_off[h]), dim3(256),
+             lds, A, (const SolveItem*)krows.ptr + krows_off[h], (const uint32_t*)srows.ptr, (const uint64_t*)ltoff.ptr,
+             (const double*)lt.ptr, (const double*)front.ptr, (const double*)yb.ptr, (const double*)xb.ptr, tb.ptr);
+      launch(ctx, height_name("lf_backward", h), lf_backward_x_kernel, dim3(krows_off[h + 1] - krows_off[h]), dim3(256),
+             lds, A, (const SolveItem*)krows.ptr + krows_off[h], (const uint64_t*)moff.ptr, (const double*)minv_t.ptr,
+             (const double*)tb.ptr, xb.ptr);
+    }
+  }
+  launch(ctx, "lf_scatter_sol", lf_scatter_sol_kernel, dim3(blocks_for(3 * (int64_t)n, 256)), dim3(256), 0, un, unv,
+         (const uint32_t*)perm.ptr, (const double*)xb.ptr, x);
 }
 
 void LprimeFactorDev::check(ddgrs_ctx* ctx) {

@@ -3,8 +3,8 @@
 // method and every index map are computed once (LprimeSymbolic, host) and reused while the positions
 // change; each step then computes K's values, assembles the fronts, factors them height by height
 // and forms L's rows and columns and the dense block inverses that the block solves use, all on the
-// device (LprimeFactorDev).  The numbers are those of factor_multifrontal up to rounding (the
-// same eliminations; products summed in another order).
+// device (LprimeFactorDev), which also solves with the factored fronts.  The numbers are those of
+// factor_multifrontal up to rounding (the same eliminations; products summed in another order).
 #pragma once
 #include "precond.hpp"
 
@@ -26,14 +26,21 @@ struct FrontTile {  // trailing-update tile of a large front: rows from i0, colu
 struct FrontPanel {  // a panel's rows [c1 + r0, c1 + r0 + kPanelItemRows) below its diagonal block
   uint32_t block, r0;
 };
+// Solves of heights with a large front: rows per work item (four per wave).
+constexpr uint32_t kSolveRows = 16;
+// LDS of a small-front solve workgroup (its matrices are staged there).
+constexpr size_t kSolveSmallLds = 64 * 1024;
+struct SolveItem {  // rows [r0, r0 + kSolveRows) of a block (or of its later front rows)
+  uint32_t block, r0;
+};
 
 struct LprimeSymbolic {
   size_t nv = 0, n = 0;
   HostCsr kpat;                   // K's pattern: each vertex and its neighbours, ascending (no values)
   std::vector<uint32_t> kdiag;    // per vertex: index of the diagonal entry in kpat
   NdBlocks nd;
-  Factor pattern;                 // perm, rows and columns of L (every front entry; no values)
-  LprimePlan plan;                // block solve schedule (no inverses)
+  std::vector<uint32_t> perm;     // the elimination order (perm[new] = old), the multiplier last
+  std::vector<uint32_t> block_of; // per row of the order: its block
   // blocks of the separator tree, then the multiplier row (the root)
   uint32_t nb = 0, H = 0, max_front = 0, max_rows = 0;
   std::vector<uint32_t> bstart, bk, bf, bparent, bheight, child_ptr, child;
@@ -42,7 +49,6 @@ struct LprimeSymbolic {
   std::vector<uint64_t> amap_k;   // per K entry: front position of its lower-triangle copy, or kNoPos
   std::vector<uint64_t> amap_c;   // per vertex: front position of (multiplier row, vertex)
   std::vector<uint32_t> ea_off, ea_map;  // per block: its update rows' positions in the parent's front
-  std::vector<uint64_t> gmap_r, gmap_c, dpos;  // front positions of L's rows, L's columns and D
   // Fronts that fit one workgroup's LDS are factored there (one launch per height); the others by
   // panels of kPanelCols columns and tiled trailing updates.
   std::vector<uint32_t> lds_blocks, lds_off;   // per height
@@ -53,6 +59,16 @@ struct LprimeSymbolic {
   std::vector<FrontPanel> panels;
   std::vector<uint32_t> tile_off, panel_off, step_c0, step_off;  // step_off: per height
   std::vector<FrontTile> tiles;
+  // solves: the blocks with a front, by height; each block's later front rows (global indices,
+  // at ea_off); the offsets of the dense inverses of the diagonal blocks (k × k row-major)
+  std::vector<uint32_t> hblocks, hoff, srows;
+  std::vector<uint64_t> moff;
+  std::vector<uint8_t> hsplit;                 // per height: solved by the large-front path
+  std::vector<uint32_t> hlds;                  // per height: LDS of the small-front path
+  std::vector<SolveItem> krows, urows;         // large-front heights' work items
+  std::vector<uint32_t> krows_off, urows_off;  // per height
+  std::vector<uint64_t> ltoff;                 // per block: its L_SJᵀ (large-front heights only)
+  std::vector<FrontTile> lt_tiles;             // transpose tiles (block, first later row, first column)
   static constexpr uint64_t kNoPos = ~0ull;
 };
 // `lds_bytes`: the LDS a workgroup may use (fronts whose lower triangle and one column fit are
@@ -62,27 +78,42 @@ LprimeSymbolic lprime_symbolic(const std::vector<uint32_t>& faces, size_t nv, co
 
 struct LprimeFactorDev {
   size_t nv = 0, n = 0;
-  uint32_t H = 0, max_children = 0, max_rows = 0;
+  uint32_t H = 0, max_children = 0, max_rows = 0, max_front = 0;
   uint64_t front_total = 0;
-  std::vector<uint32_t> lds_off, lds_rows, ea_item_off, step_off, step_c0, panel_off, tile_off;
+  std::vector<uint32_t> lds_off, lds_rows, ea_item_off, step_off, step_c0, panel_off, tile_off, hoff;
+  std::vector<uint32_t> krows_off, urows_off, hlds;
+  std::vector<uint8_t> hsplit;
   DevBuf<uint32_t> kp, kj, kdiag, bstart, bk, bf, bparent, child_ptr, child, ea_off, ea_map, lds_blocks;
   DevBuf<FrontPanel> panels;
-  DevBuf<uint64_t> foff, amap_k, amap_c, gmap_r, gmap_c, dpos;
+  DevBuf<uint32_t> perm, block_of, hblocks, srows;
+  DevBuf<uint64_t> moff;
+  DevBuf<double> minv, minv_t, ubuf, yb, xb, tb;  // block inverses M_J, M_Jᵀ; update vectors, permuted vectors
+  DevBuf<SolveItem> krows, urows;
+  DevBuf<uint64_t> ltoff;
+  DevBuf<double> lt;
+  DevBuf<FrontTile> lt_tiles;
+  DevBuf<uint64_t> foff, amap_k, amap_c;
   DevBuf<FrontItem> ea_items;
   DevBuf<FrontTile> tiles;
   DevBuf<double> front, kx, mass, crow, diagv, sums;  // sums: tr W, Σ a, ε
   DevBuf<uint32_t> bad;
   uint32_t* bad_host = nullptr;  // pinned: the first failing pivot of the last factorization
-  size_t nnz_l = 0, nnz_k = 0;
+  size_t nnz_k = 0;
+  uint32_t n_lt_tiles = 0;
   LprimeFactorDev() = default;
   LprimeFactorDev(const LprimeFactorDev&) = delete;
   LprimeFactorDev& operator=(const LprimeFactorDev&) = delete;
   ~LprimeFactorDev();
-  // Uploads the symbolic structure and sets up `lp`'s schedule and pattern.
-  void setup(ddgrs_ctx* ctx, const LprimeSymbolic& sym, LprimeDev& lp);
+  // Uploads the symbolic structure.
+  void setup(ddgrs_ctx* ctx, const LprimeSymbolic& sym);
   // Enqueues K = W + εM at the surface's positions (K's values into kx, the barycenter row into
-  // crow), the factorization of L′ and L's rows, columns, D and block inverses into `lp`.
-  void factor(ddgrs_surface* s, double eps_rel, LprimeDev& lp);
+  // crow), the factorization of L′ (in the fronts) and the dense inverses of its diagonal blocks.
+  void factor(ddgrs_surface* s, double eps_rel);
+  // x = L′⁻¹ r for a full saddle vector (3 nv + 3, interleaved positions then multipliers), the
+  // three axes together: forward by height (each front's right-hand side from its block and its
+  // children's update vectors, its block solved by the dense inverse, its update vector formed),
+  // D⁻¹, backward by height (each block from its later front rows).
+  void solve(ddgrs_ctx* ctx, const double* r, double* x);
   // After a synchronization: throws SingularSaddle if a pivot of the last factorization was zero
   // or not finite.
   void check(ddgrs_ctx* ctx);

@@ -1079,7 +1079,7 @@ void Preconditioner::setup_device(ddgrs_ctx* ctx, const LprimeSymbolic& sym) {
   nv = sym.nv;
   n = 3 * nv + 3;
   fac = std::make_unique<LprimeFactorDev>();
-  fac->setup(ctx, sym, lp);
+  fac->setup(ctx, sym);
   k_ptr = fac->kp.ptr;
   k_idx = fac->kj.ptr;
   k_val = fac->kx.ptr;
@@ -1091,7 +1091,7 @@ void Preconditioner::setup_device(ddgrs_ctx* ctx, const LprimeSymbolic& sym) {
 void Preconditioner::factor_device(ddgrs_surface* s, double s_exp, double w_tp, double eps_rel) {
   if (!fac) throw Error(DDGRS_GPU, "preconditioner", "device factorization used before its symbolic set-up");
   frac_scale = fractional_scale(s_exp, w_tp);
-  fac->factor(s, eps_rel, lp);
+  fac->factor(s, eps_rel);
   eps = -1.0;  // read on demand
 }
 
@@ -1109,12 +1109,19 @@ void Preconditioner::build_fractional(ddgrs_ctx* ctx, const std::vector<double>&
   vf.build(ctx, pos, mass, tau, chi, leaf_pairs, std::move(vertex_tree));
 }
 
+void Preconditioner::lprime_solve(ddgrs_ctx* ctx, const double* r, double* x) {
+  if (fac)
+    fac->solve(ctx, r, x);
+  else
+    lp.solve(ctx, r, x);
+}
+
 void Preconditioner::apply(ddgrs_ctx* ctx, const double* r, double* z) {
-  lp.solve(ctx, r, y.ptr);
+  lprime_solve(ctx, r, y.ptr);
   vf.apply(ctx, y.ptr, v.ptr);
   launch(ctx, "pc_augment", pc_augment_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, (uint32_t)nv, k_ptr, k_idx,
          k_val, frac_scale, (const double*)y.ptr, v.ptr);
-  lp.solve(ctx, v.ptr, z);
+  lprime_solve(ctx, v.ptr, z);
   launch(ctx, "pc_add", pc_add_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, (uint32_t)n, (const double*)y.ptr, z);
 }
 

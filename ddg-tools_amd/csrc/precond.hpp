@@ -146,6 +146,8 @@ struct Preconditioner {
   // L̃^V_{2−s} on the vertex BVH `vertex_tree` of `pos` (structure and panels on the device).
   void build_fractional(ddgrs_ctx* ctx, const std::vector<double>& pos, const std::vector<double>& mass, double tau,
                         double chi, size_t leaf_pairs, Topology vertex_tree);
+  // x = L′⁻¹ r (device pointers, length 3 nv + 3).
+  void lprime_solve(ddgrs_ctx* ctx, const double* r, double* x);
   // z = P⁻¹ r (device pointers, length 3 nv + 3).
   void apply(ddgrs_ctx* ctx, const double* r, double* z);
 };

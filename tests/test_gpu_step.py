@@ -87,6 +87,22 @@ def test_lprime_device_factor_matches_host(pkg, ctx, name, monkeypatch):
         assert rel(dev.precondition(r), host.precondition(r)) <= 1e-7
 
 
+def test_lprime_solve_paths_agree(pkg, ctx, monkeypatch):
+    """The L′ solve's small-front path (matrices staged in LDS, one workgroup per block) and its
+    large-front path (rows spread over workgroups) give the same solution."""
+    p, f = meshes(pkg)["torus_16k"]()
+    s = pkg.Surface(ctx, f, p)
+    small = pkg.Saddle(s)
+    monkeypatch.setenv("DDGRS_LPRIME_LARGE_PATH", "1")
+    large = pkg.Saddle(s)
+    monkeypatch.delenv("DDGRS_LPRIME_LARGE_PATH")
+    rng = np.random.default_rng(24)
+    r = rng.standard_normal(small.n)
+    xs, xl = small.lprime_solve(r), large.lprime_solve(r)
+    print(f"small/large path rel diff {rel(xl, xs):.2e}")
+    assert rel(xl, xs) <= 1e-9
+
+
 def test_lean_saddle_octa(pkg, ctx):
     c, o = fx.read_case("rs-octa-saddle-none"), fx.read_out("rs-octa-saddle-none")
     p, f = np.array(c["positions"]), np.array(c["faces"])
