@@ -54,6 +54,25 @@ void ctx_release(ddgrs_ctx* ctx) {
   delete ctx;
 }
 
+ddgrs_ctx* ctx_create_aux(const ddgrs_ctx* main) {
+  auto ctx = std::make_unique<ddgrs_ctx>();
+  ctx->device = main->device;
+  ctx->determinism = main->determinism;
+  ctx->prop = main->prop;
+  HIPCHECK(hipSetDevice(main->device));
+  HIPCHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  return ctx.release();
+}
+
+void ctx_merge_times(ddgrs_ctx* main, ddgrs_ctx* ctx) {
+  for (auto& kv : ctx->times) {
+    auto& t = main->times[kv.first];
+    t.launches += kv.second.launches;
+    t.ms += kv.second.ms;
+  }
+  ctx->times.clear();
+}
+
 void surface_retain(ddgrs_surface* s) { ++s->refs; }
 
 void surface_release(ddgrs_surface* s) {

@@ -1009,8 +1009,7 @@ void LprimeFactorDev::setup(ddgrs_ctx* ctx, const LprimeSymbolic& S) {
   ctx->sync();  // the symbolic structure may be released after return
 }
 
-void LprimeFactorDev::factor(ddgrs_surface* s, double eps_rel) {
-  ddgrs_ctx* ctx = s->ctx;
+void LprimeFactorDev::factor(ddgrs_ctx* ctx, ddgrs_surface* s, double eps_rel) {
   const uint32_t unv = (uint32_t)nv;
   const unsigned gv = blocks_for((int64_t)nv, 256);
   // K = W + εM and the barycenter row

@@ -2,8 +2,8 @@
 // connectivity, so the nested-dissection order, the separator tree, the fronts of the multifrontal
 // method and every index map are computed once (LprimeSymbolic, host) and reused while the positions
 // change; each step then computes K's values, assembles the fronts, factors them height by height
-// and forms L's rows and columns and the dense block inverses that the block solves use, all on the
-// device (LprimeFactorDev), which also solves with the factored fronts.  The numbers are those of
+// and forms the dense inverses of the diagonal blocks, all on the device (LprimeFactorDev), which
+// also solves with the factored fronts.  The numbers are those of
 // factor_multifrontal up to rounding (the same eliminations; products summed in another order).
 #pragma once
 #include "precond.hpp"
@@ -106,9 +106,10 @@ struct LprimeFactorDev {
   ~LprimeFactorDev();
   // Uploads the symbolic structure.
   void setup(ddgrs_ctx* ctx, const LprimeSymbolic& sym);
-  // Enqueues K = W + εM at the surface's positions (K's values into kx, the barycenter row into
-  // crow), the factorization of L′ (in the fronts) and the dense inverses of its diagonal blocks.
-  void factor(ddgrs_surface* s, double eps_rel);
+  // Enqueues on ctx's stream K = W + εM at the surface's positions (K's values into kx, the
+  // barycenter row into crow), the factorization of L′ (in the fronts) and the dense inverses of
+  // its diagonal blocks.
+  void factor(ddgrs_ctx* ctx, ddgrs_surface* s, double eps_rel);
   // x = L′⁻¹ r for a full saddle vector (3 nv + 3, interleaved positions then multipliers), the
   // three axes together: forward by height (each front's right-hand side from its block and its
   // children's update vectors, its block solved by the dense inverse, its update vector formed),

@@ -1046,7 +1046,7 @@ void Preconditioner::build(ddgrs_surface* s, double s_exp, double w_tp, double e
     build(s->ctx, plan_preconditioner(s->faces_h, (size_t)s->nv, pos, mass, s_exp, w_tp, eps_rel));
   } else {
     setup_device(s->ctx, lprime_symbolic(s->faces_h, (size_t)s->nv, pos, s->ctx->prop.sharedMemPerBlock));
-    factor_device(s, s_exp, w_tp, eps_rel);
+    factor_device(s->ctx, s, s_exp, w_tp, eps_rel);
     s->ctx->sync();
     check_factor(s->ctx);
   }
@@ -1088,10 +1088,10 @@ void Preconditioner::setup_device(ddgrs_ctx* ctx, const LprimeSymbolic& sym) {
   v.reserve(n);
 }
 
-void Preconditioner::factor_device(ddgrs_surface* s, double s_exp, double w_tp, double eps_rel) {
+void Preconditioner::factor_device(ddgrs_ctx* ctx, ddgrs_surface* s, double s_exp, double w_tp, double eps_rel) {
   if (!fac) throw Error(DDGRS_GPU, "preconditioner", "device factorization used before its symbolic set-up");
   frac_scale = fractional_scale(s_exp, w_tp);
-  fac->factor(s, eps_rel);
+  fac->factor(ctx, s, eps_rel);
   eps = -1.0;  // read on demand
 }
 

@@ -140,7 +140,7 @@ struct Preconditioner {
   // Device factorization: the symbolic structure (once per mesh), then per configuration K, L′'s
   // factor and block inverses (enqueued; check_factor() after the next synchronization).
   void setup_device(ddgrs_ctx* ctx, const LprimeSymbolic& sym);
-  void factor_device(ddgrs_surface* s, double s_exp, double w_tp, double eps_rel);
+  void factor_device(ddgrs_ctx* ctx, ddgrs_surface* s, double s_exp, double w_tp, double eps_rel);
   void check_factor(ddgrs_ctx* ctx);
   double eps_value(ddgrs_ctx* ctx);
   // L̃^V_{2−s} on the vertex BVH `vertex_tree` of `pos` (structure and panels on the device).

@@ -203,6 +203,11 @@ namespace ddgrs {
 // Lifetimes: a context lives until its owner and every surface on it have released it, a surface
 // until its owner and every handle built on it have; the destroy calls may come in any order.
 void ctx_release(ddgrs_ctx* ctx);
+// A second context on the same device with a stream of its own (default priority), for work that
+// a step runs beside its main stream; released with ctx_release.
+ddgrs_ctx* ctx_create_aux(const ddgrs_ctx* main);
+// Adds ctx's per-kernel times into main's and clears them (after both are synchronized).
+void ctx_merge_times(ddgrs_ctx* main, ddgrs_ctx* ctx);
 void surface_retain(ddgrs_surface* s);
 void surface_release(ddgrs_surface* s);
 

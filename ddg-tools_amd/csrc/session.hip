@@ -82,8 +82,14 @@ double seconds_since(std::chrono::steady_clock::time_point t) {
 }
 }  // namespace
 
+Session::~Session() {
+  if (lp_symbolic.valid()) lp_symbolic.wait();
+  if (aux) ctx_release(aux);
+}
+
 void Session::init(ddgrs_surface* s_, const StepOptions& o) {
   s = s_;
+  if (!aux) aux = ctx_create_aux(s->ctx);
   opts = o;
   exps = make_exps(o.alpha, o.beta);
   const size_t nv = (size_t)s->nv;
@@ -186,19 +192,24 @@ void Session::step(StepReport* rep) {
   const HostGeometry geo = download_geometry(s);
   const double s_exp = (exps.beta64 - 2.0) / exps.alpha64;
   mark("geometry");
-  Topology vtopo = build_vertex_tree(s, opts.leaf_size, vert_bvh);
-  mark("vertex tree");
+  // the vertex side on the second stream and another thread: vertex BVH, L′'s factorization,
+  // L̃^V (positions and areas are final on the device: the geometry download synchronized)
+  aux->profiling = ctx->profiling;
+  auto side = std::async(std::launch::async, [&, s_exp] {
+    HIPCHECK(hipSetDevice(aux->device));
+    Topology vtopo = vert_bvh.build(aux, (size_t)nv, s->positions.ptr, s->positions.ptr, s->positions.ptr, opts.leaf_size);
+    if (!lp_ready) {
+      pc.setup_device(aux, lp_symbolic.get());
+      lp_ready = true;
+    }
+    pc.factor_device(aux, s, s_exp, opts.w_tp, opts.eps_rel);
+    pc.build_fractional(aux, geo.pos, geo.vmass, 2.0 - s_exp, opts.chi, opts.leaf_pairs, std::move(vtopo));
+    aux->sync();
+  });
   Topology topo;
   Clusters clusters;
   build_face_tree(s, geo, opts.leaf_size, face_bvh, &topo, &clusters);
   mark("face tree");
-  if (!lp_ready) {
-    pc.setup_device(ctx, lp_symbolic.get());
-    lp_ready = true;
-    mark("L' set-up");
-  }
-  pc.factor_device(s, s_exp, opts.w_tp, opts.eps_rel);
-  mark("L' enqueue");
   if (!opts.energy_exact) bh.build_from(s, topo, clusters, opts.theta, opts.leaf_size);
   mark("Barnes-Hut lists");
   rep->seconds_trees = seconds_since(t);
@@ -230,8 +241,9 @@ void Session::step(StepReport* rep) {
   fm.build(s, topo, clusters, opts.chi, opts.leaf_pairs, s_exp);
   rep->seconds_metric = seconds_since(t);
   t = std::chrono::steady_clock::now();
-  pc.build_fractional(ctx, geo.pos, geo.vmass, 2.0 - s_exp, opts.chi, opts.leaf_pairs, std::move(vtopo));
-  pc.check_factor(ctx);  // the factorization ended before the synchronizations above
+  side.get();  // the second stream is synchronized
+  pc.check_factor(ctx);
+  if (ctx->profiling) ctx_merge_times(ctx, aux);
   rep->seconds_precond = seconds_since(t);
   op.fm = &fm;
   op.crow = pc.crow_dev;

@@ -42,11 +42,16 @@ struct Session {
   Preconditioner pc;
   std::future<LprimeSymbolic> lp_symbolic;  // L′'s symbolic analysis, started at init
   bool lp_ready = false;
+  ddgrs_ctx* aux = nullptr;  // second stream: vertex BVH, L̃^V and L′'s factorization of a step
   SaddleOp op;
   DeviceVecs vec;
   GmresResult last_gmres;
   DevBuf<double> g, sol, p0, partials, rows, scal, mwork, xpart;
 
+  Session() = default;
+  Session(const Session&) = delete;
+  Session& operator=(const Session&) = delete;
+  ~Session();
   // ℓ̄₀ and the barycenter target at the surface's current positions (session.rs:412 new).
   void init(ddgrs_surface* s, const StepOptions& o);
   void step(StepReport* rep);
