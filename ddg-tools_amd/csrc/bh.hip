@@ -96,9 +96,49 @@ __device__ __forceinline__ float isum8(float x) {  // sum over the 8 rows i (lan
 struct BhArgs {
   int n_leaves;
   const uint32_t *leaves, *nstart, *nend;
-  const uint32_t *lumped_off, *lumped, *exact_off, *exact, *exact_t_off, *exact_t;
-  const float4 *xhi, *xlo, *nrm;  // faces, BVH order
-  const float4 *chi, *clo;        // nodes: (X_C hi, A_C), (X_C − hi, 0)
+  const uint32_t *lumped_off, *lumped;
+  const uint32_t *ex_off, *ex_pos;  // per leaf: BVH positions of the faces of its exact nodes (list order)
+  const uint32_t *pt_off, *pt_pos;  // per leaf: BVH positions of the faces of the leaves that open it
+  const float4 *xhi, *xlo, *nrm;    // faces, BVH order
+  const float4 *chi, *clo;          // nodes: (X_C hi, A_C), (X_C − hi, 0)
+};
+
+// Entries [b, e) of a per-leaf list, 8 per step (lane slot j): the list's u32 values are loaded 64 at
+// a time (one per lane) and handed to the slots with a lane shuffle.  U steps at a time: load(v)
+// fetches a step's data for this lane's value v (every lane loads, past-the-end slots from a valid
+// address), then step(data, v, valid, entry) consumes the U steps in order, so that each wave has
+// U steps' loads in flight.  Every lane runs the same steps.
+template <int U, class Load, class Step>
+__device__ __forceinline__ void walk8(const uint32_t* __restrict__ list, uint32_t b, uint32_t e, int lane, int j,
+                                      Load&& load, Step&& step) {
+  using Data = decltype(load(0u));
+  for (uint32_t b0 = b; b0 < e; b0 += 64) {
+    const uint32_t mine = b0 + lane < e ? list[b0 + lane] : 0u;
+    const uint32_t n = min(64u, e - b0);
+    for (uint32_t s = 0; s < n; s += 8 * U) {
+      Data d[U];
+      uint32_t v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const uint32_t src = min(s + 8 * u + j, 63u);
+        v[u] = (uint32_t)__shfl((int)mine, (int)src);
+        d[u] = load(v[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (s + 8 * u < n) step(d[u], v[u], s + 8 * u + j < n, b0 + s + 8 * u + j);
+    }
+  }
+}
+
+struct NodeData {
+  float4 h, l;
+};
+struct FaceData {
+  float4 h, l;
+};
+struct FaceDataN {
+  float4 h, l, n;
 };
 
 template <int MODE>
@@ -116,13 +156,13 @@ __global__ __launch_bounds__(256) void bh_gradient_kernel(BhArgs A, KPow<MODE> p
   const float4 sh = A.xhi[ps], sl = A.xlo[ps], sn = A.nrm[ps];
   const float as = vi ? sh.w : 0.f;
   float own_a = 0.f, ox = 0.f, oy = 0.f, oz = 0.f, onx = 0.f, ony = 0.f, onz = 0.f, en = 0.f;
-  // 1. lumped nodes
-  const uint32_t l0 = A.lumped_off[k], l1 = A.lumped_off[k + 1];
-  for (uint32_t base = l0; base < l1; base += 8) {
-    const uint32_t e = base + j;
-    const bool act = e < l1;
-    const uint32_t C = A.lumped[act ? e : l0];
-    const float4 ch = A.chi[C], cl = A.clo[C];
+  // 1. lumped nodes: lane (i, j) evaluates face S_i against node C of slot j
+  auto load_node = [&](uint32_t C) { return NodeData{A.chi[C], A.clo[C]}; };
+  auto load_face = [&](uint32_t p) { return FaceData{A.xhi[p], A.xlo[p]}; };
+  auto load_face_n = [&](uint32_t p) { return FaceDataN{A.xhi[p], A.xlo[p], A.nrm[p]}; };
+  walk8<4>(A.lumped, A.lumped_off[k], A.lumped_off[k + 1], lane, j, load_node,
+           [&](const NodeData& nd, uint32_t, bool act, uint32_t e) {
+    const float4 ch = nd.h, cl = nd.l;
     const float vx = (sh.x - ch.x) + (sl.x - cl.x);
     const float vy = (sh.y - ch.y) + (sl.y - cl.y);
     const float vz = (sh.z - ch.z) + (sl.z - cl.z);
@@ -143,16 +183,13 @@ __global__ __launch_bounds__(256) void bh_gradient_kernel(BhArgs A, KPow<MODE> p
     const float b = isum8(w * as * kk);
     const float ux = isum8(-w * as * dvx), uy = isum8(-w * as * dvy), uz = isum8(-w * as * dvz);
     if (i == 0 && act) node_bu[e] = make_float4(b, ux, uy, uz);
-  }
-  // 2. exact blocks, own side: pairs (S_i, T_j)
+  });
+  // 2. exact blocks, own side: pairs (S_i, T) over the faces T of the leaf's exact nodes, slot j
   float row = 0.f, dx = 0.f, dy = 0.f, dz = 0.f, nx = 0.f, ny = 0.f, nz = 0.f;
-  for (uint32_t q = A.exact_off[k]; q < A.exact_off[k + 1]; ++q) {
-    const uint32_t C = A.exact[q];
-    const uint32_t t0 = A.nstart[C];
-    const int nt = (int)(A.nend[C] - t0);
-    const bool act = vi && j < nt && !(C == L && i == j);
-    const uint32_t pt = t0 + (j < nt ? j : 0);
-    const float4 th = A.xhi[pt], tl = A.xlo[pt];
+  walk8<4>(A.ex_pos, A.ex_off[k], A.ex_off[k + 1], lane, j, load_face,
+           [&](const FaceData& fd, uint32_t pt, bool valid, uint32_t) {
+    const bool act = vi && valid && pt != ps;
+    const float4 th = fd.h, tl = fd.l;
     const float vx = (sh.x - th.x) + (sl.x - tl.x);
     const float vy = (sh.y - th.y) + (sl.y - tl.y);
     const float vz = (sh.z - th.z) + (sl.z - tl.z);
@@ -167,16 +204,13 @@ __global__ __launch_bounds__(256) void bh_gradient_kernel(BhArgs A, KPow<MODE> p
     nx += at * cn * vx;
     ny += at * cn * vy;
     nz += at * cn * vz;
-  }
-  // 3. exact blocks, partner side: pairs (S'_j ∈ ℓ', T_i ∈ ℓ) for the leaves ℓ' that open ℓ
+  });
+  // 3. exact blocks, partner side: pairs (S', T_i) over the faces S' of the leaves that open ℓ
   float pa = 0.f, px = 0.f, py = 0.f, pz = 0.f;
-  for (uint32_t q = A.exact_t_off[k]; q < A.exact_t_off[k + 1]; ++q) {
-    const uint32_t Ls = A.leaves[A.exact_t[q]];
-    const uint32_t u0 = A.nstart[Ls];
-    const int nu = (int)(A.nend[Ls] - u0);
-    const bool act = vi && j < nu && !(Ls == L && i == j);
-    const uint32_t pu = u0 + (j < nu ? j : 0);
-    const float4 uh = A.xhi[pu], ul = A.xlo[pu], un = A.nrm[pu];
+  walk8<4>(A.pt_pos, A.pt_off[k], A.pt_off[k + 1], lane, j, load_face_n,
+           [&](const FaceDataN& fd, uint32_t pu, bool valid, uint32_t) {
+    const bool act = vi && valid && pu != ps;
+    const float4 uh = fd.h, ul = fd.l, un = fd.n;
     const float vx = (uh.x - sh.x) + (ul.x - sl.x);
     const float vy = (uh.y - sh.y) + (ul.y - sl.y);
     const float vz = (uh.z - sh.z) + (ul.z - sl.z);
@@ -189,7 +223,7 @@ __global__ __launch_bounds__(256) void bh_gradient_kernel(BhArgs A, KPow<MODE> p
     px += w * (cn * un.x - bv * vx);
     py += w * (cn * un.y - bv * vy);
     pz += w * (cn * un.z - bv * vz);
-  }
+  });
   // reduce over j and finish the own terms of S_i
   own_a = xsum8(own_a);
   ox = xsum8(ox); oy = xsum8(oy); oz = xsum8(oz);
@@ -230,32 +264,28 @@ __global__ __launch_bounds__(256) void bh_energy_kernel(BhArgs A, KPow<MODE> pw,
   const float4 sh = A.xhi[ps], sl = A.xlo[ps], sn = A.nrm[ps];
   const float as = vi ? sh.w : 0.f;
   float en = 0.f;
-  const uint32_t l0 = A.lumped_off[k], l1 = A.lumped_off[k + 1];
-  for (uint32_t base = l0; base < l1; base += 8) {
-    const uint32_t e = base + j;
-    const bool act = e < l1;
-    const uint32_t C = A.lumped[act ? e : l0];
-    const float4 ch = A.chi[C], cl = A.clo[C];
+  auto load_node = [&](uint32_t C) { return NodeData{A.chi[C], A.clo[C]}; };
+  auto load_face = [&](uint32_t p) { return FaceData{A.xhi[p], A.xlo[p]}; };
+  walk8<4>(A.lumped, A.lumped_off[k], A.lumped_off[k + 1], lane, j, load_node,
+           [&](const NodeData& nd, uint32_t, bool act, uint32_t) {
+    const float4 ch = nd.h, cl = nd.l;
     const float vx = (sh.x - ch.x) + (sl.x - cl.x);
     const float vy = (sh.y - ch.y) + (sl.y - cl.y);
     const float vz = (sh.z - ch.z) + (sl.z - cl.z);
     const float kk = kval<MODE>(pw, vx, vy, vz, sn.x, sn.y, sn.z, vx * vx + vy * vy + vz * vz);
     en += (act && vi) ? as * ch.w * kk : 0.f;
-  }
-  for (uint32_t q = A.exact_off[k]; q < A.exact_off[k + 1]; ++q) {
-    const uint32_t C = A.exact[q];
-    const uint32_t t0 = A.nstart[C];
-    const int nt = (int)(A.nend[C] - t0);
-    const bool act = vi && j < nt && !(C == L && i == j);
-    const uint32_t pt = t0 + (j < nt ? j : 0);
-    const float4 th = A.xhi[pt], tl = A.xlo[pt];
+  });
+  walk8<4>(A.ex_pos, A.ex_off[k], A.ex_off[k + 1], lane, j, load_face,
+           [&](const FaceData& fd, uint32_t pt, bool valid, uint32_t) {
+    const bool act = vi && valid && pt != ps;
+    const float4 th = fd.h, tl = fd.l;
     const float vx = (sh.x - th.x) + (sl.x - tl.x);
     const float vy = (sh.y - th.y) + (sl.y - tl.y);
     const float vz = (sh.z - th.z) + (sl.z - tl.z);
     const float r2 = act ? (vx * vx + vy * vy + vz * vz) : 1.f;
     const float kk = kval<MODE>(pw, vx, vy, vz, sn.x, sn.y, sn.z, r2);
     en += act ? as * th.w * kk : 0.f;
-  }
+  });
   en = xsum8(en);
   en = isum8(en);
   if (lane == 0) leaf_energy[k] = (double)en;
@@ -463,10 +493,10 @@ BhArgs BhEval::args() const {
   A.nend = d_end.ptr;
   A.lumped_off = d_lumped_off.ptr;
   A.lumped = d_lumped.ptr;
-  A.exact_off = d_exact_off.ptr;
-  A.exact = d_exact.ptr;
-  A.exact_t_off = d_exact_t_off.ptr;
-  A.exact_t = d_exact_t.ptr;
+  A.ex_off = d_ex_off.ptr;
+  A.ex_pos = d_ex_pos.ptr;
+  A.pt_off = d_pt_off.ptr;
+  A.pt_pos = d_pt_pos.ptr;
   A.xhi = pxhi.ptr;
   A.xlo = pxlo.ptr;
   A.nrm = pnrm.ptr;
@@ -622,6 +652,7 @@ void BhEval::build_from(ddgrs_surface* s, const Topology& t, const Clusters& c, 
     lists = BhLists();
     build_lists_device(ctx);
   }
+  flatten_lists(ctx);
   node_bu.reserve(std::max<size_t>(1, n_lumped));
   refresh(s);
 }

@@ -275,4 +275,79 @@ void BhEval::build_lists_device(ddgrs_ctx* ctx) {
          (uint32_t)n_exact, (const uint32_t*)e_tgt_s.ptr, d_exact_t_off.ptr);
 }
 
+// ---- flattened exact lists ---------------------------------------------------------------------
+
+// size[q] = faces of the node of entry q (own side: node exact[q]; partner side: leaf
+// leaves[exact_t[q]]), 0 at q = count.
+__global__ __launch_bounds__(256) void bhl_flat_size_kernel(uint32_t count, const uint32_t* __restrict__ node,
+                                                            const uint32_t* __restrict__ leaves,
+                                                            const uint32_t* __restrict__ start,
+                                                            const uint32_t* __restrict__ end,
+                                                            uint32_t* __restrict__ size) {
+  const uint32_t q = blockIdx.x * 256 + threadIdx.x;
+  if (q > count) return;
+  if (q == count) {
+    size[q] = 0;
+    return;
+  }
+  const uint32_t a = leaves ? leaves[node[q]] : node[q];
+  size[q] = end[a] - start[a];
+}
+
+__global__ __launch_bounds__(256) void bhl_flat_fill_kernel(uint32_t count, const uint32_t* __restrict__ node,
+                                                            const uint32_t* __restrict__ leaves,
+                                                            const uint32_t* __restrict__ start,
+                                                            const uint32_t* __restrict__ end,
+                                                            const uint32_t* __restrict__ fo, uint32_t* __restrict__ pos) {
+  const uint32_t q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= count) return;
+  const uint32_t a = leaves ? leaves[node[q]] : node[q];
+  uint32_t o = fo[q];
+  for (uint32_t p = start[a]; p < end[a]; ++p) pos[o++] = p;
+}
+
+// per leaf: the flat offset of its first entry (k = nl: the total)
+__global__ __launch_bounds__(256) void bhl_flat_off_kernel(uint32_t nl, const uint32_t* __restrict__ entry_off,
+                                                           const uint32_t* __restrict__ fo, uint32_t* __restrict__ off) {
+  const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+  if (k <= nl) off[k] = fo[entry_off[k]];
+}
+
+void BhEval::flatten_lists(ddgrs_ctx* ctx) {
+  const size_t nl1 = (size_t)nl + 1;
+  d_ex_off.reserve(nl1);
+  d_pt_off.reserve(nl1);
+  for (int side = 0; side < 2; ++side) {
+    const uint32_t count = (uint32_t)n_exact;
+    const uint32_t* node = side ? d_exact_t.ptr : d_exact.ptr;
+    const uint32_t* lv = side ? d_leaves.ptr : nullptr;
+    const uint32_t* entry_off = side ? d_exact_t_off.ptr : d_exact_off.ptr;
+    DevBuf<uint32_t>& size = side ? pt_size : ex_size;
+    DevBuf<uint32_t>& fo = side ? pt_fo : ex_fo;
+    DevBuf<uint32_t>& pos = side ? d_pt_pos : d_ex_pos;
+    DevBuf<uint32_t>& off = side ? d_pt_off : d_ex_off;
+    size.reserve((size_t)count + 1);
+    fo.reserve((size_t)count + 1);
+    launch(ctx, "bhl_flat_size", bhl_flat_size_kernel, dim3(blocks_for((int64_t)count + 1, 256)), dim3(256), 0, count,
+           node, lv, (const uint32_t*)d_start.ptr, (const uint32_t*)d_end.ptr, size.ptr);
+    size_t tb = 0;
+    HIPCHECK(rocprim::exclusive_scan(nullptr, tb, size.ptr, fo.ptr, 0u, (size_t)count + 1, rocprim::plus<uint32_t>(),
+                                     ctx->stream));
+    temp.reserve(tb);
+    tb = temp.n;
+    HIPCHECK(rocprim::exclusive_scan(temp.ptr, tb, size.ptr, fo.ptr, 0u, (size_t)count + 1, rocprim::plus<uint32_t>(),
+                                     ctx->stream));
+    // the total, to size the positions
+    uint32_t* pin = (uint32_t*)ctx->staging(sizeof(uint32_t));
+    download(ctx, pin, (const uint32_t*)fo.ptr + count, 1);
+    ctx->sync();
+    const uint32_t total = *pin;
+    pos.reserve(std::max<uint32_t>(total, 1));
+    launch(ctx, "bhl_flat_fill", bhl_flat_fill_kernel, dim3(blocks_for(count, 256)), dim3(256), 0, count, node, lv,
+           (const uint32_t*)d_start.ptr, (const uint32_t*)d_end.ptr, (const uint32_t*)fo.ptr, pos.ptr);
+    launch(ctx, "bhl_flat_off", bhl_flat_off_kernel, dim3(blocks_for((int64_t)nl1, 256)), dim3(256), 0, (uint32_t)nl,
+           entry_off, (const uint32_t*)fo.ptr, off.ptr);
+  }
+}
+
 }  // namespace ddgrs
