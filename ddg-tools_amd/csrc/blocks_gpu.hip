@@ -11,8 +11,9 @@
 // Per level: a classification kernel, a scan of the (admissible, near, children) counts and a
 // scatter kernel; the host reads the three totals.  Grouping: a stable radix sort by row node and a
 // binary search for the row pointers.  Panels: per leaf the column ranges of the near blocks of the
-// leaf and its ancestors, sorted by first column (the ranges of one leaf are disjoint: the blocks of
-// a chain partition the columns), then the column lists and the panel table.
+// leaf and its ancestors (each near block adds its column range to the leaves under its row node, a
+// consecutive range of leaf ordinals), sorted by first column (the ranges of one leaf are disjoint:
+// the blocks of a chain partition the columns), then the column lists and the panel table.
 #include <cstring>
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
@@ -165,61 +166,62 @@ __global__ __launch_bounds__(256) void blk_stats_kernel(uint32_t m, const uint32
   }
 }
 
-// Per leaf ordinal k < nl: near ranges, panel columns, panels and stored entries; zeros at k = nl
-// (the scans then end with the totals).
+// Each near block (A, B) gives every leaf under A (leaf ordinals [lf_A, le_A)) the column range of B:
+// per leaf the number of ranges and of columns (integer atomic sums, independent of order).
+__global__ __launch_bounds__(256) void blk_near_count_kernel(uint32_t n_near, const uint32_t* __restrict__ near_row,
+                                                             const uint32_t* __restrict__ near_col,
+                                                             const uint32_t* __restrict__ start,
+                                                             const uint32_t* __restrict__ end,
+                                                             const uint32_t* __restrict__ leaf_first,
+                                                             const uint32_t* __restrict__ leaf_end,
+                                                             uint32_t* __restrict__ rcount,
+                                                             uint32_t* __restrict__ kcount) {
+  const uint32_t q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= n_near) return;
+  const uint32_t a = near_row[q], b = near_col[q], w = end[b] - start[b];
+  for (uint32_t k = leaf_first[a]; k < leaf_end[a]; ++k) {
+    atomicAdd(&rcount[k], 1u);
+    atomicAdd(&kcount[k], w);
+  }
+}
+
+// Per leaf ordinal k < nl: panels and stored entries; zeros at k = nl (the scans then end with the
+// totals).
 __global__ __launch_bounds__(256) void blk_panel_count_kernel(uint32_t nl, const uint32_t* __restrict__ leaves,
-                                                              const uint32_t* __restrict__ parent,
                                                               const uint32_t* __restrict__ start,
                                                               const uint32_t* __restrict__ end,
-                                                              const uint32_t* __restrict__ near_ptr,
-                                                              const uint32_t* __restrict__ near_col,
-                                                              uint32_t* __restrict__ rcount,
-                                                              uint32_t* __restrict__ kcount,
+                                                              const uint32_t* __restrict__ kcount,
                                                               uint32_t* __restrict__ pcount,
                                                               uint64_t* __restrict__ entries) {
   const uint32_t k = blockIdx.x * 256 + threadIdx.x;
   if (k > nl) return;
   if (k == nl) {
-    rcount[k] = kcount[k] = pcount[k] = 0;
+    pcount[k] = 0;
     entries[k] = 0;
     return;
   }
-  const uint32_t leaf = leaves[k];
-  uint32_t R = 0, K = 0;
-  for (uint32_t a = leaf;; a = parent[a]) {
-    for (uint32_t q = near_ptr[a]; q < near_ptr[a + 1]; ++q) {
-      const uint32_t b = near_col[q];
-      K += end[b] - start[b];
-      ++R;
-    }
-    if (a == 0) break;
-  }
-  const uint32_t ns = end[leaf] - start[leaf];
-  rcount[k] = R;
-  kcount[k] = K;
+  const uint32_t leaf = leaves[k], ns = end[leaf] - start[leaf];
   pcount[k] = (ns + kPanelRows - 1) / kPanelRows;
-  entries[k] = (uint64_t)ns * K;
+  entries[k] = (uint64_t)ns * kcount[k];
 }
 
-__global__ __launch_bounds__(256) void blk_range_fill_kernel(uint32_t nl, const uint32_t* __restrict__ leaves,
-                                                             const uint32_t* __restrict__ parent,
+// The ranges of each leaf, in any order (a per-leaf cursor); they are sorted by first column next.
+__global__ __launch_bounds__(256) void blk_range_fill_kernel(uint32_t n_near, const uint32_t* __restrict__ near_row,
+                                                             const uint32_t* __restrict__ near_col,
                                                              const uint32_t* __restrict__ start,
                                                              const uint32_t* __restrict__ end,
-                                                             const uint32_t* __restrict__ near_ptr,
-                                                             const uint32_t* __restrict__ near_col,
+                                                             const uint32_t* __restrict__ leaf_first,
+                                                             const uint32_t* __restrict__ leaf_end,
                                                              const uint32_t* __restrict__ range_ptr,
-                                                             uint64_t* __restrict__ key, uint32_t* __restrict__ rend) {
-  const uint32_t k = blockIdx.x * 256 + threadIdx.x;
-  if (k >= nl) return;
-  uint32_t o = range_ptr[k];
-  for (uint32_t a = leaves[k];; a = parent[a]) {
-    for (uint32_t q = near_ptr[a]; q < near_ptr[a + 1]; ++q) {
-      const uint32_t b = near_col[q];
-      key[o] = ((uint64_t)k << 32) | start[b];
-      rend[o] = end[b];
-      ++o;
-    }
-    if (a == 0) break;
+                                                             uint32_t* __restrict__ cursor, uint64_t* __restrict__ key,
+                                                             uint32_t* __restrict__ rend) {
+  const uint32_t q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= n_near) return;
+  const uint32_t a = near_row[q], b = near_col[q];
+  for (uint32_t k = leaf_first[a]; k < leaf_end[a]; ++k) {
+    const uint32_t o = range_ptr[k] + atomicAdd(&cursor[k], 1u);
+    key[o] = ((uint64_t)k << 32) | start[b];
+    rend[o] = end[b];
   }
 }
 
@@ -410,10 +412,33 @@ BlockStats BlockTreeDev::build_structure(ddgrs_ctx* ctx, const Topology& t, cons
   S.pan_ptr.reserve(nl1);
   S.val_base.reserve(nl1);
   col_ptr.reserve(nl1);
+  // leaf ordinal ranges of the nodes (pre-order: a node's leaves are consecutive)
+  {
+    std::vector<uint32_t> lf(nn), le(nn);
+    for (int x = nn; x-- > 0;) {
+      if (t.right[x] == 0) {
+        lf[x] = t.leaf_ordinal[x];
+        le[x] = t.leaf_ordinal[x] + 1;
+      } else {
+        lf[x] = lf[x + 1];
+        le[x] = le[t.right[x]];
+      }
+    }
+    ddgrs::upload(ctx, S.leaf_first, lf);
+    ddgrs::upload(ctx, S.leaf_end, le);
+    S.cursor.reserve(nl1);
+    HIPCHECK(hipMemsetAsync(S.rcount.ptr, 0, nl1 * sizeof(uint32_t), ctx->stream));
+    HIPCHECK(hipMemsetAsync(S.kcount.ptr, 0, nl1 * sizeof(uint32_t), ctx->stream));
+    HIPCHECK(hipMemsetAsync(S.cursor.ptr, 0, nl1 * sizeof(uint32_t), ctx->stream));
+    launch(ctx, "blk_near_count", blk_near_count_kernel, dim3(blocks_for(n_near, 256)), dim3(256), 0, n_near,
+           (const uint32_t*)S.near_row.ptr, (const uint32_t*)S.near_col.ptr, (const uint32_t*)start.ptr,
+           (const uint32_t*)end.ptr, (const uint32_t*)S.leaf_first.ptr, (const uint32_t*)S.leaf_end.ptr, S.rcount.ptr,
+           S.kcount.ptr);
+    ctx->sync();  // lf, le are released on return
+  }
   launch(ctx, "blk_panel_count", blk_panel_count_kernel, dim3(blocks_for(nl1, 256)), dim3(256), 0, (uint32_t)nl,
-         (const uint32_t*)leaves.ptr, (const uint32_t*)parent.ptr, (const uint32_t*)start.ptr, (const uint32_t*)end.ptr,
-         (const uint32_t*)S.near_ptr.ptr, (const uint32_t*)S.near_col.ptr, S.rcount.ptr, S.kcount.ptr, S.pcount.ptr,
-         S.entries.ptr);
+         (const uint32_t*)leaves.ptr, (const uint32_t*)start.ptr, (const uint32_t*)end.ptr, (const uint32_t*)S.kcount.ptr,
+         S.pcount.ptr, S.entries.ptr);
   scan_exclusive<uint32_t>(ctx, S.temp, S.rcount.ptr, S.range_ptr.ptr, nl1);
   scan_exclusive<uint32_t>(ctx, S.temp, S.kcount.ptr, col_ptr.ptr, nl1);
   scan_exclusive<uint32_t>(ctx, S.temp, S.pcount.ptr, S.pan_ptr.ptr, nl1);
@@ -438,10 +463,10 @@ BlockStats BlockTreeDev::build_structure(ddgrs_ctx* ctx, const Topology& t, cons
   S.range_key_s.reserve(nr);
   S.range_end.reserve(nr);
   S.range_end_s.reserve(nr);
-  launch(ctx, "blk_range_fill", blk_range_fill_kernel, dim3(blocks_for(nl, 256)), dim3(256), 0, (uint32_t)nl,
-         (const uint32_t*)leaves.ptr, (const uint32_t*)parent.ptr, (const uint32_t*)start.ptr, (const uint32_t*)end.ptr,
-         (const uint32_t*)S.near_ptr.ptr, (const uint32_t*)S.near_col.ptr, (const uint32_t*)S.range_ptr.ptr,
-         S.range_key.ptr, S.range_end.ptr);
+  launch(ctx, "blk_range_fill", blk_range_fill_kernel, dim3(blocks_for(n_near, 256)), dim3(256), 0, n_near,
+         (const uint32_t*)S.near_row.ptr, (const uint32_t*)S.near_col.ptr, (const uint32_t*)start.ptr,
+         (const uint32_t*)end.ptr, (const uint32_t*)S.leaf_first.ptr, (const uint32_t*)S.leaf_end.ptr,
+         (const uint32_t*)S.range_ptr.ptr, S.cursor.ptr, S.range_key.ptr, S.range_end.ptr);
   if (T.ranges)
     sort_pairs<uint64_t>(ctx, S.temp, S.range_key.ptr, S.range_key_s.ptr, S.range_end.ptr, S.range_end_s.ptr, T.ranges,
                          32 + bits_for((uint64_t)nl));

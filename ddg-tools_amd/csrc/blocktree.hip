@@ -32,6 +32,42 @@ __device__ __forceinline__ float wsum64(float x) {
   x += __shfl_xor(x, 1);
   return x;
 }
+
+// Sums N per-lane values over the 64 lanes of a wave, the work halved at each step: across lane
+// bit MASK a lane keeps one half of its values and adds the partner's copy of that half (the
+// partner keeps the other), until N is odd; the rest is summed by a butterfly.  Afterwards lane L
+// holds the sums of values [base, base + n) in v[0, n), with n and base from rs_slots.
+template <int N, int MASK>
+__device__ __forceinline__ void reduce_scatter(float* v, int lane) {
+  if constexpr (MASK >= 1) {
+    if constexpr (N % 2 == 0) {
+      const bool up = (lane & MASK) != 0;
+#pragma unroll
+      for (int i = 0; i < N / 2; ++i) {
+        const float keep = up ? v[N / 2 + i] : v[i];
+        const float send = up ? v[i] : v[N / 2 + i];
+        v[i] = keep + __shfl_xor(send, MASK);
+      }
+      reduce_scatter<N / 2, MASK / 2>(v, lane);
+    } else {
+#pragma unroll
+      for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int m = MASK; m >= 1; m >>= 1) v[i] += __shfl_xor(v[i], m);
+    }
+  }
+}
+// The values that lane L holds after reduce_scatter<N, 32>: count n and first index base.
+template <int N, int MASK>
+__device__ __forceinline__ void rs_slots(int lane, int& n, int& base) {
+  if constexpr (MASK >= 1 && N % 2 == 0) {
+    rs_slots<N / 2, MASK / 2>(lane, n, base);
+    if (lane & MASK) base += N / 2;
+  } else {
+    n = N;
+    base = 0;
+  }
+}
 __device__ __forceinline__ double wsum64d(double x) {
   x += __shfl_xor(x, 32);
   x += __shfl_xor(x, 16);
@@ -228,39 +264,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void b
       }
     }
   }
-  // pairwise reduction over the wave; output (r, c) goes to lane (r C + c) mod 64
-  float mine0 = 0.f, mine1 = 0.f;
-#pragma unroll
-  for (int r = 0; r < (int)kPanelRows; ++r) {
-    if ((uint32_t)r < R) {
-#pragma unroll
-      for (int c = 0; c < C; ++c) {
-        const float t = wsum64(acc[r][c]);
-        const int idx = r * C + c;
-        if (idx < 64) {
-          if (lane == idx) mine0 = t;
-        } else {
-          if (lane == idx - 64) mine1 = t;
-        }
-      }
-    }
-  }
+  // the sums over the wave: reduce-scatter, then each lane whose low bits are 0 finishes its values
+  constexpr int NV = (int)kPanelRows * C;
+  float* flat = &acc[0][0];
+  reduce_scatter<NV, 32>(flat, lane);
+  int nmine, base;
+  rs_slots<NV, 32>(lane, nmine, base);
   const uint32_t leaf = P.leaves[k];
+  // lanes that differ only in the bits the butterfly summed over hold the same sums: the lowest writes
+  int used = 0;
+  for (int m = 32, nn = NV; m >= 1 && nn % 2 == 0; m >>= 1, nn /= 2) used |= m;
+  if ((lane & ~used & 63) == 0) {
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const int idx = lane + 64 * h;
-    if (idx >= (int)R * C) continue;
-    const int r = idx / C, c = idx % C;
-    const int kern = c < H ? 0 : NK - 1;
-    const uint32_t srow = row0 + r;
-    const float* q = u_hl + (size_t)srow * 2 * C;
-    const double us = (double)q[c] + (double)q[C + c];
-    const double far = P.alpha_leaf[(size_t)k * NK + kern] * (us - P.mean[(size_t)leaf * C + c]) +
-                       P.delta[(size_t)k * C + c];
-    const double z = (c < H ? w0 : w1) * ((double)(h == 0 ? mine0 : mine1) + far);
-    const float zh = (float)z;
-    z_hl[(size_t)srow * 2 * C + c] = zh;
-    z_hl[(size_t)srow * 2 * C + C + c] = (float)(z - (double)zh);
+    for (int q = 0; q < NV; ++q) {
+      if (q >= nmine) break;
+      const int idx = base + q;
+      const int r = idx / C, c = idx % C;
+      if ((uint32_t)r >= R) continue;
+      const int kern = c < H ? 0 : NK - 1;
+      const uint32_t srow = row0 + r;
+      const float* qv = u_hl + (size_t)srow * 2 * C;
+      const double us = (double)qv[c] + (double)qv[C + c];
+      const double far = P.alpha_leaf[(size_t)k * NK + kern] * (us - P.mean[(size_t)leaf * C + c]) +
+                         P.delta[(size_t)k * C + c];
+      const double z = (c < H ? w0 : w1) * ((double)flat[q] + far);
+      const float zh = (float)z;
+      z_hl[(size_t)srow * 2 * C + c] = zh;
+      z_hl[(size_t)srow * 2 * C + C + c] = (float)(z - (double)zh);
+    }
   }
 }
 

@@ -38,7 +38,7 @@ struct BlockCounts {
 // Device scratch of a structure build (kept between builds).
 struct BlockBuildScratch {
   DevBuf<uint32_t> pa, pb, qa, qb, adm_a, adm_b, near_a, near_b, near_row, near_col, near_ptr;
-  DevBuf<uint32_t> rcount, kcount, pcount, range_ptr, pan_ptr, range_end, range_end_s;
+  DevBuf<uint32_t> rcount, kcount, pcount, range_ptr, pan_ptr, range_end, range_end_s, leaf_first, leaf_end, cursor;
   DevBuf<uint64_t> entries, val_base, range_key, range_key_s;
   DevBuf<double> lo, hi, radius;
   DevBuf<uint8_t> cls, temp;
