@@ -145,7 +145,7 @@ template <int MODE>
 __global__ __launch_bounds__(256) void bh_gradient_kernel(BhArgs A, KPow<MODE> pw, float4* __restrict__ node_bu,
                                                           float* __restrict__ part, double* __restrict__ leaf_energy) {
   const int lane = threadIdx.x & 63;
-  const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int k = (int)xcd_wave_id();
   if (k >= A.n_leaves) return;  // whole waves exit together
   const int i = lane >> 3, j = lane & 7;
   const uint32_t L = A.leaves[k];
@@ -253,7 +253,7 @@ __global__ __launch_bounds__(256) void bh_gradient_kernel(BhArgs A, KPow<MODE> p
 template <int MODE>
 __global__ __launch_bounds__(256) void bh_energy_kernel(BhArgs A, KPow<MODE> pw, double* __restrict__ leaf_energy) {
   const int lane = threadIdx.x & 63;
-  const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int k = (int)xcd_wave_id();
   if (k >= A.n_leaves) return;
   const int i = lane >> 3, j = lane & 7;
   const uint32_t L = A.leaves[k];

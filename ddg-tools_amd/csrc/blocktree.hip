@@ -221,7 +221,7 @@ struct PanelArgs {
 template <int C, int H, int NK>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void bt_panel_kernel(PanelArgs P, const float* __restrict__ u_hl, double w0, double w1,
                                                        float* __restrict__ z_hl) {
-  const uint32_t p = wave_id();
+  const uint32_t p = xcd_wave_id();
   const int lane = threadIdx.x & 63;
   if (p >= (uint32_t)P.np) return;
   const uint32_t k = P.panel_leaf[p], row0 = P.panel_row0[p], R = P.panel_rows[p];
@@ -518,7 +518,7 @@ __global__ __launch_bounds__(256) void fm_panel_values_kernel(PanelArgs P, const
                                                               const double* __restrict__ nrm,
                                                               const double* __restrict__ area, double neg_s,
                                                               float* __restrict__ val) {
-  const uint32_t p = wave_id();
+  const uint32_t p = xcd_wave_id();
   const int lane = threadIdx.x & 63;
   if (p >= (uint32_t)P.np) return;
   const uint32_t k = P.panel_leaf[p], row0 = P.panel_row0[p], R = P.panel_rows[p];
@@ -762,7 +762,7 @@ __global__ __launch_bounds__(256) void vf_panel_values_kernel(PanelArgs P, const
                                                               const double* __restrict__ pos,
                                                               const double* __restrict__ mass, double ex,
                                                               float* __restrict__ val) {
-  const uint32_t p = wave_id();
+  const uint32_t p = xcd_wave_id();
   const int lane = threadIdx.x & 63;
   if (p >= (uint32_t)P.np) return;
   const uint32_t k = P.panel_leaf[p], row0 = P.panel_row0[p], R = P.panel_rows[p];

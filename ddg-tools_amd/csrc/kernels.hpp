@@ -7,6 +7,19 @@ namespace ddgrs {
 
 inline Scratch& scratch_of(ddgrs_ctx* ctx) { return ctx->scratch; }
 
+// MI355X places consecutive workgroups on its 8 XCDs in turn, each XCD with its own L2.  For
+// work ordered by locality (BVH leaves, panels) this gives each XCD a contiguous eighth of the
+// blocks instead: a bijection of [0, n) taking the blocks that run on XCD x to
+// [x·⌊n/8⌋ + min(x, n mod 8), …).
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t n) {
+  const uint32_t x = b & 7u, i = b >> 3, per = n >> 3, rem = n & 7u;
+  return x * per + (x < rem ? x : rem) + i;
+}
+// The wave's index in a 256-thread launch with XCD-contiguous blocks (wave-uniform).
+__device__ __forceinline__ uint32_t xcd_wave_id() {
+  return __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
+}
+
 // Exponents as the kernels take them (RS/src/kernel.rs:26 Exponents): integer exponents use
 // products, others powf.
 struct Exps {
