@@ -13,6 +13,8 @@
 // Own and partner sums are reduced over j by an xor butterfly (bitwise identical on all lanes,
 // independent of scheduling).  Pair arithmetic is FP32 with the hi/lo barycenter difference; the
 // node sums, the downward pass and the per-face epilogue are FP64.
+#include <chrono>
+#include <cstdio>
 #include <cstdlib>
 
 #include "bh.hpp"
@@ -601,8 +603,15 @@ void build_face_tree(ddgrs_surface* s, const HostGeometry& g, size_t leaf_size, 
   b.box_hi.reserve(3 * nf_);
   launch(ctx, "face_boxes", face_boxes_kernel, dim3(blocks_for(s->nf, 256)), dim3(256), 0, s->nf,
          (const uint32_t*)s->faces.ptr, (const double*)s->positions.ptr, b.box_lo.ptr, b.box_hi.ptr);
+  static const bool trace = std::getenv("DDGRS_TRACE_STEP") != nullptr;
+  auto t0 = std::chrono::steady_clock::now();
   *topo = b.build(ctx, nf_, s->bary.ptr, b.box_lo.ptr, b.box_hi.ptr, leaf_size);
+  auto t1 = std::chrono::steady_clock::now();
   *clusters = build_clusters(*topo, g.bary.data(), g.lo.data(), g.hi.data(), g.area.data(), g.nrm.data());
+  if (trace)
+    std::fprintf(stderr, "step:   face BVH %.2f ms, clusters %.2f ms\n",
+                 std::chrono::duration<double>(t1 - t0).count() * 1e3,
+                 std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count() * 1e3);
 }
 
 void build_face_tree(ddgrs_surface* s, size_t leaf_size, Topology* topo, Clusters* clusters) {

@@ -669,9 +669,8 @@ __global__ __launch_bounds__(256) void fm_inc_kernel(uint32_t m, const uint32_t*
 namespace {
 // Far coefficients, α, near values and the per-face data of a face metric whose block tree is on
 // the device.
-void face_metric_values(ddgrs_surface* s, FaceMetricDev& F, const std::vector<double>& centroid,
+void face_metric_values(ddgrs_ctx* ctx, ddgrs_surface* s, FaceMetricDev& F, const std::vector<double>& centroid,
                         const std::vector<double>& projector, const std::vector<double>& node_mass, double s_exp) {
-  ddgrs_ctx* ctx = s->ctx;
   BlockTreeDev& bt = F.bt;
   DevBuf<double> cen, proj, nmass;
   ddgrs::upload(ctx, cen, centroid);
@@ -702,7 +701,11 @@ void face_metric_values(ddgrs_surface* s, FaceMetricDev& F, const std::vector<do
 
 void FaceMetricDev::build(ddgrs_surface* s, const Topology& t, const Clusters& cl, double chi, size_t leaf_pairs,
                           double s_exp) {
-  ddgrs_ctx* ctx = s->ctx;
+  build(s->ctx, s, t, cl, chi, leaf_pairs, s_exp);
+}
+
+void FaceMetricDev::build(ddgrs_ctx* ctx, ddgrs_surface* s, const Topology& t, const Clusters& cl, double chi,
+                          size_t leaf_pairs, double s_exp) {
   topo = t;
   nv = (int)s->nv;
   nf = (int)s->nf;
@@ -713,7 +716,7 @@ void FaceMetricDev::build(ddgrs_surface* s, const Topology& t, const Clusters& c
   inc.reserve(3 * (size_t)nf);
   launch(ctx, "fm_inc", fm_inc_kernel, dim3(blocks_for(3 * (int64_t)nf, 256)), dim3(256), 0, (uint32_t)(3 * nf),
          (const uint32_t*)bt.position.ptr, (const uint32_t*)s->inc.ptr, inc.ptr);
-  face_metric_values(s, *this, cl.centroid, cl.projector, cl.mass, s_exp);
+  face_metric_values(ctx, s, *this, cl.centroid, cl.projector, cl.mass, s_exp);
 }
 
 void FaceMetricDev::build(ddgrs_surface* s, const FaceMetricPlan& P, double s_exp) {
@@ -725,7 +728,7 @@ void FaceMetricDev::build(ddgrs_surface* s, const FaceMetricPlan& P, double s_ex
   bt.upload(ctx, P.bt);
   ddgrs::upload(ctx, inc_ptr, P.inc_ptr);
   ddgrs::upload(ctx, inc, P.inc);
-  face_metric_values(s, *this, P.centroid, P.projector, P.node_mass, s_exp);
+  face_metric_values(ctx, s, *this, P.centroid, P.projector, P.node_mass, s_exp);
 }
 
 void FaceMetricDev::apply(ddgrs_ctx* ctx, const double* x, double* y) {

@@ -123,6 +123,9 @@ struct FaceMetricDev {
   // Builds on a face tree of the surface's current positions with separation χ and leaf_pairs
   // (structure and panels on the device).
   void build(ddgrs_surface* s, const Topology& t, const Clusters& c, double chi, size_t leaf_pairs, double s_exp);
+  // The same on ctx's stream (the surface's device data must be current there).
+  void build(ddgrs_ctx* ctx, ddgrs_surface* s, const Topology& t, const Clusters& c, double chi, size_t leaf_pairs,
+             double s_exp);
   // The same from a host plan.
   void build(ddgrs_surface* s, const FaceMetricPlan& p, double s_exp);
   // y = (w_hi B̃ + w_lo B̃⁰) x for one interleaved 3 nv FP64 vector (device pointers).

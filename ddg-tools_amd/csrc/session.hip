@@ -85,11 +85,13 @@ double seconds_since(std::chrono::steady_clock::time_point t) {
 Session::~Session() {
   if (lp_symbolic.valid()) lp_symbolic.wait();
   if (aux) ctx_release(aux);
+  if (aux2) ctx_release(aux2);
 }
 
 void Session::init(ddgrs_surface* s_, const StepOptions& o) {
   s = s_;
   if (!aux) aux = ctx_create_aux(s->ctx);
+  if (!aux2) aux2 = ctx_create_aux(s->ctx);
   opts = o;
   exps = make_exps(o.alpha, o.beta);
   const size_t nv = (size_t)s->nv;
@@ -210,6 +212,18 @@ void Session::step(StepReport* rep) {
   Clusters clusters;
   build_face_tree(s, geo, opts.leaf_size, face_bvh, &topo, &clusters);
   mark("face tree");
+  // the face metric's block tree on the third stream and another thread (reads the face tree and
+  // the surface's geometry, which stay unchanged until the line search)
+  fm.w_hi = opts.w_tp;
+  fm.w_lo = opts.w_tp * opts.c_lo;
+  aux2->profiling = ctx->profiling;
+  auto metric = std::async(std::launch::async, [&, s_exp] {
+    HIPCHECK(hipSetDevice(aux2->device));
+    const auto t0 = std::chrono::steady_clock::now();
+    fm.build(aux2, s, topo, clusters, opts.chi, opts.leaf_pairs, s_exp);
+    aux2->sync();
+    return seconds_since(t0);
+  });
   if (!opts.energy_exact) bh.build_from(s, topo, clusters, opts.theta, opts.leaf_size);
   mark("Barnes-Hut lists");
   rep->seconds_trees = seconds_since(t);
@@ -236,9 +250,10 @@ void Session::step(StepReport* rep) {
   }
   // 2. the solver of A′ and the direction x = A′⁻¹ [g; 0]
   t = std::chrono::steady_clock::now();
-  fm.w_hi = opts.w_tp;
-  fm.w_lo = opts.w_tp * opts.c_lo;
-  fm.build(s, topo, clusters, opts.chi, opts.leaf_pairs, s_exp);
+  const double metric_build = metric.get();  // the third stream is synchronized
+  if (trace) std::fprintf(stderr, "step: %-18s %8.2f ms (wait %.2f ms)\n", "metric build", metric_build * 1e3,
+                          seconds_since(t) * 1e3);
+  if (ctx->profiling) ctx_merge_times(ctx, aux2);
   rep->seconds_metric = seconds_since(t);
   t = std::chrono::steady_clock::now();
   side.get();  // the second stream is synchronized

@@ -42,7 +42,8 @@ struct Session {
   Preconditioner pc;
   std::future<LprimeSymbolic> lp_symbolic;  // L′'s symbolic analysis, started at init
   bool lp_ready = false;
-  ddgrs_ctx* aux = nullptr;  // second stream: vertex BVH, L̃^V and L′'s factorization of a step
+  ddgrs_ctx* aux = nullptr;   // second stream: vertex BVH, L̃^V and L′'s factorization of a step
+  ddgrs_ctx* aux2 = nullptr;  // third stream: the face metric's block tree of a step
   SaddleOp op;
   DeviceVecs vec;
   GmresResult last_gmres;
