@@ -29,12 +29,12 @@ struct BhEval {
   DevBuf<double4> sums;
   DevBuf<float> part;
   // device list build
-  DevBuf<uint32_t> d_esc, d_leaf_ordinal, d_parent, d_task_root, d_task_depth, l_count, e_count, l_task_off, e_task_off;
+  DevBuf<uint32_t> d_leaf_ordinal, d_parent, d_task_root, d_task_depth, l_count, e_count, l_task_off, e_task_off;
   DevBuf<uint32_t> l_idx, l_key_s, e_tgt, e_src, e_tgt_s;
   // flattened exact lists: per leaf the BVH positions of the faces of its exact nodes (own side)
   // and of the leaves that open it (partner side), with per-entry sizes and offsets
   DevBuf<uint32_t> d_ex_off, d_ex_pos, d_pt_off, d_pt_pos, ex_size, ex_fo, pt_size, pt_fo;
-  DevBuf<double> d_nlo, d_nhi, d_nrad;
+  DevBuf<uint8_t> d_bhnode;  // per node a 64-byte record for the list walks
   DevBuf<unsigned long long> d_pairs;
   DevBuf<uint8_t> temp;
 

@@ -28,20 +28,38 @@ namespace ddgrs {
 #pragma clang fp contract(off)
 
 namespace {
+// A node as the walks read it: box, radius, right child and escape index in one 64-byte record
+// (one cache line per visited node).
+struct alignas(64) BhNode {
+  double lo[3], hi[3], radius;
+  uint32_t right, esc;
+};
+static_assert(sizeof(BhNode) == 64, "one cache line per node");
+
 struct BhTree {
-  const uint32_t *right, *esc, *start, *end, *leaves;
-  const double *lo, *hi, *radius;
+  const BhNode* node;
+  const uint32_t *start, *end, *leaves;
   uint32_t nn;
   double th2;
 };
 
-__device__ __forceinline__ bool bh_lumped(const BhTree& T, uint32_t l, uint32_t a) {
+// The walking leaf's box and radius.
+struct LeafBox {
+  double lo[3], hi[3], radius;
+};
+
+__device__ __forceinline__ LeafBox leaf_box(const BhTree& T, uint32_t l) {
+  const BhNode& n = T.node[l];
+  return {{n.lo[0], n.lo[1], n.lo[2]}, {n.hi[0], n.hi[1], n.hi[2]}, n.radius};
+}
+
+__device__ __forceinline__ bool bh_lumped(const BhTree& T, const LeafBox& L, const BhNode& A) {
   double d2 = 0.0;
   for (int k = 0; k < 3; ++k) {
-    const double d = fmax(fmax(T.lo[3 * l + k], T.lo[3 * a + k]) - fmin(T.hi[3 * l + k], T.hi[3 * a + k]), 0.0);
+    const double d = fmax(fmax(L.lo[k], A.lo[k]) - fmin(L.hi[k], A.hi[k]), 0.0);
     d2 += d * d;
   }
-  const double h = fmax(T.radius[l], T.radius[a]);
+  const double h = fmax(L.radius, A.radius);
   return h * h < T.th2 * d2;
 }
 }  // namespace
@@ -59,6 +77,7 @@ struct BhTasks {
 template <class Emit>
 __device__ __forceinline__ void bh_task_walk(const BhTree& T, const BhTasks& W, uint32_t k, uint32_t j, Emit&& emit) {
   const uint32_t l = T.leaves[k], c = W.root[j], m = W.depth[j];
+  const LeafBox L = leaf_box(T, l);
   uint32_t path[kMaxCutDepth];
   {
     uint32_t x = c;
@@ -69,20 +88,22 @@ __device__ __forceinline__ void bh_task_walk(const BhTree& T, const BhTasks& W, 
   }
   for (uint32_t i = 0; i < m; ++i) {
     const uint32_t a = path[i];
-    const bool lum = bh_lumped(T, l, a);
-    if (lum || T.right[a] == 0) {
+    const BhNode A = T.node[a];
+    const bool lum = bh_lumped(T, L, A);
+    if (lum || A.right == 0) {
       if (c == a + (m - i)) emit(a, lum);
       return;
     }
   }
-  const uint32_t stop = T.esc[c];
+  const uint32_t stop = T.node[c].esc;
   for (uint32_t a = c; a < stop;) {
-    if (bh_lumped(T, l, a)) {
+    const BhNode A = T.node[a];
+    if (bh_lumped(T, L, A)) {
       emit(a, true);
-      a = T.esc[a];
-    } else if (T.right[a] == 0) {
+      a = A.esc;
+    } else if (A.right == 0) {
       emit(a, false);
-      a = T.esc[a];
+      a = A.esc;
     } else {
       a = a + 1;
     }
@@ -200,16 +221,24 @@ void BhEval::build_lists_device(ddgrs_ctx* ctx) {
       tdepth.push_back(t.depth[a]);
     }
   const uint32_t ntask = (uint32_t)troot.size();
-  ddgrs::upload(ctx, d_esc, esc);
+  std::vector<BhNode> nodes(nn);
+  for (int a = 0; a < nn; ++a) {
+    BhNode& r = nodes[a];
+    for (int k = 0; k < 3; ++k) {
+      r.lo[k] = clusters.lo[3 * (size_t)a + k];
+      r.hi[k] = clusters.hi[3 * (size_t)a + k];
+    }
+    r.radius = clusters.radius[a];
+    r.right = t.right[a];
+    r.esc = esc[a];
+  }
+  d_bhnode.reserve((size_t)nn * sizeof(BhNode));
+  HIPCHECK(hipMemcpyAsync(d_bhnode.ptr, nodes.data(), (size_t)nn * sizeof(BhNode), hipMemcpyHostToDevice, ctx->stream));
   ddgrs::upload(ctx, d_leaf_ordinal, t.leaf_ordinal);
   ddgrs::upload(ctx, d_parent, t.parent);
   ddgrs::upload(ctx, d_task_root, troot);
   ddgrs::upload(ctx, d_task_depth, tdepth);
-  ddgrs::upload(ctx, d_nlo, clusters.lo);
-  ddgrs::upload(ctx, d_nhi, clusters.hi);
-  ddgrs::upload(ctx, d_nrad, clusters.radius);
-  const BhTree T{d_right.ptr, d_esc.ptr, d_start.ptr, d_end.ptr, d_leaves.ptr, d_nlo.ptr, d_nhi.ptr, d_nrad.ptr,
-                 (uint32_t)nn, theta * theta};
+  const BhTree T{(const BhNode*)d_bhnode.ptr, d_start.ptr, d_end.ptr, d_leaves.ptr, (uint32_t)nn, theta * theta};
   const BhTasks W{d_task_root.ptr, d_task_depth.ptr, d_parent.ptr, ntask};
   const size_t nq1 = (size_t)nl * ntask + 1, nl1 = (size_t)nl + 1;
   if (nq1 > UINT32_MAX) throw Error(DDGRS_GPU, "bh_lists", "too many (leaf, task) pairs");

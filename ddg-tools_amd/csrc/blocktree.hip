@@ -524,25 +524,40 @@ __global__ __launch_bounds__(256) void fm_panel_values_kernel(PanelArgs P, const
   const uint32_t k = P.panel_leaf[p], row0 = P.panel_row0[p], R = P.panel_rows[p];
   const uint32_t c0 = P.col_ptr[k], K = P.col_ptr[k + 1] - c0;
   float* out = val + 2 * P.panel_val[p];
-  for (uint32_t r = 0; r < R; ++r) {
-    const uint32_t s0 = order[row0 + r];
-    for (uint32_t kk = lane; kk < K; kk += 64) {
-      const uint32_t t0 = order[P.cols[c0 + kk]];
+  // the panel's rows (wave-uniform), then one column per lane: its data loaded once for all rows
+  uint32_t srow[kPanelRows];
+#pragma unroll
+  for (int r = 0; r < (int)kPanelRows; ++r) srow[r] = order[row0 + ((uint32_t)r < R ? r : 0)];
+  for (uint32_t kk = lane; kk < K; kk += 64) {
+    const uint32_t t0 = order[P.cols[c0 + kk]];
+    const double bt[3] = {bary[3 * t0], bary[3 * t0 + 1], bary[3 * t0 + 2]};
+    const double nt3[3] = {nrm[3 * t0], nrm[3 * t0 + 1], nrm[3 * t0 + 2]};
+    const double at = area[t0];
+#pragma unroll
+    for (int r = 0; r < (int)kPanelRows; ++r) {
+      if ((uint32_t)r >= R) break;
+      const uint32_t s0 = srow[r];
       float* o = out + 2 * ((size_t)r * K + kk);
       if (s0 == t0) {
         o[0] = 0.f;
         o[1] = 0.f;
         continue;
       }
-      const uint32_t i = s0 < t0 ? s0 : t0, j = s0 < t0 ? t0 : s0;
+      const double bs[3] = {bary[3 * s0], bary[3 * s0 + 1], bary[3 * s0 + 2]};
+      const double ns3[3] = {nrm[3 * s0], nrm[3 * s0 + 1], nrm[3 * s0 + 2]};
+      // (min, max) face order as the reference evaluates the pair
+      const bool sfirst = s0 < t0;
+      const double* bi = sfirst ? bs : bt;
+      const double* bj = sfirst ? bt : bs;
+      const double* ni = sfirst ? ns3 : nt3;
+      const double* nj = sfirst ? nt3 : ns3;
       double d[3];
-      for (int x = 0; x < 3; ++x) d[x] = bary[3 * i + x] - bary[3 * j + x];
-      double ns = nrm[3 * i] * d[0] + nrm[3 * i + 1] * d[1] + nrm[3 * i + 2] * d[2];
-      double nt = nrm[3 * j] * d[0] + nrm[3 * j + 1] * d[1] + nrm[3 * j + 2] * d[2];
+      for (int x = 0; x < 3; ++x) d[x] = bi[x] - bj[x];
+      double ns = ni[0] * d[0] + ni[1] * d[1] + ni[2] * d[2];
+      double nt = nj[0] * d[0] + nj[1] * d[1] + nj[2] * d[2];
       double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
       double hi = pow(r2, neg_s);
       double lo = 0.5 * (ns * ns + nt * nt) / (r2 * r2) * hi;
-      const double at = area[t0];
       o[0] = (float)(hi * at);
       o[1] = (float)(lo * at);
     }
@@ -771,19 +786,27 @@ __global__ __launch_bounds__(256) void vf_panel_values_kernel(PanelArgs P, const
   const uint32_t k = P.panel_leaf[p], row0 = P.panel_row0[p], R = P.panel_rows[p];
   const uint32_t c0 = P.col_ptr[k], K = P.col_ptr[k + 1] - c0;
   float* out = val + P.panel_val[p];
-  for (uint32_t r = 0; r < R; ++r) {
-    const uint32_t s0 = order[row0 + r];
-    for (uint32_t kk = lane; kk < K; kk += 64) {
-      const uint32_t t0 = order[P.cols[c0 + kk]];
+  uint32_t srow[kPanelRows];
+#pragma unroll
+  for (int r = 0; r < (int)kPanelRows; ++r) srow[r] = order[row0 + ((uint32_t)r < R ? r : 0)];
+  for (uint32_t kk = lane; kk < K; kk += 64) {
+    const uint32_t t0 = order[P.cols[c0 + kk]];
+    const double pt[3] = {pos[3 * t0], pos[3 * t0 + 1], pos[3 * t0 + 2]};
+    const double mt = mass[t0];
+#pragma unroll
+    for (int r = 0; r < (int)kPanelRows; ++r) {
+      if ((uint32_t)r >= R) break;
+      const uint32_t s0 = srow[r];
       float* o = out + (size_t)r * K + kk;
       if (s0 == t0) {
         *o = 0.f;
         continue;
       }
-      const uint32_t i = s0 < t0 ? s0 : t0, j = s0 < t0 ? t0 : s0;
+      const double ps[3] = {pos[3 * s0], pos[3 * s0 + 1], pos[3 * s0 + 2]};
+      const bool sfirst = s0 < t0;
       double d[3];
-      for (int x = 0; x < 3; ++x) d[x] = pos[3 * i + x] - pos[3 * j + x];
-      *o = (float)(pow(d[0] * d[0] + d[1] * d[1] + d[2] * d[2], ex) * mass[t0]);
+      for (int x = 0; x < 3; ++x) d[x] = sfirst ? ps[x] - pt[x] : pt[x] - ps[x];
+      *o = (float)(pow(d[0] * d[0] + d[1] * d[1] + d[2] * d[2], ex) * mt);
     }
   }
 }
