@@ -185,24 +185,27 @@ __global__ __launch_bounds__(256) void blk_near_count_kernel(uint32_t n_near, co
   }
 }
 
-// Per leaf ordinal k < nl: panels and stored entries; zeros at k = nl (the scans then end with the
-// totals).
+// Per leaf ordinal k < nl: panels, their segments and stored entries; zeros at k = nl (the scans
+// then end with the totals).
 __global__ __launch_bounds__(256) void blk_panel_count_kernel(uint32_t nl, const uint32_t* __restrict__ leaves,
                                                               const uint32_t* __restrict__ start,
                                                               const uint32_t* __restrict__ end,
-                                                              const uint32_t* __restrict__ kcount,
+                                                              const uint32_t* __restrict__ kcount, uint32_t seg_cols,
                                                               uint32_t* __restrict__ pcount,
+                                                              uint32_t* __restrict__ scount,
                                                               uint64_t* __restrict__ entries) {
   const uint32_t k = blockIdx.x * 256 + threadIdx.x;
   if (k > nl) return;
   if (k == nl) {
     pcount[k] = 0;
+    scount[k] = 0;
     entries[k] = 0;
     return;
   }
-  const uint32_t leaf = leaves[k], ns = end[leaf] - start[leaf];
+  const uint32_t leaf = leaves[k], ns = end[leaf] - start[leaf], K = kcount[k];
   pcount[k] = (ns + kPanelRows - 1) / kPanelRows;
-  entries[k] = (uint64_t)ns * kcount[k];
+  scount[k] = pcount[k] * max(1u, (K + seg_cols - 1) / seg_cols);
+  entries[k] = (uint64_t)ns * K;
 }
 
 // The ranges of each leaf, in any order (a per-leaf cursor); they are sorted by first column next.
@@ -225,43 +228,59 @@ __global__ __launch_bounds__(256) void blk_range_fill_kernel(uint32_t n_near, co
   }
 }
 
-// One wave per leaf: the columns of its sorted ranges, in order.
-__global__ __launch_bounds__(256) void blk_cols_kernel(uint32_t nl, const uint32_t* __restrict__ range_ptr,
-                                                       const uint64_t* __restrict__ key,
-                                                       const uint32_t* __restrict__ rend,
-                                                       const uint32_t* __restrict__ col_ptr,
-                                                       uint32_t* __restrict__ cols) {
-  const uint32_t k = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const uint32_t lane = threadIdx.x & 63;
-  if (k >= nl) return;
-  uint32_t o = col_ptr[k];
-  for (uint32_t r = range_ptr[k]; r < range_ptr[k + 1]; ++r) {
-    const uint32_t b0 = (uint32_t)key[r], b1 = rend[r];
-    for (uint32_t x = b0 + lane; x < b1; x += 64) cols[o + (x - b0)] = x;
-    o += b1 - b0;
-  }
+// Columns of each sorted range (the ranges sorted by (leaf, first column) are the leaves' column
+// lists back to back, so their offsets are the exclusive sums of their lengths).
+__global__ __launch_bounds__(256) void blk_range_len_kernel(uint32_t nr, const uint64_t* __restrict__ key,
+                                                            const uint32_t* __restrict__ rend,
+                                                            uint32_t* __restrict__ len) {
+  const uint32_t r = blockIdx.x * 256 + threadIdx.x;
+  if (r < nr) len[r] = rend[r] - (uint32_t)key[r];
 }
 
+// One thread per range: its columns at its offset.  A range is one near block's column node, which
+// is a leaf or has at most leaf_pairs elements, so the loop is short.
+__global__ __launch_bounds__(256) void blk_cols_kernel(uint32_t nr, const uint64_t* __restrict__ key,
+                                                       const uint32_t* __restrict__ rend,
+                                                       const uint32_t* __restrict__ off,
+                                                       uint32_t* __restrict__ cols) {
+  const uint32_t r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= nr) return;
+  const uint32_t b0 = (uint32_t)key[r], b1 = rend[r], o = off[r];
+  for (uint32_t x = b0; x < b1; ++x) cols[o + (x - b0)] = x;
+}
+
+struct PanelOut {
+  uint32_t *leaf, *row0, *rows;
+  uint64_t* val;
+  uint32_t *seg_panel, *seg_col0, *seg_ptr;
+};
+
+// The panels of each leaf and their segments (panel after panel, columns ascending).
 __global__ __launch_bounds__(256) void blk_panel_fill_kernel(uint32_t nl, const uint32_t* __restrict__ leaves,
                                                              const uint32_t* __restrict__ start,
                                                              const uint32_t* __restrict__ end,
                                                              const uint32_t* __restrict__ col_ptr,
                                                              const uint32_t* __restrict__ pan_ptr,
-                                                             const uint64_t* __restrict__ val_base,
-                                                             uint32_t* __restrict__ panel_leaf,
-                                                             uint32_t* __restrict__ panel_row0,
-                                                             uint32_t* __restrict__ panel_rows,
-                                                             uint64_t* __restrict__ panel_val) {
+                                                             const uint32_t* __restrict__ seg_base,
+                                                             const uint64_t* __restrict__ val_base, uint32_t seg_cols,
+                                                             PanelOut o) {
   const uint32_t k = blockIdx.x * 256 + threadIdx.x;
   if (k >= nl) return;
   const uint32_t leaf = leaves[k], K = col_ptr[k + 1] - col_ptr[k];
-  uint32_t p = pan_ptr[k];
+  const uint32_t nseg = max(1u, (K + seg_cols - 1) / seg_cols);
+  uint32_t p = pan_ptr[k], sg = seg_base[k];
   for (uint32_t r0 = start[leaf]; r0 < end[leaf]; r0 += kPanelRows, ++p) {
-    panel_leaf[p] = k;
-    panel_row0[p] = r0;
-    panel_rows[p] = min(kPanelRows, end[leaf] - r0);
-    panel_val[p] = val_base[k] + (uint64_t)(r0 - start[leaf]) * K;
+    o.leaf[p] = k;
+    o.row0[p] = r0;
+    o.rows[p] = min(kPanelRows, end[leaf] - r0);
+    o.val[p] = val_base[k] + (uint64_t)(r0 - start[leaf]) * K;
+    o.seg_ptr[p] = sg;
+    for (uint32_t j = 0; j < nseg; ++j, ++sg) {
+      o.seg_panel[sg] = p;
+      o.seg_col0[sg] = j * seg_cols;
+    }
   }
+  if (k == nl - 1) o.seg_ptr[pan_ptr[nl]] = seg_base[nl];
 }
 
 __global__ __launch_bounds__(256) void blk_weight_kernel(uint32_t n, const uint32_t* __restrict__ order,
@@ -436,15 +455,19 @@ BlockStats BlockTreeDev::build_structure(ddgrs_ctx* ctx, const Topology& t, cons
            S.kcount.ptr);
     ctx->sync();  // lf, le are released on return
   }
+  seg_cols = panel_segment_cols();
+  S.scount.reserve(nl1);
+  S.seg_base.reserve(nl1);
   launch(ctx, "blk_panel_count", blk_panel_count_kernel, dim3(blocks_for(nl1, 256)), dim3(256), 0, (uint32_t)nl,
          (const uint32_t*)leaves.ptr, (const uint32_t*)start.ptr, (const uint32_t*)end.ptr, (const uint32_t*)S.kcount.ptr,
-         S.pcount.ptr, S.entries.ptr);
+         seg_cols, S.pcount.ptr, S.scount.ptr, S.entries.ptr);
   scan_exclusive<uint32_t>(ctx, S.temp, S.rcount.ptr, S.range_ptr.ptr, nl1);
   scan_exclusive<uint32_t>(ctx, S.temp, S.kcount.ptr, col_ptr.ptr, nl1);
   scan_exclusive<uint32_t>(ctx, S.temp, S.pcount.ptr, S.pan_ptr.ptr, nl1);
+  scan_exclusive<uint32_t>(ctx, S.temp, S.scount.ptr, S.seg_base.ptr, nl1);
   scan_exclusive<uint64_t>(ctx, S.temp, S.entries.ptr, S.val_base.ptr, nl1);
   struct Tails {
-    uint32_t ranges, cols, panels, pad;
+    uint32_t ranges, cols, panels, segs;
     uint64_t entries;
     unsigned long long sums[4];
   };
@@ -452,11 +475,13 @@ BlockStats BlockTreeDev::build_structure(ddgrs_ctx* ctx, const Topology& t, cons
   download(ctx, &tl->ranges, (const uint32_t*)S.range_ptr.ptr + nl, 1);
   download(ctx, &tl->cols, (const uint32_t*)col_ptr.ptr + nl, 1);
   download(ctx, &tl->panels, (const uint32_t*)S.pan_ptr.ptr + nl, 1);
+  download(ctx, &tl->segs, (const uint32_t*)S.seg_base.ptr + nl, 1);
   download(ctx, &tl->entries, (const uint64_t*)S.val_base.ptr + nl, 1);
   download(ctx, tl->sums, (const unsigned long long*)S.sums.ptr, 4);
   ctx->sync();
   const Tails T = *tl;
   np = (int)T.panels;
+  nseg = (int)T.segs;
   near_total = T.entries;
   const size_t nr = std::max<uint32_t>(T.ranges, 1);
   S.range_key.reserve(nr);
@@ -471,18 +496,31 @@ BlockStats BlockTreeDev::build_structure(ddgrs_ctx* ctx, const Topology& t, cons
     sort_pairs<uint64_t>(ctx, S.temp, S.range_key.ptr, S.range_key_s.ptr, S.range_end.ptr, S.range_end_s.ptr, T.ranges,
                          32 + bits_for((uint64_t)nl));
   cols.reserve(std::max<uint32_t>(T.cols, 1));
-  launch(ctx, "blk_cols", blk_cols_kernel, dim3(blocks_for(nl, 4)), dim3(256), 0, (uint32_t)nl,
-         (const uint32_t*)S.range_ptr.ptr, (const uint64_t*)S.range_key_s.ptr, (const uint32_t*)S.range_end_s.ptr,
-         (const uint32_t*)col_ptr.ptr, cols.ptr);
+  if (T.ranges) {
+    S.range_len.reserve(T.ranges);
+    S.range_off.reserve(T.ranges);
+    launch(ctx, "blk_range_len", blk_range_len_kernel, dim3(blocks_for(T.ranges, 256)), dim3(256), 0, T.ranges,
+           (const uint64_t*)S.range_key_s.ptr, (const uint32_t*)S.range_end_s.ptr, S.range_len.ptr);
+    scan_exclusive<uint32_t>(ctx, S.temp, S.range_len.ptr, S.range_off.ptr, T.ranges);
+    launch(ctx, "blk_cols", blk_cols_kernel, dim3(blocks_for(T.ranges, 256)), dim3(256), 0, T.ranges,
+           (const uint64_t*)S.range_key_s.ptr, (const uint32_t*)S.range_end_s.ptr, (const uint32_t*)S.range_off.ptr,
+           cols.ptr);
+  }
   panel_leaf.reserve(std::max(np, 1));
   panel_row0.reserve(std::max(np, 1));
   panel_rows.reserve(std::max(np, 1));
   panel_val.reserve(std::max(np, 1));
+  seg_panel.reserve(std::max(nseg, 1));
+  seg_col0.reserve(std::max(nseg, 1));
+  seg_ptr.reserve((size_t)np + 1);
   launch(ctx, "blk_panel_fill", blk_panel_fill_kernel, dim3(blocks_for(nl, 256)), dim3(256), 0, (uint32_t)nl,
          (const uint32_t*)leaves.ptr, (const uint32_t*)start.ptr, (const uint32_t*)end.ptr, (const uint32_t*)col_ptr.ptr,
-         (const uint32_t*)S.pan_ptr.ptr, (const uint64_t*)S.val_base.ptr, panel_leaf.ptr, panel_row0.ptr,
-         panel_rows.ptr, panel_val.ptr);
-  val.reserve((size_t)std::max<uint64_t>(near_total, 1) * nk);
+         (const uint32_t*)S.pan_ptr.ptr, (const uint32_t*)S.seg_base.ptr, (const uint64_t*)S.val_base.ptr, seg_cols,
+         PanelOut{panel_leaf.ptr, panel_row0.ptr, panel_rows.ptr, panel_val.ptr, seg_panel.ptr, seg_col0.ptr,
+                  seg_ptr.ptr});
+  seg_arrive.reserve(std::max(np, 1));
+  HIPCHECK(hipMemsetAsync(seg_arrive.ptr, 0, (size_t)std::max(np, 1) * sizeof(uint32_t), ctx->stream));
+  reserve_values(ctx);
   far_coef.reserve((size_t)std::max<uint32_t>(n_far, 1) * nk);
   counters.reserve(nn);
   HIPCHECK(hipMemsetAsync(counters.ptr, 0, (size_t)nn * sizeof(uint32_t), ctx->stream));

@@ -17,6 +17,7 @@
 // far field (one wave per node), leaf far terms (one thread per leaf and channel), panels (one wave
 // per panel: lanes are columns, registers hold the panel's rows × C channels).
 #include <algorithm>
+#include <cstdlib>
 
 #include "blocktree.hpp"
 #include "host_parallel.hpp"
@@ -143,18 +144,17 @@ __global__ __launch_bounds__(256) void bt_upward_kernel(TreeArgs t, const float*
     const double im = t.inv_mass[node];
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-      sigma[(size_t)node * C + c] = s[c];
+      publish(&sigma[(size_t)node * C + c], s[c]);
       mean[(size_t)node * C + c] = s[c] * im;
     }
     if (node == 0) return;
     const uint32_t p = t.parent[node];
-    __threadfence();  // publish σ_node before arriving at the parent
+    published();  // σ_node is visible before the arrival at the parent
     if (atomicAdd(&t.counters[p], 1u) == 0u) return;  // the sibling is not done: it will climb
-    __threadfence();  // see the sibling's σ
     t.counters[p] = 0u;
     const uint32_t l = p + 1, r = t.right[p];
 #pragma unroll
-    for (int c = 0; c < C; ++c) s[c] = sigma[(size_t)l * C + c] + sigma[(size_t)r * C + c];
+    for (int c = 0; c < C; ++c) s[c] = collect(&sigma[(size_t)l * C + c]) + collect(&sigma[(size_t)r * C + c]);
     node = p;
   }
 }
@@ -215,54 +215,111 @@ struct PanelArgs {
   const float* val;
   const double *alpha_leaf, *mean, *delta;
   int np;
+  const float* vzero;  // NK zeros after the values
 };
 
-// Near field of one panel plus its far terms; z = w_k (near + ᾱ_L (u_S − ū_L) + δ_L).
+// Panel segments: per segment its panel and first column, per panel its first segment.
+struct SegArgs {
+  const uint32_t *panel, *col0, *ptr;
+  uint32_t* arrive;
+  float* partial;
+  uint32_t cols;
+  int n;
+};
+
+// z of row r, channel c of panel p from its near sum: z = w_k (near + ᾱ_L (u_S − ū_L) + δ_L).
 template <int C, int H, int NK>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void bt_panel_kernel(PanelArgs P, const float* __restrict__ u_hl, double w0, double w1,
-                                                       float* __restrict__ z_hl) {
-  const uint32_t p = xcd_wave_id();
+__device__ __forceinline__ void panel_finish(const PanelArgs& P, uint32_t k, uint32_t row0, int r, int c, double near,
+                                             const float* __restrict__ u_hl, double w0, double w1,
+                                             float* __restrict__ z_hl) {
+  const int kern = c < H ? 0 : NK - 1;
+  const uint32_t srow = row0 + r;
+  const float* qv = u_hl + (size_t)srow * 2 * C;
+  const double us = (double)qv[c] + (double)qv[C + c];
+  const double far =
+      P.alpha_leaf[(size_t)k * NK + kern] * (us - P.mean[(size_t)P.leaves[k] * C + c]) + P.delta[(size_t)k * C + c];
+  const double z = (c < H ? w0 : w1) * (near + far);
+  const float zh = (float)z;
+  z_hl[(size_t)srow * 2 * C + c] = zh;
+  z_hl[(size_t)srow * 2 * C + C + c] = (float)(z - (double)zh);
+}
+
+// One 64-column chunk of a panel segment in registers: the columns' u (hi, lo) and the rows' values.
+template <int C, int NK>
+struct PanelChunk {
+  float th[C], tl[C], v0[kPanelRows], v1[kPanelRows];
+};
+
+// Lanes past the segment's end and rows past the panel's read the zero entry after the values
+// (`vzero`), so no load result is masked and the loads stay in flight until their chunk is summed.
+template <int C, int NK>
+__device__ __forceinline__ void panel_load(PanelChunk<C, NK>& x, const float* __restrict__ u_hl, uint32_t tcol,
+                                           const float* __restrict__ vbase, const float* __restrict__ vzero,
+                                           uint32_t K, uint32_t kk, bool act, uint32_t R) {
+  load_hl<C>(u_hl + (size_t)tcol * 2 * C, x.th, x.tl);
+#pragma unroll
+  for (int r = 0; r < (int)kPanelRows; ++r) {
+    const bool ok = act && (uint32_t)r < R;
+    const float* v = ok ? vbase + ((size_t)r * K + kk) * NK : vzero;
+    x.v0[r] = v[0];
+    x.v1[r] = v[NK - 1];
+  }
+}
+
+template <int C, int H, int NK>
+__device__ __forceinline__ void panel_compute(float (&acc)[kPanelRows][C], const PanelChunk<C, NK>& x,
+                                              const float* __restrict__ u_hl, uint32_t row0, uint32_t R) {
+#pragma unroll
+  for (int r = 0; r < (int)kPanelRows; ++r) {
+    if ((uint32_t)r < R) {
+      const float* srow = u_hl + (size_t)(row0 + r) * 2 * C;
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        const float d = (srow[c] - x.th[c]) + (srow[C + c] - x.tl[c]);
+        acc[r][c] += (c < H ? x.v0[r] : x.v1[r]) * d;
+      }
+    }
+  }
+}
+
+// Near field of one panel segment per wave.  A panel of one segment is finished by its wave; the
+// segments of a longer panel store their sums, and the last to arrive finishes the panel from the
+// stored sums in segment order (the result does not depend on the arrival order).  The chunks go
+// through two register sets: the loads of the next chunk are in flight while one is summed.
+template <int C, int H, int NK>
+__global__ __launch_bounds__(256) void bt_panel_kernel(PanelArgs P, SegArgs G, const float* __restrict__ u_hl,
+                                                       double w0, double w1, float* __restrict__ z_hl) {
+  const uint32_t sg = xcd_wave_id();
   const int lane = threadIdx.x & 63;
-  if (p >= (uint32_t)P.np) return;
+  if (sg >= (uint32_t)G.n) return;
+  const uint32_t p = G.panel[sg], j0 = G.col0[sg];
   const uint32_t k = P.panel_leaf[p], row0 = P.panel_row0[p], R = P.panel_rows[p];
   const uint32_t c0 = P.col_ptr[k], K = P.col_ptr[k + 1] - c0;
+  const uint32_t kend = min(K, j0 + G.cols);
+  const uint32_t nch = (kend - j0 + 63) / 64;  // chunks of the segment (at least one)
   const float* vbase = P.val + P.panel_val[p] * NK;
   float acc[kPanelRows][C];
 #pragma unroll
   for (int r = 0; r < (int)kPanelRows; ++r)
 #pragma unroll
     for (int c = 0; c < C; ++c) acc[r][c] = 0.f;
-  // The column index of the next chunk is loaded one chunk ahead; all rows' values of a chunk are
-  // loaded before any is used (clamped indices, zeroed outside the panel), so a chunk costs one
-  // round trip to memory, not one per row.
-  uint32_t tnext = P.cols[c0 + (lane < K ? lane : 0)];
-  for (uint32_t base = 0; base < K; base += 64) {
-    const uint32_t kk = base + lane;
-    const bool act = kk < K;
-    const uint32_t tcol = tnext;
-    tnext = P.cols[c0 + (kk + 64 < K ? kk + 64 : 0)];
-    float th[C], tl[C];
-    load_hl<C>(u_hl + (size_t)tcol * 2 * C, th, tl);
-    float v0[kPanelRows], v1[kPanelRows];
-#pragma unroll
-    for (int r = 0; r < (int)kPanelRows; ++r) {
-      const bool ok = act && (uint32_t)r < R;
-      const float* v = vbase + (ok ? ((size_t)r * K + kk) * NK : 0);
-      v0[r] = v[0];
-      v1[r] = v[NK - 1];
-      if (!ok) v0[r] = v1[r] = 0.f;
-    }
-#pragma unroll
-    for (int r = 0; r < (int)kPanelRows; ++r) {
-      if ((uint32_t)r < R) {
-        const float* srow = u_hl + (size_t)(row0 + r) * 2 * C;
-#pragma unroll
-        for (int c = 0; c < C; ++c) {
-          const float d = (srow[c] - th[c]) + (srow[C + c] - tl[c]);
-          acc[r][c] += (c < H ? v0[r] : v1[r]) * d;
-        }
-      }
-    }
+  // Chunk m's batch of loads: the column index of chunk m + 1, then chunk m's u and values (column
+  // indices clamped to the segment's last column, so every load is in bounds; every panel has a
+  // column).  A batch is issued before the previous chunk is summed; the loop ends after one batch
+  // past the segment's last chunk (its results unused).
+  const uint32_t last = c0 + kend - 1;
+  uint32_t cia = P.cols[min(c0 + j0 + lane, last)], cib;
+  PanelChunk<C, NK> a, b;
+  for (uint32_t m = 0;; m += 2) {
+    const uint32_t km = j0 + 64 * m + lane;
+    cib = P.cols[min(c0 + km + 64, last)];
+    panel_load<C, NK>(a, u_hl, cia, vbase, P.vzero, K, km, km < kend, R);
+    if (m > 0) panel_compute<C, H, NK>(acc, b, u_hl, row0, R);
+    if (m >= nch) break;
+    cia = P.cols[min(c0 + km + 128, last)];
+    panel_load<C, NK>(b, u_hl, cib, vbase, P.vzero, K, km + 64, km + 64 < kend, R);
+    panel_compute<C, H, NK>(acc, a, u_hl, row0, R);
+    if (m + 1 >= nch) break;
   }
   // the sums over the wave: reduce-scatter, then each lane whose low bits are 0 finishes its values
   constexpr int NV = (int)kPanelRows * C;
@@ -270,28 +327,40 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void b
   reduce_scatter<NV, 32>(flat, lane);
   int nmine, base;
   rs_slots<NV, 32>(lane, nmine, base);
-  const uint32_t leaf = P.leaves[k];
   // lanes that differ only in the bits the butterfly summed over hold the same sums: the lowest writes
   int used = 0;
   for (int m = 32, nn = NV; m >= 1 && nn % 2 == 0; m >>= 1, nn /= 2) used |= m;
-  if ((lane & ~used & 63) == 0) {
+  const bool owner = (lane & ~used & 63) == 0;
+  const uint32_t s0 = G.ptr[p], ns = G.ptr[p + 1] - s0;
+  if (ns == 1) {
+    if (owner) {
+#pragma unroll
+      for (int q = 0; q < NV; ++q) {
+        if (q >= nmine) break;
+        const int idx = base + q;
+        if ((uint32_t)(idx / C) < R) panel_finish<C, H, NK>(P, k, row0, idx / C, idx % C, (double)flat[q], u_hl, w0, w1, z_hl);
+      }
+    }
+    return;
+  }
+  if (owner) {
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
       if (q >= nmine) break;
-      const int idx = base + q;
-      const int r = idx / C, c = idx % C;
-      if ((uint32_t)r >= R) continue;
-      const int kern = c < H ? 0 : NK - 1;
-      const uint32_t srow = row0 + r;
-      const float* qv = u_hl + (size_t)srow * 2 * C;
-      const double us = (double)qv[c] + (double)qv[C + c];
-      const double far = P.alpha_leaf[(size_t)k * NK + kern] * (us - P.mean[(size_t)leaf * C + c]) +
-                         P.delta[(size_t)k * C + c];
-      const double z = (c < H ? w0 : w1) * ((double)flat[q] + far);
-      const float zh = (float)z;
-      z_hl[(size_t)srow * 2 * C + c] = zh;
-      z_hl[(size_t)srow * 2 * C + C + c] = (float)(z - (double)zh);
+      publish(&G.partial[(size_t)sg * NV + base + q], flat[q]);
     }
+  }
+  published();  // the partial sums are visible before the arrival
+  uint32_t arrived = 0;
+  if (lane == 0) arrived = atomicAdd(&G.arrive[p], 1u);
+  arrived = __shfl(arrived, 0);
+  if (arrived != ns - 1) return;
+  if (lane == 0) G.arrive[p] = 0u;
+  for (int idx = lane; idx < NV; idx += 64) {
+    if ((uint32_t)(idx / C) >= R) continue;
+    double near = 0.0;
+    for (uint32_t t = s0; t < s0 + ns; ++t) near += (double)collect(&G.partial[(size_t)t * NV + idx]);
+    panel_finish<C, H, NK>(P, k, row0, idx / C, idx % C, near, u_hl, w0, w1, z_hl);
   }
 }
 
@@ -326,6 +395,15 @@ __global__ __launch_bounds__(256) void bt_alpha_leaf_kernel(int nl, int nk, cons
     }
     alpha_leaf[(size_t)k * nk + j] = s;
   }
+}
+
+uint32_t panel_segment_cols() {
+  const char* e = std::getenv("DDGRS_SEGMENT_COLUMNS");
+  if (!e || !*e) return kSegCols;
+  const long v = std::strtol(e, nullptr, 10);
+  if (v <= 0 || v % 64 != 0 || v > (long)kSegCols)
+    invalid("DDGRS_SEGMENT_COLUMNS", e, "a positive multiple of 64, at most " + std::to_string(kSegCols));
+  return (uint32_t)v;
 }
 
 BlockTreePlan plan_block_tree(const Topology& t, const BlockStructure& s, int nk, std::vector<double> weight_bvh,
@@ -384,6 +462,17 @@ BlockTreePlan plan_block_tree(const Topology& t, const BlockStructure& s, int nk
   }
   P.np = (int)P.panel_leaf.size();
   P.near_total = total;
+  P.seg_cols = panel_segment_cols();
+  P.seg_ptr.assign(P.np + 1, 0);
+  for (int p = 0; p < P.np; ++p) {
+    const uint32_t K = P.col_ptr[P.panel_leaf[p] + 1] - P.col_ptr[P.panel_leaf[p]];
+    const uint32_t ns = std::max<uint32_t>(1, (K + P.seg_cols - 1) / P.seg_cols);
+    P.seg_ptr[p + 1] = P.seg_ptr[p] + ns;
+    for (uint32_t j = 0; j < ns; ++j) {
+      P.seg_panel.push_back((uint32_t)p);
+      P.seg_col0.push_back(j * P.seg_cols);
+    }
+  }
   P.far_ptr = s.far_ptr;
   P.far_col = s.far_col;
   P.far_row.resize(s.far_col.size());
@@ -416,7 +505,14 @@ void BlockTreeDev::upload(ddgrs_ctx* ctx, const BlockTreePlan& P) {
   ddgrs::upload(ctx, panel_row0, P.panel_row0);
   ddgrs::upload(ctx, panel_rows, P.panel_rows);
   ddgrs::upload(ctx, panel_val, P.panel_val);
-  val.reserve((size_t)std::max<uint64_t>(near_total, 1) * nk);
+  seg_cols = P.seg_cols;
+  nseg = (int)P.seg_panel.size();
+  ddgrs::upload(ctx, seg_panel, P.seg_panel);
+  ddgrs::upload(ctx, seg_col0, P.seg_col0);
+  ddgrs::upload(ctx, seg_ptr, P.seg_ptr);
+  seg_arrive.reserve(std::max(np, 1));
+  HIPCHECK(hipMemsetAsync(seg_arrive.ptr, 0, (size_t)std::max(np, 1) * sizeof(uint32_t), ctx->stream));
+  reserve_values(ctx);
   ddgrs::upload(ctx, far_ptr, P.far_ptr);
   ddgrs::upload(ctx, far_col, P.far_col);
   ddgrs::upload(ctx, far_row, P.far_row);
@@ -425,6 +521,11 @@ void BlockTreeDev::upload(ddgrs_ctx* ctx, const BlockTreePlan& P) {
   ddgrs::upload(ctx, inv_mass, P.inv_mass);
   counters.reserve(nn);
   HIPCHECK(hipMemsetAsync(counters.ptr, 0, (size_t)nn * sizeof(uint32_t), ctx->stream));
+}
+
+void BlockTreeDev::reserve_values(ddgrs_ctx* ctx) {
+  val.reserve((size_t)(near_total + 1) * nk);
+  HIPCHECK(hipMemsetAsync(val.ptr + (size_t)near_total * nk, 0, (size_t)nk * sizeof(float), ctx->stream));
 }
 
 void BlockTreeDev::finish_far(ddgrs_ctx* ctx) {
@@ -462,13 +563,16 @@ void BlockTreeDev::multiply(ddgrs_ctx* ctx, const float* u_hl, double w0, double
            0, ta, (const double*)alpha_node.ptr, (const double*)mean.ptr, (const double*)gamma.ptr, delta.ptr);
   }
   PanelArgs pa{panel_leaf.ptr, panel_row0.ptr, panel_rows.ptr, col_ptr.ptr,        cols.ptr, leaves.ptr,
-               panel_val.ptr,  val.ptr,        alpha_leaf.ptr, (const double*)mean.ptr, (const double*)delta.ptr, np};
+               panel_val.ptr,  val.ptr,        alpha_leaf.ptr, (const double*)mean.ptr, (const double*)delta.ptr, np,
+               val.ptr + (size_t)near_total * nk};
+  partial.reserve((size_t)std::max(nseg, 1) * kPanelRows * C);
+  SegArgs sa{seg_panel.ptr, seg_col0.ptr, seg_ptr.ptr, seg_arrive.ptr, partial.ptr, seg_cols, nseg};
   if (nk == 2)
-    launch(ctx, BT_NAME("bt_panel"), bt_panel_kernel<C, H, 2>, dim3(blocks_for(np, 4)), dim3(256), 0, pa, u_hl, w0, w1,
-           z_hl);
+    launch(ctx, BT_NAME("bt_panel"), bt_panel_kernel<C, H, 2>, dim3(blocks_for(nseg, 4)), dim3(256), 0, pa, sa, u_hl,
+           w0, w1, z_hl);
   else
-    launch(ctx, BT_NAME("bt_panel"), bt_panel_kernel<C, H, 1>, dim3(blocks_for(np, 4)), dim3(256), 0, pa, u_hl, w0, w1,
-           z_hl);
+    launch(ctx, BT_NAME("bt_panel"), bt_panel_kernel<C, H, 1>, dim3(blocks_for(nseg, 4)), dim3(256), 0, pa, sa, u_hl,
+           w0, w1, z_hl);
 }
 
 template void BlockTreeDev::multiply<12, 9>(ddgrs_ctx*, const float*, double, double, float*);
@@ -513,22 +617,25 @@ __global__ __launch_bounds__(256) void fm_far_coef_kernel(uint32_t nfar, const u
 
 // Near values of one panel per wave: rows r, columns kk (lanes), (min, max)-ordered kernels times
 // the column face's area; the diagonal is 0.
-__global__ __launch_bounds__(256) void fm_panel_values_kernel(PanelArgs P, const uint32_t* __restrict__ order,
+__global__ __launch_bounds__(256) void fm_panel_values_kernel(PanelArgs P, SegArgs G,
+                                                              const uint32_t* __restrict__ order,
                                                               const double* __restrict__ bary,
                                                               const double* __restrict__ nrm,
                                                               const double* __restrict__ area, double neg_s,
                                                               float* __restrict__ val) {
-  const uint32_t p = xcd_wave_id();
+  const uint32_t sg = xcd_wave_id();
   const int lane = threadIdx.x & 63;
-  if (p >= (uint32_t)P.np) return;
+  if (sg >= (uint32_t)G.n) return;
+  const uint32_t p = G.panel[sg], j0 = G.col0[sg];
   const uint32_t k = P.panel_leaf[p], row0 = P.panel_row0[p], R = P.panel_rows[p];
   const uint32_t c0 = P.col_ptr[k], K = P.col_ptr[k + 1] - c0;
+  const uint32_t kend = min(K, j0 + G.cols);
   float* out = val + 2 * P.panel_val[p];
   // the panel's rows (wave-uniform), then one column per lane: its data loaded once for all rows
   uint32_t srow[kPanelRows];
 #pragma unroll
   for (int r = 0; r < (int)kPanelRows; ++r) srow[r] = order[row0 + ((uint32_t)r < R ? r : 0)];
-  for (uint32_t kk = lane; kk < K; kk += 64) {
+  for (uint32_t kk = j0 + lane; kk < kend; kk += 64) {
     const uint32_t t0 = order[P.cols[c0 + kk]];
     const double bt[3] = {bary[3 * t0], bary[3 * t0 + 1], bary[3 * t0 + 2]};
     const double nt3[3] = {nrm[3 * t0], nrm[3 * t0 + 1], nrm[3 * t0 + 2]};
@@ -698,7 +805,8 @@ void face_metric_values(ddgrs_ctx* ctx, ddgrs_surface* s, FaceMetricDev& F, cons
   bt.finish_far(ctx);
   PanelArgs pa{bt.panel_leaf.ptr, bt.panel_row0.ptr, bt.panel_rows.ptr, bt.col_ptr.ptr, bt.cols.ptr, bt.leaves.ptr,
                bt.panel_val.ptr,  nullptr,          nullptr,           nullptr,         nullptr,     bt.np};
-  launch(ctx, "fm_panel_values", fm_panel_values_kernel, dim3(blocks_for(bt.np, 4)), dim3(256), 0, pa,
+  const SegArgs sa{bt.seg_panel.ptr, bt.seg_col0.ptr, bt.seg_ptr.ptr, nullptr, nullptr, bt.seg_cols, bt.nseg};
+  launch(ctx, "fm_panel_values", fm_panel_values_kernel, dim3(blocks_for(bt.nseg, 4)), dim3(256), 0, pa, sa,
          (const uint32_t*)bt.order.ptr, (const double*)s->bary.ptr, (const double*)s->normal.ptr,
          (const double*)s->area.ptr, neg_s, bt.val.ptr);
   const int nf = F.nf;
@@ -776,20 +884,23 @@ __global__ __launch_bounds__(256) void vf_far_coef_kernel(uint32_t nfar, const u
   coef[q] = (float)(pow(d[0] * d[0] + d[1] * d[1] + d[2] * d[2], ex) * nmass[col]);
 }
 
-__global__ __launch_bounds__(256) void vf_panel_values_kernel(PanelArgs P, const uint32_t* __restrict__ order,
+__global__ __launch_bounds__(256) void vf_panel_values_kernel(PanelArgs P, SegArgs G,
+                                                              const uint32_t* __restrict__ order,
                                                               const double* __restrict__ pos,
                                                               const double* __restrict__ mass, double ex,
                                                               float* __restrict__ val) {
-  const uint32_t p = xcd_wave_id();
+  const uint32_t sg = xcd_wave_id();
   const int lane = threadIdx.x & 63;
-  if (p >= (uint32_t)P.np) return;
+  if (sg >= (uint32_t)G.n) return;
+  const uint32_t p = G.panel[sg], j0 = G.col0[sg];
   const uint32_t k = P.panel_leaf[p], row0 = P.panel_row0[p], R = P.panel_rows[p];
   const uint32_t c0 = P.col_ptr[k], K = P.col_ptr[k + 1] - c0;
+  const uint32_t kend = min(K, j0 + G.cols);
   float* out = val + P.panel_val[p];
   uint32_t srow[kPanelRows];
 #pragma unroll
   for (int r = 0; r < (int)kPanelRows; ++r) srow[r] = order[row0 + ((uint32_t)r < R ? r : 0)];
-  for (uint32_t kk = lane; kk < K; kk += 64) {
+  for (uint32_t kk = j0 + lane; kk < kend; kk += 64) {
     const uint32_t t0 = order[P.cols[c0 + kk]];
     const double pt[3] = {pos[3 * t0], pos[3 * t0 + 1], pos[3 * t0 + 2]};
     const double mt = mass[t0];
@@ -864,7 +975,8 @@ void vertex_fractional_values(ddgrs_ctx* ctx, VertexFractionalDev& V, const Clus
   bt.finish_far(ctx);
   PanelArgs pa{bt.panel_leaf.ptr, bt.panel_row0.ptr, bt.panel_rows.ptr, bt.col_ptr.ptr, bt.cols.ptr, bt.leaves.ptr,
                bt.panel_val.ptr,  nullptr,          nullptr,           nullptr,         nullptr,     bt.np};
-  launch(ctx, "vf_panel_values", vf_panel_values_kernel, dim3(blocks_for(bt.np, 4)), dim3(256), 0, pa,
+  const SegArgs sa{bt.seg_panel.ptr, bt.seg_col0.ptr, bt.seg_ptr.ptr, nullptr, nullptr, bt.seg_cols, bt.nseg};
+  launch(ctx, "vf_panel_values", vf_panel_values_kernel, dim3(blocks_for(bt.nseg, 4)), dim3(256), 0, pa, sa,
          (const uint32_t*)bt.order.ptr, (const double*)dpos.ptr, (const double*)V.mass.ptr, ex, bt.val.ptr);
   V.u_hl.reserve(6 * (size_t)V.nv);
   V.z_hl.reserve(6 * (size_t)V.nv);

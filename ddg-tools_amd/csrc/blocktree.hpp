@@ -9,6 +9,13 @@ namespace ddgrs {
 
 // Rows of a near-field panel (a leaf's rows are split into panels of at most this many).
 constexpr uint32_t kPanelRows = 4;
+// Columns of a panel segment, the unit of work of the near-field kernels (a multiple of 64).  A few
+// leaves span large parts of a surface (the BVH keeps long thin leaves along mesh seams) and are
+// near tens of thousands of columns; split, their panels spread over many waves.
+constexpr uint32_t kSegCols = 512;
+// The segment length: kSegCols, or DDGRS_SEGMENT_COLUMNS (a positive multiple of 64, at most
+// kSegCols) when set.
+uint32_t panel_segment_cols();
 
 // Host part of a BlockTreeDev: the structure, panels and weights (built without the device).
 struct BlockTreePlan {
@@ -18,6 +25,9 @@ struct BlockTreePlan {
   std::vector<uint32_t> order, position, start, end, right, parent, leaves;
   std::vector<uint32_t> col_ptr, cols, panel_leaf, panel_row0, panel_rows, far_ptr, far_col, far_row;
   std::vector<uint64_t> panel_val;
+  // segments: per segment its panel and first column; per panel its first segment (np + 1)
+  uint32_t seg_cols = kSegCols;
+  std::vector<uint32_t> seg_panel, seg_col0, seg_ptr;
   std::vector<double> weight, inv_mass;
 };
 BlockTreePlan plan_block_tree(const Topology& t, const BlockStructure& s, int nk, std::vector<double> weight_bvh,
@@ -39,6 +49,7 @@ struct BlockCounts {
 struct BlockBuildScratch {
   DevBuf<uint32_t> pa, pb, qa, qb, adm_a, adm_b, near_a, near_b, near_row, near_col, near_ptr;
   DevBuf<uint32_t> rcount, kcount, pcount, range_ptr, pan_ptr, range_end, range_end_s, leaf_first, leaf_end, cursor;
+  DevBuf<uint32_t> scount, seg_base, range_len, range_off;
   DevBuf<uint64_t> entries, val_base, range_key, range_key_s;
   DevBuf<double> lo, hi, radius;
   DevBuf<uint8_t> cls, temp;
@@ -68,7 +79,15 @@ struct BlockTreeDev {
   DevBuf<uint32_t> order, position, start, end, right, parent, leaves;
   DevBuf<uint32_t> panel_leaf, panel_row0, panel_rows, col_ptr, cols;
   DevBuf<uint64_t> panel_val;  // per panel: first entry of its values
-  DevBuf<float> val;           // NK per entry
+  // Panel segments (at most seg_cols columns each), one wave each in the near-field kernels.  A
+  // panel of several segments is finished by the last of its segments to arrive, which sums the
+  // segments' partial sums in segment order (FP64).
+  uint32_t seg_cols = kSegCols;
+  int nseg = 0;
+  DevBuf<uint32_t> seg_panel, seg_col0, seg_ptr;
+  DevBuf<uint32_t> seg_arrive;  // per panel, zero between products
+  DevBuf<float> partial;        // per segment: kPanelRows × C near sums
+  DevBuf<float> val;           // NK per entry, then NK zeros (read by masked lanes)
   DevBuf<uint32_t> far_ptr, far_col, far_row;
   DevBuf<float> far_coef;      // NK per far entry: κ_AB A_B
   DevBuf<double> weight;       // per BVH position: a (FP64)
@@ -89,6 +108,8 @@ struct BlockTreeDev {
   BlockStats build_structure(ddgrs_ctx* ctx, const Topology& t, const Clusters& c, double chi, size_t leaf_pairs,
                              int nk, const double* elem_weight);
   void finish_far(ddgrs_ctx* ctx);
+  // Room for the near values and the zero entry after them.
+  void reserve_values(ddgrs_ctx* ctx);
   // z = w_k (H̃a·u − H̃(a u)) per BVH position and channel, channels [0, H) on kernel 0 with w0,
   // [H, C) on kernel NK − 1 with w1.  u and z are hi/lo FP32 pairs: 2 C floats per position
   // (C hi, then C lo).

@@ -20,6 +20,27 @@ __device__ __forceinline__ uint32_t xcd_wave_id() {
   return __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
 }
 
+// Hand-off of values between waves of one launch (last arriver of a counter reads what the others
+// wrote) without a device-scope fence, which on this GPU writes back and invalidates the whole L2
+// of the XCD.  The values are written with device-scope atomic stores and read with device-scope
+// atomic loads (both coherent across the XCDs' L2s); `published()` waits until the wave's stores
+// have completed, which orders them before the arrival counter's atomic add.  This is the
+// compiler's own release/acquire sequence for device scope minus the L2 write-back and
+// invalidation, which only concern data written with plain stores.
+template <class T>
+__device__ __forceinline__ void publish(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ T collect(T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void published() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
 // Exponents as the kernels take them (RS/src/kernel.rs:26 Exponents): integer exponents use
 // products, others powf.
 struct Exps {

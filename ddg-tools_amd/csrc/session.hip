@@ -173,6 +173,27 @@ void Session::project() {
   compute_face_geometry(s, s->packs, false);
 }
 
+// The near-field panels' column counts on stderr (count, mean, largest, and the 64-column chunks
+// of the longest panels), for reading the panel kernels' balance.
+static void trace_panels(ddgrs_ctx* ctx, const BlockTreeDev& bt) {
+  std::vector<uint32_t> col_ptr(bt.nl + 1), leaf(bt.np), rows(bt.np);
+  download(ctx, col_ptr.data(), bt.col_ptr.ptr, col_ptr.size());
+  download(ctx, leaf.data(), bt.panel_leaf.ptr, leaf.size());
+  download(ctx, rows.data(), bt.panel_rows.ptr, rows.size());
+  std::vector<uint32_t> k(bt.np);
+  double sum = 0.0, rsum = 0.0;
+  for (int p = 0; p < bt.np; ++p) {
+    k[p] = col_ptr[leaf[p] + 1] - col_ptr[leaf[p]];
+    sum += k[p];
+    rsum += rows[p];
+  }
+  std::sort(k.begin(), k.end());
+  if (k.empty()) return;
+  auto at = [&](double q) { return k[std::min(k.size() - 1, (size_t)(q * k.size()))]; };
+  std::fprintf(stderr, "panels: %d leaves %d panels, rows/panel %.2f, columns mean %.1f p50 %u p90 %u p99 %u max %u\n",
+               bt.nl, bt.np, rsum / bt.np, sum / bt.np, at(0.5), at(0.9), at(0.99), k.back());
+}
+
 void Session::step(StepReport* rep) {
   auto t_step = std::chrono::steady_clock::now();
   ddgrs_ctx* ctx = s->ctx;
@@ -254,6 +275,7 @@ void Session::step(StepReport* rep) {
   if (trace) std::fprintf(stderr, "step: %-18s %8.2f ms (wait %.2f ms)\n", "metric build", metric_build * 1e3,
                           seconds_since(t) * 1e3);
   if (ctx->profiling) ctx_merge_times(ctx, aux2);
+  if (trace) trace_panels(ctx, fm.bt);
   rep->seconds_metric = seconds_since(t);
   t = std::chrono::steady_clock::now();
   side.get();  // the second stream is synchronized

@@ -87,3 +87,40 @@ def test_block_build_device_equals_host(pkg, ctx, which, monkeypatch):
     assert fm_d.stats() == fm_h.stats()
     assert np.array_equal(fm_d.apply_batch(x, 1.0, 0.3), fm_h.apply_batch(x, 1.0, 0.3))
     assert np.array_equal(vf_d.apply_batch(x), vf_h.apply_batch(x))
+
+
+@pytest.mark.parametrize("seg", ["64", "192"])
+def test_panel_segments(pkg, ctx, seg, monkeypatch):
+    """Near-field panels split into column segments (panels of several segments finished from
+    their segments' stored sums) give the product of unsplit panels to FP32 summation order, on
+    both operators, and still match the oracle."""
+    p, f = pkg.generate.config_torus_16k()
+    s = pkg.Surface(ctx, f, p)
+    tau = 2.0 - (12.0 - 2.0) / 6.0
+    rng = np.random.default_rng(15)
+    x = rng.standard_normal((2, 3 * len(p)))
+    fm_a, vf_a = pkg.FaceMetricTree(s), pkg.VertexFractionalTree(s, tau)
+    monkeypatch.setenv("DDGRS_SEGMENT_COLUMNS", seg)
+    fm_b, vf_b = pkg.FaceMetricTree(s), pkg.VertexFractionalTree(s, tau)
+    monkeypatch.setenv("DDGRS_BLOCKS_ON_HOST", "1")
+    fm_h = pkg.FaceMetricTree(s)
+    monkeypatch.delenv("DDGRS_BLOCKS_ON_HOST")
+    monkeypatch.delenv("DDGRS_SEGMENT_COLUMNS")
+    ya, yb = fm_a.apply_batch(x, 1.0, 0.3), fm_b.apply_batch(x, 1.0, 0.3)
+    assert np.array_equal(yb, fm_h.apply_batch(x, 1.0, 0.3))
+    assert np.linalg.norm(ya - yb) / np.linalg.norm(ya) <= 1e-6
+    y0 = oracle.FaceMetric(f, p, chi=0.5).apply(x, 1.0, 0.3)
+    assert np.linalg.norm(yb - y0) / np.linalg.norm(y0) <= TOL
+    va, vb = vf_a.apply_batch(x), vf_b.apply_batch(x)
+    assert np.linalg.norm(va - vb) / np.linalg.norm(va) <= 1e-6
+    # applied twice: the per-panel arrival counts are back at zero after each product
+    assert np.array_equal(fm_b.apply_batch(x, 1.0, 0.3), yb)
+    assert np.array_equal(vf_b.apply_batch(x), vb)
+
+
+def test_panel_segments_invalid(pkg, ctx, monkeypatch):
+    p, f = pkg.generate.config_icosphere_l3()
+    s = pkg.Surface(ctx, f, p)
+    monkeypatch.setenv("DDGRS_SEGMENT_COLUMNS", "100")
+    with pytest.raises(Exception, match="multiple of 64"):
+        pkg.FaceMetricTree(s)
