@@ -14,6 +14,9 @@
 // leaf and its ancestors (each near block adds its column range to the leaves under its row node, a
 // consecutive range of leaf ordinals), sorted by first column (the ranges of one leaf are disjoint:
 // the blocks of a chain partition the columns), then the column lists and the panel table.
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
@@ -66,24 +69,206 @@ __device__ __forceinline__ bool separated(const BlockGeo& g, uint32_t a, uint32_
 
 }  // namespace
 
-// cls: 0 admissible, 1 near, 2 split; cnt: what the pair adds to each list.
-__global__ __launch_bounds__(256) void blk_classify_kernel(uint32_t m, const uint32_t* __restrict__ pa,
-                                                           const uint32_t* __restrict__ pb, BlockGeo g,
-                                                           uint8_t* __restrict__ cls, BlockCounts* __restrict__ cnt) {
-  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= m) return;
-  const uint32_t a = pa[i], b = pb[i];
+// The class of pair (a, b) (0 admissible, 1 near, 2 split) and what it adds to each list.
+__device__ __forceinline__ BlockCounts classify_pair(const BlockGeo& g, uint32_t a, uint32_t b, uint8_t& c) {
   const bool la = g.right[a] == 0, lb = g.right[b] == 0;
   const uint64_t na = g.end[a] - g.start[a], nb = g.end[b] - g.start[b];
-  uint8_t c;
   if (a != b && separated(g, a, b))
     c = 0;
   else if ((la && lb) || na * nb <= g.leaf_pairs)
     c = 1;
   else
     c = 2;
+  return {c == 0 ? 1u : 0u, c == 1 ? 1u : 0u, c == 2 ? (la ? 1u : 2u) * (lb ? 1u : 2u) : 0u};
+}
+
+// cls: 0 admissible, 1 near, 2 split; cnt: what the pair adds to each list.
+__global__ __launch_bounds__(256) void blk_classify_kernel(uint32_t m, const uint32_t* __restrict__ pa,
+                                                           const uint32_t* __restrict__ pb, BlockGeo g,
+                                                           uint8_t* __restrict__ cls, BlockCounts* __restrict__ cnt) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= m) return;
+  uint8_t c;
+  cnt[i] = classify_pair(g, pa[i], pb[i], c);
   cls[i] = c;
-  cnt[i] = {c == 0 ? 1u : 0u, c == 1 ? 1u : 0u, c == 2 ? (la ? 1u : 2u) * (lb ? 1u : 2u) : 0u};
+}
+
+// ---- levels driven by the device ----
+// The same classification without a host round trip per level: the level's pair count stays on the
+// device, and each kernel covers it with a fixed grid of kLevelBlocks blocks, block b taking a
+// contiguous range of the pairs.  A scan over the blocks' counts gives each block its first output
+// slot, and the block scans its range in order, so the lists come out in the same order as with
+// the per-level global scan.  List capacities are given; a level that would exceed one sets
+// `overflow` and the level and all later ones do nothing.
+constexpr uint32_t kLevelBlocks = 1024;
+
+struct LevelState {
+  uint32_t m[2];                // pairs of the level, by level parity
+  uint32_t adm_base, near_base; // list lengths before the current level
+  uint32_t n_adm, n_near;       // list lengths after it
+  uint32_t depth, max_m, overflow, pad;
+};
+struct LevelCaps {
+  uint32_t pairs, adm, near;
+};
+
+__device__ __forceinline__ void level_range(uint32_t m, uint32_t& lo, uint32_t& hi) {
+  const uint32_t per = (m + kLevelBlocks - 1) / kLevelBlocks;
+  lo = min(m, blockIdx.x * per);
+  hi = min(m, lo + per);
+}
+
+// Sum of 3 counts over the block's 256 threads (exclusive prefix per thread in `pre`, total returned).
+__device__ __forceinline__ BlockCounts block_scan_counts(BlockCounts v, BlockCounts& pre) {
+  __shared__ uint32_t sa[256], sn[256], sk[256];
+  const int t = threadIdx.x;
+  sa[t] = v.adm;
+  sn[t] = v.nearb;
+  sk[t] = v.kids;
+  __syncthreads();
+  for (int d = 1; d < 256; d <<= 1) {
+    uint32_t xa = 0, xn = 0, xk = 0;
+    if (t >= d) {
+      xa = sa[t - d];
+      xn = sn[t - d];
+      xk = sk[t - d];
+    }
+    __syncthreads();
+    sa[t] += xa;
+    sn[t] += xn;
+    sk[t] += xk;
+    __syncthreads();
+  }
+  pre = {sa[t] - v.adm, sn[t] - v.nearb, sk[t] - v.kids};
+  const BlockCounts tot{sa[255], sn[255], sk[255]};
+  __syncthreads();
+  return tot;
+}
+
+__global__ __launch_bounds__(256) void blk_level_count_kernel(const LevelState* __restrict__ st, int par,
+                                                              const uint32_t* __restrict__ pa,
+                                                              const uint32_t* __restrict__ pb, BlockGeo g,
+                                                              uint8_t* __restrict__ cls,
+                                                              BlockCounts* __restrict__ bsum) {
+  const uint32_t m = st->overflow ? 0u : st->m[par];
+  uint32_t lo, hi;
+  level_range(m, lo, hi);
+  BlockCounts acc{0, 0, 0};
+  for (uint32_t i = lo + threadIdx.x; i < hi; i += 256) {
+    uint8_t c;
+    const BlockCounts x = classify_pair(g, pa[i], pb[i], c);
+    cls[i] = c;
+    acc.adm += x.adm;
+    acc.nearb += x.nearb;
+    acc.kids += x.kids;
+  }
+  BlockCounts pre;
+  const BlockCounts tot = block_scan_counts(acc, pre);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+// One block of kLevelBlocks threads: the blocks' first output slots, the level's totals, the next
+// level's count.
+__global__ __launch_bounds__(1024) void blk_level_scan_kernel(LevelState* __restrict__ st, int par,
+                                                              const BlockCounts* __restrict__ bsum,
+                                                              BlockCounts* __restrict__ boff, LevelCaps caps) {
+  __shared__ uint32_t sa[kLevelBlocks], sn[kLevelBlocks], sk[kLevelBlocks];
+  const int t = threadIdx.x;
+  const BlockCounts v = bsum[t];
+  sa[t] = v.adm;
+  sn[t] = v.nearb;
+  sk[t] = v.kids;
+  __syncthreads();
+  for (int d = 1; d < (int)kLevelBlocks; d <<= 1) {
+    uint32_t xa = 0, xn = 0, xk = 0;
+    if (t >= d) {
+      xa = sa[t - d];
+      xn = sn[t - d];
+      xk = sk[t - d];
+    }
+    __syncthreads();
+    sa[t] += xa;
+    sn[t] += xn;
+    sk[t] += xk;
+    __syncthreads();
+  }
+  boff[t] = {sa[t] - v.adm, sn[t] - v.nearb, sk[t] - v.kids};
+  if (t == (int)kLevelBlocks - 1) {
+    const uint32_t ta = sa[t], tn = sn[t], tk = sk[t];
+    if (st->overflow) {
+      st->m[par ^ 1] = 0;
+      return;
+    }
+    const uint32_t m = st->m[par];
+    if (m > 0) ++st->depth;
+    st->max_m = max(st->max_m, m);
+    st->adm_base = st->n_adm;
+    st->near_base = st->n_near;
+    if ((uint64_t)st->n_adm + ta > caps.adm || (uint64_t)st->n_near + tn > caps.near || tk > caps.pairs) {
+      st->overflow = 1;
+      st->m[par ^ 1] = 0;
+      return;
+    }
+    st->n_adm += ta;
+    st->n_near += tn;
+    st->m[par ^ 1] = tk;
+  }
+}
+
+__global__ __launch_bounds__(256) void blk_level_scatter_kernel(const LevelState* __restrict__ st, int par,
+                                                                const uint32_t* __restrict__ pa,
+                                                                const uint32_t* __restrict__ pb,
+                                                                const uint32_t* __restrict__ right,
+                                                                const uint8_t* __restrict__ cls,
+                                                                const BlockCounts* __restrict__ boff,
+                                                                uint32_t* __restrict__ adm_a,
+                                                                uint32_t* __restrict__ adm_b,
+                                                                uint32_t* __restrict__ near_a,
+                                                                uint32_t* __restrict__ near_b,
+                                                                uint32_t* __restrict__ qa, uint32_t* __restrict__ qb) {
+  if (st->overflow) return;
+  const uint32_t m = st->m[par];
+  uint32_t lo, hi;
+  level_range(m, lo, hi);
+  BlockCounts base = boff[blockIdx.x];
+  base.adm += st->adm_base;
+  base.nearb += st->near_base;
+  for (uint32_t tile = lo; tile < hi; tile += 256) {
+    const uint32_t i = tile + threadIdx.x;
+    uint32_t a = 0, b = 0, ra = 0, rb = 0;
+    uint8_t c = 3;
+    BlockCounts x{0, 0, 0};
+    if (i < hi) {
+      a = pa[i];
+      b = pb[i];
+      c = cls[i];
+      ra = right[a];
+      rb = right[b];
+      x = {c == 0 ? 1u : 0u, c == 1 ? 1u : 0u, c == 2 ? (ra ? 2u : 1u) * (rb ? 2u : 1u) : 0u};
+    }
+    BlockCounts pre;
+    const BlockCounts tot = block_scan_counts(x, pre);
+    if (c == 0) {
+      adm_a[base.adm + pre.adm] = a;
+      adm_b[base.adm + pre.adm] = b;
+    } else if (c == 1) {
+      near_a[base.nearb + pre.nearb] = a;
+      near_b[base.nearb + pre.nearb] = b;
+    } else if (c == 2) {
+      const uint32_t ka[2] = {ra ? a + 1 : a, ra}, kb[2] = {rb ? b + 1 : b, rb};
+      const int xa = ra ? 2 : 1, xb = rb ? 2 : 1;
+      uint32_t q = base.kids + pre.kids;
+      for (int u = 0; u < xa; ++u)
+        for (int w = 0; w < xb; ++w) {
+          qa[q] = ka[u];
+          qb[q] = kb[w];
+          ++q;
+        }
+    }
+    base.adm += tot.adm;
+    base.nearb += tot.nearb;
+    base.kids += tot.kids;
+  }
 }
 
 __global__ __launch_bounds__(256) void blk_scatter_kernel(uint32_t m, const uint32_t* __restrict__ pa,
@@ -327,9 +512,120 @@ void grow_keep(ddgrs_ctx* ctx, DevBuf<uint32_t>& b, size_t used, size_t need) {
 
 }  // namespace
 
+// Per-level totals read back to the host: the level's lists are sized exactly.
+static void host_levels(ddgrs_ctx* ctx, BlockBuildScratch& S, const BlockGeo& g, const uint32_t* right, uint32_t& n_adm,
+                        uint32_t& n_near, uint32_t& depth, uint32_t& max_m) {
+  S.pa.reserve(1);
+  S.pb.reserve(1);
+  HIPCHECK(hipMemsetAsync(S.pa.ptr, 0, sizeof(uint32_t), ctx->stream));  // the root pair
+  HIPCHECK(hipMemsetAsync(S.pb.ptr, 0, sizeof(uint32_t), ctx->stream));
+  uint32_t m = 1;
+  n_adm = n_near = depth = 0;
+  max_m = 1;
+  BlockCounts* pin = (BlockCounts*)ctx->staging(2 * sizeof(BlockCounts));
+  while (m > 0) {
+    ++depth;
+    max_m = std::max(max_m, m);
+    S.cls.reserve(m);
+    S.cnt.reserve(m);
+    S.off.reserve(m);
+    launch(ctx, "blk_classify", blk_classify_kernel, dim3(blocks_for(m, 256)), dim3(256), 0, m,
+           (const uint32_t*)S.pa.ptr, (const uint32_t*)S.pb.ptr, g, S.cls.ptr, S.cnt.ptr);
+    size_t tb = 0;
+    const BlockCounts zero{0, 0, 0};
+    HIPCHECK(rocprim::exclusive_scan(nullptr, tb, S.cnt.ptr, S.off.ptr, zero, m, AddCounts(), ctx->stream));
+    S.temp.reserve(tb);
+    tb = S.temp.n;
+    HIPCHECK(rocprim::exclusive_scan(S.temp.ptr, tb, S.cnt.ptr, S.off.ptr, zero, m, AddCounts(), ctx->stream));
+    download(ctx, pin, (const BlockCounts*)S.off.ptr + (m - 1), 1);
+    download(ctx, pin + 1, (const BlockCounts*)S.cnt.ptr + (m - 1), 1);
+    ctx->sync();
+    const BlockCounts tot = AddCounts()(pin[0], pin[1]);
+    grow_keep(ctx, S.adm_a, n_adm, (size_t)n_adm + tot.adm);
+    grow_keep(ctx, S.adm_b, n_adm, (size_t)n_adm + tot.adm);
+    grow_keep(ctx, S.near_a, n_near, (size_t)n_near + tot.nearb);
+    grow_keep(ctx, S.near_b, n_near, (size_t)n_near + tot.nearb);
+    S.qa.reserve(std::max<uint32_t>(tot.kids, 1));
+    S.qb.reserve(std::max<uint32_t>(tot.kids, 1));
+    launch(ctx, "blk_scatter", blk_scatter_kernel, dim3(blocks_for(m, 256)), dim3(256), 0, m,
+           (const uint32_t*)S.pa.ptr, (const uint32_t*)S.pb.ptr, right, (const uint8_t*)S.cls.ptr,
+           (const BlockCounts*)S.off.ptr, n_adm, n_near, S.adm_a.ptr, S.adm_b.ptr, S.near_a.ptr, S.near_b.ptr,
+           S.qa.ptr, S.qb.ptr);
+    n_adm += tot.adm;
+    n_near += tot.nearb;
+    m = tot.kids;
+    std::swap(S.pa, S.qa);
+    std::swap(S.pb, S.qb);
+  }
+}
+
+// Levels driven by the device within the scratch's capacities: batches of levels launched without
+// reading anything back, then one read of the level state.  False when a list would have exceeded
+// its capacity (nothing usable was produced).
+static bool device_levels(ddgrs_ctx* ctx, BlockBuildScratch& S, const BlockGeo& g, const uint32_t* right, uint32_t& n_adm,
+                          uint32_t& n_near, uint32_t& depth, uint32_t& max_m) {
+  const LevelCaps caps{(uint32_t)std::min<uint64_t>(S.cap_pairs, UINT32_MAX),
+                       (uint32_t)std::min<uint64_t>(S.cap_adm, UINT32_MAX),
+                       (uint32_t)std::min<uint64_t>(S.cap_near, UINT32_MAX)};
+  S.pa.reserve(caps.pairs);
+  S.pb.reserve(caps.pairs);
+  S.qa.reserve(caps.pairs);
+  S.qb.reserve(caps.pairs);
+  S.cls.reserve(caps.pairs);
+  S.adm_a.reserve(caps.adm);
+  S.adm_b.reserve(caps.adm);
+  S.near_a.reserve(caps.near);
+  S.near_b.reserve(caps.near);
+  S.bsum.reserve(kLevelBlocks);
+  S.boff.reserve(kLevelBlocks);
+  S.lstate.reserve(sizeof(LevelState) / sizeof(uint32_t));
+  LevelState* st = (LevelState*)S.lstate.ptr;
+  HIPCHECK(hipMemsetAsync(st, 0, sizeof(LevelState), ctx->stream));
+  HIPCHECK(hipMemsetD32Async((hipDeviceptr_t)&st->m[0], 1u, 1, ctx->stream));  // level 0: the root pair
+  HIPCHECK(hipMemsetAsync(S.pa.ptr, 0, sizeof(uint32_t), ctx->stream));
+  HIPCHECK(hipMemsetAsync(S.pb.ptr, 0, sizeof(uint32_t), ctx->stream));
+  uint32_t *xa = S.pa.ptr, *xb = S.pb.ptr, *ya = S.qa.ptr, *yb = S.qb.ptr;
+  LevelState* hs = (LevelState*)ctx->staging(sizeof(LevelState));
+  uint32_t lev = 0, batch = S.last_depth + 2;
+  for (;;) {
+    for (uint32_t k = 0; k < batch; ++k, ++lev) {
+      const int par = (int)(lev & 1);
+      launch(ctx, "blk_level_count", blk_level_count_kernel, dim3(kLevelBlocks), dim3(256), 0, (const LevelState*)st,
+             par, (const uint32_t*)xa, (const uint32_t*)xb, g, S.cls.ptr, S.bsum.ptr);
+      launch(ctx, "blk_level_scan", blk_level_scan_kernel, dim3(1), dim3(kLevelBlocks), 0, st, par,
+             (const BlockCounts*)S.bsum.ptr, S.boff.ptr, caps);
+      launch(ctx, "blk_level_scatter", blk_level_scatter_kernel, dim3(kLevelBlocks), dim3(256), 0,
+             (const LevelState*)st, par, (const uint32_t*)xa, (const uint32_t*)xb, right, (const uint8_t*)S.cls.ptr,
+             (const BlockCounts*)S.boff.ptr, S.adm_a.ptr, S.adm_b.ptr, S.near_a.ptr, S.near_b.ptr, ya, yb);
+      std::swap(xa, ya);
+      std::swap(xb, yb);
+    }
+    download(ctx, hs, (const LevelState*)st, 1);
+    ctx->sync();
+    if (hs->overflow) return false;
+    if (hs->m[lev & 1] == 0) break;
+    batch = 4;
+  }
+  n_adm = hs->n_adm;
+  n_near = hs->n_near;
+  depth = hs->depth;
+  max_m = hs->max_m;
+  return true;
+}
+
 BlockStats BlockTreeDev::build_structure(ddgrs_ctx* ctx, const Topology& t, const Clusters& cl, double chi,
                                          size_t leaf_pairs, int nk_, const double* elem_weight) {
   BlockBuildScratch& S = scratch;
+  // DDGRS_TRACE_STEP=1: wall times of the build's phases on stderr
+  static const bool trace = std::getenv("DDGRS_TRACE_STEP") != nullptr;
+  auto t_part = std::chrono::steady_clock::now();
+  auto mark = [&](const char* what) {
+    if (!trace) return;
+    ctx->sync();
+    const auto now = std::chrono::steady_clock::now();
+    std::fprintf(stderr, "blocks: %-14s %8.2f ms\n", what, std::chrono::duration<double>(now - t_part).count() * 1e3);
+    t_part = now;
+  };
   nk = nk_;
   n = (int)t.order.size();
   nn = (int)t.n_nodes();
@@ -354,48 +650,20 @@ BlockStats BlockTreeDev::build_structure(ddgrs_ctx* ctx, const Topology& t, cons
          (const uint32_t*)order.ptr, elem_weight, weight.ptr);
   const BlockGeo g{start.ptr, end.ptr, right.ptr, S.lo.ptr, S.hi.ptr, S.radius.ptr, chi, (uint64_t)leaf_pairs};
 
-  // level-synchronous classification
-  S.pa.reserve(1);
-  S.pb.reserve(1);
-  const uint32_t root = 0;
-  HIPCHECK(hipMemcpyAsync(S.pa.ptr, &root, sizeof(root), hipMemcpyHostToDevice, ctx->stream));
-  HIPCHECK(hipMemcpyAsync(S.pb.ptr, &root, sizeof(root), hipMemcpyHostToDevice, ctx->stream));
-  uint32_t m = 1, n_adm = 0, n_near = 0, depth = 0;
-  BlockCounts* pin = (BlockCounts*)ctx->staging(2 * sizeof(BlockCounts));
-  while (m > 0) {
-    ++depth;
-    S.cls.reserve(m);
-    S.cnt.reserve(m);
-    S.off.reserve(m);
-    launch(ctx, "blk_classify", blk_classify_kernel, dim3(blocks_for(m, 256)), dim3(256), 0, m,
-           (const uint32_t*)S.pa.ptr, (const uint32_t*)S.pb.ptr, g, S.cls.ptr, S.cnt.ptr);
-    size_t tb = 0;
-    const BlockCounts zero{0, 0, 0};
-    HIPCHECK(rocprim::exclusive_scan(nullptr, tb, S.cnt.ptr, S.off.ptr, zero, m, AddCounts(), ctx->stream));
-    S.temp.reserve(tb);
-    tb = S.temp.n;
-    HIPCHECK(rocprim::exclusive_scan(S.temp.ptr, tb, S.cnt.ptr, S.off.ptr, zero, m, AddCounts(), ctx->stream));
-    download(ctx, pin, (const BlockCounts*)S.off.ptr + (m - 1), 1);
-    download(ctx, pin + 1, (const BlockCounts*)S.cnt.ptr + (m - 1), 1);
-    ctx->sync();
-    const BlockCounts tot = AddCounts()(pin[0], pin[1]);
-    grow_keep(ctx, S.adm_a, n_adm, (size_t)n_adm + tot.adm);
-    grow_keep(ctx, S.adm_b, n_adm, (size_t)n_adm + tot.adm);
-    grow_keep(ctx, S.near_a, n_near, (size_t)n_near + tot.nearb);
-    grow_keep(ctx, S.near_b, n_near, (size_t)n_near + tot.nearb);
-    S.qa.reserve(std::max<uint32_t>(tot.kids, 1));
-    S.qb.reserve(std::max<uint32_t>(tot.kids, 1));
-    launch(ctx, "blk_scatter", blk_scatter_kernel, dim3(blocks_for(m, 256)), dim3(256), 0, m,
-           (const uint32_t*)S.pa.ptr, (const uint32_t*)S.pb.ptr, (const uint32_t*)right.ptr, (const uint8_t*)S.cls.ptr,
-           (const BlockCounts*)S.off.ptr, n_adm, n_near, S.adm_a.ptr, S.adm_b.ptr, S.near_a.ptr, S.near_b.ptr,
-           S.qa.ptr, S.qb.ptr);
-    n_adm += tot.adm;
-    n_near += tot.nearb;
-    m = tot.kids;
-    std::swap(S.pa, S.qa);
-    std::swap(S.pb, S.qb);
-  }
+  mark("uploads");
+  // level-synchronous classification: driven by the device with the capacities of an earlier build,
+  // else (first build, or a list outgrew its capacity) with the totals read back per level
+  // DDGRS_BLOCK_LEVELS_ON_HOST=1 always reads the totals back per level (for comparisons)
+  const bool levels_on_host = std::getenv("DDGRS_BLOCK_LEVELS_ON_HOST") != nullptr;
+  uint32_t n_adm = 0, n_near = 0, depth = 0, max_m = 1;
+  if (!(S.cap_pairs && !levels_on_host && device_levels(ctx, S, g, right.ptr, n_adm, n_near, depth, max_m)))
+    host_levels(ctx, S, g, right.ptr, n_adm, n_near, depth, max_m);
+  S.cap_pairs = std::max<uint64_t>(S.cap_pairs, (uint64_t)max_m + max_m / 4 + 1024);
+  S.cap_adm = std::max<uint64_t>(S.cap_adm, (uint64_t)n_adm + n_adm / 4 + 1024);
+  S.cap_near = std::max<uint64_t>(S.cap_near, (uint64_t)n_near + n_near / 4 + 1024);
+  S.last_depth = depth;
 
+  mark("levels");
   // grouped by row node, creation order kept
   const int row_bits = bits_for((uint64_t)nn);
   n_far = n_adm;
@@ -421,6 +689,7 @@ BlockStats BlockTreeDev::build_structure(ddgrs_ctx* ctx, const Topology& t, cons
          (const uint32_t*)S.near_row.ptr, (const uint32_t*)S.near_col.ptr, (const uint32_t*)start.ptr,
          (const uint32_t*)end.ptr, S.sums.ptr + 2);
 
+  mark("grouping");
   // panels
   const size_t nl1 = (size_t)nl + 1;
   S.rcount.reserve(nl1);
@@ -524,6 +793,7 @@ BlockStats BlockTreeDev::build_structure(ddgrs_ctx* ctx, const Topology& t, cons
   far_coef.reserve((size_t)std::max<uint32_t>(n_far, 1) * nk);
   counters.reserve(nn);
   HIPCHECK(hipMemsetAsync(counters.ptr, 0, (size_t)nn * sizeof(uint32_t), ctx->stream));
+  mark("panels");
   BlockStats st;
   st.far_blocks = n_far;
   st.near_blocks = n_near;

@@ -17,6 +17,8 @@
 // far field (one wave per node), leaf far terms (one thread per leaf and channel), panels (one wave
 // per panel: lanes are columns, registers hold the panel's rows × C channels).
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
 #include <cstdlib>
 
 #include "blocktree.hpp"
@@ -833,6 +835,8 @@ void FaceMetricDev::build(ddgrs_ctx* ctx, ddgrs_surface* s, const Topology& t, c
   nv = (int)s->nv;
   nf = (int)s->nf;
   stats = bt.build_structure(ctx, t, cl, chi, leaf_pairs, 2, s->area.ptr);
+  static const bool trace = std::getenv("DDGRS_TRACE_STEP") != nullptr;
+  const auto t0 = std::chrono::steady_clock::now();
   inc_ptr.reserve((size_t)nv + 1);
   HIPCHECK(hipMemcpyAsync(inc_ptr.ptr, s->inc_ptr.ptr, ((size_t)nv + 1) * sizeof(uint32_t), hipMemcpyDeviceToDevice,
                           ctx->stream));
@@ -840,6 +844,11 @@ void FaceMetricDev::build(ddgrs_ctx* ctx, ddgrs_surface* s, const Topology& t, c
   launch(ctx, "fm_inc", fm_inc_kernel, dim3(blocks_for(3 * (int64_t)nf, 256)), dim3(256), 0, (uint32_t)(3 * nf),
          (const uint32_t*)bt.position.ptr, (const uint32_t*)s->inc.ptr, inc.ptr);
   face_metric_values(ctx, s, *this, cl.centroid, cl.projector, cl.mass, s_exp);
+  if (trace) {
+    ctx->sync();
+    std::fprintf(stderr, "blocks: %-14s %8.2f ms\n", "values",
+                 std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() * 1e3);
+  }
 }
 
 void FaceMetricDev::build(ddgrs_surface* s, const FaceMetricPlan& P, double s_exp) {

@@ -55,6 +55,12 @@ struct BlockBuildScratch {
   DevBuf<uint8_t> cls, temp;
   DevBuf<BlockCounts> cnt, off;
   DevBuf<unsigned long long> sums;
+  // Device-driven levels: per-block counts and offsets, the level state, and list capacities from
+  // earlier builds (0 before the first).
+  DevBuf<BlockCounts> bsum, boff;
+  DevBuf<uint32_t> lstate;
+  uint64_t cap_pairs = 0, cap_adm = 0, cap_near = 0;
+  uint32_t last_depth = 0;
 };
 
 // Device form of a block structure over a BVH, for products in difference form

@@ -206,3 +206,20 @@ def test_step_errors(pkg, ctx):
     with pytest.raises(pkg.RsError) as e:
         pkg.Session(s, pkg.StepOptions(exponents=pkg.Exponents(alpha=1.0, beta=2.0)))
     assert e.value.kind == "InvalidExponents"
+
+
+def test_device_levels_equal_host_levels(pkg, ctx, monkeypatch):
+    """Steps whose block cluster trees are classified level by level on the device (capacities from
+    the previous step) give bitwise the positions of steps that read every level's totals back."""
+    p, f = pkg.generate.config_torus_16k()
+    out = {}
+    for mode in ("device", "host"):
+        if mode == "host":
+            monkeypatch.setenv("DDGRS_BLOCK_LEVELS_ON_HOST", "1")
+        s = pkg.Surface(ctx, f, p)
+        ses = pkg.Session(s)
+        for _ in range(3):
+            ses.step()
+        out[mode] = s.positions()
+    monkeypatch.delenv("DDGRS_BLOCK_LEVELS_ON_HOST")
+    assert np.array_equal(out["device"], out["host"])
