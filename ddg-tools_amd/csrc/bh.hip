@@ -13,6 +13,7 @@
 // Own and partner sums are reduced over j by an xor butterfly (bitwise identical on all lanes,
 // independent of scheduling).  Pair arithmetic is FP32 with the hi/lo barycenter difference; the
 // node sums, the downward pass and the per-face epilogue are FP64.
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -96,7 +97,8 @@ __device__ __forceinline__ float isum8(float x) {  // sum over the 8 rows i (lan
 }
 
 struct BhArgs {
-  int n_leaves;
+  int n_leaves, n_items;
+  const uint32_t *item_leaf, *item_g0;  // per work item: leaf ordinal, first step
   const uint32_t *leaves, *nstart, *nend;
   const uint32_t *lumped_off, *lumped;
   const uint32_t *ex_off, *ex_pos;  // per leaf: BVH positions of the faces of its exact nodes (list order)
@@ -133,6 +135,16 @@ __device__ __forceinline__ void walk8(const uint32_t* __restrict__ list, uint32_
   }
 }
 
+// The entries of list [b, e) that fall in the item's steps [g0, g0 + kBhItemSteps), the list's
+// steps numbered from o (8 entries a step).
+__device__ __forceinline__ void item_piece(uint32_t b, uint32_t e, uint32_t o, uint32_t g0, uint32_t& pb,
+                                           uint32_t& pe) {
+  const uint32_t st = (e - b + 7) / 8, g1 = g0 + kBhItemSteps;
+  const uint32_t lo = g0 > o ? min(g0 - o, st) : 0u, hi = g1 > o ? min(g1 - o, st) : 0u;
+  pb = b + 8 * lo;
+  pe = max(pb, min(b + 8 * hi, e));
+}
+
 struct NodeData {
   float4 h, l;
 };
@@ -145,10 +157,12 @@ struct FaceDataN {
 
 template <int MODE>
 __global__ __launch_bounds__(256) void bh_gradient_kernel(BhArgs A, KPow<MODE> pw, float4* __restrict__ node_bu,
-                                                          float* __restrict__ part, double* __restrict__ leaf_energy) {
+                                                          float* __restrict__ part, double* __restrict__ item_energy) {
   const int lane = threadIdx.x & 63;
-  const int k = (int)xcd_wave_id();
-  if (k >= A.n_leaves) return;  // whole waves exit together
+  const int it = (int)xcd_wave_id();
+  if (it >= A.n_items) return;  // whole waves exit together
+  const int k = (int)A.item_leaf[it];
+  const uint32_t g0 = A.item_g0[it];
   const int i = lane >> 3, j = lane & 7;
   const uint32_t L = A.leaves[k];
   const uint32_t s0 = A.nstart[L];
@@ -162,8 +176,13 @@ __global__ __launch_bounds__(256) void bh_gradient_kernel(BhArgs A, KPow<MODE> p
   auto load_node = [&](uint32_t C) { return NodeData{A.chi[C], A.clo[C]}; };
   auto load_face = [&](uint32_t p) { return FaceData{A.xhi[p], A.xlo[p]}; };
   auto load_face_n = [&](uint32_t p) { return FaceDataN{A.xhi[p], A.xlo[p], A.nrm[p]}; };
-  walk8<4>(A.lumped, A.lumped_off[k], A.lumped_off[k + 1], lane, j, load_node,
-           [&](const NodeData& nd, uint32_t, bool act, uint32_t e) {
+  const uint32_t lb = A.lumped_off[k], le = A.lumped_off[k + 1], eb = A.ex_off[k], ee = A.ex_off[k + 1];
+  const uint32_t sl_steps = (le - lb + 7) / 8, se_steps = (ee - eb + 7) / 8;
+  uint32_t b1, e1, b2, e2, b3, e3;
+  item_piece(lb, le, 0, g0, b1, e1);
+  item_piece(eb, ee, sl_steps, g0, b2, e2);
+  item_piece(A.pt_off[k], A.pt_off[k + 1], sl_steps + se_steps, g0, b3, e3);
+  walk8<4>(A.lumped, b1, e1, lane, j, load_node, [&](const NodeData& nd, uint32_t, bool act, uint32_t e) {
     const float4 ch = nd.h, cl = nd.l;
     const float vx = (sh.x - ch.x) + (sl.x - cl.x);
     const float vy = (sh.y - ch.y) + (sl.y - cl.y);
@@ -188,8 +207,7 @@ __global__ __launch_bounds__(256) void bh_gradient_kernel(BhArgs A, KPow<MODE> p
   });
   // 2. exact blocks, own side: pairs (S_i, T) over the faces T of the leaf's exact nodes, slot j
   float row = 0.f, dx = 0.f, dy = 0.f, dz = 0.f, nx = 0.f, ny = 0.f, nz = 0.f;
-  walk8<4>(A.ex_pos, A.ex_off[k], A.ex_off[k + 1], lane, j, load_face,
-           [&](const FaceData& fd, uint32_t pt, bool valid, uint32_t) {
+  walk8<4>(A.ex_pos, b2, e2, lane, j, load_face, [&](const FaceData& fd, uint32_t pt, bool valid, uint32_t) {
     const bool act = vi && valid && pt != ps;
     const float4 th = fd.h, tl = fd.l;
     const float vx = (sh.x - th.x) + (sl.x - tl.x);
@@ -209,8 +227,7 @@ __global__ __launch_bounds__(256) void bh_gradient_kernel(BhArgs A, KPow<MODE> p
   });
   // 3. exact blocks, partner side: pairs (S', T_i) over the faces S' of the leaves that open ℓ
   float pa = 0.f, px = 0.f, py = 0.f, pz = 0.f;
-  walk8<4>(A.pt_pos, A.pt_off[k], A.pt_off[k + 1], lane, j, load_face_n,
-           [&](const FaceDataN& fd, uint32_t pu, bool valid, uint32_t) {
+  walk8<4>(A.pt_pos, b3, e3, lane, j, load_face_n, [&](const FaceDataN& fd, uint32_t pu, bool valid, uint32_t) {
     const bool act = vi && valid && pu != ps;
     const float4 uh = fd.h, ul = fd.l, un = fd.n;
     const float vx = (uh.x - sh.x) + (ul.x - sl.x);
@@ -236,7 +253,7 @@ __global__ __launch_bounds__(256) void bh_gradient_kernel(BhArgs A, KPow<MODE> p
   pa = xsum8(pa);
   px = xsum8(px); py = xsum8(py); pz = xsum8(pz);
   if (j == 0 && vi) {
-    float* o = part + 7 * (size_t)(s0 + i);
+    float* o = part + 7 * ((size_t)it * 8 + i);
     o[0] = (own_a + row) + pa;
     o[1] = (ox + as * dx) + px;
     o[2] = (oy + as * dy) + py;
@@ -249,14 +266,16 @@ __global__ __launch_bounds__(256) void bh_gradient_kernel(BhArgs A, KPow<MODE> p
   float el = en + ((j == 0 && vi) ? as * row : 0.f);
   el = xsum8(el);
   el = isum8(el);
-  if (lane == 0) leaf_energy[k] = (double)el;
+  if (lane == 0) item_energy[it] = (double)el;
 }
 
 template <int MODE>
-__global__ __launch_bounds__(256) void bh_energy_kernel(BhArgs A, KPow<MODE> pw, double* __restrict__ leaf_energy) {
+__global__ __launch_bounds__(256) void bh_energy_kernel(BhArgs A, KPow<MODE> pw, double* __restrict__ item_energy) {
   const int lane = threadIdx.x & 63;
-  const int k = (int)xcd_wave_id();
-  if (k >= A.n_leaves) return;
+  const int it = (int)xcd_wave_id();
+  if (it >= A.n_items) return;
+  const int k = (int)A.item_leaf[it];
+  const uint32_t g0 = A.item_g0[it];
   const int i = lane >> 3, j = lane & 7;
   const uint32_t L = A.leaves[k];
   const uint32_t s0 = A.nstart[L];
@@ -268,8 +287,11 @@ __global__ __launch_bounds__(256) void bh_energy_kernel(BhArgs A, KPow<MODE> pw,
   float en = 0.f;
   auto load_node = [&](uint32_t C) { return NodeData{A.chi[C], A.clo[C]}; };
   auto load_face = [&](uint32_t p) { return FaceData{A.xhi[p], A.xlo[p]}; };
-  walk8<4>(A.lumped, A.lumped_off[k], A.lumped_off[k + 1], lane, j, load_node,
-           [&](const NodeData& nd, uint32_t, bool act, uint32_t) {
+  const uint32_t lb = A.lumped_off[k], le = A.lumped_off[k + 1];
+  uint32_t b1, e1, b2, e2;
+  item_piece(lb, le, 0, g0, b1, e1);
+  item_piece(A.ex_off[k], A.ex_off[k + 1], (le - lb + 7) / 8, g0, b2, e2);
+  walk8<4>(A.lumped, b1, e1, lane, j, load_node, [&](const NodeData& nd, uint32_t, bool act, uint32_t) {
     const float4 ch = nd.h, cl = nd.l;
     const float vx = (sh.x - ch.x) + (sl.x - cl.x);
     const float vy = (sh.y - ch.y) + (sl.y - cl.y);
@@ -277,8 +299,7 @@ __global__ __launch_bounds__(256) void bh_energy_kernel(BhArgs A, KPow<MODE> pw,
     const float kk = kval<MODE>(pw, vx, vy, vz, sn.x, sn.y, sn.z, vx * vx + vy * vy + vz * vz);
     en += (act && vi) ? as * ch.w * kk : 0.f;
   });
-  walk8<4>(A.ex_pos, A.ex_off[k], A.ex_off[k + 1], lane, j, load_face,
-           [&](const FaceData& fd, uint32_t pt, bool valid, uint32_t) {
+  walk8<4>(A.ex_pos, b2, e2, lane, j, load_face, [&](const FaceData& fd, uint32_t pt, bool valid, uint32_t) {
     const bool act = vi && valid && pt != ps;
     const float4 th = fd.h, tl = fd.l;
     const float vx = (sh.x - th.x) + (sl.x - tl.x);
@@ -290,7 +311,7 @@ __global__ __launch_bounds__(256) void bh_energy_kernel(BhArgs A, KPow<MODE> pw,
   });
   en = xsum8(en);
   en = isum8(en);
-  if (lane == 0) leaf_energy[k] = (double)en;
+  if (lane == 0) item_energy[it] = (double)en;
 }
 
 #pragma clang fp contract(off)
@@ -389,6 +410,9 @@ __global__ __launch_bounds__(256) void bh_node_sums_kernel(int nn, const uint32_
 // and the final partials (:540-567), written in face order (7 FP64 per face).
 __global__ __launch_bounds__(256) void bh_final_kernel(int nf, const uint32_t* __restrict__ order,
                                                        const uint32_t* __restrict__ leaf_of_pos,
+                                                       const uint32_t* __restrict__ leaves,
+                                                       const uint32_t* __restrict__ nstart,
+                                                       const uint32_t* __restrict__ item_ptr,
                                                        const uint32_t* __restrict__ chain_ptr,
                                                        const uint32_t* __restrict__ chain,
                                                        const double4* __restrict__ sums, const double* __restrict__ cen,
@@ -412,13 +436,19 @@ __global__ __launch_bounds__(256) void bh_final_kernel(int nf, const uint32_t* _
     a = c;
   }
   uint32_t f = order[p];
-  const float* pp = part + 7 * (size_t)p;
+  // the pair partials of face p: the sum over its leaf's work items, in item order
+  const uint32_t row = p - nstart[leaves[k]];
+  double pp[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (uint32_t q = item_ptr[k]; q < item_ptr[k + 1]; ++q) {
+    const float* src = part + 7 * ((size_t)q * 8 + row);
+    for (int c = 0; c < 7; ++c) pp[c] += (double)src[c];
+  }
   double at = area[f];
   double d[3] = {bary[3 * f] - cen[3 * a], bary[3 * f + 1] - cen[3 * a + 1], bary[3 * f + 2] - cen[3 * a + 2]};
   double* o = out + 7 * (size_t)f;
-  o[0] = (double)pp[0] + (beta + (U[0] * d[0] + U[1] * d[1] + U[2] * d[2]));
-  for (int k2 = 0; k2 < 3; ++k2) o[1 + k2] = (double)pp[1 + k2] + at * U[k2];
-  for (int k2 = 0; k2 < 3; ++k2) o[4 + k2] = (double)pp[4 + k2];
+  o[0] = pp[0] + (beta + (U[0] * d[0] + U[1] * d[1] + U[2] * d[2]));
+  for (int k2 = 0; k2 < 3; ++k2) o[1 + k2] = pp[1 + k2] + at * U[k2];
+  for (int k2 = 0; k2 < 3; ++k2) o[4 + k2] = pp[4 + k2];
 }
 
 // ---- host ------------------------------------------------------------------------------------
@@ -467,8 +497,6 @@ void BhEval::upload_structure(ddgrs_ctx* ctx) {
   chi.reserve(nn);
   clo.reserve(nn);
   sums.reserve(nn);
-  part.reserve(7 * (size_t)nf);
-  leaf_energy.reserve(nl);
 }
 
 void BhEval::refresh(ddgrs_surface* s) {
@@ -490,6 +518,9 @@ void BhEval::refresh(ddgrs_surface* s) {
 BhArgs BhEval::args() const {
   BhArgs A;
   A.n_leaves = nl;
+  A.n_items = n_items;
+  A.item_leaf = item_leaf.ptr;
+  A.item_g0 = item_g0.ptr;
   A.leaves = d_leaves.ptr;
   A.nstart = d_start.ptr;
   A.nend = d_end.ptr;
@@ -509,37 +540,38 @@ BhArgs BhEval::args() const {
 
 void BhEval::energy(ddgrs_ctx* ctx, const Exps& e, double* total) {
   BhArgs A = args();
-  dim3 grid(blocks_for(nl, 4));
+  dim3 grid(blocks_for(n_items, 4));
   if (e.alpha_int == 6 && e.beta_int == 12)
-    launch(ctx, "bh_energy", bh_energy_kernel<0>, grid, dim3(256), 0, A, KPow<0>{e.alpha, e.beta, 6, 12}, leaf_energy.ptr);
+    launch(ctx, "bh_energy", bh_energy_kernel<0>, grid, dim3(256), 0, A, KPow<0>{e.alpha, e.beta, 6, 12}, item_energy.ptr);
   else if (e.alpha_int >= 0 && e.beta_int >= 0)
     launch(ctx, "bh_energy", bh_energy_kernel<1>, grid, dim3(256), 0, A, KPow<1>{e.alpha, e.beta, e.alpha_int, e.beta_int},
-           leaf_energy.ptr);
+           item_energy.ptr);
   else
-    launch(ctx, "bh_energy", bh_energy_kernel<2>, grid, dim3(256), 0, A, KPow<2>{e.alpha, e.beta, -1, -1}, leaf_energy.ptr);
-  sum_f64(ctx, nl, leaf_energy.ptr, total);
+    launch(ctx, "bh_energy", bh_energy_kernel<2>, grid, dim3(256), 0, A, KPow<2>{e.alpha, e.beta, -1, -1}, item_energy.ptr);
+  sum_f64(ctx, n_items, item_energy.ptr, total);
 }
 
 void BhEval::gradient(ddgrs_surface* s, const Exps& e, double* total, double* partials) {
   ddgrs_ctx* ctx = s->ctx;
   BhArgs A = args();
-  dim3 grid(blocks_for(nl, 4));
+  dim3 grid(blocks_for(n_items, 4));
   if (e.alpha_int == 6 && e.beta_int == 12)
     launch(ctx, "bh_gradient", bh_gradient_kernel<0>, grid, dim3(256), 0, A, KPow<0>{e.alpha, e.beta, 6, 12}, node_bu.ptr,
-           part.ptr, leaf_energy.ptr);
+           part.ptr, item_energy.ptr);
   else if (e.alpha_int >= 0 && e.beta_int >= 0)
     launch(ctx, "bh_gradient", bh_gradient_kernel<1>, grid, dim3(256), 0, A,
-           KPow<1>{e.alpha, e.beta, e.alpha_int, e.beta_int}, node_bu.ptr, part.ptr, leaf_energy.ptr);
+           KPow<1>{e.alpha, e.beta, e.alpha_int, e.beta_int}, node_bu.ptr, part.ptr, item_energy.ptr);
   else
     launch(ctx, "bh_gradient", bh_gradient_kernel<2>, grid, dim3(256), 0, A, KPow<2>{e.alpha, e.beta, -1, -1},
-           node_bu.ptr, part.ptr, leaf_energy.ptr);
+           node_bu.ptr, part.ptr, item_energy.ptr);
   launch(ctx, "bh_node_sums", bh_node_sums_kernel, dim3(blocks_for(nn, 4)), dim3(256), 0, nn,
          (const uint32_t*)d_lumped_t_off.ptr, (const uint32_t*)d_lumped_t.ptr, (const float4*)node_bu.ptr, sums.ptr);
   launch(ctx, "bh_final", bh_final_kernel, dim3(blocks_for(nf, 256)), dim3(256), 0, nf, (const uint32_t*)d_order.ptr,
-         (const uint32_t*)d_leaf_of_pos.ptr, (const uint32_t*)d_chain_ptr.ptr, (const uint32_t*)d_chain.ptr,
+         (const uint32_t*)d_leaf_of_pos.ptr, (const uint32_t*)d_leaves.ptr, (const uint32_t*)d_start.ptr,
+         (const uint32_t*)item_ptr.ptr, (const uint32_t*)d_chain_ptr.ptr, (const uint32_t*)d_chain.ptr,
          (const double4*)sums.ptr, (const double*)cen.ptr, (const float*)part.ptr, (const double*)s->area.ptr,
          (const double*)s->bary.ptr, partials);
-  sum_f64(ctx, nl, leaf_energy.ptr, total);
+  sum_f64(ctx, n_items, item_energy.ptr, total);
 }
 
 HostGeometry download_geometry(ddgrs_surface* s) {
@@ -662,8 +694,40 @@ void BhEval::build_from(ddgrs_surface* s, const Topology& t, const Clusters& c, 
     build_lists_device(ctx);
   }
   flatten_lists(ctx);
+  build_items(ctx);
   node_bu.reserve(std::max<size_t>(1, n_lumped));
   refresh(s);
+  if (std::getenv("DDGRS_TRACE_STEP")) trace_lists(ctx);
+}
+
+// Per-leaf list lengths on stderr (lumped nodes, exact faces, partner faces): mean, p99, largest,
+// and the 8-entry steps of the longest leaf, for reading the pair kernels' balance.
+void BhEval::trace_lists(ddgrs_ctx* ctx) const {
+  const uint32_t* offs[3] = {d_lumped_off.ptr, d_ex_off.ptr, d_pt_off.ptr};
+  const char* names[3] = {"lumped", "exact", "partner"};
+  std::vector<uint64_t> steps(nl, 0);
+  for (int q = 0; q < 3; ++q) {
+    std::vector<uint32_t> o(nl + 1);
+    download(ctx, o.data(), offs[q], o.size());
+    ctx->sync();
+    std::vector<uint32_t> len(nl);
+    double sum = 0.0;
+    for (int k = 0; k < nl; ++k) {
+      len[k] = o[k + 1] - o[k];
+      sum += len[k];
+      steps[k] += (len[k] + 7) / 8;
+    }
+    std::sort(len.begin(), len.end());
+    std::fprintf(stderr, "bh lists: %-8s mean %.1f p50 %u p99 %u max %u\n", names[q], sum / nl, len[nl / 2],
+                 len[(size_t)(0.99 * nl)], len.back());
+  }
+  uint64_t tot = 0, mx = 0;
+  for (uint64_t x : steps) {
+    tot += x;
+    mx = std::max(mx, x);
+  }
+  std::fprintf(stderr, "bh lists: steps per leaf mean %.1f max %llu (total %llu)\n", (double)tot / nl,
+               (unsigned long long)mx, (unsigned long long)tot);
 }
 
 BhLists BhEval::download_lists(ddgrs_ctx* ctx) const {

@@ -10,6 +10,11 @@ namespace ddgrs {
 
 struct BhArgs;
 
+// Steps (of 8 list entries) per work item of the Barnes–Hut pair kernels.  The BVH keeps a few long
+// thin leaves along mesh seams whose exact lists hold tens of thousands of faces (about 100 times
+// the mean); cut into items, they spread over many waves.
+constexpr uint32_t kBhItemSteps = 256;
+
 struct BhEval {
   double theta = 0.5;
   size_t leaf_size = 8;
@@ -25,7 +30,7 @@ struct BhEval {
   std::vector<uint32_t> level_off;
   // data of the current positions
   DevBuf<float4> pxhi, pxlo, pnrm, chi, clo, node_bu;
-  DevBuf<double> mass, moment, cen, leaf_energy;
+  DevBuf<double> mass, moment, cen;
   DevBuf<double4> sums;
   DevBuf<float> part;
   // device list build
@@ -35,6 +40,12 @@ struct BhEval {
   // and of the leaves that open it (partner side), with per-entry sizes and offsets
   DevBuf<uint32_t> d_ex_off, d_ex_pos, d_pt_off, d_pt_pos, ex_size, ex_fo, pt_size, pt_fo;
   DevBuf<uint8_t> d_bhnode;  // per node a 64-byte record for the list walks
+  // Work items of the pair kernels: a leaf's walk (lumped, exact and partner entries, 8 per step)
+  // cut into pieces of at most kBhItemSteps steps; per item its leaf and first step, per leaf its
+  // first item.  The per-face partials and energies are written per item and summed in item order.
+  int n_items = 0;
+  DevBuf<uint32_t> item_leaf, item_g0, item_ptr, item_cnt;
+  DevBuf<double> item_energy;
   DevBuf<unsigned long long> d_pairs;
   DevBuf<uint8_t> temp;
 
@@ -50,6 +61,9 @@ struct BhEval {
   void gradient(ddgrs_surface* s, const Exps& e, double* total, double* partials);
   // The lists on the host (copied from the device).
   BhLists download_lists(ddgrs_ctx* ctx) const;
+  // The work items of the current lists.
+  void build_items(ddgrs_ctx* ctx);
+  void trace_lists(ddgrs_ctx* ctx) const;
 
  private:
   void upload_structure(ddgrs_ctx* ctx);

@@ -342,6 +342,60 @@ __global__ __launch_bounds__(256) void bhl_flat_off_kernel(uint32_t nl, const ui
   if (k <= nl) off[k] = fo[entry_off[k]];
 }
 
+// Work items per leaf ordinal k < nl (zero at k = nl, so the scan ends with the total): the leaf's
+// steps (lumped, exact and partner entries, 8 a step) in pieces of kBhItemSteps, at least one.
+__global__ __launch_bounds__(256) void bh_item_count_kernel(uint32_t nl, const uint32_t* __restrict__ lumped_off,
+                                                            const uint32_t* __restrict__ ex_off,
+                                                            const uint32_t* __restrict__ pt_off,
+                                                            uint32_t* __restrict__ cnt) {
+  const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+  if (k > nl) return;
+  if (k == nl) {
+    cnt[k] = 0;
+    return;
+  }
+  const uint32_t steps = (lumped_off[k + 1] - lumped_off[k] + 7) / 8 + (ex_off[k + 1] - ex_off[k] + 7) / 8 +
+                         (pt_off[k + 1] - pt_off[k] + 7) / 8;
+  cnt[k] = max(1u, (steps + kBhItemSteps - 1) / kBhItemSteps);
+}
+
+__global__ __launch_bounds__(256) void bh_item_fill_kernel(uint32_t nl, const uint32_t* __restrict__ item_ptr,
+                                                           uint32_t* __restrict__ item_leaf,
+                                                           uint32_t* __restrict__ item_g0) {
+  const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= nl) return;
+  for (uint32_t q = item_ptr[k], g = 0; q < item_ptr[k + 1]; ++q, g += kBhItemSteps) {
+    item_leaf[q] = k;
+    item_g0[q] = g;
+  }
+}
+
+void BhEval::build_items(ddgrs_ctx* ctx) {
+  const size_t nl1 = (size_t)nl + 1;
+  item_cnt.reserve(nl1);
+  item_ptr.reserve(nl1);
+  launch(ctx, "bh_item_count", bh_item_count_kernel, dim3(blocks_for((int64_t)nl1, 256)), dim3(256), 0, (uint32_t)nl,
+         (const uint32_t*)d_lumped_off.ptr, (const uint32_t*)d_ex_off.ptr, (const uint32_t*)d_pt_off.ptr,
+         item_cnt.ptr);
+  size_t tb = 0;
+  HIPCHECK(rocprim::exclusive_scan(nullptr, tb, item_cnt.ptr, item_ptr.ptr, 0u, nl1, rocprim::plus<uint32_t>(),
+                                   ctx->stream));
+  temp.reserve(tb);
+  tb = temp.n;
+  HIPCHECK(rocprim::exclusive_scan(temp.ptr, tb, item_cnt.ptr, item_ptr.ptr, 0u, nl1, rocprim::plus<uint32_t>(),
+                                   ctx->stream));
+  uint32_t* pin = (uint32_t*)ctx->staging(sizeof(uint32_t));
+  download(ctx, pin, (const uint32_t*)item_ptr.ptr + nl, 1);
+  ctx->sync();
+  n_items = (int)*pin;
+  item_leaf.reserve(std::max(n_items, 1));
+  item_g0.reserve(std::max(n_items, 1));
+  launch(ctx, "bh_item_fill", bh_item_fill_kernel, dim3(blocks_for(nl, 256)), dim3(256), 0, (uint32_t)nl,
+         (const uint32_t*)item_ptr.ptr, item_leaf.ptr, item_g0.ptr);
+  part.reserve(7 * 8 * (size_t)std::max(n_items, 1));
+  item_energy.reserve(std::max(n_items, 1));
+}
+
 void BhEval::flatten_lists(ddgrs_ctx* ctx) {
   const size_t nl1 = (size_t)nl + 1;
   d_ex_off.reserve(nl1);
