@@ -34,7 +34,8 @@ struct BhEval {
   DevBuf<double4> sums;
   DevBuf<float> part;
   // device list build
-  DevBuf<uint32_t> d_leaf_ordinal, d_parent, d_task_root, d_task_depth, l_count, e_count, l_task_off, e_task_off;
+  DevBuf<uint32_t> d_leaf_ordinal, d_parent, d_task_root, d_task_depth, d_task_ptr, d_cut_base, l_count, e_count,
+      l_task_off, e_task_off;
   DevBuf<uint32_t> l_idx, l_key_s, e_tgt, e_src, e_tgt_s;
   // flattened exact lists: per leaf the BVH positions of the faces of its exact nodes (own side)
   // and of the leaves that open it (partner side), with per-entry sizes and offsets

@@ -7,7 +7,8 @@
 // the node after A's subtree (its escape index), after an opened node at A + 1.
 //
 // A leaf's walk is split into tasks, one per root of a cut of the tree (the nodes at depth D and
-// the shallower leaves, in pre-order): a task first tests the ancestors of its root from the top;
+// the shallower leaves, in pre-order; a deeper cut for the few leaves much wider than the median):
+// a task first tests the ancestors of its root from the top;
 // if one of them is lumped or a leaf, the walk never enters the root's subtree and the task emits
 // that ancestor only if its root is the first cut node below it (reached from it by left children
 // alone); otherwise the task walks its root's subtree.  The tasks of a leaf, in pre-order of their
@@ -64,21 +65,42 @@ __device__ __forceinline__ bool bh_lumped(const BhTree& T, const LeafBox& L, con
 }
 }  // namespace
 
-constexpr int kMaxCutDepth = 5;
+// Two cuts: at depth kCutDepth for most leaves, and at kLongCutDepth for leaves whose radius exceeds
+// kLongLeafRadius times the median leaf radius (the long thin leaves along mesh seams, whose walks
+// open most of the tree): their walks are split into up to 2^kLongCutDepth tasks.
+constexpr int kCutDepth = 5;
+constexpr int kLongCutDepth = 9;
+constexpr double kLongLeafRadius = 3.0;
 
 struct BhTasks {
-  const uint32_t* root;   // per task: its cut node
-  const uint32_t* depth;  // per task: the cut node's depth
+  const uint32_t* root;      // per cut node (the short cut's, then the long cut's): the node
+  const uint32_t* depth;     // per cut node: its depth
   const uint32_t* parent;
-  uint32_t count;
+  const uint32_t* task_ptr;  // per leaf ordinal: its first task (nl + 1)
+  const uint32_t* cut_base;  // per leaf ordinal: its cut's first entry in root/depth
+  uint32_t nl;
 };
 
-// Walks task j of leaf k, calling emit(node, lumped) for each entry in order.
+// The leaf ordinal k of task q (task_ptr[k] ≤ q < task_ptr[k + 1]) and the task's cut node index.
+__device__ __forceinline__ void bh_task_of(const BhTasks& W, uint32_t q, uint32_t& k, uint32_t& j) {
+  uint32_t lo = 0, hi = W.nl;  // the last k in [0, nl) with task_ptr[k] ≤ q
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (W.task_ptr[mid] <= q)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  k = lo;
+  j = W.cut_base[k] + (q - W.task_ptr[k]);
+}
+
+// Walks task j (a cut node index) of leaf k, calling emit(node, lumped) for each entry in order.
 template <class Emit>
 __device__ __forceinline__ void bh_task_walk(const BhTree& T, const BhTasks& W, uint32_t k, uint32_t j, Emit&& emit) {
   const uint32_t l = T.leaves[k], c = W.root[j], m = W.depth[j];
   const LeafBox L = leaf_box(T, l);
-  uint32_t path[kMaxCutDepth];
+  uint32_t path[kLongCutDepth];
   {
     uint32_t x = c;
     for (int i = (int)m - 1; i >= 0; --i) {
@@ -116,10 +138,11 @@ __global__ __launch_bounds__(256) void bhl_count_kernel(BhTree T, BhTasks W, uin
                                                         uint32_t* __restrict__ ecount,
                                                         unsigned long long* __restrict__ pairs) {
   __shared__ unsigned long long red[256];
-  const uint32_t q = blockIdx.x * 256 + threadIdx.x, nq = nl * W.count;
+  const uint32_t q = blockIdx.x * 256 + threadIdx.x, nq = W.task_ptr[nl];
   unsigned long long pk = 0;
   if (q < nq) {
-    const uint32_t k = q / W.count, j = q % W.count;
+    uint32_t k, j;
+    bh_task_of(W, q, k, j);
     const uint32_t l = T.leaves[k];
     const uint64_t ns = T.end[l] - T.start[l];
     uint32_t nlum = 0, nex = 0;
@@ -155,14 +178,19 @@ __global__ __launch_bounds__(256) void bhl_fill_kernel(BhTree T, BhTasks W, uint
                                                        uint32_t* __restrict__ lumped, uint32_t* __restrict__ lumped_e,
                                                        uint32_t* __restrict__ exact, uint32_t* __restrict__ exact_tgt,
                                                        uint32_t* __restrict__ exact_src) {
-  const uint32_t q = blockIdx.x * 256 + threadIdx.x, nq = nl * W.count;
+  const uint32_t q = blockIdx.x * 256 + threadIdx.x, nq = W.task_ptr[nl];
   if (q > nq) return;
-  const uint32_t k = q / W.count, j = q % W.count;
-  if (j == 0) {  // the leaf's first task (q = nq: the totals)
+  if (q == nq) {  // the totals
+    lumped_off[nl] = ltask_off[q];
+    exact_off[nl] = etask_off[q];
+    return;
+  }
+  uint32_t k, j;
+  bh_task_of(W, q, k, j);
+  if (q == W.task_ptr[k]) {  // the leaf's first task
     lumped_off[k] = ltask_off[q];
     exact_off[k] = etask_off[q];
   }
-  if (q == nq) return;
   uint32_t ql = ltask_off[q], qe = etask_off[q];
   bh_task_walk(T, W, k, j, [&](uint32_t a, bool lum) {
     if (lum) {
@@ -212,15 +240,33 @@ void BhEval::build_lists_device(ddgrs_ctx* ctx) {
       esc[a + 1] = t.right[a];
       esc[t.right[a]] = esc[a];
     }
-  // the cut: nodes at depth D and shallower leaves, in pre-order
-  const uint32_t D = std::min<uint32_t>(kMaxCutDepth, t.height > 0 ? t.height - 1 : 0);
+  // the cuts: nodes at depth D and shallower leaves, in pre-order (the short cut, then the long one)
   std::vector<uint32_t> troot, tdepth;
-  for (int a = 0; a < nn; ++a)
-    if (t.depth[a] == D || (t.depth[a] < D && t.right[a] == 0)) {
-      troot.push_back((uint32_t)a);
-      tdepth.push_back(t.depth[a]);
-    }
-  const uint32_t ntask = (uint32_t)troot.size();
+  uint32_t ncut[2];
+  for (int c = 0; c < 2; ++c) {
+    const uint32_t D = std::min<uint32_t>(c ? kLongCutDepth : kCutDepth, t.height > 0 ? t.height - 1 : 0);
+    const size_t n0 = troot.size();
+    for (int a = 0; a < nn; ++a)
+      if (t.depth[a] == D || (t.depth[a] < D && t.right[a] == 0)) {
+        troot.push_back((uint32_t)a);
+        tdepth.push_back(t.depth[a]);
+      }
+    ncut[c] = (uint32_t)(troot.size() - n0);
+  }
+  // per leaf its tasks: the long cut for leaves much wider than the median leaf
+  std::vector<double> lr(nl);
+  for (int k = 0; k < nl; ++k) lr[k] = clusters.radius[t.leaves[k]];
+  std::vector<double> srt = lr;
+  std::nth_element(srt.begin(), srt.begin() + nl / 2, srt.end());
+  const double long_r = kLongLeafRadius * (nl ? srt[nl / 2] : 0.0);
+  std::vector<uint32_t> task_ptr(nl + 1, 0), cut_base(nl);
+  for (int k = 0; k < nl; ++k) {
+    const bool wide = lr[k] > long_r;
+    cut_base[k] = wide ? ncut[0] : 0;
+    const uint64_t next = (uint64_t)task_ptr[k] + (wide ? ncut[1] : ncut[0]);
+    if (next >= UINT32_MAX) throw Error(DDGRS_GPU, "bh_lists", "too many (leaf, task) pairs");
+    task_ptr[k + 1] = (uint32_t)next;
+  }
   std::vector<BhNode> nodes(nn);
   for (int a = 0; a < nn; ++a) {
     BhNode& r = nodes[a];
@@ -238,10 +284,11 @@ void BhEval::build_lists_device(ddgrs_ctx* ctx) {
   ddgrs::upload(ctx, d_parent, t.parent);
   ddgrs::upload(ctx, d_task_root, troot);
   ddgrs::upload(ctx, d_task_depth, tdepth);
+  ddgrs::upload(ctx, d_task_ptr, task_ptr);
+  ddgrs::upload(ctx, d_cut_base, cut_base);
   const BhTree T{(const BhNode*)d_bhnode.ptr, d_start.ptr, d_end.ptr, d_leaves.ptr, (uint32_t)nn, theta * theta};
-  const BhTasks W{d_task_root.ptr, d_task_depth.ptr, d_parent.ptr, ntask};
-  const size_t nq1 = (size_t)nl * ntask + 1, nl1 = (size_t)nl + 1;
-  if (nq1 > UINT32_MAX) throw Error(DDGRS_GPU, "bh_lists", "too many (leaf, task) pairs");
+  const BhTasks W{d_task_root.ptr, d_task_depth.ptr, d_parent.ptr, d_task_ptr.ptr, d_cut_base.ptr, (uint32_t)nl};
+  const size_t nq1 = (size_t)task_ptr[nl] + 1, nl1 = (size_t)nl + 1;
   l_count.reserve(nq1);
   e_count.reserve(nq1);
   l_task_off.reserve(nq1);
