@@ -95,12 +95,14 @@ __global__ __launch_bounds__(256) void blk_classify_kernel(uint32_t m, const uin
 
 // ---- levels driven by the device ----
 // The same classification without a host round trip per level: the level's pair count stays on the
-// device, and each kernel covers it with a fixed grid of kLevelBlocks blocks, block b taking a
+// device, and each kernel covers it with a grid of at most kLevelBlocks blocks (sized from the
+// previous build's level sizes), block b taking a
 // contiguous range of the pairs.  A scan over the blocks' counts gives each block its first output
 // slot, and the block scans its range in order, so the lists come out in the same order as with
 // the per-level global scan.  List capacities are given; a level that would exceed one sets
 // `overflow` and the level and all later ones do nothing.
 constexpr uint32_t kLevelBlocks = 1024;
+constexpr uint32_t kMaxLevels = 64;  // levels whose sizes are recorded for the next build's grids
 
 struct LevelState {
   uint32_t m[2];                // pairs of the level, by level parity
@@ -113,7 +115,7 @@ struct LevelCaps {
 };
 
 __device__ __forceinline__ void level_range(uint32_t m, uint32_t& lo, uint32_t& hi) {
-  const uint32_t per = (m + kLevelBlocks - 1) / kLevelBlocks;
+  const uint32_t per = (m + gridDim.x - 1) / gridDim.x;
   lo = min(m, blockIdx.x * per);
   hi = min(m, lo + per);
 }
@@ -169,12 +171,13 @@ __global__ __launch_bounds__(256) void blk_level_count_kernel(const LevelState* 
 
 // One block of kLevelBlocks threads: the blocks' first output slots, the level's totals, the next
 // level's count.
-__global__ __launch_bounds__(1024) void blk_level_scan_kernel(LevelState* __restrict__ st, int par,
-                                                              const BlockCounts* __restrict__ bsum,
-                                                              BlockCounts* __restrict__ boff, LevelCaps caps) {
+__global__ __launch_bounds__(1024) void blk_level_scan_kernel(LevelState* __restrict__ st, int par, uint32_t nblk,
+                                                              uint32_t lev, const BlockCounts* __restrict__ bsum,
+                                                              BlockCounts* __restrict__ boff, uint32_t* __restrict__ level_m,
+                                                              LevelCaps caps) {
   __shared__ uint32_t sa[kLevelBlocks], sn[kLevelBlocks], sk[kLevelBlocks];
   const int t = threadIdx.x;
-  const BlockCounts v = bsum[t];
+  const BlockCounts v = (uint32_t)t < nblk ? bsum[t] : BlockCounts{0, 0, 0};
   sa[t] = v.adm;
   sn[t] = v.nearb;
   sk[t] = v.kids;
@@ -200,6 +203,7 @@ __global__ __launch_bounds__(1024) void blk_level_scan_kernel(LevelState* __rest
       return;
     }
     const uint32_t m = st->m[par];
+    if (lev < kMaxLevels) level_m[lev] = m;
     if (m > 0) ++st->depth;
     st->max_m = max(st->max_m, m);
     st->adm_base = st->n_adm;
@@ -523,9 +527,11 @@ static void host_levels(ddgrs_ctx* ctx, BlockBuildScratch& S, const BlockGeo& g,
   n_adm = n_near = depth = 0;
   max_m = 1;
   BlockCounts* pin = (BlockCounts*)ctx->staging(2 * sizeof(BlockCounts));
+  S.prev_m.clear();
   while (m > 0) {
     ++depth;
     max_m = std::max(max_m, m);
+    if (S.prev_m.size() < kMaxLevels) S.prev_m.push_back(m);
     S.cls.reserve(m);
     S.cnt.reserve(m);
     S.off.reserve(m);
@@ -578,6 +584,12 @@ static bool device_levels(ddgrs_ctx* ctx, BlockBuildScratch& S, const BlockGeo& 
   S.near_b.reserve(caps.near);
   S.bsum.reserve(kLevelBlocks);
   S.boff.reserve(kLevelBlocks);
+  S.level_m.reserve(kMaxLevels);
+  // a level's grid: about 512 pairs per block for the size the previous build had at that level
+  auto grid_for = [&](uint32_t lev) -> uint32_t {
+    const uint64_t m = lev < S.prev_m.size() ? S.prev_m[lev] : 0;
+    return (uint32_t)std::min<uint64_t>(kLevelBlocks, std::max<uint64_t>(1, (m + m / 4 + 511) / 512));
+  };
   S.lstate.reserve(sizeof(LevelState) / sizeof(uint32_t));
   LevelState* st = (LevelState*)S.lstate.ptr;
   HIPCHECK(hipMemsetAsync(st, 0, sizeof(LevelState), ctx->stream));
@@ -590,11 +602,12 @@ static bool device_levels(ddgrs_ctx* ctx, BlockBuildScratch& S, const BlockGeo& 
   for (;;) {
     for (uint32_t k = 0; k < batch; ++k, ++lev) {
       const int par = (int)(lev & 1);
-      launch(ctx, "blk_level_count", blk_level_count_kernel, dim3(kLevelBlocks), dim3(256), 0, (const LevelState*)st,
-             par, (const uint32_t*)xa, (const uint32_t*)xb, g, S.cls.ptr, S.bsum.ptr);
-      launch(ctx, "blk_level_scan", blk_level_scan_kernel, dim3(1), dim3(kLevelBlocks), 0, st, par,
-             (const BlockCounts*)S.bsum.ptr, S.boff.ptr, caps);
-      launch(ctx, "blk_level_scatter", blk_level_scatter_kernel, dim3(kLevelBlocks), dim3(256), 0,
+      const uint32_t nblk = grid_for(lev);
+      launch(ctx, "blk_level_count", blk_level_count_kernel, dim3(nblk), dim3(256), 0, (const LevelState*)st, par,
+             (const uint32_t*)xa, (const uint32_t*)xb, g, S.cls.ptr, S.bsum.ptr);
+      launch(ctx, "blk_level_scan", blk_level_scan_kernel, dim3(1), dim3(kLevelBlocks), 0, st, par, nblk, lev,
+             (const BlockCounts*)S.bsum.ptr, S.boff.ptr, S.level_m.ptr, caps);
+      launch(ctx, "blk_level_scatter", blk_level_scatter_kernel, dim3(nblk), dim3(256), 0,
              (const LevelState*)st, par, (const uint32_t*)xa, (const uint32_t*)xb, right, (const uint8_t*)S.cls.ptr,
              (const BlockCounts*)S.boff.ptr, S.adm_a.ptr, S.adm_b.ptr, S.near_a.ptr, S.near_b.ptr, ya, yb);
       std::swap(xa, ya);
@@ -610,6 +623,9 @@ static bool device_levels(ddgrs_ctx* ctx, BlockBuildScratch& S, const BlockGeo& 
   n_near = hs->n_near;
   depth = hs->depth;
   max_m = hs->max_m;
+  S.prev_m.resize(std::min<uint32_t>(lev, kMaxLevels));
+  download(ctx, S.prev_m.data(), (const uint32_t*)S.level_m.ptr, S.prev_m.size());
+  ctx->sync();
   return true;
 }
 

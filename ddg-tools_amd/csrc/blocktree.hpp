@@ -58,7 +58,8 @@ struct BlockBuildScratch {
   // Device-driven levels: per-block counts and offsets, the level state, and list capacities from
   // earlier builds (0 before the first).
   DevBuf<BlockCounts> bsum, boff;
-  DevBuf<uint32_t> lstate;
+  DevBuf<uint32_t> lstate, level_m;
+  std::vector<uint32_t> prev_m;  // pairs per level of the previous build (sizes the level grids)
   uint64_t cap_pairs = 0, cap_adm = 0, cap_near = 0;
   uint32_t last_depth = 0;
 };
