@@ -433,6 +433,7 @@ class _GmresReportC(ctypes.Structure):
         ("operator_applications", _I64),
         ("preconditioner_applications", _I64),
         ("converged", _I32),
+        ("extended", _I32),
         ("relative_residual", _D),
     ]
 
@@ -512,11 +513,12 @@ class GmresReport:
     preconditioner_applications: int
     converged: bool
     relative_residual: float
+    extended: bool = False  # the products switched to the extended near field (FP32 residual floor reached)
 
 
 def _gmres_report(c: _GmresReportC) -> GmresReport:
     return GmresReport(int(c.iterations), int(c.cycles), int(c.operator_applications), int(c.preconditioner_applications),
-                       bool(c.converged), float(c.relative_residual))
+                       bool(c.converged), float(c.relative_residual), bool(c.extended))
 
 
 class Saddle(_Handle):

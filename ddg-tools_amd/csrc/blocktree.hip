@@ -40,18 +40,18 @@ __device__ __forceinline__ float wsum64(float x) {
 // bit MASK a lane keeps one half of its values and adds the partner's copy of that half (the
 // partner keeps the other), until N is odd; the rest is summed by a butterfly.  Afterwards lane L
 // holds the sums of values [base, base + n) in v[0, n), with n and base from rs_slots.
-template <int N, int MASK>
-__device__ __forceinline__ void reduce_scatter(float* v, int lane) {
+template <int N, int MASK, class T>
+__device__ __forceinline__ void reduce_scatter(T* v, int lane) {
   if constexpr (MASK >= 1) {
     if constexpr (N % 2 == 0) {
       const bool up = (lane & MASK) != 0;
 #pragma unroll
       for (int i = 0; i < N / 2; ++i) {
-        const float keep = up ? v[N / 2 + i] : v[i];
-        const float send = up ? v[i] : v[N / 2 + i];
+        const T keep = up ? v[N / 2 + i] : v[i];
+        const T send = up ? v[i] : v[N / 2 + i];
         v[i] = keep + __shfl_xor(send, MASK);
       }
-      reduce_scatter<N / 2, MASK / 2>(v, lane);
+      reduce_scatter<N / 2, MASK / 2, T>(v, lane);
     } else {
 #pragma unroll
       for (int i = 0; i < N; ++i)
@@ -224,7 +224,8 @@ struct PanelArgs {
 struct SegArgs {
   const uint32_t *panel, *col0, *ptr;
   uint32_t* arrive;
-  float* partial;
+  float* partial;     // FP32 products
+  double* partial64;  // extended products
   uint32_t cols;
   int n;
 };
@@ -268,8 +269,11 @@ __device__ __forceinline__ void panel_load(PanelChunk<C, NK>& x, const float* __
   }
 }
 
-template <int C, int H, int NK>
-__device__ __forceinline__ void panel_compute(float (&acc)[kPanelRows][C], const PanelChunk<C, NK>& x,
+// Acc float: the difference and the sum in FP32.  Acc double (the extended product): the
+// difference of the hi/lo pairs exactly and the sum in FP64, so that the product's rounding is
+// that of the FP32 values alone.
+template <int C, int H, int NK, class Acc>
+__device__ __forceinline__ void panel_compute(Acc (&acc)[kPanelRows][C], const PanelChunk<C, NK>& x,
                                               const float* __restrict__ u_hl, uint32_t row0, uint32_t R) {
 #pragma unroll
   for (int r = 0; r < (int)kPanelRows; ++r) {
@@ -277,8 +281,13 @@ __device__ __forceinline__ void panel_compute(float (&acc)[kPanelRows][C], const
       const float* srow = u_hl + (size_t)(row0 + r) * 2 * C;
 #pragma unroll
       for (int c = 0; c < C; ++c) {
-        const float d = (srow[c] - x.th[c]) + (srow[C + c] - x.tl[c]);
-        acc[r][c] += (c < H ? x.v0[r] : x.v1[r]) * d;
+        if constexpr (sizeof(Acc) == sizeof(float)) {
+          const float d = (srow[c] - x.th[c]) + (srow[C + c] - x.tl[c]);
+          acc[r][c] += (c < H ? x.v0[r] : x.v1[r]) * d;
+        } else {
+          const double d = ((double)srow[c] - (double)x.th[c]) + ((double)srow[C + c] - (double)x.tl[c]);
+          acc[r][c] += (double)(c < H ? x.v0[r] : x.v1[r]) * d;
+        }
       }
     }
   }
@@ -288,7 +297,7 @@ __device__ __forceinline__ void panel_compute(float (&acc)[kPanelRows][C], const
 // segments of a longer panel store their sums, and the last to arrive finishes the panel from the
 // stored sums in segment order (the result does not depend on the arrival order).  The chunks go
 // through two register sets: the loads of the next chunk are in flight while one is summed.
-template <int C, int H, int NK>
+template <int C, int H, int NK, class Acc>
 __global__ __launch_bounds__(256) void bt_panel_kernel(PanelArgs P, SegArgs G, const float* __restrict__ u_hl,
                                                        double w0, double w1, float* __restrict__ z_hl) {
   const uint32_t sg = xcd_wave_id();
@@ -300,7 +309,7 @@ __global__ __launch_bounds__(256) void bt_panel_kernel(PanelArgs P, SegArgs G, c
   const uint32_t kend = min(K, j0 + G.cols);
   const uint32_t nch = (kend - j0 + 63) / 64;  // chunks of the segment (at least one)
   const float* vbase = P.val + P.panel_val[p] * NK;
-  float acc[kPanelRows][C];
+  Acc acc[kPanelRows][C];
 #pragma unroll
   for (int r = 0; r < (int)kPanelRows; ++r)
 #pragma unroll
@@ -316,17 +325,17 @@ __global__ __launch_bounds__(256) void bt_panel_kernel(PanelArgs P, SegArgs G, c
     const uint32_t km = j0 + 64 * m + lane;
     cib = P.cols[min(c0 + km + 64, last)];
     panel_load<C, NK>(a, u_hl, cia, vbase, P.vzero, K, km, km < kend, R);
-    if (m > 0) panel_compute<C, H, NK>(acc, b, u_hl, row0, R);
+    if (m > 0) panel_compute<C, H, NK, Acc>(acc, b, u_hl, row0, R);
     if (m >= nch) break;
     cia = P.cols[min(c0 + km + 128, last)];
     panel_load<C, NK>(b, u_hl, cib, vbase, P.vzero, K, km + 64, km + 64 < kend, R);
-    panel_compute<C, H, NK>(acc, a, u_hl, row0, R);
+    panel_compute<C, H, NK, Acc>(acc, a, u_hl, row0, R);
     if (m + 1 >= nch) break;
   }
   // the sums over the wave: reduce-scatter, then each lane whose low bits are 0 finishes its values
   constexpr int NV = (int)kPanelRows * C;
-  float* flat = &acc[0][0];
-  reduce_scatter<NV, 32>(flat, lane);
+  Acc* flat = &acc[0][0];
+  reduce_scatter<NV, 32, Acc>(flat, lane);
   int nmine, base;
   rs_slots<NV, 32>(lane, nmine, base);
   // lanes that differ only in the bits the butterfly summed over hold the same sums: the lowest writes
@@ -349,7 +358,10 @@ __global__ __launch_bounds__(256) void bt_panel_kernel(PanelArgs P, SegArgs G, c
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
       if (q >= nmine) break;
-      publish(&G.partial[(size_t)sg * NV + base + q], flat[q]);
+      if constexpr (sizeof(Acc) == sizeof(float))
+        publish(&G.partial[(size_t)sg * NV + base + q], flat[q]);
+      else
+        publish(&G.partial64[(size_t)sg * NV + base + q], flat[q]);
     }
   }
   published();  // the partial sums are visible before the arrival
@@ -361,7 +373,12 @@ __global__ __launch_bounds__(256) void bt_panel_kernel(PanelArgs P, SegArgs G, c
   for (int idx = lane; idx < NV; idx += 64) {
     if ((uint32_t)(idx / C) >= R) continue;
     double near = 0.0;
-    for (uint32_t t = s0; t < s0 + ns; ++t) near += (double)collect(&G.partial[(size_t)t * NV + idx]);
+    for (uint32_t t = s0; t < s0 + ns; ++t) {
+      if constexpr (sizeof(Acc) == sizeof(float))
+        near += (double)collect(&G.partial[(size_t)t * NV + idx]);
+      else
+        near += collect(&G.partial64[(size_t)t * NV + idx]);
+    }
     panel_finish<C, H, NK>(P, k, row0, idx / C, idx % C, near, u_hl, w0, w1, z_hl);
   }
 }
@@ -543,7 +560,7 @@ void BlockTreeDev::finish_far(ddgrs_ctx* ctx) {
 #define BT_NAME(base) (C == 12 ? base "_c12" : C == 3 ? base "_c3" : base "_c")
 
 template <int C, int H>
-void BlockTreeDev::multiply(ddgrs_ctx* ctx, const float* u_hl, double w0, double w1, float* z_hl) {
+void BlockTreeDev::multiply(ddgrs_ctx* ctx, const float* u_hl, double w0, double w1, float* z_hl, bool extended) {
   sigma.reserve((size_t)nn * C);
   mean.reserve((size_t)nn * C);
   gamma.reserve((size_t)nn * C);
@@ -567,18 +584,29 @@ void BlockTreeDev::multiply(ddgrs_ctx* ctx, const float* u_hl, double w0, double
   PanelArgs pa{panel_leaf.ptr, panel_row0.ptr, panel_rows.ptr, col_ptr.ptr,        cols.ptr, leaves.ptr,
                panel_val.ptr,  val.ptr,        alpha_leaf.ptr, (const double*)mean.ptr, (const double*)delta.ptr, np,
                val.ptr + (size_t)near_total * nk};
-  partial.reserve((size_t)std::max(nseg, 1) * kPanelRows * C);
-  SegArgs sa{seg_panel.ptr, seg_col0.ptr, seg_ptr.ptr, seg_arrive.ptr, partial.ptr, seg_cols, nseg};
-  if (nk == 2)
-    launch(ctx, BT_NAME("bt_panel"), bt_panel_kernel<C, H, 2>, dim3(blocks_for(nseg, 4)), dim3(256), 0, pa, sa, u_hl,
-           w0, w1, z_hl);
-  else
-    launch(ctx, BT_NAME("bt_panel"), bt_panel_kernel<C, H, 1>, dim3(blocks_for(nseg, 4)), dim3(256), 0, pa, sa, u_hl,
-           w0, w1, z_hl);
+  SegArgs sa{seg_panel.ptr, seg_col0.ptr, seg_ptr.ptr, seg_arrive.ptr, nullptr, nullptr, seg_cols, nseg};
+  const dim3 grid(blocks_for(nseg, 4));
+  if (!extended) {
+    partial.reserve((size_t)std::max(nseg, 1) * kPanelRows * C);
+    sa.partial = partial.ptr;
+    if (nk == 2)
+      launch(ctx, BT_NAME("bt_panel"), bt_panel_kernel<C, H, 2, float>, grid, dim3(256), 0, pa, sa, u_hl, w0, w1, z_hl);
+    else
+      launch(ctx, BT_NAME("bt_panel"), bt_panel_kernel<C, H, 1, float>, grid, dim3(256), 0, pa, sa, u_hl, w0, w1, z_hl);
+  } else {
+    partial64.reserve((size_t)std::max(nseg, 1) * kPanelRows * C);
+    sa.partial64 = partial64.ptr;
+    if (nk == 2)
+      launch(ctx, BT_NAME("bt_panel_x"), bt_panel_kernel<C, H, 2, double>, grid, dim3(256), 0, pa, sa, u_hl, w0, w1,
+             z_hl);
+    else
+      launch(ctx, BT_NAME("bt_panel_x"), bt_panel_kernel<C, H, 1, double>, grid, dim3(256), 0, pa, sa, u_hl, w0, w1,
+             z_hl);
+  }
 }
 
-template void BlockTreeDev::multiply<12, 9>(ddgrs_ctx*, const float*, double, double, float*);
-template void BlockTreeDev::multiply<3, 3>(ddgrs_ctx*, const float*, double, double, float*);
+template void BlockTreeDev::multiply<12, 9>(ddgrs_ctx*, const float*, double, double, float*, bool);
+template void BlockTreeDev::multiply<3, 3>(ddgrs_ctx*, const float*, double, double, float*, bool);
 
 // ---- face metric ---------------------------------------------------------------------------
 
@@ -807,7 +835,7 @@ void face_metric_values(ddgrs_ctx* ctx, ddgrs_surface* s, FaceMetricDev& F, cons
   bt.finish_far(ctx);
   PanelArgs pa{bt.panel_leaf.ptr, bt.panel_row0.ptr, bt.panel_rows.ptr, bt.col_ptr.ptr, bt.cols.ptr, bt.leaves.ptr,
                bt.panel_val.ptr,  nullptr,          nullptr,           nullptr,         nullptr,     bt.np};
-  const SegArgs sa{bt.seg_panel.ptr, bt.seg_col0.ptr, bt.seg_ptr.ptr, nullptr, nullptr, bt.seg_cols, bt.nseg};
+  const SegArgs sa{bt.seg_panel.ptr, bt.seg_col0.ptr, bt.seg_ptr.ptr, nullptr, nullptr, nullptr, bt.seg_cols, bt.nseg};
   launch(ctx, "fm_panel_values", fm_panel_values_kernel, dim3(blocks_for(bt.nseg, 4)), dim3(256), 0, pa, sa,
          (const uint32_t*)bt.order.ptr, (const double*)s->bary.ptr, (const double*)s->normal.ptr,
          (const double*)s->area.ptr, neg_s, bt.val.ptr);
@@ -866,7 +894,7 @@ void FaceMetricDev::build(ddgrs_surface* s, const FaceMetricPlan& P, double s_ex
 void FaceMetricDev::apply(ddgrs_ctx* ctx, const double* x, double* y) {
   launch(ctx, "fm_m", fm_m_kernel, dim3(blocks_for(nf, 256)), dim3(256), 0, nf, (const uint32_t*)vert_p.ptr,
          (const float*)df.ptr, x, m_hl.ptr);
-  bt.multiply<12, 9>(ctx, m_hl.ptr, w_hi, w_lo, z_hl.ptr);
+  bt.multiply<12, 9>(ctx, m_hl.ptr, w_hi, w_lo, z_hl.ptr, extended);
   launch(ctx, "fm_gather", fm_gather_kernel, dim3(blocks_for(nv, 256)), dim3(256), 0, nv, (const uint32_t*)inc_ptr.ptr,
          (const uint32_t*)inc.ptr, (const float*)df.ptr, (const double*)area_p.ptr, (const float*)z_hl.ptr, y);
 }
@@ -984,7 +1012,7 @@ void vertex_fractional_values(ddgrs_ctx* ctx, VertexFractionalDev& V, const Clus
   bt.finish_far(ctx);
   PanelArgs pa{bt.panel_leaf.ptr, bt.panel_row0.ptr, bt.panel_rows.ptr, bt.col_ptr.ptr, bt.cols.ptr, bt.leaves.ptr,
                bt.panel_val.ptr,  nullptr,          nullptr,           nullptr,         nullptr,     bt.np};
-  const SegArgs sa{bt.seg_panel.ptr, bt.seg_col0.ptr, bt.seg_ptr.ptr, nullptr, nullptr, bt.seg_cols, bt.nseg};
+  const SegArgs sa{bt.seg_panel.ptr, bt.seg_col0.ptr, bt.seg_ptr.ptr, nullptr, nullptr, nullptr, bt.seg_cols, bt.nseg};
   launch(ctx, "vf_panel_values", vf_panel_values_kernel, dim3(blocks_for(bt.nseg, 4)), dim3(256), 0, pa, sa,
          (const uint32_t*)bt.order.ptr, (const double*)dpos.ptr, (const double*)V.mass.ptr, ex, bt.val.ptr);
   V.u_hl.reserve(6 * (size_t)V.nv);
@@ -1016,7 +1044,7 @@ void VertexFractionalDev::build(ddgrs_ctx* ctx, const std::vector<double>& pos, 
 void VertexFractionalDev::apply(ddgrs_ctx* ctx, const double* x, double* y) {
   launch(ctx, "vf_u", vf_u_kernel, dim3(blocks_for(nv, 256)), dim3(256), 0, nv, (const uint32_t*)bt.order.ptr, x,
          u_hl.ptr);
-  bt.multiply<3, 3>(ctx, u_hl.ptr, 1.0, 1.0, z_hl.ptr);
+  bt.multiply<3, 3>(ctx, u_hl.ptr, 1.0, 1.0, z_hl.ptr, extended);
   launch(ctx, "vf_out", vf_out_kernel, dim3(blocks_for(nv, 256)), dim3(256), 0, nv, (const uint32_t*)bt.position.ptr,
          (const double*)mass.ptr, (const float*)z_hl.ptr, y);
 }

@@ -94,6 +94,7 @@ struct BlockTreeDev {
   DevBuf<uint32_t> seg_panel, seg_col0, seg_ptr;
   DevBuf<uint32_t> seg_arrive;  // per panel, zero between products
   DevBuf<float> partial;        // per segment: kPanelRows × C near sums
+  DevBuf<double> partial64;     // the same for the extended product
   DevBuf<float> val;           // NK per entry, then NK zeros (read by masked lanes)
   DevBuf<uint32_t> far_ptr, far_col, far_row;
   DevBuf<float> far_coef;      // NK per far entry: κ_AB A_B
@@ -120,8 +121,10 @@ struct BlockTreeDev {
   // z = w_k (H̃a·u − H̃(a u)) per BVH position and channel, channels [0, H) on kernel 0 with w0,
   // [H, C) on kernel NK − 1 with w1.  u and z are hi/lo FP32 pairs: 2 C floats per position
   // (C hi, then C lo).
+  // extended: the near field's differences and sums in FP64 (the values stay FP32), for solves that
+  // need a residual below the FP32 product's rounding.
   template <int C, int H>
-  void multiply(ddgrs_ctx* ctx, const float* u_hl, double w0, double w1, float* z_hl);
+  void multiply(ddgrs_ctx* ctx, const float* u_hl, double w0, double w1, float* z_hl, bool extended = false);
 };
 
 // Host part of a FaceMetricDev on a given face tree.
@@ -147,6 +150,7 @@ struct FaceMetricDev {
   DevBuf<float> m_hl, z_hl; // product scratch: 12 channels, hi/lo
   DevBuf<uint32_t> inc_ptr, inc;  // vertex → (face, corner), with faces' BVH positions
   double w_hi = 1.0, w_lo = 1.0;
+  bool extended = false;  // products with the extended near field (BlockTreeDev::multiply)
 
   // Builds on a face tree of the surface's current positions with separation χ and leaf_pairs
   // (structure and panels on the device).
@@ -181,6 +185,7 @@ struct VertexFractionalDev {
   int nv = 0;
   DevBuf<double> mass;      // per vertex (FP64)
   DevBuf<float> u_hl, z_hl; // product scratch: 3 channels, hi/lo
+  bool extended = false;    // products with the extended near field (BlockTreeDev::multiply)
   void build(ddgrs_ctx* ctx, const VertexFractionalPlan& p);
   // The same with the structure and panels built on the device; `topo` the vertex BVH of `pos`.
   void build(ddgrs_ctx* ctx, const std::vector<double>& pos, const std::vector<double>& mass, double tau, double chi,

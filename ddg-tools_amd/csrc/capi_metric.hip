@@ -32,6 +32,13 @@ bool blocks_on_host() {
   return e && e[0] == '1';
 }
 
+// DDGRS_PRODUCTS_EXTENDED=1 applies the metric and L̃^V with the extended near field (FP64
+// differences and sums), which the solves switch to when the FP32 products' rounding limits them.
+bool products_extended() {
+  const char* e = std::getenv("DDGRS_PRODUCTS_EXTENDED");
+  return e && e[0] == '1';
+}
+
 // Runs `apply(x_dev, y_dev)` on nb host vectors of length n.
 template <class F>
 void batch(ddgrs_ctx* ctx, int64_t nb, int64_t n, const double* xs, double* ys, F&& apply) {
@@ -86,7 +93,9 @@ int ddgrs_metric_apply_batch(ddgrs_metric* h, double w_hi, double w_lo, int64_t 
     h->fm.w_hi = w_hi;
     h->fm.w_lo = w_lo;
     ddgrs_ctx* ctx = h->s->ctx;
+    h->fm.extended = products_extended();
     batch(ctx, nb, 3 * h->s->nv, xs, ys, [&](const double* x, double* y) { h->fm.apply(ctx, x, y); });
+    h->fm.extended = false;
   });
 }
 
@@ -136,7 +145,9 @@ int ddgrs_vfrac_apply_batch(ddgrs_vfrac* h, int64_t nb, const double* xs, double
     need(xs, "xs");
     need(ys, "ys");
     ddgrs_ctx* ctx = h->s->ctx;
+    h->vf.extended = products_extended();
     batch(ctx, nb, 3 * h->s->nv, xs, ys, [&](const double* x, double* y) { h->vf.apply(ctx, x, y); });
+    h->vf.extended = false;
   });
 }
 

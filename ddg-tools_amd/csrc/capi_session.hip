@@ -54,6 +54,7 @@ void fill_report(const GmresResult& r, ddgrs_gmres_report* out) {
   out->operator_applications = (int64_t)r.operator_applications;
   out->preconditioner_applications = (int64_t)r.preconditioner_applications;
   out->converged = r.converged ? 1 : 0;
+  out->extended = r.extended ? 1 : 0;
   out->relative_residual = r.relative_residual;
 }
 }  // namespace
@@ -153,7 +154,15 @@ int ddgrs_saddle_solve(ddgrs_saddle* h, const double* b, double* x, int64_t rest
     GmresSettings gs{(size_t)restart, tolerance, (size_t)max_iterations};
     auto A = [&](const double* u, double* y) { h->op.apply(ctx, u, y); };
     auto P = [&](const double* r, double* z) { h->pc.apply(ctx, r, z); };
-    GmresResult r = gmres_device(ctx, n, A, P, db.ptr, dx.ptr, gs, h->vec);
+    FaceMetricDev& fm = *h->op.fm;
+    fm.extended = h->pc.vf.extended = false;
+    auto stall = [&]() {
+      if (fm.extended) return false;
+      fm.extended = h->pc.vf.extended = true;
+      return true;
+    };
+    GmresResult r = gmres_device(ctx, n, A, P, db.ptr, dx.ptr, gs, h->vec, stall);
+    fm.extended = h->pc.vf.extended = false;
     download(ctx, x, (const double*)dx.ptr, n);
     ctx->sync();
     if (report) fill_report(r, report);

@@ -170,8 +170,10 @@ __global__ void init_g_kernel(double* g, int m1, const double* nrm2) {
 
 GmresResult gmres_device(ddgrs_ctx* ctx, size_t n, const std::function<void(const double*, double*)>& op,
                          const std::function<void(const double*, double*)>& prec, const double* b, double* x,
-                         const GmresSettings& o, DeviceVecs& vec) {
+                         const GmresSettings& o, DeviceVecs& vec, const std::function<bool()>& on_stall) {
   GmresResult rep;
+  bool est_reached = false;  // the last cycle's Arnoldi estimate reached the tolerance
+  double prev_rel = 0.0;
   const size_t m = std::min(o.restart, std::max<size_t>(n, 1));
   const unsigned nb = blocks_for((int64_t)n, 256);
   HIPCHECK(hipMemsetAsync(x, 0, n * sizeof(double), ctx->stream));
@@ -220,6 +222,13 @@ GmresResult gmres_device(ddgrs_ctx* ctx, size_t n, const std::function<void(cons
       rep.converged = true;
       break;
     }
+    const bool stalled = est_reached || (rel > 0.25 * prev_rel && rel < 100.0 * o.tolerance);
+    if (rep.cycles > 0 && !rep.extended && on_stall && stalled && on_stall()) {
+      rep.extended = true;
+      est_reached = false;
+      continue;  // the explicit residual again, with the switched products
+    }
+    prev_rel = rel;
     if (rep.iterations >= o.max_iterations) break;
     rep.cycles++;
     launch(ctx, "gm_scale", scale_kernel, dim3(nb), dim3(256), 0, n, (const double*)r.ptr, (const double*)nrm2, 1, Vb.ptr);
@@ -245,6 +254,7 @@ GmresResult gmres_device(ddgrs_ctx* ctx, size_t n, const std::function<void(cons
       rep.history.push_back(est);
       if (hn == 0.0 || !std::isfinite(hn) || est <= o.tolerance || rep.iterations >= o.max_iterations || j + 1 >= m) {
         k_end = j + 1;
+        est_reached = est <= o.tolerance;
         break;
       }
       launch(ctx, "gm_scale", scale_kernel, dim3(nb), dim3(256), 0, n, (const double*)w.ptr, (const double*)nrm2, 1,

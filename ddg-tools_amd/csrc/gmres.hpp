@@ -25,6 +25,7 @@ struct GmresSettings {
 struct GmresResult {
   size_t iterations = 0, cycles = 0, operator_applications = 0, preconditioner_applications = 0;
   bool converged = false;
+  bool extended = false;  // on_stall switched the products
   double relative_residual = 0.0;
   std::vector<double> history, cycle_residuals;
 };
@@ -47,8 +48,14 @@ struct DeviceVecs {
 // classical Gram–Schmidt applied twice (two fixed-order GEMV passes per pass instead of j dependent
 // reductions), the Givens rotations and the Hessenberg back substitution run on the device; all in
 // FP64.  Vectors of length n are device pointers.
+//
+// on_stall (optional) is called once when the explicit residual stays above the tolerance although
+// the Arnoldi estimate of the cycle before reached it, or a whole cycle reduced it less than 4×
+// within 100× of the tolerance: the operator's rounding is the floor.  If it returns true (it has switched op and prec to more
+// exact products), the solve continues from the current x with a new explicit residual.
 GmresResult gmres_device(ddgrs_ctx* ctx, size_t n, const std::function<void(const double*, double*)>& op,
                          const std::function<void(const double*, double*)>& prec, const double* b, double* x,
-                         const GmresSettings& o, DeviceVecs& vec);
+                         const GmresSettings& o, DeviceVecs& vec,
+                         const std::function<bool()>& on_stall = std::function<bool()>());
 
 }  // namespace ddgrs

@@ -289,13 +289,22 @@ void Session::step(StepReport* rep) {
   GmresSettings gs{opts.restart, opts.tolerance, opts.max_iterations};
   auto A = [&](const double* x, double* y) { op.apply(ctx, x, y); };
   auto P = [&](const double* r, double* z) { pc.apply(ctx, r, z); };
-  GmresResult gr = gmres_device(ctx, n, A, P, g.ptr, sol.ptr, gs, vec);
+  // FP32 products first; the extended near field once their rounding is the residual's floor
+  fm.extended = pc.vf.extended = false;
+  auto stall = [&]() {
+    if (fm.extended) return false;
+    fm.extended = pc.vf.extended = true;
+    return true;
+  };
+  GmresResult gr = gmres_device(ctx, n, A, P, g.ptr, sol.ptr, gs, vec, stall);
+  fm.extended = pc.vf.extended = false;
   rep->seconds_solve = seconds_since(t);
   rep->gmres.iterations = (int64_t)gr.iterations;
   rep->gmres.cycles = (int64_t)gr.cycles;
   rep->gmres.operator_applications = (int64_t)gr.operator_applications;
   rep->gmres.preconditioner_applications = (int64_t)gr.preconditioner_applications;
   rep->gmres.converged = gr.converged ? 1 : 0;
+  rep->gmres.extended = gr.extended ? 1 : 0;
   rep->gmres.relative_residual = gr.relative_residual;
   last_gmres = gr;
   if (!gr.converged && !(gr.relative_residual <= 1e-1)) {

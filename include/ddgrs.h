@@ -168,6 +168,10 @@ typedef struct ddgrs_saddle_options {
 typedef struct ddgrs_gmres_report {
   int64_t iterations, cycles, operator_applications, preconditioner_applications;
   int32_t converged;
+  /* 1 when the solve switched its products to the extended near field (FP64 differences and sums)
+   * because the FP32 products' rounding kept the explicit residual above the tolerance; no
+   * counterpart in the reference, whose products are FP64 throughout. */
+  int32_t extended;
   double relative_residual; /* explicit ‖b − A x‖ / ‖b‖ at the last cycle start */
 } ddgrs_gmres_report;
 

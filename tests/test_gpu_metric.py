@@ -124,3 +124,33 @@ def test_panel_segments_invalid(pkg, ctx, monkeypatch):
     monkeypatch.setenv("DDGRS_SEGMENT_COLUMNS", "100")
     with pytest.raises(Exception, match="multiple of 64"):
         pkg.FaceMetricTree(s)
+
+
+@pytest.mark.parametrize("which", [0, 1])
+def test_extended_products(pkg, ctx, which, monkeypatch):
+    """The extended near field (FP64 differences and sums of the same FP32 values) gives the FP32
+    product to its rounding and is at least as close to the FP64 oracle, for B̃/B̃⁰ and L̃^V; on a
+    smooth input, where the FP32 sums cancel, it is much closer."""
+    name, p, f = list(meshes(pkg))[which]
+    s = pkg.Surface(ctx, f, p)
+    tau = 2.0 - (12.0 - 2.0) / 6.0
+    fm, vf = pkg.FaceMetricTree(s), pkg.VertexFractionalTree(s, tau)
+    om = oracle.FaceMetric(f, p, chi=0.5)
+    ov = oracle.VertexFractional(p, oracle.vertex_areas(f, p), tau, chi=0.5)
+    rng = np.random.default_rng(16)
+    smooth = np.concatenate([p, p * p], axis=1)[:, :3].reshape(-1)  # a smooth field: positions and squares
+    x = np.stack([rng.standard_normal(3 * len(p)), smooth])
+    y32, v32 = fm.apply_batch(x, 1.0, 0.3), vf.apply_batch(x)
+    monkeypatch.setenv("DDGRS_PRODUCTS_EXTENDED", "1")
+    y64, v64 = fm.apply_batch(x, 1.0, 0.3), vf.apply_batch(x)
+    monkeypatch.delenv("DDGRS_PRODUCTS_EXTENDED")
+    y0, v0 = om.apply(x, 1.0, 0.3), ov.apply(x)
+    for k in range(2):
+        e32 = np.linalg.norm(y32[k] - y0[k]) / np.linalg.norm(y0[k])
+        e64 = np.linalg.norm(y64[k] - y0[k]) / np.linalg.norm(y0[k])
+        f32 = np.linalg.norm(v32[k] - v0[k]) / np.linalg.norm(v0[k])
+        f64 = np.linalg.norm(v64[k] - v0[k]) / np.linalg.norm(v0[k])
+        print(f"{name} input {k}: metric FP32 {e32:.2e} extended {e64:.2e}; L^V FP32 {f32:.2e} extended {f64:.2e}")
+        assert e64 <= TOL and f64 <= TOL
+        assert e64 <= 1.01 * e32 + 1e-12 and f64 <= 1.01 * f32 + 1e-12
+        assert np.linalg.norm(y64[k] - y32[k]) / np.linalg.norm(y64[k]) <= 1e-5
