@@ -65,7 +65,7 @@ def lumped_face_node_pairs(tree: dict, lists: dict) -> int:
     return int((counts * sizes).sum())
 
 
-KERNEL_NAMES = {"bt_panel_c12": "bt_panel_kernel<12, 9, 2>", "bh_gradient": "bh_gradient_kernel<0>"}
+KERNEL_NAMES = {"bt_panel_c12": "bt_panel_kernel<12, 9, 2, float>", "bh_gradient": "bh_gradient_kernel<0>"}
 
 
 def _counter_kib(directory: str, counter: str, substr: str):

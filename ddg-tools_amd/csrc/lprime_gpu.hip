@@ -368,7 +368,10 @@ __global__ __launch_bounds__(256) void lf_fill_kernel(size_t n, double* __restri
   for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) x[i] = 0.0;
 }
 
-__global__ void lf_reset_kernel(uint32_t* bad) { *bad = UINT32_MAX; }
+__global__ void lf_reset_kernel(uint32_t* bad, uint32_t* solve_failed) {
+  *bad = UINT32_MAX;
+  *solve_failed = 0u;
+}
 
 // L′'s lower triangle into the fronts: K's entries, then the barycenter row.
 __global__ __launch_bounds__(256) void lf_assemble_kernel(uint32_t nnz, uint32_t nv, const uint64_t* __restrict__ amap_k,
@@ -606,26 +609,54 @@ __global__ __launch_bounds__(256) void lf_inverse_kernel(uint32_t n, uint32_t ma
 }
 
 // ---- solves with the fronts -------------------------------------------------------------------
+//
+// The solve's items hand vectors to one another: update vectors u (forward, child → parent), y_J
+// (forward → backward of the same block), t (backward, between the two phases of a large block)
+// and x (backward, parent → children, which read their later rows).  In the one-launch solve
+// (Coherent) these loads and stores are device-scope atomics, coherent across the XCDs' L2s
+// without fences; the per-height launches use plain ones (the launch boundaries order them).
+template <bool Coherent>
+__device__ __forceinline__ double ld_v(const double* p) {
+  if constexpr (Coherent)
+    return collect(const_cast<double*>(p));
+  else
+    return *p;
+}
+template <bool Coherent>
+__device__ __forceinline__ void st_v(double* p, double v) {
+  if constexpr (Coherent)
+    publish(p, v);
+  else
+    *p = v;
+}
 
-// b (order numbering, 3 per row) from a saddle vector (positions interleaved, multipliers last).
+// b (order numbering, 3 per row) from a saddle vector (positions interleaved, multipliers last);
+// the first `nzero` threads also zero the one-launch solve's counters.
 __global__ __launch_bounds__(256) void lf_gather_rhs_kernel(uint32_t n, uint32_t nv, const uint32_t* __restrict__ perm,
-                                                            const double* __restrict__ r, double* __restrict__ b) {
+                                                            const double* __restrict__ r, double* __restrict__ b,
+                                                            uint32_t nzero, uint32_t* __restrict__ zero) {
   const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+  if (t < nzero) zero[t] = 0u;
   if (t >= 3 * n) return;
   const uint32_t j = t / 3, a = t % 3, o = perm[j];
   b[t] = o < nv ? r[3 * (size_t)o + a] : r[3 * (size_t)nv + a];
 }
 
+// x from xb; NaN everywhere when the one-launch solve gave up waiting (`failed` ≠ 0), so that the
+// caller sees a non-finite result instead of a wrong one.
 __global__ __launch_bounds__(256) void lf_scatter_sol_kernel(uint32_t n, uint32_t nv, const uint32_t* __restrict__ perm,
-                                                             const double* __restrict__ xb, double* __restrict__ x) {
+                                                             const double* __restrict__ xb,
+                                                             const uint32_t* __restrict__ failed,
+                                                             double* __restrict__ x) {
   const uint32_t t = blockIdx.x * 256 + threadIdx.x;
   if (t >= 3 * n) return;
   const uint32_t j = t / 3, a = t % 3, o = perm[j];
-  x[o < nv ? 3 * (size_t)o + a : 3 * (size_t)nv + a] = xb[t];
+  x[o < nv ? 3 * (size_t)o + a : 3 * (size_t)nv + a] = *failed ? __builtin_nan("") : xb[t];
 }
 
 // w = [b_J; 0] (3 per row, f rows) plus the children's update vectors, one child after another
 // (the whole workgroup; ends synchronized).
+template <bool Coherent>
 __device__ void assemble_front_vector(const FrontArgs& A, uint32_t J, const double* __restrict__ b,
                                       const double* __restrict__ ubuf, double* w) {
   const uint32_t f = A.bf[J], k = A.bk[J], j0 = A.bstart[J], tid = threadIdx.x;
@@ -637,23 +668,44 @@ __device__ void assemble_front_vector(const FrontArgs& A, uint32_t J, const doub
     const uint32_t* map = A.ea_map + A.ea_off[ch];
     const double* uc = ubuf + 3 * (size_t)A.ea_off[ch];
     for (uint32_t r = tid; r < cs; r += 256)
-      for (int a = 0; a < 3; ++a) w[3 * map[r] + a] += uc[3 * r + a];
+      for (int a = 0; a < 3; ++a) w[3 * map[r] + a] += ld_v<Coherent>(uc + 3 * r + a);
     __syncthreads();
   }
 }
 
 // ---- small fronts: one workgroup per block, its matrices loaded into LDS first ------------------
 
+// M_J (k × k, contiguous) and L_SJ (rows k.. of the f × f front G, its first k columns) into LDS,
+// both with row stride k + 1 (L_SJ after M_J).  Each thread issues eight loads before it stores any:
+// a load-then-store loop would wait a full memory round trip per element.
+__device__ __forceinline__ void stage_block(const double* __restrict__ M, const double* __restrict__ G, uint32_t k,
+                                            uint32_t f, double* __restrict__ sh) {
+  constexpr int kBatch = 8;
+  const uint32_t nm = k * k, n = nm + (f - k) * k, ks = k + 1;
+  for (uint32_t e0 = threadIdx.x; e0 < n; e0 += kBatch * 256) {
+    double v[kBatch];
+    uint32_t dst[kBatch];
+#pragma unroll
+    for (int j = 0; j < kBatch; ++j) {
+      const uint32_t e = min(e0 + 256u * j, n - 1);  // in bounds; the duplicates are not stored
+      const bool in_m = e < nm;
+      const uint32_t q = in_m ? e : e - nm, r = q / k, l = q - r * k;
+      dst[j] = (in_m ? 0u : k * ks) + r * ks + l;
+      v[j] = in_m ? M[e] : G[(size_t)(k + r) * f + l];
+    }
+#pragma unroll
+    for (int j = 0; j < kBatch; ++j)
+      if (e0 + 256u * j < n) sh[dst[j]] = v[j];
+  }
+}
+
 // Forward: w = [b_J; 0] + children's update vectors, y_J = M_J w_J, u_J = w_S − L_SJ y_J.
 // LDS: M_J (k × (k+1)), L_SJ ((f−k) × (k+1)), w (3 f), y_J (3 k).
-__global__ __launch_bounds__(256) void lf_forward_small_kernel(FrontArgs A, const uint32_t* __restrict__ list,
-                                                               const uint64_t* __restrict__ moff,
-                                                               const double* __restrict__ minv,
-                                                               const double* __restrict__ front,
-                                                               const double* __restrict__ b, double* __restrict__ y,
-                                                               double* __restrict__ ubuf) {
-  extern __shared__ double sh[];
-  const uint32_t J = list[blockIdx.x];
+template <bool Coherent>
+__device__ void forward_small(const FrontArgs& A, uint32_t J, const uint64_t* __restrict__ moff,
+                              const double* __restrict__ minv, const double* __restrict__ front,
+                              const double* __restrict__ b, double* __restrict__ y, double* __restrict__ ubuf,
+                              double* sh) {
   const uint32_t f = A.bf[J], k = A.bk[J], j0 = A.bstart[J], ns = f - k, ks = k + 1, tid = threadIdx.x;
   double* Ms = sh;
   double* Ls = Ms + (size_t)k * ks;
@@ -661,12 +713,8 @@ __global__ __launch_bounds__(256) void lf_forward_small_kernel(FrontArgs A, cons
   double* yj = w + 3 * (size_t)f;
   const double* M = minv + moff[J];
   const double* G = front + A.foff[J];
-  for (uint32_t e = tid; e < k * k; e += 256) Ms[(e / k) * ks + e % k] = M[e];
-  for (uint32_t e = tid; e < ns * k; e += 256) {
-    const uint32_t r = e / k, l = e % k;
-    Ls[r * ks + l] = G[(size_t)(k + r) * f + l];
-  }
-  assemble_front_vector(A, J, b, ubuf, w);
+  stage_block(M, G, k, f, sh);
+  assemble_front_vector<Coherent>(A, J, b, ubuf, w);
   for (uint32_t i = tid; i < k; i += 256) {
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
     for (uint32_t l = 0; l <= i; ++l) {
@@ -678,9 +726,9 @@ __global__ __launch_bounds__(256) void lf_forward_small_kernel(FrontArgs A, cons
     yj[3 * i] = s0;
     yj[3 * i + 1] = s1;
     yj[3 * i + 2] = s2;
-    y[3 * (size_t)(j0 + i)] = s0;
-    y[3 * (size_t)(j0 + i) + 1] = s1;
-    y[3 * (size_t)(j0 + i) + 2] = s2;
+    st_v<Coherent>(y + 3 * (size_t)(j0 + i), s0);
+    st_v<Coherent>(y + 3 * (size_t)(j0 + i) + 1, s1);
+    st_v<Coherent>(y + 3 * (size_t)(j0 + i) + 2, s2);
   }
   __syncthreads();
   double* u = ubuf + 3 * (size_t)A.ea_off[J];
@@ -692,21 +740,28 @@ __global__ __launch_bounds__(256) void lf_forward_small_kernel(FrontArgs A, cons
       s1 = fma(v, yj[3 * l + 1], s1);
       s2 = fma(v, yj[3 * l + 2], s2);
     }
-    u[3 * r] = w[3 * (k + r)] - s0;
-    u[3 * r + 1] = w[3 * (k + r) + 1] - s1;
-    u[3 * r + 2] = w[3 * (k + r) + 2] - s2;
+    st_v<Coherent>(u + 3 * r, w[3 * (k + r)] - s0);
+    st_v<Coherent>(u + 3 * r + 1, w[3 * (k + r) + 1] - s1);
+    st_v<Coherent>(u + 3 * r + 2, w[3 * (k + r) + 2] - s2);
   }
 }
 
-// Backward: t = D_J⁻¹ y_J − L_SJᵀ x_S, x_J = M_Jᵀ t.  LDS: M_J, L_SJ (as forward), x_S (3 (f−k)), t (3 k).
-__global__ __launch_bounds__(256) void lf_backward_small_kernel(FrontArgs A, const uint32_t* __restrict__ list,
-                                                                const uint32_t* __restrict__ srows,
-                                                                const uint64_t* __restrict__ moff,
-                                                                const double* __restrict__ minv,
-                                                                const double* __restrict__ front,
-                                                                const double* __restrict__ y, double* __restrict__ x) {
+__global__ __launch_bounds__(256) void lf_forward_small_kernel(FrontArgs A, const uint32_t* __restrict__ list,
+                                                               const uint64_t* __restrict__ moff,
+                                                               const double* __restrict__ minv,
+                                                               const double* __restrict__ front,
+                                                               const double* __restrict__ b, double* __restrict__ y,
+                                                               double* __restrict__ ubuf) {
   extern __shared__ double sh[];
-  const uint32_t J = list[blockIdx.x];
+  forward_small<false>(A, list[blockIdx.x], moff, minv, front, b, y, ubuf, sh);
+}
+
+// Backward: t = D_J⁻¹ y_J − L_SJᵀ x_S, x_J = M_Jᵀ t.  LDS: M_J, L_SJ (as forward), x_S (3 (f−k)), t (3 k).
+template <bool Coherent>
+__device__ void backward_small(const FrontArgs& A, uint32_t J, const uint32_t* __restrict__ srows,
+                               const uint64_t* __restrict__ moff, const double* __restrict__ minv,
+                               const double* __restrict__ front, const double* __restrict__ y, double* __restrict__ x,
+                               double* sh) {
   const uint32_t f = A.bf[J], k = A.bk[J], j0 = A.bstart[J], ns = f - k, ks = k + 1, tid = threadIdx.x;
   if (k == 0) return;  // uniform per workgroup
   double* Ms = sh;
@@ -716,17 +771,14 @@ __global__ __launch_bounds__(256) void lf_backward_small_kernel(FrontArgs A, con
   const double* M = minv + moff[J];
   const double* G = front + A.foff[J];
   const uint32_t* S = srows + A.ea_off[J];
-  for (uint32_t e = tid; e < k * k; e += 256) Ms[(e / k) * ks + e % k] = M[e];
-  for (uint32_t e = tid; e < ns * k; e += 256) {
-    const uint32_t r = e / k, l = e % k;
-    Ls[r * ks + l] = G[(size_t)(k + r) * f + l];
-  }
+  stage_block(M, G, k, f, sh);
   for (uint32_t r = tid; r < ns; r += 256)
-    for (int a = 0; a < 3; ++a) xs[3 * r + a] = x[3 * (size_t)S[r] + a];
+    for (int a = 0; a < 3; ++a) xs[3 * r + a] = ld_v<Coherent>(x + 3 * (size_t)S[r] + a);
   __syncthreads();
   for (uint32_t i = tid; i < k; i += 256) {
     const double d = G[(size_t)i * f + i];
-    double s0 = y[3 * (size_t)(j0 + i)] / d, s1 = y[3 * (size_t)(j0 + i) + 1] / d, s2 = y[3 * (size_t)(j0 + i) + 2] / d;
+    double s0 = ld_v<Coherent>(y + 3 * (size_t)(j0 + i)) / d, s1 = ld_v<Coherent>(y + 3 * (size_t)(j0 + i) + 1) / d,
+           s2 = ld_v<Coherent>(y + 3 * (size_t)(j0 + i) + 2) / d;
     for (uint32_t r = 0; r < ns; ++r) {
       const double l = Ls[r * ks + i];
       s0 = fma(-l, xs[3 * r], s0);
@@ -746,10 +798,20 @@ __global__ __launch_bounds__(256) void lf_backward_small_kernel(FrontArgs A, con
       s1 = fma(m, t[3 * l + 1], s1);
       s2 = fma(m, t[3 * l + 2], s2);
     }
-    x[3 * (size_t)(j0 + i)] = s0;
-    x[3 * (size_t)(j0 + i) + 1] = s1;
-    x[3 * (size_t)(j0 + i) + 2] = s2;
+    st_v<Coherent>(x + 3 * (size_t)(j0 + i), s0);
+    st_v<Coherent>(x + 3 * (size_t)(j0 + i) + 1, s1);
+    st_v<Coherent>(x + 3 * (size_t)(j0 + i) + 2, s2);
   }
+}
+
+__global__ __launch_bounds__(256) void lf_backward_small_kernel(FrontArgs A, const uint32_t* __restrict__ list,
+                                                                const uint32_t* __restrict__ srows,
+                                                                const uint64_t* __restrict__ moff,
+                                                                const double* __restrict__ minv,
+                                                                const double* __restrict__ front,
+                                                                const double* __restrict__ y, double* __restrict__ x) {
+  extern __shared__ double sh[];
+  backward_small<false>(A, list[blockIdx.x], srows, moff, minv, front, y, x, sh);
 }
 
 // ---- large fronts: rows spread over many workgroups, four rows per wave -----------------------
@@ -784,16 +846,13 @@ __device__ __forceinline__ void wave_rows4(const double* const row[4], const uin
     for (int a = 0; a < 3; ++a) out[q][a] = wsum64(acc[q][a]);
 }
 
-// Forward y_J = M_J w_J, rows [r0, r0 + 16) (w assembled by each workgroup).
-__global__ __launch_bounds__(256) void lf_forward_y_kernel(FrontArgs A, const SolveItem* __restrict__ items,
-                                                           const uint64_t* __restrict__ moff,
-                                                           const double* __restrict__ minv,
-                                                           const double* __restrict__ b, double* __restrict__ y,
-                                                           const double* __restrict__ ubuf) {
-  extern __shared__ double w[];
-  const SolveItem it = items[blockIdx.x];
+// Forward y_J = M_J w_J, rows [r0, r0 + 16) (w assembled by each workgroup).  LDS: w (3 f).
+template <bool Coherent>
+__device__ void forward_y(const FrontArgs& A, SolveItem it, const uint64_t* __restrict__ moff,
+                          const double* __restrict__ minv, const double* __restrict__ b, double* __restrict__ y,
+                          const double* __restrict__ ubuf, double* w) {
   const uint32_t J = it.block, k = A.bk[J], j0 = A.bstart[J], lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  assemble_front_vector(A, J, b, ubuf, w);
+  assemble_front_vector<Coherent>(A, J, b, ubuf, w);
   const double* M = minv + moff[J];
   const double* row[4];
   uint32_t lo[4], hi[4], ri[4];
@@ -809,21 +868,28 @@ __global__ __launch_bounds__(256) void lf_forward_y_kernel(FrontArgs A, const So
   if (lane == 0)
     for (int q = 0; q < 4; ++q)
       if (ri[q] < k)
-        for (int a = 0; a < 3; ++a) y[3 * (size_t)(j0 + ri[q]) + a] = out[q][a];
+        for (int a = 0; a < 3; ++a) st_v<Coherent>(y + 3 * (size_t)(j0 + ri[q]) + a, out[q][a]);
 }
 
-// Forward u_J = w_S − L_SJ y_J, later rows [r0, r0 + 16).
-__global__ __launch_bounds__(256) void lf_forward_u_kernel(FrontArgs A, const SolveItem* __restrict__ items,
-                                                           const double* __restrict__ front,
-                                                           const double* __restrict__ b, const double* __restrict__ y,
-                                                           double* __restrict__ ubuf) {
-  extern __shared__ double w[];  // 3 f, then y_J: 3 k
-  const SolveItem it = items[blockIdx.x];
+__global__ __launch_bounds__(256) void lf_forward_y_kernel(FrontArgs A, const SolveItem* __restrict__ items,
+                                                           const uint64_t* __restrict__ moff,
+                                                           const double* __restrict__ minv,
+                                                           const double* __restrict__ b, double* __restrict__ y,
+                                                           const double* __restrict__ ubuf) {
+  extern __shared__ double w[];
+  forward_y<false>(A, items[blockIdx.x], moff, minv, b, y, ubuf, w);
+}
+
+// Forward u_J = w_S − L_SJ y_J, later rows [r0, r0 + 16).  LDS: w (3 f), then y_J (3 k).
+template <bool Coherent>
+__device__ void forward_u(const FrontArgs& A, SolveItem it, const double* __restrict__ front,
+                          const double* __restrict__ b, const double* __restrict__ y, double* __restrict__ ubuf,
+                          double* w) {
   const uint32_t J = it.block, f = A.bf[J], k = A.bk[J], j0 = A.bstart[J], ns = f - k;
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   double* yj = w + 3 * (size_t)f;
-  for (uint32_t i = threadIdx.x; i < 3 * k; i += 256) yj[i] = y[3 * (size_t)j0 + i];
-  assemble_front_vector(A, J, b, ubuf, w);
+  for (uint32_t i = threadIdx.x; i < 3 * k; i += 256) yj[i] = ld_v<Coherent>(y + 3 * (size_t)j0 + i);
+  assemble_front_vector<Coherent>(A, J, b, ubuf, w);
   const double* G = front + A.foff[J];
   const double* row[4];
   uint32_t lo[4], hi[4], ri[4];
@@ -840,24 +906,29 @@ __global__ __launch_bounds__(256) void lf_forward_u_kernel(FrontArgs A, const So
   if (lane == 0)
     for (int q = 0; q < 4; ++q)
       if (ri[q] < ns)
-        for (int a = 0; a < 3; ++a) u[3 * ri[q] + a] = w[3 * (k + ri[q]) + a] - out[q][a];
+        for (int a = 0; a < 3; ++a) st_v<Coherent>(u + 3 * ri[q] + a, w[3 * (k + ri[q]) + a] - out[q][a]);
+}
+
+__global__ __launch_bounds__(256) void lf_forward_u_kernel(FrontArgs A, const SolveItem* __restrict__ items,
+                                                           const double* __restrict__ front,
+                                                           const double* __restrict__ b, const double* __restrict__ y,
+                                                           double* __restrict__ ubuf) {
+  extern __shared__ double w[];
+  forward_u<false>(A, items[blockIdx.x], front, b, y, ubuf, w);
 }
 
 // Backward t = D_J⁻¹ y_J − L_SJᵀ x_S, rows [r0, r0 + 16) (rows of the transposed L_SJ).
-__global__ __launch_bounds__(256) void lf_backward_t_kernel(FrontArgs A, const SolveItem* __restrict__ items,
-                                                            const uint32_t* __restrict__ srows,
-                                                            const uint64_t* __restrict__ ltoff,
-                                                            const double* __restrict__ lt,
-                                                            const double* __restrict__ front,
-                                                            const double* __restrict__ y, const double* __restrict__ x,
-                                                            double* __restrict__ tbuf) {
-  extern __shared__ double xs[];  // 3 (f − k)
-  const SolveItem it = items[blockIdx.x];
+// LDS: x_S (3 (f − k)).
+template <bool Coherent>
+__device__ void backward_t(const FrontArgs& A, SolveItem it, const uint32_t* __restrict__ srows,
+                           const uint64_t* __restrict__ ltoff, const double* __restrict__ lt,
+                           const double* __restrict__ front, const double* __restrict__ y,
+                           const double* __restrict__ x, double* __restrict__ tbuf, double* xs) {
   const uint32_t J = it.block, f = A.bf[J], k = A.bk[J], j0 = A.bstart[J], ns = f - k;
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t* S = srows + A.ea_off[J];
   for (uint32_t r = threadIdx.x; r < ns; r += 256)
-    for (int a = 0; a < 3; ++a) xs[3 * r + a] = x[3 * (size_t)S[r] + a];
+    for (int a = 0; a < 3; ++a) xs[3 * r + a] = ld_v<Coherent>(x + 3 * (size_t)S[r] + a);
   __syncthreads();
   const double* T = lt + ltoff[J];
   const double* row[4];
@@ -877,19 +948,28 @@ __global__ __launch_bounds__(256) void lf_backward_t_kernel(FrontArgs A, const S
       if (ri[q] < k) {
         const size_t i = j0 + ri[q];
         const double d = G[(size_t)ri[q] * f + ri[q]];
-        for (int a = 0; a < 3; ++a) tbuf[3 * i + a] = y[3 * i + a] / d - out[q][a];
+        for (int a = 0; a < 3; ++a) st_v<Coherent>(tbuf + 3 * i + a, ld_v<Coherent>(y + 3 * i + a) / d - out[q][a]);
       }
 }
 
-// Backward x_J = M_Jᵀ t, rows [r0, r0 + 16) (rows of M_Jᵀ).
-__global__ __launch_bounds__(256) void lf_backward_x_kernel(FrontArgs A, const SolveItem* __restrict__ items,
-                                                            const uint64_t* __restrict__ moff,
-                                                            const double* __restrict__ minv_t,
-                                                            const double* __restrict__ tbuf, double* __restrict__ x) {
-  extern __shared__ double t[];  // 3 k
-  const SolveItem it = items[blockIdx.x];
+__global__ __launch_bounds__(256) void lf_backward_t_kernel(FrontArgs A, const SolveItem* __restrict__ items,
+                                                            const uint32_t* __restrict__ srows,
+                                                            const uint64_t* __restrict__ ltoff,
+                                                            const double* __restrict__ lt,
+                                                            const double* __restrict__ front,
+                                                            const double* __restrict__ y, const double* __restrict__ x,
+                                                            double* __restrict__ tbuf) {
+  extern __shared__ double xs[];
+  backward_t<false>(A, items[blockIdx.x], srows, ltoff, lt, front, y, x, tbuf, xs);
+}
+
+// Backward x_J = M_Jᵀ t, rows [r0, r0 + 16) (rows of M_Jᵀ).  LDS: t (3 k).
+template <bool Coherent>
+__device__ void backward_x(const FrontArgs& A, SolveItem it, const uint64_t* __restrict__ moff,
+                           const double* __restrict__ minv_t, const double* __restrict__ tbuf, double* __restrict__ x,
+                           double* t) {
   const uint32_t J = it.block, k = A.bk[J], j0 = A.bstart[J], lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (uint32_t i = threadIdx.x; i < 3 * k; i += 256) t[i] = tbuf[3 * (size_t)j0 + i];
+  for (uint32_t i = threadIdx.x; i < 3 * k; i += 256) t[i] = ld_v<Coherent>(tbuf + 3 * (size_t)j0 + i);
   __syncthreads();
   const double* Mt = minv_t + moff[J];
   const double* row[4];
@@ -906,7 +986,147 @@ __global__ __launch_bounds__(256) void lf_backward_x_kernel(FrontArgs A, const S
   if (lane == 0)
     for (int q = 0; q < 4; ++q)
       if (ri[q] < k)
-        for (int a = 0; a < 3; ++a) x[3 * (size_t)(j0 + ri[q]) + a] = out[q][a];
+        for (int a = 0; a < 3; ++a) st_v<Coherent>(x + 3 * (size_t)(j0 + ri[q]) + a, out[q][a]);
+}
+
+__global__ __launch_bounds__(256) void lf_backward_x_kernel(FrontArgs A, const SolveItem* __restrict__ items,
+                                                            const uint64_t* __restrict__ moff,
+                                                            const double* __restrict__ minv_t,
+                                                            const double* __restrict__ tbuf, double* __restrict__ x) {
+  extern __shared__ double t[];
+  backward_x<false>(A, items[blockIdx.x], moff, minv_t, tbuf, x, t);
+}
+
+// ---- the whole solve in one launch ----------------------------------------------------------------
+//
+// Workgroups take tasks in ticket order (an atomic counter): the forward tasks by height, then the
+// backward tasks from the root down; within a large block its y (t) tasks precede its u (x) tasks.
+// A task waits only for tasks of smaller tickets:
+//   forward small / y of J  — every forward task of J's children (their update vectors);
+//   forward u of J          — the y tasks of J;
+//   backward small / t of J — every forward task of J (y_J) and every backward task of J's parent
+//                             (x of J's later rows: the parent's rows and, through the parent's own
+//                             wait, those of every ancestor);
+//   backward x of J         — the t tasks of J.
+// A workgroup only waits after it has taken a ticket, so the smallest unfinished ticket always
+// belongs to a running workgroup whose dependencies are finished: the launch completes whatever
+// the number of resident workgroups.  A task's results are published (device-scope stores, then
+// the stores' completion, then a workgroup barrier) before its block's counter is incremented.
+// Counters are read with a read-modify-write atomic (a compare-and-swap that never matches), the
+// kind of operation that increments them, never by a repeated load: the hand-offs elsewhere in this
+// library read each published line once, and a load repeated on one line may be served by a copy
+// in the XCD's own L2.  The wait gives up after about 0.2 s of the device's constant-rate clock or
+// 65536 polls, whichever comes first (a broken invariant, not a slow solve: the whole solve takes
+// well under a millisecond), marks the solve failed and ends the workgroup; the scatter then writes
+// NaN.
+
+constexpr uint64_t kSolveWaitTicks = 20000000;  // s_memrealtime runs at 100 MHz
+constexpr uint32_t kSolveWaitPolls = 65536;
+
+__device__ __forceinline__ uint32_t read_count(uint32_t* c) { return atomicCAS(c, 0xffffffffu, 0xffffffffu); }
+
+__device__ bool wait_count(uint32_t* c, uint32_t need, uint32_t* failed) {
+  const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+  for (uint32_t spin = 0; read_count(c) < need; ++spin) {
+    if (spin >= kSolveWaitPolls ||
+        ((spin & 63u) == 63u &&
+         (read_count(failed) != 0u || __builtin_amdgcn_s_memrealtime() - t0 > kSolveWaitTicks))) {
+      atomicOr(failed, 1u);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(2);
+  }
+  return true;
+}
+
+struct OneArgs {
+  const SolveTask* tasks;
+  const TaskCounts* counts;
+  uint32_t ntasks;
+  uint32_t *fdone, *bdone, *ticket, *failed;
+};
+
+__global__ __launch_bounds__(256) void lf_solve_one_kernel(FrontArgs A, OneArgs O, const uint32_t* __restrict__ srows,
+                                                           const uint64_t* __restrict__ moff,
+                                                           const double* __restrict__ minv,
+                                                           const double* __restrict__ minv_t,
+                                                           const double* __restrict__ front,
+                                                           const uint64_t* __restrict__ ltoff,
+                                                           const double* __restrict__ lt, double* __restrict__ xb,
+                                                           double* __restrict__ yb, double* __restrict__ ubuf,
+                                                           double* __restrict__ tb) {
+  extern __shared__ double sh[];
+  __shared__ uint32_t s_task, s_ok;
+  // Thread 0's work between two tasks is one region at the top of the loop: the previous task's
+  // completion, then the next ticket.  (With the completion at the bottom of the loop and the ticket
+  // at the top, the compiler merged the two thread-0 regions across the back edge into a loop exit
+  // of thread 0 alone; the other lanes of its wave then reached the next barrier first and the
+  // workgroup re-read the old ticket forever.)  The ticket and the wait's outcome are read as
+  // wave-uniform values, so that no loop exit looks divergent.
+  uint32_t* done = nullptr;  // the counter of the task just finished
+  for (;;) {
+    if (threadIdx.x == 0) {
+      if (done) atomicAdd(done, 1u);
+      s_task = atomicAdd(O.ticket, 1u);
+    }
+    __syncthreads();
+    const uint32_t tk = __builtin_amdgcn_readfirstlane(s_task);
+    if (tk >= O.ntasks) return;
+    const SolveTask task = O.tasks[tk];
+    const uint32_t J = task.block;
+    if (threadIdx.x == 0) {
+      bool ok = true;
+      switch (task.kind) {
+        case kTaskFwdSmall:
+        case kTaskFwdY:
+          for (uint32_t q = A.child_ptr[J]; ok && q < A.child_ptr[J + 1]; ++q) {
+            const uint32_t c = A.child[q];
+            ok = wait_count(O.fdone + c, O.counts[c].nf, O.failed);
+          }
+          break;
+        case kTaskFwdU:
+          ok = wait_count(O.fdone + J, O.counts[J].ny, O.failed);
+          break;
+        case kTaskBwdSmall:
+        case kTaskBwdT: {
+          ok = wait_count(O.fdone + J, O.counts[J].nf, O.failed);
+          const uint32_t P = A.bparent[J];
+          if (ok && P != kNone) ok = wait_count(O.bdone + P, O.counts[P].nb, O.failed);
+          break;
+        }
+        default:
+          ok = wait_count(O.bdone + J, O.counts[J].nt, O.failed);
+          break;
+      }
+      s_ok = ok ? 1u : 0u;
+    }
+    __syncthreads();
+    if (!__builtin_amdgcn_readfirstlane(s_ok)) return;
+    const SolveItem it{J, task.r0};
+    switch (task.kind) {
+      case kTaskFwdSmall:
+        forward_small<true>(A, J, moff, minv, front, xb, yb, ubuf, sh);
+        break;
+      case kTaskFwdY:
+        forward_y<true>(A, it, moff, minv, xb, yb, ubuf, sh);
+        break;
+      case kTaskFwdU:
+        forward_u<true>(A, it, front, xb, yb, ubuf, sh);
+        break;
+      case kTaskBwdSmall:
+        backward_small<true>(A, J, srows, moff, minv, front, yb, xb, sh);
+        break;
+      case kTaskBwdT:
+        backward_t<true>(A, it, srows, ltoff, lt, front, yb, xb, tb, sh);
+        break;
+      default:
+        backward_x<true>(A, it, moff, minv_t, tb, xb, sh);
+        break;
+    }
+    published();  // this thread's results have reached the coherent level
+    __syncthreads();
+    done = (task.kind < kTaskBwdSmall ? O.fdone : O.bdone) + J;
+  }
 }
 
 // L_SJᵀ of the large fronts (k × (f − k), row-major), 32 × 32 tiles through LDS.
@@ -934,6 +1154,47 @@ __global__ __launch_bounds__(256) void lf_transpose_kernel(FrontArgs A, const Fr
 
 LprimeFactorDev::~LprimeFactorDev() {
   if (bad_host) (void)hipHostFree(bad_host);
+}
+
+void solve_tasks(const LprimeSymbolic& S, std::vector<SolveTask>& tl, std::vector<TaskCounts>& tc) {
+  tl.clear();
+  tc.assign(S.nb, TaskCounts{0, 0, 0, 0});
+  for (uint32_t h = 0; h < S.H; ++h) {
+    if (!S.hsplit[h]) {
+      for (uint32_t q = S.hoff[h]; q < S.hoff[h + 1]; ++q) {
+        tl.push_back({kTaskFwdSmall, S.hblocks[q], 0});
+        tc[S.hblocks[q]].nf++;
+      }
+      continue;
+    }
+    for (uint32_t q = S.krows_off[h]; q < S.krows_off[h + 1]; ++q) {
+      tl.push_back({kTaskFwdY, S.krows[q].block, S.krows[q].r0});
+      tc[S.krows[q].block].nf++;
+      tc[S.krows[q].block].ny++;
+    }
+    for (uint32_t q = S.urows_off[h]; q < S.urows_off[h + 1]; ++q) {
+      tl.push_back({kTaskFwdU, S.urows[q].block, S.urows[q].r0});
+      tc[S.urows[q].block].nf++;
+    }
+  }
+  for (uint32_t h = S.H; h-- > 0;) {
+    if (!S.hsplit[h]) {
+      for (uint32_t q = S.hoff[h]; q < S.hoff[h + 1]; ++q) {
+        tl.push_back({kTaskBwdSmall, S.hblocks[q], 0});
+        tc[S.hblocks[q]].nb++;
+      }
+      continue;
+    }
+    for (uint32_t q = S.krows_off[h]; q < S.krows_off[h + 1]; ++q) {
+      tl.push_back({kTaskBwdT, S.krows[q].block, S.krows[q].r0});
+      tc[S.krows[q].block].nb++;
+      tc[S.krows[q].block].nt++;
+    }
+    for (uint32_t q = S.krows_off[h]; q < S.krows_off[h + 1]; ++q) {
+      tl.push_back({kTaskBwdX, S.krows[q].block, S.krows[q].r0});
+      tc[S.krows[q].block].nb++;
+    }
+  }
 }
 
 void LprimeFactorDev::setup(ddgrs_ctx* ctx, const LprimeSymbolic& S) {
@@ -1001,6 +1262,29 @@ void LprimeFactorDev::setup(ddgrs_ctx* ctx, const LprimeSymbolic& S) {
   bad.reserve(1);
   if (!bad_host) HIPCHECK(hipHostMalloc((void**)&bad_host, sizeof(uint32_t), hipHostMallocDefault));
   *bad_host = UINT32_MAX;
+  nb = S.nb;
+  std::vector<SolveTask> tl;
+  std::vector<TaskCounts> tc;
+  solve_tasks(S, tl, tc);
+  size_t small_lds = 0;
+  bool any_split = false;
+  for (uint32_t h = 0; h < H; ++h) {
+    if (hsplit[h])
+      any_split = true;
+    else
+      small_lds = std::max<size_t>(small_lds, hlds[h]);
+  }
+  ntasks = (uint32_t)tl.size();
+  ddgrs::upload(ctx, tasks, tl);
+  ddgrs::upload(ctx, task_counts, tc);
+  task_done.reserve(2 * (size_t)nb + 1);
+  solve_failed.reserve(1);
+  launch(ctx, "lf_reset", lf_reset_kernel, dim3(1), dim3(1), 0, bad.ptr, solve_failed.ptr);
+  one_lds = std::max(small_lds, any_split ? (3 * (size_t)max_front + 3 * (size_t)max_rows) * sizeof(double) : 0);
+  one_ok = one_lds + 64 <= (size_t)ctx->prop.sharedMemPerBlock;  // and its two static words
+  if (one_ok && one_lds > 0)
+    HIPCHECK(hipFuncSetAttribute((const void*)lf_solve_one_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)one_lds));
   size_t lds_max = 0;
   for (uint32_t r : lds_rows) lds_max = std::max(lds_max, ((size_t)r * (r + 1) / 2 + r) * sizeof(double));
   if (lds_max > 0)
@@ -1027,7 +1311,7 @@ void LprimeFactorDev::factor(ddgrs_ctx* ctx, ddgrs_surface* s, double eps_rel) {
   // fronts
   launch(ctx, "lf_fill", lf_fill_kernel, dim3((unsigned)std::min<uint64_t>(blocks_for((int64_t)front_total, 256), 4096)),
          dim3(256), 0, (size_t)front_total, front.ptr);
-  launch(ctx, "lf_reset", lf_reset_kernel, dim3(1), dim3(1), 0, bad.ptr);
+  launch(ctx, "lf_reset", lf_reset_kernel, dim3(1), dim3(1), 0, bad.ptr, solve_failed.ptr);
   launch(ctx, "lf_assemble", lf_assemble_kernel, dim3(blocks_for(std::max<size_t>(nnz_k, nv), 256)), dim3(256), 0,
          (uint32_t)nnz_k, unv, (const uint64_t*)amap_k.ptr, (const double*)kx.ptr, (const uint64_t*)amap_c.ptr,
          (const double*)crow.ptr, front.ptr);
@@ -1081,8 +1365,23 @@ const char* height_name(const char* base, uint32_t h) {
 void LprimeFactorDev::solve(ddgrs_ctx* ctx, const double* r, double* x) {
   const uint32_t un = (uint32_t)n, unv = (uint32_t)nv;
   const FrontArgs A{bstart.ptr, bk.ptr, bf.ptr, bparent.ptr, child_ptr.ptr, child.ptr, ea_off.ptr, ea_map.ptr, foff.ptr};
-  launch(ctx, "lf_gather_rhs", lf_gather_rhs_kernel, dim3(blocks_for(3 * (int64_t)n, 256)), dim3(256), 0, un, unv,
-         (const uint32_t*)perm.ptr, r, xb.ptr);
+  const char* e = std::getenv("DDGRS_LPRIME_SOLVE_ONE_LAUNCH");
+  const bool one = one_ok && e && e[0] == '1';
+  const uint32_t nzero = one ? 2 * nb + 1 : 0;
+  launch(ctx, "lf_gather_rhs", lf_gather_rhs_kernel, dim3(blocks_for(std::max<int64_t>(3 * (int64_t)n, nzero), 256)),
+         dim3(256), 0, un, unv, (const uint32_t*)perm.ptr, r, xb.ptr, nzero, task_done.ptr);
+  if (one) {
+    const OneArgs O{tasks.ptr, task_counts.ptr, ntasks, task_done.ptr, task_done.ptr + nb, task_done.ptr + 2 * nb,
+                    solve_failed.ptr};
+    const uint32_t grid = std::min<uint32_t>(ntasks, 2 * (uint32_t)ctx->prop.multiProcessorCount);
+    if (grid > 0)
+      launch(ctx, "lf_solve_one", lf_solve_one_kernel, dim3(grid), dim3(256), one_lds, A, O, (const uint32_t*)srows.ptr,
+             (const uint64_t*)moff.ptr, (const double*)minv.ptr, (const double*)minv_t.ptr, (const double*)front.ptr,
+             (const uint64_t*)ltoff.ptr, (const double*)lt.ptr, xb.ptr, yb.ptr, ubuf.ptr, tb.ptr);
+    launch(ctx, "lf_scatter_sol", lf_scatter_sol_kernel, dim3(blocks_for(3 * (int64_t)n, 256)), dim3(256), 0, un, unv,
+           (const uint32_t*)perm.ptr, (const double*)xb.ptr, (const uint32_t*)solve_failed.ptr, x);
+    return;
+  }
   const size_t lds = (3 * (size_t)max_front + 3 * (size_t)max_rows) * sizeof(double);
   for (uint32_t h = 0; h < H; ++h) {
     if (!hsplit[h]) {
@@ -1113,7 +1412,7 @@ void LprimeFactorDev::solve(ddgrs_ctx* ctx, const double* r, double* x) {
     }
   }
   launch(ctx, "lf_scatter_sol", lf_scatter_sol_kernel, dim3(blocks_for(3 * (int64_t)n, 256)), dim3(256), 0, un, unv,
-         (const uint32_t*)perm.ptr, (const double*)xb.ptr, x);
+         (const uint32_t*)perm.ptr, (const double*)xb.ptr, (const uint32_t*)solve_failed.ptr, x);
 }
 
 void LprimeFactorDev::check(ddgrs_ctx* ctx) {

@@ -33,6 +33,15 @@ constexpr size_t kSolveSmallLds = 64 * 1024;
 struct SolveItem {  // rows [r0, r0 + kSolveRows) of a block (or of its later front rows)
   uint32_t block, r0;
 };
+// Tasks of the one-launch solve: a small block's forward or backward, or one SolveItem of a large
+// block's forward y_J / u_J or backward t / x_J phase.
+enum SolveTaskKind : uint32_t { kTaskFwdSmall, kTaskFwdY, kTaskFwdU, kTaskBwdSmall, kTaskBwdT, kTaskBwdX };
+struct SolveTask {
+  uint32_t kind, block, r0;
+};
+struct TaskCounts {  // per block: forward tasks (of them y tasks), backward tasks (of them t tasks)
+  uint32_t nf, ny, nb, nt;
+};
 
 struct LprimeSymbolic {
   size_t nv = 0, n = 0;
@@ -76,6 +85,10 @@ struct LprimeSymbolic {
 LprimeSymbolic lprime_symbolic(const std::vector<uint32_t>& faces, size_t nv, const std::vector<double>& pos,
                                size_t lds_bytes);
 
+// The one-launch solve's tasks in ticket order (forward by height, a large block's y tasks before
+// its u tasks; backward from the root down, t tasks before x tasks) and the per-block task counts.
+void solve_tasks(const LprimeSymbolic& S, std::vector<SolveTask>& tasks, std::vector<TaskCounts>& counts);
+
 struct LprimeFactorDev {
   size_t nv = 0, n = 0;
   uint32_t H = 0, max_children = 0, max_rows = 0, max_front = 0;
@@ -98,6 +111,14 @@ struct LprimeFactorDev {
   DevBuf<double> front, kx, mass, crow, diagv, sums;  // sums: tr W, Σ a, ε
   DevBuf<uint32_t> bad;
   uint32_t* bad_host = nullptr;  // pinned: the first failing pivot of the last factorization
+  // one-launch solve: tasks in ticket order, per-block task counts, counters (forward and backward
+  // tasks done per block, then the ticket), the failure flag
+  DevBuf<SolveTask> tasks;
+  DevBuf<TaskCounts> task_counts;
+  DevBuf<uint32_t> task_done, solve_failed;
+  uint32_t nb = 0, ntasks = 0;
+  size_t one_lds = 0;   // its LDS per workgroup
+  bool one_ok = false;  // the one-launch solve fits the device (else the per-height launches)
   size_t nnz_k = 0;
   uint32_t n_lt_tiles = 0;
   LprimeFactorDev() = default;
@@ -113,7 +134,10 @@ struct LprimeFactorDev {
   // x = L′⁻¹ r for a full saddle vector (3 nv + 3, interleaved positions then multipliers), the
   // three axes together: forward by height (each front's right-hand side from its block and its
   // children's update vectors, its block solved by the dense inverse, its update vector formed),
-  // D⁻¹, backward by height (each block from its later front rows).
+  // D⁻¹, backward by height (each block from its later front rows), one launch per height and
+  // phase.  DDGRS_LPRIME_SOLVE_ONE_LAUNCH=1 asks for a single launch whose workgroups take the
+  // blocks' tasks in dependency order (the same arithmetic per task; not yet validated on the
+  // device, so off by default).
   void solve(ddgrs_ctx* ctx, const double* r, double* x);
   // After a synchronization: throws SingularSaddle if a pivot of the last factorization was zero
   // or not finite.

@@ -103,6 +103,35 @@ def test_lprime_solve_paths_agree(pkg, ctx, monkeypatch):
     assert rel(xl, xs) <= 1e-9
 
 
+@pytest.mark.skip(reason="the one-launch L' solve hung on the device: the compiler merged thread 0's task "
+                  "completion and next ticket across the loop's back edge, so the workgroup re-read the old "
+                  "ticket; the restructured loop is not yet run on the device (off by default)")
+@pytest.mark.parametrize("name,large", [("torus_16k", False), ("torus_16k", True), ("bumpy_l6", False)])
+def test_lprime_one_launch_matches_per_height(pkg, ctx, name, large, monkeypatch):
+    """The one-launch L′ solve (workgroups take the blocks' tasks in ticket order and wait for the
+    tasks they depend on) against one launch per height and phase: each task does the same
+    arithmetic, so the solutions are bitwise equal.  Three right-hand sides in a row exercise the
+    counters' reset between solves; ``large`` puts every block on the row-split path (y/u and t/x
+    tasks)."""
+    gen = {"bumpy_l6": pkg.generate.config_bumpy_l6, **meshes(pkg)}[name]
+    p, f = gen()
+    s = pkg.Surface(ctx, f, p)
+    if large:
+        monkeypatch.setenv("DDGRS_LPRIME_LARGE_PATH", "1")
+    sd = pkg.Saddle(s)
+    monkeypatch.delenv("DDGRS_LPRIME_LARGE_PATH", raising=False)
+    rng = np.random.default_rng(25)
+    for i in range(3):
+        r = rng.standard_normal(sd.n)
+        monkeypatch.setenv("DDGRS_LPRIME_SOLVE_ONE_LAUNCH", "1")
+        x_one = sd.lprime_solve(r)
+        monkeypatch.delenv("DDGRS_LPRIME_SOLVE_ONE_LAUNCH")
+        x_per = sd.lprime_solve(r)
+        print(f"{name} large={large} rhs {i}: max |one - per| {np.abs(x_one - x_per).max():.3e}")
+        assert np.all(np.isfinite(x_one))
+        assert np.array_equal(x_one, x_per)
+
+
 def test_lean_saddle_octa(pkg, ctx):
     c, o = fx.read_case("rs-octa-saddle-none"), fx.read_out("rs-octa-saddle-none")
     p, f = np.array(c["positions"]), np.array(c["faces"])
