@@ -674,7 +674,18 @@ void BhEval::build_from(ddgrs_surface* s, const Topology& t, const Clusters& c, 
   topo = t;
   clusters = c;
   ddgrs_ctx* ctx = s->ctx;
+  // DDGRS_TRACE_STEP=1: wall times of the build's parts on stderr (each part synchronized)
+  static const bool trace = std::getenv("DDGRS_TRACE_STEP") != nullptr;
+  auto t_part = std::chrono::steady_clock::now();
+  auto mark = [&](const char* what) {
+    if (!trace) return;
+    ctx->sync();
+    const auto now = std::chrono::steady_clock::now();
+    std::fprintf(stderr, "bh: %-18s %8.2f ms\n", what, std::chrono::duration<double>(now - t_part).count() * 1e3);
+    t_part = now;
+  };
   upload_structure(ctx);
+  mark("structure");
   const char* host = std::getenv("DDGRS_BH_LISTS_ON_HOST");
   if (host && host[0] == '1') {
     lists = build_bh_lists(topo, clusters, theta);
@@ -693,11 +704,15 @@ void BhEval::build_from(ddgrs_surface* s, const Topology& t, const Clusters& c, 
     lists = BhLists();
     build_lists_device(ctx);
   }
+  mark("lists");
   flatten_lists(ctx);
+  mark("flatten");
   build_items(ctx);
+  mark("items");
   node_bu.reserve(std::max<size_t>(1, n_lumped));
   refresh(s);
-  if (std::getenv("DDGRS_TRACE_STEP")) trace_lists(ctx);
+  mark("node data");
+  if (trace) trace_lists(ctx);
 }
 
 // Per-leaf list lengths on stderr (lumped nodes, exact faces, partner faces): mean, p99, largest,

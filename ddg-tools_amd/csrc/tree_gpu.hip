@@ -1,5 +1,9 @@
 // BVH construction on the device (see tree_gpu.hpp).
 #include <cstring>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
@@ -121,18 +125,20 @@ __global__ __launch_bounds__(256) void bvh_scatter_kernel(uint32_t total, const 
   order[act_start[a] + (i - act_prefix[a])] = val[i];
 }
 
-// One workgroup per splitting node: suffix boxes, then prefix boxes with the cost of every
-// admissible split, reduced to the first minimum.
-__global__ __launch_bounds__(256) void bvh_split_kernel(const uint32_t* __restrict__ act_start,
-                                                        const uint32_t* __restrict__ act_end,
-                                                        const uint32_t* __restrict__ order, const double* __restrict__ lo,
-                                                        const double* __restrict__ hi, double* __restrict__ suf,
-                                                        uint32_t* __restrict__ split) {
+// Level records of the device-driven build (BvhLevel: a level's nodes are records
+// [off, off + na), `total` elements, `nch` split chunks).
+struct BvhLevel {
+  uint32_t off, na, total, nch;
+};
+
+// The split of one node [start, start + cnt) by one workgroup: suffix boxes, then prefix boxes
+// with the cost of every admissible split, reduced to the first minimum (thread 0's value).
+__device__ uint32_t split_node(uint32_t start, uint32_t cnt, const uint32_t* __restrict__ order,
+                               const double* __restrict__ lo, const double* __restrict__ hi, double* __restrict__ suf) {
   __shared__ Box6 wave_tot[4];
   __shared__ double red_c[256];
   __shared__ uint32_t red_k[256];
-  const uint32_t a = blockIdx.x, t = threadIdx.x;
-  const uint32_t start = act_start[a], cnt = act_end[a] - start;
+  const uint32_t t = threadIdx.x;
   const uint32_t* sl = order + start;
   double* sf = suf + 6 * (size_t)start;
   size_t low = (size_t)ceil(0.25 * (double)cnt);
@@ -195,7 +201,18 @@ __global__ __launch_bounds__(256) void bvh_split_kernel(const uint32_t* __restri
     }
     __syncthreads();
   }
-  if (t == 0) split[a] = red_k[0] == UINT32_MAX ? cnt / 2 : red_k[0];
+  return red_k[0] == UINT32_MAX ? cnt / 2 : red_k[0];
+}
+
+// One workgroup per splitting node.
+__global__ __launch_bounds__(256) void bvh_split_kernel(const uint32_t* __restrict__ act_start,
+                                                        const uint32_t* __restrict__ act_end,
+                                                        const uint32_t* __restrict__ order, const double* __restrict__ lo,
+                                                        const double* __restrict__ hi, double* __restrict__ suf,
+                                                        uint32_t* __restrict__ split) {
+  const uint32_t a = blockIdx.x;
+  const uint32_t k = split_node(act_start[a], act_end[a] - act_start[a], order, lo, hi, suf);
+  if (threadIdx.x == 0) split[a] = k;
 }
 
 // ---- chunked split for large nodes: several workgroups per node ------------------------------
@@ -204,7 +221,11 @@ __global__ __launch_bounds__(256) void bvh_split_kernel(const uint32_t* __restri
 // split's right box may lie in the next chunk), and the chunks' first minima are reduced per node.
 // Box unions are exact, so the boxes, costs and the chosen split equal the one-workgroup kernel's.
 
-__global__ __launch_bounds__(256) void bvh_chunk_total_kernel(const uint32_t* __restrict__ ch_seg,
+// The chunked kernels serve both builds: `lv` null (host-driven: act_* are the level's nodes,
+// the grid is the chunk count) or the device-driven level (act_* the flat records from lv->off,
+// the grid an upper bound, lv->nch the chunk count).
+__global__ __launch_bounds__(256) void bvh_chunk_total_kernel(const BvhLevel* __restrict__ lv,
+                                                              const uint32_t* __restrict__ ch_seg,
                                                               const uint32_t* __restrict__ ch_c0,
                                                               const uint32_t* __restrict__ act_start,
                                                               const uint32_t* __restrict__ act_end,
@@ -212,7 +233,13 @@ __global__ __launch_bounds__(256) void bvh_chunk_total_kernel(const uint32_t* __
                                                               const double* __restrict__ lo,
                                                               const double* __restrict__ hi, Box6* __restrict__ ctot) {
   __shared__ Box6 wave_tot[4];
-  const uint32_t c = blockIdx.x, a = ch_seg[c];
+  const uint32_t c = blockIdx.x;
+  uint32_t off = 0;
+  if (lv) {
+    if (c >= lv->nch) return;
+    off = lv->off;
+  }
+  const uint32_t a = off + ch_seg[c];
   const uint32_t start = act_start[a], cnt = act_end[a] - start;
   const uint32_t c0 = ch_c0[c], c1 = min(c0 + kSplitChunk, cnt);
   Box6 v = empty6();
@@ -223,11 +250,13 @@ __global__ __launch_bounds__(256) void bvh_chunk_total_kernel(const uint32_t* __
 }
 
 // Per chunk: the union of the node's earlier chunks (cpre) and of its later chunks (csuf).
-__global__ __launch_bounds__(256) void bvh_chunk_scan_kernel(uint32_t nch, const uint32_t* __restrict__ ch_seg,
+__global__ __launch_bounds__(256) void bvh_chunk_scan_kernel(const BvhLevel* __restrict__ lv, uint32_t nch,
+                                                             const uint32_t* __restrict__ ch_seg,
                                                              const uint32_t* __restrict__ seg_chunk0,
                                                              const Box6* __restrict__ ctot, Box6* __restrict__ cpre,
                                                              Box6* __restrict__ csuf) {
   const uint32_t c = blockIdx.x * 256 + threadIdx.x;
+  if (lv) nch = lv->nch;
   if (c >= nch) return;
   const uint32_t a = ch_seg[c], f0 = seg_chunk0[a], f1 = seg_chunk0[a + 1];
   Box6 pre = empty6(), suf = empty6();
@@ -237,7 +266,8 @@ __global__ __launch_bounds__(256) void bvh_chunk_scan_kernel(uint32_t nch, const
   csuf[c] = suf;
 }
 
-__global__ __launch_bounds__(256) void bvh_chunk_suffix_kernel(const uint32_t* __restrict__ ch_seg,
+__global__ __launch_bounds__(256) void bvh_chunk_suffix_kernel(const BvhLevel* __restrict__ lv,
+                                                               const uint32_t* __restrict__ ch_seg,
                                                                const uint32_t* __restrict__ ch_c0,
                                                                const uint32_t* __restrict__ act_start,
                                                                const uint32_t* __restrict__ act_end,
@@ -246,7 +276,13 @@ __global__ __launch_bounds__(256) void bvh_chunk_suffix_kernel(const uint32_t* _
                                                                const double* __restrict__ hi,
                                                                const Box6* __restrict__ csuf, double* __restrict__ suf) {
   __shared__ Box6 wave_tot[4];
-  const uint32_t c = blockIdx.x, a = ch_seg[c], t = threadIdx.x;
+  const uint32_t c = blockIdx.x, t = threadIdx.x;
+  uint32_t off = 0;
+  if (lv) {
+    if (c >= lv->nch) return;
+    off = lv->off;
+  }
+  const uint32_t a = off + ch_seg[c];
   const uint32_t start = act_start[a], cnt = act_end[a] - start;
   const uint32_t c0 = ch_c0[c], c1 = min(c0 + kSplitChunk, cnt);
   const uint32_t* sl = order + start;
@@ -277,7 +313,8 @@ struct SplitBest {
   uint32_t k;
 };
 
-__global__ __launch_bounds__(256) void bvh_chunk_cost_kernel(const uint32_t* __restrict__ ch_seg,
+__global__ __launch_bounds__(256) void bvh_chunk_cost_kernel(const BvhLevel* __restrict__ lv,
+                                                             const uint32_t* __restrict__ ch_seg,
                                                              const uint32_t* __restrict__ ch_c0,
                                                              const uint32_t* __restrict__ act_start,
                                                              const uint32_t* __restrict__ act_end,
@@ -290,7 +327,13 @@ __global__ __launch_bounds__(256) void bvh_chunk_cost_kernel(const uint32_t* __r
   __shared__ Box6 wave_tot[4];
   __shared__ double red_c[256];
   __shared__ uint32_t red_k[256];
-  const uint32_t c = blockIdx.x, a = ch_seg[c], t = threadIdx.x;
+  const uint32_t c = blockIdx.x, t = threadIdx.x;
+  uint32_t off = 0;
+  if (lv) {
+    if (c >= lv->nch) return;
+    off = lv->off;
+  }
+  const uint32_t a = off + ch_seg[c];
   const uint32_t start = act_start[a], cnt = act_end[a] - start;
   const uint32_t c0 = ch_c0[c], c1 = min(c0 + kSplitChunk, cnt);
   const uint32_t* sl = order + start;
@@ -337,13 +380,20 @@ __global__ __launch_bounds__(256) void bvh_chunk_cost_kernel(const uint32_t* __r
 }
 
 // Per node: the first minimum over its chunks (chunks in increasing k).
-__global__ __launch_bounds__(256) void bvh_chunk_pick_kernel(uint32_t na, const uint32_t* __restrict__ seg_chunk0,
+__global__ __launch_bounds__(256) void bvh_chunk_pick_kernel(const BvhLevel* __restrict__ lv, uint32_t na,
+                                                             const uint32_t* __restrict__ seg_chunk0,
                                                              const uint32_t* __restrict__ act_start,
                                                              const uint32_t* __restrict__ act_end,
                                                              const SplitBest* __restrict__ cbest,
                                                              uint32_t* __restrict__ split) {
-  const uint32_t a = blockIdx.x * 256 + threadIdx.x;
-  if (a >= na) return;
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  uint32_t off = 0;
+  if (lv) {
+    na = lv->na;
+    off = lv->off;
+  }
+  if (i >= na) return;
+  const uint32_t a = i;
   double best = INFINITY;
   uint32_t best_k = UINT32_MAX;
   for (uint32_t j = seg_chunk0[a]; j < seg_chunk0[a + 1]; ++j) {
@@ -353,13 +403,288 @@ __global__ __launch_bounds__(256) void bvh_chunk_pick_kernel(uint32_t na, const 
       best_k = b.k;
     }
   }
-  const uint32_t cnt = act_end[a] - act_start[a];
-  split[a] = best_k == UINT32_MAX ? cnt / 2 : best_k;
+  const uint32_t cnt = act_end[off + a] - act_start[off + a];
+  split[off + a] = best_k == UINT32_MAX ? cnt / 2 : best_k;
+}
+
+// ---- device-driven levels ----------------------------------------------------------------------
+// The host enqueues levels without reading anything back: a level's nodes are records
+// [off, off + na) of flat arrays (first element, end, element prefix, split), written by the
+// previous level's bvh_next kernel; every kernel reads its level's counts on the device and is
+// launched with an upper bound.  The host reads the splits once at the end and creates and
+// numbers the nodes exactly as the host-driven build does.
+
+struct BvhRecords {
+  uint32_t *start, *end, *prefix, *split;
+};
+
+// seg_of[order[p]] = the level's node holding position p, or the sentinel (one thread per
+// position: the nodes are disjoint ranges in increasing order).
+__global__ __launch_bounds__(256) void bvh_mark_all_kernel(uint32_t n, const BvhLevel* __restrict__ lv, BvhRecords R,
+                                                           const uint32_t* __restrict__ order,
+                                                           uint32_t* __restrict__ seg_of) {
+  const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= n) return;
+  const uint32_t off = lv->off, na = lv->na;
+  const uint32_t* st = R.start + off;
+  uint32_t l = 0, h = na;  // the number of nodes starting at or before p
+  while (l < h) {
+    const uint32_t m = (l + h) >> 1;
+    if (st[m] <= p)
+      l = m + 1;
+    else
+      h = m;
+  }
+  seg_of[order[p]] = (l > 0 && p < R.end[off + l - 1]) ? l - 1 : UINT32_MAX;
+}
+
+__global__ __launch_bounds__(256) void bvh_scatter_lv_kernel(const BvhLevel* __restrict__ lv, BvhRecords R,
+                                                             const uint32_t* __restrict__ key,
+                                                             const uint32_t* __restrict__ val,
+                                                             uint32_t* __restrict__ order) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= lv->total) return;
+  const uint32_t a = lv->off + key[i];
+  order[R.start[a] + (i - R.prefix[a])] = val[i];
+}
+
+__global__ __launch_bounds__(256) void bvh_split_lv_kernel(const BvhLevel* __restrict__ lv, BvhRecords R,
+                                                           const uint32_t* __restrict__ order,
+                                                           const double* __restrict__ lo, const double* __restrict__ hi,
+                                                           double* __restrict__ suf) {
+  if (blockIdx.x >= lv->na) return;  // uniform per workgroup
+  const uint32_t a = lv->off + blockIdx.x;
+  const uint32_t k = split_node(R.start[a], R.end[a] - R.start[a], order, lo, hi, suf);
+  if (threadIdx.x == 0) R.split[a] = k;
+}
+
+// Inclusive sums over a 256-thread workgroup of two counters; `tot` gets the workgroup's totals.
+__device__ void block_scan2(uint32_t& x, uint32_t& y, uint32_t* wx, uint32_t* wy, uint32_t& tx, uint32_t& ty) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t u = __shfl_up(x, o), v = __shfl_up(y, o);
+    if (lane >= o) {
+      x += u;
+      y += v;
+    }
+  }
+  if (lane == 63) {
+    wx[wave] = x;
+    wy[wave] = y;
+  }
+  __syncthreads();
+  uint32_t bx = 0, by = 0;
+  for (int w = 0; w < wave; ++w) {
+    bx += wx[w];
+    by += wy[w];
+  }
+  x += bx;
+  y += by;
+  tx = wx[0] + wx[1] + wx[2] + wx[3];
+  ty = wy[0] + wy[1] + wy[2] + wy[3];
+  __syncthreads();
+}
+
+// The level's split chunks (⌈cnt / kSplitChunk⌉ per node, in node order): per chunk its node and
+// first element, per node its first chunk (one workgroup).
+__global__ __launch_bounds__(256) void bvh_chunks_lv_kernel(BvhLevel* __restrict__ lv, BvhRecords R,
+                                                            uint32_t* __restrict__ ch_seg, uint32_t* __restrict__ ch_c0,
+                                                            uint32_t* __restrict__ seg_chunk0) {
+  __shared__ uint32_t wx[4], wy[4];
+  const uint32_t off = lv->off, na = lv->na;
+  uint32_t base = 0;
+  for (uint32_t a0 = 0; a0 < na; a0 += 256) {
+    const uint32_t a = a0 + threadIdx.x;
+    const uint32_t cnt = a < na ? R.end[off + a] - R.start[off + a] : 0;
+    const uint32_t c = (cnt + kSplitChunk - 1) / kSplitChunk;
+    uint32_t x = c, y = 0, tx, ty;
+    block_scan2(x, y, wx, wy, tx, ty);
+    if (a < na) {
+      const uint32_t first = base + x - c;
+      seg_chunk0[a] = first;
+      for (uint32_t j = 0; j < c; ++j) {
+        ch_seg[first + j] = a;
+        ch_c0[first + j] = j * kSplitChunk;
+      }
+    }
+    base += tx;
+  }
+  if (threadIdx.x == 0) {
+    seg_chunk0[na] = base;
+    lv->nch = base;
+  }
+}
+
+// The next level: the children of this level's nodes that split (more than leaf_size elements),
+// in order (left before right), with their element prefix (one workgroup).
+__global__ __launch_bounds__(256) void bvh_next_kernel(BvhLevel* __restrict__ lv, BvhRecords R, uint32_t leaf_size) {
+  __shared__ uint32_t wx[4], wy[4];
+  const uint32_t off = lv->off, na = lv->na, off2 = off + na;
+  uint32_t base_n = 0, base_e = 0;
+  for (uint32_t a0 = 0; a0 < na; a0 += 256) {
+    const uint32_t a = a0 + threadIdx.x;
+    uint32_t s0 = 0, m = 0, e0 = 0;
+    bool la = false, ra = false;
+    if (a < na) {
+      s0 = R.start[off + a];
+      e0 = R.end[off + a];
+      m = s0 + R.split[off + a];
+      la = m - s0 > leaf_size;
+      ra = e0 - m > leaf_size;
+    }
+    const uint32_t cn = (uint32_t)la + (uint32_t)ra, ce = (la ? m - s0 : 0) + (ra ? e0 - m : 0);
+    uint32_t x = cn, y = ce, tx, ty;
+    block_scan2(x, y, wx, wy, tx, ty);
+    uint32_t q = off2 + base_n + x - cn, pe = base_e + y - ce;
+    if (la) {
+      R.start[q] = s0;
+      R.end[q] = m;
+      R.prefix[q] = pe;
+      ++q;
+      pe += m - s0;
+    }
+    if (ra) {
+      R.start[q] = m;
+      R.end[q] = e0;
+      R.prefix[q] = pe;
+    }
+    base_n += tx;
+    base_e += ty;
+  }
+  if (threadIdx.x == 0) lv[1] = BvhLevel{off2, base_n, base_e, 0};
+}
+
+__global__ void bvh_root_kernel(uint32_t n, BvhLevel* __restrict__ lv, BvhRecords R) {
+  lv[0] = BvhLevel{0, 1, n, 0};
+  R.start[0] = 0;
+  R.end[0] = n;
+  R.prefix[0] = 0;
+}
+
+Topology GpuBvhBuilder::build_levels_on_device(ddgrs_ctx* ctx, size_t n, const double* lo, const double* hi,
+                                               size_t leaf_size) {
+  static const bool trace = std::getenv("DDGRS_TRACE_STEP") != nullptr;
+  const auto t_build = std::chrono::steady_clock::now();
+  const uint32_t un = (uint32_t)n, leaf = (uint32_t)leaf_size;
+  rec.reserve(4 * (n + 1));
+  BvhRecords R{rec.ptr, rec.ptr + (n + 1), rec.ptr + 2 * (n + 1), rec.ptr + 3 * (n + 1)};
+  lv_dev.reserve(kMaxBvhLevels + 1);
+  const uint32_t max_nodes = un / (leaf + 1) + 1;  // nodes of a level have more than leaf_size elements
+  ch_seg.reserve(un / kSplitChunk + max_nodes + 1);
+  ch_c0.reserve(un / kSplitChunk + max_nodes + 1);
+  seg_chunk0.reserve(max_nodes + 1);
+  ch_box.reserve(18 * (size_t)(un / kSplitChunk + max_nodes + 1));
+  ch_best.reserve(2 * (size_t)(un / kSplitChunk + max_nodes + 1));
+  std::vector<BvhLevel> lv_host;
+  std::vector<uint32_t> splits;
+  int levels = 0;
+  if (n > leaf_size) {
+    launch(ctx, "bvh_root", bvh_root_kernel, dim3(1), dim3(1), 0, un, lv_dev.ptr, R);
+    uint32_t* pin = (uint32_t*)ctx->staging(sizeof(BvhLevel));
+    int target = prev_levels.empty() ? 16 : (int)prev_levels.size();
+    for (;;) {
+      for (; levels < target; ++levels) {
+        const int L = levels;
+        const BvhLevel* lv = lv_dev.ptr + L;
+        // upper bounds of this level's nodes and of its largest node
+        const uint64_t na_bound = std::min<uint64_t>(L < 32 ? (1ull << L) : max_nodes, max_nodes);
+        const double size_bound = L < (int)prev_levels.size() ? 2.0 * prev_levels[L] : (double)n * std::pow(0.75, L);
+        launch(ctx, "bvh_mark", bvh_mark_all_kernel, dim3(blocks_for(un, 256)), dim3(256), 0, un, lv, R,
+               (const uint32_t*)order.ptr, seg_of.ptr);
+        launch(ctx, "bvh_part_keys", bvh_part_keys_kernel, dim3(blocks_for(un, 256)), dim3(256), 0, un,
+               (const uint32_t*)(axis_order.ptr + (size_t)(L % 3) * n), (const uint32_t*)seg_of.ptr, part_key_in.ptr,
+               part_val_in.ptr);
+        int bits = 1;
+        while ((1ull << bits) - 1 < na_bound) ++bits;  // the sentinel truncates to 2^bits − 1 ≥ na
+        size_t tb = temp.n;
+        HIPCHECK(rocprim::radix_sort_pairs(temp.ptr, tb, part_key_in.ptr, part_key_out.ptr, part_val_in.ptr,
+                                           part_val_out.ptr, n, 0, bits, ctx->stream));
+        launch(ctx, "bvh_scatter", bvh_scatter_lv_kernel, dim3(blocks_for(un, 256)), dim3(256), 0, lv, R,
+               (const uint32_t*)part_key_out.ptr, (const uint32_t*)part_val_out.ptr, order.ptr);
+        if (size_bound <= (double)kSplitChunk) {
+          launch(ctx, "bvh_split", bvh_split_lv_kernel, dim3((unsigned)na_bound), dim3(256), 0, lv, R,
+                 (const uint32_t*)order.ptr, lo, hi, suf.ptr);
+        } else {
+          const unsigned nch_bound = (unsigned)std::min<uint64_t>(un / kSplitChunk + na_bound, un / kSplitChunk + max_nodes);
+          Box6* ctot = (Box6*)ch_box.ptr;
+          Box6* cpre = ctot + (un / kSplitChunk + max_nodes + 1);
+          Box6* csuf = cpre + (un / kSplitChunk + max_nodes + 1);
+          launch(ctx, "bvh_chunks", bvh_chunks_lv_kernel, dim3(1), dim3(256), 0, lv_dev.ptr + L, R, ch_seg.ptr,
+                 ch_c0.ptr, seg_chunk0.ptr);
+          launch(ctx, "bvh_chunk_total", bvh_chunk_total_kernel, dim3(nch_bound), dim3(256), 0, lv,
+                 (const uint32_t*)ch_seg.ptr, (const uint32_t*)ch_c0.ptr, (const uint32_t*)R.start,
+                 (const uint32_t*)R.end, (const uint32_t*)order.ptr, lo, hi, ctot);
+          launch(ctx, "bvh_chunk_scan", bvh_chunk_scan_kernel, dim3(blocks_for(nch_bound, 256)), dim3(256), 0, lv, 0u,
+                 (const uint32_t*)ch_seg.ptr, (const uint32_t*)seg_chunk0.ptr, (const Box6*)ctot, cpre, csuf);
+          launch(ctx, "bvh_chunk_suffix", bvh_chunk_suffix_kernel, dim3(nch_bound), dim3(256), 0, lv,
+                 (const uint32_t*)ch_seg.ptr, (const uint32_t*)ch_c0.ptr, (const uint32_t*)R.start,
+                 (const uint32_t*)R.end, (const uint32_t*)order.ptr, lo, hi, (const Box6*)csuf, suf.ptr);
+          launch(ctx, "bvh_chunk_cost", bvh_chunk_cost_kernel, dim3(nch_bound), dim3(256), 0, lv,
+                 (const uint32_t*)ch_seg.ptr, (const uint32_t*)ch_c0.ptr, (const uint32_t*)R.start,
+                 (const uint32_t*)R.end, (const uint32_t*)order.ptr, lo, hi, (const Box6*)cpre, (const double*)suf.ptr,
+                 (SplitBest*)ch_best.ptr);
+          launch(ctx, "bvh_chunk_pick", bvh_chunk_pick_kernel, dim3(blocks_for(na_bound, 256)), dim3(256), 0, lv, 0u,
+                 (const uint32_t*)seg_chunk0.ptr, (const uint32_t*)R.start, (const uint32_t*)R.end,
+                 (const SplitBest*)ch_best.ptr, R.split);
+        }
+        launch(ctx, "bvh_next", bvh_next_kernel, dim3(1), dim3(256), 0, lv_dev.ptr + L, R, leaf);
+      }
+      download(ctx, (BvhLevel*)pin, (const BvhLevel*)lv_dev.ptr + levels, 1);
+      ctx->sync();
+      if (((BvhLevel*)pin)->na == 0) break;
+      if (levels >= (int)kMaxBvhLevels) throw Error(DDGRS_GPU, "BVH build", "more levels than the device build allows");
+      target = std::min<int>(levels + 4, (int)kMaxBvhLevels);
+    }
+    lv_host.resize(levels + 1);
+    download(ctx, lv_host.data(), (const BvhLevel*)lv_dev.ptr, lv_host.size());
+    ctx->sync();
+    splits.resize(lv_host[levels].off);
+    download(ctx, splits.data(), (const uint32_t*)R.split, splits.size());
+  }
+  std::vector<uint32_t> ord(n);
+  download(ctx, ord.data(), (const uint32_t*)order.ptr, n);
+  ctx->sync();
+  // the nodes in creation order, as the host-driven build makes them
+  std::vector<TreeSegment> segs{{0, un, 0}};
+  std::vector<size_t> level{0};
+  std::vector<uint32_t> level_max;
+  for (int L = 0; !level.empty(); ++L) {
+    std::vector<size_t> next;
+    uint32_t i = 0, biggest = 0;
+    for (size_t si : level) {
+      if (segs[si].end - segs[si].start <= leaf_size) continue;
+      if (L >= levels || i >= lv_host[L].na) throw Error(DDGRS_GPU, "BVH build", "device levels disagree with the nodes");
+      biggest = std::max(biggest, segs[si].end - segs[si].start);
+      const uint32_t mid = segs[si].start + splits[lv_host[L].off + i++];
+      TreeSegment l{segs[si].start, mid, segs[si].depth + 1};
+      TreeSegment r{mid, segs[si].end, segs[si].depth + 1};
+      segs[si].left = (int64_t)segs.size();
+      segs.push_back(l);
+      segs[si].right = (int64_t)segs.size();
+      segs.push_back(r);
+      next.push_back((size_t)segs[si].left);
+      next.push_back((size_t)segs[si].right);
+    }
+    if (i == 0) break;
+    if (i != lv_host[L].na) throw Error(DDGRS_GPU, "BVH build", "device levels disagree with the nodes");
+    level_max.push_back(biggest);
+    level = std::move(next);
+  }
+  prev_levels = std::move(level_max);
+  if (trace)
+    std::fprintf(stderr, "bvh: %zu elements, %d levels on the device, %.2f ms total\n", n, levels,
+                 std::chrono::duration<double>(std::chrono::steady_clock::now() - t_build).count() * 1e3);
+  return finish_topology(segs, std::move(ord));
 }
 
 Topology GpuBvhBuilder::build(ddgrs_ctx* ctx, size_t n, const double* pts, const double* lo, const double* hi,
                               size_t leaf_size) {
   const uint32_t un = (uint32_t)n;
+  // DDGRS_TRACE_STEP=1: levels and the time spent waiting for each level's split positions
+  static const bool trace = std::getenv("DDGRS_TRACE_STEP") != nullptr;
+  const auto t_build = std::chrono::steady_clock::now();
+  double wait_ms = 0.0;
+  int n_levels = 0;
   key_in.reserve(n);
   key_out.reserve(n);
   iota.reserve(n);
@@ -385,6 +710,8 @@ Topology GpuBvhBuilder::build(ddgrs_ctx* ctx, size_t n, const double* pts, const
                                        0, 64, ctx->stream));
   }
   launch(ctx, "bvh_iota", bvh_iota_kernel, dim3(blocks_for((int64_t)n, 256)), dim3(256), 0, un, order.ptr);
+  const char* host_levels = std::getenv("DDGRS_BVH_LEVELS_ON_HOST");
+  if (!(host_levels && host_levels[0] == '1')) return build_levels_on_device(ctx, n, lo, hi, leaf_size);
   std::vector<TreeSegment> segs{{0, un, 0}};
   std::vector<size_t> level{0};
   std::vector<uint32_t> a_start, a_end, a_prefix, a_split;
@@ -451,24 +778,34 @@ Topology GpuBvhBuilder::build(ddgrs_ctx* ctx, size_t n, const double* pts, const
       Box6* ctot = (Box6*)ch_box.ptr;
       Box6* cpre = ctot + nch;
       Box6* csuf = cpre + nch;
-      launch(ctx, "bvh_chunk_total", bvh_chunk_total_kernel, dim3(nch), dim3(256), 0, (const uint32_t*)ch_seg.ptr,
+      launch(ctx, "bvh_chunk_total", bvh_chunk_total_kernel, dim3(nch), dim3(256), 0, (const BvhLevel*)nullptr,
+             (const uint32_t*)ch_seg.ptr,
              (const uint32_t*)ch_c0.ptr, (const uint32_t*)act_start.ptr, (const uint32_t*)act_end.ptr,
              (const uint32_t*)order.ptr, lo, hi, ctot);
-      launch(ctx, "bvh_chunk_scan", bvh_chunk_scan_kernel, dim3(blocks_for(nch, 256)), dim3(256), 0, nch,
+      launch(ctx, "bvh_chunk_scan", bvh_chunk_scan_kernel, dim3(blocks_for(nch, 256)), dim3(256), 0,
+             (const BvhLevel*)nullptr, nch,
              (const uint32_t*)ch_seg.ptr, (const uint32_t*)seg_chunk0.ptr, (const Box6*)ctot, cpre, csuf);
-      launch(ctx, "bvh_chunk_suffix", bvh_chunk_suffix_kernel, dim3(nch), dim3(256), 0, (const uint32_t*)ch_seg.ptr,
+      launch(ctx, "bvh_chunk_suffix", bvh_chunk_suffix_kernel, dim3(nch), dim3(256), 0, (const BvhLevel*)nullptr,
+             (const uint32_t*)ch_seg.ptr,
              (const uint32_t*)ch_c0.ptr, (const uint32_t*)act_start.ptr, (const uint32_t*)act_end.ptr,
              (const uint32_t*)order.ptr, lo, hi, (const Box6*)csuf, suf.ptr);
-      launch(ctx, "bvh_chunk_cost", bvh_chunk_cost_kernel, dim3(nch), dim3(256), 0, (const uint32_t*)ch_seg.ptr,
+      launch(ctx, "bvh_chunk_cost", bvh_chunk_cost_kernel, dim3(nch), dim3(256), 0, (const BvhLevel*)nullptr,
+             (const uint32_t*)ch_seg.ptr,
              (const uint32_t*)ch_c0.ptr, (const uint32_t*)act_start.ptr, (const uint32_t*)act_end.ptr,
              (const uint32_t*)order.ptr, lo, hi, (const Box6*)cpre, (const double*)suf.ptr, (SplitBest*)ch_best.ptr);
-      launch(ctx, "bvh_chunk_pick", bvh_chunk_pick_kernel, dim3(blocks_for(na, 256)), dim3(256), 0, na,
+      launch(ctx, "bvh_chunk_pick", bvh_chunk_pick_kernel, dim3(blocks_for(na, 256)), dim3(256), 0,
+             (const BvhLevel*)nullptr, na,
              (const uint32_t*)seg_chunk0.ptr, (const uint32_t*)act_start.ptr, (const uint32_t*)act_end.ptr,
              (const SplitBest*)ch_best.ptr, act_split.ptr);
     }
     a_split.resize(na);
     download(ctx, a_split.data(), (const uint32_t*)act_split.ptr, na);
+    const auto t_wait = std::chrono::steady_clock::now();
     ctx->sync();
+    if (trace) {
+      wait_ms += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_wait).count() * 1e3;
+      ++n_levels;
+    }
     std::vector<size_t> next;
     for (uint32_t i = 0; i < na; ++i) {
       const size_t si = a_seg[i];
@@ -487,6 +824,9 @@ Topology GpuBvhBuilder::build(ddgrs_ctx* ctx, size_t n, const double* pts, const
   std::vector<uint32_t> ord(n);
   download(ctx, ord.data(), (const uint32_t*)order.ptr, n);
   ctx->sync();
+  if (trace)
+    std::fprintf(stderr, "bvh: %zu elements, %d levels, %.2f ms waiting for the device, %.2f ms total\n", n,
+                 n_levels, wait_ms, std::chrono::duration<double>(std::chrono::steady_clock::now() - t_build).count() * 1e3);
   return finish_topology(segs, std::move(ord));
 }
 

@@ -16,6 +16,8 @@
 
 namespace ddgrs {
 
+struct BvhLevel;
+
 struct GpuBvhBuilder {
   size_t cap = 0;
   DevBuf<uint64_t> key_in, key_out;
@@ -29,8 +31,18 @@ struct GpuBvhBuilder {
   DevBuf<double> ch_box;    // per chunk 3 boxes of 6 doubles: total, earlier chunks, later chunks
   DevBuf<uint64_t> ch_best; // per chunk (cost, split): 2 words
   std::vector<uint32_t> c_seg, c_c0, c_first;
-  // Builds over n elements with centres pts and boxes [lo, hi] (3 FP64 each, device pointers).
+  // device-driven levels: node records (first element, end, element prefix, split; n + 1 each),
+  // per-level counts, and the largest node of each level of the previous build (sizes the launches)
+  static constexpr uint32_t kMaxBvhLevels = 96;
+  DevBuf<uint32_t> rec;
+  DevBuf<struct BvhLevel> lv_dev;
+  std::vector<uint32_t> prev_levels;
+  // Builds over n elements with centres pts and boxes [lo, hi] (3 FP64 each, device pointers).  The
+  // levels are enqueued without reading back the splits (one read-back when the previous build's
+  // depth is reached, the splits once at the end); DDGRS_BVH_LEVELS_ON_HOST=1 reads each level's
+  // splits back and keeps the node list on the host (the same tree).
   Topology build(ddgrs_ctx* ctx, size_t n, const double* pts, const double* lo, const double* hi, size_t leaf_size);
+  Topology build_levels_on_device(ddgrs_ctx* ctx, size_t n, const double* lo, const double* hi, size_t leaf_size);
 };
 
 }  // namespace ddgrs

@@ -237,20 +237,21 @@ def test_step_errors(pkg, ctx):
     assert e.value.kind == "InvalidExponents"
 
 
-def test_device_levels_equal_host_levels(pkg, ctx, monkeypatch):
-    """Steps whose block cluster trees are classified level by level on the device (capacities from
-    the previous step) give bitwise the positions of steps that read every level's totals back."""
+@pytest.mark.parametrize("switch", ["DDGRS_BLOCK_LEVELS_ON_HOST", "DDGRS_BVH_LEVELS_ON_HOST"])
+def test_device_levels_equal_host_levels(pkg, ctx, monkeypatch, switch):
+    """Steps whose block cluster trees (or BVHs) are built level by level on the device, sized from
+    the previous step, give bitwise the positions of steps that read every level back to the host."""
     p, f = pkg.generate.config_torus_16k()
     out = {}
     for mode in ("device", "host"):
         if mode == "host":
-            monkeypatch.setenv("DDGRS_BLOCK_LEVELS_ON_HOST", "1")
+            monkeypatch.setenv(switch, "1")
         s = pkg.Surface(ctx, f, p)
         ses = pkg.Session(s)
         for _ in range(3):
             ses.step()
         out[mode] = s.positions()
-    monkeypatch.delenv("DDGRS_BLOCK_LEVELS_ON_HOST")
+    monkeypatch.delenv(switch)
     assert np.array_equal(out["device"], out["host"])
 
 
