@@ -997,6 +997,78 @@ __global__ __launch_bounds__(256) void lf_backward_x_kernel(FrontArgs A, const S
   backward_x<false>(A, items[blockIdx.x], moff, minv_t, tbuf, x, t);
 }
 
+// ---- large fronts: the forward in one phase -----------------------------------------------------
+//
+// With P_J = L_SJ M_J (formed once per factorization) the update vector needs no y_J:
+// u_J = w_S − P_J w_J, so the y and u rows of a height are one launch (the same quantity as
+// w_S − L_SJ y_J, summed in another order, FP64).  The backward keeps its two phases: the one-phase
+// form x_J = M_Jᵀ (D_J⁻¹ y_J) − P_Jᵀ x_S subtracts two large terms and raised L′'s residual 30-fold.
+
+// P_J = L_SJ M_J (rows r < f − k, columns l < k; M_J lower triangular: m from l), one 32 × 32 tile
+// per workgroup (the transpose tiles' items), m ascending.
+__global__ __launch_bounds__(256) void lf_pmat_kernel(FrontArgs A, const FrontTile* __restrict__ items,
+                                                      const uint64_t* __restrict__ ltoff,
+                                                      const uint64_t* __restrict__ moff,
+                                                      const double* __restrict__ minv,
+                                                      const double* __restrict__ front, double* __restrict__ pj) {
+  const FrontTile it = items[blockIdx.x];  // block, i0: first later row r, j0: first column l
+  const uint32_t J = it.block, f = A.bf[J], k = A.bk[J], ns = f - k;
+  const double* G = front + A.foff[J];
+  const double* M = minv + moff[J];
+  const uint32_t tx = threadIdx.x & 31, ty = threadIdx.x >> 5, l = it.j0 + tx;
+  if (l >= k) return;
+  for (uint32_t q = 0; q < 4; ++q) {
+    const uint32_t r = it.i0 + ty + 8 * q;
+    if (r >= ns) break;
+    const double* L = G + (size_t)(k + r) * f;
+    double acc = 0.0;
+    for (uint32_t m = l; m < k; ++m) acc = fma(L[m], M[(size_t)m * k + l], acc);
+    pj[ltoff[J] + (size_t)r * k + l] = acc;
+  }
+}
+
+// Forward u_J = w_S − P_J w_J, later rows [r0, r0 + 16).  LDS: w (3 f).
+__device__ void forward_up(const FrontArgs& A, SolveItem it, const uint64_t* __restrict__ ltoff,
+                           const double* __restrict__ pj, const double* __restrict__ b, double* __restrict__ ubuf,
+                           double* w) {
+  const uint32_t J = it.block, f = A.bf[J], k = A.bk[J], ns = f - k;
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  assemble_front_vector<false>(A, J, b, ubuf, w);
+  const double* P = pj + ltoff[J];
+  const double* row[4];
+  uint32_t lo[4], hi[4], ri[4];
+  for (int q = 0; q < 4; ++q) {
+    ri[q] = it.r0 + 4 * wave + q;
+    const bool ok = ri[q] < ns;
+    row[q] = P + (size_t)(ok ? ri[q] : 0) * k;
+    lo[q] = 0;
+    hi[q] = ok ? k : 0;
+  }
+  double out[4][3];
+  wave_rows4(row, lo, hi, w, lane, out);
+  double* u = ubuf + 3 * (size_t)A.ea_off[J];
+  if (lane == 0)
+    for (int q = 0; q < 4; ++q)
+      if (ri[q] < ns)
+        for (int a = 0; a < 3; ++a) u[3 * ri[q] + a] = w[3 * (k + ri[q]) + a] - out[q][a];
+}
+
+// One launch per large height: items [0, ny) are y rows (as the two-phase forward), the rest u rows.
+__global__ __launch_bounds__(256) void lf_forward_yu_kernel(FrontArgs A, const SolveItem* __restrict__ yitems,
+                                                            uint32_t ny, const SolveItem* __restrict__ uitems,
+                                                            const uint64_t* __restrict__ moff,
+                                                            const double* __restrict__ minv,
+                                                            const uint64_t* __restrict__ ltoff,
+                                                            const double* __restrict__ pj,
+                                                            const double* __restrict__ b, double* __restrict__ y,
+                                                            double* __restrict__ ubuf) {
+  extern __shared__ double w[];
+  if (blockIdx.x < ny)
+    forward_y<false>(A, yitems[blockIdx.x], moff, minv, b, y, ubuf, w);
+  else
+    forward_up(A, uitems[blockIdx.x - ny], ltoff, pj, b, ubuf, w);
+}
+
 // ---- the whole solve in one launch ----------------------------------------------------------------
 //
 // Workgroups take tasks in ticket order (an atomic counter): the forward tasks by height, then the
@@ -1243,6 +1315,7 @@ void LprimeFactorDev::setup(ddgrs_ctx* ctx, const LprimeSymbolic& S) {
   ddgrs::upload(ctx, lt_tiles, S.lt_tiles);
   n_lt_tiles = (uint32_t)S.lt_tiles.size();
   lt.reserve(std::max<uint64_t>(S.ltoff.back(), 1));
+  pj.reserve(std::max<uint64_t>(S.ltoff.back(), 1));
   tb.reserve(3 * n);
   ddgrs::upload(ctx, srows, S.srows);
   ddgrs::upload(ctx, moff, S.moff);
@@ -1343,6 +1416,9 @@ void LprimeFactorDev::factor(ddgrs_ctx* ctx, ddgrs_surface* s, double eps_rel) {
          (const uint64_t*)moff.ptr, (const double*)front.ptr, minv.ptr, minv_t.ptr);
   launch(ctx, "lf_transpose", lf_transpose_kernel, dim3(n_lt_tiles), dim3(256), 0, A,
          (const FrontTile*)lt_tiles.ptr, (const uint64_t*)ltoff.ptr, (const double*)front.ptr, lt.ptr);
+  launch(ctx, "lf_pmat", lf_pmat_kernel, dim3(n_lt_tiles), dim3(256), 0, A, (const FrontTile*)lt_tiles.ptr,
+         (const uint64_t*)ltoff.ptr, (const uint64_t*)moff.ptr, (const double*)minv.ptr, (const double*)front.ptr,
+         pj.ptr);
   download(ctx, bad_host, (const uint32_t*)bad.ptr, 1);
 }
 
@@ -1383,7 +1459,17 @@ void LprimeFactorDev::solve(ddgrs_ctx* ctx, const double* r, double* x) {
     return;
   }
   const size_t lds = (3 * (size_t)max_front + 3 * (size_t)max_rows) * sizeof(double);
+  const char* tp = std::getenv("DDGRS_LPRIME_SOLVE_TWO_PHASE");
+  const bool two_phase = tp && tp[0] == '1';
   for (uint32_t h = 0; h < H; ++h) {
+    if (hsplit[h] && !two_phase) {
+      const uint32_t ny = krows_off[h + 1] - krows_off[h], nu = urows_off[h + 1] - urows_off[h];
+      launch(ctx, height_name("lf_forward", h), lf_forward_yu_kernel, dim3(ny + nu), dim3(256), lds, A,
+             (const SolveItem*)krows.ptr + krows_off[h], ny, (const SolveItem*)urows.ptr + urows_off[h],
+             (const uint64_t*)moff.ptr, (const double*)minv.ptr, (const uint64_t*)ltoff.ptr, (const double*)pj.ptr,
+             (const double*)xb.ptr, yb.ptr, ubuf.ptr);
+      continue;
+    }
     if (!hsplit[h]) {
       launch(ctx, height_name("lf_forward", h), lf_forward_small_kernel, dim3(hoff[h + 1] - hoff[h]), dim3(256),
              (size_t)hlds[h], A, (const uint32_t*)hblocks.ptr + hoff[h], (const uint64_t*)moff.ptr,

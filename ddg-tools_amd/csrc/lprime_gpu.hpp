@@ -104,6 +104,7 @@ struct LprimeFactorDev {
   DevBuf<SolveItem> krows, urows;
   DevBuf<uint64_t> ltoff;
   DevBuf<double> lt;
+  DevBuf<double> pj;  // large fronts: P_J = L_SJ M_J ((f − k) × k), at ltoff
   DevBuf<FrontTile> lt_tiles;
   DevBuf<uint64_t> foff, amap_k, amap_c;
   DevBuf<FrontItem> ea_items;

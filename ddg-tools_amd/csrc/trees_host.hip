@@ -159,7 +159,7 @@ Topology finish_topology(const std::vector<TreeSegment>& segs, std::vector<uint3
 
 Clusters build_clusters(const Topology& t, const double* pts, const double* lo, const double* hi, const double* mass,
                         const double* normals) {
-  const size_t nn = t.n_nodes();
+  const size_t nn = t.n_nodes(), nl = t.leaves.size();
   Clusters c;
   c.mass.assign(nn, 0.0);
   c.centroid.assign(3 * nn, 0.0);
@@ -167,57 +167,61 @@ Clusters build_clusters(const Topology& t, const double* pts, const double* lo, 
   c.hi.assign(3 * nn, 0.0);
   c.radius.assign(nn, 0.0);
   std::vector<double> mom(3 * nn, 0.0);
-  for (size_t a = nn; a-- > 0;) {
+  // fold of (Σa, Σ a NxNx, Σ a NxNy, Σ a NxNz, Σ a NyNy, Σ a NyNz, Σ a NzNz), then P̄ = s / Σa
+  std::vector<double> ps(normals ? 7 * nn : 0, 0.0);
+  // leaves (independent: in parallel), each over its elements in order
+  parallel(nl, [&](size_t k) {
+    const uint32_t a = t.leaves[k];
     Box b = empty_box();
-    if (t.is_leaf(a)) {
-      double m = 0.0, mo[3] = {0.0, 0.0, 0.0};
+    double m = 0.0, mo[3] = {0.0, 0.0, 0.0};
+    for (uint32_t q = t.start[a]; q < t.end[a]; ++q) {
+      uint32_t e = t.order[q];
+      unite(b, lo + 3 * e, hi + 3 * e);
+      m += mass[e];
+      for (int k2 = 0; k2 < 3; ++k2) mo[k2] += mass[e] * pts[3 * e + k2];
+    }
+    c.mass[a] = m;
+    for (int k2 = 0; k2 < 3; ++k2) {
+      mom[3 * a + k2] = mo[k2];
+      c.lo[3 * a + k2] = b.lo[k2];
+      c.hi[3 * a + k2] = b.hi[k2];
+    }
+    if (normals) {
+      double* o = &ps[7 * (size_t)a];
       for (uint32_t q = t.start[a]; q < t.end[a]; ++q) {
         uint32_t e = t.order[q];
-        unite(b, lo + 3 * e, hi + 3 * e);
-        m += mass[e];
-        for (int k = 0; k < 3; ++k) mo[k] += mass[e] * pts[3 * e + k];
+        double w = mass[e];
+        const double* n = normals + 3 * e;
+        o[0] += w;
+        o[1] += w * n[0] * n[0];
+        o[2] += w * n[0] * n[1];
+        o[3] += w * n[0] * n[2];
+        o[4] += w * n[1] * n[1];
+        o[5] += w * n[1] * n[2];
+        o[6] += w * n[2] * n[2];
       }
-      c.mass[a] = m;
-      for (int k = 0; k < 3; ++k) mom[3 * a + k] = mo[k];
-    } else {
-      size_t l = a + 1, r = t.right[a];
-      unite(b, &c.lo[3 * l], &c.hi[3 * l]);
-      unite(b, &c.lo[3 * r], &c.hi[3 * r]);
-      c.mass[a] = c.mass[l] + c.mass[r];
-      for (int k = 0; k < 3; ++k) mom[3 * a + k] = mom[3 * l + k] + mom[3 * r + k];
     }
+  });
+  // internal nodes bottom-up (children have larger pre-order ids): left + right
+  for (size_t a = nn; a-- > 0;) {
+    if (t.is_leaf(a)) continue;
+    const size_t l = a + 1, r = t.right[a];
+    Box b = empty_box();
+    unite(b, &c.lo[3 * l], &c.hi[3 * l]);
+    unite(b, &c.lo[3 * r], &c.hi[3 * r]);
+    c.mass[a] = c.mass[l] + c.mass[r];
     for (int k = 0; k < 3; ++k) {
+      mom[3 * a + k] = mom[3 * l + k] + mom[3 * r + k];
       c.lo[3 * a + k] = b.lo[k];
       c.hi[3 * a + k] = b.hi[k];
     }
+    if (normals)
+      for (int k = 0; k < 7; ++k) ps[7 * a + k] = ps[7 * l + k] + ps[7 * r + k];
   }
-  if (normals) {
-    // fold of (Σa, Σ a NxNx, Σ a NxNy, Σ a NxNz, Σ a NyNy, Σ a NyNz, Σ a NzNz), then P̄ = s / Σa
-    std::vector<double> ps(7 * nn, 0.0);
-    for (size_t a = nn; a-- > 0;) {
-      double* o = &ps[7 * a];
-      if (t.is_leaf(a)) {
-        for (uint32_t q = t.start[a]; q < t.end[a]; ++q) {
-          uint32_t e = t.order[q];
-          double w = mass[e];
-          const double* n = normals + 3 * e;
-          o[0] += w;
-          o[1] += w * n[0] * n[0];
-          o[2] += w * n[0] * n[1];
-          o[3] += w * n[0] * n[2];
-          o[4] += w * n[1] * n[1];
-          o[5] += w * n[1] * n[2];
-          o[6] += w * n[2] * n[2];
-        }
-      } else {
-        for (int k = 0; k < 7; ++k) o[k] = ps[7 * (a + 1) + k] + ps[7 * t.right[a] + k];
-      }
-    }
-    c.projector.resize(6 * nn);
-    for (size_t a = 0; a < nn; ++a)
+  if (normals) c.projector.resize(6 * nn);
+  parallel(nn, [&](size_t a) {
+    if (normals)
       for (int k = 0; k < 6; ++k) c.projector[6 * a + k] = ps[7 * a + k + 1] / ps[7 * a];
-  }
-  for (size_t a = 0; a < nn; ++a) {
     double r2 = 0.0;
     for (int k = 0; k < 3; ++k) {
       double x = mom[3 * a + k] / c.mass[a];
@@ -229,7 +233,7 @@ Clusters build_clusters(const Topology& t, const double* pts, const double* lo, 
       r2 += d * d;
     }
     c.radius[a] = std::sqrt(r2);
-  }
+  });
   return c;
 }
 

@@ -89,7 +89,9 @@ def test_lprime_device_factor_matches_host(pkg, ctx, name, monkeypatch):
 
 def test_lprime_solve_paths_agree(pkg, ctx, monkeypatch):
     """The L′ solve's small-front path (matrices staged in LDS, one workgroup per block) and its
-    large-front path (rows spread over workgroups) give the same solution."""
+    large-front path (rows spread over workgroups; the forward's update vectors from P_J = L_SJ M_J)
+    solve L′ to the same backward error, and their solutions agree to the conditioning of L′
+    (different summation orders: up to ~1e-8, see above)."""
     p, f = meshes(pkg)["torus_16k"]()
     s = pkg.Surface(ctx, f, p)
     small = pkg.Saddle(s)
@@ -99,8 +101,17 @@ def test_lprime_solve_paths_agree(pkg, ctx, monkeypatch):
     rng = np.random.default_rng(24)
     r = rng.standard_normal(small.n)
     xs, xl = small.lprime_solve(r), large.lprime_solve(r)
-    print(f"small/large path rel diff {rel(xl, xs):.2e}")
-    assert rel(xl, xs) <= 1e-9
+    # the large path's default forms (u_J from P_J = L_SJ M_J, x_J from P_Jᵀ) against its two phases
+    monkeypatch.setenv("DDGRS_LPRIME_SOLVE_TWO_PHASE", "1")
+    xl2 = large.lprime_solve(r)
+    monkeypatch.delenv("DDGRS_LPRIME_SOLVE_TWO_PHASE")
+    L, _ = lprime_axis_matrix(f, p, oracle.vertex_areas(f, p))
+    res = {k: lprime_residual(L, r, v) for k, v in (("small", xs), ("large", xl), ("two phases", xl2))}
+    print(f"small/large path rel diff {rel(xl, xs):.2e}, one/two phases {rel(xl, xl2):.2e}, residuals {res}")
+    assert res["large"] <= max(10.0 * res["small"], 1e-13)
+    assert res["two phases"] <= max(10.0 * res["small"], 1e-13)
+    assert rel(xl, xs) <= 1e-8
+    assert rel(xl, xl2) <= 1e-8
 
 
 @pytest.mark.skip(reason="the one-launch L' solve hung on the device: the compiler merged thread 0's task "
