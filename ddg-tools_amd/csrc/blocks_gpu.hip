@@ -639,7 +639,8 @@ BlockStats BlockTreeDev::build_structure(ddgrs_ctx* ctx, const Topology& t, cons
     if (!trace) return;
     ctx->sync();
     const auto now = std::chrono::steady_clock::now();
-    std::fprintf(stderr, "blocks: %-14s %8.2f ms\n", what, std::chrono::duration<double>(now - t_part).count() * 1e3);
+    std::fprintf(stderr, "blocks(%d): %-14s %8.2f ms\n", n, what,
+                 std::chrono::duration<double>(now - t_part).count() * 1e3);
     t_part = now;
   };
   nk = nk_;
@@ -672,8 +673,9 @@ BlockStats BlockTreeDev::build_structure(ddgrs_ctx* ctx, const Topology& t, cons
   // DDGRS_BLOCK_LEVELS_ON_HOST=1 always reads the totals back per level (for comparisons)
   const bool levels_on_host = std::getenv("DDGRS_BLOCK_LEVELS_ON_HOST") != nullptr;
   uint32_t n_adm = 0, n_near = 0, depth = 0, max_m = 1;
-  if (!(S.cap_pairs && !levels_on_host && device_levels(ctx, S, g, right.ptr, n_adm, n_near, depth, max_m)))
-    host_levels(ctx, S, g, right.ptr, n_adm, n_near, depth, max_m);
+  const bool on_device = S.cap_pairs && !levels_on_host && device_levels(ctx, S, g, right.ptr, n_adm, n_near, depth, max_m);
+  if (!on_device) host_levels(ctx, S, g, right.ptr, n_adm, n_near, depth, max_m);
+  if (trace) std::fprintf(stderr, "blocks(%d): levels %s, depth %u\n", n, on_device ? "on the device" : "on the host", depth);
   S.cap_pairs = std::max<uint64_t>(S.cap_pairs, (uint64_t)max_m + max_m / 4 + 1024);
   S.cap_adm = std::max<uint64_t>(S.cap_adm, (uint64_t)n_adm + n_adm / 4 + 1024);
   S.cap_near = std::max<uint64_t>(S.cap_near, (uint64_t)n_near + n_near / 4 + 1024);

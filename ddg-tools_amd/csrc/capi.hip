@@ -47,6 +47,7 @@ void ctx_release(ddgrs_ctx* ctx) {
       (void)hipEventDestroy(p.stop);
     }
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->arena) (void)hipHostFree(ctx->arena);
     ctx->scratch = Scratch();
     (void)hipStreamDestroy(ctx->stream);
   } catch (...) {

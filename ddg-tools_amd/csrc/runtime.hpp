@@ -127,6 +127,29 @@ struct ddgrs_ctx {
     }
     return pinned;
   }
+  // Pinned memory that uploads are copied through: a copy from pageable memory is staged by the
+  // runtime through buffers of its own, shared by every host thread, and waits for them.  A region
+  // of the arena is reused only after a synchronization of the stream (the copies reading it are
+  // done): when the arena is full the stream is synchronized and the arena starts over (and grows
+  // when one upload needs more).
+  char* arena = nullptr;
+  size_t arena_bytes = 0, arena_used = 0;
+  void* arena_take(size_t bytes) {
+    bytes = (bytes + 255) & ~(size_t)255;
+    if (arena_used + bytes > arena_bytes) {
+      HIPCHECK(hipStreamSynchronize(stream));
+      arena_used = 0;
+      if (bytes > arena_bytes) {
+        if (arena) (void)hipHostFree(arena);
+        arena = nullptr;
+        arena_bytes = std::max<size_t>(bytes, std::max<size_t>(2 * arena_bytes, (size_t)16 << 20));
+        HIPCHECK(hipHostMalloc((void**)&arena, arena_bytes, hipHostMallocDefault));
+      }
+    }
+    void* p = arena + arena_used;
+    arena_used += bytes;
+    return p;
+  }
   // Collects finished event pairs into `times` (called after a synchronization).
   void harvest() {
     for (auto& p : pending) {
@@ -169,7 +192,10 @@ inline void launch(ddgrs_ctx* ctx, const char* name, K kernel, dim3 grid, dim3 b
 template <class T>
 inline void upload(ddgrs_ctx* ctx, DevBuf<T>& d, const T* h, size_t n) {
   d.reserve(n);
-  if (n) HIPCHECK(hipMemcpyAsync(d.ptr, h, n * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+  if (!n) return;
+  void* a = ctx->arena_take(n * sizeof(T));
+  std::memcpy(a, h, n * sizeof(T));
+  HIPCHECK(hipMemcpyAsync(d.ptr, a, n * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
 }
 template <class T>
 inline void upload(ddgrs_ctx* ctx, DevBuf<T>& d, const std::vector<T>& h) {
