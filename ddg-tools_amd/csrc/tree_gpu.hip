@@ -125,31 +125,6 @@ __global__ __launch_bounds__(256) void bvh_scatter_kernel(uint32_t total, const 
   order[act_start[a] + (i - act_prefix[a])] = val[i];
 }
 
-// The boxes of a chunk's elements [c0, c1) (at most kSplitChunk) for the iterations of a 256-thread
-// workgroup: iteration j holds element c0 + 256 j + t (forward) or c1 − 1 − 256 j − t (backward),
-// the empty box past the chunk.  All loads are issued before any is used (element indices first,
-// then the boxes; the indices clamped into the chunk), instead of one load round trip per iteration.
-constexpr int kChunkIters = 8;  // kSplitChunk / 256
-template <bool Backward>
-__device__ __forceinline__ void chunk_boxes(const uint32_t* __restrict__ sl, uint32_t c0, uint32_t c1,
-                                            const double* __restrict__ lo, const double* __restrict__ hi,
-                                            Box6 (&bx)[kChunkIters]) {
-  const int64_t t = threadIdx.x;
-  uint32_t e[kChunkIters];
-#pragma unroll
-  for (int j = 0; j < kChunkIters; ++j) {
-    const int64_t p = Backward ? (int64_t)c1 - 1 - 256 * j - t : (int64_t)c0 + 256 * j + t;
-    const int64_t pc = p < (int64_t)c0 ? (int64_t)c0 : (p >= (int64_t)c1 ? (int64_t)c1 - 1 : p);
-    e[j] = sl[pc];
-  }
-#pragma unroll
-  for (int j = 0; j < kChunkIters; ++j) {
-    const int64_t p = Backward ? (int64_t)c1 - 1 - 256 * j - t : (int64_t)c0 + 256 * j + t;
-    const Box6 b = load_box(lo, hi, e[j]);
-    bx[j] = (p >= (int64_t)c0 && p < (int64_t)c1) ? b : empty6();
-  }
-}
-
 // Level records of the device-driven build (BvhLevel: a level's nodes are records
 // [off, off + na), `total` elements, `nch` split chunks).
 struct BvhLevel {
@@ -158,7 +133,7 @@ struct BvhLevel {
 
 // The split of one node [start, start + cnt) by one workgroup: suffix boxes, then prefix boxes
 // with the cost of every admissible split, reduced to the first minimum (thread 0's value).
-__device__ uint32_t split_node_loop(uint32_t start, uint32_t cnt, const uint32_t* __restrict__ order,
+__device__ uint32_t split_node(uint32_t start, uint32_t cnt, const uint32_t* __restrict__ order,
                                const double* __restrict__ lo, const double* __restrict__ hi, double* __restrict__ suf) {
   __shared__ Box6 wave_tot[4];
   __shared__ double red_c[256];
@@ -229,100 +204,6 @@ __device__ uint32_t split_node_loop(uint32_t start, uint32_t cnt, const uint32_t
   return red_k[0] == UINT32_MAX ? cnt / 2 : red_k[0];
 }
 
-// The same for a node of at most kSplitChunk elements, every iteration's loads issued together.
-__device__ uint32_t split_node_small(uint32_t start, uint32_t cnt, const uint32_t* __restrict__ order,
-                                     const double* __restrict__ lo, const double* __restrict__ hi,
-                                     double* __restrict__ suf) {
-  __shared__ Box6 wave_tot[4];
-  __shared__ double red_c[256];
-  __shared__ uint32_t red_k[256];
-  const uint32_t t = threadIdx.x;
-  const uint32_t* sl = order + start;
-  double* sf = suf + 6 * (size_t)start;
-  size_t low = (size_t)ceil(0.25 * (double)cnt);
-  if (low < 1) low = 1;
-  if (low > cnt / 2) low = cnt / 2;
-  Box6 bx[kChunkIters];
-  chunk_boxes<true>(sl, 0, cnt, lo, hi, bx);
-  // suffix boxes: chunks from the end, lane order reversed
-  Box6 carry = empty6();
-#pragma unroll
-  for (int j = 0; j < kChunkIters; ++j) {
-    const int64_t c1 = (int64_t)cnt - 256 * j;
-    if (c1 <= 0) break;  // uniform
-    const int64_t c0 = c1 - 256 > 0 ? c1 - 256 : 0;
-    const int64_t p = c1 - 1 - (int64_t)t;
-    Box6 v = bx[j];
-    Box6 tot;
-    v = block_scan(v, wave_tot, &tot);
-    v = unite(v, carry);
-    if (p >= c0) {
-      double* o = sf + 6 * p;
-      o[0] = v.l0;
-      o[1] = v.l1;
-      o[2] = v.l2;
-      o[3] = v.h0;
-      o[4] = v.h1;
-      o[5] = v.h2;
-    }
-    carry = unite(tot, carry);
-  }
-  __syncthreads();
-  // prefix boxes and costs: the split k = p + 1 puts [0, k) left and [k, cnt) right
-  chunk_boxes<false>(sl, 0, cnt, lo, hi, bx);
-  double rd[kChunkIters];
-#pragma unroll
-  for (int j = 0; j < kChunkIters; ++j) {
-    const size_t k = (size_t)256 * j + t + 1;
-    const double* o = sf + 6 * (k < cnt ? k : (size_t)cnt - 1);  // clamped; used only for k ≤ cnt − low
-    const Box6 r = {o[0], o[1], o[2], o[3], o[4], o[5]};
-    rd[j] = diam2(r);
-  }
-  double best = INFINITY;
-  uint32_t best_k = UINT32_MAX;
-  carry = empty6();
-#pragma unroll
-  for (int j = 0; j < kChunkIters; ++j) {
-    const uint32_t c0 = 256 * j;
-    if (c0 >= cnt) break;  // uniform
-    const uint32_t p = c0 + t;
-    Box6 v = bx[j];
-    Box6 tot;
-    v = block_scan(v, wave_tot, &tot);
-    v = unite(carry, v);
-    const size_t k = (size_t)p + 1;
-    if (p < cnt && k >= low && k <= cnt - low) {
-      const double cost = diam2(v) + rd[j];
-      if (cost < best) {
-        best = cost;
-        best_k = (uint32_t)k;
-      }
-    }
-    carry = unite(carry, tot);
-  }
-  red_c[t] = best;
-  red_k[t] = best_k;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)t < w) {
-      const double c2 = red_c[t + w];
-      const uint32_t k2 = red_k[t + w];
-      if (k2 != UINT32_MAX && (red_k[t] == UINT32_MAX || c2 < red_c[t] || (c2 == red_c[t] && k2 < red_k[t]))) {
-        red_c[t] = c2;
-        red_k[t] = k2;
-      }
-    }
-    __syncthreads();
-  }
-  return red_k[0] == UINT32_MAX ? cnt / 2 : red_k[0];
-}
-
-__device__ uint32_t split_node(uint32_t start, uint32_t cnt, const uint32_t* __restrict__ order,
-                               const double* __restrict__ lo, const double* __restrict__ hi, double* __restrict__ suf) {
-  return cnt <= kSplitChunk ? split_node_small(start, cnt, order, lo, hi, suf)
-                            : split_node_loop(start, cnt, order, lo, hi, suf);
-}
-
 // One workgroup per splitting node.
 __global__ __launch_bounds__(256) void bvh_split_kernel(const uint32_t* __restrict__ act_start,
                                                         const uint32_t* __restrict__ act_end,
@@ -361,11 +242,8 @@ __global__ __launch_bounds__(256) void bvh_chunk_total_kernel(const BvhLevel* __
   const uint32_t a = off + ch_seg[c];
   const uint32_t start = act_start[a], cnt = act_end[a] - start;
   const uint32_t c0 = ch_c0[c], c1 = min(c0 + kSplitChunk, cnt);
-  Box6 bx[kChunkIters];
-  chunk_boxes<false>(order + start, c0, c1, lo, hi, bx);
   Box6 v = empty6();
-#pragma unroll
-  for (int j = 0; j < kChunkIters; ++j) v = unite(v, bx[j]);
+  for (uint32_t p = c0 + threadIdx.x; p < c1; p += 256) v = unite(v, load_box(lo, hi, order[start + p]));
   Box6 tot;
   (void)block_scan(v, wave_tot, &tot);
   if (threadIdx.x == 0) ctot[c] = tot;
@@ -410,15 +288,10 @@ __global__ __launch_bounds__(256) void bvh_chunk_suffix_kernel(const BvhLevel* _
   const uint32_t* sl = order + start;
   double* sf = suf + 6 * (size_t)start;
   Box6 carry = csuf[c];
-  Box6 bx[kChunkIters];
-  chunk_boxes<true>(sl, c0, c1, lo, hi, bx);
-#pragma unroll
-  for (int j = 0; j < kChunkIters; ++j) {
-    const int64_t e1 = (int64_t)c1 - 256 * j;
-    if (e1 <= (int64_t)c0) break;  // uniform
+  for (int64_t e1 = c1; e1 > (int64_t)c0; e1 -= 256) {
     const int64_t e0 = e1 - 256 > (int64_t)c0 ? e1 - 256 : (int64_t)c0;
     const int64_t p = e1 - 1 - (int64_t)t;
-    Box6 v = bx[j];
+    Box6 v = p >= e0 ? load_box(lo, hi, sl[p]) : empty6();
     Box6 tot;
     v = block_scan(v, wave_tot, &tot);
     v = unite(v, carry);
@@ -471,29 +344,17 @@ __global__ __launch_bounds__(256) void bvh_chunk_cost_kernel(const BvhLevel* __r
   double best = INFINITY;
   uint32_t best_k = UINT32_MAX;
   Box6 carry = cpre[c];
-  // the elements' boxes and the right parts' diameters of every iteration, loaded before the scans
-  Box6 bx[kChunkIters];
-  chunk_boxes<false>(sl, c0, c1, lo, hi, bx);
-  double rd[kChunkIters];
-#pragma unroll
-  for (int j = 0; j < kChunkIters; ++j) {
-    const size_t k = (size_t)c0 + 256 * j + t + 1;
-    const double* o = sf + 6 * (k < cnt ? k : (size_t)cnt - 1);  // clamped; used only for k ≤ cnt − low
-    const Box6 r = {o[0], o[1], o[2], o[3], o[4], o[5]};
-    rd[j] = diam2(r);
-  }
-#pragma unroll
-  for (int j = 0; j < kChunkIters; ++j) {
-    const uint32_t e0 = c0 + 256 * j;
-    if (e0 >= c1) break;  // uniform
+  for (uint32_t e0 = c0; e0 < c1; e0 += 256) {
     const uint32_t p = e0 + t;
-    Box6 v = bx[j];
+    Box6 v = p < c1 ? load_box(lo, hi, sl[p]) : empty6();
     Box6 tot;
     v = block_scan(v, wave_tot, &tot);
     v = unite(carry, v);
     const size_t k = (size_t)p + 1;
     if (p < c1 && k >= low && k <= cnt - low) {
-      const double cost = diam2(v) + rd[j];
+      const double* o = sf + 6 * k;
+      const Box6 r = {o[0], o[1], o[2], o[3], o[4], o[5]};
+      const double cost = diam2(v) + diam2(r);
       if (cost < best) {
         best = cost;
         best_k = (uint32_t)k;
