@@ -129,6 +129,18 @@ SIGNATURES = {
     "ddgrs_session_destroy": (None, [_P]),
     "ddgrs_session_step": (_I32, [_P, _P, _E]),
     "ddgrs_session_state": (_I32, [_P, _P, _E]),
+    "ddgrs_learned_create": (_I32, [_P, ctypes.c_uint64, _I64, ctypes.POINTER(_P), _E]),
+    "ddgrs_learned_destroy": (None, [_P]),
+    "ddgrs_learned_info": (_I32, [_P, _P, _P, _E]),
+    "ddgrs_learned_params": (_I32, [_P, _P, _P, _E]),
+    "ddgrs_learned_set_scale": (_I32, [_P, _D, _E]),
+    "ddgrs_learned_add_pair": (_I32, [_P, _P, _P, _P, _P, _E]),
+    "ddgrs_learned_features": (_I32, [_P, _P, _P, _E]),
+    "ddgrs_learned_blocks": (_I32, [_P, _P, _P, _E]),
+    "ddgrs_learned_loss_grad": (_I32, [_P, _I64, _P, _P, _E]),
+    "ddgrs_learned_train": (_I32, [_P, _I64, _I64, _D, _P, _E]),
+    "ddgrs_learned_apply_batch": (_I32, [_P, _P, _I64, _P, _P, _E]),
+    "ddgrs_session_set_learned": (_I32, [_P, _P, _I32, _E]),
 }
 
 
@@ -475,6 +487,10 @@ class _StepReportC(ctypes.Structure):
         ("seconds_line_search", _D),
         ("seconds_projection", _D),
         ("seconds_total", _D),
+        ("learned_mode", _I32),
+        ("learned_fallback", _I32),
+        ("learned_iterations", _I64),
+        ("seconds_learned", _D),
     ]
 
 
@@ -596,6 +612,9 @@ class StepReport:
     pairs_exact: int
     nodes_lumped: int
     seconds: dict
+    learned_mode: int = 0
+    learned_fallback: bool = False
+    learned_iterations: int = 0
 
 
 class Session(_Handle):
@@ -618,7 +637,14 @@ class Session(_Handle):
         sec = {k[len("seconds_"):]: float(getattr(r, k)) for k, _ in _StepReportC._fields_ if k.startswith("seconds_")}
         return StepReport(float(r.energy_before), float(r.energy_trial), float(r.gradient_norm), float(r.sobolev_norm_sq),
                           float(r.tau), float(r.tau0), bool(r.accepted), int(r.trials), _gmres_report(r.gmres),
-                          int(r.pairs_exact), int(r.nodes_lumped), sec)
+                          int(r.pairs_exact), int(r.nodes_lumped), sec, int(r.learned_mode), bool(r.learned_fallback),
+                          int(r.learned_iterations))
+
+    def set_learned(self, learned: "Learned | None", mode: int):
+        """Attach a learned preconditioner: mode 1 records a training pair per step (plain solve),
+        2 solves with P⁻¹ + Q (the plain solve on a miss), 0 detaches."""
+        _call(lib().ddgrs_session_set_learned, self._h, learned._h if learned is not None else None, mode)
+        self._learned = learned  # keeps it alive while attached
 
     def state(self) -> dict:
         o = np.zeros(5)
@@ -627,6 +653,75 @@ class Session(_Handle):
 
     def positions(self) -> np.ndarray:
         return self.surface.positions()
+
+
+class Learned(_Handle):
+    """The learned right preconditioner of a surface's mesh (no counterpart in the reference): a
+    mesh network giving per-vertex and per-edge 3×3 blocks, Q, added to the plain preconditioner
+    (z = P⁻¹ r + Q r).  Training, the forward pass and Q r run on the device."""
+
+    _destroy = "ddgrs_learned_destroy"
+
+    def __init__(self, surface: Surface, seed: int = 0, max_pairs: int = 64):
+        self.surface = surface
+        h = _P()
+        _call(lib().ddgrs_learned_create, surface.handle, seed, max_pairs, ctypes.byref(h))
+        self._h = h
+
+    def info(self) -> dict:
+        o = np.zeros(4, dtype=np.int64)
+        sc = ctypes.c_double()
+        _call(lib().ddgrs_learned_info, self._h, _ptr(o), ctypes.byref(sc))
+        return {"parameters": int(o[0]), "pairs": int(o[1]), "adam_steps": int(o[2]), "features": int(o[3]),
+                "scale": float(sc.value)}
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.info()["parameters"], dtype=np.float32)
+        _call(lib().ddgrs_learned_params, self._h, _ptr(p), None)
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.shape != (self.info()["parameters"],):
+            raise ValueError("wrong parameter count")
+        _call(lib().ddgrs_learned_params, self._h, None, _ptr(p))
+
+    def set_scale(self, s: float):
+        _call(lib().ddgrs_learned_set_scale, self._h, float(s))
+
+    def add_pair(self, positions, r, pr, y):
+        n = 3 * self.surface.nv + 3
+        a = [_f64(positions, (self.surface.nv, 3))] + [_f64(v, (n,)) for v in (r, pr, y)]
+        _call(lib().ddgrs_learned_add_pair, self._h, *[_ptr(v) for v in a])
+
+    def features(self, positions) -> np.ndarray:
+        f = np.zeros((self.surface.nv, 8), dtype=np.float32)
+        _call(lib().ddgrs_learned_features, self._h, _ptr(_f64(positions, (self.surface.nv, 3))), _ptr(f))
+        return f
+
+    def blocks(self, positions) -> np.ndarray:
+        b = np.zeros((self.surface.nv, 18), dtype=np.float32)
+        _call(lib().ddgrs_learned_blocks, self._h, _ptr(_f64(positions, (self.surface.nv, 3))), _ptr(b))
+        return b
+
+    def loss_grad(self, pair: int, grad: bool = True):
+        g = np.zeros(self.info()["parameters"], dtype=np.float32) if grad else None
+        l = ctypes.c_double()
+        _call(lib().ddgrs_learned_loss_grad, self._h, pair, ctypes.byref(l), _ptr(g) if grad else None)
+        return float(l.value), g
+
+    def train(self, iterations: int, batch: int = 8, lr: float = 1e-3):
+        o = np.zeros(2)
+        _call(lib().ddgrs_learned_train, self._h, iterations, batch, lr, _ptr(o))
+        return float(o[0]), float(o[1])
+
+    def apply(self, positions, rs) -> np.ndarray:
+        n = 3 * self.surface.nv + 3
+        r = _f64(rs).reshape(-1, n)
+        z = np.zeros_like(r)
+        _call(lib().ddgrs_learned_apply_batch, self._h, _ptr(_f64(positions, (self.surface.nv, 3))), r.shape[0], _ptr(r),
+              _ptr(z))
+        return z.reshape(np.shape(rs))
 
 
 def energy(surface: Surface, exponents: Exponents = Exponents(), eval=EnergyEval.Exact) -> EnergyReport:

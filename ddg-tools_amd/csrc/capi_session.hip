@@ -234,6 +234,24 @@ int ddgrs_session_step(ddgrs_session* h, ddgrs_step_report* report, ddgrs_error*
   });
 }
 
+int ddgrs_session_set_learned(ddgrs_session* h, ddgrs_learned* l, int32_t mode, ddgrs_error* err) {
+  return guarded(err, "session_set_learned", [&] {
+    need(h, "session");
+    if (mode < 0 || mode > 2) invalid("mode", std::to_string(mode), "0 (plain), 1 (record pairs) or 2 (P⁻¹ + Q)");
+    if (!l || mode == 0) {
+      h->ses.learned = nullptr;
+      h->ses.learned_mode = 0;
+      return;
+    }
+    LearnedModel* m = learned_model(l);
+    if (m->nv != h->s->nv || m->faces != h->s->faces_h)
+      invalid("learned", "a network of another mesh", "the session's surface mesh");
+    if (mode == 2 && !(m->scale > 0.f)) invalid("learned", "untrained", "a trained network (or a set scale) for mode 2");
+    h->ses.learned = m;
+    h->ses.learned_mode = mode;
+  });
+}
+
 int ddgrs_session_state(ddgrs_session* h, double* out5, ddgrs_error* err) {
   return guarded(err, "session_state", [&] {
     need(h, "session");

@@ -1,0 +1,733 @@
+// The learned right preconditioner (see learned.hpp): features, the network's forward and backward
+// passes, the loss, Adam and Q r, as kernels; pairs and training control on the host.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "capi_util.hpp"
+#include "learned.hpp"
+
+namespace ddgrs {
+
+LearnedLayout learned_layout() {
+  LearnedLayout L{};
+  size_t o = 0;
+  L.w0 = o;
+  o += (size_t)kLnFeat * kLnWidth;
+  L.b0 = o;
+  o += kLnWidth;
+  for (int l = 0; l < kLnLayers; ++l) {
+    L.w[l] = o;
+    o += (size_t)2 * kLnWidth * kLnWidth;
+    L.b[l] = o;
+    o += kLnWidth;
+  }
+  L.wo = o;
+  o += (size_t)kLnWidth * kLnOut;
+  L.bo = o;
+  o += kLnOut;
+  L.total = o;
+  return L;
+}
+
+namespace {
+
+// Y[v][j] = Σ_i X[v][i] M(i, j) (+ b[j]), X = [X1 (I1 columns) | X2 (I − I1)], M(i, j) =
+// M[i msi + j msj]; `relu`: Y = max(0, ·) with the pre-activation in `pre`.  M in LDS; a workgroup
+// takes 64 rows, four at a time (a thread per (row, column)).
+template <int I, int J>
+__global__ __launch_bounds__(256) void ln_rows_kernel(int n, const float* __restrict__ X1, int ld1, int I1,
+                                                      const float* __restrict__ X2, int ld2,
+                                                      const float* __restrict__ M, int msi, int msj,
+                                                      const float* __restrict__ bias, int relu, float* __restrict__ pre,
+                                                      float* __restrict__ Y, int ldy) {
+  __shared__ float sm[I * J];
+  __shared__ float sx[4][I];
+  const int tid = threadIdx.x, j = tid & 63, r = tid >> 6;
+  for (int e = tid; e < I * J; e += 256) sm[e] = M[(size_t)(e / J) * msi + (size_t)(e % J) * msj];
+  const int v1 = min(n, (int)blockIdx.x * 64 + 64);
+  for (int v0 = blockIdx.x * 64; v0 < v1; v0 += 4) {
+    __syncthreads();
+    for (int e = tid; e < 4 * I; e += 256) {
+      const int rr = e / I, i = e % I, v = v0 + rr;
+      sx[rr][i] = v < n ? (i < I1 ? X1[(size_t)v * ld1 + i] : X2[(size_t)v * ld2 + (i - I1)]) : 0.f;
+    }
+    __syncthreads();
+    const int v = v0 + r;
+    if (v < n && j < J) {
+      float acc = bias ? bias[j] : 0.f;
+#pragma unroll 8
+      for (int i = 0; i < I; ++i) acc = fmaf(sx[r][i], sm[i * J + j], acc);
+      if (pre) pre[(size_t)v * ldy + j] = acc;
+      Y[(size_t)v * ldy + j] = relu ? fmaxf(acc, 0.f) : acc;
+    }
+  }
+}
+
+// Partial weight gradients of one workgroup's rows: P[i][j] = Σ_v A[v][i] B[v][j] for i < I and,
+// as row I, the bias gradient Σ_v B[v][j]; A = [A1 (I1 columns) | A2].  Rows in chunks of 32 staged
+// in LDS; each thread keeps its outputs (t, t + 256, …) in registers.
+template <int I, int J>
+__global__ __launch_bounds__(256) void ln_wgrad_kernel(int n, int rows_per_block, const float* __restrict__ A1, int lda1,
+                                                       int I1, const float* __restrict__ A2, int lda2,
+                                                       const float* __restrict__ B, int ldb, float* __restrict__ part) {
+  constexpr int NO = (I + 1) * J, NQ = (NO + 255) / 256;
+  __shared__ float sa[32][I + 1];
+  __shared__ float sb[32][J];
+  const int tid = threadIdx.x;
+  float acc[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) acc[q] = 0.f;
+  const int r0 = blockIdx.x * rows_per_block, r1 = min(n, r0 + rows_per_block);
+  for (int c0 = r0; c0 < r1; c0 += 32) {
+    __syncthreads();
+    for (int e = tid; e < 32 * (I + 1); e += 256) {
+      const int rr = e / (I + 1), i = e % (I + 1), v = c0 + rr;
+      float x = 0.f;
+      if (v < r1) x = i == I ? 1.f : (i < I1 ? A1[(size_t)v * lda1 + i] : A2[(size_t)v * lda2 + (i - I1)]);
+      sa[rr][i] = x;
+    }
+    for (int e = tid; e < 32 * J; e += 256) {
+      const int rr = e / J, j = e % J, v = c0 + rr;
+      sb[rr][j] = v < r1 ? B[(size_t)v * ldb + j] : 0.f;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int o = tid + 256 * q;
+      if (o < NO) {
+        const int i = o / J, j = o % J;
+        float s = acc[q];
+        for (int rr = 0; rr < 32; ++rr) s = fmaf(sa[rr][i], sb[rr][j], s);
+        acc[q] = s;
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int o = tid + 256 * q;
+    if (o < NO) part[(size_t)blockIdx.x * NO + o] = acc[q];
+  }
+}
+
+// grad[w + i J + j] += Σ_b part[b][i][j] (i < I), grad[bias + j] += Σ_b part[b][I][j]; blocks in order.
+__global__ __launch_bounds__(256) void ln_wgrad_sum_kernel(int nb, int I, int J, const float* __restrict__ part,
+                                                           float* __restrict__ gw, float* __restrict__ gb) {
+  const int o = blockIdx.x * 256 + threadIdx.x, NO = (I + 1) * J;
+  if (o >= NO) return;
+  float s = 0.f;
+  for (int b = 0; b < nb; ++b) s += part[(size_t)b * NO + o];
+  if (o < I * J)
+    gw[o] += s;
+  else
+    gb[o - I * J] += s;
+}
+
+// m[v][c] = mean of h[u][c] over the neighbours u of v.
+__global__ __launch_bounds__(256) void ln_mean_kernel(int n, const uint32_t* __restrict__ ptr,
+                                                      const uint32_t* __restrict__ nb, const float* __restrict__ h,
+                                                      float* __restrict__ m) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= n * kLnWidth) return;
+  const int v = t / kLnWidth, c = t % kLnWidth;
+  const uint32_t q0 = ptr[v], q1 = ptr[v + 1];
+  float s = 0.f;
+  for (uint32_t q = q0; q < q1; ++q) s += h[(size_t)nb[q] * kLnWidth + c];
+  m[t] = q1 > q0 ? s / (float)(q1 - q0) : 0.f;
+}
+
+// dh[u][c] += Σ_{v ∈ N(u)} dm[v][c] / deg v (the neighbourhoods are symmetric).
+__global__ __launch_bounds__(256) void ln_mean_t_kernel(int n, const uint32_t* __restrict__ ptr,
+                                                        const uint32_t* __restrict__ nb, const float* __restrict__ dm,
+                                                        float* __restrict__ dh) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= n * kLnWidth) return;
+  const int u = t / kLnWidth, c = t % kLnWidth;
+  float s = 0.f;
+  for (uint32_t q = ptr[u]; q < ptr[u + 1]; ++q) {
+    const uint32_t v = nb[q];
+    s += dm[(size_t)v * kLnWidth + c] / (float)(ptr[v + 1] - ptr[v]);
+  }
+  dh[t] += s;
+}
+
+__global__ __launch_bounds__(256) void ln_relu_bwd_kernel(int n, const float* __restrict__ pre, float* __restrict__ d) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t < n && !(pre[t] > 0.f)) d[t] = 0.f;
+}
+
+__global__ __launch_bounds__(256) void ln_fill_kernel(size_t n, float* __restrict__ x) {
+  const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (t < n) x[t] = 0.f;
+}
+
+// q_v = s (B_v r_v + Σ_u ½ (g_v + g_u) r_u); d_v = 2 (q_v − e_v) / ‖e‖² into qd; per workgroup
+// Σ |q − e|² into lsum (FP64).
+__global__ __launch_bounds__(256) void ln_loss_kernel(int n, const uint32_t* __restrict__ ptr,
+                                                      const uint32_t* __restrict__ nb, const float* __restrict__ o,
+                                                      const float* __restrict__ r, const float* __restrict__ e,
+                                                      float s, float inv_e2, float* __restrict__ qd,
+                                                      double* __restrict__ lsum) {
+  __shared__ double red[256];
+  const int v = blockIdx.x * 256 + threadIdx.x;
+  double l = 0.0;
+  if (v < n) {
+    const float* ov = o + (size_t)v * kLnOut;
+    float q[3];
+    for (int a = 0; a < 3; ++a) {
+      float acc = 0.f;
+      for (int b = 0; b < 3; ++b) acc = fmaf(ov[3 * a + b], r[3 * v + b], acc);
+      q[a] = acc;
+    }
+    for (uint32_t k = ptr[v]; k < ptr[v + 1]; ++k) {
+      const uint32_t u = nb[k];
+      const float* ou = o + (size_t)u * kLnOut;
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) q[a] = fmaf(0.5f * (ov[9 + 3 * a + b] + ou[9 + 3 * a + b]), r[3 * u + b], q[a]);
+    }
+    for (int a = 0; a < 3; ++a) {
+      const float df = s * q[a] - e[3 * v + a];
+      qd[3 * v + a] = 2.f * df * inv_e2;
+      l += (double)df * (double)df;
+    }
+  }
+  red[threadIdx.x] = l;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) lsum[blockIdx.x] = red[0];
+}
+
+// dout[v] = s (d_v ⊗ r_v, ½ Σ_{u ∈ N(v)} (d_v ⊗ r_u + d_u ⊗ r_v)).
+__global__ __launch_bounds__(256) void ln_dout_kernel(int n, const uint32_t* __restrict__ ptr,
+                                                      const uint32_t* __restrict__ nb, const float* __restrict__ r,
+                                                      const float* __restrict__ qd, float s, float* __restrict__ dout) {
+  const int v = blockIdx.x * 256 + threadIdx.x;
+  if (v >= n) return;
+  float* d = dout + (size_t)v * kLnOut;
+  const float* dv = qd + 3 * (size_t)v;
+  const float* rv = r + 3 * (size_t)v;
+  float g[9] = {};
+  for (uint32_t k = ptr[v]; k < ptr[v + 1]; ++k) {
+    const uint32_t u = nb[k];
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) g[3 * a + b] += dv[a] * r[3 * (size_t)u + b] + qd[3 * (size_t)u + a] * rv[b];
+  }
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) {
+      d[3 * a + b] = s * dv[a] * rv[b];
+      d[9 + 3 * a + b] = 0.5f * s * g[3 * a + b];
+    }
+}
+
+// Adam (β₁ 0.9, β₂ 0.999, ε 1e-8) on g / batch, then g = 0.
+__global__ __launch_bounds__(256) void ln_adam_kernel(size_t n, float* __restrict__ p, float* __restrict__ g,
+                                                      float* __restrict__ m, float* __restrict__ v, float lr,
+                                                      float inv_batch, float c1, float c2) {
+  const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  const float gt = g[t] * inv_batch;
+  const float mt = 0.9f * m[t] + 0.1f * gt, vt = 0.999f * v[t] + 0.001f * gt * gt;
+  m[t] = mt;
+  v[t] = vt;
+  p[t] -= lr * (mt / c1) / (sqrtf(vt / c2) + 1e-8f);
+  g[t] = 0.f;
+}
+
+// z_v += s (B_v r_v + Σ_u ½ (g_v + g_u) r_u) (FP64 sums of the FP32 blocks).
+__global__ __launch_bounds__(256) void ln_apply_kernel(int n, const uint32_t* __restrict__ ptr,
+                                                       const uint32_t* __restrict__ nb, const float* __restrict__ o,
+                                                       double s, const double* __restrict__ r, double* __restrict__ z) {
+  const int v = blockIdx.x * 256 + threadIdx.x;
+  if (v >= n) return;
+  const float* ov = o + (size_t)v * kLnOut;
+  double q[3];
+  for (int a = 0; a < 3; ++a) {
+    double acc = 0.0;
+    for (int b = 0; b < 3; ++b) acc += (double)ov[3 * a + b] * r[3 * (size_t)v + b];
+    q[a] = acc;
+  }
+  for (uint32_t k = ptr[v]; k < ptr[v + 1]; ++k) {
+    const uint32_t u = nb[k];
+    const float* ou = o + (size_t)u * kLnOut;
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) q[a] += 0.5 * ((double)ov[9 + 3 * a + b] + (double)ou[9 + 3 * a + b]) * r[3 * (size_t)u + b];
+  }
+  for (int a = 0; a < 3; ++a) z[3 * (size_t)v + a] += s * q[a];
+}
+
+template <int I, int J>
+void rows(ddgrs_ctx* ctx, int n, const float* X1, int ld1, int I1, const float* X2, int ld2, const float* M, int msi,
+          int msj, const float* bias, int relu, float* pre, float* Y, int ldy) {
+  launch(ctx, "ln_rows", ln_rows_kernel<I, J>, dim3(blocks_for(n, 64)), dim3(256), 0, n, X1, ld1, I1, X2, ld2, M, msi,
+         msj, bias, relu, pre, Y, ldy);
+}
+
+constexpr int kGradBlocks = 128;
+
+template <int I, int J>
+void wgrad(ddgrs_ctx* ctx, int n, const float* A1, int lda1, int I1, const float* A2, int lda2, const float* B, int ldb,
+           float* part, float* gw, float* gb) {
+  const int per = (n + kGradBlocks - 1) / kGradBlocks;
+  launch(ctx, "ln_wgrad", ln_wgrad_kernel<I, J>, dim3(kGradBlocks), dim3(256), 0, n, per, A1, lda1, I1, A2, lda2, B, ldb,
+         part);
+  launch(ctx, "ln_wgrad_sum", ln_wgrad_sum_kernel, dim3(blocks_for((I + 1) * J, 256)), dim3(256), 0, kGradBlocks, I, J,
+         (const float*)part, gw, gb);
+}
+
+}  // namespace
+
+void LearnedModel::init(ddgrs_ctx* ctx, const std::vector<uint32_t>& f, int n, uint64_t seed) {
+  faces = f;
+  nv = n;
+  // neighbours: the mesh's edges, both directions, ascending
+  std::vector<std::vector<uint32_t>> adj(nv);
+  for (size_t t = 0; t + 2 < faces.size(); t += 3)
+    for (int c = 0; c < 3; ++c) {
+      const uint32_t a = faces[t + c], b = faces[t + (c + 1) % 3];
+      adj[a].push_back(b);
+      adj[b].push_back(a);
+    }
+  nbr_ptr_h.assign(1, 0);
+  nbr_h.clear();
+  for (int v = 0; v < nv; ++v) {
+    std::sort(adj[v].begin(), adj[v].end());
+    adj[v].erase(std::unique(adj[v].begin(), adj[v].end()), adj[v].end());
+    nbr_h.insert(nbr_h.end(), adj[v].begin(), adj[v].end());
+    nbr_ptr_h.push_back((uint32_t)nbr_h.size());
+  }
+  upload(ctx, nbr_ptr, nbr_ptr_h);
+  upload(ctx, nbr, nbr_h);
+  // parameters: hidden layers uniform ±√(6 / (fan_in + fan_out)) from a fixed 64-bit LCG, biases 0;
+  // the output layer 0 (Q = 0 before training)
+  const LearnedLayout L = learned_layout();
+  std::vector<float> p(L.total, 0.f);
+  uint64_t st = seed * 6364136223846793005ull + 1442695040888963407ull;
+  auto uni = [&]() {
+    st = st * 6364136223846793005ull + 1442695040888963407ull;
+    return (double)(st >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
+  };
+  auto fill = [&](size_t off, size_t count, int fan_in, int fan_out) {
+    const double a = std::sqrt(6.0 / (fan_in + fan_out));
+    for (size_t i = 0; i < count; ++i) p[off + i] = (float)(a * uni());
+  };
+  fill(L.w0, (size_t)kLnFeat * kLnWidth, kLnFeat, kLnWidth);
+  for (int l = 0; l < kLnLayers; ++l) fill(L.w[l], (size_t)2 * kLnWidth * kLnWidth, 2 * kLnWidth, kLnWidth);
+  upload(ctx, params, p);
+  std::vector<float> z(L.total, 0.f);
+  upload(ctx, grad, z);
+  upload(ctx, adam_m, z);
+  upload(ctx, adam_v, z);
+  gpart.reserve((size_t)kGradBlocks * (2 * kLnWidth + 1) * kLnWidth);
+  feat.reserve((size_t)nv * kLnFeat);
+  act.reserve((size_t)(kLnLayers + 1) * nv * kLnWidth);
+  pre.reserve((size_t)(kLnLayers + 1) * nv * kLnWidth);
+  mean.reserve((size_t)kLnLayers * nv * kLnWidth);
+  out.reserve((size_t)nv * kLnOut);
+  dout.reserve((size_t)nv * kLnOut);
+  dh.reserve((size_t)nv * kLnWidth);
+  dtmp.reserve((size_t)2 * nv * kLnWidth);
+  qd.reserve(3 * (size_t)nv);
+  lsum.reserve(blocks_for(nv, 256));
+  adam_t = 0;
+  blocks_ready = false;
+}
+
+void LearnedModel::features(ddgrs_ctx* ctx, const double* pos, float* dst) {
+  const size_t n = (size_t)nv;
+  double c[3] = {0, 0, 0};
+  for (size_t v = 0; v < n; ++v)
+    for (int k = 0; k < 3; ++k) c[k] += pos[3 * v + k];
+  for (int k = 0; k < 3; ++k) c[k] /= (double)n;
+  double rr = 0.0;
+  for (size_t v = 0; v < n; ++v)
+    for (int k = 0; k < 3; ++k) rr += (pos[3 * v + k] - c[k]) * (pos[3 * v + k] - c[k]);
+  const double R = std::sqrt(rr / (double)n) + 1e-300;
+  std::vector<double> vn(3 * n, 0.0), area(n, 0.0);
+  for (size_t t = 0; t + 2 < faces.size(); t += 3) {
+    const double* p0 = pos + 3 * (size_t)faces[t];
+    const double* p1 = pos + 3 * (size_t)faces[t + 1];
+    const double* p2 = pos + 3 * (size_t)faces[t + 2];
+    const double e1[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+    const double e2[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+    const double nf[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+    const double a2 = std::sqrt(nf[0] * nf[0] + nf[1] * nf[1] + nf[2] * nf[2]);
+    for (int c2 = 0; c2 < 3; ++c2) {
+      const size_t v = faces[t + c2];
+      for (int k = 0; k < 3; ++k) vn[3 * v + k] += nf[k];
+      area[v] += a2 / 6.0;
+    }
+  }
+  double ma = 0.0, el = 0.0;
+  size_t ne = 0;
+  for (size_t v = 0; v < n; ++v) {
+    ma += area[v];
+    for (uint32_t q = nbr_ptr_h[v]; q < nbr_ptr_h[v + 1]; ++q) {
+      const size_t u = nbr_h[q];
+      double d2 = 0.0;
+      for (int k = 0; k < 3; ++k) d2 += (pos[3 * u + k] - pos[3 * v + k]) * (pos[3 * u + k] - pos[3 * v + k]);
+      el += d2;
+      ++ne;
+    }
+  }
+  ma /= (double)n;
+  el = ne ? el / (double)ne : 1.0;
+  float* f = (float*)ctx->arena_take(n * kLnFeat * sizeof(float));
+  for (size_t v = 0; v < n; ++v) {
+    double nn = std::sqrt(vn[3 * v] * vn[3 * v] + vn[3 * v + 1] * vn[3 * v + 1] + vn[3 * v + 2] * vn[3 * v + 2]);
+    nn = nn > 0.0 ? nn : 1.0;
+    double lap[3] = {0, 0, 0};
+    const uint32_t q0 = nbr_ptr_h[v], q1 = nbr_ptr_h[v + 1];
+    for (uint32_t q = q0; q < q1; ++q)
+      for (int k = 0; k < 3; ++k) lap[k] += pos[3 * (size_t)nbr_h[q] + k];
+    double h = 0.0;
+    for (int k = 0; k < 3; ++k) {
+      lap[k] = q1 > q0 ? lap[k] / (double)(q1 - q0) - pos[3 * v + k] : 0.0;
+      h += lap[k] * vn[3 * v + k] / nn;
+    }
+    float* o = f + v * kLnFeat;
+    for (int k = 0; k < 3; ++k) {
+      o[k] = (float)((pos[3 * v + k] - c[k]) / R);
+      o[3 + k] = (float)(vn[3 * v + k] / nn);
+    }
+    o[6] = (float)(area[v] / ma);
+    o[7] = (float)(h / el * R);
+  }
+  HIPCHECK(hipMemcpyAsync(dst, f, n * kLnFeat * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+}
+
+void LearnedModel::forward(ddgrs_ctx* ctx, const float* f) {
+  const LearnedLayout L = learned_layout();
+  const size_t VW = (size_t)nv * kLnWidth;
+  const float* P = params.ptr;
+  rows<kLnFeat, kLnWidth>(ctx, nv, f, kLnFeat, kLnFeat, nullptr, 0, P + L.w0, kLnWidth, 1, P + L.b0, 1, pre.ptr,
+                          act.ptr, kLnWidth);
+  for (int l = 0; l < kLnLayers; ++l) {
+    const float* h = act.ptr + l * VW;
+    float* m = mean.ptr + l * VW;
+    launch(ctx, "ln_mean", ln_mean_kernel, dim3(blocks_for((int64_t)VW, 256)), dim3(256), 0, nv,
+           (const uint32_t*)nbr_ptr.ptr, (const uint32_t*)nbr.ptr, h, m);
+    rows<2 * kLnWidth, kLnWidth>(ctx, nv, h, kLnWidth, kLnWidth, m, kLnWidth, P + L.w[l], kLnWidth, 1, P + L.b[l], 1,
+                                 pre.ptr + (l + 1) * VW, act.ptr + (l + 1) * VW, kLnWidth);
+  }
+  rows<kLnWidth, kLnOut>(ctx, nv, act.ptr + kLnLayers * VW, kLnWidth, kLnWidth, nullptr, 0, P + L.wo, kLnOut, 1,
+                         P + L.bo, 0, nullptr, out.ptr, kLnOut);
+  blocks_ready = true;
+}
+
+void LearnedModel::add_pair(ddgrs_ctx* ctx, const double* pos, const double* r, const double* pr, const double* y) {
+  if (npairs >= cap_pairs) throw Error(DDGRS_INVALID_OPTION, "learned_add_pair", "the model holds no more pairs");
+  const size_t n3 = 3 * (size_t)nv;
+  features(ctx, pos, pair_feat.ptr + (size_t)npairs * nv * kLnFeat);
+  float* h = (float*)ctx->arena_take(2 * n3 * sizeof(float));
+  double e2 = 0.0, r2 = 0.0;
+  for (size_t i = 0; i < n3; ++i) {
+    const double e = y[i] - pr[i];
+    h[i] = (float)r[i];
+    h[n3 + i] = (float)e;
+    e2 += e * e;
+    r2 += r[i] * r[i];
+  }
+  HIPCHECK(hipMemcpyAsync(pair_r.ptr + npairs * n3, h, n3 * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHECK(hipMemcpyAsync(pair_e.ptr + npairs * n3, h + n3, n3 * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  pair_e2.push_back(e2 > 0.0 ? e2 : 1.0);
+  pair_ratio.push_back(r2 > 0.0 ? std::sqrt(e2 / r2) : 0.0);
+  ++npairs;
+}
+
+double LearnedModel::loss_and_grad(ddgrs_ctx* ctx, int p, bool add_grad, bool read) {
+  const LearnedLayout L = learned_layout();
+  const size_t VW = (size_t)nv * kLnWidth, n3 = 3 * (size_t)nv;
+  const float* f = pair_feat.ptr + (size_t)p * nv * kLnFeat;
+  const float* r = pair_r.ptr + (size_t)p * n3;
+  const float* e = pair_e.ptr + (size_t)p * n3;
+  forward(ctx, f);
+  const unsigned nbv = blocks_for(nv, 256);
+  launch(ctx, "ln_loss", ln_loss_kernel, dim3(nbv), dim3(256), 0, nv, (const uint32_t*)nbr_ptr.ptr,
+         (const uint32_t*)nbr.ptr, (const float*)out.ptr, r, e, scale, (float)(1.0 / pair_e2[p]), qd.ptr, lsum.ptr);
+  if (add_grad) {
+    float* G = grad.ptr;
+    const float* P = params.ptr;
+    launch(ctx, "ln_dout", ln_dout_kernel, dim3(nbv), dim3(256), 0, nv, (const uint32_t*)nbr_ptr.ptr,
+           (const uint32_t*)nbr.ptr, r, (const float*)qd.ptr, scale, dout.ptr);
+    const float* h4 = act.ptr + kLnLayers * VW;
+    wgrad<kLnWidth, kLnOut>(ctx, nv, h4, kLnWidth, kLnWidth, nullptr, 0, dout.ptr, kLnOut, gpart.ptr, G + L.wo, G + L.bo);
+    // dh = dout W_oᵀ
+    rows<kLnOut, kLnWidth>(ctx, nv, dout.ptr, kLnOut, kLnOut, nullptr, 0, P + L.wo, 1, kLnOut, nullptr, 0, nullptr,
+                           dh.ptr, kLnWidth);
+    for (int l = kLnLayers; l >= 1; --l) {
+      float* dpre = dh.ptr;  // dh ⊙ (pre_l > 0), in place
+      launch(ctx, "ln_relu_bwd", ln_relu_bwd_kernel, dim3(blocks_for((int64_t)VW, 256)), dim3(256), 0, (int)VW,
+             (const float*)(pre.ptr + l * VW), dpre);
+      const float* hp = act.ptr + (l - 1) * VW;
+      const float* mp = mean.ptr + (l - 1) * VW;
+      wgrad<2 * kLnWidth, kLnWidth>(ctx, nv, hp, kLnWidth, kLnWidth, mp, kLnWidth, dpre, kLnWidth, gpart.ptr,
+                                    G + L.w[l - 1], G + L.b[l - 1]);
+      // dh_{l−1} = dpre W_sᵀ + meanᵀ(dpre W_nᵀ)
+      float* dnext = dtmp.ptr;
+      float* dm = dtmp.ptr + VW;
+      rows<kLnWidth, kLnWidth>(ctx, nv, dpre, kLnWidth, kLnWidth, nullptr, 0, P + L.w[l - 1], 1, kLnWidth, nullptr, 0,
+                               nullptr, dnext, kLnWidth);
+      rows<kLnWidth, kLnWidth>(ctx, nv, dpre, kLnWidth, kLnWidth, nullptr, 0, P + L.w[l - 1] + kLnWidth * kLnWidth, 1,
+                               kLnWidth, nullptr, 0, nullptr, dm, kLnWidth);
+      launch(ctx, "ln_mean_t", ln_mean_t_kernel, dim3(blocks_for((int64_t)VW, 256)), dim3(256), 0, nv,
+             (const uint32_t*)nbr_ptr.ptr, (const uint32_t*)nbr.ptr, (const float*)dm, dnext);
+      HIPCHECK(hipMemcpyAsync(dh.ptr, dnext, VW * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    launch(ctx, "ln_relu_bwd", ln_relu_bwd_kernel, dim3(blocks_for((int64_t)VW, 256)), dim3(256), 0, (int)VW,
+           (const float*)pre.ptr, dh.ptr);
+    wgrad<kLnFeat, kLnWidth>(ctx, nv, f, kLnFeat, kLnFeat, nullptr, 0, dh.ptr, kLnWidth, gpart.ptr, G + L.w0, G + L.b0);
+  }
+  if (!read) return 0.0;
+  std::vector<double> ls(nbv);
+  download(ctx, ls.data(), (const double*)lsum.ptr, nbv);
+  ctx->sync();
+  double s = 0.0;
+  for (double x : ls) s += x;
+  return s / pair_e2[p];
+}
+
+void LearnedModel::train(ddgrs_ctx* ctx, int iters, int batch, double lr, double* before, double* after) {
+  if (npairs == 0) throw Error(DDGRS_INVALID_OPTION, "learned_train", "no training pairs");
+  if (scale == 0.f) {
+    std::vector<double> q = pair_ratio;
+    std::nth_element(q.begin(), q.begin() + q.size() / 2, q.end());
+    scale = (float)(q[q.size() / 2] > 0.0 ? q[q.size() / 2] : 1.0);
+  }
+  const LearnedLayout L = learned_layout();
+  auto mean_loss = [&]() {
+    double s = 0.0;
+    for (int p = 0; p < npairs; ++p) s += loss_and_grad(ctx, p, false);
+    return s / npairs;
+  };
+  if (before) *before = mean_loss();
+  launch(ctx, "ln_fill", ln_fill_kernel, dim3(blocks_for((int64_t)L.total, 256)), dim3(256), 0, L.total, grad.ptr);
+  int next = 0;
+  for (int it = 0; it < iters; ++it) {
+    for (int b = 0; b < batch; ++b) {
+      (void)loss_and_grad(ctx, next, true, false);
+      next = (next + 1) % npairs;
+    }
+    ++adam_t;
+    const float c1 = (float)(1.0 - std::pow(0.9, (double)adam_t)), c2 = (float)(1.0 - std::pow(0.999, (double)adam_t));
+    launch(ctx, "ln_adam", ln_adam_kernel, dim3(blocks_for((int64_t)L.total, 256)), dim3(256), 0, L.total, params.ptr,
+           grad.ptr, adam_m.ptr, adam_v.ptr, (float)lr, 1.f / (float)batch, c1, c2);
+  }
+  if (after) *after = mean_loss();
+  blocks_ready = false;
+}
+
+void LearnedModel::apply(ddgrs_ctx* ctx, const double* r, double* z) {
+  launch(ctx, "ln_apply", ln_apply_kernel, dim3(blocks_for(nv, 256)), dim3(256), 0, nv, (const uint32_t*)nbr_ptr.ptr,
+         (const uint32_t*)nbr.ptr, (const float*)out.ptr, (double)scale, r, z);
+}
+
+}  // namespace ddgrs
+
+// ---- C ABI -------------------------------------------------------------------------------------
+
+struct ddgrs_learned {
+  ddgrs_surface* s = nullptr;
+  ddgrs::LearnedModel m;
+};
+
+using namespace ddgrs;
+
+LearnedModel* ddgrs::learned_model(ddgrs_learned* l) { return &l->m; }
+
+namespace {
+void check_positions(ddgrs_learned* l, const double* pos) {
+  need(pos, "positions");
+  const size_t n = 3 * (size_t)l->s->nv;
+  for (size_t i = 0; i < n; ++i)
+    if (!std::isfinite(pos[i])) {
+      Error e(DDGRS_NON_FINITE_POSITION, "learned", "a position is not finite");
+      e.a = (int64_t)(i / 3);
+      throw e;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+int ddgrs_learned_create(ddgrs_surface* s, uint64_t seed, int64_t max_pairs, ddgrs_learned** out, ddgrs_error* err) {
+  return guarded(err, "learned_create", [&] {
+    need(s, "surface");
+    need(out, "out");
+    *out = nullptr;
+    if (max_pairs < 0) invalid("max_pairs", std::to_string(max_pairs), "non-negative");
+    HIPCHECK(hipSetDevice(s->ctx->device));
+    auto* l = new ddgrs_learned;
+    l->s = s;
+    surface_retain(s);
+    try {
+      LearnedModel& m = l->m;
+      m.init(s->ctx, s->faces_h, (int)s->nv, seed);
+      const size_t nv = (size_t)s->nv, cap = (size_t)max_pairs;
+      m.cap_pairs = (int)cap;
+      m.pair_feat.reserve(std::max<size_t>(1, cap) * nv * kLnFeat);
+      m.pair_r.reserve(std::max<size_t>(1, cap) * 3 * nv);
+      m.pair_e.reserve(std::max<size_t>(1, cap) * 3 * nv);
+      m.feat.reserve(nv * kLnFeat);
+      s->ctx->sync();
+    } catch (...) {
+      destroy_handle(l);
+      throw;
+    }
+    *out = l;
+  });
+}
+
+void ddgrs_learned_destroy(ddgrs_learned* l) { destroy_handle(l); }
+
+int ddgrs_learned_info(ddgrs_learned* l, int64_t* out4, double* scale, ddgrs_error* err) {
+  return guarded(err, "learned_info", [&] {
+    need(l, "learned");
+    if (out4) {
+      out4[0] = (int64_t)learned_layout().total;
+      out4[1] = l->m.npairs;
+      out4[2] = l->m.adam_t;
+      out4[3] = kLnFeat;
+    }
+    if (scale) *scale = l->m.scale;
+  });
+}
+
+int ddgrs_learned_params(ddgrs_learned* l, float* get, const float* set, ddgrs_error* err) {
+  return guarded(err, "learned_params", [&] {
+    need(l, "learned");
+    ddgrs_ctx* ctx = l->s->ctx;
+    HIPCHECK(hipSetDevice(ctx->device));
+    const size_t n = learned_layout().total;
+    if (get) {
+      download(ctx, get, (const float*)l->m.params.ptr, n);
+      ctx->sync();
+    }
+    if (set) {
+      for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(set[i])) invalid("params", "a non-finite value", "finite parameters");
+      upload(ctx, l->m.params, set, n);
+      l->m.blocks_ready = false;
+      ctx->sync();
+    }
+  });
+}
+
+int ddgrs_learned_set_scale(ddgrs_learned* l, double scale, ddgrs_error* err) {
+  return guarded(err, "learned_set_scale", [&] {
+    need(l, "learned");
+    if (!(std::isfinite(scale) && scale > 0.0)) invalid("scale", std::to_string(scale), "a finite number > 0");
+    l->m.scale = (float)scale;
+  });
+}
+
+int ddgrs_learned_add_pair(ddgrs_learned* l, const double* positions, const double* r, const double* pr,
+                           const double* y, ddgrs_error* err) {
+  return guarded(err, "learned_add_pair", [&] {
+    need(l, "learned");
+    need(r, "r");
+    need(pr, "pr");
+    need(y, "y");
+    check_positions(l, positions);
+    ddgrs_ctx* ctx = l->s->ctx;
+    HIPCHECK(hipSetDevice(ctx->device));
+    l->m.add_pair(ctx, positions, r, pr, y);
+    ctx->sync();
+  });
+}
+
+int ddgrs_learned_features(ddgrs_learned* l, const double* positions, float* feat, ddgrs_error* err) {
+  return guarded(err, "learned_features", [&] {
+    need(l, "learned");
+    need(feat, "feat");
+    check_positions(l, positions);
+    ddgrs_ctx* ctx = l->s->ctx;
+    HIPCHECK(hipSetDevice(ctx->device));
+    l->m.features(ctx, positions, l->m.feat.ptr);
+    download(ctx, feat, (const float*)l->m.feat.ptr, (size_t)l->s->nv * kLnFeat);
+    ctx->sync();
+  });
+}
+
+int ddgrs_learned_blocks(ddgrs_learned* l, const double* positions, float* blocks, ddgrs_error* err) {
+  return guarded(err, "learned_blocks", [&] {
+    need(l, "learned");
+    check_positions(l, positions);
+    ddgrs_ctx* ctx = l->s->ctx;
+    HIPCHECK(hipSetDevice(ctx->device));
+    l->m.features(ctx, positions, l->m.feat.ptr);
+    l->m.forward(ctx, l->m.feat.ptr);
+    if (blocks) download(ctx, blocks, (const float*)l->m.out.ptr, (size_t)l->s->nv * kLnOut);
+    ctx->sync();
+  });
+}
+
+int ddgrs_learned_loss_grad(ddgrs_learned* l, int64_t pair, double* loss, float* grad, ddgrs_error* err) {
+  return guarded(err, "learned_loss_grad", [&] {
+    need(l, "learned");
+    if (pair < 0 || pair >= l->m.npairs) invalid("pair", std::to_string(pair), "an index of an added pair");
+    if (!(l->m.scale > 0.f)) invalid("scale", "unset", "a scale (learned_set_scale or a training run)");
+    ddgrs_ctx* ctx = l->s->ctx;
+    HIPCHECK(hipSetDevice(ctx->device));
+    const size_t n = learned_layout().total;
+    if (grad) launch(ctx, "ln_fill", ln_fill_kernel, dim3(blocks_for((int64_t)n, 256)), dim3(256), 0, n, l->m.grad.ptr);
+    const double v = l->m.loss_and_grad(ctx, (int)pair, grad != nullptr);
+    if (loss) *loss = v;
+    if (grad) {
+      download(ctx, grad, (const float*)l->m.grad.ptr, n);
+      launch(ctx, "ln_fill", ln_fill_kernel, dim3(blocks_for((int64_t)n, 256)), dim3(256), 0, n, l->m.grad.ptr);
+    }
+    ctx->sync();
+    l->m.blocks_ready = false;
+  });
+}
+
+int ddgrs_learned_train(ddgrs_learned* l, int64_t iterations, int64_t batch, double lr, double* loss2,
+                        ddgrs_error* err) {
+  return guarded(err, "learned_train", [&] {
+    need(l, "learned");
+    if (iterations < 0) invalid("iterations", std::to_string(iterations), "non-negative");
+    if (batch < 1) invalid("batch", std::to_string(batch), "at least 1");
+    if (!(std::isfinite(lr) && lr > 0.0)) invalid("lr", std::to_string(lr), "a finite number > 0");
+    ddgrs_ctx* ctx = l->s->ctx;
+    HIPCHECK(hipSetDevice(ctx->device));
+    double b = 0.0, a = 0.0;
+    l->m.train(ctx, (int)iterations, (int)batch, lr, &b, &a);
+    ctx->sync();
+    if (loss2) {
+      loss2[0] = b;
+      loss2[1] = a;
+    }
+  });
+}
+
+int ddgrs_learned_apply_batch(ddgrs_learned* l, const double* positions, int64_t nb, const double* rs, double* zs,
+                              ddgrs_error* err) {
+  return guarded(err, "learned_apply_batch", [&] {
+    need(l, "learned");
+    need(rs, "rs");
+    need(zs, "zs");
+    if (nb < 0) invalid("nb", std::to_string(nb), "non-negative");
+    if (!(l->m.scale > 0.f)) invalid("scale", "unset", "a scale (learned_set_scale or a training run)");
+    check_positions(l, positions);
+    ddgrs_ctx* ctx = l->s->ctx;
+    HIPCHECK(hipSetDevice(ctx->device));
+    LearnedModel& m = l->m;
+    m.features(ctx, positions, m.feat.ptr);
+    m.forward(ctx, m.feat.ptr);
+    const size_t n = 3 * (size_t)l->s->nv + 3;
+    DevBuf<double> r(n), z(n);
+    for (int64_t k = 0; k < nb; ++k) {
+      HIPCHECK(hipMemcpyAsync(r.ptr, rs + k * n, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+      HIPCHECK(hipMemsetAsync(z.ptr, 0, n * sizeof(double), ctx->stream));
+      m.apply(ctx, r.ptr, z.ptr);
+      download(ctx, zs + k * n, (const double*)z.ptr, n);
+      ctx->sync();
+    }
+    m.blocks_ready = false;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,63 @@
+// The learned right preconditioner layered on the plain one: a mesh network maps per-vertex
+// geometry features to per-vertex 3×3 blocks B_v and per-vertex 3×3 factors g_v, and
+//   (Q r)_v = s (B_v r_v + Σ_{u ∈ N(v)} ½ (g_v + g_u) r_u)
+// is added to the plain preconditioner's output (z = P⁻¹ r + Q r), so GMRES stays a Krylov method
+// with a linear preconditioner.  Q is trained on pairs (r, y = A′⁻¹ r) from plain solves along
+// descent trajectories to reduce ‖P⁻¹ r + Q r − y‖² / ‖y − P⁻¹ r‖²; it starts at zero (the output
+// layer's weights are zero), where the combined preconditioner is the plain one.
+//
+// Network (FP32): h₀ = relu(F W₀ + b₀) (8 features → 64); four message-passing layers
+// h_{l+1} = relu(h_l W_s + m_l W_n + b), m_l = the mean of h_l over the vertex's neighbours;
+// heads o = h₄ W_o + b_o (64 → 18: B_v, g_v).  Training: forward, backward and Adam as kernels on
+// the device; gradients are checked against a NumPy restatement in the tests.
+#pragma once
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace ddgrs {
+
+constexpr int kLnFeat = 8, kLnWidth = 64, kLnLayers = 4, kLnOut = 18;
+
+// Offsets of the parameters in one FP32 array.
+struct LearnedLayout {
+  size_t w0, b0, w[kLnLayers], b[kLnLayers], wo, bo, total;  // w[l]: [W_s; W_n] (128 × 64)
+};
+LearnedLayout learned_layout();
+
+struct LearnedModel {
+  int nv = 0;
+  std::vector<uint32_t> faces;          // the mesh (3 per face)
+  std::vector<uint32_t> nbr_ptr_h, nbr_h;  // neighbours (mesh edges), ascending
+  DevBuf<uint32_t> nbr_ptr, nbr;
+  DevBuf<float> params, grad, adam_m, adam_v, gpart;
+  DevBuf<float> feat, act, pre, mean, out, dout, dh, dtmp, qd;  // activations and their gradients
+  DevBuf<double> lsum;                   // loss partial sums
+  // training pairs (features, r, e = y − P⁻¹ r: FP32 per vertex) and ‖e‖²
+  DevBuf<float> pair_feat, pair_r, pair_e;
+  std::vector<double> pair_e2, pair_ratio;
+  int npairs = 0, cap_pairs = 0;
+  float scale = 0.f;  // s (set from the pairs at the first training)
+  int64_t adam_t = 0;
+  bool blocks_ready = false;  // `out` holds the blocks of the last forward
+
+  void init(ddgrs_ctx* ctx, const std::vector<uint32_t>& faces, int nv, uint64_t seed);
+  // Features at positions (3 nv FP64, host) into `dst` (device, nv × 8).
+  void features(ddgrs_ctx* ctx, const double* pos, float* dst);
+  // Blocks at the given features (device) into `out` (nv × 18).
+  void forward(ddgrs_ctx* ctx, const float* f);
+  void add_pair(ddgrs_ctx* ctx, const double* pos, const double* r, const double* pr, const double* y);
+  // Loss of pair p at the current parameters (read: returned, after a synchronization; else 0)
+  // and (add_grad) its gradient added into `grad`.
+  double loss_and_grad(ddgrs_ctx* ctx, int p, bool add_grad, bool read = true);
+  // Adam over the pairs (minibatches of `batch`, `iters` updates); returns the mean loss over all
+  // pairs before and after.
+  void train(ddgrs_ctx* ctx, int iters, int batch, double lr, double* before, double* after);
+  // z += Q r for a full saddle vector (3 nv + 3, device; the blocks of the last forward).
+  void apply(ddgrs_ctx* ctx, const double* r, double* z);
+};
+
+// The model of a C-ABI handle.
+LearnedModel* learned_model(ddgrs_learned* l);
+
+}  // namespace ddgrs

@@ -296,8 +296,40 @@ void Session::step(StepReport* rep) {
     fm.extended = pc.vf.extended = true;
     return true;
   };
-  GmresResult gr = gmres_device(ctx, n, A, P, g.ptr, sol.ptr, gs, vec, stall);
+  const int lmode = learned ? learned_mode : 0;
+  rep->learned_mode = lmode;
+  GmresResult gr;
+  bool solved = false;
+  if (lmode == 2) {
+    // P⁻¹ + Q: the network's blocks at this step's positions, then the solve; on a miss the
+    // plain solve below from x₀ = 0
+    const auto tl = std::chrono::steady_clock::now();
+    learned->features(ctx, geo.pos.data(), learned->feat.ptr);
+    learned->forward(ctx, learned->feat.ptr);
+    rep->seconds_learned = seconds_since(tl);
+    auto PQ = [&](const double* r, double* z) {
+      pc.apply(ctx, r, z);
+      learned->apply(ctx, r, z);
+    };
+    gr = gmres_device(ctx, n, A, PQ, g.ptr, sol.ptr, gs, vec, stall);
+    fm.extended = pc.vf.extended = false;
+    rep->learned_iterations = (int64_t)gr.iterations;
+    solved = gr.converged;
+    rep->learned_fallback = solved ? 0 : 1;
+  }
+  if (!solved) gr = gmres_device(ctx, n, A, P, g.ptr, sol.ptr, gs, vec, stall);
   fm.extended = pc.vf.extended = false;
+  if (lmode == 1 && gr.converged) {
+    // the pair (g, P⁻¹ g, A′⁻¹ g) at this step's positions
+    pg.reserve(n);
+    pc.apply(ctx, g.ptr, pg.ptr);
+    std::vector<double> hg(n), hp(n), hy(n);
+    download(ctx, hg.data(), (const double*)g.ptr, n);
+    download(ctx, hp.data(), (const double*)pg.ptr, n);
+    download(ctx, hy.data(), (const double*)sol.ptr, n);
+    ctx->sync();
+    if (learned->npairs < learned->cap_pairs) learned->add_pair(ctx, geo.pos.data(), hg.data(), hp.data(), hy.data());
+  }
   rep->seconds_solve = seconds_since(t);
   rep->gmres.iterations = (int64_t)gr.iterations;
   rep->gmres.cycles = (int64_t)gr.cycles;

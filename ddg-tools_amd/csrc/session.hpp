@@ -6,6 +6,7 @@
 
 #include "bh.hpp"
 #include "gmres.hpp"
+#include "learned.hpp"
 #include "lprime_gpu.hpp"
 
 namespace ddgrs {
@@ -48,6 +49,10 @@ struct Session {
   DeviceVecs vec;
   GmresResult last_gmres;
   DevBuf<double> g, sol, p0, partials, rows, scal, mwork, xpart;
+  // the learned preconditioner: mode 1 records pairs, 2 solves with P⁻¹ + Q (plain on a miss)
+  LearnedModel* learned = nullptr;
+  int learned_mode = 0;
+  DevBuf<double> pg;  // P⁻¹ g of a recorded pair
 
   Session() = default;
   Session(const Session&) = delete;

@@ -60,6 +60,7 @@ typedef struct ddgrs_metric ddgrs_metric;
 typedef struct ddgrs_vfrac ddgrs_vfrac;
 typedef struct ddgrs_saddle ddgrs_saddle;
 typedef struct ddgrs_session ddgrs_session;
+typedef struct ddgrs_learned ddgrs_learned;
 
 /* ---- context ------------------------------------------------------------------------------- */
 
@@ -222,6 +223,13 @@ typedef struct ddgrs_step_report {
   int64_t pairs_exact, nodes_lumped;
   double seconds_trees, seconds_energy, seconds_metric, seconds_precond, seconds_solve, seconds_line_search,
       seconds_projection, seconds_total;
+  /* The learned preconditioner (no counterpart in the reference): the session's mode (0 plain,
+   * 1 plain solve recording a training pair, 2 P⁻¹ + Q), whether a P⁻¹ + Q solve missed the
+   * tolerance within the iteration cap and the plain solve's result was used (then `gmres` is the
+   * plain solve's report), and the iterations of the P⁻¹ + Q solve. */
+  int32_t learned_mode, learned_fallback;
+  int64_t learned_iterations;
+  double seconds_learned; /* features and the network's forward pass */
 } ddgrs_step_report;
 
 /* Fills the reference defaults ((6, 12), Barnes–Hut θ 0.5, restart 50, 1e-6, 200, c 1e-2). */
@@ -236,6 +244,47 @@ DDGRS_API void ddgrs_session_destroy(ddgrs_session* h);
 DDGRS_API int ddgrs_session_step(ddgrs_session* h, ddgrs_step_report* report, ddgrs_error* err);
 /* [ℓ̄₀, last τ, barycenter target (3)] (session.rs:647 mean_edge_length0). */
 DDGRS_API int ddgrs_session_state(ddgrs_session* h, double* out5, ddgrs_error* err);
+
+/* ---- the learned right preconditioner --------------------------------------------------------
+ * No counterpart in the reference, whose solver has the plain preconditioner only.  A mesh
+ * network (per-vertex features: position, unit normal, vertex area, mean curvature; four
+ * message-passing layers of width 64) maps the geometry to per-vertex 3×3 blocks B_v and per-edge
+ * 3×3 blocks G_uv = ½ (g_u + g_v); the step's preconditioner becomes z = P⁻¹ r + Q r with
+ *   (Q r)_v = s (B_v r_v + Σ_{u ∈ N(v)} G_uv r_u)
+ * on the vertex rows (the constraint rows unchanged), a linear map, so GMRES stays a Krylov method
+ * with a fixed right preconditioner.  Trained on pairs (r, P⁻¹ r, y = A′⁻¹ r) with the loss
+ * ‖P⁻¹ r + Q r − y‖² / ‖y − P⁻¹ r‖² (1 at Q = 0, where the output layer starts).  Vectors are
+ * saddle vectors of 3·nv + 3 (FP64, host), positions 3·nv. */
+/* A network for the surface's mesh (parameters from `seed`) with room for `max_pairs` pairs. */
+DDGRS_API int ddgrs_learned_create(ddgrs_surface* s, uint64_t seed, int64_t max_pairs, ddgrs_learned** out,
+                                   ddgrs_error* err);
+DDGRS_API void ddgrs_learned_destroy(ddgrs_learned* l);
+/* out4 = [parameters, pairs, Adam steps, features per vertex]; *scale = s (0 before training). */
+DDGRS_API int ddgrs_learned_info(ddgrs_learned* l, int64_t* out4, double* scale, ddgrs_error* err);
+/* Copies the FP32 parameters out (`get`) and then in (`set`); either may be null. */
+DDGRS_API int ddgrs_learned_params(ddgrs_learned* l, float* get, const float* set, ddgrs_error* err);
+/* Sets s (else the first training sets it to the pairs' median ‖y − P⁻¹ r‖ / ‖r‖). */
+DDGRS_API int ddgrs_learned_set_scale(ddgrs_learned* l, double scale, ddgrs_error* err);
+DDGRS_API int ddgrs_learned_add_pair(ddgrs_learned* l, const double* positions, const double* r, const double* pr,
+                                     const double* y, ddgrs_error* err);
+/* The features (nv × 8, FP32) and the network's blocks (nv × 18: B_v row-major, then g_v). */
+DDGRS_API int ddgrs_learned_features(ddgrs_learned* l, const double* positions, float* feat, ddgrs_error* err);
+DDGRS_API int ddgrs_learned_blocks(ddgrs_learned* l, const double* positions, float* blocks, ddgrs_error* err);
+/* The loss of one pair and (grad non-null) its gradient with respect to the parameters. */
+DDGRS_API int ddgrs_learned_loss_grad(ddgrs_learned* l, int64_t pair, double* loss, float* grad, ddgrs_error* err);
+/* Adam (β 0.9 / 0.999) over the pairs in order, `batch` pairs per update; loss2 = the mean loss
+ * over all pairs before and after. */
+DDGRS_API int ddgrs_learned_train(ddgrs_learned* l, int64_t iterations, int64_t batch, double lr, double* loss2,
+                                  ddgrs_error* err);
+/* zs = Q rs at the given positions (the learned part alone). */
+DDGRS_API int ddgrs_learned_apply_batch(ddgrs_learned* l, const double* positions, int64_t nb, const double* rs,
+                                        double* zs, ddgrs_error* err);
+/* Attaches `l` (mesh of the session's surface) to the session: mode 1 solves with the plain
+ * preconditioner and adds each step's pair (r = the step's right-hand side, P⁻¹ r, y = the
+ * direction) to `l`; mode 2 solves with P⁻¹ + Q and, when that solve misses the tolerance within
+ * the iteration cap, again with the plain one; mode 0 (or l null) detaches.  `l` must stay alive
+ * while attached. */
+DDGRS_API int ddgrs_session_set_learned(ddgrs_session* h, ddgrs_learned* l, int32_t mode, ddgrs_error* err);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,197 @@
+"""FP64 NumPy restatement of the learned right preconditioner — TEST INFRASTRUCTURE.
+
+The reference has no learned preconditioner (its solver uses the plain one only,
+``crates/ddg-repulsive-surfaces/src/solve/precond.rs``); this module restates the network that the
+library trains on the device, so that its features, forward pass, loss and gradient can be checked:
+
+* features (per vertex, 8): position centred at the mean vertex and divided by the RMS radius;
+  the unit sum of the incident faces' unnormalised normals; the vertex area (a third of the
+  incident face areas) over the mean vertex area; the umbrella Laplacian (mean neighbour minus
+  the vertex) along that normal over the mean squared edge length, times the RMS radius (a
+  dimensionless mean curvature);
+* network: h0 = relu(F W0 + b0); four layers h <- relu([h, mean_N(h)] W_l + b_l) of width 64;
+  o = h Wo + bo with o_v = (B_v row-major, g_v row-major);
+* (Q r)_v = s (B_v r_v + sum_{u in N(v)} (g_v + g_u)/2 r_u), loss |s q - e|^2 / |e|^2.
+
+Parity pins (``tests/test_oracle_learned.py``): the analytic gradient against central differences
+of the loss (every parameter group), the features against closed forms on a sphere (radial unit
+normals, equal areas on a regular mesh, the Laplacian's sign), Q's linearity and symmetry
+(Q is symmetric when B_v is, because the edge blocks are shared).
+
+Only tests may import this module.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+FEAT, WIDTH, LAYERS, OUT = 8, 64, 4, 18
+
+
+def layout():
+    """Offsets of the parameter groups in the flat FP32 array (the library's order)."""
+    o = 0
+    lay = {}
+
+    def take(name, count):
+        nonlocal o
+        lay[name] = (o, count)
+        o += count
+
+    take("w0", FEAT * WIDTH)
+    take("b0", WIDTH)
+    for l in range(LAYERS):
+        take(f"w{l + 1}", 2 * WIDTH * WIDTH)
+        take(f"b{l + 1}", WIDTH)
+    take("wo", WIDTH * OUT)
+    take("bo", OUT)
+    lay["total"] = o
+    return lay
+
+
+def unpack(p):
+    lay = layout()
+    p = np.asarray(p, dtype=np.float64)
+    out = {}
+    shapes = {"w0": (FEAT, WIDTH), "wo": (WIDTH, OUT)}
+    for l in range(LAYERS):
+        shapes[f"w{l + 1}"] = (2 * WIDTH, WIDTH)
+    for k, v in lay.items():
+        if k == "total":
+            continue
+        o, c = v
+        out[k] = p[o:o + c].reshape(shapes.get(k, (c,)))
+    return out
+
+
+def neighbours(faces, nv):
+    """CSR of the mesh's edges in both directions, ascending per vertex."""
+    adj = [set() for _ in range(nv)]
+    for a, b, c in np.asarray(faces):
+        for x, y in ((a, b), (b, c), (c, a)):
+            adj[x].add(int(y))
+            adj[y].add(int(x))
+    ptr = np.zeros(nv + 1, dtype=np.int64)
+    nb = []
+    for v in range(nv):
+        s = sorted(adj[v])
+        nb.extend(s)
+        ptr[v + 1] = len(nb)
+    return ptr, np.asarray(nb, dtype=np.int64)
+
+
+def features(pos, faces):
+    pos = np.asarray(pos, dtype=np.float64)
+    faces = np.asarray(faces)
+    nv = pos.shape[0]
+    ptr, nb = neighbours(faces, nv)
+    c = pos.mean(axis=0)
+    R = np.sqrt(((pos - c) ** 2).sum() / nv) + 1e-300
+    nf = np.cross(pos[faces[:, 1]] - pos[faces[:, 0]], pos[faces[:, 2]] - pos[faces[:, 0]])
+    a2 = np.linalg.norm(nf, axis=1)
+    vn = np.zeros((nv, 3))
+    area = np.zeros(nv)
+    for k in range(3):
+        np.add.at(vn, faces[:, k], nf)
+        np.add.at(area, faces[:, k], a2 / 6.0)
+    deg = np.diff(ptr)
+    src = np.repeat(np.arange(nv), deg)
+    el = (((pos[nb] - pos[src]) ** 2).sum(axis=1)).mean()
+    nn = np.linalg.norm(vn, axis=1)
+    nn[nn == 0] = 1.0
+    un = vn / nn[:, None]
+    nsum = np.zeros((nv, 3))
+    np.add.at(nsum, src, pos[nb])
+    lap = np.where(deg[:, None] > 0, nsum / np.maximum(deg, 1)[:, None] - pos, 0.0)
+    h = (lap * un).sum(axis=1)
+    f = np.zeros((nv, FEAT))
+    f[:, 0:3] = (pos - c) / R
+    f[:, 3:6] = un
+    f[:, 6] = area / area.mean()
+    f[:, 7] = h / el * R
+    return f
+
+
+def _mean(h, ptr, nb):
+    nv = ptr.shape[0] - 1
+    deg = np.diff(ptr)
+    src = np.repeat(np.arange(nv), deg)
+    m = np.zeros_like(h)
+    np.add.at(m, src, h[nb])
+    return m / np.maximum(deg, 1)[:, None]
+
+
+def _mean_t(dm, ptr, nb):
+    nv = ptr.shape[0] - 1
+    deg = np.diff(ptr)
+    src = np.repeat(np.arange(nv), deg)
+    d = np.zeros_like(dm)
+    np.add.at(d, nb, dm[src] / np.maximum(deg, 1)[src][:, None])
+    return d
+
+
+def forward(p, f, ptr, nb, keep=False):
+    P = unpack(p)
+    pre0 = f @ P["w0"] + P["b0"]
+    h = np.maximum(pre0, 0.0)
+    cache = [(f, None, pre0)]
+    for l in range(LAYERS):
+        m = _mean(h, ptr, nb)
+        x = np.concatenate([h, m], axis=1)
+        pre = x @ P[f"w{l + 1}"] + P[f"b{l + 1}"]
+        cache.append((x, m, pre))
+        h = np.maximum(pre, 0.0)
+    o = h @ P["wo"] + P["bo"]
+    return (o, h, cache) if keep else o
+
+
+def q_apply(o, r3, ptr, nb):
+    """q_v = B_v r_v + sum_u (g_v + g_u)/2 r_u (without the scale)."""
+    nv = o.shape[0]
+    B = o[:, :9].reshape(nv, 3, 3)
+    G = o[:, 9:].reshape(nv, 3, 3)
+    deg = np.diff(ptr)
+    src = np.repeat(np.arange(nv), deg)
+    q = np.einsum("vab,vb->va", B, r3)
+    edge = 0.5 * (G[src] + G[nb])
+    np.add.at(q, src, np.einsum("kab,kb->ka", edge, r3[nb]))
+    return q
+
+
+def loss_grad(p, f, ptr, nb, r, e, s):
+    """Loss |s q - e|^2 / |e|^2 and its gradient (flat, the parameter layout) by backpropagation."""
+    nv = f.shape[0]
+    r3 = np.asarray(r, dtype=np.float64)[:3 * nv].reshape(nv, 3)
+    e3 = np.asarray(e, dtype=np.float64)[:3 * nv].reshape(nv, 3)
+    e2 = (e3 ** 2).sum()
+    o, h4, cache = forward(p, f, ptr, nb, keep=True)
+    q = s * q_apply(o, r3, ptr, nb)
+    d = q - e3
+    L = (d ** 2).sum() / e2
+    qd = 2.0 * d / e2
+    deg = np.diff(ptr)
+    src = np.repeat(np.arange(nv), deg)
+    dout = np.zeros((nv, OUT))
+    dout[:, :9] = s * np.einsum("va,vb->vab", qd, r3).reshape(nv, 9)
+    gsum = np.zeros((nv, 3, 3))
+    np.add.at(gsum, src, np.einsum("ka,kb->kab", qd[src], r3[nb]) + np.einsum("ka,kb->kab", qd[nb], r3[src]))
+    dout[:, 9:] = 0.5 * s * gsum.reshape(nv, 9)
+    P = unpack(p)
+    grads = {"wo": h4.T @ dout, "bo": dout.sum(axis=0)}
+    dh = dout @ P["wo"].T
+    for l in range(LAYERS, 0, -1):
+        x, _, pre = cache[l]
+        dpre = dh * (pre > 0)
+        grads[f"w{l}"] = x.T @ dpre
+        grads[f"b{l}"] = dpre.sum(axis=0)
+        W = P[f"w{l}"]
+        dh = dpre @ W[:WIDTH].T + _mean_t(dpre @ W[WIDTH:].T, ptr, nb)
+    _, _, pre0 = cache[0]
+    dpre = dh * (pre0 > 0)
+    grads["w0"] = f.T @ dpre
+    grads["b0"] = dpre.sum(axis=0)
+    lay = layout()
+    g = np.zeros(lay["total"])
+    for k, v in lay.items():
+        if k != "total":
+            g[v[0]:v[0] + v[1]] = grads[k].reshape(-1)
+    return L, g

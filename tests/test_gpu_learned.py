@@ -1,0 +1,143 @@
+"""The learned right preconditioner on the device against its FP64 NumPy restatement (GPU).
+
+* features: ≤ 1e-5 absolute (FP32 of FP64 host values);
+* the network's blocks at random parameters: ≤ 1e-4 relative (2-norm), FP32 products;
+* loss ≤ 1e-4 relative and the gradient of every parameter group ≤ 1e-3 relative (2-norm): the
+  device's FP32 backward pass against the oracle's FP64 backpropagation (itself pinned by central
+  differences on the CPU);
+* Q r: ≤ 1e-6 relative against the oracle's Q at the device's FP32 blocks;
+* training and the gain: pairs recorded along plain descent trajectories, Adam on the device, and
+  on pairs of a held-out trajectory the trained Q brings P⁻¹ r closer to A′⁻¹ r (loss < 1, where
+  Q = 0 gives 1); with P⁻¹ + Q every step reaches the relative residual 1e-6.
+"""
+import numpy as np
+import pytest
+
+from oracle import learned as LN
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - np.asarray(b, np.float64)) / np.linalg.norm(b))
+
+
+def _random_params(seed):
+    rng = np.random.default_rng(seed)
+    p = np.zeros(LN.layout()["total"])
+    for k, v in LN.layout().items():
+        if k == "total":
+            continue
+        o, c = v
+        fan_in = {"w0": LN.FEAT, "wo": LN.WIDTH}.get(k, 2 * LN.WIDTH)
+        sd = (0.3 if k == "wo" else 1.0) / np.sqrt(fan_in) if k.startswith("w") else 0.05
+        p[o:o + c] = rng.standard_normal(c) * sd
+    return p.astype(np.float32)
+
+
+@pytest.fixture
+def l3(pkg, ctx):
+    p, f = pkg.generate.config_icosphere_l3()
+    return p, f, pkg.Surface(ctx, f, p)
+
+
+def test_features_and_blocks_match_oracle(pkg, l3):
+    p, f, s = l3
+    net = pkg.Learned(s, seed=1, max_pairs=2)
+    feat = net.features(p)
+    ref = LN.features(p, f)
+    assert np.max(np.abs(feat - ref)) <= 1e-5 * max(1.0, np.max(np.abs(ref)))
+    # the output layer starts at zero: no blocks before training
+    assert np.all(net.blocks(p) == 0.0)
+    prm = _random_params(4)
+    net.set_params(prm)
+    assert np.array_equal(net.params(), prm)
+    ptr, nb = LN.neighbours(f, p.shape[0])
+    o_ref = LN.forward(prm.astype(np.float64), feat.astype(np.float64), ptr, nb)
+    o = net.blocks(p)
+    print(f"blocks: rel {rel(o, o_ref):.2e}")
+    assert rel(o, o_ref) <= 1e-4
+
+
+def test_loss_gradient_and_apply_match_oracle(pkg, l3):
+    p, f, s = l3
+    nv = p.shape[0]
+    n = 3 * nv + 3
+    net = pkg.Learned(s, seed=2, max_pairs=2)
+    prm = _random_params(7)
+    net.set_params(prm)
+    net.set_scale(0.5)
+    rng = np.random.default_rng(11)
+    r, pr, y = rng.standard_normal(n), rng.standard_normal(n), rng.standard_normal(n)
+    net.add_pair(p, r, pr, y)
+    loss, g = net.loss_grad(0)
+    feat = net.features(p).astype(np.float64)
+    ptr, nb = LN.neighbours(f, nv)
+    # the pair is kept in FP32 on the device
+    r32, e32 = r.astype(np.float32).astype(np.float64), (y - pr).astype(np.float32).astype(np.float64)
+    e2 = float(np.sum((y - pr)[:3 * nv] ** 2))
+    L_ref, g_ref = LN.loss_grad(prm.astype(np.float64), feat, ptr, nb, r32, e32, 0.5)
+    L_ref *= float(np.sum(e32[:3 * nv] ** 2)) / e2
+    g_ref *= float(np.sum(e32[:3 * nv] ** 2)) / e2
+    print(f"loss {loss:.6g} oracle {L_ref:.6g}")
+    assert abs(loss - L_ref) <= 1e-4 * abs(L_ref)
+    for k, v in LN.layout().items():
+        if k == "total":
+            continue
+        o, c = v
+        e = rel(g[o:o + c], g_ref[o:o + c])
+        print(f"grad {k}: rel {e:.2e}")
+        assert e <= 1e-3, k
+    # Q r at the device's blocks
+    blocks = net.blocks(p).astype(np.float64)
+    z = net.apply(p, r)
+    q_ref = 0.5 * LN.q_apply(blocks, r[:3 * nv].reshape(nv, 3), ptr, nb)
+    assert rel(z[:3 * nv], q_ref.reshape(-1)) <= 1e-6
+    assert np.all(z[3 * nv:] == 0.0)
+
+
+def test_training_gains_over_plain(pkg, ctx):
+    """Pairs from the plain step's solves on two jittered level-3 icospheres train Q; on a third
+    trajectory the trained Q reduces ‖P⁻¹ r + Q r − A′⁻¹ r‖² / ‖P⁻¹ r − A′⁻¹ r‖² below 1, and
+    descent with P⁻¹ + Q converges at every step."""
+    gen = pkg.generate
+    net = None
+    for seed in (11, 12):
+        p, f = gen.config_icosphere_l3(seed)
+        s = pkg.Surface(ctx, f, p)
+        if net is None:
+            net = pkg.Learned(s, seed=3, max_pairs=64)
+        ses = pkg.Session(s)
+        ses.set_learned(net, 1)
+        for _ in range(10):
+            rep = ses.step()
+            assert rep.learned_mode == 1
+        ses.set_learned(None, 0)
+    assert net.info()["pairs"] == 20
+    before, after = net.train(400, batch=4, lr=2e-3)
+    print(f"training loss {before:.4f} -> {after:.4f}, scale {net.info()['scale']:.3g}")
+    assert abs(before - 1.0) < 1e-6 and after < before
+    # held out: pairs of a third trajectory
+    p, f = gen.config_icosphere_l3(13)
+    s = pkg.Surface(ctx, f, p)
+    ses = pkg.Session(s)
+    ses.set_learned(net, 1)
+    for _ in range(5):
+        ses.step()
+    held = [net.loss_grad(k, grad=False)[0] for k in range(20, net.info()["pairs"])]
+    print("held-out losses", ["%.3f" % x for x in held])
+    assert len(held) == 5 and np.mean(held) < 1.0
+    # P⁻¹ + Q against plain from the same positions
+    p, f = gen.config_icosphere_l3(14)
+    its = {}
+    for mode in (0, 2):
+        s = pkg.Surface(ctx, f, p)
+        ses = pkg.Session(s)
+        if mode:
+            ses.set_learned(net, 2)
+        its[mode] = []
+        for _ in range(5):
+            rep = ses.step()
+            assert rep.gmres.converged and rep.gmres.relative_residual <= 1e-6
+            its[mode].append(rep.learned_iterations if mode and not rep.learned_fallback else rep.gmres.iterations)
+    print(f"GMRES iterations plain {its[0]} learned {its[2]}")
