@@ -183,7 +183,7 @@ def scale_run(args) -> int:
             "workload": "plain Hs descent step, bumpy icosphere L7 (163,842 vertices, 327,680 faces, 64 bumps), "
                         "Barnes-Hut theta 0.25, chi 0.5, GMRES restart 50 to 1e-6",
             "faces": int(f.shape[0]),
-            "preconditioner": "plain (reference); learned not built",
+            "preconditioner": "plain (reference); the learned leg in `learned`",
             "gmres_iterations": [r.gmres.iterations for r in reps],
             # steps whose solve switched to the extended near field (the FP32 products' rounding held the
             # explicit residual above 1e-6), and steps that ended unconverged
@@ -194,6 +194,84 @@ def scale_run(args) -> int:
     }
     print(json.dumps(line))
     return 0
+
+
+def learned_leg(pkg, ctx, args, opts, p, f, plain_ms, plain_its):
+    """The learned preconditioner at L6: pairs from plain descent on `fields` bump fields other than
+    the timed one (`steps` each), Adam on the device, then the timed steps again from the same
+    start with P⁻¹ + Q (GMRES to the same 1e-6; the plain solve on a miss)."""
+    gen = pkg.generate
+    base, _ = gen.icosphere(6)
+    surf = pkg.Surface(ctx, f, p)
+    # four pairs per plain step: (g, A′⁻¹ g) and three Krylov pairs
+    npairs = args.learned_pairs or 4 * args.learned_fields * args.learned_steps
+    net = pkg.Learned(surf, seed=7, max_pairs=npairs)
+    t0 = time.perf_counter()
+    for k in range(args.learned_fields):
+        surf.set_positions(gen.bumps(base, 16, (0.0, 0.1), (0.1, 0.4), 1000 + k))
+        ses = pkg.Session(surf, opts)
+        ses.set_learned(net, 1)
+        for _ in range(args.learned_steps):
+            ses.step()
+        ses.close()
+    t_pairs = time.perf_counter() - t0
+    iters = max(1, args.learned_epochs * net.info()["pairs"] // args.learned_batch)
+    print(f"bench: learned: {net.info()['pairs']} pairs in {t_pairs:.1f} s, training {iters} updates", file=sys.stderr,
+          flush=True)
+    t0 = time.perf_counter()
+    before, after = net.train(iters, batch=args.learned_batch, lr=args.learned_lr)
+    ctx.synchronize()
+    t_train = time.perf_counter() - t0
+    surf.set_positions(p)
+    ses = pkg.Session(surf, opts)
+    ses.set_learned(net, 2)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    reps = [ses.step() for _ in range(args.steps)]
+    ctx.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    ses.close()
+    # one profiled step: the network's forward kernels against the FP32 VALU peak
+    surf.set_positions(p)
+    ses = pkg.Session(surf, opts)
+    ses.set_learned(net, 2)
+    ctx.set_profiling(True)
+    ses.step()
+    kt = ctx.kernel_times()
+    ctx.set_profiling(False)
+    ses.close()
+    nv = p.shape[0]
+    flop = 2.0 * nv * (8 * 64 + 4 * 128 * 64 + 64 * 18)
+    rows = kt.get("ln_rows")
+    fwd = None
+    if rows:
+        a = flop / (rows["ms"] * 1e-3) / 1e12
+        fwd = {"bound": "valu", "achieved": a, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": a / PEAK_FP32_TFLOPS,
+               "kernel_ms_per_step": rows["ms"], "launches": rows["launches"],
+               "work": f"2 x {nv} vertices x (8x64 + 4 x 128x64 + 64x18) multiply-adds"}
+    its = [r.learned_iterations if not r.learned_fallback else r.gmres.iterations for r in reps]
+    return {
+        "pairs": net.info()["pairs"],
+        "fields": args.learned_fields,
+        "steps_per_field": args.learned_steps,
+        "seconds_pairs": t_pairs,
+        "training_updates": iters,
+        "batch": args.learned_batch,
+        "seconds_training": t_train,
+        "loss_before": before,
+        "loss_after": after,
+        "gmres_iterations_learned": [r.learned_iterations for r in reps],
+        "gmres_iterations_plain": plain_its,
+        "fallbacks": sum(1 for r in reps if r.learned_fallback == 1),
+        "skipped_after_misses": sum(1 for r in reps if r.learned_fallback == 2),
+        "converged": all(r.gmres.converged for r in reps),
+        "ms_per_step_learned": ms,
+        "ms_per_step_plain": plain_ms,
+        "time_ratio_learned_over_plain": ms / plain_ms,
+        "mean_iterations_used": float(np.mean(its)),
+        "forward_ms_per_step": float(np.mean([r.seconds["learned"] for r in reps]) * 1e3),
+        "network_roofline": fwd,
+    }
 
 
 def main() -> int:
@@ -207,6 +285,14 @@ def main() -> int:
     ap.add_argument("--config", choices=["l6", "l7"], default="l6",
                     help="l6: the headline step (default); l7: the level-7 scale run (theta 0.25, energy drift "
                          "against the FP64 oracle after steps 1, 10 and 20, fallback rate)")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned preconditioner leg")
+    ap.add_argument("--learned-fields", type=int, default=4,
+                    help="bump fields of the training trajectories (the full training set: 40)")
+    ap.add_argument("--learned-steps", type=int, default=10, help="plain steps per field (the full set: 50)")
+    ap.add_argument("--learned-pairs", type=int, default=0, help="pairs kept (0: four per training step)")
+    ap.add_argument("--learned-epochs", type=int, default=20)
+    ap.add_argument("--learned-batch", type=int, default=8)
+    ap.add_argument("--learned-lr", type=float, default=2e-3)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU (asked for {args.gpus}); how the step shards over GPUs is described, not built.",
@@ -313,6 +399,9 @@ def main() -> int:
 
     its = [r.gmres.iterations for r in reps]
     stage = {k: float(np.mean([r.seconds[k] for r in reps]) * 1e3) for k in reps[0].seconds}
+    learned = None
+    if not args.no_learned:
+        learned = learned_leg(pkg, ctx, args, opts, p, f, ms, its)
     line = {
         "metric": METRIC,
         "value": 1e3 / ms,
@@ -330,7 +419,7 @@ def main() -> int:
             "workload": "plain Hs descent step, bumpy icosphere L6 (40,962 vertices, 81,920 faces), Barnes-Hut "
                         "theta 0.5, chi 0.5, GMRES restart 50 to 1e-6",
             "faces": nf,
-            "preconditioner": "plain (reference); learned not built",
+            "preconditioner": "plain (reference); the learned leg in `learned`",
             "gmres_iterations": its,
             "extended_solves": sum(1 for r in reps if r.gmres.extended),
             "energy": [r.energy_before for r in reps] + [reps[-1].energy_trial],
@@ -340,6 +429,7 @@ def main() -> int:
         "roofline": roof,
         "rooflines": rooflines,
         "cpu_baseline": cpu,
+        "learned": learned,
         "traffic_scale": {k: v for k, v in traffic.items() if k.endswith("_scale")},
         "kernel_ms_step1": {k: round(v["ms"], 4) for k, v in sorted(kt.items(), key=lambda kv: -kv[1]["ms"])[:16]},
     }

@@ -613,7 +613,7 @@ class StepReport:
     nodes_lumped: int
     seconds: dict
     learned_mode: int = 0
-    learned_fallback: bool = False
+    learned_fallback: int = 0  # 1: stagnated, plain used; 2: Q off after consecutive misses
     learned_iterations: int = 0
 
 
@@ -637,7 +637,7 @@ class Session(_Handle):
         sec = {k[len("seconds_"):]: float(getattr(r, k)) for k, _ in _StepReportC._fields_ if k.startswith("seconds_")}
         return StepReport(float(r.energy_before), float(r.energy_trial), float(r.gradient_norm), float(r.sobolev_norm_sq),
                           float(r.tau), float(r.tau0), bool(r.accepted), int(r.trials), _gmres_report(r.gmres),
-                          int(r.pairs_exact), int(r.nodes_lumped), sec, int(r.learned_mode), bool(r.learned_fallback),
+                          int(r.pairs_exact), int(r.nodes_lumped), sec, int(r.learned_mode), int(r.learned_fallback),
                           int(r.learned_iterations))
 
     def set_learned(self, learned: "Learned | None", mode: int):

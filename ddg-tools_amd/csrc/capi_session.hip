@@ -249,6 +249,7 @@ int ddgrs_session_set_learned(ddgrs_session* h, ddgrs_learned* l, int32_t mode, 
     if (mode == 2 && !(m->scale > 0.f)) invalid("learned", "untrained", "a trained network (or a set scale) for mode 2");
     h->ses.learned = m;
     h->ses.learned_mode = mode;
+    h->ses.learned_misses = 0;
   });
 }
 

@@ -300,7 +300,9 @@ void Session::step(StepReport* rep) {
   rep->learned_mode = lmode;
   GmresResult gr;
   bool solved = false;
-  if (lmode == 2) {
+  if (lmode == 2 && learned_misses >= kLearnedMissLimit) {
+    rep->learned_fallback = 2;  // Q off for the rest of the session after consecutive misses
+  } else if (lmode == 2) {
     // P⁻¹ + Q: the network's blocks at this step's positions, then the solve; on a miss the
     // plain solve below from x₀ = 0
     const auto tl = std::chrono::steady_clock::now();
@@ -311,24 +313,45 @@ void Session::step(StepReport* rep) {
       pc.apply(ctx, r, z);
       learned->apply(ctx, r, z);
     };
-    gr = gmres_device(ctx, n, A, PQ, g.ptr, sol.ptr, gs, vec, stall);
+    // stagnation bound: twice the last converged solve's iterations (+5; one restart cycle before
+    // any) before the plain fallback
+    GmresSettings gl = gs;
+    gl.max_iterations = std::min(gs.max_iterations, last_iterations > 0 ? 2 * last_iterations + 5 : gs.restart);
+    gr = gmres_device(ctx, n, A, PQ, g.ptr, sol.ptr, gl, vec, stall);
     fm.extended = pc.vf.extended = false;
     rep->learned_iterations = (int64_t)gr.iterations;
     solved = gr.converged;
     rep->learned_fallback = solved ? 0 : 1;
+    learned_misses = solved ? 0 : learned_misses + 1;
   }
   if (!solved) gr = gmres_device(ctx, n, A, P, g.ptr, sol.ptr, gs, vec, stall);
   fm.extended = pc.vf.extended = false;
+  if (gr.converged) last_iterations = gr.iterations;
   if (lmode == 1 && gr.converged) {
-    // the pair (g, P⁻¹ g, A′⁻¹ g) at this step's positions
+    // the pair (g, P⁻¹ g, A′⁻¹ g) at this step's positions, then pairs of the vectors the plain
+    // solve's Krylov space is made of: with z₀ = P⁻¹ g, r_k = A′ z_{k−1} has A′⁻¹ r_k = z_{k−1}
+    // exactly, and z_k = P⁻¹ r_k
     pg.reserve(n);
+    kr.reserve(n);
+    kz.reserve(n);
     pc.apply(ctx, g.ptr, pg.ptr);
-    std::vector<double> hg(n), hp(n), hy(n);
-    download(ctx, hg.data(), (const double*)g.ptr, n);
-    download(ctx, hp.data(), (const double*)pg.ptr, n);
-    download(ctx, hy.data(), (const double*)sol.ptr, n);
-    ctx->sync();
-    if (learned->npairs < learned->cap_pairs) learned->add_pair(ctx, geo.pos.data(), hg.data(), hp.data(), hy.data());
+    std::vector<double> hr(n), hp(n), hy(n);
+    auto add = [&](const double* r, const double* pr, const double* y) {
+      if (learned->npairs >= learned->cap_pairs) return;
+      download(ctx, hr.data(), r, n);
+      download(ctx, hp.data(), pr, n);
+      download(ctx, hy.data(), y, n);
+      ctx->sync();
+      learned->add_pair(ctx, geo.pos.data(), hr.data(), hp.data(), hy.data());
+    };
+    add(g.ptr, pg.ptr, sol.ptr);
+    double *z = pg.ptr, *r = kr.ptr, *pz = kz.ptr;
+    for (int k = 0; k < kKrylovPairs && learned->npairs < learned->cap_pairs; ++k) {
+      op.apply(ctx, z, r);
+      pc.apply(ctx, r, pz);
+      add(r, pz, z);
+      std::swap(z, pz);
+    }
   }
   rep->seconds_solve = seconds_since(t);
   rep->gmres.iterations = (int64_t)gr.iterations;

@@ -52,7 +52,11 @@ struct Session {
   // the learned preconditioner: mode 1 records pairs, 2 solves with P⁻¹ + Q (plain on a miss)
   LearnedModel* learned = nullptr;
   int learned_mode = 0;
-  DevBuf<double> pg;  // P⁻¹ g of a recorded pair
+  DevBuf<double> pg, kr, kz;  // P⁻¹ g and the Krylov vectors of recorded pairs
+  size_t last_iterations = 0;  // iterations of the last converged solve
+  int learned_misses = 0;        // consecutive P⁻¹ + Q solves that fell back to the plain one
+  static constexpr int kLearnedMissLimit = 2;  // after that many, the session solves plain
+  static constexpr int kKrylovPairs = 3;  // pairs (A′ z, z) recorded per step after (g, A′⁻¹ g)
 
   Session() = default;
   Session(const Session&) = delete;

@@ -224,9 +224,10 @@ typedef struct ddgrs_step_report {
   double seconds_trees, seconds_energy, seconds_metric, seconds_precond, seconds_solve, seconds_line_search,
       seconds_projection, seconds_total;
   /* The learned preconditioner (no counterpart in the reference): the session's mode (0 plain,
-   * 1 plain solve recording a training pair, 2 P⁻¹ + Q), whether a P⁻¹ + Q solve missed the
-   * tolerance within the iteration cap and the plain solve's result was used (then `gmres` is the
-   * plain solve's report), and the iterations of the P⁻¹ + Q solve. */
+   * 1 plain solve recording training pairs, 2 P⁻¹ + Q); learned_fallback 1 when the P⁻¹ + Q solve
+   * stagnated and the plain solve's result was used (then `gmres` is the plain solve's report),
+   * 2 when Q was not tried because the two solves before fell back (the session then stays plain
+   * until set_learned is called again); the iterations of the P⁻¹ + Q solve. */
   int32_t learned_mode, learned_fallback;
   int64_t learned_iterations;
   double seconds_learned; /* features and the network's forward pass */
@@ -280,10 +281,12 @@ DDGRS_API int ddgrs_learned_train(ddgrs_learned* l, int64_t iterations, int64_t 
 DDGRS_API int ddgrs_learned_apply_batch(ddgrs_learned* l, const double* positions, int64_t nb, const double* rs,
                                         double* zs, ddgrs_error* err);
 /* Attaches `l` (mesh of the session's surface) to the session: mode 1 solves with the plain
- * preconditioner and adds each step's pair (r = the step's right-hand side, P⁻¹ r, y = the
- * direction) to `l`; mode 2 solves with P⁻¹ + Q and, when that solve misses the tolerance within
- * the iteration cap, again with the plain one; mode 0 (or l null) detaches.  `l` must stay alive
- * while attached. */
+ * preconditioner and adds each step's pairs to `l` (r = the step's right-hand side g with y = the
+ * direction, then three Krylov pairs r_k = A′ z_{k−1}, y = z_{k−1}, z₀ = P⁻¹ g, z_k = P⁻¹ r_k);
+ * mode 2 solves with P⁻¹ + Q and, when that solve has not reached the tolerance within twice the
+ * last converged solve's iterations plus 5 (one restart cycle before any converged solve; at
+ * most the iteration cap), again with the plain one;
+ * mode 0 (or l null) detaches.  `l` must stay alive while attached. */
 DDGRS_API int ddgrs_session_set_learned(ddgrs_session* h, ddgrs_learned* l, int32_t mode, ddgrs_error* err);
 
 #ifdef __cplusplus

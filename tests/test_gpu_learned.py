@@ -98,23 +98,24 @@ def test_loss_gradient_and_apply_match_oracle(pkg, l3):
 
 def test_training_gains_over_plain(pkg, ctx):
     """Pairs from the plain step's solves on two jittered level-3 icospheres train Q; on a third
-    trajectory the trained Q reduces ‖P⁻¹ r + Q r − A′⁻¹ r‖² / ‖P⁻¹ r − A′⁻¹ r‖² below 1, and
-    descent with P⁻¹ + Q converges at every step."""
+    trajectory the trained Q reduces ‖P⁻¹ r + Q r − A′⁻¹ r‖² / ‖P⁻¹ r − A′⁻¹ r‖² below 1 on
+    average, and descent with P⁻¹ + Q (the plain solve on stagnation) converges at every step."""
     gen = pkg.generate
     net = None
     for seed in (11, 12):
         p, f = gen.config_icosphere_l3(seed)
         s = pkg.Surface(ctx, f, p)
         if net is None:
-            net = pkg.Learned(s, seed=3, max_pairs=64)
+            net = pkg.Learned(s, seed=3, max_pairs=128)
         ses = pkg.Session(s)
         ses.set_learned(net, 1)
         for _ in range(10):
             rep = ses.step()
             assert rep.learned_mode == 1
         ses.set_learned(None, 0)
-    assert net.info()["pairs"] == 20
-    before, after = net.train(400, batch=4, lr=2e-3)
+    # per step: (g, A′⁻¹ g) and three Krylov pairs
+    assert net.info()["pairs"] == 80
+    before, after = net.train(400, batch=8, lr=2e-3)
     print(f"training loss {before:.4f} -> {after:.4f}, scale {net.info()['scale']:.3g}")
     assert abs(before - 1.0) < 1e-6 and after < before
     # held out: pairs of a third trajectory
@@ -124,9 +125,9 @@ def test_training_gains_over_plain(pkg, ctx):
     ses.set_learned(net, 1)
     for _ in range(5):
         ses.step()
-    held = [net.loss_grad(k, grad=False)[0] for k in range(20, net.info()["pairs"])]
+    held = [net.loss_grad(k, grad=False)[0] for k in range(80, net.info()["pairs"])]
     print("held-out losses", ["%.3f" % x for x in held])
-    assert len(held) == 5 and np.mean(held) < 1.0
+    assert len(held) == 20 and np.mean(held) < 1.0
     # P⁻¹ + Q against plain from the same positions
     p, f = gen.config_icosphere_l3(14)
     its = {}
