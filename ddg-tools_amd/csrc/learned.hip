@@ -92,14 +92,25 @@ __global__ __launch_bounds__(256) void ln_wgrad_kernel(int n, int rows_per_block
       sb[rr][j] = v < r1 ? B[(size_t)v * ldb + j] : 0.f;
     }
     __syncthreads();
+    if constexpr (J == 64) {
+      // thread (j, i0) owns rows i0 + 4q of column j: one B value per row of the chunk, A broadcast
+      const int j = tid & 63, i0 = tid >> 6;
+      for (int rr = 0; rr < 32; ++rr) {
+        const float b = sb[rr][j];
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      const int o = tid + 256 * q;
-      if (o < NO) {
-        const int i = o / J, j = o % J;
-        float s = acc[q];
-        for (int rr = 0; rr < 32; ++rr) s = fmaf(sa[rr][i], sb[rr][j], s);
-        acc[q] = s;
+        for (int q = 0; q < NQ; ++q)
+          if (i0 + 4 * q <= I) acc[q] = fmaf(sa[rr][i0 + 4 * q], b, acc[q]);
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        const int o = tid + 256 * q;
+        if (o < NO) {
+          const int i = o / J, j = o % J;
+          float s = acc[q];
+          for (int rr = 0; rr < 32; ++rr) s = fmaf(sa[rr][i], sb[rr][j], s);
+          acc[q] = s;
+        }
       }
     }
   }
@@ -110,17 +121,26 @@ __global__ __launch_bounds__(256) void ln_wgrad_kernel(int n, int rows_per_block
   }
 }
 
-// grad[w + i J + j] += Σ_b part[b][i][j] (i < I), grad[bias + j] += Σ_b part[b][I][j]; blocks in order.
+// grad[w + i J + j] += Σ_b part[b][i][j] (i < I), grad[bias + j] += Σ_b part[b][I][j].  A workgroup
+// takes 64 outputs; its four waves sum the blocks b ≡ wave (mod 4) in order, then the four sums are
+// added in wave order (a fixed summation tree).
 __global__ __launch_bounds__(256) void ln_wgrad_sum_kernel(int nb, int I, int J, const float* __restrict__ part,
                                                            float* __restrict__ gw, float* __restrict__ gb) {
-  const int o = blockIdx.x * 256 + threadIdx.x, NO = (I + 1) * J;
-  if (o >= NO) return;
+  __shared__ float red[4][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, NO = (I + 1) * J;
+  const int o = blockIdx.x * 64 + lane;
   float s = 0.f;
-  for (int b = 0; b < nb; ++b) s += part[(size_t)b * NO + o];
-  if (o < I * J)
-    gw[o] += s;
-  else
-    gb[o - I * J] += s;
+  if (o < NO)
+    for (int b = w; b < nb; b += 4) s += part[(size_t)b * NO + o];
+  red[w][lane] = s;
+  __syncthreads();
+  if (w == 0 && o < NO) {
+    const float t = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+    if (o < I * J)
+      gw[o] += t;
+    else
+      gb[o - I * J] += t;
+  }
 }
 
 // m[v][c] = mean of h[u][c] over the neighbours u of v.
@@ -265,7 +285,7 @@ void rows(ddgrs_ctx* ctx, int n, const float* X1, int ld1, int I1, const float* 
          msj, bias, relu, pre, Y, ldy);
 }
 
-constexpr int kGradBlocks = 128;
+constexpr int kGradBlocks = 512;
 
 template <int I, int J>
 void wgrad(ddgrs_ctx* ctx, int n, const float* A1, int lda1, int I1, const float* A2, int lda2, const float* B, int ldb,
@@ -273,7 +293,7 @@ void wgrad(ddgrs_ctx* ctx, int n, const float* A1, int lda1, int I1, const float
   const int per = (n + kGradBlocks - 1) / kGradBlocks;
   launch(ctx, "ln_wgrad", ln_wgrad_kernel<I, J>, dim3(kGradBlocks), dim3(256), 0, n, per, A1, lda1, I1, A2, lda2, B, ldb,
          part);
-  launch(ctx, "ln_wgrad_sum", ln_wgrad_sum_kernel, dim3(blocks_for((I + 1) * J, 256)), dim3(256), 0, kGradBlocks, I, J,
+  launch(ctx, "ln_wgrad_sum", ln_wgrad_sum_kernel, dim3(blocks_for((I + 1) * J, 64)), dim3(256), 0, kGradBlocks, I, J,
          (const float*)part, gw, gb);
 }
 
