@@ -96,7 +96,7 @@ def measure_traffic(timeout_s: int = 240):
         for counter in ("FETCH_SIZE", "WRITE_SIZE"):
             runs = {
                 "bench": [sys.executable, os.path.abspath(__file__), "--steps", "1", "--warmup", "0",
-                          "--no-cpu-baseline", "--no-traffic"],
+                          "--no-cpu-baseline", "--no-traffic", "--no-learned"],
                 "copy": [probe, "256", "3"],
             }
             for tag, cmd in runs.items():
