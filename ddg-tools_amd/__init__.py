@@ -642,7 +642,7 @@ class Session(_Handle):
 
     def set_learned(self, learned: "Learned | None", mode: int):
         """Attach a learned preconditioner: mode 1 records a training pair per step (plain solve),
-        2 solves with P⁻¹ + Q (the plain solve on a miss), 0 detaches."""
+        2 solves with (I + Q) P⁻¹ (the plain solve on a miss), 0 detaches."""
         _call(lib().ddgrs_session_set_learned, self._h, learned._h if learned is not None else None, mode)
         self._learned = learned  # keeps it alive while attached
 
@@ -658,7 +658,7 @@ class Session(_Handle):
 class Learned(_Handle):
     """The learned right preconditioner of a surface's mesh (no counterpart in the reference): a
     mesh network giving per-vertex and per-edge 3×3 blocks, Q, added to the plain preconditioner
-    (z = P⁻¹ r + Q r).  Training, the forward pass and Q r run on the device."""
+    (z = (I + Q) P⁻¹ r).  Training, the forward pass and Q x run on the device."""
 
     _destroy = "ddgrs_learned_destroy"
 

@@ -237,7 +237,7 @@ int ddgrs_session_step(ddgrs_session* h, ddgrs_step_report* report, ddgrs_error*
 int ddgrs_session_set_learned(ddgrs_session* h, ddgrs_learned* l, int32_t mode, ddgrs_error* err) {
   return guarded(err, "session_set_learned", [&] {
     need(h, "session");
-    if (mode < 0 || mode > 2) invalid("mode", std::to_string(mode), "0 (plain), 1 (record pairs) or 2 (P⁻¹ + Q)");
+    if (mode < 0 || mode > 2) invalid("mode", std::to_string(mode), "0 (plain), 1 (record pairs) or 2 ((I + Q) P⁻¹)");
     if (!l || mode == 0) {
       h->ses.learned = nullptr;
       h->ses.learned_mode = 0;

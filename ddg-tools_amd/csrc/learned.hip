@@ -1,5 +1,5 @@
 // The learned right preconditioner (see learned.hpp): features, the network's forward and backward
-// passes, the loss, Adam and Q r, as kernels; pairs and training control on the host.
+// passes, the loss, Adam and Q x, as kernels; pairs and training control on the host.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -442,13 +442,21 @@ void LearnedModel::add_pair(ddgrs_ctx* ctx, const double* pos, const double* r, 
   const size_t n3 = 3 * (size_t)nv;
   features(ctx, pos, pair_feat.ptr + (size_t)npairs * nv * kLnFeat);
   float* h = (float*)ctx->arena_take(2 * n3 * sizeof(float));
+  // the scalar c = ⟨y, P⁻¹ r⟩ / ‖P⁻¹ r‖² leaves GMRES's Krylov spaces unchanged: Q learns what
+  // remains of y after it, from c P⁻¹ r
+  double yp = 0.0, pp = 0.0;
+  for (size_t i = 0; i < n3; ++i) {
+    yp += y[i] * pr[i];
+    pp += pr[i] * pr[i];
+  }
+  const double c = pp > 0.0 ? yp / pp : 1.0;
   double e2 = 0.0, r2 = 0.0;
   for (size_t i = 0; i < n3; ++i) {
-    const double e = y[i] - pr[i];
-    h[i] = (float)r[i];
+    const double x = c * pr[i], e = y[i] - x;
+    h[i] = (float)x;
     h[n3 + i] = (float)e;
     e2 += e * e;
-    r2 += r[i] * r[i];
+    r2 += x * x;
   }
   HIPCHECK(hipMemcpyAsync(pair_r.ptr + npairs * n3, h, n3 * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
   HIPCHECK(hipMemcpyAsync(pair_e.ptr + npairs * n3, h + n3, n3 * sizeof(float), hipMemcpyHostToDevice, ctx->stream));

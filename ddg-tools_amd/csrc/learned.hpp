@@ -1,10 +1,11 @@
 // The learned right preconditioner layered on the plain one: a mesh network maps per-vertex
 // geometry features to per-vertex 3×3 blocks B_v and per-vertex 3×3 factors g_v, and
-//   (Q r)_v = s (B_v r_v + Σ_{u ∈ N(v)} ½ (g_v + g_u) r_u)
-// is added to the plain preconditioner's output (z = P⁻¹ r + Q r), so GMRES stays a Krylov method
+//   (Q x)_v = s (B_v x_v + Σ_{u ∈ N(v)} ½ (g_v + g_u) x_u)
+// corrects the plain preconditioner's output: z = (I + Q) P⁻¹ r, so GMRES stays a Krylov method
 // with a linear preconditioner.  Q is trained on pairs (r, y = A′⁻¹ r) from plain solves along
-// descent trajectories to reduce ‖P⁻¹ r + Q r − y‖² / ‖y − P⁻¹ r‖²; it starts at zero (the output
-// layer's weights are zero), where the combined preconditioner is the plain one.
+// descent trajectories: with x = c P⁻¹ r, c = ⟨y, P⁻¹ r⟩ / ‖P⁻¹ r‖² (a scalar leaves the Krylov
+// spaces unchanged), it reduces ‖x + Q x − y‖² / ‖y − x‖²; it starts at zero (the output layer's
+// weights are zero), where the combined preconditioner is the plain one.
 //
 // Network (FP32): h₀ = relu(F W₀ + b₀) (8 features → 64); four message-passing layers
 // h_{l+1} = relu(h_l W_s + m_l W_n + b), m_l = the mean of h_l over the vertex's neighbours;
@@ -33,7 +34,7 @@ struct LearnedModel {
   DevBuf<float> params, grad, adam_m, adam_v, gpart;
   DevBuf<float> feat, act, pre, mean, out, dout, dh, dtmp, qd;  // activations and their gradients
   DevBuf<double> lsum;                   // loss partial sums
-  // training pairs (features, r, e = y − P⁻¹ r: FP32 per vertex) and ‖e‖²
+  // training pairs (features, x = c P⁻¹ r, e = y − x: FP32 per vertex) and ‖e‖²
   DevBuf<float> pair_feat, pair_r, pair_e;
   std::vector<double> pair_e2, pair_ratio;
   int npairs = 0, cap_pairs = 0;
@@ -53,7 +54,7 @@ struct LearnedModel {
   // Adam over the pairs (minibatches of `batch`, `iters` updates); returns the mean loss over all
   // pairs before and after.
   void train(ddgrs_ctx* ctx, int iters, int batch, double lr, double* before, double* after);
-  // z += Q r for a full saddle vector (3 nv + 3, device; the blocks of the last forward).
+  // z += Q x for a full saddle vector x (3 nv + 3, device; the blocks of the last forward).
   void apply(ddgrs_ctx* ctx, const double* r, double* z);
 };
 

@@ -303,15 +303,17 @@ void Session::step(StepReport* rep) {
   if (lmode == 2 && learned_misses >= kLearnedMissLimit) {
     rep->learned_fallback = 2;  // Q off for the rest of the session after consecutive misses
   } else if (lmode == 2) {
-    // P⁻¹ + Q: the network's blocks at this step's positions, then the solve; on a miss the
+    // (I + Q) P⁻¹: the network's blocks at this step's positions, then the solve; on a miss the
     // plain solve below from x₀ = 0
     const auto tl = std::chrono::steady_clock::now();
     learned->features(ctx, geo.pos.data(), learned->feat.ptr);
     learned->forward(ctx, learned->feat.ptr);
     rep->seconds_learned = seconds_since(tl);
+    kz.reserve(n);
     auto PQ = [&](const double* r, double* z) {
       pc.apply(ctx, r, z);
-      learned->apply(ctx, r, z);
+      HIPCHECK(hipMemcpyAsync(kz.ptr, z, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+      learned->apply(ctx, kz.ptr, z);
     };
     // stagnation bound: twice the last converged solve's iterations (+5; one restart cycle before
     // any) before the plain fallback

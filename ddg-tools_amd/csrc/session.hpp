@@ -49,12 +49,12 @@ struct Session {
   DeviceVecs vec;
   GmresResult last_gmres;
   DevBuf<double> g, sol, p0, partials, rows, scal, mwork, xpart;
-  // the learned preconditioner: mode 1 records pairs, 2 solves with P⁻¹ + Q (plain on a miss)
+  // the learned preconditioner: mode 1 records pairs, 2 solves with (I + Q) P⁻¹ (plain on a miss)
   LearnedModel* learned = nullptr;
   int learned_mode = 0;
   DevBuf<double> pg, kr, kz;  // P⁻¹ g and the Krylov vectors of recorded pairs
   size_t last_iterations = 0;  // iterations of the last converged solve
-  int learned_misses = 0;        // consecutive P⁻¹ + Q solves that fell back to the plain one
+  int learned_misses = 0;        // consecutive (I + Q) P⁻¹ solves that fell back to the plain one
   static constexpr int kLearnedMissLimit = 2;  // after that many, the session solves plain
   static constexpr int kKrylovPairs = 3;  // pairs (A′ z, z) recorded per step after (g, A′⁻¹ g)
 

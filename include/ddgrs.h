@@ -224,10 +224,10 @@ typedef struct ddgrs_step_report {
   double seconds_trees, seconds_energy, seconds_metric, seconds_precond, seconds_solve, seconds_line_search,
       seconds_projection, seconds_total;
   /* The learned preconditioner (no counterpart in the reference): the session's mode (0 plain,
-   * 1 plain solve recording training pairs, 2 P⁻¹ + Q); learned_fallback 1 when the P⁻¹ + Q solve
+   * 1 plain solve recording training pairs, 2 (I + Q) P⁻¹); learned_fallback 1 when that solve
    * stagnated and the plain solve's result was used (then `gmres` is the plain solve's report),
    * 2 when Q was not tried because the two solves before fell back (the session then stays plain
-   * until set_learned is called again); the iterations of the P⁻¹ + Q solve. */
+   * until set_learned is called again); the iterations of the (I + Q) P⁻¹ solve. */
   int32_t learned_mode, learned_fallback;
   int64_t learned_iterations;
   double seconds_learned; /* features and the network's forward pass */
@@ -250,11 +250,12 @@ DDGRS_API int ddgrs_session_state(ddgrs_session* h, double* out5, ddgrs_error* e
  * No counterpart in the reference, whose solver has the plain preconditioner only.  A mesh
  * network (per-vertex features: position, unit normal, vertex area, mean curvature; four
  * message-passing layers of width 64) maps the geometry to per-vertex 3×3 blocks B_v and per-edge
- * 3×3 blocks G_uv = ½ (g_u + g_v); the step's preconditioner becomes z = P⁻¹ r + Q r with
- *   (Q r)_v = s (B_v r_v + Σ_{u ∈ N(v)} G_uv r_u)
+ * 3×3 blocks G_uv = ½ (g_u + g_v); the step's preconditioner becomes z = (I + Q) P⁻¹ r with
+ *   (Q x)_v = s (B_v x_v + Σ_{u ∈ N(v)} G_uv x_u)
  * on the vertex rows (the constraint rows unchanged), a linear map, so GMRES stays a Krylov method
- * with a fixed right preconditioner.  Trained on pairs (r, P⁻¹ r, y = A′⁻¹ r) with the loss
- * ‖P⁻¹ r + Q r − y‖² / ‖y − P⁻¹ r‖² (1 at Q = 0, where the output layer starts).  Vectors are
+ * with a fixed right preconditioner.  Trained on pairs (r, P⁻¹ r, y = A′⁻¹ r): with the scalar
+ * c = ⟨y, P⁻¹ r⟩ / ‖P⁻¹ r‖² (a scalar does not change GMRES's Krylov spaces) and x = c P⁻¹ r, the
+ * loss is ‖x + Q x − y‖² / ‖y − x‖² (1 at Q = 0, where the output layer starts).  Vectors are
  * saddle vectors of 3·nv + 3 (FP64, host), positions 3·nv. */
 /* A network for the surface's mesh (parameters from `seed`) with room for `max_pairs` pairs. */
 DDGRS_API int ddgrs_learned_create(ddgrs_surface* s, uint64_t seed, int64_t max_pairs, ddgrs_learned** out,
@@ -264,7 +265,7 @@ DDGRS_API void ddgrs_learned_destroy(ddgrs_learned* l);
 DDGRS_API int ddgrs_learned_info(ddgrs_learned* l, int64_t* out4, double* scale, ddgrs_error* err);
 /* Copies the FP32 parameters out (`get`) and then in (`set`); either may be null. */
 DDGRS_API int ddgrs_learned_params(ddgrs_learned* l, float* get, const float* set, ddgrs_error* err);
-/* Sets s (else the first training sets it to the pairs' median ‖y − P⁻¹ r‖ / ‖r‖). */
+/* Sets s (else the first training sets it to the pairs' median ‖y − x‖ / ‖x‖). */
 DDGRS_API int ddgrs_learned_set_scale(ddgrs_learned* l, double scale, ddgrs_error* err);
 DDGRS_API int ddgrs_learned_add_pair(ddgrs_learned* l, const double* positions, const double* r, const double* pr,
                                      const double* y, ddgrs_error* err);
@@ -277,13 +278,13 @@ DDGRS_API int ddgrs_learned_loss_grad(ddgrs_learned* l, int64_t pair, double* lo
  * over all pairs before and after. */
 DDGRS_API int ddgrs_learned_train(ddgrs_learned* l, int64_t iterations, int64_t batch, double lr, double* loss2,
                                   ddgrs_error* err);
-/* zs = Q rs at the given positions (the learned part alone). */
+/* zs = Q rs at the given positions (the learned map alone, applied to the given vectors). */
 DDGRS_API int ddgrs_learned_apply_batch(ddgrs_learned* l, const double* positions, int64_t nb, const double* rs,
                                         double* zs, ddgrs_error* err);
 /* Attaches `l` (mesh of the session's surface) to the session: mode 1 solves with the plain
  * preconditioner and adds each step's pairs to `l` (r = the step's right-hand side g with y = the
  * direction, then three Krylov pairs r_k = A′ z_{k−1}, y = z_{k−1}, z₀ = P⁻¹ g, z_k = P⁻¹ r_k);
- * mode 2 solves with P⁻¹ + Q and, when that solve has not reached the tolerance within twice the
+ * mode 2 solves with (I + Q) P⁻¹ and, when that solve has not reached the tolerance within twice the
  * last converged solve's iterations plus 5 (one restart cycle before any converged solve; at
  * most the iteration cap), again with the plain one;
  * mode 0 (or l null) detaches.  `l` must stay alive while attached. */

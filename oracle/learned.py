@@ -11,7 +11,8 @@ library trains on the device, so that its features, forward pass, loss and gradi
   dimensionless mean curvature);
 * network: h0 = relu(F W0 + b0); four layers h <- relu([h, mean_N(h)] W_l + b_l) of width 64;
   o = h Wo + bo with o_v = (B_v row-major, g_v row-major);
-* (Q r)_v = s (B_v r_v + sum_{u in N(v)} (g_v + g_u)/2 r_u), loss |s q - e|^2 / |e|^2.
+* (Q x)_v = s (B_v x_v + sum_{u in N(v)} (g_v + g_u)/2 x_u), loss |s q - e|^2 / |e|^2 for a stored
+  pair (x, e) (the library stores x = c P^-1 r and e = y - x).
 
 Parity pins (``tests/test_oracle_learned.py``): the analytic gradient against central differences
 of the loss (every parameter group), the features against closed forms on a sphere (radial unit

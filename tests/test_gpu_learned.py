@@ -5,10 +5,10 @@
 * loss ≤ 1e-4 relative and the gradient of every parameter group ≤ 1e-3 relative (2-norm): the
   device's FP32 backward pass against the oracle's FP64 backpropagation (itself pinned by central
   differences on the CPU);
-* Q r: ≤ 1e-6 relative against the oracle's Q at the device's FP32 blocks;
+* Q x: ≤ 1e-6 relative against the oracle's Q at the device's FP32 blocks;
 * training and the gain: pairs recorded along plain descent trajectories, Adam on the device, and
-  on pairs of a held-out trajectory the trained Q brings P⁻¹ r closer to A′⁻¹ r (loss < 1, where
-  Q = 0 gives 1); with P⁻¹ + Q every step reaches the relative residual 1e-6.
+  on pairs of a held-out trajectory the trained Q brings x = c P⁻¹ r closer to A′⁻¹ r (c the
+  best scalar; loss < 1, where Q = 0 gives 1); with (I + Q) P⁻¹ every step reaches 1e-6.
 """
 import numpy as np
 import pytest
@@ -74,7 +74,7 @@ def test_loss_gradient_and_apply_match_oracle(pkg, l3):
     feat = net.features(p).astype(np.float64)
     ptr, nb = LN.neighbours(f, nv)
     # the pair is kept in FP32 on the device
-    r32, e32 = r.astype(np.float32).astype(np.float64), (y - pr).astype(np.float32).astype(np.float64)
+    r32, e32 = pr.astype(np.float32).astype(np.float64), (y - pr).astype(np.float32).astype(np.float64)
     e2 = float(np.sum((y - pr)[:3 * nv] ** 2))
     L_ref, g_ref = LN.loss_grad(prm.astype(np.float64), feat, ptr, nb, r32, e32, 0.5)
     L_ref *= float(np.sum(e32[:3 * nv] ** 2)) / e2
@@ -88,7 +88,7 @@ def test_loss_gradient_and_apply_match_oracle(pkg, l3):
         e = rel(g[o:o + c], g_ref[o:o + c])
         print(f"grad {k}: rel {e:.2e}")
         assert e <= 1e-3, k
-    # Q r at the device's blocks
+    # Q x at the device's blocks
     blocks = net.blocks(p).astype(np.float64)
     z = net.apply(p, r)
     q_ref = 0.5 * LN.q_apply(blocks, r[:3 * nv].reshape(nv, 3), ptr, nb)
@@ -98,8 +98,9 @@ def test_loss_gradient_and_apply_match_oracle(pkg, l3):
 
 def test_training_gains_over_plain(pkg, ctx):
     """Pairs from the plain step's solves on two jittered level-3 icospheres train Q; on a third
-    trajectory the trained Q reduces ‖P⁻¹ r + Q r − A′⁻¹ r‖² / ‖P⁻¹ r − A′⁻¹ r‖² below 1 on
-    average, and descent with P⁻¹ + Q (the plain solve on stagnation) converges at every step."""
+    trajectory the trained Q reduces ‖x + Q x − A′⁻¹ r‖² / ‖x − A′⁻¹ r‖² (x = c P⁻¹ r, c the best
+    scalar) below 1 on average, and descent with (I + Q) P⁻¹ (the plain solve on stagnation)
+    converges at every step."""
     gen = pkg.generate
     net = None
     for seed in (11, 12):
@@ -128,7 +129,7 @@ def test_training_gains_over_plain(pkg, ctx):
     held = [net.loss_grad(k, grad=False)[0] for k in range(80, net.info()["pairs"])]
     print("held-out losses", ["%.3f" % x for x in held])
     assert len(held) == 20 and np.mean(held) < 1.0
-    # P⁻¹ + Q against plain from the same positions
+    # (I + Q) P⁻¹ against plain from the same positions
     p, f = gen.config_icosphere_l3(14)
     its = {}
     for mode in (0, 2):
