@@ -73,9 +73,12 @@ def test_loss_gradient_and_apply_match_oracle(pkg, l3):
     loss, g = net.loss_grad(0)
     feat = net.features(p).astype(np.float64)
     ptr, nb = LN.neighbours(f, nv)
-    # the pair is kept in FP32 on the device
-    r32, e32 = pr.astype(np.float32).astype(np.float64), (y - pr).astype(np.float32).astype(np.float64)
-    e2 = float(np.sum((y - pr)[:3 * nv] ** 2))
+    # the pair is kept as x = c P⁻¹ r (c the best scalar) and e = y − x, in FP32 on the device
+    m = 3 * nv
+    c = float(np.dot(y[:m], pr[:m]) / np.dot(pr[:m], pr[:m]))
+    x, e = c * pr, y - c * pr
+    r32, e32 = x.astype(np.float32).astype(np.float64), e.astype(np.float32).astype(np.float64)
+    e2 = float(np.sum(e[:m] ** 2))
     L_ref, g_ref = LN.loss_grad(prm.astype(np.float64), feat, ptr, nb, r32, e32, 0.5)
     L_ref *= float(np.sum(e32[:3 * nv] ** 2)) / e2
     g_ref *= float(np.sum(e32[:3 * nv] ** 2)) / e2
