@@ -64,6 +64,64 @@ __global__ __launch_bounds__(256) void ln_rows_kernel(int n, const float* __rest
   }
 }
 
+// The same product for J = 64 on the matrix cores (FP32 MFMA 16×16×4, FP32 accumulation): a
+// workgroup takes 64 rows in two passes of 32; per pass wave w owns the 16-row group w & 1 and the
+// two 16-column tiles 2 (w >> 1) and 2 (w >> 1) + 1.  M (K × 64, K = I rounded up to 4, zero rows
+// appended) and the pass's 32 rows of X (row stride K + 1) are staged in LDS.  Lane l supplies
+// A(l % 16, k₀ + l / 16) and B(k₀ + l / 16, 16 t + l % 16) and holds D(4 (l / 16) + v, l % 16).
+using f32x4 = __attribute__((__vector_size__(4 * sizeof(float)))) float;
+
+template <int I>
+__global__ __launch_bounds__(256) void ln_rows_mfma_kernel(int n, const float* __restrict__ X1, int ld1, int I1,
+                                                           const float* __restrict__ X2, int ld2,
+                                                           const float* __restrict__ M, int msi, int msj,
+                                                           const float* __restrict__ bias, int relu,
+                                                           float* __restrict__ pre, float* __restrict__ Y, int ldy) {
+  constexpr int K = (I + 3) / 4 * 4, J = 64;
+  __shared__ float sm[K * J];
+  __shared__ float sx[32][K + 1];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  for (int e = tid; e < K * J; e += 256) {
+    const int i = e / J, j = e % J;
+    sm[e] = i < I ? M[(size_t)i * msi + (size_t)j * msj] : 0.f;
+  }
+  const int rg = w & 1, c0 = 32 * (w >> 1), li = lane & 15, lk = lane >> 4;
+  for (int p = 0; p < 2; ++p) {
+    const int v0 = blockIdx.x * 64 + 32 * p;
+    __syncthreads();
+    for (int e = tid; e < 32 * K; e += 256) {
+      const int rr = e / K, i = e % K, v = v0 + rr;
+      float x = 0.f;
+      if (v < n && i < I) x = i < I1 ? X1[(size_t)v * ld1 + i] : X2[(size_t)v * ld2 + (i - I1)];
+      sx[rr][i] = x;
+    }
+    __syncthreads();
+    f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+    for (int k0 = 0; k0 < K; k0 += 4) {
+      const float a = sx[16 * rg + li][k0 + lk];
+      const float b0 = sm[(k0 + lk) * J + c0 + li];
+      const float b1 = sm[(k0 + lk) * J + c0 + 16 + li];
+      acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b0, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b1, acc1, 0, 0, 0);
+    }
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int j = c0 + 16 * t + li;
+      const float bj = bias ? bias[j] : 0.f;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int v = v0 + 16 * rg + 4 * lk + q;
+        if (v < n) {
+          const float y = (t ? acc1[q] : acc0[q]) + bj;
+          if (pre) pre[(size_t)v * ldy + j] = y;
+          Y[(size_t)v * ldy + j] = relu ? fmaxf(y, 0.f) : y;
+        }
+      }
+    }
+  }
+}
+
 // Partial weight gradients of one workgroup's rows: P[i][j] = Σ_v A[v][i] B[v][j] for i < I and,
 // as row I, the bias gradient Σ_v B[v][j]; A = [A1 (I1 columns) | A2].  Rows in chunks of 32 staged
 // in LDS; each thread keeps its outputs (t, t + 256, …) in registers.
@@ -92,25 +150,14 @@ __global__ __launch_bounds__(256) void ln_wgrad_kernel(int n, int rows_per_block
       sb[rr][j] = v < r1 ? B[(size_t)v * ldb + j] : 0.f;
     }
     __syncthreads();
-    if constexpr (J == 64) {
-      // thread (j, i0) owns rows i0 + 4q of column j: one B value per row of the chunk, A broadcast
-      const int j = tid & 63, i0 = tid >> 6;
-      for (int rr = 0; rr < 32; ++rr) {
-        const float b = sb[rr][j];
 #pragma unroll
-        for (int q = 0; q < NQ; ++q)
-          if (i0 + 4 * q <= I) acc[q] = fmaf(sa[rr][i0 + 4 * q], b, acc[q]);
-      }
-    } else {
-#pragma unroll
-      for (int q = 0; q < NQ; ++q) {
-        const int o = tid + 256 * q;
-        if (o < NO) {
-          const int i = o / J, j = o % J;
-          float s = acc[q];
-          for (int rr = 0; rr < 32; ++rr) s = fmaf(sa[rr][i], sb[rr][j], s);
-          acc[q] = s;
-        }
+    for (int q = 0; q < NQ; ++q) {
+      const int o = tid + 256 * q;
+      if (o < NO) {
+        const int i = o / J, j = o % J;
+        float s = acc[q];
+        for (int rr = 0; rr < 32; ++rr) s = fmaf(sa[rr][i], sb[rr][j], s);
+        acc[q] = s;
       }
     }
   }
@@ -119,6 +166,52 @@ __global__ __launch_bounds__(256) void ln_wgrad_kernel(int n, int rows_per_block
     const int o = tid + 256 * q;
     if (o < NO) part[(size_t)blockIdx.x * NO + o] = acc[q];
   }
+}
+
+// The same partial gradients for J = 64 on the matrix cores (FP32 MFMA 16×16×4): the output
+// (I + 1) × 64, rows padded to 16·NT, is NT × 4 tiles; wave w owns column tile w and all NT row
+// tiles.  Per step of 4 rows of the chunk, lane l supplies Aᵀ(16 t + l % 16, k₀ + l / 16) =
+// A(k₀ + l / 16, 16 t + l % 16) and B(k₀ + l / 16, 16 w + l % 16).
+template <int I>
+__global__ __launch_bounds__(256) void ln_wgrad_mfma_kernel(int n, int rows_per_block, const float* __restrict__ A1,
+                                                            int lda1, int I1, const float* __restrict__ A2, int lda2,
+                                                            const float* __restrict__ B, int ldb,
+                                                            float* __restrict__ part) {
+  constexpr int J = 64, NT = (I + 16) / 16, SA = 16 * NT, NO = (I + 1) * J;
+  __shared__ float sa[32][SA];
+  __shared__ float sb[32][J];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, li = lane & 15, lk = lane >> 4;
+  f32x4 acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int r0 = blockIdx.x * rows_per_block, r1 = min(n, r0 + rows_per_block);
+  for (int c0 = r0; c0 < r1; c0 += 32) {
+    __syncthreads();
+    for (int e = tid; e < 32 * SA; e += 256) {
+      const int rr = e / SA, i = e % SA, v = c0 + rr;
+      float x = 0.f;
+      if (v < r1 && i <= I) x = i == I ? 1.f : (i < I1 ? A1[(size_t)v * lda1 + i] : A2[(size_t)v * lda2 + (i - I1)]);
+      sa[rr][i] = x;
+    }
+    for (int e = tid; e < 32 * J; e += 256) {
+      const int rr = e / J, j = e % J, v = c0 + rr;
+      sb[rr][j] = v < r1 ? B[(size_t)v * ldb + j] : 0.f;
+    }
+    __syncthreads();
+#pragma unroll 2
+    for (int k0 = 0; k0 < 32; k0 += 4) {
+      const float b = sb[k0 + lk][16 * w + li];
+#pragma unroll
+      for (int t = 0; t < NT; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(sa[k0 + lk][16 * t + li], b, acc[t], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NT; ++t)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int i = 16 * t + 4 * lk + q;
+      if (i <= I) part[(size_t)blockIdx.x * NO + (size_t)i * J + 16 * w + li] = acc[t][q];
+    }
 }
 
 // grad[w + i J + j] += Σ_b part[b][i][j] (i < I), grad[bias + j] += Σ_b part[b][I][j].  A workgroup
@@ -281,8 +374,12 @@ __global__ __launch_bounds__(256) void ln_apply_kernel(int n, const uint32_t* __
 template <int I, int J>
 void rows(ddgrs_ctx* ctx, int n, const float* X1, int ld1, int I1, const float* X2, int ld2, const float* M, int msi,
           int msj, const float* bias, int relu, float* pre, float* Y, int ldy) {
-  launch(ctx, "ln_rows", ln_rows_kernel<I, J>, dim3(blocks_for(n, 64)), dim3(256), 0, n, X1, ld1, I1, X2, ld2, M, msi,
-         msj, bias, relu, pre, Y, ldy);
+  if constexpr (J == 64)
+    launch(ctx, "ln_rows", ln_rows_mfma_kernel<I>, dim3(blocks_for(n, 64)), dim3(256), 0, n, X1, ld1, I1, X2, ld2, M,
+           msi, msj, bias, relu, pre, Y, ldy);
+  else
+    launch(ctx, "ln_rows", ln_rows_kernel<I, J>, dim3(blocks_for(n, 64)), dim3(256), 0, n, X1, ld1, I1, X2, ld2, M, msi,
+           msj, bias, relu, pre, Y, ldy);
 }
 
 constexpr int kGradBlocks = 512;
@@ -291,8 +388,12 @@ template <int I, int J>
 void wgrad(ddgrs_ctx* ctx, int n, const float* A1, int lda1, int I1, const float* A2, int lda2, const float* B, int ldb,
            float* part, float* gw, float* gb) {
   const int per = (n + kGradBlocks - 1) / kGradBlocks;
-  launch(ctx, "ln_wgrad", ln_wgrad_kernel<I, J>, dim3(kGradBlocks), dim3(256), 0, n, per, A1, lda1, I1, A2, lda2, B, ldb,
-         part);
+  if constexpr (J == 64)
+    launch(ctx, "ln_wgrad", ln_wgrad_mfma_kernel<I>, dim3(kGradBlocks), dim3(256), 0, n, per, A1, lda1, I1, A2, lda2, B,
+           ldb, part);
+  else
+    launch(ctx, "ln_wgrad", ln_wgrad_kernel<I, J>, dim3(kGradBlocks), dim3(256), 0, n, per, A1, lda1, I1, A2, lda2, B,
+           ldb, part);
   launch(ctx, "ln_wgrad_sum", ln_wgrad_sum_kernel, dim3(blocks_for((I + 1) * J, 64)), dim3(256), 0, kGradBlocks, I, J,
          (const float*)part, gw, gb);
 }
