@@ -269,6 +269,14 @@ void Session::step(StepReport* rep) {
     Error e(DDGRS_NON_FINITE_ENERGY, "step", "the energy or its differential is not finite");
     throw e;
   }
+  // the learned preconditioner's blocks at this step's positions, while the other streams finish
+  const bool use_learned = learned && learned_mode == 2 && learned_misses < kLearnedMissLimit;
+  if (use_learned) {
+    const auto tl = std::chrono::steady_clock::now();
+    learned->features(ctx, geo.pos.data(), learned->feat.ptr);
+    learned->forward(ctx, learned->feat.ptr);
+    rep->seconds_learned = seconds_since(tl);
+  }
   // 2. the solver of A′ and the direction x = A′⁻¹ [g; 0]
   t = std::chrono::steady_clock::now();
   const double metric_build = metric.get();  // the third stream is synchronized
@@ -300,15 +308,10 @@ void Session::step(StepReport* rep) {
   rep->learned_mode = lmode;
   GmresResult gr;
   bool solved = false;
-  if (lmode == 2 && learned_misses >= kLearnedMissLimit) {
+  if (lmode == 2 && !use_learned) {
     rep->learned_fallback = 2;  // Q off for the rest of the session after consecutive misses
   } else if (lmode == 2) {
-    // (I + Q) P⁻¹: the network's blocks at this step's positions, then the solve; on a miss the
-    // plain solve below from x₀ = 0
-    const auto tl = std::chrono::steady_clock::now();
-    learned->features(ctx, geo.pos.data(), learned->feat.ptr);
-    learned->forward(ctx, learned->feat.ptr);
-    rep->seconds_learned = seconds_since(tl);
+    // (I + Q) P⁻¹ with the blocks formed above; on a miss the plain solve below from x₀ = 0
     kz.reserve(n);
     auto PQ = [&](const double* r, double* z) {
       pc.apply(ctx, r, z);
