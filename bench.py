@@ -166,6 +166,12 @@ def scale_run(args) -> int:
             print(f"bench: L7 step {k}: energy {e_dev:.12g}, oracle {e_orc:.12g}", file=sys.stderr, flush=True)
     ses.close()
     ms = (t1 - t0) * 1e3 / steps
+    learned = None
+    if not args.no_learned:
+        # retrained on trajectories at this size (fields of 64 bumps like the timed one)
+        surf.set_positions(p)
+        learned = learned_leg(pkg, ctx, args, opts, p, f, ms, [r.gmres.iterations for r in reps], level=7,
+                              bumps=(64, (0.0, 0.05), (0.05, 0.2)), steps=steps)
     line = {
         "metric": "Hs descent steps/sec at 327,680 faces (level-7 scale run)",
         "value": 1e3 / ms,
@@ -191,24 +197,27 @@ def scale_run(args) -> int:
             "unconverged_steps": sum(1 for r in reps if not r.gmres.converged),
             "energy_drift": drift,
         },
+        "learned": learned,
     }
     print(json.dumps(line))
     return 0
 
 
-def learned_leg(pkg, ctx, args, opts, p, f, plain_ms, plain_its):
-    """The learned preconditioner at L6: pairs from plain descent on `fields` bump fields other than
-    the timed one (`steps` each), Adam on the device, then the timed steps again from the same
-    start with P⁻¹ + Q (GMRES to the same 1e-6; the plain solve on a miss)."""
+def learned_leg(pkg, ctx, args, opts, p, f, plain_ms, plain_its, level=6, bumps=(16, (0.0, 0.1), (0.1, 0.4)),
+                steps=None):
+    """The learned preconditioner: pairs from plain descent on `fields` bump fields other than the
+    timed one (`steps` each), Adam on the device, then the timed steps again from the same start
+    with (I + Q) P⁻¹ (GMRES to the same 1e-6; the plain solve on a miss)."""
     gen = pkg.generate
-    base, _ = gen.icosphere(6)
+    steps = steps or args.steps
+    base, _ = gen.icosphere(level)
     surf = pkg.Surface(ctx, f, p)
     # four pairs per plain step: (g, A′⁻¹ g) and three Krylov pairs
     npairs = args.learned_pairs or 4 * args.learned_fields * args.learned_steps
     net = pkg.Learned(surf, seed=7, max_pairs=npairs)
     t0 = time.perf_counter()
     for k in range(args.learned_fields):
-        surf.set_positions(gen.bumps(base, 16, (0.0, 0.1), (0.1, 0.4), 1000 + k))
+        surf.set_positions(gen.bumps(base, bumps[0], bumps[1], bumps[2], 1000 + k))
         ses = pkg.Session(surf, opts)
         ses.set_learned(net, 1)
         for _ in range(args.learned_steps):
@@ -227,9 +236,9 @@ def learned_leg(pkg, ctx, args, opts, p, f, plain_ms, plain_its):
     ses.set_learned(net, 2)
     ctx.synchronize()
     t0 = time.perf_counter()
-    reps = [ses.step() for _ in range(args.steps)]
+    reps = [ses.step() for _ in range(steps)]
     ctx.synchronize()
-    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    ms = (time.perf_counter() - t0) * 1e3 / steps
     ses.close()
     # one profiled step: the network's forward kernels against the FP32 VALU peak
     surf.set_positions(p)
@@ -264,6 +273,7 @@ def learned_leg(pkg, ctx, args, opts, p, f, plain_ms, plain_its):
         "gmres_iterations_learned": [r.learned_iterations for r in reps],
         "gmres_iterations_plain": plain_its,
         "fallbacks": sum(1 for r in reps if r.learned_fallback == 1),
+        "fallback_rate": sum(1 for r in reps if r.learned_fallback) / len(reps),
         "skipped_after_misses": sum(1 for r in reps if r.learned_fallback == 2),
         "converged": all(r.gmres.converged for r in reps),
         "ms_per_step_learned": ms,
