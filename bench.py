@@ -193,7 +193,7 @@ def scale_run(args) -> int:
             "gmres_iterations": [r.gmres.iterations for r in reps],
             # steps whose solve switched to the extended near field (the FP32 products' rounding held the
             # explicit residual above 1e-6), and steps that ended unconverged
-            "fallback_rate": sum(1 for r in reps if r.gmres.extended) / steps,
+            "extended_solve_rate": sum(1 for r in reps if r.gmres.extended) / steps,
             "unconverged_steps": sum(1 for r in reps if not r.gmres.converged),
             "energy_drift": drift,
         },
