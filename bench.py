@@ -104,8 +104,8 @@ def measure_traffic(timeout_s: int = 240):
                 r = subprocess.run(["timeout", "-s", "KILL", str(timeout_s), prof, "--pmc", counter, "--output-format",
                                     "csv", "-d", d, "-o", "run", "--", *cmd], cwd="/tmp", env=env,
                                    stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
-                if r.returncode not in (0, 139) and not glob.glob(os.path.join(d, "**", "*.csv"), recursive=True):
-                    return {}
+                if r.returncode != 0:
+                    return {"error": f"{tag} under --pmc {counter} exited with {r.returncode}"}
             copy_kib = _counter_kib(os.path.join(tmp, f"{counter}_copy"), counter, "copy_kernel")
             if not copy_kib:
                 return {}
@@ -442,6 +442,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "learned": learned,
         "traffic_scale": {k: v for k, v in traffic.items() if k.endswith("_scale")},
+        "traffic_error": traffic.get("error"),
         "kernel_ms_step1": {k: round(v["ms"], 4) for k, v in sorted(kt.items(), key=lambda kv: -kv[1]["ms"])[:16]},
     }
     print(json.dumps(line))

@@ -19,9 +19,11 @@ binding raises; there is no CPU fallback.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import dataclasses
 import os
+import weakref
 
 import numpy as np
 
@@ -65,6 +67,24 @@ class _Err(ctypes.Structure):
 
 
 _lib = None
+
+# Live handles by teardown rank: sessions (which point at attached learned networks) first, then
+# the other handles built on a surface, then surfaces, then contexts.  The atexit hook closes them
+# in that order while the interpreter and the HIP runtime are still whole, so no destroy call is
+# left to the finalizers of interpreter shutdown, whose order is arbitrary.
+_LIVE = [weakref.WeakSet() for _ in range(4)]
+
+
+def _register(obj, rank: int):
+    _LIVE[rank].add(obj)
+
+
+@atexit.register
+def close_all():
+    """Closes every live handle in dependency order (also usable before a process forks or ends)."""
+    for live in _LIVE:
+        for obj in list(live):
+            obj.close()
 
 
 def library_path() -> str:
@@ -129,6 +149,7 @@ SIGNATURES = {
     "ddgrs_session_destroy": (None, [_P]),
     "ddgrs_session_step": (_I32, [_P, _P, _E]),
     "ddgrs_session_state": (_I32, [_P, _P, _E]),
+    "ddgrs_session_trials": (_I32, [_P, _I64, _P, _P, _P, ctypes.POINTER(_I64), _E]),
     "ddgrs_learned_create": (_I32, [_P, ctypes.c_uint64, _I64, ctypes.POINTER(_P), _E]),
     "ddgrs_learned_destroy": (None, [_P]),
     "ddgrs_learned_info": (_I32, [_P, _P, _P, _E]),
@@ -174,6 +195,7 @@ class Context:
         h = _P()
         _call(lib().ddgrs_ctx_create, ordinal, determinism, ctypes.byref(h))
         self._h = h
+        _register(self, 3)
 
     @property
     def handle(self):
@@ -224,6 +246,7 @@ class Surface:
         h = _P()
         _call(lib().ddgrs_surface_create, ctx.handle, self.nv, self.nf, _ptr(self.faces), _ptr(p), ctypes.byref(h))
         self._h = h
+        _register(self, 2)
 
     @property
     def handle(self):
@@ -309,6 +332,7 @@ class BhTree:
         h = _P()
         _call(lib().ddgrs_bh_build, surface.handle, opts.theta, opts.leaf_size, ctypes.byref(h))
         self._h = h
+        _register(self, 1)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -377,6 +401,12 @@ class ClusterTreeOptions:
 
 class _Handle:
     _destroy = None
+    _rank = 1  # teardown order (see close_all)
+
+    def __setattr__(self, name, value):
+        object.__setattr__(self, name, value)
+        if name == "_h" and value is not None:
+            _register(self, self._rank)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -463,6 +493,8 @@ class _StepOptionsC(ctypes.Structure):
         ("armijo_c", _D),
         ("tau_min_rel", _D),
         ("max_halvings", _I64),
+        ("start_kind", _I32),
+        ("start_value", _D),
     ]
 
 
@@ -491,6 +523,10 @@ class _StepReportC(ctypes.Structure):
         ("learned_fallback", _I32),
         ("learned_iterations", _I64),
         ("seconds_learned", _D),
+        ("energy_after", _D),
+        ("direction_norm", _D),
+        ("direction_max", _D),
+        ("halvings", _I64),
     ]
 
 
@@ -576,10 +612,42 @@ class Saddle(_Handle):
         return x, _gmres_report(rep)
 
 
+class StartRule:
+    """The line search's first step (src/step/options.rs ``StartRule``; session.rs:1604): values
+    are ``(kind, value)`` pairs for ``StepOptions.start``."""
+
+    GROW_PREVIOUS, ABSOLUTE, MAX_DISPLACEMENT, REFERENCE = 0, 1, 2, 3
+
+    @staticmethod
+    def grow_previous(lengths: float = 1.0):
+        """``GrowPrevious{factor: 2, first: MaxDisplacement{lengths}}`` (the default)."""
+        return (StartRule.GROW_PREVIOUS, float(lengths))
+
+    @staticmethod
+    def absolute(tau: float):
+        return (StartRule.ABSOLUTE, float(tau))
+
+    @staticmethod
+    def max_displacement(lengths: float):
+        return (StartRule.MAX_DISPLACEMENT, float(lengths))
+
+    @staticmethod
+    def reference():
+        """``Reference``: τ₀ = 2 / ‖x‖."""
+        return (StartRule.REFERENCE, 0.0)
+
+
+class TrialOutcome:
+    """``TrialOutcome`` (src/step/session.rs)."""
+
+    ACCEPTED, ARMIJO_FAILED, NON_FINITE = 0, 1, 2
+
+
 @dataclasses.dataclass(frozen=True)
 class StepOptions:
     """A step of the Hs flow (src/step/session.rs ``FlowOptions`` with remeshing and CCD off, the
-    global barycenter constraint, ``StartRule::GrowPrevious{2, MaxDisplacement{1}}``)."""
+    global barycenter constraint; ``start`` defaults to ``StartRule::GrowPrevious{2,
+    MaxDisplacement{1}}``, ``max_halvings`` < 0 is no cap)."""
 
     exponents: Exponents = Exponents()
     energy_exact: bool = False
@@ -589,11 +657,13 @@ class StepOptions:
     armijo_c: float = 1e-2
     tau_min_rel: float = 1e-12
     max_halvings: int = -1
+    start: tuple = (0, 1.0)
 
     def _c(self) -> _StepOptionsC:
         return _StepOptionsC(self.exponents.alpha, self.exponents.beta, 1 if self.energy_exact else 0, self.theta,
                              self.saddle._c(), self.gmres.restart, self.gmres.tolerance, self.gmres.max_iterations,
-                             self.armijo_c, self.tau_min_rel, self.max_halvings)
+                             self.armijo_c, self.tau_min_rel, self.max_halvings, int(self.start[0]),
+                             float(self.start[1]))
 
 
 @dataclasses.dataclass(frozen=True)
@@ -615,6 +685,11 @@ class StepReport:
     learned_mode: int = 0
     learned_fallback: int = 0  # 1: stagnated, plain used; 2: Q off after consecutive misses
     learned_iterations: int = 0
+    energy_after: float = 0.0  # F at the projected positions
+    direction_norm: float = 0.0
+    direction_max: float = 0.0
+    halvings: int = 0
+    trial_list: tuple = ()  # (τ, F, TrialOutcome) per trial, in order
 
 
 class Session(_Handle):
@@ -622,6 +697,7 @@ class Session(_Handle):
     moves the surface (``Session::step``, :724)."""
 
     _destroy = "ddgrs_session_destroy"
+    _rank = 0
 
     def __init__(self, surface: Surface, options: StepOptions = StepOptions()):
         self.surface = surface
@@ -638,7 +714,18 @@ class Session(_Handle):
         return StepReport(float(r.energy_before), float(r.energy_trial), float(r.gradient_norm), float(r.sobolev_norm_sq),
                           float(r.tau), float(r.tau0), bool(r.accepted), int(r.trials), _gmres_report(r.gmres),
                           int(r.pairs_exact), int(r.nodes_lumped), sec, int(r.learned_mode), int(r.learned_fallback),
-                          int(r.learned_iterations))
+                          int(r.learned_iterations), float(r.energy_after), float(r.direction_norm),
+                          float(r.direction_max), int(r.halvings), self.trials())
+
+    def trials(self) -> tuple:
+        """The last step's trials as (τ, F, ``TrialOutcome``) in order (``StepReport::trials``)."""
+        n = _I64()
+        _call(lib().ddgrs_session_trials, self._h, 0, None, None, None, ctypes.byref(n))
+        k = int(n.value)
+        tau, f, o = np.zeros(k), np.zeros(k), np.zeros(k, dtype=np.int32)
+        if k:
+            _call(lib().ddgrs_session_trials, self._h, k, _ptr(tau), _ptr(f), _ptr(o), ctypes.byref(n))
+        return tuple((float(a), float(b), int(c)) for a, b, c in zip(tau, f, o))
 
     def set_learned(self, learned: "Learned | None", mode: int):
         """Attach a learned preconditioner: mode 1 records a training pair per step (plain solve),

@@ -182,6 +182,8 @@ void ddgrs_step_options_default(ddgrs_step_options* o) {
   o->armijo_c = 1e-2;
   o->tau_min_rel = 1e-12;
   o->max_halvings = -1;
+  o->start_kind = DDGRS_START_GROW_PREVIOUS;
+  o->start_value = 1.0;
 }
 
 int ddgrs_session_create(ddgrs_surface* s, const ddgrs_step_options* o, ddgrs_session** out, ddgrs_error* err) {
@@ -199,6 +201,10 @@ int ddgrs_session_create(ddgrs_surface* s, const ddgrs_step_options* o, ddgrs_se
       invalid("armijo_c", std::to_string(o->armijo_c), "in (0, 1)");
     if (!(std::isfinite(o->tau_min_rel) && o->tau_min_rel > 0.0))
       invalid("tau_min_rel", std::to_string(o->tau_min_rel), "a finite number > 0");
+    if (o->start_kind < DDGRS_START_GROW_PREVIOUS || o->start_kind > DDGRS_START_REFERENCE)
+      invalid("start_kind", std::to_string(o->start_kind), "0 (grow previous), 1 (absolute), 2 (max displacement) or 3 (reference)");
+    if (o->start_kind != DDGRS_START_REFERENCE && !(std::isfinite(o->start_value) && o->start_value > 0.0))
+      invalid("start_value", std::to_string(o->start_value), "a finite number > 0");
     StepOptions so;
     so.alpha = o->alpha;
     so.beta = o->beta;
@@ -216,6 +222,8 @@ int ddgrs_session_create(ddgrs_surface* s, const ddgrs_step_options* o, ddgrs_se
     so.armijo_c = o->armijo_c;
     so.tau_min_rel = o->tau_min_rel;
     so.max_halvings = (long)o->max_halvings;
+    so.start_kind = o->start_kind;
+    so.start_value = o->start_value;
     auto h = std::make_unique<ddgrs_session>();
     h->s = s;
     h->ses.init(s, so);
@@ -231,6 +239,21 @@ int ddgrs_session_step(ddgrs_session* h, ddgrs_step_report* report, ddgrs_error*
     need(h, "session");
     ddgrs_step_report local;
     h->ses.step(report ? report : &local);
+  });
+}
+
+int ddgrs_session_trials(ddgrs_session* h, int64_t capacity, double* tau, double* energy, int32_t* outcome,
+                         int64_t* count, ddgrs_error* err) {
+  return guarded(err, "session_trials", [&] {
+    need(h, "session");
+    need(count, "count");
+    const auto& tr = h->ses.trials;
+    *count = (int64_t)tr.size();
+    for (int64_t i = 0; i < capacity && i < (int64_t)tr.size(); ++i) {
+      if (tau) tau[i] = tr[i].tau;
+      if (energy) energy[i] = tr[i].energy;
+      if (outcome) outcome[i] = tr[i].outcome;
+    }
   });
 }
 

@@ -1135,11 +1135,14 @@ __global__ __launch_bounds__(256) void lf_solve_one_kernel(FrontArgs A, OneArgs 
   // of thread 0 alone; the other lanes of its wave then reached the next barrier first and the
   // workgroup re-read the old ticket forever.)  The ticket and the wait's outcome are read as
   // wave-uniform values, so that no loop exit looks divergent.
+  // The loop is bounded on its own: a workgroup takes at most ntasks + 1 tickets (each iteration
+  // takes a new one, so a correct run leaves after at most that many), and leaves as soon as
+  // another workgroup has flagged a failed wait.
   uint32_t* done = nullptr;  // the counter of the task just finished
-  for (;;) {
+  for (uint32_t iter = 0; iter <= O.ntasks; ++iter) {
     if (threadIdx.x == 0) {
       if (done) atomicAdd(done, 1u);
-      s_task = atomicAdd(O.ticket, 1u);
+      s_task = read_count(O.failed) != 0u ? O.ntasks : atomicAdd(O.ticket, 1u);
     }
     __syncthreads();
     const uint32_t tk = __builtin_amdgcn_readfirstlane(s_task);
@@ -1199,6 +1202,7 @@ __global__ __launch_bounds__(256) void lf_solve_one_kernel(FrontArgs A, OneArgs 
     __syncthreads();
     done = (task.kind < kTaskBwdSmall ? O.fdone : O.bdone) + J;
   }
+  if (threadIdx.x == 0) atomicOr(O.failed, 1u);  // more tickets than tasks: the solve is invalid
 }
 
 // L_SJᵀ of the large fronts (k × (f − k), row-major), 32 × 32 tiles through LDS.
@@ -1441,8 +1445,12 @@ const char* height_name(const char* base, uint32_t h) {
 void LprimeFactorDev::solve(ddgrs_ctx* ctx, const double* r, double* x) {
   const uint32_t un = (uint32_t)n, unv = (uint32_t)nv;
   const FrontArgs A{bstart.ptr, bk.ptr, bf.ptr, bparent.ptr, child_ptr.ptr, child.ptr, ea_off.ptr, ea_map.ptr, foff.ptr};
+  // The one-launch solve hung on the device twice and its restructured loop has not run there
+  // since; until it has, asking for it is an error rather than a launch.
   const char* e = std::getenv("DDGRS_LPRIME_SOLVE_ONE_LAUNCH");
-  const bool one = one_ok && e && e[0] == '1';
+  if (e && e[0] == '1')
+    invalid("DDGRS_LPRIME_SOLVE_ONE_LAUNCH", e, "unset: the one-launch L' solve is disabled (it hung on the device)");
+  const bool one = false;
   const uint32_t nzero = one ? 2 * nb + 1 : 0;
   launch(ctx, "lf_gather_rhs", lf_gather_rhs_kernel, dim3(blocks_for(std::max<int64_t>(3 * (int64_t)n, nzero), 256)),
          dim3(256), 0, un, unv, (const uint32_t*)perm.ptr, r, xb.ptr, nzero, task_done.ptr);

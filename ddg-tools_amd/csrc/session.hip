@@ -1,9 +1,9 @@
 // One Hs descent step on the device (RS/src/step/session.rs:749 step_inner, remeshing and CCD off):
 // energy and differential (Barnes–Hut on frozen lists, or exact all pairs), the hierarchical
 // saddle solver of the step (metric tree on the energy's face tree, iterative.rs:487-560), the
-// direction by right-preconditioned GMRES, the Armijo line search with the GrowPrevious /
-// MaxDisplacement start (session.rs:991-1069, :1604), and the projection onto the global
-// barycenter constraint.  Positions and every vector stay on the device; the host reads scalars
+// direction by right-preconditioned GMRES, the Armijo line search with the reference's start rules
+// and halving cap (session.rs:906, :958-1069, :1604), and the projection onto the global
+// barycenter constraint with F at the projected positions.  Positions and every vector stay on the device; the host reads scalars
 // (energies, the Armijo test, norms) and builds the tree structures.
 #include <algorithm>
 #include <chrono>
@@ -50,29 +50,34 @@ __global__ __launch_bounds__(256) void translate_kernel(int64_t nv, double* __re
   pos[3 * v + 2] += sz;
 }
 
-// max over v of |x_v| (Euclidean) and of |x_i|: order-free maxima, two levels.
+// Per block of 256 vertices: max over v of |x_v| (Euclidean), max of |x_i| (order-free maxima) and
+// Σ x_i² (a fixed tree; the host adds the blocks in order).
 __global__ __launch_bounds__(256) void xmax_kernel(int64_t nv, const double* __restrict__ x, double* __restrict__ part) {
-  __shared__ double s0[256], s1[256];
+  __shared__ double s0[256], s1[256], s2[256];
   int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  double d = 0.0, m = 0.0;
+  double d = 0.0, m = 0.0, q = 0.0;
   if (v < nv) {
     double a = x[3 * v], b = x[3 * v + 1], c = x[3 * v + 2];
-    d = sqrt(a * a + b * b + c * c);
+    q = a * a + b * b + c * c;
+    d = sqrt(q);
     m = fmax(fabs(a), fmax(fabs(b), fabs(c)));
   }
   s0[threadIdx.x] = d;
   s1[threadIdx.x] = m;
+  s2[threadIdx.x] = q;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) {
       s0[threadIdx.x] = fmax(s0[threadIdx.x], s0[threadIdx.x + w]);
       s1[threadIdx.x] = fmax(s1[threadIdx.x], s1[threadIdx.x + w]);
+      s2[threadIdx.x] = s2[threadIdx.x] + s2[threadIdx.x + w];
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = s0[0];
-    part[2 * blockIdx.x + 1] = s1[0];
+    part[3 * blockIdx.x] = s0[0];
+    part[3 * blockIdx.x + 1] = s1[0];
+    part[3 * blockIdx.x + 2] = s2[0];
   }
 }
 
@@ -134,7 +139,7 @@ void Session::init(ddgrs_surface* s_, const StepOptions& o) {
   rows.reserve((size_t)s->nf);
   scal.reserve(16);
   mwork.reserve(4 * nv);
-  xpart.reserve(2 * (size_t)blocks_for((int64_t)nv, 256));
+  xpart.reserve(3 * (size_t)blocks_for((int64_t)nv, 256));
 }
 
 double Session::read_scalar(const double* d) {
@@ -372,41 +377,56 @@ void Session::step(StepReport* rep) {
     e.value = gr.relative_residual;
     throw e;
   }
-  // ⟨g, x⟩, ‖x‖_∞ and the largest vertex displacement
+  // ⟨g, x⟩, ‖x‖, ‖x‖_∞ and the largest vertex displacement
   vec.dots(ctx, g.ptr, 1, sol.ptr, scal.ptr + 1);
   const unsigned nbv = blocks_for(nv, 256);
   launch(ctx, "xmax", xmax_kernel, dim3(nbv), dim3(256), 0, nv, (const double*)sol.ptr, xpart.ptr);
-  std::vector<double> xp(2 * (size_t)nbv);
-  download(ctx, xp.data(), (const double*)xpart.ptr, 2 * (size_t)nbv);
+  std::vector<double> xp(3 * (size_t)nbv);
+  download(ctx, xp.data(), (const double*)xpart.ptr, 3 * (size_t)nbv);
   const double slope = read_scalar(scal.ptr + 1);
-  double maxd = 0.0, xinf = 0.0;
+  double maxd = 0.0, xinf = 0.0, xn2 = 0.0;
   for (unsigned b = 0; b < nbv; ++b) {
-    maxd = std::fmax(maxd, xp[2 * b]);
-    xinf = std::fmax(xinf, xp[2 * b + 1]);
+    maxd = std::fmax(maxd, xp[3 * b]);
+    xinf = std::fmax(xinf, xp[3 * b + 1]);
+    xn2 += xp[3 * b + 2];
   }
+  const double xnorm = std::sqrt(xn2);
   rep->sobolev_norm_sq = slope;
-  // 3. line search (Armijo, halving; start GrowPrevious{2, MaxDisplacement{1}})
+  rep->direction_norm = xnorm;
+  rep->direction_max = xinf;
+  // 3. line search (Armijo, halving; the start rule of session.rs:1604 start_tau)
   t = std::chrono::steady_clock::now();
-  const double tau0 = last_tau > 0.0 ? 2.0 * last_tau : l0 / maxd;
+  double tau0;
+  switch (opts.start_kind) {
+    case DDGRS_START_ABSOLUTE: tau0 = opts.start_value; break;
+    case DDGRS_START_MAX_DISPLACEMENT: tau0 = opts.start_value * l0 / maxd; break;
+    case DDGRS_START_REFERENCE: tau0 = 2.0 / xnorm; break;
+    default: tau0 = last_tau > 0.0 ? 2.0 * last_tau : opts.start_value * l0 / maxd; break;
+  }
   const double tau_min = opts.tau_min_rel * l0 / xinf;
   rep->tau0 = tau0;
   HIPCHECK(hipMemcpyAsync(p0.ptr, s->positions.ptr, nd * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
   bool accepted = false;
   double tau = tau0;
+  trials.clear();
   if (slope > 0.0 && std::isfinite(tau0) && tau0 > 0.0) {
-    while (tau >= tau_min && (opts.max_halvings < 0 || rep->trials <= opts.max_halvings)) {
+    for (;;) {
+      if (!(tau >= tau_min) || (opts.max_halvings >= 0 && rep->halvings > opts.max_halvings)) break;
       launch(ctx, "trial", trial_kernel, dim3(blocks_for((int64_t)nd, 256)), dim3(256), 0, nd, (const double*)p0.ptr,
              (const double*)sol.ptr, tau, s->positions.ptr);
       compute_face_geometry(s, s->packs, false);
       if (!opts.energy_exact) bh.refresh(s);
       const double ft = energy_value();
       rep->trials++;
-      if (std::isfinite(ft) && ft <= f0 - opts.armijo_c * tau * slope) {
+      const bool ok = std::isfinite(ft) && ft <= f0 - opts.armijo_c * tau * slope;
+      trials.push_back({tau, std::isfinite(ft) ? ft : NAN, !std::isfinite(ft) ? 2 : ok ? 0 : 1});
+      if (ok) {
         accepted = true;
         rep->energy_trial = ft;
         break;
       }
       tau *= 0.5;
+      rep->halvings++;
     }
   }
   if (!accepted) {
@@ -415,9 +435,12 @@ void Session::step(StepReport* rep) {
     rep->energy_trial = f0;
   }
   rep->seconds_line_search = seconds_since(t);
-  // 4. projection onto the barycenter constraint
+  // 4. projection onto the barycenter constraint, then F at the projected positions
+  // (session.rs:1505 `after`)
   t = std::chrono::steady_clock::now();
   project();
+  if (!opts.energy_exact) bh.refresh(s);
+  rep->energy_after = energy_value();
   rep->seconds_projection = seconds_since(t);
   rep->accepted = accepted ? 1 : 0;
   rep->tau = accepted ? tau : 0.0;

@@ -26,6 +26,14 @@ struct StepOptions {
   double armijo_c = 1e-2;
   double tau_min_rel = 1e-12;    // τ_min = tau_min_rel · ℓ̄₀ / ‖x‖_∞
   long max_halvings = -1;        // no cap when negative
+  int start_kind = DDGRS_START_GROW_PREVIOUS;
+  double start_value = 1.0;      // τ (absolute) or the lengths of MaxDisplacement
+};
+
+// TrialOutcome (RS/src/step/session.rs): 0 accepted, 1 Armijo failed, 2 non-finite.
+struct TrialRecord {
+  double tau, energy;
+  int32_t outcome;
 };
 
 using StepReport = ddgrs_step_report;
@@ -53,6 +61,7 @@ struct Session {
   LearnedModel* learned = nullptr;
   int learned_mode = 0;
   DevBuf<double> pg, kr, kz;  // P⁻¹ g and the Krylov vectors of recorded pairs
+  std::vector<TrialRecord> trials;  // the last step's line search
   size_t last_iterations = 0;  // iterations of the last converged solve
   int learned_misses = 0;        // consecutive (I + Q) P⁻¹ solves that fell back to the plain one
   static constexpr int kLearnedMissLimit = 2;  // after that many, the session solves plain

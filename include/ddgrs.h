@@ -200,8 +200,16 @@ DDGRS_API int ddgrs_saddle_solve(ddgrs_saddle* h, const double* b, double* x, in
 
 /* ---- the descent session -------------------------------------------------------------------- */
 
+/* The line search's first step (RS/src/step/options.rs StartRule; session.rs:1604 start_tau). */
+enum ddgrs_start_kind {
+  DDGRS_START_GROW_PREVIOUS = 0, /* GrowPrevious{factor 2, first MaxDisplacement{start_value}} */
+  DDGRS_START_ABSOLUTE = 1,      /* Absolute{tau = start_value} */
+  DDGRS_START_MAX_DISPLACEMENT = 2, /* MaxDisplacement{lengths = start_value}: start_value ℓ̄₀ / max_v |x_v| */
+  DDGRS_START_REFERENCE = 3      /* Reference: 2 / ‖x‖ */
+};
+
 /* Options of a step (RS/src/step/session.rs FlowOptions with the Hs method, remeshing and CCD
- * off, the global barycenter constraint, StartRule::GrowPrevious{2, MaxDisplacement{1}}). */
+ * off, the global barycenter constraint; the default start is GrowPrevious{2, MaxDisplacement{1}}). */
 typedef struct ddgrs_step_options {
   double alpha, beta;
   int32_t energy_exact;    /* 1: exact all pairs; 0: Barnes–Hut */
@@ -212,10 +220,14 @@ typedef struct ddgrs_step_options {
   int64_t max_iterations;
   double armijo_c;         /* sufficient decrease constant */
   double tau_min_rel;      /* τ_min = tau_min_rel · ℓ̄₀ / ‖x‖_∞ */
-  int64_t max_halvings;    /* < 0: no cap */
+  int64_t max_halvings;    /* < 0: no cap (LineSearch::max_halvings) */
+  int32_t start_kind;      /* enum ddgrs_start_kind */
+  double start_value;      /* τ of Absolute, the lengths of MaxDisplacement (GrowPrevious's first) */
 } ddgrs_step_options;
 
-/* A step's report (RS/src/step/session.rs:146 StepReport, the fields this path produces). */
+/* A step's report (RS/src/step/session.rs:146 StepReport, the fields this path produces).
+ * accepted 0 is StepStatus::Stuck: the unmoved positions are projected.  energy_after is F at the
+ * projected positions; direction_norm ‖x‖ and direction_max ‖x‖_∞ of the direction. */
 typedef struct ddgrs_step_report {
   double energy_before, energy_trial, gradient_norm, sobolev_norm_sq, tau, tau0;
   int32_t accepted, trials;
@@ -231,7 +243,15 @@ typedef struct ddgrs_step_report {
   int32_t learned_mode, learned_fallback;
   int64_t learned_iterations;
   double seconds_learned; /* features and the network's forward pass */
+  double energy_after, direction_norm, direction_max;
+  int64_t halvings;
 } ddgrs_step_report;
+
+/* The trials of the last step in order (StepReport::trials, RS/src/step/session.rs:136 Trial): τ,
+ * F (NaN when not finite) and the outcome (TrialOutcome: 0 accepted, 1 Armijo failed, 2 non-finite);
+ * at most `capacity` are written, *count is the number of trials. */
+DDGRS_API int ddgrs_session_trials(ddgrs_session* h, int64_t capacity, double* tau, double* energy, int32_t* outcome,
+                                   int64_t* count, ddgrs_error* err);
 
 /* Fills the reference defaults ((6, 12), Barnes–Hut θ 0.5, restart 50, 1e-6, 200, c 1e-2). */
 DDGRS_API void ddgrs_step_options_default(ddgrs_step_options* o);

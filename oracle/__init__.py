@@ -19,9 +19,10 @@ Parity pins (see the oracle tests, ``tests/test_oracle_*.py``):
 * metric B, B0 (chi = 0): the Lean ``rs-*-metric`` fixtures; the null space; symmetry;
 * preconditioner and saddle solve: the Lean ``rs-icosa-precond`` and ``rs-octa-saddle-none``
   fixtures, an independent numpy assembly of K and L', GMRES against a dense solve;
-* the descent step (``Session.step``): parity unpinned beyond properties (Armijo condition,
-  energy decrease, barycenter kept) -- the reference's step fixtures all carry area, volume or
-  pin constraints that this path does not build.
+* the descent step (``Session.step``): the Lean ``rs-icosa-step-stuck`` fixture (barycenter only,
+  exact energy, dense metric, absolute start, capped halvings: F0, slope, ||x||, lbar, tau0, every
+  trial and the stuck result at the reference's 1e-10), plus the Armijo condition, the energy
+  decrease and the kept barycenter of accepted steps.
 """
 from __future__ import annotations
 
@@ -115,10 +116,11 @@ def _declare(L):
     L.orc_precond_apply.argtypes = [_P, _P, _P, _INT]
     L.orc_lprime_solve.argtypes = [_P, _P, _P]
     L.orc_saddle_solve.argtypes = [_P, _P, _P, _I64, _D, _I64, _INT, _P]
-    L.orc_session_create.argtypes = [_I64, _I64, _P, _P, _D, _D, _D, _D, _I64, _D, _I64, ctypes.POINTER(_INT)]
+    L.orc_face_operators.argtypes = [_I64, _I64, _P, _P, _P, _P, _P]
+    L.orc_session_create.argtypes = [_I64, _I64, _P, _P, _D, _D, _P, _P, ctypes.POINTER(_INT)]
     L.orc_session_create.restype = _P
     L.orc_session_free.argtypes = [_P]
-    L.orc_session_step.argtypes = [_P, _INT, _P]
+    L.orc_session_step.argtypes = [_P, _INT, _P, _P, _I64]
     L.orc_session_step.restype = _INT
     L.orc_session_positions.argtypes = [_P, _P]
     L.orc_fractional_symbol_ratio.argtypes = [_D]
@@ -349,15 +351,29 @@ class Saddle:
 
 
 class Session:
-    """The oracle's Hs descent session (step/session.rs:280) with Barnes–Hut energy, the hierarchical
-    saddle, a global barycenter constraint, remeshing and CCD off."""
+    """The oracle's Hs descent session (step/session.rs:280) with Barnes–Hut or exact energy, the
+    hierarchical saddle (χ = 0: the dense metric), the global barycenter constraint, remeshing and
+    CCD off.
 
-    def __init__(self, faces, positions, alpha=6.0, beta=12.0, theta=0.5, chi=0.5, restart=50, tol=1e-6, maxit=200):
+    ``start`` is the line search's first step (step/options.rs ``StartRule``, session.rs:1604):
+    ``("grow", lengths)`` = GrowPrevious{2, MaxDisplacement{lengths}} (the default),
+    ``("abs", tau)``, ``("maxdisp", lengths)`` or ``("reference",)`` (2/‖x‖).
+    ``max_halvings`` < 0: no cap."""
+
+    START_KINDS = {"grow": 0, "abs": 1, "maxdisp": 2, "reference": 3}
+
+    def __init__(self, faces, positions, alpha=6.0, beta=12.0, theta=0.5, chi=0.5, restart=50, tol=1e-6, maxit=200,
+                 energy_exact=False, start=("grow", 1.0), armijo_c=1e-2, tau_min_rel=1e-12, max_halvings=-1,
+                 c_lo=1.0, leaf_size=8):
         f, p = _mesh(faces, positions)
         self.nv = p.shape[0]
         st = _INT(0)
-        self._h = lib().orc_session_create(self.nv, f.shape[0], _ptr(f), _ptr(p), alpha, beta, theta, chi, restart, tol,
-                                           maxit, ctypes.byref(st))
+        kind = self.START_KINDS[start[0]]
+        value = float(start[1]) if len(start) > 1 else 0.0
+        of = np.array([theta, chi, tol, value, armijo_c, tau_min_rel, c_lo], dtype=np.float64)
+        oi = np.array([restart, maxit, 1 if energy_exact else 0, kind, max_halvings, leaf_size], dtype=np.int64)
+        self._h = lib().orc_session_create(self.nv, f.shape[0], _ptr(f), _ptr(p), alpha, beta, _ptr(of), _ptr(oi),
+                                           ctypes.byref(st))
         if not self._h:
             raise OracleError(st.value, "session")
 
@@ -368,13 +384,20 @@ class Session:
             self._h = None
 
     def step(self, threads=0):
-        rep = np.zeros(9)
-        st = lib().orc_session_step(self._h, threads, _ptr(rep))
+        """One step; the report's ``trials`` is a list of (τ, F, outcome) with outcome 0 accepted,
+        1 Armijo failed, 2 non-finite."""
+        cap = 256
+        rep = np.zeros(16)
+        tr = np.zeros((cap, 3))
+        st = lib().orc_session_step(self._h, threads, _ptr(rep), _ptr(tr), cap)
         if st != 0:
             raise OracleError(st, "step")
         keys = ["energy0", "energy1", "slope", "tau", "accepted", "trials", "gmres_iterations", "gmres_converged",
-                "gmres_relative_residual"]
-        return dict(zip(keys, rep.tolist()))
+                "gmres_relative_residual", "energy_after", "tau0", "gradient_norm", "direction_norm", "direction_max",
+                "halvings", "lbar"]
+        out = dict(zip(keys, rep.tolist()))
+        out["trial_list"] = [(float(t), float(e), int(o)) for t, e, o in tr[: min(cap, int(out["trials"]))]]
+        return out
 
     def positions(self):
         out = np.zeros((self.nv, 3))
@@ -384,3 +407,12 @@ class Session:
 
 def fractional_symbol_ratio(s):
     return lib().orc_fractional_symbol_ratio(s)
+
+
+def face_operators(faces, positions, u):
+    """(ū per face, D_f u per face (nf × 3)) for one vertex function u (metric/mod.rs:200-225)."""
+    f, p = _mesh(faces, positions)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    avg, grad = np.zeros(f.shape[0]), np.zeros((f.shape[0], 3))
+    lib().orc_face_operators(p.shape[0], f.shape[0], _ptr(f), _ptr(p), _ptr(u), _ptr(avg), _ptr(grad))
+    return avg, grad

@@ -296,8 +296,10 @@ void orc_saddle_solve(void* h, const double* rhs, double* x, int64_t restart, do
   report[5] = (double)rep.preconditioner_applications;
 }
 
+// opts_f: [θ, χ, tolerance, start value, Armijo c, τ_min rel, c_lo];
+// opts_i: [restart, max iterations, exact energy, start kind, max halvings (< 0: none), leaf size]
 void* orc_session_create(int64_t nv, int64_t nf, const uint32_t* faces, const double* pos, double alpha, double beta,
-                         double theta, double chi, int64_t restart, double tol, int64_t maxit, int* status) {
+                         const double* opts_f, const int64_t* opts_i, int* status) {
   Error err;
   Exponents e;
   if (!make_exps(alpha, beta, &e, &err)) {
@@ -305,11 +307,19 @@ void* orc_session_create(int64_t nv, int64_t nf, const uint32_t* faces, const do
     return nullptr;
   }
   SessionOptions o;
-  o.theta = theta;
-  o.saddle.chi = chi;
-  o.gmres.restart = (size_t)restart;
-  o.gmres.tolerance = tol;
-  o.gmres.max_iterations = (size_t)maxit;
+  o.theta = opts_f[0];
+  o.saddle.chi = opts_f[1];
+  o.gmres.tolerance = opts_f[2];
+  o.start_value = opts_f[3];
+  o.armijo_c = opts_f[4];
+  o.tau_min_rel = opts_f[5];
+  o.saddle.c_lo = opts_f[6];
+  o.gmres.restart = (size_t)opts_i[0];
+  o.gmres.max_iterations = (size_t)opts_i[1];
+  o.energy_exact = opts_i[2] != 0;
+  o.start_kind = (int)opts_i[3];
+  o.max_halvings = (long)opts_i[4];
+  o.saddle.leaf_size = (size_t)opts_i[5];
   auto* s = new Session();
   if (!session_create(make_mesh(nv, nf, faces), make_pos(nv, pos), e, o, s, &err)) {
     delete s;
@@ -322,11 +332,14 @@ void* orc_session_create(int64_t nv, int64_t nf, const uint32_t* faces, const do
 
 void orc_session_free(void* h) { delete static_cast<Session*>(h); }
 
-// report: [energy0, energy1, slope, tau, accepted, trials, gmres iterations, gmres converged, gmres relres]
-int orc_session_step(void* h, int threads, double* report) {
+// report: [energy0, energy1, slope, tau, accepted, trials, gmres iterations, gmres converged,
+// gmres relres, energy after the projection, τ₀, ‖g‖, ‖x‖, ‖x‖_∞, halvings, ℓ̄₀];
+// trials (cap × 3): per trial [τ, F, outcome (0 accepted, 1 Armijo failed, 2 non-finite)]
+int orc_session_step(void* h, int threads, double* report, double* trials, int64_t cap) {
   Error err;
   StepReport rep;
-  if (!session_step(*static_cast<Session*>(h), threads, &rep, &err)) return err.status;
+  auto* s = static_cast<Session*>(h);
+  if (!session_step(*s, threads, &rep, &err)) return err.status;
   report[0] = rep.energy0;
   report[1] = rep.energy1;
   report[2] = rep.slope;
@@ -336,6 +349,18 @@ int orc_session_step(void* h, int threads, double* report) {
   report[6] = (double)rep.gmres.iterations;
   report[7] = rep.gmres.converged ? 1.0 : 0.0;
   report[8] = rep.gmres.relative_residual;
+  report[9] = rep.energy_after;
+  report[10] = rep.tau0;
+  report[11] = rep.gradient_norm;
+  report[12] = rep.direction_norm;
+  report[13] = rep.direction_max;
+  report[14] = (double)rep.halvings;
+  report[15] = s->l0;
+  for (size_t t = 0; t < rep.trial.size() && (int64_t)t < cap; ++t) {
+    trials[3 * t] = rep.trial[t].tau;
+    trials[3 * t + 1] = rep.trial[t].energy;
+    trials[3 * t + 2] = (double)rep.trial[t].outcome;
+  }
   return OK;
 }
 
@@ -346,5 +371,22 @@ void orc_session_positions(void* h, double* out) {
 }
 
 double orc_fractional_symbol_ratio(double s) { return fractional_symbol_ratio(s); }
+
+// D_f u and the average ū per face (metric/mod.rs:200 `average`, :210 `gradient` with the
+// coefficients of :120 `df_coefficients`); avg: nf, grad: 3·nf.
+void orc_face_operators(int64_t nv, int64_t nf, const uint32_t* faces, const double* pos, const double* u,
+                        double* avg, double* grad) {
+  Mesh m = make_mesh(nv, nf, faces);
+  Positions p = make_pos(nv, pos);
+  for (int64_t f = 0; f < nf; ++f) {
+    const auto& t = m.faces[f];
+    avg[f] = (u[t[0]] + u[t[1]] + u[t[2]]) / 3.0;
+    const auto g = df_coefficients(corners(p, t));
+    double out[3] = {0.0, 0.0, 0.0};
+    for (int c = 0; c < 3; ++c)
+      for (int r = 0; r < 3; ++r) out[r] += u[t[c]] * g[c][r];
+    for (int r = 0; r < 3; ++r) grad[3 * f + r] = out[r];
+  }
+}
 
 }  // extern "C"

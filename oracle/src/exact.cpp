@@ -2,7 +2,7 @@
 //
 // Restates crates/ddg-repulsive-surfaces/src/energy/exact.rs:
 //   face_partials :87 (the pair loop :100-116, rows combined in face order :126-131),
-//   energy_and_differential :146, culprit_pair :165,
+//   energy_value :61, energy_and_differential :146, culprit_pair :165,
 // and crates/ddg-repulsive-surfaces/src/surface.rs:513 `validate_positions`.
 #include <algorithm>
 #include <thread>
@@ -93,6 +93,21 @@ FacePartials exact_face_partials(const FaceGeometry& g, const Exponents& e, int 
   out.energy = 0.0;
   for (int64_t s = 0; s < nf; ++s) out.energy += e_rows[s];
   return out;
+}
+
+// exact.rs:61 `energy_value`: rows a_S Σ_{T≠S} a_T K(X_S − X_T, N_S), summed in face order.
+double exact_energy_value(const FaceGeometry& g, const Exponents& e, int threads) {
+  const int64_t nf = (int64_t)g.areas.size();
+  std::vector<double> rows((size_t)nf);
+  parallel_for(nf, threads, [&](int64_t s) {
+    double acc = 0.0;
+    for (int64_t t = 0; t < nf; ++t)
+      if (t != s) acc += g.areas[t] * kernel_of_v(e, sub(g.barycenters[s], g.barycenters[t]), g.normals[s]);
+    rows[s] = g.areas[s] * acc;
+  });
+  double sum = 0.0;
+  for (int64_t s = 0; s < nf; ++s) sum += rows[s];
+  return sum;
 }
 
 // exact.rs:165 `culprit_pair`.

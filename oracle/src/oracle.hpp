@@ -13,6 +13,7 @@ struct FacePartials {
 };
 
 FacePartials exact_face_partials(const FaceGeometry& g, const Exponents& e, int threads);
+double exact_energy_value(const FaceGeometry& g, const Exponents& e, int threads);
 bool exact_energy_and_differential(const Mesh& m, const Positions& p, const Exponents& e, int threads,
                                    double* energy, std::vector<V3>* diff, FacePartials* partials,
                                    Error* err);
@@ -279,10 +280,18 @@ GmresReport gmres_right(const std::function<void(const double*, double*)>& op,
                         const std::function<void(const double*, double*)>& prec, const std::vector<double>& b,
                         const GmresOptions& o, std::vector<double>* x);
 
+// The line search's first step (step/options.rs StartRule; session.rs:1604 start_tau).
+enum StartKind { START_GROW_MAXDISP = 0, START_ABSOLUTE = 1, START_MAXDISP = 2, START_REFERENCE = 3 };
 struct SessionOptions {
+  bool energy_exact = false;  // EnergyEval::Exact instead of Barnes–Hut
   double theta = 0.5;
   SaddleOptions saddle;
   GmresOptions gmres;
+  int start_kind = START_GROW_MAXDISP;
+  double start_value = 1.0;   // τ (absolute) or the lengths of MaxDisplacement
+  double armijo_c = 1e-2;     // ArmijoRule::Standard{c}
+  double tau_min_rel = 1e-12;
+  long max_halvings = -1;     // < 0: no cap
 };
 struct Session {
   Mesh mesh;
@@ -292,10 +301,17 @@ struct Session {
   double l0 = 0.0, last_tau = 0.0;
   V3 barycenter_target{0, 0, 0};
 };
+// TrialOutcome (session.rs): 0 accepted, 1 Armijo failed, 2 non-finite.
+struct Trial {
+  double tau = 0.0, energy = 0.0;
+  int outcome = 0;
+};
 struct StepReport {
-  double energy0 = 0.0, energy1 = 0.0, slope = 0.0, tau = 0.0;
+  double energy0 = 0.0, energy1 = 0.0, energy_after = 0.0, slope = 0.0, tau = 0.0, tau0 = 0.0;
+  double gradient_norm = 0.0, direction_norm = 0.0, direction_max = 0.0;
   bool accepted = false;
-  size_t trials = 0;
+  size_t trials = 0, halvings = 0;
+  std::vector<Trial> trial;
   GmresReport gmres;
 };
 bool session_create(const Mesh& m, const Positions& p, const Exponents& e, const SessionOptions& o, Session* s,

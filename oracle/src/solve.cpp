@@ -12,8 +12,9 @@
 //   crates/ddg-repulsive-surfaces/src/solve/gmres.rs:311-583 GmresRun (right preconditioning,
 //       x0 = 0, MGS fused with the next dot in chunks of 4096, Givens, explicit residual per cycle)
 //   crates/ddg-repulsive-surfaces/src/step/session.rs:749 step_inner (energy, direction, Armijo
-//       line search with the GrowPrevious / MaxDisplacement start :1604, barycenter projection),
-//       with remeshing and CCD off.
+//       line search with the StartRule kinds of :1604, max_halvings, the trial records, the
+//       barycenter projection and F after it), with remeshing and CCD off; the objective's exact
+//       or Barnes–Hut evaluation (step/objective.rs:151, energy/exact.rs:61 energy_value).
 //
 // The L′ factor is this oracle's own: a nested-dissection order by coordinate bisection and the
 // up-looking LDLᵀ of T. Davis ("Algorithm 849: A concise sparse Cholesky factorization package",
@@ -23,10 +24,11 @@
 //
 // Pins: the Lean `rs-icosa-precond` fixture (ε, L′ solves, P⁻¹ r), the Lean `rs-octa-saddle-none`
 // fixture (the direction x = A′⁻¹ [g; 0] and ⟨g, x⟩ with the dense metric), an independent numpy
-// assembly of K and L′, GMRES against a dense solve.  session_step: parity unpinned beyond the
-// properties its test checks (the accepted step satisfies the Armijo condition, lowers the
-// energy and keeps the barycenter): the reference's step fixtures all carry area, volume or pin
-// constraints, which this path does not build.
+// assembly of K and L′, GMRES against a dense solve.  session_step: the Lean `rs-icosa-step-stuck`
+// fixture (barycenter only, exact energy, dense metric = χ 0, absolute start, three halvings:
+// F₀, ⟨g, x⟩, ‖x‖, ℓ̄₀, τ₀, every trial's τ, outcome and F, the stuck result and F after the
+// projection, tests/oracle.rs:557-640 tolerances), plus the Armijo / energy / barycenter
+// properties of accepted steps.
 #include <algorithm>
 #include <numeric>
 
@@ -448,19 +450,57 @@ GmresReport gmres_right(const std::function<void(const double*, double*)>& op,
   return rep;
 }
 
+// Area-weighted barycenter (constraints/mod.rs:448 `barycenter`, vertex areas :487 `areas`).
+static V3 barycenter_of(const Mesh& m, const Positions& p) {
+  FaceGeometry gg = face_geometry(m, p);
+  auto va = vertex_areas(m, gg.areas);
+  double tot = 0.0;
+  V3 bc{0, 0, 0};
+  for (size_t v = 0; v < (size_t)m.nv; ++v) {
+    for (int k = 0; k < 3; ++k) bc[k] += va[v] * p[v][k];
+    tot += va[v];
+  }
+  return scale(bc, 1.0 / tot);
+}
+
+// session.rs:1604 `start_tau`.
+static double start_tau(const SessionOptions& o, double last_tau, double xnorm, double max_disp, double l0) {
+  switch (o.start_kind) {
+    case START_ABSOLUTE: return o.start_value;
+    case START_MAXDISP: return o.start_value * l0 / max_disp;
+    case START_REFERENCE: return 2.0 / xnorm;
+    default: return last_tau > 0.0 ? 2.0 * last_tau : o.start_value * l0 / max_disp;
+  }
+}
+
 bool session_step(Session& s, int threads, StepReport* rep, Error* err) {
   const Mesh& m = s.mesh;
   const size_t nv = (size_t)m.nv, nd = 3 * nv;
-  // 1. trees, lists, F and dF (Barnes–Hut; objective.rs:71,151)
+  const bool exact = s.opts.energy_exact;
+  // 1. trees, lists, F and dF (objective.rs:71 build, :151 evaluate)
   BhTree bh;
-  if (!bh_build(m, s.positions, s.opts.theta, s.opts.saddle.leaf_size, threads, &bh, err)) return false;
   double f0;
   std::vector<V3> gv;
-  if (!bh_energy_and_differential(bh, s.exps, threads, &f0, &gv, err)) return false;
+  if (exact) {
+    if (!exact_energy_and_differential(m, s.positions, s.exps, threads, &f0, &gv, nullptr, err)) return false;
+  } else {
+    if (!bh_build(m, s.positions, s.opts.theta, s.opts.saddle.leaf_size, threads, &bh, err)) return false;
+    if (!bh_energy_and_differential(bh, s.exps, threads, &f0, &gv, err)) return false;
+  }
+  // F at trial positions, value only; a geometric failure is a rejected (non-finite) trial
+  auto value_at = [&](const Positions& q) {
+    Error e2;
+    if (exact) {
+      if (!validate_positions(m, q, &e2)) return (double)NAN;
+      return exact_energy_value(face_geometry(m, q), s.exps, threads);
+    }
+    if (!bh_refit(bh, q, &e2)) return (double)NAN;
+    return bh_energy(bh, s.exps, threads);
+  };
   std::vector<double> g(nd);
   for (size_t v = 0; v < nv; ++v)
     for (int k = 0; k < 3; ++k) g[3 * v + k] = gv[v][k];
-  // 2. the solver and the direction x = A′⁻¹ [g; 0]
+  // 2. the solver and the direction x = A′⁻¹ [g; 0] (session.rs:820-860; k = 0 Schur columns)
   Saddle sd;
   if (!saddle_build(m, s.positions, s.exps, s.opts.saddle, threads, &sd, err)) return false;
   std::vector<double> rhs(nd + 3, 0.0), sol;
@@ -474,52 +514,56 @@ bool session_step(Session& s, int threads, StepReport* rep, Error* err) {
     return false;
   }
   std::vector<double> x(sol.begin(), sol.begin() + nd);
-  double slope = 0.0, xinf = 0.0, xn2 = 0.0, maxd = 0.0;
+  double slope = 0.0, xinf = 0.0, xn2 = 0.0, maxd = 0.0, gn2 = 0.0;
   for (size_t i = 0; i < nd; ++i) {
     slope += g[i] * x[i];
+    gn2 += g[i] * g[i];
     xinf = std::max(xinf, std::fabs(x[i]));
     xn2 += x[i] * x[i];
   }
   for (size_t v = 0; v < nv; ++v)
     maxd = std::max(maxd, std::sqrt(x[3 * v] * x[3 * v] + x[3 * v + 1] * x[3 * v + 1] + x[3 * v + 2] * x[3 * v + 2]));
-  // 3. Armijo line search (session.rs:991-1069; StartRule::GrowPrevious{2, MaxDisplacement{1}})
-  double tau0 = s.last_tau > 0.0 ? 2.0 * s.last_tau : s.l0 / maxd;
-  double tau_min = 1e-12 * s.l0 / xinf;
+  const double xnorm = std::sqrt(xn2);
+  // 3. Armijo line search (session.rs:906, :958-1069)
+  const double tau0 = start_tau(s.opts, s.last_tau, xnorm, maxd, s.l0);
+  const double tau_min = s.opts.tau_min_rel * s.l0 / xinf;
   rep->energy0 = f0;
   rep->slope = slope;
+  rep->gradient_norm = std::sqrt(gn2);
+  rep->direction_norm = xnorm;
+  rep->direction_max = xinf;
+  rep->tau0 = tau0;
   rep->accepted = false;
-  rep->trials = 0;
+  rep->trials = rep->halvings = 0;
+  rep->trial.clear();
   Positions trial(nv);
   double tau = tau0;
   if (slope > 0.0 && std::isfinite(tau0) && tau0 > 0.0) {
-    while (tau >= tau_min) {
+    for (;;) {
+      if (!(tau >= tau_min) || (s.opts.max_halvings >= 0 && (long)rep->halvings > s.opts.max_halvings)) break;
       for (size_t v = 0; v < nv; ++v)
         for (int k = 0; k < 3; ++k) trial[v][k] = s.positions[v][k] - tau * x[3 * v + k];
-      Error e2;
-      double ft = NAN;
-      if (bh_refit(bh, trial, &e2)) ft = bh_energy(bh, s.exps, threads);
+      const double ft = value_at(trial);
+      const bool ok = std::isfinite(ft) && ft <= f0 - s.opts.armijo_c * tau * slope;
+      rep->trial.push_back({tau, std::isfinite(ft) ? ft : NAN, !std::isfinite(ft) ? 2 : ok ? 0 : 1});
       rep->trials++;
-      if (std::isfinite(ft) && ft <= f0 - 1e-2 * tau * slope) {
+      if (ok) {
         rep->accepted = true;
         rep->energy1 = ft;
         break;
       }
       tau *= 0.5;
+      rep->halvings++;
     }
   }
-  // 4. projection: translate to the target barycenter (areas at the new positions)
+  if (!rep->accepted) rep->energy1 = f0;
+  // 4. projection (constraints/mod.rs project_positional): translate by −Φ, Φ the barycenter's
+  // offset from its target (areas at the moved positions); then F at the projected positions
   Positions np = rep->accepted ? trial : s.positions;
-  FaceGeometry gg = face_geometry(m, np);
-  auto va = vertex_areas(m, gg.areas);
-  double tot = 0.0;
-  V3 bc{0, 0, 0};
-  for (size_t v = 0; v < nv; ++v) {
-    tot += va[v];
-    for (int k = 0; k < 3; ++k) bc[k] += va[v] * np[v][k];
-  }
-  for (int k = 0; k < 3; ++k) bc[k] /= tot;
+  const V3 bc = barycenter_of(m, np);
   for (size_t v = 0; v < nv; ++v)
-    for (int k = 0; k < 3; ++k) np[v][k] += s.barycenter_target[k] - bc[k];
+    for (int k = 0; k < 3; ++k) np[v][k] -= bc[k] - s.barycenter_target[k];
+  rep->energy_after = value_at(np);
   s.positions = np;
   if (rep->accepted) s.last_tau = tau;
   rep->tau = rep->accepted ? tau : 0.0;
@@ -534,15 +578,7 @@ bool session_create(const Mesh& m, const Positions& p, const Exponents& e, const
   s->exps = e;
   s->opts = o;
   s->l0 = mean_edge_length(m, p);
-  FaceGeometry gg = face_geometry(m, p);
-  auto va = vertex_areas(m, gg.areas);
-  double tot = 0.0;
-  V3 bc{0, 0, 0};
-  for (size_t v = 0; v < (size_t)m.nv; ++v) {
-    tot += va[v];
-    for (int k = 0; k < 3; ++k) bc[k] += va[v] * p[v][k];
-  }
-  for (int k = 0; k < 3; ++k) s->barycenter_target[k] = bc[k] / tot;
+  s->barycenter_target = barycenter_of(m, p);
   return true;
 }
 

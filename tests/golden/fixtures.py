@@ -55,6 +55,14 @@ def read_case(name: str) -> dict:
             c["faces"].append([int(x) for x in t[1:4]])
         elif k == "c":
             c["constraints"].append(t[1:])
+        elif k == "armijo":
+            c["armijo"] = hexf(t[1])
+        elif k == "start":
+            c["start"] = (t[1],) + tuple(hexf(x) for x in t[2:])
+        elif k == "maxhalv":
+            c["maxhalv"] = int(t[1])
+        elif k == "newton":
+            c["newton"] = (t[1], int(t[2]))
     c.setdefault("exponents", (6.0, 12.0))
     c["positions"] = np.array(c["positions"], dtype=np.float64)
     c["faces"] = np.array(c["faces"], dtype=np.uint32)
@@ -77,8 +85,17 @@ def read_out(name: str) -> dict:
             o["rows"].setdefault(k, {})[int(t[1])] = [hexf(x) for x in t[2:]]
         elif k == "M":
             o["M"] = [hexf(x) for x in t[1:]]
-        elif k in ("tau", "s", "eps", "pivot", "gx", "cnx", "cqx"):
+        elif k in ("tau", "s", "eps", "pivot", "gx", "cnx", "cqx", "energy0", "slope", "xnorm", "lbar", "tau0",
+                   "energy1", "energy-final"):
             o[k] = hexf(t[1])
+        elif k == "trial":
+            o.setdefault("trials", []).append((int(t[1]), hexf(t[2]), t[3], hexf(t[4]), hexf(t[5])))
+        elif k == "result":
+            o["result"] = (t[1], hexf(t[2]), int(t[3]))
+        elif k == "newton":
+            o["newton"] = int(t[1])
+        elif k in ("phi", "phi-before"):
+            o[k] = [hexf(x) for x in t[1:]]
         elif k == "df":
             o.setdefault("df", []).append((int(t[1]), int(t[2]), [hexf(x) for x in t[3:]]))
         elif k == "mesh":

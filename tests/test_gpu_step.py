@@ -14,6 +14,12 @@
 * The step: energy before ≤ 1e-5 relative (the Barnes–Hut energy on the same tree and θ), the
   accepted trial energy ≤ 1e-5, the same number of line-search trials, τ ≤ 1e-4 relative, moved
   positions ≤ 1e-4 of the displacement, the barycenter kept to 1e-12.
+* Lean golden ``rs-icosa-step-stuck`` through the device session (exact energy, χ = 0, absolute
+  start 1e6, three halvings): F₀ and every trial energy ≤ 1e-5 relative (FP32 pair kernels),
+  ⟨g, x⟩ and ‖x‖ ≤ 1e-4 (GMRES to 1e-6 on FP32 metric values), τ₀ and each τ exactly, every
+  outcome, the stuck result after four halvings, the positions unchanged.
+* C1 proper: one exact-energy step on the level-3 icosphere (1,280 faces) against the oracle's
+  exact-energy step: the same trial records, F before / after ≤ 1e-5, τ ≤ 1e-4.
 """
 import numpy as np
 import pytest
@@ -114,9 +120,20 @@ def test_lprime_solve_paths_agree(pkg, ctx, monkeypatch):
     assert rel(xl, xl2) <= 1e-8
 
 
-@pytest.mark.skip(reason="the one-launch L' solve hung on the device: the compiler merged thread 0's task "
+def test_lprime_one_launch_is_refused(pkg, ctx, monkeypatch):
+    """The one-launch L′ solve is disabled: asking for it raises InvalidOption before any launch."""
+    p, f = pkg.generate.config_icosphere_l3()
+    sd = pkg.Saddle(pkg.Surface(ctx, f, p))
+    monkeypatch.setenv("DDGRS_LPRIME_SOLVE_ONE_LAUNCH", "1")
+    with pytest.raises(pkg.RsError) as e:
+        sd.lprime_solve(np.ones(sd.n))
+    assert e.value.kind == "InvalidOption"
+
+
+@pytest.mark.skip(reason="the one-launch L' solve hung on the device twice (the compiler merged thread 0's task "
                   "completion and next ticket across the loop's back edge, so the workgroup re-read the old "
-                  "ticket; the restructured loop is not yet run on the device (off by default)")
+                  "ticket); the restructured, now also iteration-bounded loop has not run on the device, and the "
+                  "library refuses the opt-in until it has")
 @pytest.mark.parametrize("name,large", [("torus_16k", False), ("torus_16k", True), ("bumpy_l6", False)])
 def test_lprime_one_launch_matches_per_height(pkg, ctx, name, large, monkeypatch):
     """The one-launch L′ solve (workgroups take the blocks' tasks in ticket order and wait for the
@@ -282,3 +299,56 @@ def test_long_trajectory_converges(pkg, ctx):
         assert r.gmres.converged and r.gmres.relative_residual <= 1e-6
         assert r.energy_trial < r.energy_before
     assert extended >= 1
+
+
+def test_step_lean_stuck(pkg, ctx):
+    c, o = fx.read_case("rs-icosa-step-stuck"), fx.read_out("rs-icosa-step-stuck")
+    p, f = c["positions"], c["faces"]
+    s = pkg.Surface(ctx, f, p)
+    opts = pkg.StepOptions(exponents=pkg.Exponents(*c["exponents"]), energy_exact=True,
+                           saddle=pkg.SaddleOptions(chi=0.0), armijo_c=c["armijo"], max_halvings=c["maxhalv"],
+                           start=pkg.StartRule.absolute(c["start"][1]))
+    ses = pkg.Session(s, opts)
+    r = ses.step()
+    relv = lambda a, b: abs(a - b) / abs(b)
+    print(f"stuck: F0 {r.energy_before:.12g} / {o['energy0']:.12g}, slope {r.sobolev_norm_sq:.10g} / {o['slope']:.10g}, "
+          f"|x| {r.direction_norm:.10g} / {o['xnorm']:.10g}, trials {r.trial_list}")
+    assert relv(r.energy_before, o["energy0"]) <= 1e-5
+    assert relv(r.sobolev_norm_sq, o["slope"]) <= 1e-4
+    assert relv(r.direction_norm, o["xnorm"]) <= 1e-4
+    assert relv(ses.state()["mean_edge_length0"], o["lbar"]) <= 1e-12
+    assert r.tau0 == o["tau0"]
+    want = {"accepted": 0, "armijo-failed": 1, "non-finite": 2}
+    assert len(r.trial_list) == len(o["trials"]) == r.trials
+    for (tau, fe, outcome), (_, wtau, wout, wf, _) in zip(r.trial_list, o["trials"]):
+        assert outcome == want[wout] and tau == wtau
+        assert relv(fe, wf) <= 1e-4
+    assert not r.accepted and r.tau == 0.0 and r.halvings == o["result"][2]
+    assert relv(r.energy_trial, o["energy1"]) <= 1e-5
+    assert relv(r.energy_after, o["energy-final"]) <= 1e-5
+    assert np.abs(s.positions() - p).max() <= 1e-12
+
+
+def test_step_c1_exact_energy(pkg, ctx):
+    p, f = pkg.generate.config_icosphere_l3()
+    s = pkg.Surface(ctx, f, p)
+    ses = pkg.Session(s, pkg.StepOptions(energy_exact=True))
+    ors = oracle.Session(f, p, energy_exact=True)
+    r = ses.step()
+    r0 = ors.step(threads=16)
+    x, x0 = s.positions(), ors.positions()
+    print(f"C1: E {r.energy_before:.12g} / {r0['energy0']:.12g} -> {r.energy_after:.12g} / {r0['energy_after']:.12g}; "
+          f"gmres {r.gmres.iterations} / {r0['gmres_iterations']:.0f}; trials {r.trial_list} / {r0['trial_list']}")
+    assert abs(r.energy_before - r0["energy0"]) <= 1e-5 * abs(r0["energy0"])
+    assert abs(r.sobolev_norm_sq - r0["slope"]) <= 1e-4 * abs(r0["slope"])
+    assert abs(r.direction_norm - r0["direction_norm"]) <= 1e-4 * r0["direction_norm"]
+    assert len(r.trial_list) == len(r0["trial_list"])
+    for (tau, fe, out), (tau0, fe0, out0) in zip(r.trial_list, r0["trial_list"]):
+        assert out == out0 and abs(tau - tau0) <= 1e-4 * tau0
+        assert abs(fe - fe0) <= 1e-5 * abs(fe0)
+    assert r.accepted and r0["accepted"] == 1.0
+    assert abs(r.energy_trial - r0["energy1"]) <= 1e-5 * abs(r0["energy1"])
+    assert abs(r.energy_after - r0["energy_after"]) <= 1e-5 * abs(r0["energy_after"])
+    assert abs(r.gmres.iterations - r0["gmres_iterations"]) <= 1
+    assert r.gmres.converged and r.gmres.relative_residual <= 1e-6
+    assert np.abs(x - x0).max() <= 1e-4 * np.abs(x0 - p).max()

@@ -4,6 +4,10 @@
   to 1e-12 (crates/ddg-repulsive-surfaces/tests/hierarchical.rs:169, oracle.rs:41);
 * χ = 0.5 stays within 5 % of χ = 0 (hierarchical.rs:192);
 * the null space (B̃⁰ 1 = 0 exactly through H̃a, hierarchical.rs:4-16) and symmetry;
+* D_f and the face average of the Lean golden ``rs-{tetra,icosa,patch,mobius}-df`` fixtures to
+  1e-12 normwise (tests/oracle.rs:262 ``run_df``, metric/mod.rs:120-225), through the same
+  ``df_coefficients`` the metric products use; plus D_f 1 = 0 and D_f of a linear function
+  equal to its tangential gradient;
 * the vertex-pair L^V_τ at χ = 0 equals the brute-force formula
   Σ_{v≠w} (u_v − u_w)² a_v a_w |f_v − f_w|^{−(2τ+2)} (dense.rs:479).
 """
@@ -97,3 +101,30 @@ def test_vertex_fractional_brute_force(torus):
     assert np.abs(y - want).max() <= 1e-11 * np.abs(want).max()
     approx = oracle.VertexFractional(p, mass, tau, chi=0.5)
     assert np.linalg.norm(approx.apply(x) - want) / np.linalg.norm(want) < 5e-2
+
+
+@pytest.mark.parametrize("name", ["rs-tetra-df", "rs-icosa-df", "rs-patch-df", "rs-mobius-df"])
+def test_lean_df_fixtures(name):
+    c, o = fx.read_case(name), fx.read_out(name)
+    got, want = [], []
+    for k, f, vals in o["df"]:
+        avg, grad = oracle.face_operators(c["faces"], c["positions"], c["u"][k])
+        got += [avg[f], *grad[f]]
+        want += vals
+    got, want = np.array(got), np.array(want)
+    assert len(want) >= 8
+    assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
+
+
+def test_df_identities():
+    """D_f 1 = 0, and for u = ⟨a, f⟩ (linear) D_f u = a − ⟨a, N⟩ N on every face."""
+    c = fx.read_case("rs-icosa-df")
+    p, f = c["positions"], c["faces"].astype(np.int64)
+    _, g1 = oracle.face_operators(f, p, np.ones(len(p)))
+    assert np.abs(g1).max() <= 1e-14
+    a = np.array([0.3, -1.1, 0.7])
+    _, g = oracle.face_operators(f, p, p @ a)
+    n = np.cross(p[f[:, 1]] - p[f[:, 0]], p[f[:, 2]] - p[f[:, 0]])
+    n /= np.linalg.norm(n, axis=1)[:, None]
+    tang = a[None, :] - (n @ a)[:, None] * n
+    assert np.abs(g - tang).max() <= 1e-13

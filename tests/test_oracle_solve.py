@@ -11,6 +11,11 @@
 * Lean golden ``rs-octa-saddle-none``: the direction x = A′⁻¹ [g; 0] with the dense metric
   (χ = 0, w = (1, 1), (4, 8), barycenter rows) and ⟨g, x⟩ — pins A′, the constraint rows and GMRES;
 * ε and L′ against an independent numpy assembly (cotan W, lumped M, barycenter row);
+* Lean golden ``rs-icosa-step-stuck``: one step of the session (exact energy, the dense metric
+  χ = 0, the barycenter only, an absolute start τ₀ = 1e6, at most three halvings) reproduces F₀,
+  ⟨g, x⟩, ‖x‖, ℓ̄₀, τ₀, every trial's τ, outcome and F, the stuck result with four halvings and F
+  after the projection (tests/oracle.rs:557-640, 1e-10; ℓ̄₀ 1e-12) — the trial energies at
+  τ ≥ 1.25e5 are F(f − τ x) with τ x dominating, so they pin the direction's shape;
 * one descent step lowers the energy and keeps the barycenter.
 """
 import math
@@ -132,3 +137,43 @@ def test_descent_step_lowers_energy_and_keeps_barycenter(small):
     vq = oracle.vertex_areas(f, q)
     b1 = (vq[:, None] * q).sum(axis=0) / vq.sum()
     assert np.abs(b1 - b0).max() <= 1e-14
+
+
+def step_stuck_session(fx_case):
+    c = fx_case
+    return oracle.Session(c["faces"], c["positions"], *c["exponents"], chi=0.0, tol=1e-13, maxit=500,
+                          energy_exact=True, start=("abs", c["start"][1]), armijo_c=c["armijo"],
+                          max_halvings=c["maxhalv"])
+
+
+def test_lean_step_stuck_icosa():
+    c, o = fx.read_case("rs-icosa-step-stuck"), fx.read_out("rs-icosa-step-stuck")
+    assert c["weights"] == (1.0, 1.0, 0.0) and c["constraints"] == [["barycenter", "all"]]
+    assert c["start"][0] == "abs" and c["newton"] == ("tol", 5)
+    rep = step_stuck_session(c).step(1)
+    rel = lambda a, b: abs(a - b) / abs(b)
+    assert rel(rep["energy0"], o["energy0"]) <= 1e-10
+    assert rel(rep["slope"], o["slope"]) <= 1e-10
+    assert rel(rep["direction_norm"], o["xnorm"]) <= 1e-10
+    assert rel(rep["lbar"], o["lbar"]) <= 1e-12
+    assert rel(rep["tau0"], o["tau0"]) <= 1e-10
+    want = {"accepted": 0, "armijo-failed": 1, "non-finite": 2}
+    assert len(rep["trial_list"]) == len(o["trials"]) == 4
+    for (tau, f, outcome), (_, wtau, wout, wf, _) in zip(rep["trial_list"], o["trials"]):
+        assert outcome == want[wout]
+        assert rel(tau, wtau) <= 1e-10
+        assert rel(f, wf) <= 1e-10
+    assert o["result"][0] == "stuck" and rep["accepted"] == 0.0
+    assert rep["halvings"] == o["result"][2] == 4 and rep["tau"] == o["result"][1] == 0.0
+    assert rel(rep["energy1"], o["energy1"]) <= 1e-10
+    assert rel(rep["energy_after"], o["energy-final"]) <= 1e-10
+    assert o["newton"] == 0 and o["phi"] == [0.0, 0.0, 0.0]
+
+
+def test_lean_step_stuck_catches_a_wrong_halving_cap():
+    """The same replay with the cap off by one takes a fifth trial (τ = 6.25e4) that the fixture
+    does not have: the trial records pin the cap rule `halvings > max_halvings`."""
+    c = dict(fx.read_case("rs-icosa-step-stuck"))
+    c["maxhalv"] = c["maxhalv"] + 1
+    rep = step_stuck_session(c).step(1)
+    assert len(rep["trial_list"]) == 5
