@@ -133,6 +133,8 @@ SIGNATURES = {
     "ddgrs_metric_destroy": (None, [_P]),
     "ddgrs_metric_apply_batch": (_I32, [_P, _D, _D, _I64, _P, _P, _E]),
     "ddgrs_metric_stats": (_I32, [_P, _P, _E]),
+    "ddgrs_metric_blocks": (_I32, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _E]),
+    "ddgrs_vfrac_blocks": (_I32, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _E]),
     "ddgrs_vfrac_build": (_I32, [_P, _D, _D, _I64, _I64, ctypes.POINTER(_P), _E]),
     "ddgrs_vfrac_destroy": (None, [_P]),
     "ddgrs_vfrac_apply_batch": (_I32, [_P, _I64, _P, _P, _E]),
@@ -426,6 +428,19 @@ def _batch(fn, h, n, x, *extra):
     return ys[0] if one else ys
 
 
+def _block_lists(fn, h) -> dict:
+    """The tree and block lists of a block cluster tree handle (``ddgrs_*_blocks``)."""
+    c = np.zeros(4, dtype=np.int64)
+    _call(fn, h, _ptr(c), *([None] * 8))
+    n, nn, nfar, nnear = (int(v) for v in c)
+    out = {k: np.zeros(m, dtype=np.uint32) for k, m in (("order", n), ("start", nn), ("end", nn), ("right", nn),
+                                                        ("far_ptr", nn + 1), ("far_col", nfar),
+                                                        ("near_ptr", nn + 1), ("near_col", nnear))}
+    _call(fn, h, _ptr(c), *[_ptr(out[k]) for k in ("order", "start", "end", "right", "far_ptr", "far_col",
+                                                      "near_ptr", "near_col")])
+    return out
+
+
 class FaceMetricTree(_Handle):
     """B̃ and B̃⁰ of one configuration (``FaceMetricTree``, src/metric/hierarchical.rs:283)."""
 
@@ -448,6 +463,11 @@ class FaceMetricTree(_Handle):
         keys = ["admissible_blocks", "inadmissible_blocks", "admissible_pairs", "near_entries", "nodes", "depth", "near_total"]
         return dict(zip(keys, (int(v) for v in o)))
 
+    def blocks(self) -> dict:
+        """The face tree (order, start, end, right) and the admissible (far) and inadmissible (near)
+        block lists per row node in creation order (src/spatial/block_tree.rs:129-209)."""
+        return _block_lists(lib().ddgrs_metric_blocks, self._h)
+
 
 class VertexFractionalTree(_Handle):
     """L̃^V_τ over vertex pairs (``VertexFractionalTree``, src/metric/hierarchical.rs:1003)."""
@@ -462,6 +482,10 @@ class VertexFractionalTree(_Handle):
 
     def apply_batch(self, x):
         return _batch(lib().ddgrs_vfrac_apply_batch, self._h, 3 * self.surface.nv, x)
+
+    def blocks(self) -> dict:
+        """The vertex tree and block lists, as ``FaceMetricTree.blocks`` (hierarchical.rs:1017)."""
+        return _block_lists(lib().ddgrs_vfrac_blocks, self._h)
 
 
 class _SaddleOptionsC(ctypes.Structure):

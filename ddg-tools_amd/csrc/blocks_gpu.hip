@@ -694,6 +694,7 @@ BlockStats BlockTreeDev::build_structure(ddgrs_ctx* ctx, const Topology& t, cons
   S.near_row.reserve(std::max<uint32_t>(n_near, 1));
   S.near_col.reserve(std::max<uint32_t>(n_near, 1));
   S.near_ptr.reserve(nn + 1);
+  this->n_near = n_near;
   if (n_near)
     sort_pairs<uint32_t>(ctx, S.temp, S.near_a.ptr, S.near_row.ptr, S.near_b.ptr, S.near_col.ptr, n_near, row_bits);
   launch(ctx, "blk_row_ptr", blk_row_ptr_kernel, dim3(blocks_for(nn + 1, 256)), dim3(256), 0, (uint32_t)nn, n_near,
@@ -820,6 +821,31 @@ BlockStats BlockTreeDev::build_structure(ddgrs_ctx* ctx, const Topology& t, cons
   st.near_entries = T.sums[3];
   st.depth = depth;
   return st;
+}
+
+void BlockTreeDev::set_near_lists(ddgrs_ctx* ctx, const BlockStructure& s) {
+  n_near = (uint32_t)s.near_col.size();
+  ddgrs::upload(ctx, scratch.near_ptr, s.near_ptr);
+  ddgrs::upload(ctx, scratch.near_col, s.near_col);
+}
+
+void BlockTreeDev::lists(ddgrs_ctx* ctx, std::vector<uint32_t>* order_h, std::vector<uint32_t>* start_h,
+                         std::vector<uint32_t>* end_h, std::vector<uint32_t>* right_h,
+                         std::vector<uint32_t>* far_ptr_h, std::vector<uint32_t>* far_col_h,
+                         std::vector<uint32_t>* near_ptr_h, std::vector<uint32_t>* near_col_h) {
+  auto get = [&](std::vector<uint32_t>* h, const DevBuf<uint32_t>& d, size_t count) {
+    h->assign(count, 0u);
+    download(ctx, h->data(), (const uint32_t*)d.ptr, count);
+  };
+  get(order_h, order, (size_t)n);
+  get(start_h, start, (size_t)nn);
+  get(end_h, end, (size_t)nn);
+  get(right_h, right, (size_t)nn);
+  get(far_ptr_h, far_ptr, (size_t)nn + 1);
+  get(far_col_h, far_col, n_far);
+  get(near_ptr_h, scratch.near_ptr, (size_t)nn + 1);
+  get(near_col_h, scratch.near_col, n_near);
+  ctx->sync();
 }
 
 }  // namespace ddgrs

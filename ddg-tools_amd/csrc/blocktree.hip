@@ -886,6 +886,7 @@ void FaceMetricDev::build(ddgrs_surface* s, const FaceMetricPlan& P, double s_ex
   nv = (int)s->nv;
   nf = (int)s->nf;
   bt.upload(ctx, P.bt);
+  bt.set_near_lists(ctx, P.bs);
   ddgrs::upload(ctx, inc_ptr, P.inc_ptr);
   ddgrs::upload(ctx, inc, P.inc);
   face_metric_values(ctx, s, *this, P.centroid, P.projector, P.node_mass, s_exp);
@@ -1026,6 +1027,7 @@ void VertexFractionalDev::build(ddgrs_ctx* ctx, const VertexFractionalPlan& P) {
   topo = P.topo;
   stats = block_stats(P.bs);
   bt.upload(ctx, P.bt);
+  bt.set_near_lists(ctx, P.bs);
   ddgrs::upload(ctx, mass, P.mass);
   vertex_fractional_values(ctx, *this, P.clusters, P.pos, P.tau);
 }

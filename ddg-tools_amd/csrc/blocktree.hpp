@@ -82,7 +82,7 @@ struct BlockTreeDev {
   int nk = 1;
   int n = 0, nn = 0, nl = 0, np = 0;
   uint64_t near_total = 0;  // stored near entries (Σ over panels of rows × columns)
-  uint32_t n_far = 0;
+  uint32_t n_far = 0, n_near = 0;
   DevBuf<uint32_t> order, position, start, end, right, parent, leaves;
   DevBuf<uint32_t> panel_leaf, panel_row0, panel_rows, col_ptr, cols;
   DevBuf<uint64_t> panel_val;  // per panel: first entry of its values
@@ -116,6 +116,15 @@ struct BlockTreeDev {
   BlockStats build_structure(ddgrs_ctx* ctx, const Topology& t, const Clusters& c, double chi, size_t leaf_pairs,
                              int nk, const double* elem_weight);
   void finish_far(ddgrs_ctx* ctx);
+  // A host-planned structure's near lists, for lists() (upload() carries the panels only).
+  void set_near_lists(ddgrs_ctx* ctx, const BlockStructure& s);
+  // The tree and the grouped block lists of the last build, on the host: order (n), start, end,
+  // right (nn), far_ptr and near_ptr (nn + 1), far_col (n_far) and near_col (n_near); per row node
+  // the column nodes in creation order (RS/src/spatial/block_tree.rs:198 group_by_row).
+  void lists(ddgrs_ctx* ctx, std::vector<uint32_t>* order_h, std::vector<uint32_t>* start_h,
+             std::vector<uint32_t>* end_h, std::vector<uint32_t>* right_h, std::vector<uint32_t>* far_ptr_h,
+             std::vector<uint32_t>* far_col_h, std::vector<uint32_t>* near_ptr_h,
+             std::vector<uint32_t>* near_col_h);
   // Room for the near values and the zero entry after them.
   void reserve_values(ddgrs_ctx* ctx);
   // z = w_k (H̃a·u − H̃(a u)) per BVH position and channel, channels [0, H) on kernel 0 with w0,

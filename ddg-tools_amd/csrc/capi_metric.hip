@@ -1,6 +1,7 @@
 // C ABI of the hierarchical metric B̃, B̃⁰ and of the vertex-pair fractional operator.
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 
 #include "bh.hpp"
 #include "blocktree.hpp"
@@ -114,6 +115,34 @@ int ddgrs_metric_stats(ddgrs_metric* h, int64_t* out, ddgrs_error* err) {
   });
 }
 
+namespace {
+// Copies out a block tree's node arrays and grouped lists (any pointer may be null; counts4 =
+// [elements, nodes, admissible blocks, near blocks]).
+void export_lists(ddgrs_ctx* ctx, BlockTreeDev& bt, int64_t* counts4, uint32_t* order, uint32_t* start, uint32_t* end,
+                  uint32_t* right, uint32_t* far_ptr, uint32_t* far_col, uint32_t* near_ptr, uint32_t* near_col) {
+  std::vector<uint32_t> v[8];
+  bt.lists(ctx, &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6], &v[7]);
+  if (counts4) {
+    counts4[0] = bt.n;
+    counts4[1] = bt.nn;
+    counts4[2] = bt.n_far;
+    counts4[3] = bt.n_near;
+  }
+  uint32_t* outs[8] = {order, start, end, right, far_ptr, far_col, near_ptr, near_col};
+  for (int k = 0; k < 8; ++k)
+    if (outs[k] && !v[k].empty()) std::memcpy(outs[k], v[k].data(), v[k].size() * sizeof(uint32_t));
+}
+}  // namespace
+
+int ddgrs_metric_blocks(ddgrs_metric* h, int64_t* counts4, uint32_t* order, uint32_t* start, uint32_t* end,
+                        uint32_t* right, uint32_t* far_ptr, uint32_t* far_col, uint32_t* near_ptr, uint32_t* near_col,
+                        ddgrs_error* err) {
+  return guarded(err, "metric_blocks", [&] {
+    need(h, "metric");
+    export_lists(h->s->ctx, h->fm.bt, counts4, order, start, end, right, far_ptr, far_col, near_ptr, near_col);
+  });
+}
+
 int ddgrs_vfrac_build(ddgrs_surface* s, double tau, double chi, int64_t leaf_size, int64_t leaf_pairs,
                       ddgrs_vfrac** out, ddgrs_error* err) {
   return guarded(err, "vfrac_build", [&] {
@@ -138,6 +167,15 @@ int ddgrs_vfrac_build(ddgrs_surface* s, double tau, double chi, int64_t leaf_siz
 }
 
 void ddgrs_vfrac_destroy(ddgrs_vfrac* h) { destroy_handle(h); }
+
+int ddgrs_vfrac_blocks(ddgrs_vfrac* h, int64_t* counts4, uint32_t* order, uint32_t* start, uint32_t* end,
+                       uint32_t* right, uint32_t* far_ptr, uint32_t* far_col, uint32_t* near_ptr, uint32_t* near_col,
+                       ddgrs_error* err) {
+  return guarded(err, "vfrac_blocks", [&] {
+    need(h, "vfrac");
+    export_lists(h->s->ctx, h->vf.bt, counts4, order, start, end, right, far_ptr, far_col, near_ptr, near_col);
+  });
+}
 
 int ddgrs_vfrac_apply_batch(ddgrs_vfrac* h, int64_t nb, const double* xs, double* ys, ddgrs_error* err) {
   return guarded(err, "vfrac_apply", [&] {

@@ -144,6 +144,17 @@ DDGRS_API int ddgrs_metric_apply_batch(ddgrs_metric* m, double w_hi, double w_lo
  * near entries] (BctStats, RS/src/spatial/block_tree.rs:401). */
 DDGRS_API int ddgrs_metric_stats(ddgrs_metric* m, int64_t* out7, ddgrs_error* err);
 
+/* The block cluster tree and its block lists, for the bit-exact check against the CPU build
+ * (RS/src/spatial/block_tree.rs:129 build_impl: blocks level by level, then grouped by row node,
+ * creation order kept, :198 group_by_row).  Any array pointer may be NULL (call with all NULL to
+ * read the sizes first): counts4 = [elements, nodes, admissible blocks, inadmissible blocks];
+ * order (elements: BVH position → element), start, end, right (nodes; pre-order numbering),
+ * far_ptr and near_ptr (nodes + 1: CSR offsets per row node), far_col (admissible blocks) and
+ * near_col (inadmissible blocks: dense near field): the column nodes. */
+DDGRS_API int ddgrs_metric_blocks(ddgrs_metric* m, int64_t* counts4, uint32_t* order, uint32_t* start, uint32_t* end,
+                                  uint32_t* right, uint32_t* far_ptr, uint32_t* far_col, uint32_t* near_ptr,
+                                  uint32_t* near_col, ddgrs_error* err);
+
 /* The vertex-pair L̃^V_τ (dual areas of the surface) on a vertex BVH.  Replaces
  * VertexFractionalTree::for_surface / build_on_gpu (hierarchical.rs:1157,1092). */
 DDGRS_API int ddgrs_vfrac_build(ddgrs_surface* s, double tau, double chi, int64_t leaf_size, int64_t leaf_pairs,
@@ -151,6 +162,11 @@ DDGRS_API int ddgrs_vfrac_build(ddgrs_surface* s, double tau, double chi, int64_
 DDGRS_API void ddgrs_vfrac_destroy(ddgrs_vfrac* v);
 /* Replaces VertexFractionalTree::apply_batch (hierarchical.rs:1274). */
 DDGRS_API int ddgrs_vfrac_apply_batch(ddgrs_vfrac* v, int64_t nb, const double* xs, double* ys, ddgrs_error* err);
+/* The vertex tree and its block lists, as ddgrs_metric_blocks (the vertex BVH and the blocks of
+ * VertexFractionalTree::build, hierarchical.rs:1017). */
+DDGRS_API int ddgrs_vfrac_blocks(ddgrs_vfrac* v, int64_t* counts4, uint32_t* order, uint32_t* start, uint32_t* end,
+                                 uint32_t* right, uint32_t* far_ptr, uint32_t* far_col, uint32_t* near_ptr,
+                                 uint32_t* near_col, ddgrs_error* err);
 
 /* ---- the step's saddle solver --------------------------------------------------------------- */
 

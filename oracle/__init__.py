@@ -117,6 +117,8 @@ def _declare(L):
     L.orc_lprime_solve.argtypes = [_P, _P, _P]
     L.orc_saddle_solve.argtypes = [_P, _P, _P, _I64, _D, _I64, _INT, _P]
     L.orc_face_operators.argtypes = [_I64, _I64, _P, _P, _P, _P, _P]
+    L.orc_metric_blocks.argtypes = [_P] * 10
+    L.orc_vfrac_blocks.argtypes = [_P] * 10
     L.orc_session_create.argtypes = [_I64, _I64, _P, _P, _D, _D, _P, _P, ctypes.POINTER(_INT)]
     L.orc_session_create.restype = _P
     L.orc_session_free.argtypes = [_P]
@@ -222,6 +224,17 @@ class BhTree:
         return float(e[0]), d, fp
 
 
+def _block_lists(fn, h):
+    c = np.zeros(4, dtype=np.int64)
+    fn(h, _ptr(c), *([None] * 8))
+    n, nn, nfar, nnear = (int(v) for v in c)
+    names = ("order", "start", "end", "right", "far_ptr", "far_col", "near_ptr", "near_col")
+    sizes = (n, nn, nn, nn, nn + 1, nfar, nn + 1, nnear)
+    out = {k: np.zeros(m, dtype=np.uint32) for k, m in zip(names, sizes)}
+    fn(h, _ptr(c), *[_ptr(out[k]) for k in names])
+    return out
+
+
 class FaceMetric:
     """The oracle's hierarchical B̃, B̃⁰ (hierarchical.rs:283 ``FaceMetricTree``)."""
 
@@ -256,6 +269,11 @@ class FaceMetric:
         keys = ["admissible_blocks", "inadmissible_blocks", "admissible_pairs", "near_entries", "nodes", "depth", "near_total"]
         return dict(zip(keys, (int(x) for x in o)))
 
+    def blocks(self):
+        """The face tree and the admissible / inadmissible block lists per row node in creation
+        order (block_tree.rs:129-209)."""
+        return _block_lists(lib().orc_metric_blocks, self._h)
+
     def ha(self):
         out = np.zeros((self.nf, 2))
         lib().orc_metric_ha(self._h, _ptr(out))
@@ -287,6 +305,10 @@ class VertexFractional:
         ys = np.zeros_like(xs)
         lib().orc_vfrac_apply(self._h, xs.shape[0], _ptr(xs), _ptr(ys), self.threads)
         return ys[0] if one else ys
+
+    def blocks(self):
+        """The vertex tree and its block lists (as ``FaceMetric.blocks``)."""
+        return _block_lists(lib().orc_vfrac_blocks, self._h)
 
 
 def vertex_areas(faces, positions):

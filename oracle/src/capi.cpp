@@ -1,3 +1,4 @@
+#include <algorithm>
 // FP64 oracle — TEST INFRASTRUCTURE ONLY: a flat C interface for the Python test harness
 // (oracle/__init__.py loads it through ctypes).  Arrays are flat: faces `3·nf` u32, positions and
 // differentials `3·nv` f64 interleaved (`3 v + axis`), per-face partials `7·nf`
@@ -197,6 +198,45 @@ void orc_metric_stats(void* h, int64_t* out) {
 }
 
 // H̃a per face (2 per face: hi, lo), original order.
+// A block tree's node arrays and grouped lists (block_tree.rs:129-209); any pointer may be null;
+// counts4 = [elements, nodes, admissible blocks, inadmissible blocks].
+static void block_lists(const BlockTree& t, int64_t* counts4, uint32_t* order, uint32_t* start, uint32_t* end,
+                        uint32_t* right, uint32_t* far_ptr, uint32_t* far_col, uint32_t* near_ptr, uint32_t* near_col) {
+  const size_t nn = t.start.size();
+  if (counts4) {
+    counts4[0] = (int64_t)t.order.size();
+    counts4[1] = (int64_t)nn;
+    counts4[2] = (int64_t)t.far_col.size();
+    counts4[3] = (int64_t)t.near_col.size();
+  }
+  auto put = [](uint32_t* out, const std::vector<uint32_t>& v) {
+    if (out) std::copy(v.begin(), v.end(), out);
+  };
+  auto put64 = [](uint32_t* out, const std::vector<uint64_t>& v) {
+    if (out)
+      for (size_t i = 0; i < v.size(); ++i) out[i] = (uint32_t)v[i];
+  };
+  put(order, t.order);
+  put(start, t.start);
+  put(end, t.end);
+  put(right, t.right);
+  put64(far_ptr, t.far_ptr);
+  put(far_col, t.far_col);
+  put64(near_ptr, t.near_ptr);
+  put(near_col, t.near_col);
+}
+
+void orc_metric_blocks(void* h, int64_t* counts4, uint32_t* order, uint32_t* start, uint32_t* end, uint32_t* right,
+                       uint32_t* far_ptr, uint32_t* far_col, uint32_t* near_ptr, uint32_t* near_col) {
+  block_lists(static_cast<FaceMetric*>(h)->bct, counts4, order, start, end, right, far_ptr, far_col, near_ptr, near_col);
+}
+
+void orc_vfrac_blocks(void* h, int64_t* counts4, uint32_t* order, uint32_t* start, uint32_t* end, uint32_t* right,
+                      uint32_t* far_ptr, uint32_t* far_col, uint32_t* near_ptr, uint32_t* near_col) {
+  block_lists(static_cast<VertexFractional*>(h)->bct, counts4, order, start, end, right, far_ptr, far_col, near_ptr,
+              near_col);
+}
+
 void orc_metric_ha(void* h, double* out) {
   auto* fm = static_cast<FaceMetric*>(h);
   std::memcpy(out, fm->ha.data(), fm->ha.size() * 8);

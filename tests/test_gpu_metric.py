@@ -3,6 +3,11 @@
 B̃, B̃⁰ (face pairs, χ = 0.5, leaf 8, leaf_pairs 64) and L̃^V_{2−s} (vertex pairs) in FP32
 (near and far values evaluated in FP64, stored FP32; FP32 accumulation) against the oracle's FP64
 product on the same block tree: relative error ≤ 1e-5 in the 2-norm; block counts equal.
+
+The block cluster trees built on the device equal the oracle's CPU build bit for bit: the BVH
+(order, start, end, right) and the admissible and inadmissible block lists per row node in
+creation order (RS/src/spatial/block_tree.rs:129-209), for the face tree and the vertex tree, on
+the level-3 icosphere, the 16,384-face torus and the level-6 bumpy icosphere.
 """
 import numpy as np
 import pytest
@@ -12,6 +17,16 @@ import oracle
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-5
+
+
+BLOCK_KEYS = ("order", "start", "end", "right", "far_ptr", "far_col", "near_ptr", "near_col")
+
+
+def assert_same_blocks(dev, orc, what):
+    for k in BLOCK_KEYS:
+        assert dev[k].shape == orc[k].shape and np.array_equal(dev[k], orc[k]), (what, k, dev[k].shape, orc[k].shape)
+    print(f"{what}: {len(dev['order'])} elements, {len(dev['start'])} nodes, {len(dev['far_col'])} admissible and "
+          f"{len(dev['near_col'])} inadmissible blocks, equal to the CPU build")
 
 
 def meshes(pkg):
@@ -28,6 +43,7 @@ def test_face_metric(pkg, ctx, which):
     st, so = fm.stats(), om.stats()
     for k in ("admissible_blocks", "inadmissible_blocks", "admissible_pairs", "near_entries", "nodes", "depth", "near_total"):
         assert st[k] == so[k], (k, st[k], so[k])
+    assert_same_blocks(fm.blocks(), om.blocks(), f"{name} face tree")
     rng = np.random.default_rng(11)
     x = rng.standard_normal((3, 3 * len(p)))
     for w_hi, w_lo in [(1.0, 1.0), (1.0, 0.0), (0.0, 1.0)]:
@@ -50,6 +66,7 @@ def test_vertex_fractional(pkg, ctx, which):
     vf = pkg.VertexFractionalTree(s, tau)
     mass = oracle.vertex_areas(f, p)
     ov = oracle.VertexFractional(p, mass, tau, chi=0.5)
+    assert_same_blocks(vf.blocks(), ov.blocks(), f"{name} vertex tree")
     rng = np.random.default_rng(12)
     x = rng.standard_normal((2, 3 * len(p)))
     y, y0 = vf.apply_batch(x), ov.apply(x)
@@ -64,6 +81,7 @@ def test_face_metric_l6(pkg, ctx):
     fm = pkg.FaceMetricTree(s)
     om = oracle.FaceMetric(f, p, chi=0.5)
     assert fm.stats() == om.stats()
+    assert_same_blocks(fm.blocks(), om.blocks(), "bumpy_l6 face tree")
     rng = np.random.default_rng(13)
     x = rng.standard_normal(3 * len(p))
     y, y0 = fm.apply_batch(x), om.apply(x)
@@ -154,3 +172,23 @@ def test_extended_products(pkg, ctx, which, monkeypatch):
         assert e64 <= TOL and f64 <= TOL
         assert e64 <= 1.01 * e32 + 1e-12 and f64 <= 1.01 * f32 + 1e-12
         assert np.linalg.norm(y64[k] - y32[k]) / np.linalg.norm(y64[k]) <= 1e-5
+
+
+def test_vertex_tree_blocks_l6(pkg, ctx):
+    p, f = pkg.generate.config_bumpy_l6()
+    s = pkg.Surface(ctx, f, p)
+    tau = 2.0 - (12.0 - 2.0) / 6.0
+    vf = pkg.VertexFractionalTree(s, tau)
+    ov = oracle.VertexFractional(p, oracle.vertex_areas(f, p), tau, chi=0.5)
+    assert_same_blocks(vf.blocks(), ov.blocks(), "bumpy_l6 vertex tree")
+
+
+def test_host_planned_blocks_export_the_same_lists(pkg, ctx, monkeypatch):
+    """The lists read back from a host-planned build equal those of the device build."""
+    p, f = pkg.generate.config_torus_16k()
+    s = pkg.Surface(ctx, f, p)
+    dev = pkg.FaceMetricTree(s).blocks()
+    monkeypatch.setenv("DDGRS_BLOCKS_ON_HOST", "1")
+    host = pkg.FaceMetricTree(s).blocks()
+    monkeypatch.delenv("DDGRS_BLOCKS_ON_HOST")
+    assert_same_blocks(dev, host, "torus_16k host plan")

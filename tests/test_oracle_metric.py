@@ -128,3 +128,26 @@ def test_df_identities():
     n /= np.linalg.norm(n, axis=1)[:, None]
     tang = a[None, :] - (n @ a)[:, None] * n
     assert np.abs(g - tang).max() <= 1e-13
+
+
+def test_block_lists_partition_every_pair():
+    """The oracle's admissible and inadmissible blocks cover every ordered element pair exactly
+    once (block_tree.rs: the level-by-level split of (root, root)), admissible blocks are never
+    diagonal, and the counts match the stats."""
+    c = fx.read_case("rs-torus-metric")
+    p, f = c["positions"], c["faces"]
+    fm = oracle.FaceMetric(f, p, chi=0.5, leaf_size=2, leaf_pairs=4)
+    b = fm.blocks()
+    n = len(b["order"])
+    cover = np.zeros((n, n), dtype=np.int64)
+    for key in ("far", "near"):
+        ptr, col = b[key + "_ptr"], b[key + "_col"]
+        for a in range(len(b["start"])):
+            for q in range(ptr[a], ptr[a + 1]):
+                c_ = col[q]
+                if key == "far":
+                    assert a != c_
+                cover[b["start"][a]:b["end"][a], b["start"][c_]:b["end"][c_]] += 1
+    assert np.all(cover == 1)
+    st = fm.stats()
+    assert len(b["far_col"]) == st["admissible_blocks"] and len(b["near_col"]) == st["inadmissible_blocks"]
