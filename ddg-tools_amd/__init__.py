@@ -121,6 +121,7 @@ SIGNATURES = {
     "ddgrs_surface_set_positions": (_I32, [_P, _P, _E]),
     "ddgrs_surface_get_positions": (_I32, [_P, _P, _E]),
     "ddgrs_exact_energy_and_differential": (_I32, [_P, _D, _D, ctypes.POINTER(_D), _P, _P, _E]),
+    "ddgrs_exact_energy_and_differential_noalloc": (_I32, [_P, _D, _D, ctypes.POINTER(_D), _P, _P, _E]),
     "ddgrs_bh_build": (_I32, [_P, _D, _I64, ctypes.POINTER(_P), _E]),
     "ddgrs_bh_destroy": (None, [_P]),
     "ddgrs_bh_refit": (_I32, [_P, _P, _E]),
@@ -320,6 +321,18 @@ def exact_energy_and_differential(surface: Surface, exponents: Exponents = Expon
     fp = np.zeros((surface.nf, 7)) if partials else None
     _call(lib().ddgrs_exact_energy_and_differential, surface.handle, exponents.alpha, exponents.beta, ctypes.byref(e),
           _ptr(d), _ptr(fp) if partials else None)
+    return (e.value, d, fp) if partials else (e.value, d)
+
+
+def exact_energy_and_differential_noalloc(surface: Surface, exponents: Exponents = Exponents(), partials: bool = False):
+    """As ``exact_energy_and_differential`` through the entry point that allocates nothing (its
+    scratch is a fixed per-device array: at most 131,072 faces and vertices; overlapping calls on
+    one device race)."""
+    e = _D()
+    d = np.zeros((surface.nv, 3))
+    fp = np.zeros((surface.nf, 7)) if partials else None
+    _call(lib().ddgrs_exact_energy_and_differential_noalloc, surface.handle, exponents.alpha, exponents.beta,
+          ctypes.byref(e), _ptr(d), _ptr(fp) if partials else None)
     return (e.value, d, fp) if partials else (e.value, d)
 
 

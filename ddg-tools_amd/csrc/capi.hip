@@ -106,6 +106,7 @@ int ddgrs_ctx_create(int32_t ordinal, int32_t determinism, ddgrs_ctx** out, ddgr
     HIPCHECK(hipSetDevice(ordinal));
     HIPCHECK(hipGetDeviceProperties(&ctx->prop, ordinal));
     HIPCHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    noalloc_setup(ordinal);
     *out = ctx.release();
   });
 }

@@ -59,7 +59,14 @@ void pull_back(ddgrs_surface* s, const double* partials, double* out);
 void sum_f64(ddgrs_ctx* ctx, int64_t n, const double* x, double* out);
 
 // Exact all-pairs energy: per-face partials (7·nf, FP64) and per-face energy rows (nf, FP64).
+// chunk_partials: exact_pairs_chunk_elems(nf) float2 of device memory for the per-column-chunk sums,
+// or null for the context's scratch.
 void exact_pairs(ddgrs_ctx* ctx, int64_t nf, const FacePacks& packs, const double* area, const Exps& e,
-                 double* partials, double* row_energy);
+                 double* partials, double* row_energy, float2* chunk_partials = nullptr);
+// The float2 count of exact_pairs' per-column-chunk sums for nf faces (the partition depends on nf only).
+size_t exact_pairs_chunk_elems(int64_t nf);
+// Looks up the current device's address of the no-allocation entry point's scratch array (once per
+// device; called when a context is created on it).
+void noalloc_setup(int device);
 
 }  // namespace ddgrs

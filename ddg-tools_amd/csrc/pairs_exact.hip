@@ -176,17 +176,35 @@ __global__ __launch_bounds__(256) void exact_combine_kernel(int nf, int P, const
   }
 }
 
-void exact_pairs(ddgrs_ctx* ctx, int64_t nf64, const FacePacks& packs, const double* area, const Exps& e,
-                 double* partials, double* row_energy) {
-  if (nf64 > INT32_MAX / 8) invalid("faces", std::to_string(nf64), "at most 2^28 faces for the all-pairs kernel");
-  const int nf = (int)nf64;
+namespace {
+// The column partition: P chunks of `chunk` columns (a multiple of the tile), P · ⌈nf/256⌉ ≥ 2048
+// workgroups where nf allows; it depends on nf only, so results do not depend on the device.
+void pair_partition(int nf, int* P_out, int* chunk_out) {
   const int rb = (nf + kPairRows - 1) / kPairRows;
   int P = (2048 + rb - 1) / rb;
   P = std::max(1, std::min(P, (nf + kPairRows - 1) / kPairRows));
   int chunk = (nf + P - 1) / P;
   chunk = (chunk + kPairTile - 1) / kPairTile * kPairTile;
-  P = (nf + chunk - 1) / chunk;
-  float2* part = scratch_of(ctx).f2((size_t)P * 8 * nf);
+  *P_out = (nf + chunk - 1) / chunk;
+  *chunk_out = chunk;
+}
+}  // namespace
+
+size_t exact_pairs_chunk_elems(int64_t nf) {
+  if (nf <= 0 || nf > INT32_MAX / 8) return 0;
+  int P, chunk;
+  pair_partition((int)nf, &P, &chunk);
+  return (size_t)P * 8 * (size_t)nf;
+}
+
+void exact_pairs(ddgrs_ctx* ctx, int64_t nf64, const FacePacks& packs, const double* area, const Exps& e,
+                 double* partials, double* row_energy, float2* chunk_partials) {
+  if (nf64 > INT32_MAX / 8) invalid("faces", std::to_string(nf64), "at most 2^28 faces for the all-pairs kernel");
+  const int nf = (int)nf64;
+  const int rb = (nf + kPairRows - 1) / kPairRows;
+  int P, chunk;
+  pair_partition(nf, &P, &chunk);
+  float2* part = chunk_partials ? chunk_partials : scratch_of(ctx).f2((size_t)P * 8 * nf);
   dim3 grid(rb, P);
   if (e.alpha_int == 6 && e.beta_int == 12) {
     PairPow<0> pw{e.alpha, e.beta, 6, 12};

@@ -101,6 +101,17 @@ DDGRS_API int ddgrs_surface_get_positions(ddgrs_surface* s, double* positions, d
 DDGRS_API int ddgrs_exact_energy_and_differential(ddgrs_surface* s, double alpha, double beta, double* energy,
                                         double* differential, double* face_partials, ddgrs_error* err);
 
+/* The same as ddgrs_exact_energy_and_differential (same kernels, same results bit for bit), but
+ * allocating nothing: no counterpart in the reference; for callers that cannot allocate (tight
+ * serving loops, code that must not call hipMalloc).  The kernels' scratch memory is one 52 MiB
+ * array per device, resident until the process ends, so the mesh is limited to 131,072 faces and
+ * 131,072 vertices (larger: DDGRS_INVALID_OPTION, before any work).  Calls that overlap on one
+ * device share that array and race; serialize them.  Outputs are host memory as in the allocating
+ * entry point (the copies are made straight into them). */
+DDGRS_API int ddgrs_exact_energy_and_differential_noalloc(ddgrs_surface* s, double alpha, double beta, double* energy,
+                                                          double* differential, double* face_partials,
+                                                          ddgrs_error* err);
+
 /* ---- Barnes–Hut --------------------------------------------------------------------------- */
 
 /* Builds the face BVH (leaf size, split rule of RC/src/spatial/bvh.rs:126) and the frozen
