@@ -64,3 +64,32 @@ def test_struct_layouts_match_binding(tmp_path):
         assert c[name] == ctypes.sizeof(cls), name
         for f, _ in cls._fields_:
             assert c[f"{name}.{f}"] == getattr(cls, f).offset, (name, f)
+
+
+def _rust_stub():
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    return text[text.index("```rust\n#![allow"):]
+
+
+def test_rust_stub_structs_match_header():
+    """The -sys crate stub in INTEGRATION.md declares every header struct with the same fields in
+    the same order (the binding's ctypes structs, whose layout the test above checks against gcc)."""
+    pkg = load_package()
+    stub = _rust_stub()
+    pairs = {
+        "ddgrs_saddle_options": pkg._SaddleOptionsC,
+        "ddgrs_gmres_report": pkg._GmresReportC,
+        "ddgrs_step_options": pkg._StepOptionsC,
+        "ddgrs_step_report": pkg._StepReportC,
+    }
+    for name, cls in pairs.items():
+        body = re.search(r"pub struct " + name + r"\s*\{(.*?)\}", stub, re.S).group(1)
+        fields = re.findall(r"pub (\w+):", body)
+        assert fields == [f for f, _ in cls._fields_], name
+
+
+def test_rust_stub_declares_every_entry_point():
+    stub = _rust_stub()
+    declared = set(re.findall(r"pub fn (ddgrs_\w+)\(", stub))
+    missing = [s for s in declared_symbols() if s not in declared]
+    assert not missing, missing
