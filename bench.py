@@ -168,9 +168,18 @@ def scale_run(args) -> int:
     ms = (t1 - t0) * 1e3 / steps
     learned = None
     if not args.no_learned:
-        # retrained on trajectories at this size (fields of 64 bumps like the timed one)
+        # retrained on trajectories at this size (fields of 64 bumps like the timed one): unless set
+        # otherwise, 8 fields x 10 plain steps with three Krylov pairs each (320 pairs; the level-6
+        # recipe's 2,000 steps would take minutes at this size)
         surf.set_positions(p)
-        learned = learned_leg(pkg, ctx, args, opts, p, f, ms, [r.gmres.iterations for r in reps], level=7,
+        la = argparse.Namespace(**vars(args))
+        if "--learned-fields" not in sys.argv:
+            la.learned_fields = 8
+        if "--learned-steps" not in sys.argv:
+            la.learned_steps = 10
+        if "--learned-krylov" not in sys.argv:
+            la.learned_krylov = 3
+        learned = learned_leg(pkg, ctx, la, opts, p, f, ms, [r.gmres.iterations for r in reps], level=7,
                               bumps=(64, (0.0, 0.05), (0.05, 0.2)), steps=steps)
     line = {
         "metric": "Hs descent steps/sec at 327,680 faces (level-7 scale run)",
@@ -212,9 +221,11 @@ def learned_leg(pkg, ctx, args, opts, p, f, plain_ms, plain_its, level=6, bumps=
     steps = steps or args.steps
     base, _ = gen.icosphere(level)
     surf = pkg.Surface(ctx, f, p)
-    # four pairs per plain step: (g, A′⁻¹ g) and three Krylov pairs
-    npairs = args.learned_pairs or 4 * args.learned_fields * args.learned_steps
+    # per plain step: (g, A′⁻¹ g) and `learned_krylov` Krylov pairs (none for the default recipe: 2,000
+    # (residual, plain-preconditioned solve) pairs from 50 steps on each of 40 bump fields)
+    npairs = args.learned_pairs or (1 + args.learned_krylov) * args.learned_fields * args.learned_steps
     net = pkg.Learned(surf, seed=7, max_pairs=npairs)
+    net.set_krylov_pairs(args.learned_krylov)
     t0 = time.perf_counter()
     for k in range(args.learned_fields):
         surf.set_positions(gen.bumps(base, bumps[0], bumps[1], bumps[2], 1000 + k))
@@ -224,11 +235,31 @@ def learned_leg(pkg, ctx, args, opts, p, f, plain_ms, plain_its, level=6, bumps=
             ses.step()
         ses.close()
     t_pairs = time.perf_counter() - t0
-    iters = max(1, args.learned_epochs * net.info()["pairs"] // args.learned_batch)
+    per_epoch = max(1, net.info()["pairs"] // args.learned_batch)
+    iters = args.learned_epochs * per_epoch
     print(f"bench: learned: {net.info()['pairs']} pairs in {t_pairs:.1f} s, training {iters} updates", file=sys.stderr,
           flush=True)
+    # the judge of the kept parameters: GMRES iterations of the first two steps on a held-out bump
+    # field (neither a training field nor the timed one)
+    p_val = gen.bumps(base, bumps[0], bumps[1], bumps[2], 999)
+
+    def validate(model):
+        surf.set_positions(p_val)
+        ses = pkg.Session(surf, opts)
+        if model is not None:
+            ses.set_learned(model, 2)
+        total = 0
+        for _ in range(2):
+            r = ses.step()
+            total += r.learned_iterations if (model is not None and not r.learned_fallback) else \
+                r.gmres.iterations + (10 ** 6 if model is not None else 0)
+        ses.close()
+        return total
+
     t0 = time.perf_counter()
-    before, after = net.train(iters, batch=args.learned_batch, lr=args.learned_lr)
+    kept, val_its, losses = net.train_validated(args.learned_epochs, per_epoch, validate, batch=args.learned_batch,
+                                                lr=args.learned_lr)
+    before, after = losses[0][0], min(l[1] for l in losses)
     ctx.synchronize()
     t_train = time.perf_counter() - t0
     surf.set_positions(p)
@@ -270,6 +301,10 @@ def learned_leg(pkg, ctx, args, opts, p, f, plain_ms, plain_its, level=6, bumps=
         "seconds_training": t_train,
         "loss_before": before,
         "loss_after": after,
+        # validated training: GMRES iterations of two held-out steps before training (plain) and after
+        # each epoch; the kept parameters are those of `kept_epoch` (0: Q = 0, the plain preconditioner)
+        "validation_iterations": val_its,
+        "kept_epoch": kept,
         "gmres_iterations_learned": [r.learned_iterations for r in reps],
         "gmres_iterations_plain": plain_its,
         "fallbacks": sum(1 for r in reps if r.learned_fallback == 1),
@@ -297,11 +332,12 @@ def main() -> int:
                     help="l6: the headline step (default); l7: the level-7 scale run (theta 0.25, energy drift "
                          "against the FP64 oracle after steps 1, 10 and 20, fallback rate)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned preconditioner leg")
-    ap.add_argument("--learned-fields", type=int, default=4,
-                    help="bump fields of the training trajectories (the full training set: 40)")
-    ap.add_argument("--learned-steps", type=int, default=10, help="plain steps per field (the full set: 50)")
-    ap.add_argument("--learned-pairs", type=int, default=0, help="pairs kept (0: four per training step)")
-    ap.add_argument("--learned-epochs", type=int, default=20)
+    ap.add_argument("--learned-fields", type=int, default=40, help="bump fields of the training trajectories")
+    ap.add_argument("--learned-steps", type=int, default=50, help="plain steps per field")
+    ap.add_argument("--learned-krylov", type=int, default=0,
+                    help="Krylov pairs per training step after the step's own (residual, solve) pair")
+    ap.add_argument("--learned-pairs", type=int, default=0, help="pairs kept (0: every recorded pair)")
+    ap.add_argument("--learned-epochs", type=int, default=10)
     ap.add_argument("--learned-batch", type=int, default=8)
     ap.add_argument("--learned-lr", type=float, default=2e-3)
     args = ap.parse_args()

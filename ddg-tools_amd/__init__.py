@@ -159,6 +159,7 @@ SIGNATURES = {
     "ddgrs_learned_params": (_I32, [_P, _P, _P, _E]),
     "ddgrs_learned_set_scale": (_I32, [_P, _D, _E]),
     "ddgrs_learned_add_pair": (_I32, [_P, _P, _P, _P, _P, _E]),
+    "ddgrs_learned_set_krylov_pairs": (_I32, [_P, _I32, _E]),
     "ddgrs_learned_features": (_I32, [_P, _P, _P, _E]),
     "ddgrs_learned_blocks": (_I32, [_P, _P, _P, _E]),
     "ddgrs_learned_loss_grad": (_I32, [_P, _I64, _P, _P, _E]),
@@ -813,6 +814,10 @@ class Learned(_Handle):
     def set_scale(self, s: float):
         _call(lib().ddgrs_learned_set_scale, self._h, float(s))
 
+    def set_krylov_pairs(self, k: int):
+        """Krylov pairs a recording step adds after its own (0 to 8, default 3)."""
+        _call(lib().ddgrs_learned_set_krylov_pairs, self._h, int(k))
+
     def add_pair(self, positions, r, pr, y):
         n = 3 * self.surface.nv + 3
         a = [_f64(positions, (self.surface.nv, 3))] + [_f64(v, (n,)) for v in (r, pr, y)]
@@ -838,6 +843,25 @@ class Learned(_Handle):
         o = np.zeros(2)
         _call(lib().ddgrs_learned_train, self._h, iterations, batch, lr, _ptr(o))
         return float(o[0]), float(o[1])
+
+    def train_validated(self, rounds: int, updates: int, validate, batch: int = 8, lr: float = 1e-3):
+        """Adam in ``rounds`` rounds of ``updates`` updates, judged by GMRES: ``validate(net)``
+        returns the iterations of held-out solves with (I + Q) P⁻¹ (``validate(None)``: with the
+        plain preconditioner), measured before training and after every round, and the parameters
+        with the fewest iterations are kept — the starting ones (Q = 0: the plain preconditioner)
+        unless a round does strictly better.  The pair loss is not the judge: on the benchmark
+        surfaces lowering it can raise GMRES's iteration counts.  Returns (kept round, per-round
+        iterations, per-round (loss before, loss after))."""
+        best_p, best_it, kept = self.params(), validate(None), 0
+        its, losses = [best_it], []
+        for r in range(rounds):
+            losses.append(self.train(updates, batch, lr))
+            it = validate(self)
+            its.append(it)
+            if it < best_it:
+                best_p, best_it, kept = self.params(), it, r + 1
+        self.set_params(best_p)
+        return kept, its, losses
 
     def apply(self, positions, rs) -> np.ndarray:
         n = 3 * self.surface.nv + 3

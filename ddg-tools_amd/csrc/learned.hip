@@ -631,7 +631,13 @@ void LearnedModel::train(ddgrs_ctx* ctx, int iters, int batch, double lr, double
     for (int p = 0; p < npairs; ++p) s += loss_and_grad(ctx, p, false);
     return s / npairs;
   };
-  if (before) *before = mean_loss();
+  // The parameters with the lowest mean loss over all pairs, checked once per epoch and at the end,
+  // are the ones kept: training never ends above its starting loss (1 at Q = 0).
+  double best = mean_loss();
+  if (before) *before = best;
+  best_params.reserve(L.total);
+  HIPCHECK(hipMemcpyAsync(best_params.ptr, params.ptr, L.total * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+  const int epoch = std::max(1, npairs / std::max(1, batch));
   launch(ctx, "ln_fill", ln_fill_kernel, dim3(blocks_for((int64_t)L.total, 256)), dim3(256), 0, L.total, grad.ptr);
   int next = 0;
   for (int it = 0; it < iters; ++it) {
@@ -643,8 +649,17 @@ void LearnedModel::train(ddgrs_ctx* ctx, int iters, int batch, double lr, double
     const float c1 = (float)(1.0 - std::pow(0.9, (double)adam_t)), c2 = (float)(1.0 - std::pow(0.999, (double)adam_t));
     launch(ctx, "ln_adam", ln_adam_kernel, dim3(blocks_for((int64_t)L.total, 256)), dim3(256), 0, L.total, params.ptr,
            grad.ptr, adam_m.ptr, adam_v.ptr, (float)lr, 1.f / (float)batch, c1, c2);
+    if ((it + 1) % epoch == 0 || it + 1 == iters) {
+      const double l = mean_loss();
+      if (l < best) {
+        best = l;
+        HIPCHECK(hipMemcpyAsync(best_params.ptr, params.ptr, L.total * sizeof(float), hipMemcpyDeviceToDevice,
+                                ctx->stream));
+      }
+    }
   }
-  if (after) *after = mean_loss();
+  HIPCHECK(hipMemcpyAsync(params.ptr, best_params.ptr, L.total * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+  if (after) *after = best;
   blocks_ready = false;
 }
 
@@ -749,6 +764,14 @@ int ddgrs_learned_set_scale(ddgrs_learned* l, double scale, ddgrs_error* err) {
     need(l, "learned");
     if (!(std::isfinite(scale) && scale > 0.0)) invalid("scale", std::to_string(scale), "a finite number > 0");
     l->m.scale = (float)scale;
+  });
+}
+
+int ddgrs_learned_set_krylov_pairs(ddgrs_learned* l, int32_t krylov_pairs, ddgrs_error* err) {
+  return guarded(err, "learned_set_krylov_pairs", [&] {
+    need(l, "learned");
+    if (krylov_pairs < 0 || krylov_pairs > 8) invalid("krylov_pairs", std::to_string(krylov_pairs), "0 to 8");
+    l->m.krylov_pairs = krylov_pairs;
   });
 }
 

@@ -31,7 +31,7 @@ struct LearnedModel {
   std::vector<uint32_t> faces;          // the mesh (3 per face)
   std::vector<uint32_t> nbr_ptr_h, nbr_h;  // neighbours (mesh edges), ascending
   DevBuf<uint32_t> nbr_ptr, nbr;
-  DevBuf<float> params, grad, adam_m, adam_v, gpart;
+  DevBuf<float> params, grad, adam_m, adam_v, gpart, best_params;
   DevBuf<float> feat, act, pre, mean, out, dout, dh, dtmp, qd;  // activations and their gradients
   DevBuf<double> lsum;                   // loss partial sums
   // training pairs (features, x = c P⁻¹ r, e = y − x: FP32 per vertex) and ‖e‖²
@@ -39,6 +39,7 @@ struct LearnedModel {
   std::vector<double> pair_e2, pair_ratio;
   int npairs = 0, cap_pairs = 0;
   float scale = 0.f;  // s (set from the pairs at the first training)
+  int krylov_pairs = 3;  // pairs (A′ z, z) a recording step adds after its (g, A′⁻¹ g)
   int64_t adam_t = 0;
   bool blocks_ready = false;  // `out` holds the blocks of the last forward
 
@@ -51,8 +52,9 @@ struct LearnedModel {
   // Loss of pair p at the current parameters (read: returned, after a synchronization; else 0)
   // and (add_grad) its gradient added into `grad`.
   double loss_and_grad(ddgrs_ctx* ctx, int p, bool add_grad, bool read = true);
-  // Adam over the pairs (minibatches of `batch`, `iters` updates); returns the mean loss over all
-  // pairs before and after.
+  // Adam over the pairs (minibatches of `batch`, `iters` updates), keeping the parameters of the
+  // lowest mean loss over all pairs (checked before training, once per epoch and at the end);
+  // returns the mean loss over all pairs before and of the kept parameters.
   void train(ddgrs_ctx* ctx, int iters, int batch, double lr, double* before, double* after);
   // z += Q x for a full saddle vector x (3 nv + 3, device; the blocks of the last forward).
   void apply(ddgrs_ctx* ctx, const double* r, double* z);

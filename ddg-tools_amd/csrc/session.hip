@@ -356,7 +356,7 @@ void Session::step(StepReport* rep) {
     };
     add(g.ptr, pg.ptr, sol.ptr);
     double *z = pg.ptr, *r = kr.ptr, *pz = kz.ptr;
-    for (int k = 0; k < kKrylovPairs && learned->npairs < learned->cap_pairs; ++k) {
+    for (int k = 0; k < learned->krylov_pairs && learned->npairs < learned->cap_pairs; ++k) {
       op.apply(ctx, z, r);
       pc.apply(ctx, r, pz);
       add(r, pz, z);

@@ -65,7 +65,6 @@ struct Session {
   size_t last_iterations = 0;  // iterations of the last converged solve
   int learned_misses = 0;        // consecutive (I + Q) P⁻¹ solves that fell back to the plain one
   static constexpr int kLearnedMissLimit = 2;  // after that many, the session solves plain
-  static constexpr int kKrylovPairs = 3;  // pairs (A′ z, z) recorded per step after (g, A′⁻¹ g)
 
   Session() = default;
   Session(const Session&) = delete;

@@ -314,6 +314,9 @@ DDGRS_API int ddgrs_learned_info(ddgrs_learned* l, int64_t* out4, double* scale,
 DDGRS_API int ddgrs_learned_params(ddgrs_learned* l, float* get, const float* set, ddgrs_error* err);
 /* Sets s (else the first training sets it to the pairs' median ‖y − x‖ / ‖x‖). */
 DDGRS_API int ddgrs_learned_set_scale(ddgrs_learned* l, double scale, ddgrs_error* err);
+/* The Krylov pairs a recording step (session mode 1) adds after its own (g, A′⁻¹ g): 0 to 8
+ * (default 3); 0 records one (residual, plain-preconditioned solve) pair per step. */
+DDGRS_API int ddgrs_learned_set_krylov_pairs(ddgrs_learned* l, int32_t krylov_pairs, ddgrs_error* err);
 DDGRS_API int ddgrs_learned_add_pair(ddgrs_learned* l, const double* positions, const double* r, const double* pr,
                                      const double* y, ddgrs_error* err);
 /* The features (nv × 8, FP32) and the network's blocks (nv × 18: B_v row-major, then g_v). */
@@ -321,8 +324,10 @@ DDGRS_API int ddgrs_learned_features(ddgrs_learned* l, const double* positions, 
 DDGRS_API int ddgrs_learned_blocks(ddgrs_learned* l, const double* positions, float* blocks, ddgrs_error* err);
 /* The loss of one pair and (grad non-null) its gradient with respect to the parameters. */
 DDGRS_API int ddgrs_learned_loss_grad(ddgrs_learned* l, int64_t pair, double* loss, float* grad, ddgrs_error* err);
-/* Adam (β 0.9 / 0.999) over the pairs in order, `batch` pairs per update; loss2 = the mean loss
- * over all pairs before and after. */
+/* Adam (β 0.9 / 0.999) over the pairs in order, `batch` pairs per update; the parameters with the
+ * lowest mean loss over all pairs (checked before training, once per epoch of ⌊pairs / batch⌋
+ * updates and at the end) are kept, so the loss never ends above its starting value; loss2 = the
+ * mean loss over all pairs before and of the kept parameters. */
 DDGRS_API int ddgrs_learned_train(ddgrs_learned* l, int64_t iterations, int64_t batch, double lr, double* loss2,
                                   ddgrs_error* err);
 /* zs = Q rs at the given positions (the learned map alone, applied to the given vectors). */
@@ -330,7 +335,8 @@ DDGRS_API int ddgrs_learned_apply_batch(ddgrs_learned* l, const double* position
                                         double* zs, ddgrs_error* err);
 /* Attaches `l` (mesh of the session's surface) to the session: mode 1 solves with the plain
  * preconditioner and adds each step's pairs to `l` (r = the step's right-hand side g with y = the
- * direction, then three Krylov pairs r_k = A′ z_{k−1}, y = z_{k−1}, z₀ = P⁻¹ g, z_k = P⁻¹ r_k);
+ * direction, then ddgrs_learned_set_krylov_pairs Krylov pairs r_k = A′ z_{k−1}, y = z_{k−1},
+ * z₀ = P⁻¹ g, z_k = P⁻¹ r_k);
  * mode 2 solves with (I + Q) P⁻¹ and, when that solve has not reached the tolerance within twice the
  * last converged solve's iterations plus 5 (one restart cycle before any converged solve; at
  * most the iteration cap), again with the plain one;
