@@ -153,6 +153,7 @@ SIGNATURES = {
     "ddgrs_session_step": (_I32, [_P, _P, _E]),
     "ddgrs_session_state": (_I32, [_P, _P, _E]),
     "ddgrs_session_trials": (_I32, [_P, _I64, _P, _P, _P, ctypes.POINTER(_I64), _E]),
+    "ddgrs_session_fallbacks": (_I32, [_P, ctypes.c_char_p, _I64, ctypes.POINTER(_I64), _E]),
     "ddgrs_learned_create": (_I32, [_P, ctypes.c_uint64, _I64, ctypes.POINTER(_P), _E]),
     "ddgrs_learned_destroy": (None, [_P]),
     "ddgrs_learned_info": (_I32, [_P, _P, _P, _E]),
@@ -533,6 +534,10 @@ class _StepOptionsC(ctypes.Structure):
         ("max_halvings", _I64),
         ("start_kind", _I32),
         ("start_value", _D),
+        ("robust", _I32),
+        ("retry_max_iterations", _I64),
+        ("dense_max_vertices", _I64),
+        ("diverged_residual", _D),
     ]
 
 
@@ -565,6 +570,7 @@ class _StepReportC(ctypes.Structure):
         ("direction_norm", _D),
         ("direction_max", _D),
         ("halvings", _I64),
+        ("fallback_stage", _I32),
     ]
 
 
@@ -696,12 +702,18 @@ class StepOptions:
     tau_min_rel: float = 1e-12
     max_halvings: int = -1
     start: tuple = (0, 1.0)
+    # the recovery chain of a diverged solve (src/step/direction.rs:259 RobustHierarchical)
+    robust: bool = True
+    retry_max_iterations: int = 0  # 0: max(4 max_iterations, 800)
+    dense_max_vertices: int = 6000
+    diverged_residual: float = 1e-1  # an unconverged solve above it has diverged (the reference's 0.1)
 
     def _c(self) -> _StepOptionsC:
         return _StepOptionsC(self.exponents.alpha, self.exponents.beta, 1 if self.energy_exact else 0, self.theta,
                              self.saddle._c(), self.gmres.restart, self.gmres.tolerance, self.gmres.max_iterations,
                              self.armijo_c, self.tau_min_rel, self.max_halvings, int(self.start[0]),
-                             float(self.start[1]))
+                             float(self.start[1]), 1 if self.robust else 0, self.retry_max_iterations,
+                             self.dense_max_vertices, self.diverged_residual)
 
 
 @dataclasses.dataclass(frozen=True)
@@ -728,6 +740,8 @@ class StepReport:
     direction_max: float = 0.0
     halvings: int = 0
     trial_list: tuple = ()  # (τ, F, TrialOutcome) per trial, in order
+    fallback_stage: int = 0  # 1: the GMRES retry solved the step, 2: the dense LU did
+    fallbacks: tuple = ()  # the escalations as text (LinearReport::fallbacks)
 
 
 class Session(_Handle):
@@ -753,7 +767,15 @@ class Session(_Handle):
                           float(r.tau), float(r.tau0), bool(r.accepted), int(r.trials), _gmres_report(r.gmres),
                           int(r.pairs_exact), int(r.nodes_lumped), sec, int(r.learned_mode), int(r.learned_fallback),
                           int(r.learned_iterations), float(r.energy_after), float(r.direction_norm),
-                          float(r.direction_max), int(r.halvings), self.trials())
+                          float(r.direction_max), int(r.halvings), self.trials(), int(r.fallback_stage),
+                          self.fallbacks())
+
+    def fallbacks(self) -> tuple:
+        """The last step's escalations of the recovery chain, as text."""
+        n = _I64()
+        buf = ctypes.create_string_buffer(4096)
+        _call(lib().ddgrs_session_fallbacks, self._h, buf, 4096, ctypes.byref(n))
+        return tuple(x for x in buf.value.decode().split("\n") if x)
 
     def trials(self) -> tuple:
         """The last step's trials as (τ, F, ``TrialOutcome``) in order (``StepReport::trials``)."""

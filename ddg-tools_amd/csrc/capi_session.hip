@@ -3,6 +3,7 @@
 #include <cstring>
 
 #include "capi_util.hpp"
+#include "robust.hpp"
 #include "session.hpp"
 
 using namespace ddgrs;
@@ -184,6 +185,10 @@ void ddgrs_step_options_default(ddgrs_step_options* o) {
   o->max_halvings = -1;
   o->start_kind = DDGRS_START_GROW_PREVIOUS;
   o->start_value = 1.0;
+  o->robust = 1;
+  o->retry_max_iterations = 0;
+  o->dense_max_vertices = kDenseFallbackMaxVertices;
+  o->diverged_residual = 1e-1;
 }
 
 int ddgrs_session_create(ddgrs_surface* s, const ddgrs_step_options* o, ddgrs_session** out, ddgrs_error* err) {
@@ -224,6 +229,16 @@ int ddgrs_session_create(ddgrs_surface* s, const ddgrs_step_options* o, ddgrs_se
     so.max_halvings = (long)o->max_halvings;
     so.start_kind = o->start_kind;
     so.start_value = o->start_value;
+    if (o->retry_max_iterations < 0)
+      invalid("retry_max_iterations", std::to_string(o->retry_max_iterations), "0 (the default cap) or more");
+    if (o->dense_max_vertices < 0)
+      invalid("dense_max_vertices", std::to_string(o->dense_max_vertices), "non-negative");
+    so.robust = o->robust != 0;
+    so.retry_max_iterations = (long)o->retry_max_iterations;
+    so.dense_max_vertices = o->dense_max_vertices;
+    if (!(o->diverged_residual >= 0.0 && std::isfinite(o->diverged_residual)))
+      invalid("diverged_residual", std::to_string(o->diverged_residual), "a finite number >= 0");
+    so.diverged_residual = o->diverged_residual;
     auto h = std::make_unique<ddgrs_session>();
     h->s = s;
     h->ses.init(s, so);
@@ -254,6 +269,21 @@ int ddgrs_session_trials(ddgrs_session* h, int64_t capacity, double* tau, double
       if (energy) energy[i] = tr[i].energy;
       if (outcome) outcome[i] = tr[i].outcome;
     }
+  });
+}
+
+int ddgrs_session_fallbacks(ddgrs_session* h, char* text, int64_t capacity, int64_t* count, ddgrs_error* err) {
+  return guarded(err, "session_fallbacks", [&] {
+    need(h, "session");
+    need(count, "count");
+    const auto& ev = h->ses.fallbacks;
+    *count = (int64_t)ev.size();
+    if (!text || capacity <= 0) return;
+    std::string all;
+    for (const auto& e : ev) all += e + "\n";
+    const size_t m = std::min<size_t>(all.size(), (size_t)capacity - 1);
+    std::memcpy(text, all.data(), m);
+    text[m] = '\0';
   });
 }
 

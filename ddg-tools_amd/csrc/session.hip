@@ -16,6 +16,7 @@
 #include "bh.hpp"
 #include "capi_util.hpp"
 #include "gmres.hpp"
+#include "robust.hpp"
 #include "session.hpp"
 
 namespace ddgrs {
@@ -336,6 +337,41 @@ void Session::step(StepReport* rep) {
   }
   if (!solved) gr = gmres_device(ctx, n, A, P, g.ptr, sol.ptr, gs, vec, stall);
   fm.extended = pc.vf.extended = false;
+  // the recovery chain of a diverged solve (RS/src/step/direction.rs:277 escalate): right GMRES
+  // with restart 60 and at least 800 iterations, then a dense LU of the exact metric's saddle
+  // system for meshes of at most dense_max_vertices vertices
+  fallbacks.clear();
+  if (!gr.converged && !(gr.relative_residual <= opts.diverged_residual) && opts.robust) {
+    const double r0 = gr.relative_residual;
+    char line[256];
+    GmresSettings g2{60, opts.tolerance,
+                     opts.retry_max_iterations > 0 ? (size_t)opts.retry_max_iterations
+                                                   : std::max<size_t>(4 * opts.max_iterations, 800)};
+    GmresResult gr2 = gmres_device(ctx, n, A, P, g.ptr, sol.ptr, g2, vec, stall);
+    fm.extended = pc.vf.extended = false;
+    if (gr2.converged || gr2.relative_residual <= opts.diverged_residual) {
+      std::snprintf(line, sizeof line,
+                    "gmres diverged (relative residual %.3e): retried with right GMRES, augmented preconditioner on "
+                    "vertex pairs, restart 60, at most %zu iterations (final residual %.3e)",
+                    r0, g2.max_iterations, gr2.relative_residual);
+      fallbacks.push_back(line);
+      gr = gr2;
+      rep->fallback_stage = 1;
+    } else if (nv <= opts.dense_max_vertices) {
+      const double rr = dense_saddle_solve(ctx, s, topo, clusters, s_exp, fm.w_hi, fm.w_lo, pc.crow_dev, g.ptr, sol.ptr);
+      std::snprintf(line, sizeof line,
+                    "gmres diverged (relative residual %.3e): dense LU of the %lld x %lld metric (residual %.3e)", r0,
+                    (long long)nv, (long long)nv, rr);
+      fallbacks.push_back(line);
+      gr = GmresResult();
+      gr.converged = rr <= opts.tolerance;
+      gr.relative_residual = rr;
+      rep->fallback_stage = 2;
+    } else {
+      gr = gr2;
+      gr.relative_residual = r0;  // SolverDiverged reports the first solve's residual
+    }
+  }
   if (gr.converged) last_iterations = gr.iterations;
   if (lmode == 1 && gr.converged) {
     // the pair (g, P⁻¹ g, A′⁻¹ g) at this step's positions, then pairs of the vectors the plain
@@ -372,7 +408,7 @@ void Session::step(StepReport* rep) {
   rep->gmres.extended = gr.extended ? 1 : 0;
   rep->gmres.relative_residual = gr.relative_residual;
   last_gmres = gr;
-  if (!gr.converged && !(gr.relative_residual <= 1e-1)) {
+  if (!gr.converged && !(gr.relative_residual <= opts.diverged_residual)) {
     Error e(DDGRS_SOLVER_DIVERGED, "gmres", "true relative residual above 0.1 after the iteration cap");
     e.value = gr.relative_residual;
     throw e;

@@ -3,6 +3,8 @@
 // barycenter projection.
 #pragma once
 #include <future>
+#include <string>
+#include <vector>
 
 #include "bh.hpp"
 #include "gmres.hpp"
@@ -28,6 +30,10 @@ struct StepOptions {
   long max_halvings = -1;        // no cap when negative
   int start_kind = DDGRS_START_GROW_PREVIOUS;
   double start_value = 1.0;      // τ (absolute) or the lengths of MaxDisplacement
+  bool robust = true;            // the recovery chain of a diverged solve
+  long retry_max_iterations = 0; // its GMRES retry's cap (0: max(4 max_iterations, 800))
+  int64_t dense_max_vertices = 6000;  // its dense LU's largest mesh
+  double diverged_residual = 1e-1;    // an unconverged solve above it has diverged
 };
 
 // TrialOutcome (RS/src/step/session.rs): 0 accepted, 1 Armijo failed, 2 non-finite.
@@ -62,6 +68,7 @@ struct Session {
   int learned_mode = 0;
   DevBuf<double> pg, kr, kz;  // P⁻¹ g and the Krylov vectors of recorded pairs
   std::vector<TrialRecord> trials;  // the last step's line search
+  std::vector<std::string> fallbacks;  // the last step's escalations (LinearReport::fallbacks)
   size_t last_iterations = 0;  // iterations of the last converged solve
   int learned_misses = 0;        // consecutive (I + Q) P⁻¹ solves that fell back to the plain one
   static constexpr int kLearnedMissLimit = 2;  // after that many, the session solves plain

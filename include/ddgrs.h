@@ -250,6 +250,16 @@ typedef struct ddgrs_step_options {
   int64_t max_halvings;    /* < 0: no cap (LineSearch::max_halvings) */
   int32_t start_kind;      /* enum ddgrs_start_kind */
   double start_value;      /* τ of Absolute, the lengths of MaxDisplacement (GrowPrevious's first) */
+  /* The recovery chain of a diverged solve (RS/src/step/direction.rs:259-338 RobustHierarchical):
+   * robust 1 retries with right GMRES at restart 60 and retry_max_iterations (0: at least 800, four
+   * times max_iterations), then solves by a dense LU of the exact metric's saddle system on meshes
+   * of at most dense_max_vertices (6000) vertices; robust 0 returns SolverDiverged at once.
+   * diverged_residual: a solve that ends unconverged with its explicit relative residual above it
+   * has diverged (the reference's 0.1, RS/src/solve/gmres.rs). */
+  int32_t robust;
+  int64_t retry_max_iterations;
+  int64_t dense_max_vertices;
+  double diverged_residual;
 } ddgrs_step_options;
 
 /* A step's report (RS/src/step/session.rs:146 StepReport, the fields this path produces).
@@ -272,7 +282,14 @@ typedef struct ddgrs_step_report {
   double seconds_learned; /* features and the network's forward pass */
   double energy_after, direction_norm, direction_max;
   int64_t halvings;
+  /* The recovery chain: 0 not needed, 1 the GMRES retry solved the step (then `gmres` is its
+   * report), 2 the dense LU did (gmres.relative_residual is its explicit residual). */
+  int32_t fallback_stage;
 } ddgrs_step_report;
+
+/* The escalations of the last step as text (LinearReport::fallbacks, RS/src/solve/mod.rs), one a
+ * line, NUL-terminated within `capacity` bytes; *count = the number of escalations. */
+DDGRS_API int ddgrs_session_fallbacks(ddgrs_session* h, char* text, int64_t capacity, int64_t* count, ddgrs_error* err);
 
 /* The trials of the last step in order (StepReport::trials, RS/src/step/session.rs:136 Trial): τ,
  * F (NaN when not finite) and the outcome (TrialOutcome: 0 accepted, 1 Armijo failed, 2 non-finite);

@@ -352,3 +352,47 @@ def test_step_c1_exact_energy(pkg, ctx):
     assert abs(r.gmres.iterations - r0["gmres_iterations"]) <= 1
     assert r.gmres.converged and r.gmres.relative_residual <= 1e-6
     assert np.abs(x - x0).max() <= 1e-4 * np.abs(x0 - p).max()
+
+
+def test_recovery_chain_retry(pkg, ctx):
+    """A diverged solve (forced from the host: one GMRES iteration, every unconverged solve counted
+    as diverged) is retried with right GMRES at restart 60 and 800 iterations (direction.rs:277
+    escalate, stage 1); the step then equals the ordinary one."""
+    p, f = pkg.generate.config_icosphere_l3()
+    ref = pkg.Session(pkg.Surface(ctx, f, p)).step()
+    s = pkg.Surface(ctx, f, p)
+    r = pkg.Session(s, pkg.StepOptions(gmres=pkg.GmresOptions(max_iterations=1), diverged_residual=1e-30)).step()
+    print(f"retry: stage {r.fallback_stage}, {r.fallbacks}, gmres {r.gmres.iterations} residual "
+          f"{r.gmres.relative_residual:.2e}")
+    assert r.fallback_stage == 1 and len(r.fallbacks) == 1 and "retried" in r.fallbacks[0]
+    assert r.gmres.converged and r.gmres.relative_residual <= 1e-6
+    assert r.trials == ref.trials and abs(r.tau - ref.tau) <= 1e-4 * ref.tau
+    assert abs(r.energy_trial - ref.energy_trial) <= 1e-6 * abs(ref.energy_trial)
+
+
+def test_recovery_chain_dense(pkg, ctx):
+    """When the retry diverges too, a mesh of at most 6,000 vertices is solved by a dense LU of
+    the exact metric's saddle system on the device (direction.rs:277 escalate, stage 2): the step
+    equals the oracle's step with the exact (χ = 0) metric solved to 1e-12."""
+    p, f = pkg.generate.config_icosphere_l3()
+    s = pkg.Surface(ctx, f, p)
+    opts = pkg.StepOptions(gmres=pkg.GmresOptions(max_iterations=1), diverged_residual=1e-30, retry_max_iterations=1)
+    r = pkg.Session(s, opts).step()
+    r0 = oracle.Session(f, p, chi=0.0, tol=1e-12, maxit=500).step(threads=16)
+    print(f"dense: stage {r.fallback_stage}, {r.fallbacks}; tau {r.tau:.8g} / {r0['tau']:.8g}, "
+          f"E {r.energy_trial:.12g} / {r0['energy1']:.12g}")
+    assert r.fallback_stage == 2 and len(r.fallbacks) == 1 and "dense LU" in r.fallbacks[0]
+    assert r.gmres.converged and r.gmres.relative_residual <= 1e-10
+    assert abs(r.sobolev_norm_sq - r0["slope"]) <= 1e-5 * abs(r0["slope"])
+    assert r.trials == int(r0["trials"]) and abs(r.tau - r0["tau"]) <= 1e-5 * r0["tau"]
+    assert abs(r.energy_trial - r0["energy1"]) <= 1e-5 * abs(r0["energy1"])
+
+
+def test_recovery_chain_off(pkg, ctx):
+    p, f = pkg.generate.config_icosphere_l3()
+    s = pkg.Surface(ctx, f, p)
+    ses = pkg.Session(s, pkg.StepOptions(gmres=pkg.GmresOptions(max_iterations=1), diverged_residual=1e-30,
+                                         robust=False))
+    with pytest.raises(pkg.RsError) as e:
+        ses.step()
+    assert e.value.kind == "SolverDiverged"
