@@ -320,13 +320,61 @@ def learned_leg(pkg, ctx, args, opts, p, f, plain_ms, plain_its, level=6, bumps=
     }
 
 
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for ln in fh:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(args, f, p, opts) -> dict:
+    """The FP64 oracle's step (this repository's C++ restatement of the reference's Rust CPU path,
+    which cannot be built here) on the host cores, on a bounded sample of the same workload: on all
+    the threads the GPU's share of the box gives (16 at most), `cpu_warmup` warm-up steps then the
+    median of `cpu_steps` consecutive steps from the seeded input; one step each on 4 threads and
+    on 1 thread; per-phase medians."""
+    import oracle
+
+    nproc = os.cpu_count() or 1
+    full = min(16, nproc)
+    out = {"unit": "steps/s", "kind": "port", "nproc": nproc, "cpu_model": _cpu_model()}
+    runs = {}
+    for threads, warm, steps in ((full, args.cpu_warmup, args.cpu_steps), (4, 0, 1), (1, 0, 1)):
+        if threads > nproc or threads in runs:
+            continue
+        print(f"bench: oracle baseline, {warm} + {steps} step(s) on {threads} threads", file=sys.stderr, flush=True)
+        ors = oracle.Session(f, p, theta=opts.theta)
+        for _ in range(warm):
+            ors.step(threads=threads)
+        times, phases = [], []
+        for _ in range(steps):
+            c0 = time.perf_counter()
+            rep = ors.step(threads=threads)
+            times.append(time.perf_counter() - c0)
+            phases.append(rep["seconds"])
+        med = float(np.median(times))
+        runs[threads] = {"steps_per_s": 1.0 / med, "median_s": med, "steps": steps, "warmup": warm,
+                         "phases_s": {k: float(np.median([ph[k] for ph in phases])) for k in phases[0]}}
+    out["value"] = runs[full]["steps_per_s"]
+    out["cores"] = full
+    out["runs"] = {str(k): v for k, v in runs.items()}
+    out["sample"] = (f"FP64 oracle Hs step (same workload, Barnes-Hut theta 0.5): {full} threads, {args.cpu_warmup} "
+                     f"warm-up + median of {args.cpu_steps} steps from the seeded input; 4 and 1 threads, one step each")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=5, help="timed oracle steps on all threads (median)")
+    ap.add_argument("--cpu-warmup", type=int, default=2, help="oracle warm-up steps on all threads")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 --pmc traffic passes")
     ap.add_argument("--config", choices=["l6", "l7"], default="l6",
                     help="l6: the headline step (default); l7: the level-7 scale run (theta 0.25, energy drift "
@@ -426,23 +474,7 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu_baseline:
-        import oracle
-
-        threads = min(16, os.cpu_count() or 1)
-        print(f"bench: oracle baseline, {args.cpu_steps} step(s) on {threads} threads", file=sys.stderr, flush=True)
-        ors = oracle.Session(f, p, theta=opts.theta)
-        c0 = time.perf_counter()
-        for _ in range(args.cpu_steps):
-            ors.step(threads=threads)
-        c1 = time.perf_counter()
-        cpu = {
-            "value": args.cpu_steps / (c1 - c0),
-            "unit": "steps/s",
-            "cores": threads,
-            "kind": "port",
-            "sample": f"FP64 oracle Hs step (same workload, Barnes-Hut theta 0.5), the first {args.cpu_steps} steps "
-                      f"from the seeded input, {c1 - c0:.1f} s",
-        }
+        cpu = cpu_baseline(args, f, p, opts)
 
     its = [r.gmres.iterations for r in reps]
     stage = {k: float(np.mean([r.seconds[k] for r in reps]) * 1e3) for k in reps[0].seconds}

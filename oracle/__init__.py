@@ -409,7 +409,7 @@ class Session:
         """One step; the report's ``trials`` is a list of (τ, F, outcome) with outcome 0 accepted,
         1 Armijo failed, 2 non-finite."""
         cap = 256
-        rep = np.zeros(16)
+        rep = np.zeros(21)
         tr = np.zeros((cap, 3))
         st = lib().orc_session_step(self._h, threads, _ptr(rep), _ptr(tr), cap)
         if st != 0:
@@ -418,6 +418,8 @@ class Session:
                 "gmres_relative_residual", "energy_after", "tau0", "gradient_norm", "direction_norm", "direction_max",
                 "halvings", "lbar"]
         out = dict(zip(keys, rep.tolist()))
+        out["seconds"] = dict(zip(["trees_energy", "solver_build", "gmres", "line_search", "projection"],
+                                  rep[16:21].tolist()))
         out["trial_list"] = [(float(t), float(e), int(o)) for t, e, o in tr[: min(cap, int(out["trials"]))]]
         return out
 

@@ -373,7 +373,8 @@ void* orc_session_create(int64_t nv, int64_t nf, const uint32_t* faces, const do
 void orc_session_free(void* h) { delete static_cast<Session*>(h); }
 
 // report: [energy0, energy1, slope, tau, accepted, trials, gmres iterations, gmres converged,
-// gmres relres, energy after the projection, τ₀, ‖g‖, ‖x‖, ‖x‖_∞, halvings, ℓ̄₀];
+// gmres relres, energy after the projection, τ₀, ‖g‖, ‖x‖, ‖x‖_∞, halvings, ℓ̄₀, then the wall
+// seconds of trees + F, dF / the solver's build / GMRES / the line search / the projection];
 // trials (cap × 3): per trial [τ, F, outcome (0 accepted, 1 Armijo failed, 2 non-finite)]
 int orc_session_step(void* h, int threads, double* report, double* trials, int64_t cap) {
   Error err;
@@ -396,6 +397,7 @@ int orc_session_step(void* h, int threads, double* report, double* trials, int64
   report[13] = rep.direction_max;
   report[14] = (double)rep.halvings;
   report[15] = s->l0;
+  for (int k = 0; k < 5; ++k) report[16 + k] = rep.seconds[k];
   for (size_t t = 0; t < rep.trial.size() && (int64_t)t < cap; ++t) {
     trials[3 * t] = rep.trial[t].tau;
     trials[3 * t + 1] = rep.trial[t].energy;

@@ -313,6 +313,8 @@ struct StepReport {
   size_t trials = 0, halvings = 0;
   std::vector<Trial> trial;
   GmresReport gmres;
+  // wall seconds of the phases: trees and F, dF; the solver's build; GMRES; line search; projection
+  double seconds[5] = {0, 0, 0, 0, 0};
 };
 bool session_create(const Mesh& m, const Positions& p, const Exponents& e, const SessionOptions& o, Session* s,
                     Error* err);

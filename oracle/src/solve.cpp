@@ -30,6 +30,7 @@
 // projection, tests/oracle.rs:557-640 tolerances), plus the Armijo / energy / barycenter
 // properties of accepted steps.
 #include <algorithm>
+#include <chrono>
 #include <numeric>
 
 #include "oracle.hpp"
@@ -477,6 +478,12 @@ bool session_step(Session& s, int threads, StepReport* rep, Error* err) {
   const Mesh& m = s.mesh;
   const size_t nv = (size_t)m.nv, nd = 3 * nv;
   const bool exact = s.opts.energy_exact;
+  auto clock = std::chrono::steady_clock::now();
+  auto lap = [&](int k) {
+    const auto now = std::chrono::steady_clock::now();
+    rep->seconds[k] = std::chrono::duration<double>(now - clock).count();
+    clock = now;
+  };
   // 1. trees, lists, F and dF (objective.rs:71 build, :151 evaluate)
   BhTree bh;
   double f0;
@@ -500,14 +507,17 @@ bool session_step(Session& s, int threads, StepReport* rep, Error* err) {
   std::vector<double> g(nd);
   for (size_t v = 0; v < nv; ++v)
     for (int k = 0; k < 3; ++k) g[3 * v + k] = gv[v][k];
+  lap(0);
   // 2. the solver and the direction x = A′⁻¹ [g; 0] (session.rs:820-860; k = 0 Schur columns)
   Saddle sd;
   if (!saddle_build(m, s.positions, s.exps, s.opts.saddle, threads, &sd, err)) return false;
+  lap(1);
   std::vector<double> rhs(nd + 3, 0.0), sol;
   std::copy(g.begin(), g.end(), rhs.begin());
   auto op = [&](const double* x, double* y) { saddle_apply(sd, x, y, threads); };
   auto pc = [&](const double* r, double* z) { precond_apply(sd, r, z, threads); };
   rep->gmres = gmres_right(op, pc, rhs, s.opts.gmres, &sol);
+  lap(2);
   if (!rep->gmres.converged && !(rep->gmres.relative_residual <= 1e-1)) {
     err->status = SOLVER_DIVERGED;
     err->detail = "GMRES diverged";
@@ -557,6 +567,7 @@ bool session_step(Session& s, int threads, StepReport* rep, Error* err) {
     }
   }
   if (!rep->accepted) rep->energy1 = f0;
+  lap(3);
   // 4. projection (constraints/mod.rs project_positional): translate by −Φ, Φ the barycenter's
   // offset from its target (areas at the moved positions); then F at the projected positions
   Positions np = rep->accepted ? trial : s.positions;
@@ -564,6 +575,7 @@ bool session_step(Session& s, int threads, StepReport* rep, Error* err) {
   for (size_t v = 0; v < nv; ++v)
     for (int k = 0; k < 3; ++k) np[v][k] -= bc[k] - s.barycenter_target[k];
   rep->energy_after = value_at(np);
+  lap(4);
   s.positions = np;
   if (rep->accepted) s.last_tau = tau;
   rep->tau = rep->accepted ? tau : 0.0;
