@@ -40,6 +40,11 @@ KINDS = {
     6: "DegenerateFace",
     7: "NonFinitePosition",
     8: "Gpu",
+    # raised by the Python layer (io.py, api.py)
+    9: "Parse",
+    10: "Io",
+    11: "PolygonFace",
+    12: "Unsupported",
 }
 
 
@@ -909,3 +914,5 @@ def energy(surface: Surface, exponents: Exponents = Exponents(), eval=EnergyEval
 
 
 from . import generate  # noqa: E402,F401
+from . import io  # noqa: E402,F401
+from . import api  # noqa: E402,F401

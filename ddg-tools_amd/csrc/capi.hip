@@ -117,7 +117,12 @@ int ddgrs_device_info(ddgrs_ctx* ctx, char* name, int64_t name_len, int64_t* com
                       ddgrs_error* err) {
   return guarded(err, "device_info", [&] {
     need(ctx, "ctx");
-    if (name && name_len > 0) std::snprintf(name, (size_t)name_len, "%s (%s)", ctx->prop.name, ctx->prop.gcnArchName);
+    if (name && name_len > 0) {
+      if (ctx->prop.name[0])
+        std::snprintf(name, (size_t)name_len, "%s (%s)", ctx->prop.name, ctx->prop.gcnArchName);
+      else  // some runtimes leave the marketing name empty
+        std::snprintf(name, (size_t)name_len, "AMD GPU (%s)", ctx->prop.gcnArchName);
+    }
     if (compute_units) *compute_units = ctx->prop.multiProcessorCount;
     if (memory_bytes) *memory_bytes = (int64_t)ctx->prop.totalGlobalMem;
   });
