@@ -159,6 +159,8 @@ SIGNATURES = {
     "ddgrs_session_state": (_I32, [_P, _P, _E]),
     "ddgrs_session_trials": (_I32, [_P, _I64, _P, _P, _P, ctypes.POINTER(_I64), _E]),
     "ddgrs_session_fallbacks": (_I32, [_P, ctypes.c_char_p, _I64, ctypes.POINTER(_I64), _E]),
+    "ddgrs_session_gmres_history": (_I32, [_P, _P, _I64, _P, _I64, _P, _E]),
+    "ddgrs_saddle_gmres_history": (_I32, [_P, _P, _I64, _P, _I64, _P, _E]),
     "ddgrs_learned_create": (_I32, [_P, ctypes.c_uint64, _I64, ctypes.POINTER(_P), _E]),
     "ddgrs_learned_destroy": (None, [_P]),
     "ddgrs_learned_info": (_I32, [_P, _P, _P, _E]),
@@ -615,11 +617,23 @@ class GmresReport:
     converged: bool
     relative_residual: float
     extended: bool = False  # the products switched to the extended near field (FP32 residual floor reached)
+    cycle_residuals: tuple = ()  # the explicit relative residual at every cycle start and at the end
+    history: tuple = ()  # the Arnoldi estimate after every iteration
 
 
-def _gmres_report(c: _GmresReportC) -> GmresReport:
+def _gmres_history(fn, h) -> tuple:
+    """(cycle_residuals, history) of the last solve through ``fn`` (a ``*_gmres_history`` entry)."""
+    counts = np.zeros(2, dtype=np.int64)
+    _call(fn, h, None, 0, None, 0, _ptr(counts))
+    c = np.zeros(int(counts[0]))
+    hist = np.zeros(int(counts[1]))
+    _call(fn, h, _ptr(c), c.size, _ptr(hist), hist.size, _ptr(counts))
+    return tuple(float(v) for v in c), tuple(float(v) for v in hist)
+
+
+def _gmres_report(c: _GmresReportC, history: tuple = ((), ())) -> GmresReport:
     return GmresReport(int(c.iterations), int(c.cycles), int(c.operator_applications), int(c.preconditioner_applications),
-                       bool(c.converged), float(c.relative_residual), bool(c.extended))
+                       bool(c.converged), float(c.relative_residual), bool(c.extended), *history)
 
 
 class Saddle(_Handle):
@@ -658,7 +672,7 @@ class Saddle(_Handle):
         rep = _GmresReportC()
         _call(lib().ddgrs_saddle_solve, self._h, _ptr(bb), _ptr(x), options.restart, options.tolerance,
               options.max_iterations, ctypes.byref(rep))
-        return x, _gmres_report(rep)
+        return x, _gmres_report(rep, _gmres_history(lib().ddgrs_saddle_gmres_history, self._h))
 
 
 class StartRule:
@@ -769,7 +783,8 @@ class Session(_Handle):
         _call(lib().ddgrs_session_step, self._h, ctypes.byref(r))
         sec = {k[len("seconds_"):]: float(getattr(r, k)) for k, _ in _StepReportC._fields_ if k.startswith("seconds_")}
         return StepReport(float(r.energy_before), float(r.energy_trial), float(r.gradient_norm), float(r.sobolev_norm_sq),
-                          float(r.tau), float(r.tau0), bool(r.accepted), int(r.trials), _gmres_report(r.gmres),
+                          float(r.tau), float(r.tau0), bool(r.accepted), int(r.trials),
+                          _gmres_report(r.gmres, _gmres_history(lib().ddgrs_session_gmres_history, self._h)),
                           int(r.pairs_exact), int(r.nodes_lumped), sec, int(r.learned_mode), int(r.learned_fallback),
                           int(r.learned_iterations), float(r.energy_after), float(r.direction_norm),
                           float(r.direction_max), int(r.halvings), self.trials(), int(r.fallback_stage),

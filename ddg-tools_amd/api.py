@@ -145,7 +145,8 @@ class RepulsiveSurfaceSession:
             "trials": [{"tau": t, "energy": e, "outcome": ("accepted", "armijo_failed", "non_finite")[o]}
                        for t, e, o in r.trial_list],
             "linear": {"iterations": g.iterations, "cycles": g.cycles, "converged": g.converged,
-                       "relative_residual": g.relative_residual, "fallbacks": list(r.fallbacks)},
+                       "relative_residual": g.relative_residual, "cycle_residuals": list(g.cycle_residuals),
+                       "history": list(g.history), "fallbacks": list(r.fallbacks)},
             "remesh": None, "seconds": dict(r.seconds),
         }
         return self._report

@@ -14,6 +14,7 @@ struct ddgrs_saddle {
   Preconditioner pc;
   SaddleOp op;
   DeviceVecs vec;
+  GmresResult last;  // the last solve's report, with its cycle residuals and history
 };
 
 struct ddgrs_session {
@@ -47,6 +48,14 @@ void batch(ddgrs_ctx* ctx, int64_t nb, int64_t n, const double* xs, double* ys, 
     HIPCHECK(hipMemcpyAsync(ys + q * n, dy.ptr, n * 8, hipMemcpyDeviceToHost, ctx->stream));
   }
   ctx->sync();
+}
+
+void copy_history(const GmresResult& r, double* cycle_residuals, int64_t cap_cycles, double* history,
+                  int64_t cap_history, int64_t* counts2) {
+  counts2[0] = (int64_t)r.cycle_residuals.size();
+  counts2[1] = (int64_t)r.history.size();
+  for (int64_t i = 0; cycle_residuals && i < cap_cycles && i < counts2[0]; ++i) cycle_residuals[i] = r.cycle_residuals[i];
+  for (int64_t i = 0; history && i < cap_history && i < counts2[1]; ++i) history[i] = r.history[i];
 }
 
 void fill_report(const GmresResult& r, ddgrs_gmres_report* out) {
@@ -166,7 +175,17 @@ int ddgrs_saddle_solve(ddgrs_saddle* h, const double* b, double* x, int64_t rest
     fm.extended = h->pc.vf.extended = false;
     download(ctx, x, (const double*)dx.ptr, n);
     ctx->sync();
+    h->last = r;
     if (report) fill_report(r, report);
+  });
+}
+
+int ddgrs_saddle_gmres_history(ddgrs_saddle* h, double* cycle_residuals, int64_t cap_cycles, double* history,
+                               int64_t cap_history, int64_t* counts2, ddgrs_error* err) {
+  return guarded(err, "saddle_gmres_history", [&] {
+    need(h, "saddle");
+    need(counts2, "counts");
+    copy_history(h->last, cycle_residuals, cap_cycles, history, cap_history, counts2);
   });
 }
 
@@ -269,6 +288,15 @@ int ddgrs_session_trials(ddgrs_session* h, int64_t capacity, double* tau, double
       if (energy) energy[i] = tr[i].energy;
       if (outcome) outcome[i] = tr[i].outcome;
     }
+  });
+}
+
+int ddgrs_session_gmres_history(ddgrs_session* h, double* cycle_residuals, int64_t cap_cycles, double* history,
+                                int64_t cap_history, int64_t* counts2, ddgrs_error* err) {
+  return guarded(err, "session_gmres_history", [&] {
+    need(h, "session");
+    need(counts2, "counts");
+    copy_history(h->ses.last_gmres, cycle_residuals, cap_cycles, history, cap_history, counts2);
   });
 }
 

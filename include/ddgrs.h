@@ -224,6 +224,12 @@ DDGRS_API int ddgrs_saddle_lprime_batch(ddgrs_saddle* h, int64_t nb, const doubl
  * GmresSaddle::solve_many, options gmres.rs:138). */
 DDGRS_API int ddgrs_saddle_solve(ddgrs_saddle* h, const double* b, double* x, int64_t restart, double tolerance,
                                  int64_t max_iterations, ddgrs_gmres_report* report, ddgrs_error* err);
+/* The last solve's stopping measure (relative, explicit) at every cycle start and at the end, and
+ * its Arnoldi estimate after every iteration (RS/src/solve/gmres.rs:214-217
+ * GmresReport::cycle_residuals, ::history): at most cap_cycles / cap_history values are written
+ * (either array may be NULL); counts2 = [cycle residuals, history entries]. */
+DDGRS_API int ddgrs_saddle_gmres_history(ddgrs_saddle* h, double* cycle_residuals, int64_t cap_cycles, double* history,
+                                         int64_t cap_history, int64_t* counts2, ddgrs_error* err);
 
 /* ---- the descent session -------------------------------------------------------------------- */
 
@@ -286,6 +292,11 @@ typedef struct ddgrs_step_report {
    * report), 2 the dense LU did (gmres.relative_residual is its explicit residual). */
   int32_t fallback_stage;
 } ddgrs_step_report;
+
+/* ddgrs_saddle_gmres_history for the last step's direction solve (the last GMRES run of the
+ * recovery chain when it escalated; empty after a dense stage). */
+DDGRS_API int ddgrs_session_gmres_history(ddgrs_session* h, double* cycle_residuals, int64_t cap_cycles,
+                                          double* history, int64_t cap_history, int64_t* counts2, ddgrs_error* err);
 
 /* The escalations of the last step as text (LinearReport::fallbacks, RS/src/solve/mod.rs), one a
  * line, NUL-terminated within `capacity` bytes; *count = the number of escalations. */

@@ -116,6 +116,7 @@ def _declare(L):
     L.orc_precond_apply.argtypes = [_P, _P, _P, _INT]
     L.orc_lprime_solve.argtypes = [_P, _P, _P]
     L.orc_saddle_solve.argtypes = [_P, _P, _P, _I64, _D, _I64, _INT, _P]
+    L.orc_saddle_solve_history.argtypes = [_P, _P, _P, _I64, _D, _I64, _INT, _P, _P, _I64, _P, _I64, _P]
     L.orc_face_operators.argtypes = [_I64, _I64, _P, _P, _P, _P, _P]
     L.orc_metric_blocks.argtypes = [_P] * 10
     L.orc_vfrac_blocks.argtypes = [_P] * 10
@@ -366,10 +367,16 @@ class Saddle:
         rhs = np.ascontiguousarray(rhs, dtype=np.float64)
         x = np.zeros(self.n)
         rep = np.zeros(6)
-        lib().orc_saddle_solve(self._h, _ptr(rhs), _ptr(x), restart, tol, maxit, self.threads, _ptr(rep))
+        cap = int(maxit) + int(maxit) // max(int(restart), 1) + 4
+        cyc, hist, counts = np.zeros(cap), np.zeros(cap), np.zeros(2, dtype=np.int64)
+        lib().orc_saddle_solve_history(self._h, _ptr(rhs), _ptr(x), restart, tol, maxit, self.threads, _ptr(rep),
+                                       _ptr(cyc), cap, _ptr(hist), cap, _ptr(counts))
         keys = ["iterations", "cycles", "converged", "relative_residual", "operator_applications",
                 "preconditioner_applications"]
-        return x, dict(zip(keys, rep.tolist()))
+        out = dict(zip(keys, rep.tolist()))
+        out["cycle_residuals"] = cyc[: min(int(counts[0]), cap)].tolist()
+        out["history"] = hist[: min(int(counts[1]), cap)].tolist()
+        return x, out
 
 
 class Session:

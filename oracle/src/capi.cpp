@@ -314,9 +314,12 @@ void orc_lprime_solve(void* h, const double* r, double* x) {
   }
 }
 
-// report: [iterations, cycles, converged, relative residual, operator applications, preconditioner applications]
-void orc_saddle_solve(void* h, const double* rhs, double* x, int64_t restart, double tol, int64_t maxit, int threads,
-                      double* report) {
+// report: [iterations, cycles, converged, relative residual, operator applications, preconditioner applications];
+// with history outputs: at most cap_c cycle residuals and cap_h Arnoldi estimates (gmres.rs:214-217),
+// counts = [cycle residuals, estimates]
+void orc_saddle_solve_history(void* h, const double* rhs, double* x, int64_t restart, double tol, int64_t maxit,
+                              int threads, double* report, double* cycle_residuals, int64_t cap_c, double* history,
+                              int64_t cap_h, int64_t* counts) {
   auto* sd = static_cast<Saddle*>(h);
   const size_t n = 3 * sd->nv + 3;
   std::vector<double> b(rhs, rhs + n), sol;
@@ -334,6 +337,17 @@ void orc_saddle_solve(void* h, const double* rhs, double* x, int64_t restart, do
   report[3] = rep.relative_residual;
   report[4] = (double)rep.operator_applications;
   report[5] = (double)rep.preconditioner_applications;
+  if (counts) {
+    counts[0] = (int64_t)rep.cycle_residuals.size();
+    counts[1] = (int64_t)rep.history.size();
+    for (int64_t i = 0; cycle_residuals && i < cap_c && i < counts[0]; ++i) cycle_residuals[i] = rep.cycle_residuals[i];
+    for (int64_t i = 0; history && i < cap_h && i < counts[1]; ++i) history[i] = rep.history[i];
+  }
+}
+
+void orc_saddle_solve(void* h, const double* rhs, double* x, int64_t restart, double tol, int64_t maxit, int threads,
+                      double* report) {
+  orc_saddle_solve_history(h, rhs, x, restart, tol, maxit, threads, report, nullptr, 0, nullptr, 0, nullptr);
 }
 
 // opts_f: [θ, χ, tolerance, start value, Armijo c, τ_min rel, c_lo];

@@ -199,6 +199,15 @@ def test_saddle_solve(pkg, ctx, name):
     assert err <= 1e-4
     # the device solution solves the oracle's FP64 system to the metric's FP32 accuracy
     assert rel(od.apply(x), b) <= 1e-4
+    # the cycle residuals and Arnoldi estimates (gmres.rs:214-217) follow the oracle's while the
+    # residual is above the FP32 products' rounding
+    assert len(rep.history) == rep.iterations and len(rep.cycle_residuals) >= rep.cycles + 1
+    assert abs(rep.cycle_residuals[0] - 1.0) <= 1e-12 and rep.cycle_residuals[-1] == rep.relative_residual
+    h0 = rep0["history"]
+    for k in range(min(len(h0), len(rep.history))):
+        if h0[k] < 1e-4:
+            break
+        assert abs(rep.history[k] - h0[k]) <= 1e-2 * h0[k], (k, rep.history[k], h0[k])
 
 
 @pytest.mark.parametrize("name,theta,steps", [("icosphere_l3", 0.5, 3), ("torus_16k", 0.25, 1)])
@@ -223,6 +232,7 @@ def test_step_parity(pkg, ctx, name, theta, steps):
         assert abs(r.energy_trial - r0["energy1"]) <= 1e-5 * abs(r0["energy1"])
         assert abs(r.gmres.iterations - r0["gmres_iterations"]) <= 1
         assert r.gmres.converged and r.gmres.relative_residual <= 1e-6
+        assert len(r.gmres.history) == r.gmres.iterations and r.gmres.cycle_residuals[-1] == r.gmres.relative_residual
         assert r.energy_trial < r.energy_before
         moved = np.abs(x0 - p).max()
         assert np.abs(x - x0).max() <= 1e-4 * moved

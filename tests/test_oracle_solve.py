@@ -125,6 +125,52 @@ def test_gmres_against_dense_solve(small):
     assert np.linalg.norm(A @ x - rhs) <= 1e-9 * np.linalg.norm(rhs)
 
 
+def test_gmres_history_is_the_minimal_residual(small):
+    """GmresReport::history (gmres.rs:217): after k iterations of the first cycle the Arnoldi
+    estimate is min over y in K_k(A P^-1, b) of ||b - A P^-1 y|| / ||b||, computed here by least
+    squares on an orthonormal Krylov basis of the dense A P^-1; cycle_residuals (gmres.rs:214) start
+    at 1, each later one is the explicit residual that the previous cycle's last estimate predicts,
+    and the last is the reported relative residual."""
+    p, f = small
+    sd = oracle.Saddle(f, p, chi=0.5)
+    n = sd.n
+    eye = np.eye(n)
+    A = np.stack([sd.apply(e) for e in eye], axis=1)
+    Pinv = np.stack([sd.precond(e) for e in eye], axis=1)
+    M = A @ Pinv
+    _, d, _ = oracle.exact(f, p)
+    b = np.zeros(n)
+    b[:-3] = d.reshape(-1)
+    _, rep = sd.solve(b, restart=50, tol=1e-8, maxit=50)
+    hist = rep["history"]
+    assert len(hist) == rep["iterations"] >= 5
+    Q = (b / np.linalg.norm(b))[:, None]
+    # P^-1 has the near-null space of L' (eps = 1e-10 tr W / tr M), so the dense A P^-1 carries
+    # rounding of about 1e-10 relative to its norm: compared while the residual is above 1e-6
+    checked = 0
+    for k, est in enumerate(hist[:12], 1):
+        MQ = M @ Q
+        c = np.linalg.lstsq(MQ, b, rcond=None)[0]
+        want = np.linalg.norm(b - MQ @ c) / np.linalg.norm(b)
+        if want < 1e-6:
+            break
+        assert abs(est - want) <= 1e-5 * want, (k, est, want)
+        checked += 1
+        v = M @ Q[:, -1]
+        for _ in range(2):
+            v -= Q @ (Q.T @ v)
+        Q = np.hstack([Q, (v / np.linalg.norm(v))[:, None]])
+    assert checked >= 5
+    # restarted: 3 cycles of 4 iterations
+    _, rep = sd.solve(b, restart=4, tol=1e-12, maxit=12)
+    cyc, hist = rep["cycle_residuals"], rep["history"]
+    assert rep["cycles"] == 3 and len(hist) == 12 and len(cyc) == 4
+    assert cyc[0] == 1.0 and cyc[-1] == rep["relative_residual"]
+    for c in range(1, 4):
+        assert abs(cyc[c] - hist[4 * c - 1]) <= 1e-7 * cyc[c] + 1e-15, (c, cyc[c], hist[4 * c - 1])
+    assert all(h1 <= h0 * (1 + 1e-12) for h0, h1 in zip(hist, hist[1:]))
+
+
 def test_descent_step_lowers_energy_and_keeps_barycenter(small):
     p, f = small
     s = oracle.Session(f, p, theta=0.5)
