@@ -824,7 +824,7 @@ namespace {
 void face_metric_values(ddgrs_ctx* ctx, ddgrs_surface* s, FaceMetricDev& F, const std::vector<double>& centroid,
                         const std::vector<double>& projector, const std::vector<double>& node_mass, double s_exp) {
   BlockTreeDev& bt = F.bt;
-  DevBuf<double> cen, proj, nmass;
+  DevBuf<double>&cen = F.node_cen, &proj = F.node_proj, &nmass = F.node_mass;
   ddgrs::upload(ctx, cen, centroid);
   ddgrs::upload(ctx, proj, projector);
   ddgrs::upload(ctx, nmass, node_mass);
@@ -848,7 +848,6 @@ void face_metric_values(ddgrs_ctx* ctx, ddgrs_surface* s, FaceMetricDev& F, cons
          F.area_p.ptr, F.vert_p.ptr);
   F.m_hl.reserve(24 * (size_t)nf);
   F.z_hl.reserve(24 * (size_t)nf);
-  ctx->sync();  // the local buffers above are released on return
 }
 }  // namespace
 
@@ -1003,7 +1002,7 @@ void vertex_fractional_values(ddgrs_ctx* ctx, VertexFractionalDev& V, const Clus
                               double tau) {
   BlockTreeDev& bt = V.bt;
   const double ex = -(tau + 1.0);
-  DevBuf<double> cen, dpos, nmass;
+  DevBuf<double>&cen = V.node_cen, &dpos = V.pos_dev, &nmass = V.node_mass;
   ddgrs::upload(ctx, cen, cl.centroid);
   ddgrs::upload(ctx, dpos, pos);
   ddgrs::upload(ctx, nmass, cl.mass);
@@ -1018,7 +1017,6 @@ void vertex_fractional_values(ddgrs_ctx* ctx, VertexFractionalDev& V, const Clus
          (const uint32_t*)bt.order.ptr, (const double*)dpos.ptr, (const double*)V.mass.ptr, ex, bt.val.ptr);
   V.u_hl.reserve(6 * (size_t)V.nv);
   V.z_hl.reserve(6 * (size_t)V.nv);
-  ctx->sync();
 }
 }  // namespace
 

@@ -158,6 +158,7 @@ struct FaceMetricDev {
   DevBuf<uint32_t> vert_p;  // 3 per BVH position
   DevBuf<float> m_hl, z_hl; // product scratch: 12 channels, hi/lo
   DevBuf<uint32_t> inc_ptr, inc;  // vertex → (face, corner), with faces' BVH positions
+  DevBuf<double> node_cen, node_proj, node_mass;  // the build's cluster data (far coefficients)
   double w_hi = 1.0, w_lo = 1.0;
   bool extended = false;  // products with the extended near field (BlockTreeDev::multiply)
 
@@ -194,6 +195,7 @@ struct VertexFractionalDev {
   int nv = 0;
   DevBuf<double> mass;      // per vertex (FP64)
   DevBuf<float> u_hl, z_hl; // product scratch: 3 channels, hi/lo
+  DevBuf<double> node_cen, node_mass, pos_dev;  // the build's cluster data and positions
   bool extended = false;    // products with the extended near field (BlockTreeDev::multiply)
   void build(ddgrs_ctx* ctx, const VertexFractionalPlan& p);
   // The same with the structure and panels built on the device; `topo` the vertex BVH of `pos`.

@@ -54,9 +54,11 @@ struct DevBuf {
     n = count;
     if (count) HIPCHECK(hipMalloc(&ptr, count * sizeof(T)));
   }
-  // Grows (never shrinks) the allocation; contents are not kept.
+  // Grows (never shrinks) the allocation, with a quarter of headroom: sizes that creep up from one
+  // step to the next then stop reallocating after a few steps (hipFree waits for the whole device,
+  // which would serialize the step's streams).  Contents are not kept.
   void reserve(size_t count) {
-    if (count > n || !ptr) alloc(count > 0 ? count : 1);
+    if (count > n || !ptr) alloc(count > 0 ? count + count / 4 : 1);
   }
   void release() {
     if (ptr) (void)hipFree(ptr);
