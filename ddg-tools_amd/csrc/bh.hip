@@ -669,10 +669,14 @@ void BhEval::build(ddgrs_surface* s, double theta_, size_t leaf_size_) {
 }
 
 void BhEval::build_from(ddgrs_surface* s, const Topology& t, const Clusters& c, double theta_, size_t leaf_size_) {
-  theta = theta_;
-  leaf_size = leaf_size_;
   topo = t;
   clusters = c;
+  build_from_members(s, theta_, leaf_size_);
+}
+
+void BhEval::build_from_members(ddgrs_surface* s, double theta_, size_t leaf_size_) {
+  theta = theta_;
+  leaf_size = leaf_size_;
   ddgrs_ctx* ctx = s->ctx;
   // DDGRS_TRACE_STEP=1: wall times of the build's parts on stderr (each part synchronized)
   static const bool trace = std::getenv("DDGRS_TRACE_STEP") != nullptr;

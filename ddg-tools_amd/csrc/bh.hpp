@@ -54,6 +54,8 @@ struct BhEval {
   void build(ddgrs_surface* s, double theta, size_t leaf_size);
   // Lists on a face tree of the surface's current positions.
   void build_from(ddgrs_surface* s, const Topology& t, const Clusters& c, double theta, size_t leaf_size);
+  // The same on the tree already in topo and clusters (no copy).
+  void build_from_members(ddgrs_surface* s, double theta, size_t leaf_size);
   // Face packs in BVH order and node aggregates of the surface's current positions.
   void refresh(ddgrs_surface* s);
   // Ẽ into total[0] (device).

@@ -267,7 +267,8 @@ void BhEval::build_lists_device(ddgrs_ctx* ctx) {
     if (next >= UINT32_MAX) throw Error(DDGRS_GPU, "bh_lists", "too many (leaf, task) pairs");
     task_ptr[k + 1] = (uint32_t)next;
   }
-  std::vector<BhNode> nodes(nn);
+  // pinned staging: a copy from pageable memory would wait for the stream
+  BhNode* nodes = (BhNode*)ctx->arena_take((size_t)nn * sizeof(BhNode));
   for (int a = 0; a < nn; ++a) {
     BhNode& r = nodes[a];
     for (int k = 0; k < 3; ++k) {
@@ -279,7 +280,7 @@ void BhEval::build_lists_device(ddgrs_ctx* ctx) {
     r.esc = esc[a];
   }
   d_bhnode.reserve((size_t)nn * sizeof(BhNode));
-  HIPCHECK(hipMemcpyAsync(d_bhnode.ptr, nodes.data(), (size_t)nn * sizeof(BhNode), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHECK(hipMemcpyAsync(d_bhnode.ptr, nodes, (size_t)nn * sizeof(BhNode), hipMemcpyHostToDevice, ctx->stream));
   ddgrs::upload(ctx, d_leaf_ordinal, t.leaf_ordinal);
   ddgrs::upload(ctx, d_parent, t.parent);
   ddgrs::upload(ctx, d_task_root, troot);

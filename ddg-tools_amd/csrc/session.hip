@@ -235,9 +235,11 @@ void Session::step(StepReport* rep) {
     pc.build_fractional(aux, geo.pos, geo.vmass, 2.0 - s_exp, opts.chi, opts.leaf_pairs, std::move(vtopo));
     aux->sync();
   });
-  Topology topo;
-  Clusters clusters;
-  build_face_tree(s, geo, opts.leaf_size, face_bvh, &topo, &clusters);
+  // the face tree is built into the Barnes–Hut evaluator's members, which the step's other users
+  // read (nothing rebuilds it before the next step)
+  build_face_tree(s, geo, opts.leaf_size, face_bvh, &bh.topo, &bh.clusters);
+  const Topology& topo = bh.topo;
+  const Clusters& clusters = bh.clusters;
   mark("face tree");
   // the face metric's block tree on the third stream and another thread (reads the face tree and
   // the surface's geometry, which stay unchanged until the line search)
@@ -251,7 +253,7 @@ void Session::step(StepReport* rep) {
     aux2->sync();
     return seconds_since(t0);
   });
-  if (!opts.energy_exact) bh.build_from(s, topo, clusters, opts.theta, opts.leaf_size);
+  if (!opts.energy_exact) bh.build_from_members(s, opts.theta, opts.leaf_size);
   mark("Barnes-Hut lists");
   rep->seconds_trees = seconds_since(t);
   t = std::chrono::steady_clock::now();
