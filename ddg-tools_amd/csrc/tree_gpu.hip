@@ -485,6 +485,26 @@ __device__ void block_scan2(uint32_t& x, uint32_t& y, uint32_t* wx, uint32_t* wy
   __syncthreads();
 }
 
+// Inclusive sum over a workgroup of W waves; tot gets the workgroup's total.
+template <int W>
+__device__ void block_scan_n(uint32_t& x, uint32_t* wx, uint32_t& tot) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t u = __shfl_up(x, o);
+    if (lane >= o) x += u;
+  }
+  if (lane == 63) wx[wave] = x;
+  __syncthreads();
+  uint32_t b = 0, all = 0;
+  for (int w = 0; w < W; ++w) {
+    if (w < wave) b += wx[w];
+    all += wx[w];
+  }
+  x += b;
+  tot = all;
+  __syncthreads();
+}
+
 // The level's split chunks (⌈cnt / kSplitChunk⌉ per node, in node order): per chunk its node and
 // first element, per node its first chunk (one workgroup).
 __global__ __launch_bounds__(256) void bvh_chunks_lv_kernel(BvhLevel* __restrict__ lv, BvhRecords R,
@@ -561,9 +581,237 @@ __global__ void bvh_root_kernel(uint32_t n, BvhLevel* __restrict__ lv, BvhRecord
   R.prefix[0] = 0;
 }
 
+// ---- subtrees finished in one workgroup ---------------------------------------------------------
+// Below a cut depth every node has at most kSubMax elements; one workgroup builds each such node's
+// whole subtree in LDS.  The elements get local ids (their rank in the node's axis-0 order) and the
+// workgroup keeps, per axis, the node-ordered element sequence in that axis's (key, index) order.
+// Level by level, each wave splits one node at a time: the node's elements in its axis order are
+// its range of that axis's sequence, the split is the first minimum of the same costs as
+// split_node's (chunk unions, then per-chunk prefix and suffix scans; unions are exact), and every
+// axis sequence is then stably partitioned into the children's ranges (ballot ranks), which keeps
+// each child's elements in every axis order.  A child of at most leaf_size elements writes its
+// elements to the final order in its parent's axis order, as the level-by-level build leaves them.
+// The splits go out in level order per subtree; the host numbers the nodes as before.
+constexpr uint32_t kSubMax = 2048;
+constexpr uint32_t kSubNodes = kSubMax / 2;  // nodes of one local level (each holds 2 or more elements)
+
+struct SubtreeArgs {
+  const BvhLevel* lv;     // the cut level's records
+  BvhRecords R;
+  const uint32_t* lists;  // 3 n: per axis, each cut node's elements in that axis's order
+  uint32_t* inv;          // n: element → local id (scratch)
+  const double *lo, *hi;
+  uint32_t* order;        // n: the leaves' elements are written here
+  uint32_t* sub_split;    // n: per cut node, from its first position, its subtree's splits in level order
+  uint32_t* sub_count;    // per cut node: splits written, UINT32_MAX when it has more than kSubMax elements
+  uint32_t n, leaf, axis0;
+};
+
+namespace {
+__device__ __forceinline__ Box6 shfl_down6(const Box6& b, int o) {
+  return {__shfl_down(b.l0, o), __shfl_down(b.l1, o), __shfl_down(b.l2, o),
+          __shfl_down(b.h0, o), __shfl_down(b.h1, o), __shfl_down(b.h2, o)};
+}
+
+// lane `src` (wave-uniform) of every component, by scalar reads of the lane
+__device__ __forceinline__ double readlane_d(double x, int src) {
+  const uint64_t u = (uint64_t)__double_as_longlong(x);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, src);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), src);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+__device__ __forceinline__ Box6 shfl6(const Box6& b, int src) {
+  return {readlane_d(b.l0, src), readlane_d(b.l1, src), readlane_d(b.l2, src),
+          readlane_d(b.h0, src), readlane_d(b.h1, src), readlane_d(b.h2, src)};
+}
+
+// inclusive unions over lanes 0..lane (fwd) or lane..63 (bwd)
+__device__ __forceinline__ Box6 wave_union_fwd(Box6 v, int lane) {
+  for (int o = 1; o < 64; o <<= 1) {
+    const Box6 t = shfl_up6(v, o);
+    if (lane >= o) v = unite(t, v);
+  }
+  return v;
+}
+
+__device__ __forceinline__ Box6 wave_union_bwd(Box6 v, int lane) {
+  for (int o = 1; o < 64; o <<= 1) {
+    const Box6 t = shfl_down6(v, o);
+    if (lane + o < 64) v = unite(v, t);
+  }
+  return v;
+}
+
+// The split of the node at local positions [s, e) along the axis whose sequence is L (one wave).
+__device__ uint32_t wave_split(uint32_t s, uint32_t e, const uint16_t* __restrict__ L, const uint32_t* __restrict__ gid,
+                               const double* __restrict__ lo, const double* __restrict__ hi, int lane) {
+  const uint32_t cnt = e - s, nch = (cnt + 63) / 64;
+  size_t low = (size_t)ceil(0.25 * (double)cnt);
+  if (low < 1) low = 1;
+  if (low > cnt / 2) low = cnt / 2;
+  // lane c: the unions of the chunks before c and after c (none for a node of one chunk)
+  Box6 pre = empty6(), post = empty6();
+  if (nch > 1) {
+    Box6 mine = empty6();
+    for (uint32_t c = 0; c < nch; ++c) {
+      const uint32_t p = s + c * 64 + lane;
+      Box6 v = p < e ? load_box(lo, hi, gid[L[p]]) : empty6();
+      v = shfl6(wave_union_fwd(v, lane), 63);
+      if ((uint32_t)lane == c) mine = v;
+    }
+    pre = shfl_up6(wave_union_fwd(mine, lane), 1);
+    if (lane == 0) pre = empty6();
+    post = shfl_down6(wave_union_bwd(mine, lane), 1);
+    if (lane == 63) post = empty6();
+  }
+  double best = INFINITY;
+  uint32_t best_k = UINT32_MAX;
+  for (uint32_t c = 0; c < nch; ++c) {
+    const Box6 pre_c = shfl6(pre, c), post_c = shfl6(post, c);
+    const uint32_t p = s + c * 64 + lane;
+    const Box6 v = p < e ? load_box(lo, hi, gid[L[p]]) : empty6();
+    const Box6 left = unite(pre_c, wave_union_fwd(v, lane));  // positions s..p
+    Box6 rest = shfl_down6(wave_union_bwd(v, lane), 1);        // positions p+1.. to the chunk's end
+    if (lane == 63) rest = empty6();
+    const Box6 right = unite(rest, post_c);
+    const size_t k = (size_t)(p - s) + 1;
+    if (p < e && k >= low && k <= cnt - low) {
+      const double cost = diam2(left) + diam2(right);
+      if (cost < best) {
+        best = cost;
+        best_k = (uint32_t)k;
+      }
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const double c2 = __shfl_xor(best, o);
+    const uint32_t k2 = __shfl_xor(best_k, o);
+    if (k2 != UINT32_MAX && (best_k == UINT32_MAX || c2 < best || (c2 == best && k2 < best_k))) {
+      best = c2;
+      best_k = k2;
+    }
+  }
+  return best_k == UINT32_MAX ? cnt / 2 : best_k;
+}
+}  // namespace
+
+constexpr uint32_t kSubThreads = 512;
+
+__global__ __launch_bounds__(512) void bvh_subtree_kernel(SubtreeArgs A) {
+  __shared__ uint32_t gid[kSubMax];
+  __shared__ uint16_t seq[2][3][kSubMax];  // [level parity][axis][position] → local id
+  __shared__ uint8_t flag[kSubMax];        // local id → in the left child
+  __shared__ uint32_t nodes[2][kSubNodes]; // per local level: s | e << 16
+  __shared__ uint16_t ksplit[kSubNodes];
+  __shared__ uint32_t count[2];
+  __shared__ uint32_t wx[kSubThreads / 64], wy[kSubThreads / 64];
+  const uint32_t g = blockIdx.x, t = threadIdx.x;
+  const int lane = (int)(t & 63), wave = (int)(t >> 6);
+  if (g >= A.lv->na) return;  // uniform per workgroup
+  const uint32_t a = A.lv->off + g, s0 = A.R.start[a], m = A.R.end[a] - s0;
+  if (m > kSubMax) {
+    if (t == 0) A.sub_count[g] = UINT32_MAX;
+    return;
+  }
+  for (uint32_t i = t; i < m; i += kSubThreads) {
+    const uint32_t x = A.lists[s0 + i];
+    gid[i] = x;
+    A.inv[x] = i;
+    seq[0][0][i] = (uint16_t)i;
+  }
+  __threadfence_block();
+  __syncthreads();
+  for (int b = 1; b < 3; ++b)
+    for (uint32_t i = t; i < m; i += kSubThreads) seq[0][b][i] = (uint16_t)A.inv[A.lists[(size_t)b * A.n + s0 + i]];
+  if (t == 0) {
+    nodes[0][0] = m << 16;
+    count[0] = 1;
+  }
+  __syncthreads();
+  uint32_t nsplit = 0;
+  for (uint32_t d = 0;; ++d) {
+    const uint32_t par = d & 1, nc = count[par];
+    if (nc == 0) break;
+    const uint32_t ax = (A.axis0 + d) % 3;
+    for (uint32_t i = (uint32_t)wave; i < nc; i += kSubThreads / 64) {
+      const uint32_t se = nodes[par][i], s = se & 0xffffu, e = se >> 16;
+      const uint16_t* src = seq[par][ax];
+      const uint32_t k = wave_split(s, e, src, gid, A.lo, A.hi, lane), mid = s + k;
+      if (lane == 0) ksplit[i] = (uint16_t)k;
+      for (uint32_t p = s + lane; p < e; p += 64) flag[src[p]] = p < mid ? 1 : 0;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const unsigned long long below = (1ull << lane) - 1ull;
+      for (int b = 0; b < 3; ++b) {
+        const uint16_t* S = seq[par][b];
+        uint16_t* D = seq[par ^ 1][b];
+        uint32_t nl = 0, nr = 0;
+        for (uint32_t c0 = s; c0 < e; c0 += 64) {
+          const uint32_t p = c0 + lane;
+          const bool valid = p < e;
+          const uint16_t x = valid ? S[p] : 0;
+          const bool f = valid && flag[x];
+          const bool r = valid && !flag[x];
+          const unsigned long long bl = __ballot(f), br = __ballot(r);
+          if (f) D[s + nl + __popcll(bl & below)] = x;
+          if (r) D[mid + nr + __popcll(br & below)] = x;
+          nl += __popcll(bl);
+          nr += __popcll(br);
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      // children of at most leaf_size elements are leaves: their elements in this axis's order
+      const uint16_t* dst = seq[par ^ 1][ax];
+      if (mid - s <= A.leaf)
+        for (uint32_t p = s + lane; p < mid; p += 64) A.order[s0 + p] = gid[dst[p]];
+      if (e - mid <= A.leaf)
+        for (uint32_t p = mid + lane; p < e; p += 64) A.order[s0 + p] = gid[dst[p]];
+    }
+    __syncthreads();
+    // the splits in level order, and the next level: the children that split, left before right
+    for (uint32_t i = t; i < nc; i += kSubThreads) A.sub_split[s0 + nsplit + i] = ksplit[i];
+    nsplit += nc;
+    uint32_t base = 0;
+    for (uint32_t i0 = 0; i0 < nc; i0 += kSubThreads) {
+      const uint32_t i = i0 + t;
+      uint32_t s = 0, e = 0, mid = 0;
+      bool la = false, ra = false;
+      if (i < nc) {
+        const uint32_t se = nodes[par][i];
+        s = se & 0xffffu;
+        e = se >> 16;
+        mid = s + ksplit[i];
+        la = mid - s > A.leaf;
+        ra = e - mid > A.leaf;
+      }
+      const uint32_t cn = (uint32_t)la + (uint32_t)ra;
+      uint32_t x = cn, y = 0, tx, ty;
+      block_scan_n<kSubThreads / 64>(x, wx, tx);
+      (void)y;
+      (void)wy;
+      (void)ty;
+      uint32_t q = base + x - cn;
+      if (la) nodes[par ^ 1][q++] = s | (mid << 16);
+      if (ra) nodes[par ^ 1][q] = mid | (e << 16);
+      base += tx;
+    }
+    if (t == 0) count[par ^ 1] = base;
+    __syncthreads();
+  }
+  if (t == 0) A.sub_count[g] = nsplit;
+}
+
 Topology GpuBvhBuilder::build_levels_on_device(ddgrs_ctx* ctx, size_t n, const double* lo, const double* hi,
-                                               size_t leaf_size) {
+                                               size_t leaf_size, bool allow_cut) {
   static const bool trace = std::getenv("DDGRS_TRACE_STEP") != nullptr;
+  // DDGRS_BVH_SUBTREES=0: every level through the level loop.  DDGRS_BVH_CUT_SIZE: the cut goes to
+  // the first level whose largest node had at most this many elements in the previous build.
+  const char* sub_env = std::getenv("DDGRS_BVH_SUBTREES");
+  const char* cut_env = std::getenv("DDGRS_BVH_CUT_SIZE");
+  const bool subtrees = !(sub_env && sub_env[0] == '0');
+  const double cut_size = cut_env ? std::atof(cut_env) : 0.5 * kSubMax;
   const auto t_build = std::chrono::steady_clock::now();
   const uint32_t un = (uint32_t)n, leaf = (uint32_t)leaf_size;
   rec.reserve(4 * (n + 1));
@@ -576,70 +824,146 @@ Topology GpuBvhBuilder::build_levels_on_device(ddgrs_ctx* ctx, size_t n, const d
   ch_box.reserve(18 * (size_t)(un / kSplitChunk + max_nodes + 1));
   ch_best.reserve(2 * (size_t)(un / kSplitChunk + max_nodes + 1));
   std::vector<BvhLevel> lv_host;
-  std::vector<uint32_t> splits;
+  std::vector<uint32_t> splits, sub_count_h, sub_split_h;
   int levels = 0;
+  // the cut: the first level whose largest node the previous build found small enough for one
+  // workgroup's subtree build
+  int cut = -1;
+  if (allow_cut && subtrees) {
+    for (size_t L = 0; L < prev_levels.size(); ++L)
+      if ((double)prev_levels[L] <= cut_size) {
+        cut = (int)L;
+        break;
+      }
+    // a cut build records the levels down to its cut only: below them the largest node shrinks by
+    // at least a quarter a level (the split rule keeps a quarter on each side)
+    if (cut < 0 && !prev_levels.empty()) {
+      double est = prev_levels.back();
+      int L = (int)prev_levels.size() - 1;
+      while (est > cut_size && L < (int)kMaxBvhLevels - 1) {
+        est *= 0.75;
+        ++L;
+      }
+      if (est <= cut_size) cut = L;
+    }
+  }
   if (n > leaf_size) {
     launch(ctx, "bvh_root", bvh_root_kernel, dim3(1), dim3(1), 0, un, lv_dev.ptr, R);
     uint32_t* pin = (uint32_t*)ctx->staging(sizeof(BvhLevel));
-    int target = prev_levels.empty() ? 16 : (int)prev_levels.size();
-    for (;;) {
-      for (; levels < target; ++levels) {
-        const int L = levels;
-        const BvhLevel* lv = lv_dev.ptr + L;
-        // upper bounds of this level's nodes and of its largest node
-        const uint64_t na_bound = std::min<uint64_t>(L < 32 ? (1ull << L) : max_nodes, max_nodes);
-        const double size_bound = L < (int)prev_levels.size() ? 2.0 * prev_levels[L] : (double)n * std::pow(0.75, L);
-        launch(ctx, "bvh_mark", bvh_mark_all_kernel, dim3(blocks_for(un, 256)), dim3(256), 0, un, lv, R,
-               (const uint32_t*)order.ptr, seg_of.ptr);
+    // one level: its nodes' elements in the level's axis order, the splits, the next level's nodes
+    auto run_level = [&](int L) {
+      const BvhLevel* lv = lv_dev.ptr + L;
+      // upper bounds of this level's nodes and of its largest node
+      const uint64_t na_bound = std::min<uint64_t>(L < 32 ? (1ull << L) : max_nodes, max_nodes);
+      const double size_bound = L < (int)prev_levels.size() ? 2.0 * prev_levels[L] : (double)n * std::pow(0.75, L);
+      launch(ctx, "bvh_mark", bvh_mark_all_kernel, dim3(blocks_for(un, 256)), dim3(256), 0, un, lv, R,
+             (const uint32_t*)order.ptr, seg_of.ptr);
+      launch(ctx, "bvh_part_keys", bvh_part_keys_kernel, dim3(blocks_for(un, 256)), dim3(256), 0, un,
+             (const uint32_t*)(axis_order.ptr + (size_t)(L % 3) * n), (const uint32_t*)seg_of.ptr, part_key_in.ptr,
+             part_val_in.ptr);
+      int bits = 1;
+      while ((1ull << bits) - 1 < na_bound) ++bits;  // the sentinel truncates to 2^bits − 1 ≥ na
+      size_t tb = temp.n;
+      HIPCHECK(rocprim::radix_sort_pairs(temp.ptr, tb, part_key_in.ptr, part_key_out.ptr, part_val_in.ptr,
+                                         part_val_out.ptr, n, 0, bits, ctx->stream));
+      launch(ctx, "bvh_scatter", bvh_scatter_lv_kernel, dim3(blocks_for(un, 256)), dim3(256), 0, lv, R,
+             (const uint32_t*)part_key_out.ptr, (const uint32_t*)part_val_out.ptr, order.ptr);
+      if (size_bound <= (double)kSplitChunk) {
+        launch(ctx, "bvh_split", bvh_split_lv_kernel, dim3((unsigned)na_bound), dim3(256), 0, lv, R,
+               (const uint32_t*)order.ptr, lo, hi, suf.ptr);
+      } else {
+        const unsigned nch_bound = (unsigned)std::min<uint64_t>(un / kSplitChunk + na_bound, un / kSplitChunk + max_nodes);
+        Box6* ctot = (Box6*)ch_box.ptr;
+        Box6* cpre = ctot + (un / kSplitChunk + max_nodes + 1);
+        Box6* csuf = cpre + (un / kSplitChunk + max_nodes + 1);
+        launch(ctx, "bvh_chunks", bvh_chunks_lv_kernel, dim3(1), dim3(256), 0, lv_dev.ptr + L, R, ch_seg.ptr,
+               ch_c0.ptr, seg_chunk0.ptr);
+        launch(ctx, "bvh_chunk_total", bvh_chunk_total_kernel, dim3(nch_bound), dim3(256), 0, lv,
+               (const uint32_t*)ch_seg.ptr, (const uint32_t*)ch_c0.ptr, (const uint32_t*)R.start,
+               (const uint32_t*)R.end, (const uint32_t*)order.ptr, lo, hi, ctot);
+        launch(ctx, "bvh_chunk_scan", bvh_chunk_scan_kernel, dim3(blocks_for(nch_bound, 256)), dim3(256), 0, lv, 0u,
+               (const uint32_t*)ch_seg.ptr, (const uint32_t*)seg_chunk0.ptr, (const Box6*)ctot, cpre, csuf);
+        launch(ctx, "bvh_chunk_suffix", bvh_chunk_suffix_kernel, dim3(nch_bound), dim3(256), 0, lv,
+               (const uint32_t*)ch_seg.ptr, (const uint32_t*)ch_c0.ptr, (const uint32_t*)R.start,
+               (const uint32_t*)R.end, (const uint32_t*)order.ptr, lo, hi, (const Box6*)csuf, suf.ptr);
+        launch(ctx, "bvh_chunk_cost", bvh_chunk_cost_kernel, dim3(nch_bound), dim3(256), 0, lv,
+               (const uint32_t*)ch_seg.ptr, (const uint32_t*)ch_c0.ptr, (const uint32_t*)R.start,
+               (const uint32_t*)R.end, (const uint32_t*)order.ptr, lo, hi, (const Box6*)cpre, (const double*)suf.ptr,
+               (SplitBest*)ch_best.ptr);
+        launch(ctx, "bvh_chunk_pick", bvh_chunk_pick_kernel, dim3(blocks_for(na_bound, 256)), dim3(256), 0, lv, 0u,
+               (const uint32_t*)seg_chunk0.ptr, (const uint32_t*)R.start, (const uint32_t*)R.end,
+               (const SplitBest*)ch_best.ptr, R.split);
+      }
+      launch(ctx, "bvh_next", bvh_next_kernel, dim3(1), dim3(256), 0, lv_dev.ptr + L, R, leaf);
+    };
+    if (cut >= 0) {
+      for (; levels < cut; ++levels) run_level(levels);
+      // the cut level: its nodes' elements in each axis's order, then one workgroup per node
+      const BvhLevel* lv = lv_dev.ptr + cut;
+      const uint64_t na_bound = std::min<uint64_t>(cut < 32 ? (1ull << cut) : max_nodes, max_nodes);
+      int bits = 1;
+      while ((1ull << bits) - 1 < na_bound) ++bits;
+      cut_lists.reserve(3 * n);
+      sub_split.reserve(n);
+      sub_count.reserve(na_bound);
+      launch(ctx, "bvh_mark", bvh_mark_all_kernel, dim3(blocks_for(un, 256)), dim3(256), 0, un, lv, R,
+             (const uint32_t*)order.ptr, seg_of.ptr);
+      for (int b = 0; b < 3; ++b) {
         launch(ctx, "bvh_part_keys", bvh_part_keys_kernel, dim3(blocks_for(un, 256)), dim3(256), 0, un,
-               (const uint32_t*)(axis_order.ptr + (size_t)(L % 3) * n), (const uint32_t*)seg_of.ptr, part_key_in.ptr,
+               (const uint32_t*)(axis_order.ptr + (size_t)b * n), (const uint32_t*)seg_of.ptr, part_key_in.ptr,
                part_val_in.ptr);
-        int bits = 1;
-        while ((1ull << bits) - 1 < na_bound) ++bits;  // the sentinel truncates to 2^bits − 1 ≥ na
         size_t tb = temp.n;
         HIPCHECK(rocprim::radix_sort_pairs(temp.ptr, tb, part_key_in.ptr, part_key_out.ptr, part_val_in.ptr,
                                            part_val_out.ptr, n, 0, bits, ctx->stream));
         launch(ctx, "bvh_scatter", bvh_scatter_lv_kernel, dim3(blocks_for(un, 256)), dim3(256), 0, lv, R,
-               (const uint32_t*)part_key_out.ptr, (const uint32_t*)part_val_out.ptr, order.ptr);
-        if (size_bound <= (double)kSplitChunk) {
-          launch(ctx, "bvh_split", bvh_split_lv_kernel, dim3((unsigned)na_bound), dim3(256), 0, lv, R,
-                 (const uint32_t*)order.ptr, lo, hi, suf.ptr);
-        } else {
-          const unsigned nch_bound = (unsigned)std::min<uint64_t>(un / kSplitChunk + na_bound, un / kSplitChunk + max_nodes);
-          Box6* ctot = (Box6*)ch_box.ptr;
-          Box6* cpre = ctot + (un / kSplitChunk + max_nodes + 1);
-          Box6* csuf = cpre + (un / kSplitChunk + max_nodes + 1);
-          launch(ctx, "bvh_chunks", bvh_chunks_lv_kernel, dim3(1), dim3(256), 0, lv_dev.ptr + L, R, ch_seg.ptr,
-                 ch_c0.ptr, seg_chunk0.ptr);
-          launch(ctx, "bvh_chunk_total", bvh_chunk_total_kernel, dim3(nch_bound), dim3(256), 0, lv,
-                 (const uint32_t*)ch_seg.ptr, (const uint32_t*)ch_c0.ptr, (const uint32_t*)R.start,
-                 (const uint32_t*)R.end, (const uint32_t*)order.ptr, lo, hi, ctot);
-          launch(ctx, "bvh_chunk_scan", bvh_chunk_scan_kernel, dim3(blocks_for(nch_bound, 256)), dim3(256), 0, lv, 0u,
-                 (const uint32_t*)ch_seg.ptr, (const uint32_t*)seg_chunk0.ptr, (const Box6*)ctot, cpre, csuf);
-          launch(ctx, "bvh_chunk_suffix", bvh_chunk_suffix_kernel, dim3(nch_bound), dim3(256), 0, lv,
-                 (const uint32_t*)ch_seg.ptr, (const uint32_t*)ch_c0.ptr, (const uint32_t*)R.start,
-                 (const uint32_t*)R.end, (const uint32_t*)order.ptr, lo, hi, (const Box6*)csuf, suf.ptr);
-          launch(ctx, "bvh_chunk_cost", bvh_chunk_cost_kernel, dim3(nch_bound), dim3(256), 0, lv,
-                 (const uint32_t*)ch_seg.ptr, (const uint32_t*)ch_c0.ptr, (const uint32_t*)R.start,
-                 (const uint32_t*)R.end, (const uint32_t*)order.ptr, lo, hi, (const Box6*)cpre, (const double*)suf.ptr,
-                 (SplitBest*)ch_best.ptr);
-          launch(ctx, "bvh_chunk_pick", bvh_chunk_pick_kernel, dim3(blocks_for(na_bound, 256)), dim3(256), 0, lv, 0u,
-                 (const uint32_t*)seg_chunk0.ptr, (const uint32_t*)R.start, (const uint32_t*)R.end,
-                 (const SplitBest*)ch_best.ptr, R.split);
-        }
-        launch(ctx, "bvh_next", bvh_next_kernel, dim3(1), dim3(256), 0, lv_dev.ptr + L, R, leaf);
+               (const uint32_t*)part_key_out.ptr, (const uint32_t*)part_val_out.ptr, cut_lists.ptr + (size_t)b * n);
       }
+      // seg_of is free again: the subtrees' element → local id map
+      const SubtreeArgs sa{lv,       R,      (const uint32_t*)cut_lists.ptr, seg_of.ptr, lo,
+                           hi,       order.ptr, sub_split.ptr, sub_count.ptr, un,
+                           leaf,     (uint32_t)(cut % 3)};
+      launch(ctx, "bvh_subtree", bvh_subtree_kernel, dim3((unsigned)na_bound), dim3(kSubThreads), 0, sa);
+      levels = cut;
+      lv_host.resize(cut + 1);
+      download(ctx, lv_host.data(), (const BvhLevel*)lv_dev.ptr, lv_host.size());
+      sub_count_h.resize(na_bound);
+      download(ctx, sub_count_h.data(), (const uint32_t*)sub_count.ptr, sub_count_h.size());
+      ctx->sync();
+      uint32_t too_large = UINT32_MAX;
+      for (uint32_t g = 0; g < lv_host[cut].na && too_large == UINT32_MAX; ++g)
+        if (sub_count_h[g] == UINT32_MAX) too_large = g;
+      if (too_large == UINT32_MAX) {
+        const size_t rec_bound = std::min<uint64_t>(cut < 32 ? (1ull << cut) : n + 1, n + 1);  // records above the cut
+        splits.resize(rec_bound);
+        download(ctx, splits.data(), (const uint32_t*)R.split, splits.size());
+        sub_split_h.resize(n);
+        download(ctx, sub_split_h.data(), (const uint32_t*)sub_split.ptr, n);
+      } else {
+        // a node too large for one workgroup: the level loop goes on from the cut level (the cut
+        // level's records stand, and every cut node's range of `order` still holds its elements)
+        if (trace)
+          std::fprintf(stderr, "bvh: %zu elements, cut at level %d: node %u of %u too large\n", n, cut, too_large,
+                       lv_host[cut].na);
+        cut = -1;
+      }
+    }
+    int target = prev_levels.empty() ? 16 : (int)prev_levels.size();
+    for (;;) {
+      if (cut >= 0) break;
+      for (; levels < target; ++levels) run_level(levels);
       download(ctx, (BvhLevel*)pin, (const BvhLevel*)lv_dev.ptr + levels, 1);
       ctx->sync();
       if (((BvhLevel*)pin)->na == 0) break;
       if (levels >= (int)kMaxBvhLevels) throw Error(DDGRS_GPU, "BVH build", "more levels than the device build allows");
       target = std::min<int>(levels + 4, (int)kMaxBvhLevels);
     }
-    lv_host.resize(levels + 1);
-    download(ctx, lv_host.data(), (const BvhLevel*)lv_dev.ptr, lv_host.size());
-    ctx->sync();
-    splits.resize(lv_host[levels].off);
-    download(ctx, splits.data(), (const uint32_t*)R.split, splits.size());
+    if (cut < 0) {
+      lv_host.resize(levels + 1);
+      download(ctx, lv_host.data(), (const BvhLevel*)lv_dev.ptr, lv_host.size());
+      ctx->sync();
+      splits.resize(lv_host[levels].off);
+      download(ctx, splits.data(), (const uint32_t*)R.split, splits.size());
+    }
   }
   std::vector<uint32_t> ord(n);
   download(ctx, ord.data(), (const uint32_t*)order.ptr, n);
@@ -651,6 +975,38 @@ Topology GpuBvhBuilder::build_levels_on_device(ddgrs_ctx* ctx, size_t n, const d
   for (int L = 0; !level.empty(); ++L) {
     std::vector<size_t> next;
     uint32_t i = 0, biggest = 0;
+    if (L == cut) {
+      // each cut node's subtree from its splits in level order, left child before right
+      for (size_t si : level) {
+        if (segs[si].end - segs[si].start <= leaf_size) continue;
+        if (i >= lv_host[L].na) throw Error(DDGRS_GPU, "BVH build", "device levels disagree with the nodes");
+        biggest = std::max(biggest, segs[si].end - segs[si].start);
+        const uint32_t s0 = segs[si].start, cnt = sub_count_h[i++];
+        uint32_t j = 0;
+        std::vector<size_t> q{si}, nq;
+        while (!q.empty()) {
+          nq.clear();
+          for (size_t u : q) {
+            if (j >= cnt || s0 + j >= sub_split_h.size())
+              throw Error(DDGRS_GPU, "BVH build", "a subtree's splits disagree with its nodes");
+            const uint32_t mid = segs[u].start + sub_split_h[s0 + j++];
+            TreeSegment l{segs[u].start, mid, segs[u].depth + 1};
+            TreeSegment r{mid, segs[u].end, segs[u].depth + 1};
+            segs[u].left = (int64_t)segs.size();
+            segs.push_back(l);
+            segs[u].right = (int64_t)segs.size();
+            segs.push_back(r);
+            if (l.end - l.start > leaf_size) nq.push_back((size_t)segs[u].left);
+            if (r.end - r.start > leaf_size) nq.push_back((size_t)segs[u].right);
+          }
+          std::swap(q, nq);
+        }
+        if (j != cnt) throw Error(DDGRS_GPU, "BVH build", "a subtree's splits disagree with its nodes");
+      }
+      if (i != lv_host[L].na) throw Error(DDGRS_GPU, "BVH build", "device levels disagree with the nodes");
+      if (i > 0) level_max.push_back(biggest);
+      break;
+    }
     for (size_t si : level) {
       if (segs[si].end - segs[si].start <= leaf_size) continue;
       if (L >= levels || i >= lv_host[L].na) throw Error(DDGRS_GPU, "BVH build", "device levels disagree with the nodes");
@@ -672,7 +1028,8 @@ Topology GpuBvhBuilder::build_levels_on_device(ddgrs_ctx* ctx, size_t n, const d
   }
   prev_levels = std::move(level_max);
   if (trace)
-    std::fprintf(stderr, "bvh: %zu elements, %d levels on the device, %.2f ms total\n", n, levels,
+    std::fprintf(stderr, "bvh: %zu elements, %d levels on the device%s, %.2f ms total\n", n, levels,
+                 cut >= 0 ? " then the subtrees" : "",
                  std::chrono::duration<double>(std::chrono::steady_clock::now() - t_build).count() * 1e3);
   return finish_topology(segs, std::move(ord));
 }

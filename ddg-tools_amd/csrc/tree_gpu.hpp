@@ -35,6 +35,8 @@ struct GpuBvhBuilder {
   // per-level counts, and the largest node of each level of the previous build (sizes the launches)
   static constexpr uint32_t kMaxBvhLevels = 96;
   DevBuf<uint32_t> rec;
+  // the cut: per axis the cut nodes' elements in axis order, per cut node its splits and their count
+  DevBuf<uint32_t> cut_lists, sub_split, sub_count;
   DevBuf<struct BvhLevel> lv_dev;
   std::vector<uint32_t> prev_levels;
   // Builds over n elements with centres pts and boxes [lo, hi] (3 FP64 each, device pointers).  The
@@ -42,7 +44,11 @@ struct GpuBvhBuilder {
   // depth is reached, the splits once at the end); DDGRS_BVH_LEVELS_ON_HOST=1 reads each level's
   // splits back and keeps the node list on the host (the same tree).
   Topology build(ddgrs_ctx* ctx, size_t n, const double* pts, const double* lo, const double* hi, size_t leaf_size);
-  Topology build_levels_on_device(ddgrs_ctx* ctx, size_t n, const double* lo, const double* hi, size_t leaf_size);
+  // Levels above a cut depth as below, the subtrees of the cut's nodes one workgroup each (from the
+  // second build on, the cut placed from the previous build's node sizes; with allow_cut false every
+  // level runs, and a cut node too large for one workgroup makes the level loop go on from the cut).
+  Topology build_levels_on_device(ddgrs_ctx* ctx, size_t n, const double* lo, const double* hi, size_t leaf_size,
+                                  bool allow_cut = true);
 };
 
 }  // namespace ddgrs

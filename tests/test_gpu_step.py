@@ -293,6 +293,30 @@ def test_device_levels_equal_host_levels(pkg, ctx, monkeypatch, switch):
     assert np.array_equal(out["device"], out["host"])
 
 
+@pytest.mark.parametrize("name,cut_size", [("torus_16k", None), ("icosphere_l3", None), ("bumpy_l6", None),
+                                           ("torus_16k", "1e9")])
+def test_bvh_subtrees_equal_level_loop(pkg, ctx, monkeypatch, name, cut_size):
+    """From the second build on, the BVH's nodes below a cut depth are built one subtree per
+    workgroup (the whole tree at L3); the steps are bitwise those of BVHs built level by level.  A
+    cut placed above nodes too large for one workgroup (cut size 1e9: the root) falls back to the
+    level loop."""
+    p, f = meshes(pkg)[name]() if name != "bumpy_l6" else pkg.generate.config_bumpy_l6()
+    out = {}
+    for mode in ("subtrees", "levels"):
+        if mode == "levels":
+            monkeypatch.setenv("DDGRS_BVH_SUBTREES", "0")
+        elif cut_size:
+            monkeypatch.setenv("DDGRS_BVH_CUT_SIZE", cut_size)
+        s = pkg.Surface(ctx, f, p)
+        ses = pkg.Session(s)
+        for _ in range(3):
+            ses.step()
+        out[mode] = s.positions()
+        ses.close()
+        s.close()
+    assert np.array_equal(out["subtrees"], out["levels"])
+
+
 def test_long_trajectory_converges(pkg, ctx):
     """Fourteen steps at L6: once the gradient is small and smooth, the FP32 products' rounding
     keeps the explicit residual just above 1e-6 (the Arnoldi estimate reaches it); the solve then
