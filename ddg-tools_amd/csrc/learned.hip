@@ -64,12 +64,22 @@ __global__ __launch_bounds__(256) void ln_rows_kernel(int n, const float* __rest
   }
 }
 
-// The same product for J = 64 on the matrix cores (FP32 MFMA 16×16×4, FP32 accumulation): a
+// The same product for J = 64 on the matrix cores, BF16 inputs with FP32 accumulation (MFMA
+// 16×16×16 bf16): X and M are rounded to BF16 (nearest, ties to even) as they are staged in LDS; a
 // workgroup takes 64 rows in two passes of 32; per pass wave w owns the 16-row group w & 1 and the
-// two 16-column tiles 2 (w >> 1) and 2 (w >> 1) + 1.  M (K × 64, K = I rounded up to 4, zero rows
-// appended) and the pass's 32 rows of X (row stride K + 1) are staged in LDS.  Lane l supplies
-// A(l % 16, k₀ + l / 16) and B(k₀ + l / 16, 16 t + l % 16) and holds D(4 (l / 16) + v, l % 16).
+// two 16-column tiles 2 (w >> 1) and 2 (w >> 1) + 1.  M (K × 64, K = I rounded up to 16, zero rows
+// appended) and the pass's 32 rows of X (row stride K + 4) are staged in LDS.  Lane l supplies
+// A(l % 16, k₀ + 4 (l / 16) + q) and B(k₀ + 4 (l / 16) + q, 16 t + l % 16), q < 4, and holds
+// D(4 (l / 16) + v, l % 16).
 using f32x4 = __attribute__((__vector_size__(4 * sizeof(float)))) float;
+using bf16x4 = __attribute__((__vector_size__(4 * sizeof(short)))) short;
+
+// FP32 → BF16, round to nearest, ties to even (finite inputs)
+__device__ __forceinline__ uint16_t to_bf16(float x) {
+  uint32_t u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
 
 template <int I>
 __global__ __launch_bounds__(256) void ln_rows_mfma_kernel(int n, const float* __restrict__ X1, int ld1, int I1,
@@ -77,13 +87,13 @@ __global__ __launch_bounds__(256) void ln_rows_mfma_kernel(int n, const float* _
                                                            const float* __restrict__ M, int msi, int msj,
                                                            const float* __restrict__ bias, int relu,
                                                            float* __restrict__ pre, float* __restrict__ Y, int ldy) {
-  constexpr int K = (I + 3) / 4 * 4, J = 64;
-  __shared__ float sm[K * J];
-  __shared__ float sx[32][K + 1];
+  constexpr int K = (I + 15) / 16 * 16, J = 64, SX = K + 4;
+  __shared__ uint16_t sm[K * J];
+  __shared__ uint16_t sx[32 * SX];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   for (int e = tid; e < K * J; e += 256) {
     const int i = e / J, j = e % J;
-    sm[e] = i < I ? M[(size_t)i * msi + (size_t)j * msj] : 0.f;
+    sm[e] = i < I ? to_bf16(M[(size_t)i * msi + (size_t)j * msj]) : (uint16_t)0;
   }
   const int rg = w & 1, c0 = 32 * (w >> 1), li = lane & 15, lk = lane >> 4;
   for (int p = 0; p < 2; ++p) {
@@ -93,17 +103,22 @@ __global__ __launch_bounds__(256) void ln_rows_mfma_kernel(int n, const float* _
       const int rr = e / K, i = e % K, v = v0 + rr;
       float x = 0.f;
       if (v < n && i < I) x = i < I1 ? X1[(size_t)v * ld1 + i] : X2[(size_t)v * ld2 + (i - I1)];
-      sx[rr][i] = x;
+      sx[rr * SX + i] = to_bf16(x);
     }
     __syncthreads();
     f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll 4
-    for (int k0 = 0; k0 < K; k0 += 4) {
-      const float a = sx[16 * rg + li][k0 + lk];
-      const float b0 = sm[(k0 + lk) * J + c0 + li];
-      const float b1 = sm[(k0 + lk) * J + c0 + 16 + li];
-      acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b0, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b1, acc1, 0, 0, 0);
+#pragma unroll 2
+    for (int k0 = 0; k0 < K; k0 += 16) {
+      bf16x4 a, b0, b1;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int k = k0 + 4 * lk + q;
+        a[q] = (short)sx[(16 * rg + li) * SX + k];
+        b0[q] = (short)sm[k * J + c0 + li];
+        b1[q] = (short)sm[k * J + c0 + 16 + li];
+      }
+      acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b0, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b1, acc1, 0, 0, 0);
     }
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
@@ -168,18 +183,19 @@ __global__ __launch_bounds__(256) void ln_wgrad_kernel(int n, int rows_per_block
   }
 }
 
-// The same partial gradients for J = 64 on the matrix cores (FP32 MFMA 16×16×4): the output
-// (I + 1) × 64, rows padded to 16·NT, is NT × 4 tiles; wave w owns column tile w and all NT row
-// tiles.  Per step of 4 rows of the chunk, lane l supplies Aᵀ(16 t + l % 16, k₀ + l / 16) =
-// A(k₀ + l / 16, 16 t + l % 16) and B(k₀ + l / 16, 16 w + l % 16).
+// The same partial gradients for J = 64 on the matrix cores, BF16 inputs with FP32 accumulation
+// (MFMA 16×16×16 bf16; A and B rounded to BF16 as they are staged): the output (I + 1) × 64, rows
+// padded to 16·NT, is NT × 4 tiles; wave w owns column tile w and all NT row tiles.  Per step of 16
+// rows of the chunk, lane l supplies Aᵀ(16 t + l % 16, k₀ + 4 (l / 16) + q) =
+// A(k₀ + 4 (l / 16) + q, 16 t + l % 16) and B(k₀ + 4 (l / 16) + q, 16 w + l % 16), q < 4.
 template <int I>
 __global__ __launch_bounds__(256) void ln_wgrad_mfma_kernel(int n, int rows_per_block, const float* __restrict__ A1,
                                                             int lda1, int I1, const float* __restrict__ A2, int lda2,
                                                             const float* __restrict__ B, int ldb,
                                                             float* __restrict__ part) {
   constexpr int J = 64, NT = (I + 16) / 16, SA = 16 * NT, NO = (I + 1) * J;
-  __shared__ float sa[32][SA];
-  __shared__ float sb[32][J];
+  __shared__ uint16_t sa[32][SA + 2];
+  __shared__ uint16_t sb[32][J + 2];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, li = lane & 15, lk = lane >> 4;
   f32x4 acc[NT];
 #pragma unroll
@@ -191,18 +207,25 @@ __global__ __launch_bounds__(256) void ln_wgrad_mfma_kernel(int n, int rows_per_
       const int rr = e / SA, i = e % SA, v = c0 + rr;
       float x = 0.f;
       if (v < r1 && i <= I) x = i == I ? 1.f : (i < I1 ? A1[(size_t)v * lda1 + i] : A2[(size_t)v * lda2 + (i - I1)]);
-      sa[rr][i] = x;
+      sa[rr][i] = to_bf16(x);
     }
     for (int e = tid; e < 32 * J; e += 256) {
       const int rr = e / J, j = e % J, v = c0 + rr;
-      sb[rr][j] = v < r1 ? B[(size_t)v * ldb + j] : 0.f;
+      sb[rr][j] = to_bf16(v < r1 ? B[(size_t)v * ldb + j] : 0.f);
     }
     __syncthreads();
-#pragma unroll 2
-    for (int k0 = 0; k0 < 32; k0 += 4) {
-      const float b = sb[k0 + lk][16 * w + li];
 #pragma unroll
-      for (int t = 0; t < NT; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(sa[k0 + lk][16 * t + li], b, acc[t], 0, 0, 0);
+    for (int k0 = 0; k0 < 32; k0 += 16) {
+      bf16x4 b;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) b[q] = (short)sb[k0 + 4 * lk + q][16 * w + li];
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        bf16x4 a;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) a[q] = (short)sa[k0 + 4 * lk + q][16 * t + li];
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+      }
     }
   }
 #pragma unroll
@@ -375,10 +398,10 @@ template <int I, int J>
 void rows(ddgrs_ctx* ctx, int n, const float* X1, int ld1, int I1, const float* X2, int ld2, const float* M, int msi,
           int msj, const float* bias, int relu, float* pre, float* Y, int ldy) {
   if constexpr (J == 64)
-    launch(ctx, "ln_rows", ln_rows_mfma_kernel<I>, dim3(blocks_for(n, 64)), dim3(256), 0, n, X1, ld1, I1, X2, ld2, M,
+    launch(ctx, "ln_rows_mfma", ln_rows_mfma_kernel<I>, dim3(blocks_for(n, 64)), dim3(256), 0, n, X1, ld1, I1, X2, ld2, M,
            msi, msj, bias, relu, pre, Y, ldy);
   else
-    launch(ctx, "ln_rows", ln_rows_kernel<I, J>, dim3(blocks_for(n, 64)), dim3(256), 0, n, X1, ld1, I1, X2, ld2, M, msi,
+    launch(ctx, "ln_rows_valu", ln_rows_kernel<I, J>, dim3(blocks_for(n, 64)), dim3(256), 0, n, X1, ld1, I1, X2, ld2, M, msi,
            msj, bias, relu, pre, Y, ldy);
 }
 
@@ -389,10 +412,10 @@ void wgrad(ddgrs_ctx* ctx, int n, const float* A1, int lda1, int I1, const float
            float* part, float* gw, float* gb) {
   const int per = (n + kGradBlocks - 1) / kGradBlocks;
   if constexpr (J == 64)
-    launch(ctx, "ln_wgrad", ln_wgrad_mfma_kernel<I>, dim3(kGradBlocks), dim3(256), 0, n, per, A1, lda1, I1, A2, lda2, B,
+    launch(ctx, "ln_wgrad_mfma", ln_wgrad_mfma_kernel<I>, dim3(kGradBlocks), dim3(256), 0, n, per, A1, lda1, I1, A2, lda2, B,
            ldb, part);
   else
-    launch(ctx, "ln_wgrad", ln_wgrad_kernel<I, J>, dim3(kGradBlocks), dim3(256), 0, n, per, A1, lda1, I1, A2, lda2, B,
+    launch(ctx, "ln_wgrad_valu", ln_wgrad_kernel<I, J>, dim3(kGradBlocks), dim3(256), 0, n, per, A1, lda1, I1, A2, lda2, B,
            ldb, part);
   launch(ctx, "ln_wgrad_sum", ln_wgrad_sum_kernel, dim3(blocks_for((I + 1) * J, 64)), dim3(256), 0, kGradBlocks, I, J,
          (const float*)part, gw, gb);

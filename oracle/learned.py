@@ -12,12 +12,19 @@ library trains on the device, so that its features, forward pass, loss and gradi
 * network: h0 = relu(F W0 + b0); four layers h <- relu([h, mean_N(h)] W_l + b_l) of width 64;
   o = h Wo + bo with o_v = (B_v row-major, g_v row-major);
 * (Q x)_v = s (B_v x_v + sum_{u in N(v)} (g_v + g_u)/2 x_u), loss |s q - e|^2 / |e|^2 for a stored
-  pair (x, e) (the library stores x = c P^-1 r and e = y - x).
+  pair (x, e) (the library stores x = c P^-1 r and e = y - x);
+* ``bf16=True`` rounds both inputs of the products that the library runs on BF16 matrix cores
+  (every product with 64 output columns: the input and message layers, the back-propagated
+  activations through every layer, the weight and bias gradients of the 64-wide layers) to BF16,
+  nearest with ties to even, and accumulates exactly; the 18-output head and its weight gradient
+  stay unrounded (FP32 on the device).
 
 Parity pins (``tests/test_oracle_learned.py``): the analytic gradient against central differences
 of the loss (every parameter group), the features against closed forms on a sphere (radial unit
 normals, equal areas on a regular mesh, the Laplacian's sign), Q's linearity and symmetry
-(Q is symmetric when B_v is, because the edge blocks are shared).
+(Q is symmetric when B_v is, because the edge blocks are shared); the BF16 rounding against hand
+values (ties to even, carries into the exponent), and the BF16 products against the unrounded ones
+when every input is already a BF16 value.
 
 Only tests may import this module.
 """
@@ -26,6 +33,17 @@ from __future__ import annotations
 import numpy as np
 
 FEAT, WIDTH, LAYERS, OUT = 8, 64, 4, 18
+
+
+def bf16(x):
+    """x rounded to FP32, then to BF16 (the upper 16 bits; nearest, ties to even), as float64."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def _mm(a, b, rounded):
+    return bf16(a) @ bf16(b) if rounded else a @ b
 
 
 def layout():
@@ -130,15 +148,15 @@ def _mean_t(dm, ptr, nb):
     return d
 
 
-def forward(p, f, ptr, nb, keep=False):
+def forward(p, f, ptr, nb, keep=False, bf16=False):
     P = unpack(p)
-    pre0 = f @ P["w0"] + P["b0"]
+    pre0 = _mm(f, P["w0"], bf16) + P["b0"]
     h = np.maximum(pre0, 0.0)
     cache = [(f, None, pre0)]
     for l in range(LAYERS):
         m = _mean(h, ptr, nb)
         x = np.concatenate([h, m], axis=1)
-        pre = x @ P[f"w{l + 1}"] + P[f"b{l + 1}"]
+        pre = _mm(x, P[f"w{l + 1}"], bf16) + P[f"b{l + 1}"]
         cache.append((x, m, pre))
         h = np.maximum(pre, 0.0)
     o = h @ P["wo"] + P["bo"]
@@ -158,13 +176,14 @@ def q_apply(o, r3, ptr, nb):
     return q
 
 
-def loss_grad(p, f, ptr, nb, r, e, s):
+def loss_grad(p, f, ptr, nb, r, e, s, bf16=False):
     """Loss |s q - e|^2 / |e|^2 and its gradient (flat, the parameter layout) by backpropagation."""
     nv = f.shape[0]
     r3 = np.asarray(r, dtype=np.float64)[:3 * nv].reshape(nv, 3)
     e3 = np.asarray(e, dtype=np.float64)[:3 * nv].reshape(nv, 3)
     e2 = (e3 ** 2).sum()
-    o, h4, cache = forward(p, f, ptr, nb, keep=True)
+    o, h4, cache = forward(p, f, ptr, nb, keep=True, bf16=bf16)
+    rb = globals()["bf16"] if bf16 else (lambda a: a)
     q = s * q_apply(o, r3, ptr, nb)
     d = q - e3
     L = (d ** 2).sum() / e2
@@ -178,18 +197,18 @@ def loss_grad(p, f, ptr, nb, r, e, s):
     dout[:, 9:] = 0.5 * s * gsum.reshape(nv, 9)
     P = unpack(p)
     grads = {"wo": h4.T @ dout, "bo": dout.sum(axis=0)}
-    dh = dout @ P["wo"].T
+    dh = _mm(dout, P["wo"].T, bf16)
     for l in range(LAYERS, 0, -1):
         x, _, pre = cache[l]
         dpre = dh * (pre > 0)
-        grads[f"w{l}"] = x.T @ dpre
-        grads[f"b{l}"] = dpre.sum(axis=0)
+        grads[f"w{l}"] = _mm(x.T, dpre, bf16)
+        grads[f"b{l}"] = rb(dpre).sum(axis=0)
         W = P[f"w{l}"]
-        dh = dpre @ W[:WIDTH].T + _mean_t(dpre @ W[WIDTH:].T, ptr, nb)
+        dh = _mm(dpre, W[:WIDTH].T, bf16) + _mean_t(_mm(dpre, W[WIDTH:].T, bf16), ptr, nb)
     _, _, pre0 = cache[0]
     dpre = dh * (pre0 > 0)
-    grads["w0"] = f.T @ dpre
-    grads["b0"] = dpre.sum(axis=0)
+    grads["w0"] = _mm(f.T, dpre, bf16)
+    grads["b0"] = rb(dpre).sum(axis=0)
     lay = layout()
     g = np.zeros(lay["total"])
     for k, v in lay.items():

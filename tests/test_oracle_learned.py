@@ -83,3 +83,31 @@ def test_q_is_linear_and_symmetric_with_symmetric_vertex_blocks():
     f = LN.features(pos, faces)
     r, e = rng.standard_normal(3 * nv + 3), rng.standard_normal(3 * nv + 3)
     assert abs(LN.loss_grad(p, f, ptr, nb, r, e, 1.0)[0] - 1.0) < 1e-15
+
+
+def test_bf16_rounding_hand_values():
+    """BF16 keeps 7 fraction bits: half an ulp ties to the even neighbour, above it rounds up, and
+    a tie below a power of two carries into the exponent."""
+    u = 2.0 ** -7
+    x = np.array([1.0, 1 + u / 2, 1 + u + u / 2, 1 + u / 2 + 2.0 ** -20, 2 - u / 2, -(1 + u + u / 2), 0.0, 3.0])
+    want = np.array([1.0, 1.0, 1 + 2 * u, 1 + u, 2.0, -(1 + 2 * u), 0.0, 3.0])
+    assert np.array_equal(LN.bf16(x), want)
+
+
+def test_bf16_products_on_bf16_inputs_are_exact():
+    """Products whose inputs are BF16 values already are the unrounded products; the BF16 loss and
+    gradient stay within BF16 rounding of the unrounded ones."""
+    rng = np.random.default_rng(3)
+    a, b = LN.bf16(rng.standard_normal((40, 16))), LN.bf16(rng.standard_normal((16, 64)))
+    assert np.array_equal(LN._mm(a, b, True), a @ b)
+    pos, faces = _mesh(1)
+    ptr, nb = LN.neighbours(faces, pos.shape[0])
+    f = LN.features(pos, faces)
+    p = _random_params(2)
+    n = 3 * pos.shape[0] + 3
+    r, e = rng.standard_normal(n), rng.standard_normal(n)
+    L0, g0 = LN.loss_grad(p, f, ptr, nb, r, e, 0.7)
+    L1, g1 = LN.loss_grad(p, f, ptr, nb, r, e, 0.7, bf16=True)
+    assert abs(L1 - L0) <= 3e-2 * abs(L0)
+    assert np.linalg.norm(g1 - g0) <= 5e-2 * np.linalg.norm(g0)
+    assert not np.array_equal(g1, g0)
