@@ -126,6 +126,8 @@ def _declare(L):
     L.orc_session_step.argtypes = [_P, _INT, _P, _P, _I64]
     L.orc_session_step.restype = _INT
     L.orc_session_positions.argtypes = [_P, _P]
+    L.orc_session_constraints.argtypes = [_P, _P, _I64]
+    L.orc_constraint_rows.argtypes = [_I64, _I64, _P, _P, _P, _I64, _P, _P]
     L.orc_fractional_symbol_ratio.argtypes = [_D]
     L.orc_fractional_symbol_ratio.restype = _D
 
@@ -387,20 +389,32 @@ class Session:
     ``start`` is the line search's first step (step/options.rs ``StartRule``, session.rs:1604):
     ``("grow", lengths)`` = GrowPrevious{2, MaxDisplacement{lengths}} (the default),
     ``("abs", tau)``, ``("maxdisp", lengths)`` or ``("reference",)`` (2/‖x‖).
-    ``max_halvings`` < 0: no cap."""
+    ``max_halvings`` < 0: no cap.
+
+    ``constraints``: nonlinear rows after the barycenter's, at most two of ``("area", target)`` and
+    ``("volume", target)`` (target None: the value at creation), enforced through the Schur
+    complement (solve/schur.rs) and the projection's Newton rounds: ``newton`` = ``("tol", n)``
+    (UntilTolerance{max: n}) or ``("fixed", n)``."""
 
     START_KINDS = {"grow": 0, "abs": 1, "maxdisp": 2, "reference": 3}
+    ROW_KINDS = {"area": 1, "volume": 2}
 
     def __init__(self, faces, positions, alpha=6.0, beta=12.0, theta=0.5, chi=0.5, restart=50, tol=1e-6, maxit=200,
                  energy_exact=False, start=("grow", 1.0), armijo_c=1e-2, tau_min_rel=1e-12, max_halvings=-1,
-                 c_lo=1.0, leaf_size=8):
+                 c_lo=1.0, leaf_size=8, constraints=(), newton=("tol", 5)):
         f, p = _mesh(faces, positions)
         self.nv = p.shape[0]
         st = _INT(0)
         kind = self.START_KINDS[start[0]]
         value = float(start[1]) if len(start) > 1 else 0.0
-        of = np.array([theta, chi, tol, value, armijo_c, tau_min_rel, c_lo], dtype=np.float64)
-        oi = np.array([restart, maxit, 1 if energy_exact else 0, kind, max_halvings, leaf_size], dtype=np.int64)
+        if len(constraints) > 2:
+            raise ValueError("at most two nonlinear rows")
+        kinds = [self.ROW_KINDS[c[0]] for c in constraints] + [0] * (2 - len(constraints))
+        targets = [np.nan if c[1] is None else float(c[1]) for c in constraints] + [np.nan] * (2 - len(constraints))
+        self.n_rows = 3 + len(constraints)
+        of = np.array([theta, chi, tol, value, armijo_c, tau_min_rel, c_lo] + targets, dtype=np.float64)
+        oi = np.array([restart, maxit, 1 if energy_exact else 0, kind, max_halvings, leaf_size] + kinds +
+                      [1 if newton[0] == "fixed" else 0, int(newton[1])], dtype=np.int64)
         self._h = lib().orc_session_create(self.nv, f.shape[0], _ptr(f), _ptr(p), alpha, beta, _ptr(of), _ptr(oi),
                                            ctypes.byref(st))
         if not self._h:
@@ -428,12 +442,28 @@ class Session:
         out["seconds"] = dict(zip(["trees_energy", "solver_build", "gmres", "line_search", "projection"],
                                   rep[16:21].tolist()))
         out["trial_list"] = [(float(t), float(e), int(o)) for t, e, o in tr[: min(cap, int(out["trials"]))]]
+        c = np.zeros(3 + 2 * self.n_rows)
+        lib().orc_session_constraints(self._h, _ptr(c), c.size)
+        out["newton_rounds"] = int(c[1])
+        out["min_pivot"] = float(c[2])
+        out["phi_before"] = c[3:3 + self.n_rows].tolist()
+        out["phi"] = c[3 + self.n_rows:].tolist()
         return out
 
     def positions(self):
         out = np.zeros((self.nv, 3))
         lib().orc_session_positions(self._h, _ptr(out))
         return out
+
+
+def constraint_rows(faces, positions, kinds):
+    """(values, gradients (k × 3 nv)) of the rows ``kinds`` ("area", "volume") at the positions
+    (constraints/mod.rs:462-647)."""
+    f, p = _mesh(faces, positions)
+    k = np.array([Session.ROW_KINDS[x] for x in kinds], dtype=np.int64)
+    vals, grads = np.zeros(len(k)), np.zeros((len(k), 3 * p.shape[0]))
+    lib().orc_constraint_rows(p.shape[0], f.shape[0], _ptr(f), _ptr(p), _ptr(k), len(k), _ptr(vals), _ptr(grads))
+    return vals, grads
 
 
 def fractional_symbol_ratio(s):

@@ -350,8 +350,11 @@ void orc_saddle_solve(void* h, const double* rhs, double* x, int64_t restart, do
   orc_saddle_solve_history(h, rhs, x, restart, tol, maxit, threads, report, nullptr, 0, nullptr, 0, nullptr);
 }
 
-// opts_f: [θ, χ, tolerance, start value, Armijo c, τ_min rel, c_lo];
-// opts_i: [restart, max iterations, exact energy, start kind, max halvings (< 0: none), leaf size]
+// opts_f: [θ, χ, tolerance, start value, Armijo c, τ_min rel, c_lo, target of row 1, of row 2
+// (NaN: the value at creation)];
+// opts_i: [restart, max iterations, exact energy, start kind, max halvings (< 0: none), leaf size,
+// row 1 and row 2 after the barycenter (0 none, 1 area, 2 volume), Newton (0: until tolerance,
+// 1: fixed), Newton rounds]
 void* orc_session_create(int64_t nv, int64_t nf, const uint32_t* faces, const double* pos, double alpha, double beta,
                          const double* opts_f, const int64_t* opts_i, int* status) {
   Error err;
@@ -374,6 +377,13 @@ void* orc_session_create(int64_t nv, int64_t nf, const uint32_t* faces, const do
   o.start_kind = (int)opts_i[3];
   o.max_halvings = (long)opts_i[4];
   o.saddle.leaf_size = (size_t)opts_i[5];
+  for (int r = 0; r < 2; ++r)
+    if (opts_i[6 + r] == ROW_AREA || opts_i[6 + r] == ROW_VOLUME) {
+      o.rows.push_back((int)opts_i[6 + r]);
+      o.targets.push_back(opts_f[7 + r]);
+    }
+  o.newton_fixed = opts_i[8] != 0;
+  o.newton_count = (int)opts_i[9];
   auto* s = new Session();
   if (!session_create(make_mesh(nv, nf, faces), make_pos(nv, pos), e, o, s, &err)) {
     delete s;
@@ -418,6 +428,28 @@ int orc_session_step(void* h, int threads, double* report, double* trials, int64
     trials[3 * t + 2] = (double)rep.trial[t].outcome;
   }
   return OK;
+}
+
+// out: [rows (3 + nonlinear), Newton rounds, smallest kept Schur pivot, Φ before the projection
+// (rows), Φ after it (rows)] of the last step; cap doubles at most
+void orc_session_constraints(void* h, double* out, int64_t cap) {
+  auto* s = static_cast<Session*>(h);
+  std::vector<double> v{(double)s->last_phi.size(), (double)s->last_newton_rounds, s->last_min_pivot};
+  v.insert(v.end(), s->last_phi_before.begin(), s->last_phi_before.end());
+  v.insert(v.end(), s->last_phi.begin(), s->last_phi.end());
+  for (size_t i = 0; i < v.size() && (int64_t)i < cap; ++i) out[i] = v[i];
+}
+
+// Values (rows of kind 1 area, 2 volume) and their gradients (3 nv each) at positions.
+void orc_constraint_rows(int64_t nv, int64_t nf, const uint32_t* faces, const double* pos, const int64_t* kinds,
+                         int64_t k, double* values, double* grads) {
+  const Mesh m = make_mesh(nv, nf, faces);
+  const Positions p = make_pos(nv, pos);
+  for (int64_t r = 0; r < k; ++r) {
+    values[r] = row_value(m, p, (int)kinds[r]);
+    const auto g = row_gradient(m, p, (int)kinds[r]);
+    std::copy(g.begin(), g.end(), grads + r * 3 * nv);
+  }
 }
 
 void orc_session_positions(void* h, double* out) {

@@ -282,6 +282,32 @@ GmresReport gmres_right(const std::function<void(const double*, double*)>& op,
 
 // The line search's first step (step/options.rs StartRule; session.rs:1604 start_tau).
 enum StartKind { START_GROW_MAXDISP = 0, START_ABSOLUTE = 1, START_MAXDISP = 2, START_REFERENCE = 3 };
+// Nonlinear constraint rows (constraints/mod.rs RowKind::Area, ::Volume) of one component.
+enum { ROW_AREA = 1, ROW_VOLUME = 2 };
+double total_area(const Mesh& m, const Positions& p);
+double signed_volume(const Mesh& m, const Positions& p);
+std::vector<double> area_gradient(const Mesh& m, const Positions& p);    // 3 nv
+std::vector<double> volume_gradient(const Mesh& m, const Positions& p);  // 3 nv
+double row_value(const Mesh& m, const Positions& p, int kind);
+std::vector<double> row_gradient(const Mesh& m, const Positions& p, int kind);
+double row_tolerance(double target);  // 1e-8 |target|, or 1e-8 for a zero target
+
+// The Schur complement of the nonlinear rows (schur.rs SchurComplement): C_N (k dense rows of
+// 3 nv), Z = A′⁻¹ C_Nᵀ (position blocks), S = −C_N Z, the equilibration and the pivoted Cholesky
+// factor of the kept rows.
+struct Schur {
+  std::vector<std::vector<double>> c, z;
+  std::vector<double> s, d, l;
+  std::vector<size_t> kept;  // pivot order
+  double min_pivot = 1.0;
+};
+void schur_build(Schur* S, std::vector<std::vector<double>> c, std::vector<std::vector<double>> z);
+std::vector<double> schur_solve_n(const Schur& S, const std::vector<double>& v);
+std::vector<double> schur_combine(const Schur& S, const std::vector<double>& c);
+std::vector<double> schur_apply_c(const Schur& S, const double* v);
+std::vector<double> schur_direction(const Schur& S, const std::vector<double>& y);   // y − Z N⁻¹ C_N y
+std::vector<double> schur_correction(const Schur& S, const std::vector<double>& phi);  // Z N⁻¹ Φ_N
+
 struct SessionOptions {
   bool energy_exact = false;  // EnergyEval::Exact instead of Barnes–Hut
   double theta = 0.5;
@@ -292,6 +318,13 @@ struct SessionOptions {
   double armijo_c = 1e-2;     // ArmijoRule::Standard{c}
   double tau_min_rel = 1e-12;
   long max_halvings = -1;     // < 0: no cap
+  // nonlinear rows after the barycenter's (ROW_AREA, ROW_VOLUME), their targets (NaN: the value
+  // at creation) and the projection's Newton rounds (schur.rs NewtonSteps: Fixed(n) or
+  // UntilTolerance{max: n})
+  std::vector<int> rows;
+  std::vector<double> targets;
+  bool newton_fixed = false;
+  int newton_count = 5;
 };
 struct Session {
   Mesh mesh;
@@ -300,6 +333,11 @@ struct Session {
   SessionOptions opts;
   double l0 = 0.0, last_tau = 0.0;
   V3 barycenter_target{0, 0, 0};
+  std::vector<double> row_targets;  // of the nonlinear rows
+  // the last step's constraint rows (as in StepReport)
+  std::vector<double> last_phi_before, last_phi;
+  int last_newton_rounds = 0;
+  double last_min_pivot = 1.0;
 };
 // TrialOutcome (session.rs): 0 accepted, 1 Armijo failed, 2 non-finite.
 struct Trial {
@@ -315,6 +353,11 @@ struct StepReport {
   GmresReport gmres;
   // wall seconds of the phases: trees and F, dF; the solver's build; GMRES; line search; projection
   double seconds[5] = {0, 0, 0, 0, 0};
+  // the constraint rows: Φ (barycenter x, y, z, then the nonlinear rows) at the moved positions
+  // before the projection and after it, the Newton rounds kept, the smallest kept Schur pivot
+  std::vector<double> phi_before, phi;
+  int newton_rounds = 0;
+  double min_pivot = 1.0;
 };
 bool session_create(const Mesh& m, const Positions& p, const Exponents& e, const SessionOptions& o, Session* s,
                     Error* err);

@@ -508,22 +508,42 @@ bool session_step(Session& s, int threads, StepReport* rep, Error* err) {
   for (size_t v = 0; v < nv; ++v)
     for (int k = 0; k < 3; ++k) g[3 * v + k] = gv[v][k];
   lap(0);
-  // 2. the solver and the direction x = A′⁻¹ [g; 0] (session.rs:820-860; k = 0 Schur columns)
+  // 2. the solver and the direction (session.rs:820-860): with nonlinear rows, the Schur columns
+  // z_r = A′⁻¹ [C_N row r; 0] and y = A′⁻¹ [g; 0] (schur.rs:101 with_direction: one batch, the
+  // same results as separate solves), then x = y − Z N⁻¹ C_N y; without them x = y
   Saddle sd;
   if (!saddle_build(m, s.positions, s.exps, s.opts.saddle, threads, &sd, err)) return false;
   lap(1);
-  std::vector<double> rhs(nd + 3, 0.0), sol;
-  std::copy(g.begin(), g.end(), rhs.begin());
   auto op = [&](const double* x, double* y) { saddle_apply(sd, x, y, threads); };
   auto pc = [&](const double* r, double* z) { precond_apply(sd, r, z, threads); };
-  rep->gmres = gmres_right(op, pc, rhs, s.opts.gmres, &sol);
-  lap(2);
-  if (!rep->gmres.converged && !(rep->gmres.relative_residual <= 1e-1)) {
-    err->status = SOLVER_DIVERGED;
-    err->detail = "GMRES diverged";
-    return false;
+  auto solve = [&](const std::vector<double>& b, std::vector<double>* out, GmresReport* gr) {
+    std::vector<double> rhs(nd + 3, 0.0), sol;
+    std::copy(b.begin(), b.end(), rhs.begin());
+    *gr = gmres_right(op, pc, rhs, s.opts.gmres, &sol);
+    if (!gr->converged && !(gr->relative_residual <= 1e-1)) {
+      err->status = SOLVER_DIVERGED;
+      err->detail = "GMRES diverged";
+      return false;
+    }
+    out->assign(sol.begin(), sol.begin() + nd);
+    return true;
+  };
+  const size_t kn = s.opts.rows.size();
+  Schur schur;
+  std::vector<std::vector<double>> crow(kn), zcol(kn);
+  for (size_t r = 0; r < kn; ++r) {
+    crow[r] = row_gradient(m, s.positions, s.opts.rows[r]);
+    GmresReport gr;
+    if (!solve(crow[r], &zcol[r], &gr)) return false;
   }
-  std::vector<double> x(sol.begin(), sol.begin() + nd);
+  std::vector<double> x;
+  if (!solve(g, &x, &rep->gmres)) return false;
+  if (kn) {
+    schur_build(&schur, std::move(crow), std::move(zcol));
+    x = schur_direction(schur, x);
+  }
+  rep->min_pivot = schur.min_pivot;
+  lap(2);
   double slope = 0.0, xinf = 0.0, xn2 = 0.0, maxd = 0.0, gn2 = 0.0;
   for (size_t i = 0; i < nd; ++i) {
     slope += g[i] * x[i];
@@ -568,14 +588,67 @@ bool session_step(Session& s, int threads, StepReport* rep, Error* err) {
   }
   if (!rep->accepted) rep->energy1 = f0;
   lap(3);
-  // 4. projection (constraints/mod.rs project_positional): translate by −Φ, Φ the barycenter's
-  // offset from its target (areas at the moved positions); then F at the projected positions
+  // 4. projection (schur.rs:495 project): the positional projection (constraints/mod.rs:657:
+  // translate by −Φ, Φ the barycenter's offset from its target, areas at the moved positions),
+  // then Newton rounds f ← f − Z N⁻¹ Φ_N, each followed by the positional projection and kept only
+  // if it lowers max_r |Φ_r| / tol_r over the kept rows; then F at the projected positions
+  auto values = [&](const Positions& q) {
+    std::vector<double> phi(3 + kn);
+    const V3 b = barycenter_of(m, q);
+    for (int k = 0; k < 3; ++k) phi[k] = b[k] - s.barycenter_target[k];
+    for (size_t r = 0; r < kn; ++r) phi[3 + r] = row_value(m, q, s.opts.rows[r]) - s.row_targets[r];
+    return phi;
+  };
+  auto project_positional = [&](Positions& q) {
+    const V3 b = barycenter_of(m, q);
+    for (size_t v = 0; v < nv; ++v)
+      for (int k = 0; k < 3; ++k) q[v][k] -= b[k] - s.barycenter_target[k];
+  };
   Positions np = rep->accepted ? trial : s.positions;
-  const V3 bc = barycenter_of(m, np);
-  for (size_t v = 0; v < nv; ++v)
-    for (int k = 0; k < 3; ++k) np[v][k] -= bc[k] - s.barycenter_target[k];
+  rep->phi_before = values(np);
+  project_positional(np);
+  std::vector<double> phi = values(np);
+  rep->newton_rounds = 0;
+  if (kn && !schur.kept.empty()) {
+    std::vector<size_t> kept = schur.kept;
+    std::sort(kept.begin(), kept.end());
+    auto measure = [&](const std::vector<double>& ph) {
+      double mm = 0.0;
+      for (size_t i : kept) {
+        const double v = std::fabs(ph[3 + i]) / row_tolerance(s.row_targets[i]);
+        if (std::isnan(v) || std::isnan(mm))
+          mm = NAN;
+        else
+          mm = std::max(mm, v);
+      }
+      return mm;
+    };
+    double mcur = measure(phi);
+    const bool until = !s.opts.newton_fixed;
+    while (rep->newton_rounds < s.opts.newton_count) {
+      if (until && mcur <= 1.0) break;
+      const std::vector<double> phin(phi.begin() + 3, phi.end());
+      const std::vector<double> h = schur_correction(schur, phin);
+      Positions t2 = np;
+      for (size_t v = 0; v < nv; ++v)
+        for (int k = 0; k < 3; ++k) t2[v][k] -= h[3 * v + k];
+      project_positional(t2);
+      const std::vector<double> pt = values(t2);
+      const double mt = measure(pt);
+      if (!(mt < mcur)) break;
+      np = t2;
+      phi = pt;
+      mcur = mt;
+      rep->newton_rounds++;
+    }
+  }
+  rep->phi = phi;
   rep->energy_after = value_at(np);
   lap(4);
+  s.last_phi_before = rep->phi_before;
+  s.last_phi = rep->phi;
+  s.last_newton_rounds = rep->newton_rounds;
+  s.last_min_pivot = rep->min_pivot;
   s.positions = np;
   if (rep->accepted) s.last_tau = tau;
   rep->tau = rep->accepted ? tau : 0.0;
@@ -591,6 +664,11 @@ bool session_create(const Mesh& m, const Positions& p, const Exponents& e, const
   s->opts = o;
   s->l0 = mean_edge_length(m, p);
   s->barycenter_target = barycenter_of(m, p);
+  s->row_targets.clear();
+  for (size_t r = 0; r < o.rows.size(); ++r) {
+    const double t = r < o.targets.size() ? o.targets[r] : NAN;
+    s->row_targets.push_back(std::isnan(t) ? row_value(m, p, o.rows[r]) : t);
+  }
   return true;
 }
 

@@ -102,6 +102,8 @@ def read_out(name: str) -> dict:
             o["mesh"] = [int(x) for x in t[1:]]
         elif k == "rows":
             o["nrows"] = [int(x) for x in t[1:]]
+        elif k == "row":
+            o.setdefault("crows", []).append((int(t[1]), t[2], t[3], hexf(t[4])))
     return o
 
 
