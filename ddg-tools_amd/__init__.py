@@ -160,6 +160,7 @@ SIGNATURES = {
     "ddgrs_session_trials": (_I32, [_P, _I64, _P, _P, _P, ctypes.POINTER(_I64), _E]),
     "ddgrs_session_fallbacks": (_I32, [_P, ctypes.c_char_p, _I64, ctypes.POINTER(_I64), _E]),
     "ddgrs_session_gmres_history": (_I32, [_P, _P, _I64, _P, _I64, _P, _E]),
+    "ddgrs_session_constraints": (_I32, [_P, _P, _P, _I64, ctypes.POINTER(_I64), _E]),
     "ddgrs_saddle_gmres_history": (_I32, [_P, _P, _I64, _P, _I64, _P, _E]),
     "ddgrs_learned_create": (_I32, [_P, ctypes.c_uint64, _I64, ctypes.POINTER(_P), _E]),
     "ddgrs_learned_destroy": (None, [_P]),
@@ -545,6 +546,10 @@ class _StepOptionsC(ctypes.Structure):
         ("retry_max_iterations", _I64),
         ("dense_max_vertices", _I64),
         ("diverged_residual", _D),
+        ("rows", _I32 * 2),
+        ("row_targets", _D * 2),
+        ("newton_fixed", _I32),
+        ("newton_count", _I32),
     ]
 
 
@@ -578,6 +583,8 @@ class _StepReportC(ctypes.Structure):
         ("direction_max", _D),
         ("halvings", _I64),
         ("fallback_stage", _I32),
+        ("newton_rounds", _I32),
+        ("min_pivot", _D),
     ]
 
 
@@ -726,13 +733,28 @@ class StepOptions:
     retry_max_iterations: int = 0  # 0: max(4 max_iterations, 800)
     dense_max_vertices: int = 6000
     diverged_residual: float = 1e-1  # an unconverged solve above it has diverged (the reference's 0.1)
+    # nonlinear rows after the barycenter's: up to two of ("area", target) and ("volume", target),
+    # target None for the value at creation (src/constraints/mod.rs Constraint::Area, ::Volume);
+    # Newton rounds of the projection: ("tol", n) = UntilTolerance{max: n}, ("fixed", n)
+    constraints: tuple = ()
+    newton: tuple = ("tol", 5)
+
+    _ROW_KINDS = {"area": 1, "volume": 2}
 
     def _c(self) -> _StepOptionsC:
+        if len(self.constraints) > 2:
+            raise RsError(1, "options", "constraints: at most the area and the volume rows")
+        rows = [self._ROW_KINDS[k] for k, _ in self.constraints] + [0] * (2 - len(self.constraints))
+        targets = [float("nan") if t is None else float(t) for _, t in self.constraints]
+        targets += [float("nan")] * (2 - len(targets))
+        if self.newton[0] not in ("tol", "fixed"):
+            raise RsError(1, "options", f"newton = {self.newton!r}: ('tol', n) or ('fixed', n)")
         return _StepOptionsC(self.exponents.alpha, self.exponents.beta, 1 if self.energy_exact else 0, self.theta,
                              self.saddle._c(), self.gmres.restart, self.gmres.tolerance, self.gmres.max_iterations,
                              self.armijo_c, self.tau_min_rel, self.max_halvings, int(self.start[0]),
                              float(self.start[1]), 1 if self.robust else 0, self.retry_max_iterations,
-                             self.dense_max_vertices, self.diverged_residual)
+                             self.dense_max_vertices, self.diverged_residual, (_I32 * 2)(*rows),
+                             (_D * 2)(*targets), 1 if self.newton[0] == "fixed" else 0, int(self.newton[1]))
 
 
 @dataclasses.dataclass(frozen=True)
@@ -761,6 +783,8 @@ class StepReport:
     trial_list: tuple = ()  # (τ, F, TrialOutcome) per trial, in order
     fallback_stage: int = 0  # 1: the GMRES retry solved the step, 2: the dense LU did
     fallbacks: tuple = ()  # the escalations as text (LinearReport::fallbacks)
+    newton_rounds: int = 0  # Newton rounds kept by the projection (nonlinear rows)
+    min_pivot: float = 1.0  # the smallest kept equilibrated Schur pivot
 
 
 class Session(_Handle):
@@ -788,7 +812,17 @@ class Session(_Handle):
                           int(r.pairs_exact), int(r.nodes_lumped), sec, int(r.learned_mode), int(r.learned_fallback),
                           int(r.learned_iterations), float(r.energy_after), float(r.direction_norm),
                           float(r.direction_max), int(r.halvings), self.trials(), int(r.fallback_stage),
-                          self.fallbacks())
+                          self.fallbacks(), int(r.newton_rounds), float(r.min_pivot))
+
+    def constraints(self) -> tuple:
+        """(Φ before the projection, Φ after it) of the last step: the barycenter's x, y, z, then
+        the nonlinear rows; empty without nonlinear rows."""
+        n = _I64()
+        _call(lib().ddgrs_session_constraints, self._h, None, None, 0, ctypes.byref(n))
+        a, b = np.zeros(int(n.value)), np.zeros(int(n.value))
+        if n.value:
+            _call(lib().ddgrs_session_constraints, self._h, _ptr(a), _ptr(b), int(n.value), ctypes.byref(n))
+        return tuple(a.tolist()), tuple(b.tolist())
 
     def fallbacks(self) -> tuple:
         """The last step's escalations of the recovery chain, as text."""

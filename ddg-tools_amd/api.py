@@ -11,8 +11,13 @@ path (the reference's ``device`` takes ``"cpu"``, ``"cuda"`` or ``"auto"``, :829
 
 Positions and faces may be flat (3 per vertex / face) or (n, 3).  The options are the reference's
 names; those that select something this path does not run (remeshing, CCD, other metrics,
-energies, optimizers, penalties, constraints other than the barycenter) raise ``Unsupported``
-instead of being ignored, and an unknown name is ``InvalidOption``.
+energies, optimizers, penalties, pins, per-component rows, target schedules) raise ``Unsupported``
+instead of being ignored, and an unknown name is ``InvalidOption``.  ``constraints`` takes the
+reference's list of dicts (``crates/ddg-py/src/repulsive_surfaces.rs:629``): the global
+``{"type": "barycenter"}`` (required: it is the saddle's own row) and up to one each of
+``{"type": "area", "target": t}`` and ``{"type": "volume", "target": t}`` (target optional: the
+value at creation); ``newton_steps`` (fixed rounds) or ``newton_max`` (rounds until tolerance,
+default 5) set the projection's Newton rounds.
 """
 from __future__ import annotations
 
@@ -35,6 +40,7 @@ DEFAULTS = {
     "armijo": "standard", "start_lengths": 1.0, "start_factor": 2.0, "start_tau": None, "start": "grow_previous",
     "tau_min_rel": 1e-12, "max_halvings": None, "remesh": False, "ccd": False, "max_iterations": 1000,
     "stuck_limit": 5, "stop": "energy_stall", "stop_relative": 1e-7, "stop_window": 10, "threads": None,
+    "constraints": None, "barycenter": True, "newton_steps": None, "newton_max": 5,
 }
 # the reference's names that only select paths outside this one: allowed at their default only
 _ONLY_DEFAULT = {"energy", "metric", "gmres_side", "gpu_determinism", "method", "barycenter_mode", "barycenter_shift",
@@ -50,7 +56,7 @@ def _options(options: dict):
     for k, v in options.items():
         if k == "cuda_ordinal":
             k = "rocm_ordinal"
-        if k in ("penalties", "constraints", "pins", "pin_targets", "pin_move", "ccd_obstacles", "schedules"):
+        if k in ("penalties", "pins", "pin_targets", "pin_move", "ccd_obstacles", "schedules"):
             if v:
                 raise _unsupported(k, v, "penalties, extra constraints and obstacles")
             continue
@@ -69,6 +75,11 @@ def _options(options: dict):
         raise RsError(1, "options", f"energy_eval = {o['energy_eval']!r}: 'barnes_hut' or 'exact'")
     if o["stop"] not in ("energy_stall", "off"):
         raise _unsupported("stop", o["stop"], "this stop rule")
+    rows = _constraints(o["constraints"], o["barycenter"])
+    if o["newton_steps"] is not None:
+        newton = ("fixed", int(o["newton_steps"]))
+    else:
+        newton = ("tol", int(o["newton_max"]))
     kinds = {"grow_previous": StartRule.grow_previous(o["start_lengths"]),
              "max_displacement": StartRule.max_displacement(o["start_lengths"]), "reference": StartRule.reference()}
     if o["start"] == "absolute":
@@ -90,8 +101,48 @@ def _options(options: dict):
         tau_min_rel=float(o["tau_min_rel"]),
         max_halvings=-1 if o["max_halvings"] is None else int(o["max_halvings"]),
         start=start,
+        constraints=tuple(rows),
+        newton=newton,
     )
     return o, step
+
+
+def _constraints(value, barycenter):
+    """The nonlinear rows of the reference's ``constraints`` list (repulsive_surfaces.rs:629)."""
+    if value is None:
+        if not barycenter:
+            raise _unsupported("barycenter", False, "a step without the barycenter row")
+        return []
+    if not isinstance(value, (list, tuple)):
+        raise RsError(1, "options", f"constraints: expected a list of dicts, got {type(value).__name__}")
+    rows, bary, seen = [], False, set()
+    for i, c in enumerate(value):
+        what = f"constraints[{i}]"
+        if not isinstance(c, dict):
+            raise RsError(1, "options", f"{what}: expected a dict")
+        kind = c.get("type")
+        if kind not in ("barycenter", "pins", "area", "volume"):
+            raise RsError(1, "options", f"{what}.type = {kind!r}: 'barycenter', 'pins', 'area' or 'volume'")
+        if kind == "pins":
+            raise _unsupported(what, c, "pinned vertices")
+        if c.get("component") is not None:
+            raise _unsupported(what, c, "a per-component row")
+        if kind == "barycenter":
+            if c.get("target") is not None or set(c) - {"type", "component", "target"}:
+                raise _unsupported(what, c, "a barycenter target")
+            bary = True
+            continue
+        extra = set(c) - {"type", "component", "target"}
+        if extra:
+            raise _unsupported(what, c, f"{sorted(extra)} (growth schedules)")
+        if kind in seen:
+            raise _unsupported(what, c, f"a second {kind} row")
+        seen.add(kind)
+        t = c.get("target")
+        rows.append((kind, None if t is None else float(t)))
+    if not bary:
+        raise _unsupported("constraints", value, "a step without the barycenter row")
+    return rows
 
 
 def _mesh(positions, faces):
@@ -136,12 +187,14 @@ class RepulsiveSurfaceSession:
         elif self._iteration >= int(self._options["max_iterations"]):
             self._status = "max_iterations"
         g = r.gmres
+        phi_before, phi = self._session.constraints()
         self._report = {
             "iteration": self._iteration, "status": "accepted" if r.accepted else "stuck",
             "session_status": self._status, "n_vertices": self._surface.nv, "n_faces": self._surface.nf,
             "energy_before": r.energy_before, "energy_after": r.energy_after, "tangent_point_after": r.energy_after,
             "penalties_after": [], "gradient_norm": r.gradient_norm, "sobolev_norm_sq": r.sobolev_norm_sq,
-            "tau": r.tau, "tau0": r.tau0, "tau_ccd": None, "halvings": r.halvings, "newton_steps": 0,
+            "tau": r.tau, "tau0": r.tau0, "tau_ccd": None, "halvings": r.halvings, "newton_steps": r.newton_rounds,
+            "phi_before": list(phi_before), "phi": list(phi), "schur_pivot": r.min_pivot,
             "trials": [{"tau": t, "energy": e, "outcome": ("accepted", "armijo_failed", "non_finite")[o]}
                        for t, e, o in r.trial_list],
             "linear": {"iterations": g.iterations, "cycles": g.cycles, "converged": g.converged,

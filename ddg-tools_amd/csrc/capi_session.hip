@@ -208,6 +208,10 @@ void ddgrs_step_options_default(ddgrs_step_options* o) {
   o->retry_max_iterations = 0;
   o->dense_max_vertices = kDenseFallbackMaxVertices;
   o->diverged_residual = 1e-1;
+  o->rows[0] = o->rows[1] = 0;
+  o->row_targets[0] = o->row_targets[1] = NAN;
+  o->newton_fixed = 0;
+  o->newton_count = 5;
 }
 
 int ddgrs_session_create(ddgrs_surface* s, const ddgrs_step_options* o, ddgrs_session** out, ddgrs_error* err) {
@@ -258,6 +262,19 @@ int ddgrs_session_create(ddgrs_surface* s, const ddgrs_step_options* o, ddgrs_se
     if (!(o->diverged_residual >= 0.0 && std::isfinite(o->diverged_residual)))
       invalid("diverged_residual", std::to_string(o->diverged_residual), "a finite number >= 0");
     so.diverged_residual = o->diverged_residual;
+    for (int r = 0; r < 2; ++r) {
+      if (o->rows[r] < 0 || o->rows[r] > 2)
+        invalid("rows", std::to_string(o->rows[r]), "0 (none), 1 (total area) or 2 (enclosed volume)");
+      if (!std::isnan(o->row_targets[r]) && !std::isfinite(o->row_targets[r]))
+        invalid("row_targets", std::to_string(o->row_targets[r]), "a finite target or NaN (the value at creation)");
+      so.rows[r] = o->rows[r];
+      so.row_targets[r] = o->row_targets[r];
+    }
+    if (o->rows[0] != 0 && o->rows[0] == o->rows[1])
+      invalid("rows", std::to_string(o->rows[1]), "each nonlinear row at most once");
+    if (o->newton_count < 0) invalid("newton_count", std::to_string(o->newton_count), "non-negative");
+    so.newton_fixed = o->newton_fixed != 0;
+    so.newton_count = o->newton_count;
     auto h = std::make_unique<ddgrs_session>();
     h->s = s;
     h->ses.init(s, so);
@@ -297,6 +314,21 @@ int ddgrs_session_gmres_history(ddgrs_session* h, double* cycle_residuals, int64
     need(h, "session");
     need(counts2, "counts");
     copy_history(h->ses.last_gmres, cycle_residuals, cap_cycles, history, cap_history, counts2);
+  });
+}
+
+int ddgrs_session_constraints(ddgrs_session* h, double* phi_before, double* phi, int64_t capacity, int64_t* nrows,
+                              ddgrs_error* err) {
+  return guarded(err, "session_constraints", [&] {
+    need(h, "session");
+    need(nrows, "nrows");
+    const auto& a = h->ses.last_phi_before;
+    const auto& b = h->ses.last_phi;
+    *nrows = (int64_t)b.size();
+    for (int64_t i = 0; i < capacity && i < (int64_t)b.size(); ++i) {
+      if (phi_before) phi_before[i] = a[i];
+      if (phi) phi[i] = b[i];
+    }
   });
 }
 

@@ -125,6 +125,17 @@ void Session::init(ddgrs_surface* s_, const StepOptions& o) {
   }
   for (int k = 0; k < 3; ++k) target[k] = bc[k] / tot;
   last_tau = 0.0;
+  // the nonlinear rows and their targets (the values at creation unless given)
+  nl_rows.clear();
+  nl_targets.clear();
+  const HostMesh hm{s->faces_h.data(), s->nf, s->nv};
+  for (int r = 0; r < 2; ++r)
+    if (o.rows[r] == kRowArea || o.rows[r] == kRowVolume) {
+      nl_rows.push_back(o.rows[r]);
+      nl_targets.push_back(std::isnan(o.row_targets[r]) ? row_value(hm, pos.data(), o.rows[r]) : o.row_targets[r]);
+    }
+  last_phi_before.clear();
+  last_phi.clear();
   // the mesh's L′ structure (fixed while only the positions change), on another thread
   lp_ready = false;
   const size_t lds = s->ctx->prop.sharedMemPerBlock;
@@ -176,6 +187,69 @@ void Session::project() {
   for (int k = 0; k < 3; ++k) shift[k] = target[k] - pin[1 + k] / tot;
   launch(ctx, "translate", translate_kernel, dim3(blocks_for(nv, 256)), dim3(256), 0, nv, s->positions.ptr, shift[0],
          shift[1], shift[2]);
+  compute_face_geometry(s, s->packs, false);
+}
+
+// schur.rs:495 project with nonlinear rows, on the host in FP64: the positional projection
+// (constraints/mod.rs:657: translate by the barycenter's offset from its target, areas at the
+// positions), then Newton rounds f ← f − Z N⁻¹ Φ_N, each followed by the positional projection and
+// kept only if it lowers max_r |Φ_r| / tol_r over the kept rows; Φ before and after is kept.
+void Session::project_constrained(StepReport* rep) {
+  ddgrs_ctx* ctx = s->ctx;
+  const int64_t nv = s->nv;
+  const size_t nd = 3 * (size_t)nv, kn = nl_rows.size();
+  const HostMesh hm{s->faces_h.data(), s->nf, nv};
+  std::vector<double> pos(nd);
+  download(ctx, pos.data(), (const double*)s->positions.ptr, nd);
+  ctx->sync();
+  auto values = [&](const std::vector<double>& q) {
+    std::vector<double> phi(3 + kn);
+    double b[3];
+    area_barycenter(hm, q.data(), b);
+    for (int k = 0; k < 3; ++k) phi[k] = b[k] - target[k];
+    for (size_t r = 0; r < kn; ++r) phi[3 + r] = row_value(hm, q.data(), nl_rows[r]) - nl_targets[r];
+    return phi;
+  };
+  auto positional = [&](std::vector<double>& q) {
+    double b[3];
+    area_barycenter(hm, q.data(), b);
+    for (int64_t v = 0; v < nv; ++v)
+      for (int k = 0; k < 3; ++k) q[3 * v + k] -= b[k] - target[k];
+  };
+  last_phi_before = values(pos);
+  positional(pos);
+  std::vector<double> phi = values(pos);
+  int rounds = 0;
+  std::vector<size_t> kept = schur.kept;
+  std::sort(kept.begin(), kept.end());
+  if (!kept.empty()) {
+    auto measure = [&](const std::vector<double>& ph) {
+      double m = 0.0;
+      for (size_t i : kept) {
+        const double v = std::fabs(ph[3 + i]) / row_tolerance(nl_targets[i]);
+        m = (std::isnan(v) || std::isnan(m)) ? NAN : std::max(m, v);
+      }
+      return m;
+    };
+    double m = measure(phi);
+    while (rounds < opts.newton_count) {
+      if (!opts.newton_fixed && m <= 1.0) break;
+      const std::vector<double> h = schur.correction(std::vector<double>(phi.begin() + 3, phi.end()));
+      std::vector<double> trial = pos;
+      for (size_t i = 0; i < nd; ++i) trial[i] -= h[i];
+      positional(trial);
+      const std::vector<double> pt = values(trial);
+      const double mt = measure(pt);
+      if (!(mt < m)) break;
+      pos = std::move(trial);
+      phi = pt;
+      m = mt;
+      ++rounds;
+    }
+  }
+  last_phi = phi;
+  rep->newton_rounds = rounds;
+  upload(ctx, s->positions, pos.data(), nd);
   compute_face_geometry(s, s->packs, false);
 }
 
@@ -415,6 +489,38 @@ void Session::step(StepReport* rep) {
     e.value = gr.relative_residual;
     throw e;
   }
+  // the nonlinear rows (schur.rs:101 with_direction): z_r = A′⁻¹ [C_N row r; 0] with the step's
+  // operator and preconditioner, S = −C_N Z and its factor, then x = y − Z N⁻¹ C_N y (host, FP64)
+  rep->min_pivot = 1.0;
+  if (!nl_rows.empty()) {
+    const HostMesh hm{s->faces_h.data(), nf, nv};
+    std::vector<std::vector<double>> crow, zcol;
+    std::vector<double> hb(n, 0.0), hz(n);
+    for (int kind : nl_rows) {
+      crow.push_back(row_gradient(hm, geo.pos.data(), kind));
+      std::copy(crow.back().begin(), crow.back().end(), hb.begin());
+      upload(ctx, zrhs, hb.data(), n);
+      zsol.reserve(n);
+      GmresResult gz = gmres_device(ctx, n, A, P, zrhs.ptr, zsol.ptr, gs, vec, stall);
+      fm.extended = pc.vf.extended = false;
+      if (!gz.converged && !(gz.relative_residual <= opts.diverged_residual)) {
+        Error e(DDGRS_SOLVER_DIVERGED, "gmres", "a Schur column's solve: true relative residual above 0.1");
+        e.value = gz.relative_residual;
+        throw e;
+      }
+      download(ctx, hz.data(), (const double*)zsol.ptr, n);
+      ctx->sync();
+      zcol.emplace_back(hz.begin(), hz.begin() + (long)nd);
+    }
+    schur.build(std::move(crow), std::move(zcol));
+    std::vector<double> y(nd);
+    download(ctx, y.data(), (const double*)sol.ptr, nd);
+    ctx->sync();
+    const std::vector<double> x = schur.direction(y);
+    HIPCHECK(hipMemcpyAsync(sol.ptr, x.data(), nd * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    ctx->sync();
+    rep->min_pivot = schur.min_pivot;
+  }
   // ⟨g, x⟩, ‖x‖, ‖x‖_∞ and the largest vertex displacement
   vec.dots(ctx, g.ptr, 1, sol.ptr, scal.ptr + 1);
   const unsigned nbv = blocks_for(nv, 256);
@@ -476,7 +582,10 @@ void Session::step(StepReport* rep) {
   // 4. projection onto the barycenter constraint, then F at the projected positions
   // (session.rs:1505 `after`)
   t = std::chrono::steady_clock::now();
-  project();
+  if (nl_rows.empty())
+    project();
+  else
+    project_constrained(rep);
   if (!opts.energy_exact) bh.refresh(s);
   rep->energy_after = energy_value();
   rep->seconds_projection = seconds_since(t);

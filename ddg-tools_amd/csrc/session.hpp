@@ -6,7 +6,10 @@
 #include <string>
 #include <vector>
 
+#include <cmath>
+
 #include "bh.hpp"
+#include "constraints.hpp"
 #include "gmres.hpp"
 #include "learned.hpp"
 #include "lprime_gpu.hpp"
@@ -34,6 +37,11 @@ struct StepOptions {
   long retry_max_iterations = 0; // its GMRES retry's cap (0: max(4 max_iterations, 800))
   int64_t dense_max_vertices = 6000;  // its dense LU's largest mesh
   double diverged_residual = 1e-1;    // an unconverged solve above it has diverged
+  // nonlinear rows after the barycenter's (kRowArea, kRowVolume), targets (NaN: at creation), Newton
+  int rows[2] = {0, 0};
+  double row_targets[2] = {NAN, NAN};
+  bool newton_fixed = false;
+  int newton_count = 5;
 };
 
 // TrialOutcome (RS/src/step/session.rs): 0 accepted, 1 Armijo failed, 2 non-finite.
@@ -69,6 +77,12 @@ struct Session {
   DevBuf<double> pg, kr, kz;  // P⁻¹ g and the Krylov vectors of recorded pairs
   std::vector<TrialRecord> trials;  // the last step's line search
   std::vector<std::string> fallbacks;  // the last step's escalations (LinearReport::fallbacks)
+  // the nonlinear constraint rows, their targets, the last step's Schur complement and Φ
+  std::vector<int> nl_rows;
+  std::vector<double> nl_targets;
+  SchurRows schur;
+  std::vector<double> last_phi_before, last_phi;
+  DevBuf<double> zrhs, zsol;
   size_t last_iterations = 0;  // iterations of the last converged solve
   int learned_misses = 0;        // consecutive (I + Q) P⁻¹ solves that fell back to the plain one
   static constexpr int kLearnedMissLimit = 2;  // after that many, the session solves plain
@@ -85,6 +99,8 @@ struct Session {
   double read_scalar(const double* d);
   double energy_value();
   void project();
+  // the projection with nonlinear rows: the barycenter translation and the Newton rounds (host)
+  void project_constrained(StepReport* rep);
 };
 
 }  // namespace ddgrs

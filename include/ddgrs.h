@@ -266,6 +266,18 @@ typedef struct ddgrs_step_options {
   int64_t retry_max_iterations;
   int64_t dense_max_vertices;
   double diverged_residual;
+  /* Nonlinear constraint rows after the barycenter's (RS/src/constraints/mod.rs Constraint::Area,
+   * ::Volume over the whole mesh): rows[i] 0 none, 1 the total area, 2 the enclosed volume; their
+   * targets (NaN: the value at session creation).  They enter the direction through the Schur
+   * complement of their rows (RS/src/solve/schur.rs:101 SchurComplement::with_direction: one more
+   * GMRES solve per row) and the projection through Newton rounds (schur.rs:495 project):
+   * newton_fixed 1 takes exactly newton_count rounds (NewtonSteps::Fixed), 0 repeats while a kept
+   * row is above 1e-8 |target| (NewtonSteps::UntilTolerance{max: newton_count}); each round is
+   * kept only if it lowers max |Φ_r| / tol_r. */
+  int32_t rows[2];
+  double row_targets[2];
+  int32_t newton_fixed;
+  int32_t newton_count;
 } ddgrs_step_options;
 
 /* A step's report (RS/src/step/session.rs:146 StepReport, the fields this path produces).
@@ -291,6 +303,10 @@ typedef struct ddgrs_step_report {
   /* The recovery chain: 0 not needed, 1 the GMRES retry solved the step (then `gmres` is its
    * report), 2 the dense LU did (gmres.relative_residual is its explicit residual). */
   int32_t fallback_stage;
+  /* The nonlinear rows: Newton rounds kept by the projection (StepReport::newton_steps) and the
+   * smallest kept equilibrated Schur pivot (LinearReport::schur_pivot; 1 without rows). */
+  int32_t newton_rounds;
+  double min_pivot;
 } ddgrs_step_report;
 
 /* ddgrs_saddle_gmres_history for the last step's direction solve (the last GMRES run of the
@@ -301,6 +317,13 @@ DDGRS_API int ddgrs_session_gmres_history(ddgrs_session* h, double* cycle_residu
 /* The escalations of the last step as text (LinearReport::fallbacks, RS/src/solve/mod.rs), one a
  * line, NUL-terminated within `capacity` bytes; *count = the number of escalations. */
 DDGRS_API int ddgrs_session_fallbacks(ddgrs_session* h, char* text, int64_t capacity, int64_t* count, ddgrs_error* err);
+
+/* Φ of the constraint rows (the barycenter's x, y, z, then the nonlinear rows) at the last step's
+ * moved positions before the projection and after it (schur.rs:495 Projection::phi); *nrows = 3 + the
+ * nonlinear rows, or 0 when the session has none (its projection is the barycenter translation
+ * alone).  At most `capacity` values of each are written. */
+DDGRS_API int ddgrs_session_constraints(ddgrs_session* h, double* phi_before, double* phi, int64_t capacity,
+                                        int64_t* nrows, ddgrs_error* err);
 
 /* The trials of the last step in order (StepReport::trials, RS/src/step/session.rs:136 Trial): τ,
  * F (NaN when not finite) and the outcome (TrialOutcome: 0 accepted, 1 Armijo failed, 2 non-finite);

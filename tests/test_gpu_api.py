@@ -28,6 +28,22 @@ def test_session_api_matches_native(pkg, ctx, tmp_path):
     assert api.iteration == 0 and np.array_equal(api.positions.reshape(-1, 3), p1)
 
 
+def test_session_api_constraints(pkg, ctx):
+    """The reference's ``constraints`` list (area and volume held) through the API: the same steps
+    as the native session with the same rows, Φ reported before and after the projection."""
+    p, f = pkg.generate.config_icosphere_l3()
+    cons = [{"type": "barycenter"}, {"type": "area"}, {"type": "volume"}]
+    api = pkg.api.RepulsiveSurfaceSession(p, f, device="rocm", constraints=cons, energy_eval="exact", chi=0.0)
+    native = pkg.Session(pkg.Surface(ctx, f, p), pkg.StepOptions(energy_exact=True, saddle=pkg.SaddleOptions(chi=0.0),
+                                                                 constraints=(("area", None), ("volume", None))))
+    for _ in range(2):
+        a, n = api.step(), native.step()
+        assert a["status"] == "accepted" and a["energy_after"] == n.energy_after
+        assert a["newton_steps"] == n.newton_rounds >= 1 and len(a["phi"]) == 5
+        print("phi after", a["phi"], "rounds", a["newton_steps"])
+    assert np.array_equal(api.positions.reshape(-1, 3), native.positions())
+
+
 def test_surface_energy_and_gpu_info(pkg, ctx):
     p, f = pkg.generate.config_icosphere_l3()
     e, d, rep = pkg.api.surface_energy(p, f, energy_eval="exact")

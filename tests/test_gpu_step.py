@@ -363,6 +363,69 @@ def test_step_lean_stuck(pkg, ctx):
     assert np.abs(s.positions() - p).max() <= 1e-12
 
 
+@pytest.mark.parametrize("name", ["rs-icosa-step", "rs-mobius-step", "rs-torus-step-ref"])
+def test_step_lean_constrained(pkg, ctx, name):
+    """The reference's constrained step fixtures (area and volume rows through the Schur complement
+    and the projection's Newton rounds, RS/src/solve/schur.rs:101,495) through the device session
+    with exact energy and the exact metric, at the FP32 metric's accuracy: the direction, the
+    trials, the constraint values after the projection and the positions."""
+    c, o = fx.read_case(name), fx.read_out(name)
+    p, f = c["positions"], c["faces"]
+    rows = []
+    for cc in c["constraints"]:
+        if cc[0] in ("area", "volume"):
+            rows.append((cc[0], fx.hexf(cc[2]) if len(cc) > 2 else None))
+    st = c["start"]
+    start = (pkg.StartRule.max_displacement(st[1]) if st[0] == "maxdisp" else
+             pkg.StartRule.absolute(st[1]) if st[0] == "abs" else pkg.StartRule.reference())
+    s = pkg.Surface(ctx, f, p)
+    opts = pkg.StepOptions(exponents=pkg.Exponents(*c["exponents"]), energy_exact=True,
+                           saddle=pkg.SaddleOptions(chi=0.0), armijo_c=c["armijo"], max_halvings=c["maxhalv"],
+                           start=start, constraints=tuple(rows), newton=c["newton"])
+    ses = pkg.Session(s, opts)
+    r = ses.step()
+    phi_before, phi = ses.constraints()
+    relv = lambda a, b: abs(a - b) / abs(b)
+    print(f"{name}: F0 {r.energy_before:.12g} / {o['energy0']:.12g}, slope {r.sobolev_norm_sq:.10g} / {o['slope']:.10g}, "
+          f"|x| {r.direction_norm:.10g} / {o['xnorm']:.10g}, newton {r.newton_rounds} / {o['newton']}, "
+          f"pivot {r.min_pivot:.4g}, phi {phi}")
+    assert relv(r.energy_before, o["energy0"]) <= 1e-5
+    assert relv(r.sobolev_norm_sq, o["slope"]) <= 1e-4
+    assert relv(r.direction_norm, o["xnorm"]) <= 1e-4
+    assert relv(r.tau0, o["tau0"]) <= 1e-4
+    want = {"accepted": 0, "armijo-failed": 1, "non-finite": 2}
+    assert len(r.trial_list) == len(o["trials"])
+    for (tau, fe, outcome), (_, wtau, wout, wf, _) in zip(r.trial_list, o["trials"]):
+        assert outcome == want[wout] and relv(tau, wtau) <= 1e-4
+        assert relv(fe, wf) <= 1e-4
+    assert r.accepted and r.halvings == o["result"][2]
+    assert relv(r.energy_trial, o["energy1"]) <= 1e-4
+    assert len(phi) == len(o["phi"]) == 3 + len(rows)
+    # the rows' targets: given, or the values at creation (the oracle's rows at the input)
+    v0, _ = oracle.constraint_rows(f, p, [k for k, _ in rows])
+    targets = [t if t is not None else v for (_, t), v in zip(rows, v0)]
+    # Φ before the projection follows the step's motion; after it the barycenter rows vanish and
+    # the nonlinear rows reach their tolerance 1e-8 |target| (rounds until tolerance), or match the
+    # fixture's after its fixed rounds
+    scale = max(1.0, max(abs(x) for x in o["phi-before"]))
+    assert np.max(np.abs(np.array(phi_before) - np.array(o["phi-before"]))) <= 1e-4 * scale
+    assert np.max(np.abs(phi[:3])) <= 1e-12 * max(1.0, np.abs(p).max())
+    for i, (kind, _) in enumerate(rows):
+        w = o["phi"][3 + i]
+        if c["newton"][0] == "tol":
+            assert 1 <= r.newton_rounds <= c["newton"][1]
+            assert abs(phi[3 + i]) <= 1e-8 * abs(targets[i]), (kind, phi[3 + i], w)
+        else:
+            assert r.newton_rounds == o["newton"]
+            assert abs(phi[3 + i] - w) <= 1e-4 * abs(w), (kind, phi[3 + i], w)
+    assert relv(r.energy_after, o["energy-final"]) <= 1e-4
+    want_x = np.array([o["rows"]["x"][i] for i in range(len(o["rows"]["x"]))])
+    moved = np.abs(want_x - p).max()
+    assert np.abs(s.positions() - want_x).max() <= 1e-4 * moved
+    ses.close()
+    s.close()
+
+
 def test_step_c1_exact_energy(pkg, ctx):
     p, f = pkg.generate.config_icosphere_l3()
     s = pkg.Surface(ctx, f, p)

@@ -54,8 +54,18 @@ def test_session_options():
     assert step.exponents == pkg.Exponents(4.0, 8.0) and step.energy_exact and step.saddle.chi == 0.25
     assert step.saddle.c_lo == 0.0 and step.gmres.tolerance == 1e-8 and step.max_halvings == 3
     assert step.start == pkg.StartRule.absolute(0.5)
+    o, step = pkg.api._options({"constraints": [{"type": "barycenter"}, {"type": "area"},
+                                                {"type": "volume", "target": 2.5}], "newton_steps": 1})
+    assert step.constraints == (("area", None), ("volume", 2.5)) and step.newton == ("fixed", 1)
+    assert pkg.api._options({})[1].constraints == () and pkg.api._options({})[1].newton == ("tol", 5)
     for bad, kind in (({"remesh": True}, "Unsupported"), ({"device": "cuda"}, "Unsupported"),
                       ({"metric": "h1"}, "Unsupported"), ({"penalties": [1]}, "Unsupported"),
+                      ({"constraints": [{"type": "area"}]}, "Unsupported"),
+                      ({"constraints": [{"type": "barycenter"}, {"type": "pins", "vertices": [0]}]}, "Unsupported"),
+                      ({"constraints": [{"type": "barycenter"}, {"type": "area", "component": 0}]}, "Unsupported"),
+                      ({"constraints": [{"type": "barycenter"}, {"type": "area", "factor": 1.1, "steps": 3}]},
+                       "Unsupported"),
+                      ({"constraints": [{"type": "normal"}]}, "InvalidOption"), ({"barycenter": False}, "Unsupported"),
                       ({"no_such_option": 1}, "InvalidOption"), ({"start": "absolute"}, "InvalidOption")):
         with pytest.raises(pkg.RsError) as e:
             pkg.api._options(bad)
