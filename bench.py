@@ -282,17 +282,18 @@ def learned_leg(pkg, ctx, args, opts, p, f, plain_ms, plain_its, level=6, bumps=
     ctx.set_profiling(False)
     ses.close()
     nv = p.shape[0]
-    flop = 2.0 * nv * (8 * 64 + 4 * 128 * 64)
+    ne = net.edges().shape[0]
+    flop = 2.0 * (nv * (8 * 64 + 4 * 128 * 64) + ne * 64 * 64)
     rows = kt.get("ln_rows_mfma")
     head = kt.get("ln_rows_valu")
     fwd = None
     if rows:
         a = flop / (rows["ms"] * 1e-3) / 1e12
-        # the layers of width 64 (BF16 inputs, FP32 accumulation); the 64 x 18 head runs on VALU
+        # the layers of width 64 (BF16 inputs, FP32 accumulation); the two 64 x 9 heads run on VALU
         fwd = {"bound": "mfma", "achieved": a, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s", "frac": a / PEAK_BF16_TFLOPS,
                "kernel_ms_per_step": rows["ms"], "launches": rows["launches"],
                "head_valu_ms_per_step": head["ms"] if head else None,
-               "work": f"2 x {nv} vertices x (8x64 + 4 x 128x64) multiply-adds, BF16 inputs"}
+               "work": f"2 x ({nv} vertices x (8x64 + 4 x 128x64) + {ne} edges x 64x64) multiply-adds, BF16 inputs"}
     its = [r.learned_iterations if not r.learned_fallback else r.gmres.iterations for r in reps]
     return {
         "pairs": net.info()["pairs"],

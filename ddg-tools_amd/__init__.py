@@ -904,8 +904,17 @@ class Learned(_Handle):
         _call(lib().ddgrs_learned_features, self._h, _ptr(_f64(positions, (self.surface.nv, 3))), _ptr(f))
         return f
 
+    def edges(self) -> np.ndarray:
+        """The mesh's undirected edges {u < v} in the network's order (ascending), ne × 2."""
+        f = np.asarray(self.surface.faces, dtype=np.int64).reshape(-1, 3)
+        e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
+        e = np.unique(np.sort(e, axis=1), axis=0)
+        return e
+
     def blocks(self, positions) -> np.ndarray:
-        b = np.zeros((self.surface.nv, 18), dtype=np.float32)
+        """The network's blocks at the positions: B_v (nv rows of 9, row-major), then E_uv (one row
+        of 9 per edge of ``edges()``)."""
+        b = np.zeros((self.surface.nv + self.edges().shape[0], 9), dtype=np.float32)
         _call(lib().ddgrs_learned_blocks, self._h, _ptr(_f64(positions, (self.surface.nv, 3))), _ptr(b))
         return b
 

@@ -22,9 +22,17 @@ LearnedLayout learned_layout() {
     L.b[l] = o;
     o += kLnWidth;
   }
-  L.wo = o;
+  L.wv = o;
   o += (size_t)kLnWidth * kLnOut;
-  L.bo = o;
+  L.bv = o;
+  o += kLnOut;
+  L.we1 = o;
+  o += (size_t)kLnWidth * kLnWidth;
+  L.be1 = o;
+  o += kLnWidth;
+  L.we2 = o;
+  o += (size_t)kLnWidth * kLnOut;
+  L.be2 = o;
   o += kLnOut;
   L.total = o;
   return L;
@@ -299,8 +307,10 @@ __global__ __launch_bounds__(256) void ln_fill_kernel(size_t n, float* __restric
 
 // q_v = s (B_v r_v + Σ_u ½ (g_v + g_u) r_u); d_v = 2 (q_v − e_v) / ‖e‖² into qd; per workgroup
 // Σ |q − e|² into lsum (FP64).
+// q_v = B_v r_v + Σ_{u ∈ N(v)} E_uv r_u; qd = 2 (s q − e) / |e|², loss partial sums per block.
 __global__ __launch_bounds__(256) void ln_loss_kernel(int n, const uint32_t* __restrict__ ptr,
-                                                      const uint32_t* __restrict__ nb, const float* __restrict__ o,
+                                                      const uint32_t* __restrict__ nb, const uint32_t* __restrict__ emap,
+                                                      const float* __restrict__ o, const float* __restrict__ eo,
                                                       const float* __restrict__ r, const float* __restrict__ e,
                                                       float s, float inv_e2, float* __restrict__ qd,
                                                       double* __restrict__ lsum) {
@@ -317,9 +327,9 @@ __global__ __launch_bounds__(256) void ln_loss_kernel(int n, const uint32_t* __r
     }
     for (uint32_t k = ptr[v]; k < ptr[v + 1]; ++k) {
       const uint32_t u = nb[k];
-      const float* ou = o + (size_t)u * kLnOut;
+      const float* E = eo + (size_t)emap[k] * kLnOut;
       for (int a = 0; a < 3; ++a)
-        for (int b = 0; b < 3; ++b) q[a] = fmaf(0.5f * (ov[9 + 3 * a + b] + ou[9 + 3 * a + b]), r[3 * u + b], q[a]);
+        for (int b = 0; b < 3; ++b) q[a] = fmaf(E[3 * a + b], r[3 * u + b], q[a]);
     }
     for (int a = 0; a < 3; ++a) {
       const float df = s * q[a] - e[3 * v + a];
@@ -336,26 +346,47 @@ __global__ __launch_bounds__(256) void ln_loss_kernel(int n, const uint32_t* __r
   if (threadIdx.x == 0) lsum[blockIdx.x] = red[0];
 }
 
-// dout[v] = s (d_v ⊗ r_v, ½ Σ_{u ∈ N(v)} (d_v ⊗ r_u + d_u ⊗ r_v)).
-__global__ __launch_bounds__(256) void ln_dout_kernel(int n, const uint32_t* __restrict__ ptr,
-                                                      const uint32_t* __restrict__ nb, const float* __restrict__ r,
-                                                      const float* __restrict__ qd, float s, float* __restrict__ dout) {
+// dB_v = s d_v ⊗ r_v.
+__global__ __launch_bounds__(256) void ln_dout_kernel(int n, const float* __restrict__ r, const float* __restrict__ qd,
+                                                      float s, float* __restrict__ dout) {
   const int v = blockIdx.x * 256 + threadIdx.x;
   if (v >= n) return;
   float* d = dout + (size_t)v * kLnOut;
-  const float* dv = qd + 3 * (size_t)v;
-  const float* rv = r + 3 * (size_t)v;
-  float g[9] = {};
-  for (uint32_t k = ptr[v]; k < ptr[v + 1]; ++k) {
-    const uint32_t u = nb[k];
-    for (int a = 0; a < 3; ++a)
-      for (int b = 0; b < 3; ++b) g[3 * a + b] += dv[a] * r[3 * (size_t)u + b] + qd[3 * (size_t)u + a] * rv[b];
-  }
   for (int a = 0; a < 3; ++a)
-    for (int b = 0; b < 3; ++b) {
-      d[3 * a + b] = s * dv[a] * rv[b];
-      d[9 + 3 * a + b] = 0.5f * s * g[3 * a + b];
-    }
+    for (int b = 0; b < 3; ++b) d[3 * a + b] = s * qd[3 * (size_t)v + a] * r[3 * (size_t)v + b];
+}
+
+// dE_uv = s (d_u ⊗ r_v + d_v ⊗ r_u) per edge {u, v}.
+__global__ __launch_bounds__(256) void ln_edge_dout_kernel(int ne, const uint32_t* __restrict__ edge,
+                                                           const float* __restrict__ r, const float* __restrict__ qd,
+                                                           float s, float* __restrict__ dout) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= ne) return;
+  const size_t u = edge[2 * (size_t)k], v = edge[2 * (size_t)k + 1];
+  float* d = dout + (size_t)k * kLnOut;
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) d[3 * a + b] = s * (qd[3 * u + a] * r[3 * v + b] + qd[3 * v + a] * r[3 * u + b]);
+}
+
+// The edge head's input h_u + h_v (64 channels) per edge.
+__global__ __launch_bounds__(256) void ln_edge_in_kernel(int ne, const uint32_t* __restrict__ edge,
+                                                         const float* __restrict__ h, float* __restrict__ x) {
+  const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (size_t)ne * kLnWidth) return;
+  const size_t k = t / kLnWidth, c = t % kLnWidth;
+  x[t] = h[(size_t)edge[2 * k] * kLnWidth + c] + h[(size_t)edge[2 * k + 1] * kLnWidth + c];
+}
+
+// dh_v += Σ_{u ∈ N(v)} dx_{edge {u, v}} (neighbour order: a fixed sum per vertex).
+__global__ __launch_bounds__(256) void ln_edge_gather_kernel(int n, const uint32_t* __restrict__ ptr,
+                                                             const uint32_t* __restrict__ emap,
+                                                             const float* __restrict__ dx, float* __restrict__ dh) {
+  const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (size_t)n * kLnWidth) return;
+  const size_t v = t / kLnWidth, c = t % kLnWidth;
+  float acc = dh[t];
+  for (uint32_t k = ptr[v]; k < ptr[v + 1]; ++k) acc += dx[(size_t)emap[k] * kLnWidth + c];
+  dh[t] = acc;
 }
 
 // Adam (β₁ 0.9, β₂ 0.999, ε 1e-8) on g / batch, then g = 0.
@@ -372,9 +403,10 @@ __global__ __launch_bounds__(256) void ln_adam_kernel(size_t n, float* __restric
   g[t] = 0.f;
 }
 
-// z_v += s (B_v r_v + Σ_u ½ (g_v + g_u) r_u) (FP64 sums of the FP32 blocks).
+// z_v += s (B_v r_v + Σ_u E_uv r_u) (FP64 sums of the FP32 blocks).
 __global__ __launch_bounds__(256) void ln_apply_kernel(int n, const uint32_t* __restrict__ ptr,
-                                                       const uint32_t* __restrict__ nb, const float* __restrict__ o,
+                                                       const uint32_t* __restrict__ nb, const uint32_t* __restrict__ emap,
+                                                       const float* __restrict__ o, const float* __restrict__ eo,
                                                        double s, const double* __restrict__ r, double* __restrict__ z) {
   const int v = blockIdx.x * 256 + threadIdx.x;
   if (v >= n) return;
@@ -387,9 +419,9 @@ __global__ __launch_bounds__(256) void ln_apply_kernel(int n, const uint32_t* __
   }
   for (uint32_t k = ptr[v]; k < ptr[v + 1]; ++k) {
     const uint32_t u = nb[k];
-    const float* ou = o + (size_t)u * kLnOut;
+    const float* E = eo + (size_t)emap[k] * kLnOut;
     for (int a = 0; a < 3; ++a)
-      for (int b = 0; b < 3; ++b) q[a] += 0.5 * ((double)ov[9 + 3 * a + b] + (double)ou[9 + 3 * a + b]) * r[3 * (size_t)u + b];
+      for (int b = 0; b < 3; ++b) q[a] += (double)E[3 * a + b] * r[3 * (size_t)u + b];
   }
   for (int a = 0; a < 3; ++a) z[3 * (size_t)v + a] += s * q[a];
 }
@@ -444,6 +476,30 @@ void LearnedModel::init(ddgrs_ctx* ctx, const std::vector<uint32_t>& f, int n, u
   }
   upload(ctx, nbr_ptr, nbr_ptr_h);
   upload(ctx, nbr, nbr_h);
+  // undirected edges {u < v}: vertex u's larger neighbours in order, u ascending
+  edge_h.clear();
+  std::vector<uint32_t> first(nv + 1, 0);
+  for (int u = 0; u < nv; ++u) {
+    first[u] = (uint32_t)(edge_h.size() / 2);
+    for (uint32_t k = nbr_ptr_h[u]; k < nbr_ptr_h[u + 1]; ++k)
+      if (nbr_h[k] > (uint32_t)u) {
+        edge_h.push_back((uint32_t)u);
+        edge_h.push_back(nbr_h[k]);
+      }
+  }
+  ne = (int)(edge_h.size() / 2);
+  emap_h.assign(nbr_h.size(), 0);
+  for (int v = 0; v < nv; ++v)
+    for (uint32_t k = nbr_ptr_h[v]; k < nbr_ptr_h[v + 1]; ++k) {
+      const uint32_t u = nbr_h[k], a = std::min<uint32_t>(u, v), b = std::max<uint32_t>(u, v);
+      // the edge {a, b}: a's larger neighbours are in ascending order from first[a]
+      uint32_t id = first[a];
+      for (uint32_t q = nbr_ptr_h[a]; q < nbr_ptr_h[a + 1] && nbr_h[q] != b; ++q)
+        if (nbr_h[q] > a) ++id;
+      emap_h[k] = id;
+    }
+  upload(ctx, edge, edge_h);
+  upload(ctx, emap, emap_h);
   // parameters: hidden layers uniform ±√(6 / (fan_in + fan_out)) from a fixed 64-bit LCG, biases 0;
   // the output layer 0 (Q = 0 before training)
   const LearnedLayout L = learned_layout();
@@ -459,6 +515,7 @@ void LearnedModel::init(ddgrs_ctx* ctx, const std::vector<uint32_t>& f, int n, u
   };
   fill(L.w0, (size_t)kLnFeat * kLnWidth, kLnFeat, kLnWidth);
   for (int l = 0; l < kLnLayers; ++l) fill(L.w[l], (size_t)2 * kLnWidth * kLnWidth, 2 * kLnWidth, kLnWidth);
+  fill(L.we1, (size_t)kLnWidth * kLnWidth, kLnWidth, kLnWidth);
   upload(ctx, params, p);
   std::vector<float> z(L.total, 0.f);
   upload(ctx, grad, z);
@@ -471,6 +528,13 @@ void LearnedModel::init(ddgrs_ctx* ctx, const std::vector<uint32_t>& f, int n, u
   mean.reserve((size_t)kLnLayers * nv * kLnWidth);
   out.reserve((size_t)nv * kLnOut);
   dout.reserve((size_t)nv * kLnOut);
+  const size_t EW = (size_t)std::max(ne, 1) * kLnWidth;
+  e_in.reserve(EW);
+  e_pre.reserve(EW);
+  e_act.reserve(EW);
+  e_d.reserve(EW);
+  e_out.reserve((size_t)std::max(ne, 1) * kLnOut);
+  e_dout.reserve((size_t)std::max(ne, 1) * kLnOut);
   dh.reserve((size_t)nv * kLnWidth);
   dtmp.reserve((size_t)2 * nv * kLnWidth);
   qd.reserve(3 * (size_t)nv);
@@ -556,8 +620,18 @@ void LearnedModel::forward(ddgrs_ctx* ctx, const float* f) {
     rows<2 * kLnWidth, kLnWidth>(ctx, nv, h, kLnWidth, kLnWidth, m, kLnWidth, P + L.w[l], kLnWidth, 1, P + L.b[l], 1,
                                  pre.ptr + (l + 1) * VW, act.ptr + (l + 1) * VW, kLnWidth);
   }
-  rows<kLnWidth, kLnOut>(ctx, nv, act.ptr + kLnLayers * VW, kLnWidth, kLnWidth, nullptr, 0, P + L.wo, kLnOut, 1,
-                         P + L.bo, 0, nullptr, out.ptr, kLnOut);
+  const float* h4 = act.ptr + kLnLayers * VW;
+  rows<kLnWidth, kLnOut>(ctx, nv, h4, kLnWidth, kLnWidth, nullptr, 0, P + L.wv, kLnOut, 1, P + L.bv, 0, nullptr,
+                         out.ptr, kLnOut);
+  if (ne > 0) {
+    const size_t EW = (size_t)ne * kLnWidth;
+    launch(ctx, "ln_edge_in", ln_edge_in_kernel, dim3(blocks_for((int64_t)EW, 256)), dim3(256), 0, ne,
+           (const uint32_t*)edge.ptr, h4, e_in.ptr);
+    rows<kLnWidth, kLnWidth>(ctx, ne, e_in.ptr, kLnWidth, kLnWidth, nullptr, 0, P + L.we1, kLnWidth, 1, P + L.be1, 1,
+                             e_pre.ptr, e_act.ptr, kLnWidth);
+    rows<kLnWidth, kLnOut>(ctx, ne, e_act.ptr, kLnWidth, kLnWidth, nullptr, 0, P + L.we2, kLnOut, 1, P + L.be2, 0,
+                           nullptr, e_out.ptr, kLnOut);
+  }
   blocks_ready = true;
 }
 
@@ -598,17 +672,36 @@ double LearnedModel::loss_and_grad(ddgrs_ctx* ctx, int p, bool add_grad, bool re
   forward(ctx, f);
   const unsigned nbv = blocks_for(nv, 256);
   launch(ctx, "ln_loss", ln_loss_kernel, dim3(nbv), dim3(256), 0, nv, (const uint32_t*)nbr_ptr.ptr,
-         (const uint32_t*)nbr.ptr, (const float*)out.ptr, r, e, scale, (float)(1.0 / pair_e2[p]), qd.ptr, lsum.ptr);
+         (const uint32_t*)nbr.ptr, (const uint32_t*)emap.ptr, (const float*)out.ptr, (const float*)e_out.ptr, r, e,
+         scale, (float)(1.0 / pair_e2[p]), qd.ptr, lsum.ptr);
   if (add_grad) {
     float* G = grad.ptr;
     const float* P = params.ptr;
-    launch(ctx, "ln_dout", ln_dout_kernel, dim3(nbv), dim3(256), 0, nv, (const uint32_t*)nbr_ptr.ptr,
-           (const uint32_t*)nbr.ptr, r, (const float*)qd.ptr, scale, dout.ptr);
+    launch(ctx, "ln_dout", ln_dout_kernel, dim3(nbv), dim3(256), 0, nv, r, (const float*)qd.ptr, scale, dout.ptr);
     const float* h4 = act.ptr + kLnLayers * VW;
-    wgrad<kLnWidth, kLnOut>(ctx, nv, h4, kLnWidth, kLnWidth, nullptr, 0, dout.ptr, kLnOut, gpart.ptr, G + L.wo, G + L.bo);
-    // dh = dout W_oᵀ
-    rows<kLnOut, kLnWidth>(ctx, nv, dout.ptr, kLnOut, kLnOut, nullptr, 0, P + L.wo, 1, kLnOut, nullptr, 0, nullptr,
+    wgrad<kLnWidth, kLnOut>(ctx, nv, h4, kLnWidth, kLnWidth, nullptr, 0, dout.ptr, kLnOut, gpart.ptr, G + L.wv, G + L.bv);
+    // dh = dB W_vᵀ, then the edge head's share: dE → dt = dE W_e2ᵀ → ⊙ relu′ → dx = · W_e1ᵀ,
+    // added to both ends of every edge
+    rows<kLnOut, kLnWidth>(ctx, nv, dout.ptr, kLnOut, kLnOut, nullptr, 0, P + L.wv, 1, kLnOut, nullptr, 0, nullptr,
                            dh.ptr, kLnWidth);
+    if (ne > 0) {
+      const size_t EW = (size_t)ne * kLnWidth;
+      launch(ctx, "ln_edge_dout", ln_edge_dout_kernel, dim3(blocks_for(ne, 256)), dim3(256), 0, ne,
+             (const uint32_t*)edge.ptr, r, (const float*)qd.ptr, scale, e_dout.ptr);
+      wgrad<kLnWidth, kLnOut>(ctx, ne, e_act.ptr, kLnWidth, kLnWidth, nullptr, 0, e_dout.ptr, kLnOut, gpart.ptr,
+                              G + L.we2, G + L.be2);
+      rows<kLnOut, kLnWidth>(ctx, ne, e_dout.ptr, kLnOut, kLnOut, nullptr, 0, P + L.we2, 1, kLnOut, nullptr, 0, nullptr,
+                             e_d.ptr, kLnWidth);
+      launch(ctx, "ln_relu_bwd", ln_relu_bwd_kernel, dim3(blocks_for((int64_t)EW, 256)), dim3(256), 0, (int)EW,
+             (const float*)e_pre.ptr, e_d.ptr);
+      wgrad<kLnWidth, kLnWidth>(ctx, ne, e_in.ptr, kLnWidth, kLnWidth, nullptr, 0, e_d.ptr, kLnWidth, gpart.ptr,
+                                G + L.we1, G + L.be1);
+      // dx = dpre W_e1ᵀ into e_in (its forward value is no longer needed)
+      rows<kLnWidth, kLnWidth>(ctx, ne, e_d.ptr, kLnWidth, kLnWidth, nullptr, 0, P + L.we1, 1, kLnWidth, nullptr, 0,
+                               nullptr, e_in.ptr, kLnWidth);
+      launch(ctx, "ln_edge_gather", ln_edge_gather_kernel, dim3(blocks_for((int64_t)VW, 256)), dim3(256), 0, nv,
+             (const uint32_t*)nbr_ptr.ptr, (const uint32_t*)emap.ptr, (const float*)e_in.ptr, dh.ptr);
+    }
     for (int l = kLnLayers; l >= 1; --l) {
       float* dpre = dh.ptr;  // dh ⊙ (pre_l > 0), in place
       launch(ctx, "ln_relu_bwd", ln_relu_bwd_kernel, dim3(blocks_for((int64_t)VW, 256)), dim3(256), 0, (int)VW,
@@ -688,7 +781,8 @@ void LearnedModel::train(ddgrs_ctx* ctx, int iters, int batch, double lr, double
 
 void LearnedModel::apply(ddgrs_ctx* ctx, const double* r, double* z) {
   launch(ctx, "ln_apply", ln_apply_kernel, dim3(blocks_for(nv, 256)), dim3(256), 0, nv, (const uint32_t*)nbr_ptr.ptr,
-         (const uint32_t*)nbr.ptr, (const float*)out.ptr, (double)scale, r, z);
+         (const uint32_t*)nbr.ptr, (const uint32_t*)emap.ptr, (const float*)out.ptr, (const float*)e_out.ptr,
+         (double)scale, r, z);
 }
 
 }  // namespace ddgrs
@@ -834,7 +928,10 @@ int ddgrs_learned_blocks(ddgrs_learned* l, const double* positions, float* block
     HIPCHECK(hipSetDevice(ctx->device));
     l->m.features(ctx, positions, l->m.feat.ptr);
     l->m.forward(ctx, l->m.feat.ptr);
-    if (blocks) download(ctx, blocks, (const float*)l->m.out.ptr, (size_t)l->s->nv * kLnOut);
+    if (blocks) {
+      download(ctx, blocks, (const float*)l->m.out.ptr, (size_t)l->s->nv * kLnOut);
+      download(ctx, blocks + (size_t)l->s->nv * kLnOut, (const float*)l->m.e_out.ptr, (size_t)l->m.ne * kLnOut);
+    }
     ctx->sync();
   });
 }

@@ -370,7 +370,8 @@ DDGRS_API int ddgrs_learned_set_scale(ddgrs_learned* l, double scale, ddgrs_erro
 DDGRS_API int ddgrs_learned_set_krylov_pairs(ddgrs_learned* l, int32_t krylov_pairs, ddgrs_error* err);
 DDGRS_API int ddgrs_learned_add_pair(ddgrs_learned* l, const double* positions, const double* r, const double* pr,
                                      const double* y, ddgrs_error* err);
-/* The features (nv × 8, FP32) and the network's blocks (nv × 18: B_v row-major, then g_v). */
+/* The features (nv × 8, FP32) and the network's blocks: B_v (nv × 9, row-major), then E_uv (one row
+ * of 9 per undirected edge {u < v}, edges in ascending (u, v) order). */
 DDGRS_API int ddgrs_learned_features(ddgrs_learned* l, const double* positions, float* feat, ddgrs_error* err);
 DDGRS_API int ddgrs_learned_blocks(ddgrs_learned* l, const double* positions, float* blocks, ddgrs_error* err);
 /* The loss of one pair and (grad non-null) its gradient with respect to the parameters. */

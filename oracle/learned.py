@@ -10,14 +10,16 @@ library trains on the device, so that its features, forward pass, loss and gradi
   the vertex) along that normal over the mean squared edge length, times the RMS radius (a
   dimensionless mean curvature);
 * network: h0 = relu(F W0 + b0); four layers h <- relu([h, mean_N(h)] W_l + b_l) of width 64;
-  o = h Wo + bo with o_v = (B_v row-major, g_v row-major);
-* (Q x)_v = s (B_v x_v + sum_{u in N(v)} (g_v + g_u)/2 x_u), loss |s q - e|^2 / |e|^2 for a stored
-  pair (x, e) (the library stores x = c P^-1 r and e = y - x);
+  vertex head B_v = h_v Wv + bv (9, row-major); edge head per undirected edge {u < v} (ascending):
+  t = relu((h_u + h_v) We1 + be1) (64), E_uv = t We2 + be2 (9, row-major);
+* (Q x)_v = s (B_v x_v + sum_{u in N(v)} E_uv x_u), loss |s q - e|^2 / |e|^2 for a stored pair
+  (x, e) (the library stores x = c P^-1 r and e = y - x); ``forward`` returns the blocks as one
+  array: the nv vertex rows, then the edge rows;
 * ``bf16=True`` rounds both inputs of the products that the library runs on BF16 matrix cores
-  (every product with 64 output columns: the input and message layers, the back-propagated
-  activations through every layer, the weight and bias gradients of the 64-wide layers) to BF16,
-  nearest with ties to even, and accumulates exactly; the 18-output head and its weight gradient
-  stay unrounded (FP32 on the device).
+  (every product with 64 output columns: the input, message and edge layers, the back-propagated
+  activations through every layer and head, the weight and bias gradients of the 64-wide layers)
+  to BF16, nearest with ties to even, and accumulates exactly; the 9-output heads and their weight
+  gradients stay unrounded (FP32 on the device).
 
 Parity pins (``tests/test_oracle_learned.py``): the analytic gradient against central differences
 of the loss (every parameter group), the features against closed forms on a sphere (radial unit
@@ -32,7 +34,7 @@ from __future__ import annotations
 
 import numpy as np
 
-FEAT, WIDTH, LAYERS, OUT = 8, 64, 4, 18
+FEAT, WIDTH, LAYERS, OUT = 8, 64, 4, 9
 
 
 def bf16(x):
@@ -61,8 +63,12 @@ def layout():
     for l in range(LAYERS):
         take(f"w{l + 1}", 2 * WIDTH * WIDTH)
         take(f"b{l + 1}", WIDTH)
-    take("wo", WIDTH * OUT)
-    take("bo", OUT)
+    take("wv", WIDTH * OUT)
+    take("bv", OUT)
+    take("we1", WIDTH * WIDTH)
+    take("be1", WIDTH)
+    take("we2", WIDTH * OUT)
+    take("be2", OUT)
     lay["total"] = o
     return lay
 
@@ -71,7 +77,7 @@ def unpack(p):
     lay = layout()
     p = np.asarray(p, dtype=np.float64)
     out = {}
-    shapes = {"w0": (FEAT, WIDTH), "wo": (WIDTH, OUT)}
+    shapes = {"w0": (FEAT, WIDTH), "wv": (WIDTH, OUT), "we1": (WIDTH, WIDTH), "we2": (WIDTH, OUT)}
     for l in range(LAYERS):
         shapes[f"w{l + 1}"] = (2 * WIDTH, WIDTH)
     for k, v in lay.items():
@@ -96,6 +102,16 @@ def neighbours(faces, nv):
         nb.extend(s)
         ptr[v + 1] = len(nb)
     return ptr, np.asarray(nb, dtype=np.int64)
+
+
+def edges(ptr, nb):
+    """The undirected edges {u < v} in ascending order (ne × 2) and, per neighbour entry of the
+    CSR, its edge."""
+    nv = ptr.shape[0] - 1
+    src = np.repeat(np.arange(nv), np.diff(ptr))
+    e = np.stack([np.minimum(src, nb), np.maximum(src, nb)], axis=1)
+    uniq, emap = np.unique(e, axis=0, return_inverse=True)
+    return uniq, emap.reshape(-1)
 
 
 def features(pos, faces):
@@ -159,20 +175,25 @@ def forward(p, f, ptr, nb, keep=False, bf16=False):
         pre = _mm(x, P[f"w{l + 1}"], bf16) + P[f"b{l + 1}"]
         cache.append((x, m, pre))
         h = np.maximum(pre, 0.0)
-    o = h @ P["wo"] + P["bo"]
-    return (o, h, cache) if keep else o
+    B = h @ P["wv"] + P["bv"]
+    E, _ = edges(ptr, nb)
+    x = h[E[:, 0]] + h[E[:, 1]]
+    epre = _mm(x, P["we1"], bf16) + P["be1"]
+    t = np.maximum(epre, 0.0)
+    Eo = t @ P["we2"] + P["be2"]
+    o = np.concatenate([B, Eo])
+    return (o, h, cache, (x, epre, t)) if keep else o
 
 
 def q_apply(o, r3, ptr, nb):
-    """q_v = B_v r_v + sum_u (g_v + g_u)/2 r_u (without the scale)."""
-    nv = o.shape[0]
-    B = o[:, :9].reshape(nv, 3, 3)
-    G = o[:, 9:].reshape(nv, 3, 3)
-    deg = np.diff(ptr)
-    src = np.repeat(np.arange(nv), deg)
+    """q_v = B_v r_v + sum_u E_uv r_u (without the scale); o = the vertex rows, then the edge rows."""
+    nv = ptr.shape[0] - 1
+    B = o[:nv].reshape(nv, 3, 3)
+    Eo = o[nv:].reshape(-1, 3, 3)
+    _, emap = edges(ptr, nb)
+    src = np.repeat(np.arange(nv), np.diff(ptr))
     q = np.einsum("vab,vb->va", B, r3)
-    edge = 0.5 * (G[src] + G[nb])
-    np.add.at(q, src, np.einsum("kab,kb->ka", edge, r3[nb]))
+    np.add.at(q, src, np.einsum("kab,kb->ka", Eo[emap], r3[nb]))
     return q
 
 
@@ -182,22 +203,25 @@ def loss_grad(p, f, ptr, nb, r, e, s, bf16=False):
     r3 = np.asarray(r, dtype=np.float64)[:3 * nv].reshape(nv, 3)
     e3 = np.asarray(e, dtype=np.float64)[:3 * nv].reshape(nv, 3)
     e2 = (e3 ** 2).sum()
-    o, h4, cache = forward(p, f, ptr, nb, keep=True, bf16=bf16)
+    o, h4, cache, (ex, epre, et) = forward(p, f, ptr, nb, keep=True, bf16=bf16)
     rb = globals()["bf16"] if bf16 else (lambda a: a)
     q = s * q_apply(o, r3, ptr, nb)
     d = q - e3
     L = (d ** 2).sum() / e2
     qd = 2.0 * d / e2
-    deg = np.diff(ptr)
-    src = np.repeat(np.arange(nv), deg)
-    dout = np.zeros((nv, OUT))
-    dout[:, :9] = s * np.einsum("va,vb->vab", qd, r3).reshape(nv, 9)
-    gsum = np.zeros((nv, 3, 3))
-    np.add.at(gsum, src, np.einsum("ka,kb->kab", qd[src], r3[nb]) + np.einsum("ka,kb->kab", qd[nb], r3[src]))
-    dout[:, 9:] = 0.5 * s * gsum.reshape(nv, 9)
+    E, _ = edges(ptr, nb)
+    u, v = E[:, 0], E[:, 1]
+    dB = s * np.einsum("va,vb->vab", qd, r3).reshape(nv, 9)
+    dE = s * (np.einsum("ka,kb->kab", qd[u], r3[v]) + np.einsum("ka,kb->kab", qd[v], r3[u])).reshape(-1, 9)
     P = unpack(p)
-    grads = {"wo": h4.T @ dout, "bo": dout.sum(axis=0)}
-    dh = _mm(dout, P["wo"].T, bf16)
+    grads = {"wv": h4.T @ dB, "bv": dB.sum(axis=0), "we2": et.T @ dE, "be2": dE.sum(axis=0)}
+    dh = _mm(dB, P["wv"].T, bf16)
+    dpe = _mm(dE, P["we2"].T, bf16) * (epre > 0)
+    grads["we1"] = _mm(ex.T, dpe, bf16)
+    grads["be1"] = rb(dpe).sum(axis=0)
+    dx = _mm(dpe, P["we1"].T, bf16)
+    np.add.at(dh, u, dx)
+    np.add.at(dh, v, dx)
     for l in range(LAYERS, 0, -1):
         x, _, pre = cache[l]
         dpre = dh * (pre > 0)

@@ -62,22 +62,29 @@ def test_features_on_a_sphere():
 
 
 def test_q_is_linear_and_symmetric_with_symmetric_vertex_blocks():
+    """Q is linear; with symmetric vertex and edge blocks it is symmetric (each edge block serves
+    both directions of its edge); a non-symmetric edge block breaks it."""
     pos, faces = _mesh(1)
     nv = pos.shape[0]
     ptr, nb = LN.neighbours(faces, nv)
+    E, emap = LN.edges(ptr, nb)
+    assert E.shape[0] == len(nb) // 2 and np.all(E[:, 0] < E[:, 1])
     rng = np.random.default_rng(3)
-    o = rng.standard_normal((nv, 18))
-    B = o[:, :9].reshape(nv, 3, 3)
-    o[:, :9] = (B + B.transpose(0, 2, 1)).reshape(nv, 9) / 2
-    o[:, 9:] = ((o[:, 9:].reshape(nv, 3, 3) + o[:, 9:].reshape(nv, 3, 3).transpose(0, 2, 1)) / 2).reshape(nv, 9)
+    o = rng.standard_normal((nv + E.shape[0], 9))
+    M = o.reshape(-1, 3, 3)
+    o = ((M + M.transpose(0, 2, 1)) / 2).reshape(-1, 9)
     x, y = rng.standard_normal((nv, 3)), rng.standard_normal((nv, 3))
     qx, qy = LN.q_apply(o, x, ptr, nb), LN.q_apply(o, y, ptr, nb)
     assert np.allclose(LN.q_apply(o, 2 * x - y, ptr, nb), 2 * qx - qy)
     assert abs((qx * y).sum() - (x * qy).sum()) < 1e-10 * np.abs(qx).sum()
-    # at Q = 0 (the output layer zero) the loss is 1
+    o2 = o.copy()
+    o2[nv + 5, 1] += 1.0
+    q2x, q2y = LN.q_apply(o2, x, ptr, nb), LN.q_apply(o2, y, ptr, nb)
+    assert abs((q2x * y).sum() - (x * q2y).sum()) > 1e-3
+    # at Q = 0 (the output layers zero) the loss is 1
     p = _random_params(2)
     lay = LN.layout()
-    for k in ("wo", "bo"):
+    for k in ("wv", "bv", "we2", "be2"):
         o_, c = lay[k]
         p[o_:o_ + c] = 0.0
     f = LN.features(pos, faces)
