@@ -161,6 +161,7 @@ SIGNATURES = {
     "ddgrs_session_fallbacks": (_I32, [_P, ctypes.c_char_p, _I64, ctypes.POINTER(_I64), _E]),
     "ddgrs_session_gmres_history": (_I32, [_P, _P, _I64, _P, _I64, _P, _E]),
     "ddgrs_session_constraints": (_I32, [_P, _P, _P, _I64, ctypes.POINTER(_I64), _E]),
+    "ddgrs_schur": (_I32, [_I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _E]),
     "ddgrs_saddle_gmres_history": (_I32, [_P, _P, _I64, _P, _I64, _P, _E]),
     "ddgrs_learned_create": (_I32, [_P, ctypes.c_uint64, _I64, ctypes.POINTER(_P), _E]),
     "ddgrs_learned_destroy": (None, [_P]),
@@ -680,6 +681,20 @@ class Saddle(_Handle):
         _call(lib().ddgrs_saddle_solve, self._h, _ptr(bb), _ptr(x), options.restart, options.tolerance,
               options.max_iterations, ctypes.byref(rep))
         return x, _gmres_report(rep, _gmres_history(lib().ddgrs_saddle_gmres_history, self._h))
+
+
+def schur(c, z, y, phi):
+    """The step's Schur complement on the host (``SchurComplement``, src/solve/schur.rs:63): from the
+    rows c (k × nd) and the columns z = A′⁻¹ C_Nᵀ (k × nd), (x = y − Z N⁻¹ C_N y, h = Z N⁻¹ Φ_N,
+    S = −C_N Z, the smallest kept pivot)."""
+    c = np.ascontiguousarray(c, dtype=np.float64)
+    z = np.ascontiguousarray(z, dtype=np.float64)
+    k, nd = c.shape
+    y = _f64(y, (nd,))
+    phi = _f64(phi, (k,))
+    x, h, S, mp = np.zeros(nd), np.zeros(nd), np.zeros((k, k)), ctypes.c_double()
+    _call(lib().ddgrs_schur, k, nd, _ptr(c), _ptr(z), _ptr(y), _ptr(phi), _ptr(x), _ptr(h), _ptr(S), ctypes.byref(mp))
+    return x, h, S, float(mp.value)
 
 
 class StartRule:

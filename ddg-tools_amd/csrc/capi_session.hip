@@ -317,6 +317,33 @@ int ddgrs_session_gmres_history(ddgrs_session* h, double* cycle_residuals, int64
   });
 }
 
+int ddgrs_schur(int64_t k, int64_t nd, const double* c, const double* z, const double* y, const double* phi,
+                double* x, double* h, double* s, double* min_pivot, ddgrs_error* err) {
+  return guarded(err, "schur", [&] {
+    if (k < 0 || k > 64) invalid("k", std::to_string(k), "0 to 64 rows");
+    if (nd < 0) invalid("nd", std::to_string(nd), "non-negative");
+    need(c, "c");
+    need(z, "z");
+    need(y, "y");
+    need(phi, "phi");
+    need(x, "x");
+    need(h, "h");
+    std::vector<std::vector<double>> cr((size_t)k), zc((size_t)k);
+    for (int64_t r = 0; r < k; ++r) {
+      cr[r].assign(c + r * nd, c + (r + 1) * nd);
+      zc[r].assign(z + r * nd, z + (r + 1) * nd);
+    }
+    SchurRows S;
+    S.build(std::move(cr), std::move(zc));
+    const auto xv = S.direction(std::vector<double>(y, y + nd));
+    const auto hv = S.correction(std::vector<double>(phi, phi + k));
+    std::copy(xv.begin(), xv.end(), x);
+    std::copy(hv.begin(), hv.end(), h);
+    if (s) std::copy(S.s.begin(), S.s.end(), s);
+    if (min_pivot) *min_pivot = S.min_pivot;
+  });
+}
+
 int ddgrs_session_constraints(ddgrs_session* h, double* phi_before, double* phi, int64_t capacity, int64_t* nrows,
                               ddgrs_error* err) {
   return guarded(err, "session_constraints", [&] {

@@ -325,6 +325,14 @@ DDGRS_API int ddgrs_session_fallbacks(ddgrs_session* h, char* text, int64_t capa
 DDGRS_API int ddgrs_session_constraints(ddgrs_session* h, double* phi_before, double* phi, int64_t capacity,
                                         int64_t* nrows, ddgrs_error* err);
 
+/* The Schur complement of k nonlinear rows on the host (RS/src/solve/schur.rs:63 SchurComplement:
+ * from_solutions :142, factor :185, direction_from :318, correction :341), the step's own code: from
+ * the rows c (k × nd, row-major) and the columns z = A′⁻¹ C_Nᵀ (k × nd, position blocks), the
+ * direction x = y − Z N⁻¹ C_N y, the correction h = Z N⁻¹ Φ_N (phi: k values), S = −C_N Z (k × k,
+ * as computed) and the smallest kept equilibrated pivot.  Needs no device. */
+DDGRS_API int ddgrs_schur(int64_t k, int64_t nd, const double* c, const double* z, const double* y, const double* phi,
+                          double* x, double* h, double* s, double* min_pivot, ddgrs_error* err);
+
 /* The trials of the last step in order (StepReport::trials, RS/src/step/session.rs:136 Trial): τ,
  * F (NaN when not finite) and the outcome (TrialOutcome: 0 accepted, 1 Armijo failed, 2 non-finite);
  * at most `capacity` are written, *count is the number of trials. */

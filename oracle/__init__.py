@@ -105,7 +105,7 @@ def _declare(L):
     L.orc_vfrac_free.argtypes = [_P]
     L.orc_vfrac_apply.argtypes = [_P, _I64, _P, _P, _INT]
     L.orc_vertex_areas.argtypes = [_I64, _I64, _P, _P, _P]
-    L.orc_saddle_build.argtypes = [_I64, _I64, _P, _P, _D, _D, _D, _I64, _I64, _D, _INT, ctypes.POINTER(_INT)]
+    L.orc_saddle_build.argtypes = [_I64, _I64, _P, _P, _D, _D, _D, _I64, _I64, _D, _D, _INT, ctypes.POINTER(_INT)]
     L.orc_saddle_build.restype = _P
     L.orc_saddle_free.argtypes = [_P]
     L.orc_saddle_eps.argtypes = [_P]
@@ -128,6 +128,7 @@ def _declare(L):
     L.orc_session_positions.argtypes = [_P, _P]
     L.orc_session_constraints.argtypes = [_P, _P, _I64]
     L.orc_constraint_rows.argtypes = [_I64, _I64, _P, _P, _P, _I64, _P, _P]
+    L.orc_schur.argtypes = [_I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P]
     L.orc_fractional_symbol_ratio.argtypes = [_D]
     L.orc_fractional_symbol_ratio.restype = _D
 
@@ -326,13 +327,13 @@ class Saddle:
     barycenter row (solve/iterative.rs:126, solve/precond.rs:645)."""
 
     def __init__(self, faces, positions, alpha=6.0, beta=12.0, chi=0.5, leaf_size=8, leaf_pairs=64, eps_rel=1e-10,
-                 threads=0):
+                 threads=0, c_lo=1.0):
         f, p = _mesh(faces, positions)
         self.nv, self.threads = p.shape[0], threads
         self.n = 3 * self.nv + 3
         st = _INT(0)
         self._h = lib().orc_saddle_build(self.nv, f.shape[0], _ptr(f), _ptr(p), alpha, beta, chi, leaf_size, leaf_pairs,
-                                         eps_rel, threads, ctypes.byref(st))
+                                         eps_rel, c_lo, threads, ctypes.byref(st))
         if not self._h:
             raise OracleError(st.value, "saddle build")
 
@@ -464,6 +465,19 @@ def constraint_rows(faces, positions, kinds):
     vals, grads = np.zeros(len(k)), np.zeros((len(k), 3 * p.shape[0]))
     lib().orc_constraint_rows(p.shape[0], f.shape[0], _ptr(f), _ptr(p), _ptr(k), len(k), _ptr(vals), _ptr(grads))
     return vals, grads
+
+
+def schur(c, z, y, phi):
+    """(x = y − Z N⁻¹ C_N y, h = Z N⁻¹ Φ_N, S = −C_N Z, smallest kept pivot) for the rows c (k × nd)
+    and the columns z = A′⁻¹ C_Nᵀ (k × nd, position blocks) (solve/schur.rs)."""
+    c = np.ascontiguousarray(c, dtype=np.float64)
+    z = np.ascontiguousarray(z, dtype=np.float64)
+    k, nd = c.shape
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    phi = np.ascontiguousarray(phi, dtype=np.float64)
+    x, h, S, mp = np.zeros(nd), np.zeros(nd), np.zeros((k, k)), ctypes.c_double()
+    lib().orc_schur(k, nd, _ptr(c), _ptr(z), _ptr(y), _ptr(phi), _ptr(x), _ptr(h), _ptr(S), ctypes.byref(mp))
+    return x, h, S, mp.value
 
 
 def fractional_symbol_ratio(s):

@@ -272,7 +272,8 @@ void orc_vertex_areas(int64_t nv, int64_t nf, const uint32_t* faces, const doubl
 // ---- saddle solver, preconditioner, GMRES, descent step ----
 
 void* orc_saddle_build(int64_t nv, int64_t nf, const uint32_t* faces, const double* pos, double alpha, double beta,
-                       double chi, int64_t leaf_size, int64_t leaf_pairs, double eps_rel, int threads, int* status) {
+                       double chi, int64_t leaf_size, int64_t leaf_pairs, double eps_rel, double c_lo, int threads,
+                       int* status) {
   Error err;
   Exponents e;
   if (!make_exps(alpha, beta, &e, &err)) {
@@ -284,6 +285,7 @@ void* orc_saddle_build(int64_t nv, int64_t nf, const uint32_t* faces, const doub
   o.leaf_size = (size_t)leaf_size;
   o.leaf_pairs = (size_t)leaf_pairs;
   o.eps_rel = eps_rel;
+  o.c_lo = c_lo;
   auto* sd = new Saddle();
   if (!saddle_build(make_mesh(nv, nf, faces), make_pos(nv, pos), e, o, threads, sd, &err)) {
     delete sd;
@@ -438,6 +440,25 @@ void orc_session_constraints(void* h, double* out, int64_t cap) {
   v.insert(v.end(), s->last_phi_before.begin(), s->last_phi_before.end());
   v.insert(v.end(), s->last_phi.begin(), s->last_phi.end());
   for (size_t i = 0; i < v.size() && (int64_t)i < cap; ++i) out[i] = v[i];
+}
+
+// The Schur complement of k rows (c: k dense rows of nd, z: the k columns A′⁻¹ C_Nᵀ, position
+// blocks): x = y − Z N⁻¹ C_N y, h = Z N⁻¹ Φ_N, S (k × k as computed), the smallest kept pivot.
+void orc_schur(int64_t k, int64_t nd, const double* c, const double* z, const double* y, const double* phi,
+               double* x_out, double* h_out, double* s_out, double* min_pivot) {
+  std::vector<std::vector<double>> cr(k), zc(k);
+  for (int64_t r = 0; r < k; ++r) {
+    cr[r].assign(c + r * nd, c + (r + 1) * nd);
+    zc[r].assign(z + r * nd, z + (r + 1) * nd);
+  }
+  Schur S;
+  schur_build(&S, cr, zc);
+  const auto x = schur_direction(S, std::vector<double>(y, y + nd));
+  const auto h = schur_correction(S, std::vector<double>(phi, phi + k));
+  std::copy(x.begin(), x.end(), x_out);
+  std::copy(h.begin(), h.end(), h_out);
+  std::copy(S.s.begin(), S.s.end(), s_out);
+  *min_pivot = S.min_pivot;
 }
 
 // Values (rows of kind 1 area, 2 volume) and their gradients (3 nv each) at positions.

@@ -94,7 +94,7 @@ def read_out(name: str) -> dict:
             o["result"] = (t[1], hexf(t[2]), int(t[3]))
         elif k == "newton":
             o["newton"] = int(t[1])
-        elif k in ("phi", "phi-before"):
+        elif k in ("phi", "phi-before", "phin"):
             o[k] = [hexf(x) for x in t[1:]]
         elif k == "df":
             o.setdefault("df", []).append((int(t[1]), int(t[2]), [hexf(x) for x in t[3:]]))

@@ -133,3 +133,49 @@ def test_lean_constrained_step_catches_a_missing_newton_projection():
     rep = constrained_session(c).step(1)
     assert rep["newton_rounds"] == 0
     assert np.max(np.abs(np.array(rep["phi"][3:]) - np.array(o["phi"][3:]))) > 1e-4
+
+
+@pytest.mark.parametrize("name", ["rs-icosa-saddle", "rs-mobius-saddle", "rs-torus-saddle"])
+def test_lean_saddle_with_schur_rows(name):
+    """The reference's saddle fixtures with nonlinear rows (RS/tests/oracle.rs:377-419): with the
+    exact metric's FP64 solves z_r = A′⁻¹ [C_N row r; 0] and y = A′⁻¹ [g; 0], the oracle's Schur
+    complement and the step's own (``ddgrs_schur``, host code) give the fixture's S, the direction
+    x, ⟨g, x⟩ and the Newton correction h to SOLVE = 1e-10."""
+    from conftest import load_package
+    pkg = load_package()
+    c, o = fx.read_case(name), fx.read_out(name)
+    p, f = c["positions"], c["faces"]
+    nv = p.shape[0]
+    rows, bary = _targets(c)
+    assert bary and rows and all(t is not None for _, t in rows)
+    w = c["weights"]
+    sd = oracle.Saddle(f, p, *c["exponents"], chi=0.0, c_lo=1.0 if w[1] != 0.0 else 0.0)
+    vals, C = oracle.constraint_rows(f, p, [k for k, _ in rows])
+    phin = vals - np.array([t for _, t in rows])
+    assert np.max(np.abs(phin - np.array(o["phin"]))) <= 1e-10
+    _, g, _ = oracle.exact(f, p, *c["exponents"])
+    g = g.reshape(-1)
+    want_g = np.array([o["rows"]["g"][i] for i in range(nv)]).reshape(-1)
+    assert np.abs(g - want_g).max() <= 1e-12 * np.abs(want_g).max()
+
+    def solve(b):
+        rhs = np.zeros(sd.n)
+        rhs[: 3 * nv] = b
+        x, rep = sd.solve(rhs, tol=1e-13, maxit=500)
+        assert rep["converged"] == 1.0
+        return x[: 3 * nv]
+
+    Z = np.stack([solve(C[r]) for r in range(C.shape[0])])
+    y = solve(g)
+    want_x = np.array([o["rows"]["x"][i] for i in range(nv)]).reshape(-1)
+    want_h = np.array([o["rows"]["h"][i] for i in range(nv)]).reshape(-1)
+    want_S = fx.matrix(o, "S")
+    for label, impl in (("oracle", oracle.schur), ("library", pkg.schur)):
+        x, h, S, pivot = impl(C, Z, y, phin)
+        print(name, label, "S", S.tolist(), "pivot", pivot)
+        assert np.abs(S - want_S).max() <= 1e-10 * np.abs(want_S).max(), label
+        assert np.abs(x - want_x).max() <= 1e-10 * np.abs(want_x).max(), label
+        assert abs(g @ x - o["gx"]) <= 1e-10 * np.linalg.norm(g) * np.linalg.norm(x), label
+        assert np.abs(h - want_h).max() <= 1e-10 * np.abs(want_h).max(), label
+        # C_N x = 0 on the kept rows (the fixture's cnx is round-off)
+        assert np.abs(C @ x).max() <= 1e-10 * np.abs(C).max() * np.abs(x).max()
