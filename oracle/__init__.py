@@ -105,7 +105,8 @@ def _declare(L):
     L.orc_vfrac_free.argtypes = [_P]
     L.orc_vfrac_apply.argtypes = [_P, _I64, _P, _P, _INT]
     L.orc_vertex_areas.argtypes = [_I64, _I64, _P, _P, _P]
-    L.orc_saddle_build.argtypes = [_I64, _I64, _P, _P, _D, _D, _D, _I64, _I64, _D, _D, _INT, ctypes.POINTER(_INT)]
+    L.orc_saddle_build.argtypes = [_I64, _I64, _P, _P, _D, _D, _D, _I64, _I64, _D, _D, _INT, ctypes.POINTER(_INT),
+                                   _I64, _I64, _I64]
     L.orc_saddle_build.restype = _P
     L.orc_saddle_free.argtypes = [_P]
     L.orc_saddle_eps.argtypes = [_P]
@@ -326,14 +327,19 @@ class Saddle:
     """The oracle's hierarchical saddle A′ with the augmented vertex-pair preconditioner and a global
     barycenter row (solve/iterative.rs:126, solve/precond.rs:645)."""
 
+    FORMS = {"augmented": 0, "congruence": 1}
+    SCALES = {"symbol": 0, "unit": 1}
+
     def __init__(self, faces, positions, alpha=6.0, beta=12.0, chi=0.5, leaf_size=8, leaf_pairs=64, eps_rel=1e-10,
-                 threads=0, c_lo=1.0):
+                 threads=0, c_lo=1.0, form="augmented", scale="symbol", refine=False):
+        """``form``, ``scale`` and ``refine`` are PreconditionerOptions' (solve/precond.rs:466)."""
         f, p = _mesh(faces, positions)
         self.nv, self.threads = p.shape[0], threads
         self.n = 3 * self.nv + 3
         st = _INT(0)
         self._h = lib().orc_saddle_build(self.nv, f.shape[0], _ptr(f), _ptr(p), alpha, beta, chi, leaf_size, leaf_pairs,
-                                         eps_rel, c_lo, threads, ctypes.byref(st))
+                                         eps_rel, c_lo, threads, ctypes.byref(st), self.FORMS[form], self.SCALES[scale],
+                                         1 if refine else 0)
         if not self._h:
             raise OracleError(st.value, "saddle build")
 

@@ -273,7 +273,7 @@ void orc_vertex_areas(int64_t nv, int64_t nf, const uint32_t* faces, const doubl
 
 void* orc_saddle_build(int64_t nv, int64_t nf, const uint32_t* faces, const double* pos, double alpha, double beta,
                        double chi, int64_t leaf_size, int64_t leaf_pairs, double eps_rel, double c_lo, int threads,
-                       int* status) {
+                       int* status, int64_t form, int64_t scale, int64_t refine) {
   Error err;
   Exponents e;
   if (!make_exps(alpha, beta, &e, &err)) {
@@ -286,6 +286,9 @@ void* orc_saddle_build(int64_t nv, int64_t nf, const uint32_t* faces, const doub
   o.leaf_pairs = (size_t)leaf_pairs;
   o.eps_rel = eps_rel;
   o.c_lo = c_lo;
+  o.form = (int)form;
+  o.scale = (int)scale;
+  o.refine = refine != 0;
   auto* sd = new Saddle();
   if (!saddle_build(make_mesh(nv, nf, faces), make_pos(nv, pos), e, o, threads, sd, &err)) {
     delete sd;

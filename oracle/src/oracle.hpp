@@ -244,16 +244,22 @@ bool ldlt_factor(const Csr& a, const std::vector<uint32_t>& perm, Ldlt* f, Error
 void ldlt_solve(const Ldlt& f, const double* b, double* x);
 double fractional_symbol_ratio(double s);
 
+// precond.rs:466 PreconditionerOptions: form (0 Augmented, 1 Congruence), scale (0 Symbol, 1 Unit),
+// refine (one step of iterative refinement in every L′ solve of an application).
 struct SaddleOptions {
   double chi = 0.5;
   size_t leaf_size = 8, leaf_pairs = 64;
   double w_tp = 1.0, c_lo = 1.0;
   double eps_rel = 1e-10;
+  int form = 0, scale = 0;
+  bool refine = false;
 };
 
 struct Saddle {
   size_t nv = 0;
   double w_tp = 1.0, c_lo = 1.0, eps = 0.0, frac_scale = 1.0;
+  int form = 0;
+  bool refine = false;
   FaceMetric metric;
   std::vector<double> mass, crow;
   Csr k, lprime_matrix;

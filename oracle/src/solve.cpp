@@ -274,7 +274,9 @@ bool saddle_build(const Mesh& m, const Positions& p, const Exponents& e, const S
   const double s = e.s();
   if (!vertex_fractional_build(p, out->mass, 2.0 - s, o.chi, o.leaf_size, o.leaf_pairs, threads, &out->frac, err))
     return false;
-  out->frac_scale = 1.0 / fractional_symbol_ratio(s) / o.w_tp;
+  out->frac_scale = o.scale == 1 ? 1.0 : 1.0 / fractional_symbol_ratio(s) / o.w_tp;
+  out->form = o.form;
+  out->refine = o.refine;
   return true;
 }
 
@@ -306,22 +308,48 @@ static void lprime_solve(const Saddle& sd, const double* r, double* x) {
   }
 }
 
-// precond.rs:1026 apply_fine, augmented form: z = L′⁻¹ v + y, y = L′⁻¹ r,
-// v = s L^V y − (K ⊗ I₃) y on the positions, 0 on the multipliers.
+// L′⁻¹ r with one step of iterative refinement when the saddle asks for it (precond.rs:1031
+// solve_many_on(.., refine)): x ← x + L′⁻¹ (r − L′ x), L′ the assembled scalar saddle per axis.
+static void lprime_solve_opt(const Saddle& sd, const double* r, double* x) {
+  lprime_solve(sd, r, x);
+  if (!sd.refine) return;
+  const size_t nv = sd.nv, nd = 3 * nv, n = nd + 3;
+  std::vector<double> res(n), dx(n);
+  const Csr& L = sd.lprime_matrix;
+  for (int a = 0; a < 3; ++a)
+    for (size_t i = 0; i <= nv; ++i) {
+      double acc = 0.0;
+      for (size_t q = L.ptr[i]; q < L.ptr[i + 1]; ++q) {
+        const size_t j = L.idx[q];
+        acc += L.val[q] * (j < nv ? x[3 * j + a] : x[nd + a]);
+      }
+      const size_t t = i < nv ? 3 * i + a : nd + a;
+      res[t] = r[t] - acc;
+    }
+  lprime_solve(sd, res.data(), dx.data());
+  for (size_t i = 0; i < n; ++i) x[i] += dx[i];
+}
+
+// precond.rs:1026 apply_fine: y = L′⁻¹ r, v = L^V y on the positions and 0 on the multipliers;
+// augmented form (default): v ← s v − (K ⊗ I₃) y, z = L′⁻¹ v + y; congruence form: z = L′⁻¹ v.
 void precond_apply(const Saddle& sd, const double* r, double* z, int threads) {
   const size_t nv = sd.nv, nd = 3 * nv, n = nd + 3;
   std::vector<double> y(n), v(n, 0.0);
-  lprime_solve(sd, r, y.data());
+  lprime_solve_opt(sd, r, y.data());
   vertex_fractional_apply(sd.frac, 1, y.data(), v.data(), threads);
-  for (size_t i = 0; i < nd; ++i) v[i] *= sd.frac_scale;
-  for (size_t j = 0; j < nv; ++j) {
-    double acc[3] = {0.0, 0.0, 0.0};
-    for (size_t q = sd.k.ptr[j]; q < sd.k.ptr[j + 1]; ++q)
-      for (int a = 0; a < 3; ++a) acc[a] += sd.k.val[q] * y[3 * (size_t)sd.k.idx[q] + a];
-    for (int a = 0; a < 3; ++a) v[3 * j + a] -= acc[a];
+  if (sd.form == 0) {
+    if (sd.frac_scale != 1.0)
+      for (size_t i = 0; i < nd; ++i) v[i] *= sd.frac_scale;
+    for (size_t j = 0; j < nv; ++j) {
+      double acc[3] = {0.0, 0.0, 0.0};
+      for (size_t q = sd.k.ptr[j]; q < sd.k.ptr[j + 1]; ++q)
+        for (int a = 0; a < 3; ++a) acc[a] += sd.k.val[q] * y[3 * (size_t)sd.k.idx[q] + a];
+      for (int a = 0; a < 3; ++a) v[3 * j + a] -= acc[a];
+    }
   }
-  lprime_solve(sd, v.data(), z);
-  for (size_t i = 0; i < n; ++i) z[i] += y[i];
+  lprime_solve_opt(sd, v.data(), z);
+  if (sd.form == 0)
+    for (size_t i = 0; i < n; ++i) z[i] += y[i];
 }
 
 namespace {

@@ -223,3 +223,30 @@ def test_lean_step_stuck_catches_a_wrong_halving_cap():
     c["maxhalv"] = c["maxhalv"] + 1
     rep = step_stuck_session(c).step(1)
     assert len(rep["trial_list"]) == 5
+
+
+def test_preconditioner_variants(small):
+    """PreconditionerOptions (solve/precond.rs:466-582): with y = L′⁻¹ r and v = [L^V y; 0],
+    augmented P⁻¹_s r = L′⁻¹ ((s L^V − K) y) + y and congruence P⁻¹ r = L′⁻¹ v, so
+    P⁻¹_symbol − P⁻¹_unit = (s − 1) P⁻¹_congruence (s the symbol scale); the congruence form is
+    symmetric and its range meets the barycenter rows (C₁ z = 0, precond.rs:586-594); one step of
+    refinement in the L′ solves changes the application by round-off only."""
+    p, f = small
+    sym = oracle.Saddle(f, p, chi=0.5)
+    unit = oracle.Saddle(f, p, chi=0.5, scale="unit")
+    cong = oracle.Saddle(f, p, chi=0.5, form="congruence")
+    refd = oracle.Saddle(f, p, chi=0.5, refine=True)
+    rng = np.random.default_rng(4)
+    r, u = rng.standard_normal(sym.n), rng.standard_normal(sym.n)
+    zs, zu, zc = sym.precond(r), unit.precond(r), cong.precond(r)
+    scale = oracle.lib().orc_saddle_frac_scale(sym._h)
+    assert scale != 1.0 and oracle.lib().orc_saddle_frac_scale(unit._h) == 1.0
+    assert np.abs((zs - zu) - (scale - 1.0) * zc).max() <= 1e-9 * np.abs(zs - zu).max()
+    assert abs(u @ cong.precond(r) - r @ cong.precond(u)) <= 1e-9 * np.abs(u) @ np.abs(zc)
+    nv = sym.nv
+    va = oracle.vertex_areas(f, p)
+    for k in range(3):
+        assert abs((va / va.sum()) @ zc[k: 3 * nv: 3]) <= 1e-9 * np.abs(zc).max()
+    zr = refd.precond(r)
+    assert not np.array_equal(zr, zs)
+    assert np.abs(zr - zs).max() <= 1e-6 * np.abs(zs).max()
