@@ -152,6 +152,7 @@ SIGNATURES = {
     "ddgrs_saddle_precond_batch": (_I32, [_P, _I64, _P, _P, _E]),
     "ddgrs_saddle_lprime_batch": (_I32, [_P, _I64, _P, _P, _E]),
     "ddgrs_saddle_solve": (_I32, [_P, _P, _P, _I64, _D, _I64, _P, _E]),
+    "ddgrs_saddle_solve_many": (_I32, [_P, _I64, _P, _P, _I64, _D, _I64, _P, _E]),
     "ddgrs_step_options_default": (None, [_P]),
     "ddgrs_session_create": (_I32, [_P, _P, ctypes.POINTER(_P), _E]),
     "ddgrs_session_destroy": (None, [_P]),
@@ -513,7 +514,17 @@ class VertexFractionalTree(_Handle):
 
 
 class _SaddleOptionsC(ctypes.Structure):
-    _fields_ = [("chi", _D), ("leaf_size", _I64), ("leaf_pairs", _I64), ("w_tp", _D), ("c_lo", _D), ("eps_rel", _D)]
+    _fields_ = [
+        ("chi", _D),
+        ("leaf_size", _I64),
+        ("leaf_pairs", _I64),
+        ("w_tp", _D),
+        ("c_lo", _D),
+        ("eps_rel", _D),
+        ("form", _I32),
+        ("scale", _I32),
+        ("refine", _I32),
+    ]
 
 
 class _GmresReportC(ctypes.Structure):
@@ -600,9 +611,20 @@ class SaddleOptions:
     w_tp: float = 1.0
     c_lo: float = 1.0
     eps_rel: float = 1e-10
+    # PreconditionerOptions::form ("augmented" or "congruence"), ::scale of the fractional factor
+    # ("symbol" or "unit") and ::refine (one step of iterative refinement per L' solve)
+    form: str = "augmented"
+    scale: str = "symbol"
+    refine: bool = False
 
     def _c(self) -> _SaddleOptionsC:
-        return _SaddleOptionsC(self.chi, self.leaf_size, self.leaf_pairs, self.w_tp, self.c_lo, self.eps_rel)
+        forms, scales = {"augmented": 0, "congruence": 1}, {"symbol": 0, "unit": 1}
+        if self.form not in forms:
+            raise RsError(1, "options", f"form {self.form!r}: 'augmented' or 'congruence'")
+        if self.scale not in scales:
+            raise RsError(1, "options", f"scale {self.scale!r}: 'symbol' or 'unit'")
+        return _SaddleOptionsC(self.chi, self.leaf_size, self.leaf_pairs, self.w_tp, self.c_lo, self.eps_rel,
+                               forms[self.form], scales[self.scale], int(bool(self.refine)))
 
 
 @dataclasses.dataclass(frozen=True)
@@ -681,6 +703,20 @@ class Saddle(_Handle):
         _call(lib().ddgrs_saddle_solve, self._h, _ptr(bb), _ptr(x), options.restart, options.tolerance,
               options.max_iterations, ctypes.byref(rep))
         return x, _gmres_report(rep, _gmres_history(lib().ddgrs_saddle_gmres_history, self._h))
+
+    def solve_many(self, bs, options: GmresOptions = GmresOptions()):
+        """``GmresSaddle::solve_many`` (src/solve/gmres.rs:801): x_i = A′⁻¹ b_i for the rows of bs
+        (nb × (3 nv + 3)) and one ``GmresReport`` per row (histories left empty; the history
+        entry point holds the last solve's)."""
+        bb = np.ascontiguousarray(bs, dtype=np.float64)
+        if bb.ndim != 2 or bb.shape[1] != self.n:
+            raise RsError(1, "saddle_solve_many", f"right-hand sides of shape {bb.shape}, want (nb, {self.n})")
+        nb = bb.shape[0]
+        xs = np.zeros((nb, self.n))
+        reps = (_GmresReportC * max(nb, 1))()
+        _call(lib().ddgrs_saddle_solve_many, self._h, nb, _ptr(bb), _ptr(xs), options.restart, options.tolerance,
+              options.max_iterations, reps)
+        return xs, [_gmres_report(reps[i]) for i in range(nb)]
 
 
 def schur(c, z, y, phi):

@@ -30,6 +30,9 @@ void check_saddle_options(const ddgrs_saddle_options& o) {
   if (!(std::isfinite(o.w_tp) && o.w_tp > 0.0)) invalid("w_tp", std::to_string(o.w_tp), "a finite weight > 0");
   if (!(std::isfinite(o.c_lo) && o.c_lo >= 0.0)) invalid("c_lo", std::to_string(o.c_lo), "a finite weight >= 0");
   if (!(std::isfinite(o.eps_rel) && o.eps_rel > 0.0)) invalid("eps_rel", std::to_string(o.eps_rel), "a finite number > 0");
+  if (o.form != 0 && o.form != 1) invalid("form", std::to_string(o.form), "0 (augmented) or 1 (congruence)");
+  if (o.scale != 0 && o.scale != 1) invalid("scale", std::to_string(o.scale), "0 (symbol) or 1 (unit)");
+  if (o.refine != 0 && o.refine != 1) invalid("refine", std::to_string(o.refine), "0 or 1");
 }
 
 // gmres.rs:168 GmresOptions::validate
@@ -79,6 +82,9 @@ void ddgrs_saddle_options_default(ddgrs_saddle_options* o) {
   o->w_tp = 1.0;
   o->c_lo = 1.0;
   o->eps_rel = 1e-10;
+  o->form = 0;
+  o->scale = 0;
+  o->refine = 0;
 }
 
 int ddgrs_saddle_build(ddgrs_surface* s, double alpha, double beta, const ddgrs_saddle_options* o, ddgrs_saddle** out,
@@ -99,6 +105,9 @@ int ddgrs_saddle_build(ddgrs_surface* s, double alpha, double beta, const ddgrs_
     h->fm.w_hi = o->w_tp;
     h->fm.w_lo = o->w_tp * o->c_lo;
     h->fm.build(s, t, c, o->chi, (size_t)o->leaf_pairs, s_exp);
+    h->pc.form = o->form;
+    h->pc.unit_scale = o->scale == 1;
+    h->pc.refine = o->refine == 1;
     h->pc.build(s, s_exp, o->w_tp, o->eps_rel, o->chi, (size_t)o->leaf_size, (size_t)o->leaf_pairs);
     h->op.fm = &h->fm;
     h->op.crow = h->pc.crow_dev;
@@ -116,7 +125,7 @@ int ddgrs_saddle_info(ddgrs_saddle* h, double* out2, ddgrs_error* err) {
     need(h, "saddle");
     need(out2, "out");
     out2[0] = h->pc.eps_value(h->s->ctx);
-    out2[1] = h->pc.frac_scale;
+    out2[1] = h->pc.scale();
   });
 }
 
@@ -150,6 +159,27 @@ int ddgrs_saddle_lprime_batch(ddgrs_saddle* h, int64_t nb, const double* rs, dou
   });
 }
 
+namespace {
+// One right-preconditioned GMRES solve of A′ x = b on device vectors (x starts at zero); the
+// extended products take over when the FP32 products stall.
+GmresResult saddle_gmres(ddgrs_saddle* h, const double* b, double* x, const GmresSettings& gs) {
+  ddgrs_ctx* ctx = h->s->ctx;
+  const size_t n = 3 * (size_t)h->s->nv + 3;
+  auto A = [&](const double* u, double* y) { h->op.apply(ctx, u, y); };
+  auto P = [&](const double* r, double* z) { h->pc.apply(ctx, r, z); };
+  FaceMetricDev& fm = *h->op.fm;
+  fm.extended = h->pc.vf.extended = false;
+  auto stall = [&]() {
+    if (fm.extended) return false;
+    fm.extended = h->pc.vf.extended = true;
+    return true;
+  };
+  GmresResult r = gmres_device(ctx, n, A, P, b, x, gs, h->vec, stall);
+  fm.extended = h->pc.vf.extended = false;
+  return r;
+}
+}  // namespace
+
 int ddgrs_saddle_solve(ddgrs_saddle* h, const double* b, double* x, int64_t restart, double tolerance,
                        int64_t max_iterations, ddgrs_gmres_report* report, ddgrs_error* err) {
   return guarded(err, "saddle_solve", [&] {
@@ -161,22 +191,35 @@ int ddgrs_saddle_solve(ddgrs_saddle* h, const double* b, double* x, int64_t rest
     const size_t n = 3 * (size_t)h->s->nv + 3;
     DevBuf<double> db(n), dx(n);
     upload(ctx, db, b, n);
-    GmresSettings gs{(size_t)restart, tolerance, (size_t)max_iterations};
-    auto A = [&](const double* u, double* y) { h->op.apply(ctx, u, y); };
-    auto P = [&](const double* r, double* z) { h->pc.apply(ctx, r, z); };
-    FaceMetricDev& fm = *h->op.fm;
-    fm.extended = h->pc.vf.extended = false;
-    auto stall = [&]() {
-      if (fm.extended) return false;
-      fm.extended = h->pc.vf.extended = true;
-      return true;
-    };
-    GmresResult r = gmres_device(ctx, n, A, P, db.ptr, dx.ptr, gs, h->vec, stall);
-    fm.extended = h->pc.vf.extended = false;
+    GmresResult r = saddle_gmres(h, db.ptr, dx.ptr, GmresSettings{(size_t)restart, tolerance, (size_t)max_iterations});
     download(ctx, x, (const double*)dx.ptr, n);
     ctx->sync();
     h->last = r;
     if (report) fill_report(r, report);
+  });
+}
+
+int ddgrs_saddle_solve_many(ddgrs_saddle* h, int64_t nb, const double* bs, double* xs, int64_t restart,
+                            double tolerance, int64_t max_iterations, ddgrs_gmres_report* reports, ddgrs_error* err) {
+  return guarded(err, "saddle_solve_many", [&] {
+    need(h, "saddle");
+    if (nb < 0) invalid("nb", std::to_string(nb), "non-negative");
+    if (nb == 0) return;
+    need(bs, "bs");
+    need(xs, "xs");
+    check_gmres(restart, tolerance, max_iterations);
+    ddgrs_ctx* ctx = h->s->ctx;
+    const size_t n = 3 * (size_t)h->s->nv + 3;
+    DevBuf<double> db(n * (size_t)nb), dx(n * (size_t)nb);
+    upload(ctx, db, bs, n * (size_t)nb);
+    const GmresSettings gs{(size_t)restart, tolerance, (size_t)max_iterations};
+    for (int64_t i = 0; i < nb; ++i) {
+      GmresResult r = saddle_gmres(h, db.ptr + n * (size_t)i, dx.ptr + n * (size_t)i, gs);
+      if (reports) fill_report(r, reports + i);
+      h->last = std::move(r);
+    }
+    download(ctx, xs, (const double*)dx.ptr, n * (size_t)nb);
+    ctx->sync();
   });
 }
 
@@ -244,6 +287,9 @@ int ddgrs_session_create(ddgrs_surface* s, const ddgrs_step_options* o, ddgrs_se
     so.w_tp = o->saddle.w_tp;
     so.c_lo = o->saddle.c_lo;
     so.eps_rel = o->saddle.eps_rel;
+    so.pc_form = o->saddle.form;
+    so.pc_unit_scale = o->saddle.scale == 1;
+    so.pc_refine = o->saddle.refine == 1;
     so.restart = (size_t)o->restart;
     so.tolerance = o->tolerance;
     so.max_iterations = (size_t)o->max_iterations;

@@ -891,17 +891,19 @@ void LprimeDev::solve(ddgrs_ctx* ctx, const double* r, double* x) {
          (const uint32_t*)perm.ptr, (const double*)xb.ptr, x);
 }
 
-// v = s·v − (K ⊗ I₃) y on the positions, 0 on the multipliers (precond.rs:1046-1066).
+// Augmented form: v = s·v − (K ⊗ I₃) y on the positions, 0 on the multipliers (precond.rs:1046-1066);
+// congruence form: only the multipliers' zeros.
 __global__ __launch_bounds__(256) void pc_augment_kernel(uint32_t nv, const uint32_t* __restrict__ kp,
                                                          const uint32_t* __restrict__ kj, const double* __restrict__ kx,
                                                          double scale, const double* __restrict__ y,
-                                                         double* __restrict__ v) {
+                                                         double* __restrict__ v, int augmented) {
   uint32_t t = blockIdx.x * 256 + threadIdx.x;
   if (t >= 3 * nv + 3) return;
   if (t >= 3 * nv) {
     v[t] = 0.0;
     return;
   }
+  if (!augmented) return;  // congruence form: L^V y as it stands
   uint32_t j = t / 3, a = t % 3;
   double acc = 0.0;
   for (uint32_t q = kp[j]; q < kp[j + 1]; ++q) acc += kx[q] * y[3 * (size_t)kj[q] + a];
@@ -911,6 +913,57 @@ __global__ __launch_bounds__(256) void pc_augment_kernel(uint32_t nv, const uint
 __global__ __launch_bounds__(256) void pc_add_kernel(uint32_t n, const double* __restrict__ y, double* __restrict__ z) {
   uint32_t t = blockIdx.x * 256 + threadIdx.x;
   if (t < n) z[t] += y[t];
+}
+
+__device__ inline double pc_sum256(double v, double* sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// The refinement residual r − L′ x, L′ = [[K ⊗ I₃, cᵀ], [c, 0]] per axis: the position rows
+// r − (K x + c_v λ) here, and per workgroup partial sums of c · x per axis for the multiplier rows.
+__global__ __launch_bounds__(256) void pc_lres_kernel(uint32_t nv, const uint32_t* __restrict__ kp,
+                                                      const uint32_t* __restrict__ kj, const double* __restrict__ kx,
+                                                      const double* __restrict__ crow, const double* __restrict__ r,
+                                                      const double* __restrict__ x, double* __restrict__ res,
+                                                      double* __restrict__ partial) {
+  __shared__ double sh[256];
+  const size_t nd = 3 * (size_t)nv;
+  const uint32_t j = blockIdx.x * 256 + threadIdx.x;
+  double s[3] = {0.0, 0.0, 0.0};
+  if (j < nv) {
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (uint32_t q = kp[j]; q < kp[j + 1]; ++q) {
+      const double k = kx[q];
+      const size_t c = 3 * (size_t)kj[q];
+      for (int a = 0; a < 3; ++a) acc[a] += k * x[c + a];
+    }
+    const double c = crow[j];
+    for (int a = 0; a < 3; ++a) {
+      res[3 * (size_t)j + a] = r[3 * (size_t)j + a] - (acc[a] + c * x[nd + a]);
+      s[a] = c * x[3 * (size_t)j + a];
+    }
+  }
+  for (int a = 0; a < 3; ++a) {
+    const double t = pc_sum256(s[a], sh);
+    if (threadIdx.x == 0) partial[3 * (size_t)blockIdx.x + a] = t;
+  }
+}
+
+__global__ void pc_lres_final_kernel(uint32_t nblocks, uint32_t nv, const double* __restrict__ partial,
+                                     const double* __restrict__ r, double* __restrict__ res) {
+  const int a = threadIdx.x;
+  if (a >= 3) return;
+  double s = 0.0;
+  for (uint32_t b = 0; b < nblocks; ++b) s += partial[3 * (size_t)b + a];
+  res[3 * (size_t)nv + a] = r[3 * (size_t)nv + a] - s;
 }
 
 HostCsr cotan_stiffness(const std::vector<uint32_t>& faces, size_t nv, const std::vector<double>& pos,
@@ -1116,13 +1169,30 @@ void Preconditioner::lprime_solve(ddgrs_ctx* ctx, const double* r, double* x) {
     lp.solve(ctx, r, x);
 }
 
+void Preconditioner::lprime_solve_refined(ddgrs_ctx* ctx, const double* r, double* x) {
+  lprime_solve(ctx, r, x);
+  if (!refine) return;
+  // x ← x + L′⁻¹ (r − L′ x) (precond.rs:1031 solve_many_on with refine)
+  const unsigned nb = blocks_for((int64_t)nv, 256);
+  res.reserve(n);
+  dx.reserve(n);
+  part.reserve(3 * (size_t)nb);
+  launch(ctx, "pc_lres", pc_lres_kernel, dim3(nb), dim3(256), 0, (uint32_t)nv, k_ptr, k_idx, k_val, crow_dev, r,
+         (const double*)x, res.ptr, part.ptr);
+  launch(ctx, "pc_lres_final", pc_lres_final_kernel, dim3(1), dim3(64), 0, (uint32_t)nb, (uint32_t)nv,
+         (const double*)part.ptr, r, res.ptr);
+  lprime_solve(ctx, res.ptr, dx.ptr);
+  launch(ctx, "pc_add", pc_add_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, (uint32_t)n, (const double*)dx.ptr, x);
+}
+
 void Preconditioner::apply(ddgrs_ctx* ctx, const double* r, double* z) {
-  lprime_solve(ctx, r, y.ptr);
+  lprime_solve_refined(ctx, r, y.ptr);
   vf.apply(ctx, y.ptr, v.ptr);
   launch(ctx, "pc_augment", pc_augment_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, (uint32_t)nv, k_ptr, k_idx,
-         k_val, frac_scale, (const double*)y.ptr, v.ptr);
-  lprime_solve(ctx, v.ptr, z);
-  launch(ctx, "pc_add", pc_add_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, (uint32_t)n, (const double*)y.ptr, z);
+         k_val, scale(), (const double*)y.ptr, v.ptr, (int)(form == kFormAugmented));
+  lprime_solve_refined(ctx, v.ptr, z);
+  if (form == kFormAugmented)
+    launch(ctx, "pc_add", pc_add_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, (uint32_t)n, (const double*)y.ptr, z);
 }
 
 }  // namespace ddgrs

@@ -116,6 +116,8 @@ HostCsr cotan_stiffness(const std::vector<uint32_t>& faces, size_t nv, const std
 struct LprimeSymbolic;
 struct LprimeFactorDev;
 
+enum { kFormAugmented = 0, kFormCongruence = 1 };
+
 struct Preconditioner {
   size_t nv = 0, n = 0;
   double eps = 0.0, frac_scale = 1.0;
@@ -130,6 +132,14 @@ struct Preconditioner {
   // What apply() reads: K and the barycenter row, from the host plan or the device factorization.
   const uint32_t *k_ptr = nullptr, *k_idx = nullptr;
   const double *k_val = nullptr, *crow_dev = nullptr;
+  // PreconditionerOptions (precond.rs:466): the form (kFormAugmented: z = L′⁻¹[(s L^V − K) y; 0] + y,
+  // kFormCongruence: z = L′⁻¹[L^V y; 0], with y = L′⁻¹ r), the fractional factor's scale (the planar
+  // symbol 1/(w_TP 4 I(σ) I(2−s)), or 1 with unit_scale) and one step of iterative refinement in
+  // every L′ solve (refine).
+  int form = kFormAugmented;
+  bool unit_scale = false, refine = false;
+  DevBuf<double> res, dx, part;  // refinement work
+  double scale() const { return unit_scale ? 1.0 : frac_scale; }
 
   Preconditioner();
   ~Preconditioner();
@@ -148,6 +158,8 @@ struct Preconditioner {
                         double chi, size_t leaf_pairs, Topology vertex_tree);
   // x = L′⁻¹ r (device pointers, length 3 nv + 3).
   void lprime_solve(ddgrs_ctx* ctx, const double* r, double* x);
+  // The same, followed by one step of iterative refinement when `refine` is set.
+  void lprime_solve_refined(ddgrs_ctx* ctx, const double* r, double* x);
   // z = P⁻¹ r (device pointers, length 3 nv + 3).
   void apply(ddgrs_ctx* ctx, const double* r, double* z);
 };

@@ -305,6 +305,9 @@ void Session::step(StepReport* rep) {
       pc.setup_device(aux, lp_symbolic.get());
       lp_ready = true;
     }
+    pc.form = opts.pc_form;
+    pc.unit_scale = opts.pc_unit_scale;
+    pc.refine = opts.pc_refine;
     pc.factor_device(aux, s, s_exp, opts.w_tp, opts.eps_rel);
     pc.build_fractional(aux, geo.pos, geo.vmass, 2.0 - s_exp, opts.chi, opts.leaf_pairs, std::move(vtopo));
     aux->sync();

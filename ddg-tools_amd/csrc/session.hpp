@@ -25,6 +25,9 @@ struct StepOptions {
   size_t leaf_pairs = 64;        // dense block threshold
   double w_tp = 1.0, c_lo = 1.0; // metric weights of B̃ and B̃⁰
   double eps_rel = 1e-10;        // ε of K = W + ε M relative to trW / trM
+  int pc_form = 0;               // preconditioner form: 0 augmented, 1 congruence
+  bool pc_unit_scale = false;    // L̃^V unscaled instead of the planar symbol's scale
+  bool pc_refine = false;        // one step of iterative refinement per L′ solve
   size_t restart = 50;
   double tolerance = 1e-6;
   size_t max_iterations = 200;

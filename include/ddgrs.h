@@ -190,6 +190,14 @@ typedef struct ddgrs_saddle_options {
   double w_tp;        /* weight of B̃ */
   double c_lo;        /* weight of B̃⁰ relative to w_tp */
   double eps_rel;     /* K = W + ε M with ε = eps_rel · tr W / tr M */
+  /* PreconditionerOptions::form: 0 Augmented, P⁻¹ = L′⁻¹[(s L^V − K) y; 0] + y with y = L′⁻¹ r;
+   * 1 Congruence, P⁻¹ = L′⁻¹[L^V y; 0] (RS/src/solve/precond.rs:1026 apply_fine). */
+  int32_t form;
+  /* PreconditionerOptions::scale: 0 Symbol, s = 1/(w_tp 4 I(σ) I(2−s)); 1 Unit, s = 1
+   * (precond.rs:502 FractionalScale).  No effect on the congruence form. */
+  int32_t scale;
+  /* PreconditionerOptions::refine: 1 adds one step of iterative refinement to every L′ solve. */
+  int32_t refine;
 } ddgrs_saddle_options;
 
 /* The GMRES report (RS/src/solve/gmres.rs:200 GmresReport). */
@@ -203,7 +211,8 @@ typedef struct ddgrs_gmres_report {
   double relative_residual; /* explicit ‖b − A x‖ / ‖b‖ at the last cycle start */
 } ddgrs_gmres_report;
 
-/* Fills the reference defaults (χ 0.5, leaf 8, 64 pairs, w 1, c_lo 1, ε 1e-10). */
+/* Fills the reference defaults (χ 0.5, leaf 8, 64 pairs, w 1, c_lo 1, ε 1e-10, augmented form,
+ * symbol scale, no refinement). */
 DDGRS_API void ddgrs_saddle_options_default(ddgrs_saddle_options* o);
 /* The solver of A′ = [[w (B̃ + c_lo B̃⁰), C_Qᵀ], [C_Q, 0]] with the global barycenter rows C_Q
  * (k_Q = 3) at the surface's current positions: the metric tree, L′ (sparse LDLᵀ in a nested
@@ -220,10 +229,17 @@ DDGRS_API int ddgrs_saddle_apply_batch(ddgrs_saddle* h, int64_t nb, const double
 DDGRS_API int ddgrs_saddle_precond_batch(ddgrs_saddle* h, int64_t nb, const double* rs, double* zs, ddgrs_error* err);
 /* x = L′⁻¹ r (RS/src/solve/precond.rs:384 LaplacianSaddle::solve). */
 DDGRS_API int ddgrs_saddle_lprime_batch(ddgrs_saddle* h, int64_t nb, const double* rs, double* xs, ddgrs_error* err);
-/* x = A′⁻¹ b by right-preconditioned restarted GMRES on the device (RS/src/solve/gmres.rs:801
- * GmresSaddle::solve_many, options gmres.rs:138). */
+/* x = A′⁻¹ b by right-preconditioned restarted GMRES on the device (RS/src/solve/gmres.rs:783
+ * GmresSaddle::solve, options gmres.rs:138). */
 DDGRS_API int ddgrs_saddle_solve(ddgrs_saddle* h, const double* b, double* x, int64_t restart, double tolerance,
                                  int64_t max_iterations, ddgrs_gmres_report* report, ddgrs_error* err);
+/* x_i = A′⁻¹ b_i for nb right-hand sides of 3·nv + 3 each (bs, xs row-major nb × (3·nv + 3)), one
+ * report per right-hand side (reports may be NULL).  Replaces GmresSaddle::solve_many
+ * (RS/src/solve/gmres.rs:801 → gmres_batch_on): every solve is the one ddgrs_saddle_solve makes
+ * for its right-hand side, bit for bit; the history entry point then reports the last of them. */
+DDGRS_API int ddgrs_saddle_solve_many(ddgrs_saddle* h, int64_t nb, const double* bs, double* xs, int64_t restart,
+                                      double tolerance, int64_t max_iterations, ddgrs_gmres_report* reports,
+                                      ddgrs_error* err);
 /* The last solve's stopping measure (relative, explicit) at every cycle start and at the end, and
  * its Arnoldi estimate after every iteration (RS/src/solve/gmres.rs:214-217
  * GmresReport::cycle_residuals, ::history): at most cap_cycles / cap_history values are written

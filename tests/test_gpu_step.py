@@ -69,6 +69,61 @@ def test_saddle_parts(pkg, ctx, name):
     assert e_p <= 1e-5
 
 
+@pytest.mark.parametrize(
+    "form,scale,refine",
+    [("congruence", "symbol", False), ("augmented", "unit", False), ("augmented", "symbol", True), ("congruence", "unit", True)],
+)
+def test_preconditioner_variants(pkg, ctx, form, scale, refine):
+    """PreconditionerOptions' form, fractional scale and refinement (precond.rs:466) on the device
+    against the oracle's restatement of the same variant, and GMRES with the variant reaching the
+    default preconditioner's solution."""
+    p, f = pkg.generate.config_icosphere_l3()
+    s = pkg.Surface(ctx, f, p)
+    sd = pkg.Saddle(s, options=pkg.SaddleOptions(form=form, scale=scale, refine=refine))
+    od = oracle.Saddle(f, p, form=form, scale=scale, refine=refine)
+    base = pkg.Saddle(s)
+    info = sd.info()
+    want_scale = 1.0 if scale == "unit" else base.info()["fractional_scale"]
+    assert info["fractional_scale"] == want_scale
+    rng = np.random.default_rng(31)
+    x = rng.standard_normal((2, od.n))
+    e_p = max(rel(sd.precondition(x[q]), od.precond(x[q])) for q in range(2))
+    # the step's right-hand side [g; 0] with Σ_v g_v = 0, as a translation-invariant energy's
+    # differential has: then the multipliers of the solution vanish (λ_a = Σ_v g_va, since B̃ kills
+    # translations and Σ c_v = 1).  The congruence form's range has no multiplier component, so
+    # right-preconditioned GMRES with it cannot reach any other right-hand side (precond.rs:586-594).
+    b = x[0].copy()
+    b[-3:] = 0.0
+    g = b[:-3].reshape(-1, 3)
+    g -= g.mean(axis=0)
+    xb, rb = base.solve(b)
+    xv, rv = sd.solve(b)
+    print(f"{form}/{scale}/refine={refine}: P^-1 {e_p:.2e}; GMRES {rv.iterations} it (default {rb.iterations}), "
+          f"rel diff {rel(xv, xb):.2e}")
+    assert e_p <= 1e-5
+    assert rv.converged and rb.converged
+    assert rel(xv, xb) <= 1e-4
+
+
+def test_solve_many_matches_solve(pkg, ctx):
+    """GmresSaddle::solve_many (gmres.rs:801): each right-hand side's solution and report are the
+    single solve's, bit for bit; an empty batch is a no-op."""
+    p, f = pkg.generate.config_icosphere_l3()
+    s = pkg.Surface(ctx, f, p)
+    sd = pkg.Saddle(s)
+    rng = np.random.default_rng(37)
+    bs = rng.standard_normal((3, sd.n))
+    xs, reps = sd.solve_many(bs)
+    for i in range(3):
+        x1, r1 = sd.solve(bs[i])
+        assert np.array_equal(xs[i], x1)
+        assert (reps[i].iterations, reps[i].cycles, reps[i].relative_residual) == (r1.iterations, r1.cycles, r1.relative_residual)
+    xs0, reps0 = sd.solve_many(np.zeros((0, sd.n)))
+    assert xs0.shape == (0, sd.n) and reps0 == []
+    with pytest.raises(pkg.RsError):
+        sd.solve_many(bs[:, :-1])
+
+
 @pytest.mark.parametrize("name", ["icosphere_l3", "torus_16k"])
 def test_lprime_device_factor_matches_host(pkg, ctx, name, monkeypatch):
     """L′ factored on the device (per-mesh symbolic analysis, fronts eliminated on the GPU) against
