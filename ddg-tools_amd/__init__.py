@@ -24,6 +24,7 @@ import ctypes
 import dataclasses
 import os
 import weakref
+from typing import Optional
 
 import numpy as np
 
@@ -125,6 +126,11 @@ SIGNATURES = {
     "ddgrs_surface_destroy": (None, [_P]),
     "ddgrs_surface_set_positions": (_I32, [_P, _P, _E]),
     "ddgrs_surface_get_positions": (_I32, [_P, _P, _E]),
+    "ddgrs_surface_edges": (_I32, [_P, _P, _I64, ctypes.POINTER(_I64), _E]),
+    "ddgrs_ccd_options_default": (None, [_P]),
+    "ddgrs_ccd_first_contact": (_I32, [_P, _P, _P, _D, _P, _P, _P, _E]),
+    "ddgrs_ccd_intersecting_pairs": (_I32, [_P, _P, _P, _I64, ctypes.POINTER(_I64), _E]),
+    "ddgrs_ccd_band": (_I32, [_P, _P, _P, _P, _E]),
     "ddgrs_exact_energy_and_differential": (_I32, [_P, _D, _D, ctypes.POINTER(_D), _P, _P, _E]),
     "ddgrs_exact_energy_and_differential_noalloc": (_I32, [_P, _D, _D, ctypes.POINTER(_D), _P, _P, _E]),
     "ddgrs_bh_build": (_I32, [_P, _D, _I64, ctypes.POINTER(_P), _E]),
@@ -285,6 +291,123 @@ class Surface:
         out = np.zeros((self.nv, 3))
         _call(lib().ddgrs_surface_get_positions, self._h, _ptr(out))
         return out
+
+    def edges(self) -> np.ndarray:
+        """The sorted unique edges (ne × 2, a < b) (``Surface::edges``, src/surface.rs:564)."""
+        ne = _I64()
+        _call(lib().ddgrs_surface_edges, self._h, None, 0, ctypes.byref(ne))
+        out = np.zeros((ne.value, 2), dtype=np.uint32)
+        _call(lib().ddgrs_surface_edges, self._h, _ptr(out), ne.value, ctypes.byref(ne))
+        return out
+
+
+# ---- continuous collision detection (src/ccd.rs) --------------------------------------------
+
+
+class _CcdOptionsC(ctypes.Structure):
+    _fields_ = [("gap", _D), ("reference_length", _D), ("max_iterations", _I64), ("close_factor", _D),
+                ("leaf_size", _I64)]
+
+
+class _CcdReportC(ctypes.Structure):
+    _fields_ = [
+        ("tau_max", _D),
+        ("has_contact", _I32),
+        ("contact_kind", _I32),
+        ("contact_a", _I64),
+        ("contact_b", _I64),
+        ("gap_abs", _D),
+        ("iterations", _I64),
+        ("pairs_tested", _I64),
+        ("close_pairs", _I64),
+        ("touching_pairs", _I64),
+        ("exempt_pairs", _I64),
+        ("windows", _I32),
+        ("seconds", _D),
+    ]
+
+
+@dataclasses.dataclass(frozen=True)
+class CcdOptions:
+    """``CcdOptions`` (src/ccd.rs:86): gap relative to the reference length (None: the mean edge
+    length of the start positions), advances per pair and window, the closing fraction of pairs
+    that start inside the gap, the broad-phase leaf size."""
+
+    gap: float = 0.05
+    reference_length: Optional[float] = None
+    max_iterations: int = 100
+    close_factor: float = 0.5
+    leaf_size: int = 8
+
+    def _c(self) -> _CcdOptionsC:
+        ref = float("nan") if self.reference_length is None else float(self.reference_length)
+        return _CcdOptionsC(self.gap, ref, self.max_iterations, self.close_factor, self.leaf_size)
+
+
+@dataclasses.dataclass(frozen=True)
+class CcdReport:
+    """``CcdReport`` (src/ccd.rs:214); ``contact`` is None or (τ, kind, a, b) with kind
+    "vertex_triangle" (vertex a, triangle b) or "edge_edge" (edge ids a < b of ``Surface.edges``)."""
+
+    tau_max: float
+    contact: Optional[tuple]
+    gap_abs: float
+    iterations: int
+    pairs_tested: int
+    windows: int
+    close_pairs: int
+    touching_pairs: int
+    exempt_pairs: int
+    seconds: float
+
+
+def first_contact(surface: Surface, velocity, tau_end: float, options: CcdOptions = CcdOptions(), start=None,
+                  band=None) -> CcdReport:
+    """``first_contact_with_band`` (src/ccd.rs:781) of start + t·velocity on [0, tau_end] on the
+    device (start None: the surface's positions; band None or per-vertex flags)."""
+    u = _f64(velocity).reshape(-1, 3)
+    if u.shape[0] != surface.nv:
+        raise RsError(1, "ccd_first_contact", f"{u.shape[0]} velocities for {surface.nv} vertices")
+    st = None if start is None else _f64(start).reshape(-1, 3)
+    if st is not None and st.shape[0] != surface.nv:
+        raise RsError(1, "ccd_first_contact", f"{st.shape[0]} positions for {surface.nv} vertices")
+    b = None if band is None else np.ascontiguousarray(band, dtype=np.uint8).reshape(-1)
+    if b is not None and b.shape[0] != surface.nv:
+        raise RsError(1, "ccd_first_contact", f"band: {b.shape[0]} flags for {surface.nv} vertices")
+    o = options._c()
+    r = _CcdReportC()
+    _call(lib().ddgrs_ccd_first_contact, surface.handle, None if st is None else _ptr(st), _ptr(u), float(tau_end),
+          ctypes.byref(o), None if b is None else _ptr(b), ctypes.byref(r))
+    return _ccd_report(r)
+
+
+def _ccd_report(r: _CcdReportC) -> CcdReport:
+    kinds = ("vertex_triangle", "edge_edge")
+    contact = (float(r.tau_max), kinds[r.contact_kind], int(r.contact_a), int(r.contact_b)) if r.has_contact else None
+    return CcdReport(float(r.tau_max), contact, float(r.gap_abs), int(r.iterations), int(r.pairs_tested),
+                     int(r.windows), int(r.close_pairs), int(r.touching_pairs), int(r.exempt_pairs), float(r.seconds))
+
+
+def intersecting_pairs(surface: Surface, positions=None) -> np.ndarray:
+    """The sorted face pairs (k × 2, f < g) that intersect (``intersecting_pairs``, src/ccd.rs:1229)."""
+    p = None if positions is None else _f64(positions).reshape(-1, 3)
+    n = _I64()
+    _call(lib().ddgrs_ccd_intersecting_pairs, surface.handle, None if p is None else _ptr(p), None, 0, ctypes.byref(n))
+    out = np.zeros((n.value, 2), dtype=np.uint32)
+    if n.value:
+        _call(lib().ddgrs_ccd_intersecting_pairs, surface.handle, None if p is None else _ptr(p), _ptr(out), n.value,
+              ctypes.byref(n))
+    return out
+
+
+def intersection_band(surface: Surface, positions=None):
+    """``IntersectionBand::new`` (src/ccd.rs:1292): (per-vertex flags, intersecting pairs, vertices
+    in the band)."""
+    p = None if positions is None else _f64(positions).reshape(-1, 3)
+    flags = np.zeros(surface.nv, dtype=np.uint8)
+    counts = np.zeros(2, dtype=np.int64)
+    _call(lib().ddgrs_ccd_band, surface.handle, None if p is None else _ptr(p), _ptr(flags), _ptr(counts))
+    return flags, int(counts[0]), int(counts[1])
 
 
 # ---- energy (src/energy/mod.rs) ----------------------------------------------------------------
@@ -562,6 +685,8 @@ class _StepOptionsC(ctypes.Structure):
         ("row_targets", _D * 2),
         ("newton_fixed", _I32),
         ("newton_count", _I32),
+        ("ccd", _I32),
+        ("ccd_options", _CcdOptionsC),
     ]
 
 
@@ -597,6 +722,10 @@ class _StepReportC(ctypes.Structure):
         ("fallback_stage", _I32),
         ("newton_rounds", _I32),
         ("min_pivot", _D),
+        ("has_ccd", _I32),
+        ("tau_ccd", _D),
+        ("ccd", _CcdReportC),
+        ("seconds_ccd", _D),
     ]
 
 
@@ -766,9 +895,10 @@ class TrialOutcome:
 
 @dataclasses.dataclass(frozen=True)
 class StepOptions:
-    """A step of the Hs flow (src/step/session.rs ``FlowOptions`` with remeshing and CCD off, the
-    global barycenter constraint; ``start`` defaults to ``StartRule::GrowPrevious{2,
-    MaxDisplacement{1}}``, ``max_halvings`` < 0 is no cap)."""
+    """A step of the Hs flow (src/step/session.rs ``FlowOptions`` with remeshing off, the global
+    barycenter constraint; ``start`` defaults to ``StartRule::GrowPrevious{2,
+    MaxDisplacement{1}}``, ``max_halvings`` < 0 is no cap; ``ccd`` None is
+    ``LineSearchOptions::ccd`` None, else the CCD cap of τ₀ with those options)."""
 
     exponents: Exponents = Exponents()
     energy_exact: bool = False
@@ -789,6 +919,7 @@ class StepOptions:
     # Newton rounds of the projection: ("tol", n) = UntilTolerance{max: n}, ("fixed", n)
     constraints: tuple = ()
     newton: tuple = ("tol", 5)
+    ccd: Optional[CcdOptions] = None
 
     _ROW_KINDS = {"area": 1, "volume": 2}
 
@@ -805,7 +936,8 @@ class StepOptions:
                              self.armijo_c, self.tau_min_rel, self.max_halvings, int(self.start[0]),
                              float(self.start[1]), 1 if self.robust else 0, self.retry_max_iterations,
                              self.dense_max_vertices, self.diverged_residual, (_I32 * 2)(*rows),
-                             (_D * 2)(*targets), 1 if self.newton[0] == "fixed" else 0, int(self.newton[1]))
+                             (_D * 2)(*targets), 1 if self.newton[0] == "fixed" else 0, int(self.newton[1]),
+                             0 if self.ccd is None else 1, (self.ccd or CcdOptions())._c())
 
 
 @dataclasses.dataclass(frozen=True)
@@ -836,6 +968,8 @@ class StepReport:
     fallbacks: tuple = ()  # the escalations as text (LinearReport::fallbacks)
     newton_rounds: int = 0  # Newton rounds kept by the projection (nonlinear rows)
     min_pivot: float = 1.0  # the smallest kept equilibrated Schur pivot
+    tau_ccd: Optional[float] = None  # the CCD's τ_max (tau0 is after its cap)
+    ccd: Optional["CcdReport"] = None
 
 
 class Session(_Handle):
@@ -863,7 +997,8 @@ class Session(_Handle):
                           int(r.pairs_exact), int(r.nodes_lumped), sec, int(r.learned_mode), int(r.learned_fallback),
                           int(r.learned_iterations), float(r.energy_after), float(r.direction_norm),
                           float(r.direction_max), int(r.halvings), self.trials(), int(r.fallback_stage),
-                          self.fallbacks(), int(r.newton_rounds), float(r.min_pivot))
+                          self.fallbacks(), int(r.newton_rounds), float(r.min_pivot),
+                          float(r.tau_ccd) if r.has_ccd else None, _ccd_report(r.ccd) if r.has_ccd else None)
 
     def constraints(self) -> tuple:
         """(Φ before the projection, Φ after it) of the last step: the barycenter's x, y, z, then

@@ -255,6 +255,8 @@ void ddgrs_step_options_default(ddgrs_step_options* o) {
   o->row_targets[0] = o->row_targets[1] = NAN;
   o->newton_fixed = 0;
   o->newton_count = 5;
+  o->ccd = 0;
+  ddgrs_ccd_options_default(&o->ccd_options);
 }
 
 int ddgrs_session_create(ddgrs_surface* s, const ddgrs_step_options* o, ddgrs_session** out, ddgrs_error* err) {
@@ -321,6 +323,20 @@ int ddgrs_session_create(ddgrs_surface* s, const ddgrs_step_options* o, ddgrs_se
     if (o->newton_count < 0) invalid("newton_count", std::to_string(o->newton_count), "non-negative");
     so.newton_fixed = o->newton_fixed != 0;
     so.newton_count = o->newton_count;
+    if (o->ccd != 0 && o->ccd != 1) invalid("ccd", std::to_string(o->ccd), "0 or 1");
+    so.ccd = o->ccd == 1;
+    if (so.ccd) {
+      const ddgrs_ccd_options& c = o->ccd_options;
+      if (c.max_iterations < 1 || c.max_iterations > UINT32_MAX)
+        invalid("ccd.max_iterations", std::to_string(c.max_iterations), "at least 1");
+      if (c.leaf_size < 1 || c.leaf_size > (1 << 20)) invalid("ccd.leaf_size", std::to_string(c.leaf_size), "at least 1");
+      so.ccd_opts.gap = c.gap;
+      so.ccd_opts.reference_length = c.reference_length;
+      so.ccd_opts.close_factor = c.close_factor;
+      so.ccd_opts.max_iterations = (uint32_t)c.max_iterations;
+      so.ccd_opts.leaf_size = (uint32_t)c.leaf_size;
+      check_ccd_options(so.ccd_opts);
+    }
     auto h = std::make_unique<ddgrs_session>();
     h->s = s;
     h->ses.init(s, so);

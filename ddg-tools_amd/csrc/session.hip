@@ -125,6 +125,7 @@ void Session::init(ddgrs_surface* s_, const StepOptions& o) {
   }
   for (int k = 0; k < 3; ++k) target[k] = bc[k] / tot;
   last_tau = 0.0;
+  if (opts.ccd) ccd.set_mesh(s->ctx, s->faces_h, (uint32_t)nv);
   // the nonlinear rows and their targets (the values at creation unless given)
   nl_rows.clear();
   nl_targets.clear();
@@ -551,6 +552,35 @@ void Session::step(StepReport* rep) {
     default: tau0 = last_tau > 0.0 ? 2.0 * last_tau : opts.start_value * l0 / maxd; break;
   }
   const double tau_min = opts.tau_min_rel * l0 / xinf;
+  // the CCD cap (session.rs:917-956): the band of the faces that intersect at the start, then
+  // τ₀ ← min(τ₀, ⅔ τ_max) along −x when a contact is found
+  if (opts.ccd) {
+    const auto tc = std::chrono::steady_clock::now();
+    const size_t np = ccd.band_flags(ctx, s->faces.ptr, s->positions.ptr);
+    if (std::isfinite(tau0) && tau0 > 0.0) {
+      CcdOpts co = opts.ccd_opts;
+      if (std::isnan(co.reference_length)) co.reference_length = l0;
+      const CcdResult cr = ccd.first_contact(ctx, s->faces.ptr, s->positions.ptr, geo.pos.data(), sol.ptr, -1.0, tau0,
+                                             co, np ? ccd.band.ptr : nullptr);
+      if (cr.has_contact) tau0 = std::fmin(tau0, 2.0 / 3.0 * cr.tau_max);
+      rep->has_ccd = 1;
+      rep->tau_ccd = cr.tau_max;
+      rep->ccd.tau_max = cr.tau_max;
+      rep->ccd.has_contact = cr.has_contact ? 1 : 0;
+      rep->ccd.contact_kind = cr.kind;
+      rep->ccd.contact_a = cr.a;
+      rep->ccd.contact_b = cr.b;
+      rep->ccd.gap_abs = cr.gap_abs;
+      rep->ccd.iterations = (int64_t)cr.iterations;
+      rep->ccd.pairs_tested = (int64_t)cr.pairs_tested;
+      rep->ccd.close_pairs = (int64_t)cr.close_pairs;
+      rep->ccd.touching_pairs = (int64_t)cr.touching_pairs;
+      rep->ccd.exempt_pairs = (int64_t)cr.exempt_pairs;
+      rep->ccd.windows = (int32_t)cr.windows;
+    }
+    rep->seconds_ccd = seconds_since(tc);
+    rep->ccd.seconds = rep->seconds_ccd;
+  }
   rep->tau0 = tau0;
   HIPCHECK(hipMemcpyAsync(p0.ptr, s->positions.ptr, nd * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
   bool accepted = false;

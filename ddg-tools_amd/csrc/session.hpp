@@ -9,6 +9,7 @@
 #include <cmath>
 
 #include "bh.hpp"
+#include "ccd.hpp"
 #include "constraints.hpp"
 #include "gmres.hpp"
 #include "learned.hpp"
@@ -45,6 +46,8 @@ struct StepOptions {
   double row_targets[2] = {NAN, NAN};
   bool newton_fixed = false;
   int newton_count = 5;
+  bool ccd = false;               // the CCD cap of τ₀ (with the exempt band)
+  CcdOpts ccd_opts;
 };
 
 // TrialOutcome (RS/src/step/session.rs): 0 accepted, 1 Armijo failed, 2 non-finite.
@@ -66,6 +69,7 @@ struct Session {
   GpuBvhBuilder face_bvh, vert_bvh;
   FaceMetricDev fm;
   Preconditioner pc;
+  CcdDev ccd;              // the CCD cap of the line search (edges set at init when it is on)
   std::future<LprimeSymbolic> lp_symbolic;  // L′'s symbolic analysis, started at init
   bool lp_ready = false;
   ddgrs_ctx* aux = nullptr;   // second stream: vertex BVH, L̃^V and L′'s factorization of a step

@@ -247,6 +247,54 @@ DDGRS_API int ddgrs_saddle_solve_many(ddgrs_saddle* h, int64_t nb, const double*
 DDGRS_API int ddgrs_saddle_gmres_history(ddgrs_saddle* h, double* cycle_residuals, int64_t cap_cycles, double* history,
                                          int64_t cap_history, int64_t* counts2, ddgrs_error* err);
 
+/* ---- continuous collision detection ------------------------------------------------------- */
+
+/* CcdOptions (RS/src/ccd.rs:86). */
+typedef struct ddgrs_ccd_options {
+  double gap;              /* the gap relative to the reference length (0.05) */
+  double reference_length; /* ℓ̄; NaN (None): the mean edge length of the start positions */
+  int64_t max_iterations;  /* advances per pair and window (100) */
+  double close_factor;     /* a pair closer than the gap at t = 0 may close to this fraction (0.5) */
+  int64_t leaf_size;       /* leaf size of the broad-phase box trees (8) */
+} ddgrs_ccd_options;
+
+/* CcdReport (RS/src/ccd.rs:214).  contact_kind 0: Primitive::VertexTriangle{vertex a, triangle b};
+ * 1: Primitive::EdgeEdge{a < b} (ids of the sorted unique edge list, ddgrs_surface_edges). */
+typedef struct ddgrs_ccd_report {
+  double tau_max;
+  int32_t has_contact, contact_kind;
+  int64_t contact_a, contact_b;
+  double gap_abs;
+  int64_t iterations, pairs_tested, close_pairs, touching_pairs, exempt_pairs;
+  int32_t windows;
+  double seconds;
+} ddgrs_ccd_report;
+
+/* Fills CcdOptions::default(). */
+DDGRS_API void ddgrs_ccd_options_default(ddgrs_ccd_options* o);
+/* The largest τ_max ≤ tau_end such that start + t·velocity keeps every non-adjacent vertex–triangle
+ * and edge–edge pair of the surface at distance ≥ gap·ℓ̄ on [0, τ_max] (pairs closer at t = 0: at
+ * close_factor of their distance; touching pairs: free), broad and narrow phase on the device.
+ * start: 3·nv positions, or NULL for the surface's current ones; velocity: 3·nv; band: nv flags
+ * (IntersectionBand, ddgrs_ccd_band) or NULL.  Every pair's advance, the windows and the counts
+ * are the reference's, so the report equals its report.  Replaces first_contact_with_band
+ * (RS/src/ccd.rs:781) for the mesh alone (static obstacles are not on this path). */
+DDGRS_API int ddgrs_ccd_first_contact(ddgrs_surface* s, const double* start, const double* velocity, double tau_end,
+                                      const ddgrs_ccd_options* o, const uint8_t* band, ddgrs_ccd_report* report,
+                                      ddgrs_error* err);
+/* The face pairs [f, g], f < g, that intersect at the positions (NULL: the surface's current), sorted;
+ * at most cap are written to pairs (2 each), *count is their number.  Replaces intersecting_pairs
+ * (RS/src/ccd.rs:1229). */
+DDGRS_API int ddgrs_ccd_intersecting_pairs(ddgrs_surface* s, const double* positions, uint32_t* pairs, int64_t cap,
+                                           int64_t* count, ddgrs_error* err);
+/* IntersectionBand::new (RS/src/ccd.rs:1292): flags (nv) of the vertices of intersecting faces
+ * widened by one ring; counts2 = [intersecting pairs, vertices in the band]. */
+DDGRS_API int ddgrs_ccd_band(ddgrs_surface* s, const double* positions, uint8_t* flags, int64_t* counts2,
+                             ddgrs_error* err);
+/* The sorted unique edges (a < b) of the surface (Surface::edges, RS/src/surface.rs:564): *ne edges,
+ * at most cap written to edges (2 each; edges may be NULL). */
+DDGRS_API int ddgrs_surface_edges(ddgrs_surface* s, uint32_t* edges, int64_t cap, int64_t* ne, ddgrs_error* err);
+
 /* ---- the descent session -------------------------------------------------------------------- */
 
 /* The line search's first step (RS/src/step/options.rs StartRule; session.rs:1604 start_tau). */
@@ -294,6 +342,13 @@ typedef struct ddgrs_step_options {
   double row_targets[2];
   int32_t newton_fixed;
   int32_t newton_count;
+  /* Continuous collision detection in the line search (LineSearchOptions::ccd, RS/src/step/
+   * session.rs:917-956): with ccd 1 every step finds the faces that intersect at its start (their
+   * vertices widened by one ring are exempt, IntersectionBand) and, for a finite τ₀ > 0, caps τ₀ at
+   * ⅔ τ_max of ddgrs_ccd_first_contact along −x (ccd_options.reference_length NaN: the session's
+   * ℓ̄₀). */
+  int32_t ccd;
+  ddgrs_ccd_options ccd_options;
 } ddgrs_step_options;
 
 /* A step's report (RS/src/step/session.rs:146 StepReport, the fields this path produces).
@@ -323,6 +378,12 @@ typedef struct ddgrs_step_report {
    * smallest kept equilibrated Schur pivot (LinearReport::schur_pivot; 1 without rows). */
   int32_t newton_rounds;
   double min_pivot;
+  /* The CCD (StepReport::tau_ccd, ::ccd): has_ccd 1 when it ran (ccd on and τ₀ finite and > 0),
+   * then tau_ccd = ccd.tau_max (tau0 above is after the cap); seconds_ccd the band and the CCD. */
+  int32_t has_ccd;
+  double tau_ccd;
+  ddgrs_ccd_report ccd;
+  double seconds_ccd;
 } ddgrs_step_report;
 
 /* ddgrs_saddle_gmres_history for the last step's direction solve (the last GMRES run of the

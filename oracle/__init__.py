@@ -130,6 +130,17 @@ def _declare(L):
     L.orc_session_constraints.argtypes = [_P, _P, _I64]
     L.orc_constraint_rows.argtypes = [_I64, _I64, _P, _P, _P, _I64, _P, _P]
     L.orc_schur.argtypes = [_I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P]
+    L.orc_session_ccd.restype = _INT
+    L.orc_session_ccd.argtypes = [_P, _P, _P]
+    L.orc_ccd_first_contact.restype = _INT
+    L.orc_ccd_first_contact.argtypes = [_I64, _I64, _P, _P, _P, _D, _P, _P, _P, _P, _P]
+    L.orc_intersecting_pairs.restype = _I64
+    L.orc_intersecting_pairs.argtypes = [_I64, _I64, _P, _P, _P, _I64]
+    L.orc_intersection_band.argtypes = [_I64, _I64, _P, _P, _P, _P]
+    L.orc_min_distance.restype = _D
+    L.orc_min_distance.argtypes = [_I64, _I64, _P, _P]
+    L.orc_ccd_primitive.restype = _D
+    L.orc_ccd_primitive.argtypes = [_INT, _P]
     L.orc_fractional_symbol_ratio.argtypes = [_D]
     L.orc_fractional_symbol_ratio.restype = _D
 
@@ -408,7 +419,7 @@ class Session:
 
     def __init__(self, faces, positions, alpha=6.0, beta=12.0, theta=0.5, chi=0.5, restart=50, tol=1e-6, maxit=200,
                  energy_exact=False, start=("grow", 1.0), armijo_c=1e-2, tau_min_rel=1e-12, max_halvings=-1,
-                 c_lo=1.0, leaf_size=8, constraints=(), newton=("tol", 5)):
+                 c_lo=1.0, leaf_size=8, constraints=(), newton=("tol", 5), ccd=None):
         f, p = _mesh(faces, positions)
         self.nv = p.shape[0]
         st = _INT(0)
@@ -419,9 +430,14 @@ class Session:
         kinds = [self.ROW_KINDS[c[0]] for c in constraints] + [0] * (2 - len(constraints))
         targets = [np.nan if c[1] is None else float(c[1]) for c in constraints] + [np.nan] * (2 - len(constraints))
         self.n_rows = 3 + len(constraints)
-        of = np.array([theta, chi, tol, value, armijo_c, tau_min_rel, c_lo] + targets, dtype=np.float64)
+        c = dict(gap=0.05, reference_length=None, max_iterations=100, close_factor=0.5, leaf_size=8)
+        c.update(ccd or {})
+        ref = np.nan if c["reference_length"] is None else float(c["reference_length"])
+        of = np.array([theta, chi, tol, value, armijo_c, tau_min_rel, c_lo] + targets +
+                      [c["gap"], ref, c["close_factor"]], dtype=np.float64)
         oi = np.array([restart, maxit, 1 if energy_exact else 0, kind, max_halvings, leaf_size] + kinds +
-                      [1 if newton[0] == "fixed" else 0, int(newton[1])], dtype=np.int64)
+                      [1 if newton[0] == "fixed" else 0, int(newton[1]), 0 if ccd is None else 1,
+                       c["max_iterations"], c["leaf_size"]], dtype=np.int64)
         self._h = lib().orc_session_create(self.nv, f.shape[0], _ptr(f), _ptr(p), alpha, beta, _ptr(of), _ptr(oi),
                                            ctypes.byref(st))
         if not self._h:
@@ -455,6 +471,15 @@ class Session:
         out["min_pivot"] = float(c[2])
         out["phi_before"] = c[3:3 + self.n_rows].tolist()
         out["phi"] = c[3 + self.n_rows:].tolist()
+        of, oi = np.zeros(3), np.zeros(10, dtype=np.int64)
+        if lib().orc_session_ccd(self._h, _ptr(of), _ptr(oi)):
+            kinds = ("vertex_triangle", "edge_edge")
+            out["ccd"] = {"tau_max": float(of[0]), "gap_abs": float(of[1]),
+                          "contact": (float(of[2]), kinds[int(oi[1])], int(oi[2]), int(oi[3])) if oi[0] else None,
+                          "iterations": int(oi[4]), "pairs_tested": int(oi[5]), "windows": int(oi[6]),
+                          "close_pairs": int(oi[7]), "touching_pairs": int(oi[8]), "exempt_pairs": int(oi[9])}
+        else:
+            out["ccd"] = None
         return out
 
     def positions(self):
@@ -497,3 +522,74 @@ def face_operators(faces, positions, u):
     avg, grad = np.zeros(f.shape[0]), np.zeros((f.shape[0], 3))
     lib().orc_face_operators(p.shape[0], f.shape[0], _ptr(f), _ptr(p), _ptr(u), _ptr(avg), _ptr(grad))
     return avg, grad
+
+
+# ---- continuous collision detection (ccd.rs) ------------------------------------------------
+
+
+def ccd_first_contact(faces, positions, velocity, tau_end, gap=0.05, reference_length=None, max_iterations=100,
+                      close_factor=0.5, leaf_size=8, band=None):
+    """``first_contact_with_band`` (ccd.rs:781) of the motion positions + t velocity on [0, tau_end],
+    mesh against mesh.  Returns a dict: tau_max, gap_abs, contact (None or (tau, kind, a, b): kind
+    "vertex_triangle" (vertex a, triangle b) or "edge_edge" (edge ids a < b of the sorted edge
+    list)), iterations, pairs_tested, windows, close_pairs, touching_pairs, exempt_pairs."""
+    f, p = _mesh(faces, positions)
+    u = np.ascontiguousarray(velocity, dtype=np.float64).reshape(-1, 3)
+    nv = p.shape[0]
+    if u.shape[0] != nv:
+        raise OracleError(1, "velocity: one vector per vertex")
+    of = np.array([gap, np.nan if reference_length is None else reference_length, close_factor])
+    oi = np.array([max_iterations, leaf_size], dtype=np.int64)
+    b = None if band is None else np.ascontiguousarray(band, dtype=np.uint8)
+    out_f, out_i = np.zeros(3), np.zeros(10, dtype=np.int64)
+    st = lib().orc_ccd_first_contact(nv, f.shape[0], _ptr(f), _ptr(p), _ptr(u), float(tau_end), _ptr(of), _ptr(oi),
+                                     None if b is None else _ptr(b), _ptr(out_f), _ptr(out_i))
+    if st != 0:
+        raise OracleError(st, "ccd")
+    kinds = ("vertex_triangle", "edge_edge")
+    return {
+        "tau_max": float(out_f[0]),
+        "gap_abs": float(out_f[1]),
+        "contact": (float(out_f[2]), kinds[int(out_i[1])], int(out_i[2]), int(out_i[3])) if out_i[0] else None,
+        "iterations": int(out_i[4]),
+        "pairs_tested": int(out_i[5]),
+        "windows": int(out_i[6]),
+        "close_pairs": int(out_i[7]),
+        "touching_pairs": int(out_i[8]),
+        "exempt_pairs": int(out_i[9]),
+    }
+
+
+def intersecting_pairs(faces, positions):
+    """The sorted face pairs (k × 2) that intersect (ccd.rs:1229)."""
+    f, p = _mesh(faces, positions)
+    n = lib().orc_intersecting_pairs(p.shape[0], f.shape[0], _ptr(f), _ptr(p), None, 0)
+    out = np.zeros((n, 2), dtype=np.uint32)
+    if n:
+        lib().orc_intersecting_pairs(p.shape[0], f.shape[0], _ptr(f), _ptr(p), _ptr(out), n)
+    return out
+
+
+def intersection_band(faces, positions):
+    """(per-vertex flags, intersecting pairs, vertices in the band) of ``IntersectionBand::new``
+    (ccd.rs:1292)."""
+    f, p = _mesh(faces, positions)
+    flags, counts = np.zeros(p.shape[0], dtype=np.uint8), np.zeros(2, dtype=np.int64)
+    lib().orc_intersection_band(p.shape[0], f.shape[0], _ptr(f), _ptr(p), _ptr(flags), _ptr(counts))
+    return flags, int(counts[0]), int(counts[1])
+
+
+def min_distance(faces, positions):
+    """The smallest distance over the non-adjacent vertex-triangle and edge-edge pairs (ccd.rs:1364)."""
+    f, p = _mesh(faces, positions)
+    return lib().orc_min_distance(p.shape[0], f.shape[0], _ptr(f), _ptr(p))
+
+
+def ccd_primitive(kind, *points):
+    """The curves crate's distances and the static tests on points (3-vectors): "point_triangle"
+    (p, a, b, c), "segment_segment" (p1, q1, p2, q2), "point_segment" (p, a, b),
+    "segment_triangle" (p, q, a, b, c: 1.0 if they meet), "triangles" (6 points: 1.0 if they
+    intersect)."""
+    k = {"point_triangle": 0, "segment_segment": 1, "point_segment": 2, "segment_triangle": 3, "triangles": 4}[kind]
+    x = np.ascontiguousarray(np.array(points, dtype=np.float64).reshape(-1))
+    return lib().orc_ccd_primitive(k, _ptr(x))

@@ -389,6 +389,19 @@ void* orc_session_create(int64_t nv, int64_t nf, const uint32_t* faces, const do
     }
   o.newton_fixed = opts_i[8] != 0;
   o.newton_count = (int)opts_i[9];
+  o.ccd = opts_i[10] != 0;
+  o.ccd_opts.gap = opts_f[9];
+  o.ccd_opts.reference_length = opts_f[10];
+  o.ccd_opts.close_factor = opts_f[11];
+  o.ccd_opts.max_iterations = (uint32_t)opts_i[11];
+  o.ccd_opts.leaf_size = (size_t)opts_i[12];
+  if (o.ccd) {
+    Error cerr;
+    if (!ccd_validate(o.ccd_opts, &cerr)) {
+      *status = cerr.status;
+      return nullptr;
+    }
+  }
   auto* s = new Session();
   if (!session_create(make_mesh(nv, nf, faces), make_pos(nv, pos), e, o, s, &err)) {
     delete s;
@@ -498,6 +511,90 @@ void orc_face_operators(int64_t nv, int64_t nf, const uint32_t* faces, const dou
     for (int c = 0; c < 3; ++c)
       for (int r = 0; r < 3; ++r) out[r] += u[t[c]] * g[c][r];
     for (int r = 0; r < 3; ++r) grad[3 * f + r] = out[r];
+  }
+}
+
+// ---- CCD -------------------------------------------------------------------------------------
+
+// The last step's CCD report in the layout of orc_ccd_first_contact; returns 1 when it ran.
+int orc_session_ccd(void* h, double* out_f, int64_t* out_i) {
+  auto* s = static_cast<Session*>(h);
+  const CcdReport& r = s->last_ccd;
+  out_f[0] = r.tau_max;
+  out_f[1] = r.gap_abs;
+  out_f[2] = r.contact.tau;
+  const int64_t vals[10] = {r.has_contact, r.contact.kind, r.contact.a, r.contact.b, (int64_t)r.iterations,
+                            (int64_t)r.pairs_tested, r.windows, (int64_t)r.close_pairs, (int64_t)r.touching_pairs,
+                            (int64_t)r.exempt_pairs};
+  for (int k = 0; k < 10; ++k) out_i[k] = vals[k];
+  return s->last_has_ccd ? 1 : 0;
+}
+
+// opts_f: gap, reference_length (NaN: none), close_factor; opts_i: max_iterations, leaf_size.
+// band: nv flags or NULL.  out_f: tau_max, gap_abs, contact tau; out_i: has_contact, kind, a, b,
+// iterations, pairs_tested, windows, close_pairs, touching_pairs, exempt_pairs.
+int orc_ccd_first_contact(int64_t nv, int64_t nf, const uint32_t* faces, const double* start, const double* vel,
+                          double tau_end, const double* opts_f, const int64_t* opts_i, const uint8_t* band,
+                          double* out_f, int64_t* out_i) {
+  Mesh m = make_mesh(nv, nf, faces);
+  Positions s = make_pos(nv, start), u = make_pos(nv, vel);
+  CcdOptions o;
+  o.gap = opts_f[0];
+  o.reference_length = opts_f[1];
+  o.close_factor = opts_f[2];
+  o.max_iterations = (uint32_t)opts_i[0];
+  o.leaf_size = (size_t)opts_i[1];
+  std::vector<uint8_t> b;
+  if (band) b.assign(band, band + nv);
+  CcdReport r;
+  Error err;
+  if (!first_contact(m, s, u, tau_end, o, band ? &b : nullptr, &r, &err)) return err.status;
+  out_f[0] = r.tau_max;
+  out_f[1] = r.gap_abs;
+  out_f[2] = r.contact.tau;
+  const int64_t vals[10] = {r.has_contact, r.contact.kind, r.contact.a, r.contact.b, (int64_t)r.iterations,
+                            (int64_t)r.pairs_tested, r.windows, (int64_t)r.close_pairs, (int64_t)r.touching_pairs,
+                            (int64_t)r.exempt_pairs};
+  for (int k = 0; k < 10; ++k) out_i[k] = vals[k];
+  return 0;
+}
+
+// The sorted intersecting face pairs: at most cap written to pairs (2 each); returns the count.
+int64_t orc_intersecting_pairs(int64_t nv, int64_t nf, const uint32_t* faces, const double* pos, uint32_t* pairs,
+                               int64_t cap) {
+  const auto out = intersecting_pairs(make_mesh(nv, nf, faces), make_pos(nv, pos));
+  for (size_t i = 0; i < out.size() && (int64_t)i < cap; ++i) {
+    pairs[2 * i] = out[i][0];
+    pairs[2 * i + 1] = out[i][1];
+  }
+  return (int64_t)out.size();
+}
+
+// The band flags (nv); counts2 = [intersecting pairs, vertices in the band].
+void orc_intersection_band(int64_t nv, int64_t nf, const uint32_t* faces, const double* pos, uint8_t* flags,
+                           int64_t* counts2) {
+  size_t np = 0, nb = 0;
+  const auto b = intersection_band(make_mesh(nv, nf, faces), make_pos(nv, pos), &np, &nb);
+  std::copy(b.begin(), b.end(), flags);
+  counts2[0] = (int64_t)np;
+  counts2[1] = (int64_t)nb;
+}
+
+double orc_min_distance(int64_t nv, int64_t nf, const uint32_t* faces, const double* pos) {
+  return min_distance(make_mesh(nv, nf, faces), make_pos(nv, pos));
+}
+
+// kind 0: point p (x[0..3]) to triangle x[3..12]; 1: segment x[0..6] to segment x[6..12];
+// 2: point to segment x[3..9]; 3: segment x[0..6] meets triangle x[6..15] (1.0 or 0.0);
+// 4: triangles x[0..9] and x[9..18] intersect.
+double orc_ccd_primitive(int kind, const double* x) {
+  auto v = [&](int i) { return V3{x[3 * i], x[3 * i + 1], x[3 * i + 2]}; };
+  switch (kind) {
+    case 0: return point_triangle_distance(v(0), v(1), v(2), v(3));
+    case 1: return segment_segment_distance(v(0), v(1), v(2), v(3));
+    case 2: return point_segment_distance(v(0), v(1), v(2));
+    case 3: return segment_meets_triangle(v(0), v(1), {v(2), v(3), v(4)}) ? 1.0 : 0.0;
+    default: return triangles_intersect({v(0), v(1), v(2)}, {v(3), v(4), v(5)}) ? 1.0 : 0.0;
   }
 }
 

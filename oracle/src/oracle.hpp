@@ -314,6 +314,28 @@ std::vector<double> schur_apply_c(const Schur& S, const double* v);
 std::vector<double> schur_direction(const Schur& S, const std::vector<double>& y);   // y − Z N⁻¹ C_N y
 std::vector<double> schur_correction(const Schur& S, const std::vector<double>& phi);  // Z N⁻¹ Φ_N
 
+// ---- continuous collision detection options and report (ccd.cpp) ----
+struct CcdOptions {
+  double gap = 0.05;                     // relative to the reference length
+  double reference_length = std::nan("");  // NaN: the mean edge length of the start positions
+  uint32_t max_iterations = 100;
+  double close_factor = 0.5;
+  size_t leaf_size = 8;
+};
+// kind 0: vertex a against triangle b; kind 1: edges a < b (ids of the sorted unique edge list)
+struct CcdContact {
+  double tau = 0.0;
+  int kind = 0;
+  uint32_t a = 0, b = 0;
+};
+struct CcdReport {
+  double tau_max = 0.0, gap_abs = 0.0;
+  bool has_contact = false;
+  CcdContact contact;
+  uint64_t iterations = 0, pairs_tested = 0, close_pairs = 0, touching_pairs = 0, exempt_pairs = 0;
+  uint32_t windows = 0;
+};
+
 struct SessionOptions {
   bool energy_exact = false;  // EnergyEval::Exact instead of Barnes–Hut
   double theta = 0.5;
@@ -331,6 +353,9 @@ struct SessionOptions {
   std::vector<double> targets;
   bool newton_fixed = false;
   int newton_count = 5;
+  // the CCD cap of τ₀ (session.rs:917-956), reference_length NaN: the session's ℓ̄₀
+  bool ccd = false;
+  CcdOptions ccd_opts;
 };
 struct Session {
   Mesh mesh;
@@ -342,6 +367,8 @@ struct Session {
   std::vector<double> row_targets;  // of the nonlinear rows
   // the last step's constraint rows (as in StepReport)
   std::vector<double> last_phi_before, last_phi;
+  bool last_has_ccd = false;
+  CcdReport last_ccd;
   int last_newton_rounds = 0;
   double last_min_pivot = 1.0;
 };
@@ -364,9 +391,27 @@ struct StepReport {
   std::vector<double> phi_before, phi;
   int newton_rounds = 0;
   double min_pivot = 1.0;
+  // the CCD (when it ran: ccd on, τ₀ finite and positive); tau0 above is after its cap
+  bool has_ccd = false;
+  CcdReport ccd;
 };
 bool session_create(const Mesh& m, const Positions& p, const Exponents& e, const SessionOptions& o, Session* s,
                     Error* err);
 bool session_step(Session& s, int threads, StepReport* rep, Error* err);
+
+// ---- continuous collision detection (ccd.cpp; crates/ddg-repulsive-surfaces/src/ccd.rs) ----
+double point_segment_distance(V3 p, V3 a, V3 b);
+double segment_segment_distance(V3 p1, V3 q1, V3 p2, V3 q2);
+double point_triangle_distance(V3 p, V3 a, V3 b, V3 c);
+bool segment_meets_triangle(V3 p, V3 q, const std::array<V3, 3>& tri);
+bool triangles_intersect(const std::array<V3, 3>& t1, const std::array<V3, 3>& t2);
+std::vector<std::array<uint32_t, 2>> mesh_edges(const Mesh& m);
+std::vector<std::array<uint32_t, 2>> intersecting_pairs(const Mesh& m, const Positions& p);
+std::vector<uint8_t> intersection_band(const Mesh& m, const Positions& p, size_t* n_pairs, size_t* n_vertices);
+double min_distance(const Mesh& m, const Positions& p);
+bool ccd_validate(const CcdOptions& o, Error* err);
+// band: per-vertex flags or nullptr (an empty band is no band)
+bool first_contact(const Mesh& m, const Positions& start, const Positions& vel, double tau_end, const CcdOptions& o,
+                   const std::vector<uint8_t>* band, CcdReport* rep, Error* err);
 
 }  // namespace orc

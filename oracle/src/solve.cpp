@@ -583,8 +583,24 @@ bool session_step(Session& s, int threads, StepReport* rep, Error* err) {
     maxd = std::max(maxd, std::sqrt(x[3 * v] * x[3 * v] + x[3 * v + 1] * x[3 * v + 1] + x[3 * v + 2] * x[3 * v + 2]));
   const double xnorm = std::sqrt(xn2);
   // 3. Armijo line search (session.rs:906, :958-1069)
-  const double tau0 = start_tau(s.opts, s.last_tau, xnorm, maxd, s.l0);
+  double tau0 = start_tau(s.opts, s.last_tau, xnorm, maxd, s.l0);
   const double tau_min = s.opts.tau_min_rel * s.l0 / xinf;
+  // the CCD cap (session.rs:917-956): the band of the faces that intersect at the start, then
+  // τ₀ ← min(τ₀, ⅔ τ_max) of first_contact_with_band along −x
+  rep->has_ccd = false;
+  if (s.opts.ccd) {
+    size_t np = 0, nb = 0;
+    const std::vector<uint8_t> band = intersection_band(m, s.positions, &np, &nb);
+    if (std::isfinite(tau0) && tau0 > 0.0) {
+      CcdOptions co = s.opts.ccd_opts;
+      if (std::isnan(co.reference_length)) co.reference_length = s.l0;
+      Positions vel(nv);
+      for (size_t v = 0; v < nv; ++v) vel[v] = {-x[3 * v], -x[3 * v + 1], -x[3 * v + 2]};
+      if (!first_contact(m, s.positions, vel, tau0, co, nb ? &band : nullptr, &rep->ccd, err)) return false;
+      if (rep->ccd.has_contact) tau0 = fmin_(tau0, 2.0 / 3.0 * rep->ccd.tau_max);
+      rep->has_ccd = true;
+    }
+  }
   rep->energy0 = f0;
   rep->slope = slope;
   rep->gradient_norm = std::sqrt(gn2);
@@ -675,6 +691,8 @@ bool session_step(Session& s, int threads, StepReport* rep, Error* err) {
   lap(4);
   s.last_phi_before = rep->phi_before;
   s.last_phi = rep->phi;
+  s.last_has_ccd = rep->has_ccd;
+  s.last_ccd = rep->ccd;
   s.last_newton_rounds = rep->newton_rounds;
   s.last_min_pivot = rep->min_pivot;
   s.positions = np;
