@@ -371,6 +371,54 @@ def cpu_baseline(args, f, p, opts) -> dict:
     return out
 
 
+def ccd_leg(pkg, ctx, args, p, f) -> dict:
+    """The continuous collision detection at the same mesh (first_contact_with_band, ccd.rs:781):
+    a smooth synthetic motion of four mean edge lengths for its fastest vertex over [0, 4] (three
+    windows), the device's median of 5 calls after one warm-up, the FP64 oracle once on one host
+    thread (the CPU baseline of this leg; its report must equal the device's bit for bit), and the
+    step with the CCD cap on (3 steps after one warm-up: ms per step and the CCD's share)."""
+    vel = np.stack([np.sin(3 * p[:, 1]), np.cos(2 * p[:, 2]), -2.0 * p[:, 2] * np.abs(p[:, 0])], axis=1)
+    e = {(min(a, b), max(a, b)) for t in f.tolist() for a, b in ((t[0], t[1]), (t[1], t[2]), (t[2], t[0]))}
+    ea = np.array(sorted(e))
+    l0 = float(np.mean(np.linalg.norm(p[ea[:, 0]] - p[ea[:, 1]], axis=1)))
+    vel *= l0 / np.max(np.linalg.norm(vel, axis=1))
+    surf = pkg.Surface(ctx, f, p)
+    pkg.first_contact(surf, vel, 4.0)
+    times, rep = [], None
+    for _ in range(5):
+        c0 = time.perf_counter()
+        rep = pkg.first_contact(surf, vel, 4.0)
+        times.append(time.perf_counter() - c0)
+    out = {"workload": "first_contact_with_band, 81,920 faces, smooth synthetic motion of 4 mean edge lengths",
+           "device_ms": float(np.median(times) * 1e3), "windows": rep.windows, "pairs_tested": rep.pairs_tested,
+           "iterations": rep.iterations, "contact": rep.contact}
+    if not args.no_cpu_baseline:
+        import oracle
+
+        print("bench: oracle CCD baseline on one thread", file=sys.stderr, flush=True)
+        c0 = time.perf_counter()
+        ref = oracle.ccd_first_contact(f, p, vel, 4.0)
+        out["cpu_ms"] = (time.perf_counter() - c0) * 1e3
+        out["cpu_kind"] = "port"
+        out["cpu_cores"] = 1
+        out["matches_oracle"] = (rep.tau_max == ref["tau_max"] and rep.contact == ref["contact"]
+                                 and (rep.iterations, rep.pairs_tested, rep.windows)
+                                 == (ref["iterations"], ref["pairs_tested"], ref["windows"]))
+    opts = pkg.StepOptions(ccd=pkg.CcdOptions())
+    ses = pkg.Session(surf, opts)
+    ses.step()
+    ctx.synchronize()
+    c0 = time.perf_counter()
+    reps = [ses.step() for _ in range(3)]
+    ctx.synchronize()
+    out["step_ms_with_ccd"] = (time.perf_counter() - c0) * 1e3 / 3
+    out["step_ccd_ms"] = float(np.mean([r.seconds["ccd"] for r in reps]) * 1e3)
+    out["step_tau_ccd"] = [r.tau_ccd for r in reps]
+    ses.close()
+    surf.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -384,6 +432,7 @@ def main() -> int:
                     help="l6: the headline step (default); l7: the level-7 scale run (theta 0.25, energy drift "
                          "against the FP64 oracle after steps 1, 10 and 20, fallback rate)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned preconditioner leg")
+    ap.add_argument("--no-ccd", action="store_true", help="skip the collision detection leg")
     ap.add_argument("--learned-fields", type=int, default=40, help="bump fields of the training trajectories")
     ap.add_argument("--learned-steps", type=int, default=50, help="plain steps per field")
     ap.add_argument("--learned-krylov", type=int, default=0,
@@ -485,6 +534,7 @@ def main() -> int:
     learned = None
     if not args.no_learned:
         learned = learned_leg(pkg, ctx, args, opts, p, f, ms, its)
+    ccd = None if args.no_ccd else ccd_leg(pkg, ctx, args, np.asarray(p), np.asarray(f, dtype=np.uint32))
     line = {
         "metric": METRIC,
         "value": 1e3 / ms,
@@ -513,6 +563,7 @@ def main() -> int:
         "rooflines": rooflines,
         "cpu_baseline": cpu,
         "learned": learned,
+        "ccd": ccd,
         "traffic_scale": {k: v for k, v in traffic.items() if k.endswith("_scale")},
         "traffic_error": traffic.get("error"),
         "kernel_ms_step1": {k: round(v["ms"], 4) for k, v in sorted(kt.items(), key=lambda kv: -kv[1]["ms"])[:16]},

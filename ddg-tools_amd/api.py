@@ -25,7 +25,7 @@ import time
 
 import numpy as np
 
-from . import (Context, EnergyEval, Exponents, GmresOptions, RsError, SaddleOptions, Session, StartRule, StepOptions,
+from . import (CcdOptions, Context, EnergyEval, Exponents, GmresOptions, RsError, SaddleOptions, Session, StartRule, StepOptions,
                Surface, BhOptions, energy as _energy)
 
 UNSUPPORTED = 12
@@ -38,13 +38,14 @@ DEFAULTS = {
     "backend": "hierarchical", "theta": 0.5, "leaf_size": 8, "energy_eval": "barnes_hut", "method":
     "projected_gradient", "barycenter_mode": "auto", "barycenter_shift": False, "armijo_c": 1e-2,
     "armijo": "standard", "start_lengths": 1.0, "start_factor": 2.0, "start_tau": None, "start": "grow_previous",
-    "tau_min_rel": 1e-12, "max_halvings": None, "remesh": False, "ccd": False, "max_iterations": 1000,
+    "tau_min_rel": 1e-12, "max_halvings": None, "remesh": False, "ccd": True, "ccd_gap": 0.05, "ccd_max_iterations": 100,
+    "ccd_close_factor": 0.5, "max_iterations": 1000,
     "stuck_limit": 5, "stop": "energy_stall", "stop_relative": 1e-7, "stop_window": 10, "threads": None,
     "constraints": None, "barycenter": True, "newton_steps": None, "newton_max": 5,
 }
 # the reference's names that only select paths outside this one: allowed at their default only
 _ONLY_DEFAULT = {"energy", "metric", "gmres_side", "gpu_determinism", "method", "barycenter_mode", "barycenter_shift",
-                 "armijo", "start_factor", "remesh", "ccd"}
+                 "armijo", "start_factor", "remesh"}
 
 
 def _unsupported(name, value, what):
@@ -103,6 +104,9 @@ def _options(options: dict):
         start=start,
         constraints=tuple(rows),
         newton=newton,
+        # LineSearchOptions::ccd (repulsive_surfaces.rs:1042-1048): on by default
+        ccd=CcdOptions(gap=float(o["ccd_gap"]), max_iterations=int(o["ccd_max_iterations"]),
+                       close_factor=float(o["ccd_close_factor"])) if o["ccd"] else None,
     )
     return o, step
 
@@ -193,7 +197,7 @@ class RepulsiveSurfaceSession:
             "session_status": self._status, "n_vertices": self._surface.nv, "n_faces": self._surface.nf,
             "energy_before": r.energy_before, "energy_after": r.energy_after, "tangent_point_after": r.energy_after,
             "penalties_after": [], "gradient_norm": r.gradient_norm, "sobolev_norm_sq": r.sobolev_norm_sq,
-            "tau": r.tau, "tau0": r.tau0, "tau_ccd": None, "halvings": r.halvings, "newton_steps": r.newton_rounds,
+            "tau": r.tau, "tau0": r.tau0, "tau_ccd": r.tau_ccd, "halvings": r.halvings, "newton_steps": r.newton_rounds,
             "phi_before": list(phi_before), "phi": list(phi), "schur_pivot": r.min_pivot,
             "trials": [{"tau": t, "energy": e, "outcome": ("accepted", "armijo_failed", "non_finite")[o]}
                        for t, e, o in r.trial_list],

@@ -14,10 +14,11 @@ def test_session_api_matches_native(pkg, ctx, tmp_path):
     pkg.io.write_obj(path, p, f)
     p1, f1 = pkg.io.read_obj(path)
     api = pkg.api.RepulsiveSurfaceSession(p1.reshape(-1), f1.reshape(-1), device="rocm", theta=0.5)
-    native = pkg.Session(pkg.Surface(ctx, f, p))
+    native = pkg.Session(pkg.Surface(ctx, f, p), pkg.StepOptions(ccd=pkg.CcdOptions()))  # the API's default CCD
     for _ in range(2):
         a, n = api.step(), native.step()
         assert a["status"] == "accepted" and a["energy_before"] == n.energy_before and a["tau"] == n.tau
+        assert a["tau_ccd"] == n.tau_ccd and n.ccd is not None
         assert a["linear"]["converged"] and a["linear"]["iterations"] == n.gmres.iterations
     assert np.array_equal(api.positions.reshape(-1, 3), native.positions())
     out = api.run(max_steps=2)
@@ -35,7 +36,8 @@ def test_session_api_constraints(pkg, ctx):
     cons = [{"type": "barycenter"}, {"type": "area"}, {"type": "volume"}]
     api = pkg.api.RepulsiveSurfaceSession(p, f, device="rocm", constraints=cons, energy_eval="exact", chi=0.0)
     native = pkg.Session(pkg.Surface(ctx, f, p), pkg.StepOptions(energy_exact=True, saddle=pkg.SaddleOptions(chi=0.0),
-                                                                 constraints=(("area", None), ("volume", None))))
+                                                                 constraints=(("area", None), ("volume", None)),
+                                                                 ccd=pkg.CcdOptions()))
     for _ in range(2):
         a, n = api.step(), native.step()
         assert a["status"] == "accepted" and a["energy_after"] == n.energy_after

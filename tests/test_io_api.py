@@ -58,6 +58,11 @@ def test_session_options():
                                                 {"type": "volume", "target": 2.5}], "newton_steps": 1})
     assert step.constraints == (("area", None), ("volume", 2.5)) and step.newton == ("fixed", 1)
     assert pkg.api._options({})[1].constraints == () and pkg.api._options({})[1].newton == ("tol", 5)
+    # the CCD cap is on by default, as in the reference (repulsive_surfaces.rs:1048)
+    assert pkg.api._options({})[1].ccd == pkg.CcdOptions()
+    assert pkg.api._options({"ccd": False})[1].ccd is None
+    c = pkg.api._options({"ccd_gap": 0.1, "ccd_max_iterations": 7, "ccd_close_factor": 0.25})[1].ccd
+    assert (c.gap, c.max_iterations, c.close_factor, c.reference_length) == (0.1, 7, 0.25, None)
     for bad, kind in (({"remesh": True}, "Unsupported"), ({"device": "cuda"}, "Unsupported"),
                       ({"metric": "h1"}, "Unsupported"), ({"penalties": [1]}, "Unsupported"),
                       ({"constraints": [{"type": "area"}]}, "Unsupported"),
