@@ -143,6 +143,16 @@ def _declare(L):
     L.orc_ccd_primitive.argtypes = [_INT, _P]
     L.orc_fractional_symbol_ratio.argtypes = [_D]
     L.orc_fractional_symbol_ratio.restype = _D
+    L.orc_remesh.argtypes = [_I64, _I64, _P, _P, _I64, _P, _D, _P, _P, _D, _P, _P, ctypes.POINTER(_INT),
+                             ctypes.c_char_p, _I64]
+    L.orc_remesh.restype = _P
+    L.orc_remesh_op.argtypes = [_I64, _I64, _P, _P, _I64, _P, _INT, ctypes.c_uint32, ctypes.c_uint32, _INT, _INT,
+                                ctypes.POINTER(_INT), ctypes.POINTER(_INT), ctypes.POINTER(_INT), ctypes.c_char_p, _I64]
+    L.orc_remesh_op.restype = _P
+    L.orc_remeshed_sizes.argtypes = [_P, _P]
+    L.orc_remeshed_read.argtypes = [_P] * 8
+    L.orc_remeshed_free.argtypes = [_P]
+    L.orc_remesh_geom.argtypes = [_INT, _P, _P]
 
 
 def _ptr(a: np.ndarray):
@@ -593,3 +603,86 @@ def ccd_primitive(kind, *points):
     k = {"point_triangle": 0, "segment_segment": 1, "point_segment": 2, "segment_triangle": 3, "triangles": 4}[kind]
     x = np.ascontiguousarray(np.array(points, dtype=np.float64).reshape(-1))
     return lib().orc_ccd_primitive(k, _ptr(x))
+
+
+# ---- remeshing (remesh/mod.rs, remesh/mesh.rs) ----------------------------------------------
+
+# mesh.rs:75 ``RefusalKind::ALL``
+REFUSAL_KINDS = ("NoSuchEdge", "BothPinned", "LinkCondition", "Foldover", "RepeatedFace", "NonManifoldResult",
+                 "BoundaryEdge", "NotDelaunay", "ExistingEdge", "Guard", "DegenerateFace")
+_REPORT_HEAD = ("splits", "collapses", "flips", "splits_deferred", "smoothing_moves", "smoothing_halved",
+                "smoothing_refused", "smoothing_skipped", "flip_cap_hits", "n_vertices", "n_faces", "guard_refused",
+                "guard_tested", "guard_exempt")
+
+
+def _remeshed(h) -> dict:
+    """Reads and frees a Remeshed handle: positions, faces, face_parents, vertex map (CSR), pinned."""
+    L = lib()
+    try:
+        n = np.zeros(4, dtype=np.int64)
+        L.orc_remeshed_sizes(h, _ptr(n))
+        nv, nf, nm, npin = (int(x) for x in n)
+        out = {"positions": np.zeros((nv, 3)), "faces": np.zeros((nf, 3), dtype=np.uint32),
+               "face_parents": np.zeros(nf, dtype=np.uint32), "map_offsets": np.zeros(nv + 1, dtype=np.uint32),
+               "map_sources": np.zeros(nm, dtype=np.uint32), "map_weights": np.zeros(nm),
+               "pinned": np.zeros(npin, dtype=np.uint32)}
+        L.orc_remeshed_read(h, *[_ptr(out[k]) for k in ("positions", "faces", "face_parents", "map_offsets",
+                                                         "map_sources", "map_weights", "pinned")])
+        return out
+    finally:
+        L.orc_remeshed_free(h)
+
+
+def remesh(faces, positions, l0, pinned=(), split=1.5, collapse=0.5, rounds=5, rho=0.5, flip_cap_per_vertex=100,
+           order="flip_then_smooth", enabled=("split", "collapse", "flip", "smooth"), max_growth=2.0,
+           guard_gap=None, band=None):
+    """One remeshing pass (mod.rs:235 ``remesh``).  ``guard_gap``: the absolute gap of the CCD
+    collapse guard (ccd.rs:1448; None: no guard); ``band``: its per-vertex exempt flags.  Returns
+    (result dict as ``_remeshed``, report dict with the refusal counts per kind)."""
+    f, p = _mesh(faces, positions)
+    pins = np.ascontiguousarray(pinned, dtype=np.uint32)
+    of = np.array([split, collapse, rho, max_growth], dtype=np.float64)
+    oi = np.array([rounds, flip_cap_per_vertex, 0 if order == "flip_then_smooth" else 1,
+                   "split" in enabled, "collapse" in enabled, "flip" in enabled, "smooth" in enabled], dtype=np.int64)
+    b = None if band is None else np.ascontiguousarray(band, dtype=np.uint8)
+    rep = np.zeros(47, dtype=np.int64)
+    st = _INT(0)
+    msg = ctypes.create_string_buffer(512)
+    h = lib().orc_remesh(p.shape[0], f.shape[0], _ptr(f), _ptr(p), pins.size, _ptr(pins), float(l0), _ptr(of),
+                         _ptr(oi), 0.0 if guard_gap is None else float(guard_gap), None if b is None else _ptr(b),
+                         _ptr(rep), ctypes.byref(st), msg, 512)
+    if not h:
+        raise OracleError(st.value, msg.value.decode())
+    report = {k: int(v) for k, v in zip(_REPORT_HEAD, rep[:14])}
+    for j, name in enumerate(("refused_splits", "refused_collapses", "refused_flips")):
+        report[name] = {k: int(v) for k, v in zip(REFUSAL_KINDS, rep[14 + 11 * j: 25 + 11 * j])}
+    return _remeshed(h), report
+
+
+def remesh_op(faces, positions, op, a, b, pinned=(), delaunay=False, foldover=False):
+    """One ``EditMesh`` operation (mesh.rs:592 split, :679 collapse, :888 flip) and ``finish``.
+    Returns (refusal kind or None, reordered, result dict)."""
+    f, p = _mesh(faces, positions)
+    pins = np.ascontiguousarray(pinned, dtype=np.uint32)
+    ref, reo, st = _INT(0), _INT(0), _INT(0)
+    msg = ctypes.create_string_buffer(512)
+    h = lib().orc_remesh_op(p.shape[0], f.shape[0], _ptr(f), _ptr(p), pins.size, _ptr(pins),
+                            {"split": 0, "collapse": 1, "flip": 2}[op], a, b, int(delaunay), int(foldover),
+                            ctypes.byref(ref), ctypes.byref(reo), ctypes.byref(st), msg, 512)
+    if not h:
+        raise OracleError(st.value, msg.value.decode())
+    return (None if ref.value == 0 else REFUSAL_KINDS[ref.value - 1]), bool(reo.value), _remeshed(h)
+
+
+def remesh_geom(kind, *points):
+    """The remesher's predicates (geom.rs): "circumcenter" (3 points → 3-vector), "delaunay"
+    (p, q, c, d → bool), "folds" (before, after normals → bool), "triangle_distance" (6 points)."""
+    k = {"circumcenter": 0, "delaunay": 1, "folds": 2, "triangle_distance": 3}[kind]
+    x = np.ascontiguousarray(np.array(points, dtype=np.float64).reshape(-1))
+    out = np.zeros(3)
+    lib().orc_remesh_geom(k, _ptr(x), _ptr(out))
+    if k == 0:
+        return out
+    if k in (1, 2):
+        return bool(out[0])
+    return float(out[0])

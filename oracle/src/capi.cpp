@@ -598,4 +598,136 @@ double orc_ccd_primitive(int kind, const double* x) {
   }
 }
 
+
+// ---- remeshing -------------------------------------------------------------------------------
+// Options: f = [split, collapse, rho, max_growth], i = [rounds, flip_cap_per_vertex, order,
+// split on, collapse on, flip on, smooth on].  Report (47 int64): splits, collapses, flips,
+// splits_deferred, smoothing moves / halved / refused / skipped, flip_cap_hits, n_vertices, n_faces,
+// guard refused / tested / exempt, then refused splits, collapses, flips (11 kinds each, in the
+// order of mesh.rs:75 RefusalKind::ALL).
+namespace {
+RemeshOptions remesh_options(const double* f, const int64_t* i) {
+  RemeshOptions o;
+  o.split = f[0];
+  o.collapse = f[1];
+  o.rho = f[2];
+  o.max_growth = f[3];
+  o.rounds = (int)i[0];
+  o.flip_cap_per_vertex = (size_t)i[1];
+  o.order = (int)i[2];
+  o.en_split = i[3] != 0;
+  o.en_collapse = i[4] != 0;
+  o.en_flip = i[5] != 0;
+  o.en_smooth = i[6] != 0;
+  return o;
+}
+void write_remesh_report(const RemeshReport& r, int64_t* out) {
+  const int64_t head[14] = {(int64_t)r.splits,          (int64_t)r.collapses,         (int64_t)r.flips,
+                            (int64_t)r.splits_deferred, (int64_t)r.smoothing_moves,   (int64_t)r.smoothing_halved,
+                            (int64_t)r.smoothing_refused, (int64_t)r.smoothing_skipped, (int64_t)r.flip_cap_hits,
+                            (int64_t)r.n_vertices,      (int64_t)r.n_faces,           (int64_t)r.guard_refused,
+                            (int64_t)r.guard_tested,    (int64_t)r.guard_exempt};
+  for (int k = 0; k < 14; ++k) out[k] = head[k];
+  for (int k = 0; k < 11; ++k) {
+    out[14 + k] = (int64_t)r.refused_splits[k];
+    out[25 + k] = (int64_t)r.refused_collapses[k];
+    out[36 + k] = (int64_t)r.refused_flips[k];
+  }
+}
+}  // namespace
+
+void* orc_remesh(int64_t nv, int64_t nf, const uint32_t* faces, const double* pos, int64_t npin, const uint32_t* pins,
+                 double l0, const double* opts_f, const int64_t* opts_i, double guard_gap, const uint8_t* band,
+                 int64_t* report, int* status, char* msg, int64_t msglen) {
+  Error err;
+  std::vector<uint32_t> pv(pins, pins + npin);
+  std::vector<uint8_t> bv;
+  if (band) bv.assign(band, band + nv);
+  auto* out = new Remeshed();
+  RemeshReport rep;
+  if (!remesh(make_mesh(nv, nf, faces), make_pos(nv, pos), pv, l0, remesh_options(opts_f, opts_i), guard_gap,
+              band ? &bv : nullptr, out, &rep, &err)) {
+    delete out;
+    *status = fail(err, msg, msglen, nullptr);
+    return nullptr;
+  }
+  write_remesh_report(rep, report);
+  *status = OK;
+  return out;
+}
+
+// Single operation (op 0 split, 1 collapse, 2 flip) then finish; *refusal the RemeshRefusal.
+void* orc_remesh_op(int64_t nv, int64_t nf, const uint32_t* faces, const double* pos, int64_t npin,
+                    const uint32_t* pins, int op, uint32_t a, uint32_t b, int delaunay, int foldover, int* refusal,
+                    int* reordered, int* status, char* msg, int64_t msglen) {
+  Error err;
+  std::vector<uint32_t> pv(pins, pins + npin);
+  EditMesh* m = nullptr;
+  if (!edit_mesh_new(make_mesh(nv, nf, faces), make_pos(nv, pos), pv, &m, &err)) {
+    *status = fail(err, msg, msglen, nullptr);
+    return nullptr;
+  }
+  bool ro = false;
+  *refusal = edit_mesh_op(*m, op, a, b, delaunay != 0, foldover != 0, &ro);
+  *reordered = ro ? 1 : 0;
+  auto* out = new Remeshed();
+  const bool ok = edit_mesh_finish(*m, out, &err);
+  edit_mesh_free(m);
+  if (!ok) {
+    delete out;
+    *status = fail(err, msg, msglen, nullptr);
+    return nullptr;
+  }
+  *status = OK;
+  return out;
+}
+
+// sizes: nv, nf, entries of the vertex map, pins
+void orc_remeshed_sizes(void* h, int64_t* sizes) {
+  auto* r = (Remeshed*)h;
+  sizes[0] = (int64_t)r->positions.size();
+  sizes[1] = (int64_t)r->faces.size();
+  sizes[2] = (int64_t)r->map_sources.size();
+  sizes[3] = (int64_t)r->pinned.size();
+}
+
+void orc_remeshed_read(void* h, double* pos, uint32_t* faces, uint32_t* parents, uint32_t* map_offsets,
+                       uint32_t* map_sources, double* map_weights, uint32_t* pins) {
+  auto* r = (Remeshed*)h;
+  for (size_t v = 0; v < r->positions.size(); ++v)
+    for (int k = 0; k < 3; ++k) pos[3 * v + k] = r->positions[v][k];
+  for (size_t f = 0; f < r->faces.size(); ++f) {
+    for (int k = 0; k < 3; ++k) faces[3 * f + k] = r->faces[f][k];
+    parents[f] = r->face_parents[f];
+  }
+  std::copy(r->map_offsets.begin(), r->map_offsets.end(), map_offsets);
+  std::copy(r->map_sources.begin(), r->map_sources.end(), map_sources);
+  std::copy(r->map_weights.begin(), r->map_weights.end(), map_weights);
+  std::copy(r->pinned.begin(), r->pinned.end(), pins);
+}
+
+void orc_remeshed_free(void* h) { delete (Remeshed*)h; }
+
+// Geometry predicates of the remesher (geom.rs): kind 0 circumcenter (x: 9 → out 3), 1 Delaunay
+// test (x: p q c d, 12 → out[0] 0/1), 2 folds (x: before, after, 6 → out[0] 0/1), 3 triangle
+// distance (x: 18 → out[0]).
+void orc_remesh_geom(int kind, const double* x, double* out) {
+  auto v = [&](int i) { return V3{x[3 * i], x[3 * i + 1], x[3 * i + 2]}; };
+  switch (kind) {
+    case 0: {
+      const V3 c = remesh_circumcenter({v(0), v(1), v(2)});
+      for (int k = 0; k < 3; ++k) out[k] = c[k];
+      break;
+    }
+    case 1:
+      out[0] = remesh_violates_delaunay(v(0), v(1), v(2), v(3)) ? 1.0 : 0.0;
+      break;
+    case 2:
+      out[0] = remesh_folds(v(0), v(1)) ? 1.0 : 0.0;
+      break;
+    default:
+      out[0] = remesh_triangle_distance({v(0), v(1), v(2)}, {v(3), v(4), v(5)});
+  }
+}
+
 }  // extern "C"

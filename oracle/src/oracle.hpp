@@ -414,4 +414,66 @@ bool ccd_validate(const CcdOptions& o, Error* err);
 bool first_contact(const Mesh& m, const Positions& start, const Positions& vel, double tau_end, const CcdOptions& o,
                    const std::vector<uint8_t>* band, CcdReport* rep, Error* err);
 
+// ---- remeshing (remesh.cpp; crates/ddg-repulsive-surfaces/src/remesh/) ------------------------
+// Refusal kinds in the order of mesh.rs:75 `RefusalKind::ALL` (1-based; 0 = done).
+enum RemeshRefusal : int {
+  REMESH_OK = 0,
+  REMESH_NO_SUCH_EDGE = 1,
+  REMESH_BOTH_PINNED = 2,
+  REMESH_LINK_CONDITION = 3,
+  REMESH_FOLDOVER = 4,
+  REMESH_REPEATED_FACE = 5,
+  REMESH_NON_MANIFOLD_RESULT = 6,
+  REMESH_BOUNDARY_EDGE = 7,
+  REMESH_NOT_DELAUNAY = 8,
+  REMESH_EXISTING_EDGE = 9,
+  REMESH_GUARD = 10,
+  REMESH_DEGENERATE_FACE = 11,
+};
+// mod.rs:96 `RemeshOptions` (order 0: flip then smooth, 1: smooth then flip)
+struct RemeshOptions {
+  double split = 1.5, collapse = 0.5;
+  int rounds = 5;
+  double rho = 0.5;
+  size_t flip_cap_per_vertex = 100;
+  int order = 0;
+  bool en_split = true, en_collapse = true, en_flip = true, en_smooth = true;
+  double max_growth = 2.0;
+};
+// mod.rs:187 `RemeshReport` (+ the collapse guard's counters)
+struct RemeshReport {
+  uint64_t splits = 0, collapses = 0, flips = 0;
+  uint64_t refused_splits[11] = {}, splits_deferred = 0, refused_collapses[11] = {}, refused_flips[11] = {};
+  uint64_t smoothing_moves = 0, smoothing_halved = 0, smoothing_refused = 0, smoothing_skipped = 0;
+  uint32_t flip_cap_hits = 0;
+  size_t n_vertices = 0, n_faces = 0;
+  uint64_t guard_refused = 0, guard_tested = 0, guard_exempt = 0;
+};
+// mesh.rs:210 `Remeshed` with the vertex map in CSR form (transport.rs:15)
+struct Remeshed {
+  Positions positions;
+  std::vector<std::array<uint32_t, 3>> faces;
+  std::vector<uint32_t> face_parents, map_offsets, map_sources, pinned;
+  std::vector<double> map_weights;
+  size_t n_old = 0;
+};
+struct EditMesh;
+struct CollapseGuard;
+bool edit_mesh_new(const Mesh& s, const Positions& p, const std::vector<uint32_t>& pins, EditMesh** out, Error* err);
+void edit_mesh_free(EditMesh* m);
+// op 0 split, 1 collapse, 2 flip; returns a RemeshRefusal
+int edit_mesh_op(EditMesh& m, int op, uint32_t a, uint32_t b, bool delaunay, bool foldover, bool* reordered);
+bool edit_mesh_finish(const EditMesh& m, Remeshed* out, Error* err);
+// guard_gap > 0: the CCD collapse guard with that absolute gap and the optional band
+bool remesh(const Mesh& s, const Positions& p, const std::vector<uint32_t>& pins, double l0, const RemeshOptions& o,
+            double guard_gap, const std::vector<uint8_t>* band, Remeshed* out, RemeshReport* rep, Error* err);
+double remesh_angle_between(V3 u, V3 v);
+V3 remesh_tri_normal(const std::array<V3, 3>& p);
+bool remesh_folds(V3 before, V3 after);
+bool remesh_violates_delaunay(V3 p, V3 q, V3 c, V3 d);
+V3 remesh_circumcenter(const std::array<V3, 3>& p);
+V3 remesh_barycenter(const std::array<V3, 3>& p);
+bool remesh_smoothing_displacement(V3 x, double rho, const std::vector<std::pair<double, V3>>& faces, V3* u);
+double remesh_triangle_distance(const std::array<V3, 3>& t1, const std::array<V3, 3>& t2);
+
 }  // namespace orc

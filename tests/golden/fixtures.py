@@ -63,6 +63,12 @@ def read_case(name: str) -> dict:
             c["maxhalv"] = int(t[1])
         elif k == "newton":
             c["newton"] = (t[1], int(t[2]))
+        elif k == "op":
+            c["op"] = (t[1], int(t[2]), int(t[3]))
+        elif k == "pinned":
+            c["pinned"] = [int(x) for x in t[1:]]
+        elif k == "legality":
+            c["legality"] = set(t[1:])
     c.setdefault("exponents", (6.0, 12.0))
     c["positions"] = np.array(c["positions"], dtype=np.float64)
     c["faces"] = np.array(c["faces"], dtype=np.uint32)
@@ -104,6 +110,14 @@ def read_out(name: str) -> dict:
             o["nrows"] = [int(x) for x in t[1:]]
         elif k == "row":
             o.setdefault("crows", []).append((int(t[1]), t[2], t[3], hexf(t[4])))
+        elif k == "p":
+            o.setdefault("positions", []).append([hexf(x) for x in t[1:4]])
+        elif k == "f":
+            o.setdefault("faces", []).append([int(x) for x in t[1:4]])
+        elif k == "parent":
+            o.setdefault("parent", []).append(int(t[2]))
+        elif k == "vmap":
+            o.setdefault("vmap", []).append([int(x) for x in t[2:]])
     return o
 
 
