@@ -183,6 +183,13 @@ SIGNATURES = {
     "ddgrs_learned_train": (_I32, [_P, _I64, _I64, _D, _P, _E]),
     "ddgrs_learned_apply_batch": (_I32, [_P, _P, _I64, _P, _P, _E]),
     "ddgrs_session_set_learned": (_I32, [_P, _P, _I32, _E]),
+    "ddgrs_remesh_options_default": (None, [_P]),
+    "ddgrs_remesh": (_I32, [_P, _I64, _I64, _P, _P, _I64, _P, _D, _P, _P, ctypes.POINTER(_P), _P, _E]),
+    "ddgrs_remesh_op": (_I32, [_P, _I64, _I64, _P, _P, _I64, _P, _I32, ctypes.c_uint32, ctypes.c_uint32, _I32, _I32,
+                                ctypes.POINTER(_I32), ctypes.POINTER(_I32), ctypes.POINTER(_P), _E]),
+    "ddgrs_remeshed_sizes": (_I32, [_P, _P, _E]),
+    "ddgrs_remeshed_read": (_I32, [_P, _P, _P, _P, _P, _P, _P, _P, _E]),
+    "ddgrs_remeshed_destroy": (None, [_P]),
 }
 
 
@@ -408,6 +415,212 @@ def intersection_band(surface: Surface, positions=None):
     counts = np.zeros(2, dtype=np.int64)
     _call(lib().ddgrs_ccd_band, surface.handle, None if p is None else _ptr(p), _ptr(flags), _ptr(counts))
     return flags, int(counts[0]), int(counts[1])
+
+
+# ---- remeshing (src/remesh/) -------------------------------------------------------------------
+
+
+class _RemeshOptionsC(ctypes.Structure):
+    _fields_ = [
+        ("split", _D),
+        ("collapse", _D),
+        ("rounds", _I32),
+        ("rho", _D),
+        ("flip_cap_per_vertex", _I64),
+        ("order", _I32),
+        ("split_on", _I32),
+        ("collapse_on", _I32),
+        ("flip_on", _I32),
+        ("smooth_on", _I32),
+        ("max_growth", _D),
+        ("guard_gap", _D),
+    ]
+
+
+class _RemeshReportC(ctypes.Structure):
+    _fields_ = [
+        ("splits", _I64),
+        ("collapses", _I64),
+        ("flips", _I64),
+        ("splits_deferred", _I64),
+        ("refused_splits", _I64 * 11),
+        ("refused_collapses", _I64 * 11),
+        ("refused_flips", _I64 * 11),
+        ("smoothing_moves", _I64),
+        ("smoothing_halved", _I64),
+        ("smoothing_refused", _I64),
+        ("smoothing_skipped", _I64),
+        ("flip_cap_hits", _I64),
+        ("n_vertices", _I64),
+        ("n_faces", _I64),
+        ("smoothing_iterations", _I64),
+        ("seconds", _D),
+    ]
+
+
+# src/remesh/mesh.rs:75 ``RefusalKind::ALL``
+REFUSAL_KINDS = ("NoSuchEdge", "BothPinned", "LinkCondition", "Foldover", "RepeatedFace", "NonManifoldResult",
+                 "BoundaryEdge", "NotDelaunay", "ExistingEdge", "Guard", "DegenerateFace")
+
+
+@dataclasses.dataclass(frozen=True)
+class EnabledOps:
+    """``EnabledOps`` (src/remesh/mod.rs:80)."""
+
+    split: bool = True
+    collapse: bool = True
+    flip: bool = True
+    smooth: bool = True
+
+
+@dataclasses.dataclass(frozen=True)
+class RemeshOptions:
+    """``RemeshOptions`` (src/remesh/mod.rs:96): split / collapse thresholds as multiples of L0,
+    flip and smooth rounds, smoothing step ρ, flip cap per vertex, order "flip_then_smooth" (the
+    paper, default) or "smooth_then_flip", the parts that run, the growth cap (``math.inf``
+    disables it).  ``guard_collapses`` is the session's choice (the CCD gap rule on collapses, with
+    ``CcdOptions().gap``); ``remesh`` takes the guard explicitly."""
+
+    split: float = 1.5
+    collapse: float = 0.5
+    rounds: int = 5
+    rho: float = 0.5
+    flip_cap_per_vertex: int = 100
+    order: str = "flip_then_smooth"
+    enabled_ops: EnabledOps = EnabledOps()
+    max_growth: float = 2.0
+    guard_collapses: bool = True
+
+    def _c(self, guard_gap: float) -> _RemeshOptionsC:
+        if self.order not in ("flip_then_smooth", "smooth_then_flip"):
+            raise RsError(1, "remesh", f"order: {self.order!r}: flip_then_smooth or smooth_then_flip")
+        e = self.enabled_ops
+        return _RemeshOptionsC(self.split, self.collapse, self.rounds, self.rho, self.flip_cap_per_vertex,
+                               0 if self.order == "flip_then_smooth" else 1, int(e.split), int(e.collapse),
+                               int(e.flip), int(e.smooth), self.max_growth, guard_gap)
+
+
+@dataclasses.dataclass(frozen=True)
+class VertexMap:
+    """``VertexMap`` (src/remesh/transport.rs:15): new vertex → old vertices and weights (CSR)."""
+
+    offsets: np.ndarray
+    sources: np.ndarray
+    weights: np.ndarray
+    n_old: int
+
+    def entry(self, v: int):
+        lo, hi = int(self.offsets[v]), int(self.offsets[v + 1])
+        return self.sources[lo:hi], self.weights[lo:hi]
+
+    def transport(self, old) -> np.ndarray:
+        """new[v] = Σ w_i old[i] for per-vertex data (transport.rs:83)."""
+        a = np.asarray(old, dtype=np.float64)
+        if a.shape[0] != self.n_old:
+            raise RsError(1, "transport", f"{a.shape[0]} values for {self.n_old} old vertices")
+        counts = np.diff(self.offsets.astype(np.int64))
+        rows = np.repeat(np.arange(len(counts)), counts)
+        w = self.weights.reshape((-1,) + (1,) * (a.ndim - 1))
+        out = np.zeros((len(counts),) + a.shape[1:])
+        np.add.at(out, rows, w * a[self.sources])
+        return out
+
+
+@dataclasses.dataclass(frozen=True)
+class Remeshed:
+    """``Remeshed`` (src/remesh/mesh.rs:210): faces, positions, the vertex map, the face parents
+    and the new ids of the input pins."""
+
+    faces: np.ndarray
+    positions: np.ndarray
+    vertex_map: VertexMap
+    face_parents: np.ndarray
+    pinned: np.ndarray
+
+
+@dataclasses.dataclass(frozen=True)
+class RemeshReport:
+    """``RemeshReport`` (src/remesh/mod.rs:187); refusals as {kind: count}."""
+
+    splits: int
+    collapses: int
+    flips: int
+    refused_splits: dict
+    splits_deferred: int
+    refused_collapses: dict
+    refused_flips: dict
+    smoothing_moves: int
+    smoothing_halved: int
+    smoothing_refused: int
+    smoothing_skipped: int
+    flip_cap_hits: int
+    n_vertices: int
+    n_faces: int
+    seconds: float
+    smoothing_iterations: int = 0
+
+
+def _read_remeshed(h, n_old: int) -> Remeshed:
+    L = lib()
+    try:
+        n = np.zeros(4, dtype=np.int64)
+        _call(L.ddgrs_remeshed_sizes, h, _ptr(n))
+        nv, nf, nm, npin = (int(x) for x in n)
+        pos = np.zeros((nv, 3))
+        faces = np.zeros((nf, 3), dtype=np.uint32)
+        parents = np.zeros(nf, dtype=np.uint32)
+        off = np.zeros(nv + 1, dtype=np.uint32)
+        src = np.zeros(nm, dtype=np.uint32)
+        w = np.zeros(nm)
+        pins = np.zeros(npin, dtype=np.uint32)
+        _call(L.ddgrs_remeshed_read, h, _ptr(pos), _ptr(faces), _ptr(parents), _ptr(off), _ptr(src), _ptr(w),
+              _ptr(pins))
+        return Remeshed(faces, pos, VertexMap(off, src, w, n_old), parents, pins)
+    finally:
+        L.ddgrs_remeshed_destroy(h)
+
+
+def remesh(ctx: "Context", faces, positions, l0: float, options: RemeshOptions = RemeshOptions(), pinned=(),
+           guard_gap: Optional[float] = None, band=None):
+    """``remesh`` (src/remesh/mod.rs:235) on the device: one pass of splits, collapses, Delaunay
+    flips and tangential smoothing.  ``guard_gap``: the absolute gap of the collapse guard (the
+    session's ``CcdCollapseGuard``, src/ccd.rs:1448; None: no guard); ``band``: its exempt vertices
+    (``intersection_band``).  Returns (Remeshed, RemeshReport)."""
+    f = np.ascontiguousarray(faces, dtype=np.uint32).reshape(-1, 3)
+    p = _f64(positions).reshape(-1, 3)
+    pins = np.ascontiguousarray(pinned, dtype=np.uint32).reshape(-1)
+    b = None if band is None else np.ascontiguousarray(band, dtype=np.uint8).reshape(-1)
+    if b is not None and b.shape[0] != p.shape[0]:
+        raise RsError(1, "remesh", f"band: {b.shape[0]} flags for {p.shape[0]} vertices")
+    o = options._c(0.0 if guard_gap is None else float(guard_gap))
+    h = _P()
+    r = _RemeshReportC()
+    _call(lib().ddgrs_remesh, ctx.handle, p.shape[0], f.shape[0], _ptr(f), _ptr(p), pins.size,
+          _ptr(pins) if pins.size else None, float(l0), ctypes.byref(o), None if b is None else _ptr(b),
+          ctypes.byref(h), ctypes.byref(r))
+    out = _read_remeshed(h, p.shape[0])
+    kinds = lambda arr: {k: int(v) for k, v in zip(REFUSAL_KINDS, arr)}
+    rep = RemeshReport(int(r.splits), int(r.collapses), int(r.flips), kinds(r.refused_splits), int(r.splits_deferred),
+                       kinds(r.refused_collapses), kinds(r.refused_flips), int(r.smoothing_moves),
+                       int(r.smoothing_halved), int(r.smoothing_refused), int(r.smoothing_skipped),
+                       int(r.flip_cap_hits), int(r.n_vertices), int(r.n_faces), float(r.seconds),
+                       int(r.smoothing_iterations))
+    return out, rep
+
+
+def remesh_op(ctx: "Context", faces, positions, op: str, a: int, b: int, pinned=(), delaunay: bool = False,
+              foldover: bool = False):
+    """One ``EditMesh`` operation on the device (src/remesh/mesh.rs:592 split, :679 collapse, :888
+    flip) and ``finish``: (refusal kind or None, reordered, Remeshed)."""
+    f = np.ascontiguousarray(faces, dtype=np.uint32).reshape(-1, 3)
+    p = _f64(positions).reshape(-1, 3)
+    pins = np.ascontiguousarray(pinned, dtype=np.uint32).reshape(-1)
+    ref, reo, h = _I32(), _I32(), _P()
+    _call(lib().ddgrs_remesh_op, ctx.handle, p.shape[0], f.shape[0], _ptr(f), _ptr(p), pins.size,
+          _ptr(pins) if pins.size else None, {"split": 0, "collapse": 1, "flip": 2}[op], a, b, int(delaunay),
+          int(foldover), ctypes.byref(ref), ctypes.byref(reo), ctypes.byref(h))
+    out = _read_remeshed(h, p.shape[0])
+    return (None if ref.value == 0 else REFUSAL_KINDS[ref.value - 1]), bool(reo.value), out
 
 
 # ---- energy (src/energy/mod.rs) ----------------------------------------------------------------

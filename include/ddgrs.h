@@ -480,6 +480,67 @@ DDGRS_API int ddgrs_learned_apply_batch(ddgrs_learned* l, const double* position
  * mode 0 (or l null) detaches.  `l` must stay alive while attached. */
 DDGRS_API int ddgrs_session_set_learned(ddgrs_session* h, ddgrs_learned* l, int32_t mode, ddgrs_error* err);
 
+/* ---- remeshing (RS/src/remesh/) ------------------------------------------------------------- */
+
+typedef struct ddgrs_remeshed ddgrs_remeshed;
+
+/* RemeshOptions (RS/src/remesh/mod.rs:96) and the collapse guard: split / collapse thresholds as
+ * multiples of L0, flip/smooth rounds, smoothing step rho, flip cap per vertex, order (0 flip then
+ * smooth, the default; 1 smooth then flip), which parts run, max_growth (INFINITY disables it);
+ * guard_gap > 0 checks every collapse with the CCD gap rule at that absolute gap (the session's
+ * CcdCollapseGuard, RS/src/ccd.rs:1448; 0: no guard). */
+typedef struct ddgrs_remesh_options {
+  double split, collapse;
+  int32_t rounds;
+  double rho;
+  int64_t flip_cap_per_vertex;
+  int32_t order;
+  int32_t split_on, collapse_on, flip_on, smooth_on;
+  double max_growth;
+  double guard_gap;
+} ddgrs_remesh_options;
+
+/* RemeshReport (RS/src/remesh/mod.rs:187).  refused_*[k]: refusals of kind k in the order of
+ * RefusalKind::ALL (RS/src/remesh/mesh.rs:75: NoSuchEdge, BothPinned, LinkCondition, Foldover,
+ * RepeatedFace, NonManifoldResult, BoundaryEdge, NotDelaunay, ExistingEdge, Guard, DegenerateFace);
+ * smoothing_iterations: passes of the parallel guarded smoothing over all rounds (no counterpart). */
+typedef struct ddgrs_remesh_report {
+  int64_t splits, collapses, flips, splits_deferred;
+  int64_t refused_splits[11], refused_collapses[11], refused_flips[11];
+  int64_t smoothing_moves, smoothing_halved, smoothing_refused, smoothing_skipped;
+  int64_t flip_cap_hits;
+  int64_t n_vertices, n_faces;
+  int64_t smoothing_iterations;
+  double seconds;
+} ddgrs_remesh_report;
+
+DDGRS_API void ddgrs_remesh_options_default(ddgrs_remesh_options* o);
+/* One remeshing pass of the mesh (nv vertices, nf faces, positions 3·nv) on the device: splits of
+ * the long edges, collapses of the short ones shortest first, Delaunay flips and tangential
+ * smoothing, the result numbered as the reference numbers it.  pinned: npinned vertex ids (never
+ * moved or removed); band: nv flags exempting pairs of the collapse guard (IntersectionBand,
+ * ddgrs_ccd_band) or NULL.  Replaces remesh (RS/src/remesh/mod.rs:235) with the session's guard.
+ * The result (read with ddgrs_remeshed_*) must be destroyed. */
+DDGRS_API int ddgrs_remesh(ddgrs_ctx* ctx, int64_t nv, int64_t nf, const uint32_t* faces, const double* positions,
+                           int64_t npinned, const uint32_t* pinned, double l0, const ddgrs_remesh_options* o,
+                           const uint8_t* band, ddgrs_remeshed** out, ddgrs_remesh_report* report, ddgrs_error* err);
+/* One EditMesh operation (op 0 split, 1 collapse, 2 flip; RS/src/remesh/mesh.rs:592, :679, :888)
+ * with the optional Delaunay / foldover checks, then EditMesh::finish (:991).  *refusal: 0 done,
+ * else 1 + the RefusalKind index; *reordered: the flip's FlipOutcome::reordered. */
+DDGRS_API int ddgrs_remesh_op(ddgrs_ctx* ctx, int64_t nv, int64_t nf, const uint32_t* faces, const double* positions,
+                              int64_t npinned, const uint32_t* pinned, int32_t op, uint32_t a, uint32_t b,
+                              int32_t delaunay, int32_t foldover, int32_t* refusal, int32_t* reordered,
+                              ddgrs_remeshed** out, ddgrs_error* err);
+/* sizes4 = [vertices, faces, vertex-map entries, pins] (Remeshed, RS/src/remesh/mesh.rs:210). */
+DDGRS_API int ddgrs_remeshed_sizes(const ddgrs_remeshed* r, int64_t* sizes4, ddgrs_error* err);
+/* positions 3·nv, faces 3·nf, face_parents nf, the vertex map (VertexMap, RS/src/remesh/
+ * transport.rs:15) as map_offsets nv + 1, map_sources / map_weights per entry, and the new ids of
+ * the input pins in the caller's order. */
+DDGRS_API int ddgrs_remeshed_read(const ddgrs_remeshed* r, double* positions, uint32_t* faces, uint32_t* face_parents,
+                                  uint32_t* map_offsets, uint32_t* map_sources, double* map_weights, uint32_t* pinned,
+                                  ddgrs_error* err);
+DDGRS_API void ddgrs_remeshed_destroy(ddgrs_remeshed* r);
+
 #ifdef __cplusplus
 }
 #endif

@@ -60,6 +60,8 @@ def test_struct_layouts_match_binding(tmp_path):
         "ddgrs_step_report": pkg._StepReportC,
         "ddgrs_ccd_options": pkg._CcdOptionsC,
         "ddgrs_ccd_report": pkg._CcdReportC,
+        "ddgrs_remesh_options": pkg._RemeshOptionsC,
+        "ddgrs_remesh_report": pkg._RemeshReportC,
     }
     c = _c_layout(tmp_path, {k: [f for f, _ in v._fields_] for k, v in pairs.items()})
     for name, cls in pairs.items():
@@ -85,6 +87,8 @@ def test_rust_stub_structs_match_header():
         "ddgrs_step_report": pkg._StepReportC,
         "ddgrs_ccd_options": pkg._CcdOptionsC,
         "ddgrs_ccd_report": pkg._CcdReportC,
+        "ddgrs_remesh_options": pkg._RemeshOptionsC,
+        "ddgrs_remesh_report": pkg._RemeshReportC,
     }
     for name, cls in pairs.items():
         body = re.search(r"pub struct " + name + r"\s*\{(.*?)\}", stub, re.S).group(1)
