@@ -419,6 +419,64 @@ def ccd_leg(pkg, ctx, args, p, f) -> dict:
     return out
 
 
+def remesh_leg(pkg, ctx, args, p, f, opts) -> dict:
+    """One remeshing pass as the reference's step runs it (session.rs:1109-1187): the positions after
+    one descent step, L0 the input's mean edge length, default RemeshOptions and the collapse guard
+    at the default CCD gap (0.05 L0) with the band of the faces that intersect now.  The device's
+    median of 5 passes after one warm-up against the FP64 oracle's pass on one host thread (the CPU
+    baseline of this leg; the device's faces, parents, vertex map and report must equal the
+    oracle's, positions to the printed error); then a stress pass of a harder deformation (16
+    bumps of up to 0.3 on the sphere) with thousands of splits and flips."""
+    e = {(min(a, b), max(a, b)) for t in f.tolist() for a, b in ((t[0], t[1]), (t[1], t[2]), (t[2], t[0]))}
+    ea = np.array(sorted(e))
+    l0 = float(np.mean(np.linalg.norm(p[ea[:, 0]] - p[ea[:, 1]], axis=1)))
+    surf = pkg.Surface(ctx, f, p)
+    ses = pkg.Session(surf, opts)
+    ses.step()
+    q = ses.positions()
+    ses.close()
+    band = pkg.intersection_band(surf, q)[0]
+    surf.close()
+    gap = pkg.CcdOptions().gap * l0
+    out = {}
+    for name, pos in (("after_step", q), ("stress", None)):
+        if pos is None:
+            p6, f6 = pkg.generate.icosphere(6)
+            pos = pkg.generate.bumps(np.asarray(p6), 16, (0.0, 0.3), (0.1, 0.4), 3)
+            band = None
+        pkg.remesh(ctx, f, pos, l0, pkg.RemeshOptions(), guard_gap=gap, band=band)
+        times, res = [], None
+        for _ in range(5):
+            c0 = time.perf_counter()
+            res = pkg.remesh(ctx, f, pos, l0, pkg.RemeshOptions(), guard_gap=gap, band=band)
+            times.append(time.perf_counter() - c0)
+        dev, rep = res
+        o = {"device_ms": float(np.median(times) * 1e3), "splits": rep.splits, "collapses": rep.collapses,
+             "flips": rep.flips, "smoothing_moves": rep.smoothing_moves, "smoothing_iterations": rep.smoothing_iterations,
+             "faces_out": rep.n_faces}
+        if not args.no_cpu_baseline:
+            import oracle
+
+            print(f"bench: oracle remeshing pass ({name}) on one thread", file=sys.stderr, flush=True)
+            c0 = time.perf_counter()
+            ref, rref = oracle.remesh(f, pos, l0, guard_gap=gap, band=band)
+            o["cpu_ms"] = (time.perf_counter() - c0) * 1e3
+            o["cpu_kind"] = "port"
+            o["cpu_cores"] = 1
+            same = (dev.faces.shape == ref["faces"].shape and bool((dev.faces == ref["faces"]).all())
+                    and bool((dev.face_parents == ref["face_parents"]).all())
+                    and bool((dev.vertex_map.sources == ref["map_sources"]).all())
+                    and all(getattr(rep, k) == rref[k] for k in ("splits", "collapses", "flips", "smoothing_moves")))
+            o["matches_oracle"] = same
+            o["max_position_error"] = (float(np.abs(dev.positions - ref["positions"]).max())
+                                       if dev.positions.shape == ref["positions"].shape else None)
+        out[name] = o
+    out["workload"] = ("remesh (remesh/mod.rs:235) of 81,920 faces, L0 = the input's mean edge length, default "
+                       "options, collapse guard at gap 0.05 L0; after_step: the positions after one Hs step; "
+                       "stress: 16 bumps of up to 0.3")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -433,6 +491,7 @@ def main() -> int:
                          "against the FP64 oracle after steps 1, 10 and 20, fallback rate)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned preconditioner leg")
     ap.add_argument("--no-ccd", action="store_true", help="skip the collision detection leg")
+    ap.add_argument("--no-remesh", action="store_true", help="skip the remeshing leg")
     ap.add_argument("--learned-fields", type=int, default=40, help="bump fields of the training trajectories")
     ap.add_argument("--learned-steps", type=int, default=50, help="plain steps per field")
     ap.add_argument("--learned-krylov", type=int, default=0,
@@ -535,6 +594,8 @@ def main() -> int:
     if not args.no_learned:
         learned = learned_leg(pkg, ctx, args, opts, p, f, ms, its)
     ccd = None if args.no_ccd else ccd_leg(pkg, ctx, args, np.asarray(p), np.asarray(f, dtype=np.uint32))
+    remesh = None if args.no_remesh else remesh_leg(pkg, ctx, args, np.asarray(p), np.asarray(f, dtype=np.uint32),
+                                                     opts)
     line = {
         "metric": METRIC,
         "value": 1e3 / ms,
@@ -564,6 +625,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "learned": learned,
         "ccd": ccd,
+        "remesh": remesh,
         "traffic_scale": {k: v for k, v in traffic.items() if k.endswith("_scale")},
         "traffic_error": traffic.get("error"),
         "kernel_ms_step1": {k: round(v["ms"], 4) for k, v in sorted(kt.items(), key=lambda kv: -kv[1]["ms"])[:16]},

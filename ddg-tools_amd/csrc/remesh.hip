@@ -13,15 +13,15 @@
 //
 // Which edits happen, and in which order, is the reference's:
 // * splits: the input's long edges in lexicographic order (the longest within the growth budget
-//   when they exceed it), each applied by the edit workgroup;
+//   when they exceed it), each applied by the edit wave;
 // * collapses: the short edges sorted by (length bits, min, max) on the device, merged by the edit
-//   workgroup with a heap of the edges that collapses requeue — exactly the reference's binary
+//   wave with a heap of the edges that collapses requeue — exactly the reference's binary
 //   heap order, stale entries skipped by its own tests; the collapse guard is evaluated by the
 //   whole workgroup over every live face (see guard_scan);
 // * flips, per round: the reference pops every edge in lexicographic order, then the edges its
 //   flips requeue.  An edge's Delaunay status changes only when a flip replaces one of its faces,
 //   and those edges are exactly the flip's quad edges, which the reference queues.  So the wide
-//   kernel classifies every edge once, and the edit workgroup visits in lexicographic order only
+//   kernel classifies every edge once, and the edit wave visits in lexicographic order only
 //   the violating edges and the still-pending quad edges of earlier flips (each re-tested when
 //   visited), then the requeued ones in FIFO order: the same flips in the same order;
 // * smoothing: displacements for all vertices at once (Jacobi, as the reference); the guarded
@@ -336,15 +336,136 @@ __device__ bool vertex_ok(const Mesh& m, uint32_t v) {
   return true;
 }
 
-// mesh.rs:577 `remove_vface` (swap-remove)
-__device__ void remove_vface(const Mesh& m, uint32_t v, uint32_t f) {
-  const uint32_t k = m.vfn[v];
-  for (uint32_t i = 0; i < k; ++i)
-    if (m.vf[(size_t)v * kVF + i] == f) {
-      m.vf[(size_t)v * kVF + i] = m.vf[(size_t)v * kVF + k - 1];
-      m.vfn[v] = k - 1;
-      return;
+// ---- wave-cooperative checks of the edit wave -------------------------------------------------
+// The edit kernels run one wave whose 64 lanes all execute the same edits with the same values
+// (every lane writes what the others write); the star checks below spread a vertex's faces over the
+// lanes (lane i: face i of the list), so a check costs a few wave instructions per face instead of a
+// lane walking arrays in private memory.
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t x) {
+  for (int o = 32; o >= 1; o >>= 1) x = min(x, (uint32_t)__shfl_xor((int)x, o));
+  return x;
+}
+__device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long x) {
+  for (int o = 32; o >= 1; o >>= 1) {
+    const unsigned long long y = __shfl_xor(x, o);
+    x = y < x ? y : x;
+  }
+  return x;
+}
+__device__ __forceinline__ int first_lane(bool p) {
+  const unsigned long long b = __ballot(p);
+  return b ? __ffsll((long long)b) - 1 : -1;
+}
+
+// `fan(v).is_some()` (mesh.rs:475) by the whole wave (the faces' relative orientations do not
+// decide it, so they are not tracked).
+__device__ bool fan_ok_wave(const Mesh& m, uint32_t v) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t nfs = m.vfn[v];
+  if (nfs == 0 || nfs > (uint32_t)kVF) return false;
+  const bool act = (uint32_t)lane < nfs;
+  const uint32_t f = act ? m.vf[(size_t)v * kVF + lane] : kNone;
+  uint32_t wa = kNone, wb = kNone;
+  if (act) wedge(F(m, f), v, &wa, &wb);
+  uint32_t ca = 0, cb = 0;
+  for (uint32_t j = 0; j < nfs; ++j) {
+    const uint32_t ja = (uint32_t)__shfl((int)wa, (int)j), jb = (uint32_t)__shfl((int)wb, (int)j);
+    ca += (ja == wa ? 1u : 0u) + (jb == wa ? 1u : 0u);
+    cb += (ja == wb ? 1u : 0u) + (jb == wb ? 1u : 0u);
+  }
+  if (__ballot(act && (ca > 2 || cb > 2))) return false;
+  const bool ea = act && ca == 1, eb = act && cb == 1;
+  const int nends = __popcll(__ballot(ea)) + __popcll(__ballot(eb));
+  if (nends != 0 && nends != 2) return false;
+  uint32_t cur, entry;
+  if (nends == 2) {
+    // the smallest (face, neighbour) of the ends
+    unsigned long long k = ~0ull;
+    if (ea) k = ((unsigned long long)f << 32) | wa;
+    if (eb) {
+      const unsigned long long kb = ((unsigned long long)f << 32) | wb;
+      k = kb < k ? kb : k;
     }
+    k = wave_min_u64(k);
+    cur = (uint32_t)(k >> 32);
+    entry = (uint32_t)(k & 0xffffffffull);
+  } else {
+    cur = wave_min_u32(f);
+    entry = (uint32_t)__shfl((int)wa, first_lane(f == cur));
+  }
+  const uint32_t start = cur;
+  uint32_t c = 1;
+  for (;;) {
+    const int ci = first_lane(f == cur);
+    const uint32_t cwa = (uint32_t)__shfl((int)wa, ci), cwb = (uint32_t)__shfl((int)wb, ci);
+    const uint32_t exit = cwa == entry ? cwb : cwa;
+    const uint32_t next = wave_min_u32(act && f != cur && (wa == exit || wb == exit) ? f : kNone);
+    if (next == kNone || next == start) break;
+    if (c > nfs) return false;
+    ++c;
+    cur = next;
+    entry = exit;
+  }
+  return c == nfs;
+}
+
+// mesh.rs:558 `vertex_ok` by the whole wave.
+__device__ bool vertex_ok_wave(const Mesh& m, uint32_t v) {
+  if (!m.valive[v]) return true;
+  if (!fan_ok_wave(m, v)) return false;
+  const int lane = threadIdx.x & 63;
+  const uint32_t k = m.vfn[v];
+  const bool act = (uint32_t)lane < k;
+  uint32_t s3[3] = {kNone, kNone, kNone};
+  if (act) {
+    const Tri t = F(m, m.vf[(size_t)v * kVF + lane]);
+    const uint32_t lo = min(t.v[0], t.v[1]), hi = max(t.v[0], t.v[1]);
+    s3[0] = min(lo, t.v[2]);
+    s3[2] = max(hi, t.v[2]);
+    s3[1] = max(lo, min(hi, t.v[2]));
+  }
+  bool dup = false;
+  for (uint32_t j = 0; j < k; ++j) {
+    const uint32_t a = (uint32_t)__shfl((int)s3[0], (int)j), b = (uint32_t)__shfl((int)s3[1], (int)j),
+                   c = (uint32_t)__shfl((int)s3[2], (int)j);
+    dup = dup || (act && (uint32_t)lane != j && a == s3[0] && b == s3[1] && c == s3[2]);
+  }
+  return __ballot(dup) == 0;
+}
+
+// `edge_faces` (mesh.rs:381) by the whole wave: lane i tests face i of a's list; the matches are
+// taken in ascending order by repeated wave minima.
+__device__ int edge_faces_wave(const Mesh& m, uint32_t a, uint32_t b, uint32_t* out) {
+  if (!valid_vertex(m, a) || !valid_vertex(m, b) || a == b) return 0;
+  const int lane = threadIdx.x & 63;
+  const uint32_t k = m.vfn[a];
+  uint32_t f = kNone;
+  if ((uint32_t)lane < k) {
+    const uint32_t g = m.vf[(size_t)a * kVF + lane];
+    if (has(F(m, g), b)) f = g;
+  }
+  const int n = __popcll(__ballot(f != kNone));
+  if (n > 8) flag(m, F_EDGE);
+  uint32_t last = 0;
+  int c = 0;
+  for (; c < n && c < 8; ++c) {
+    const uint32_t x = wave_min_u32((f != kNone && (c == 0 || f > last)) ? f : kNone);
+    out[c] = x;
+    last = x;
+  }
+  return c;
+}
+
+// mesh.rs:577 `remove_vface` (swap-remove), by the whole wave: lane i compares entry i.
+__device__ void remove_vface(const Mesh& m, uint32_t v, uint32_t f) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t k = m.vfn[v];
+  const int i = first_lane((uint32_t)lane < k && m.vf[(size_t)v * kVF + lane] == f);
+  if (i < 0) return;
+  const uint32_t last = m.vf[(size_t)v * kVF + k - 1];
+  m.vf[(size_t)v * kVF + i] = last;
+  m.vfn[v] = k - 1;
 }
 __device__ void push_vface(const Mesh& m, uint32_t v, uint32_t f) {
   const uint32_t k = m.vfn[v];
@@ -409,7 +530,7 @@ __device__ bool combine(const Mesh& m, uint32_t a, double wa, uint32_t b, double
 
 __device__ int split_edge(const Mesh& m, uint32_t a, uint32_t b) {
   uint32_t inc[8];
-  const int ni = edge_faces(m, a, b, inc);
+  const int ni = edge_faces_wave(m, a, b, inc);
   if (ni == 0) return R_NO_SUCH_EDGE;
   const uint32_t mv = m.cnt->nv;
   const V mid = vscale(vadd(P(m, a), P(m, b)), 0.5);
@@ -455,7 +576,7 @@ __device__ int split_edge(const Mesh& m, uint32_t a, uint32_t b) {
 
 __device__ int flip_edge(const Mesh& m, uint32_t a, uint32_t b, bool delaunay, bool foldover, bool* reordered) {
   uint32_t inc[8];
-  const int ni = edge_faces(m, a, b, inc);
+  const int ni = edge_faces_wave(m, a, b, inc);
   *reordered = false;
   if (ni == 0) return R_NO_SUCH_EDGE;
   if (ni != 2) return R_BOUNDARY;
@@ -465,7 +586,7 @@ __device__ int flip_edge(const Mesh& m, uint32_t a, uint32_t b, bool delaunay, b
   if (!orient(t1, a, b, &p, &q, &c) || !orient(t2, a, b, &p2, &q2, &d)) return R_NO_SUCH_EDGE;
   *reordered = p2 == p;
   uint32_t tmp[8];
-  if (edge_faces(m, c, d, tmp) != 0 || c == d) return R_EXISTING;
+  if (edge_faces_wave(m, c, d, tmp) != 0 || c == d) return R_EXISTING;
   const V xp = P(m, p), xq = P(m, q), xc = P(m, c), xd = P(m, d);
   if (delaunay && !violates_delaunay(xp, xq, xc, xd)) return R_NOT_DELAUNAY;
   if (foldover) {
@@ -476,27 +597,22 @@ __device__ int flip_edge(const Mesh& m, uint32_t a, uint32_t b, bool delaunay, b
     if (folds(n1, tri_normal(xp, xd, xc)) || folds(n2, tri_normal(xd, xq, xc))) return R_FOLDOVER;
   }
   if (degenerate(xp, xd, xc) || degenerate(xd, xq, xc)) return R_DEGENERATE;
-  // snapshot of the four vertices' face lists for the undo
-  const uint32_t vs[4] = {p, q, c, d};
-  uint32_t snap[4][kVF], snapn[4];
-  for (int i = 0; i < 4; ++i) {
-    snapn[i] = m.vfn[vs[i]];
-    for (uint32_t j = 0; j < snapn[i]; ++j) snap[i][j] = m.vf[(size_t)vs[i] * kVF + j];
-  }
   setF(m, f1, p, d, c);
   setF(m, f2, d, q, c);
   remove_vface(m, p, f2);
   remove_vface(m, q, f1);
   push_vface(m, c, f2);
   push_vface(m, d, f1);
+  const uint32_t vs[4] = {p, q, c, d};
   for (int i = 0; i < 4; ++i)
-    if (!vertex_ok(m, vs[i])) {
+    if (!vertex_ok_wave(m, vs[i])) {
+      // undone by the inverse edits (the order of a face list never changes a result)
       setF(m, f1, t1.v[0], t1.v[1], t1.v[2]);
       setF(m, f2, t2.v[0], t2.v[1], t2.v[2]);
-      for (int k = 0; k < 4; ++k) {
-        m.vfn[vs[k]] = snapn[k];
-        for (uint32_t j = 0; j < snapn[k]; ++j) m.vf[(size_t)vs[k] * kVF + j] = snap[k][j];
-      }
+      remove_vface(m, c, f2);
+      remove_vface(m, d, f1);
+      push_vface(m, p, f2);
+      push_vface(m, q, f1);
       return R_NON_MANIFOLD;
     }
   return R_OK;
@@ -519,7 +635,7 @@ struct Collapse {
 // The checks up to the guard: returns R_OK when the collapse may proceed.
 __device__ int collapse_check(const Mesh& m, uint32_t a, uint32_t b, bool foldover, Collapse& C) {
   C.a = a, C.b = b;
-  C.ni = edge_faces(m, a, b, C.inc);
+  C.ni = edge_faces_wave(m, a, b, C.inc);
   if (C.ni == 0) return R_NO_SUCH_EDGE;
   if (m.pinned[a] && m.pinned[b]) return R_BOTH_PINNED;
   {
@@ -663,7 +779,7 @@ __device__ int collapse_apply(const Mesh& m, const Collapse& C) {
   isort(check, nc);
   bool ok = true;
   for (int i = 0; i < nc && ok; ++i)
-    if (i == 0 || check[i] != check[i - 1]) ok = vertex_ok(m, check[i]);
+    if (i == 0 || check[i] != check[i - 1]) ok = vertex_ok_wave(m, check[i]);
   if (!ok) {
     for (int i = 0; i < nt; ++i) {
       m.vfn[touched[i]] = snapn[i];
@@ -843,10 +959,10 @@ __global__ void rm_budget(uint32_t nl, const uint32_t* order, const uint32_t* ad
 }
 
 
-// Splits of the kept long edges in lexicographic order (mod.rs:290-297), by one thread.
+// Splits of the kept long edges in lexicographic order (mod.rs:290-297), by one wave.
 __global__ void rm_split_engine(Mesh m, const uint32_t* ea, const uint32_t* eb, const uint32_t* list,
                                 const uint32_t* keep, uint32_t nl) {
-  if (threadIdx.x != 0) return;
+  if (threadIdx.x >= 64) return;
   for (uint32_t i = 0; i < nl; ++i) {
     if (keep && !keep[i]) continue;
     const uint32_t e = list[i];
@@ -937,7 +1053,7 @@ __global__ __launch_bounds__(kGuardThreads) void rm_collapse_engine(Mesh m, Coll
   Collapse C;
   uint32_t si = 0, hn = 0;
   for (;;) {
-    if (tid == 0) {
+    if (tid < 64) {
       s_cmd = 0;
       while (!m.cnt->flags) {
         HeapEntry e;
@@ -955,7 +1071,7 @@ __global__ __launch_bounds__(kGuardThreads) void rm_collapse_engine(Mesh m, Coll
         const uint32_t a = e.a, b = e.b;
         if (!m.valive[a] || !m.valive[b]) continue;
         uint32_t tmp[8];
-        if (edge_faces(m, a, b, tmp) == 0) continue;
+        if (edge_faces_wave(m, a, b, tmp) == 0) continue;
         const double l = vnorm(vsub(P(m, a), P(m, b)));
         if ((unsigned long long)__double_as_longlong(l) != e.bits || l >= A.thr) continue;
         const int r = collapse_check(m, a, b, true, C);
@@ -1048,7 +1164,7 @@ __global__ __launch_bounds__(kGuardThreads) void rm_collapse_engine(Mesh m, Coll
       }
     }
     __syncthreads();
-    if (tid == 0) {
+    if (tid < 64) {
       if (s_refuse) {
         ++m.cnt->refused_collapse[R_GUARD - 1];
       } else if (collapse_apply(m, C) == R_OK) {
@@ -1122,20 +1238,19 @@ __device__ uint32_t lex_position(const FlipArgs& A, uint32_t lo, uint32_t hi) {
   }
   return (l < A.eoff[lo + 1] && A.eb[l] == hi) ? l : kNone;
 }
-__device__ bool queued(const FlipArgs& A, uint32_t lo, uint32_t hi) {
+// The requeued edges (lo, hi) are kept per lower end; lane i compares entry i (at most kNb).
+__device__ int queued_lane(const FlipArgs& A, uint32_t lo, uint32_t hi) {
+  const int lane = threadIdx.x & 63;
   const uint32_t n = A.qn[lo];
-  for (uint32_t i = 0; i < n; ++i)
-    if (A.q[(size_t)lo * kNb + i] == hi) return true;
-  return false;
+  return first_lane((uint32_t)lane < n && A.q[(size_t)lo * kNb + lane] == hi);
 }
+__device__ bool queued(const FlipArgs& A, uint32_t lo, uint32_t hi) { return queued_lane(A, lo, hi) >= 0; }
 __device__ void unqueue(const FlipArgs& A, uint32_t lo, uint32_t hi) {
+  const int i = queued_lane(A, lo, hi);
+  if (i < 0) return;
   const uint32_t n = A.qn[lo];
-  for (uint32_t i = 0; i < n; ++i)
-    if (A.q[(size_t)lo * kNb + i] == hi) {
-      A.q[(size_t)lo * kNb + i] = A.q[(size_t)lo * kNb + n - 1];
-      A.qn[lo] = n - 1;
-      return;
-    }
+  A.q[(size_t)lo * kNb + i] = A.q[(size_t)lo * kNb + n - 1];
+  A.qn[lo] = n - 1;
 }
 __device__ void rheap_push(uint32_t* h, uint32_t& n, uint32_t x) {
   uint32_t i = n++;
@@ -1162,16 +1277,16 @@ __device__ uint32_t rheap_pop(uint32_t* h, uint32_t& n) {
   return top;
 }
 
-// One round of flips by one thread: the reference's queue visited through the violating edges
-// and the pending quad edges (pass over the lexicographic list), then its FIFO of requeued edges.
+// One round of flips by one wave: the reference's queue visited through the violating edges and
+// the pending quad edges (pass over the lexicographic list), then its FIFO of requeued edges.
 __global__ void rm_flip_engine(Mesh m, FlipArgs A) {
-  if (threadIdx.x != 0) return;
+  if (threadIdx.x >= 64) return;
   unsigned long long flips = 0, head = 0, tail = 0;
   uint32_t i0 = 0, rn = 0, cursor = 0, last = kNone;
   bool abort = false;
   auto process = [&](uint32_t a, uint32_t b) {
     uint32_t inc[8];
-    if (edge_faces(m, a, b, inc) != 2) return;
+    if (edge_faces_wave(m, a, b, inc) != 2) return;
     uint32_t cd[2];
     for (int i = 0; i < 2; ++i) {
       const Tri t = F(m, inc[i]);
@@ -1932,7 +2047,7 @@ void remesh_pass(ddgrs_ctx* ctx, const RemeshInput& in, double l0, const ddgrs_r
 
 namespace {
 __global__ void rm_single_op(Mesh m, int op, uint32_t a, uint32_t b, int delaunay, int foldover, int* result) {
-  if (threadIdx.x != 0) return;
+  if (threadIdx.x >= 64) return;
   int r = R_OK;
   bool reo = false;
   if (op == 0) {
