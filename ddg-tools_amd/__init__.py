@@ -164,6 +164,8 @@ SIGNATURES = {
     "ddgrs_session_destroy": (None, [_P]),
     "ddgrs_session_step": (_I32, [_P, _P, _E]),
     "ddgrs_session_state": (_I32, [_P, _P, _E]),
+    "ddgrs_session_set_state": (_I32, [_P, _P, _E]),
+    "ddgrs_session_row_targets": (_I32, [_P, _P, _E]),
     "ddgrs_session_trials": (_I32, [_P, _I64, _P, _P, _P, ctypes.POINTER(_I64), _E]),
     "ddgrs_session_fallbacks": (_I32, [_P, ctypes.c_char_p, _I64, ctypes.POINTER(_I64), _E]),
     "ddgrs_session_gmres_history": (_I32, [_P, _P, _I64, _P, _I64, _P, _E]),
@@ -1250,6 +1252,18 @@ class Session(_Handle):
         o = np.zeros(5)
         _call(lib().ddgrs_session_state, self._h, _ptr(o))
         return {"mean_edge_length0": float(o[0]), "last_tau": float(o[1]), "barycenter_target": o[2:5].copy()}
+
+    def set_state(self, mean_edge_length0: float, last_tau: float, barycenter_target) -> None:
+        """Continues a run on a new surface (after a remeshing): ℓ̄₀, the last accepted τ and the
+        barycenter target of the session it replaces."""
+        o = np.array([mean_edge_length0, last_tau, *np.asarray(barycenter_target, dtype=np.float64).reshape(3)])
+        _call(lib().ddgrs_session_set_state, self._h, _ptr(o))
+
+    def row_targets(self) -> tuple:
+        """The area / volume rows' targets (NaN for no row), as given or at creation."""
+        o = np.zeros(2)
+        _call(lib().ddgrs_session_row_targets, self._h, _ptr(o))
+        return float(o[0]), float(o[1])
 
     def positions(self) -> np.ndarray:
         return self.surface.positions()

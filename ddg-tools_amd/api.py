@@ -10,9 +10,15 @@ path (the reference's ``device`` takes ``"cpu"``, ``"cuda"`` or ``"auto"``, :829
 * ``gpu_info()`` → ``available``, ``devices`` (ordinal, name, memory_bytes), ``reason``.
 
 Positions and faces may be flat (3 per vertex / face) or (n, 3).  The options are the reference's
-names; those that select something this path does not run (remeshing, CCD, other metrics,
-energies, optimizers, penalties, pins, per-component rows, target schedules) raise ``Unsupported``
-instead of being ignored, and an unknown name is ``InvalidOption``.  ``constraints`` takes the
+names; those that select something this path does not run (other metrics, energies, optimizers,
+penalties, pins, obstacles, per-component rows, target schedules) raise ``Unsupported`` instead of
+being ignored, and an unknown name is ``InvalidOption``.  Remeshing is on by default as in the
+reference (``remesh``, ``remesh_split``, ``remesh_collapse``, ``remesh_rounds``, ``remesh_rho``,
+``remesh_flip_cap``, ``remesh_order``, ``remesh_ops``; ``crates/ddg-py/src/repulsive_surfaces.rs:
+1062-1111``): after every step the device remeshes the projected positions with the collapse
+guard (the line search's CCD gap, or the default one with the band of the faces that intersect
+then), and a remeshing that changed the mesh continues the run on a new device session with the
+previous one's ℓ̄₀, last τ and targets.  ``constraints`` takes the
 reference's list of dicts (``crates/ddg-py/src/repulsive_surfaces.rs:629``): the global
 ``{"type": "barycenter"}`` (required: it is the saddle's own row) and up to one each of
 ``{"type": "area", "target": t}`` and ``{"type": "volume", "target": t}`` (target optional: the
@@ -21,12 +27,13 @@ default 5) set the projection's Newton rounds.
 """
 from __future__ import annotations
 
+import dataclasses
 import time
 
 import numpy as np
 
-from . import (CcdOptions, Context, EnergyEval, Exponents, GmresOptions, RsError, SaddleOptions, Session, StartRule, StepOptions,
-               Surface, BhOptions, energy as _energy)
+from . import (CcdOptions, Context, EnabledOps, EnergyEval, Exponents, GmresOptions, RemeshOptions, RsError, SaddleOptions,
+               Session, StartRule, StepOptions, Surface, BhOptions, energy as _energy, intersection_band, remesh as _remesh)
 
 UNSUPPORTED = 12
 
@@ -38,14 +45,16 @@ DEFAULTS = {
     "backend": "hierarchical", "theta": 0.5, "leaf_size": 8, "energy_eval": "barnes_hut", "method":
     "projected_gradient", "barycenter_mode": "auto", "barycenter_shift": False, "armijo_c": 1e-2,
     "armijo": "standard", "start_lengths": 1.0, "start_factor": 2.0, "start_tau": None, "start": "grow_previous",
-    "tau_min_rel": 1e-12, "max_halvings": None, "remesh": False, "ccd": True, "ccd_gap": 0.05, "ccd_max_iterations": 100,
+    "tau_min_rel": 1e-12, "max_halvings": None, "remesh": True, "remesh_split": 1.5, "remesh_collapse": 0.5,
+    "remesh_rounds": 5, "remesh_rho": 0.5, "remesh_flip_cap": 100, "remesh_order": "flip_then_smooth", "remesh_ops": None,
+    "ccd": True, "ccd_gap": 0.05, "ccd_max_iterations": 100,
     "ccd_close_factor": 0.5, "max_iterations": 1000,
     "stuck_limit": 5, "stop": "energy_stall", "stop_relative": 1e-7, "stop_window": 10, "threads": None,
     "constraints": None, "barycenter": True, "newton_steps": None, "newton_max": 5,
 }
 # the reference's names that only select paths outside this one: allowed at their default only
 _ONLY_DEFAULT = {"energy", "metric", "gmres_side", "gpu_determinism", "method", "barycenter_mode", "barycenter_shift",
-                 "armijo", "start_factor", "remesh"}
+                 "armijo", "start_factor"}
 
 
 def _unsupported(name, value, what):
@@ -76,6 +85,7 @@ def _options(options: dict):
         raise RsError(1, "options", f"energy_eval = {o['energy_eval']!r}: 'barnes_hut' or 'exact'")
     if o["stop"] not in ("energy_stall", "off"):
         raise _unsupported("stop", o["stop"], "this stop rule")
+    o["_remesh"] = _remesh_options(o) if o["remesh"] else None
     rows = _constraints(o["constraints"], o["barycenter"])
     if o["newton_steps"] is not None:
         newton = ("fixed", int(o["newton_steps"]))
@@ -109,6 +119,25 @@ def _options(options: dict):
                        close_factor=float(o["ccd_close_factor"])) if o["ccd"] else None,
     )
     return o, step
+
+
+def _remesh_options(o) -> RemeshOptions:
+    """RemeshOptions from the remesh_* options (repulsive_surfaces.rs:1062-1111)."""
+    if o["remesh_order"] not in ("flip_then_smooth", "smooth_then_flip"):
+        raise RsError(1, "options", f"remesh_order = {o['remesh_order']!r}: flip_then_smooth or smooth_then_flip")
+    ops = o["remesh_ops"]
+    if ops is None:
+        enabled = EnabledOps()
+    else:
+        if not isinstance(ops, (list, tuple)):
+            raise RsError(1, "options", f"remesh_ops: expected a list of names, got {type(ops).__name__}")
+        bad = [x for x in ops if x not in ("split", "collapse", "flip", "smooth")]
+        if bad:
+            raise RsError(1, "options", f"remesh_ops: {bad[0]!r} is not valid; use split, collapse, flip, smooth")
+        enabled = EnabledOps(*(k in ops for k in ("split", "collapse", "flip", "smooth")))
+    return RemeshOptions(split=float(o["remesh_split"]), collapse=float(o["remesh_collapse"]),
+                         rounds=int(o["remesh_rounds"]), rho=float(o["remesh_rho"]),
+                         flip_cap_per_vertex=int(o["remesh_flip_cap"]), order=o["remesh_order"], enabled_ops=enabled)
 
 
 def _constraints(value, barycenter):
@@ -157,16 +186,19 @@ def _mesh(positions, faces):
 
 class RepulsiveSurfaceSession:
     """``RepulsiveSurfaceSession`` (crates/ddg-py/src/python/repulsive_surfaces.rs:404) on the
-    device: remeshing and CCD off, the barycenter constraint."""
+    device: the barycenter row (and optional area / volume rows), the CCD cap in the line search
+    and remeshing after every step (session.rs:1107-1187), as the reference's defaults."""
 
     def __init__(self, positions, faces, **options):
         self._options, self._step_options = _options(options)
-        self._p0, self._faces = _mesh(positions, faces)
+        self._p0, self._faces0 = _mesh(positions, faces)
         self._ctx = Context(int(self._options["rocm_ordinal"]))
         self.reset()
 
     def reset(self):
         """Back to the initial mesh with the same options."""
+        self._faces = self._faces0
+        self._face_origin = np.arange(self._faces.shape[0], dtype=np.int32)
         self._surface = Surface(self._ctx, self._faces, self._p0)
         self._session = Session(self._surface, self._step_options)
         self._iteration, self._stuck, self._report = 0, 0, None
@@ -174,7 +206,45 @@ class RepulsiveSurfaceSession:
         self._status = "running"
         self._energy = None
 
+    def _remesh_after_step(self, band):
+        """Step 5 of Session::step_inner (session.rs:1107-1187): the device remeshes the projected
+        positions with the collapse guard; a changed mesh gets a new surface and session that keep
+        ℓ̄₀, the last τ and the targets (the face origins compose through the face parents)."""
+        st = self._session.state()
+        l0 = st["mean_edge_length0"]
+        q = self._surface.positions()
+        if self._options["ccd"]:
+            gap = float(self._options["ccd_gap"]) * l0
+        else:
+            gap = CcdOptions().gap * l0
+            band = intersection_band(self._surface, q)[0]
+        out, rep = _remesh(self._ctx, self._faces, q, l0, self._options["_remesh"], guard_gap=gap, band=band)
+        if out.faces.shape == self._faces.shape and np.array_equal(out.faces, self._faces):
+            self._surface.set_positions(out.positions)
+        else:
+            targets = self._session.row_targets()
+            rows = tuple((kind, targets[i]) for i, (kind, _) in enumerate(self._step_options.constraints))
+            self._session.close()
+            self._surface.close()
+            self._faces = out.faces
+            self._face_origin = self._face_origin[out.face_parents]
+            self._surface = Surface(self._ctx, self._faces, out.positions)
+            self._session = Session(self._surface, dataclasses.replace(self._step_options, constraints=rows))
+            self._session.set_state(l0, st["last_tau"], st["barycenter_target"])
+        return {"splits": rep.splits, "collapses": rep.collapses, "flips": rep.flips,
+                "refused_splits": rep.refused_splits, "splits_deferred": rep.splits_deferred,
+                "refused_collapses": rep.refused_collapses, "refused_flips": rep.refused_flips,
+                "smoothing_moves": rep.smoothing_moves, "smoothing_halved": rep.smoothing_halved,
+                "smoothing_refused": rep.smoothing_refused, "smoothing_skipped": rep.smoothing_skipped,
+                "flip_cap_hits": rep.flip_cap_hits, "n_vertices": rep.n_vertices, "n_faces": rep.n_faces,
+                "seconds": rep.seconds}
+
     def step(self) -> dict:
+        remeshing = self._options["_remesh"] is not None
+        band = None
+        if remeshing and self._options["ccd"]:
+            # the faces that intersect at the start of the step (session.rs:920-928)
+            band = intersection_band(self._surface, self._surface.positions())[0]
         r = self._session.step()
         self._iteration += 1
         self._stuck = 0 if r.accepted else self._stuck + 1
@@ -192,6 +262,7 @@ class RepulsiveSurfaceSession:
             self._status = "max_iterations"
         g = r.gmres
         phi_before, phi = self._session.constraints()
+        remesh_report = self._remesh_after_step(band) if remeshing else None
         self._report = {
             "iteration": self._iteration, "status": "accepted" if r.accepted else "stuck",
             "session_status": self._status, "n_vertices": self._surface.nv, "n_faces": self._surface.nf,
@@ -204,7 +275,7 @@ class RepulsiveSurfaceSession:
             "linear": {"iterations": g.iterations, "cycles": g.cycles, "converged": g.converged,
                        "relative_residual": g.relative_residual, "cycle_residuals": list(g.cycle_residuals),
                        "history": list(g.history), "fallbacks": list(r.fallbacks)},
-            "remesh": None, "seconds": dict(r.seconds),
+            "remesh": remesh_report, "seconds": dict(r.seconds),
         }
         return self._report
 
@@ -232,7 +303,7 @@ class RepulsiveSurfaceSession:
     def snapshot(self) -> dict:
         st = self._session.state()
         return {"positions": self.positions, "faces": self.faces, "pinned": np.zeros(0, dtype=np.int32),
-                "face_origin": np.arange(self._surface.nf, dtype=np.int32), "iteration": self._iteration,
+                "face_origin": self._face_origin.copy(), "iteration": self._iteration,
                 "energy": self._energy, "status": self._status, "error": None,
                 "mean_edge_length0": st["mean_edge_length0"], "report": self._report}
 

@@ -455,6 +455,32 @@ int ddgrs_session_set_learned(ddgrs_session* h, ddgrs_learned* l, int32_t mode, 
   });
 }
 
+int ddgrs_session_set_state(ddgrs_session* h, const double* in5, ddgrs_error* err) {
+  return guarded(err, "session_set_state", [&] {
+    need(h, "session");
+    need(in5, "state");
+    for (int k = 0; k < 5; ++k)
+      if (!std::isfinite(in5[k])) invalid("state", std::to_string(in5[k]), "finite values");
+    if (!(in5[0] > 0.0)) invalid("mean_edge_length0", std::to_string(in5[0]), "positive");
+    if (in5[1] < 0.0) invalid("last_tau", std::to_string(in5[1]), "non-negative");
+    h->ses.l0 = in5[0];
+    h->ses.last_tau = in5[1];
+    for (int k = 0; k < 3; ++k) h->ses.target[k] = in5[2 + k];
+  });
+}
+
+int ddgrs_session_row_targets(ddgrs_session* h, double* out2, ddgrs_error* err) {
+  return guarded(err, "session_row_targets", [&] {
+    need(h, "session");
+    need(out2, "out");
+    out2[0] = out2[1] = NAN;
+    size_t j = 0;
+    for (int r = 0; r < 2; ++r)
+      if (h->ses.opts.rows[r] == kRowArea || h->ses.opts.rows[r] == kRowVolume)
+        out2[r] = j < h->ses.nl_targets.size() ? h->ses.nl_targets[j++] : NAN;
+  });
+}
+
 int ddgrs_session_state(ddgrs_session* h, double* out5, ddgrs_error* err) {
   return guarded(err, "session_state", [&] {
     need(h, "session");

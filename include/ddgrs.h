@@ -428,6 +428,12 @@ DDGRS_API void ddgrs_session_destroy(ddgrs_session* h);
 DDGRS_API int ddgrs_session_step(ddgrs_session* h, ddgrs_step_report* report, ddgrs_error* err);
 /* [ℓ̄₀, last τ, barycenter target (3)] (session.rs:647 mean_edge_length0). */
 DDGRS_API int ddgrs_session_state(ddgrs_session* h, double* out5, ddgrs_error* err);
+/* Sets [ℓ̄₀, last τ, barycenter target (3)]: a session created on a remeshed surface continues the
+ * run (session.rs:1193 keeps ℓ̄₀, the line search's last τ and the targets across a remeshing). */
+DDGRS_API int ddgrs_session_set_state(ddgrs_session* h, const double* in5, ddgrs_error* err);
+/* The targets of the area / volume rows (in the order of ddgrs_step_options.rows; NaN for none):
+ * the given ones or the values at creation. */
+DDGRS_API int ddgrs_session_row_targets(ddgrs_session* h, double* out2, ddgrs_error* err);
 
 /* ---- the learned right preconditioner --------------------------------------------------------
  * No counterpart in the reference, whose solver has the plain preconditioner only.  A mesh

@@ -63,7 +63,17 @@ def test_session_options():
     assert pkg.api._options({"ccd": False})[1].ccd is None
     c = pkg.api._options({"ccd_gap": 0.1, "ccd_max_iterations": 7, "ccd_close_factor": 0.25})[1].ccd
     assert (c.gap, c.max_iterations, c.close_factor, c.reference_length) == (0.1, 7, 0.25, None)
-    for bad, kind in (({"remesh": True}, "Unsupported"), ({"device": "cuda"}, "Unsupported"),
+    # remeshing is on by default with the reference's RemeshOptions (repulsive_surfaces.rs:1062-1111)
+    assert pkg.api._options({})[0]["_remesh"] == pkg.RemeshOptions()
+    assert pkg.api._options({"remesh": False})[0]["_remesh"] is None
+    r = pkg.api._options({"remesh_split": 1.8, "remesh_collapse": 0.4, "remesh_rounds": 2, "remesh_rho": 0.25,
+                          "remesh_flip_cap": 7, "remesh_order": "smooth_then_flip",
+                          "remesh_ops": ["flip", "smooth"]})[0]["_remesh"]
+    assert (r.split, r.collapse, r.rounds, r.rho, r.flip_cap_per_vertex, r.order) == (1.8, 0.4, 2, 0.25, 7,
+                                                                                       "smooth_then_flip")
+    assert r.enabled_ops == pkg.EnabledOps(False, False, True, True)
+    for bad, kind in (({"remesh_order": "sideways"}, "InvalidOption"), ({"remesh_ops": ["twist"]}, "InvalidOption"),
+                      ({"remesh_ops": "flip"}, "InvalidOption"), ({"device": "cuda"}, "Unsupported"),
                       ({"metric": "h1"}, "Unsupported"), ({"penalties": [1]}, "Unsupported"),
                       ({"constraints": [{"type": "area"}]}, "Unsupported"),
                       ({"constraints": [{"type": "barycenter"}, {"type": "pins", "vertices": [0]}]}, "Unsupported"),
