@@ -218,6 +218,42 @@ LprimeSymbolic lprime_symbolic(const std::vector<uint32_t>& faces, size_t nv, co
   }
   for (uint32_t b = 0; b < nb; ++b)
     S.ltoff[b + 1] = S.ltoff[b] + (S.hsplit[S.bheight[b]] ? (uint64_t)S.bk[b] * (S.bf[b] - S.bk[b]) : 0);
+  // subtrees of the lowest small-front heights whose fronts fit a wave's LDS slice, at most
+  // kSubLevels of them (DDGRS_LPRIME_SUBTREES=<levels> sets that bound; 0: every height by its own
+  // launches)
+  const char* sub_env = std::getenv("DDGRS_LPRIME_SUBTREES");
+  const uint32_t max_levels = sub_env ? (uint32_t)std::strtoul(sub_env, nullptr, 10) : kSubLevels;
+  for (uint32_t h = 0; h < std::min(S.H, max_levels) && !S.hsplit[h]; ++h) {
+    uint32_t need = 0;
+    for (uint32_t q = S.hoff[h]; q < S.hoff[h + 1]; ++q)
+      need = std::max(need, 3 * (S.bf[S.hblocks[q]] + S.bk[S.hblocks[q]]));
+    if (need > kSubWaveDoubles) break;
+    S.sub_levels = h + 1;
+    S.sub_wave_doubles = std::max(S.sub_wave_doubles, need);
+  }
+  if (S.sub_levels > 0) {
+    const uint32_t L = S.sub_levels;
+    std::vector<std::vector<uint32_t>> lv(L);
+    std::vector<uint32_t> stack;
+    S.sub_lvl_off.assign(1, 0);
+    for (uint32_t b = 0; b < nb; ++b) {
+      const uint32_t p = S.bparent[b];
+      if (S.bheight[b] >= L || (p != kNone && S.bheight[p] < L)) continue;
+      for (auto& v : lv) v.clear();
+      stack.assign(1, b);
+      while (!stack.empty()) {
+        const uint32_t c = stack.back();
+        stack.pop_back();
+        if (S.bf[c] > 0) lv[S.bheight[c]].push_back(c);
+        for (uint32_t q = S.child_ptr[c]; q < S.child_ptr[c + 1]; ++q) stack.push_back(S.child[q]);
+      }
+      for (auto& v : lv) {
+        std::sort(v.begin(), v.end());
+        S.sub_fronts.insert(S.sub_fronts.end(), v.begin(), v.end());
+        S.sub_lvl_off.push_back((uint32_t)S.sub_fronts.size());
+      }
+    }
+  }
   // work lists per height
   const size_t lds_doubles = lds_bytes / sizeof(double);
   auto fits_lds = [&](uint32_t f) { return (uint64_t)f * (f + 1) / 2 + f <= lds_doubles; };
@@ -814,6 +850,185 @@ __global__ __launch_bounds__(256) void lf_backward_small_kernel(FrontArgs A, con
   backward_small<false>(A, list[blockIdx.x], srows, moff, minv, front, y, x, sh);
 }
 
+// ---- subtrees of the lowest heights: one workgroup per subtree, one front per wave ---------------
+//
+// A workgroup takes its subtree's blocks level by level (forward from the leaves up, backward from
+// the subtree's root down), each wave one block at a time, the levels separated by workgroup
+// barriers: a block's update vector (forward) or its later rows' x (backward) come from the same
+// workgroup's previous level, or from launches before this one.  The matrices are read where they
+// lie, each lane one row of the block (M_J through M_Jᵀ, so a wave's loads of one column are
+// contiguous); every sum runs over the same terms in the same order as in the small-front kernels
+// above, so the results are bitwise those of one launch per height.
+
+// Loads a lane issues before it uses the first of them.
+constexpr uint32_t kSubBatch = 16;
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Forward of block J by one wave.  LDS slice: w (3 f), y_J (3 k).
+__device__ void forward_wave(const FrontArgs& A, uint32_t J, const uint64_t* __restrict__ moff,
+                             const double* __restrict__ minv_t, const double* __restrict__ front,
+                             const double* __restrict__ b, double* __restrict__ y, double* __restrict__ ubuf,
+                             double* w, uint32_t lane) {
+  const uint32_t f = A.bf[J], k = A.bk[J], j0 = A.bstart[J], ns = f - k;
+  double* yj = w + 3 * (size_t)f;
+  for (uint32_t i = lane; i < f; i += 64)
+    for (int a = 0; a < 3; ++a) w[3 * i + a] = i < k ? b[3 * (size_t)(j0 + i) + a] : 0.0;
+  wave_sync();
+  for (uint32_t q = A.child_ptr[J]; q < A.child_ptr[J + 1]; ++q) {
+    const uint32_t ch = A.child[q], cs = A.bf[ch] - A.bk[ch];
+    const uint32_t* map = A.ea_map + A.ea_off[ch];
+    const double* uc = ubuf + 3 * (size_t)A.ea_off[ch];
+    for (uint32_t r = lane; r < cs; r += 64)
+      for (int a = 0; a < 3; ++a) w[3 * map[r] + a] += uc[3 * r + a];
+    wave_sync();
+  }
+  const double* Mt = minv_t + moff[J];
+  for (uint32_t i = lane; i < k; i += 64) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (uint32_t l0 = 0; l0 <= i; l0 += kSubBatch) {
+      double m[kSubBatch];
+#pragma unroll
+      for (uint32_t j = 0; j < kSubBatch; ++j) m[j] = l0 + j <= i ? Mt[(size_t)(l0 + j) * k + i] : 0.0;
+#pragma unroll
+      for (uint32_t j = 0; j < kSubBatch; ++j) {
+        const uint32_t l = l0 + j;
+        if (l <= i) {
+          s0 = fma(m[j], w[3 * l], s0);
+          s1 = fma(m[j], w[3 * l + 1], s1);
+          s2 = fma(m[j], w[3 * l + 2], s2);
+        }
+      }
+    }
+    yj[3 * i] = s0;
+    yj[3 * i + 1] = s1;
+    yj[3 * i + 2] = s2;
+    y[3 * (size_t)(j0 + i)] = s0;
+    y[3 * (size_t)(j0 + i) + 1] = s1;
+    y[3 * (size_t)(j0 + i) + 2] = s2;
+  }
+  wave_sync();
+  const double* G = front + A.foff[J];
+  double* u = ubuf + 3 * (size_t)A.ea_off[J];
+  for (uint32_t r = lane; r < ns; r += 64) {
+    const double* row = G + (size_t)(k + r) * f;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (uint32_t l0 = 0; l0 < k; l0 += kSubBatch) {
+      double v[kSubBatch];
+#pragma unroll
+      for (uint32_t j = 0; j < kSubBatch; ++j) v[j] = l0 + j < k ? row[l0 + j] : 0.0;
+#pragma unroll
+      for (uint32_t j = 0; j < kSubBatch; ++j) {
+        const uint32_t l = l0 + j;
+        if (l < k) {
+          s0 = fma(v[j], yj[3 * l], s0);
+          s1 = fma(v[j], yj[3 * l + 1], s1);
+          s2 = fma(v[j], yj[3 * l + 2], s2);
+        }
+      }
+    }
+    u[3 * r] = w[3 * (k + r)] - s0;
+    u[3 * r + 1] = w[3 * (k + r) + 1] - s1;
+    u[3 * r + 2] = w[3 * (k + r) + 2] - s2;
+  }
+  wave_sync();  // the slice is reused by the wave's next block
+}
+
+// Backward of block J by one wave.  LDS slice: x_S (3 (f − k)), t (3 k).
+__device__ void backward_wave(const FrontArgs& A, uint32_t J, const uint32_t* __restrict__ srows,
+                              const uint64_t* __restrict__ moff, const double* __restrict__ minv,
+                              const double* __restrict__ front, const double* __restrict__ y, double* __restrict__ x,
+                              double* xs, uint32_t lane) {
+  const uint32_t f = A.bf[J], k = A.bk[J], j0 = A.bstart[J], ns = f - k;
+  if (k == 0) return;  // uniform per wave
+  double* t = xs + 3 * (size_t)ns;
+  const double* G = front + A.foff[J];
+  const uint32_t* S = srows + A.ea_off[J];
+  for (uint32_t r = lane; r < ns; r += 64)
+    for (int a = 0; a < 3; ++a) xs[3 * r + a] = x[3 * (size_t)S[r] + a];
+  wave_sync();
+  for (uint32_t i = lane; i < k; i += 64) {
+    const double d = G[(size_t)i * f + i];
+    double s0 = y[3 * (size_t)(j0 + i)] / d, s1 = y[3 * (size_t)(j0 + i) + 1] / d, s2 = y[3 * (size_t)(j0 + i) + 2] / d;
+    for (uint32_t r0 = 0; r0 < ns; r0 += kSubBatch) {
+      double v[kSubBatch];
+#pragma unroll
+      for (uint32_t j = 0; j < kSubBatch; ++j) v[j] = r0 + j < ns ? G[(size_t)(k + r0 + j) * f + i] : 0.0;
+#pragma unroll
+      for (uint32_t j = 0; j < kSubBatch; ++j) {
+        const uint32_t r = r0 + j;
+        if (r < ns) {
+          s0 = fma(-v[j], xs[3 * r], s0);
+          s1 = fma(-v[j], xs[3 * r + 1], s1);
+          s2 = fma(-v[j], xs[3 * r + 2], s2);
+        }
+      }
+    }
+    t[3 * i] = s0;
+    t[3 * i + 1] = s1;
+    t[3 * i + 2] = s2;
+  }
+  wave_sync();
+  const double* M = minv + moff[J];
+  for (uint32_t i = lane; i < k; i += 64) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (uint32_t l0 = i; l0 < k; l0 += kSubBatch) {
+      double m[kSubBatch];
+#pragma unroll
+      for (uint32_t j = 0; j < kSubBatch; ++j) m[j] = l0 + j < k ? M[(size_t)(l0 + j) * k + i] : 0.0;
+#pragma unroll
+      for (uint32_t j = 0; j < kSubBatch; ++j) {
+        const uint32_t l = l0 + j;
+        if (l < k) {
+          s0 = fma(m[j], t[3 * l], s0);
+          s1 = fma(m[j], t[3 * l + 1], s1);
+          s2 = fma(m[j], t[3 * l + 2], s2);
+        }
+      }
+    }
+    x[3 * (size_t)(j0 + i)] = s0;
+    x[3 * (size_t)(j0 + i) + 1] = s1;
+    x[3 * (size_t)(j0 + i) + 2] = s2;
+  }
+  wave_sync();
+}
+
+// One workgroup per subtree: `levels` level ranges of `fronts` at lvl[blockIdx.x · levels ...].
+__global__ __launch_bounds__(64 * kSubWaves) void lf_forward_sub_kernel(
+    FrontArgs A, uint32_t levels, uint32_t wave_doubles, const uint32_t* __restrict__ fronts,
+    const uint32_t* __restrict__ lvl, const uint64_t* __restrict__ moff, const double* __restrict__ minv_t,
+    const double* __restrict__ front, const double* __restrict__ b, double* __restrict__ y, double* __restrict__ ubuf) {
+  extern __shared__ double sh[];
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double* w = sh + (size_t)wave * wave_doubles;
+  const uint32_t* L = lvl + (size_t)blockIdx.x * levels;
+  for (uint32_t h = 0; h < levels; ++h) {
+    for (uint32_t q = L[h] + wave; q < L[h + 1]; q += kSubWaves)
+      forward_wave(A, fronts[q], moff, minv_t, front, b, y, ubuf, w, lane);
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(64 * kSubWaves) void lf_backward_sub_kernel(
+    FrontArgs A, uint32_t levels, uint32_t wave_doubles, const uint32_t* __restrict__ fronts,
+    const uint32_t* __restrict__ lvl, const uint32_t* __restrict__ srows, const uint64_t* __restrict__ moff,
+    const double* __restrict__ minv, const double* __restrict__ front, const double* __restrict__ y,
+    double* __restrict__ x) {
+  extern __shared__ double sh[];
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double* xs = sh + (size_t)wave * wave_doubles;
+  const uint32_t* L = lvl + (size_t)blockIdx.x * levels;
+  for (uint32_t h = levels; h-- > 0;) {
+    for (uint32_t q = L[h] + wave; q < L[h + 1]; q += kSubWaves)
+      backward_wave(A, fronts[q], srows, moff, minv, front, y, x, xs, lane);
+    __syncthreads();
+  }
+}
+
 // ---- large fronts: rows spread over many workgroups, four rows per wave -----------------------
 
 // Σ_l row_q[l] v[l] (3 axes) for the four rows of a wave: rows q with length len[q], reduced over
@@ -1351,6 +1566,13 @@ void LprimeFactorDev::setup(ddgrs_ctx* ctx, const LprimeSymbolic& S) {
     else
       small_lds = std::max<size_t>(small_lds, hlds[h]);
   }
+  sub_levels = S.sub_levels;
+  sub_wave_doubles = S.sub_wave_doubles;
+  nsub = sub_levels ? (uint32_t)(S.sub_lvl_off.size() - 1) / sub_levels : 0;
+  if (nsub) {
+    ddgrs::upload(ctx, sub_fronts, S.sub_fronts);
+    ddgrs::upload(ctx, sub_lvl_off, S.sub_lvl_off);
+  }
   ntasks = (uint32_t)tl.size();
   ddgrs::upload(ctx, tasks, tl);
   ddgrs::upload(ctx, task_counts, tc);
@@ -1469,7 +1691,13 @@ void LprimeFactorDev::solve(ddgrs_ctx* ctx, const double* r, double* x) {
   const size_t lds = (3 * (size_t)max_front + 3 * (size_t)max_rows) * sizeof(double);
   const char* tp = std::getenv("DDGRS_LPRIME_SOLVE_TWO_PHASE");
   const bool two_phase = tp && tp[0] == '1';
-  for (uint32_t h = 0; h < H; ++h) {
+  const size_t sub_lds = (size_t)kSubWaves * sub_wave_doubles * sizeof(double);
+  if (nsub)
+    launch(ctx, "lf_forward_sub", lf_forward_sub_kernel, dim3(nsub), dim3(64 * kSubWaves), sub_lds, A, sub_levels,
+           sub_wave_doubles, (const uint32_t*)sub_fronts.ptr, (const uint32_t*)sub_lvl_off.ptr,
+           (const uint64_t*)moff.ptr, (const double*)minv_t.ptr, (const double*)front.ptr, (const double*)xb.ptr,
+           yb.ptr, ubuf.ptr);
+  for (uint32_t h = sub_levels; h < H; ++h) {
     if (hsplit[h] && !two_phase) {
       const uint32_t ny = krows_off[h + 1] - krows_off[h], nu = urows_off[h + 1] - urows_off[h];
       launch(ctx, height_name("lf_forward", h), lf_forward_yu_kernel, dim3(ny + nu), dim3(256), lds, A,
@@ -1491,7 +1719,7 @@ void LprimeFactorDev::solve(ddgrs_ctx* ctx, const double* r, double* x) {
              (const double*)yb.ptr, ubuf.ptr);
     }
   }
-  for (uint32_t h = H; h-- > 0;) {
+  for (uint32_t h = H; h-- > sub_levels;) {
     if (!hsplit[h]) {
       launch(ctx, height_name("lf_backward", h), lf_backward_small_kernel, dim3(hoff[h + 1] - hoff[h]), dim3(256),
              (size_t)hlds[h], A, (const uint32_t*)hblocks.ptr + hoff[h], (const uint32_t*)srows.ptr,
@@ -1505,6 +1733,11 @@ void LprimeFactorDev::solve(ddgrs_ctx* ctx, const double* r, double* x) {
              (const double*)tb.ptr, xb.ptr);
     }
   }
+  if (nsub)
+    launch(ctx, "lf_backward_sub", lf_backward_sub_kernel, dim3(nsub), dim3(64 * kSubWaves), sub_lds, A, sub_levels,
+           sub_wave_doubles, (const uint32_t*)sub_fronts.ptr, (const uint32_t*)sub_lvl_off.ptr,
+           (const uint32_t*)srows.ptr, (const uint64_t*)moff.ptr, (const double*)minv.ptr, (const double*)front.ptr,
+           (const double*)yb.ptr, xb.ptr);
   launch(ctx, "lf_scatter_sol", lf_scatter_sol_kernel, dim3(blocks_for(3 * (int64_t)n, 256)), dim3(256), 0, un, unv,
          (const uint32_t*)perm.ptr, (const double*)xb.ptr, (const uint32_t*)solve_failed.ptr, x);
 }

@@ -30,6 +30,11 @@ struct FrontPanel {  // a panel's rows [c1 + r0, c1 + r0 + kPanelItemRows) below
 constexpr uint32_t kSolveRows = 16;
 // LDS of a small-front solve workgroup (its matrices are staged there).
 constexpr size_t kSolveSmallLds = 64 * 1024;
+// Subtree solves of the lowest heights: waves per workgroup (one front per wave at a time) and the
+// largest LDS slice of a wave, in doubles (w or x_S and y_J or t: 3 f + 3 k).
+constexpr uint32_t kSubWaves = 8;
+constexpr uint32_t kSubWaveDoubles = 1024;
+constexpr uint32_t kSubLevels = 3;
 struct SolveItem {  // rows [r0, r0 + kSolveRows) of a block (or of its later front rows)
   uint32_t block, r0;
 };
@@ -78,6 +83,12 @@ struct LprimeSymbolic {
   std::vector<uint32_t> krows_off, urows_off;  // per height
   std::vector<uint64_t> ltoff;                 // per block: its L_SJᵀ (large-front heights only)
   std::vector<FrontTile> lt_tiles;             // transpose tiles (block, first later row, first column)
+  // Heights [0, sub_levels) are solved by one launch per phase, one workgroup per subtree of those
+  // heights (rooted at a block whose parent is higher, or at the root), level by level inside it.
+  // sub_fronts: each subtree's blocks, level-major; the range of subtree s at level h is
+  // [sub_lvl_off[s·sub_levels + h], sub_lvl_off[s·sub_levels + h + 1]).
+  uint32_t sub_levels = 0, sub_wave_doubles = 0;
+  std::vector<uint32_t> sub_fronts, sub_lvl_off;
   static constexpr uint64_t kNoPos = ~0ull;
 };
 // `lds_bytes`: the LDS a workgroup may use (fronts whose lower triangle and one column fit are
@@ -117,6 +128,8 @@ struct LprimeFactorDev {
   DevBuf<SolveTask> tasks;
   DevBuf<TaskCounts> task_counts;
   DevBuf<uint32_t> task_done, solve_failed;
+  DevBuf<uint32_t> sub_fronts, sub_lvl_off;
+  uint32_t sub_levels = 0, sub_wave_doubles = 0, nsub = 0;
   uint32_t nb = 0, ntasks = 0;
   size_t one_lds = 0;   // its LDS per workgroup
   bool one_ok = false;  // the one-launch solve fits the device (else the per-height launches)
@@ -135,8 +148,8 @@ struct LprimeFactorDev {
   // x = L′⁻¹ r for a full saddle vector (3 nv + 3, interleaved positions then multipliers), the
   // three axes together: forward by height (each front's right-hand side from its block and its
   // children's update vectors, its block solved by the dense inverse, its update vector formed),
-  // D⁻¹, backward by height (each block from its later front rows), one launch per height and
-  // phase.  DDGRS_LPRIME_SOLVE_ONE_LAUNCH=1 asks for a single launch whose workgroups take the
+  // D⁻¹, backward by height (each block from its later front rows); the lowest heights in one
+  // launch per phase (a workgroup per subtree), the others one launch per height and phase.  DDGRS_LPRIME_SOLVE_ONE_LAUNCH=1 asks for a single launch whose workgroups take the
   // blocks' tasks in dependency order (the same arithmetic per task; not yet validated on the
   // device, so off by default).
   void solve(ddgrs_ctx* ctx, const double* r, double* x);

@@ -175,6 +175,27 @@ def test_lprime_solve_paths_agree(pkg, ctx, monkeypatch):
     assert rel(xl, xl2) <= 1e-8
 
 
+@pytest.mark.parametrize("name", ["icosphere_l3", "torus_16k", "bumpy_l6"])
+def test_lprime_subtree_launches_match_per_height(pkg, ctx, name, monkeypatch):
+    """The lowest heights' solves in one launch per phase (one workgroup per subtree, a block per
+    wave) against one launch per height and phase: each block's sums run over the same terms in the
+    same order, so the solutions are bitwise equal.  Three right-hand sides in a row."""
+    gen = {"bumpy_l6": pkg.generate.config_bumpy_l6, **meshes(pkg)}[name]
+    p, f = gen()
+    s = pkg.Surface(ctx, f, p)
+    sub = pkg.Saddle(s)
+    monkeypatch.setenv("DDGRS_LPRIME_SUBTREES", "0")
+    per = pkg.Saddle(s)
+    monkeypatch.delenv("DDGRS_LPRIME_SUBTREES")
+    rng = np.random.default_rng(26)
+    for i in range(3):
+        r = rng.standard_normal(sub.n)
+        xs, xp = sub.lprime_solve(r), per.lprime_solve(r)
+        print(f"{name} rhs {i}: max |subtree - per height| {np.abs(xs - xp).max():.3e}", flush=True)
+        assert np.all(np.isfinite(xs))
+        assert np.array_equal(xs, xp)
+
+
 def test_lprime_one_launch_is_refused(pkg, ctx, monkeypatch):
     """The one-launch L′ solve is disabled: asking for it raises InvalidOption before any launch."""
     p, f = pkg.generate.config_icosphere_l3()
