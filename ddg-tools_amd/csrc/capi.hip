@@ -48,6 +48,7 @@ void ctx_release(ddgrs_ctx* ctx) {
     }
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     if (ctx->arena) (void)hipHostFree(ctx->arena);
+    for (char* a : ctx->arena_retired) (void)hipHostFree(a);
     ctx->scratch = Scratch();
     (void)hipStreamDestroy(ctx->stream);
   } catch (...) {

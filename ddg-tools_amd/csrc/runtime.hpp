@@ -131,22 +131,30 @@ struct ddgrs_ctx {
   }
   // Pinned memory that uploads are copied through: a copy from pageable memory is staged by the
   // runtime through buffers of its own, shared by every host thread, and waits for them.  A region
-  // of the arena is reused only after a synchronization of the stream (the copies reading it are
-  // done): when the arena is full the stream is synchronized and the arena starts over (and grows
-  // when one upload needs more).
+  // of the arena is reused only after a synchronization of the stream (sync() starts the arena
+  // over: the copies reading it are done).  When the arena is full before that, the uploads
+  // continue in a new arena twice as large and the full one is kept until the context is released
+  // (copies may still be reading it): waiting for the stream instead would also wait for every
+  // kernel queued before the upload.  After the first steps one arena holds a step's uploads.
+  // Past kArenaGrowLimit bytes the arena stops growing and a full one waits for the stream.
+  static constexpr size_t kArenaGrowLimit = (size_t)256 << 20;
   char* arena = nullptr;
   size_t arena_bytes = 0, arena_used = 0;
+  std::vector<char*> arena_retired;
   void* arena_take(size_t bytes) {
     bytes = (bytes + 255) & ~(size_t)255;
-    if (arena_used + bytes > arena_bytes) {
+    if (arena_used + bytes > arena_bytes && bytes <= arena_bytes && 2 * arena_bytes > kArenaGrowLimit) {
       HIPCHECK(hipStreamSynchronize(stream));
       arena_used = 0;
-      if (bytes > arena_bytes) {
-        if (arena) (void)hipHostFree(arena);
-        arena = nullptr;
-        arena_bytes = std::max<size_t>(bytes, std::max<size_t>(2 * arena_bytes, (size_t)16 << 20));
-        HIPCHECK(hipHostMalloc((void**)&arena, arena_bytes, hipHostMallocDefault));
-      }
+    }
+    if (arena_used + bytes > arena_bytes) {
+      const size_t grown = std::max<size_t>(bytes, std::max<size_t>(2 * arena_bytes, (size_t)16 << 20));
+      char* fresh = nullptr;
+      HIPCHECK(hipHostMalloc((void**)&fresh, grown, hipHostMallocDefault));
+      if (arena) arena_retired.push_back(arena);
+      arena = fresh;
+      arena_bytes = grown;
+      arena_used = 0;
     }
     void* p = arena + arena_used;
     arena_used += bytes;
@@ -167,6 +175,7 @@ struct ddgrs_ctx {
   }
   void sync() {
     HIPCHECK(hipStreamSynchronize(stream));
+    arena_used = 0;
     if (!pending.empty()) harvest();
   }
 };
