@@ -103,6 +103,7 @@ __global__ __launch_bounds__(256) void blk_classify_kernel(uint32_t m, const uin
 // `overflow` and the level and all later ones do nothing.
 constexpr uint32_t kLevelBlocks = 1024;
 constexpr uint32_t kMaxLevels = 64;  // levels whose sizes are recorded for the next build's grids
+constexpr uint64_t kFirstCapPerNode = 64;  // a first build's list capacities per tree node
 
 struct LevelState {
   uint32_t m[2];                // pairs of the level, by level parity
@@ -587,6 +588,7 @@ static bool device_levels(ddgrs_ctx* ctx, BlockBuildScratch& S, const BlockGeo& 
   S.level_m.reserve(kMaxLevels);
   // a level's grid: about 512 pairs per block for the size the previous build had at that level
   auto grid_for = [&](uint32_t lev) -> uint32_t {
+    if (S.prev_m.empty()) return kLevelBlocks / 4;  // a first build: no sizes yet
     const uint64_t m = lev < S.prev_m.size() ? S.prev_m[lev] : 0;
     return (uint32_t)std::min<uint64_t>(kLevelBlocks, std::max<uint64_t>(1, (m + m / 4 + 511) / 512));
   };
@@ -673,7 +675,18 @@ BlockStats BlockTreeDev::build_structure(ddgrs_ctx* ctx, const Topology& t, cons
   // DDGRS_BLOCK_LEVELS_ON_HOST=1 always reads the totals back per level (for comparisons)
   const bool levels_on_host = std::getenv("DDGRS_BLOCK_LEVELS_ON_HOST") != nullptr;
   uint32_t n_adm = 0, n_near = 0, depth = 0, max_m = 1;
-  const bool on_device = S.cap_pairs && !levels_on_host && device_levels(ctx, S, g, right.ptr, n_adm, n_near, depth, max_m);
+  // a first build sizes its lists from the node count; a list that outgrows its capacity makes the
+  // device pass start over with four times the capacities (the host pass after the third)
+  if (!S.cap_pairs) S.cap_pairs = S.cap_adm = S.cap_near = kFirstCapPerNode * (uint64_t)nn + 4096;
+  bool on_device = false;
+  for (int attempt = 0; attempt < 3 && !levels_on_host && !on_device; ++attempt) {
+    on_device = device_levels(ctx, S, g, right.ptr, n_adm, n_near, depth, max_m);
+    if (!on_device) {
+      S.cap_pairs *= 4;
+      S.cap_adm *= 4;
+      S.cap_near *= 4;
+    }
+  }
   if (!on_device) host_levels(ctx, S, g, right.ptr, n_adm, n_near, depth, max_m);
   if (trace) std::fprintf(stderr, "blocks(%d): levels %s, depth %u\n", n, on_device ? "on the device" : "on the host", depth);
   S.cap_pairs = std::max<uint64_t>(S.cap_pairs, (uint64_t)max_m + max_m / 4 + 1024);

@@ -829,17 +829,18 @@ Topology GpuBvhBuilder::build_levels_on_device(ddgrs_ctx* ctx, size_t n, const d
   // the cut: the first level whose largest node the previous build found small enough for one
   // workgroup's subtree build
   int cut = -1;
-  if (allow_cut && subtrees) {
+  if (allow_cut && subtrees && n > leaf_size) {  // (a root of at most leaf_size elements is the tree)
     for (size_t L = 0; L < prev_levels.size(); ++L)
       if ((double)prev_levels[L] <= cut_size) {
         cut = (int)L;
         break;
       }
     // a cut build records the levels down to its cut only: below them the largest node shrinks by
-    // at least a quarter a level (the split rule keeps a quarter on each side)
-    if (cut < 0 && !prev_levels.empty()) {
-      double est = prev_levels.back();
-      int L = (int)prev_levels.size() - 1;
+    // at least a quarter a level (the split rule keeps a quarter on each side); a first build
+    // starts that bound from the root
+    if (cut < 0) {
+      double est = prev_levels.empty() ? (double)n : prev_levels.back();
+      int L = prev_levels.empty() ? 0 : (int)prev_levels.size() - 1;
       while (est > cut_size && L < (int)kMaxBvhLevels - 1) {
         est *= 0.75;
         ++L;
@@ -976,6 +977,7 @@ Topology GpuBvhBuilder::build_levels_on_device(ddgrs_ctx* ctx, size_t n, const d
     std::vector<size_t> next;
     uint32_t i = 0, biggest = 0;
     if (L == cut) {
+      if ((size_t)L >= lv_host.size()) throw Error(DDGRS_GPU, "BVH build", "the cut level was not built");
       // each cut node's subtree from its splits in level order, left child before right
       for (size_t si : level) {
         if (segs[si].end - segs[si].start <= leaf_size) continue;
