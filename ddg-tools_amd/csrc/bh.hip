@@ -627,6 +627,144 @@ __global__ __launch_bounds__(256) void face_boxes_kernel(int64_t nf, const uint3
   }
 }
 
+// ---- cluster aggregates on the device ------------------------------------------------------------
+//
+// The sums of build_clusters (trees_host.hip) with the same operations in the same order: each
+// leaf over its elements in tree order, each internal node as left + right (one launch per depth,
+// deepest first), then centroid, radius and projector per node.  Box corners use the C library's
+// fmin / fmax (the first argument when it is smaller, or when the second is NaN).  No contraction:
+// every product is rounded before its sum.
+namespace {
+
+__device__ __forceinline__ double lib_fmin(double x, double y) { return (x < y || y != y) ? x : y; }
+__device__ __forceinline__ double lib_fmax(double x, double y) { return (x > y || y != y) ? x : y; }
+
+struct ClusterOut {  // offsets into one buffer of 27 doubles per node
+  double *mass, *centroid, *lo, *hi, *radius, *projector, *mom, *ps;
+};
+ClusterOut cluster_out(double* b, size_t nn) {
+  return {b, b + nn, b + 4 * nn, b + 7 * nn, b + 10 * nn, b + 11 * nn, b + 17 * nn, b + 20 * nn};
+}
+
+__global__ __launch_bounds__(256) void cl_leaf_kernel(uint32_t nl, const uint32_t* __restrict__ leaves,
+                                                      const uint32_t* __restrict__ start,
+                                                      const uint32_t* __restrict__ end,
+                                                      const uint32_t* __restrict__ order, const double* __restrict__ pts,
+                                                      const double* __restrict__ lo, const double* __restrict__ hi,
+                                                      const double* __restrict__ mass, const double* __restrict__ nrm,
+                                                      ClusterOut C) {
+#pragma clang fp contract(off)
+  const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= nl) return;
+  const uint32_t a = leaves[k];
+  double bl[3] = {INFINITY, INFINITY, INFINITY}, bh[3] = {-INFINITY, -INFINITY, -INFINITY};
+  double m = 0.0, mo[3] = {0.0, 0.0, 0.0}, o[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (uint32_t q = start[a]; q < end[a]; ++q) {
+    const uint32_t e = order[q];
+    const double w = mass[e];
+    for (int j = 0; j < 3; ++j) {
+      bl[j] = lib_fmin(bl[j], lo[3 * (size_t)e + j]);
+      bh[j] = lib_fmax(bh[j], hi[3 * (size_t)e + j]);
+      mo[j] += w * pts[3 * (size_t)e + j];
+    }
+    m += w;
+    if (nrm) {
+      const double n0 = nrm[3 * (size_t)e], n1 = nrm[3 * (size_t)e + 1], n2 = nrm[3 * (size_t)e + 2];
+      o[0] += w;
+      o[1] += w * n0 * n0;
+      o[2] += w * n0 * n1;
+      o[3] += w * n0 * n2;
+      o[4] += w * n1 * n1;
+      o[5] += w * n1 * n2;
+      o[6] += w * n2 * n2;
+    }
+  }
+  C.mass[a] = m;
+  for (int j = 0; j < 3; ++j) {
+    C.mom[3 * (size_t)a + j] = mo[j];
+    C.lo[3 * (size_t)a + j] = bl[j];
+    C.hi[3 * (size_t)a + j] = bh[j];
+  }
+  if (nrm)
+    for (int j = 0; j < 7; ++j) C.ps[7 * (size_t)a + j] = o[j];
+}
+
+__global__ __launch_bounds__(256) void cl_internal_kernel(uint32_t nn, uint32_t d, const uint32_t* __restrict__ depth,
+                                                          const uint32_t* __restrict__ right, int with_ps, ClusterOut C) {
+#pragma clang fp contract(off)
+  const uint32_t a = blockIdx.x * 256 + threadIdx.x;
+  if (a >= nn || depth[a] != d || right[a] == 0) return;
+  const size_t l = a + 1, r = right[a];
+  C.mass[a] = C.mass[l] + C.mass[r];
+  for (int j = 0; j < 3; ++j) {
+    C.mom[3 * a + j] = C.mom[3 * l + j] + C.mom[3 * r + j];
+    C.lo[3 * a + j] = lib_fmin(lib_fmin(INFINITY, C.lo[3 * l + j]), C.lo[3 * r + j]);
+    C.hi[3 * a + j] = lib_fmax(lib_fmax(-INFINITY, C.hi[3 * l + j]), C.hi[3 * r + j]);
+  }
+  if (with_ps)
+    for (int j = 0; j < 7; ++j) C.ps[7 * a + j] = C.ps[7 * l + j] + C.ps[7 * r + j];
+}
+
+__global__ __launch_bounds__(256) void cl_final_kernel(uint32_t nn, int with_ps, ClusterOut C) {
+#pragma clang fp contract(off)
+  const uint32_t a = blockIdx.x * 256 + threadIdx.x;
+  if (a >= nn) return;
+  if (with_ps)
+    for (int j = 0; j < 6; ++j) C.projector[6 * (size_t)a + j] = C.ps[7 * (size_t)a + j + 1] / C.ps[7 * (size_t)a];
+  double x[3], r2 = 0.0;
+  for (int j = 0; j < 3; ++j) {
+    x[j] = C.mom[3 * (size_t)a + j] / C.mass[a];
+    C.centroid[3 * (size_t)a + j] = x[j];
+  }
+  for (int j = 0; j < 3; ++j) {
+    const double dd = lib_fmax(C.hi[3 * (size_t)a + j] - x[j], x[j] - C.lo[3 * (size_t)a + j]);
+    r2 += dd * dd;
+  }
+  C.radius[a] = sqrt(r2);
+}
+
+}  // namespace
+
+// build_clusters (trees_host.hip) of a tree built by `b`, computed on the device from element
+// data on the device and read back.
+Clusters build_clusters_device(ddgrs_ctx* ctx, GpuBvhBuilder& b, const Topology& t, const double* pts,
+                               const double* lo, const double* hi, const double* mass, const double* nrm) {
+  const size_t n = t.order.size(), nn = t.n_nodes(), nl = t.leaves.size();
+  Clusters c;
+  if (nn == 0) return c;
+  const size_t words = n + 4 * nn + nl;
+  uint32_t* h = (uint32_t*)ctx->arena_take(words * sizeof(uint32_t));
+  uint32_t* p = h;
+  for (const std::vector<uint32_t>* v : {&t.order, &t.start, &t.end, &t.right, &t.depth, &t.leaves}) {
+    std::copy(v->begin(), v->end(), p);
+    p += v->size();
+  }
+  b.cl_topo.reserve(words);
+  HIPCHECK(hipMemcpyAsync(b.cl_topo.ptr, h, words * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+  const uint32_t* order = b.cl_topo.ptr;
+  const uint32_t *start = order + n, *end = start + nn, *right = end + nn, *depth = right + nn, *leaves = depth + nn;
+  b.cl_buf.reserve(27 * nn);
+  const ClusterOut C = cluster_out(b.cl_buf.ptr, nn);
+  const int with_ps = nrm != nullptr;
+  launch(ctx, "cl_leaf", cl_leaf_kernel, dim3(blocks_for((int64_t)nl, 256)), dim3(256), 0, (uint32_t)nl, leaves, start,
+         end, order, pts, lo, hi, mass, nrm, C);
+  for (uint32_t d = t.height; d-- > 0;)
+    launch(ctx, "cl_internal", cl_internal_kernel, dim3(blocks_for((int64_t)nn, 256)), dim3(256), 0, (uint32_t)nn, d,
+           depth, right, with_ps, C);
+  launch(ctx, "cl_final", cl_final_kernel, dim3(blocks_for((int64_t)nn, 256)), dim3(256), 0, (uint32_t)nn, with_ps, C);
+  const size_t nout = (with_ps ? 17 : 11) * nn;
+  double* out = (double*)ctx->staging(nout * sizeof(double));
+  download(ctx, out, (const double*)b.cl_buf.ptr, nout);
+  ctx->sync();
+  c.mass.assign(out, out + nn);
+  c.centroid.assign(out + nn, out + 4 * nn);
+  c.lo.assign(out + 4 * nn, out + 7 * nn);
+  c.hi.assign(out + 7 * nn, out + 10 * nn);
+  c.radius.assign(out + 10 * nn, out + 11 * nn);
+  if (with_ps) c.projector.assign(out + 11 * nn, out + 17 * nn);
+  return c;
+}
+
 void build_face_tree(ddgrs_surface* s, const HostGeometry& g, size_t leaf_size, GpuBvhBuilder& b, Topology* topo,
                      Clusters* clusters) {
   ddgrs_ctx* ctx = s->ctx;
@@ -639,7 +777,8 @@ void build_face_tree(ddgrs_surface* s, const HostGeometry& g, size_t leaf_size, 
   auto t0 = std::chrono::steady_clock::now();
   *topo = b.build(ctx, nf_, s->bary.ptr, b.box_lo.ptr, b.box_hi.ptr, leaf_size);
   auto t1 = std::chrono::steady_clock::now();
-  *clusters = build_clusters(*topo, g.bary.data(), g.lo.data(), g.hi.data(), g.area.data(), g.nrm.data());
+  *clusters = build_clusters_device(ctx, b, *topo, s->bary.ptr, b.box_lo.ptr, b.box_hi.ptr, s->area.ptr,
+                                    s->normal.ptr);
   if (trace)
     std::fprintf(stderr, "step:   face BVH %.2f ms, clusters %.2f ms\n",
                  std::chrono::duration<double>(t1 - t0).count() * 1e3,

@@ -25,6 +25,10 @@ struct GpuBvhBuilder {
   DevBuf<uint32_t> act_start, act_end, act_prefix, act_split;
   DevBuf<double> suf;  // 6 per element: suffix box lo, hi
   DevBuf<double> box_lo, box_hi;  // element boxes when the caller has none on the device
+  // cluster aggregates of a built tree on the device: its topology (order, start, end, right,
+  // depth, leaves) and the per-node sums and records
+  DevBuf<uint32_t> cl_topo;
+  DevBuf<double> cl_buf;
   DevBuf<uint8_t> temp;
   // chunked split of large nodes: per chunk its node and first element, per node its first chunk
   DevBuf<uint32_t> ch_seg, ch_c0, seg_chunk0;
