@@ -578,28 +578,12 @@ HostGeometry download_geometry(ddgrs_surface* s) {
   ddgrs_ctx* ctx = s->ctx;
   const size_t nf_ = (size_t)s->nf, nv = (size_t)s->nv;
   HostGeometry g;
-  g.pos.resize(3 * nv);
-  g.area.resize(nf_);
-  g.bary.resize(3 * nf_);
-  g.nrm.resize(3 * nf_);
-  download(ctx, g.pos.data(), (const double*)s->positions.ptr, 3 * nv);
-  download(ctx, g.area.data(), (const double*)s->area.ptr, nf_);
-  download(ctx, g.bary.data(), (const double*)s->bary.ptr, 3 * nf_);
-  download(ctx, g.nrm.data(), (const double*)s->normal.ptr, 3 * nf_);
+  double* pin = (double*)ctx->staging((3 * nv + nf_) * sizeof(double));
+  download(ctx, pin, (const double*)s->positions.ptr, 3 * nv);
+  download(ctx, pin + 3 * nv, (const double*)s->area.ptr, nf_);
   ctx->sync();
-  g.lo.resize(3 * nf_);
-  g.hi.resize(3 * nf_);
-  for (size_t f = 0; f < nf_; ++f)
-    for (int k = 0; k < 3; ++k) {
-      double a = INFINITY, b = -INFINITY;
-      for (int c = 0; c < 3; ++c) {
-        double x = g.pos[3 * (size_t)s->faces_h[3 * f + c] + k];
-        a = std::fmin(a, x);
-        b = std::fmax(b, x);
-      }
-      g.lo[3 * f + k] = a;
-      g.hi[3 * f + k] = b;
-    }
+  g.pos.assign(pin, pin + 3 * nv);
+  g.area.assign(pin + 3 * nv, pin + 3 * nv + nf_);
   // RS/src/geometry.rs:194 vertex_areas (faces ascending)
   g.vmass.assign(nv, 0.0);
   for (size_t f = 0; f < nf_; ++f) {
@@ -765,8 +749,7 @@ Clusters build_clusters_device(ddgrs_ctx* ctx, GpuBvhBuilder& b, const Topology&
   return c;
 }
 
-void build_face_tree(ddgrs_surface* s, const HostGeometry& g, size_t leaf_size, GpuBvhBuilder& b, Topology* topo,
-                     Clusters* clusters) {
+void build_face_tree(ddgrs_surface* s, size_t leaf_size, GpuBvhBuilder& b, Topology* topo, Clusters* clusters) {
   ddgrs_ctx* ctx = s->ctx;
   const size_t nf_ = (size_t)s->nf;
   b.box_lo.reserve(3 * nf_);
@@ -787,7 +770,7 @@ void build_face_tree(ddgrs_surface* s, const HostGeometry& g, size_t leaf_size, 
 
 void build_face_tree(ddgrs_surface* s, size_t leaf_size, Topology* topo, Clusters* clusters) {
   GpuBvhBuilder b;
-  build_face_tree(s, download_geometry(s), leaf_size, b, topo, clusters);
+  build_face_tree(s, leaf_size, b, topo, clusters);
 }
 
 Topology build_vertex_tree(ddgrs_surface* s, size_t leaf_size, GpuBvhBuilder& b) {

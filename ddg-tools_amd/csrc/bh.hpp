@@ -80,16 +80,15 @@ struct BhEval {
 }  // namespace ddgrs
 
 namespace ddgrs {
-// The surface's current positions and face geometry on the host (one synchronisation).
+// The surface's current positions, face areas and vertex areas on the host (one synchronisation).
 struct HostGeometry {
-  std::vector<double> pos, area, bary, nrm, lo, hi;  // 3 nv, nf, 3 nf, 3 nf, 3 nf (face boxes), 3 nf
-  std::vector<double> vmass;                           // vertex areas a_v = ⅓ Σ_{S∋v} a_S
+  std::vector<double> pos, area;  // 3 nv, nf
+  std::vector<double> vmass;      // vertex areas a_v = ⅓ Σ_{S∋v} a_S
 };
 HostGeometry download_geometry(ddgrs_surface* s);
-// Face BVH (topology built on the device, build-time clusters with projectors on the host from the
-// snapshot g of the same positions).
-void build_face_tree(ddgrs_surface* s, const HostGeometry& g, size_t leaf_size, GpuBvhBuilder& b, Topology* topo,
-                     Clusters* clusters);
+// Face BVH (topology built on the device) and its build-time clusters with projectors (computed on
+// the device, read back).
+void build_face_tree(ddgrs_surface* s, size_t leaf_size, GpuBvhBuilder& b, Topology* topo, Clusters* clusters);
 // Face BVH at the surface's current positions.
 void build_face_tree(ddgrs_surface* s, size_t leaf_size, Topology* topo, Clusters* clusters);
 // Vertex BVH (points f_v) at the surface's current positions, built on the device.

@@ -315,7 +315,7 @@ void Session::step(StepReport* rep) {
   });
   // the face tree is built into the Barnes–Hut evaluator's members, which the step's other users
   // read (nothing rebuilds it before the next step)
-  build_face_tree(s, geo, opts.leaf_size, face_bvh, &bh.topo, &bh.clusters);
+  build_face_tree(s, opts.leaf_size, face_bvh, &bh.topo, &bh.clusters);
   const Topology& topo = bh.topo;
   const Clusters& clusters = bh.clusters;
   mark("face tree");
