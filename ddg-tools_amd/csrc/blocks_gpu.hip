@@ -103,7 +103,9 @@ __global__ __launch_bounds__(256) void blk_classify_kernel(uint32_t m, const uin
 // `overflow` and the level and all later ones do nothing.
 constexpr uint32_t kLevelBlocks = 1024;
 constexpr uint32_t kMaxLevels = 64;  // levels whose sizes are recorded for the next build's grids
-constexpr uint64_t kFirstCapPerNode = 64;  // a first build's list capacities per tree node
+// A first build's list capacities per tree node: pairs of a level, admissible blocks, near blocks
+// (surface meshes of 10⁴–10⁵ faces have about 23, 70 and 30 per node).
+constexpr uint64_t kFirstPairsPerNode = 48, kFirstAdmPerNode = 128, kFirstNearPerNode = 64;
 
 struct LevelState {
   uint32_t m[2];                // pairs of the level, by level parity
@@ -677,18 +679,25 @@ BlockStats BlockTreeDev::build_structure(ddgrs_ctx* ctx, const Topology& t, cons
   uint32_t n_adm = 0, n_near = 0, depth = 0, max_m = 1;
   // a first build sizes its lists from the node count; a list that outgrows its capacity makes the
   // device pass start over with four times the capacities (the host pass after the third)
-  if (!S.cap_pairs) S.cap_pairs = S.cap_adm = S.cap_near = kFirstCapPerNode * (uint64_t)nn + 4096;
+  if (!S.cap_pairs) {
+    S.cap_pairs = kFirstPairsPerNode * (uint64_t)nn + 4096;
+    S.cap_adm = kFirstAdmPerNode * (uint64_t)nn + 4096;
+    S.cap_near = kFirstNearPerNode * (uint64_t)nn + 4096;
+  }
   bool on_device = false;
   for (int attempt = 0; attempt < 3 && !levels_on_host && !on_device; ++attempt) {
     on_device = device_levels(ctx, S, g, right.ptr, n_adm, n_near, depth, max_m);
     if (!on_device) {
+      if (trace) std::fprintf(stderr, "blocks(%d): device levels overflowed their capacities\n", n);
       S.cap_pairs *= 4;
       S.cap_adm *= 4;
       S.cap_near *= 4;
     }
   }
   if (!on_device) host_levels(ctx, S, g, right.ptr, n_adm, n_near, depth, max_m);
-  if (trace) std::fprintf(stderr, "blocks(%d): levels %s, depth %u\n", n, on_device ? "on the device" : "on the host", depth);
+  if (trace)
+    std::fprintf(stderr, "blocks(%d): levels %s, depth %u, largest level %u pairs, %u admissible, %u near\n", n,
+                 on_device ? "on the device" : "on the host", depth, max_m, n_adm, n_near);
   S.cap_pairs = std::max<uint64_t>(S.cap_pairs, (uint64_t)max_m + max_m / 4 + 1024);
   S.cap_adm = std::max<uint64_t>(S.cap_adm, (uint64_t)n_adm + n_adm / 4 + 1024);
   S.cap_near = std::max<uint64_t>(S.cap_near, (uint64_t)n_near + n_near / 4 + 1024);
