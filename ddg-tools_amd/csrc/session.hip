@@ -137,7 +137,8 @@ void Session::init(ddgrs_surface* s_, const StepOptions& o) {
     }
   last_phi_before.clear();
   last_phi.clear();
-  // the mesh's L′ structure (fixed while only the positions change), on another thread
+  // the mesh's L′ structure (fixed while only the positions change) on another thread while the
+  // step's buffers are set up, then uploaded: per-mesh work, done once before the first step
   lp_ready = false;
   const size_t lds = s->ctx->prop.sharedMemPerBlock;
   lp_symbolic = std::async(std::launch::async, [this, nv, lds, p = std::move(pos)] {
@@ -153,6 +154,9 @@ void Session::init(ddgrs_surface* s_, const StepOptions& o) {
   scal.reserve(16);
   mwork.reserve(4 * nv);
   xpart.reserve(3 * (size_t)blocks_for((int64_t)nv, 256));
+  HIPCHECK(hipSetDevice(aux->device));
+  pc.setup_device(aux, lp_symbolic.get());
+  lp_ready = true;
 }
 
 double Session::read_scalar(const double* d) {
