@@ -716,10 +716,10 @@ Clusters build_clusters_device(ddgrs_ctx* ctx, GpuBvhBuilder& b, const Topology&
   const size_t n = t.order.size(), nn = t.n_nodes(), nl = t.leaves.size();
   Clusters c;
   if (nn == 0) return c;
-  const size_t words = n + 4 * nn + nl;
+  const size_t words = n + 5 * nn + nl;
   uint32_t* h = (uint32_t*)ctx->arena_take(words * sizeof(uint32_t));
   uint32_t* p = h;
-  for (const std::vector<uint32_t>* v : {&t.order, &t.start, &t.end, &t.right, &t.depth, &t.leaves}) {
+  for (const std::vector<uint32_t>* v : {&t.order, &t.start, &t.end, &t.right, &t.depth, &t.leaves, &t.parent}) {
     std::copy(v->begin(), v->end(), p);
     p += v->size();
   }
@@ -747,6 +747,14 @@ Clusters build_clusters_device(ddgrs_ctx* ctx, GpuBvhBuilder& b, const Topology&
   c.radius.assign(out + 10 * nn, out + 11 * nn);
   if (with_ps) c.projector.assign(out + 11 * nn, out + 17 * nn);
   return c;
+}
+
+DeviceTree device_tree(const GpuBvhBuilder& b, const Topology& t) {
+  const size_t n = t.order.size(), nn = t.n_nodes(), nl = t.leaves.size();
+  const uint32_t* o = b.cl_topo.ptr;
+  const ClusterOut C = cluster_out(b.cl_buf.ptr, nn);
+  return {o, o + n, o + n + nn, o + n + 2 * nn, o + n + 4 * nn + nl, o + n + 4 * nn,
+          C.mass, C.centroid, C.lo, C.hi, C.radius, C.projector};
 }
 
 void build_face_tree(ddgrs_surface* s, size_t leaf_size, GpuBvhBuilder& b, Topology* topo, Clusters* clusters) {

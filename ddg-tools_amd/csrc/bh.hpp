@@ -89,6 +89,8 @@ HostGeometry download_geometry(ddgrs_surface* s);
 // Face BVH (topology built on the device) and its build-time clusters with projectors (computed on
 // the device, read back).
 void build_face_tree(ddgrs_surface* s, size_t leaf_size, GpuBvhBuilder& b, Topology* topo, Clusters* clusters);
+// The tree and clusters of b's last build_face_tree, on the device (valid until b builds again).
+DeviceTree device_tree(const GpuBvhBuilder& b, const Topology& t);
 // Face BVH at the surface's current positions.
 void build_face_tree(ddgrs_surface* s, size_t leaf_size, Topology* topo, Clusters* clusters);
 // Vertex BVH (points f_v) at the surface's current positions, built on the device.

@@ -83,6 +83,16 @@ __device__ __forceinline__ BlockCounts classify_pair(const BlockGeo& g, uint32_t
 }
 
 // cls: 0 admissible, 1 near, 2 split; cnt: what the pair adds to each list.
+// position[order[p]] = p (p < n) and inv_mass[a] = 1 / mass[a] (a < nn).
+__global__ __launch_bounds__(256) void blk_tree_maps_kernel(uint32_t n, uint32_t nn, const uint32_t* __restrict__ order,
+                                                            const double* __restrict__ mass,
+                                                            uint32_t* __restrict__ position,
+                                                            double* __restrict__ inv_mass) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) position[order[i]] = i;
+  if (i < nn) inv_mass[i] = 1.0 / mass[i];
+}
+
 __global__ __launch_bounds__(256) void blk_classify_kernel(uint32_t m, const uint32_t* __restrict__ pa,
                                                            const uint32_t* __restrict__ pb, BlockGeo g,
                                                            uint8_t* __restrict__ cls, BlockCounts* __restrict__ cnt) {
@@ -634,7 +644,7 @@ static bool device_levels(ddgrs_ctx* ctx, BlockBuildScratch& S, const BlockGeo& 
 }
 
 BlockStats BlockTreeDev::build_structure(ddgrs_ctx* ctx, const Topology& t, const Clusters& cl, double chi,
-                                         size_t leaf_pairs, int nk_, const double* elem_weight) {
+                                         size_t leaf_pairs, int nk_, const double* elem_weight, const DeviceTree* dt) {
   BlockBuildScratch& S = scratch;
   // DDGRS_TRACE_STEP=1: wall times of the build's phases on stderr
   static const bool trace = std::getenv("DDGRS_TRACE_STEP") != nullptr;
@@ -651,21 +661,41 @@ BlockStats BlockTreeDev::build_structure(ddgrs_ctx* ctx, const Topology& t, cons
   n = (int)t.order.size();
   nn = (int)t.n_nodes();
   nl = (int)t.leaves.size();
-  ddgrs::upload(ctx, order, t.order);
-  std::vector<uint32_t> pos(n);
-  for (int p = 0; p < n; ++p) pos[t.order[p]] = (uint32_t)p;
-  ddgrs::upload(ctx, position, pos);
-  ddgrs::upload(ctx, start, t.start);
-  ddgrs::upload(ctx, end, t.end);
-  ddgrs::upload(ctx, right, t.right);
-  ddgrs::upload(ctx, parent, t.parent);
-  ddgrs::upload(ctx, leaves, t.leaves);
-  std::vector<double> im(nn);
-  for (int a = 0; a < nn; ++a) im[a] = 1.0 / cl.mass[a];
-  ddgrs::upload(ctx, inv_mass, im);
-  ddgrs::upload(ctx, S.lo, cl.lo);
-  ddgrs::upload(ctx, S.hi, cl.hi);
-  ddgrs::upload(ctx, S.radius, cl.radius);
+  if (dt) {
+    auto copy = [&](auto& d, const auto* src, size_t cnt) {
+      d.reserve(cnt);
+      HIPCHECK(hipMemcpyAsync(d.ptr, src, cnt * sizeof(*src), hipMemcpyDeviceToDevice, ctx->stream));
+    };
+    copy(order, dt->order, (size_t)n);
+    copy(start, dt->start, (size_t)nn);
+    copy(end, dt->end, (size_t)nn);
+    copy(right, dt->right, (size_t)nn);
+    copy(parent, dt->parent, (size_t)nn);
+    copy(leaves, dt->leaves, (size_t)nl);
+    copy(S.lo, dt->lo, 3 * (size_t)nn);
+    copy(S.hi, dt->hi, 3 * (size_t)nn);
+    copy(S.radius, dt->radius, (size_t)nn);
+    position.reserve(n);
+    inv_mass.reserve(nn);
+    launch(ctx, "blk_tree_maps", blk_tree_maps_kernel, dim3(blocks_for(std::max(n, nn), 256)), dim3(256), 0,
+           (uint32_t)n, (uint32_t)nn, dt->order, dt->mass, position.ptr, inv_mass.ptr);
+  } else {
+    ddgrs::upload(ctx, order, t.order);
+    std::vector<uint32_t> pos(n);
+    for (int p = 0; p < n; ++p) pos[t.order[p]] = (uint32_t)p;
+    ddgrs::upload(ctx, position, pos);
+    ddgrs::upload(ctx, start, t.start);
+    ddgrs::upload(ctx, end, t.end);
+    ddgrs::upload(ctx, right, t.right);
+    ddgrs::upload(ctx, parent, t.parent);
+    ddgrs::upload(ctx, leaves, t.leaves);
+    std::vector<double> im(nn);
+    for (int a = 0; a < nn; ++a) im[a] = 1.0 / cl.mass[a];
+    ddgrs::upload(ctx, inv_mass, im);
+    ddgrs::upload(ctx, S.lo, cl.lo);
+    ddgrs::upload(ctx, S.hi, cl.hi);
+    ddgrs::upload(ctx, S.radius, cl.radius);
+  }
   weight.reserve(n);
   launch(ctx, "blk_weight", blk_weight_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, (uint32_t)n,
          (const uint32_t*)order.ptr, elem_weight, weight.ptr);

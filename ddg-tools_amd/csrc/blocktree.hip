@@ -822,12 +822,23 @@ namespace {
 // Far coefficients, α, near values and the per-face data of a face metric whose block tree is on
 // the device.
 void face_metric_values(ddgrs_ctx* ctx, ddgrs_surface* s, FaceMetricDev& F, const std::vector<double>& centroid,
-                        const std::vector<double>& projector, const std::vector<double>& node_mass, double s_exp) {
+                        const std::vector<double>& projector, const std::vector<double>& node_mass, double s_exp,
+                        const DeviceTree* dt = nullptr) {
   BlockTreeDev& bt = F.bt;
   DevBuf<double>&cen = F.node_cen, &proj = F.node_proj, &nmass = F.node_mass;
-  ddgrs::upload(ctx, cen, centroid);
-  ddgrs::upload(ctx, proj, projector);
-  ddgrs::upload(ctx, nmass, node_mass);
+  if (dt) {
+    const size_t nn = node_mass.size();
+    cen.reserve(3 * nn);
+    proj.reserve(6 * nn);
+    nmass.reserve(nn);
+    HIPCHECK(hipMemcpyAsync(cen.ptr, dt->centroid, 3 * nn * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHECK(hipMemcpyAsync(proj.ptr, dt->projector, 6 * nn * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHECK(hipMemcpyAsync(nmass.ptr, dt->mass, nn * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+  } else {
+    ddgrs::upload(ctx, cen, centroid);
+    ddgrs::upload(ctx, proj, projector);
+    ddgrs::upload(ctx, nmass, node_mass);
+  }
   const double neg_s = -s_exp;
   launch(ctx, "fm_far_coef", fm_far_coef_kernel, dim3(blocks_for(bt.n_far, 256)), dim3(256), 0, bt.n_far,
          (const uint32_t*)bt.far_row.ptr, (const uint32_t*)bt.far_col.ptr, NodeGeo{cen.ptr, proj.ptr, nmass.ptr}, neg_s,
@@ -857,11 +868,11 @@ void FaceMetricDev::build(ddgrs_surface* s, const Topology& t, const Clusters& c
 }
 
 void FaceMetricDev::build(ddgrs_ctx* ctx, ddgrs_surface* s, const Topology& t, const Clusters& cl, double chi,
-                          size_t leaf_pairs, double s_exp) {
+                          size_t leaf_pairs, double s_exp, const DeviceTree* dt) {
   topo = t;
   nv = (int)s->nv;
   nf = (int)s->nf;
-  stats = bt.build_structure(ctx, t, cl, chi, leaf_pairs, 2, s->area.ptr);
+  stats = bt.build_structure(ctx, t, cl, chi, leaf_pairs, 2, s->area.ptr, dt);
   static const bool trace = std::getenv("DDGRS_TRACE_STEP") != nullptr;
   const auto t0 = std::chrono::steady_clock::now();
   inc_ptr.reserve((size_t)nv + 1);
@@ -870,7 +881,7 @@ void FaceMetricDev::build(ddgrs_ctx* ctx, ddgrs_surface* s, const Topology& t, c
   inc.reserve(3 * (size_t)nf);
   launch(ctx, "fm_inc", fm_inc_kernel, dim3(blocks_for(3 * (int64_t)nf, 256)), dim3(256), 0, (uint32_t)(3 * nf),
          (const uint32_t*)bt.position.ptr, (const uint32_t*)s->inc.ptr, inc.ptr);
-  face_metric_values(ctx, s, *this, cl.centroid, cl.projector, cl.mass, s_exp);
+  face_metric_values(ctx, s, *this, cl.centroid, cl.projector, cl.mass, s_exp, dt);
   if (trace) {
     ctx->sync();
     std::fprintf(stderr, "blocks: %-14s %8.2f ms\n", "values",

@@ -113,8 +113,9 @@ struct BlockTreeDev {
   // upload(plan_block_tree(...)) of build_block_structure's lists): level-synchronous
   // classification of the block pairs, stable grouping by row node, panels from the near lists.
   // `elem_weight`: per element in the tree's input numbering (device pointer).
+  // `dt`: the tree and clusters on the device (copied there instead of uploaded), or null.
   BlockStats build_structure(ddgrs_ctx* ctx, const Topology& t, const Clusters& c, double chi, size_t leaf_pairs,
-                             int nk, const double* elem_weight);
+                             int nk, const double* elem_weight, const DeviceTree* dt = nullptr);
   void finish_far(ddgrs_ctx* ctx);
   // A host-planned structure's near lists, for lists() (upload() carries the panels only).
   void set_near_lists(ddgrs_ctx* ctx, const BlockStructure& s);
@@ -167,7 +168,7 @@ struct FaceMetricDev {
   void build(ddgrs_surface* s, const Topology& t, const Clusters& c, double chi, size_t leaf_pairs, double s_exp);
   // The same on ctx's stream (the surface's device data must be current there).
   void build(ddgrs_ctx* ctx, ddgrs_surface* s, const Topology& t, const Clusters& c, double chi, size_t leaf_pairs,
-             double s_exp);
+             double s_exp, const DeviceTree* dt = nullptr);
   // The same from a host plan.
   void build(ddgrs_surface* s, const FaceMetricPlan& p, double s_exp);
   // y = (w_hi B̃ + w_lo B̃⁰) x for one interleaved 3 nv FP64 vector (device pointers).

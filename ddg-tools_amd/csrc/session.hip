@@ -322,6 +322,7 @@ void Session::step(StepReport* rep) {
   build_face_tree(s, opts.leaf_size, face_bvh, &bh.topo, &bh.clusters);
   const Topology& topo = bh.topo;
   const Clusters& clusters = bh.clusters;
+  const DeviceTree dtree = device_tree(face_bvh, topo);
   mark("face tree");
   // the face metric's block tree on the third stream and another thread (reads the face tree and
   // the surface's geometry, which stay unchanged until the line search)
@@ -331,7 +332,7 @@ void Session::step(StepReport* rep) {
   auto metric = std::async(std::launch::async, [&, s_exp] {
     HIPCHECK(hipSetDevice(aux2->device));
     const auto t0 = std::chrono::steady_clock::now();
-    fm.build(aux2, s, topo, clusters, opts.chi, opts.leaf_pairs, s_exp);
+    fm.build(aux2, s, topo, clusters, opts.chi, opts.leaf_pairs, s_exp, &dtree);
     aux2->sync();
     return seconds_since(t0);
   });

@@ -30,6 +30,13 @@ Topology finish_topology(const std::vector<TreeSegment>& segs, std::vector<uint3
 
 // Build-time cluster data (RS/src/spatial/mod.rs:80 Cluster), summed bottom-up in the reference's
 // order: area, centroid, box of the element boxes, radius to the farthest box corner.
+// A tree and its clusters already on the device (the face tree of a step): the block tree's build
+// copies them on the device instead of uploading the host copies.
+struct DeviceTree {
+  const uint32_t *order, *start, *end, *right, *parent, *leaves;
+  const double *mass, *centroid, *lo, *hi, *radius, *projector;
+};
+
 struct Clusters {
   std::vector<double> mass, centroid, lo, hi, radius;  // n, 3n, 3n, 3n, n
   std::vector<double> projector;  // 6n: P̄ = Σ a N Nᵀ / Σ a, upper triangle xx xy xz yy yz zz (face trees)
