@@ -419,7 +419,9 @@ DDGRS_API int ddgrs_session_trials(ddgrs_session* h, int64_t capacity, double* t
 /* Fills the reference defaults ((6, 12), Barnes–Hut θ 0.5, restart 50, 1e-6, 200, c 1e-2). */
 DDGRS_API void ddgrs_step_options_default(ddgrs_step_options* o);
 /* A session on the surface (which it moves): ℓ̄₀ and the barycenter target at the current
- * positions.  Replaces Session::new (RS/src/step/session.rs:371). */
+ * positions, and the mesh's L′ symbolic analysis (elimination order, fronts) uploaded to the
+ * device, so that the first step does no per-mesh work.  Replaces Session::new
+ * (RS/src/step/session.rs:371). */
 DDGRS_API int ddgrs_session_create(ddgrs_surface* s, const ddgrs_step_options* o, ddgrs_session** out,
                                    ddgrs_error* err);
 DDGRS_API void ddgrs_session_destroy(ddgrs_session* h);
